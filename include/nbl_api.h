@@ -1,0 +1,236 @@
+/*
+ * nbl_api.h — C-ABI of the MI355X (gfx950) nabla_solve hot path.
+ *
+ * Plain C: POD descriptors, pointers and sizes only. No C++ exception crosses
+ * this boundary; every entry point returns an int status (NBL_OK == 0, < 0 on
+ * error) and the message of the last failure on the calling thread is
+ * available from nbl_last_error().  The C++20 host layer (the nabla/flow headers in
+ * the package) wraps these calls and rethrows std::runtime_error, which is the
+ * reference's error convention (core/src/flow/ns_solver.cpp:830,
+ * core/src/flow/case.cpp:303-314).
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   nbl_grid_desc / nbl_face_bc      <- nabla::flow::UniformGrid, FaceBC
+ *                                       (core/include/nabla/flow/grid.hpp:17-73)
+ *   nbl_flow_cfg / nbl_step_report   <- nabla::flow::FlowConfig, StepReport
+ *                                       (core/include/nabla/flow/ns_solver.hpp:14-49)
+ *   nbl_run_*                        <- nabla::flow::NSSolver
+ *                                       (core/include/nabla/flow/ns_solver.hpp:58-89)
+ *   nbl_poisson_*                    <- nabla::flow::PoissonSolver / MgPoisson /
+ *                                       CgPoisson (core/include/nabla/flow/poisson.hpp:20-89)
+ *   nbl_octree_*                     <- nabla::mesh::Octree connectivity and
+ *                                       nabla::flow::OctreePoisson
+ *                                       (core/include/nabla/mesh/octree.hpp:104-178,
+ *                                        core/include/nabla/flow/octree_poisson.hpp:29-55)
+ *
+ * Threading: one nbl_run / nbl_poisson / nbl_octree is used by one host thread
+ * at a time; each owns its own HIP stream.  A context may be shared.
+ */
+#ifndef NBL_API_H
+#define NBL_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NBL_OK 0
+#define NBL_ERR_INVALID (-1)   /* bad argument / size mismatch            */
+#define NBL_ERR_DEVICE (-2)    /* HIP runtime or kernel launch failure     */
+#define NBL_ERR_STATE (-3)     /* call not valid in the object's state     */
+#define NBL_ERR_CAPACITY (-4)  /* request exceeds a fixed capacity         */
+
+/* Boundary kinds, same order as nabla::flow::BoundaryType (grid.hpp:17). */
+enum { NBL_BC_WALL = 0, NBL_BC_INLET = 1, NBL_BC_OUTLET = 2, NBL_BC_PERIODIC = 3 };
+/* Inlet profiles, same order as nabla::flow::InletProfile (grid.hpp:18). */
+enum { NBL_INLET_UNIFORM = 0, NBL_INLET_PARABOLIC = 1 };
+/* Convection schemes, same order as nabla::flow::ConvectionScheme (ns_solver.hpp:12). */
+enum { NBL_CONV_CENTRAL = 0, NBL_CONV_WENO5 = 1 };
+/* Pressure-solve precision. */
+enum {
+  NBL_PREC_FP64 = 0,  /* plain: every level of the V-cycle in FP64                  */
+  NBL_PREC_MIXED = 1, /* FP32 V-cycle inside FP64 flexible CG (FP64 residual/dots)   */
+  NBL_PREC_MIXED_BF16 = 2 /* as MIXED, with BF16 smoother scratch on the finest level */
+};
+
+/* One domain face (grid.hpp:20-26). */
+typedef struct nbl_face_bc {
+  int type;                 /* NBL_BC_*                                  */
+  double wall_velocity[3];  /* Wall: prescribed velocity (lid)            */
+  int profile;              /* NBL_INLET_*                               */
+  double inlet_speed;       /* bulk inlet speed                          */
+  int inlet_axis;           /* velocity component the inlet drives        */
+} nbl_face_bc;
+
+/* Uniform MAC grid (grid.hpp:32-73).  Faces: 0=-x 1=+x 2=-y 3=+y 4=-z 5=+z.
+ * `solid` is a host array of nx*ny*nz bytes in the compact order
+ * c = i + nx*(j + ny*k) (grid.hpp:48-53); NULL means no solid cells. */
+typedef struct nbl_grid_desc {
+  int nx, ny, nz;
+  double origin[3];
+  double dx, dy, dz;
+  nbl_face_bc face[6];
+  const uint8_t* solid;
+} nbl_grid_desc;
+
+/* FlowConfig (ns_solver.hpp:14-32).  Defaults: nbl_flow_cfg_default(). */
+typedef struct nbl_flow_cfg {
+  double reynolds;
+  double ref_length;
+  double velocity_scale;
+  double density;
+  double cfl;
+  double body_force[3];
+  int convection;           /* NBL_CONV_* */
+  double poisson_tol;
+  int poisson_max_iters;
+  double tol_continuity;
+  double tol_mass;
+  double ref_area;          /* 0 => ref_length^2 */
+} nbl_flow_cfg;
+
+/* Options of the GPU pressure solver that have no reference counterpart. */
+typedef struct nbl_solver_opts {
+  int precision;            /* NBL_PREC_*                                   */
+  int pre_smooth;           /* damped-Jacobi sweeps before restriction (2)  */
+  int post_smooth;          /* sweeps after prolongation (2)                */
+  int check_every;          /* host polls the device convergence flag after
+                               this many flexible-CG iterations (>=1)       */
+} nbl_solver_opts;
+
+/* StepReport (ns_solver.hpp:34-49). */
+typedef struct nbl_step_report {
+  int step;
+  double t;
+  double dt;
+  double cfl;
+  double momentum_residual;
+  double continuity_residual;
+  double divergence_max;
+  double mass_error;
+  int poisson_iters;
+  double poisson_residual;
+  double cd;
+  double cl;
+  uint64_t cell_count;
+  int accepted;
+} nbl_step_report;
+
+/* PoissonSolver::Report (poisson.hpp:25-29). */
+typedef struct nbl_poisson_report {
+  int iterations;
+  double residual;
+  int converged;
+} nbl_poisson_report;
+
+typedef struct nbl_ctx nbl_ctx;
+typedef struct nbl_run nbl_run;
+typedef struct nbl_poisson nbl_poisson;
+typedef struct nbl_octree nbl_octree;
+
+/* ---- defaults and errors ------------------------------------------------ */
+void nbl_flow_cfg_default(nbl_flow_cfg* cfg);        /* FlowConfig{} values        */
+void nbl_solver_opts_default(nbl_solver_opts* opts); /* FP64, V(2,2), check 1     */
+const char* nbl_last_error(void);
+const char* nbl_version(void);
+
+/* ---- device context ------------------------------------------------------ */
+int nbl_ctx_create(int device, nbl_ctx** out);
+int nbl_ctx_destroy(nbl_ctx* ctx);
+/* HBM bytes currently held by runs / solvers / octrees of this context. */
+int nbl_ctx_bytes_in_use(nbl_ctx* ctx, uint64_t* bytes);
+
+/* ---- MAC Navier-Stokes run (NSSolver, ns_solver.hpp:58-89) ----------------
+ * State stays in HBM between calls.  Host arrays use the reference's face
+ * layouts: u (nx+1)*ny*nz, v nx*(ny+1)*nz, w nx*ny*(nz+1), p nx*ny*nz
+ * (ns_solver.hpp:93-110).  Passing NULL for `opts` selects the defaults.     */
+int nbl_run_create(nbl_ctx* ctx, const nbl_grid_desc* grid, const nbl_flow_cfg* cfg,
+                   const nbl_solver_opts* opts, nbl_run** out);
+int nbl_run_destroy(nbl_run* run);
+/* NSSolver::initialize (ns_solver.cpp:95-112). */
+int nbl_run_initialize(nbl_run* run);
+/* NSSolver::setState (ns_solver.cpp:826-839); sizes are checked. */
+int nbl_run_set_state(nbl_run* run, const double* u, size_t nu, const double* v, size_t nv,
+                      const double* w, size_t nw, const double* p, size_t np, int step,
+                      double t, double dt_last);
+/* uFace/vFace/wFace/pressure + stepIndex/time/dtLast (ns_solver.hpp:142-160). Any
+ * pointer may be NULL. */
+int nbl_run_get_state(nbl_run* run, double* u, double* v, double* w, double* p, int* step,
+                      double* t, double* dt_last);
+/* NSSolver::computeDt (ns_solver.cpp:581-603). */
+int nbl_run_compute_dt(nbl_run* run, double* dt);
+/* NSSolver::step(dt) (ns_solver.cpp:767-824); dt <= 0 selects step() with the
+ * CFL dt (ns_solver.cpp:765), computed on the device. */
+int nbl_run_step(nbl_run* run, double dt, nbl_step_report* rep);
+/* Advance `nsteps` CFL steps without a host round trip per step for the
+ * report; `reps` (may be NULL) receives nsteps reports. */
+int nbl_run_step_many(nbl_run* run, int nsteps, nbl_step_report* reps);
+/* Device pointers of the resident state (u, v, w, p; reference layouts). */
+int nbl_run_device_state(nbl_run* run, double** u, double** v, double** w, double** p);
+/* Right-hand side of the momentum equation for the current state, as
+ * NSSolver::computeRhs (ns_solver.cpp:311-473) leaves it: ru, rv, rw in the
+ * face layouts, zero on boundary-plane and solid-touching faces.  The state's
+ * boundary velocities are first enforced (ns_solver.cpp:775).              */
+int nbl_run_rhs(nbl_run* run, double* ru, double* rv, double* rw);
+/* NSSolver::cellVelocity(component) (ns_solver.cpp:690-701). */
+int nbl_run_cell_velocity(nbl_run* run, int component, double* out);
+
+/* ---- pressure Poisson plugin (PoissonSolver, poisson.hpp:20-32) ----------
+ * kind: 0 = MG-PCG "mg-pcg" (MgPoisson, multigrid.cpp:206-355),
+ *       1 = Jacobi-PCG "cg-jacobi" (CgPoisson, poisson.cpp:136-252).
+ * Solve A x = b (A = -Laplacian with the grid's closure).  x is assigned
+ * (length nx*ny*nz).  Non-convergence is reported, not an error.            */
+int nbl_poisson_create(nbl_ctx* ctx, const nbl_grid_desc* grid, int kind, double tol,
+                       int max_iters, const nbl_solver_opts* opts, nbl_poisson** out);
+int nbl_poisson_destroy(nbl_poisson* ps);
+/* Host buffers. */
+int nbl_poisson_solve(nbl_poisson* ps, const double* b, double* x, nbl_poisson_report* rep);
+/* Device buffers (HBM resident, length nx*ny*nz). */
+int nbl_poisson_solve_device(nbl_poisson* ps, const double* d_b, double* d_x,
+                             nbl_poisson_report* rep);
+/* CgPoisson::applyA (poisson.cpp:102-134), host buffers. */
+int nbl_poisson_apply(nbl_poisson* ps, const double* x, double* out);
+/* CgPoisson::buildDiagonal (poisson.cpp:78-91) and the number of multigrid
+ * levels with each level's dims (dims: 3*nlevels ints, may be NULL). */
+int nbl_poisson_diagonal(nbl_poisson* ps, double* diag);
+int nbl_poisson_levels(nbl_poisson* ps, int* nlevels, int* dims);
+/* One level-0 damped-Jacobi sweep x <- x + 0.8 (b - A x)/diag on fluid cells,
+ * without the null-space projection (multigrid.cpp:102-109). */
+int nbl_poisson_smooth_once(nbl_poisson* ps, const double* b, double* x);
+/* Level-0 -> level-1 full-weighting restriction of r (multigrid.cpp:127-150,
+ * without the projection) and level-1 -> level-0 prolongation x += P e
+ * (multigrid.cpp:158-173). */
+int nbl_poisson_restrict(nbl_poisson* ps, const double* rf, double* rc);
+int nbl_poisson_prolong(nbl_poisson* ps, const double* ec, double* xf);
+/* Timings of the last solve: milliseconds of device time and the number of
+ * preconditioner applications. */
+int nbl_poisson_last_timing(nbl_poisson* ps, double* ms, int* vcycles);
+
+/* ---- octree (Octree connectivity + OctreePoisson) ----------------------
+ * Leaves are given in any order (keys = Morton anchors on the 2^max_level grid,
+ * morton.hpp:40-42); they are Morton-sorted on the device.  perm[i] (sorted
+ * position -> input index) is returned by nbl_octree_permutation.           */
+int nbl_octree_create(nbl_ctx* ctx, const uint64_t* keys, const uint8_t* levels,
+                      const uint8_t* masks, size_t n, const double origin[3],
+                      const double extent[3], int max_level, nbl_octree** out);
+int nbl_octree_destroy(nbl_octree* oc);
+int nbl_octree_count(nbl_octree* oc, size_t* n, size_t* nnz);
+int nbl_octree_permutation(nbl_octree* oc, uint64_t* perm);
+/* Face table in sorted order, as Octree::faceNeighbors (octree.cpp:229-283)
+ * returns it: for leaf i and direction d (0..5), count[6*i+d] in {0,1,..,4}
+ * and nbr[24*i + 4*d + m] = sorted index of the m-th neighbour (-1 unused). */
+int nbl_octree_face_table(nbl_octree* oc, uint8_t* count, int32_t* nbr);
+/* OctreePoisson::applyA / buildDiagonal (octree_poisson.cpp:64-105); arrays
+ * are in the caller's input order. */
+int nbl_octree_apply(nbl_octree* oc, const double* x, double* out);
+int nbl_octree_diagonal(nbl_octree* oc, double* diag);
+/* OctreePoisson::solve (octree_poisson.cpp:107-197): Jacobi-PCG, singular. */
+int nbl_octree_poisson_solve(nbl_octree* oc, const double* b, double* x, double tol,
+                             int max_iters, nbl_poisson_report* rep);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NBL_API_H */

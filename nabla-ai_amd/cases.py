@@ -1,0 +1,197 @@
+"""Case builders and synthetic inputs, shared by tests and bench.py.
+
+Reference builders (core/src/flow/case.cpp:25-134) are mirrored as
+`lid_cavity`, `channel` and `wall_mounted_cube`; the measured workloads are
+
+* `taylor_green(n)`  — n^3 periodic box, L = 2*pi, Re 1600, CFL 0.5, WENO-5,
+  pressure tolerance 1e-8, analytic Taylor-Green initial state;
+* `cube_channel()`   — 192x64x64 channel (h = 1, 16 cells per h), uniform
+  inlet U = 1 on x-, outlet on x+, no-slip floor/ceiling, periodic span, the
+  staircase cube [3,4]x[0,1]x[1.5,2.5] (cell-centre rule of case.cpp:113-122),
+  Re_h 500, CFL 0.7, WENO-5, 1 % seeded uniform perturbation of the initial
+  interior u-faces.
+
+The PRNG is splitmix64 with doubles from the top 53 bits, fully specified so
+inputs are reproducible on any host.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+import nbl
+
+MASK64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(seed: int, n: int) -> np.ndarray:
+    """n successive splitmix64 outputs from `seed` (uint64)."""
+    with np.errstate(over="ignore"):
+        k = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + k * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(seed: int, n: int) -> np.ndarray:
+    return (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+
+def normal(seed: int, n: int) -> np.ndarray:
+    """Box-Muller N(0,1) from splitmix64 uniforms."""
+    m = (n + 1) // 2
+    u = uniform01(seed, 2 * m)
+    u1 = np.maximum(u[0::2], 2.0 ** -60)
+    u2 = u[1::2]
+    r = np.sqrt(-2.0 * np.log(u1))
+    z = np.concatenate([r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)])
+    return z[:n]
+
+
+def _faces(*types):
+    return [dict(type=t) for t in types]
+
+
+# ---- reference builders (case.cpp:25-134) ---------------------------------
+def lid_cavity(n: int, reynolds: float, lid: float = 1.0):
+    faces = _faces(nbl.BC_WALL, nbl.BC_WALL, nbl.BC_WALL, nbl.BC_WALL,
+                   nbl.BC_PERIODIC, nbl.BC_PERIODIC)
+    faces[3]["wall_velocity"] = (lid, 0.0, 0.0)
+    g = nbl.Grid(n, n, 1, 1.0 / n, 1.0 / n, 1.0 / n, faces=faces)
+    cfg = nbl.flow_cfg(reynolds=reynolds, ref_length=1.0, velocity_scale=lid, cfl=0.8,
+                       convection=nbl.CONV_CENTRAL)
+    return g, cfg
+
+
+def channel(nx: int, ny: int, reynolds: float, bulk: float = 1.0):
+    ly, lx = 1.0, 6.0
+    faces = _faces(nbl.BC_INLET, nbl.BC_OUTLET, nbl.BC_WALL, nbl.BC_WALL,
+                   nbl.BC_PERIODIC, nbl.BC_PERIODIC)
+    faces[0].update(profile=nbl.INLET_PARABOLIC, inlet_speed=bulk, inlet_axis=0)
+    g = nbl.Grid(nx, ny, 1, lx / nx, ly / ny, ly / ny, faces=faces)
+    cfg = nbl.flow_cfg(reynolds=reynolds, ref_length=ly, velocity_scale=bulk, cfl=0.8,
+                       convection=nbl.CONV_CENTRAL)
+    return g, cfg
+
+
+def cube_solid(nx, ny, nz, dx, dy, dz, x0, x1, y0, y1, z0, z1):
+    i = (np.arange(nx) + 0.5) * dx
+    j = (np.arange(ny) + 0.5) * dy
+    k = (np.arange(nz) + 0.5) * dz
+    X, Y, Z = np.meshgrid(i, j, k, indexing="ij")
+    s = (X >= x0) & (X <= x1) & (Y >= y0) & (Y <= y1) & (Z >= z0) & (Z <= z1)
+    return np.ascontiguousarray(s.transpose(2, 1, 0)).astype(np.uint8).reshape(-1)
+
+
+def wall_mounted_cube(res: int, reynolds: float):
+    h = 1.0
+    dx = h / res
+    nx, ny, nz = round(14.0 * res), round(3.0 * res), round(6.4 * res)
+    faces = _faces(nbl.BC_INLET, nbl.BC_OUTLET, nbl.BC_WALL, nbl.BC_WALL,
+                   nbl.BC_PERIODIC, nbl.BC_PERIODIC)
+    faces[0].update(profile=nbl.INLET_UNIFORM, inlet_speed=1.0, inlet_axis=0)
+    zc = 3.2 * h
+    solid = cube_solid(nx, ny, nz, dx, dx, dx, 3.0 * h, 4.0 * h, 0.0, h, zc - 0.5 * h, zc + 0.5 * h)
+    g = nbl.Grid(nx, ny, nz, dx, faces=faces, solid=solid)
+    cfg = nbl.flow_cfg(reynolds=reynolds, ref_length=h, velocity_scale=1.0, ref_area=h * h,
+                       cfl=0.7, convection=nbl.CONV_WENO5)
+    return g, cfg
+
+
+# ---- measured workloads -----------------------------------------------------
+def taylor_green(n: int = 64, reynolds: float = 1600.0, tol: float = 1e-8):
+    L = 2.0 * math.pi
+    h = L / n
+    g = nbl.Grid(n, n, n, h, faces=_faces(*([nbl.BC_PERIODIC] * 6)))
+    cfg = nbl.flow_cfg(reynolds=reynolds, ref_length=1.0, velocity_scale=1.0, cfl=0.5,
+                       convection=nbl.CONV_WENO5, poisson_tol=tol, poisson_max_iters=5000)
+    return g, cfg
+
+
+def taylor_green_state(g: nbl.Grid):
+    """u = sin x cos y cos z on x-faces, v = -cos x sin y cos z on y-faces, w = p = 0.
+    The duplicated periodic faces (i = nx, j = ny) equal their i = 0 / j = 0 copies."""
+    nx, ny, nz, h = g.nx, g.ny, g.nz, g.dx
+    xf = np.arange(nx + 1) * h
+    xc = (np.arange(nx) + 0.5) * h
+    yf = np.arange(ny + 1) * h
+    yc = (np.arange(ny) + 0.5) * h
+    zc = (np.arange(nz) + 0.5) * h
+    Z, Y, X = np.meshgrid(zc, yc, xf, indexing="ij")
+    u = np.sin(X) * np.cos(Y) * np.cos(Z)
+    u[:, :, nx] = u[:, :, 0]
+    Z, Y, X = np.meshgrid(zc, yf, xc, indexing="ij")
+    v = -np.cos(X) * np.sin(Y) * np.cos(Z)
+    v[:, ny, :] = v[:, 0, :]
+    w = np.zeros(g.shapes[2])
+    p = np.zeros(g.shapes[3])
+    return u.reshape(-1).copy(), v.reshape(-1).copy(), w, p
+
+
+def cube_channel(nx=192, ny=64, nz=64, res=16, reynolds=500.0, tol=1e-8):
+    h = 1.0
+    dx = h / res
+    faces = _faces(nbl.BC_INLET, nbl.BC_OUTLET, nbl.BC_WALL, nbl.BC_WALL,
+                   nbl.BC_PERIODIC, nbl.BC_PERIODIC)
+    faces[0].update(profile=nbl.INLET_UNIFORM, inlet_speed=1.0, inlet_axis=0)
+    solid = cube_solid(nx, ny, nz, dx, dx, dx, 3.0 * h, 4.0 * h, 0.0, h,
+                       0.5 * nz * dx - 0.5 * h, 0.5 * nz * dx + 0.5 * h)
+    g = nbl.Grid(nx, ny, nz, dx, faces=faces, solid=solid)
+    cfg = nbl.flow_cfg(reynolds=reynolds, ref_length=h, velocity_scale=1.0, ref_area=h * h,
+                       cfl=0.7, convection=nbl.CONV_WENO5, poisson_tol=tol,
+                       poisson_max_iters=5000)
+    return g, cfg
+
+
+def cube_channel_state(g: nbl.Grid, seed: int = 20240601, amp: float = 0.01):
+    """Initial state: u = 1 + amp*xi on interior x-faces (xi ~ U(-1,1)), inlet
+    faces 1, solid-touching faces 0, v = w = p = 0."""
+    nx, ny, nz = g.nx, g.ny, g.nz
+    u = np.ones((nz, ny, nx + 1))
+    xi = 2.0 * uniform01(seed, nz * ny * (nx - 1)).reshape(nz, ny, nx - 1) - 1.0
+    u[:, :, 1:nx] += amp * xi
+    s = g.solid.reshape(nz, ny, nx).astype(bool)
+    touch = np.zeros((nz, ny, nx + 1), bool)
+    touch[:, :, :nx] |= s
+    touch[:, :, 1:] |= s
+    u[touch] = 0.0
+    u[:, :, nx] = u[:, :, nx - 1]
+    v = np.zeros(g.shapes[1])
+    w = np.zeros(g.shapes[2])
+    p = np.zeros(g.shapes[3])
+    return u.reshape(-1).copy(), v, w, p
+
+
+def kinetic_energy_enstrophy(g: nbl.Grid, u, v, w):
+    """KE = (1/2N) sum |u_c|^2 and enstrophy = (1/2N) sum |omega_c|^2 with
+    cell-centred velocities (ns_solver.cpp:677-688) and periodic central
+    differences for the vorticity."""
+    nx, ny, nz = g.nx, g.ny, g.nz
+    U = u.reshape(nz, ny, nx + 1)
+    V = v.reshape(nz, ny + 1, nx)
+    W = w.reshape(nz + 1, ny, nx)
+    uc = 0.5 * (U[:, :, :-1] + U[:, :, 1:])
+    vc = 0.5 * (V[:, :-1, :] + V[:, 1:, :])
+    wc = 0.5 * (W[:-1] + W[1:]) if nz > 1 else np.zeros((nz, ny, nx))
+    n = nx * ny * nz
+    ke = float(np.sum(uc * uc + vc * vc + wc * wc)) / (2.0 * n)
+
+    def d(f, axis, h):
+        return (np.roll(f, -1, axis) - np.roll(f, 1, axis)) / (2.0 * h)
+    ox = d(wc, 1, g.dy) - d(vc, 0, g.dz)
+    oy = d(uc, 0, g.dz) - d(wc, 2, g.dx)
+    oz = d(vc, 2, g.dx) - d(uc, 1, g.dy)
+    ens = float(np.sum(ox * ox + oy * oy + oz * oz)) / (2.0 * n)
+    return ke, ens
+
+
+def poisson_rhs(g: nbl.Grid, seed: int):
+    """RHS ~ N(0,1) i.i.d. on fluid cells, projected to zero mean."""
+    b = normal(seed, g.cells)
+    fl = g.solid == 0
+    b[~fl] = 0.0
+    b[fl] -= b[fl].mean()
+    return b

@@ -1,0 +1,425 @@
+// api.hip — the extern "C" boundary (include/nbl_api.h).  Every entry point
+// catches C++ exceptions and turns them into a status code plus a
+// thread-local message; nothing throws across the ABI.
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "nbl_api.h"
+#include "octree.hpp"
+#include "solver.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return NBL_OK;
+  } catch (const std::invalid_argument& e) {
+    g_err = e.what();
+    return NBL_ERR_INVALID;
+  } catch (const nbl::HipError& e) {
+    g_err = e.what();
+    return NBL_ERR_DEVICE;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return NBL_ERR_CAPACITY;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return NBL_ERR_STATE;
+  } catch (...) {
+    g_err = "unknown error";
+    return NBL_ERR_STATE;
+  }
+}
+
+void need(bool ok, const char* what) {
+  if (!ok) throw std::invalid_argument(what);
+}
+
+void useDevice(nbl_ctx* ctx) { NBL_HIP(hipSetDevice(ctx->device)); }
+
+nbl_solver_opts optsOr(const nbl_solver_opts* o) {
+  nbl_solver_opts d;
+  nbl_solver_opts_default(&d);
+  return o ? *o : d;
+}
+
+}  // namespace
+
+extern "C" {
+
+void nbl_flow_cfg_default(nbl_flow_cfg* c) {  // FlowConfig{} (ns_solver.hpp:14-32)
+  if (!c) return;
+  c->reynolds = 100.0;
+  c->ref_length = 1.0;
+  c->velocity_scale = 1.0;
+  c->density = 1.0;
+  c->cfl = 0.8;
+  c->body_force[0] = c->body_force[1] = c->body_force[2] = 0.0;
+  c->convection = NBL_CONV_CENTRAL;
+  c->poisson_tol = 1e-9;
+  c->poisson_max_iters = 5000;
+  c->tol_continuity = 1e-7;
+  c->tol_mass = 1e-8;
+  c->ref_area = 0.0;
+}
+
+void nbl_solver_opts_default(nbl_solver_opts* o) {
+  if (!o) return;
+  o->precision = NBL_PREC_FP64;
+  o->pre_smooth = 2;
+  o->post_smooth = 2;
+  o->check_every = 1;
+}
+
+const char* nbl_last_error(void) { return g_err.c_str(); }
+const char* nbl_version(void) { return "nabla-mi355x 0.1.0 (gfx950)"; }
+
+int nbl_ctx_create(int device, nbl_ctx** out) {
+  return guarded([&] {
+    need(out != nullptr, "out is NULL");
+    int n = 0;
+    NBL_HIP(hipGetDeviceCount(&n));
+    need(device >= 0 && device < n, "device index out of range");
+    NBL_HIP(hipSetDevice(device));
+    auto* c = new nbl_ctx();
+    c->device = device;
+    *out = c;
+  });
+}
+
+int nbl_ctx_destroy(nbl_ctx* ctx) {
+  return guarded([&] { delete ctx; });
+}
+
+int nbl_ctx_bytes_in_use(nbl_ctx* ctx, uint64_t* bytes) {
+  return guarded([&] {
+    need(ctx && bytes, "NULL argument");
+    *bytes = ctx->bytes.load();
+  });
+}
+
+// ---- runs -----------------------------------------------------------------
+int nbl_run_create(nbl_ctx* ctx, const nbl_grid_desc* grid, const nbl_flow_cfg* cfg,
+                   const nbl_solver_opts* opts, nbl_run** out) {
+  return guarded([&] {
+    need(ctx && grid && cfg && out, "NULL argument");
+    useDevice(ctx);
+    auto r = std::make_unique<nbl_run>();
+    r->ctx = ctx;
+    r->impl = std::make_unique<nbl::RunDev>(ctx, nbl::HostGrid::from(grid), *cfg, optsOr(opts));
+    *out = r.release();
+  });
+}
+
+int nbl_run_destroy(nbl_run* run) {
+  return guarded([&] { delete run; });
+}
+
+int nbl_run_initialize(nbl_run* run) {
+  return guarded([&] {
+    need(run, "run is NULL");
+    useDevice(run->ctx);
+    run->impl->initialize();
+  });
+}
+
+int nbl_run_set_state(nbl_run* run, const double* u, size_t nu, const double* v, size_t nv, const double* w,
+                      size_t nw, const double* p, size_t np, int step, double t, double dt_last) {
+  return guarded([&] {
+    need(run, "run is NULL");
+    useDevice(run->ctx);
+    run->impl->setState(u, nu, v, nv, w, nw, p, np, step, t, dt_last);
+  });
+}
+
+int nbl_run_get_state(nbl_run* run, double* u, double* v, double* w, double* p, int* step, double* t,
+                      double* dt_last) {
+  return guarded([&] {
+    need(run, "run is NULL");
+    useDevice(run->ctx);
+    run->impl->getState(u, v, w, p, step, t, dt_last);
+  });
+}
+
+int nbl_run_compute_dt(nbl_run* run, double* dt) {
+  return guarded([&] {
+    need(run && dt, "NULL argument");
+    useDevice(run->ctx);
+    *dt = run->impl->computeDt();
+  });
+}
+
+int nbl_run_step(nbl_run* run, double dt, nbl_step_report* rep) {
+  return guarded([&] {
+    need(run, "run is NULL");
+    useDevice(run->ctx);
+    const nbl_step_report r = run->impl->step(dt);
+    if (rep) *rep = r;
+  });
+}
+
+int nbl_run_step_many(nbl_run* run, int nsteps, nbl_step_report* reps) {
+  return guarded([&] {
+    need(run, "run is NULL");
+    need(nsteps >= 0, "nsteps must be >= 0");
+    useDevice(run->ctx);
+    for (int s = 0; s < nsteps; ++s) {
+      const nbl_step_report r = run->impl->step(0.0);
+      if (reps) reps[s] = r;
+    }
+  });
+}
+
+int nbl_run_device_state(nbl_run* run, double** u, double** v, double** w, double** p) {
+  return guarded([&] {
+    need(run, "run is NULL");
+    if (u) *u = run->impl->u();
+    if (v) *v = run->impl->v();
+    if (w) *w = run->impl->w();
+    if (p) *p = run->impl->p();
+  });
+}
+
+int nbl_run_rhs(nbl_run* run, double* ru, double* rv, double* rw) {
+  return guarded([&] {
+    need(run && ru && rv && rw, "NULL argument");
+    useDevice(run->ctx);
+    run->impl->rhs(ru, rv, rw);
+  });
+}
+
+int nbl_run_cell_velocity(nbl_run* run, int component, double* out) {
+  return guarded([&] {
+    need(run && out, "NULL argument");
+    useDevice(run->ctx);
+    run->impl->cellVelocity(component, out);
+  });
+}
+
+// ---- pressure plugin ---------------------------------------------------------
+int nbl_poisson_create(nbl_ctx* ctx, const nbl_grid_desc* grid, int kind, double tol, int max_iters,
+                       const nbl_solver_opts* opts, nbl_poisson** out) {
+  return guarded([&] {
+    need(ctx && grid && out, "NULL argument");
+    useDevice(ctx);
+    auto ps = std::make_unique<nbl_poisson>();
+    ps->ctx = ctx;
+    ps->grid = nbl::HostGrid::from(grid);
+    ps->impl = std::make_unique<nbl::PoissonDev>(ctx, ps->grid, kind, tol, max_iters, optsOr(opts));
+    NBL_HIP(hipStreamCreate(&ps->stream));
+    *out = ps.release();
+  });
+}
+
+int nbl_poisson_destroy(nbl_poisson* ps) {
+  return guarded([&] {
+    if (!ps) return;
+    if (ps->stream) {
+      (void)hipStreamSynchronize(ps->stream);
+      (void)hipStreamDestroy(ps->stream);
+    }
+    delete ps;
+  });
+}
+
+namespace {
+struct Staging {  // device copies of host vectors for the host-buffer entry points
+  nbl::DevBuf a, b;
+  Staging(nbl_ctx* ctx, size_t n) : a(ctx, n * 8), b(ctx, n * 8) {}
+};
+}  // namespace
+
+int nbl_poisson_solve(nbl_poisson* ps, const double* b, double* x, nbl_poisson_report* rep) {
+  return guarded([&] {
+    need(ps && b && x, "NULL argument");
+    useDevice(ps->ctx);
+    const size_t n = ps->grid.cells();
+    Staging st(ps->ctx, n);
+    NBL_HIP(hipMemcpyAsync(st.a.as<void>(), b, n * 8, hipMemcpyHostToDevice, ps->stream));
+    const nbl_poisson_report r = ps->impl->solve(st.a.as<double>(), st.b.as<double>(), ps->stream);
+    NBL_HIP(hipMemcpyAsync(x, st.b.as<void>(), n * 8, hipMemcpyDeviceToHost, ps->stream));
+    NBL_HIP(hipStreamSynchronize(ps->stream));
+    if (rep) *rep = r;
+  });
+}
+
+int nbl_poisson_solve_device(nbl_poisson* ps, const double* d_b, double* d_x, nbl_poisson_report* rep) {
+  return guarded([&] {
+    need(ps && d_b && d_x, "NULL argument");
+    useDevice(ps->ctx);
+    const nbl_poisson_report r = ps->impl->solve(d_b, d_x, ps->stream);
+    NBL_HIP(hipStreamSynchronize(ps->stream));
+    if (rep) *rep = r;
+  });
+}
+
+int nbl_poisson_apply(nbl_poisson* ps, const double* x, double* out) {
+  return guarded([&] {
+    need(ps && x && out, "NULL argument");
+    useDevice(ps->ctx);
+    const size_t n = ps->grid.cells();
+    Staging st(ps->ctx, n);
+    NBL_HIP(hipMemcpyAsync(st.a.as<void>(), x, n * 8, hipMemcpyHostToDevice, ps->stream));
+    nbl::launch_apply(ps->impl->level(0), st.a.as<double>(), st.b.as<double>(), ps->stream);
+    NBL_HIP(hipMemcpyAsync(out, st.b.as<void>(), n * 8, hipMemcpyDeviceToHost, ps->stream));
+    NBL_HIP(hipStreamSynchronize(ps->stream));
+  });
+}
+
+int nbl_poisson_diagonal(nbl_poisson* ps, double* diag) {
+  return guarded([&] {
+    need(ps && diag, "NULL argument");
+    useDevice(ps->ctx);
+    const size_t n = ps->grid.cells();
+    nbl::DevBuf d(ps->ctx, n * 8);
+    nbl::launch_diag(ps->impl->level(0), d.as<double>(), ps->stream);
+    NBL_HIP(hipMemcpyAsync(diag, d.as<void>(), n * 8, hipMemcpyDeviceToHost, ps->stream));
+    NBL_HIP(hipStreamSynchronize(ps->stream));
+  });
+}
+
+int nbl_poisson_levels(nbl_poisson* ps, int* nlevels, int* dims) {
+  return guarded([&] {
+    need(ps && nlevels, "NULL argument");
+    *nlevels = ps->impl->nlevels();
+    if (dims)
+      for (int l = 0; l < ps->impl->nlevels(); ++l) {
+        const nbl::MgLevel& L = ps->impl->level(l);
+        dims[3 * l] = L.nx;
+        dims[3 * l + 1] = L.ny;
+        dims[3 * l + 2] = L.nz;
+      }
+  });
+}
+
+int nbl_poisson_smooth_once(nbl_poisson* ps, const double* b, double* x) {
+  return guarded([&] {
+    need(ps && b && x, "NULL argument");
+    need(ps->impl->kind() == 0, "smoother exists for the mg-pcg solver only");
+    useDevice(ps->ctx);
+    const size_t n = ps->grid.cells();
+    Staging st(ps->ctx, n);
+    nbl::DevBuf xo(ps->ctx, n * 8);
+    NBL_HIP(hipMemcpyAsync(st.a.as<void>(), b, n * 8, hipMemcpyHostToDevice, ps->stream));
+    NBL_HIP(hipMemcpyAsync(st.b.as<void>(), x, n * 8, hipMemcpyHostToDevice, ps->stream));
+    nbl::launch_mg_sweep<double>(ps->impl->level(0), st.a.as<double>(), nullptr, st.b.as<double>(), nullptr, false,
+                                 nullptr, nullptr, nullptr, xo.as<double>(), nullptr, ps->impl->red(), nullptr,
+                                 ps->stream);
+    NBL_HIP(hipMemcpyAsync(x, xo.as<void>(), n * 8, hipMemcpyDeviceToHost, ps->stream));
+    NBL_HIP(hipStreamSynchronize(ps->stream));
+  });
+}
+
+int nbl_poisson_restrict(nbl_poisson* ps, const double* rf, double* rc) {
+  return guarded([&] {
+    need(ps && rf && rc, "NULL argument");
+    need(ps->impl->nlevels() >= 2, "grid has a single multigrid level");
+    useDevice(ps->ctx);
+    const nbl::MgLevel& L = ps->impl->level(0);
+    const nbl::MgLevel& Lc = ps->impl->level(1);
+    nbl::DevBuf a(ps->ctx, size_t(L.n) * 8), c(ps->ctx, size_t(Lc.n) * 8);
+    NBL_HIP(hipMemcpyAsync(a.as<void>(), rf, size_t(L.n) * 8, hipMemcpyHostToDevice, ps->stream));
+    nbl::launch_restrict_plain(L, Lc, a.as<double>(), c.as<double>(), ps->stream);
+    NBL_HIP(hipMemcpyAsync(rc, c.as<void>(), size_t(Lc.n) * 8, hipMemcpyDeviceToHost, ps->stream));
+    NBL_HIP(hipStreamSynchronize(ps->stream));
+  });
+}
+
+int nbl_poisson_prolong(nbl_poisson* ps, const double* ec, double* xf) {
+  return guarded([&] {
+    need(ps && ec && xf, "NULL argument");
+    need(ps->impl->nlevels() >= 2, "grid has a single multigrid level");
+    useDevice(ps->ctx);
+    const nbl::MgLevel& L = ps->impl->level(0);
+    const nbl::MgLevel& Lc = ps->impl->level(1);
+    nbl::DevBuf a(ps->ctx, size_t(L.n) * 8), c(ps->ctx, size_t(Lc.n) * 8);
+    NBL_HIP(hipMemcpyAsync(a.as<void>(), xf, size_t(L.n) * 8, hipMemcpyHostToDevice, ps->stream));
+    NBL_HIP(hipMemcpyAsync(c.as<void>(), ec, size_t(Lc.n) * 8, hipMemcpyHostToDevice, ps->stream));
+    nbl::launch_prolong_plain(L, Lc, c.as<double>(), a.as<double>(), ps->stream);
+    NBL_HIP(hipMemcpyAsync(xf, a.as<void>(), size_t(L.n) * 8, hipMemcpyDeviceToHost, ps->stream));
+    NBL_HIP(hipStreamSynchronize(ps->stream));
+  });
+}
+
+int nbl_poisson_last_timing(nbl_poisson* ps, double* ms, int* vcycles) {
+  return guarded([&] {
+    need(ps, "NULL argument");
+    if (ms) *ms = ps->impl->lastMs();
+    if (vcycles) *vcycles = ps->impl->lastVcycles();
+  });
+}
+
+// ---- octree ---------------------------------------------------------------------
+int nbl_octree_create(nbl_ctx* ctx, const uint64_t* keys, const uint8_t* levels, const uint8_t* masks, size_t n,
+                      const double origin[3], const double extent[3], int max_level, nbl_octree** out) {
+  return guarded([&] {
+    need(ctx && keys && levels && masks && origin && extent && out, "NULL argument");
+    useDevice(ctx);
+    auto o = std::make_unique<nbl_octree>();
+    o->ctx = ctx;
+    o->impl = std::make_unique<nbl::OctreeDev>(ctx, keys, levels, masks, n, origin, extent, max_level);
+    *out = o.release();
+  });
+}
+
+int nbl_octree_destroy(nbl_octree* oc) {
+  return guarded([&] { delete oc; });
+}
+
+int nbl_octree_count(nbl_octree* oc, size_t* n, size_t* nnz) {
+  return guarded([&] {
+    need(oc, "octree is NULL");
+    if (n) *n = oc->impl->size();
+    if (nnz) *nnz = oc->impl->nnz();
+  });
+}
+
+int nbl_octree_permutation(nbl_octree* oc, uint64_t* perm) {
+  return guarded([&] {
+    need(oc && perm, "NULL argument");
+    useDevice(oc->ctx);
+    oc->impl->permutation(perm);
+  });
+}
+
+int nbl_octree_face_table(nbl_octree* oc, uint8_t* count, int32_t* nbr) {
+  return guarded([&] {
+    need(oc, "octree is NULL");
+    useDevice(oc->ctx);
+    oc->impl->faceTable(count, nbr);
+  });
+}
+
+int nbl_octree_apply(nbl_octree* oc, const double* x, double* out) {
+  return guarded([&] {
+    need(oc && x && out, "NULL argument");
+    useDevice(oc->ctx);
+    oc->impl->apply(x, out);
+  });
+}
+
+int nbl_octree_diagonal(nbl_octree* oc, double* diag) {
+  return guarded([&] {
+    need(oc && diag, "NULL argument");
+    useDevice(oc->ctx);
+    oc->impl->diagonal(diag);
+  });
+}
+
+int nbl_octree_poisson_solve(nbl_octree* oc, const double* b, double* x, double tol, int max_iters,
+                             nbl_poisson_report* rep) {
+  return guarded([&] {
+    need(oc && b && x, "NULL argument");
+    useDevice(oc->ctx);
+    const nbl_poisson_report r = oc->impl->solve(b, x, tol, max_iters);
+    if (rep) *rep = r;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,153 @@
+// dev.hpp — POD descriptors shared by host code and gfx950 kernels, and the
+// host-side launchers each kernel file exports.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace nbl {
+
+// Uniform MAC grid as the kernels see it (reference grid.hpp:32-73).
+struct DevGrid {
+  int nx, ny, nz;
+  double dx, dy, dz;
+  double ox, oy, oz;
+  int bc[6];              // NBL_BC_* per face 0=-x .. 5=+z
+  double wall[6][3];      // wall velocity per face
+  int profile[6];         // inlet profile per face
+  double inletSpeed[6];
+  int inletAxis[6];
+  const uint8_t* solid;   // device, nx*ny*nz
+  int hasSolid;
+};
+
+// One multigrid level.  code[c]: bits 2d..2d+1 = coupling kind across face d
+// (0 none, 1 fluid neighbour, 2 outlet Dirichlet; d = 0..5 = -x,+x,-y,+y,-z,+z),
+// bit 12 = fluid cell.  The diagonal is rebuilt from the code in the same
+// accumulation order as the reference's diagOf (poisson.cpp:54-74).
+struct MgLevel {
+  int nx, ny, nz;
+  long long n;
+  double h2[3];           // 1/dx^2, 1/dy^2, 1/dz^2
+  const uint16_t* code;
+  int fx, fy, fz;         // factor from the next-finer level (coarse levels)
+  long long nf;           // fluid cells
+};
+
+constexpr uint16_t kFluidBit = 1u << 12;
+
+__host__ __device__ inline int kindOf(uint16_t code, int dir) { return (code >> (2 * dir)) & 3; }
+
+// Scalars of one flexible-CG / CG solve, resident on the device.  Kernels of
+// an iteration read `done` first and do nothing once the solve has stopped,
+// so the host may enqueue iterations ahead of the convergence test.
+struct SolveCtl {
+  double tol, bnorm, rz, pAp, alpha, beta, rr, resid;
+  int it, done, converged, broke;
+  int vcycles, pad_;
+  double slot[64];  // reduction results / pending means
+};
+
+// Per-run time-step scalars (NSSolver step state).
+struct StepCtl {
+  double dt;       // dt of the current step (CFL or fixed)
+  double t;
+  int step;
+  int pad_;
+  double cfl_max;  // max inverse CFL time scale found by the last CFL pass
+};
+
+// Step report fields computed on the device (mirrors nbl_step_report).
+struct DevReport {
+  int step;
+  double t, dt, cfl, mom, divl2, divmax, mass;
+  int piters;
+  double pres, cd, cl;
+  unsigned long long cells;
+  int accepted;
+};
+
+// ---- launchers (ns.hip) ----------------------------------------------------
+struct NsArgs {
+  DevGrid g;
+  double nu;
+  double bodyForce[3];
+  int convection;
+  double cfl, velocityScale, density, refLength, refArea, tolContinuity, tolMass;
+};
+
+void launch_bc_planes(const NsArgs& a, double* u, double* v, double* w, bool outflow, hipStream_t s);
+void launch_bc_solid(const NsArgs& a, double* u, double* v, double* w, const int* list,
+                     const int counts[3], hipStream_t s);
+void launch_rhs_rk(const NsArgs& a, const double* u0, const double* v0, const double* w0,
+                   const double* u, const double* v, const double* w, double* uo, double* vo,
+                   double* wo, int stage, const StepCtl* sc, hipStream_t s);
+void launch_divergence(const NsArgs& a, const double* u, const double* v, const double* w,
+                       double* b, hipStream_t s);
+void launch_correct(const NsArgs& a, double* u, double* v, double* w, const double* phi,
+                    double* p, const StepCtl* sc, hipStream_t s);
+void launch_cfl(const NsArgs& a, const double* u, const double* v, const double* w, StepCtl* sc,
+                double fixed_dt, RedBuf rb, hipStream_t s);
+void launch_diagnostics(const NsArgs& a, const double* u, const double* v, const double* w,
+                        const double* u0, const double* v0, const double* w0, const double* p,
+                        StepCtl* sc, DevReport* out, RedBuf rb, hipStream_t s);
+void launch_cell_velocity(const NsArgs& a, const double* u, const double* v, const double* w,
+                          int comp, double* out, hipStream_t s);
+void launch_fill(double* a, long long n, double v, hipStream_t s);
+
+// ---- launchers (mg.hip) ----------------------------------------------------
+// Multigrid pieces are templated on the storage/arithmetic type T (double for
+// the plain path, float for the mixed V-cycle).
+template <class T>
+void launch_mg_sweep(const MgLevel& L, const T* b, const double* mb, const T* x, const double* mx,
+                     bool zeroGuess, const T* ec, const double* mec, const MgLevel* Lc, T* xo,
+                     double* meanOut, RedBuf rb, const SolveCtl* ctl, hipStream_t s);
+template <class T>
+void launch_mg_resid_restrict(const MgLevel& L, const MgLevel& Lc, const T* b, const double* mb,
+                              const T* x, const double* mx, T* rc, double* meanOut, RedBuf rb,
+                              const SolveCtl* ctl, hipStream_t s);
+template <class T>
+void launch_mg_coarse(const MgLevel& L, bool singular, const T* b, const double* mb, T* x,
+                      T* scratch, const SolveCtl* ctl, hipStream_t s);
+template <class T>
+void launch_mg_finish(const MgLevel& L, const T* x, const double* mx, double* z, double* meanOut,
+                      RedBuf rb, const SolveCtl* ctl, hipStream_t s);
+template <class T>
+void launch_to_level0(const MgLevel& L, const double* r, T* b, const SolveCtl* ctl, hipStream_t s);
+
+void launch_apply(const MgLevel& L, const double* x, double* out, hipStream_t s);
+void launch_diag(const MgLevel& L, double* out, hipStream_t s);
+void launch_restrict_plain(const MgLevel& L, const MgLevel& Lc, const double* rf, double* rc,
+                           hipStream_t s);
+void launch_prolong_plain(const MgLevel& L, const MgLevel& Lc, const double* ec, double* xf,
+                          hipStream_t s);
+
+// Flexible CG (MgPoisson::solve) and Jacobi CG (CgPoisson::solve) pieces.
+void launch_fcg_init(const MgLevel& L, bool singular, bool maskSolid, const double* b, double* rhs,
+                     double* x, SolveCtl* ctl, RedBuf rb, hipStream_t s);
+void launch_fcg_init2(const MgLevel& L, bool singular, bool earlyExit, double* rhs, double* r,
+                      SolveCtl* ctl, RedBuf rb, hipStream_t s);
+void launch_fcg_start(const MgLevel& L, const double* z, const double* mz, const double* r,
+                      double* zm, double* p, SolveCtl* ctl, RedBuf rb, hipStream_t s);
+void launch_fcg_apply(const MgLevel& L, const double* p, double* Ap, SolveCtl* ctl, RedBuf rb,
+                      hipStream_t s);
+void launch_fcg_update(const MgLevel& L, bool singular, double* x, const double* p,
+                       const double* r, const double* Ap, double* rn, SolveCtl* ctl, RedBuf rb,
+                       hipStream_t s);
+void launch_fcg_norm(const MgLevel& L, double* rn, SolveCtl* ctl, RedBuf rb, hipStream_t s);
+void launch_fcg_dots(const MgLevel& L, double* z, const double* mz, const double* rn,
+                     const double* r, SolveCtl* ctl, RedBuf rb, hipStream_t s);
+void launch_fcg_pupdate(const MgLevel& L, double* p, const double* z, double* r,
+                        const double* rn, SolveCtl* ctl, hipStream_t s);
+void launch_fcg_final(const MgLevel& L, bool singular, const double* x, const double* rhs,
+                      double* xo, SolveCtl* ctl, RedBuf rb, double convTol, hipStream_t s);
+void launch_cg_z(const MgLevel& L, const double* r, double* z, SolveCtl* ctl, RedBuf rb,
+                 bool first, double* p, hipStream_t s);
+void launch_cg_update(const MgLevel& L, double* x, const double* p, double* r, const double* Ap,
+                      SolveCtl* ctl, RedBuf rb, hipStream_t s);
+void launch_cg_pupdate(const MgLevel& L, double* p, const double* z, SolveCtl* ctl, hipStream_t s);
+
+}  // namespace nbl

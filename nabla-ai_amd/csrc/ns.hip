@@ -1,0 +1,796 @@
+// ns.hip — gfx950 kernels of the MAC time step (reference core/src/flow/ns_solver.cpp).
+//
+// Layout in HBM: the reference's own face arrays (u: (nx+1)*ny*nz, v:
+// nx*(ny+1)*nz, w: nx*ny*(nz+1), compact index i fastest) and compact cell
+// arrays.  Every kernel is a streaming pass over one flattened index space
+// (faces of u|v|w, optionally followed by cells), one element per thread, so
+// loads along i are coalesced.  Arithmetic follows the reference expression
+// order, and the library is built with -ffp-contract=off, so per-element
+// results are bit-identical to the reference; only reductions (CFL max,
+// diagnostics sums) use a different (deterministic, tree) order.
+#include "dev.hpp"
+#include "nbl_api.h"
+
+namespace nbl {
+namespace {
+
+struct Idx {  // staggered index helpers (ns_solver.hpp:93-110)
+  int nx, ny, nz;
+  __device__ long long u(int i, int j, int k) const { return i + (long long)(nx + 1) * (j + (long long)ny * k); }
+  __device__ long long v(int i, int j, int k) const { return i + (long long)nx * (j + (long long)(ny + 1) * k); }
+  __device__ long long w(int i, int j, int k) const { return i + (long long)nx * (j + (long long)ny * k); }
+  __device__ long long c(int i, int j, int k) const { return i + (long long)nx * (j + (long long)ny * k); }
+};
+
+__device__ __forceinline__ int wrapi(int a, int n) { return (a % n + n) % n; }
+
+__device__ __forceinline__ bool solidAt(const DevGrid& g, int i, int j, int k) {
+  if (i < 0 || j < 0 || k < 0 || i >= g.nx || j >= g.ny || k >= g.nz) return false;
+  return g.solid[i + (long long)g.nx * (j + (long long)g.ny * k)] != 0;
+}
+
+// Inlet velocity component `comp` on face f at transverse index a (used as the
+// y index by the parabolic profile), ns_solver.cpp:71-93.
+__device__ double inletComp(const DevGrid& g, int f, int a, int comp) {
+  double mag = g.inletSpeed[f];
+  if (g.profile[f] == NBL_INLET_PARABOLIC) {
+    const double H = g.ny * g.dy;
+    const double yc = g.oy + 0.5 * H;
+    const double y = g.oy + (a + 0.5) * g.dy;
+    const double r = (y - yc) / (0.5 * H);
+    mag = 1.5 * g.inletSpeed[f] * (1.0 - r * r);
+  }
+  const int ax = g.inletAxis[f] == 0 ? 0 : (g.inletAxis[f] == 1 ? 1 : 2);
+  return ax == comp ? mag : 0.0;
+}
+
+// Reflections pending while a ghost index is resolved: the reference evaluates
+// `2*X - inner` recursively, so they are applied innermost first.
+struct Refl {
+  double X[3];
+  int d = 0;
+  __device__ bool push(double x) {
+    if (d >= 3) return false;
+    X[d++] = x;
+    return true;
+  }
+  __device__ double fold(double val) const {
+    for (int q = d - 1; q >= 0; --q) val = 2.0 * X[q] - val;
+    return val;
+  }
+};
+
+// Ghost-aware reads (ns_solver.cpp:116-219), resolved iteratively.
+__device__ double ghostU(const DevGrid& g, const double* __restrict__ u, int i, int j, int k) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const Idx I{nx, ny, nz};
+  Refl R;
+  for (int guard = 0; guard < 12; ++guard) {
+    if (k < 0 || k >= nz) {
+      if (g.bc[4] == NBL_BC_PERIODIC) { k = wrapi(k, nz); continue; }
+      if (!R.push(g.wall[k < 0 ? 4 : 5][0])) break;
+      k = k < 0 ? -1 - k : 2 * nz - 1 - k;
+      continue;
+    }
+    if (j < 0 || j >= ny) {
+      if (g.bc[2] == NBL_BC_PERIODIC) { j = wrapi(j, ny); continue; }
+      if (!R.push(g.wall[j < 0 ? 2 : 3][0])) break;
+      j = j < 0 ? -1 - j : 2 * ny - 1 - j;
+      continue;
+    }
+    if (i < 0 || i > nx) {
+      if (g.bc[0] == NBL_BC_PERIODIC) { i = wrapi(i, nx); continue; }
+      const int ib = i < 0 ? 0 : nx;
+      if (g.bc[i < 0 ? 0 : 1] == NBL_BC_OUTLET) return R.fold(u[I.u(ib, j, k)]);
+      if (!R.push(u[I.u(ib, j, k)])) break;
+      i = 2 * ib - i;
+      continue;
+    }
+    return R.fold(u[I.u(i, j, k)]);
+  }
+  return 0.0;
+}
+
+__device__ double ghostV(const DevGrid& g, const double* __restrict__ v, int i, int j, int k) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const Idx I{nx, ny, nz};
+  Refl R;
+  for (int guard = 0; guard < 12; ++guard) {
+    if (k < 0 || k >= nz) {
+      if (g.bc[4] == NBL_BC_PERIODIC) { k = wrapi(k, nz); continue; }
+      if (!R.push(g.wall[k < 0 ? 4 : 5][1])) break;
+      k = k < 0 ? -1 - k : 2 * nz - 1 - k;
+      continue;
+    }
+    if (i < 0 || i >= nx) {
+      if (g.bc[0] == NBL_BC_PERIODIC) { i = wrapi(i, nx); continue; }
+      const int f = i < 0 ? 0 : 1;
+      const int im = i < 0 ? -1 - i : 2 * nx - 1 - i;
+      if (g.bc[f] == NBL_BC_INLET) {
+        if (!R.push(inletComp(g, f, j, 1))) break;
+        i = im;
+        continue;
+      }
+      if (g.bc[f] == NBL_BC_OUTLET) return R.fold(v[I.v(i < 0 ? 0 : nx - 1, j, k)]);
+      if (!R.push(g.wall[f][1])) break;
+      i = im;
+      continue;
+    }
+    if (j < 0 || j > ny) {
+      if (g.bc[2] == NBL_BC_PERIODIC) { j = wrapi(j, ny); continue; }
+      const int jb = j < 0 ? 0 : ny;
+      if (g.bc[j < 0 ? 2 : 3] == NBL_BC_OUTLET) return R.fold(v[I.v(i, jb, k)]);
+      if (!R.push(v[I.v(i, jb, k)])) break;
+      j = 2 * jb - j;
+      continue;
+    }
+    return R.fold(v[I.v(i, j, k)]);
+  }
+  return 0.0;
+}
+
+__device__ double ghostW(const DevGrid& g, const double* __restrict__ w, int i, int j, int k) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const Idx I{nx, ny, nz};
+  Refl R;
+  for (int guard = 0; guard < 12; ++guard) {
+    if (j < 0 || j >= ny) {
+      if (g.bc[2] == NBL_BC_PERIODIC) { j = wrapi(j, ny); continue; }
+      if (!R.push(g.wall[j < 0 ? 2 : 3][2])) break;
+      j = j < 0 ? -1 - j : 2 * ny - 1 - j;
+      continue;
+    }
+    if (i < 0 || i >= nx) {
+      if (g.bc[0] == NBL_BC_PERIODIC) { i = wrapi(i, nx); continue; }
+      const int f = i < 0 ? 0 : 1;
+      if (g.bc[f] == NBL_BC_OUTLET) return R.fold(w[I.w(i < 0 ? 0 : nx - 1, j, k)]);
+      if (!R.push(g.wall[f][2])) break;
+      i = i < 0 ? -1 - i : 2 * nx - 1 - i;
+      continue;
+    }
+    if (k < 0 || k > nz) {
+      if (g.bc[4] == NBL_BC_PERIODIC) { k = wrapi(k, nz); continue; }
+      const int kb = k < 0 ? 0 : nz;
+      if (!R.push(w[I.w(i, j, kb)])) break;
+      k = 2 * kb - k;
+      continue;
+    }
+    return R.fold(w[I.w(i, j, k)]);
+  }
+  return 0.0;
+}
+
+// Jiang-Shu WENO-5 at the right face of v2 (ns_solver.cpp:15-31), same
+// expression order as the reference.
+__device__ __forceinline__ double weno5(double v0, double v1, double v2, double v3, double v4) {
+  const double e = 1e-12;
+  const double s0 = 13.0 / 12.0 * (v0 - 2 * v1 + v2) * (v0 - 2 * v1 + v2) +
+                    0.25 * (v0 - 4 * v1 + 3 * v2) * (v0 - 4 * v1 + 3 * v2);
+  const double s1 = 13.0 / 12.0 * (v1 - 2 * v2 + v3) * (v1 - 2 * v2 + v3) + 0.25 * (v1 - v3) * (v1 - v3);
+  const double s2 = 13.0 / 12.0 * (v2 - 2 * v3 + v4) * (v2 - 2 * v3 + v4) +
+                    0.25 * (3 * v2 - 4 * v3 + v4) * (3 * v2 - 4 * v3 + v4);
+  const double a0 = 0.1 / ((e + s0) * (e + s0));
+  const double a1 = 0.6 / ((e + s1) * (e + s1));
+  const double a2 = 0.3 / ((e + s2) * (e + s2));
+  const double sw = a0 + a1 + a2;
+  const double q0 = (2 * v0 - 7 * v1 + 11 * v2) / 6.0;
+  const double q1 = (-v1 + 5 * v2 + 2 * v3) / 6.0;
+  const double q2 = (2 * v2 + 5 * v3 - v4) / 6.0;
+  return (a0 * q0 + a1 * q1 + a2 * q2) / sw;
+}
+
+__device__ __forceinline__ double recon(int sc, double adv, double qm2, double qm1, double q0,
+                                        double qp1, double qp2, double qp3) {
+  if (sc == NBL_CONV_CENTRAL) return 0.5 * (q0 + qp1);
+  return adv >= 0.0 ? weno5(qm2, qm1, q0, qp1, qp2) : weno5(qp3, qp2, qp1, q0, qm1);
+}
+
+// Stencil readers: Fast for stencils wholly inside the arrays, Ghost otherwise.
+struct FastRd {
+  const DevGrid* g;
+  const double *u, *v, *w;
+  Idx I;
+  __device__ double U(int i, int j, int k) const { return u[I.u(i, j, k)]; }
+  __device__ double V(int i, int j, int k) const { return v[I.v(i, j, k)]; }
+  __device__ double W(int i, int j, int k) const { return w[I.w(i, j, k)]; }
+};
+struct GhostRd {
+  const DevGrid* g;
+  const double *u, *v, *w;
+  Idx I;
+  __device__ double U(int i, int j, int k) const { return ghostU(*g, u, i, j, k); }
+  __device__ double V(int i, int j, int k) const { return ghostV(*g, v, i, j, k); }
+  __device__ double W(int i, int j, int k) const { return ghostW(*g, w, i, j, k); }
+};
+
+// u-momentum at x-face (i,j,k) (ns_solver.cpp:323-372).
+template <class Rd>
+__device__ double rhsU(const NsArgs& A, const Rd& r, int i, int j, int k) {
+  const DevGrid& g = A.g;
+  const int sc = A.convection;
+  const bool flat = g.nz == 1;
+  const double aL = 0.5 * (r.U(i - 1, j, k) + r.U(i, j, k));
+  const double aR = 0.5 * (r.U(i, j, k) + r.U(i + 1, j, k));
+  const double qL = recon(sc, aL, r.U(i - 3, j, k), r.U(i - 2, j, k), r.U(i - 1, j, k), r.U(i, j, k),
+                          r.U(i + 1, j, k), r.U(i + 2, j, k));
+  const double qR = recon(sc, aR, r.U(i - 2, j, k), r.U(i - 1, j, k), r.U(i, j, k), r.U(i + 1, j, k),
+                          r.U(i + 2, j, k), r.U(i + 3, j, k));
+  const double fx = (aR * qR - aL * qL) / g.dx;
+  const double vT = 0.5 * (r.V(i - 1, j + 1, k) + r.V(i, j + 1, k));
+  const double vB = 0.5 * (r.V(i - 1, j, k) + r.V(i, j, k));
+  const double qT = recon(sc, vT, r.U(i, j - 2, k), r.U(i, j - 1, k), r.U(i, j, k), r.U(i, j + 1, k),
+                          r.U(i, j + 2, k), r.U(i, j + 3, k));
+  const double qB = recon(sc, vB, r.U(i, j - 3, k), r.U(i, j - 2, k), r.U(i, j - 1, k), r.U(i, j, k),
+                          r.U(i, j + 1, k), r.U(i, j + 2, k));
+  const double fy = (vT * qT - vB * qB) / g.dy;
+  double fz = 0.0;
+  if (!flat) {
+    const double wT = 0.5 * (r.W(i - 1, j, k + 1) + r.W(i, j, k + 1));
+    const double wB = 0.5 * (r.W(i - 1, j, k) + r.W(i, j, k));
+    const double zT = recon(sc, wT, r.U(i, j, k - 2), r.U(i, j, k - 1), r.U(i, j, k), r.U(i, j, k + 1),
+                            r.U(i, j, k + 2), r.U(i, j, k + 3));
+    const double zB = recon(sc, wB, r.U(i, j, k - 3), r.U(i, j, k - 2), r.U(i, j, k - 1), r.U(i, j, k),
+                            r.U(i, j, k + 1), r.U(i, j, k + 2));
+    fz = (wT * zT - wB * zB) / g.dz;
+  }
+  double lap = (r.U(i + 1, j, k) - 2 * r.U(i, j, k) + r.U(i - 1, j, k)) / (g.dx * g.dx) +
+               (r.U(i, j + 1, k) - 2 * r.U(i, j, k) + r.U(i, j - 1, k)) / (g.dy * g.dy);
+  if (!flat) lap += (r.U(i, j, k + 1) - 2 * r.U(i, j, k) + r.U(i, j, k - 1)) / (g.dz * g.dz);
+  return -(fx + fy + fz) + A.nu * lap + A.bodyForce[0];
+}
+
+// v-momentum at y-face (i,j,k) (ns_solver.cpp:377-424).
+template <class Rd>
+__device__ double rhsV(const NsArgs& A, const Rd& r, int i, int j, int k) {
+  const DevGrid& g = A.g;
+  const int sc = A.convection;
+  const bool flat = g.nz == 1;
+  const double bB = 0.5 * (r.V(i, j - 1, k) + r.V(i, j, k));
+  const double bT = 0.5 * (r.V(i, j, k) + r.V(i, j + 1, k));
+  const double qB = recon(sc, bB, r.V(i, j - 3, k), r.V(i, j - 2, k), r.V(i, j - 1, k), r.V(i, j, k),
+                          r.V(i, j + 1, k), r.V(i, j + 2, k));
+  const double qT = recon(sc, bT, r.V(i, j - 2, k), r.V(i, j - 1, k), r.V(i, j, k), r.V(i, j + 1, k),
+                          r.V(i, j + 2, k), r.V(i, j + 3, k));
+  const double fy = (bT * qT - bB * qB) / g.dy;
+  const double uR = 0.5 * (r.U(i + 1, j - 1, k) + r.U(i + 1, j, k));
+  const double uL = 0.5 * (r.U(i, j - 1, k) + r.U(i, j, k));
+  const double qR = recon(sc, uR, r.V(i - 2, j, k), r.V(i - 1, j, k), r.V(i, j, k), r.V(i + 1, j, k),
+                          r.V(i + 2, j, k), r.V(i + 3, j, k));
+  const double qL = recon(sc, uL, r.V(i - 3, j, k), r.V(i - 2, j, k), r.V(i - 1, j, k), r.V(i, j, k),
+                          r.V(i + 1, j, k), r.V(i + 2, j, k));
+  const double fx = (uR * qR - uL * qL) / g.dx;
+  double fz = 0.0;
+  if (!flat) {
+    const double wR = 0.5 * (r.W(i, j - 1, k + 1) + r.W(i, j, k + 1));
+    const double wL = 0.5 * (r.W(i, j - 1, k) + r.W(i, j, k));
+    const double zR = recon(sc, wR, r.V(i, j, k - 2), r.V(i, j, k - 1), r.V(i, j, k), r.V(i, j, k + 1),
+                            r.V(i, j, k + 2), r.V(i, j, k + 3));
+    const double zL = recon(sc, wL, r.V(i, j, k - 3), r.V(i, j, k - 2), r.V(i, j, k - 1), r.V(i, j, k),
+                            r.V(i, j, k + 1), r.V(i, j, k + 2));
+    fz = (wR * zR - wL * zL) / g.dz;
+  }
+  double lap = (r.V(i + 1, j, k) - 2 * r.V(i, j, k) + r.V(i - 1, j, k)) / (g.dx * g.dx) +
+               (r.V(i, j + 1, k) - 2 * r.V(i, j, k) + r.V(i, j - 1, k)) / (g.dy * g.dy);
+  if (!flat) lap += (r.V(i, j, k + 1) - 2 * r.V(i, j, k) + r.V(i, j, k - 1)) / (g.dz * g.dz);
+  return -(fx + fy + fz) + A.nu * lap + A.bodyForce[1];
+}
+
+// w-momentum at z-face (i,j,k) (ns_solver.cpp:427-472).
+template <class Rd>
+__device__ double rhsW(const NsArgs& A, const Rd& r, int i, int j, int k) {
+  const DevGrid& g = A.g;
+  const int sc = A.convection;
+  const double cB = 0.5 * (r.W(i, j, k - 1) + r.W(i, j, k));
+  const double cT = 0.5 * (r.W(i, j, k) + r.W(i, j, k + 1));
+  const double qB = recon(sc, cB, r.W(i, j, k - 3), r.W(i, j, k - 2), r.W(i, j, k - 1), r.W(i, j, k),
+                          r.W(i, j, k + 1), r.W(i, j, k + 2));
+  const double qT = recon(sc, cT, r.W(i, j, k - 2), r.W(i, j, k - 1), r.W(i, j, k), r.W(i, j, k + 1),
+                          r.W(i, j, k + 2), r.W(i, j, k + 3));
+  const double fz = (cT * qT - cB * qB) / g.dz;
+  const double uR = 0.5 * (r.U(i + 1, j, k - 1) + r.U(i + 1, j, k));
+  const double uL = 0.5 * (r.U(i, j, k - 1) + r.U(i, j, k));
+  const double xR = recon(sc, uR, r.W(i - 2, j, k), r.W(i - 1, j, k), r.W(i, j, k), r.W(i + 1, j, k),
+                          r.W(i + 2, j, k), r.W(i + 3, j, k));
+  const double xL = recon(sc, uL, r.W(i - 3, j, k), r.W(i - 2, j, k), r.W(i - 1, j, k), r.W(i, j, k),
+                          r.W(i + 1, j, k), r.W(i + 2, j, k));
+  const double fx = (uR * xR - uL * xL) / g.dx;
+  const double vR = 0.5 * (r.V(i, j + 1, k - 1) + r.V(i, j + 1, k));
+  const double vL = 0.5 * (r.V(i, j, k - 1) + r.V(i, j, k));
+  const double yR = recon(sc, vR, r.W(i, j - 2, k), r.W(i, j - 1, k), r.W(i, j, k), r.W(i, j + 1, k),
+                          r.W(i, j + 2, k), r.W(i, j + 3, k));
+  const double yL = recon(sc, vL, r.W(i, j - 3, k), r.W(i, j - 2, k), r.W(i, j - 1, k), r.W(i, j, k),
+                          r.W(i, j + 1, k), r.W(i, j + 2, k));
+  const double fy = (vR * yR - vL * yL) / g.dy;
+  const double lap = (r.W(i + 1, j, k) - 2 * r.W(i, j, k) + r.W(i - 1, j, k)) / (g.dx * g.dx) +
+                     (r.W(i, j + 1, k) - 2 * r.W(i, j, k) + r.W(i, j - 1, k)) / (g.dy * g.dy) +
+                     (r.W(i, j, k + 1) - 2 * r.W(i, j, k) + r.W(i, j, k - 1)) / (g.dz * g.dz);
+  return -(fx + fy + fz) + A.nu * lap + A.bodyForce[2];
+}
+
+// True when every index of a +-3 stencil around (i,j,k) of component comp lies
+// inside the stored arrays (no ghost resolution needed).
+__device__ __forceinline__ bool interior(const DevGrid& g, int comp, int i, int j, int k) {
+  const bool flat = g.nz == 1;
+  const int mx = comp == 0 ? g.nx : g.nx - 1;
+  const int my = comp == 1 ? g.ny : g.ny - 1;
+  const int mz = comp == 2 ? g.nz : g.nz - 1;
+  if (i < 3 || i + 3 > mx || j < 3 || j + 3 > my) return false;
+  if (!flat && (k < 3 || k + 3 > mz)) return false;
+  return true;
+}
+
+// RHS + SSP-RK3 stage combine over u|v|w faces (ns_solver.cpp:776-794).
+__global__ void __launch_bounds__(kBlock) k_rhs_rk(NsArgs A, const double* __restrict__ u0,
+                                                   const double* __restrict__ v0,
+                                                   const double* __restrict__ w0,
+                                                   const double* __restrict__ u,
+                                                   const double* __restrict__ v,
+                                                   const double* __restrict__ w, double* __restrict__ uo,
+                                                   double* __restrict__ vo, double* __restrict__ wo,
+                                                   int stage, const StepCtl* __restrict__ sc) {
+  const DevGrid& g = A.g;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const long long nu = (long long)(nx + 1) * ny * nz, nv = (long long)nx * (ny + 1) * nz;
+  const long long nw = (long long)nx * ny * (nz + 1);
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nu + nv + nw) return;
+  const double dt = sc->dt;
+  int comp;
+  int i, j, k;
+  if (t < nu) {
+    comp = 0;
+    i = int(t % (nx + 1));
+    long long q = t / (nx + 1);
+    j = int(q % ny);
+    k = int(q / ny);
+  } else if (t < nu + nv) {
+    comp = 1;
+    t -= nu;
+    i = int(t % nx);
+    long long q = t / nx;
+    j = int(q % (ny + 1));
+    k = int(q / (ny + 1));
+  } else {
+    comp = 2;
+    t -= nu + nv;
+    i = int(t % nx);
+    long long q = t / nx;
+    j = int(q % ny);
+    k = int(q / ny);
+  }
+  double R = 0.0;
+  bool active;
+  if (comp == 0) active = i >= 1 && i < nx && !solidAt(g, i - 1, j, k) && !solidAt(g, i, j, k);
+  else if (comp == 1) active = j >= 1 && j < ny && !solidAt(g, i, j - 1, k) && !solidAt(g, i, j, k);
+  else active = nz > 1 && k >= 1 && k < nz && !solidAt(g, i, j, k - 1) && !solidAt(g, i, j, k);
+  if (active) {
+    const Idx I{nx, ny, nz};
+    if (interior(g, comp, i, j, k)) {
+      FastRd r{&g, u, v, w, I};
+      R = comp == 0 ? rhsU(A, r, i, j, k) : (comp == 1 ? rhsV(A, r, i, j, k) : rhsW(A, r, i, j, k));
+    } else {
+      GhostRd r{&g, u, v, w, I};
+      R = comp == 0 ? rhsU(A, r, i, j, k) : (comp == 1 ? rhsV(A, r, i, j, k) : rhsW(A, r, i, j, k));
+    }
+  }
+  const double* a0 = comp == 0 ? u0 : (comp == 1 ? v0 : w0);
+  const double* ac = comp == 0 ? u : (comp == 1 ? v : w);
+  double* ao = comp == 0 ? uo : (comp == 1 ? vo : wo);
+  const double x0 = a0[t], xc = ac[t];
+  double out;
+  if (stage == 0) out = R;
+  else if (stage == 1) out = x0 + dt * R;
+  else if (stage == 2) out = 0.75 * x0 + 0.25 * (xc + dt * R);
+  else out = (1.0 / 3.0) * x0 + (2.0 / 3.0) * (xc + dt * R);
+  ao[t] = out;
+}
+
+// Boundary planes of applyVelocityBC (ns_solver.cpp:224-276).
+__global__ void k_bc_planes(NsArgs A, double* __restrict__ u, double* __restrict__ v,
+                            double* __restrict__ w, int outflow) {
+  const DevGrid& g = A.g;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const Idx I{nx, ny, nz};
+  const long long px = 2LL * ny * nz, py = 2LL * nx * nz, pz = nz > 1 ? 2LL * nx * ny : 0;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < px) {
+    const int s = int(t & 1);
+    const long long q = t >> 1;
+    const int j = int(q % ny), k = int(q / ny);
+    const int i = s == 0 ? 0 : nx;
+    const int ty = g.bc[s];
+    if (ty == NBL_BC_WALL) u[I.u(i, j, k)] = 0.0;
+    else if (ty == NBL_BC_INLET) u[I.u(i, j, k)] = inletComp(g, s, j, 0);
+    else if (ty == NBL_BC_OUTLET && outflow) u[I.u(i, j, k)] = u[I.u(s == 0 ? 1 : nx - 1, j, k)];
+    return;
+  }
+  t -= px;
+  if (t < py) {
+    const int s = int(t & 1);
+    const long long q = t >> 1;
+    const int i = int(q % nx), k = int(q / nx);
+    const int f = 2 + s, j = s == 0 ? 0 : ny;
+    const int ty = g.bc[f];
+    if (ty == NBL_BC_WALL) v[I.v(i, j, k)] = 0.0;
+    else if (ty == NBL_BC_INLET) v[I.v(i, j, k)] = inletComp(g, f, i, 1);
+    else if (ty == NBL_BC_OUTLET && outflow) v[I.v(i, j, k)] = v[I.v(i, s == 0 ? 1 : ny - 1, k)];
+    return;
+  }
+  t -= py;
+  if (t < pz) {
+    const int s = int(t & 1);
+    const long long q = t >> 1;
+    const int i = int(q % nx), j = int(q / nx);
+    const int f = 4 + s, k = s == 0 ? 0 : nz;
+    const int ty = g.bc[f];
+    if (ty == NBL_BC_WALL) w[I.w(i, j, k)] = 0.0;
+    else if (ty == NBL_BC_INLET) w[I.w(i, j, k)] = inletComp(g, f, i, 2);
+    else if (ty == NBL_BC_OUTLET && outflow) w[I.w(i, j, k)] = w[I.w(i, j, s == 0 ? 1 : nz - 1)];
+  }
+}
+
+// Solid-touching faces (precomputed lists) are zeroed (ns_solver.cpp:277-308).
+__global__ void k_bc_solid(double* __restrict__ u, double* __restrict__ v, double* __restrict__ w,
+                           const int* __restrict__ list, int nu, int nv, int nw) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nu) u[list[t]] = 0.0;
+  else if (t < nu + nv) v[list[t]] = 0.0;
+  else if (t < nu + nv + nw) w[list[t]] = 0.0;
+}
+
+// b = -div u on fluid cells, 0 in solids (ns_solver.cpp:482-498).
+__global__ void __launch_bounds__(kBlock) k_divergence(NsArgs A, const double* __restrict__ u,
+                                                       const double* __restrict__ v,
+                                                       const double* __restrict__ w,
+                                                       double* __restrict__ b) {
+  const DevGrid& g = A.g;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const long long n = (long long)nx * ny * nz;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int i = int(c % nx);
+  const long long q = c / nx;
+  const int j = int(q % ny), k = int(q / ny);
+  if (g.hasSolid && g.solid[c]) {
+    b[c] = 0.0;
+    return;
+  }
+  const Idx I{nx, ny, nz};
+  double d = (u[I.u(i + 1, j, k)] - u[I.u(i, j, k)]) / g.dx + (v[I.v(i, j + 1, k)] - v[I.v(i, j, k)]) / g.dy;
+  if (nz > 1) d += (w[I.w(i, j, k + 1)] - w[I.w(i, j, k)]) / g.dz;
+  b[c] = -d;
+}
+
+// phi with boundary ghosts (ns_solver.cpp:508-526).
+__device__ __forceinline__ double phiAt(const DevGrid& g, const double* __restrict__ phi, int i, int j, int k) {
+  int q[3] = {i, j, k};
+  const int n[3] = {g.nx, g.ny, g.nz};
+  double s = 1.0;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    if (q[ax] >= 0 && q[ax] < n[ax]) continue;
+    if (g.bc[2 * ax] == NBL_BC_PERIODIC) {
+      q[ax] = wrapi(q[ax], n[ax]);
+      continue;
+    }
+    const int f = 2 * ax + (q[ax] < 0 ? 0 : 1);
+    q[ax] = q[ax] < 0 ? 0 : n[ax] - 1;
+    s *= g.bc[f] == NBL_BC_OUTLET ? -1.0 : 1.0;
+  }
+  return s * phi[q[0] + (long long)g.nx * (q[1] + (long long)g.ny * q[2])];
+}
+
+// Face-flux correction u -= grad(phi) on non-solid faces, then the
+// post-projection boundary values (applyVelocityBC without the outlet copy),
+// and p = phi/dt (ns_solver.cpp:531-578).
+__global__ void __launch_bounds__(kBlock) k_correct(NsArgs A, double* __restrict__ u, double* __restrict__ v,
+                                                    double* __restrict__ w, const double* __restrict__ phi,
+                                                    double* __restrict__ p, const StepCtl* __restrict__ sc) {
+  const DevGrid& g = A.g;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const long long nu = (long long)(nx + 1) * ny * nz, nv = (long long)nx * (ny + 1) * nz;
+  const long long nw = nz > 1 ? (long long)nx * ny * (nz + 1) : 0;
+  const long long nc = (long long)nx * ny * nz;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nu) {
+    const int i = int(t % (nx + 1));
+    const long long q = t / (nx + 1);
+    const int j = int(q % ny), k = int(q / ny);
+    if (solidAt(g, i - 1, j, k) || solidAt(g, i, j, k)) {
+      u[t] = 0.0;
+      return;
+    }
+    double val = u[t] - (phiAt(g, phi, i, j, k) - phiAt(g, phi, i - 1, j, k)) / g.dx;
+    if (i == 0 || i == nx) {
+      const int f = i == 0 ? 0 : 1;
+      if (g.bc[f] == NBL_BC_WALL) val = 0.0;
+      else if (g.bc[f] == NBL_BC_INLET) val = inletComp(g, f, j, 0);
+    }
+    u[t] = val;
+    return;
+  }
+  t -= nu;
+  if (t < nv) {
+    const int i = int(t % nx);
+    const long long q = t / nx;
+    const int j = int(q % (ny + 1)), k = int(q / (ny + 1));
+    if (solidAt(g, i, j - 1, k) || solidAt(g, i, j, k)) {
+      v[t] = 0.0;
+      return;
+    }
+    double val = v[t] - (phiAt(g, phi, i, j, k) - phiAt(g, phi, i, j - 1, k)) / g.dy;
+    if (j == 0 || j == ny) {
+      const int f = j == 0 ? 2 : 3;
+      if (g.bc[f] == NBL_BC_WALL) val = 0.0;
+      else if (g.bc[f] == NBL_BC_INLET) val = inletComp(g, f, i, 1);
+    }
+    v[t] = val;
+    return;
+  }
+  t -= nv;
+  if (t < nw) {
+    const int i = int(t % nx);
+    const long long q = t / nx;
+    const int j = int(q % ny), k = int(q / ny);
+    if (solidAt(g, i, j, k - 1) || solidAt(g, i, j, k)) {
+      w[t] = 0.0;
+      return;
+    }
+    double val = w[t] - (phiAt(g, phi, i, j, k) - phiAt(g, phi, i, j, k - 1)) / g.dz;
+    if (k == 0 || k == nz) {
+      const int f = k == 0 ? 4 : 5;
+      if (g.bc[f] == NBL_BC_WALL) val = 0.0;
+      else if (g.bc[f] == NBL_BC_INLET) val = inletComp(g, f, i, 2);
+    }
+    w[t] = val;
+    return;
+  }
+  t -= nw;
+  if (t < nc) p[t] = phi[t] / sc->dt;
+}
+
+// CFL time step (ns_solver.cpp:581-603): max over fluid cells, then dt.
+__global__ void __launch_bounds__(kBlock) k_cfl(NsArgs A, const double* __restrict__ u,
+                                                const double* __restrict__ v, const double* __restrict__ w,
+                                                StepCtl* sc, RedBuf rb) {
+  const DevGrid& g = A.g;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const bool flat = nz == 1;
+  const long long n = (long long)nx * ny * nz;
+  const Idx I{nx, ny, nz};
+  const double visc = 2.0 * A.nu * (1.0 / (g.dx * g.dx) + 1.0 / (g.dy * g.dy) + (flat ? 0.0 : 1.0 / (g.dz * g.dz)));
+  double m[1] = {0.0};
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+    if (g.hasSolid && g.solid[c]) continue;
+    const int i = int(c % nx);
+    const long long q = c / nx;
+    const int j = int(q % ny), k = int(q / ny);
+    const double a = fabs(0.5 * (u[I.u(i, j, k)] + u[I.u(i + 1, j, k)]));
+    const double b = fabs(0.5 * (v[I.v(i, j, k)] + v[I.v(i, j + 1, k)]));
+    const double cc = flat ? 0.0 : fabs(0.5 * (w[I.w(i, j, k)] + w[I.w(i, j, k + 1)]));
+    const double s = a / g.dx + b / g.dy + (flat ? 0.0 : cc / g.dz) + visc;
+    m[0] = fmax(m[0], s);
+  }
+  double tot[1];
+  if (gridFold<1, 1u>(m, rb, tot) && threadIdx.x == 0) {
+    const double base = visc + 1e-300;
+    const double mx = fmax(base, tot[0]);
+    sc->cfl_max = mx;
+    sc->dt = A.cfl / mx;
+  }
+}
+
+__global__ void k_set_dt(StepCtl* sc, double dt) { sc->dt = dt; }
+
+__global__ void k_fill(double* a, long long n, double v) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = v;
+}
+
+// End-of-step diagnostics (ns_solver.cpp:797-823, 605-675, 703-763).
+__global__ void __launch_bounds__(kBlock) k_diagnostics(NsArgs A, const double* __restrict__ u,
+                                                        const double* __restrict__ v,
+                                                        const double* __restrict__ w,
+                                                        const double* __restrict__ u0,
+                                                        const double* __restrict__ v0,
+                                                        const double* __restrict__ w0,
+                                                        const double* __restrict__ p, StepCtl* sc,
+                                                        DevReport* out, RedBuf rb) {
+  const DevGrid& g = A.g;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const Idx I{nx, ny, nz};
+  const long long nu = (long long)(nx + 1) * ny * nz, nv = (long long)nx * (ny + 1) * nz;
+  const long long nw = (long long)nx * ny * (nz + 1), nc = (long long)nx * ny * nz;
+  const long long ntot = nu + nv + nw + nc;
+  const double ax = g.dy * g.dz, ay = g.dx * g.dz, az = g.dx * g.dy;
+  const double mu = A.nu * A.density;
+  // acc: 0 du^2, 1 div^2, 2 fluid count, 3 mass net, 4 fx, 5 fy, 6 max|div|
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ntot; t += (long long)gridDim.x * blockDim.x) {
+    if (t < nu) {
+      const double d = u[t] - u0[t];
+      acc[0] += d * d;
+      const int i = int(t % (nx + 1));
+      if (i == 0) acc[3] += (u[t] - u[t + nx]) * ax;
+      continue;
+    }
+    long long q = t - nu;
+    if (q < nv) {
+      const double d = v[q] - v0[q];
+      acc[0] += d * d;
+      const int j = int((q / nx) % (ny + 1));
+      if (j == 0) acc[3] += (v[q] - v[q + (long long)nx * ny]) * ay;
+      continue;
+    }
+    q -= nv;
+    if (q < nw) {
+      const double d = w[q] - w0[q];
+      acc[0] += d * d;
+      if (nz > 1 && q < (long long)nx * ny) acc[3] += (w[q] - w[q + (long long)nx * ny * nz]) * az;
+      continue;
+    }
+    q -= nw;
+    const int i = int(q % nx);
+    const long long r = q / nx;
+    const int j = int(r % ny), k = int(r / ny);
+    if (solidAt(g, i, j, k)) continue;
+    double dv = (u[I.u(i + 1, j, k)] - u[I.u(i, j, k)]) / g.dx + (v[I.v(i, j + 1, k)] - v[I.v(i, j, k)]) / g.dy;
+    if (nz > 1) dv += (w[I.w(i, j, k + 1)] - w[I.w(i, j, k)]) / g.dz;
+    acc[1] += dv * dv;
+    acc[2] += 1.0;
+    acc[6] = fmax(acc[6], fabs(dv));
+    if (g.hasSolid) {
+      const double pf = p[q];
+      const double uc = 0.5 * (u[I.u(i, j, k)] + u[I.u(i + 1, j, k)]);
+      const double vc = 0.5 * (v[I.v(i, j, k)] + v[I.v(i, j + 1, k)]);
+      const int nd = nz > 1 ? 6 : 4;
+      for (int d = 0; d < nd; ++d) {
+        const int si = i + (d == 0 ? -1 : d == 1 ? 1 : 0), sj = j + (d == 2 ? -1 : d == 3 ? 1 : 0);
+        const int sk = k + (d == 4 ? -1 : d == 5 ? 1 : 0);
+        if (!solidAt(g, si, sj, sk)) continue;
+        const double nxo = d == 0 ? 1.0 : d == 1 ? -1.0 : 0.0;
+        const double nyo = d == 2 ? 1.0 : d == 3 ? -1.0 : 0.0;
+        const int axis = d / 2;
+        const double area = axis == 0 ? ax : axis == 1 ? ay : az;
+        const double dn = axis == 0 ? g.dx : axis == 1 ? g.dy : g.dz;
+        acc[4] += -pf * nxo * area;
+        acc[5] += -pf * nyo * area;
+        if (axis != 0) acc[4] += mu * uc / (0.5 * dn) * area;
+        if (axis != 1) acc[5] += mu * vc / (0.5 * dn) * area;
+      }
+    }
+  }
+  double tot[7];
+  if (!gridFold<7, 1u << 6>(acc, rb, tot) || threadIdx.x != 0) return;
+  const double dt = sc->dt;
+  const double n = double(nu + nv + nw);
+  const double mom = sqrt(tot[0] / n) / (dt * A.velocityScale + 1e-300);
+  const double divl2 = tot[2] > 0.0 ? sqrt(tot[1] / tot[2]) : 0.0;
+  const double mass = fabs(tot[3]) / (A.velocityScale * (ny * ay) + 1e-300);
+  double cd = 0.0, cl = 0.0;
+  if (g.hasSolid) {
+    double aref = A.refArea;
+    if (aref <= 0.0) aref = A.refLength * A.refLength;
+    const double qd = 0.5 * A.density * A.velocityScale * A.velocityScale * aref + 1e-300;
+    cd = tot[4] / qd;
+    cl = tot[5] / qd;
+  }
+  sc->step += 1;
+  sc->t += dt;
+  out->step = sc->step;
+  out->t = sc->t;
+  out->dt = dt;
+  out->cfl = A.cfl;
+  out->mom = mom;
+  out->divl2 = divl2;
+  out->divmax = tot[6];
+  out->mass = mass;
+  out->cd = cd;
+  out->cl = cl;
+  out->cells = (unsigned long long)nc;
+  out->accepted = (divl2 < A.tolContinuity && mass < A.tolMass) ? 1 : 0;
+}
+
+// Cell-centred velocity (ns_solver.cpp:677-701).
+__global__ void k_cell_velocity(NsArgs A, const double* __restrict__ u, const double* __restrict__ v,
+                                const double* __restrict__ w, int comp, double* __restrict__ out) {
+  const DevGrid& g = A.g;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const long long n = (long long)nx * ny * nz;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const Idx I{nx, ny, nz};
+  const int i = int(c % nx);
+  const long long q = c / nx;
+  const int j = int(q % ny), k = int(q / ny);
+  double r;
+  if (comp == 0) r = 0.5 * (u[I.u(i, j, k)] + u[I.u(i + 1, j, k)]);
+  else if (comp == 1) r = 0.5 * (v[I.v(i, j, k)] + v[I.v(i, j + 1, k)]);
+  else r = nz == 1 ? 0.0 : 0.5 * (w[I.w(i, j, k)] + w[I.w(i, j, k + 1)]);
+  out[c] = r;
+}
+
+}  // namespace
+}  // namespace nbl
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+namespace nbl {
+
+static long long faces(const DevGrid& g) {
+  return (long long)(g.nx + 1) * g.ny * g.nz + (long long)g.nx * (g.ny + 1) * g.nz +
+         (long long)g.nx * g.ny * (g.nz + 1);
+}
+
+void launch_bc_planes(const NsArgs& a, double* u, double* v, double* w, bool outflow, hipStream_t s) {
+  const DevGrid& g = a.g;
+  const long long n = 2LL * g.ny * g.nz + 2LL * g.nx * g.nz + (g.nz > 1 ? 2LL * g.nx * g.ny : 0);
+  k_bc_planes<<<gridFor(n), kBlock, 0, s>>>(a, u, v, w, outflow ? 1 : 0);
+  NBL_LAUNCH_CHECK();
+}
+
+void launch_bc_solid(const NsArgs& a, double* u, double* v, double* w, const int* list,
+                     const int counts[3], hipStream_t s) {
+  (void)a;
+  const int n = counts[0] + counts[1] + counts[2];
+  if (n == 0) return;
+  k_bc_solid<<<gridFor(n), kBlock, 0, s>>>(u, v, w, list, counts[0], counts[1], counts[2]);
+  NBL_LAUNCH_CHECK();
+}
+
+void launch_rhs_rk(const NsArgs& a, const double* u0, const double* v0, const double* w0,
+                   const double* u, const double* v, const double* w, double* uo, double* vo,
+                   double* wo, int stage, const StepCtl* sc, hipStream_t s) {
+  k_rhs_rk<<<gridFor(faces(a.g)), kBlock, 0, s>>>(a, u0, v0, w0, u, v, w, uo, vo, wo, stage, sc);
+  NBL_LAUNCH_CHECK();
+}
+
+void launch_divergence(const NsArgs& a, const double* u, const double* v, const double* w,
+                       double* b, hipStream_t s) {
+  const long long n = (long long)a.g.nx * a.g.ny * a.g.nz;
+  k_divergence<<<gridFor(n), kBlock, 0, s>>>(a, u, v, w, b);
+  NBL_LAUNCH_CHECK();
+}
+
+void launch_correct(const NsArgs& a, double* u, double* v, double* w, const double* phi,
+                    double* p, const StepCtl* sc, hipStream_t s) {
+  const long long n = faces(a.g) + (long long)a.g.nx * a.g.ny * a.g.nz;
+  k_correct<<<gridFor(n), kBlock, 0, s>>>(a, u, v, w, phi, p, sc);
+  NBL_LAUNCH_CHECK();
+}
+
+void launch_cfl(const NsArgs& a, const double* u, const double* v, const double* w, StepCtl* sc,
+                double fixed_dt, RedBuf rb, hipStream_t s) {
+  if (fixed_dt > 0.0) {
+    k_set_dt<<<1, 1, 0, s>>>(sc, fixed_dt);
+  } else {
+    const long long n = (long long)a.g.nx * a.g.ny * a.g.nz;
+    k_cfl<<<gridFor(n, kBlock, kMaxRedBlocks), kBlock, 0, s>>>(a, u, v, w, sc, rb);
+  }
+  NBL_LAUNCH_CHECK();
+}
+
+void launch_diagnostics(const NsArgs& a, const double* u, const double* v, const double* w,
+                        const double* u0, const double* v0, const double* w0, const double* p,
+                        StepCtl* sc, DevReport* out, RedBuf rb, hipStream_t s) {
+  const long long n = faces(a.g) + (long long)a.g.nx * a.g.ny * a.g.nz;
+  k_diagnostics<<<gridFor(n, kBlock, kMaxRedBlocks), kBlock, 0, s>>>(a, u, v, w, u0, v0, w0, p, sc,
+                                                                      out, rb);
+  NBL_LAUNCH_CHECK();
+}
+
+void launch_fill(double* a, long long n, double v, hipStream_t s) {
+  if (n <= 0) return;
+  k_fill<<<gridFor(n), kBlock, 0, s>>>(a, n, v);
+  NBL_LAUNCH_CHECK();
+}
+
+void launch_cell_velocity(const NsArgs& a, const double* u, const double* v, const double* w,
+                          int comp, double* out, hipStream_t s) {
+  const long long n = (long long)a.g.nx * a.g.ny * a.g.nz;
+  k_cell_velocity<<<gridFor(n), kBlock, 0, s>>>(a, u, v, w, comp, out);
+  NBL_LAUNCH_CHECK();
+}
+
+}  // namespace nbl

@@ -1,0 +1,616 @@
+// octree.hip — octree leaves on the GPU (reference core/src/mesh/octree.cpp,
+// core/include/nabla/mesh/morton.hpp, core/src/flow/octree_poisson.cpp).
+//
+// Layout in HBM: leaves Morton-sorted (SoA: key u64, level u8, mask u8).  The
+// reference finds neighbours through a hash map probed at every operator
+// application (octree.cpp:229-277); here the same probes run once, as binary
+// searches over the sorted keys, and produce
+//   * a face table (count + up to 4 neighbours per leaf and direction, in the
+//     reference's order) — exported for the connectivity parity check, and
+//   * a CSR operator: per fluid leaf, its fluid neighbours in the reference's
+//     accumulation order with the exact face coefficient area/dist computed
+//     from the reference's centre/size formulas, plus the diagonal.
+// OctreePoisson::applyA is then one gather pass over the CSR, bit-identical to
+// the reference per element.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#include "octree.hpp"
+
+namespace nbl {
+namespace {
+
+// 21 bits per axis: x at bit 3i, y at 3i+1, z at 3i+2 (morton.hpp:15-48).
+__host__ __device__ __forceinline__ uint64_t spread3(uint32_t a) {
+  uint64_t x = a & 0x1fffffull;
+  x = (x | (x << 32)) & 0x001f00000000ffffull;
+  x = (x | (x << 16)) & 0x001f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+__host__ __device__ __forceinline__ uint32_t gather3(uint64_t x) {
+  x &= 0x1249249249249249ull;
+  x = (x ^ (x >> 2)) & 0x10c30c30c30c30c3ull;
+  x = (x ^ (x >> 4)) & 0x100f00f00f00f00full;
+  x = (x ^ (x >> 8)) & 0x001f0000ff0000ffull;
+  x = (x ^ (x >> 16)) & 0x001f00000000ffffull;
+  x = (x ^ (x >> 32)) & 0x1fffffull;
+  return uint32_t(x);
+}
+__host__ __device__ __forceinline__ uint64_t menc(uint32_t x, uint32_t y, uint32_t z) {
+  return spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
+}
+
+__device__ __forceinline__ long long findKey(const uint64_t* __restrict__ keys, long long n, uint64_t k) {
+  long long lo = 0, hi = n;
+  while (lo < hi) {
+    const long long mid = (lo + hi) >> 1;
+    if (keys[mid] < k) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < n && keys[lo] == k) ? lo : -1;
+}
+
+// Neighbours across face `dir` of leaf i (octree.cpp:229-277).
+__global__ void k_faces(long long n, int maxL, const uint64_t* __restrict__ keys, const uint8_t* __restrict__ lev,
+                        uint8_t* __restrict__ cnt, int32_t* __restrict__ nbr) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 6 * n) return;
+  const long long i = t / 6;
+  const int dir = int(t % 6);
+  const int axis = dir >> 1;
+  const bool plus = dir & 1;
+  const uint64_t key = keys[i];
+  const int L = lev[i];
+  const uint32_t a[3] = {gather3(key), gather3(key >> 1), gather3(key >> 2)};
+  const uint32_t size = 1u << (maxL - L);
+  int64_t q[3] = {a[0], a[1], a[2]};
+  q[axis] += plus ? int64_t(size) : -int64_t(size);
+  int32_t* out = nbr + 4 * t;
+  out[0] = out[1] = out[2] = out[3] = -1;
+  if (q[axis] < 0 || q[axis] >= (int64_t(1) << maxL)) {
+    cnt[t] = 0;
+    return;
+  }
+  const uint32_t u[3] = {uint32_t(q[0]), uint32_t(q[1]), uint32_t(q[2])};
+  for (int l = L; l >= 0; --l) {
+    const uint32_t m = ~((1u << (maxL - l)) - 1u);
+    const long long hit = findKey(keys, n, menc(u[0] & m, u[1] & m, u[2] & m));
+    if (hit >= 0 && lev[hit] == l) {
+      out[0] = int32_t(hit);
+      cnt[t] = 1;
+      return;
+    }
+  }
+  int c = 0;
+  if (L + 1 <= maxL) {
+    const uint32_t cs = 1u << (maxL - L - 1);
+    const int t1 = (axis + 1) % 3, t2 = (axis + 2) % 3;
+    const uint32_t base = plus ? u[axis] : u[axis] + size - cs;
+    for (int k1 = 0; k1 < 2; ++k1)
+      for (int k2 = 0; k2 < 2; ++k2) {
+        uint32_t b[3];
+        b[axis] = base;
+        b[t1] = u[t1] + uint32_t(k1) * cs;
+        b[t2] = u[t2] + uint32_t(k2) * cs;
+        const long long hit = findKey(keys, n, menc(b[0], b[1], b[2]));
+        if (hit >= 0 && lev[hit] == L + 1) out[c++] = int32_t(hit);
+      }
+  }
+  cnt[t] = uint8_t(c);
+}
+
+struct Geo {
+  double org[3], fine[3];
+  int maxL;
+};
+
+// Reference centre (octree.cpp:84-91) and size (octree.cpp:97-100).
+__device__ __forceinline__ double centerAxis(const Geo& G, uint64_t key, int lvl, int axis) {
+  const uint32_t a = gather3(key >> axis);
+  const uint32_t sz = 1u << (G.maxL - lvl);
+  const double units = double(a) + 0.5 * double(sz);
+  return G.org[axis] + units * G.fine[axis];
+}
+__device__ __forceinline__ double sizeAxis(const Geo& G, int lvl, int axis) {
+  return double(1u << (G.maxL - lvl)) * G.fine[axis];
+}
+// area of the finer cell / centre distance (octree_poisson.cpp:24-32)
+__device__ __forceinline__ double faceCoeff(const Geo& G, uint64_t ki, int li, uint64_t kj, int lj, int axis) {
+  const int t1 = (axis + 1) % 3, t2 = (axis + 2) % 3;
+  const int lf = lj > li ? lj : li;
+  const double area = sizeAxis(G, lf, t1) * sizeAxis(G, lf, t2);
+  const double dist = fabs(centerAxis(G, ki, li, axis) - centerAxis(G, kj, lj, axis));
+  return dist > 0.0 ? area / dist : 0.0;
+}
+
+__global__ void k_row_count(long long n, const uint8_t* __restrict__ mask, const uint8_t* __restrict__ cnt,
+                            const int32_t* __restrict__ nbr, int32_t* __restrict__ rowCount) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c = 0;
+  if (mask[i] == 0) {
+    for (int d = 0; d < 6; ++d)
+      for (int m = 0; m < cnt[6 * i + d]; ++m)
+        if (mask[nbr[24 * i + 4 * d + m]] == 0) ++c;
+  }
+  rowCount[i] = c;
+}
+
+__global__ void k_row_fill(long long n, Geo G, const uint64_t* __restrict__ keys, const uint8_t* __restrict__ lev,
+                           const uint8_t* __restrict__ mask, const uint8_t* __restrict__ cnt,
+                           const int32_t* __restrict__ nbr, const int32_t* __restrict__ rowPtr,
+                           int32_t* __restrict__ col, double* __restrict__ coef, double* __restrict__ diag) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (mask[i] != 0) {
+    diag[i] = 1.0;
+    return;
+  }
+  int e = rowPtr[i];
+  double d = 0.0;
+  for (int dd = 0; dd < 6; ++dd)
+    for (int m = 0; m < cnt[6 * i + dd]; ++m) {
+      const int j = nbr[24 * i + 4 * dd + m];
+      if (mask[j] != 0) continue;
+      const double c = faceCoeff(G, keys[i], lev[i], keys[j], lev[j], dd >> 1);
+      col[e] = j;
+      coef[e] = c;
+      ++e;
+      d += c;
+    }
+  diag[i] = d > 0.0 ? d : 1.0;
+}
+
+// (A x)_i = sum_e coef_e (x_i - x_col_e) on fluid rows, x_i elsewhere
+// (octree_poisson.cpp:64-85).
+__device__ __forceinline__ double octRow(long long i, const uint8_t* __restrict__ mask, const int32_t* __restrict__ rowPtr,
+                                         const int32_t* __restrict__ col, const double* __restrict__ coef,
+                                         const double* __restrict__ x, double m, bool hasM) {
+  double xi = x[i];
+  if (mask[i] != 0) return hasM ? 0.0 : xi;
+  if (hasM) xi = xi - m;
+  double acc = 0.0;
+  const int e1 = rowPtr[i + 1];
+  for (int e = rowPtr[i]; e < e1; ++e) {
+    double xj = x[col[e]];
+    if (hasM) xj = xj - m;
+    acc += coef[e] * (xi - xj);
+  }
+  return acc;
+}
+
+__global__ void k_oct_apply(long long n, const uint8_t* __restrict__ mask, const int32_t* __restrict__ rowPtr,
+                            const int32_t* __restrict__ col, const double* __restrict__ coef,
+                            const double* __restrict__ x, double* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = octRow(i, mask, rowPtr, col, coef, x, 0.0, false);
+}
+
+__global__ void k_gather(long long n, const uint64_t* __restrict__ perm, const double* __restrict__ src,
+                         double* __restrict__ dst) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[perm[i]];
+}
+__global__ void k_scatter(long long n, const uint64_t* __restrict__ perm, const double* __restrict__ src,
+                          double* __restrict__ dst) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[perm[i]] = src[i];
+}
+
+__global__ void k_iota(long long n, uint64_t* __restrict__ a) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = uint64_t(i);
+}
+template <class T>
+__global__ void k_permute(long long n, const uint64_t* __restrict__ perm, const T* __restrict__ src, T* __restrict__ dst) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[perm[i]];
+}
+
+// Input validation: duplicates, misaligned anchors, levels out of range.
+__global__ void k_validate(long long n, int maxL, const uint64_t* __restrict__ keys, const uint8_t* __restrict__ lev,
+                           unsigned* __restrict__ bad) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int L = lev[i];
+  if (L > maxL) {
+    atomicOr(bad, 1u);
+    return;
+  }
+  const uint32_t sz = 1u << (maxL - L);
+  const uint64_t k = keys[i];
+  const uint32_t a[3] = {gather3(k), gather3(k >> 1), gather3(k >> 2)};
+  for (int q = 0; q < 3; ++q)
+    if ((a[q] & (sz - 1u)) || a[q] + uint64_t(sz) > (uint64_t(1) << maxL)) atomicOr(bad, 2u);
+  if (k >> 63) atomicOr(bad, 2u);
+  if (i > 0 && keys[i - 1] == k) atomicOr(bad, 4u);
+}
+
+// ---- Jacobi-PCG on the leaves (octree_poisson.cpp:107-197) ------------------
+enum : int { O_MEAN_RHS = 0, O_MEAN_R = 1, O_MEAN_Z = 2, O_MEAN_X = 3 };
+
+#define OGRID(c, n) \
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < (n); c += (long long)gridDim.x * blockDim.x)
+
+__global__ void __launch_bounds__(kBlock) k_o_init(long long n, long long nf, const uint8_t* __restrict__ mask,
+                                                   const double* __restrict__ b, double* __restrict__ rhs,
+                                                   double* __restrict__ x, SolveCtl* ctl, RedBuf rb) {
+  double acc[1] = {0.0};
+  OGRID(c, n) {
+    const double v = mask[c] == 0 ? b[c] : 0.0;
+    rhs[c] = v;
+    x[c] = 0.0;
+    acc[0] += v;
+  }
+  double tot[1];
+  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) ctl->slot[O_MEAN_RHS] = nf ? tot[0] / double(nf) : 0.0;
+}
+
+// rhs <- rhs - mean (fluid), 0 elsewhere; r = rhs; z' = r/diag (fluid); fold
+// |rhs|^2 and sum(z').
+__global__ void __launch_bounds__(kBlock) k_o_init2(long long n, long long nf, const uint8_t* __restrict__ mask,
+                                                    const double* __restrict__ diag, double* __restrict__ rhs,
+                                                    double* __restrict__ r, double* __restrict__ z, SolveCtl* ctl,
+                                                    RedBuf rb) {
+  const double m = ctl->slot[O_MEAN_RHS];
+  double acc[2] = {0.0, 0.0};
+  OGRID(c, n) {
+    const bool fl = mask[c] == 0;
+    const double v = (fl && nf) ? rhs[c] - m : (nf ? 0.0 : rhs[c]);
+    rhs[c] = v;
+    r[c] = v;
+    const double zv = fl ? v / diag[c] : 0.0;
+    z[c] = zv;
+    acc[0] += v * v;
+    if (fl) acc[1] += zv;
+  }
+  double tot[2];
+  if (gridFold<2>(acc, rb, tot) && threadIdx.x == 0) {
+    ctl->bnorm = sqrt(tot[0]) + 1e-300;
+    ctl->slot[O_MEAN_Z] = nf ? tot[1] / double(nf) : 0.0;
+    ctl->it = 0;
+    ctl->done = 0;
+    ctl->converged = 0;
+    ctl->broke = 0;
+  }
+}
+
+// z <- z' - mean (fluid), 0 elsewhere; fold r.z.  first: p = z and rz = r.z;
+// otherwise beta = (r.z)/rz.
+__global__ void __launch_bounds__(kBlock) k_o_zdot(long long n, long long nf, const uint8_t* __restrict__ mask,
+                                                   const double* __restrict__ r, double* __restrict__ z,
+                                                   double* __restrict__ p, int first, SolveCtl* ctl, RedBuf rb) {
+  if (ctl->done) return;
+  const double m = ctl->slot[O_MEAN_Z];
+  double acc[1] = {0.0};
+  OGRID(c, n) {
+    const double zv = mask[c] == 0 ? (nf ? z[c] - m : z[c]) : 0.0;
+    z[c] = zv;
+    if (first) p[c] = zv;
+    acc[0] += r[c] * zv;
+  }
+  double tot[1];
+  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) {
+    if (first) {
+      ctl->rz = tot[0];
+    } else {
+      ctl->beta = tot[0] / ctl->rz;
+      ctl->rz = tot[0];
+      ctl->it += 1;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_o_apply_dot(long long n, const uint8_t* __restrict__ mask,
+                                                        const int32_t* __restrict__ rowPtr, const int32_t* __restrict__ col,
+                                                        const double* __restrict__ coef, const double* __restrict__ p,
+                                                        double* __restrict__ Ap, SolveCtl* ctl, RedBuf rb) {
+  if (ctl->done) return;
+  double acc[1] = {0.0};
+  OGRID(c, n) {
+    const double v = octRow(c, mask, rowPtr, col, coef, p, 0.0, false);
+    Ap[c] = v;
+    acc[0] += p[c] * v;
+  }
+  double tot[1];
+  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) {
+    if (fabs(tot[0]) < 1e-300) {
+      ctl->done = 1;
+      ctl->broke = 1;
+    } else {
+      ctl->alpha = ctl->rz / tot[0];
+    }
+  }
+}
+
+// x += alpha p; r' = r - alpha Ap; fold sum over fluid of r'.
+__global__ void __launch_bounds__(kBlock) k_o_update(long long n, long long nf, const uint8_t* __restrict__ mask,
+                                                     double* __restrict__ x, const double* __restrict__ p,
+                                                     double* __restrict__ r, const double* __restrict__ Ap,
+                                                     SolveCtl* ctl, RedBuf rb) {
+  if (ctl->done) return;
+  const double a = ctl->alpha;
+  double acc[1] = {0.0};
+  OGRID(c, n) {
+    x[c] = x[c] + a * p[c];
+    const double v = r[c] - a * Ap[c];
+    r[c] = v;
+    if (mask[c] == 0) acc[0] += v;
+  }
+  double tot[1];
+  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) ctl->slot[O_MEAN_R] = nf ? tot[0] / double(nf) : 0.0;
+}
+
+// r <- r' - mean (fluid), 0 elsewhere; convergence test on |r|; z' = r/diag
+// (fluid) and its fluid sum.
+__global__ void __launch_bounds__(kBlock) k_o_rnorm(long long n, long long nf, const uint8_t* __restrict__ mask,
+                                                    const double* __restrict__ diag, double* __restrict__ r,
+                                                    double* __restrict__ z, SolveCtl* ctl, RedBuf rb) {
+  if (ctl->done) return;
+  const double m = ctl->slot[O_MEAN_R];
+  double acc[2] = {0.0, 0.0};
+  OGRID(c, n) {
+    const bool fl = mask[c] == 0;
+    const double v = fl ? (nf ? r[c] - m : r[c]) : (nf ? 0.0 : r[c]);
+    r[c] = v;
+    const double zv = fl ? v / diag[c] : 0.0;
+    z[c] = zv;
+    acc[0] += v * v;
+    if (fl) acc[1] += zv;
+  }
+  double tot[2];
+  if (gridFold<2>(acc, rb, tot) && threadIdx.x == 0) {
+    ctl->slot[O_MEAN_Z] = nf ? tot[1] / double(nf) : 0.0;
+    if (sqrt(tot[0]) <= ctl->tol * ctl->bnorm) {
+      ctl->it += 1;
+      ctl->converged = 1;
+      ctl->done = 1;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_o_pupdate(long long n, double* __restrict__ p, const double* __restrict__ z,
+                                                      SolveCtl* ctl) {
+  if (ctl->done) return;
+  const double be = ctl->beta;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) p[c] = z[c] + be * p[c];
+}
+
+__global__ void __launch_bounds__(kBlock) k_o_xsum(long long n, long long nf, const uint8_t* __restrict__ mask,
+                                                   const double* __restrict__ x, SolveCtl* ctl, RedBuf rb) {
+  double acc[1] = {0.0};
+  OGRID(c, n) if (mask[c] == 0) acc[0] += x[c];
+  double tot[1];
+  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) ctl->slot[O_MEAN_X] = nf ? tot[0] / double(nf) : 0.0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_o_final(long long n, long long nf, const uint8_t* __restrict__ mask,
+                                                    const int32_t* __restrict__ rowPtr, const int32_t* __restrict__ col,
+                                                    const double* __restrict__ coef, const double* __restrict__ x,
+                                                    const double* __restrict__ rhs, double* __restrict__ xo,
+                                                    SolveCtl* ctl, RedBuf rb) {
+  const double m = ctl->slot[O_MEAN_X];
+  const bool hasM = nf > 0;
+  double acc[1] = {0.0};
+  OGRID(c, n) {
+    const double Ax = octRow(c, mask, rowPtr, col, coef, x, m, hasM);
+    const double xv = mask[c] == 0 ? (hasM ? x[c] - m : x[c]) : (hasM ? 0.0 : x[c]);
+    xo[c] = xv;
+    const double e = rhs[c] - Ax;
+    acc[0] += e * e;
+  }
+  double tot[1];
+  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) {
+    ctl->resid = sqrt(tot[0]) / ctl->bnorm;
+    ctl->converged = (ctl->converged || ctl->resid <= 1e-6) ? 1 : 0;
+  }
+}
+
+int ogrid(long long n) { return gridFor(n, kBlock, kMaxRedBlocks); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// OctreeDev
+// ---------------------------------------------------------------------------
+OctreeDev::OctreeDev(nbl_ctx* ctx, const uint64_t* keys, const uint8_t* levels, const uint8_t* masks, size_t n,
+                     const double origin[3], const double extent[3], int maxLevel)
+    : ctx_(ctx), n_(n), maxL_(maxLevel) {
+  if (n == 0) throw std::invalid_argument("octree: no leaves");
+  if (maxLevel < 1 || maxLevel > 21) throw std::invalid_argument("Octree: maxLevel must be in [1, 21]");
+  if (!(extent[0] > 0 && extent[1] > 0 && extent[2] > 0)) throw std::invalid_argument("Octree: extent must be positive");
+  if (n >= (size_t(1) << 31)) throw std::invalid_argument("octree: too many leaves");
+  const double inv = 1.0 / double(1u << maxLevel);
+  for (int a = 0; a < 3; ++a) {
+    org_[a] = origin[a];
+    fine_[a] = extent[a] * inv;
+  }
+  NBL_HIP(hipStreamCreate(&stream_));
+  hipStream_t s = stream_;
+  const long long N = (long long)n;
+  DevBuf kin(ctx, n * 8), idx(ctx, n * 8), lin(ctx, n), min_(ctx, n);
+  keys_ = DevBuf(ctx, n * 8);
+  perm_ = DevBuf(ctx, n * 8);
+  lev_ = DevBuf(ctx, n);
+  mask_ = DevBuf(ctx, n);
+  NBL_HIP(hipMemcpyAsync(kin.as<void>(), keys, n * 8, hipMemcpyHostToDevice, s));
+  NBL_HIP(hipMemcpyAsync(lin.as<void>(), levels, n, hipMemcpyHostToDevice, s));
+  NBL_HIP(hipMemcpyAsync(min_.as<void>(), masks, n, hipMemcpyHostToDevice, s));
+  k_iota<<<gridFor(N), kBlock, 0, s>>>(N, idx.as<uint64_t>());
+  NBL_LAUNCH_CHECK();
+  size_t tmpBytes = 0;
+  NBL_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpBytes, kin.as<uint64_t>(), keys_.as<uint64_t>(),
+                                             idx.as<uint64_t>(), perm_.as<uint64_t>(), int(n), 0, 64, s));
+  DevBuf tmp(ctx, tmpBytes);
+  NBL_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.as<void>(), tmpBytes, kin.as<uint64_t>(), keys_.as<uint64_t>(),
+                                             idx.as<uint64_t>(), perm_.as<uint64_t>(), int(n), 0, 64, s));
+  k_permute<uint8_t><<<gridFor(N), kBlock, 0, s>>>(N, perm_.as<uint64_t>(), lin.as<uint8_t>(), lev_.as<uint8_t>());
+  k_permute<uint8_t><<<gridFor(N), kBlock, 0, s>>>(N, perm_.as<uint64_t>(), min_.as<uint8_t>(), mask_.as<uint8_t>());
+  NBL_LAUNCH_CHECK();
+  DevBuf bad(ctx, 4);
+  NBL_HIP(hipMemsetAsync(bad.as<void>(), 0, 4, s));
+  k_validate<<<gridFor(N), kBlock, 0, s>>>(N, maxLevel, keys_.as<uint64_t>(), lev_.as<uint8_t>(), bad.as<unsigned>());
+  NBL_LAUNCH_CHECK();
+  unsigned hbad = 0;
+  NBL_HIP(hipMemcpyAsync(&hbad, bad.as<void>(), 4, hipMemcpyDeviceToHost, s));
+  NBL_HIP(hipStreamSynchronize(s));
+  if (hbad & 1u) throw std::invalid_argument("octree: level above maxLevel");
+  if (hbad & 2u) throw std::invalid_argument("octree: key is not a valid anchor for its level");
+  if (hbad & 4u) throw std::invalid_argument("octree: duplicate leaf key");
+  // face table
+  cnt_ = DevBuf(ctx, 6 * n);
+  nbr_ = DevBuf(ctx, 24 * n * sizeof(int32_t));
+  k_faces<<<gridFor(6 * N), kBlock, 0, s>>>(N, maxLevel, keys_.as<uint64_t>(), lev_.as<uint8_t>(), cnt_.as<uint8_t>(),
+                                            nbr_.as<int32_t>());
+  NBL_LAUNCH_CHECK();
+  // CSR of the fluid-fluid couplings
+  DevBuf rowCount(ctx, (n + 1) * sizeof(int32_t));
+  rowPtr_ = DevBuf(ctx, (n + 1) * sizeof(int32_t));
+  NBL_HIP(hipMemsetAsync(rowCount.as<void>(), 0, (n + 1) * sizeof(int32_t), s));
+  k_row_count<<<gridFor(N), kBlock, 0, s>>>(N, mask_.as<uint8_t>(), cnt_.as<uint8_t>(), nbr_.as<int32_t>(),
+                                            rowCount.as<int32_t>());
+  NBL_LAUNCH_CHECK();
+  size_t scanBytes = 0;
+  NBL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, rowCount.as<int32_t>(), rowPtr_.as<int32_t>(),
+                                           int(n + 1), s));
+  DevBuf scanTmp(ctx, scanBytes);
+  NBL_HIP(hipcub::DeviceScan::ExclusiveSum(scanTmp.as<void>(), scanBytes, rowCount.as<int32_t>(),
+                                           rowPtr_.as<int32_t>(), int(n + 1), s));
+  int32_t nnz = 0;
+  NBL_HIP(hipMemcpyAsync(&nnz, rowPtr_.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NBL_HIP(hipStreamSynchronize(s));
+  nnz_ = size_t(nnz);
+  col_ = DevBuf(ctx, std::max<size_t>(1, nnz_) * sizeof(int32_t));
+  coef_ = DevBuf(ctx, std::max<size_t>(1, nnz_) * sizeof(double));
+  diag_ = DevBuf(ctx, n * sizeof(double));
+  Geo G;
+  for (int a = 0; a < 3; ++a) {
+    G.org[a] = org_[a];
+    G.fine[a] = fine_[a];
+  }
+  G.maxL = maxLevel;
+  k_row_fill<<<gridFor(N), kBlock, 0, s>>>(N, G, keys_.as<uint64_t>(), lev_.as<uint8_t>(), mask_.as<uint8_t>(),
+                                           cnt_.as<uint8_t>(), nbr_.as<int32_t>(), rowPtr_.as<int32_t>(),
+                                           col_.as<int32_t>(), coef_.as<double>(), diag_.as<double>());
+  NBL_LAUNCH_CHECK();
+  std::vector<uint8_t> hm(masks, masks + n);
+  nf_ = 0;
+  for (uint8_t m : hm) nf_ += m == 0;
+  // solver vectors
+  for (auto* b : {&x_, &rhs_, &r_, &z_, &p_, &Ap_, &t0_, &t1_}) *b = DevBuf(ctx, n * 8);
+  ctl_ = DevBuf(ctx, sizeof(SolveCtl));
+  red_ = DevBuf(ctx, sizeof(double) * kMaxRedVals * kMaxRedBlocks);
+  ticket_ = DevBuf(ctx, 256);
+  NBL_HIP(hipMemsetAsync(ticket_.as<void>(), 0, 256, s));
+  NBL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hctl_), 2 * sizeof(SolveCtl), hipHostMallocDefault));
+  NBL_HIP(hipStreamSynchronize(s));
+}
+
+OctreeDev::~OctreeDev() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  if (hctl_) (void)hipHostFree(hctl_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void OctreeDev::permutation(uint64_t* perm) {
+  NBL_HIP(hipMemcpyAsync(perm, perm_.as<void>(), n_ * 8, hipMemcpyDeviceToHost, stream_));
+  NBL_HIP(hipStreamSynchronize(stream_));
+}
+
+void OctreeDev::faceTable(uint8_t* count, int32_t* nbr) {
+  if (count) NBL_HIP(hipMemcpyAsync(count, cnt_.as<void>(), 6 * n_, hipMemcpyDeviceToHost, stream_));
+  if (nbr) NBL_HIP(hipMemcpyAsync(nbr, nbr_.as<void>(), 24 * n_ * sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
+  NBL_HIP(hipStreamSynchronize(stream_));
+}
+
+void OctreeDev::toSorted(const double* hostIn, double* dSorted) {
+  NBL_HIP(hipMemcpyAsync(t1_.as<void>(), hostIn, n_ * 8, hipMemcpyHostToDevice, stream_));
+  k_gather<<<gridFor((long long)n_), kBlock, 0, stream_>>>((long long)n_, perm_.as<uint64_t>(), t1_.as<double>(), dSorted);
+  NBL_LAUNCH_CHECK();
+}
+
+void OctreeDev::fromSorted(const double* dSorted, double* hostOut) {
+  k_scatter<<<gridFor((long long)n_), kBlock, 0, stream_>>>((long long)n_, perm_.as<uint64_t>(), dSorted, t1_.as<double>());
+  NBL_LAUNCH_CHECK();
+  NBL_HIP(hipMemcpyAsync(hostOut, t1_.as<void>(), n_ * 8, hipMemcpyDeviceToHost, stream_));
+  NBL_HIP(hipStreamSynchronize(stream_));
+}
+
+void OctreeDev::applySorted(const double* x, double* out, hipStream_t s) {
+  k_oct_apply<<<gridFor((long long)n_), kBlock, 0, s>>>((long long)n_, mask_.as<uint8_t>(), rowPtr_.as<int32_t>(),
+                                                         col_.as<int32_t>(), coef_.as<double>(), x, out);
+  NBL_LAUNCH_CHECK();
+}
+
+void OctreeDev::apply(const double* x, double* out) {
+  toSorted(x, t0_.as<double>());
+  applySorted(t0_.as<double>(), Ap_.as<double>(), stream_);
+  fromSorted(Ap_.as<double>(), out);
+}
+
+void OctreeDev::diagonal(double* out) { fromSorted(diag_.as<double>(), out); }
+
+nbl_poisson_report OctreeDev::solveSorted(const double* b, double* xo, double tol, int maxIters, hipStream_t s) {
+  SolveCtl* ctl = ctl_.as<SolveCtl>();
+  const RedBuf rb{red_.as<double>(), ticket_.as<unsigned>()};
+  const long long N = (long long)n_, nf = (long long)nf_;
+  const int g = ogrid(N);
+  hctl_[1] = SolveCtl{};
+  hctl_[1].tol = tol;
+  NBL_HIP(hipMemcpyAsync(ctl, &hctl_[1], sizeof(SolveCtl), hipMemcpyHostToDevice, s));
+  const uint8_t* mk = mask_.as<uint8_t>();
+  const int32_t* rp = rowPtr_.as<int32_t>();
+  const int32_t* cl = col_.as<int32_t>();
+  const double* cf = coef_.as<double>();
+  const double* dg = diag_.as<double>();
+  double *x = x_.as<double>(), *rhs = rhs_.as<double>(), *r = r_.as<double>(), *z = z_.as<double>();
+  double *p = p_.as<double>(), *Ap = Ap_.as<double>();
+  k_o_init<<<g, kBlock, 0, s>>>(N, nf, mk, b, rhs, x, ctl, rb);
+  k_o_init2<<<g, kBlock, 0, s>>>(N, nf, mk, dg, rhs, r, z, ctl, rb);
+  k_o_zdot<<<g, kBlock, 0, s>>>(N, nf, mk, r, z, p, 1, ctl, rb);
+  NBL_LAUNCH_CHECK();
+  int enq = 0;
+  int batch = std::clamp(predicted_, 1, std::max(1, maxIters));
+  while (enq < maxIters) {
+    for (int q = 0; q < batch && enq < maxIters; ++q, ++enq) {
+      k_o_apply_dot<<<g, kBlock, 0, s>>>(N, mk, rp, cl, cf, p, Ap, ctl, rb);
+      k_o_update<<<g, kBlock, 0, s>>>(N, nf, mk, x, p, r, Ap, ctl, rb);
+      k_o_rnorm<<<g, kBlock, 0, s>>>(N, nf, mk, dg, r, z, ctl, rb);
+      k_o_zdot<<<g, kBlock, 0, s>>>(N, nf, mk, r, z, p, 0, ctl, rb);
+      k_o_pupdate<<<gridFor(N), kBlock, 0, s>>>(N, p, z, ctl);
+    }
+    NBL_LAUNCH_CHECK();
+    NBL_HIP(hipMemcpyAsync(hctl_, ctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, s));
+    NBL_HIP(hipStreamSynchronize(s));
+    if (hctl_[0].done) break;
+    batch = std::max(8, batch / 4);
+  }
+  k_o_xsum<<<g, kBlock, 0, s>>>(N, nf, mk, x, ctl, rb);
+  k_o_final<<<g, kBlock, 0, s>>>(N, nf, mk, rp, cl, cf, x, rhs, xo, ctl, rb);
+  NBL_LAUNCH_CHECK();
+  NBL_HIP(hipMemcpyAsync(hctl_, ctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, s));
+  NBL_HIP(hipStreamSynchronize(s));
+  const SolveCtl& h = hctl_[0];
+  if (h.converged && h.it > 0) predicted_ = h.it;
+  return nbl_poisson_report{h.it, h.resid, h.converged};
+}
+
+nbl_poisson_report OctreeDev::solve(const double* b, double* x, double tol, int maxIters) {
+  if (maxIters < 0) throw std::invalid_argument("max iterations must be >= 0");
+  // b is consumed by the first kernel of the solve, so the result may reuse its buffer
+  toSorted(b, t0_.as<double>());
+  const nbl_poisson_report rep = solveSorted(t0_.as<double>(), t0_.as<double>(), tol, maxIters, stream_);
+  fromSorted(t0_.as<double>(), x);
+  return rep;
+}
+
+}  // namespace nbl

@@ -1,0 +1,584 @@
+// solver.hip — host orchestration of the device pressure solvers and the
+// device-resident MAC run.  Kernel launches only; no per-element host work
+// after set-up.
+#include "solver.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <stdexcept>
+
+namespace nbl {
+
+// ---------------------------------------------------------------------------
+// Device buffers
+// ---------------------------------------------------------------------------
+DevBuf::DevBuf(nbl_ctx* ctx, size_t bytes) : n_(bytes), ctx_(ctx) {
+  if (bytes == 0) return;
+  NBL_HIP(hipMalloc(&p_, bytes));
+  if (ctx_) ctx_->bytes += bytes;
+}
+DevBuf::~DevBuf() {
+  if (p_) {
+    (void)hipFree(p_);
+    if (ctx_) ctx_->bytes -= n_;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Grid and hierarchy (multigrid.cpp:210-266, poisson.cpp:18-52)
+// ---------------------------------------------------------------------------
+HostGrid HostGrid::from(const nbl_grid_desc* d) {
+  if (!d) throw std::invalid_argument("grid descriptor is NULL");
+  if (d->nx < 1 || d->ny < 1 || d->nz < 1) throw std::invalid_argument("grid dimensions must be >= 1");
+  if (!(d->dx > 0 && d->dy > 0 && d->dz > 0)) throw std::invalid_argument("grid spacings must be > 0");
+  HostGrid g;
+  g.nx = d->nx;
+  g.ny = d->ny;
+  g.nz = d->nz;
+  for (int a = 0; a < 3; ++a) g.org[a] = d->origin[a];
+  g.dx = d->dx;
+  g.dy = d->dy;
+  g.dz = d->dz;
+  for (int f = 0; f < 6; ++f) {
+    g.face[f] = d->face[f];
+    if (g.face[f].type < NBL_BC_WALL || g.face[f].type > NBL_BC_PERIODIC)
+      throw std::invalid_argument("unknown boundary type");
+  }
+  for (int a = 0; a < 3; ++a)
+    if ((g.face[2 * a].type == NBL_BC_PERIODIC) != (g.face[2 * a + 1].type == NBL_BC_PERIODIC))
+      throw std::invalid_argument("periodic boundaries must be paired");
+  g.solid.assign(g.cells(), 0);
+  if (d->solid) std::memcpy(g.solid.data(), d->solid, g.cells());
+  return g;
+}
+
+bool hasOutlet(const HostGrid& g) {
+  for (const auto& f : g.face)
+    if (f.type == NBL_BC_OUTLET) return true;
+  return false;
+}
+
+static void fillCodes(HostLevel& L) {
+  const HostGrid& g = L.g;
+  const int n[3] = {g.nx, g.ny, g.nz};
+  L.code.assign(g.cells(), 0);
+  L.nf = 0;
+  for (int k = 0; k < g.nz; ++k)
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i) {
+        const size_t c = size_t(i) + size_t(g.nx) * (size_t(j) + size_t(g.ny) * k);
+        if (g.solid[c]) continue;
+        uint16_t code = kFluidBit;
+        ++L.nf;
+        for (int dir = 0; dir < 6; ++dir) {
+          const int axis = dir / 2, side = (dir & 1) ? 1 : -1;
+          if (axis == 2 && g.nz == 1) continue;
+          int q[3] = {i, j, k};
+          q[axis] += side;
+          int kind;
+          if (q[axis] >= 0 && q[axis] < n[axis]) {
+            kind = 1;
+          } else {
+            const int t = g.face[2 * axis + (side > 0 ? 1 : 0)].type;
+            if (t == NBL_BC_PERIODIC) {
+              q[axis] = (q[axis] + n[axis]) % n[axis];
+              kind = 1;
+            } else {
+              kind = t == NBL_BC_OUTLET ? 2 : 0;
+            }
+          }
+          if (kind == 1 && g.solid[size_t(q[0]) + size_t(g.nx) * (size_t(q[1]) + size_t(g.ny) * q[2])]) kind = 0;
+          code |= uint16_t(kind << (2 * dir));
+        }
+        L.code[c] = code;
+      }
+}
+
+std::vector<HostLevel> buildLevels(const HostGrid& g, bool multilevel) {
+  std::vector<HostLevel> lv;
+  HostLevel L0;
+  L0.g = g;
+  fillCodes(L0);
+  lv.push_back(std::move(L0));
+  while (multilevel && lv.size() < 8) {
+    const HostGrid& f = lv.back().g;
+    const int fx = f.nx >= 4 ? 2 : 1, fy = f.ny >= 4 ? 2 : 1, fz = f.nz >= 4 ? 2 : 1;
+    if (fx == 1 && fy == 1 && fz == 1) break;
+    HostLevel C;
+    C.f[0] = fx;
+    C.f[1] = fy;
+    C.f[2] = fz;
+    HostGrid& cg = C.g;
+    cg.nx = (f.nx + fx - 1) / fx;
+    cg.ny = (f.ny + fy - 1) / fy;
+    cg.nz = (f.nz + fz - 1) / fz;
+    cg.dx = f.dx * double(f.nx) / double(cg.nx);
+    cg.dy = f.dy * double(f.ny) / double(cg.ny);
+    cg.dz = f.dz * double(f.nz) / double(cg.nz);
+    for (int a = 0; a < 3; ++a) cg.org[a] = f.org[a];
+    for (int q = 0; q < 6; ++q) cg.face[q] = f.face[q];
+    std::vector<uint8_t> any(cg.cells(), 0);
+    const HostLevel& F = lv.back();
+    for (int k = 0; k < f.nz; ++k)
+      for (int j = 0; j < f.ny; ++j)
+        for (int i = 0; i < f.nx; ++i) {
+          const size_t c = size_t(i) + size_t(f.nx) * (size_t(j) + size_t(f.ny) * k);
+          if (F.code[c] & kFluidBit)
+            any[size_t(i / fx) + size_t(cg.nx) * (size_t(j / fy) + size_t(cg.ny) * (k / fz))] = 1;
+        }
+    cg.solid.assign(cg.cells(), 0);
+    for (size_t q = 0; q < cg.solid.size(); ++q) cg.solid[q] = any[q] ? 0 : 1;
+    fillCodes(C);
+    lv.push_back(std::move(C));
+    if (lv.back().g.cells() <= 2048) break;
+  }
+  return lv;
+}
+
+// ---------------------------------------------------------------------------
+// Pressure solver
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kSlotMeanZ = 3;
+int slotB(int l) { return 8 + 4 * l; }
+int slotX(int l, int buf) { return 8 + 4 * l + 1 + buf; }
+// Address of ctl->slot[i] computed without touching device memory.
+double* slotPtr(SolveCtl* ctl, int i) {
+  return reinterpret_cast<double*>(reinterpret_cast<char*>(ctl) + offsetof(SolveCtl, slot)) + i;
+}
+}  // namespace
+
+PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, int maxIters,
+                       const nbl_solver_opts& opts)
+    : ctx_(ctx), kind_(kind), tol_(tol), maxIt_(maxIters), opts_(opts) {
+  if (kind != 0 && kind != 1) throw std::invalid_argument("poisson kind must be 0 (mg-pcg) or 1 (cg-jacobi)");
+  if (maxIters < 0) throw std::invalid_argument("max iterations must be >= 0");
+  if (opts.pre_smooth < 1 || opts.post_smooth < 1)
+    throw std::invalid_argument("pre/post smoothing sweeps must be >= 1");
+  if (opts.precision < NBL_PREC_FP64 || opts.precision > NBL_PREC_MIXED_BF16)
+    throw std::invalid_argument("unknown precision");
+  singular_ = !hasOutlet(g);
+  mixed_ = kind == 0 && opts.precision != NBL_PREC_FP64;
+  hl_ = buildLevels(g, kind == 0);
+  const int nl = int(hl_.size());
+  const size_t ts = mixed_ ? sizeof(float) : sizeof(double);
+  for (int l = 0; l < nl; ++l) {
+    const HostLevel& H = hl_[l];
+    codes_.emplace_back(ctx, H.code.size() * sizeof(uint16_t));
+    NBL_HIP(hipMemcpy(codes_.back().as<void>(), H.code.data(), H.code.size() * sizeof(uint16_t),
+                      hipMemcpyHostToDevice));
+    MgLevel m{};
+    m.nx = H.g.nx;
+    m.ny = H.g.ny;
+    m.nz = H.g.nz;
+    m.n = (long long)H.g.cells();
+    m.h2[0] = 1.0 / (H.g.dx * H.g.dx);
+    m.h2[1] = 1.0 / (H.g.dy * H.g.dy);
+    m.h2[2] = 1.0 / (H.g.dz * H.g.dz);
+    m.code = codes_.back().as<uint16_t>();
+    m.fx = H.f[0];
+    m.fy = H.f[1];
+    m.fz = H.f[2];
+    m.nf = H.nf;
+    dl_.push_back(m);
+    const size_t n = H.g.cells();
+    lb_.emplace_back(ctx, l > 0 ? n * ts : 0);
+    lx0_.emplace_back(ctx, kind == 0 ? n * ts : 0);
+    lx1_.emplace_back(ctx, kind == 0 ? n * ts : 0);
+  }
+  if (kind == 0) coarseScr_ = DevBuf(ctx, 4 * hl_.back().g.cells() * ts);
+  const size_t n0 = hl_[0].g.cells();
+  if (mixed_) b0f_ = DevBuf(ctx, n0 * sizeof(float));
+  x_ = DevBuf(ctx, n0 * 8);
+  rhs_ = DevBuf(ctx, n0 * 8);
+  r_ = DevBuf(ctx, n0 * 8);
+  rn_ = DevBuf(ctx, n0 * 8);
+  z_ = DevBuf(ctx, n0 * 8);
+  p_ = DevBuf(ctx, n0 * 8);
+  Ap_ = DevBuf(ctx, n0 * 8);
+  ctl_ = DevBuf(ctx, sizeof(SolveCtl));
+  red_ = DevBuf(ctx, sizeof(double) * kMaxRedVals * kMaxRedBlocks);
+  ticket_ = DevBuf(ctx, 256);
+  NBL_HIP(hipMemset(ticket_.as<void>(), 0, 256));
+  NBL_HIP(hipMemset(ctl_.as<void>(), 0, sizeof(SolveCtl)));
+  NBL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hctl_), sizeof(SolveCtl) * 2, hipHostMallocDefault));
+  std::memset(hctl_, 0, sizeof(SolveCtl) * 2);
+  NBL_HIP(hipEventCreate(&e0_));
+  NBL_HIP(hipEventCreate(&e1_));
+  predicted_ = std::max(1, opts.check_every);
+}
+
+PoissonDev::~PoissonDev() {
+  if (hctl_) (void)hipHostFree(hctl_);
+  if (e0_) (void)hipEventDestroy(e0_);
+  if (e1_) (void)hipEventDestroy(e1_);
+}
+
+template <class T>
+void PoissonDev::vcycle(int l, const T* b, const double* mb, const T** xout, const double** mxout,
+                        hipStream_t s) {
+  SolveCtl* ctl = ctl_.as<SolveCtl>();
+  const MgLevel& L = dl_[l];
+  const RedBuf rb = red();
+  if (l + 1 == nlevels()) {
+    T* x = lx0_[l].as<T>();
+    launch_mg_coarse<T>(L, singular_, b, mb, x, coarseScr_.as<T>(), ctl, s);
+    *xout = x;
+    *mxout = nullptr;
+    return;
+  }
+  T* bufs[2] = {lx0_[l].as<T>(), lx1_[l].as<T>()};
+  double* ms[2] = {singular_ ? slotPtr(ctl, slotX(l, 0)) : nullptr, singular_ ? slotPtr(ctl, slotX(l, 1)) : nullptr};
+  int cur = -1;
+  for (int q = 0; q < opts_.pre_smooth; ++q) {
+    const int o = cur == 0 ? 1 : 0;
+    launch_mg_sweep<T>(L, b, mb, cur < 0 ? nullptr : bufs[cur], cur < 0 ? nullptr : ms[cur], cur < 0,
+                       nullptr, nullptr, nullptr, bufs[o], ms[o], rb, ctl, s);
+    cur = o;
+  }
+  const MgLevel& Lc = dl_[l + 1];
+  T* bc = lb_[l + 1].as<T>();
+  double* mbc = singular_ ? slotPtr(ctl, slotB(l + 1)) : nullptr;
+  launch_mg_resid_restrict<T>(L, Lc, b, mb, bufs[cur], ms[cur], bc, mbc, rb, ctl, s);
+  const T* ec = nullptr;
+  const double* mec = nullptr;
+  vcycle<T>(l + 1, bc, mbc, &ec, &mec, s);
+  for (int q = 0; q < opts_.post_smooth; ++q) {
+    const int o = 1 - cur;
+    launch_mg_sweep<T>(L, b, mb, bufs[cur], ms[cur], false, q == 0 ? ec : nullptr, q == 0 ? mec : nullptr,
+                       &Lc, bufs[o], ms[o], rb, ctl, s);
+    cur = o;
+  }
+  *xout = bufs[cur];
+  *mxout = ms[cur];
+}
+
+void PoissonDev::minv(const double* r, hipStream_t s) {
+  SolveCtl* ctl = ctl_.as<SolveCtl>();
+  double* mz = singular_ ? slotPtr(ctl, kSlotMeanZ) : nullptr;
+  if (mixed_) {
+    launch_to_level0<float>(dl_[0], r, b0f_.as<float>(), ctl, s);
+    const float* x = nullptr;
+    const double* mx = nullptr;
+    vcycle<float>(0, b0f_.as<float>(), nullptr, &x, &mx, s);
+    launch_mg_finish<float>(dl_[0], x, mx, z_.as<double>(), mz, red(), ctl, s);
+  } else {
+    const double* x = nullptr;
+    const double* mx = nullptr;
+    vcycle<double>(0, r, nullptr, &x, &mx, s);
+    launch_mg_finish<double>(dl_[0], x, mx, z_.as<double>(), mz, red(), ctl, s);
+  }
+  ++vcount_;
+}
+
+void PoissonDev::enqueueIteration(hipStream_t s) {
+  SolveCtl* ctl = ctl_.as<SolveCtl>();
+  const MgLevel& L = dl_[0];
+  const RedBuf rb = red();
+  double* x = x_.as<double>();
+  double* r = r_.as<double>();
+  double* z = z_.as<double>();
+  double* p = p_.as<double>();
+  double* Ap = Ap_.as<double>();
+  launch_fcg_apply(L, p, Ap, ctl, rb, s);
+  if (kind_ == 0) {
+    double* rn = rn_.as<double>();
+    launch_fcg_update(L, singular_, x, p, r, Ap, rn, ctl, rb, s);
+    if (singular_) launch_fcg_norm(L, rn, ctl, rb, s);
+    minv(rn, s);
+    launch_fcg_dots(L, z, singular_ ? slotPtr(ctl, kSlotMeanZ) : nullptr, rn, r, ctl, rb, s);
+    launch_fcg_pupdate(L, p, z, r, rn, ctl, s);
+  } else {
+    launch_cg_update(L, x, p, r, Ap, ctl, rb, s);
+    launch_cg_z(L, r, z, ctl, rb, false, p, s);
+    launch_cg_pupdate(L, p, z, ctl, s);
+  }
+}
+
+void PoissonDev::readCtl(hipStream_t s) {
+  NBL_HIP(hipMemcpyAsync(hctl_, ctl_.as<void>(), sizeof(SolveCtl), hipMemcpyDeviceToHost, s));
+  NBL_HIP(hipStreamSynchronize(s));
+}
+
+nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t s) {
+  SolveCtl* ctl = ctl_.as<SolveCtl>();
+  const MgLevel& L = dl_[0];
+  const RedBuf rb = red();
+  // fresh scalars: tolerance set, everything else cleared by the init kernels
+  hctl_[1] = SolveCtl{};
+  hctl_[1].tol = tol_;
+  NBL_HIP(hipMemcpyAsync(ctl_.as<void>(), &hctl_[1], sizeof(SolveCtl), hipMemcpyHostToDevice, s));
+  NBL_HIP(hipEventRecord(e0_, s));
+  vcount_ = 0;
+  double* x = x_.as<double>();
+  double* rhs = rhs_.as<double>();
+  double* r = r_.as<double>();
+  double* z = z_.as<double>();
+  double* p = p_.as<double>();
+  if (kind_ == 0) {
+    launch_fcg_init(L, singular_, true, d_b, rhs, x, ctl, rb, s);
+    launch_fcg_init2(L, singular_, true, rhs, r, ctl, rb, s);
+    minv(r, s);
+    launch_fcg_start(L, z, singular_ ? slotPtr(ctl, kSlotMeanZ) : nullptr, r, z, p, ctl, rb, s);
+  } else {
+    launch_fcg_init(L, singular_, false, d_b, rhs, x, ctl, rb, s);
+    launch_fcg_init2(L, singular_, false, rhs, r, ctl, rb, s);
+    launch_cg_z(L, r, z, ctl, rb, true, p, s);
+  }
+  int enq = 0;
+  int batch = std::clamp(predicted_, 1, std::max(1, maxIt_));
+  const int step = std::max(1, opts_.check_every);
+  while (enq < maxIt_) {
+    for (int q = 0; q < batch && enq < maxIt_; ++q, ++enq) enqueueIteration(s);
+    readCtl(s);
+    if (hctl_[0].done) break;
+    batch = step;
+  }
+  launch_fcg_final(L, singular_, x, rhs, d_x, ctl, rb, kind_ == 0 ? tol_ : 1e-6, s);
+  NBL_HIP(hipEventRecord(e1_, s));
+  readCtl(s);
+  float ms = 0.f;
+  NBL_HIP(hipEventElapsedTime(&ms, e0_, e1_));
+  lastMs_ = ms;
+  lastV_ = vcount_;
+  const SolveCtl& h = hctl_[0];
+  if (h.converged && h.it > 0) predicted_ = h.it;
+  return nbl_poisson_report{h.it, h.resid, h.converged};
+}
+
+// ---------------------------------------------------------------------------
+// MAC run
+// ---------------------------------------------------------------------------
+RunDev::RunDev(nbl_ctx* ctx, const HostGrid& g, const nbl_flow_cfg& cfg, const nbl_solver_opts& opts)
+    : ctx_(ctx), g_(g), cfg_(cfg) {
+  if (!(cfg.reynolds > 0)) throw std::invalid_argument("reynolds must be > 0");
+  NBL_HIP(hipStreamCreate(&stream_));
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  sz_[0] = size_t(nx + 1) * ny * nz;
+  sz_[1] = size_t(nx) * (ny + 1) * nz;
+  sz_[2] = size_t(nx) * ny * (nz + 1);
+  sz_[3] = size_t(nx) * ny * nz;
+  solid_ = DevBuf(ctx, sz_[3]);
+  NBL_HIP(hipMemcpy(solid_.as<void>(), g.solid.data(), sz_[3], hipMemcpyHostToDevice));
+  bool anySolid = false;
+  for (uint8_t s : g.solid) anySolid |= s != 0;
+  // faces touching a solid cell, per component (ns_solver.cpp:277-308)
+  std::vector<int> list;
+  auto sol = [&](int i, int j, int k) {
+    if (i < 0 || j < 0 || k < 0 || i >= nx || j >= ny || k >= nz) return false;
+    return g.solid[size_t(i) + size_t(nx) * (size_t(j) + size_t(ny) * k)] != 0;
+  };
+  if (anySolid) {
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i <= nx; ++i)
+          if (sol(i - 1, j, k) || sol(i, j, k)) list.push_back(int(i + (nx + 1) * (j + ny * k)));
+    solidCounts_[0] = int(list.size());
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j <= ny; ++j)
+        for (int i = 0; i < nx; ++i)
+          if (sol(i, j - 1, k) || sol(i, j, k)) list.push_back(int(i + nx * (j + (ny + 1) * k)));
+    solidCounts_[1] = int(list.size()) - solidCounts_[0];
+    if (nz > 1)
+      for (int k = 0; k <= nz; ++k)
+        for (int j = 0; j < ny; ++j)
+          for (int i = 0; i < nx; ++i)
+            if (sol(i, j, k - 1) || sol(i, j, k)) list.push_back(int(i + nx * (j + ny * k)));
+    solidCounts_[2] = int(list.size()) - solidCounts_[0] - solidCounts_[1];
+    solidList_ = DevBuf(ctx, list.size() * sizeof(int));
+    NBL_HIP(hipMemcpy(solidList_.as<void>(), list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  DevGrid& d = args_.g;
+  d.nx = nx;
+  d.ny = ny;
+  d.nz = nz;
+  d.dx = g.dx;
+  d.dy = g.dy;
+  d.dz = g.dz;
+  d.ox = g.org[0];
+  d.oy = g.org[1];
+  d.oz = g.org[2];
+  for (int f = 0; f < 6; ++f) {
+    d.bc[f] = g.face[f].type;
+    for (int a = 0; a < 3; ++a) d.wall[f][a] = g.face[f].wall_velocity[a];
+    d.profile[f] = g.face[f].profile;
+    d.inletSpeed[f] = g.face[f].inlet_speed;
+    d.inletAxis[f] = g.face[f].inlet_axis;
+  }
+  d.solid = solid_.as<uint8_t>();
+  d.hasSolid = anySolid ? 1 : 0;
+  args_.nu = cfg.velocity_scale * cfg.ref_length / cfg.reynolds;
+  for (int a = 0; a < 3; ++a) args_.bodyForce[a] = cfg.body_force[a];
+  args_.convection = cfg.convection;
+  args_.cfl = cfg.cfl;
+  args_.velocityScale = cfg.velocity_scale;
+  args_.density = cfg.density;
+  args_.refLength = cfg.ref_length;
+  args_.refArea = cfg.ref_area;
+  args_.tolContinuity = cfg.tol_continuity;
+  args_.tolMass = cfg.tol_mass;
+  for (int q = 0; q < 3; ++q) {
+    bufA_[q] = DevBuf(ctx, sz_[q] * 8);
+    bufB_[q] = DevBuf(ctx, sz_[q] * 8);
+    buf0_[q] = DevBuf(ctx, sz_[q] * 8);
+    cur_[q] = bufA_[q].as<double>();
+    alt_[q] = bufB_[q].as<double>();
+    NBL_HIP(hipMemset(cur_[q], 0, sz_[q] * 8));
+  }
+  p_ = DevBuf(ctx, sz_[3] * 8);
+  b_ = DevBuf(ctx, sz_[3] * 8);
+  phi_ = DevBuf(ctx, sz_[3] * 8);
+  NBL_HIP(hipMemset(p_.as<void>(), 0, sz_[3] * 8));
+  sc_ = DevBuf(ctx, sizeof(StepCtl));
+  rep_ = DevBuf(ctx, sizeof(DevReport));
+  red_ = DevBuf(ctx, sizeof(double) * kMaxRedVals * kMaxRedBlocks);
+  ticket_ = DevBuf(ctx, 256);
+  NBL_HIP(hipMemset(ticket_.as<void>(), 0, 256));
+  NBL_HIP(hipMemset(sc_.as<void>(), 0, sizeof(StepCtl)));
+  NBL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hsc_), sizeof(StepCtl), hipHostMallocDefault));
+  NBL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hrep_), sizeof(DevReport), hipHostMallocDefault));
+  std::memset(hsc_, 0, sizeof(StepCtl));
+  nbl_solver_opts o = opts;
+  poisson_ = std::make_unique<PoissonDev>(ctx, g, 0, cfg.poisson_tol, cfg.poisson_max_iters, o);
+}
+
+RunDev::~RunDev() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  poisson_.reset();
+  if (hsc_) (void)hipHostFree(hsc_);
+  if (hrep_) (void)hipHostFree(hrep_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void RunDev::bc(bool outflow) {
+  launch_bc_planes(args_, cur_[0], cur_[1], cur_[2], outflow, stream_);
+  if (args_.g.hasSolid)
+    launch_bc_solid(args_, cur_[0], cur_[1], cur_[2], solidList_.as<int>(), solidCounts_, stream_);
+}
+
+void RunDev::initialize() {  // ns_solver.cpp:95-112
+  for (int q = 0; q < 3; ++q) NBL_HIP(hipMemsetAsync(cur_[q], 0, sz_[q] * 8, stream_));
+  NBL_HIP(hipMemsetAsync(p_.as<void>(), 0, sz_[3] * 8, stream_));
+  for (int f = 0; f < 6; ++f)
+    if (g_.face[f].type == NBL_BC_INLET && g_.face[f].inlet_axis == 0) {
+      launch_fill(cur_[0], (long long)sz_[0], g_.face[f].inlet_speed, stream_);
+      break;
+    }
+  bc(true);
+  hsc_->step = 0;
+  hsc_->t = 0.0;
+  hsc_->dt = 0.0;
+  NBL_HIP(hipMemcpyAsync(sc_.as<void>(), hsc_, sizeof(StepCtl), hipMemcpyHostToDevice, stream_));
+  NBL_HIP(hipStreamSynchronize(stream_));
+  dtLast_ = 0.0;
+}
+
+void RunDev::setState(const double* u, size_t nu, const double* v, size_t nv, const double* w, size_t nw,
+                      const double* p, size_t np, int step, double t, double dtLast) {
+  if (nu != sz_[0] || nv != sz_[1] || nw != sz_[2] || np != sz_[3])
+    throw std::invalid_argument("NSSolver::setState: size mismatch");
+  if (!u || !v || !w || !p) throw std::invalid_argument("setState: NULL array");
+  const double* src[3] = {u, v, w};
+  for (int q = 0; q < 3; ++q)
+    NBL_HIP(hipMemcpyAsync(cur_[q], src[q], sz_[q] * 8, hipMemcpyHostToDevice, stream_));
+  NBL_HIP(hipMemcpyAsync(p_.as<void>(), p, sz_[3] * 8, hipMemcpyHostToDevice, stream_));
+  hsc_->step = step;
+  hsc_->t = t;
+  hsc_->dt = dtLast;
+  NBL_HIP(hipMemcpyAsync(sc_.as<void>(), hsc_, sizeof(StepCtl), hipMemcpyHostToDevice, stream_));
+  NBL_HIP(hipStreamSynchronize(stream_));
+  dtLast_ = dtLast;
+}
+
+void RunDev::getState(double* u, double* v, double* w, double* p, int* step, double* t, double* dtLast) {
+  double* dst[3] = {u, v, w};
+  for (int q = 0; q < 3; ++q)
+    if (dst[q]) NBL_HIP(hipMemcpyAsync(dst[q], cur_[q], sz_[q] * 8, hipMemcpyDeviceToHost, stream_));
+  if (p) NBL_HIP(hipMemcpyAsync(p, p_.as<void>(), sz_[3] * 8, hipMemcpyDeviceToHost, stream_));
+  NBL_HIP(hipMemcpyAsync(hsc_, sc_.as<void>(), sizeof(StepCtl), hipMemcpyDeviceToHost, stream_));
+  NBL_HIP(hipStreamSynchronize(stream_));
+  if (step) *step = hsc_->step;
+  if (t) *t = hsc_->t;
+  if (dtLast) *dtLast = dtLast_;
+}
+
+double RunDev::computeDt() {
+  const RedBuf rb{red_.as<double>(), ticket_.as<unsigned>()};
+  launch_cfl(args_, cur_[0], cur_[1], cur_[2], sc_.as<StepCtl>(), 0.0, rb, stream_);
+  NBL_HIP(hipMemcpyAsync(hsc_, sc_.as<void>(), sizeof(StepCtl), hipMemcpyDeviceToHost, stream_));
+  NBL_HIP(hipStreamSynchronize(stream_));
+  return hsc_->dt;
+}
+
+void RunDev::project(int stageIdx, nbl_poisson_report& rep) {  // ns_solver.cpp:475-579
+  (void)stageIdx;
+  bc(true);
+  launch_divergence(args_, cur_[0], cur_[1], cur_[2], b_.as<double>(), stream_);
+  rep = poisson_->solve(b_.as<double>(), phi_.as<double>(), stream_);
+  launch_correct(args_, cur_[0], cur_[1], cur_[2], phi_.as<double>(), p_.as<double>(), sc_.as<StepCtl>(),
+                 stream_);
+}
+
+nbl_step_report RunDev::step(double dt) {  // ns_solver.cpp:765-824
+  const RedBuf rb{red_.as<double>(), ticket_.as<unsigned>()};
+  StepCtl* sc = sc_.as<StepCtl>();
+  launch_cfl(args_, cur_[0], cur_[1], cur_[2], sc, dt > 0.0 ? dt : 0.0, rb, stream_);
+  double* u0[3] = {buf0_[0].as<double>(), buf0_[1].as<double>(), buf0_[2].as<double>()};
+  for (int q = 0; q < 3; ++q)
+    NBL_HIP(hipMemcpyAsync(u0[q], cur_[q], sz_[q] * 8, hipMemcpyDeviceToDevice, stream_));
+  bc(true);
+  nbl_poisson_report pr[3];
+  for (int stage = 1; stage <= 3; ++stage) {
+    launch_rhs_rk(args_, u0[0], u0[1], u0[2], cur_[0], cur_[1], cur_[2], alt_[0], alt_[1], alt_[2], stage, sc,
+                  stream_);
+    std::swap(cur_, alt_);
+    project(stage, pr[stage - 1]);
+  }
+  launch_diagnostics(args_, cur_[0], cur_[1], cur_[2], u0[0], u0[1], u0[2], p_.as<double>(), sc,
+                     rep_.as<DevReport>(), rb, stream_);
+  NBL_HIP(hipMemcpyAsync(hrep_, rep_.as<void>(), sizeof(DevReport), hipMemcpyDeviceToHost, stream_));
+  NBL_HIP(hipStreamSynchronize(stream_));
+  nbl_step_report r{};
+  r.step = hrep_->step;
+  r.t = hrep_->t;
+  r.dt = hrep_->dt;
+  r.cfl = hrep_->cfl;
+  r.momentum_residual = hrep_->mom;
+  r.continuity_residual = hrep_->divl2;
+  r.divergence_max = hrep_->divmax;
+  r.mass_error = hrep_->mass;
+  r.poisson_iters = pr[0].iterations + pr[1].iterations + pr[2].iterations;
+  r.poisson_residual = std::max({pr[0].residual, pr[1].residual, pr[2].residual});
+  r.cd = hrep_->cd;
+  r.cl = hrep_->cl;
+  r.cell_count = hrep_->cells;
+  r.accepted = hrep_->accepted;
+  dtLast_ = r.dt;
+  return r;
+}
+
+void RunDev::rhs(double* ru, double* rv, double* rw) {
+  // R of the current state after applyVelocityBC (ns_solver.cpp:775-776), on a copy
+  for (int q = 0; q < 3; ++q)
+    NBL_HIP(hipMemcpyAsync(alt_[q], cur_[q], sz_[q] * 8, hipMemcpyDeviceToDevice, stream_));
+  launch_bc_planes(args_, alt_[0], alt_[1], alt_[2], true, stream_);
+  if (args_.g.hasSolid)
+    launch_bc_solid(args_, alt_[0], alt_[1], alt_[2], solidList_.as<int>(), solidCounts_, stream_);
+  double* out[3] = {buf0_[0].as<double>(), buf0_[1].as<double>(), buf0_[2].as<double>()};
+  launch_rhs_rk(args_, alt_[0], alt_[1], alt_[2], alt_[0], alt_[1], alt_[2], out[0], out[1], out[2], 0,
+                sc_.as<StepCtl>(), stream_);
+  double* dst[3] = {ru, rv, rw};
+  for (int q = 0; q < 3; ++q) NBL_HIP(hipMemcpyAsync(dst[q], out[q], sz_[q] * 8, hipMemcpyDeviceToHost, stream_));
+  NBL_HIP(hipStreamSynchronize(stream_));
+}
+
+void RunDev::cellVelocity(int comp, double* out) {
+  if (comp < 0 || comp > 2) throw std::invalid_argument("component must be 0, 1 or 2");
+  launch_cell_velocity(args_, cur_[0], cur_[1], cur_[2], comp, b_.as<double>(), stream_);
+  NBL_HIP(hipMemcpyAsync(out, b_.as<void>(), sz_[3] * 8, hipMemcpyDeviceToHost, stream_));
+  NBL_HIP(hipStreamSynchronize(stream_));
+}
+
+}  // namespace nbl

@@ -1,0 +1,181 @@
+// solver.hpp — host-side objects behind the C-ABI: device context, the
+// uniform pressure solvers (MG-PCG and Jacobi-CG) and the MAC run.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "dev.hpp"
+#include "nbl_api.h"
+
+struct nbl_ctx {
+  int device = 0;
+  std::atomic<uint64_t> bytes{0};
+};
+
+namespace nbl {
+
+// Device allocation owned by an object and accounted to its context.
+class DevBuf {
+ public:
+  DevBuf() = default;
+  DevBuf(nbl_ctx* ctx, size_t bytes);
+  ~DevBuf();
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept { swap(o); }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    swap(o);
+    return *this;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p_);
+  }
+  size_t bytes() const { return n_; }
+
+ private:
+  void swap(DevBuf& o) noexcept {
+    std::swap(p_, o.p_);
+    std::swap(n_, o.n_);
+    std::swap(ctx_, o.ctx_);
+  }
+  void* p_ = nullptr;
+  size_t n_ = 0;
+  nbl_ctx* ctx_ = nullptr;
+};
+
+// Host copy of a uniform grid (reference UniformGrid).
+struct HostGrid {
+  int nx = 1, ny = 1, nz = 1;
+  double org[3] = {0, 0, 0};
+  double dx = 1, dy = 1, dz = 1;
+  nbl_face_bc face[6]{};
+  std::vector<uint8_t> solid;
+  size_t cells() const { return size_t(nx) * ny * nz; }
+  static HostGrid from(const nbl_grid_desc* d);
+};
+
+// One level of the agglomeration hierarchy, host side.
+struct HostLevel {
+  HostGrid g;
+  std::vector<uint16_t> code;
+  long long nf = 0;
+  int f[3] = {1, 1, 1};
+};
+
+// Builds the levels as MgPoisson does (multigrid.cpp:210-266).
+std::vector<HostLevel> buildLevels(const HostGrid& g, bool multilevel);
+bool hasOutlet(const HostGrid& g);
+
+// Uniform pressure solver on the device.  kind 0 = flexible CG with one
+// V(pre,post) cycle per iteration (MgPoisson), kind 1 = Jacobi CG (CgPoisson).
+class PoissonDev {
+ public:
+  PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, int maxIters,
+             const nbl_solver_opts& opts);
+  ~PoissonDev();
+  // Solve with device buffers on `stream` (b and x may be any device arrays of
+  // length n).  Returns the reference's Report.
+  nbl_poisson_report solve(const double* d_b, double* d_x, hipStream_t stream);
+  const std::vector<HostLevel>& levels() const { return hl_; }
+  const MgLevel& level(int l) const { return dl_[l]; }
+  int nlevels() const { return int(dl_.size()); }
+  bool singular() const { return singular_; }
+  int kind() const { return kind_; }
+  double lastMs() const { return lastMs_; }
+  int lastVcycles() const { return lastV_; }
+  RedBuf red() const { return RedBuf{red_.as<double>(), ticket_.as<unsigned>()}; }
+
+ private:
+  template <class T>
+  void vcycle(int l, const T* b, const double* mb, const T** xout, const double** mxout, hipStream_t s);
+  void minv(const double* r, hipStream_t s);
+  void enqueueIteration(hipStream_t s);
+  void readCtl(hipStream_t s);
+
+  nbl_ctx* ctx_;
+  int kind_;
+  double tol_;
+  int maxIt_;
+  nbl_solver_opts opts_;
+  bool singular_ = false;
+  bool mixed_ = false;
+  std::vector<HostLevel> hl_;
+  std::vector<MgLevel> dl_;
+  std::vector<DevBuf> codes_;
+  // per level: b (levels >= 1), two x buffers, coarse scratch
+  std::vector<DevBuf> lb_, lx0_, lx1_;
+  DevBuf coarseScr_;
+  DevBuf b0f_;  // FP32 level-0 right-hand side (mixed)
+  // flexible-CG vectors (FP64)
+  DevBuf x_, rhs_, r_, rn_, z_, p_, Ap_;
+  DevBuf ctl_, red_, ticket_;
+  SolveCtl* hctl_ = nullptr;  // pinned mirror
+  hipEvent_t e0_ = nullptr, e1_ = nullptr;
+  double lastMs_ = 0;
+  int lastV_ = 0;
+  int predicted_ = 1;
+  int vcount_ = 0;
+};
+
+// A device-resident MAC Navier-Stokes run (NSSolver).
+class RunDev {
+ public:
+  RunDev(nbl_ctx* ctx, const HostGrid& g, const nbl_flow_cfg& cfg, const nbl_solver_opts& opts);
+  ~RunDev();
+  void initialize();
+  void setState(const double* u, size_t nu, const double* v, size_t nv, const double* w, size_t nw,
+                const double* p, size_t np, int step, double t, double dtLast);
+  void getState(double* u, double* v, double* w, double* p, int* step, double* t, double* dtLast);
+  double computeDt();
+  nbl_step_report step(double dt);
+  void rhs(double* ru, double* rv, double* rw);
+  void cellVelocity(int comp, double* out);
+  double* u() const { return cur_[0]; }
+  double* v() const { return cur_[1]; }
+  double* w() const { return cur_[2]; }
+  double* p() const { return p_.as<double>(); }
+  hipStream_t stream() const { return stream_; }
+  size_t sizes(int q) const { return sz_[q]; }
+
+ private:
+  void bc(bool outflow);
+  void project(int stageIdx, nbl_poisson_report& rep);
+
+  nbl_ctx* ctx_;
+  HostGrid g_;
+  nbl_flow_cfg cfg_;
+  NsArgs args_{};
+  hipStream_t stream_ = nullptr;
+  size_t sz_[4];
+  DevBuf solid_, solidList_;
+  int solidCounts_[3] = {0, 0, 0};
+  DevBuf bufA_[3], bufB_[3], buf0_[3], p_, b_, phi_;
+  double* cur_[3];
+  double* alt_[3];
+  DevBuf sc_, rep_, red_, ticket_;
+  StepCtl* hsc_ = nullptr;
+  DevReport* hrep_ = nullptr;
+  std::unique_ptr<PoissonDev> poisson_;
+  double dtLast_ = 0.0;
+};
+
+}  // namespace nbl
+
+struct nbl_run {
+  nbl_ctx* ctx;
+  std::unique_ptr<nbl::RunDev> impl;
+};
+struct nbl_poisson {
+  nbl_ctx* ctx;
+  std::unique_ptr<nbl::PoissonDev> impl;
+  hipStream_t stream = nullptr;
+  nbl::HostGrid grid;
+};

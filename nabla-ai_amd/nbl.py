@@ -1,0 +1,429 @@
+"""ctypes view of the nbl C-ABI (include/nbl_api.h) for tests and bench.py.
+
+The product's host layer is C++20 (nabla/flow headers in this package); this
+module only mirrors the POD descriptors and binds the C entry points so that
+Python tests can drive the HIP library the way the reference's Catch2 tests
+drive nabla_core in-process (core/tests/test_flow.cpp:103-105).
+
+`load()` raises if the HIP library is not built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+LIB_PATH = PKG / "lib" / "libnabla_hip.so"
+
+BC_WALL, BC_INLET, BC_OUTLET, BC_PERIODIC = 0, 1, 2, 3
+INLET_UNIFORM, INLET_PARABOLIC = 0, 1
+CONV_CENTRAL, CONV_WENO5 = 0, 1
+PREC_FP64, PREC_MIXED, PREC_MIXED_BF16 = 0, 1, 2
+
+
+class FaceBC(C.Structure):
+    _fields_ = [("type", C.c_int), ("wall_velocity", C.c_double * 3), ("profile", C.c_int),
+                ("inlet_speed", C.c_double), ("inlet_axis", C.c_int)]
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("origin", C.c_double * 3),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("face", FaceBC * 6), ("solid", C.POINTER(C.c_uint8))]
+
+
+class FlowCfg(C.Structure):
+    _fields_ = [("reynolds", C.c_double), ("ref_length", C.c_double),
+                ("velocity_scale", C.c_double), ("density", C.c_double), ("cfl", C.c_double),
+                ("body_force", C.c_double * 3), ("convection", C.c_int),
+                ("poisson_tol", C.c_double), ("poisson_max_iters", C.c_int),
+                ("tol_continuity", C.c_double), ("tol_mass", C.c_double),
+                ("ref_area", C.c_double)]
+
+
+class SolverOpts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("pre_smooth", C.c_int), ("post_smooth", C.c_int),
+                ("check_every", C.c_int)]
+
+
+class StepReport(C.Structure):
+    _fields_ = [("step", C.c_int), ("t", C.c_double), ("dt", C.c_double), ("cfl", C.c_double),
+                ("momentum_residual", C.c_double), ("continuity_residual", C.c_double),
+                ("divergence_max", C.c_double), ("mass_error", C.c_double),
+                ("poisson_iters", C.c_int), ("poisson_residual", C.c_double),
+                ("cd", C.c_double), ("cl", C.c_double), ("cell_count", C.c_uint64),
+                ("accepted", C.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class PoissonReport(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("residual", C.c_double), ("converged", C.c_int)]
+
+
+P_D = C.POINTER(C.c_double)
+P_U8 = C.POINTER(C.c_uint8)
+P_U64 = C.POINTER(C.c_uint64)
+P_I32 = C.POINTER(C.c_int32)
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(P_D)
+
+
+# --------------------------------------------------------------------------
+# Grid / config builders (defaults of FlowConfig, ns_solver.hpp:14-32)
+# --------------------------------------------------------------------------
+class Grid:
+    """Host description of a uniform MAC grid; keeps the solid array alive."""
+
+    def __init__(self, nx, ny, nz, dx, dy=None, dz=None, origin=(0.0, 0.0, 0.0),
+                 faces=None, solid=None):
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.dx = float(dx)
+        self.dy = float(dx if dy is None else dy)
+        self.dz = float(dx if dz is None else dz)
+        self.origin = tuple(float(o) for o in origin)
+        self.faces = faces if faces is not None else [dict(type=BC_WALL) for _ in range(6)]
+        n = self.nx * self.ny * self.nz
+        self.solid = (np.zeros(n, np.uint8) if solid is None
+                      else np.ascontiguousarray(solid, dtype=np.uint8).reshape(-1))
+        assert self.solid.size == n
+
+    @property
+    def cells(self):
+        return self.nx * self.ny * self.nz
+
+    @property
+    def shapes(self):
+        nx, ny, nz = self.nx, self.ny, self.nz
+        return ((nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1), nx * ny * nz)
+
+    def desc(self) -> GridDesc:
+        d = GridDesc()
+        d.nx, d.ny, d.nz = self.nx, self.ny, self.nz
+        d.origin[:] = self.origin
+        d.dx, d.dy, d.dz = self.dx, self.dy, self.dz
+        for f, spec in enumerate(self.faces):
+            d.face[f].type = spec.get("type", BC_WALL)
+            d.face[f].wall_velocity[:] = spec.get("wall_velocity", (0.0, 0.0, 0.0))
+            d.face[f].profile = spec.get("profile", INLET_UNIFORM)
+            d.face[f].inlet_speed = spec.get("inlet_speed", 0.0)
+            d.face[f].inlet_axis = spec.get("inlet_axis", 0)
+        d.solid = self.solid.ctypes.data_as(P_U8)
+        d._keep = self.solid  # noqa: keep the buffer alive with the descriptor
+        return d
+
+
+def flow_cfg(**kw) -> FlowCfg:
+    c = FlowCfg()
+    c.reynolds, c.ref_length, c.velocity_scale, c.density, c.cfl = 100.0, 1.0, 1.0, 1.0, 0.8
+    c.body_force[:] = (0.0, 0.0, 0.0)
+    c.convection = CONV_CENTRAL
+    c.poisson_tol, c.poisson_max_iters = 1e-9, 5000
+    c.tol_continuity, c.tol_mass, c.ref_area = 1e-7, 1e-8, 0.0
+    for k, v in kw.items():
+        if k == "body_force":
+            c.body_force[:] = v
+        else:
+            setattr(c, k, v)
+    return c
+
+
+def solver_opts(precision=PREC_FP64, pre=2, post=2, check_every=1) -> SolverOpts:
+    o = SolverOpts()
+    o.precision, o.pre_smooth, o.post_smooth, o.check_every = precision, pre, post, check_every
+    return o
+
+
+# --------------------------------------------------------------------------
+# Library loading
+# --------------------------------------------------------------------------
+_lib = None
+
+
+def _proto(lib, name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+
+
+def load(path: os.PathLike | None = None):
+    """Load the HIP library; raises if it is missing (no fallback path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"nbl: HIP library not built ({p}); run __graft_entry__.build()")
+    lib = C.CDLL(str(p))
+    V = C.c_void_p
+    i, d, sz = C.c_int, C.c_double, C.c_size_t
+    _proto(lib, "nbl_last_error", C.c_char_p)
+    _proto(lib, "nbl_version", C.c_char_p)
+    _proto(lib, "nbl_flow_cfg_default", None, C.POINTER(FlowCfg))
+    _proto(lib, "nbl_solver_opts_default", None, C.POINTER(SolverOpts))
+    _proto(lib, "nbl_ctx_create", i, i, C.POINTER(V))
+    _proto(lib, "nbl_ctx_destroy", i, V)
+    _proto(lib, "nbl_ctx_bytes_in_use", i, V, P_U64)
+    _proto(lib, "nbl_run_create", i, V, C.POINTER(GridDesc), C.POINTER(FlowCfg),
+           C.POINTER(SolverOpts), C.POINTER(V))
+    _proto(lib, "nbl_run_destroy", i, V)
+    _proto(lib, "nbl_run_initialize", i, V)
+    _proto(lib, "nbl_run_set_state", i, V, P_D, sz, P_D, sz, P_D, sz, P_D, sz, i, d, d)
+    _proto(lib, "nbl_run_get_state", i, V, P_D, P_D, P_D, P_D, C.POINTER(i), P_D, P_D)
+    _proto(lib, "nbl_run_compute_dt", i, V, P_D)
+    _proto(lib, "nbl_run_step", i, V, d, C.POINTER(StepReport))
+    _proto(lib, "nbl_run_step_many", i, V, i, C.POINTER(StepReport))
+    _proto(lib, "nbl_run_device_state", i, V, C.POINTER(P_D), C.POINTER(P_D), C.POINTER(P_D),
+           C.POINTER(P_D))
+    _proto(lib, "nbl_run_rhs", i, V, P_D, P_D, P_D)
+    _proto(lib, "nbl_run_cell_velocity", i, V, i, P_D)
+    _proto(lib, "nbl_poisson_create", i, V, C.POINTER(GridDesc), i, d, i, C.POINTER(SolverOpts),
+           C.POINTER(V))
+    _proto(lib, "nbl_poisson_destroy", i, V)
+    _proto(lib, "nbl_poisson_solve", i, V, P_D, P_D, C.POINTER(PoissonReport))
+    _proto(lib, "nbl_poisson_solve_device", i, V, C.c_void_p, C.c_void_p, C.POINTER(PoissonReport))
+    _proto(lib, "nbl_poisson_apply", i, V, P_D, P_D)
+    _proto(lib, "nbl_poisson_diagonal", i, V, P_D)
+    _proto(lib, "nbl_poisson_levels", i, V, C.POINTER(i), C.POINTER(i))
+    _proto(lib, "nbl_poisson_smooth_once", i, V, P_D, P_D)
+    _proto(lib, "nbl_poisson_restrict", i, V, P_D, P_D)
+    _proto(lib, "nbl_poisson_prolong", i, V, P_D, P_D)
+    _proto(lib, "nbl_poisson_last_timing", i, V, P_D, C.POINTER(i))
+    _proto(lib, "nbl_octree_create", i, V, P_U64, P_U8, P_U8, sz, P_D, P_D, i, C.POINTER(V))
+    _proto(lib, "nbl_octree_destroy", i, V)
+    _proto(lib, "nbl_octree_count", i, V, C.POINTER(sz), C.POINTER(sz))
+    _proto(lib, "nbl_octree_permutation", i, V, P_U64)
+    _proto(lib, "nbl_octree_face_table", i, V, P_U8, P_I32)
+    _proto(lib, "nbl_octree_apply", i, V, P_D, P_D)
+    _proto(lib, "nbl_octree_diagonal", i, V, P_D)
+    _proto(lib, "nbl_octree_poisson_solve", i, V, P_D, P_D, d, i, C.POINTER(PoissonReport))
+    _lib = lib
+    return lib
+
+
+class NblError(RuntimeError):
+    pass
+
+
+def check(rc: int):
+    if rc != 0:
+        msg = _lib.nbl_last_error().decode() if _lib else "?"
+        raise NblError(f"nbl error {rc}: {msg}")
+
+
+class Ctx:
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        self.h = C.c_void_p()
+        check(self.lib.nbl_ctx_create(device, C.byref(self.h)))
+
+    def bytes_in_use(self) -> int:
+        v = C.c_uint64()
+        check(self.lib.nbl_ctx_bytes_in_use(self.h, C.byref(v)))
+        return v.value
+
+    def close(self):
+        if self.h:
+            check(self.lib.nbl_ctx_destroy(self.h))
+            self.h = C.c_void_p()
+
+
+class Run:
+    """nbl_run: device-resident NSSolver (ns_solver.hpp:58-89)."""
+
+    def __init__(self, ctx: Ctx, grid: Grid, cfg: FlowCfg, opts: SolverOpts | None = None):
+        self.lib, self.grid = ctx.lib, grid
+        self._desc = grid.desc()
+        self.h = C.c_void_p()
+        check(self.lib.nbl_run_create(ctx.h, C.byref(self._desc), C.byref(cfg),
+                                      C.byref(opts) if opts is not None else None,
+                                      C.byref(self.h)))
+
+    def initialize(self):
+        check(self.lib.nbl_run_initialize(self.h))
+
+    def set_state(self, u, v, w, p, step=0, t=0.0, dt_last=0.0):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w, p)]
+        check(self.lib.nbl_run_set_state(self.h, dptr(arrs[0]), arrs[0].size, dptr(arrs[1]),
+                                         arrs[1].size, dptr(arrs[2]), arrs[2].size,
+                                         dptr(arrs[3]), arrs[3].size, step, t, dt_last))
+
+    def get_state(self):
+        su, sv, sw, sp = self.grid.shapes
+        u, v, w, p = (np.empty(s) for s in (su, sv, sw, sp))
+        st, t, dt = C.c_int(), C.c_double(), C.c_double()
+        check(self.lib.nbl_run_get_state(self.h, dptr(u), dptr(v), dptr(w), dptr(p),
+                                         C.byref(st), C.byref(t), C.byref(dt)))
+        return u, v, w, p, st.value, t.value, dt.value
+
+    def compute_dt(self) -> float:
+        v = C.c_double()
+        check(self.lib.nbl_run_compute_dt(self.h, C.byref(v)))
+        return v.value
+
+    def step(self, dt: float = 0.0) -> StepReport:
+        r = StepReport()
+        check(self.lib.nbl_run_step(self.h, dt, C.byref(r)))
+        return r
+
+    def step_many(self, n: int):
+        reps = (StepReport * n)()
+        check(self.lib.nbl_run_step_many(self.h, n, reps))
+        return list(reps)
+
+    def rhs(self):
+        su, sv, sw, _ = self.grid.shapes
+        a, b, c = np.empty(su), np.empty(sv), np.empty(sw)
+        check(self.lib.nbl_run_rhs(self.h, dptr(a), dptr(b), dptr(c)))
+        return a, b, c
+
+    def cell_velocity(self, comp: int):
+        out = np.empty(self.grid.cells)
+        check(self.lib.nbl_run_cell_velocity(self.h, comp, dptr(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            check(self.lib.nbl_run_destroy(self.h))
+            self.h = C.c_void_p()
+
+
+class Poisson:
+    """nbl_poisson: the PoissonSolver plugin (poisson.hpp:20-32)."""
+
+    MG, CG = 0, 1
+
+    def __init__(self, ctx: Ctx, grid: Grid, kind=0, tol=1e-9, max_iters=100,
+                 opts: SolverOpts | None = None):
+        self.lib, self.grid = ctx.lib, grid
+        self._desc = grid.desc()
+        self.h = C.c_void_p()
+        check(self.lib.nbl_poisson_create(ctx.h, C.byref(self._desc), kind, tol, max_iters,
+                                          C.byref(opts) if opts is not None else None,
+                                          C.byref(self.h)))
+
+    def solve(self, b):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.empty(self.grid.cells)
+        rep = PoissonReport()
+        check(self.lib.nbl_poisson_solve(self.h, dptr(b), dptr(x), C.byref(rep)))
+        return x, rep
+
+    def solve_device(self, d_b: int, d_x: int):
+        rep = PoissonReport()
+        check(self.lib.nbl_poisson_solve_device(self.h, C.c_void_p(d_b), C.c_void_p(d_x),
+                                                C.byref(rep)))
+        return rep
+
+    def apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.empty(self.grid.cells)
+        check(self.lib.nbl_poisson_apply(self.h, dptr(x), dptr(out)))
+        return out
+
+    def diagonal(self):
+        out = np.empty(self.grid.cells)
+        check(self.lib.nbl_poisson_diagonal(self.h, dptr(out)))
+        return out
+
+    def levels(self):
+        n = C.c_int()
+        dims = (C.c_int * 24)()
+        check(self.lib.nbl_poisson_levels(self.h, C.byref(n), dims))
+        return [tuple(dims[3 * l:3 * l + 3]) for l in range(n.value)]
+
+    def smooth_once(self, b, x):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.array(x, dtype=np.float64)
+        check(self.lib.nbl_poisson_smooth_once(self.h, dptr(b), dptr(x)))
+        return x
+
+    def restrict(self, rf):
+        lv = self.levels()
+        rf = np.ascontiguousarray(rf, dtype=np.float64)
+        rc = np.empty(int(np.prod(lv[1])))
+        check(self.lib.nbl_poisson_restrict(self.h, dptr(rf), dptr(rc)))
+        return rc
+
+    def prolong(self, ec, xf):
+        ec = np.ascontiguousarray(ec, dtype=np.float64)
+        xf = np.array(xf, dtype=np.float64)
+        check(self.lib.nbl_poisson_prolong(self.h, dptr(ec), dptr(xf)))
+        return xf
+
+    def last_timing(self):
+        ms, vc = C.c_double(), C.c_int()
+        check(self.lib.nbl_poisson_last_timing(self.h, C.byref(ms), C.byref(vc)))
+        return ms.value, vc.value
+
+    def close(self):
+        if self.h:
+            check(self.lib.nbl_poisson_destroy(self.h))
+            self.h = C.c_void_p()
+
+
+class Octree:
+    """nbl_octree: Morton-sorted leaves, face tables and OctreePoisson."""
+
+    def __init__(self, ctx: Ctx, keys, levels, masks, origin, extent, max_level):
+        self.lib = ctx.lib
+        self.keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        self.levels = np.ascontiguousarray(levels, dtype=np.uint8)
+        self.masks = np.ascontiguousarray(masks, dtype=np.uint8)
+        self.n = self.keys.size
+        o = (C.c_double * 3)(*origin)
+        e = (C.c_double * 3)(*extent)
+        self.h = C.c_void_p()
+        check(self.lib.nbl_octree_create(ctx.h, self.keys.ctypes.data_as(P_U64),
+                                         self.levels.ctypes.data_as(P_U8),
+                                         self.masks.ctypes.data_as(P_U8), self.n, o, e,
+                                         max_level, C.byref(self.h)))
+
+    def counts(self):
+        n, nnz = C.c_size_t(), C.c_size_t()
+        check(self.lib.nbl_octree_count(self.h, C.byref(n), C.byref(nnz)))
+        return n.value, nnz.value
+
+    def permutation(self):
+        perm = np.empty(self.n, np.uint64)
+        check(self.lib.nbl_octree_permutation(self.h, perm.ctypes.data_as(P_U64)))
+        return perm
+
+    def face_table(self):
+        cnt = np.empty(6 * self.n, np.uint8)
+        nbr = np.empty(24 * self.n, np.int32)
+        check(self.lib.nbl_octree_face_table(self.h, cnt.ctypes.data_as(P_U8),
+                                             nbr.ctypes.data_as(P_I32)))
+        return cnt.reshape(self.n, 6), nbr.reshape(self.n, 6, 4)
+
+    def apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.empty(self.n)
+        check(self.lib.nbl_octree_apply(self.h, dptr(x), dptr(out)))
+        return out
+
+    def diagonal(self):
+        out = np.empty(self.n)
+        check(self.lib.nbl_octree_diagonal(self.h, dptr(out)))
+        return out
+
+    def solve(self, b, tol=1e-9, max_iters=2000):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.empty(self.n)
+        rep = PoissonReport()
+        check(self.lib.nbl_octree_poisson_solve(self.h, dptr(b), dptr(x), tol, max_iters,
+                                                C.byref(rep)))
+        return x, rep
+
+    def close(self):
+        if self.h:
+            check(self.lib.nbl_octree_destroy(self.h))
+            self.h = C.c_void_p()

@@ -1,0 +1,263 @@
+"""ctypes binding of the oracle — TEST INFRASTRUCTURE ONLY.
+
+Two libraries share one signature set (oracle/cpu/oracle.h):
+  * ``orc_*`` in oracle/lib/liboracle.so — the serial FP64 restatement of the
+    reference hot path (the checker);
+  * ``ref_*`` in oracle/_ref/libnabla_ref.so — the reference's own sources
+    compiled where they lie plus oracle/ref_shim.cpp (only where it was built).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import
+this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE.parent / "nabla-ai_amd"))
+import nbl  # noqa: E402  (POD descriptors only)
+
+ORACLE_LIB = HERE / "lib" / "liboracle.so"
+REF_LIB = HERE / "_ref" / "libnabla_ref.so"
+
+P_D, P_U8, P_U64 = nbl.P_D, nbl.P_U8, nbl.P_U64
+dptr = nbl.dptr
+
+
+class OracleLib:
+    def __init__(self, kind: str = "orc"):
+        path = ORACLE_LIB if kind == "orc" else REF_LIB
+        if not path.exists():
+            raise FileNotFoundError(path)
+        self.kind = kind
+        self.lib = C.CDLL(str(path))
+        V, i, d, sz = C.c_void_p, C.c_int, C.c_double, C.c_size_t
+        G, F = C.POINTER(nbl.GridDesc), C.POINTER(nbl.FlowCfg)
+        R, PR = C.POINTER(nbl.StepReport), C.POINTER(nbl.PoissonReport)
+        spec = {
+            "ns_create": (V, G, F), "ns_destroy": (None, V), "ns_initialize": (None, V),
+            "ns_set_state": (i, V, P_D, sz, P_D, sz, P_D, sz, P_D, sz, i, d, d),
+            "ns_get_state": (None, V, P_D, P_D, P_D, P_D, C.POINTER(i), P_D, P_D),
+            "ns_compute_dt": (d, V), "ns_step": (None, V, d, R),
+            "ns_cell_velocity": (None, V, i, P_D),
+            "apply_a": (None, G, P_D, P_D), "diagonal": (None, G, P_D),
+            "mg_solve": (None, G, P_D, P_D, d, i, PR), "cg_solve": (None, G, P_D, P_D, d, i, PR),
+            "morton_encode": (C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32),
+            "morton_decode": (None, C.c_uint64, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                              C.POINTER(C.c_uint32)),
+            "oct_create": (V, C.POINTER(d), C.POINTER(d), i), "oct_destroy": (None, V),
+            "oct_count": (sz, V), "oct_refine_uniform": (None, V, i),
+            "oct_refine_box": (None, V, C.POINTER(d), C.POINTER(d), i),
+            "oct_refine_key": (i, V, C.c_uint64), "oct_set_mask": (None, V, sz, C.c_uint8),
+            "oct_dump": (None, V, P_U64, P_U8, P_U8),
+            "oct_face_neighbors": (i, V, sz, i, P_U64), "oct_is_balanced": (i, V),
+            "oct_centers": (None, V, P_D), "oct_apply": (None, V, P_D, P_D),
+            "oct_diagonal": (None, V, P_D), "oct_solve": (None, V, P_D, P_D, d, i, PR),
+        }
+        if kind == "orc":
+            spec.update({
+                "ns_rhs": (None, V, P_D, P_D, P_D), "mg_levels": (i, G, C.POINTER(i)),
+                "mg_smooth_once": (None, G, P_D, P_D), "mg_restrict": (None, G, P_D, P_D),
+                "mg_prolong": (None, G, P_D, P_D),
+            })
+        for name, (res, *args) in spec.items():
+            f = getattr(self.lib, f"{kind}_{name}")
+            f.restype, f.argtypes = res, list(args)
+            setattr(self, name, f)
+
+    # ---- uniform ----------------------------------------------------------
+    def apply(self, grid: nbl.Grid, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.empty(grid.cells)
+        d = grid.desc()
+        self.apply_a(C.byref(d), dptr(x), dptr(out))
+        return out
+
+    def diag(self, grid: nbl.Grid):
+        out = np.empty(grid.cells)
+        d = grid.desc()
+        self.diagonal(C.byref(d), dptr(out))
+        return out
+
+    def mg(self, grid: nbl.Grid, b, tol=1e-9, max_cycles=100):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.empty(grid.cells)
+        rep = nbl.PoissonReport()
+        d = grid.desc()
+        self.mg_solve(C.byref(d), dptr(b), dptr(x), tol, max_cycles, C.byref(rep))
+        return x, rep
+
+    def cg(self, grid: nbl.Grid, b, tol=1e-10, max_iters=5000):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.empty(grid.cells)
+        rep = nbl.PoissonReport()
+        d = grid.desc()
+        self.cg_solve(C.byref(d), dptr(b), dptr(x), tol, max_iters, C.byref(rep))
+        return x, rep
+
+    def levels(self, grid: nbl.Grid):
+        dims = (C.c_int * 24)()
+        d = grid.desc()
+        n = self.mg_levels(C.byref(d), dims)
+        return [tuple(dims[3 * l:3 * l + 3]) for l in range(n)]
+
+    def smooth(self, grid, b, x):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.array(x, dtype=np.float64)
+        d = grid.desc()
+        self.mg_smooth_once(C.byref(d), dptr(b), dptr(x))
+        return x
+
+    def restrict(self, grid, rf):
+        lv = self.levels(grid)
+        rf = np.ascontiguousarray(rf, dtype=np.float64)
+        rc = np.zeros(int(np.prod(lv[1])))
+        d = grid.desc()
+        self.mg_restrict(C.byref(d), dptr(rf), dptr(rc))
+        return rc
+
+    def prolong(self, grid, ec, xf):
+        ec = np.ascontiguousarray(ec, dtype=np.float64)
+        xf = np.array(xf, dtype=np.float64)
+        d = grid.desc()
+        self.mg_prolong(C.byref(d), dptr(ec), dptr(xf))
+        return xf
+
+    def ns(self, grid: nbl.Grid, cfg: nbl.FlowCfg):
+        return NS(self, grid, cfg)
+
+    def octree(self, origin, extent, max_level):
+        return Oct(self, origin, extent, max_level)
+
+
+class NS:
+    def __init__(self, o: OracleLib, grid: nbl.Grid, cfg: nbl.FlowCfg):
+        self.o, self.grid = o, grid
+        self._d = grid.desc()
+        self._cfg = cfg
+        self.h = o.ns_create(C.byref(self._d), C.byref(cfg))
+
+    def initialize(self):
+        self.o.ns_initialize(self.h)
+
+    def set_state(self, u, v, w, p, step=0, t=0.0, dt_last=0.0):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (u, v, w, p)]
+        rc = self.o.ns_set_state(self.h, dptr(a[0]), a[0].size, dptr(a[1]), a[1].size,
+                                 dptr(a[2]), a[2].size, dptr(a[3]), a[3].size, step, t, dt_last)
+        if rc != 0:
+            raise RuntimeError("setState: size mismatch")
+
+    def get_state(self):
+        su, sv, sw, sp = self.grid.shapes
+        u, v, w, p = (np.empty(s) for s in (su, sv, sw, sp))
+        st, t, dt = C.c_int(), C.c_double(), C.c_double()
+        self.o.ns_get_state(self.h, dptr(u), dptr(v), dptr(w), dptr(p), C.byref(st),
+                            C.byref(t), C.byref(dt))
+        return u, v, w, p, st.value, t.value, dt.value
+
+    def compute_dt(self):
+        return self.o.ns_compute_dt(self.h)
+
+    def step(self, dt=0.0):
+        r = nbl.StepReport()
+        self.o.ns_step(self.h, dt, C.byref(r))
+        return r
+
+    def rhs(self):
+        su, sv, sw, _ = self.grid.shapes
+        a, b, c = np.empty(su), np.empty(sv), np.empty(sw)
+        self.o.ns_rhs(self.h, dptr(a), dptr(b), dptr(c))
+        return a, b, c
+
+    def cell_velocity(self, comp):
+        out = np.empty(self.grid.cells)
+        self.o.ns_cell_velocity(self.h, comp, dptr(out))
+        return out
+
+    def close(self):
+        if self.h:
+            self.o.ns_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+
+class Oct:
+    def __init__(self, o: OracleLib, origin, extent, max_level):
+        self.o = o
+        self.origin, self.extent, self.max_level = tuple(origin), tuple(extent), max_level
+        self.h = o.oct_create((C.c_double * 3)(*origin), (C.c_double * 3)(*extent), max_level)
+        if not self.h:
+            raise ValueError("octree: invalid arguments")
+
+    @property
+    def n(self):
+        return self.o.oct_count(self.h)
+
+    def refine_uniform(self, times):
+        self.o.oct_refine_uniform(self.h, times)
+
+    def refine_box(self, lo, hi, times=1):
+        self.o.oct_refine_box(self.h, (C.c_double * 3)(*lo), (C.c_double * 3)(*hi), times)
+
+    def refine_key(self, key):
+        if self.o.oct_refine_key(self.h, int(key)) != 0:
+            raise RuntimeError("refine failed")
+
+    def set_mask(self, i, m):
+        self.o.oct_set_mask(self.h, int(i), int(m))
+
+    def dump(self):
+        n = self.n
+        k = np.empty(n, np.uint64)
+        l = np.empty(n, np.uint8)
+        m = np.empty(n, np.uint8)
+        self.o.oct_dump(self.h, k.ctypes.data_as(P_U64), l.ctypes.data_as(P_U8),
+                        m.ctypes.data_as(P_U8))
+        return k, l, m
+
+    def neighbors(self, i, d):
+        buf = (C.c_uint64 * 4)()
+        c = self.o.oct_face_neighbors(self.h, int(i), int(d), buf)
+        return [int(buf[m]) for m in range(c)]
+
+    def balanced(self):
+        return bool(self.o.oct_is_balanced(self.h))
+
+    def centers(self):
+        out = np.empty(3 * self.n)
+        self.o.oct_centers(self.h, dptr(out))
+        return out.reshape(-1, 3)
+
+    def apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.empty(self.n)
+        self.o.oct_apply(self.h, dptr(x), dptr(out))
+        return out
+
+    def diagonal(self):
+        out = np.empty(self.n)
+        self.o.oct_diagonal(self.h, dptr(out))
+        return out
+
+    def solve(self, b, tol=1e-9, max_iters=2000):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.empty(self.n)
+        rep = nbl.PoissonReport()
+        self.o.oct_solve(self.h, dptr(b), dptr(x), tol, max_iters, C.byref(rep))
+        return x, rep
+
+    def close(self):
+        if self.h:
+            self.o.oct_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+
+def available(kind: str) -> bool:
+    return (ORACLE_LIB if kind == "orc" else REF_LIB).exists()

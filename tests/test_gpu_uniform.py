@@ -1,0 +1,207 @@
+"""GPU parity of the uniform MAC path against the oracle (oracle/cpu), through
+the C-ABI.
+
+Bars:
+* per-element kernels (applyA, diagonal, one smoother sweep, restriction,
+  prolongation, the momentum right-hand side) are bit-identical: the kernels
+  evaluate the reference's expressions in the reference's order with FP
+  contraction off;
+* solves and time steps contain reductions whose summation order differs from
+  the reference's serial loops, so they are compared with tolerances: the
+  same iteration counts, solutions within rel-L2 1e-9 (solve tolerance
+  1e-10), fields within rel-L2 1e-10 per step sequence.
+"""
+import numpy as np
+import pytest
+
+import cases
+import nbl
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    d = np.linalg.norm(a - b)
+    s = np.linalg.norm(b)
+    return d / s if s > 0 else d
+
+
+def grids():
+    out = []
+    g, _ = cases.lid_cavity(12, 100.0)
+    out.append(("lid2d", g))
+    g, _ = cases.channel(16, 8, 100.0)
+    out.append(("chan2d", g))
+    g, _ = cases.taylor_green(8)
+    out.append(("tgv8", g))
+    faces = [dict(type=nbl.BC_INLET, inlet_speed=1.0), dict(type=nbl.BC_OUTLET),
+             dict(type=nbl.BC_WALL), dict(type=nbl.BC_WALL, wall_velocity=(0.3, 0.0, 0.1)),
+             dict(type=nbl.BC_PERIODIC), dict(type=nbl.BC_PERIODIC)]
+    solid = np.zeros((6, 8, 12), np.uint8)
+    solid[2:4, 0:3, 4:7] = 1
+    out.append(("box3d", nbl.Grid(12, 8, 6, 0.1, 0.12, 0.15, faces=faces, solid=solid)))
+    faces2 = [dict(type=nbl.BC_WALL), dict(type=nbl.BC_WALL),
+              dict(type=nbl.BC_INLET, profile=nbl.INLET_PARABOLIC, inlet_speed=0.7, inlet_axis=1),
+              dict(type=nbl.BC_OUTLET), dict(type=nbl.BC_WALL), dict(type=nbl.BC_OUTLET)]
+    out.append(("odd3d", nbl.Grid(10, 9, 7, 0.2, faces=faces2)))
+    g, _ = cases.cube_channel(nx=48, ny=16, nz=16, res=4)
+    out.append(("cube48", g))
+    return out
+
+
+GRIDS = grids()
+IDS = [n for n, _ in GRIDS]
+
+
+@pytest.mark.parametrize("name,g", GRIDS, ids=IDS)
+def test_operator_pieces_bitwise(ctx, orc, name, g):
+    ps = nbl.Poisson(ctx, g, kind=0, tol=1e-10, max_iters=100)
+    try:
+        x = cases.normal(5, g.cells)
+        b = cases.normal(6, g.cells)
+        assert np.array_equal(ps.apply(x), orc.apply(g, x))
+        assert np.array_equal(ps.diagonal(), orc.diag(g))
+        assert ps.levels() == orc.levels(g)
+        assert np.array_equal(ps.smooth_once(b, x), orc.smooth(g, b, x))
+        if len(ps.levels()) > 1:
+            assert np.array_equal(ps.restrict(b), orc.restrict(g, b))
+            ec = cases.normal(8, int(np.prod(ps.levels()[1])))
+            assert np.array_equal(ps.prolong(ec, x), orc.prolong(g, ec, x))
+    finally:
+        ps.close()
+
+
+@pytest.mark.parametrize("name,g", GRIDS, ids=IDS)
+def test_mg_pcg_matches_oracle(ctx, orc, name, g):
+    b = cases.poisson_rhs(g, 11)
+    ps = nbl.Poisson(ctx, g, kind=0, tol=1e-10, max_iters=100)
+    try:
+        x, rep = ps.solve(b)
+        xo, ro = orc.mg(g, b, 1e-10, 100)
+        assert rep.converged and ro.converged
+        assert abs(rep.iterations - ro.iterations) <= 1
+        assert rep.residual <= 1e-10 * 1.01
+        assert rel(x, xo) < 1e-9
+        # a second solve on the same object reuses the hierarchy and agrees
+        x2, rep2 = ps.solve(b)
+        assert np.array_equal(x2, x) and rep2.iterations == rep.iterations
+    finally:
+        ps.close()
+
+
+@pytest.mark.parametrize("name,g", GRIDS[:4], ids=IDS[:4])
+def test_cg_jacobi_matches_oracle(ctx, orc, name, g):
+    b = cases.poisson_rhs(g, 13)
+    ps = nbl.Poisson(ctx, g, kind=1, tol=1e-10, max_iters=3000)
+    try:
+        x, rep = ps.solve(b)
+        xo, ro = orc.cg(g, b, 1e-10, 3000)
+        assert abs(rep.iterations - ro.iterations) <= 2
+        assert rel(x, xo) < 1e-8
+        assert rep.residual < 1e-9
+    finally:
+        ps.close()
+
+
+def test_mg_manufactured_solid_block(ctx):
+    """core/tests/test_flow.cpp:63-117: 32x16x16 inlet/outlet/walls/periodic with
+    a wall-mounted block; x_exact random on fluid; <= 50 cycles, err < 1e-6."""
+    faces = [dict(type=nbl.BC_INLET), dict(type=nbl.BC_OUTLET), dict(type=nbl.BC_WALL),
+             dict(type=nbl.BC_WALL), dict(type=nbl.BC_PERIODIC), dict(type=nbl.BC_PERIODIC)]
+    solid = np.zeros((16, 16, 32), np.uint8)
+    solid[6:10, 0:6, 12:18] = 1
+    g = nbl.Grid(32, 16, 16, 1.0 / 16, faces=faces, solid=solid)
+    xe = 2.0 * cases.uniform01(12345, g.cells) - 1.0
+    xe[g.solid == 1] = 0.0
+    ps = nbl.Poisson(ctx, g, kind=0, tol=1e-10, max_iters=100)
+    b = ps.apply(xe)
+    x, rep = ps.solve(b)
+    ps.close()
+    assert rep.converged and rep.residual < 1e-8 and rep.iterations <= 50
+    assert np.sqrt(np.mean((x - xe) ** 2)) < 1e-6
+
+
+@pytest.mark.parametrize("conv", [nbl.CONV_CENTRAL, nbl.CONV_WENO5], ids=["central", "weno5"])
+@pytest.mark.parametrize("name,g", GRIDS, ids=IDS)
+def test_rhs_bitwise(ctx, orc, name, g, conv):
+    cfg = nbl.flow_cfg(reynolds=200.0, convection=conv, body_force=(0.01, -0.02, 0.003))
+    st = [cases.normal(s, n) * 0.3 for s, n in zip((1, 2, 3, 4), g.shapes)]
+    run = nbl.Run(ctx, g, cfg)
+    o = orc.ns(g, cfg)
+    try:
+        run.set_state(*st)
+        o.set_state(*st)
+        for a, b in zip(run.rhs(), o.rhs()):
+            assert np.array_equal(a, b)
+        for comp in range(3):
+            assert np.array_equal(run.cell_velocity(comp), o.cell_velocity(comp))
+    finally:
+        run.close()
+
+
+@pytest.mark.parametrize("conv", [nbl.CONV_CENTRAL, nbl.CONV_WENO5], ids=["central", "weno5"])
+@pytest.mark.parametrize("name,g", GRIDS, ids=IDS)
+def test_steps_match_oracle(ctx, orc, name, g, conv):
+    cfg = nbl.flow_cfg(reynolds=200.0, convection=conv, poisson_tol=1e-10, cfl=0.5,
+                       body_force=(0.01, 0.0, 0.0))
+    st = [cases.normal(s, n) * 0.3 for s, n in zip((1, 2, 3, 4), g.shapes)]
+    run = nbl.Run(ctx, g, cfg)
+    o = orc.ns(g, cfg)
+    try:
+        run.set_state(*st)
+        o.set_state(*st)
+        assert run.compute_dt() == o.compute_dt()
+        for _ in range(3):
+            r, q = run.step(), o.step()
+            assert r.step == q.step and r.poisson_iters == q.poisson_iters
+            assert r.dt == pytest.approx(q.dt, rel=1e-12)
+            assert r.t == pytest.approx(q.t, rel=1e-12)
+            assert r.continuity_residual == pytest.approx(q.continuity_residual, rel=1e-3, abs=1e-13)
+            assert r.momentum_residual == pytest.approx(q.momentum_residual, rel=1e-9)
+            assert r.mass_error == pytest.approx(q.mass_error, rel=1e-6, abs=1e-13)
+            assert r.cd == pytest.approx(q.cd, rel=1e-9, abs=1e-12)
+            assert r.cl == pytest.approx(q.cl, rel=1e-9, abs=1e-12)
+            assert r.accepted == q.accepted
+        a, b = run.get_state(), o.get_state()
+        for fa, fb in zip(a[:4], b[:4]):
+            assert rel(fa, fb) < 1e-10
+        assert a[4] == b[4]
+    finally:
+        run.close()
+
+
+def test_initialize_matches_oracle(ctx, orc):
+    g, cfg = cases.wall_mounted_cube(4, 500.0)
+    run = nbl.Run(ctx, g, cfg)
+    o = orc.ns(g, cfg)
+    run.initialize()
+    o.initialize()
+    for fa, fb in zip(run.get_state()[:4], o.get_state()[:4]):
+        assert np.array_equal(fa, fb)
+    run.close()
+
+
+def test_set_state_rejects_wrong_sizes(ctx):
+    g, cfg = cases.lid_cavity(8, 100.0)
+    run = nbl.Run(ctx, g, cfg)
+    su, sv, sw, sp = g.shapes
+    with pytest.raises(nbl.NblError):
+        run.set_state(np.zeros(su + 1), np.zeros(sv), np.zeros(sw), np.zeros(sp))
+    run.close()
+
+
+def test_tgv16_ten_steps(ctx, orc):
+    g, cfg = cases.taylor_green(16)
+    st = cases.taylor_green_state(g)
+    run = nbl.Run(ctx, g, cfg)
+    o = orc.ns(g, cfg)
+    run.set_state(*st)
+    o.set_state(*st)
+    for _ in range(10):
+        r, q = run.step(), o.step()
+        assert r.poisson_iters == q.poisson_iters
+    a, b = run.get_state(), o.get_state()
+    for fa, fb in zip(a[:4], b[:4]):
+        assert rel(fa, fb) < 1e-10
+    run.close()
