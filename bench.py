@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Benchmark of the nabla_solve per-step hot path on one MI355X.
+
+Workload (one "step" = one SSP-RK3 MAC time step, three projections):
+192x64x64 cube channel (h = 1, 16 cells per h), uniform inlet U = 1 with a
+seeded 1 % perturbation, no-slip floor/ceiling, periodic span, convective
+outlet, staircase cube [3,4]x[0,1]x[1.5,2.5], Re_h 500, CFL 0.7, WENO-5,
+MG-PCG pressure solve to 1e-8 relative residual; FP64 throughout (the plain
+path).  The same steps are also run with the FP32 V-cycle inside the FP64
+flexible CG ("mixed") and reported beside it.
+
+Prints ONE JSON line (rank 0).  `value` is whole-job Mcell-steps/s of the FP64
+path with the state resident in HBM.  `roofline` is the finest-level smoother
+sweep, timed live with HIP events on its own stream; `cpu_baseline` is the
+reference solver (oracle/_ref, else the oracle restatement) on one host core
+over a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "nabla-ai_amd"))
+sys.path.insert(0, str(ROOT / "oracle"))
+
+import numpy as np  # noqa: E402
+
+import cases  # noqa: E402
+import nbl  # noqa: E402
+
+METRIC = "Mcell-steps/s + MG-PCG solve ms at rel. resid 1e-8; p50/p99 served-step latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def run_steps(ctx, g, cfg, st, precision, warmup, steps):
+    run = nbl.Run(ctx, g, cfg, nbl.solver_opts(precision=precision))
+    run.set_state(*st)
+    for _ in range(warmup):
+        run.step()
+    reps, solve_ms, iters = [], [], []
+    t0 = time.perf_counter()
+    for s in range(steps):
+        r = run.step()
+        ms, it, _vc = run.last_solves()
+        reps.append(r.as_dict())
+        solve_ms.extend(ms)
+        iters.extend(it)
+        if (s + 1) % max(1, steps // 5) == 0:
+            log(f"prec={precision} step {s + 1}/{steps}")
+    elapsed = time.perf_counter() - t0
+    state = run.get_state()
+    run.close()
+    return reps, solve_ms, iters, elapsed, state
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def roofline(ctx, g, reps=200):
+    ps = nbl.Poisson(ctx, g, kind=0, tol=1e-8, max_iters=5000)
+    ms, by = ps.profile(0, reps)
+    ps.close()
+    ach = by / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_sweep_w<double> (finest-level damped-Jacobi sweep + fused fluid sum)",
+            "avg_launch_us": round(ms * 1e3, 3), "bytes_per_launch": by, "cells": g.cells}
+
+
+def cpu_baseline(g, cfg, st, steps):
+    import oracle_py
+    kind = "reference" if oracle_py.available("ref") else "port"
+    lib = oracle_py.OracleLib("ref" if kind == "reference" else "orc")
+    try:
+        old = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {min(old)})
+    except (AttributeError, OSError):
+        old = None
+    try:
+        s = lib.ns(g, cfg)
+        s.set_state(*st)
+        t0 = time.perf_counter()
+        for q in range(steps):
+            s.step()
+            log(f"cpu baseline step {q + 1}/{steps}")
+        el = time.perf_counter() - t0
+        s.close()
+    finally:
+        if old is not None:
+            os.sched_setaffinity(0, old)
+    return {"value": round(g.cells * steps / el / 1e6, 5), "unit": "Mcell-steps/s", "cores": 1,
+            "kind": kind, "seconds_per_step": round(el / steps, 3),
+            "sample": f"{steps} steps of the same workload from the same initial state, 1 thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only the timed FP64 steps (for profiler runs)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(independent runs shard across GPUs without exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    g, cfg = cases.cube_channel()
+    st = cases.cube_channel_state(g)
+    ctx = nbl.Ctx(0)
+    log(f"workload 192x64x64 cube channel, {g.cells} cells, warmup {a.warmup}, steps {a.steps}")
+    reps, solve_ms, iters, el, state = run_steps(ctx, g, cfg, st, nbl.PREC_FP64, a.warmup, a.steps)
+    ms_step = el / a.steps * 1e3
+    value = g.cells * a.steps / el / 1e6
+    log(f"fp64: {ms_step:.3f} ms/step, {value:.2f} Mcell-steps/s")
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded splitmix64 perturbation of the inflow)",
+        "config": {"workload": "cube-channel 192x64x64, Re_h 500, WENO-5, SSP-RK3, MG-PCG tol 1e-8",
+                   "cells": g.cells, "precision": "fp64"},
+        "mg_pcg_solve_ms_median": round(statistics.median(solve_ms), 4),
+        "poisson_iters_per_solve_mean": round(float(np.mean(iters)), 2),
+        "max_poisson_residual": max(r["poisson_residual"] for r in reps),
+        "served_p50_ms": None, "served_p99_ms": None,
+    }
+    if a.profile_only:
+        print(json.dumps(out), flush=True)
+        return
+    if not a.no_mixed:
+        mreps, mms, mit, mel, mstate = run_steps(ctx, g, cfg, st, nbl.PREC_MIXED, a.warmup, a.steps)
+        mval = g.cells * a.steps / mel / 1e6
+        cd = np.array([r["cd"] for r in reps])
+        mcd = np.array([r["cd"] for r in mreps])
+        cl = np.array([r["cl"] for r in reps])
+        mcl = np.array([r["cl"] for r in mreps])
+        out["mixed_fp32_vcycle"] = {
+            "value": round(mval, 3), "ms_per_step": round(mel / a.steps * 1e3, 4),
+            "speedup_vs_fp64": round(mval / value, 3),
+            "mg_pcg_solve_ms_median": round(statistics.median(mms), 4),
+            "poisson_iters_per_solve_mean": round(float(np.mean(mit)), 2),
+            "max_poisson_residual": max(r["poisson_residual"] for r in mreps),
+            "rel_l2_vs_fp64": {k: rel(x, y) for k, x, y in zip("uvwp", mstate[:4], state[:4])},
+            "cd_max_rel_err": float(np.max(np.abs(mcd - cd) / np.maximum(np.abs(cd), 1e-30))),
+            "cl_max_abs_err": float(np.max(np.abs(mcl - cl))),
+        }
+        log(f"mixed: {mval:.2f} Mcell-steps/s")
+    out["roofline"] = roofline(ctx, g)
+    big = nbl.Grid(256, 256, 256, 1.0 / 256)
+    out["roofline_256cubed"] = roofline(ctx, big, reps=50)
+    ctx.close()
+    if not a.no_cpu and a.cpu_steps > 0:
+        out["cpu_baseline"] = cpu_baseline(g, cfg, st, a.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -177,6 +177,11 @@ int nbl_run_rhs(nbl_run* run, double* ru, double* rv, double* rw);
 /* NSSolver::cellVelocity(component) (ns_solver.cpp:690-701). */
 int nbl_run_cell_velocity(nbl_run* run, int component, double* out);
 
+/* Per-stage pressure solves of the last step: device milliseconds, flexible-CG
+ * iterations and V-cycles (any pointer may be NULL; arrays of 3).  No
+ * reference counterpart (the reference keeps no timings). */
+int nbl_run_last_solves(nbl_run* run, double* ms, int* iters, int* vcycles);
+
 /* ---- pressure Poisson plugin (PoissonSolver, poisson.hpp:20-32) ----------
  * kind: 0 = MG-PCG "mg-pcg" (MgPoisson, multigrid.cpp:206-355),
  *       1 = Jacobi-PCG "cg-jacobi" (CgPoisson, poisson.cpp:136-252).
@@ -207,6 +212,16 @@ int nbl_poisson_prolong(nbl_poisson* ps, const double* ec, double* xf);
 /* Timings of the last solve: milliseconds of device time and the number of
  * preconditioner applications. */
 int nbl_poisson_last_timing(nbl_poisson* ps, double* ms, int* vcycles);
+/* Measurement hook (no reference counterpart): launch one kernel of the solve
+ * `reps` times back to back on the solver's stream between two HIP events and
+ * return the average duration of one launch (ms) and its algorithmic HBM
+ * bytes (every array the kernel must touch counted once):
+ *   kernel 0: finest-level damped-Jacobi sweep with its fused fluid sum
+ *             (reads x, b, 2-byte code; writes x),
+ *   kernel 1: finest-level A p with the p.Ap fold (reads p, code; writes Ap),
+ *   kernel 2: finest-level residual + restriction (reads b, x, code; writes
+ *             the coarse residual). */
+int nbl_poisson_profile(nbl_poisson* ps, int kernel, int reps, double* ms, double* bytes);
 
 /* ---- octree (Octree connectivity + OctreePoisson) ----------------------
  * Leaves are given in any order (keys = Morton anchors on the 2^max_level grid,

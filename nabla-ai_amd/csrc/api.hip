@@ -202,6 +202,13 @@ int nbl_run_cell_velocity(nbl_run* run, int component, double* out) {
   });
 }
 
+int nbl_run_last_solves(nbl_run* run, double* ms, int* iters, int* vcycles) {
+  return guarded([&] {
+    need(run, "run is NULL");
+    run->impl->lastSolves(ms, iters, vcycles);
+  });
+}
+
 // ---- pressure plugin ---------------------------------------------------------
 int nbl_poisson_create(nbl_ctx* ctx, const nbl_grid_desc* grid, int kind, double tol, int max_iters,
                        const nbl_solver_opts* opts, nbl_poisson** out) {
@@ -352,6 +359,14 @@ int nbl_poisson_last_timing(nbl_poisson* ps, double* ms, int* vcycles) {
     need(ps, "NULL argument");
     if (ms) *ms = ps->impl->lastMs();
     if (vcycles) *vcycles = ps->impl->lastVcycles();
+  });
+}
+
+int nbl_poisson_profile(nbl_poisson* ps, int kernel, int reps, double* ms, double* bytes) {
+  return guarded([&] {
+    need(ps && ms && bytes, "NULL argument");
+    useDevice(ps->ctx);
+    ps->impl->profile(kernel, reps, ps->stream, ms, bytes);
   });
 }
 

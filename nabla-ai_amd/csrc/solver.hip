@@ -348,6 +348,65 @@ nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t
   return nbl_poisson_report{h.it, h.resid, h.converged};
 }
 
+void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double* bytes) {
+  if (kernel < 0 || kernel > 2) throw std::invalid_argument("profile: kernel must be 0, 1 or 2");
+  if (reps < 1) throw std::invalid_argument("profile: reps must be >= 1");
+  if (kind_ != 0) throw std::invalid_argument("profile: mg-pcg solver only");
+  if (kernel == 2 && nlevels() < 2) throw std::invalid_argument("profile: single-level grid");
+  const MgLevel& L = dl_[0];
+  const double n = double(L.n);
+  const double sz = mixed_ ? 4.0 : 8.0;
+  SolveCtl* ctl = ctl_.as<SolveCtl>();
+  double* m0 = singular_ ? slotPtr(ctl, slotX(0, 0)) : nullptr;
+  double* m1 = singular_ ? slotPtr(ctl, slotX(0, 1)) : nullptr;
+  // a live solve leaves ctl->done set; clear it so the kernels do their work
+  hctl_[1] = SolveCtl{};
+  NBL_HIP(hipMemcpyAsync(ctl, &hctl_[1], sizeof(SolveCtl), hipMemcpyHostToDevice, s));
+  NBL_HIP(hipMemsetAsync(lx0_[0].as<void>(), 0, L.n * size_t(sz), s));
+  NBL_HIP(hipMemsetAsync(lx1_[0].as<void>(), 0, L.n * size_t(sz), s));
+  NBL_HIP(hipMemsetAsync(r_.as<void>(), 0, L.n * 8, s));
+  NBL_HIP(hipMemsetAsync(p_.as<void>(), 0, L.n * 8, s));
+  if (mixed_) NBL_HIP(hipMemsetAsync(b0f_.as<void>(), 0, L.n * 4, s));
+  if (singular_) {
+    NBL_HIP(hipMemsetAsync(m0, 0, 8, s));
+    NBL_HIP(hipMemsetAsync(m1, 0, 8, s));
+  }
+  auto one = [&](int q) {
+    if (kernel == 0) {
+      if (mixed_)
+        launch_mg_sweep<float>(L, b0f_.as<float>(), nullptr, lx0_[0].as<float>(), m0, false, nullptr, nullptr,
+                               nullptr, lx1_[0].as<float>(), m1, red(), ctl, s);
+      else
+        launch_mg_sweep<double>(L, r_.as<double>(), nullptr, (q & 1) ? lx1_[0].as<double>() : lx0_[0].as<double>(),
+                                (q & 1) ? m1 : m0, false, nullptr, nullptr, nullptr,
+                                (q & 1) ? lx0_[0].as<double>() : lx1_[0].as<double>(), (q & 1) ? m0 : m1, red(), ctl,
+                                s);
+    } else if (kernel == 1) {
+      launch_fcg_apply(L, p_.as<double>(), Ap_.as<double>(), ctl, red(), s);
+    } else {
+      if (mixed_)
+        launch_mg_resid_restrict<float>(L, dl_[1], b0f_.as<float>(), nullptr, lx0_[0].as<float>(), m0,
+                                        lb_[1].as<float>(), singular_ ? slotPtr(ctl, slotB(1)) : nullptr, red(), ctl,
+                                        s);
+      else
+        launch_mg_resid_restrict<double>(L, dl_[1], r_.as<double>(), nullptr, lx0_[0].as<double>(), m0,
+                                         lb_[1].as<double>(), singular_ ? slotPtr(ctl, slotB(1)) : nullptr, red(),
+                                         ctl, s);
+    }
+  };
+  one(0);  // warm-up
+  NBL_HIP(hipEventRecord(e0_, s));
+  for (int q = 0; q < reps; ++q) one(q);
+  NBL_HIP(hipEventRecord(e1_, s));
+  NBL_HIP(hipEventSynchronize(e1_));
+  float t = 0.f;
+  NBL_HIP(hipEventElapsedTime(&t, e0_, e1_));
+  *ms = double(t) / reps;
+  if (kernel == 0) *bytes = n * (3.0 * sz + 2.0);
+  else if (kernel == 1) *bytes = n * (8.0 + 8.0 + 2.0);
+  else *bytes = n * (2.0 * sz + 2.0) + double(dl_[1].n) * sz;
+}
+
 // ---------------------------------------------------------------------------
 // MAC run
 // ---------------------------------------------------------------------------
@@ -513,10 +572,12 @@ double RunDev::computeDt() {
 }
 
 void RunDev::project(int stageIdx, nbl_poisson_report& rep) {  // ns_solver.cpp:475-579
-  (void)stageIdx;
   bc(true);
   launch_divergence(args_, cur_[0], cur_[1], cur_[2], b_.as<double>(), stream_);
   rep = poisson_->solve(b_.as<double>(), phi_.as<double>(), stream_);
+  solveMs_[stageIdx - 1] = poisson_->lastMs();
+  solveIt_[stageIdx - 1] = rep.iterations;
+  solveV_[stageIdx - 1] = poisson_->lastVcycles();
   launch_correct(args_, cur_[0], cur_[1], cur_[2], phi_.as<double>(), p_.as<double>(), sc_.as<StepCtl>(),
                  stream_);
 }

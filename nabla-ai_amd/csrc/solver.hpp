@@ -92,6 +92,8 @@ class PoissonDev {
   double lastMs() const { return lastMs_; }
   int lastVcycles() const { return lastV_; }
   RedBuf red() const { return RedBuf{red_.as<double>(), ticket_.as<unsigned>()}; }
+  // Average ms per launch and algorithmic bytes of one solve kernel.
+  void profile(int kernel, int reps, hipStream_t s, double* ms, double* bytes);
 
  private:
   template <class T>
@@ -143,6 +145,13 @@ class RunDev {
   double* w() const { return cur_[2]; }
   double* p() const { return p_.as<double>(); }
   hipStream_t stream() const { return stream_; }
+  void lastSolves(double* ms, int* iters, int* vcycles) const {
+    for (int q = 0; q < 3; ++q) {
+      if (ms) ms[q] = solveMs_[q];
+      if (iters) iters[q] = solveIt_[q];
+      if (vcycles) vcycles[q] = solveV_[q];
+    }
+  }
   size_t sizes(int q) const { return sz_[q]; }
 
  private:
@@ -165,6 +174,8 @@ class RunDev {
   DevReport* hrep_ = nullptr;
   std::unique_ptr<PoissonDev> poisson_;
   double dtLast_ = 0.0;
+  double solveMs_[3] = {0, 0, 0};
+  int solveIt_[3] = {0, 0, 0}, solveV_[3] = {0, 0, 0};
 };
 
 }  // namespace nbl

@@ -185,6 +185,8 @@ def load(path: os.PathLike | None = None):
            C.POINTER(P_D))
     _proto(lib, "nbl_run_rhs", i, V, P_D, P_D, P_D)
     _proto(lib, "nbl_run_cell_velocity", i, V, i, P_D)
+    _proto(lib, "nbl_run_last_solves", i, V, P_D, C.POINTER(i), C.POINTER(i))
+    _proto(lib, "nbl_poisson_profile", i, V, i, i, P_D, P_D)
     _proto(lib, "nbl_poisson_create", i, V, C.POINTER(GridDesc), i, d, i, C.POINTER(SolverOpts),
            C.POINTER(V))
     _proto(lib, "nbl_poisson_destroy", i, V)
@@ -285,6 +287,13 @@ class Run:
         check(self.lib.nbl_run_rhs(self.h, dptr(a), dptr(b), dptr(c)))
         return a, b, c
 
+    def last_solves(self):
+        ms = (C.c_double * 3)()
+        it = (C.c_int * 3)()
+        vc = (C.c_int * 3)()
+        check(self.lib.nbl_run_last_solves(self.h, ms, it, vc))
+        return list(ms), list(it), list(vc)
+
     def cell_velocity(self, comp: int):
         out = np.empty(self.grid.cells)
         check(self.lib.nbl_run_cell_velocity(self.h, comp, dptr(out)))
@@ -358,6 +367,11 @@ class Poisson:
         xf = np.array(xf, dtype=np.float64)
         check(self.lib.nbl_poisson_prolong(self.h, dptr(ec), dptr(xf)))
         return xf
+
+    def profile(self, kernel: int, reps: int = 100):
+        ms, by = C.c_double(), C.c_double()
+        check(self.lib.nbl_poisson_profile(self.h, kernel, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
 
     def last_timing(self):
         ms, vc = C.c_double(), C.c_int()

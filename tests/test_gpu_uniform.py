@@ -205,3 +205,38 @@ def test_tgv16_ten_steps(ctx, orc):
     for fa, fb in zip(a[:4], b[:4]):
         assert rel(fa, fb) < 1e-10
     run.close()
+
+
+@pytest.mark.parametrize("name,g", GRIDS, ids=IDS)
+def test_mixed_precision_reaches_fp64_tolerance(ctx, orc, name, g):
+    """FP32 V-cycle inside the FP64 flexible CG: the FP64 residual still reaches
+    the requested tolerance, and the solution agrees with the FP64 oracle to it."""
+    b = cases.poisson_rhs(g, 17)
+    ps = nbl.Poisson(ctx, g, kind=0, tol=1e-10, max_iters=200, opts=nbl.solver_opts(nbl.PREC_MIXED))
+    try:
+        x, rep = ps.solve(b)
+    finally:
+        ps.close()
+    xo, ro = orc.mg(g, b, 1e-10, 200)
+    assert rep.converged and rep.residual <= 1e-10 * 1.01
+    assert rep.iterations <= ro.iterations + 3
+    assert rel(x, xo) < 1e-8
+
+
+def test_mixed_steps_track_fp64(ctx):
+    g, cfg = cases.cube_channel(nx=48, ny=16, nz=16, res=4)
+    cfg.poisson_tol = 1e-8
+    st = cases.cube_channel_state(g)
+    out = []
+    for prec in (nbl.PREC_FP64, nbl.PREC_MIXED):
+        run = nbl.Run(ctx, g, cfg, nbl.solver_opts(prec))
+        run.set_state(*st)
+        reps = [run.step() for _ in range(5)]
+        out.append((reps, run.get_state()))
+        run.close()
+    (r64, s64), (r32, s32) = out
+    for a, b in zip(r64, r32):
+        assert b.poisson_residual <= 1e-8 and a.poisson_residual <= 1e-8
+        assert b.cd == pytest.approx(a.cd, rel=1e-6)
+    for fa, fb in zip(s64[:3], s32[:3]):
+        assert rel(fb, fa) < 1e-7
