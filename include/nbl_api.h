@@ -129,6 +129,7 @@ typedef struct nbl_ctx nbl_ctx;
 typedef struct nbl_run nbl_run;
 typedef struct nbl_poisson nbl_poisson;
 typedef struct nbl_octree nbl_octree;
+typedef struct nbl_tree nbl_tree;
 
 /* ---- defaults and errors ------------------------------------------------ */
 void nbl_flow_cfg_default(nbl_flow_cfg* cfg);        /* FlowConfig{} values        */
@@ -222,6 +223,21 @@ int nbl_poisson_last_timing(nbl_poisson* ps, double* ms, int* vcycles);
  *   kernel 2: finest-level residual + restriction (reads b, x, code; writes
  *             the coarse residual). */
 int nbl_poisson_profile(nbl_poisson* ps, int kernel, int reps, double* ms, double* bytes);
+
+/* ---- octree construction (host; Octree::refineUniform / refineByMorton /
+ * isBalanced, octree.cpp:325-388,544-560) ------------------------------------
+ * A leaf set under construction, kept sorted by Morton key.  Leaf indices in
+ * the calls below refer to that sorted order. */
+int nbl_tree_create(const double origin[3], const double extent[3], int max_level, nbl_tree** out);
+int nbl_tree_destroy(nbl_tree* t);
+int nbl_tree_count(nbl_tree* t, size_t* n);
+int nbl_tree_refine_uniform(nbl_tree* t, int times);
+/* Split every flagged leaf (flags: n bytes, n == count), refining coarser face
+ * neighbours first so the tree stays 2:1 balanced, as refineByMorton does. */
+int nbl_tree_refine_flagged(nbl_tree* t, const uint8_t* flags, size_t n);
+int nbl_tree_is_balanced(nbl_tree* t, int* balanced);
+/* keys, levels (n each) and cell centres (3n, octree.cpp:84-95); any may be NULL. */
+int nbl_tree_leaves(nbl_tree* t, uint64_t* keys, uint8_t* levels, double* centers);
 
 /* ---- octree (Octree connectivity + OctreePoisson) ----------------------
  * Leaves are given in any order (keys = Morton anchors on the 2^max_level grid,

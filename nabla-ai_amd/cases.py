@@ -195,3 +195,34 @@ def poisson_rhs(g: nbl.Grid, seed: int):
     b[~fl] = 0.0
     b[fl] -= b[fl].mean()
     return b
+
+
+# ---- octree workload (cube wake) ---------------------------------------------
+CUBE_BOX = (3.0, 4.0, 0.0, 1.0, 1.5, 2.5)
+WAKE_BOX = (4.0, 10.0, 0.0, 2.0, 1.0, 3.0)
+C3_ORIGIN = (0.0, 0.0, 0.0)
+C3_EXTENT = (12.0, 4.0, 4.0)
+
+
+def c3_predicate(centers: np.ndarray) -> np.ndarray:
+    """Refinement rule of the cube-wake octree: centre within 1.5 h of the cube's
+    bounding box, or inside the wake box [0,6h] behind the rear face, +-1 h about
+    the span centre, [0,2h] above the floor (h = 1)."""
+    x, y, z = centers[:, 0], centers[:, 1], centers[:, 2]
+    x0, x1, y0, y1, z0, z1 = CUBE_BOX
+    dx = np.maximum(np.maximum(x0 - x, 0.0), x - x1)
+    dy = np.maximum(np.maximum(y0 - y, 0.0), y - y1)
+    dz = np.maximum(np.maximum(z0 - z, 0.0), z - z1)
+    near = dx * dx + dy * dy + dz * dz <= 1.5 * 1.5
+    a0, a1, b0, b1, c0, c1 = WAKE_BOX
+    wake = (x >= a0) & (x <= a1) & (y >= b0) & (y <= b1) & (z >= c0) & (z <= c1)
+    return near | wake
+
+
+def c3_solid(centers: np.ndarray) -> np.ndarray:
+    """InsideSolid mask (2) for leaves whose centre lies in the cube
+    (core/src/adaptive/run.cpp:67-75), Fluid (0) elsewhere."""
+    x, y, z = centers[:, 0], centers[:, 1], centers[:, 2]
+    x0, x1, y0, y1, z0, z1 = CUBE_BOX
+    inside = (x >= x0) & (x <= x1) & (y >= y0) & (y <= y1) & (z >= z0) & (z <= z1)
+    return np.where(inside, 2, 0).astype(np.uint8)

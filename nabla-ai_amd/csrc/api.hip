@@ -9,6 +9,7 @@
 #include "nbl_api.h"
 #include "octree.hpp"
 #include "solver.hpp"
+#include "tree.hpp"
 
 namespace {
 
@@ -367,6 +368,48 @@ int nbl_poisson_profile(nbl_poisson* ps, int kernel, int reps, double* ms, doubl
     need(ps && ms && bytes, "NULL argument");
     useDevice(ps->ctx);
     ps->impl->profile(kernel, reps, ps->stream, ms, bytes);
+  });
+}
+
+// ---- octree construction ----------------------------------------------------------
+int nbl_tree_create(const double origin[3], const double extent[3], int max_level, nbl_tree** out) {
+  return guarded([&] {
+    need(origin && extent && out, "NULL argument");
+    *out = new nbl_tree(origin, extent, max_level);
+  });
+}
+int nbl_tree_destroy(nbl_tree* t) {
+  return guarded([&] { delete t; });
+}
+int nbl_tree_count(nbl_tree* t, size_t* n) {
+  return guarded([&] {
+    need(t && n, "NULL argument");
+    *n = t->impl.size();
+  });
+}
+int nbl_tree_refine_uniform(nbl_tree* t, int times) {
+  return guarded([&] {
+    need(t, "tree is NULL");
+    need(times >= 0, "times must be >= 0");
+    t->impl.refineUniform(times);
+  });
+}
+int nbl_tree_refine_flagged(nbl_tree* t, const uint8_t* flags, size_t n) {
+  return guarded([&] {
+    need(t && flags, "NULL argument");
+    t->impl.refineFlagged(flags, n);
+  });
+}
+int nbl_tree_is_balanced(nbl_tree* t, int* balanced) {
+  return guarded([&] {
+    need(t && balanced, "NULL argument");
+    *balanced = t->impl.balanced() ? 1 : 0;
+  });
+}
+int nbl_tree_leaves(nbl_tree* t, uint64_t* keys, uint8_t* levels, double* centers) {
+  return guarded([&] {
+    need(t, "tree is NULL");
+    t->impl.leaves(keys, levels, centers);
   });
 }
 

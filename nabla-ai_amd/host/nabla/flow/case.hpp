@@ -1,0 +1,57 @@
+// Case description, run driver and the reference's file formats
+// (core/include/nabla/flow/case.hpp:12-57): case.json in; diagnostics.jsonl,
+// NBFCK01 checkpoints and VTU snapshots out, byte-compatible.
+#pragma once
+
+#include <cstddef>
+#include <string>
+
+#include "nabla/flow/grid.hpp"
+#include "nabla/flow/ns_solver.hpp"
+
+namespace nabla::flow {
+
+struct RunOptions {
+  std::string runDir = "run";
+  int maxSteps = 20000;
+  double stopTime = 0.0;
+  double steadyTol = 0.0;
+  int snapshotEvery = 0;
+  int checkpointEvery = 0;
+  std::string restartFrom;
+  bool quiet = false;
+};
+
+struct CaseSpec {
+  std::string name = "case";
+  UniformGrid grid;
+  FlowConfig flow;
+  RunOptions run;
+  Precision precision = Precision::FP64;  // optional "precision" key
+};
+
+struct RunSummary {
+  int steps = 0;
+  double time = 0.0;
+  double momentumResidual = 0.0;
+  double continuityResidual = 0.0;
+  double massError = 0.0;
+  double cd = 0.0;
+  double cl = 0.0;
+  std::size_t cells = 0;
+  bool steady = false;
+  std::string runDir;
+};
+
+CaseSpec makeLidCavity(int n, double reynolds, double lidVelocity = 1.0);
+CaseSpec makeChannel(int nx, int ny, double reynolds, double bulkVelocity = 1.0);
+CaseSpec makeWallMountedCube(int resolution, double reynolds);
+
+CaseSpec loadCase(const std::string& jsonPath);
+RunSummary runCase(const CaseSpec& spec);
+
+void writeGridVtu(const NSSolver& solver, const std::string& path);
+void writeCheckpoint(const NSSolver& solver, const std::string& path);
+void readCheckpoint(NSSolver& solver, const std::string& path);
+
+}  // namespace nabla::flow

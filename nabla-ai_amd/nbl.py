@@ -199,6 +199,13 @@ def load(path: os.PathLike | None = None):
     _proto(lib, "nbl_poisson_restrict", i, V, P_D, P_D)
     _proto(lib, "nbl_poisson_prolong", i, V, P_D, P_D)
     _proto(lib, "nbl_poisson_last_timing", i, V, P_D, C.POINTER(i))
+    _proto(lib, "nbl_tree_create", i, P_D, P_D, i, C.POINTER(V))
+    _proto(lib, "nbl_tree_destroy", i, V)
+    _proto(lib, "nbl_tree_count", i, V, C.POINTER(sz))
+    _proto(lib, "nbl_tree_refine_uniform", i, V, i)
+    _proto(lib, "nbl_tree_refine_flagged", i, V, P_U8, sz)
+    _proto(lib, "nbl_tree_is_balanced", i, V, C.POINTER(i))
+    _proto(lib, "nbl_tree_leaves", i, V, P_U64, P_U8, P_D)
     _proto(lib, "nbl_octree_create", i, V, P_U64, P_U8, P_U8, sz, P_D, P_D, i, C.POINTER(V))
     _proto(lib, "nbl_octree_destroy", i, V)
     _proto(lib, "nbl_octree_count", i, V, C.POINTER(sz), C.POINTER(sz))
@@ -382,6 +389,62 @@ class Poisson:
         if self.h:
             check(self.lib.nbl_poisson_destroy(self.h))
             self.h = C.c_void_p()
+
+
+class Tree:
+    """nbl_tree: host-side balanced octree construction (sorted leaves)."""
+
+    def __init__(self, origin, extent, max_level):
+        self.lib = load()
+        self.origin, self.extent, self.max_level = tuple(origin), tuple(extent), max_level
+        self.h = C.c_void_p()
+        check(self.lib.nbl_tree_create((C.c_double * 3)(*origin), (C.c_double * 3)(*extent),
+                                       max_level, C.byref(self.h)))
+
+    @property
+    def n(self):
+        v = C.c_size_t()
+        check(self.lib.nbl_tree_count(self.h, C.byref(v)))
+        return v.value
+
+    def refine_uniform(self, times):
+        check(self.lib.nbl_tree_refine_uniform(self.h, times))
+
+    def refine_flagged(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.uint8)
+        check(self.lib.nbl_tree_refine_flagged(self.h, f.ctypes.data_as(P_U8), f.size))
+
+    def balanced(self):
+        v = C.c_int()
+        check(self.lib.nbl_tree_is_balanced(self.h, C.byref(v)))
+        return bool(v.value)
+
+    def leaves(self):
+        n = self.n
+        k = np.empty(n, np.uint64)
+        l = np.empty(n, np.uint8)
+        c = np.empty(3 * n)
+        check(self.lib.nbl_tree_leaves(self.h, k.ctypes.data_as(P_U64), l.ctypes.data_as(P_U8),
+                                       dptr(c)))
+        return k, l, c.reshape(n, 3)
+
+    def close(self):
+        if self.h:
+            check(self.lib.nbl_tree_destroy(self.h))
+            self.h = C.c_void_p()
+
+
+def build_tree(origin, extent, max_level, base, predicate):
+    """Refine uniformly to `base`, then refine (balanced) every leaf below
+    max_level whose centre satisfies `predicate` until none is left."""
+    t = Tree(origin, extent, max_level)
+    t.refine_uniform(base)
+    while True:
+        _k, l, c = t.leaves()
+        f = predicate(c) & (l < max_level)
+        if not f.any():
+            return t
+        t.refine_flagged(f)
 
 
 class Octree:

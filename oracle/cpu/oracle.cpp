@@ -1426,6 +1426,11 @@ int orc_oct_refine_key(void* h, uint64_t key) {
     return -1;
   }
 }
+int orc_oct_level_of(void* h, uint64_t key) {
+  const Oct& o = *static_cast<Oct*>(h);
+  const long i = o.find(key);
+  return i < 0 ? -1 : int(o.lev[size_t(i)]);
+}
 void orc_oct_set_mask(void* h, size_t i, uint8_t m) { static_cast<Oct*>(h)->msk[i] = m; }
 void orc_oct_dump(void* h, uint64_t* keys, uint8_t* levels, uint8_t* masks) {
   const Oct& o = *static_cast<Oct*>(h);

@@ -63,6 +63,7 @@ size_t orc_oct_count(void* h);
 void orc_oct_refine_uniform(void* h, int times);
 void orc_oct_refine_box(void* h, const double lo[3], const double hi[3], int times);
 int orc_oct_refine_key(void* h, uint64_t key);
+int orc_oct_level_of(void* h, uint64_t key);
 void orc_oct_set_mask(void* h, size_t i, uint8_t m);
 void orc_oct_dump(void* h, uint64_t* keys, uint8_t* levels, uint8_t* masks);
 int orc_oct_face_neighbors(void* h, size_t i, int dir, uint64_t* out);

@@ -53,6 +53,7 @@ class OracleLib:
             "oct_count": (sz, V), "oct_refine_uniform": (None, V, i),
             "oct_refine_box": (None, V, C.POINTER(d), C.POINTER(d), i),
             "oct_refine_key": (i, V, C.c_uint64), "oct_set_mask": (None, V, sz, C.c_uint8),
+            "oct_level_of": (i, V, C.c_uint64),
             "oct_dump": (None, V, P_U64, P_U8, P_U8),
             "oct_face_neighbors": (i, V, sz, i, P_U64), "oct_is_balanced": (i, V),
             "oct_centers": (None, V, P_D), "oct_apply": (None, V, P_D, P_D),
@@ -207,6 +208,14 @@ class Oct:
     def refine_key(self, key):
         if self.o.oct_refine_key(self.h, int(key)) != 0:
             raise RuntimeError("refine failed")
+
+    def refine_flagged_pass(self, keys, levels):
+        """Refine (balanced) each listed leaf that still exists at its listed
+        level; a leaf already split earlier in the pass is skipped."""
+        for k, l in zip(keys, levels):
+            if self.o.oct_level_of(self.h, int(k)) == int(l):
+                if self.o.oct_refine_key(self.h, int(k)) != 0:
+                    raise RuntimeError("refine failed")
 
     def set_mask(self, i, m):
         self.o.oct_set_mask(self.h, int(i), int(m))

@@ -184,6 +184,11 @@ int ref_oct_refine_key(void* h, uint64_t key) {
     return -1;
   }
 }
+int ref_oct_level_of(void* h, uint64_t key) {
+  const auto& t = *static_cast<mesh::Octree*>(h);
+  const size_t i = t.find(key);
+  return i == mesh::Octree::npos ? -1 : t.level(i);
+}
 void ref_oct_set_mask(void* h, size_t i, uint8_t m) {
   static_cast<mesh::Octree*>(h)->setMask(i, static_cast<mesh::CellMask>(m));
 }

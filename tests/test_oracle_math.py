@@ -1,0 +1,259 @@
+"""Pin the oracle against what the mathematics fixes, independently of the
+reference's code: closed forms, invariants and the assertions of the
+reference's own tests (core/tests/*.cpp).  Each check is chosen so a plausible
+slip (dropped term, wrong sign or index, transposed operand) fails it."""
+import math
+
+import numpy as np
+import pytest
+
+import cases
+import nbl
+
+
+def periodic_grid(nx, ny, nz, h):
+    return nbl.Grid(nx, ny, nz, h, faces=[dict(type=nbl.BC_PERIODIC)] * 6)
+
+
+def test_apply_fourier_eigenvalues(orc):
+    """On a periodic grid cos(k.x) is an eigenvector of the 7-point -Laplacian
+    with eigenvalue sum_a (2 - 2 cos(k_a h_a)) / h_a^2."""
+    nx, ny, nz = 16, 12, 10
+    g = nbl.Grid(nx, ny, nz, 0.1, 0.2, 0.3, faces=[dict(type=nbl.BC_PERIODIC)] * 6)
+    kx, ky, kz = 2 * np.pi * 3 / (nx * 0.1), 2 * np.pi * 2 / (ny * 0.2), 2 * np.pi * 1 / (nz * 0.3)
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    phi = np.cos(kx * (i + 0.5) * 0.1 + ky * (j + 0.5) * 0.2 + kz * (k + 0.5) * 0.3)
+    phi = np.ascontiguousarray(phi.transpose(2, 1, 0)).reshape(-1)
+    lam = sum((2 - 2 * math.cos(kk * hh)) / hh ** 2 for kk, hh in ((kx, 0.1), (ky, 0.2), (kz, 0.3)))
+    assert np.allclose(orc.apply(g, phi), lam * phi, rtol=0, atol=1e-11 * lam)
+
+
+def test_apply_symmetric_and_outlet_dirichlet(orc):
+    """A is symmetric (SPD) with walls, outlet, periodic span and a solid block;
+    the constant lies in the null space exactly when there is no outlet."""
+    faces = [dict(type=nbl.BC_INLET), dict(type=nbl.BC_OUTLET), dict(type=nbl.BC_WALL),
+             dict(type=nbl.BC_WALL), dict(type=nbl.BC_PERIODIC), dict(type=nbl.BC_PERIODIC)]
+    solid = np.zeros((6, 8, 12), np.uint8)
+    solid[2:4, 0:3, 4:7] = 1
+    g = nbl.Grid(12, 8, 6, 0.1, 0.12, 0.15, faces=faces, solid=solid)
+    a, b = cases.normal(1, g.cells), cases.normal(2, g.cells)
+    a[g.solid == 1] = 0
+    b[g.solid == 1] = 0
+    assert a @ orc.apply(g, b) == pytest.approx(b @ orc.apply(g, a), rel=1e-12)
+    one = (g.solid == 0).astype(float)
+    r = orc.apply(g, one)
+    # non-zero only on the outlet column (the Dirichlet ghost -phi contributes 2/h^2)
+    assert np.all(r.reshape(6, 8, 12)[:, :, :-1][solid[:, :, :-1] == 0] == 0)
+    assert np.allclose(r.reshape(6, 8, 12)[:, :, -1][solid[:, :, -1] == 0], 2 / 0.1 ** 2)
+    faces[1] = dict(type=nbl.BC_WALL)
+    g2 = nbl.Grid(12, 8, 6, 0.1, 0.12, 0.15, faces=faces, solid=solid)
+    assert np.all(orc.apply(g2, one)[g2.solid == 0] == 0)
+
+
+def test_cg_manufactured_periodic(orc):
+    """core/tests/test_flow.cpp:21-61 (16x16x1 periodic, cos*cos)."""
+    g = nbl.Grid(16, 16, 1, 1 / 16, faces=[dict(type=nbl.BC_PERIODIC)] * 6)
+    x = (np.arange(16) + 0.5) / 16
+    X, Y = np.meshgrid(x, x, indexing="xy")
+    phi = (np.cos(2 * np.pi * X) * np.cos(2 * np.pi * Y)).reshape(-1)
+    b = orc.apply(g, phi)
+    xs, rep = orc.cg(g, b, 1e-12, 5000)
+    assert rep.residual < 1e-8
+    e = (xs - xs.mean()) - (phi - phi.mean())
+    assert np.sqrt(np.mean(e * e)) < 1e-6
+
+
+def test_mg_manufactured_periodic_singular(orc):
+    """core/tests/test_flow.cpp:119-159 (24x24 periodic, MG-PCG, <= 50 cycles)."""
+    g = nbl.Grid(24, 24, 1, 1 / 24, faces=[dict(type=nbl.BC_PERIODIC)] * 6)
+    x = (np.arange(24) + 0.5) / 24
+    X, Y = np.meshgrid(x, x, indexing="xy")
+    phi = (np.cos(2 * np.pi * X) * np.cos(2 * np.pi * Y)).reshape(-1)
+    xs, rep = orc.mg(g, orc.apply(g, phi), 1e-12, 100)
+    assert rep.residual < 1e-8 and rep.iterations <= 50
+    e = (xs - xs.mean()) - (phi - phi.mean())
+    assert np.sqrt(np.mean(e * e)) < 1e-6
+
+
+def test_uniform_stream_is_a_fixed_point(orc):
+    """Uniform periodic flow: convection and diffusion vanish identically, the
+    projection is idle, and the SSP-RK3 combine returns u^n exactly
+    (1/3 u + 2/3 u reproduces u only when the stage weights sum to one)."""
+    g = periodic_grid(8, 8, 8, 0.25)
+    for conv in (nbl.CONV_CENTRAL, nbl.CONV_WENO5):
+        cfg = nbl.flow_cfg(reynolds=50.0, convection=conv)
+        s = orc.ns(g, cfg)
+        su, sv, sw, sp = g.shapes
+        u, v, w = np.full(su, 0.75), np.full(sv, -0.5), np.full(sw, 0.25)
+        s.set_state(u, v, w, np.zeros(sp))
+        for a in s.rhs():
+            assert np.all(a == 0.0)
+        s.step(0.01)
+        uu, vv, ww, *_ = s.get_state()
+        assert np.allclose(uu, 0.75, atol=1e-15) and np.allclose(vv, -0.5, atol=1e-15)
+        assert np.allclose(ww, 0.25, atol=1e-15)
+
+
+def test_rhs_diffusion_only_matches_laplacian(orc):
+    """With v = w = 0 and u = sin(2 pi y / L) (a shear wave), central convection
+    vanishes and ru = nu * (u(j+1) - 2u + u(j-1))/dy^2 exactly."""
+    n, L = 16, 1.0
+    h = L / n
+    g = periodic_grid(n, n, n, h)
+    cfg = nbl.flow_cfg(reynolds=10.0, convection=nbl.CONV_CENTRAL, ref_length=1.0)
+    s = orc.ns(g, cfg)
+    su, sv, sw, sp = g.shapes
+    y = (np.arange(n) + 0.5) * h
+    u = np.broadcast_to(np.sin(2 * np.pi * y / L)[None, :, None], (n, n, n + 1)).copy().reshape(-1)
+    s.set_state(u, np.zeros(sv), np.zeros(sw), np.zeros(sp))
+    ru, rv, rw = s.rhs()
+    U = u.reshape(n, n, n + 1)
+    lap = (np.roll(U, -1, 1) - 2 * U + np.roll(U, 1, 1)) / (h * h)
+    want = 0.1 * lap
+    want[:, :, 0] = 0.0
+    want[:, :, n] = 0.0  # boundary-face planes get no momentum RHS (ns_solver.cpp:325)
+    assert np.allclose(ru.reshape(n, n, n + 1), want, rtol=1e-13, atol=1e-13)
+    assert np.all(rv == 0) and np.all(rw == 0)
+
+
+def test_weno_constant_and_linear_reconstruction(orc):
+    """WENO-5 reproduces constants exactly and linear data to rounding: with a
+    linear u(x) in a periodic-free direction the convective flux difference of
+    u*u equals d(u^2)/dx of the reconstructed faces.  Checked through the RHS
+    of a 1-D advected ramp away from boundaries."""
+    n = 24
+    g = nbl.Grid(n, 4, 4, 0.1, faces=[dict(type=nbl.BC_WALL)] * 2 + [dict(type=nbl.BC_PERIODIC)] * 4)
+    cfg = nbl.flow_cfg(reynolds=1e12, convection=nbl.CONV_WENO5)
+    s = orc.ns(g, cfg)
+    su, sv, sw, sp = g.shapes
+    xf = np.arange(n + 1) * 0.1
+    u = np.broadcast_to((1.0 + 0.5 * xf)[None, None, :], (4, 4, n + 1)).copy().reshape(-1)
+    s.set_state(u, np.zeros(sv), np.zeros(sw), np.zeros(sp))
+    ru, *_ = s.rhs()
+    # interior faces away from the walls: u linear => face values are the
+    # midpoints, d(u^2)/dx = 2 u u' = 2 (1 + 0.5 x) 0.5
+    R = ru.reshape(4, 4, n + 1)[:, :, 4:n - 3]
+    x = xf[4:n - 3]
+    assert np.allclose(R, -(1.0 + 0.5 * x), rtol=1e-10, atol=1e-12)
+
+
+def test_projection_drives_divergence_down(orc):
+    """core/tests/test_flow.cpp:161-172: lid cavity 24^2, tol 1e-10, 5 steps."""
+    g, cfg = cases.lid_cavity(24, 100.0)
+    cfg.poisson_tol = 1e-10
+    s = orc.ns(g, cfg)
+    s.initialize()
+    for _ in range(5):
+        r = s.step()
+    assert r.continuity_residual < 1e-7
+
+
+def test_tgv_early_energy_decay(orc):
+    """Taylor-Green: kinetic energy decays like exp(-6 nu t) at early times
+    (a 1e-3-level sanity check of the physics, not a parity pin)."""
+    g, cfg = cases.taylor_green(32)
+    cfg.reynolds = 100.0
+    s = orc.ns(g, cfg)
+    st = cases.taylor_green_state(g)
+    s.set_state(*st)
+    ke0, _ = cases.kinetic_energy_enstrophy(g, *st[:3])
+    for _ in range(5):
+        r = s.step()
+    u, v, w, *_ = s.get_state()
+    ke, _ = cases.kinetic_energy_enstrophy(g, u, v, w)
+    assert ke / ke0 == pytest.approx(math.exp(-6 * 0.01 * r.t), rel=2e-3)
+
+
+def test_morton_constants_and_roundtrip(orc):
+    """core/tests/test_morton.cpp:10-39."""
+    assert orc.morton_encode(1, 0, 0) == 1 and orc.morton_encode(0, 1, 0) == 2
+    assert orc.morton_encode(0, 0, 1) == 4
+    rng = np.random.default_rng(0xC0FFEE)
+    import ctypes as C
+    for x, y, z in rng.integers(0, 1 << 21, size=(2000, 3)):
+        k = orc.morton_encode(int(x), int(y), int(z))
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        orc.morton_decode(k, C.byref(a), C.byref(b), C.byref(c))
+        assert (a.value, b.value, c.value) == (x, y, z)
+    m = (1 << 21) - 1
+    assert orc.morton_encode(m, m, m) == (1 << 63) - 1
+
+
+def brute_neighbours(k, l, max_level, i, d):
+    """Leaves sharing a positive-area face patch with leaf i across face d."""
+    def box(q):
+        x = [int(sum(((int(k[q]) >> (3 * b + a)) & 1) << b for b in range(21))) for a in range(3)]
+        s = 1 << (max_level - int(l[q]))
+        return x, s
+    (ai, si) = box(i)
+    axis, plus = d // 2, d & 1
+    plane = ai[axis] + si if plus else ai[axis]
+    out = []
+    for q in range(len(k)):
+        if q == i:
+            continue
+        aq, sq = box(q)
+        face = aq[axis] if plus else aq[axis] + sq
+        if face != plane:
+            continue
+        ok = True
+        for t in range(3):
+            if t == axis:
+                continue
+            lo, hi = max(ai[t], aq[t]), min(ai[t] + si, aq[t] + sq)
+            ok &= hi > lo
+        if ok:
+            out.append(q)
+    return sorted(out)
+
+
+def test_octree_connectivity_matches_geometry(orc):
+    """Face neighbours are exactly the leaves that share a positive-area face
+    patch (brute force over all pairs), with 1 neighbour at same/coarser level
+    and 4 across a 2:1 jump (core/tests/test_octree.cpp:136-170)."""
+    t = orc.octree((0, 0, 0), (1, 1, 1), 4)
+    t.refine_uniform(2)
+    k, l, _m = t.dump()
+    t.refine_key(k[21])
+    t.refine_key(k[42])
+    k, l, _m = t.dump()
+    assert t.balanced()
+    saw4 = False
+    for i in range(t.n):
+        for d in range(6):
+            nb = t.neighbors(i, d)
+            assert sorted(nb) == brute_neighbours(k, l, 4, i, d)
+            assert len(nb) in (0, 1, 4)
+            saw4 |= len(nb) == 4
+    assert saw4
+
+
+def test_octree_operator_conservative_and_symmetric(orc):
+    """core/tests/test_octree_poisson.cpp:99-145: sum(Ap) = 0 and a.Ab = b.Aa."""
+    t = orc.octree((0, 0, 0), (1, 1, 1), 5)
+    t.refine_uniform(2)
+    t.refine_box((0, 0, 0), (0.5, 0.5, 0.5), 1)
+    p = cases.normal(2024, t.n)
+    Ap = t.apply(p)
+    assert abs(Ap.sum()) < 1e-9 * (1 + np.abs(Ap).max())
+    a, b = cases.normal(7, t.n), cases.normal(8, t.n)
+    assert a @ t.apply(b) == pytest.approx(b @ t.apply(a), rel=1e-10)
+
+
+def test_octree_uniform_equals_scaled_uniform_operator(orc):
+    """On a uniform all-fluid tree, OctreePoisson::applyA equals cell volume
+    times the uniform 7-point operator with wall (Neumann) boundaries."""
+    t = orc.octree((0, 0, 0), (1, 1, 1), 3)
+    t.refine_uniform(3)
+    k, _l, _m = t.dump()
+    c = t.centers()
+    n = 8
+    h = 1.0 / n
+    idx = np.floor(c / h).astype(int)
+    lin = idx[:, 0] + n * (idx[:, 1] + n * idx[:, 2])
+    g = nbl.Grid(n, n, n, h)
+    x = cases.normal(9, t.n)
+    xu = np.empty(n ** 3)
+    xu[lin] = x
+    Au = orc.apply(g, xu)[lin]
+    assert np.allclose(t.apply(x), h ** 3 * Au, rtol=1e-12, atol=1e-12)
