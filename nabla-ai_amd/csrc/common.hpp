@@ -3,11 +3,12 @@
 // * HIP error handling that turns a failed runtime call into an nbl status and
 //   a thread-local message (no exception crosses the C-ABI).
 // * Deterministic grid reductions: every block folds its values with a fixed
-//   wavefront/LDS tree, writes one partial, and the last block to arrive (ticket
-//   counter) folds the partials in index order.  The grid size of a reducing
-//   launch is fixed per problem size, so the result is bit-reproducible run to
-//   run.  The last block also resets the ticket, so no memset is needed between
-//   launches (graph-capture friendly).
+//   wavefront/LDS tree and writes one partial per value; a one-block finalize
+//   kernel then folds the partials in index order and applies the scalar
+//   update (alpha, beta, a mean, the convergence test ...).  Launch shapes are
+//   fixed per problem size, so results are bit-reproducible run to run.  No
+//   atomics and no fences: a single ticket counter hit by thousands of blocks
+//   serialises at the memory side and cost ~50 us per reduction.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,7 +21,6 @@ namespace nbl {
 
 constexpr int kBlock = 256;          // threads per block for streaming kernels
 constexpr int kWave = 64;            // CDNA wavefront
-constexpr int kMaxRedBlocks = 2048;  // cap on blocks of a reducing launch
 constexpr int kMaxRedVals = 8;       // values reduced together by one launch
 
 struct HipError : std::runtime_error {
@@ -43,10 +43,11 @@ inline int gridFor(long long n, int block = kBlock, int cap = 1 << 30) {
   return int(g);
 }
 
-// Scratch for deterministic reductions: partial sums per block and a ticket.
+// Scratch for deterministic reductions: kMaxRedVals rows of `stride` partials
+// (one per block of the largest reducing launch).
 struct RedBuf {
-  double* partials;   // kMaxRedVals * kMaxRedBlocks
-  unsigned* ticket;   // one counter, zero between launches
+  double* partials;
+  int stride;
 };
 
 // ---------------------------------------------------------------------------
@@ -67,15 +68,16 @@ __device__ __forceinline__ double waveMax(double v) {
 // thread 0.  Bit q of MAXMASK selects max instead of sum for value q.
 template <int K, unsigned MAXMASK>
 __device__ __forceinline__ void blockFold(double (&v)[K], double* sh) {
-  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  const int nw = blockDim.x / kWave;
+  const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  const int lane = tid & (kWave - 1), wid = tid / kWave;
+  const int nw = (blockDim.x * blockDim.y * blockDim.z) / kWave;
 #pragma unroll
   for (int q = 0; q < K; ++q) {
     v[q] = ((MAXMASK >> q) & 1u) ? waveMax(v[q]) : waveSum(v[q]);
     if (lane == 0) sh[q * 16 + wid] = v[q];
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
 #pragma unroll
     for (int q = 0; q < K; ++q) {
       double a = sh[q * 16];
@@ -87,43 +89,42 @@ __device__ __forceinline__ void blockFold(double (&v)[K], double* sh) {
   __syncthreads();
 }
 
-// Grid-wide deterministic reduction of K per-thread values (sum, or max for
-// the values flagged in MAXMASK; max operands must be >= 0).  Returns true in
-// every thread of the last block to finish; `out` then holds the totals in
-// thread 0.  Requires blockDim.x <= 1024 and gridDim.x <= kMaxRedBlocks.
+// Write this block's K folded values to partials[q * stride + blockLinear].
+// Plain stores: the consumer is a later kernel (k_fin-style finalize), so the
+// kernel boundary orders them and no fence or atomic is needed.
 template <int K, unsigned MAXMASK = 0u>
-__device__ __forceinline__ bool gridFold(double (&v)[K], RedBuf rb, double (&out)[K]) {
-  static_assert(K <= kMaxRedVals, "too many values");
+__device__ __forceinline__ void blockToPartials(double (&v)[K], double* partials, int stride, int blockLinear) {
   __shared__ double sh[K * 16];
-  __shared__ int last;
   blockFold<K, MAXMASK>(v, sh);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x + threadIdx.y * blockDim.x == 0) {
 #pragma unroll
-    for (int q = 0; q < K; ++q) rb.partials[q * kMaxRedBlocks + blockIdx.x] = v[q];
-    __threadfence();
-    const unsigned t = atomicAdd(rb.ticket, 1u);
-    last = (t == gridDim.x - 1) ? 1 : 0;
+    for (int q = 0; q < K; ++q) partials[q * stride + blockLinear] = v[q];
   }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
+}
+
+// Fold P partials per value in a fixed order inside one block; the totals are
+// returned in every thread.
+template <int K, unsigned MAXMASK = 0u>
+__device__ __forceinline__ void foldPartials(const double* partials, int stride, int P, double (&out)[K]) {
+  __shared__ double sh[K * 16];
+  __shared__ double res[K];
   double a[K];
 #pragma unroll
   for (int q = 0; q < K; ++q) {
     a[q] = 0.0;
-    for (int b = threadIdx.x; b < int(gridDim.x); b += blockDim.x) {
-      const double pv = __hip_atomic_load(&rb.partials[q * kMaxRedBlocks + b], __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
+    for (int b = threadIdx.x; b < P; b += blockDim.x) {
+      const double pv = partials[q * stride + b];
       a[q] = ((MAXMASK >> q) & 1u) ? fmax(a[q], pv) : a[q] + pv;
     }
   }
   blockFold<K, MAXMASK>(a, sh);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
+    for (int q = 0; q < K; ++q) res[q] = a[q];
+  __syncthreads();
 #pragma unroll
-    for (int q = 0; q < K; ++q) out[q] = a[q];
-    *rb.ticket = 0u;
-  }
-  return true;
+  for (int q = 0; q < K; ++q) out[q] = res[q];
 }
+
+constexpr int kFinThreads = 1024;  // threads of a one-block finalize kernel
 
 }  // namespace nbl

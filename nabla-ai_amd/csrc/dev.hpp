@@ -47,8 +47,40 @@ __host__ __device__ inline int kindOf(uint16_t code, int dir) { return (code >> 
 struct SolveCtl {
   double tol, bnorm, rz, pAp, alpha, beta, rr, resid;
   int it, done, converged, broke;
-  int vcycles, pad_;
+  int early, pad_;  // early: converged before the first iteration
   double slot[64];  // reduction results / pending means
+};
+
+// Launch shape of a cell kernel on an nx x ny x nz level: 256-thread blocks,
+// bx threads along i (64, 32, 16 or 8) times 256/bx rows of j, one k per grid z.
+struct Geo3 {
+  dim3 grid, block;
+  int blocks;
+};
+Geo3 geo3(int nx, int ny, int nz);
+
+// Scalar update applied by the one-block finalize kernel after a reduction.
+enum FinOp : int {
+  FIN_MEAN = 0,   // slot[slot] = sum / nf
+  FIN_BNORM,      // |b| -> bnorm, reset counters, optional early exit
+  FIN_RZ,         // rz = r.z
+  FIN_ALPHA,      // p.Ap -> alpha (or stop on a vanishing p.Ap)
+  FIN_CONV,       // |r|^2 -> convergence test
+  FIN_BETA_FLEX,  // (z.rn - z.r)/rz
+  FIN_BETA,       // (r.z)/rz
+  FIN_RESID,      // |rhs - A x|^2 -> true relative residual
+  FIN_BNORM_MEAN, // (|b|^2, sum) -> bnorm + counters reset, slot[slot] = sum / nf
+  FIN_CONV_MEAN,  // (|r|^2, sum) -> convergence test, slot[slot] = sum / nf
+};
+struct FinArgs {
+  int op;
+  int K;          // values folded (1 or 2)
+  int nparts;     // partials written by the reducing kernel (its block count)
+  int slot;
+  long long nf;
+  double convTol;
+  int earlyExit;
+  int predicated; // skip when the solve has stopped
 };
 
 // Per-run time-step scalars (NSSolver step state).
@@ -84,13 +116,15 @@ void launch_bc_solid(const NsArgs& a, double* u, double* v, double* w, const int
                      const int counts[3], hipStream_t s);
 void launch_rhs_rk(const NsArgs& a, const double* u0, const double* v0, const double* w0,
                    const double* u, const double* v, const double* w, double* uo, double* vo,
-                   double* wo, int stage, const StepCtl* sc, hipStream_t s);
+                   double* wo, int stage, const StepCtl* sc, const DevGrid* gd, hipStream_t s);
 void launch_divergence(const NsArgs& a, const double* u, const double* v, const double* w,
                        double* b, hipStream_t s);
 void launch_correct(const NsArgs& a, double* u, double* v, double* w, const double* phi,
                     double* p, const StepCtl* sc, hipStream_t s);
+// Reducing NS launches write partials and run their own finalize kernel.
 void launch_cfl(const NsArgs& a, const double* u, const double* v, const double* w, StepCtl* sc,
                 double fixed_dt, RedBuf rb, hipStream_t s);
+int ns_partials(const DevGrid& g);  // partial slots the NS reductions need
 void launch_diagnostics(const NsArgs& a, const double* u, const double* v, const double* w,
                         const double* u0, const double* v0, const double* w0, const double* p,
                         StepCtl* sc, DevReport* out, RedBuf rb, hipStream_t s);
@@ -101,19 +135,20 @@ void launch_fill(double* a, long long n, double v, hipStream_t s);
 // ---- launchers (mg.hip) ----------------------------------------------------
 // Multigrid pieces are templated on the storage/arithmetic type T (double for
 // the plain path, float for the mixed V-cycle).
+void launch_fin(RedBuf rb, const FinArgs& a, SolveCtl* ctl, hipStream_t s);
 template <class T>
 void launch_mg_sweep(const MgLevel& L, const T* b, const double* mb, const T* x, const double* mx,
-                     bool zeroGuess, const T* ec, const double* mec, const MgLevel* Lc, T* xo,
-                     double* meanOut, RedBuf rb, const SolveCtl* ctl, hipStream_t s);
+                     bool zeroGuess, const T* ec, const double* mec, const MgLevel* Lc, T* xo, bool fold,
+                     RedBuf rb, const SolveCtl* ctl, hipStream_t s);
 template <class T>
 void launch_mg_resid_restrict(const MgLevel& L, const MgLevel& Lc, const T* b, const double* mb,
-                              const T* x, const double* mx, T* rc, double* meanOut, RedBuf rb,
+                              const T* x, const double* mx, T* rc, bool fold, RedBuf rb,
                               const SolveCtl* ctl, hipStream_t s);
 template <class T>
 void launch_mg_coarse(const MgLevel& L, bool singular, const T* b, const double* mb, T* x,
                       T* scratch, const SolveCtl* ctl, hipStream_t s);
 template <class T>
-void launch_mg_finish(const MgLevel& L, const T* x, const double* mx, double* z, double* meanOut,
+void launch_mg_finish(const MgLevel& L, const T* x, const double* mx, double* z, bool fold,
                       RedBuf rb, const SolveCtl* ctl, hipStream_t s);
 template <class T>
 void launch_to_level0(const MgLevel& L, const double* r, T* b, const SolveCtl* ctl, hipStream_t s);
@@ -126,28 +161,28 @@ void launch_prolong_plain(const MgLevel& L, const MgLevel& Lc, const double* ec,
                           hipStream_t s);
 
 // Flexible CG (MgPoisson::solve) and Jacobi CG (CgPoisson::solve) pieces.
-void launch_fcg_init(const MgLevel& L, bool singular, bool maskSolid, const double* b, double* rhs,
-                     double* x, SolveCtl* ctl, RedBuf rb, hipStream_t s);
-void launch_fcg_init2(const MgLevel& L, bool singular, bool earlyExit, double* rhs, double* r,
-                      SolveCtl* ctl, RedBuf rb, hipStream_t s);
-void launch_fcg_start(const MgLevel& L, const double* z, const double* mz, const double* r,
-                      double* zm, double* p, SolveCtl* ctl, RedBuf rb, hipStream_t s);
-void launch_fcg_apply(const MgLevel& L, const double* p, double* Ap, SolveCtl* ctl, RedBuf rb,
+// Each reducing launcher writes one partial per block of geo3(level 0).
+void launch_fcg_init(const MgLevel& L, bool maskSolid, const double* b, double* rhs, double* x, RedBuf rb,
+                     hipStream_t s);
+void launch_fcg_init2(const MgLevel& L, bool singular, double* rhs, double* r, const SolveCtl* ctl, RedBuf rb,
                       hipStream_t s);
-void launch_fcg_update(const MgLevel& L, bool singular, double* x, const double* p,
-                       const double* r, const double* Ap, double* rn, SolveCtl* ctl, RedBuf rb,
-                       hipStream_t s);
-void launch_fcg_norm(const MgLevel& L, double* rn, SolveCtl* ctl, RedBuf rb, hipStream_t s);
-void launch_fcg_dots(const MgLevel& L, double* z, const double* mz, const double* rn,
-                     const double* r, SolveCtl* ctl, RedBuf rb, hipStream_t s);
-void launch_fcg_pupdate(const MgLevel& L, double* p, const double* z, double* r,
-                        const double* rn, SolveCtl* ctl, hipStream_t s);
-void launch_fcg_final(const MgLevel& L, bool singular, const double* x, const double* rhs,
-                      double* xo, SolveCtl* ctl, RedBuf rb, double convTol, hipStream_t s);
-void launch_cg_z(const MgLevel& L, const double* r, double* z, SolveCtl* ctl, RedBuf rb,
-                 bool first, double* p, hipStream_t s);
+void launch_fcg_start(const MgLevel& L, double* z, const double* mz, const double* r, double* p,
+                      const SolveCtl* ctl, RedBuf rb, hipStream_t s);
+void launch_fcg_apply(const MgLevel& L, const double* p, double* Ap, const SolveCtl* ctl, RedBuf rb,
+                      hipStream_t s);
+void launch_fcg_update(const MgLevel& L, bool singular, double* x, const double* p, const double* r,
+                       const double* Ap, double* rn, const SolveCtl* ctl, RedBuf rb, hipStream_t s);
+void launch_fcg_norm(const MgLevel& L, double* rn, const SolveCtl* ctl, RedBuf rb, hipStream_t s);
+void launch_fcg_dots(const MgLevel& L, double* z, const double* mz, const double* rn, const double* r,
+                     const SolveCtl* ctl, RedBuf rb, hipStream_t s);
+void launch_fcg_pupdate(const MgLevel& L, double* p, const double* z, double* r, const double* rn,
+                        const SolveCtl* ctl, hipStream_t s);
+void launch_fluid_sum(const MgLevel& L, const double* x, RedBuf rb, hipStream_t s);
+void launch_fcg_final(const MgLevel& L, bool singular, const double* x, const double* rhs, double* xo,
+                      const SolveCtl* ctl, RedBuf rb, hipStream_t s);
+void launch_cg_z(const MgLevel& L, const double* r, double* z, bool first, double* p, const SolveCtl* ctl,
+                 RedBuf rb, hipStream_t s);
 void launch_cg_update(const MgLevel& L, double* x, const double* p, double* r, const double* Ap,
-                      SolveCtl* ctl, RedBuf rb, hipStream_t s);
-void launch_cg_pupdate(const MgLevel& L, double* p, const double* z, SolveCtl* ctl, hipStream_t s);
+                      const SolveCtl* ctl, RedBuf rb, hipStream_t s);
 
 }  // namespace nbl

@@ -4,37 +4,66 @@
 // Memory model: one compact array per vector and level (i fastest) plus a
 // 16-bit per-cell coupling code; the 7-point operator, its diagonal and the
 // periodic wrap are all rebuilt from the code, so a sweep moves 3 vectors and
-// 2 bytes per cell.  The reference's null-space projection ("removeMean",
-// multigrid.cpp:81-99) is applied lazily: the kernel that produces a vector
-// also folds its fluid sum, and every consumer subtracts the resulting mean
-// when it reads the vector.  x - mean is the same rounded value whether it
-// is stored first or formed on the fly, so this is bit-identical per element
-// to the reference while saving one pass and one launch per projection.
+// 2 bytes per cell.  Cell kernels run on a 3-D launch (64..8 threads along i,
+// 4..32 rows of j per block, one k per grid z), so (i, j, k) come from the
+// launch coordinates with no integer division and loads along i coalesce.
+//
+// The reference's null-space projection ("removeMean", multigrid.cpp:81-99)
+// is applied lazily: the kernel that produces a vector also writes per-block
+// partial sums of it, a one-block finalize kernel turns them into the mean,
+// and every consumer subtracts the mean when it reads the vector.  x - mean is
+// the same rounded value whether it is stored first or formed on the fly, so
+// this is bit-identical per element to the reference.
 //
 // Every kernel of a flexible-CG iteration first checks ctl->done, so the host
 // can enqueue iterations before it knows whether the solve has converged.
 #include "dev.hpp"
 
 namespace nbl {
-namespace {
 
-__device__ __forceinline__ long long nbIndex(const MgLevel& L, long long c, int i, int j, int k, int dir) {
-  const long long sx = 1, sy = L.nx, sz = (long long)L.nx * L.ny;
-  switch (dir) {
-    case 0: return i > 0 ? c - sx : c + (L.nx - 1) * sx;
-    case 1: return i < L.nx - 1 ? c + sx : c - (L.nx - 1) * sx;
-    case 2: return j > 0 ? c - sy : c + (L.ny - 1) * sy;
-    case 3: return j < L.ny - 1 ? c + sy : c - (L.ny - 1) * sy;
-    case 4: return k > 0 ? c - sz : c + (L.nz - 1) * sz;
-    default: return k < L.nz - 1 ? c + sz : c - (L.nz - 1) * sz;
-  }
+Geo3 geo3(int nx, int ny, int nz) {
+  const int bx = nx >= 64 ? 64 : nx >= 32 ? 32 : nx >= 16 ? 16 : 8;
+  const int by = kBlock / bx;
+  Geo3 g;
+  g.block = dim3(bx, by, 1);
+  g.grid = dim3((nx + bx - 1) / bx, (ny + by - 1) / by, nz);
+  g.blocks = int(g.grid.x * g.grid.y * g.grid.z);
+  return g;
 }
 
-__device__ __forceinline__ void decode(const MgLevel& L, long long c, int& i, int& j, int& k) {
-  i = int(c % L.nx);
-  const long long q = c / L.nx;
-  j = int(q % L.ny);
-  k = int(q / L.ny);
+namespace {
+
+struct Cell {
+  int i, j, k, c, blk;
+  bool valid;
+};
+
+__device__ __forceinline__ Cell cellOf(int nx, int ny, int nz) {
+  Cell q;
+  q.i = blockIdx.x * blockDim.x + threadIdx.x;
+  q.j = blockIdx.y * blockDim.y + threadIdx.y;
+  q.k = blockIdx.z;
+  q.valid = q.i < nx && q.j < ny && q.k < nz;
+  q.c = q.i + nx * (q.j + ny * q.k);
+  q.blk = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  return q;
+}
+
+// Neighbour across face `dir` (periodic wrap where the code says kind 1 at a
+// domain face): flat index and coordinates.
+__device__ __forceinline__ int nbOf(const MgLevel& L, int c, int i, int j, int k, int dir, int& ii, int& jj, int& kk) {
+  ii = i;
+  jj = j;
+  kk = k;
+  const int sy = L.nx, sz = L.nx * L.ny;
+  switch (dir) {
+    case 0: if (i > 0) { ii = i - 1; return c - 1; } ii = L.nx - 1; return c + L.nx - 1;
+    case 1: if (i < L.nx - 1) { ii = i + 1; return c + 1; } ii = 0; return c - (L.nx - 1);
+    case 2: if (j > 0) { jj = j - 1; return c - sy; } jj = L.ny - 1; return c + (L.ny - 1) * sy;
+    case 3: if (j < L.ny - 1) { jj = j + 1; return c + sy; } jj = 0; return c - (L.ny - 1) * sy;
+    case 4: if (k > 0) { kk = k - 1; return c - sz; } kk = L.nz - 1; return c + (L.nz - 1) * sz;
+    default: if (k < L.nz - 1) { kk = k + 1; return c + sz; } kk = 0; return c - (L.nz - 1) * sz;
+  }
 }
 
 // Value of x at a fluid cell as the reference holds it at that point of the
@@ -48,11 +77,11 @@ struct XView {
   int ncx, ncy, fx, fy, fz;
   T em;
   bool hasEm;
-  __device__ __forceinline__ T at(long long c, int i, int j, int k) const {
+  __device__ __forceinline__ T at(int c, int i, int j, int k) const {
     T v = x[c];
     if (hasM) v = v - m;
     if (ec) {
-      T e = ec[(i / fx) + (long long)ncx * ((j / fy) + (long long)ncy * (k / fz))];
+      T e = ec[(i / fx) + ncx * ((j / fy) + ncy * (k / fz))];
       if (hasEm) e = e - em;
       v = v + e;
     }
@@ -60,20 +89,15 @@ struct XView {
   }
 };
 
-__device__ __forceinline__ void stepTo(int dir, int& i, int& j, int& k, const MgLevel& L) {
-  switch (dir) {
-    case 0: i = i > 0 ? i - 1 : L.nx - 1; break;
-    case 1: i = i < L.nx - 1 ? i + 1 : 0; break;
-    case 2: j = j > 0 ? j - 1 : L.ny - 1; break;
-    case 3: j = j < L.ny - 1 ? j + 1 : 0; break;
-    case 4: k = k > 0 ? k - 1 : L.nz - 1; break;
-    default: k = k < L.nz - 1 ? k + 1 : 0; break;
-  }
+template <class T>
+__device__ __forceinline__ XView<T> plainView(const T* x) {
+  return XView<T>{x, T(0), false, nullptr, 1, 1, 1, 1, 1, T(0), false};
 }
 
-// (A x)_c for a fluid cell, and the diagonal sum dg (poisson.cpp:102-134).
+// (A x)_c for a fluid cell and the diagonal sum dg (poisson.cpp:102-134): the
+// couplings are accumulated in the reference's face order.
 template <class T, bool ZERO>
-__device__ __forceinline__ T applyRow(const MgLevel& L, uint16_t code, long long c, int i, int j, int k,
+__device__ __forceinline__ T applyRow(const MgLevel& L, uint16_t code, int c, int i, int j, int k,
                                       const XView<T>& X, T& dg) {
   const T h2[3] = {T(L.h2[0]), T(L.h2[1]), T(L.h2[2])};
   T d = T(0), off = T(0);
@@ -84,9 +108,9 @@ __device__ __forceinline__ T applyRow(const MgLevel& L, uint16_t code, long long
     if (kd == 1) {
       d += h2[a];
       if (!ZERO) {
-        int ii = i, jj = j, kk = k;
-        stepTo(dir, ii, jj, kk, L);
-        off += h2[a] * X.at(nbIndex(L, c, i, j, k, dir), ii, jj, kk);
+        int ii, jj, kk;
+        const int nb = nbOf(L, c, i, j, k, dir, ii, jj, kk);
+        off += h2[a] * X.at(nb, ii, jj, kk);
       } else {
         off += h2[a] * T(0);
       }
@@ -99,11 +123,233 @@ __device__ __forceinline__ T applyRow(const MgLevel& L, uint16_t code, long long
   return d * xc - off;
 }
 
-// Coarsest level: Jacobi-PCG to 1e-6 in at most 500 iterations, as
-// CgPoisson(1e-6, 500).solve (multigrid.cpp:178-183, poisson.cpp:136-240),
-// run by one workgroup with block-wide reductions.
-constexpr int kCoarseThreads = 1024;
+// ---------------------------------------------------------------------------
+// Smoother: x <- x + 0.8 (b - A x)/diag on fluid cells, 0 elsewhere
+// (multigrid.cpp:102-114), with the fused prolongation of the parent
+// correction (multigrid.cpp:158-173) when `ec` is given, and per-block fluid
+// sums of the result for the projection that follows a sweep on singular
+// systems.  Pending means (`mb`, `mx`, `mec`) are device slots, read here.
+// ---------------------------------------------------------------------------
+template <class T, bool ZERO>
+__global__ void __launch_bounds__(kBlock) k_sweep(MgLevel L, const T* __restrict__ b, const double* mb,
+                                                  const T* __restrict__ x, const double* mx, const T* ec,
+                                                  const double* mec, int ncx, int ncy, int fx, int fy, int fz,
+                                                  T* __restrict__ xo, int fold, RedBuf rb, const SolveCtl* ctl) {
+  if (ctl && ctl->done) return;
+  const XView<T> X{x, mx ? T(*mx) : T(0), mx != nullptr, ec, ncx, ncy, fx, fy, fz, mec ? T(*mec) : T(0),
+                   mec != nullptr};
+  const Cell q = cellOf(L.nx, L.ny, L.nz);
+  double acc[1] = {0.0};
+  if (q.valid) {
+    const uint16_t code = L.code[q.c];
+    if (!(code & kFluidBit)) {
+      xo[q.c] = T(0);
+    } else {
+      T dg;
+      const T Ax = applyRow<T, ZERO>(L, code, q.c, q.i, q.j, q.k, X, dg);
+      const T xc = ZERO ? T(0) : X.at(q.c, q.i, q.j, q.k);
+      const T diag = dg > T(0) ? dg : T(1);
+      T bv = b[q.c];
+      if (mb) bv = bv - T(*mb);
+      const T xn = xc + T(0.8) * (bv - Ax) / diag;
+      xo[q.c] = xn;
+      acc[0] = double(xn);
+    }
+  }
+  if (fold) blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
+}
 
+// Residual r = b - A x on fine fluid cells restricted to the coarse grid as
+// the mean over fluid children, children summed in the reference's k,j,i
+// order (multigrid.cpp:116-154); one thread per coarse cell.  ZEROX: x == 0
+// (restriction of b itself).
+template <class T, bool ZEROX>
+__global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc, const T* __restrict__ b,
+                                                           const double* mb, const T* __restrict__ x,
+                                                           const double* mx, T* __restrict__ rc, int fold,
+                                                           RedBuf rb, const SolveCtl* ctl) {
+  if (ctl && ctl->done) return;
+  const XView<T> X{x, mx ? T(*mx) : T(0), mx != nullptr, nullptr, 1, 1, 1, 1, 1, T(0), false};
+  const Cell Q = cellOf(Lc.nx, Lc.ny, Lc.nz);
+  double acc[1] = {0.0};
+  if (Q.valid) {
+    if (!(Lc.code[Q.c] & kFluidBit)) {
+      rc[Q.c] = T(0);
+    } else {
+      const T bm = mb ? T(*mb) : T(0);
+      T s = T(0);
+      int cnt = 0;
+      const int k1 = min((Q.k + 1) * Lc.fz, L.nz), j1 = min((Q.j + 1) * Lc.fy, L.ny);
+      const int i1 = min((Q.i + 1) * Lc.fx, L.nx);
+      for (int k = Q.k * Lc.fz; k < k1; ++k)
+        for (int j = Q.j * Lc.fy; j < j1; ++j)
+          for (int i = Q.i * Lc.fx; i < i1; ++i) {
+            const int c = i + L.nx * (j + L.ny * k);
+            const uint16_t code = L.code[c];
+            if (!(code & kFluidBit)) continue;
+            T dg;
+            const T Ax = applyRow<T, ZEROX>(L, code, c, i, j, k, X, dg);
+            T bv = b[c];
+            if (mb) bv = bv - bm;
+            s += bv - Ax;
+            ++cnt;
+          }
+      const T r = s / T(cnt);
+      rc[Q.c] = r;
+      acc[0] = double(r);
+    }
+  }
+  if (fold) blockToPartials<1>(acc, rb.partials, rb.stride, Q.blk);
+}
+
+// ---------------------------------------------------------------------------
+// Coarsest level: Jacobi-PCG to 1e-6 in at most 500 iterations, as
+// CgPoisson(1e-6, 500).solve (multigrid.cpp:178-183, poisson.cpp:136-240).
+// One workgroup; x, r, z, p and the diagonal of its cells live in registers,
+// p is mirrored in LDS for the stencil, neighbour LDS indices are resolved once,
+// and an iteration costs three workgroup barriers.
+// ---------------------------------------------------------------------------
+constexpr int kCoarseT = 512;
+constexpr int kCoarseMax = 4096;
+constexpr int kCPT = kCoarseMax / kCoarseT;
+
+template <class T>
+__global__ void __launch_bounds__(kCoarseT) k_coarse_lds(MgLevel L, int singular, const T* __restrict__ b,
+                                                         const double* mb, T* __restrict__ xout,
+                                                         const SolveCtl* ctl) {
+  if (ctl && ctl->done) return;
+  __shared__ T pl[kCoarseMax];
+  __shared__ double red[3][kCoarseT / kWave];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+  constexpr int nw = kCoarseT / kWave;
+  const int N = int(L.n);
+  const T h2[3] = {T(L.h2[0]), T(L.h2[1]), T(L.h2[2])};
+  T x[kCPT], r[kCPT], z[kCPT], p[kCPT], dia[kCPT], dsum[kCPT];
+  int nb[kCPT][6];
+  bool fl[kCPT], live[kCPT];
+  const T bm = mb ? T(*mb) : T(0);
+  auto bsum = [&](double v, int slot) -> double {
+    v = waveSum(v);
+    if (lane == 0) red[slot][wid] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < nw; ++w) s += red[slot][w];
+    return s;
+  };
+  double s0 = 0.0;
+#pragma unroll
+  for (int q = 0; q < kCPT; ++q) {
+    const int c = tid + q * kCoarseT;
+    live[q] = c < N;
+    fl[q] = false;
+    x[q] = r[q] = z[q] = p[q] = dsum[q] = T(0);
+    dia[q] = T(1);
+#pragma unroll
+    for (int d = 0; d < 6; ++d) nb[q][d] = -1;
+    if (!live[q]) continue;
+    const uint16_t code = L.code[c];
+    fl[q] = code & kFluidBit;
+    T v = b[c];
+    if (mb && fl[q]) v = v - bm;
+    r[q] = v;
+    if (!fl[q]) continue;
+    s0 += double(v);
+    const int i = c % L.nx, j = (c / L.nx) % L.ny, k = c / (L.nx * L.ny);
+    T dg = T(0);
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+      const int kd = kindOf(code, d);
+      if (kd == 1) {
+        int ii, jj, kk;
+        nb[q][d] = nbOf(L, c, i, j, k, d, ii, jj, kk);
+        dg += h2[d >> 1];
+      } else if (kd == 2) {
+        dg += T(2) * h2[d >> 1];
+      }
+    }
+    dsum[q] = dg;
+    dia[q] = dg > T(0) ? dg : T(1);
+  }
+  if (singular && L.nf > 0) {
+    const T m = T(bsum(s0, 0) / double(L.nf));
+#pragma unroll
+    for (int q = 0; q < kCPT; ++q) r[q] = fl[q] ? r[q] - m : T(0);
+  }
+  double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+  for (int q = 0; q < kCPT; ++q) {
+    if (!live[q]) continue;
+    z[q] = r[q] / dia[q];
+    p[q] = z[q];
+    pl[tid + q * kCoarseT] = p[q];
+    a0 += double(r[q]) * double(z[q]);
+    a1 += double(r[q]) * double(r[q]);
+  }
+  double rz = bsum(a0, 1);
+  const double bn = sqrt(bsum(a1, 2)) + 1e-300;
+  __syncthreads();
+  for (int it = 0; it < 500; ++it) {
+    T Ap[kCPT];
+    double sp = 0.0;
+#pragma unroll
+    for (int q = 0; q < kCPT; ++q) {
+      Ap[q] = T(0);
+      if (!live[q]) continue;
+      if (fl[q]) {
+        T off = T(0);
+#pragma unroll
+        for (int dd = 0; dd < 6; ++dd)
+          if (nb[q][dd] >= 0) off += h2[dd >> 1] * pl[nb[q][dd]];
+        Ap[q] = dsum[q] * p[q] - off;
+      } else {
+        Ap[q] = p[q];
+      }
+      sp += double(p[q]) * double(Ap[q]);
+    }
+    const double pAp = bsum(sp, 0);
+    if (fabs(pAp) < 1e-300) break;
+    const double alpha = rz / pAp;
+    double rr = 0.0, rz2 = 0.0;
+#pragma unroll
+    for (int q = 0; q < kCPT; ++q) {
+      if (!live[q]) continue;
+      x[q] = x[q] + T(alpha) * p[q];
+      r[q] = r[q] - T(alpha) * Ap[q];
+      z[q] = r[q] / dia[q];
+      rr += double(r[q]) * double(r[q]);
+      rz2 += double(r[q]) * double(z[q]);
+    }
+    rr = bsum(rr, 1);
+    rz2 = bsum(rz2, 2);
+    if (sqrt(rr) <= 1e-6 * bn) break;
+    const double beta = rz2 / rz;
+    rz = rz2;
+#pragma unroll
+    for (int q = 0; q < kCPT; ++q) {
+      if (!live[q]) continue;
+      p[q] = z[q] + T(beta) * p[q];
+      pl[tid + q * kCoarseT] = p[q];
+    }
+    __syncthreads();
+  }
+  if (singular && L.nf > 0) {
+    double sx = 0.0;
+#pragma unroll
+    for (int q = 0; q < kCPT; ++q)
+      if (fl[q]) sx += double(x[q]);
+    const T m = T(bsum(sx, 0) / double(L.nf));
+#pragma unroll
+    for (int q = 0; q < kCPT; ++q)
+      if (fl[q]) x[q] = x[q] - m;
+  }
+#pragma unroll
+  for (int q = 0; q < kCPT; ++q)
+    if (live[q]) xout[tid + q * kCoarseT] = x[q];
+}
+
+// Same solve for coarsest levels above kCoarseMax cells (grids that stop
+// coarsening early): vectors in global memory, one workgroup.
 template <int K>
 __device__ __forceinline__ void blockAll(double (&v)[K], double* sh, double* bc) {
   blockFold<K, 0u>(v, sh);
@@ -115,22 +361,26 @@ __device__ __forceinline__ void blockAll(double (&v)[K], double* sh, double* bc)
 }
 
 template <class T>
-__global__ void __launch_bounds__(kCoarseThreads) k_coarse_cg(MgLevel L, int singular, const T* __restrict__ b,
-                                                              const double* mb, T* __restrict__ x,
-                                                              T* __restrict__ scr, const SolveCtl* ctl) {
+__global__ void __launch_bounds__(1024) k_coarse_global(MgLevel L, int singular, const T* __restrict__ b,
+                                                        const double* mb, T* __restrict__ x, T* __restrict__ scr,
+                                                        const SolveCtl* ctl) {
   if (ctl && ctl->done) return;
   __shared__ double sh[2 * 16];
   __shared__ double bc[2];
-  const long long N = L.n;
+  const int N = int(L.n);
   T* r = scr;
   T* z = scr + N;
   T* p = scr + 2 * N;
   T* Ap = scr + 3 * N;
   const T bm = mb ? T(*mb) : T(0);
   const int tid = threadIdx.x, nt = blockDim.x;
-  // rhs (kept in r), projected for singular systems
+  auto ijk = [&](int c, int& i, int& j, int& k) {
+    i = c % L.nx;
+    j = (c / L.nx) % L.ny;
+    k = c / (L.nx * L.ny);
+  };
   double s1[1] = {0.0};
-  for (long long c = tid; c < N; c += nt) {
+  for (int c = tid; c < N; c += nt) {
     const bool fl = L.code[c] & kFluidBit;
     T v = b[c];
     if (mb && fl) v = v - bm;
@@ -141,22 +391,22 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse_cg(MgLevel L, int sin
   blockAll<1>(s1, sh, bc);
   if (singular && L.nf > 0) {
     const T m = T(s1[0] / double(L.nf));
-    for (long long c = tid; c < N; c += nt) r[c] = (L.code[c] & kFluidBit) ? r[c] - m : T(0);
+    for (int c = tid; c < N; c += nt) r[c] = (L.code[c] & kFluidBit) ? r[c] - m : T(0);
   }
   __syncthreads();
-  const XView<T> P{p, T(0), false, nullptr, 1, 1, 1, 1, 1, T(0), false};
+  const XView<T> P = plainView<T>(p);
+  const XView<T> Z = plainView<T>(r);
+  auto diagOf = [&](int c, uint16_t code) {
+    if (!(code & kFluidBit)) return T(1);
+    int i, j, k;
+    ijk(c, i, j, k);
+    T dg;
+    applyRow<T, true>(L, code, c, i, j, k, Z, dg);
+    return dg > T(0) ? dg : T(1);
+  };
   double s2[2] = {0.0, 0.0};
-  for (long long c = tid; c < N; c += nt) {
-    const uint16_t code = L.code[c];
-    T dg = T(0);
-    if (code & kFluidBit) {
-      int i, j, k;
-      decode(L, c, i, j, k);
-      const XView<T> Z{r, T(0), false, nullptr, 1, 1, 1, 1, 1, T(0), false};
-      applyRow<T, true>(L, code, c, i, j, k, Z, dg);
-    }
-    const T diag = (code & kFluidBit) ? (dg > T(0) ? dg : T(1)) : T(1);
-    const T zv = r[c] / diag;
+  for (int c = tid; c < N; c += nt) {
+    const T zv = r[c] / diagOf(c, L.code[c]);
     z[c] = zv;
     p[c] = zv;
     s2[0] += double(r[c]) * double(zv);
@@ -167,12 +417,12 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse_cg(MgLevel L, int sin
   const double bn = sqrt(s2[1]) + 1e-300;
   for (int it = 0; it < 500; ++it) {
     double s[1] = {0.0};
-    for (long long c = tid; c < N; c += nt) {
+    for (int c = tid; c < N; c += nt) {
       const uint16_t code = L.code[c];
       T v;
       if (code & kFluidBit) {
         int i, j, k;
-        decode(L, c, i, j, k);
+        ijk(c, i, j, k);
         T dg;
         v = applyRow<T, false>(L, code, c, i, j, k, P, dg);
       } else {
@@ -182,24 +432,14 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse_cg(MgLevel L, int sin
       s[0] += double(p[c]) * double(v);
     }
     blockAll<1>(s, sh, bc);
-    const double pAp = s[0];
-    if (fabs(pAp) < 1e-300) break;
-    const double alpha = rz / pAp;
+    if (fabs(s[0]) < 1e-300) break;
+    const double alpha = rz / s[0];
     double q2[2] = {0.0, 0.0};
-    for (long long c = tid; c < N; c += nt) {
-      const uint16_t code = L.code[c];
+    for (int c = tid; c < N; c += nt) {
       x[c] = x[c] + T(alpha) * p[c];
       const T rv = r[c] - T(alpha) * Ap[c];
       r[c] = rv;
-      T dg = T(0);
-      if (code & kFluidBit) {
-        int i, j, k;
-        decode(L, c, i, j, k);
-        const XView<T> Z{r, T(0), false, nullptr, 1, 1, 1, 1, 1, T(0), false};
-        applyRow<T, true>(L, code, c, i, j, k, Z, dg);
-      }
-      const T diag = (code & kFluidBit) ? (dg > T(0) ? dg : T(1)) : T(1);
-      const T zv = rv / diag;
+      const T zv = rv / diagOf(c, L.code[c]);
       z[c] = zv;
       q2[0] += double(rv) * double(rv);
       q2[1] += double(rv) * double(zv);
@@ -208,46 +448,44 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse_cg(MgLevel L, int sin
     if (sqrt(q2[0]) <= 1e-6 * bn) break;
     const double beta = q2[1] / rz;
     rz = q2[1];
-    for (long long c = tid; c < N; c += nt) p[c] = z[c] + T(beta) * p[c];
+    for (int c = tid; c < N; c += nt) p[c] = z[c] + T(beta) * p[c];
     __syncthreads();
   }
   if (singular && L.nf > 0) {
     double sx[1] = {0.0};
-    for (long long c = tid; c < N; c += nt)
+    for (int c = tid; c < N; c += nt)
       if (L.code[c] & kFluidBit) sx[0] += double(x[c]);
     blockAll<1>(sx, sh, bc);
     const T m = T(sx[0] / double(L.nf));
-    for (long long c = tid; c < N; c += nt)
+    for (int c = tid; c < N; c += nt)
       if (L.code[c] & kFluidBit) x[c] = x[c] - m;
   }
 }
 
-// Level-0 preconditioner output: z = x - pending mean (fluid), 0 elsewhere;
-// folds the fluid sum of z for Minv's own projection (multigrid.cpp:283-289).
+// Level-0 preconditioner output: z = x - pending mean (fluid), 0 elsewhere,
+// with fluid partial sums for Minv's own projection (multigrid.cpp:283-289).
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_finish(MgLevel L, const T* __restrict__ x, const double* mx,
-                                                   double* __restrict__ z, double* meanOut, RedBuf rb,
+                                                   double* __restrict__ z, int fold, RedBuf rb,
                                                    const SolveCtl* ctl) {
   if (ctl && ctl->done) return;
-  const T m = mx ? T(*mx) : T(0);
+  const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
-  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < L.n; c += (long long)gridDim.x * blockDim.x) {
-    const bool fl = L.code[c] & kFluidBit;
-    T v = x[c];
-    if (mx && fl) v = v - m;
+  if (q.valid) {
+    const bool fl = L.code[q.c] & kFluidBit;
+    T v = x[q.c];
+    if (mx && fl) v = v - T(*mx);
     const double zv = fl ? double(v) : 0.0;
-    z[c] = zv;
-    if (fl) acc[0] += zv;
+    z[q.c] = zv;
+    acc[0] = fl ? zv : 0.0;
   }
-  if (!meanOut) return;
-  double tot[1];
-  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) *meanOut = tot[0] / double(L.nf);
+  if (fold) blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
 template <class T>
-__global__ void k_to_level0(long long n, const double* __restrict__ r, T* __restrict__ b, const SolveCtl* ctl) {
+__global__ void k_to_level0(int n, const double* __restrict__ r, T* __restrict__ b, const SolveCtl* ctl) {
   if (ctl && ctl->done) return;
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < n) b[c] = T(r[c]);
 }
 
@@ -255,554 +493,497 @@ __global__ void k_to_level0(long long n, const double* __restrict__ r, T* __rest
 // Plain operator pieces (kernel-level parity hooks)
 // ---------------------------------------------------------------------------
 __global__ void k_apply(MgLevel L, const double* __restrict__ x, double* __restrict__ out) {
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= L.n) return;
-  const uint16_t code = L.code[c];
+  const Cell q = cellOf(L.nx, L.ny, L.nz);
+  if (!q.valid) return;
+  const uint16_t code = L.code[q.c];
   if (!(code & kFluidBit)) {
-    out[c] = x[c];
+    out[q.c] = x[q.c];
     return;
   }
-  int i, j, k;
-  decode(L, c, i, j, k);
-  const XView<double> X{x, 0.0, false, nullptr, 1, 1, 1, 1, 1, 0.0, false};
   double dg;
-  out[c] = applyRow<double, false>(L, code, c, i, j, k, X, dg);
+  out[q.c] = applyRow<double, false>(L, code, q.c, q.i, q.j, q.k, plainView<double>(x), dg);
 }
 
 __global__ void k_diag(MgLevel L, double* __restrict__ out) {
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= L.n) return;
-  const uint16_t code = L.code[c];
+  const Cell q = cellOf(L.nx, L.ny, L.nz);
+  if (!q.valid) return;
+  const uint16_t code = L.code[q.c];
   if (!(code & kFluidBit)) {
-    out[c] = 1.0;
+    out[q.c] = 1.0;
     return;
   }
-  int i, j, k;
-  decode(L, c, i, j, k);
-  const XView<double> X{nullptr, 0.0, false, nullptr, 1, 1, 1, 1, 1, 0.0, false};
   double dg;
-  applyRow<double, true>(L, code, c, i, j, k, X, dg);
-  out[c] = dg > 0.0 ? dg : 1.0;
+  applyRow<double, true>(L, code, q.c, q.i, q.j, q.k, plainView<double>(nullptr), dg);
+  out[q.c] = dg > 0.0 ? dg : 1.0;
 }
 
 __global__ void k_prolong(MgLevel L, MgLevel Lc, const double* __restrict__ ec, double* __restrict__ xf) {
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= L.n) return;
-  if (!(L.code[c] & kFluidBit)) return;
-  int i, j, k;
-  decode(L, c, i, j, k);
-  xf[c] = xf[c] + ec[(i / Lc.fx) + (long long)Lc.nx * ((j / Lc.fy) + (long long)Lc.ny * (k / Lc.fz))];
+  const Cell q = cellOf(L.nx, L.ny, L.nz);
+  if (!q.valid || !(L.code[q.c] & kFluidBit)) return;
+  xf[q.c] = xf[q.c] + ec[(q.i / Lc.fx) + Lc.nx * ((q.j / Lc.fy) + Lc.ny * (q.k / Lc.fz))];
 }
 
 // ---------------------------------------------------------------------------
 // Flexible CG (MgPoisson::solve, multigrid.cpp:206-355) and Jacobi CG
-// (CgPoisson::solve, poisson.cpp:136-252).  Scalars live in SolveCtl.
+// (CgPoisson::solve, poisson.cpp:136-252).  Scalars live in SolveCtl; each
+// reducing kernel writes per-block partials and is followed by k_fin.
 // ---------------------------------------------------------------------------
 enum : int { S_MEAN_RHS = 0, S_MEAN_RN = 1, S_MEAN_X = 2 };
 
-#define GRID_LOOP(c, n) \
-  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < (n); c += (long long)gridDim.x * blockDim.x)
-
-__global__ void __launch_bounds__(kBlock) k_fcg_init(MgLevel L, int singular, int maskSolid, const double* __restrict__ b,
-                                                     double* __restrict__ rhs, double* __restrict__ x,
-                                                     SolveCtl* ctl, RedBuf rb) {
+__global__ void __launch_bounds__(kBlock) k_fcg_init(MgLevel L, int maskSolid, const double* __restrict__ b,
+                                                     double* __restrict__ rhs, double* __restrict__ x, RedBuf rb) {
+  const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
-  GRID_LOOP(c, L.n) {
-    const bool fl = L.code[c] & kFluidBit;
-    const double v = (fl || !maskSolid) ? b[c] : 0.0;
-    rhs[c] = v;
-    x[c] = 0.0;
-    if (fl) acc[0] += v;
+  if (q.valid) {
+    const bool fl = L.code[q.c] & kFluidBit;
+    const double v = (fl || !maskSolid) ? b[q.c] : 0.0;
+    rhs[q.c] = v;
+    x[q.c] = 0.0;
+    acc[0] = fl ? v : 0.0;
   }
-  if (!singular) return;
-  double tot[1];
-  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) ctl->slot[S_MEAN_RHS] = tot[0] / double(L.nf);
+  blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
-__global__ void __launch_bounds__(kBlock) k_fcg_init2(MgLevel L, int singular, int earlyExit, double* __restrict__ rhs,
-                                                      double* __restrict__ r, SolveCtl* ctl, RedBuf rb) {
-  const double m = singular ? ctl->slot[S_MEAN_RHS] : 0.0;
+// rhs <- rhs - mean (fluid) / 0 (solid) on singular systems; r = rhs; |rhs|^2.
+__global__ void __launch_bounds__(kBlock) k_fcg_init2(MgLevel L, int singular, double* __restrict__ rhs,
+                                                      double* __restrict__ r, const SolveCtl* ctl, RedBuf rb) {
+  const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
-  GRID_LOOP(c, L.n) {
-    double v = rhs[c];
+  if (q.valid) {
+    double v = rhs[q.c];
     if (singular) {
-      v = (L.code[c] & kFluidBit) ? v - m : 0.0;
-      rhs[c] = v;
+      v = (L.code[q.c] & kFluidBit) ? v - ctl->slot[S_MEAN_RHS] : 0.0;
+      rhs[q.c] = v;
     }
-    r[c] = v;
-    acc[0] += v * v;
+    r[q.c] = v;
+    acc[0] = v * v;
   }
-  double tot[1];
-  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) {
-    const double bn = sqrt(tot[0]) + 1e-300;
-    ctl->bnorm = bn;
-    ctl->it = 0;
-    ctl->done = 0;
-    ctl->converged = 0;
-    ctl->broke = 0;
-    ctl->vcycles = 0;
-    if (earlyExit && sqrt(tot[0]) <= ctl->tol * bn) {  // multigrid.cpp:292-298
-      ctl->done = 1;
-      ctl->converged = 1;
-      ctl->resid = sqrt(tot[0]) / bn;
-      ctl->slot[63] = 1.0;  // marks "returned before the loop"
-    } else {
-      ctl->slot[63] = 0.0;
-    }
-  }
+  blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
-// z <- z - pending mean (fluid); p = z; rz = r.z
+// z <- z - pending mean (fluid); p = z; r.z
 __global__ void __launch_bounds__(kBlock) k_fcg_start(MgLevel L, double* __restrict__ z, const double* mz,
                                                       const double* __restrict__ r, double* __restrict__ p,
-                                                      SolveCtl* ctl, RedBuf rb) {
+                                                      const SolveCtl* ctl, RedBuf rb) {
   if (ctl->done) return;
-  const double m = mz ? *mz : 0.0;
+  const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
-  GRID_LOOP(c, L.n) {
-    double v = z[c];
-    if (mz && (L.code[c] & kFluidBit)) v = v - m;
-    z[c] = v;
-    p[c] = v;
-    acc[0] += r[c] * v;
+  if (q.valid) {
+    double v = z[q.c];
+    if (mz && (L.code[q.c] & kFluidBit)) v = v - *mz;
+    z[q.c] = v;
+    p[q.c] = v;
+    acc[0] = r[q.c] * v;
   }
-  double tot[1];
-  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) ctl->rz = tot[0];
+  blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
 __global__ void __launch_bounds__(kBlock) k_fcg_apply(MgLevel L, const double* __restrict__ p,
-                                                      double* __restrict__ Ap, SolveCtl* ctl, RedBuf rb) {
+                                                      double* __restrict__ Ap, const SolveCtl* ctl, RedBuf rb) {
   if (ctl->done) return;
-  const XView<double> X{p, 0.0, false, nullptr, 1, 1, 1, 1, 1, 0.0, false};
+  const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
-  GRID_LOOP(c, L.n) {
-    const uint16_t code = L.code[c];
+  if (q.valid) {
+    const uint16_t code = L.code[q.c];
     double v;
     if (code & kFluidBit) {
-      int i, j, k;
-      decode(L, c, i, j, k);
       double dg;
-      v = applyRow<double, false>(L, code, c, i, j, k, X, dg);
+      v = applyRow<double, false>(L, code, q.c, q.i, q.j, q.k, plainView<double>(p), dg);
     } else {
-      v = p[c];
+      v = p[q.c];
     }
-    Ap[c] = v;
-    acc[0] += p[c] * v;
+    Ap[q.c] = v;
+    acc[0] = p[q.c] * v;
   }
-  double tot[1];
-  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) {
-    ctl->pAp = tot[0];
-    if (fabs(tot[0]) < 1e-300) {
-      ctl->done = 1;
-      ctl->broke = 1;
-    } else {
-      ctl->alpha = ctl->rz / tot[0];
-    }
-  }
+  blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
-__device__ __forceinline__ void convergeCheck(SolveCtl* ctl, double rr) {
-  if (sqrt(rr) <= ctl->tol * ctl->bnorm) {
-    ctl->it += 1;
-    ctl->converged = 1;
-    ctl->done = 1;
-  }
-}
-
-// x += alpha p; rn = r - alpha Ap; fold sum(rn) (singular) or |rn|^2.
+// x += alpha p; rn = r - alpha Ap; fluid sum of rn (singular) or |rn|^2.
 __global__ void __launch_bounds__(kBlock) k_fcg_update(MgLevel L, int singular, double* __restrict__ x,
                                                        const double* __restrict__ p, const double* __restrict__ r,
                                                        const double* __restrict__ Ap, double* __restrict__ rn,
-                                                       SolveCtl* ctl, RedBuf rb) {
+                                                       const SolveCtl* ctl, RedBuf rb) {
   if (ctl->done) return;
   const double a = ctl->alpha;
+  const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
-  GRID_LOOP(c, L.n) {
-    x[c] = x[c] + a * p[c];
-    const double v = r[c] - a * Ap[c];
-    rn[c] = v;
-    if (singular) {
-      if (L.code[c] & kFluidBit) acc[0] += v;
-    } else {
-      acc[0] += v * v;
-    }
+  if (q.valid) {
+    x[q.c] = x[q.c] + a * p[q.c];
+    const double v = r[q.c] - a * Ap[q.c];
+    rn[q.c] = v;
+    if (singular) acc[0] = (L.code[q.c] & kFluidBit) ? v : 0.0;
+    else acc[0] = v * v;
   }
-  double tot[1];
-  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) {
-    if (singular) ctl->slot[S_MEAN_RN] = tot[0] / double(L.nf);
-    else convergeCheck(ctl, tot[0]);
-  }
+  blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
-__global__ void __launch_bounds__(kBlock) k_fcg_norm(MgLevel L, double* __restrict__ rn, SolveCtl* ctl, RedBuf rb) {
+__global__ void __launch_bounds__(kBlock) k_fcg_norm(MgLevel L, double* __restrict__ rn, const SolveCtl* ctl,
+                                                     RedBuf rb) {
   if (ctl->done) return;
-  const double m = ctl->slot[S_MEAN_RN];
+  const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
-  GRID_LOOP(c, L.n) {
-    const double v = (L.code[c] & kFluidBit) ? rn[c] - m : 0.0;
-    rn[c] = v;
-    acc[0] += v * v;
+  if (q.valid) {
+    const double v = (L.code[q.c] & kFluidBit) ? rn[q.c] - ctl->slot[S_MEAN_RN] : 0.0;
+    rn[q.c] = v;
+    acc[0] = v * v;
   }
-  double tot[1];
-  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) convergeCheck(ctl, tot[0]);
+  blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
-// z <- z - pending mean; a1 = z.rn, a2 = z.r; beta = (a1 - a2)/rz (flexible,
-// Polak-Ribiere, multigrid.cpp:328-339).
+// z <- z - pending mean; z.rn and z.r (flexible Polak-Ribiere beta,
+// multigrid.cpp:328-339)
 __global__ void __launch_bounds__(kBlock) k_fcg_dots(MgLevel L, double* __restrict__ z, const double* mz,
                                                      const double* __restrict__ rn, const double* __restrict__ r,
-                                                     SolveCtl* ctl, RedBuf rb, int maxIt) {
+                                                     const SolveCtl* ctl, RedBuf rb) {
   if (ctl->done) return;
-  const double m = mz ? *mz : 0.0;
+  const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[2] = {0.0, 0.0};
-  GRID_LOOP(c, L.n) {
-    double v = z[c];
-    if (mz && (L.code[c] & kFluidBit)) v = v - m;
-    z[c] = v;
-    acc[0] += v * rn[c];
-    acc[1] += v * r[c];
+  if (q.valid) {
+    double v = z[q.c];
+    if (mz && (L.code[q.c] & kFluidBit)) v = v - *mz;
+    z[q.c] = v;
+    acc[0] = v * rn[q.c];
+    acc[1] = v * r[q.c];
   }
-  double tot[2];
-  if (gridFold<2>(acc, rb, tot) && threadIdx.x == 0) {
-    ctl->beta = (tot[0] - tot[1]) / ctl->rz;
-    ctl->rz = tot[0];
-    ctl->it += 1;
-    if (ctl->it >= maxIt) ctl->done = 1;
-  }
+  blockToPartials<2>(acc, rb.partials, rb.stride, q.blk);
 }
 
-__global__ void __launch_bounds__(kBlock) k_fcg_pupdate(long long n, double* __restrict__ p, const double* __restrict__ z,
+__global__ void __launch_bounds__(kBlock) k_fcg_pupdate(int n, double* __restrict__ p, const double* __restrict__ z,
                                                         double* __restrict__ r, const double* __restrict__ rn,
-                                                        SolveCtl* ctl) {
+                                                        const SolveCtl* ctl) {
   if (ctl->done) return;
   const double be = ctl->beta;
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   p[c] = z[c] + be * p[c];
   if (r) r[c] = rn[c];
 }
 
-__global__ void __launch_bounds__(kBlock) k_fluid_sum(MgLevel L, const double* __restrict__ x, SolveCtl* ctl,
-                                                      RedBuf rb) {
+__global__ void __launch_bounds__(kBlock) k_fluid_sum(MgLevel L, const double* __restrict__ x, RedBuf rb) {
+  const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
-  GRID_LOOP(c, L.n) if (L.code[c] & kFluidBit) acc[0] += x[c];
-  double tot[1];
-  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) ctl->slot[S_MEAN_X] = tot[0] / double(L.nf);
+  if (q.valid && (L.code[q.c] & kFluidBit)) acc[0] = x[q.c];
+  blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
-// Final projection of x and the true relative residual (multigrid.cpp:342-354).
+// Final projection of x and the true residual (multigrid.cpp:342-354).
 __global__ void __launch_bounds__(kBlock) k_fcg_final(MgLevel L, int singular, const double* __restrict__ x,
                                                       const double* __restrict__ rhs, double* __restrict__ xo,
-                                                      SolveCtl* ctl, RedBuf rb, double convTol) {
+                                                      const SolveCtl* ctl, RedBuf rb) {
   const double m = singular ? ctl->slot[S_MEAN_X] : 0.0;
   const XView<double> X{x, m, singular != 0, nullptr, 1, 1, 1, 1, 1, 0.0, false};
+  const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
-  GRID_LOOP(c, L.n) {
-    const uint16_t code = L.code[c];
+  if (q.valid) {
+    const uint16_t code = L.code[q.c];
     double Ax, xv;
     if (code & kFluidBit) {
-      int i, j, k;
-      decode(L, c, i, j, k);
       double dg;
-      Ax = applyRow<double, false>(L, code, c, i, j, k, X, dg);
-      xv = singular ? x[c] - m : x[c];
+      Ax = applyRow<double, false>(L, code, q.c, q.i, q.j, q.k, X, dg);
+      xv = singular ? x[q.c] - m : x[q.c];
     } else {
-      xv = singular ? 0.0 : x[c];
+      xv = singular ? 0.0 : x[q.c];
       Ax = xv;
     }
-    xo[c] = xv;
-    const double e = rhs[c] - Ax;
-    acc[0] += e * e;
+    xo[q.c] = xv;
+    const double e = rhs[q.c] - Ax;
+    acc[0] = e * e;
   }
-  double tot[1];
-  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) {
-    if (ctl->slot[63] != 0.0) return;  // returned before the loop: keep that report
-    ctl->resid = sqrt(tot[0]) / ctl->bnorm;
-    ctl->converged = (ctl->converged || ctl->resid <= convTol) ? 1 : 0;
-  }
+  blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
-// Jacobi-CG pieces: z = r/diag and its dot with r.
+// Jacobi-CG: z = r/diag and r.z (first: also p = z).
 __global__ void __launch_bounds__(kBlock) k_cg_z(MgLevel L, const double* __restrict__ r, double* __restrict__ z,
-                                                 double* __restrict__ p, SolveCtl* ctl, RedBuf rb, int first,
-                                                 int maxIt) {
+                                                 double* __restrict__ p, int first, const SolveCtl* ctl, RedBuf rb) {
   if (ctl->done) return;
-  const XView<double> X{nullptr, 0.0, false, nullptr, 1, 1, 1, 1, 1, 0.0, false};
+  const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
-  GRID_LOOP(c, L.n) {
-    const uint16_t code = L.code[c];
+  if (q.valid) {
+    const uint16_t code = L.code[q.c];
     double diag = 1.0;
     if (code & kFluidBit) {
-      int i, j, k;
-      decode(L, c, i, j, k);
       double dg;
-      applyRow<double, true>(L, code, c, i, j, k, X, dg);
+      applyRow<double, true>(L, code, q.c, q.i, q.j, q.k, plainView<double>(nullptr), dg);
       diag = dg > 0.0 ? dg : 1.0;
     }
-    const double zv = r[c] / diag;
-    z[c] = zv;
-    if (first) p[c] = zv;
-    acc[0] += r[c] * zv;
+    const double zv = r[q.c] / diag;
+    z[q.c] = zv;
+    if (first) p[q.c] = zv;
+    acc[0] = r[q.c] * zv;
   }
-  double tot[1];
-  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) {
-    if (first) {
-      ctl->rz = tot[0];
-    } else {
-      ctl->beta = tot[0] / ctl->rz;
-      ctl->rz = tot[0];
-      ctl->it += 1;
-      if (ctl->it >= maxIt) ctl->done = 1;
-    }
-  }
+  blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
 __global__ void __launch_bounds__(kBlock) k_cg_update(MgLevel L, double* __restrict__ x, const double* __restrict__ p,
                                                       double* __restrict__ r, const double* __restrict__ Ap,
-                                                      SolveCtl* ctl, RedBuf rb) {
+                                                      const SolveCtl* ctl, RedBuf rb) {
   if (ctl->done) return;
   const double a = ctl->alpha;
+  const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
-  GRID_LOOP(c, L.n) {
-    x[c] = x[c] + a * p[c];
-    const double v = r[c] - a * Ap[c];
-    r[c] = v;
-    acc[0] += v * v;
+  if (q.valid) {
+    x[q.c] = x[q.c] + a * p[q.c];
+    const double v = r[q.c] - a * Ap[q.c];
+    r[q.c] = v;
+    acc[0] = v * v;
   }
-  double tot[1];
-  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) convergeCheck(ctl, tot[0]);
+  blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
-int redGrid(long long n) { return gridFor(n, kBlock, kMaxRedBlocks); }
+// One-block finalize: fold the partials of the preceding reducing kernel and
+// apply the scalar update `op`.
+__global__ void __launch_bounds__(kFinThreads) k_fin(RedBuf rb, FinArgs a, SolveCtl* ctl) {
+  if (a.predicated && ctl->done) return;
+  double t[2];
+  if (a.K == 2) {
+    foldPartials<2>(rb.partials, rb.stride, a.nparts, t);
+  } else {
+    double t1[1];
+    foldPartials<1>(rb.partials, rb.stride, a.nparts, t1);
+    t[0] = t1[0];
+    t[1] = 0.0;
+  }
+  if (threadIdx.x != 0) return;
+  switch (a.op) {
+    case FIN_MEAN:
+      ctl->slot[a.slot] = a.nf > 0 ? t[0] / double(a.nf) : 0.0;
+      break;
+    case FIN_BNORM: {  // multigrid.cpp:281, 292-298
+      const double bn = sqrt(t[0]) + 1e-300;
+      ctl->bnorm = bn;
+      ctl->it = 0;
+      ctl->done = 0;
+      ctl->converged = 0;
+      ctl->broke = 0;
+      ctl->early = 0;
+      if (a.earlyExit && sqrt(t[0]) <= ctl->tol * bn) {
+        ctl->done = 1;
+        ctl->converged = 1;
+        ctl->early = 1;
+        ctl->resid = sqrt(t[0]) / bn;
+      }
+      break;
+    }
+    case FIN_RZ:
+      ctl->rz = t[0];
+      break;
+    case FIN_ALPHA:
+      ctl->pAp = t[0];
+      if (fabs(t[0]) < 1e-300) {
+        ctl->done = 1;
+        ctl->broke = 1;
+      } else {
+        ctl->alpha = ctl->rz / t[0];
+      }
+      break;
+    case FIN_CONV:
+      ctl->rr = t[0];
+      if (sqrt(t[0]) <= ctl->tol * ctl->bnorm) {
+        ctl->it += 1;
+        ctl->converged = 1;
+        ctl->done = 1;
+      }
+      break;
+    case FIN_BETA_FLEX:
+      ctl->beta = (t[0] - t[1]) / ctl->rz;
+      ctl->rz = t[0];
+      ctl->it += 1;
+      break;
+    case FIN_BETA:
+      ctl->beta = t[0] / ctl->rz;
+      ctl->rz = t[0];
+      ctl->it += 1;
+      break;
+    case FIN_RESID:
+      if (ctl->early) break;  // returned before the loop: keep that report
+      ctl->resid = sqrt(t[0]) / ctl->bnorm;
+      ctl->converged = (ctl->converged || ctl->resid <= a.convTol) ? 1 : 0;
+      break;
+    case FIN_BNORM_MEAN:
+      ctl->bnorm = sqrt(t[0]) + 1e-300;
+      ctl->slot[a.slot] = a.nf > 0 ? t[1] / double(a.nf) : 0.0;
+      ctl->it = 0;
+      ctl->done = 0;
+      ctl->converged = 0;
+      ctl->broke = 0;
+      ctl->early = 0;
+      break;
+    case FIN_CONV_MEAN:
+      ctl->slot[a.slot] = a.nf > 0 ? t[1] / double(a.nf) : 0.0;
+      ctl->rr = t[0];
+      if (sqrt(t[0]) <= ctl->tol * ctl->bnorm) {
+        ctl->it += 1;
+        ctl->converged = 1;
+        ctl->done = 1;
+      }
+      break;
+  }
+}
 
 }  // namespace
 
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-// Smoother: x <- x + 0.8 (b - A x)/diag on fluid cells, 0 elsewhere
-// (multigrid.cpp:102-114), with the fused prolongation of the parent
-// correction (multigrid.cpp:158-173) when `ec` is given, and the fluid sum of
-// the result for the projection that follows a sweep on singular systems.
-// Pending means (`mb`, `mx`) are device slots, read here.
-template <class T, bool ZERO>
-__global__ void __launch_bounds__(kBlock) k_sweep_w(MgLevel L, const T* __restrict__ b, const double* mb,
-                                                    const T* __restrict__ x, const double* mx, const T* ec,
-                                                    const double* mec, int ncx, int ncy, int fx, int fy, int fz,
-                                                    T* __restrict__ xo, double* meanOut, RedBuf rb,
-                                                    const SolveCtl* ctl) {
-  if (ctl && ctl->done) return;
-  XView<T> X{x, mx ? T(*mx) : T(0), mx != nullptr, ec, ncx, ncy, fx, fy, fz, mec ? T(*mec) : T(0), mec != nullptr};
-  const T bm = mb ? T(*mb) : T(0);
-  double acc[1] = {0.0};
-  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < L.n; c += (long long)gridDim.x * blockDim.x) {
-    const uint16_t code = L.code[c];
-    if (!(code & kFluidBit)) {
-      xo[c] = T(0);
-      continue;
-    }
-    int i, j, k;
-    decode(L, c, i, j, k);
-    T dg;
-    const T Ax = applyRow<T, ZERO>(L, code, c, i, j, k, X, dg);
-    const T xc = ZERO ? T(0) : X.at(c, i, j, k);
-    const T diag = dg > T(0) ? dg : T(1);
-    T bv = b[c];
-    if (mb) bv = bv - bm;
-    const T xn = xc + T(0.8) * (bv - Ax) / diag;
-    xo[c] = xn;
-    acc[0] += double(xn);
-  }
-  if (!meanOut) return;
-  double tot[1];
-  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) *meanOut = tot[0] / double(L.nf);
-}
-
-// Residual r = b - A x on fine fluid cells restricted to the coarse grid as
-// the mean over fluid children, children summed in the reference's k,j,i
-// order (multigrid.cpp:116-154).  ZEROX: x == 0 (restriction of b itself).
-template <class T, bool ZEROX>
-__global__ void __launch_bounds__(kBlock) k_resid_restrict_w(MgLevel L, MgLevel Lc, const T* __restrict__ b,
-                                                             const double* mb, const T* __restrict__ x,
-                                                             const double* mx, T* __restrict__ rc, double* meanOut,
-                                                             RedBuf rb, const SolveCtl* ctl) {
-  if (ctl && ctl->done) return;
-  XView<T> X{x, mx ? T(*mx) : T(0), mx != nullptr, nullptr, 1, 1, 1, 1, 1, T(0), false};
-  const T bm = mb ? T(*mb) : T(0);
-  double acc[1] = {0.0};
-  for (long long C = (long long)blockIdx.x * blockDim.x + threadIdx.x; C < Lc.n; C += (long long)gridDim.x * blockDim.x) {
-    if (!(Lc.code[C] & kFluidBit)) {
-      rc[C] = T(0);
-      continue;
-    }
-    int I, J, K;
-    decode(Lc, C, I, J, K);
-    T s = T(0);
-    int cnt = 0;
-    const int k1 = min((K + 1) * Lc.fz, L.nz), j1 = min((J + 1) * Lc.fy, L.ny), i1 = min((I + 1) * Lc.fx, L.nx);
-    for (int k = K * Lc.fz; k < k1; ++k)
-      for (int j = J * Lc.fy; j < j1; ++j)
-        for (int i = I * Lc.fx; i < i1; ++i) {
-          const long long c = i + (long long)L.nx * (j + (long long)L.ny * k);
-          const uint16_t code = L.code[c];
-          if (!(code & kFluidBit)) continue;
-          T dg;
-          const T Ax = applyRow<T, ZEROX>(L, code, c, i, j, k, X, dg);
-          T bv = b[c];
-          if (mb) bv = bv - bm;
-          s += bv - Ax;
-          ++cnt;
-        }
-    const T r = s / T(cnt);
-    rc[C] = r;
-    acc[0] += double(r);
-  }
-  if (!meanOut) return;
-  double tot[1];
-  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) *meanOut = tot[0] / double(Lc.nf);
+void launch_fin(RedBuf rb, const FinArgs& a, SolveCtl* ctl, hipStream_t s) {
+  k_fin<<<1, kFinThreads, 0, s>>>(rb, a, ctl);
+  NBL_LAUNCH_CHECK();
 }
 
 template <class T>
 void launch_mg_sweep(const MgLevel& L, const T* b, const double* mb, const T* x, const double* mx,
-                     bool zeroGuess, const T* ec, const double* mec, const MgLevel* Lc, T* xo,
-                     double* meanOut, RedBuf rb, const SolveCtl* ctl, hipStream_t s) {
-  const int g = redGrid(L.n);
+                     bool zeroGuess, const T* ec, const double* mec, const MgLevel* Lc, T* xo, bool fold,
+                     RedBuf rb, const SolveCtl* ctl, hipStream_t s) {
+  const Geo3 g = geo3(L.nx, L.ny, L.nz);
   const int ncx = Lc ? Lc->nx : 1, ncy = Lc ? Lc->ny : 1;
   const int fx = Lc ? Lc->fx : 1, fy = Lc ? Lc->fy : 1, fz = Lc ? Lc->fz : 1;
   if (zeroGuess)
-    k_sweep_w<T, true><<<g, kBlock, 0, s>>>(L, b, mb, x, mx, nullptr, nullptr, 1, 1, 1, 1, 1, xo, meanOut, rb,
-                                            ctl);
+    k_sweep<T, true><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, nullptr, nullptr, 1, 1, 1, 1, 1, xo, fold ? 1 : 0,
+                                                rb, ctl);
   else
-    k_sweep_w<T, false><<<g, kBlock, 0, s>>>(L, b, mb, x, mx, ec, mec, ncx, ncy, fx, fy, fz, xo, meanOut, rb,
-                                             ctl);
+    k_sweep<T, false><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, ec, mec, ncx, ncy, fx, fy, fz, xo, fold ? 1 : 0,
+                                                 rb, ctl);
   NBL_LAUNCH_CHECK();
 }
 
 template <class T>
-void launch_mg_resid_restrict(const MgLevel& L, const MgLevel& Lc, const T* b, const double* mb,
-                              const T* x, const double* mx, T* rc, double* meanOut, RedBuf rb,
-                              const SolveCtl* ctl, hipStream_t s) {
-  k_resid_restrict_w<T, false><<<redGrid(Lc.n), kBlock, 0, s>>>(L, Lc, b, mb, x, mx, rc, meanOut, rb, ctl);
+void launch_mg_resid_restrict(const MgLevel& L, const MgLevel& Lc, const T* b, const double* mb, const T* x,
+                              const double* mx, T* rc, bool fold, RedBuf rb, const SolveCtl* ctl, hipStream_t s) {
+  const Geo3 g = geo3(Lc.nx, Lc.ny, Lc.nz);
+  k_resid_restrict<T, false><<<g.grid, g.block, 0, s>>>(L, Lc, b, mb, x, mx, rc, fold ? 1 : 0, rb, ctl);
   NBL_LAUNCH_CHECK();
 }
 
 template <class T>
-void launch_mg_coarse(const MgLevel& L, bool singular, const T* b, const double* mb, T* x,
-                      T* scratch, const SolveCtl* ctl, hipStream_t s) {
-  k_coarse_cg<T><<<1, kCoarseThreads, 0, s>>>(L, singular ? 1 : 0, b, mb, x, scratch, ctl);
+void launch_mg_coarse(const MgLevel& L, bool singular, const T* b, const double* mb, T* x, T* scratch,
+                      const SolveCtl* ctl, hipStream_t s) {
+  if (L.n <= kCoarseMax)
+    k_coarse_lds<T><<<1, kCoarseT, 0, s>>>(L, singular ? 1 : 0, b, mb, x, ctl);
+  else
+    k_coarse_global<T><<<1, 1024, 0, s>>>(L, singular ? 1 : 0, b, mb, x, scratch, ctl);
   NBL_LAUNCH_CHECK();
 }
 
 template <class T>
-void launch_mg_finish(const MgLevel& L, const T* x, const double* mx, double* z, double* meanOut,
-                      RedBuf rb, const SolveCtl* ctl, hipStream_t s) {
-  k_finish<T><<<redGrid(L.n), kBlock, 0, s>>>(L, x, mx, z, meanOut, rb, ctl);
+void launch_mg_finish(const MgLevel& L, const T* x, const double* mx, double* z, bool fold, RedBuf rb,
+                      const SolveCtl* ctl, hipStream_t s) {
+  const Geo3 g = geo3(L.nx, L.ny, L.nz);
+  k_finish<T><<<g.grid, g.block, 0, s>>>(L, x, mx, z, fold ? 1 : 0, rb, ctl);
   NBL_LAUNCH_CHECK();
 }
 
 template <class T>
 void launch_to_level0(const MgLevel& L, const double* r, T* b, const SolveCtl* ctl, hipStream_t s) {
-  k_to_level0<T><<<gridFor(L.n), kBlock, 0, s>>>(L.n, r, b, ctl);
+  k_to_level0<T><<<gridFor(L.n), kBlock, 0, s>>>(int(L.n), r, b, ctl);
   NBL_LAUNCH_CHECK();
 }
 
-#define NBL_INST(T)                                                                                        \
-  template void launch_mg_sweep<T>(const MgLevel&, const T*, const double*, const T*, const double*, bool,  \
-                                   const T*, const double*, const MgLevel*, T*, double*, RedBuf,            \
-                                   const SolveCtl*, hipStream_t);                                          \
-  template void launch_mg_resid_restrict<T>(const MgLevel&, const MgLevel&, const T*, const double*,       \
-                                            const T*, const double*, T*, double*, RedBuf, const SolveCtl*, \
-                                            hipStream_t);                                                  \
-  template void launch_mg_coarse<T>(const MgLevel&, bool, const T*, const double*, T*, T*,                 \
-                                    const SolveCtl*, hipStream_t);                                         \
-  template void launch_mg_finish<T>(const MgLevel&, const T*, const double*, double*, double*, RedBuf,     \
-                                    const SolveCtl*, hipStream_t);                                         \
+#define NBL_INST(T)                                                                                          \
+  template void launch_mg_sweep<T>(const MgLevel&, const T*, const double*, const T*, const double*, bool,    \
+                                   const T*, const double*, const MgLevel*, T*, bool, RedBuf, const SolveCtl*, \
+                                   hipStream_t);                                                             \
+  template void launch_mg_resid_restrict<T>(const MgLevel&, const MgLevel&, const T*, const double*,         \
+                                            const T*, const double*, T*, bool, RedBuf, const SolveCtl*,      \
+                                            hipStream_t);                                                    \
+  template void launch_mg_coarse<T>(const MgLevel&, bool, const T*, const double*, T*, T*,                   \
+                                    const SolveCtl*, hipStream_t);                                           \
+  template void launch_mg_finish<T>(const MgLevel&, const T*, const double*, double*, bool, RedBuf,          \
+                                    const SolveCtl*, hipStream_t);                                           \
   template void launch_to_level0<T>(const MgLevel&, const double*, T*, const SolveCtl*, hipStream_t);
 NBL_INST(double)
 NBL_INST(float)
 #undef NBL_INST
 
 void launch_apply(const MgLevel& L, const double* x, double* out, hipStream_t s) {
-  k_apply<<<gridFor(L.n), kBlock, 0, s>>>(L, x, out);
+  const Geo3 g = geo3(L.nx, L.ny, L.nz);
+  k_apply<<<g.grid, g.block, 0, s>>>(L, x, out);
   NBL_LAUNCH_CHECK();
 }
 void launch_diag(const MgLevel& L, double* out, hipStream_t s) {
-  k_diag<<<gridFor(L.n), kBlock, 0, s>>>(L, out);
+  const Geo3 g = geo3(L.nx, L.ny, L.nz);
+  k_diag<<<g.grid, g.block, 0, s>>>(L, out);
   NBL_LAUNCH_CHECK();
 }
 void launch_restrict_plain(const MgLevel& L, const MgLevel& Lc, const double* rf, double* rc, hipStream_t s) {
   // rf - A*0 == rf exactly, so the fused kernel with a zero x restricts rf itself
-  k_resid_restrict_w<double, true><<<redGrid(Lc.n), kBlock, 0, s>>>(L, Lc, rf, nullptr, nullptr, nullptr, rc, nullptr,
-                                                                RedBuf{nullptr, nullptr}, nullptr);
+  const Geo3 g = geo3(Lc.nx, Lc.ny, Lc.nz);
+  k_resid_restrict<double, true><<<g.grid, g.block, 0, s>>>(L, Lc, rf, nullptr, nullptr, nullptr, rc, 0,
+                                                            RedBuf{nullptr, 0}, nullptr);
   NBL_LAUNCH_CHECK();
 }
 void launch_prolong_plain(const MgLevel& L, const MgLevel& Lc, const double* ec, double* xf, hipStream_t s) {
-  k_prolong<<<gridFor(L.n), kBlock, 0, s>>>(L, Lc, ec, xf);
+  const Geo3 g = geo3(L.nx, L.ny, L.nz);
+  k_prolong<<<g.grid, g.block, 0, s>>>(L, Lc, ec, xf);
   NBL_LAUNCH_CHECK();
 }
 
-void launch_fcg_init(const MgLevel& L, bool singular, bool maskSolid, const double* b, double* rhs,
-                     double* x, SolveCtl* ctl, RedBuf rb, hipStream_t s) {
-  k_fcg_init<<<redGrid(L.n), kBlock, 0, s>>>(L, singular ? 1 : 0, maskSolid ? 1 : 0, b, rhs, x, ctl, rb);
+void launch_fcg_init(const MgLevel& L, bool maskSolid, const double* b, double* rhs, double* x, RedBuf rb,
+                     hipStream_t s) {
+  const Geo3 g = geo3(L.nx, L.ny, L.nz);
+  k_fcg_init<<<g.grid, g.block, 0, s>>>(L, maskSolid ? 1 : 0, b, rhs, x, rb);
   NBL_LAUNCH_CHECK();
 }
-void launch_fcg_init2(const MgLevel& L, bool singular, bool earlyExit, double* rhs, double* r,
-                      SolveCtl* ctl, RedBuf rb, hipStream_t s) {
-  k_fcg_init2<<<redGrid(L.n), kBlock, 0, s>>>(L, singular ? 1 : 0, earlyExit ? 1 : 0, rhs, r, ctl, rb);
-  NBL_LAUNCH_CHECK();
-}
-void launch_fcg_start(const MgLevel& L, const double* z, const double* mz, const double* r,
-                      double* zm, double* p, SolveCtl* ctl, RedBuf rb, hipStream_t s) {
-  (void)z;
-  k_fcg_start<<<redGrid(L.n), kBlock, 0, s>>>(L, zm, mz, r, p, ctl, rb);
-  NBL_LAUNCH_CHECK();
-}
-void launch_fcg_apply(const MgLevel& L, const double* p, double* Ap, SolveCtl* ctl, RedBuf rb,
+void launch_fcg_init2(const MgLevel& L, bool singular, double* rhs, double* r, const SolveCtl* ctl, RedBuf rb,
                       hipStream_t s) {
-  k_fcg_apply<<<redGrid(L.n), kBlock, 0, s>>>(L, p, Ap, ctl, rb);
+  const Geo3 g = geo3(L.nx, L.ny, L.nz);
+  k_fcg_init2<<<g.grid, g.block, 0, s>>>(L, singular ? 1 : 0, rhs, r, ctl, rb);
   NBL_LAUNCH_CHECK();
 }
-void launch_fcg_update(const MgLevel& L, bool singular, double* x, const double* p,
-                       const double* r, const double* Ap, double* rn, SolveCtl* ctl, RedBuf rb,
-                       hipStream_t s) {
-  k_fcg_update<<<redGrid(L.n), kBlock, 0, s>>>(L, singular ? 1 : 0, x, p, r, Ap, rn, ctl, rb);
+void launch_fcg_start(const MgLevel& L, double* z, const double* mz, const double* r, double* p,
+                      const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
+  const Geo3 g = geo3(L.nx, L.ny, L.nz);
+  k_fcg_start<<<g.grid, g.block, 0, s>>>(L, z, mz, r, p, ctl, rb);
   NBL_LAUNCH_CHECK();
 }
-void launch_fcg_norm(const MgLevel& L, double* rn, SolveCtl* ctl, RedBuf rb, hipStream_t s) {
-  k_fcg_norm<<<redGrid(L.n), kBlock, 0, s>>>(L, rn, ctl, rb);
+void launch_fcg_apply(const MgLevel& L, const double* p, double* Ap, const SolveCtl* ctl, RedBuf rb,
+                      hipStream_t s) {
+  const Geo3 g = geo3(L.nx, L.ny, L.nz);
+  k_fcg_apply<<<g.grid, g.block, 0, s>>>(L, p, Ap, ctl, rb);
   NBL_LAUNCH_CHECK();
 }
-void launch_fcg_dots(const MgLevel& L, double* z, const double* mz, const double* rn,
-                     const double* r, SolveCtl* ctl, RedBuf rb, hipStream_t s) {
-  k_fcg_dots<<<redGrid(L.n), kBlock, 0, s>>>(L, z, mz, rn, r, ctl, rb, 1 << 30);
+void launch_fcg_update(const MgLevel& L, bool singular, double* x, const double* p, const double* r,
+                       const double* Ap, double* rn, const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
+  const Geo3 g = geo3(L.nx, L.ny, L.nz);
+  k_fcg_update<<<g.grid, g.block, 0, s>>>(L, singular ? 1 : 0, x, p, r, Ap, rn, ctl, rb);
   NBL_LAUNCH_CHECK();
 }
-void launch_fcg_pupdate(const MgLevel& L, double* p, const double* z, double* r,
-                        const double* rn, SolveCtl* ctl, hipStream_t s) {
-  k_fcg_pupdate<<<gridFor(L.n), kBlock, 0, s>>>(L.n, p, z, r, rn, ctl);
+void launch_fcg_norm(const MgLevel& L, double* rn, const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
+  const Geo3 g = geo3(L.nx, L.ny, L.nz);
+  k_fcg_norm<<<g.grid, g.block, 0, s>>>(L, rn, ctl, rb);
   NBL_LAUNCH_CHECK();
 }
-void launch_fcg_final(const MgLevel& L, bool singular, const double* x, const double* rhs,
-                      double* xo, SolveCtl* ctl, RedBuf rb, double convTol, hipStream_t s) {
-  if (singular) {
-    k_fluid_sum<<<redGrid(L.n), kBlock, 0, s>>>(L, x, ctl, rb);
-    NBL_LAUNCH_CHECK();
-  }
-  k_fcg_final<<<redGrid(L.n), kBlock, 0, s>>>(L, singular ? 1 : 0, x, rhs, xo, ctl, rb, convTol);
+void launch_fcg_dots(const MgLevel& L, double* z, const double* mz, const double* rn, const double* r,
+                     const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
+  const Geo3 g = geo3(L.nx, L.ny, L.nz);
+  k_fcg_dots<<<g.grid, g.block, 0, s>>>(L, z, mz, rn, r, ctl, rb);
   NBL_LAUNCH_CHECK();
 }
-void launch_cg_z(const MgLevel& L, const double* r, double* z, SolveCtl* ctl, RedBuf rb,
-                 bool first, double* p, hipStream_t s) {
-  k_cg_z<<<redGrid(L.n), kBlock, 0, s>>>(L, r, z, p, ctl, rb, first ? 1 : 0, 1 << 30);
+void launch_fcg_pupdate(const MgLevel& L, double* p, const double* z, double* r, const double* rn,
+                        const SolveCtl* ctl, hipStream_t s) {
+  k_fcg_pupdate<<<gridFor(L.n), kBlock, 0, s>>>(int(L.n), p, z, r, rn, ctl);
+  NBL_LAUNCH_CHECK();
+}
+void launch_fluid_sum(const MgLevel& L, const double* x, RedBuf rb, hipStream_t s) {
+  const Geo3 g = geo3(L.nx, L.ny, L.nz);
+  k_fluid_sum<<<g.grid, g.block, 0, s>>>(L, x, rb);
+  NBL_LAUNCH_CHECK();
+}
+void launch_fcg_final(const MgLevel& L, bool singular, const double* x, const double* rhs, double* xo,
+                      const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
+  const Geo3 g = geo3(L.nx, L.ny, L.nz);
+  k_fcg_final<<<g.grid, g.block, 0, s>>>(L, singular ? 1 : 0, x, rhs, xo, ctl, rb);
+  NBL_LAUNCH_CHECK();
+}
+void launch_cg_z(const MgLevel& L, const double* r, double* z, bool first, double* p, const SolveCtl* ctl,
+                 RedBuf rb, hipStream_t s) {
+  const Geo3 g = geo3(L.nx, L.ny, L.nz);
+  k_cg_z<<<g.grid, g.block, 0, s>>>(L, r, z, p, first ? 1 : 0, ctl, rb);
   NBL_LAUNCH_CHECK();
 }
 void launch_cg_update(const MgLevel& L, double* x, const double* p, double* r, const double* Ap,
-                      SolveCtl* ctl, RedBuf rb, hipStream_t s) {
-  k_cg_update<<<redGrid(L.n), kBlock, 0, s>>>(L, x, p, r, Ap, ctl, rb);
-  NBL_LAUNCH_CHECK();
-}
-void launch_cg_pupdate(const MgLevel& L, double* p, const double* z, SolveCtl* ctl, hipStream_t s) {
-  k_fcg_pupdate<<<gridFor(L.n), kBlock, 0, s>>>(L.n, p, z, nullptr, nullptr, ctl);
+                      const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
+  const Geo3 g = geo3(L.nx, L.ny, L.nz);
+  k_cg_update<<<g.grid, g.block, 0, s>>>(L, x, p, r, Ap, ctl, rb);
   NBL_LAUNCH_CHECK();
 }
 

@@ -14,19 +14,19 @@
 namespace nbl {
 namespace {
 
-struct Idx {  // staggered index helpers (ns_solver.hpp:93-110)
+struct Idx {  // staggered index helpers (ns_solver.hpp:93-110); arrays < 2^31 elements
   int nx, ny, nz;
-  __device__ long long u(int i, int j, int k) const { return i + (long long)(nx + 1) * (j + (long long)ny * k); }
-  __device__ long long v(int i, int j, int k) const { return i + (long long)nx * (j + (long long)(ny + 1) * k); }
-  __device__ long long w(int i, int j, int k) const { return i + (long long)nx * (j + (long long)ny * k); }
-  __device__ long long c(int i, int j, int k) const { return i + (long long)nx * (j + (long long)ny * k); }
+  __device__ int u(int i, int j, int k) const { return i + (nx + 1) * (j + ny * k); }
+  __device__ int v(int i, int j, int k) const { return i + nx * (j + (ny + 1) * k); }
+  __device__ int w(int i, int j, int k) const { return i + nx * (j + ny * k); }
+  __device__ int c(int i, int j, int k) const { return i + nx * (j + ny * k); }
 };
 
 __device__ __forceinline__ int wrapi(int a, int n) { return (a % n + n) % n; }
 
 __device__ __forceinline__ bool solidAt(const DevGrid& g, int i, int j, int k) {
   if (i < 0 || j < 0 || k < 0 || i >= g.nx || j >= g.ny || k >= g.nz) return false;
-  return g.solid[i + (long long)g.nx * (j + (long long)g.ny * k)] != 0;
+  return g.solid[i + g.nx * (j + g.ny * k)] != 0;
 }
 
 // Inlet velocity component `comp` on face f at transverse index a (used as the
@@ -47,15 +47,20 @@ __device__ double inletComp(const DevGrid& g, int f, int a, int comp) {
 // Reflections pending while a ghost index is resolved: the reference evaluates
 // `2*X - inner` recursively, so they are applied innermost first.
 struct Refl {
-  double X[3];
+  double X0 = 0.0, X1 = 0.0, X2 = 0.0;
   int d = 0;
-  __device__ bool push(double x) {
-    if (d >= 3) return false;
-    X[d++] = x;
+  __device__ __forceinline__ bool push(double x) {
+    if (d == 0) X0 = x;
+    else if (d == 1) X1 = x;
+    else if (d == 2) X2 = x;
+    else return false;
+    ++d;
     return true;
   }
-  __device__ double fold(double val) const {
-    for (int q = d - 1; q >= 0; --q) val = 2.0 * X[q] - val;
+  __device__ __forceinline__ double fold(double val) const {
+    if (d >= 3) val = 2.0 * X2 - val;
+    if (d >= 2) val = 2.0 * X1 - val;
+    if (d >= 1) val = 2.0 * X0 - val;
     return val;
   }
 };
@@ -186,8 +191,10 @@ __device__ __forceinline__ double recon(int sc, double adv, double qm2, double q
 }
 
 // Stencil readers: Fast for stencils wholly inside the arrays, Ghost otherwise.
+// GhostRd reads the face tables through a device-memory copy of the grid: the
+// ghost resolvers index them with run-time face numbers, which on a by-value
+// kernel argument would force a private (scratch) copy of the whole struct.
 struct FastRd {
-  const DevGrid* g;
   const double *u, *v, *w;
   Idx I;
   __device__ double U(int i, int j, int k) const { return u[I.u(i, j, k)]; }
@@ -327,36 +334,38 @@ __global__ void __launch_bounds__(kBlock) k_rhs_rk(NsArgs A, const double* __res
                                                    const double* __restrict__ v,
                                                    const double* __restrict__ w, double* __restrict__ uo,
                                                    double* __restrict__ vo, double* __restrict__ wo,
-                                                   int stage, const StepCtl* __restrict__ sc) {
+                                                   int stage, const StepCtl* __restrict__ sc,
+                                                   const DevGrid* __restrict__ gd) {
   const DevGrid& g = A.g;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const long long nu = (long long)(nx + 1) * ny * nz, nv = (long long)nx * (ny + 1) * nz;
-  const long long nw = (long long)nx * ny * (nz + 1);
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned nu = unsigned(nx + 1) * ny * nz, nv = unsigned(nx) * (ny + 1) * nz;
+  const unsigned nw = unsigned(nx) * ny * (nz + 1);
+  unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nu + nv + nw) return;
   const double dt = sc->dt;
   int comp;
   int i, j, k;
   if (t < nu) {
     comp = 0;
-    i = int(t % (nx + 1));
-    long long q = t / (nx + 1);
-    j = int(q % ny);
-    k = int(q / ny);
+    const unsigned rx = unsigned(nx + 1);
+    i = int(t % rx);
+    const unsigned q = t / rx;
+    j = int(q % unsigned(ny));
+    k = int(q / unsigned(ny));
   } else if (t < nu + nv) {
     comp = 1;
     t -= nu;
-    i = int(t % nx);
-    long long q = t / nx;
-    j = int(q % (ny + 1));
-    k = int(q / (ny + 1));
+    i = int(t % unsigned(nx));
+    const unsigned q = t / unsigned(nx);
+    j = int(q % unsigned(ny + 1));
+    k = int(q / unsigned(ny + 1));
   } else {
     comp = 2;
     t -= nu + nv;
-    i = int(t % nx);
-    long long q = t / nx;
-    j = int(q % ny);
-    k = int(q / ny);
+    i = int(t % unsigned(nx));
+    const unsigned q = t / unsigned(nx);
+    j = int(q % unsigned(ny));
+    k = int(q / unsigned(ny));
   }
   double R = 0.0;
   bool active;
@@ -366,10 +375,10 @@ __global__ void __launch_bounds__(kBlock) k_rhs_rk(NsArgs A, const double* __res
   if (active) {
     const Idx I{nx, ny, nz};
     if (interior(g, comp, i, j, k)) {
-      FastRd r{&g, u, v, w, I};
+      FastRd r{u, v, w, I};
       R = comp == 0 ? rhsU(A, r, i, j, k) : (comp == 1 ? rhsV(A, r, i, j, k) : rhsW(A, r, i, j, k));
     } else {
-      GhostRd r{&g, u, v, w, I};
+      GhostRd r{gd, u, v, w, I};
       R = comp == 0 ? rhsU(A, r, i, j, k) : (comp == 1 ? rhsV(A, r, i, j, k) : rhsW(A, r, i, j, k));
     }
   }
@@ -445,12 +454,12 @@ __global__ void __launch_bounds__(kBlock) k_divergence(NsArgs A, const double* _
                                                        double* __restrict__ b) {
   const DevGrid& g = A.g;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const long long n = (long long)nx * ny * nz;
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned n = unsigned(nx) * ny * nz;
+  const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  const int i = int(c % nx);
-  const long long q = c / nx;
-  const int j = int(q % ny), k = int(q / ny);
+  const int i = int(c % unsigned(nx));
+  const unsigned q = c / unsigned(nx);
+  const int j = int(q % unsigned(ny)), k = int(q / unsigned(ny));
   if (g.hasSolid && g.solid[c]) {
     b[c] = 0.0;
     return;
@@ -477,7 +486,7 @@ __device__ __forceinline__ double phiAt(const DevGrid& g, const double* __restri
     q[ax] = q[ax] < 0 ? 0 : n[ax] - 1;
     s *= g.bc[f] == NBL_BC_OUTLET ? -1.0 : 1.0;
   }
-  return s * phi[q[0] + (long long)g.nx * (q[1] + (long long)g.ny * q[2])];
+  return s * phi[q[0] + g.nx * (q[1] + g.ny * q[2])];
 }
 
 // Face-flux correction u -= grad(phi) on non-solid faces, then the
@@ -488,14 +497,14 @@ __global__ void __launch_bounds__(kBlock) k_correct(NsArgs A, double* __restrict
                                                     double* __restrict__ p, const StepCtl* __restrict__ sc) {
   const DevGrid& g = A.g;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const long long nu = (long long)(nx + 1) * ny * nz, nv = (long long)nx * (ny + 1) * nz;
-  const long long nw = nz > 1 ? (long long)nx * ny * (nz + 1) : 0;
-  const long long nc = (long long)nx * ny * nz;
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned nu = unsigned(nx + 1) * ny * nz, nv = unsigned(nx) * (ny + 1) * nz;
+  const unsigned nw = nz > 1 ? unsigned(nx) * ny * (nz + 1) : 0;
+  const unsigned nc = unsigned(nx) * ny * nz;
+  unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < nu) {
-    const int i = int(t % (nx + 1));
-    const long long q = t / (nx + 1);
-    const int j = int(q % ny), k = int(q / ny);
+    const int i = int(t % unsigned(nx + 1));
+    const unsigned q = t / unsigned(nx + 1);
+    const int j = int(q % unsigned(ny)), k = int(q / unsigned(ny));
     if (solidAt(g, i - 1, j, k) || solidAt(g, i, j, k)) {
       u[t] = 0.0;
       return;
@@ -511,9 +520,9 @@ __global__ void __launch_bounds__(kBlock) k_correct(NsArgs A, double* __restrict
   }
   t -= nu;
   if (t < nv) {
-    const int i = int(t % nx);
-    const long long q = t / nx;
-    const int j = int(q % (ny + 1)), k = int(q / (ny + 1));
+    const int i = int(t % unsigned(nx));
+    const unsigned q = t / unsigned(nx);
+    const int j = int(q % unsigned(ny + 1)), k = int(q / unsigned(ny + 1));
     if (solidAt(g, i, j - 1, k) || solidAt(g, i, j, k)) {
       v[t] = 0.0;
       return;
@@ -529,9 +538,9 @@ __global__ void __launch_bounds__(kBlock) k_correct(NsArgs A, double* __restrict
   }
   t -= nv;
   if (t < nw) {
-    const int i = int(t % nx);
-    const long long q = t / nx;
-    const int j = int(q % ny), k = int(q / ny);
+    const int i = int(t % unsigned(nx));
+    const unsigned q = t / unsigned(nx);
+    const int j = int(q % unsigned(ny)), k = int(q / unsigned(ny));
     if (solidAt(g, i, j, k - 1) || solidAt(g, i, j, k)) {
       w[t] = 0.0;
       return;
@@ -549,35 +558,47 @@ __global__ void __launch_bounds__(kBlock) k_correct(NsArgs A, double* __restrict
   if (t < nc) p[t] = phi[t] / sc->dt;
 }
 
-// CFL time step (ns_solver.cpp:581-603): max over fluid cells, then dt.
+// CFL time step (ns_solver.cpp:581-603): per-block max over fluid cells, then
+// k_cfl_fin folds the partials and sets dt.
+constexpr int kNsBlocks = 4096;  // cap on blocks of the NS reductions (grid-stride)
+
+__device__ __forceinline__ double viscTerm(const NsArgs& A) {
+  const DevGrid& g = A.g;
+  const bool flat = g.nz == 1;
+  return 2.0 * A.nu * (1.0 / (g.dx * g.dx) + 1.0 / (g.dy * g.dy) + (flat ? 0.0 : 1.0 / (g.dz * g.dz)));
+}
+
 __global__ void __launch_bounds__(kBlock) k_cfl(NsArgs A, const double* __restrict__ u,
                                                 const double* __restrict__ v, const double* __restrict__ w,
-                                                StepCtl* sc, RedBuf rb) {
+                                                RedBuf rb) {
   const DevGrid& g = A.g;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const bool flat = nz == 1;
-  const long long n = (long long)nx * ny * nz;
+  const unsigned n = unsigned(nx) * ny * nz;
   const Idx I{nx, ny, nz};
-  const double visc = 2.0 * A.nu * (1.0 / (g.dx * g.dx) + 1.0 / (g.dy * g.dy) + (flat ? 0.0 : 1.0 / (g.dz * g.dz)));
+  const double visc = viscTerm(A);
   double m[1] = {0.0};
-  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+  for (unsigned c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
     if (g.hasSolid && g.solid[c]) continue;
-    const int i = int(c % nx);
-    const long long q = c / nx;
-    const int j = int(q % ny), k = int(q / ny);
+    const int i = int(c % unsigned(nx));
+    const unsigned q = c / unsigned(nx);
+    const int j = int(q % unsigned(ny)), k = int(q / unsigned(ny));
     const double a = fabs(0.5 * (u[I.u(i, j, k)] + u[I.u(i + 1, j, k)]));
     const double b = fabs(0.5 * (v[I.v(i, j, k)] + v[I.v(i, j + 1, k)]));
     const double cc = flat ? 0.0 : fabs(0.5 * (w[I.w(i, j, k)] + w[I.w(i, j, k + 1)]));
-    const double s = a / g.dx + b / g.dy + (flat ? 0.0 : cc / g.dz) + visc;
-    m[0] = fmax(m[0], s);
+    const double sv = a / g.dx + b / g.dy + (flat ? 0.0 : cc / g.dz) + visc;
+    m[0] = fmax(m[0], sv);
   }
-  double tot[1];
-  if (gridFold<1, 1u>(m, rb, tot) && threadIdx.x == 0) {
-    const double base = visc + 1e-300;
-    const double mx = fmax(base, tot[0]);
-    sc->cfl_max = mx;
-    sc->dt = A.cfl / mx;
-  }
+  blockToPartials<1, 1u>(m, rb.partials, rb.stride, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kFinThreads) k_cfl_fin(NsArgs A, RedBuf rb, int nparts, StepCtl* sc) {
+  double t[1];
+  foldPartials<1, 1u>(rb.partials, rb.stride, nparts, t);
+  if (threadIdx.x != 0) return;
+  const double mx = fmax(viscTerm(A) + 1e-300, t[0]);
+  sc->cfl_max = mx;
+  sc->dt = A.cfl / mx;
 }
 
 __global__ void k_set_dt(StepCtl* sc, double dt) { sc->dt = dt; }
@@ -599,40 +620,40 @@ __global__ void __launch_bounds__(kBlock) k_diagnostics(NsArgs A, const double* 
   const DevGrid& g = A.g;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const Idx I{nx, ny, nz};
-  const long long nu = (long long)(nx + 1) * ny * nz, nv = (long long)nx * (ny + 1) * nz;
-  const long long nw = (long long)nx * ny * (nz + 1), nc = (long long)nx * ny * nz;
-  const long long ntot = nu + nv + nw + nc;
+  const unsigned nu = unsigned(nx + 1) * ny * nz, nv = unsigned(nx) * (ny + 1) * nz;
+  const unsigned nw = unsigned(nx) * ny * (nz + 1), nc = unsigned(nx) * ny * nz;
+  const unsigned ntot = nu + nv + nw + nc;
   const double ax = g.dy * g.dz, ay = g.dx * g.dz, az = g.dx * g.dy;
   const double mu = A.nu * A.density;
   // acc: 0 du^2, 1 div^2, 2 fluid count, 3 mass net, 4 fx, 5 fy, 6 max|div|
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ntot; t += (long long)gridDim.x * blockDim.x) {
+  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < ntot; t += gridDim.x * blockDim.x) {
     if (t < nu) {
       const double d = u[t] - u0[t];
       acc[0] += d * d;
-      const int i = int(t % (nx + 1));
+      const int i = int(t % unsigned(nx + 1));
       if (i == 0) acc[3] += (u[t] - u[t + nx]) * ax;
       continue;
     }
-    long long q = t - nu;
+    unsigned q = t - nu;
     if (q < nv) {
       const double d = v[q] - v0[q];
       acc[0] += d * d;
-      const int j = int((q / nx) % (ny + 1));
-      if (j == 0) acc[3] += (v[q] - v[q + (long long)nx * ny]) * ay;
+      const int j = int((q / unsigned(nx)) % unsigned(ny + 1));
+      if (j == 0) acc[3] += (v[q] - v[q + unsigned(nx) * ny]) * ay;
       continue;
     }
     q -= nv;
     if (q < nw) {
       const double d = w[q] - w0[q];
       acc[0] += d * d;
-      if (nz > 1 && q < (long long)nx * ny) acc[3] += (w[q] - w[q + (long long)nx * ny * nz]) * az;
+      if (nz > 1 && q < unsigned(nx) * ny) acc[3] += (w[q] - w[q + unsigned(nx) * ny * nz]) * az;
       continue;
     }
     q -= nw;
-    const int i = int(q % nx);
-    const long long r = q / nx;
-    const int j = int(r % ny), k = int(r / ny);
+    const int i = int(q % unsigned(nx));
+    const unsigned r = q / unsigned(nx);
+    const int j = int(r % unsigned(ny)), k = int(r / unsigned(ny));
     if (solidAt(g, i, j, k)) continue;
     double dv = (u[I.u(i + 1, j, k)] - u[I.u(i, j, k)]) / g.dx + (v[I.v(i, j + 1, k)] - v[I.v(i, j, k)]) / g.dy;
     if (nz > 1) dv += (w[I.w(i, j, k + 1)] - w[I.w(i, j, k)]) / g.dz;
@@ -660,10 +681,21 @@ __global__ void __launch_bounds__(kBlock) k_diagnostics(NsArgs A, const double* 
       }
     }
   }
+  blockToPartials<7, 1u << 6>(acc, rb.partials, rb.stride, blockIdx.x);
+}
+
+// Step report from the folded diagnostics (ns_solver.cpp:797-823).
+__global__ void __launch_bounds__(kFinThreads) k_diag_fin(NsArgs A, RedBuf rb, int nparts, StepCtl* sc,
+                                                          DevReport* out) {
+  const DevGrid& g = A.g;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const double ay = g.dx * g.dz;
+  const double nu = double(nx + 1) * ny * nz, nv = double(nx) * (ny + 1) * nz, nw = double(nx) * ny * (nz + 1);
   double tot[7];
-  if (!gridFold<7, 1u << 6>(acc, rb, tot) || threadIdx.x != 0) return;
+  foldPartials<7, 1u << 6>(rb.partials, rb.stride, nparts, tot);
+  if (threadIdx.x != 0) return;
   const double dt = sc->dt;
-  const double n = double(nu + nv + nw);
+  const double n = nu + nv + nw;
   const double mom = sqrt(tot[0] / n) / (dt * A.velocityScale + 1e-300);
   const double divl2 = tot[2] > 0.0 ? sqrt(tot[1] / tot[2]) : 0.0;
   const double mass = fabs(tot[3]) / (A.velocityScale * (ny * ay) + 1e-300);
@@ -687,7 +719,7 @@ __global__ void __launch_bounds__(kBlock) k_diagnostics(NsArgs A, const double* 
   out->mass = mass;
   out->cd = cd;
   out->cl = cl;
-  out->cells = (unsigned long long)nc;
+  out->cells = (unsigned long long)nx * ny * nz;
   out->accepted = (divl2 < A.tolContinuity && mass < A.tolMass) ? 1 : 0;
 }
 
@@ -696,13 +728,13 @@ __global__ void k_cell_velocity(NsArgs A, const double* __restrict__ u, const do
                                 const double* __restrict__ w, int comp, double* __restrict__ out) {
   const DevGrid& g = A.g;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const long long n = (long long)nx * ny * nz;
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned n = unsigned(nx) * ny * nz;
+  const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const Idx I{nx, ny, nz};
-  const int i = int(c % nx);
-  const long long q = c / nx;
-  const int j = int(q % ny), k = int(q / ny);
+  const int i = int(c % unsigned(nx));
+  const unsigned q = c / unsigned(nx);
+  const int j = int(q % unsigned(ny)), k = int(q / unsigned(ny));
   double r;
   if (comp == 0) r = 0.5 * (u[I.u(i, j, k)] + u[I.u(i + 1, j, k)]);
   else if (comp == 1) r = 0.5 * (v[I.v(i, j, k)] + v[I.v(i, j + 1, k)]);
@@ -741,8 +773,8 @@ void launch_bc_solid(const NsArgs& a, double* u, double* v, double* w, const int
 
 void launch_rhs_rk(const NsArgs& a, const double* u0, const double* v0, const double* w0,
                    const double* u, const double* v, const double* w, double* uo, double* vo,
-                   double* wo, int stage, const StepCtl* sc, hipStream_t s) {
-  k_rhs_rk<<<gridFor(faces(a.g)), kBlock, 0, s>>>(a, u0, v0, w0, u, v, w, uo, vo, wo, stage, sc);
+                   double* wo, int stage, const StepCtl* sc, const DevGrid* gd, hipStream_t s) {
+  k_rhs_rk<<<gridFor(faces(a.g)), kBlock, 0, s>>>(a, u0, v0, w0, u, v, w, uo, vo, wo, stage, sc, gd);
   NBL_LAUNCH_CHECK();
 }
 
@@ -760,13 +792,20 @@ void launch_correct(const NsArgs& a, double* u, double* v, double* w, const doub
   NBL_LAUNCH_CHECK();
 }
 
+int ns_partials(const DevGrid& g) {
+  (void)g;
+  return kNsBlocks;
+}
+
 void launch_cfl(const NsArgs& a, const double* u, const double* v, const double* w, StepCtl* sc,
                 double fixed_dt, RedBuf rb, hipStream_t s) {
   if (fixed_dt > 0.0) {
     k_set_dt<<<1, 1, 0, s>>>(sc, fixed_dt);
   } else {
     const long long n = (long long)a.g.nx * a.g.ny * a.g.nz;
-    k_cfl<<<gridFor(n, kBlock, kMaxRedBlocks), kBlock, 0, s>>>(a, u, v, w, sc, rb);
+    const int g = gridFor(n, kBlock, kNsBlocks);
+    k_cfl<<<g, kBlock, 0, s>>>(a, u, v, w, rb);
+    k_cfl_fin<<<1, kFinThreads, 0, s>>>(a, rb, g, sc);
   }
   NBL_LAUNCH_CHECK();
 }
@@ -775,8 +814,9 @@ void launch_diagnostics(const NsArgs& a, const double* u, const double* v, const
                         const double* u0, const double* v0, const double* w0, const double* p,
                         StepCtl* sc, DevReport* out, RedBuf rb, hipStream_t s) {
   const long long n = faces(a.g) + (long long)a.g.nx * a.g.ny * a.g.nz;
-  k_diagnostics<<<gridFor(n, kBlock, kMaxRedBlocks), kBlock, 0, s>>>(a, u, v, w, u0, v0, w0, p, sc,
-                                                                      out, rb);
+  const int g = gridFor(n, kBlock, kNsBlocks);
+  k_diagnostics<<<g, kBlock, 0, s>>>(a, u, v, w, u0, v0, w0, p, sc, out, rb);
+  k_diag_fin<<<1, kFinThreads, 0, s>>>(a, rb, g, sc, out);
   NBL_LAUNCH_CHECK();
 }
 

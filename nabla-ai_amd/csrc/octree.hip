@@ -235,14 +235,16 @@ __global__ void k_validate(long long n, int maxL, const uint64_t* __restrict__ k
 }
 
 // ---- Jacobi-PCG on the leaves (octree_poisson.cpp:107-197) ------------------
+// Every reducing kernel writes per-block partials (grid-stride over at most
+// kOctBlocks blocks) and is followed by the shared finalize kernel (k_fin).
 enum : int { O_MEAN_RHS = 0, O_MEAN_R = 1, O_MEAN_Z = 2, O_MEAN_X = 3 };
+constexpr int kOctBlocks = 4096;
 
-#define OGRID(c, n) \
-  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < (n); c += (long long)gridDim.x * blockDim.x)
+#define OGRID(c, n) for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < (n); c += (long long)gridDim.x * blockDim.x)
 
-__global__ void __launch_bounds__(kBlock) k_o_init(long long n, long long nf, const uint8_t* __restrict__ mask,
+__global__ void __launch_bounds__(kBlock) k_o_init(long long n, const uint8_t* __restrict__ mask,
                                                    const double* __restrict__ b, double* __restrict__ rhs,
-                                                   double* __restrict__ x, SolveCtl* ctl, RedBuf rb) {
+                                                   double* __restrict__ x, RedBuf rb) {
   double acc[1] = {0.0};
   OGRID(c, n) {
     const double v = mask[c] == 0 ? b[c] : 0.0;
@@ -250,16 +252,15 @@ __global__ void __launch_bounds__(kBlock) k_o_init(long long n, long long nf, co
     x[c] = 0.0;
     acc[0] += v;
   }
-  double tot[1];
-  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) ctl->slot[O_MEAN_RHS] = nf ? tot[0] / double(nf) : 0.0;
+  blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
 }
 
-// rhs <- rhs - mean (fluid), 0 elsewhere; r = rhs; z' = r/diag (fluid); fold
-// |rhs|^2 and sum(z').
+// rhs <- rhs - mean (fluid), 0 elsewhere; r = rhs; z' = r/diag (fluid);
+// partials of |rhs|^2 and sum(z').
 __global__ void __launch_bounds__(kBlock) k_o_init2(long long n, long long nf, const uint8_t* __restrict__ mask,
                                                     const double* __restrict__ diag, double* __restrict__ rhs,
-                                                    double* __restrict__ r, double* __restrict__ z, SolveCtl* ctl,
-                                                    RedBuf rb) {
+                                                    double* __restrict__ r, double* __restrict__ z,
+                                                    const SolveCtl* ctl, RedBuf rb) {
   const double m = ctl->slot[O_MEAN_RHS];
   double acc[2] = {0.0, 0.0};
   OGRID(c, n) {
@@ -272,22 +273,13 @@ __global__ void __launch_bounds__(kBlock) k_o_init2(long long n, long long nf, c
     acc[0] += v * v;
     if (fl) acc[1] += zv;
   }
-  double tot[2];
-  if (gridFold<2>(acc, rb, tot) && threadIdx.x == 0) {
-    ctl->bnorm = sqrt(tot[0]) + 1e-300;
-    ctl->slot[O_MEAN_Z] = nf ? tot[1] / double(nf) : 0.0;
-    ctl->it = 0;
-    ctl->done = 0;
-    ctl->converged = 0;
-    ctl->broke = 0;
-  }
+  blockToPartials<2>(acc, rb.partials, rb.stride, blockIdx.x);
 }
 
-// z <- z' - mean (fluid), 0 elsewhere; fold r.z.  first: p = z and rz = r.z;
-// otherwise beta = (r.z)/rz.
+// z <- z' - mean (fluid), 0 elsewhere; partials of r.z (first: also p = z).
 __global__ void __launch_bounds__(kBlock) k_o_zdot(long long n, long long nf, const uint8_t* __restrict__ mask,
                                                    const double* __restrict__ r, double* __restrict__ z,
-                                                   double* __restrict__ p, int first, SolveCtl* ctl, RedBuf rb) {
+                                                   double* __restrict__ p, int first, const SolveCtl* ctl, RedBuf rb) {
   if (ctl->done) return;
   const double m = ctl->slot[O_MEAN_Z];
   double acc[1] = {0.0};
@@ -297,22 +289,13 @@ __global__ void __launch_bounds__(kBlock) k_o_zdot(long long n, long long nf, co
     if (first) p[c] = zv;
     acc[0] += r[c] * zv;
   }
-  double tot[1];
-  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) {
-    if (first) {
-      ctl->rz = tot[0];
-    } else {
-      ctl->beta = tot[0] / ctl->rz;
-      ctl->rz = tot[0];
-      ctl->it += 1;
-    }
-  }
+  blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
 }
 
 __global__ void __launch_bounds__(kBlock) k_o_apply_dot(long long n, const uint8_t* __restrict__ mask,
                                                         const int32_t* __restrict__ rowPtr, const int32_t* __restrict__ col,
                                                         const double* __restrict__ coef, const double* __restrict__ p,
-                                                        double* __restrict__ Ap, SolveCtl* ctl, RedBuf rb) {
+                                                        double* __restrict__ Ap, const SolveCtl* ctl, RedBuf rb) {
   if (ctl->done) return;
   double acc[1] = {0.0};
   OGRID(c, n) {
@@ -320,22 +303,14 @@ __global__ void __launch_bounds__(kBlock) k_o_apply_dot(long long n, const uint8
     Ap[c] = v;
     acc[0] += p[c] * v;
   }
-  double tot[1];
-  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) {
-    if (fabs(tot[0]) < 1e-300) {
-      ctl->done = 1;
-      ctl->broke = 1;
-    } else {
-      ctl->alpha = ctl->rz / tot[0];
-    }
-  }
+  blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
 }
 
-// x += alpha p; r' = r - alpha Ap; fold sum over fluid of r'.
-__global__ void __launch_bounds__(kBlock) k_o_update(long long n, long long nf, const uint8_t* __restrict__ mask,
+// x += alpha p; r' = r - alpha Ap; partial fluid sums of r'.
+__global__ void __launch_bounds__(kBlock) k_o_update(long long n, const uint8_t* __restrict__ mask,
                                                      double* __restrict__ x, const double* __restrict__ p,
                                                      double* __restrict__ r, const double* __restrict__ Ap,
-                                                     SolveCtl* ctl, RedBuf rb) {
+                                                     const SolveCtl* ctl, RedBuf rb) {
   if (ctl->done) return;
   const double a = ctl->alpha;
   double acc[1] = {0.0};
@@ -345,15 +320,14 @@ __global__ void __launch_bounds__(kBlock) k_o_update(long long n, long long nf, 
     r[c] = v;
     if (mask[c] == 0) acc[0] += v;
   }
-  double tot[1];
-  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) ctl->slot[O_MEAN_R] = nf ? tot[0] / double(nf) : 0.0;
+  blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
 }
 
-// r <- r' - mean (fluid), 0 elsewhere; convergence test on |r|; z' = r/diag
-// (fluid) and its fluid sum.
+// r <- r' - mean (fluid), 0 elsewhere; z' = r/diag (fluid); partials of |r|^2
+// and sum(z').
 __global__ void __launch_bounds__(kBlock) k_o_rnorm(long long n, long long nf, const uint8_t* __restrict__ mask,
                                                     const double* __restrict__ diag, double* __restrict__ r,
-                                                    double* __restrict__ z, SolveCtl* ctl, RedBuf rb) {
+                                                    double* __restrict__ z, const SolveCtl* ctl, RedBuf rb) {
   if (ctl->done) return;
   const double m = ctl->slot[O_MEAN_R];
   double acc[2] = {0.0, 0.0};
@@ -366,38 +340,29 @@ __global__ void __launch_bounds__(kBlock) k_o_rnorm(long long n, long long nf, c
     acc[0] += v * v;
     if (fl) acc[1] += zv;
   }
-  double tot[2];
-  if (gridFold<2>(acc, rb, tot) && threadIdx.x == 0) {
-    ctl->slot[O_MEAN_Z] = nf ? tot[1] / double(nf) : 0.0;
-    if (sqrt(tot[0]) <= ctl->tol * ctl->bnorm) {
-      ctl->it += 1;
-      ctl->converged = 1;
-      ctl->done = 1;
-    }
-  }
+  blockToPartials<2>(acc, rb.partials, rb.stride, blockIdx.x);
 }
 
 __global__ void __launch_bounds__(kBlock) k_o_pupdate(long long n, double* __restrict__ p, const double* __restrict__ z,
-                                                      SolveCtl* ctl) {
+                                                      const SolveCtl* ctl) {
   if (ctl->done) return;
   const double be = ctl->beta;
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c < n) p[c] = z[c] + be * p[c];
 }
 
-__global__ void __launch_bounds__(kBlock) k_o_xsum(long long n, long long nf, const uint8_t* __restrict__ mask,
-                                                   const double* __restrict__ x, SolveCtl* ctl, RedBuf rb) {
+__global__ void __launch_bounds__(kBlock) k_o_xsum(long long n, const uint8_t* __restrict__ mask,
+                                                   const double* __restrict__ x, RedBuf rb) {
   double acc[1] = {0.0};
   OGRID(c, n) if (mask[c] == 0) acc[0] += x[c];
-  double tot[1];
-  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) ctl->slot[O_MEAN_X] = nf ? tot[0] / double(nf) : 0.0;
+  blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
 }
 
 __global__ void __launch_bounds__(kBlock) k_o_final(long long n, long long nf, const uint8_t* __restrict__ mask,
                                                     const int32_t* __restrict__ rowPtr, const int32_t* __restrict__ col,
                                                     const double* __restrict__ coef, const double* __restrict__ x,
                                                     const double* __restrict__ rhs, double* __restrict__ xo,
-                                                    SolveCtl* ctl, RedBuf rb) {
+                                                    const SolveCtl* ctl, RedBuf rb) {
   const double m = ctl->slot[O_MEAN_X];
   const bool hasM = nf > 0;
   double acc[1] = {0.0};
@@ -408,14 +373,10 @@ __global__ void __launch_bounds__(kBlock) k_o_final(long long n, long long nf, c
     const double e = rhs[c] - Ax;
     acc[0] += e * e;
   }
-  double tot[1];
-  if (gridFold<1>(acc, rb, tot) && threadIdx.x == 0) {
-    ctl->resid = sqrt(tot[0]) / ctl->bnorm;
-    ctl->converged = (ctl->converged || ctl->resid <= 1e-6) ? 1 : 0;
-  }
+  blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
 }
 
-int ogrid(long long n) { return gridFor(n, kBlock, kMaxRedBlocks); }
+int ogrid(long long n) { return gridFor(n, kBlock, kOctBlocks); }
 
 }  // namespace
 
@@ -508,9 +469,7 @@ OctreeDev::OctreeDev(nbl_ctx* ctx, const uint64_t* keys, const uint8_t* levels, 
   // solver vectors
   for (auto* b : {&x_, &rhs_, &r_, &z_, &p_, &Ap_, &t0_, &t1_}) *b = DevBuf(ctx, n * 8);
   ctl_ = DevBuf(ctx, sizeof(SolveCtl));
-  red_ = DevBuf(ctx, sizeof(double) * kMaxRedVals * kMaxRedBlocks);
-  ticket_ = DevBuf(ctx, 256);
-  NBL_HIP(hipMemsetAsync(ticket_.as<void>(), 0, 256, s));
+  red_ = DevBuf(ctx, sizeof(double) * kMaxRedVals * kOctBlocks);
   NBL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hctl_), 2 * sizeof(SolveCtl), hipHostMallocDefault));
   NBL_HIP(hipStreamSynchronize(s));
 }
@@ -561,7 +520,7 @@ void OctreeDev::diagonal(double* out) { fromSorted(diag_.as<double>(), out); }
 
 nbl_poisson_report OctreeDev::solveSorted(const double* b, double* xo, double tol, int maxIters, hipStream_t s) {
   SolveCtl* ctl = ctl_.as<SolveCtl>();
-  const RedBuf rb{red_.as<double>(), ticket_.as<unsigned>()};
+  const RedBuf rb{red_.as<double>(), kOctBlocks};
   const long long N = (long long)n_, nf = (long long)nf_;
   const int g = ogrid(N);
   hctl_[1] = SolveCtl{};
@@ -574,18 +533,36 @@ nbl_poisson_report OctreeDev::solveSorted(const double* b, double* xo, double to
   const double* dg = diag_.as<double>();
   double *x = x_.as<double>(), *rhs = rhs_.as<double>(), *r = r_.as<double>(), *z = z_.as<double>();
   double *p = p_.as<double>(), *Ap = Ap_.as<double>();
-  k_o_init<<<g, kBlock, 0, s>>>(N, nf, mk, b, rhs, x, ctl, rb);
+  auto fin = [&](int op, int K, int slot, bool pred) {
+    FinArgs a{};
+    a.op = op;
+    a.K = K;
+    a.nparts = g;
+    a.slot = slot;
+    a.nf = nf;
+    a.convTol = 1e-6;
+    a.predicated = pred ? 1 : 0;
+    launch_fin(rb, a, ctl, s);
+  };
+  k_o_init<<<g, kBlock, 0, s>>>(N, mk, b, rhs, x, rb);
+  fin(FIN_MEAN, 1, O_MEAN_RHS, false);
   k_o_init2<<<g, kBlock, 0, s>>>(N, nf, mk, dg, rhs, r, z, ctl, rb);
+  fin(FIN_BNORM_MEAN, 2, O_MEAN_Z, false);
   k_o_zdot<<<g, kBlock, 0, s>>>(N, nf, mk, r, z, p, 1, ctl, rb);
+  fin(FIN_RZ, 1, 0, true);
   NBL_LAUNCH_CHECK();
   int enq = 0;
   int batch = std::clamp(predicted_, 1, std::max(1, maxIters));
   while (enq < maxIters) {
     for (int q = 0; q < batch && enq < maxIters; ++q, ++enq) {
       k_o_apply_dot<<<g, kBlock, 0, s>>>(N, mk, rp, cl, cf, p, Ap, ctl, rb);
-      k_o_update<<<g, kBlock, 0, s>>>(N, nf, mk, x, p, r, Ap, ctl, rb);
+      fin(FIN_ALPHA, 1, 0, true);
+      k_o_update<<<g, kBlock, 0, s>>>(N, mk, x, p, r, Ap, ctl, rb);
+      fin(FIN_MEAN, 1, O_MEAN_R, true);
       k_o_rnorm<<<g, kBlock, 0, s>>>(N, nf, mk, dg, r, z, ctl, rb);
+      fin(FIN_CONV_MEAN, 2, O_MEAN_Z, true);
       k_o_zdot<<<g, kBlock, 0, s>>>(N, nf, mk, r, z, p, 0, ctl, rb);
+      fin(FIN_BETA, 1, 0, true);
       k_o_pupdate<<<gridFor(N), kBlock, 0, s>>>(N, p, z, ctl);
     }
     NBL_LAUNCH_CHECK();
@@ -594,8 +571,10 @@ nbl_poisson_report OctreeDev::solveSorted(const double* b, double* xo, double to
     if (hctl_[0].done) break;
     batch = std::max(8, batch / 4);
   }
-  k_o_xsum<<<g, kBlock, 0, s>>>(N, nf, mk, x, ctl, rb);
+  k_o_xsum<<<g, kBlock, 0, s>>>(N, mk, x, rb);
+  fin(FIN_MEAN, 1, O_MEAN_X, false);
   k_o_final<<<g, kBlock, 0, s>>>(N, nf, mk, rp, cl, cf, x, rhs, xo, ctl, rb);
+  fin(FIN_RESID, 1, 0, false);
   NBL_LAUNCH_CHECK();
   NBL_HIP(hipMemcpyAsync(hctl_, ctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, s));
   NBL_HIP(hipStreamSynchronize(s));

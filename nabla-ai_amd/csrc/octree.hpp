@@ -35,7 +35,7 @@ class OctreeDev {
   hipStream_t stream_ = nullptr;
   DevBuf keys_, perm_, lev_, mask_, cnt_, nbr_, rowPtr_, col_, coef_, diag_;
   DevBuf x_, rhs_, r_, z_, p_, Ap_, t0_, t1_;
-  DevBuf ctl_, red_, ticket_;
+  DevBuf ctl_, red_;
   SolveCtl* hctl_ = nullptr;
   int predicted_ = 16;
 };

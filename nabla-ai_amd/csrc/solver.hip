@@ -141,7 +141,8 @@ std::vector<HostLevel> buildLevels(const HostGrid& g, bool multilevel) {
 // Pressure solver
 // ---------------------------------------------------------------------------
 namespace {
-constexpr int kSlotMeanZ = 3;
+constexpr int kSlotMeanZ = 3;  // pending mean of the preconditioner output z
+constexpr int kSlotMeanRhs = 0, kSlotMeanRn = 1, kSlotMeanX = 2;
 int slotB(int l) { return 8 + 4 * l; }
 int slotX(int l, int buf) { return 8 + 4 * l + 1 + buf; }
 // Address of ctl->slot[i] computed without touching device memory.
@@ -162,8 +163,11 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
   singular_ = !hasOutlet(g);
   mixed_ = kind == 0 && opts.precision != NBL_PREC_FP64;
   hl_ = buildLevels(g, kind == 0);
+  // the null-space projections are no-ops without fluid cells
+  project_ = singular_ && hl_[0].nf > 0;
   const int nl = int(hl_.size());
   const size_t ts = mixed_ ? sizeof(float) : sizeof(double);
+  int stride = 1;
   for (int l = 0; l < nl; ++l) {
     const HostLevel& H = hl_[l];
     codes_.emplace_back(ctx, H.code.size() * sizeof(uint16_t));
@@ -183,11 +187,15 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
     m.fz = H.f[2];
     m.nf = H.nf;
     dl_.push_back(m);
+    blocks_.push_back(geo3(m.nx, m.ny, m.nz).blocks);
+    stride = std::max(stride, blocks_.back());
     const size_t n = H.g.cells();
     lb_.emplace_back(ctx, l > 0 ? n * ts : 0);
     lx0_.emplace_back(ctx, kind == 0 ? n * ts : 0);
     lx1_.emplace_back(ctx, kind == 0 ? n * ts : 0);
   }
+  if (size_t(hl_[0].g.cells()) >= (size_t(1) << 31))
+    throw std::invalid_argument("grid too large: cell indices must fit in 32 bits");
   if (kind == 0) coarseScr_ = DevBuf(ctx, 4 * hl_.back().g.cells() * ts);
   const size_t n0 = hl_[0].g.cells();
   if (mixed_) b0f_ = DevBuf(ctx, n0 * sizeof(float));
@@ -199,9 +207,8 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
   p_ = DevBuf(ctx, n0 * 8);
   Ap_ = DevBuf(ctx, n0 * 8);
   ctl_ = DevBuf(ctx, sizeof(SolveCtl));
-  red_ = DevBuf(ctx, sizeof(double) * kMaxRedVals * kMaxRedBlocks);
-  ticket_ = DevBuf(ctx, 256);
-  NBL_HIP(hipMemset(ticket_.as<void>(), 0, 256));
+  redStride_ = stride;
+  red_ = DevBuf(ctx, sizeof(double) * kMaxRedVals * size_t(stride));
   NBL_HIP(hipMemset(ctl_.as<void>(), 0, sizeof(SolveCtl)));
   NBL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hctl_), sizeof(SolveCtl) * 2, hipHostMallocDefault));
   std::memset(hctl_, 0, sizeof(SolveCtl) * 2);
@@ -216,6 +223,23 @@ PoissonDev::~PoissonDev() {
   if (e1_) (void)hipEventDestroy(e1_);
 }
 
+void PoissonDev::fin(int op, int K, int nparts, int slot, long long nf, bool predicated, hipStream_t s,
+                     int earlyExit, double convTol) {
+  FinArgs a{};
+  a.op = op;
+  a.K = K;
+  a.nparts = nparts;
+  a.slot = slot;
+  a.nf = nf;
+  a.convTol = convTol;
+  a.earlyExit = earlyExit;
+  a.predicated = predicated ? 1 : 0;
+  launch_fin(red(), a, ctl_.as<SolveCtl>(), s);
+}
+
+// One V(pre, post) cycle from level l (multigrid.cpp:185-198).  Each sweep and
+// each restriction leaves its null-space mean pending in a slot; consumers
+// subtract it on read.
 template <class T>
 void PoissonDev::vcycle(int l, const T* b, const double* mb, const T** xout, const double** mxout,
                         hipStream_t s) {
@@ -224,76 +248,95 @@ void PoissonDev::vcycle(int l, const T* b, const double* mb, const T** xout, con
   const RedBuf rb = red();
   if (l + 1 == nlevels()) {
     T* x = lx0_[l].as<T>();
-    launch_mg_coarse<T>(L, singular_, b, mb, x, coarseScr_.as<T>(), ctl, s);
+    launch_mg_coarse<T>(L, project_, b, mb, x, coarseScr_.as<T>(), ctl, s);
     *xout = x;
     *mxout = nullptr;
     return;
   }
   T* bufs[2] = {lx0_[l].as<T>(), lx1_[l].as<T>()};
-  double* ms[2] = {singular_ ? slotPtr(ctl, slotX(l, 0)) : nullptr, singular_ ? slotPtr(ctl, slotX(l, 1)) : nullptr};
+  double* ms[2] = {project_ ? slotPtr(ctl, slotX(l, 0)) : nullptr,
+                   project_ ? slotPtr(ctl, slotX(l, 1)) : nullptr};
   int cur = -1;
   for (int q = 0; q < opts_.pre_smooth; ++q) {
     const int o = cur == 0 ? 1 : 0;
-    launch_mg_sweep<T>(L, b, mb, cur < 0 ? nullptr : bufs[cur], cur < 0 ? nullptr : ms[cur], cur < 0,
-                       nullptr, nullptr, nullptr, bufs[o], ms[o], rb, ctl, s);
+    launch_mg_sweep<T>(L, b, mb, cur < 0 ? nullptr : bufs[cur], cur < 0 ? nullptr : ms[cur], cur < 0, nullptr,
+                       nullptr, nullptr, bufs[o], project_, rb, ctl, s);
+    if (project_) fin(FIN_MEAN, 1, blocks_[l], slotX(l, o), L.nf, true, s);
     cur = o;
   }
   const MgLevel& Lc = dl_[l + 1];
   T* bc = lb_[l + 1].as<T>();
-  double* mbc = singular_ ? slotPtr(ctl, slotB(l + 1)) : nullptr;
-  launch_mg_resid_restrict<T>(L, Lc, b, mb, bufs[cur], ms[cur], bc, mbc, rb, ctl, s);
+  double* mbc = project_ ? slotPtr(ctl, slotB(l + 1)) : nullptr;
+  launch_mg_resid_restrict<T>(L, Lc, b, mb, bufs[cur], ms[cur], bc, project_, rb, ctl, s);
+  if (project_) fin(FIN_MEAN, 1, blocks_[l + 1], slotB(l + 1), Lc.nf, true, s);
   const T* ec = nullptr;
   const double* mec = nullptr;
   vcycle<T>(l + 1, bc, mbc, &ec, &mec, s);
   for (int q = 0; q < opts_.post_smooth; ++q) {
     const int o = 1 - cur;
-    launch_mg_sweep<T>(L, b, mb, bufs[cur], ms[cur], false, q == 0 ? ec : nullptr, q == 0 ? mec : nullptr,
-                       &Lc, bufs[o], ms[o], rb, ctl, s);
+    launch_mg_sweep<T>(L, b, mb, bufs[cur], ms[cur], false, q == 0 ? ec : nullptr, q == 0 ? mec : nullptr, &Lc,
+                       bufs[o], project_, rb, ctl, s);
+    if (project_) fin(FIN_MEAN, 1, blocks_[l], slotX(l, o), L.nf, true, s);
     cur = o;
   }
   *xout = bufs[cur];
   *mxout = ms[cur];
 }
 
+// z = Minv r: one V-cycle from a zero guess, then the level-0 projection
+// (multigrid.cpp:283-289).  z's mean is left pending in kSlotMeanZ.
 void PoissonDev::minv(const double* r, hipStream_t s) {
   SolveCtl* ctl = ctl_.as<SolveCtl>();
-  double* mz = singular_ ? slotPtr(ctl, kSlotMeanZ) : nullptr;
+  const MgLevel& L = dl_[0];
   if (mixed_) {
-    launch_to_level0<float>(dl_[0], r, b0f_.as<float>(), ctl, s);
+    launch_to_level0<float>(L, r, b0f_.as<float>(), ctl, s);
     const float* x = nullptr;
     const double* mx = nullptr;
     vcycle<float>(0, b0f_.as<float>(), nullptr, &x, &mx, s);
-    launch_mg_finish<float>(dl_[0], x, mx, z_.as<double>(), mz, red(), ctl, s);
+    launch_mg_finish<float>(L, x, mx, z_.as<double>(), project_, red(), ctl, s);
   } else {
     const double* x = nullptr;
     const double* mx = nullptr;
     vcycle<double>(0, r, nullptr, &x, &mx, s);
-    launch_mg_finish<double>(dl_[0], x, mx, z_.as<double>(), mz, red(), ctl, s);
+    launch_mg_finish<double>(L, x, mx, z_.as<double>(), project_, red(), ctl, s);
   }
+  if (project_) fin(FIN_MEAN, 1, blocks_[0], kSlotMeanZ, L.nf, true, s);
   ++vcount_;
 }
 
+// One iteration of the flexible CG loop (multigrid.cpp:309-340) or of the
+// Jacobi CG loop (poisson.cpp:197-225); every launch is a no-op once the
+// device has set ctl->done.
 void PoissonDev::enqueueIteration(hipStream_t s) {
   SolveCtl* ctl = ctl_.as<SolveCtl>();
   const MgLevel& L = dl_[0];
   const RedBuf rb = red();
+  const int nb = blocks_[0];
   double* x = x_.as<double>();
   double* r = r_.as<double>();
   double* z = z_.as<double>();
   double* p = p_.as<double>();
   double* Ap = Ap_.as<double>();
   launch_fcg_apply(L, p, Ap, ctl, rb, s);
+  fin(FIN_ALPHA, 1, nb, 0, 0, true, s);
   if (kind_ == 0) {
     double* rn = rn_.as<double>();
-    launch_fcg_update(L, singular_, x, p, r, Ap, rn, ctl, rb, s);
-    if (singular_) launch_fcg_norm(L, rn, ctl, rb, s);
+    launch_fcg_update(L, project_, x, p, r, Ap, rn, ctl, rb, s);
+    if (project_) {
+      fin(FIN_MEAN, 1, nb, kSlotMeanRn, L.nf, true, s);
+      launch_fcg_norm(L, rn, ctl, rb, s);
+    }
+    fin(FIN_CONV, 1, nb, 0, 0, true, s);
     minv(rn, s);
-    launch_fcg_dots(L, z, singular_ ? slotPtr(ctl, kSlotMeanZ) : nullptr, rn, r, ctl, rb, s);
+    launch_fcg_dots(L, z, project_ ? slotPtr(ctl, kSlotMeanZ) : nullptr, rn, r, ctl, rb, s);
+    fin(FIN_BETA_FLEX, 2, nb, 0, 0, true, s);
     launch_fcg_pupdate(L, p, z, r, rn, ctl, s);
   } else {
     launch_cg_update(L, x, p, r, Ap, ctl, rb, s);
-    launch_cg_z(L, r, z, ctl, rb, false, p, s);
-    launch_cg_pupdate(L, p, z, ctl, s);
+    fin(FIN_CONV, 1, nb, 0, 0, true, s);
+    launch_cg_z(L, r, z, false, p, ctl, rb, s);
+    fin(FIN_BETA, 1, nb, 0, 0, true, s);
+    launch_fcg_pupdate(L, p, z, nullptr, nullptr, ctl, s);
   }
 }
 
@@ -302,11 +345,14 @@ void PoissonDev::readCtl(hipStream_t s) {
   NBL_HIP(hipStreamSynchronize(s));
 }
 
+// MgPoisson::solve (multigrid.cpp:268-355) / CgPoisson::solve
+// (poisson.cpp:136-252).  The host enqueues the iteration count the previous
+// solve needed, then polls the device flag every check_every iterations.
 nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t s) {
   SolveCtl* ctl = ctl_.as<SolveCtl>();
   const MgLevel& L = dl_[0];
   const RedBuf rb = red();
-  // fresh scalars: tolerance set, everything else cleared by the init kernels
+  const int nb = blocks_[0];
   hctl_[1] = SolveCtl{};
   hctl_[1].tol = tol_;
   NBL_HIP(hipMemcpyAsync(ctl_.as<void>(), &hctl_[1], sizeof(SolveCtl), hipMemcpyHostToDevice, s));
@@ -317,16 +363,19 @@ nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t
   double* r = r_.as<double>();
   double* z = z_.as<double>();
   double* p = p_.as<double>();
-  if (kind_ == 0) {
-    launch_fcg_init(L, singular_, true, d_b, rhs, x, ctl, rb, s);
-    launch_fcg_init2(L, singular_, true, rhs, r, ctl, rb, s);
+  const bool mg = kind_ == 0;
+  // MgPoisson zeroes the right-hand side on solid cells; CgPoisson does not
+  launch_fcg_init(L, mg, d_b, rhs, x, rb, s);
+  if (project_) fin(FIN_MEAN, 1, nb, kSlotMeanRhs, L.nf, false, s);
+  launch_fcg_init2(L, project_, rhs, r, ctl, rb, s);
+  fin(FIN_BNORM, 1, nb, 0, 0, false, s, mg ? 1 : 0);
+  if (mg) {
     minv(r, s);
-    launch_fcg_start(L, z, singular_ ? slotPtr(ctl, kSlotMeanZ) : nullptr, r, z, p, ctl, rb, s);
+    launch_fcg_start(L, z, project_ ? slotPtr(ctl, kSlotMeanZ) : nullptr, r, p, ctl, rb, s);
   } else {
-    launch_fcg_init(L, singular_, false, d_b, rhs, x, ctl, rb, s);
-    launch_fcg_init2(L, singular_, false, rhs, r, ctl, rb, s);
-    launch_cg_z(L, r, z, ctl, rb, true, p, s);
+    launch_cg_z(L, r, z, true, p, ctl, rb, s);
   }
+  fin(FIN_RZ, 1, nb, 0, 0, true, s);
   int enq = 0;
   int batch = std::clamp(predicted_, 1, std::max(1, maxIt_));
   const int step = std::max(1, opts_.check_every);
@@ -336,7 +385,12 @@ nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t
     if (hctl_[0].done) break;
     batch = step;
   }
-  launch_fcg_final(L, singular_, x, rhs, d_x, ctl, rb, kind_ == 0 ? tol_ : 1e-6, s);
+  if (project_) {
+    launch_fluid_sum(L, x, rb, s);
+    fin(FIN_MEAN, 1, nb, kSlotMeanX, L.nf, false, s);
+  }
+  launch_fcg_final(L, project_, x, rhs, d_x, ctl, rb, s);
+  fin(FIN_RESID, 1, nb, 0, 0, false, s, 0, mg ? tol_ : 1e-6);
   NBL_HIP(hipEventRecord(e1_, s));
   readCtl(s);
   float ms = 0.f;
@@ -357,9 +411,10 @@ void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double
   const double n = double(L.n);
   const double sz = mixed_ ? 4.0 : 8.0;
   SolveCtl* ctl = ctl_.as<SolveCtl>();
-  double* m0 = singular_ ? slotPtr(ctl, slotX(0, 0)) : nullptr;
-  double* m1 = singular_ ? slotPtr(ctl, slotX(0, 1)) : nullptr;
-  // a live solve leaves ctl->done set; clear it so the kernels do their work
+  double* m0 = project_ ? slotPtr(ctl, slotX(0, 0)) : nullptr;
+  double* m1 = project_ ? slotPtr(ctl, slotX(0, 1)) : nullptr;
+  // a live solve leaves ctl->done set; clear it (and every pending mean) so the
+  // kernels do their work
   hctl_[1] = SolveCtl{};
   NBL_HIP(hipMemcpyAsync(ctl, &hctl_[1], sizeof(SolveCtl), hipMemcpyHostToDevice, s));
   NBL_HIP(hipMemsetAsync(lx0_[0].as<void>(), 0, L.n * size_t(sz), s));
@@ -367,31 +422,26 @@ void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double
   NBL_HIP(hipMemsetAsync(r_.as<void>(), 0, L.n * 8, s));
   NBL_HIP(hipMemsetAsync(p_.as<void>(), 0, L.n * 8, s));
   if (mixed_) NBL_HIP(hipMemsetAsync(b0f_.as<void>(), 0, L.n * 4, s));
-  if (singular_) {
-    NBL_HIP(hipMemsetAsync(m0, 0, 8, s));
-    NBL_HIP(hipMemsetAsync(m1, 0, 8, s));
-  }
+  const RedBuf rb = red();
   auto one = [&](int q) {
     if (kernel == 0) {
       if (mixed_)
-        launch_mg_sweep<float>(L, b0f_.as<float>(), nullptr, lx0_[0].as<float>(), m0, false, nullptr, nullptr,
-                               nullptr, lx1_[0].as<float>(), m1, red(), ctl, s);
+        launch_mg_sweep<float>(L, b0f_.as<float>(), nullptr, (q & 1) ? lx1_[0].as<float>() : lx0_[0].as<float>(),
+                               (q & 1) ? m1 : m0, false, nullptr, nullptr, nullptr,
+                               (q & 1) ? lx0_[0].as<float>() : lx1_[0].as<float>(), project_, rb, ctl, s);
       else
         launch_mg_sweep<double>(L, r_.as<double>(), nullptr, (q & 1) ? lx1_[0].as<double>() : lx0_[0].as<double>(),
                                 (q & 1) ? m1 : m0, false, nullptr, nullptr, nullptr,
-                                (q & 1) ? lx0_[0].as<double>() : lx1_[0].as<double>(), (q & 1) ? m0 : m1, red(), ctl,
-                                s);
+                                (q & 1) ? lx0_[0].as<double>() : lx1_[0].as<double>(), project_, rb, ctl, s);
     } else if (kernel == 1) {
-      launch_fcg_apply(L, p_.as<double>(), Ap_.as<double>(), ctl, red(), s);
+      launch_fcg_apply(L, p_.as<double>(), Ap_.as<double>(), ctl, rb, s);
     } else {
       if (mixed_)
         launch_mg_resid_restrict<float>(L, dl_[1], b0f_.as<float>(), nullptr, lx0_[0].as<float>(), m0,
-                                        lb_[1].as<float>(), singular_ ? slotPtr(ctl, slotB(1)) : nullptr, red(), ctl,
-                                        s);
+                                        lb_[1].as<float>(), project_, rb, ctl, s);
       else
         launch_mg_resid_restrict<double>(L, dl_[1], r_.as<double>(), nullptr, lx0_[0].as<double>(), m0,
-                                         lb_[1].as<double>(), singular_ ? slotPtr(ctl, slotB(1)) : nullptr, red(),
-                                         ctl, s);
+                                         lb_[1].as<double>(), project_, rb, ctl, s);
     }
   };
   one(0);  // warm-up
@@ -402,9 +452,10 @@ void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double
   float t = 0.f;
   NBL_HIP(hipEventElapsedTime(&t, e0_, e1_));
   *ms = double(t) / reps;
+  // algorithmic bytes: every input and output vector once, plus the 2-byte code
   if (kernel == 0) *bytes = n * (3.0 * sz + 2.0);
   else if (kernel == 1) *bytes = n * (8.0 + 8.0 + 2.0);
-  else *bytes = n * (2.0 * sz + 2.0) + double(dl_[1].n) * sz;
+  else *bytes = n * (2.0 * sz + 2.0) + double(dl_[1].n) * (sz + 2.0);
 }
 
 // ---------------------------------------------------------------------------
@@ -492,9 +543,10 @@ RunDev::RunDev(nbl_ctx* ctx, const HostGrid& g, const nbl_flow_cfg& cfg, const n
   NBL_HIP(hipMemset(p_.as<void>(), 0, sz_[3] * 8));
   sc_ = DevBuf(ctx, sizeof(StepCtl));
   rep_ = DevBuf(ctx, sizeof(DevReport));
-  red_ = DevBuf(ctx, sizeof(double) * kMaxRedVals * kMaxRedBlocks);
-  ticket_ = DevBuf(ctx, 256);
-  NBL_HIP(hipMemset(ticket_.as<void>(), 0, 256));
+  gridDev_ = DevBuf(ctx, sizeof(DevGrid));
+  NBL_HIP(hipMemcpy(gridDev_.as<void>(), &args_.g, sizeof(DevGrid), hipMemcpyHostToDevice));
+  redStride_ = ns_partials(args_.g);
+  red_ = DevBuf(ctx, sizeof(double) * kMaxRedVals * size_t(redStride_));
   NBL_HIP(hipMemset(sc_.as<void>(), 0, sizeof(StepCtl)));
   NBL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hsc_), sizeof(StepCtl), hipHostMallocDefault));
   NBL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hrep_), sizeof(DevReport), hipHostMallocDefault));
@@ -564,7 +616,7 @@ void RunDev::getState(double* u, double* v, double* w, double* p, int* step, dou
 }
 
 double RunDev::computeDt() {
-  const RedBuf rb{red_.as<double>(), ticket_.as<unsigned>()};
+  const RedBuf rb{red_.as<double>(), redStride_};
   launch_cfl(args_, cur_[0], cur_[1], cur_[2], sc_.as<StepCtl>(), 0.0, rb, stream_);
   NBL_HIP(hipMemcpyAsync(hsc_, sc_.as<void>(), sizeof(StepCtl), hipMemcpyDeviceToHost, stream_));
   NBL_HIP(hipStreamSynchronize(stream_));
@@ -583,7 +635,7 @@ void RunDev::project(int stageIdx, nbl_poisson_report& rep) {  // ns_solver.cpp:
 }
 
 nbl_step_report RunDev::step(double dt) {  // ns_solver.cpp:765-824
-  const RedBuf rb{red_.as<double>(), ticket_.as<unsigned>()};
+  const RedBuf rb{red_.as<double>(), redStride_};
   StepCtl* sc = sc_.as<StepCtl>();
   launch_cfl(args_, cur_[0], cur_[1], cur_[2], sc, dt > 0.0 ? dt : 0.0, rb, stream_);
   double* u0[3] = {buf0_[0].as<double>(), buf0_[1].as<double>(), buf0_[2].as<double>()};
@@ -593,7 +645,7 @@ nbl_step_report RunDev::step(double dt) {  // ns_solver.cpp:765-824
   nbl_poisson_report pr[3];
   for (int stage = 1; stage <= 3; ++stage) {
     launch_rhs_rk(args_, u0[0], u0[1], u0[2], cur_[0], cur_[1], cur_[2], alt_[0], alt_[1], alt_[2], stage, sc,
-                  stream_);
+                  gridDev_.as<DevGrid>(), stream_);
     std::swap(cur_, alt_);
     project(stage, pr[stage - 1]);
   }
@@ -629,7 +681,7 @@ void RunDev::rhs(double* ru, double* rv, double* rw) {
     launch_bc_solid(args_, alt_[0], alt_[1], alt_[2], solidList_.as<int>(), solidCounts_, stream_);
   double* out[3] = {buf0_[0].as<double>(), buf0_[1].as<double>(), buf0_[2].as<double>()};
   launch_rhs_rk(args_, alt_[0], alt_[1], alt_[2], alt_[0], alt_[1], alt_[2], out[0], out[1], out[2], 0,
-                sc_.as<StepCtl>(), stream_);
+                sc_.as<StepCtl>(), gridDev_.as<DevGrid>(), stream_);
   double* dst[3] = {ru, rv, rw};
   for (int q = 0; q < 3; ++q) NBL_HIP(hipMemcpyAsync(dst[q], out[q], sz_[q] * 8, hipMemcpyDeviceToHost, stream_));
   NBL_HIP(hipStreamSynchronize(stream_));

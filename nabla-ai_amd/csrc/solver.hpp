@@ -91,7 +91,9 @@ class PoissonDev {
   int kind() const { return kind_; }
   double lastMs() const { return lastMs_; }
   int lastVcycles() const { return lastV_; }
-  RedBuf red() const { return RedBuf{red_.as<double>(), ticket_.as<unsigned>()}; }
+  RedBuf red() const { return RedBuf{red_.as<double>(), redStride_}; }
+  int blocks(int l) const { return blocks_[l]; }
+  bool projects() const { return project_; }
   // Average ms per launch and algorithmic bytes of one solve kernel.
   void profile(int kernel, int reps, hipStream_t s, double* ms, double* bytes);
 
@@ -99,6 +101,8 @@ class PoissonDev {
   template <class T>
   void vcycle(int l, const T* b, const double* mb, const T** xout, const double** mxout, hipStream_t s);
   void minv(const double* r, hipStream_t s);
+  void fin(int op, int K, int nparts, int slot, long long nf, bool predicated, hipStream_t s, int earlyExit = 0,
+           double convTol = 0.0);
   void enqueueIteration(hipStream_t s);
   void readCtl(hipStream_t s);
 
@@ -108,9 +112,11 @@ class PoissonDev {
   int maxIt_;
   nbl_solver_opts opts_;
   bool singular_ = false;
+  bool project_ = false;  // singular with fluid cells: null-space projections apply
   bool mixed_ = false;
   std::vector<HostLevel> hl_;
   std::vector<MgLevel> dl_;
+  std::vector<int> blocks_;  // geo3 block count per level (= partials per reduction)
   std::vector<DevBuf> codes_;
   // per level: b (levels >= 1), two x buffers, coarse scratch
   std::vector<DevBuf> lb_, lx0_, lx1_;
@@ -118,7 +124,8 @@ class PoissonDev {
   DevBuf b0f_;  // FP32 level-0 right-hand side (mixed)
   // flexible-CG vectors (FP64)
   DevBuf x_, rhs_, r_, rn_, z_, p_, Ap_;
-  DevBuf ctl_, red_, ticket_;
+  DevBuf ctl_, red_;
+  int redStride_ = 1;
   SolveCtl* hctl_ = nullptr;  // pinned mirror
   hipEvent_t e0_ = nullptr, e1_ = nullptr;
   double lastMs_ = 0;
@@ -169,7 +176,8 @@ class RunDev {
   DevBuf bufA_[3], bufB_[3], buf0_[3], p_, b_, phi_;
   double* cur_[3];
   double* alt_[3];
-  DevBuf sc_, rep_, red_, ticket_;
+  DevBuf sc_, rep_, red_, gridDev_;
+  int redStride_ = 1;
   StepCtl* hsc_ = nullptr;
   DevReport* hrep_ = nullptr;
   std::unique_ptr<PoissonDev> poisson_;
