@@ -226,3 +226,20 @@ def c3_solid(centers: np.ndarray) -> np.ndarray:
     x0, x1, y0, y1, z0, z1 = CUBE_BOX
     inside = (x >= x0) & (x <= x1) & (y >= y0) & (y <= y1) & (z >= z0) & (z <= z1)
     return np.where(inside, 2, 0).astype(np.uint8)
+
+
+def face_stream_sha256(keys: np.ndarray, cnt: np.ndarray, nbr: np.ndarray) -> str:
+    """SHA-256 of an octree's connectivity in a layout-independent form: leaves
+    in ascending key order, each with its 6 neighbour counts and the 6x4
+    neighbour keys in face order (unused slots all ones).  `nbr` holds leaf
+    indices into `keys` (-1 unused), in any leaf order matching `keys`."""
+    import hashlib
+    keys = np.asarray(keys, np.uint64)
+    o = np.argsort(keys, kind="stable")
+    nb = np.asarray(nbr, np.int64).reshape(len(keys), 6, 4)[o]
+    nk = np.where(nb >= 0, keys[np.maximum(nb, 0)], np.uint64(0xFFFFFFFFFFFFFFFF)).astype(np.uint64)
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(keys[o]).tobytes())
+    h.update(np.ascontiguousarray(np.asarray(cnt, np.uint8).reshape(len(keys), 6)[o]).tobytes())
+    h.update(np.ascontiguousarray(nk).tobytes())
+    return h.hexdigest()

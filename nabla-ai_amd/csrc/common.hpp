@@ -53,16 +53,38 @@ struct RedBuf {
 // ---------------------------------------------------------------------------
 // Device helpers
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double waveSum(double v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
-  return v;
+// Wavefront reductions with DPP lane moves (no LDS traffic): pairs and quads
+// (quad_perm), rows of 16 (row_ror 4, 8), then row_bcast15 / row_bcast31
+// carry rows 0..2 into row 3, so lane 63 ends with the total, which readlane
+// hands to every lane.  The order is fixed, so results are reproducible.
+// Every lane of the wavefront must be active.
+template <int CTRL, int ROWMASK, bool MAX>
+__device__ __forceinline__ double dppCombine(double v) {
+  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+  const int lo = int(unsigned(b)), hi = int(unsigned(b >> 32));
+  // rows the move does not write keep `old`: 0 for a sum, the lane's own value for a max
+  const int tlo = __builtin_amdgcn_update_dpp(MAX ? lo : 0, lo, CTRL, ROWMASK, 0xF, false);
+  const int thi = __builtin_amdgcn_update_dpp(MAX ? hi : 0, hi, CTRL, ROWMASK, 0xF, false);
+  const double t =
+      __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(unsigned(thi)) << 32) | unsigned(tlo)));
+  return MAX ? fmax(v, t) : v + t;
 }
-__device__ __forceinline__ double waveMax(double v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
-  return v;
+
+template <bool MAX>
+__device__ __forceinline__ double waveReduce(double v) {
+  v = dppCombine<0xB1, 0xF, MAX>(v);   // quad_perm [1,0,3,2]
+  v = dppCombine<0x4E, 0xF, MAX>(v);   // quad_perm [2,3,0,1]
+  v = dppCombine<0x124, 0xF, MAX>(v);  // row_ror:4
+  v = dppCombine<0x128, 0xF, MAX>(v);  // row_ror:8
+  v = dppCombine<0x142, 0xA, MAX>(v);  // row_bcast:15 into rows 1, 3
+  v = dppCombine<0x143, 0xC, MAX>(v);  // row_bcast:31 into rows 2, 3
+  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+  const int lo = __builtin_amdgcn_readlane(int(unsigned(b)), kWave - 1);
+  const int hi = __builtin_amdgcn_readlane(int(unsigned(b >> 32)), kWave - 1);
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(unsigned(hi)) << 32) | unsigned(lo)));
 }
+__device__ __forceinline__ double waveSum(double v) { return waveReduce<false>(v); }
+__device__ __forceinline__ double waveMax(double v) { return waveReduce<true>(v); }
 
 // Fold K values per thread over the block (fixed order); result valid in
 // thread 0.  Bit q of MAXMASK selects max instead of sum for value q.

@@ -111,12 +111,26 @@ struct NsArgs {
   double cfl, velocityScale, density, refLength, refArea, tolContinuity, tolMass;
 };
 
+// Padded copies of u, v, w used by the RHS stencils: kPadHalo ghost layers on
+// every side.  Element (i,j,k) of component c sits at
+// p[c][origin[c] + i + sx[c]*j + sxy[c]*k]; n[c] elements are allocated.
+constexpr int kPadHalo = 3;
+struct PadBufs {
+  double* p[3];
+  long long origin[3];
+  int sx[3], sxy[3];
+  long long n[3];
+};
+PadBufs padLayout(const DevGrid& g);  // sizes and strides; the caller sets p[]
+
 void launch_bc_planes(const NsArgs& a, double* u, double* v, double* w, bool outflow, hipStream_t s);
 void launch_bc_solid(const NsArgs& a, double* u, double* v, double* w, const int* list,
                      const int counts[3], hipStream_t s);
-void launch_rhs_rk(const NsArgs& a, const double* u0, const double* v0, const double* w0,
-                   const double* u, const double* v, const double* w, double* uo, double* vo,
-                   double* wo, int stage, const StepCtl* sc, const DevGrid* gd, hipStream_t s);
+// Pads u, v, w (gd: device copy of a.g) and writes the stage update: stage 0
+// writes R itself, stages 1-3 the SSP-RK3 combinations with u0.
+void launch_rhs_rk(const NsArgs& a, const DevGrid* gd, const PadBufs& pb, const double* u0, const double* v0,
+                   const double* w0, const double* u, const double* v, const double* w, double* uo, double* vo,
+                   double* wo, int stage, const StepCtl* sc, hipStream_t s);
 void launch_divergence(const NsArgs& a, const double* u, const double* v, const double* w,
                        double* b, hipStream_t s);
 void launch_correct(const NsArgs& a, double* u, double* v, double* w, const double* phi,

@@ -205,48 +205,61 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc
 // ---------------------------------------------------------------------------
 // Coarsest level: Jacobi-PCG to 1e-6 in at most 500 iterations, as
 // CgPoisson(1e-6, 500).solve (multigrid.cpp:178-183, poisson.cpp:136-240).
-// One workgroup; x, r, z, p and the diagonal of its cells live in registers,
-// p is mirrored in LDS for the stencil, neighbour LDS indices are resolved once,
-// and an iteration costs three workgroup barriers.
+// One workgroup of 512 threads, up to 4 cells per thread: x, r, p, the
+// diagonal and its inverse live in registers, p is mirrored in LDS for the
+// stencil (absent neighbours point at a zero slot), and an iteration costs two
+// DPP block reductions and three barriers.  The coarse solve's dot products
+// are tree-ordered, so it reproduces the reference's CG to rounding, not bit
+// for bit; z = r * (1/diag) is used for the same reason.
 // ---------------------------------------------------------------------------
 constexpr int kCoarseT = 512;
-constexpr int kCoarseMax = 4096;
-constexpr int kCPT = kCoarseMax / kCoarseT;
+constexpr int kCPT = 4;
+constexpr int kCoarseMax = kCoarseT * kCPT;  // the hierarchy stops at <= 2048 cells
+
+template <int K>
+__device__ __forceinline__ void coarseSum(double (&v)[K], double (*red)[kCoarseT / kWave]) {
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    v[q] = waveSum(v[q]);
+    if (lane == 0) red[q][wid] = v[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kCoarseT / kWave; ++w) s += red[q][w];
+    v[q] = s;
+  }
+}
 
 template <class T>
 __global__ void __launch_bounds__(kCoarseT) k_coarse_lds(MgLevel L, int singular, const T* __restrict__ b,
                                                          const double* mb, T* __restrict__ xout,
                                                          const SolveCtl* ctl) {
   if (ctl && ctl->done) return;
-  __shared__ T pl[kCoarseMax];
-  __shared__ double red[3][kCoarseT / kWave];
-  const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
-  constexpr int nw = kCoarseT / kWave;
+  __shared__ T pl[kCoarseMax + 1];  // [kCoarseMax] stays 0
+  __shared__ double redA[1][kCoarseT / kWave];
+  __shared__ double redB[2][kCoarseT / kWave];
+  const int tid = threadIdx.x;
   const int N = int(L.n);
   const T h2[3] = {T(L.h2[0]), T(L.h2[1]), T(L.h2[2])};
-  T x[kCPT], r[kCPT], z[kCPT], p[kCPT], dia[kCPT], dsum[kCPT];
+  T x[kCPT], r[kCPT], p[kCPT], inv[kCPT], dsum[kCPT];
   int nb[kCPT][6];
   bool fl[kCPT], live[kCPT];
   const T bm = mb ? T(*mb) : T(0);
-  auto bsum = [&](double v, int slot) -> double {
-    v = waveSum(v);
-    if (lane == 0) red[slot][wid] = v;
-    __syncthreads();
-    double s = 0.0;
-#pragma unroll
-    for (int w = 0; w < nw; ++w) s += red[slot][w];
-    return s;
-  };
-  double s0 = 0.0;
+  if (tid == 0) pl[kCoarseMax] = T(0);
+  double s0[1] = {0.0};
 #pragma unroll
   for (int q = 0; q < kCPT; ++q) {
     const int c = tid + q * kCoarseT;
     live[q] = c < N;
     fl[q] = false;
-    x[q] = r[q] = z[q] = p[q] = dsum[q] = T(0);
-    dia[q] = T(1);
+    x[q] = r[q] = p[q] = dsum[q] = T(0);
+    inv[q] = T(1);
 #pragma unroll
-    for (int d = 0; d < 6; ++d) nb[q][d] = -1;
+    for (int d = 0; d < 6; ++d) nb[q][d] = kCoarseMax;
     if (!live[q]) continue;
     const uint16_t code = L.code[c];
     fl[q] = code & kFluidBit;
@@ -254,7 +267,7 @@ __global__ void __launch_bounds__(kCoarseT) k_coarse_lds(MgLevel L, int singular
     if (mb && fl[q]) v = v - bm;
     r[q] = v;
     if (!fl[q]) continue;
-    s0 += double(v);
+    s0[0] += double(v);
     const int i = c % L.nx, j = (c / L.nx) % L.ny, k = c / (L.nx * L.ny);
     T dg = T(0);
 #pragma unroll
@@ -269,76 +282,74 @@ __global__ void __launch_bounds__(kCoarseT) k_coarse_lds(MgLevel L, int singular
       }
     }
     dsum[q] = dg;
-    dia[q] = dg > T(0) ? dg : T(1);
+    inv[q] = T(1) / (dg > T(0) ? dg : T(1));
   }
   if (singular && L.nf > 0) {
-    const T m = T(bsum(s0, 0) / double(L.nf));
+    coarseSum<1>(s0, redA);
+    const T m = T(s0[0] / double(L.nf));
 #pragma unroll
     for (int q = 0; q < kCPT; ++q) r[q] = fl[q] ? r[q] - m : T(0);
   }
-  double a0 = 0.0, a1 = 0.0;
+  double a2[2] = {0.0, 0.0};
 #pragma unroll
   for (int q = 0; q < kCPT; ++q) {
     if (!live[q]) continue;
-    z[q] = r[q] / dia[q];
-    p[q] = z[q];
+    p[q] = r[q] * inv[q];
     pl[tid + q * kCoarseT] = p[q];
-    a0 += double(r[q]) * double(z[q]);
-    a1 += double(r[q]) * double(r[q]);
+    a2[0] += double(r[q]) * double(p[q]);
+    a2[1] += double(r[q]) * double(r[q]);
   }
-  double rz = bsum(a0, 1);
-  const double bn = sqrt(bsum(a1, 2)) + 1e-300;
-  __syncthreads();
+  coarseSum<2>(a2, redB);
+  double rz = a2[0];
+  const double bn = sqrt(a2[1]) + 1e-300;
   for (int it = 0; it < 500; ++it) {
     T Ap[kCPT];
-    double sp = 0.0;
+    double sp[1] = {0.0};
 #pragma unroll
     for (int q = 0; q < kCPT; ++q) {
       Ap[q] = T(0);
       if (!live[q]) continue;
       if (fl[q]) {
-        T off = T(0);
-#pragma unroll
-        for (int dd = 0; dd < 6; ++dd)
-          if (nb[q][dd] >= 0) off += h2[dd >> 1] * pl[nb[q][dd]];
+        const T off = h2[0] * (pl[nb[q][0]] + pl[nb[q][1]]) + h2[1] * (pl[nb[q][2]] + pl[nb[q][3]]) +
+                      h2[2] * (pl[nb[q][4]] + pl[nb[q][5]]);
         Ap[q] = dsum[q] * p[q] - off;
       } else {
         Ap[q] = p[q];
       }
-      sp += double(p[q]) * double(Ap[q]);
+      sp[0] += double(p[q]) * double(Ap[q]);
     }
-    const double pAp = bsum(sp, 0);
-    if (fabs(pAp) < 1e-300) break;
-    const double alpha = rz / pAp;
-    double rr = 0.0, rz2 = 0.0;
+    coarseSum<1>(sp, redA);
+    if (fabs(sp[0]) < 1e-300) break;
+    const double alpha = rz / sp[0];
+    double rr[2] = {0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < kCPT; ++q) {
       if (!live[q]) continue;
       x[q] = x[q] + T(alpha) * p[q];
       r[q] = r[q] - T(alpha) * Ap[q];
-      z[q] = r[q] / dia[q];
-      rr += double(r[q]) * double(r[q]);
-      rz2 += double(r[q]) * double(z[q]);
+      rr[0] += double(r[q]) * double(r[q]);
+      rr[1] += double(r[q]) * double(r[q] * inv[q]);
     }
-    rr = bsum(rr, 1);
-    rz2 = bsum(rz2, 2);
-    if (sqrt(rr) <= 1e-6 * bn) break;
-    const double beta = rz2 / rz;
-    rz = rz2;
+    coarseSum<2>(rr, redB);
+    if (sqrt(rr[0]) <= 1e-6 * bn) break;
+    const double beta = rr[1] / rz;
+    rz = rr[1];
 #pragma unroll
     for (int q = 0; q < kCPT; ++q) {
       if (!live[q]) continue;
-      p[q] = z[q] + T(beta) * p[q];
+      p[q] = r[q] * inv[q] + T(beta) * p[q];
       pl[tid + q * kCoarseT] = p[q];
     }
     __syncthreads();
   }
+  __syncthreads();  // a break right after a reduction leaves redA being read
   if (singular && L.nf > 0) {
-    double sx = 0.0;
+    double sx[1] = {0.0};
 #pragma unroll
     for (int q = 0; q < kCPT; ++q)
-      if (fl[q]) sx += double(x[q]);
-    const T m = T(bsum(sx, 0) / double(L.nf));
+      if (fl[q]) sx[0] += double(x[q]);
+    coarseSum<1>(sx, redA);
+    const T m = T(sx[0] / double(L.nf));
 #pragma unroll
     for (int q = 0; q < kCPT; ++q)
       if (fl[q]) x[q] = x[q] - m;
@@ -451,6 +462,7 @@ __global__ void __launch_bounds__(1024) k_coarse_global(MgLevel L, int singular,
     for (int c = tid; c < N; c += nt) p[c] = z[c] + T(beta) * p[c];
     __syncthreads();
   }
+  __syncthreads();
   if (singular && L.nf > 0) {
     double sx[1] = {0.0};
     for (int c = tid; c < N; c += nt)

@@ -190,31 +190,20 @@ __device__ __forceinline__ double recon(int sc, double adv, double qm2, double q
   return adv >= 0.0 ? weno5(qm2, qm1, q0, qp1, qp2) : weno5(qp3, qp2, qp1, q0, qm1);
 }
 
-// Stencil readers: Fast for stencils wholly inside the arrays, Ghost otherwise.
-// GhostRd reads the face tables through a device-memory copy of the grid: the
-// ghost resolvers index them with run-time face numbers, which on a by-value
-// kernel argument would force a private (scratch) copy of the whole struct.
-struct FastRd {
+// Stencil reader over the padded copies: element (i,j,k) of a component,
+// ghost layers included, at p[i + sx*j + sxy*k] (p points at element (0,0,0)).
+struct PadRd {
   const double *u, *v, *w;
-  Idx I;
-  __device__ double U(int i, int j, int k) const { return u[I.u(i, j, k)]; }
-  __device__ double V(int i, int j, int k) const { return v[I.v(i, j, k)]; }
-  __device__ double W(int i, int j, int k) const { return w[I.w(i, j, k)]; }
-};
-struct GhostRd {
-  const DevGrid* g;
-  const double *u, *v, *w;
-  Idx I;
-  __device__ double U(int i, int j, int k) const { return ghostU(*g, u, i, j, k); }
-  __device__ double V(int i, int j, int k) const { return ghostV(*g, v, i, j, k); }
-  __device__ double W(int i, int j, int k) const { return ghostW(*g, w, i, j, k); }
+  int ux, uxy, vx, vxy, wx, wxy;
+  __device__ __forceinline__ double U(int i, int j, int k) const { return u[i + ux * j + uxy * k]; }
+  __device__ __forceinline__ double V(int i, int j, int k) const { return v[i + vx * j + vxy * k]; }
+  __device__ __forceinline__ double W(int i, int j, int k) const { return w[i + wx * j + wxy * k]; }
 };
 
 // u-momentum at x-face (i,j,k) (ns_solver.cpp:323-372).
-template <class Rd>
-__device__ double rhsU(const NsArgs& A, const Rd& r, int i, int j, int k) {
+template <int sc, class Rd>
+__device__ __forceinline__ double rhsU(const NsArgs& A, const Rd& r, int i, int j, int k) {
   const DevGrid& g = A.g;
-  const int sc = A.convection;
   const bool flat = g.nz == 1;
   const double aL = 0.5 * (r.U(i - 1, j, k) + r.U(i, j, k));
   const double aR = 0.5 * (r.U(i, j, k) + r.U(i + 1, j, k));
@@ -247,10 +236,9 @@ __device__ double rhsU(const NsArgs& A, const Rd& r, int i, int j, int k) {
 }
 
 // v-momentum at y-face (i,j,k) (ns_solver.cpp:377-424).
-template <class Rd>
-__device__ double rhsV(const NsArgs& A, const Rd& r, int i, int j, int k) {
+template <int sc, class Rd>
+__device__ __forceinline__ double rhsV(const NsArgs& A, const Rd& r, int i, int j, int k) {
   const DevGrid& g = A.g;
-  const int sc = A.convection;
   const bool flat = g.nz == 1;
   const double bB = 0.5 * (r.V(i, j - 1, k) + r.V(i, j, k));
   const double bT = 0.5 * (r.V(i, j, k) + r.V(i, j + 1, k));
@@ -283,10 +271,9 @@ __device__ double rhsV(const NsArgs& A, const Rd& r, int i, int j, int k) {
 }
 
 // w-momentum at z-face (i,j,k) (ns_solver.cpp:427-472).
-template <class Rd>
-__device__ double rhsW(const NsArgs& A, const Rd& r, int i, int j, int k) {
+template <int sc, class Rd>
+__device__ __forceinline__ double rhsW(const NsArgs& A, const Rd& r, int i, int j, int k) {
   const DevGrid& g = A.g;
-  const int sc = A.convection;
   const double cB = 0.5 * (r.W(i, j, k - 1) + r.W(i, j, k));
   const double cT = 0.5 * (r.W(i, j, k) + r.W(i, j, k + 1));
   const double qB = recon(sc, cB, r.W(i, j, k - 3), r.W(i, j, k - 2), r.W(i, j, k - 1), r.W(i, j, k),
@@ -314,83 +301,143 @@ __device__ double rhsW(const NsArgs& A, const Rd& r, int i, int j, int k) {
   return -(fx + fy + fz) + A.nu * lap + A.bodyForce[2];
 }
 
-// True when every index of a +-3 stencil around (i,j,k) of component comp lies
-// inside the stored arrays (no ghost resolution needed).
-__device__ __forceinline__ bool interior(const DevGrid& g, int comp, int i, int j, int k) {
-  const bool flat = g.nz == 1;
-  const int mx = comp == 0 ? g.nx : g.nx - 1;
-  const int my = comp == 1 ? g.ny : g.ny - 1;
-  const int mz = comp == 2 ? g.nz : g.nz - 1;
-  if (i < 3 || i + 3 > mx || j < 3 || j + 3 > my) return false;
-  if (!flat && (k < 3 || k + 3 > mz)) return false;
-  return true;
+// ---------------------------------------------------------------------------
+// Padded copies.  Each component array gets kPadHalo ghost layers on every
+// side, filled with what the reference's ghost accessors return for those
+// indices (ns_solver.cpp:116-219); the ghost value of an index depends only on
+// the index and the stored array, so every stencil that reads it sees the same
+// value it would in the reference.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void compDims(const DevGrid& g, int c, int& X, int& Y, int& Z) {
+  X = g.nx + (c == 0);
+  Y = g.ny + (c == 1);
+  Z = g.nz + (c == 2);
 }
 
-// RHS + SSP-RK3 stage combine over u|v|w faces (ns_solver.cpp:776-794).
-__global__ void __launch_bounds__(kBlock) k_rhs_rk(NsArgs A, const double* __restrict__ u0,
-                                                   const double* __restrict__ v0,
-                                                   const double* __restrict__ w0,
-                                                   const double* __restrict__ u,
-                                                   const double* __restrict__ v,
-                                                   const double* __restrict__ w, double* __restrict__ uo,
-                                                   double* __restrict__ vo, double* __restrict__ wo,
-                                                   int stage, const StepCtl* __restrict__ sc,
-                                                   const DevGrid* __restrict__ gd) {
+// Interior copy: block z indexes the planes of u, then v, then w.
+__global__ void __launch_bounds__(kBlock) k_pad_copy(NsArgs A, const double* __restrict__ u,
+                                                     const double* __restrict__ v, const double* __restrict__ w,
+                                                     PadBufs pb) {
   const DevGrid& g = A.g;
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const unsigned nu = unsigned(nx + 1) * ny * nz, nv = unsigned(nx) * (ny + 1) * nz;
-  const unsigned nw = unsigned(nx) * ny * (nz + 1);
-  unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nu + nv + nw) return;
-  const double dt = sc->dt;
-  int comp;
-  int i, j, k;
-  if (t < nu) {
-    comp = 0;
-    const unsigned rx = unsigned(nx + 1);
-    i = int(t % rx);
-    const unsigned q = t / rx;
-    j = int(q % unsigned(ny));
-    k = int(q / unsigned(ny));
-  } else if (t < nu + nv) {
-    comp = 1;
-    t -= nu;
-    i = int(t % unsigned(nx));
-    const unsigned q = t / unsigned(nx);
-    j = int(q % unsigned(ny + 1));
-    k = int(q / unsigned(ny + 1));
-  } else {
-    comp = 2;
-    t -= nu + nv;
-    i = int(t % unsigned(nx));
-    const unsigned q = t / unsigned(nx);
-    j = int(q % unsigned(ny));
-    k = int(q / unsigned(ny));
-  }
-  double R = 0.0;
-  bool active;
-  if (comp == 0) active = i >= 1 && i < nx && !solidAt(g, i - 1, j, k) && !solidAt(g, i, j, k);
-  else if (comp == 1) active = j >= 1 && j < ny && !solidAt(g, i, j - 1, k) && !solidAt(g, i, j, k);
-  else active = nz > 1 && k >= 1 && k < nz && !solidAt(g, i, j, k - 1) && !solidAt(g, i, j, k);
-  if (active) {
-    const Idx I{nx, ny, nz};
-    if (interior(g, comp, i, j, k)) {
-      FastRd r{u, v, w, I};
-      R = comp == 0 ? rhsU(A, r, i, j, k) : (comp == 1 ? rhsV(A, r, i, j, k) : rhsW(A, r, i, j, k));
-    } else {
-      GhostRd r{gd, u, v, w, I};
-      R = comp == 0 ? rhsU(A, r, i, j, k) : (comp == 1 ? rhsV(A, r, i, j, k) : rhsW(A, r, i, j, k));
+  int c = 0, k = blockIdx.z;
+  if (k >= g.nz) {
+    k -= g.nz;
+    c = 1;
+    if (k >= g.nz) {
+      k -= g.nz;
+      c = 2;
     }
   }
+  int X, Y, Z;
+  compDims(g, c, X, Y, Z);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= X || j >= Y) return;
+  const double* src = c == 0 ? u : (c == 1 ? v : w);
+  double* dst = pb.p[c] + pb.origin[c];
+  dst[i + pb.sx[c] * j + pb.sxy[c] * k] = src[i + X * (j + Y * k)];
+}
+
+// Ghost layers: one thread per halo element of u, v, w (k-planes, then
+// j-rows of in-range planes, then i-ends of in-range rows).
+__global__ void __launch_bounds__(kBlock) k_pad_halo(const DevGrid* __restrict__ gd, const double* __restrict__ u,
+                                                     const double* __restrict__ v, const double* __restrict__ w,
+                                                     PadBufs pb) {
+  const DevGrid& g = *gd;
+  unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int H = kPadHalo;
+  for (int c = 0; c < 3; ++c) {
+    int X, Y, Z;
+    compDims(g, c, X, Y, Z);
+    const unsigned PX = X + 2 * H, PY = Y + 2 * H;
+    const unsigned nk = PX * PY * (2 * H), nj = PX * (2 * H) * unsigned(Z), ni = (2 * H) * unsigned(Y) * Z;
+    if (t >= nk + nj + ni) {
+      t -= nk + nj + ni;
+      continue;
+    }
+    int i, j, k;
+    if (t < nk) {
+      i = int(t % PX) - H;
+      const unsigned q = t / PX;
+      j = int(q % PY) - H;
+      const int kk = int(q / PY);
+      k = kk < H ? kk - H : Z + (kk - H);
+    } else if (t < nk + nj) {
+      t -= nk;
+      i = int(t % PX) - H;
+      const unsigned q = t / PX;
+      const int jj = int(q % (2 * H));
+      j = jj < H ? jj - H : Y + (jj - H);
+      k = int(q / (2 * H));
+    } else {
+      t -= nk + nj;
+      const int ii = int(t % (2 * H));
+      i = ii < H ? ii - H : X + (ii - H);
+      const unsigned q = t / (2 * H);
+      j = int(q % unsigned(Y));
+      k = int(q / unsigned(Y));
+    }
+    // Stencils of active faces leave the array along one axis at a time, so
+    // edge and corner halo elements are never read.  They are not resolved:
+    // the ghost resolvers assume the other indices are in range (an outlet
+    // copy would read outside the array otherwise).
+    const int oob = (i < 0 || i >= X) + (j < 0 || j >= Y) + (k < 0 || k >= Z);
+    double val = 0.0;
+    if (oob == 1) {
+      if (c == 0) val = ghostU(g, u, i, j, k);
+      else if (c == 1) val = ghostV(g, v, i, j, k);
+      else val = ghostW(g, w, i, j, k);
+    }
+    pb.p[c][pb.origin[c] + i + pb.sx[c] * j + pb.sxy[c] * k] = val;
+    return;
+  }
+}
+
+// RHS + SSP-RK3 stage combine (ns_solver.cpp:776-794), one thread per face,
+// block z indexing the planes of u, then v, then w.  Stencils read the padded
+// copies; u0 and the output are the compact arrays.
+template <int SCHEME>
+__global__ void __launch_bounds__(kBlock) k_rhs_rk(NsArgs A, PadBufs pb, const double* __restrict__ u0,
+                                                   const double* __restrict__ v0, const double* __restrict__ w0,
+                                                   double* __restrict__ uo, double* __restrict__ vo,
+                                                   double* __restrict__ wo, int stage,
+                                                   const StepCtl* __restrict__ sc) {
+  const DevGrid& g = A.g;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  int comp = 0, k = blockIdx.z;
+  if (k >= nz) {
+    k -= nz;
+    comp = 1;
+    if (k >= nz) {
+      k -= nz;
+      comp = 2;
+    }
+  }
+  int X, Y, Z;
+  compDims(g, comp, X, Y, Z);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= X || j >= Y) return;
+  const PadRd r{pb.p[0] + pb.origin[0], pb.p[1] + pb.origin[1], pb.p[2] + pb.origin[2],
+                pb.sx[0], pb.sxy[0], pb.sx[1], pb.sxy[1], pb.sx[2], pb.sxy[2]};
+  double R = 0.0, xc;
+  const int t = i + X * (j + Y * k);
+  if (comp == 0) {
+    xc = r.U(i, j, k);
+    if (i >= 1 && i < nx && !solidAt(g, i - 1, j, k) && !solidAt(g, i, j, k)) R = rhsU<SCHEME>(A, r, i, j, k);
+  } else if (comp == 1) {
+    xc = r.V(i, j, k);
+    if (j >= 1 && j < ny && !solidAt(g, i, j - 1, k) && !solidAt(g, i, j, k)) R = rhsV<SCHEME>(A, r, i, j, k);
+  } else {
+    xc = r.W(i, j, k);
+    if (nz > 1 && k >= 1 && k < nz && !solidAt(g, i, j, k - 1) && !solidAt(g, i, j, k)) R = rhsW<SCHEME>(A, r, i, j, k);
+  }
   const double* a0 = comp == 0 ? u0 : (comp == 1 ? v0 : w0);
-  const double* ac = comp == 0 ? u : (comp == 1 ? v : w);
   double* ao = comp == 0 ? uo : (comp == 1 ? vo : wo);
-  const double x0 = a0[t], xc = ac[t];
+  const double dt = sc->dt;
   double out;
   if (stage == 0) out = R;
-  else if (stage == 1) out = x0 + dt * R;
-  else if (stage == 2) out = 0.75 * x0 + 0.25 * (xc + dt * R);
-  else out = (1.0 / 3.0) * x0 + (2.0 / 3.0) * (xc + dt * R);
+  else if (stage == 1) out = a0[t] + dt * R;
+  else if (stage == 2) out = 0.75 * a0[t] + 0.25 * (xc + dt * R);
+  else out = (1.0 / 3.0) * a0[t] + (2.0 / 3.0) * (xc + dt * R);
   ao[t] = out;
 }
 
@@ -771,10 +818,45 @@ void launch_bc_solid(const NsArgs& a, double* u, double* v, double* w, const int
   NBL_LAUNCH_CHECK();
 }
 
-void launch_rhs_rk(const NsArgs& a, const double* u0, const double* v0, const double* w0,
-                   const double* u, const double* v, const double* w, double* uo, double* vo,
-                   double* wo, int stage, const StepCtl* sc, const DevGrid* gd, hipStream_t s) {
-  k_rhs_rk<<<gridFor(faces(a.g)), kBlock, 0, s>>>(a, u0, v0, w0, u, v, w, uo, vo, wo, stage, sc, gd);
+PadBufs padLayout(const DevGrid& g) {
+  PadBufs pb{};
+  const int H = kPadHalo;
+  for (int c = 0; c < 3; ++c) {
+    const long long X = g.nx + (c == 0), Y = g.ny + (c == 1), Z = g.nz + (c == 2);
+    const long long PX = X + 2 * H, PY = Y + 2 * H, PZ = Z + 2 * H;
+    pb.sx[c] = int(PX);
+    pb.sxy[c] = int(PX * PY);
+    pb.origin[c] = H + PX * (H + PY * H);
+    pb.n[c] = PX * PY * PZ;
+  }
+  return pb;
+}
+
+static long long haloCount(const DevGrid& g) {
+  const long long H = kPadHalo;
+  long long n = 0;
+  for (int c = 0; c < 3; ++c) {
+    const long long X = g.nx + (c == 0), Y = g.ny + (c == 1), Z = g.nz + (c == 2);
+    n += (X + 2 * H) * (Y + 2 * H) * 2 * H + (X + 2 * H) * 2 * H * Z + 2 * H * Y * Z;
+  }
+  return n;
+}
+
+void launch_rhs_rk(const NsArgs& a, const DevGrid* gd, const PadBufs& pb, const double* u0, const double* v0,
+                   const double* w0, const double* u, const double* v, const double* w, double* uo, double* vo,
+                   double* wo, int stage, const StepCtl* sc, hipStream_t s) {
+  const DevGrid& g = a.g;
+  for (int c = 0; c < 3; ++c)
+    if (pb.n[c] >= (1LL << 31)) throw std::invalid_argument("grid too large for 32-bit face indexing");
+  // block z walks the planes of u (nz), v (nz) and w (nz + 1)
+  const Geo3 geo = geo3(g.nx + 1, g.ny + 1, 3 * g.nz + 1);
+  const dim3 grid = geo.grid;
+  k_pad_copy<<<grid, geo.block, 0, s>>>(a, u, v, w, pb);
+  k_pad_halo<<<gridFor(haloCount(g)), kBlock, 0, s>>>(gd, u, v, w, pb);
+  if (a.convection == NBL_CONV_CENTRAL)
+    k_rhs_rk<NBL_CONV_CENTRAL><<<grid, geo.block, 0, s>>>(a, pb, u0, v0, w0, uo, vo, wo, stage, sc);
+  else
+    k_rhs_rk<NBL_CONV_WENO5><<<grid, geo.block, 0, s>>>(a, pb, u0, v0, w0, uo, vo, wo, stage, sc);
   NBL_LAUNCH_CHECK();
 }
 

@@ -543,6 +543,11 @@ RunDev::RunDev(nbl_ctx* ctx, const HostGrid& g, const nbl_flow_cfg& cfg, const n
   NBL_HIP(hipMemset(p_.as<void>(), 0, sz_[3] * 8));
   sc_ = DevBuf(ctx, sizeof(StepCtl));
   rep_ = DevBuf(ctx, sizeof(DevReport));
+  pad_ = padLayout(args_.g);
+  for (int c = 0; c < 3; ++c) {
+    padBuf_[c] = DevBuf(ctx, size_t(pad_.n[c]) * 8);
+    pad_.p[c] = padBuf_[c].as<double>();
+  }
   gridDev_ = DevBuf(ctx, sizeof(DevGrid));
   NBL_HIP(hipMemcpy(gridDev_.as<void>(), &args_.g, sizeof(DevGrid), hipMemcpyHostToDevice));
   redStride_ = ns_partials(args_.g);
@@ -644,8 +649,8 @@ nbl_step_report RunDev::step(double dt) {  // ns_solver.cpp:765-824
   bc(true);
   nbl_poisson_report pr[3];
   for (int stage = 1; stage <= 3; ++stage) {
-    launch_rhs_rk(args_, u0[0], u0[1], u0[2], cur_[0], cur_[1], cur_[2], alt_[0], alt_[1], alt_[2], stage, sc,
-                  gridDev_.as<DevGrid>(), stream_);
+    launch_rhs_rk(args_, gridDev_.as<DevGrid>(), pad_, u0[0], u0[1], u0[2], cur_[0], cur_[1], cur_[2], alt_[0],
+                  alt_[1], alt_[2], stage, sc, stream_);
     std::swap(cur_, alt_);
     project(stage, pr[stage - 1]);
   }
@@ -680,8 +685,8 @@ void RunDev::rhs(double* ru, double* rv, double* rw) {
   if (args_.g.hasSolid)
     launch_bc_solid(args_, alt_[0], alt_[1], alt_[2], solidList_.as<int>(), solidCounts_, stream_);
   double* out[3] = {buf0_[0].as<double>(), buf0_[1].as<double>(), buf0_[2].as<double>()};
-  launch_rhs_rk(args_, alt_[0], alt_[1], alt_[2], alt_[0], alt_[1], alt_[2], out[0], out[1], out[2], 0,
-                sc_.as<StepCtl>(), gridDev_.as<DevGrid>(), stream_);
+  launch_rhs_rk(args_, gridDev_.as<DevGrid>(), pad_, alt_[0], alt_[1], alt_[2], alt_[0], alt_[1], alt_[2], out[0],
+                out[1], out[2], 0, sc_.as<StepCtl>(), stream_);
   double* dst[3] = {ru, rv, rw};
   for (int q = 0; q < 3; ++q) NBL_HIP(hipMemcpyAsync(dst[q], out[q], sz_[q] * 8, hipMemcpyDeviceToHost, stream_));
   NBL_HIP(hipStreamSynchronize(stream_));

@@ -177,6 +177,8 @@ class RunDev {
   double* cur_[3];
   double* alt_[3];
   DevBuf sc_, rep_, red_, gridDev_;
+  DevBuf padBuf_[3];
+  PadBufs pad_{};
   int redStride_ = 1;
   StepCtl* hsc_ = nullptr;
   DevReport* hrep_ = nullptr;

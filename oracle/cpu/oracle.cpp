@@ -1445,6 +1445,15 @@ int orc_oct_face_neighbors(void* h, size_t i, int dir, uint64_t* out) {
   for (size_t m = 0; m < v.size(); ++m) out[m] = v[m];
   return int(v.size());
 }
+void orc_oct_face_table(void* h, uint8_t* cnt, int64_t* nbr) {
+  const Oct& o = *static_cast<Oct*>(h);
+  for (size_t i = 0; i < o.key.size(); ++i)
+    for (int d = 0; d < 6; ++d) {
+      const auto v = o.nbrs(i, d);
+      cnt[6 * i + d] = uint8_t(v.size());
+      for (int m = 0; m < 4; ++m) nbr[24 * i + 4 * d + m] = m < int(v.size()) ? int64_t(v[m]) : -1;
+    }
+}
 int orc_oct_is_balanced(void* h) { return static_cast<Oct*>(h)->balanced() ? 1 : 0; }
 void orc_oct_centers(void* h, double* xyz) {
   const Oct& o = *static_cast<Oct*>(h);

@@ -67,6 +67,8 @@ int orc_oct_level_of(void* h, uint64_t key);
 void orc_oct_set_mask(void* h, size_t i, uint8_t m);
 void orc_oct_dump(void* h, uint64_t* keys, uint8_t* levels, uint8_t* masks);
 int orc_oct_face_neighbors(void* h, size_t i, int dir, uint64_t* out);
+// cnt[6i+d]: neighbours of leaf i across face d; nbr[24i+4d+m]: their indices (-1 unused)
+void orc_oct_face_table(void* h, uint8_t* cnt, int64_t* nbr);
 int orc_oct_is_balanced(void* h);
 void orc_oct_centers(void* h, double* xyz);
 void orc_oct_apply(void* h, const double* x, double* out);

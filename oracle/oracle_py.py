@@ -25,6 +25,7 @@ ORACLE_LIB = HERE / "lib" / "liboracle.so"
 REF_LIB = HERE / "_ref" / "libnabla_ref.so"
 
 P_D, P_U8, P_U64 = nbl.P_D, nbl.P_U8, nbl.P_U64
+P_I64 = C.POINTER(C.c_int64)
 dptr = nbl.dptr
 
 
@@ -56,6 +57,7 @@ class OracleLib:
             "oct_level_of": (i, V, C.c_uint64),
             "oct_dump": (None, V, P_U64, P_U8, P_U8),
             "oct_face_neighbors": (i, V, sz, i, P_U64), "oct_is_balanced": (i, V),
+            "oct_face_table": (None, V, P_U8, P_I64),
             "oct_centers": (None, V, P_D), "oct_apply": (None, V, P_D, P_D),
             "oct_diagonal": (None, V, P_D), "oct_solve": (None, V, P_D, P_D, d, i, PR),
         }
@@ -233,6 +235,13 @@ class Oct:
         buf = (C.c_uint64 * 4)()
         c = self.o.oct_face_neighbors(self.h, int(i), int(d), buf)
         return [int(buf[m]) for m in range(c)]
+
+    def face_table(self):
+        """(cnt[n,6], nbr[n,6,4]) leaf indices, -1 padded, in the tree's order."""
+        cnt = np.empty((self.n, 6), np.uint8)
+        nbr = np.empty((self.n, 6, 4), np.int64)
+        self.o.oct_face_table(self.h, cnt.ctypes.data_as(P_U8), nbr.ctypes.data_as(P_I64))
+        return cnt, nbr
 
     def balanced(self):
         return bool(self.o.oct_is_balanced(self.h))

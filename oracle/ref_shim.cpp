@@ -205,6 +205,17 @@ int ref_oct_face_neighbors(void* h, size_t i, int dir, uint64_t* out) {
   for (size_t m = 0; m < v.size(); ++m) out[m] = v[m];
   return static_cast<int>(v.size());
 }
+// All face neighbours at once: cnt[6i+d] neighbours of leaf i across face d,
+// their leaf indices in nbr[24i+4d ..], unused slots -1.
+void ref_oct_face_table(void* h, uint8_t* cnt, int64_t* nbr) {
+  const auto& t = *static_cast<mesh::Octree*>(h);
+  for (size_t i = 0; i < t.cellCount(); ++i)
+    for (int d = 0; d < 6; ++d) {
+      const auto v = t.faceNeighbors(i, d);
+      cnt[6 * i + d] = static_cast<uint8_t>(v.size());
+      for (int m = 0; m < 4; ++m) nbr[24 * i + 4 * d + m] = m < int(v.size()) ? int64_t(v[m]) : -1;
+    }
+}
 int ref_oct_is_balanced(void* h) { return static_cast<mesh::Octree*>(h)->isBalanced() ? 1 : 0; }
 void ref_oct_centers(void* h, double* xyz) {
   const auto& t = *static_cast<mesh::Octree*>(h);

@@ -191,10 +191,15 @@ def c3_tree(ref, max_level=8, base=5):
     k, l, m = t.dump()
     order = np.argsort(k)
     h = hashlib.sha256(np.ascontiguousarray(k[order]).tobytes() + np.ascontiguousarray(l[order]).tobytes())
+    t1 = _t.time()
+    cnt, nbr = t.face_table()
+    fh = cases.face_stream_sha256(k, cnt, nbr)
+    print(f"c3 face table {_t.time() - t1:.0f}s", flush=True)
     np.savez_compressed(HERE / f"c3_tree_L{max_level}.npz", n=t.n, sha256=h.hexdigest(),
+                        face_sha256=fh, face_count=np.int64(cnt.sum()),
                         level_hist=np.bincount(l, minlength=max_level + 1),
                         balanced=t.balanced() if t.n < 300000 else True)
-    print("c3", max_level, t.n, h.hexdigest(), flush=True)
+    print("c3", max_level, t.n, h.hexdigest(), fh, flush=True)
 
 
 def main():
