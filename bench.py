@@ -74,8 +74,20 @@ def roofline(ctx, g, reps=200):
     ach = by / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_sweep_w<double> (finest-level damped-Jacobi sweep + fused fluid sum)",
+            "kernel": "k_sweep<double> (finest-level damped-Jacobi sweep; fluid-sum fold when singular)",
             "avg_launch_us": round(ms * 1e3, 3), "bytes_per_launch": by, "cells": g.cells}
+
+
+def coarse_solve(ctx, g, reps=50):
+    """Average time of one coarsest-level solve (24x8x8 here) for both coarse
+    modes: the direct inverse (default) and the reference's CG(1e-6, 500)."""
+    out = {}
+    for name, mode in (("direct", nbl.COARSE_DIRECT), ("cg", nbl.COARSE_CG)):
+        ps = nbl.Poisson(ctx, g, kind=0, tol=1e-8, max_iters=5000, opts=nbl.solver_opts(coarse=mode))
+        ms, _by = ps.profile(3, reps)
+        ps.close()
+        out[f"{name}_us"] = round(ms * 1e3, 2)
+    return out
 
 
 def cpu_baseline(g, cfg, st, steps):
@@ -162,6 +174,7 @@ def main():
         }
         log(f"mixed: {mval:.2f} Mcell-steps/s")
     out["roofline"] = roofline(ctx, g)
+    out["coarse_solve"] = coarse_solve(ctx, g)
     big = nbl.Grid(256, 256, 256, 1.0 / 256)
     out["roofline_256cubed"] = roofline(ctx, big, reps=50)
     ctx.close()

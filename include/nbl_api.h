@@ -54,6 +54,15 @@ enum {
   NBL_PREC_MIXED = 1, /* FP32 V-cycle inside FP64 flexible CG (FP64 residual/dots)   */
   NBL_PREC_MIXED_BF16 = 2 /* as MIXED, with BF16 smoother scratch on the finest level */
 };
+/* Coarsest-level solve of the V-cycle.  The reference runs CgPoisson(1e-6, 500)
+   there ("near-exact", multigrid.cpp:178-183). */
+enum {
+  NBL_COARSE_DIRECT = 0, /* exact: the level's inverse, formed once on the device
+                            (projected onto the fluid mean-zero space when the
+                            system is singular); falls back to CG when it cannot
+                            be formed or the level exceeds 4096 cells          */
+  NBL_COARSE_CG = 1      /* the reference's Jacobi-PCG to 1e-6, <= 500 iterations */
+};
 
 /* One domain face (grid.hpp:20-26). */
 typedef struct nbl_face_bc {
@@ -98,6 +107,7 @@ typedef struct nbl_solver_opts {
   int post_smooth;          /* sweeps after prolongation (2)                */
   int check_every;          /* host polls the device convergence flag after
                                this many flexible-CG iterations (>=1)       */
+  int coarse;               /* NBL_COARSE_* (default DIRECT)                */
 } nbl_solver_opts;
 
 /* StepReport (ns_solver.hpp:34-49). */
@@ -133,7 +143,7 @@ typedef struct nbl_tree nbl_tree;
 
 /* ---- defaults and errors ------------------------------------------------ */
 void nbl_flow_cfg_default(nbl_flow_cfg* cfg);        /* FlowConfig{} values        */
-void nbl_solver_opts_default(nbl_solver_opts* opts); /* FP64, V(2,2), check 1     */
+void nbl_solver_opts_default(nbl_solver_opts* opts); /* FP64, V(2,2), check 1, direct coarse */
 const char* nbl_last_error(void);
 const char* nbl_version(void);
 
@@ -221,7 +231,9 @@ int nbl_poisson_last_timing(nbl_poisson* ps, double* ms, int* vcycles);
  *             (reads x, b, 2-byte code; writes x),
  *   kernel 1: finest-level A p with the p.Ap fold (reads p, code; writes Ap),
  *   kernel 2: finest-level residual + restriction (reads b, x, code; writes
- *             the coarse residual). */
+ *             the coarse residual),
+ *   kernel 3: one coarsest-level solve (direct: reads the n^2 inverse, b and
+ *             code, writes x; CG: counted as b, x and code only). */
 int nbl_poisson_profile(nbl_poisson* ps, int kernel, int reps, double* ms, double* bytes);
 
 /* ---- octree construction (host; Octree::refineUniform / refineByMorton /

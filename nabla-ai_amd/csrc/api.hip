@@ -76,6 +76,7 @@ void nbl_solver_opts_default(nbl_solver_opts* o) {
   o->pre_smooth = 2;
   o->post_smooth = 2;
   o->check_every = 1;
+  o->coarse = NBL_COARSE_DIRECT;
 }
 
 const char* nbl_last_error(void) { return g_err.c_str(); }

@@ -149,4 +149,31 @@ __device__ __forceinline__ void foldPartials(const double* partials, int stride,
 
 constexpr int kFinThreads = 1024;  // threads of a one-block finalize kernel
 
+// XCD-aware tile order.  Workgroups are handed to the 8 XCDs round-robin by
+// linear block id, and each XCD has its own L2; renumbering so that XCD x
+// receives the contiguous tile range [x*q + min(x, rem), ...) keeps the planes
+// a stencil shares with its neighbours inside one L2.  A bijection on [0, B).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcdTile(int b, int B) {
+  const int q = B / kXcds, rem = B % kXcds;
+  const int x = b % kXcds, idx = b / kXcds;
+  return x * q + min(x, rem) + idx;
+}
+// 3-D block coordinates of this workgroup's tile under xcdTile.
+struct Blk3 {
+  int x, y, z, lin;
+};
+__device__ __forceinline__ Blk3 blk3() {
+  const int B = int(gridDim.x * gridDim.y * gridDim.z);
+  const int b = int(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+  const int t = xcdTile(b, B);
+  Blk3 r;
+  r.lin = t;
+  r.x = t % int(gridDim.x);
+  const int u = t / int(gridDim.x);
+  r.y = u % int(gridDim.y);
+  r.z = u / int(gridDim.y);
+  return r;
+}
+
 }  // namespace nbl

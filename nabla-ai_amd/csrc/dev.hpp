@@ -58,6 +58,10 @@ struct Geo3 {
   int blocks;
 };
 Geo3 geo3(int nx, int ny, int nz);
+// Launch shape of the fused residual + restriction from L to Lc: one thread per
+// fine cell, blocks covering whole coarse cells (its block count is the
+// number of partials the fold writes).
+Geo3 restrictGeo(const MgLevel& L, const MgLevel& Lc);
 
 // Scalar update applied by the one-block finalize kernel after a reduction.
 enum FinOp : int {
@@ -161,6 +165,16 @@ void launch_mg_resid_restrict(const MgLevel& L, const MgLevel& Lc, const T* b, c
 template <class T>
 void launch_mg_coarse(const MgLevel& L, bool singular, const T* b, const double* mb, T* x,
                       T* scratch, const SolveCtl* ctl, hipStream_t s);
+// Direct coarse solve: one Gauss-Jordan step on the n x 2n augmented matrix
+// (sets *fail when a pivot is not above `tiny`), the projected inverse from the
+// eliminated matrix (scratch: n + 1 doubles), and x = Minv (b - mb) per solve
+// (FP32 levels use the FP32 copy Mf).
+void launch_gj_step(double* G, int n, int k, double* colk, double* rowk, int* fail, double tiny, hipStream_t s);
+void launch_inv_store(const double* G, int n, const uint16_t* code, long long nf, bool singular, double* scratch,
+                      double* M, float* Mf, hipStream_t s);
+template <class T>
+void launch_mg_coarse_direct(const MgLevel& L, const double* M, const float* Mf, const T* b, const double* mb, T* x,
+                             const SolveCtl* ctl, hipStream_t s);
 template <class T>
 void launch_mg_finish(const MgLevel& L, const T* x, const double* mx, double* z, bool fold,
                       RedBuf rb, const SolveCtl* ctl, hipStream_t s);

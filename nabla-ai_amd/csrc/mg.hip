@@ -40,12 +40,13 @@ struct Cell {
 
 __device__ __forceinline__ Cell cellOf(int nx, int ny, int nz) {
   Cell q;
-  q.i = blockIdx.x * blockDim.x + threadIdx.x;
-  q.j = blockIdx.y * blockDim.y + threadIdx.y;
-  q.k = blockIdx.z;
+  const Blk3 B = blk3();
+  q.i = B.x * blockDim.x + threadIdx.x;
+  q.j = B.y * blockDim.y + threadIdx.y;
+  q.k = B.z;
   q.valid = q.i < nx && q.j < ny && q.k < nz;
   q.c = q.i + nx * (q.j + ny * q.k);
-  q.blk = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  q.blk = B.lin;
   return q;
 }
 
@@ -160,8 +161,10 @@ __global__ void __launch_bounds__(kBlock) k_sweep(MgLevel L, const T* __restrict
 }
 
 // Residual r = b - A x on fine fluid cells restricted to the coarse grid as
-// the mean over fluid children, children summed in the reference's k,j,i
-// order (multigrid.cpp:116-154); one thread per coarse cell.  ZEROX: x == 0
+// the mean over fluid children (multigrid.cpp:116-154).  One thread per fine
+// cell; a block covers whole coarse cells (blockDim.z = fz, even x/y extents),
+// stages its residuals in LDS, and the thread at each coarse cell's first
+// child sums the children in the reference's k, j, i order.  ZEROX: x == 0
 // (restriction of b itself).
 template <class T, bool ZEROX>
 __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc, const T* __restrict__ b,
@@ -169,51 +172,75 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc
                                                            const double* mx, T* __restrict__ rc, int fold,
                                                            RedBuf rb, const SolveCtl* ctl) {
   if (ctl && ctl->done) return;
+  __shared__ T rs[kBlock];
+  __shared__ unsigned char fls[kBlock];
   const XView<T> X{x, mx ? T(*mx) : T(0), mx != nullptr, nullptr, 1, 1, 1, 1, 1, T(0), false};
-  const Cell Q = cellOf(Lc.nx, Lc.ny, Lc.nz);
-  double acc[1] = {0.0};
-  if (Q.valid) {
-    if (!(Lc.code[Q.c] & kFluidBit)) {
-      rc[Q.c] = T(0);
-    } else {
-      const T bm = mb ? T(*mb) : T(0);
-      T s = T(0);
-      int cnt = 0;
-      const int k1 = min((Q.k + 1) * Lc.fz, L.nz), j1 = min((Q.j + 1) * Lc.fy, L.ny);
-      const int i1 = min((Q.i + 1) * Lc.fx, L.nx);
-      for (int k = Q.k * Lc.fz; k < k1; ++k)
-        for (int j = Q.j * Lc.fy; j < j1; ++j)
-          for (int i = Q.i * Lc.fx; i < i1; ++i) {
-            const int c = i + L.nx * (j + L.ny * k);
-            const uint16_t code = L.code[c];
-            if (!(code & kFluidBit)) continue;
-            T dg;
-            const T Ax = applyRow<T, ZEROX>(L, code, c, i, j, k, X, dg);
-            T bv = b[c];
-            if (mb) bv = bv - bm;
-            s += bv - Ax;
-            ++cnt;
-          }
-      const T r = s / T(cnt);
-      rc[Q.c] = r;
-      acc[0] = double(r);
+  const Blk3 B = blk3();
+  const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
+  const int bx = blockDim.x, by = blockDim.y;
+  const int i = B.x * bx + tx, j = B.y * by + ty, k = B.z * int(blockDim.z) + tz;
+  const int tid = tx + bx * (ty + by * tz);
+  T r = T(0);
+  bool fl = false;
+  if (i < L.nx && j < L.ny && k < L.nz) {
+    const int c = i + L.nx * (j + L.ny * k);
+    const uint16_t code = L.code[c];
+    if (code & kFluidBit) {
+      fl = true;
+      T dg;
+      const T Ax = applyRow<T, ZEROX>(L, code, c, i, j, k, X, dg);
+      T bv = b[c];
+      if (mb) bv = bv - T(*mb);
+      r = bv - Ax;
     }
   }
-  if (fold) blockToPartials<1>(acc, rb.partials, rb.stride, Q.blk);
+  rs[tid] = r;
+  fls[tid] = fl ? 1 : 0;
+  __syncthreads();
+  double acc[1] = {0.0};
+  if (tz == 0 && tx % Lc.fx == 0 && ty % Lc.fy == 0) {
+    const int I = i / Lc.fx, J = j / Lc.fy, K = k / Lc.fz;
+    if (I < Lc.nx && J < Lc.ny && K < Lc.nz) {
+      const int C = I + Lc.nx * (J + Lc.ny * K);
+      if (!(Lc.code[C] & kFluidBit)) {
+        rc[C] = T(0);
+      } else {
+        T s = T(0);
+        int cnt = 0;
+        for (int dk = 0; dk < Lc.fz; ++dk)
+          for (int dj = 0; dj < Lc.fy; ++dj)
+            for (int di = 0; di < Lc.fx; ++di) {
+              const int t = (tx + di) + bx * ((ty + dj) + by * dk);
+              if (fls[t]) {
+                s += rs[t];
+                ++cnt;
+              }
+            }
+        const T rv = s / T(cnt);
+        rc[C] = rv;
+        acc[0] = double(rv);
+      }
+    }
+  }
+  if (fold) blockToPartials<1>(acc, rb.partials, rb.stride, B.lin);
 }
 
 // ---------------------------------------------------------------------------
 // Coarsest level: Jacobi-PCG to 1e-6 in at most 500 iterations, as
 // CgPoisson(1e-6, 500).solve (multigrid.cpp:178-183, poisson.cpp:136-240).
-// One workgroup of 512 threads, up to 4 cells per thread: x, r, p, the
-// diagonal and its inverse live in registers, p is mirrored in LDS for the
-// stencil (absent neighbours point at a zero slot), and an iteration costs two
-// DPP block reductions and three barriers.  The coarse solve's dot products
-// are tree-ordered, so it reproduces the reference's CG to rounding, not bit
-// for bit; z = r * (1/diag) is used for the same reason.
+// One workgroup of 256 threads (one wavefront per SIMD: the per-wavefront
+// scalar work of an iteration is then done once per SIMD), up to 8 cells per
+// thread, every vector in
+// registers and the preconditioned residual u mirrored in LDS for the
+// stencil (absent neighbours read a zero slot).  The iteration is the
+// Chronopoulos-Gear form of PCG (s = A p carried by recurrence), which needs a
+// single fused reduction of (r.u, Au.u, r.r) and two barriers per iteration;
+// in exact arithmetic its iterates are the reference's.  The coarse solve's
+// dot products are tree-ordered, so it matches the reference's CG to rounding,
+// not bit for bit, and u = r * (1/diag) is used for the same reason.
 // ---------------------------------------------------------------------------
-constexpr int kCoarseT = 512;
-constexpr int kCPT = 4;
+constexpr int kCoarseT = 256;
+constexpr int kCPT = 8;
 constexpr int kCoarseMax = kCoarseT * kCPT;  // the hierarchy stops at <= 2048 cells
 
 template <int K>
@@ -234,29 +261,32 @@ __device__ __forceinline__ void coarseSum(double (&v)[K], double (*red)[kCoarseT
   }
 }
 
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return fmaf(a, b, c); }
+
 template <class T>
 __global__ void __launch_bounds__(kCoarseT) k_coarse_lds(MgLevel L, int singular, const T* __restrict__ b,
                                                          const double* mb, T* __restrict__ xout,
                                                          const SolveCtl* ctl) {
   if (ctl && ctl->done) return;
-  __shared__ T pl[kCoarseMax + 1];  // [kCoarseMax] stays 0
-  __shared__ double redA[1][kCoarseT / kWave];
-  __shared__ double redB[2][kCoarseT / kWave];
+  __shared__ T ul[kCoarseMax + 1];  // [kCoarseMax] stays 0
+  __shared__ double red1[1][kCoarseT / kWave];
+  __shared__ double red3[3][kCoarseT / kWave];
   const int tid = threadIdx.x;
   const int N = int(L.n);
   const T h2[3] = {T(L.h2[0]), T(L.h2[1]), T(L.h2[2])};
-  T x[kCPT], r[kCPT], p[kCPT], inv[kCPT], dsum[kCPT];
+  T x[kCPT], r[kCPT], p[kCPT], sv[kCPT], inv[kCPT], dsum[kCPT];
   int nb[kCPT][6];
   bool fl[kCPT], live[kCPT];
   const T bm = mb ? T(*mb) : T(0);
-  if (tid == 0) pl[kCoarseMax] = T(0);
+  if (tid == 0) ul[kCoarseMax] = T(0);
   double s0[1] = {0.0};
 #pragma unroll
   for (int q = 0; q < kCPT; ++q) {
     const int c = tid + q * kCoarseT;
     live[q] = c < N;
     fl[q] = false;
-    x[q] = r[q] = p[q] = dsum[q] = T(0);
+    x[q] = r[q] = p[q] = sv[q] = dsum[q] = T(0);
     inv[q] = T(1);
 #pragma unroll
     for (int d = 0; d < 6; ++d) nb[q][d] = kCoarseMax;
@@ -285,70 +315,70 @@ __global__ void __launch_bounds__(kCoarseT) k_coarse_lds(MgLevel L, int singular
     inv[q] = T(1) / (dg > T(0) ? dg : T(1));
   }
   if (singular && L.nf > 0) {
-    coarseSum<1>(s0, redA);
+    coarseSum<1>(s0, red1);
     const T m = T(s0[0] / double(L.nf));
 #pragma unroll
     for (int q = 0; q < kCPT; ++q) r[q] = fl[q] ? r[q] - m : T(0);
   }
-  double a2[2] = {0.0, 0.0};
-#pragma unroll
-  for (int q = 0; q < kCPT; ++q) {
-    if (!live[q]) continue;
-    p[q] = r[q] * inv[q];
-    pl[tid + q * kCoarseT] = p[q];
-    a2[0] += double(r[q]) * double(p[q]);
-    a2[1] += double(r[q]) * double(r[q]);
-  }
-  coarseSum<2>(a2, redB);
-  double rz = a2[0];
-  const double bn = sqrt(a2[1]) + 1e-300;
-  for (int it = 0; it < 500; ++it) {
-    T Ap[kCPT];
-    double sp[1] = {0.0};
+  double bn = 0.0, bn2 = 0.0, gammaOld = 1.0, alphaOld = 1.0;
+  for (int it = 0;; ++it) {
+    // u = M^-1 r to LDS; w = A u; (r.u, w.u, r.r)
+    T u[kCPT];
 #pragma unroll
     for (int q = 0; q < kCPT; ++q) {
-      Ap[q] = T(0);
-      if (!live[q]) continue;
-      if (fl[q]) {
-        const T off = h2[0] * (pl[nb[q][0]] + pl[nb[q][1]]) + h2[1] * (pl[nb[q][2]] + pl[nb[q][3]]) +
-                      h2[2] * (pl[nb[q][4]] + pl[nb[q][5]]);
-        Ap[q] = dsum[q] * p[q] - off;
-      } else {
-        Ap[q] = p[q];
-      }
-      sp[0] += double(p[q]) * double(Ap[q]);
-    }
-    coarseSum<1>(sp, redA);
-    if (fabs(sp[0]) < 1e-300) break;
-    const double alpha = rz / sp[0];
-    double rr[2] = {0.0, 0.0};
-#pragma unroll
-    for (int q = 0; q < kCPT; ++q) {
-      if (!live[q]) continue;
-      x[q] = x[q] + T(alpha) * p[q];
-      r[q] = r[q] - T(alpha) * Ap[q];
-      rr[0] += double(r[q]) * double(r[q]);
-      rr[1] += double(r[q]) * double(r[q] * inv[q]);
-    }
-    coarseSum<2>(rr, redB);
-    if (sqrt(rr[0]) <= 1e-6 * bn) break;
-    const double beta = rr[1] / rz;
-    rz = rr[1];
-#pragma unroll
-    for (int q = 0; q < kCPT; ++q) {
-      if (!live[q]) continue;
-      p[q] = r[q] * inv[q] + T(beta) * p[q];
-      pl[tid + q * kCoarseT] = p[q];
+      u[q] = r[q] * inv[q];
+      if (live[q]) ul[tid + q * kCoarseT] = u[q];
     }
     __syncthreads();
+    T w[kCPT];
+    double d3[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < kCPT; ++q) {
+      w[q] = T(0);
+      if (!live[q]) continue;
+      if (fl[q]) {
+        T off = h2[0] * (ul[nb[q][0]] + ul[nb[q][1]]);
+        off = fmaT(h2[1], ul[nb[q][2]] + ul[nb[q][3]], off);
+        off = fmaT(h2[2], ul[nb[q][4]] + ul[nb[q][5]], off);
+        w[q] = fmaT(dsum[q], u[q], -off);
+      } else {
+        w[q] = u[q];
+      }
+      d3[0] = fma(double(r[q]), double(u[q]), d3[0]);
+      d3[1] = fma(double(w[q]), double(u[q]), d3[1]);
+      d3[2] = fma(double(r[q]), double(r[q]), d3[2]);
+    }
+    coarseSum<3>(d3, red3);
+    if (it == 0) {
+      bn = sqrt(d3[2]) + 1e-300;
+      bn2 = (1e-6 * bn) * (1e-6 * bn);
+    } else if (d3[2] <= bn2) {
+      break;  // |r_it| <= 1e-6 |b| after update it
+    }
+    if (it == 500) break;
+    const double gamma = d3[0];
+    const double beta = it == 0 ? 0.0 : gamma / gammaOld;
+    const double den = it == 0 ? d3[1] : d3[1] - beta * gamma / alphaOld;  // = p.Ap
+    if (fabs(den) < 1e-300) break;
+    const double alpha = gamma / den;
+    const T a = T(alpha), be = T(beta);
+#pragma unroll
+    for (int q = 0; q < kCPT; ++q) {
+      p[q] = fmaT(be, p[q], u[q]);
+      sv[q] = fmaT(be, sv[q], w[q]);
+      x[q] = fmaT(a, p[q], x[q]);
+      r[q] = fmaT(-a, sv[q], r[q]);
+    }
+    gammaOld = gamma;
+    alphaOld = alpha;
   }
-  __syncthreads();  // a break right after a reduction leaves redA being read
+  __syncthreads();  // a break right after the reduction leaves red3 being read
   if (singular && L.nf > 0) {
     double sx[1] = {0.0};
 #pragma unroll
     for (int q = 0; q < kCPT; ++q)
       if (fl[q]) sx[0] += double(x[q]);
-    coarseSum<1>(sx, redA);
+    coarseSum<1>(sx, red1);
     const T m = T(sx[0] / double(L.nf));
 #pragma unroll
     for (int q = 0; q < kCPT; ++q)
@@ -357,6 +387,108 @@ __global__ void __launch_bounds__(kCoarseT) k_coarse_lds(MgLevel L, int singular
 #pragma unroll
   for (int q = 0; q < kCPT; ++q)
     if (live[q]) xout[tid + q * kCoarseT] = x[q];
+}
+
+// ---------------------------------------------------------------------------
+// Direct coarsest-level solve.  The level's operator (identity rows for solid
+// cells; on singular systems the fluid block plus alpha*11^T) is inverted once
+// by Gauss-Jordan elimination on the device, projected with the fluid
+// mean-removal P on both sides when singular (P A^+ P: the minimum-norm,
+// mean-free solution that CG from zero followed by removeMean reaches), and
+// each coarse solve is then one dense matrix-vector product.
+// ---------------------------------------------------------------------------
+// Step k of Gauss-Jordan on the n x 2n augmented matrix G (row-major): the
+// pivot row divided by the pivot, and column k, are copied out first so the
+// update kernel has no read/write overlap.
+__global__ void k_gj_copy(const double* __restrict__ G, int n, int k, double* __restrict__ colk,
+                          double* __restrict__ rowk, int* fail, double tiny) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = 2 * n;
+  const double piv = G[size_t(k) * w + k];
+  if (t == 0 && !(piv > tiny)) *fail = 1;
+  if (t < w) rowk[t] = G[size_t(k) * w + t] / piv;
+  if (t < n) colk[t] = G[size_t(t) * w + k];
+}
+
+__global__ void k_gj_update(double* __restrict__ G, int n, int k, const double* __restrict__ colk,
+                            const double* __restrict__ rowk) {
+  const int j = k + blockIdx.x * blockDim.x + threadIdx.x;  // columns left of k are settled
+  const int i = blockIdx.y;
+  const int w = 2 * n;
+  if (j >= w) return;
+  double* g = G + size_t(i) * w + j;
+  *g = i == k ? rowk[j] : *g - colk[i] * rowk[j];
+}
+
+// Row means of the inverse over fluid columns (one block per row), for P M P.
+__global__ void __launch_bounds__(kBlock) k_inv_rowmean(const double* __restrict__ G, int n,
+                                                        const uint16_t* __restrict__ code, long long nf,
+                                                        double* __restrict__ rm) {
+  const int i = blockIdx.x;
+  const double* row = G + size_t(i) * 2 * n + n;
+  double a[1] = {0.0};
+  for (int j = threadIdx.x; j < n; j += blockDim.x)
+    if (code[j] & kFluidBit) a[0] += row[j];
+  __shared__ double sh[16];
+  blockFold<1, 0u>(a, sh);
+  if (threadIdx.x == 0) rm[i] = (code[i] & kFluidBit) ? a[0] / double(nf) : 0.0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_inv_total(const double* __restrict__ rm, int n,
+                                                      const uint16_t* __restrict__ code, long long nf,
+                                                      double* __restrict__ tot) {
+  double a[1] = {0.0};
+  for (int j = threadIdx.x; j < n; j += blockDim.x)
+    if (code[j] & kFluidBit) a[0] += rm[j];
+  __shared__ double sh[16];
+  blockFold<1, 0u>(a, sh);
+  if (threadIdx.x == 0) *tot = a[0] / double(nf);
+}
+
+// Final coarse inverse (n x n, FP64 and FP32 copies) from the right half of G.
+__global__ void k_inv_store(const double* __restrict__ G, int n, const uint16_t* __restrict__ code, int singular,
+                            const double* __restrict__ rm, const double* __restrict__ tot,
+                            double* __restrict__ M, float* __restrict__ Mf) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= n) return;
+  double v = G[size_t(i) * 2 * n + n + j];
+  if (singular) {
+    const bool fi = code[i] & kFluidBit, fj = code[j] & kFluidBit;
+    v = (fi && fj) ? ((v - rm[i]) - rm[j]) + *tot : 0.0;
+  }
+  M[size_t(i) * n + j] = v;
+  Mf[size_t(i) * n + j] = float(v);
+}
+
+// x = Minv (b - mb on fluid cells): kDirectRows rows per block, each thread
+// striding the columns; fixed-order block fold.
+constexpr int kDirectRows = 4;
+template <class T, class MT>
+__global__ void __launch_bounds__(kBlock) k_coarse_direct(MgLevel L, const MT* __restrict__ M,
+                                                          const T* __restrict__ b, const double* mb,
+                                                          T* __restrict__ x, const SolveCtl* ctl) {
+  if (ctl && ctl->done) return;
+  const int n = int(L.n);
+  const int r0 = blockIdx.x * kDirectRows;
+  const double m = mb ? *mb : 0.0;
+  double acc[kDirectRows];
+#pragma unroll
+  for (int r = 0; r < kDirectRows; ++r) acc[r] = 0.0;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    T bj = b[j];
+    if (mb && (L.code[j] & kFluidBit)) bj = bj - T(m);
+    const double bv = double(bj);
+#pragma unroll
+    for (int r = 0; r < kDirectRows; ++r)
+      if (r0 + r < n) acc[r] = fma(double(M[size_t(r0 + r) * n + j]), bv, acc[r]);
+  }
+  __shared__ double sh[kDirectRows * 16];
+  blockFold<kDirectRows, 0u>(acc, sh);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int r = 0; r < kDirectRows; ++r)
+      if (r0 + r < n) x[r0 + r] = T(acc[r]);
 }
 
 // Same solve for coarsest levels above kCoarseMax cells (grids that stop
@@ -858,10 +990,21 @@ void launch_mg_sweep(const MgLevel& L, const T* b, const double* mb, const T* x,
   NBL_LAUNCH_CHECK();
 }
 
+Geo3 restrictGeo(const MgLevel& L, const MgLevel& Lc) {
+  const int bz = Lc.fz;
+  const int bx = L.nx >= 64 ? 64 : L.nx >= 32 ? 32 : 16;
+  const int by = kBlock / (bx * bz);
+  Geo3 g;
+  g.block = dim3(bx, by, bz);
+  g.grid = dim3((L.nx + bx - 1) / bx, (L.ny + by - 1) / by, (L.nz + bz - 1) / bz);
+  g.blocks = int(g.grid.x * g.grid.y * g.grid.z);
+  return g;
+}
+
 template <class T>
 void launch_mg_resid_restrict(const MgLevel& L, const MgLevel& Lc, const T* b, const double* mb, const T* x,
                               const double* mx, T* rc, bool fold, RedBuf rb, const SolveCtl* ctl, hipStream_t s) {
-  const Geo3 g = geo3(Lc.nx, Lc.ny, Lc.nz);
+  const Geo3 g = restrictGeo(L, Lc);
   k_resid_restrict<T, false><<<g.grid, g.block, 0, s>>>(L, Lc, b, mb, x, mx, rc, fold ? 1 : 0, rb, ctl);
   NBL_LAUNCH_CHECK();
 }
@@ -890,6 +1033,35 @@ void launch_to_level0(const MgLevel& L, const double* r, T* b, const SolveCtl* c
   NBL_LAUNCH_CHECK();
 }
 
+void launch_gj_step(double* G, int n, int k, double* colk, double* rowk, int* fail, double tiny, hipStream_t s) {
+  k_gj_copy<<<gridFor(2LL * n), kBlock, 0, s>>>(G, n, k, colk, rowk, fail, tiny);
+  const dim3 grid((2 * n - k + kBlock - 1) / kBlock, n, 1);
+  k_gj_update<<<grid, kBlock, 0, s>>>(G, n, k, colk, rowk);
+  NBL_LAUNCH_CHECK();
+}
+
+void launch_inv_store(const double* G, int n, const uint16_t* code, long long nf, bool singular, double* scratch,
+                      double* M, float* Mf, hipStream_t s) {
+  if (singular) {
+    k_inv_rowmean<<<n, kBlock, 0, s>>>(G, n, code, nf, scratch);
+    k_inv_total<<<1, kBlock, 0, s>>>(scratch, n, code, nf, scratch + n);
+  }
+  const dim3 grid((n + kBlock - 1) / kBlock, n, 1);
+  k_inv_store<<<grid, kBlock, 0, s>>>(G, n, code, singular ? 1 : 0, scratch, scratch + n, M, Mf);
+  NBL_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_mg_coarse_direct(const MgLevel& L, const double* M, const float* Mf, const T* b, const double* mb, T* x,
+                             const SolveCtl* ctl, hipStream_t s) {
+  const int blocks = int((L.n + kDirectRows - 1) / kDirectRows);
+  if constexpr (sizeof(T) == sizeof(float))
+    k_coarse_direct<T, float><<<blocks, kBlock, 0, s>>>(L, Mf, b, mb, x, ctl);
+  else
+    k_coarse_direct<T, double><<<blocks, kBlock, 0, s>>>(L, M, b, mb, x, ctl);
+  NBL_LAUNCH_CHECK();
+}
+
 #define NBL_INST(T)                                                                                          \
   template void launch_mg_sweep<T>(const MgLevel&, const T*, const double*, const T*, const double*, bool,    \
                                    const T*, const double*, const MgLevel*, T*, bool, RedBuf, const SolveCtl*, \
@@ -901,7 +1073,9 @@ void launch_to_level0(const MgLevel& L, const double* r, T* b, const SolveCtl* c
                                     const SolveCtl*, hipStream_t);                                           \
   template void launch_mg_finish<T>(const MgLevel&, const T*, const double*, double*, bool, RedBuf,          \
                                     const SolveCtl*, hipStream_t);                                           \
-  template void launch_to_level0<T>(const MgLevel&, const double*, T*, const SolveCtl*, hipStream_t);
+  template void launch_to_level0<T>(const MgLevel&, const double*, T*, const SolveCtl*, hipStream_t);        \
+  template void launch_mg_coarse_direct<T>(const MgLevel&, const double*, const float*, const T*,            \
+                                           const double*, T*, const SolveCtl*, hipStream_t);
 NBL_INST(double)
 NBL_INST(float)
 #undef NBL_INST
@@ -918,7 +1092,7 @@ void launch_diag(const MgLevel& L, double* out, hipStream_t s) {
 }
 void launch_restrict_plain(const MgLevel& L, const MgLevel& Lc, const double* rf, double* rc, hipStream_t s) {
   // rf - A*0 == rf exactly, so the fused kernel with a zero x restricts rf itself
-  const Geo3 g = geo3(Lc.nx, Lc.ny, Lc.nz);
+  const Geo3 g = restrictGeo(L, Lc);
   k_resid_restrict<double, true><<<g.grid, g.block, 0, s>>>(L, Lc, rf, nullptr, nullptr, nullptr, rc, 0,
                                                             RedBuf{nullptr, 0}, nullptr);
   NBL_LAUNCH_CHECK();

@@ -403,7 +403,8 @@ __global__ void __launch_bounds__(kBlock) k_rhs_rk(NsArgs A, PadBufs pb, const d
                                                    const StepCtl* __restrict__ sc) {
   const DevGrid& g = A.g;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  int comp = 0, k = blockIdx.z;
+  const Blk3 B = blk3();
+  int comp = 0, k = B.z;
   if (k >= nz) {
     k -= nz;
     comp = 1;
@@ -414,7 +415,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs_rk(NsArgs A, PadBufs pb, const d
   }
   int X, Y, Z;
   compDims(g, comp, X, Y, Z);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = B.x * blockDim.x + threadIdx.x, j = B.y * blockDim.y + threadIdx.y;
   if (i >= X || j >= Y) return;
   const PadRd r{pb.p[0] + pb.origin[0], pb.p[1] + pb.origin[1], pb.p[2] + pb.origin[2],
                 pb.sx[0], pb.sxy[0], pb.sx[1], pb.sxy[1], pb.sx[2], pb.sxy[2]};

@@ -189,6 +189,8 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
     dl_.push_back(m);
     blocks_.push_back(geo3(m.nx, m.ny, m.nz).blocks);
     stride = std::max(stride, blocks_.back());
+    rblocks_.push_back(l > 0 ? restrictGeo(dl_[l - 1], m).blocks : 0);
+    stride = std::max(stride, rblocks_.back());
     const size_t n = H.g.cells();
     lb_.emplace_back(ctx, l > 0 ? n * ts : 0);
     lx0_.emplace_back(ctx, kind == 0 ? n * ts : 0);
@@ -197,6 +199,8 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
   if (size_t(hl_[0].g.cells()) >= (size_t(1) << 31))
     throw std::invalid_argument("grid too large: cell indices must fit in 32 bits");
   if (kind == 0) coarseScr_ = DevBuf(ctx, 4 * hl_.back().g.cells() * ts);
+  if (opts.coarse != NBL_COARSE_DIRECT && opts.coarse != NBL_COARSE_CG)
+    throw std::invalid_argument("unknown coarse solve kind");
   const size_t n0 = hl_[0].g.cells();
   if (mixed_) b0f_ = DevBuf(ctx, n0 * sizeof(float));
   x_ = DevBuf(ctx, n0 * 8);
@@ -215,6 +219,76 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
   NBL_HIP(hipEventCreate(&e0_));
   NBL_HIP(hipEventCreate(&e1_));
   predicted_ = std::max(1, opts.check_every);
+  if (kind == 0 && nl > 1 && opts.coarse == NBL_COARSE_DIRECT) direct_ = buildDirectCoarse();
+}
+
+// Dense inverse of the coarsest level's operator (multigrid.cpp:178-183 runs
+// CgPoisson to 1e-6 there; an exact solve gives the same flexible-CG iteration
+// counts and agrees with it far inside the outer tolerance).  Rows follow
+// applyRow: fluid rows carry the 7-point couplings decoded from the level's
+// face codes (periodic wraps included), solid rows are identity.  A singular
+// system gets alpha*11^T on the fluid block and the inverse is projected
+// afterwards.  Returns false (CG is used instead) when a pivot vanishes, e.g.
+// for a fluid region the outlet does not reach.
+bool PoissonDev::buildDirectCoarse() {
+  const HostLevel& H = hl_.back();
+  const long long nn = (long long)H.g.cells();
+  if (nn > kDirectCoarseMax) return false;
+  const int n = int(nn);
+  const int dims[3] = {H.g.nx, H.g.ny, H.g.nz};
+  const double h2[3] = {1.0 / (H.g.dx * H.g.dx), 1.0 / (H.g.dy * H.g.dy), 1.0 / (H.g.dz * H.g.dz)};
+  const size_t w = size_t(2) * n;
+  std::vector<double> G(size_t(n) * w, 0.0);
+  double dmax = 0.0, dfl = 0.0;
+  for (int c = 0; c < n; ++c) {
+    double* row = G.data() + size_t(c) * w;
+    row[n + c] = 1.0;
+    const uint16_t code = H.code[size_t(c)];
+    if (!(code & kFluidBit)) {
+      row[c] = 1.0;
+      continue;
+    }
+    const int q0[3] = {c % dims[0], (c / dims[0]) % dims[1], c / (dims[0] * dims[1])};
+    double d = 0.0;
+    for (int dir = 0; dir < 6; ++dir) {
+      const int kd = kindOf(code, dir), a = dir >> 1;
+      if (kd == 2) d += 2.0 * h2[a];
+      if (kd != 1) continue;
+      d += h2[a];
+      int q[3] = {q0[0], q0[1], q0[2]};
+      q[a] += (dir & 1) ? 1 : -1;
+      q[a] = (q[a] + dims[a]) % dims[a];
+      row[q[0] + dims[0] * (q[1] + dims[1] * q[2])] -= h2[a];
+    }
+    row[c] += d;
+    dmax = std::max(dmax, d);
+    dfl += d;
+  }
+  if (project_) {
+    const double alpha = dfl / double(H.nf) / double(H.nf);
+    for (int i = 0; i < n; ++i) {
+      if (!(H.code[size_t(i)] & kFluidBit)) continue;
+      double* row = G.data() + size_t(i) * w;
+      for (int j = 0; j < n; ++j)
+        if (H.code[size_t(j)] & kFluidBit) row[j] += alpha;
+    }
+  }
+  hipStream_t s = nullptr;
+  DevBuf g(ctx_, G.size() * sizeof(double)), col(ctx_, size_t(n) * 8), rowk(ctx_, w * 8), fail(ctx_, 256);
+  NBL_HIP(hipMemcpy(g.as<void>(), G.data(), G.size() * sizeof(double), hipMemcpyHostToDevice));
+  NBL_HIP(hipMemset(fail.as<void>(), 0, 256));
+  const double tiny = 1e-9 * std::max(dmax, 1.0);
+  for (int k = 0; k < n; ++k) launch_gj_step(g.as<double>(), n, k, col.as<double>(), rowk.as<double>(), fail.as<int>(), tiny, s);
+  int failed = 0;
+  NBL_HIP(hipMemcpy(&failed, fail.as<void>(), sizeof(int), hipMemcpyDeviceToHost));
+  if (failed) return false;
+  coarseM_ = DevBuf(ctx_, size_t(n) * n * sizeof(double));
+  coarseMf_ = DevBuf(ctx_, size_t(n) * n * sizeof(float));
+  DevBuf scratch(ctx_, size_t(n + 1) * sizeof(double));
+  launch_inv_store(g.as<double>(), n, dl_.back().code, H.nf, project_, scratch.as<double>(), coarseM_.as<double>(),
+                   coarseMf_.as<float>(), s);
+  NBL_HIP(hipDeviceSynchronize());
+  return true;
 }
 
 PoissonDev::~PoissonDev() {
@@ -248,7 +322,10 @@ void PoissonDev::vcycle(int l, const T* b, const double* mb, const T** xout, con
   const RedBuf rb = red();
   if (l + 1 == nlevels()) {
     T* x = lx0_[l].as<T>();
-    launch_mg_coarse<T>(L, project_, b, mb, x, coarseScr_.as<T>(), ctl, s);
+    if (direct_)
+      launch_mg_coarse_direct<T>(L, coarseM_.as<double>(), coarseMf_.as<float>(), b, mb, x, ctl, s);
+    else
+      launch_mg_coarse<T>(L, project_, b, mb, x, coarseScr_.as<T>(), ctl, s);
     *xout = x;
     *mxout = nullptr;
     return;
@@ -268,7 +345,7 @@ void PoissonDev::vcycle(int l, const T* b, const double* mb, const T** xout, con
   T* bc = lb_[l + 1].as<T>();
   double* mbc = project_ ? slotPtr(ctl, slotB(l + 1)) : nullptr;
   launch_mg_resid_restrict<T>(L, Lc, b, mb, bufs[cur], ms[cur], bc, project_, rb, ctl, s);
-  if (project_) fin(FIN_MEAN, 1, blocks_[l + 1], slotB(l + 1), Lc.nf, true, s);
+  if (project_) fin(FIN_MEAN, 1, rblocks_[l + 1], slotB(l + 1), Lc.nf, true, s);
   const T* ec = nullptr;
   const double* mec = nullptr;
   vcycle<T>(l + 1, bc, mbc, &ec, &mec, s);
@@ -403,10 +480,10 @@ nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t
 }
 
 void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double* bytes) {
-  if (kernel < 0 || kernel > 2) throw std::invalid_argument("profile: kernel must be 0, 1 or 2");
+  if (kernel < 0 || kernel > 3) throw std::invalid_argument("profile: kernel must be 0, 1, 2 or 3");
   if (reps < 1) throw std::invalid_argument("profile: reps must be >= 1");
   if (kind_ != 0) throw std::invalid_argument("profile: mg-pcg solver only");
-  if (kernel == 2 && nlevels() < 2) throw std::invalid_argument("profile: single-level grid");
+  if ((kernel == 2 || kernel == 3) && nlevels() < 2) throw std::invalid_argument("profile: single-level grid");
   const MgLevel& L = dl_[0];
   const double n = double(L.n);
   const double sz = mixed_ ? 4.0 : 8.0;
@@ -435,6 +512,17 @@ void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double
                                 (q & 1) ? lx0_[0].as<double>() : lx1_[0].as<double>(), project_, rb, ctl, s);
     } else if (kernel == 1) {
       launch_fcg_apply(L, p_.as<double>(), Ap_.as<double>(), ctl, rb, s);
+    } else if (kernel == 3) {
+      const int lc = nlevels() - 1;
+      if (mixed_) {
+        const float* xo = nullptr;
+        const double* mo = nullptr;
+        vcycle<float>(lc, lb_[lc].as<float>(), nullptr, &xo, &mo, s);
+      } else {
+        const double* xo = nullptr;
+        const double* mo = nullptr;
+        vcycle<double>(lc, lb_[lc].as<double>(), nullptr, &xo, &mo, s);
+      }
     } else {
       if (mixed_)
         launch_mg_resid_restrict<float>(L, dl_[1], b0f_.as<float>(), nullptr, lx0_[0].as<float>(), m0,
@@ -453,8 +541,10 @@ void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double
   NBL_HIP(hipEventElapsedTime(&t, e0_, e1_));
   *ms = double(t) / reps;
   // algorithmic bytes: every input and output vector once, plus the 2-byte code
+  const double nc = double(dl_.back().n);
   if (kernel == 0) *bytes = n * (3.0 * sz + 2.0);
   else if (kernel == 1) *bytes = n * (8.0 + 8.0 + 2.0);
+  else if (kernel == 3) *bytes = direct_ ? nc * nc * sz + nc * (2.0 * sz + 2.0) : nc * (2.0 * sz + 2.0);
   else *bytes = n * (2.0 * sz + 2.0) + double(dl_[1].n) * (sz + 2.0);
 }
 

@@ -87,6 +87,7 @@ class PoissonDev {
   const std::vector<HostLevel>& levels() const { return hl_; }
   const MgLevel& level(int l) const { return dl_[l]; }
   int nlevels() const { return int(dl_.size()); }
+  bool directCoarse() const { return direct_; }
   bool singular() const { return singular_; }
   int kind() const { return kind_; }
   double lastMs() const { return lastMs_; }
@@ -101,6 +102,7 @@ class PoissonDev {
   template <class T>
   void vcycle(int l, const T* b, const double* mb, const T** xout, const double** mxout, hipStream_t s);
   void minv(const double* r, hipStream_t s);
+  bool buildDirectCoarse();
   void fin(int op, int K, int nparts, int slot, long long nf, bool predicated, hipStream_t s, int earlyExit = 0,
            double convTol = 0.0);
   void enqueueIteration(hipStream_t s);
@@ -116,11 +118,15 @@ class PoissonDev {
   bool mixed_ = false;
   std::vector<HostLevel> hl_;
   std::vector<MgLevel> dl_;
-  std::vector<int> blocks_;  // geo3 block count per level (= partials per reduction)
+  std::vector<int> blocks_;   // geo3 block count per level (= partials per reduction)
+  std::vector<int> rblocks_;  // blocks of the restriction into level l (l >= 1)
   std::vector<DevBuf> codes_;
   // per level: b (levels >= 1), two x buffers, coarse scratch
   std::vector<DevBuf> lb_, lx0_, lx1_;
   DevBuf coarseScr_;
+  static constexpr long long kDirectCoarseMax = 4096;  // cells of a coarsest level solved directly
+  bool direct_ = false;
+  DevBuf coarseM_, coarseMf_;  // coarsest-level inverse (FP64, FP32)
   DevBuf b0f_;  // FP32 level-0 right-hand side (mixed)
   // flexible-CG vectors (FP64)
   DevBuf x_, rhs_, r_, rn_, z_, p_, Ap_;

@@ -23,6 +23,7 @@ BC_WALL, BC_INLET, BC_OUTLET, BC_PERIODIC = 0, 1, 2, 3
 INLET_UNIFORM, INLET_PARABOLIC = 0, 1
 CONV_CENTRAL, CONV_WENO5 = 0, 1
 PREC_FP64, PREC_MIXED, PREC_MIXED_BF16 = 0, 1, 2
+COARSE_DIRECT, COARSE_CG = 0, 1
 
 
 class FaceBC(C.Structure):
@@ -47,7 +48,7 @@ class FlowCfg(C.Structure):
 
 class SolverOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("pre_smooth", C.c_int), ("post_smooth", C.c_int),
-                ("check_every", C.c_int)]
+                ("check_every", C.c_int), ("coarse", C.c_int)]
 
 
 class StepReport(C.Structure):
@@ -136,9 +137,10 @@ def flow_cfg(**kw) -> FlowCfg:
     return c
 
 
-def solver_opts(precision=PREC_FP64, pre=2, post=2, check_every=1) -> SolverOpts:
+def solver_opts(precision=PREC_FP64, pre=2, post=2, check_every=1, coarse=None) -> SolverOpts:
     o = SolverOpts()
     o.precision, o.pre_smooth, o.post_smooth, o.check_every = precision, pre, post, check_every
+    o.coarse = COARSE_DIRECT if coarse is None else coarse
     return o
 
 

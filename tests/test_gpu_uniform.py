@@ -72,10 +72,11 @@ def test_operator_pieces_bitwise(ctx, orc, name, g):
         ps.close()
 
 
+@pytest.mark.parametrize("coarse", [nbl.COARSE_DIRECT, nbl.COARSE_CG], ids=["direct", "cg"])
 @pytest.mark.parametrize("name,g", GRIDS, ids=IDS)
-def test_mg_pcg_matches_oracle(ctx, orc, name, g):
+def test_mg_pcg_matches_oracle(ctx, orc, name, g, coarse):
     b = cases.poisson_rhs(g, 11)
-    ps = nbl.Poisson(ctx, g, kind=0, tol=1e-10, max_iters=100)
+    ps = nbl.Poisson(ctx, g, kind=0, tol=1e-10, max_iters=100, opts=nbl.solver_opts(coarse=coarse))
     try:
         x, rep = ps.solve(b)
         xo, ro = orc.mg(g, b, 1e-10, 100)
@@ -88,6 +89,24 @@ def test_mg_pcg_matches_oracle(ctx, orc, name, g):
         assert np.array_equal(x2, x) and rep2.iterations == rep.iterations
     finally:
         ps.close()
+
+
+@pytest.mark.parametrize("name", ["cube192", "tgv64"])
+def test_direct_coarse_matches_cg_coarse(ctx, name):
+    """The exact coarsest-level solve and the reference's CG(1e-6, 500) there
+    give the same flexible-CG iteration counts and solutions that agree far
+    inside the outer tolerance (1e-8 solve, agreement <= 1e-11)."""
+    g = cases.cube_channel()[0] if name == "cube192" else cases.taylor_green(64)[0]
+    out = []
+    for coarse in (nbl.COARSE_DIRECT, nbl.COARSE_CG):
+        ps = nbl.Poisson(ctx, g, kind=0, tol=1e-8, max_iters=100, opts=nbl.solver_opts(coarse=coarse))
+        try:
+            out.append([ps.solve(cases.poisson_rhs(g, s)) for s in (1, 2, 3)])
+        finally:
+            ps.close()
+    for (xd, rd), (xc, rc) in zip(*out):
+        assert rd.converged and rc.converged and rd.iterations == rc.iterations
+        assert rel(xd, xc) < 1e-11
 
 
 @pytest.mark.parametrize("name,g", GRIDS[:4], ids=IDS[:4])
