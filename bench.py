@@ -90,6 +90,69 @@ def coarse_solve(ctx, g, reps=50):
     return out
 
 
+def octree_c3(ctx, with_cpu=True):
+    """The cube-wake octree (4,294,942 leaves, levels 5..8, the cube's leaves
+    InsideSolid): device setup (sort, validation, face table, operator), the
+    SpMV roofline, and a Jacobi-PCG solve to 1e-9 of a seeded mean-free
+    right-hand side (OctreePoisson::solve).  CPU side: the reference's
+    faceNeighbors sweep over all leaves and one applyA, on one core."""
+    t0 = time.perf_counter()
+    t = nbl.build_tree(cases.C3_ORIGIN, cases.C3_EXTENT, 8, 5, cases.c3_predicate)
+    k, l, c = t.leaves()
+    host_build_s = time.perf_counter() - t0
+    m = cases.c3_solid(c)
+    d = nbl.Octree(ctx, k, l, m, cases.C3_ORIGIN, cases.C3_EXTENT, 8)
+    setup_ms, _ = d.last_timing()
+    n, nnz = d.counts()
+    ms, by = d.profile(0, 50)
+    fl = m == 0
+    b = cases.normal(11, k.size)
+    b[~fl] = 0.0
+    b[fl] -= b[fl].mean()
+    _x, rep = d.solve(b, 1e-9, 20000)
+    _, solve_ms = d.last_timing()
+    d.close()
+    ach = by / (ms * 1e-3) / 1e9
+    out = {"leaves": n, "couplings": nnz, "host_build_s": round(host_build_s, 3),
+           "device_setup_ms": round(setup_ms, 3),
+           "jacobi_pcg": {"tol": 1e-9, "iterations": rep.iterations, "residual": rep.residual,
+                          "converged": bool(rep.converged), "solve_ms": round(solve_ms, 3),
+                          "ms_per_iteration": round(solve_ms / max(rep.iterations, 1), 5)},
+           "spmv_roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": round(ach / HBM_PEAK_GBS, 4), "avg_launch_us": round(ms * 1e3, 3),
+                             "bytes_per_launch": by}}
+    log(f"octree: {n} leaves, setup {setup_ms:.1f} ms, {rep.iterations} iterations in {solve_ms:.1f} ms")
+    if with_cpu:
+        import oracle_py
+        kind = "reference" if oracle_py.available("ref") else "port"
+        lib = oracle_py.OracleLib("ref" if kind == "reference" else "orc")
+        rt = lib.octree(cases.C3_ORIGIN, cases.C3_EXTENT, 8)
+        rt.refine_uniform(5)
+        while True:
+            rk, rl, _m = rt.dump()
+            pred = cases.c3_predicate(rt.centers()) & (rl < 8)
+            if not pred.any():
+                break
+            rt.refine_flagged_pass(rk[pred], rl[pred])
+        rk, _rl, _m = rt.dump()
+        for i in np.nonzero(cases.c3_solid(rt.centers()))[0]:
+            rt.set_mask(int(i), 2)
+        t1 = time.perf_counter()
+        rt.face_table()
+        conn_s = time.perf_counter() - t1
+        xr = cases.normal(3, rt.n)
+        t1 = time.perf_counter()
+        rt.apply(xr)
+        apply_s = time.perf_counter() - t1
+        rt.close()
+        out["cpu_baseline"] = {"kind": kind, "cores": 1, "connectivity_s": round(conn_s, 3),
+                               "applyA_s": round(apply_s, 3),
+                               "est_solve_s": round(apply_s * rep.iterations, 1),
+                               "sample": "faceNeighbors over every leaf and one OctreePoisson::applyA on the same "
+                                         "tree; est_solve_s = applyA_s x the device iteration count"}
+    return out
+
+
 def cpu_baseline(g, cfg, st, steps):
     import oracle_py
     kind = "reference" if oracle_py.available("ref") else "port"
@@ -124,8 +187,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-octree", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the timed FP64 steps (for profiler runs)")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the roofline kernel loops and a 256 MiB copy (for counter runs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -135,6 +201,12 @@ def main():
     g, cfg = cases.cube_channel()
     st = cases.cube_channel_state(g)
     ctx = nbl.Ctx(0)
+    if a.roofline_only:
+        out = {"roofline": roofline(ctx, g), "roofline_256cubed": roofline(ctx, nbl.Grid(256, 256, 256, 1.0 / 256), 50),
+               "copy_256MiB_ms": ctx.probe_copy(256 << 20, 20)}
+        ctx.close()
+        print(json.dumps(out), flush=True)
+        return
     log(f"workload 192x64x64 cube channel, {g.cells} cells, warmup {a.warmup}, steps {a.steps}")
     reps, solve_ms, iters, el, state = run_steps(ctx, g, cfg, st, nbl.PREC_FP64, a.warmup, a.steps)
     ms_step = el / a.steps * 1e3
@@ -177,6 +249,8 @@ def main():
     out["coarse_solve"] = coarse_solve(ctx, g)
     big = nbl.Grid(256, 256, 256, 1.0 / 256)
     out["roofline_256cubed"] = roofline(ctx, big, reps=50)
+    if not a.no_octree:
+        out["octree_c3"] = octree_c3(ctx, with_cpu=not a.no_cpu)
     ctx.close()
     if not a.no_cpu and a.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(g, cfg, st, a.cpu_steps)

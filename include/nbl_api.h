@@ -152,6 +152,10 @@ int nbl_ctx_create(int device, nbl_ctx** out);
 int nbl_ctx_destroy(nbl_ctx* ctx);
 /* HBM bytes currently held by runs / solvers / octrees of this context. */
 int nbl_ctx_bytes_in_use(nbl_ctx* ctx, uint64_t* bytes);
+/* No counterpart in the reference: measurement.  Average time (ms) of one
+ * device-to-device copy of `bytes` (a plain copy kernel of known traffic, used
+ * to calibrate hardware byte counters). */
+int nbl_ctx_probe_copy(nbl_ctx* ctx, uint64_t bytes, int reps, double* ms);
 
 /* ---- MAC Navier-Stokes run (NSSolver, ns_solver.hpp:58-89) ----------------
  * State stays in HBM between calls.  Host arrays use the reference's face
@@ -272,6 +276,13 @@ int nbl_octree_diagonal(nbl_octree* oc, double* diag);
 /* OctreePoisson::solve (octree_poisson.cpp:107-197): Jacobi-PCG, singular. */
 int nbl_octree_poisson_solve(nbl_octree* oc, const double* b, double* x, double tol,
                              int max_iters, nbl_poisson_report* rep);
+/* No counterpart in the reference: measurement.  Average ms and algorithmic
+ * bytes of one launch of an octree kernel (kernel 0: the SpMV, reading row
+ * pointers, masks, x and writing y once per leaf plus a column index and a
+ * coefficient per coupling), and the device time of construction (sort,
+ * validation, face table, operator) and of the last poisson_solve. */
+int nbl_octree_profile(nbl_octree* oc, int kernel, int reps, double* ms, double* bytes);
+int nbl_octree_last_timing(nbl_octree* oc, double* setup_ms, double* solve_ms);
 
 #ifdef __cplusplus
 }

@@ -106,6 +106,32 @@ int nbl_ctx_bytes_in_use(nbl_ctx* ctx, uint64_t* bytes) {
   });
 }
 
+int nbl_ctx_probe_copy(nbl_ctx* ctx, uint64_t bytes, int reps, double* ms) {
+  return guarded([&] {
+    need(ctx && ms && bytes > 0 && reps > 0, "bad probe arguments");
+    useDevice(ctx);
+    nbl::DevBuf a(ctx, bytes), b(ctx, bytes);
+    hipStream_t s;
+    NBL_HIP(hipStreamCreate(&s));
+    hipEvent_t e0, e1;
+    NBL_HIP(hipEventCreate(&e0));
+    NBL_HIP(hipEventCreate(&e1));
+    NBL_HIP(hipMemsetAsync(a.as<void>(), 0, bytes, s));
+    NBL_HIP(hipMemcpyAsync(b.as<void>(), a.as<void>(), bytes, hipMemcpyDeviceToDevice, s));
+    NBL_HIP(hipEventRecord(e0, s));
+    for (int q = 0; q < reps; ++q)
+      NBL_HIP(hipMemcpyAsync(b.as<void>(), a.as<void>(), bytes, hipMemcpyDeviceToDevice, s));
+    NBL_HIP(hipEventRecord(e1, s));
+    NBL_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    NBL_HIP(hipEventElapsedTime(&t, e0, e1));
+    *ms = double(t) / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+  });
+}
+
 // ---- runs -----------------------------------------------------------------
 int nbl_run_create(nbl_ctx* ctx, const nbl_grid_desc* grid, const nbl_flow_cfg* cfg,
                    const nbl_solver_opts* opts, nbl_run** out) {
@@ -478,6 +504,22 @@ int nbl_octree_poisson_solve(nbl_octree* oc, const double* b, double* x, double 
     useDevice(oc->ctx);
     const nbl_poisson_report r = oc->impl->solve(b, x, tol, max_iters);
     if (rep) *rep = r;
+  });
+}
+
+int nbl_octree_profile(nbl_octree* oc, int kernel, int reps, double* ms, double* bytes) {
+  return guarded([&] {
+    need(oc && ms && bytes, "NULL argument");
+    useDevice(oc->ctx);
+    oc->impl->profile(kernel, reps, ms, bytes);
+  });
+}
+
+int nbl_octree_last_timing(nbl_octree* oc, double* setup_ms, double* solve_ms) {
+  return guarded([&] {
+    need(oc, "NULL argument");
+    if (setup_ms) *setup_ms = oc->impl->setupMs();
+    if (solve_ms) *solve_ms = oc->impl->lastSolveMs();
   });
 }
 

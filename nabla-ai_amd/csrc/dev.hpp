@@ -158,6 +158,10 @@ template <class T>
 void launch_mg_sweep(const MgLevel& L, const T* b, const double* mb, const T* x, const double* mx,
                      bool zeroGuess, const T* ec, const double* mec, const MgLevel* Lc, T* xo, bool fold,
                      RedBuf rb, const SolveCtl* ctl, hipStream_t s);
+// The first two sweeps from a zero guess in one pass (no projection between
+// them: non-singular systems only).
+template <class T>
+void launch_mg_sweep_pair(const MgLevel& L, const T* b, T* xo, const SolveCtl* ctl, hipStream_t s);
 template <class T>
 void launch_mg_resid_restrict(const MgLevel& L, const MgLevel& Lc, const T* b, const double* mb,
                               const T* x, const double* mx, T* rc, bool fold, RedBuf rb,
@@ -203,6 +207,10 @@ void launch_fcg_update(const MgLevel& L, bool singular, double* x, const double*
 void launch_fcg_norm(const MgLevel& L, double* rn, const SolveCtl* ctl, RedBuf rb, hipStream_t s);
 void launch_fcg_dots(const MgLevel& L, double* z, const double* mz, const double* rn, const double* r,
                      const SolveCtl* ctl, RedBuf rb, hipStream_t s);
+// k_finish fused into k_fcg_dots (no pending means): z = V-cycle output x.
+template <class T>
+void launch_fcg_dots_x(const MgLevel& L, const T* x, double* z, const double* rn, const double* r,
+                       const SolveCtl* ctl, RedBuf rb, hipStream_t s);
 void launch_fcg_pupdate(const MgLevel& L, double* p, const double* z, double* r, const double* rn,
                         const SolveCtl* ctl, hipStream_t s);
 void launch_fluid_sum(const MgLevel& L, const double* x, RedBuf rb, hipStream_t s);

@@ -68,21 +68,22 @@ __device__ __forceinline__ int nbOf(const MgLevel& L, int c, int i, int j, int k
 }
 
 // Value of x at a fluid cell as the reference holds it at that point of the
-// cycle: x - pending mean (+ the parent's coarse correction after prolongation).
-template <class T>
+// cycle: x - pending mean (+ the parent's coarse correction after prolongation,
+// PROLONG; the parent of (i,j,k) is (i>>sx, j>>sy, k>>sz), factors being 1 or 2).
+template <class T, bool PROLONG = false>
 struct XView {
   const T* __restrict__ x;
   T m;
   bool hasM;
-  const T* __restrict__ ec;  // non-null: add the parent's correction ec[parent] - em
-  int ncx, ncy, fx, fy, fz;
+  const T* __restrict__ ec;  // PROLONG: add the parent's correction ec[parent] - em
+  int ncx, ncy, sx, sy, sz;
   T em;
   bool hasEm;
   __device__ __forceinline__ T at(int c, int i, int j, int k) const {
     T v = x[c];
     if (hasM) v = v - m;
-    if (ec) {
-      T e = ec[(i / fx) + ncx * ((j / fy) + ncy * (k / fz))];
+    if constexpr (PROLONG) {
+      T e = ec[(i >> sx) + ncx * ((j >> sy) + ncy * (k >> sz))];
       if (hasEm) e = e - em;
       v = v + e;
     }
@@ -92,15 +93,29 @@ struct XView {
 
 template <class T>
 __device__ __forceinline__ XView<T> plainView(const T* x) {
-  return XView<T>{x, T(0), false, nullptr, 1, 1, 1, 1, 1, T(0), false};
+  return XView<T>{x, T(0), false, nullptr, 1, 1, 0, 0, 0, T(0), false};
 }
 
 // (A x)_c for a fluid cell and the diagonal sum dg (poisson.cpp:102-134): the
-// couplings are accumulated in the reference's face order.
-template <class T, bool ZERO>
-__device__ __forceinline__ T applyRow(const MgLevel& L, uint16_t code, int c, int i, int j, int k,
-                                      const XView<T>& X, T& dg) {
+// couplings are accumulated in the reference's face order.  The six neighbour
+// values are read before the face kinds are decoded (every neighbour index is
+// in range: nbOf wraps), so a wavefront issues all its stencil loads in one
+// round trip instead of waiting for the face code first.
+template <class T, bool ZERO, class XV>
+__device__ __forceinline__ T applyRow(const MgLevel& L, uint16_t code, int c, int i, int j, int k, const XV& X,
+                                      T& dg) {
   const T h2[3] = {T(L.h2[0]), T(L.h2[1]), T(L.h2[2])};
+  T xn[6];
+  T xc = T(0);
+  if constexpr (!ZERO) {
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir) {
+      int ii, jj, kk;
+      const int nb = nbOf(L, c, i, j, k, dir, ii, jj, kk);
+      xn[dir] = X.at(nb, ii, jj, kk);
+    }
+    xc = X.at(c, i, j, k);
+  }
   T d = T(0), off = T(0);
 #pragma unroll
   for (int dir = 0; dir < 6; ++dir) {
@@ -108,37 +123,52 @@ __device__ __forceinline__ T applyRow(const MgLevel& L, uint16_t code, int c, in
     const int a = dir >> 1;
     if (kd == 1) {
       d += h2[a];
-      if (!ZERO) {
-        int ii, jj, kk;
-        const int nb = nbOf(L, c, i, j, k, dir, ii, jj, kk);
-        off += h2[a] * X.at(nb, ii, jj, kk);
-      } else {
-        off += h2[a] * T(0);
-      }
+      off += h2[a] * (ZERO ? T(0) : xn[dir]);
     } else if (kd == 2) {
       d += T(2) * h2[a];
     }
   }
   dg = d;
-  const T xc = ZERO ? T(0) : X.at(c, i, j, k);
   return d * xc - off;
 }
 
 // ---------------------------------------------------------------------------
 // Smoother: x <- x + 0.8 (b - A x)/diag on fluid cells, 0 elsewhere
 // (multigrid.cpp:102-114), with the fused prolongation of the parent
-// correction (multigrid.cpp:158-173) when `ec` is given, and per-block fluid
-// sums of the result for the projection that follows a sweep on singular
-// systems.  Pending means (`mb`, `mx`, `mec`) are device slots, read here.
+// correction (multigrid.cpp:158-173) in SW_PROLONG, and per-block fluid sums
+// of the result for the projection that follows a sweep on singular systems.
+// Pending means (`mb`, `mx`, `mec`) are device slots, read here.
+// SW_PAIR runs the first two sweeps from a zero guess in one pass (systems
+// without the projection between sweeps): the first sweep is pointwise,
+// x1 = 0 + 0.8 (b - 0)/diag, so each cell re-evaluates it at its stencil points.
 // ---------------------------------------------------------------------------
-template <class T, bool ZERO>
+enum SweepMode : int { SW_PLAIN = 0, SW_ZERO = 1, SW_PROLONG = 2, SW_PAIR = 3 };
+
+// First sweep from zero at one cell, in the expression order of k_sweep.
+template <class T>
+__device__ __forceinline__ T firstSweep(const MgLevel& L, uint16_t code, int c, int i, int j, int k, T bv) {
+  if (!(code & kFluidBit)) return T(0);
+  T dg;
+  const T Ax = applyRow<T, true>(L, code, c, i, j, k, plainView<T>(nullptr), dg);
+  const T diag = dg > T(0) ? dg : T(1);
+  return T(0) + T(0.8) * (bv - Ax) / diag;
+}
+
+template <class T>
+struct FirstSweepView {
+  const MgLevel* L;
+  const T* __restrict__ b;
+  __device__ __forceinline__ T at(int c, int i, int j, int k) const {
+    return firstSweep<T>(*L, L->code[c], c, i, j, k, b[c]);
+  }
+};
+
+template <class T, int MODE>
 __global__ void __launch_bounds__(kBlock) k_sweep(MgLevel L, const T* __restrict__ b, const double* mb,
                                                   const T* __restrict__ x, const double* mx, const T* ec,
-                                                  const double* mec, int ncx, int ncy, int fx, int fy, int fz,
+                                                  const double* mec, int ncx, int ncy, int sx, int sy, int sz,
                                                   T* __restrict__ xo, int fold, RedBuf rb, const SolveCtl* ctl) {
   if (ctl && ctl->done) return;
-  const XView<T> X{x, mx ? T(*mx) : T(0), mx != nullptr, ec, ncx, ncy, fx, fy, fz, mec ? T(*mec) : T(0),
-                   mec != nullptr};
   const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
   if (q.valid) {
@@ -146,9 +176,21 @@ __global__ void __launch_bounds__(kBlock) k_sweep(MgLevel L, const T* __restrict
     if (!(code & kFluidBit)) {
       xo[q.c] = T(0);
     } else {
-      T dg;
-      const T Ax = applyRow<T, ZERO>(L, code, q.c, q.i, q.j, q.k, X, dg);
-      const T xc = ZERO ? T(0) : X.at(q.c, q.i, q.j, q.k);
+      T dg, Ax, xc;
+      if constexpr (MODE == SW_PAIR) {
+        const FirstSweepView<T> X{&L, b};
+        Ax = applyRow<T, false>(L, code, q.c, q.i, q.j, q.k, X, dg);
+        xc = X.at(q.c, q.i, q.j, q.k);
+      } else if constexpr (MODE == SW_PROLONG) {
+        const XView<T, true> X{x, mx ? T(*mx) : T(0), mx != nullptr, ec, ncx, ncy, sx, sy, sz,
+                               mec ? T(*mec) : T(0), mec != nullptr};
+        Ax = applyRow<T, false>(L, code, q.c, q.i, q.j, q.k, X, dg);
+        xc = X.at(q.c, q.i, q.j, q.k);
+      } else {
+        const XView<T> X{x, mx ? T(*mx) : T(0), mx != nullptr, nullptr, 1, 1, 0, 0, 0, T(0), false};
+        Ax = applyRow<T, MODE == SW_ZERO>(L, code, q.c, q.i, q.j, q.k, X, dg);
+        xc = MODE == SW_ZERO ? T(0) : X.at(q.c, q.i, q.j, q.k);
+      }
       const T diag = dg > T(0) ? dg : T(1);
       T bv = b[q.c];
       if (mb) bv = bv - T(*mb);
@@ -174,7 +216,7 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc
   if (ctl && ctl->done) return;
   __shared__ T rs[kBlock];
   __shared__ unsigned char fls[kBlock];
-  const XView<T> X{x, mx ? T(*mx) : T(0), mx != nullptr, nullptr, 1, 1, 1, 1, 1, T(0), false};
+  const XView<T> X{x, mx ? T(*mx) : T(0), mx != nullptr, nullptr, 1, 1, 0, 0, 0, T(0), false};
   const Blk3 B = blk3();
   const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
   const int bx = blockDim.x, by = blockDim.y;
@@ -792,6 +834,24 @@ __global__ void __launch_bounds__(kBlock) k_fcg_dots(MgLevel L, double* __restri
   blockToPartials<2>(acc, rb.partials, rb.stride, q.blk);
 }
 
+// k_finish + k_fcg_dots in one pass for systems without projections: z is the
+// V-cycle output on fluid cells (converted to FP64), 0 elsewhere.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_fcg_dots_x(MgLevel L, const T* __restrict__ x, double* __restrict__ z,
+                                                       const double* __restrict__ rn, const double* __restrict__ r,
+                                                       const SolveCtl* ctl, RedBuf rb) {
+  if (ctl->done) return;
+  const Cell q = cellOf(L.nx, L.ny, L.nz);
+  double acc[2] = {0.0, 0.0};
+  if (q.valid) {
+    const double v = (L.code[q.c] & kFluidBit) ? double(x[q.c]) : 0.0;
+    z[q.c] = v;
+    acc[0] = v * rn[q.c];
+    acc[1] = v * r[q.c];
+  }
+  blockToPartials<2>(acc, rb.partials, rb.stride, q.blk);
+}
+
 __global__ void __launch_bounds__(kBlock) k_fcg_pupdate(int n, double* __restrict__ p, const double* __restrict__ z,
                                                         double* __restrict__ r, const double* __restrict__ rn,
                                                         const SolveCtl* ctl) {
@@ -815,7 +875,7 @@ __global__ void __launch_bounds__(kBlock) k_fcg_final(MgLevel L, int singular, c
                                                       const double* __restrict__ rhs, double* __restrict__ xo,
                                                       const SolveCtl* ctl, RedBuf rb) {
   const double m = singular ? ctl->slot[S_MEAN_X] : 0.0;
-  const XView<double> X{x, m, singular != 0, nullptr, 1, 1, 1, 1, 1, 0.0, false};
+  const XView<double> X{x, m, singular != 0, nullptr, 1, 1, 0, 0, 0, 0.0, false};
   const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
   if (q.valid) {
@@ -980,13 +1040,22 @@ void launch_mg_sweep(const MgLevel& L, const T* b, const double* mb, const T* x,
                      RedBuf rb, const SolveCtl* ctl, hipStream_t s) {
   const Geo3 g = geo3(L.nx, L.ny, L.nz);
   const int ncx = Lc ? Lc->nx : 1, ncy = Lc ? Lc->ny : 1;
-  const int fx = Lc ? Lc->fx : 1, fy = Lc ? Lc->fy : 1, fz = Lc ? Lc->fz : 1;
+  const int sx = Lc ? Lc->fx - 1 : 0, sy = Lc ? Lc->fy - 1 : 0, sz = Lc ? Lc->fz - 1 : 0;
+  const int f = fold ? 1 : 0;
   if (zeroGuess)
-    k_sweep<T, true><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, nullptr, nullptr, 1, 1, 1, 1, 1, xo, fold ? 1 : 0,
-                                                rb, ctl);
+    k_sweep<T, SW_ZERO><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, nullptr, nullptr, 1, 1, 0, 0, 0, xo, f, rb, ctl);
+  else if (ec)
+    k_sweep<T, SW_PROLONG><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, ec, mec, ncx, ncy, sx, sy, sz, xo, f, rb, ctl);
   else
-    k_sweep<T, false><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, ec, mec, ncx, ncy, fx, fy, fz, xo, fold ? 1 : 0,
-                                                 rb, ctl);
+    k_sweep<T, SW_PLAIN><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, nullptr, nullptr, 1, 1, 0, 0, 0, xo, f, rb, ctl);
+  NBL_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_mg_sweep_pair(const MgLevel& L, const T* b, T* xo, const SolveCtl* ctl, hipStream_t s) {
+  const Geo3 g = geo3(L.nx, L.ny, L.nz);
+  k_sweep<T, SW_PAIR><<<g.grid, g.block, 0, s>>>(L, b, nullptr, nullptr, nullptr, nullptr, nullptr, 1, 1, 0, 0, 0, xo,
+                                                 0, RedBuf{nullptr, 0}, ctl);
   NBL_LAUNCH_CHECK();
 }
 
@@ -1074,6 +1143,7 @@ void launch_mg_coarse_direct(const MgLevel& L, const double* M, const float* Mf,
   template void launch_mg_finish<T>(const MgLevel&, const T*, const double*, double*, bool, RedBuf,          \
                                     const SolveCtl*, hipStream_t);                                           \
   template void launch_to_level0<T>(const MgLevel&, const double*, T*, const SolveCtl*, hipStream_t);        \
+  template void launch_mg_sweep_pair<T>(const MgLevel&, const T*, T*, const SolveCtl*, hipStream_t);        \
   template void launch_mg_coarse_direct<T>(const MgLevel&, const double*, const float*, const T*,            \
                                            const double*, T*, const SolveCtl*, hipStream_t);
 NBL_INST(double)
@@ -1144,6 +1214,17 @@ void launch_fcg_dots(const MgLevel& L, double* z, const double* mz, const double
   k_fcg_dots<<<g.grid, g.block, 0, s>>>(L, z, mz, rn, r, ctl, rb);
   NBL_LAUNCH_CHECK();
 }
+template <class T>
+void launch_fcg_dots_x(const MgLevel& L, const T* x, double* z, const double* rn, const double* r,
+                       const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
+  const Geo3 g = geo3(L.nx, L.ny, L.nz);
+  k_fcg_dots_x<T><<<g.grid, g.block, 0, s>>>(L, x, z, rn, r, ctl, rb);
+  NBL_LAUNCH_CHECK();
+}
+template void launch_fcg_dots_x<double>(const MgLevel&, const double*, double*, const double*, const double*,
+                                        const SolveCtl*, RedBuf, hipStream_t);
+template void launch_fcg_dots_x<float>(const MgLevel&, const float*, double*, const double*, const double*,
+                                       const SolveCtl*, RedBuf, hipStream_t);
 void launch_fcg_pupdate(const MgLevel& L, double* p, const double* z, double* r, const double* rn,
                         const SolveCtl* ctl, hipStream_t s) {
   k_fcg_pupdate<<<gridFor(L.n), kBlock, 0, s>>>(int(L.n), p, z, r, rn, ctl);

@@ -396,6 +396,8 @@ OctreeDev::OctreeDev(nbl_ctx* ctx, const uint64_t* keys, const uint8_t* levels, 
     fine_[a] = extent[a] * inv;
   }
   NBL_HIP(hipStreamCreate(&stream_));
+  NBL_HIP(hipEventCreate(&e0_));
+  NBL_HIP(hipEventCreate(&e1_));
   hipStream_t s = stream_;
   const long long N = (long long)n;
   DevBuf kin(ctx, n * 8), idx(ctx, n * 8), lin(ctx, n), min_(ctx, n);
@@ -406,6 +408,7 @@ OctreeDev::OctreeDev(nbl_ctx* ctx, const uint64_t* keys, const uint8_t* levels, 
   NBL_HIP(hipMemcpyAsync(kin.as<void>(), keys, n * 8, hipMemcpyHostToDevice, s));
   NBL_HIP(hipMemcpyAsync(lin.as<void>(), levels, n, hipMemcpyHostToDevice, s));
   NBL_HIP(hipMemcpyAsync(min_.as<void>(), masks, n, hipMemcpyHostToDevice, s));
+  NBL_HIP(hipEventRecord(e0_, s));
   k_iota<<<gridFor(N), kBlock, 0, s>>>(N, idx.as<uint64_t>());
   NBL_LAUNCH_CHECK();
   size_t tmpBytes = 0;
@@ -463,6 +466,7 @@ OctreeDev::OctreeDev(nbl_ctx* ctx, const uint64_t* keys, const uint8_t* levels, 
                                            cnt_.as<uint8_t>(), nbr_.as<int32_t>(), rowPtr_.as<int32_t>(),
                                            col_.as<int32_t>(), coef_.as<double>(), diag_.as<double>());
   NBL_LAUNCH_CHECK();
+  NBL_HIP(hipEventRecord(e1_, s));
   std::vector<uint8_t> hm(masks, masks + n);
   nf_ = 0;
   for (uint8_t m : hm) nf_ += m == 0;
@@ -472,10 +476,15 @@ OctreeDev::OctreeDev(nbl_ctx* ctx, const uint64_t* keys, const uint8_t* levels, 
   red_ = DevBuf(ctx, sizeof(double) * kMaxRedVals * kOctBlocks);
   NBL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hctl_), 2 * sizeof(SolveCtl), hipHostMallocDefault));
   NBL_HIP(hipStreamSynchronize(s));
+  float t = 0.f;
+  NBL_HIP(hipEventElapsedTime(&t, e0_, e1_));
+  setupMs_ = t;
 }
 
 OctreeDev::~OctreeDev() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  if (e0_) (void)hipEventDestroy(e0_);
+  if (e1_) (void)hipEventDestroy(e1_);
   if (hctl_) (void)hipHostFree(hctl_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -518,6 +527,22 @@ void OctreeDev::apply(const double* x, double* out) {
 
 void OctreeDev::diagonal(double* out) { fromSorted(diag_.as<double>(), out); }
 
+void OctreeDev::profile(int kernel, int reps, double* ms, double* bytes) {
+  if (kernel != 0) throw std::invalid_argument("octree profile: kernel must be 0 (SpMV)");
+  if (reps < 1) throw std::invalid_argument("octree profile: reps must be >= 1");
+  NBL_HIP(hipMemsetAsync(t0_.as<void>(), 0, n_ * 8, stream_));
+  applySorted(t0_.as<double>(), Ap_.as<double>(), stream_);
+  NBL_HIP(hipEventRecord(e0_, stream_));
+  for (int q = 0; q < reps; ++q) applySorted(t0_.as<double>(), Ap_.as<double>(), stream_);
+  NBL_HIP(hipEventRecord(e1_, stream_));
+  NBL_HIP(hipEventSynchronize(e1_));
+  float t = 0.f;
+  NBL_HIP(hipEventElapsedTime(&t, e0_, e1_));
+  *ms = double(t) / reps;
+  // row pointer, mask, x and y once per leaf; column index and coefficient per coupling
+  *bytes = double(n_) * (4.0 + 1.0 + 8.0 + 8.0) + double(nnz_) * (4.0 + 8.0);
+}
+
 nbl_poisson_report OctreeDev::solveSorted(const double* b, double* xo, double tol, int maxIters, hipStream_t s) {
   SolveCtl* ctl = ctl_.as<SolveCtl>();
   const RedBuf rb{red_.as<double>(), kOctBlocks};
@@ -526,6 +551,7 @@ nbl_poisson_report OctreeDev::solveSorted(const double* b, double* xo, double to
   hctl_[1] = SolveCtl{};
   hctl_[1].tol = tol;
   NBL_HIP(hipMemcpyAsync(ctl, &hctl_[1], sizeof(SolveCtl), hipMemcpyHostToDevice, s));
+  NBL_HIP(hipEventRecord(e0_, s));
   const uint8_t* mk = mask_.as<uint8_t>();
   const int32_t* rp = rowPtr_.as<int32_t>();
   const int32_t* cl = col_.as<int32_t>();
@@ -576,8 +602,12 @@ nbl_poisson_report OctreeDev::solveSorted(const double* b, double* xo, double to
   k_o_final<<<g, kBlock, 0, s>>>(N, nf, mk, rp, cl, cf, x, rhs, xo, ctl, rb);
   fin(FIN_RESID, 1, 0, false);
   NBL_LAUNCH_CHECK();
+  NBL_HIP(hipEventRecord(e1_, s));
   NBL_HIP(hipMemcpyAsync(hctl_, ctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, s));
   NBL_HIP(hipStreamSynchronize(s));
+  float tms = 0.f;
+  NBL_HIP(hipEventElapsedTime(&tms, e0_, e1_));
+  lastMs_ = tms;
   const SolveCtl& h = hctl_[0];
   if (h.converged && h.it > 0) predicted_ = h.it;
   return nbl_poisson_report{h.it, h.resid, h.converged};

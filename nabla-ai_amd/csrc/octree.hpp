@@ -23,6 +23,11 @@ class OctreeDev {
   void applySorted(const double* x, double* out, hipStream_t s);
   nbl_poisson_report solveSorted(const double* b, double* x, double tol, int maxIters, hipStream_t s);
   hipStream_t stream() const { return stream_; }
+  // Device time of construction (sort, validation, face table, CSR) and of the
+  // last solve (ms), and a timing loop over one kernel (0: the SpMV).
+  double setupMs() const { return setupMs_; }
+  double lastSolveMs() const { return lastMs_; }
+  void profile(int kernel, int reps, double* ms, double* bytes);
 
  private:
   void toSorted(const double* hostIn, double* dSorted);
@@ -38,6 +43,8 @@ class OctreeDev {
   DevBuf ctl_, red_;
   SolveCtl* hctl_ = nullptr;
   int predicted_ = 16;
+  hipEvent_t e0_ = nullptr, e1_ = nullptr;
+  double setupMs_ = 0.0, lastMs_ = 0.0;
 };
 
 }  // namespace nbl

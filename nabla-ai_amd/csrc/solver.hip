@@ -334,7 +334,13 @@ void PoissonDev::vcycle(int l, const T* b, const double* mb, const T** xout, con
   double* ms[2] = {project_ ? slotPtr(ctl, slotX(l, 0)) : nullptr,
                    project_ ? slotPtr(ctl, slotX(l, 1)) : nullptr};
   int cur = -1;
-  for (int q = 0; q < opts_.pre_smooth; ++q) {
+  int q0 = 0;
+  if (!project_ && opts_.pre_smooth >= 2) {  // sweeps 1 and 2 in one pass
+    launch_mg_sweep_pair<T>(L, b, bufs[0], ctl, s);
+    cur = 0;
+    q0 = 2;
+  }
+  for (int q = q0; q < opts_.pre_smooth; ++q) {
     const int o = cur == 0 ? 1 : 0;
     launch_mg_sweep<T>(L, b, mb, cur < 0 ? nullptr : bufs[cur], cur < 0 ? nullptr : ms[cur], cur < 0, nullptr,
                        nullptr, nullptr, bufs[o], project_, rb, ctl, s);
@@ -361,21 +367,27 @@ void PoissonDev::vcycle(int l, const T* b, const double* mb, const T** xout, con
 }
 
 // z = Minv r: one V-cycle from a zero guess, then the level-0 projection
-// (multigrid.cpp:283-289).  z's mean is left pending in kSlotMeanZ.
-void PoissonDev::minv(const double* r, hipStream_t s) {
+// (multigrid.cpp:283-289).  z's mean is left pending in kSlotMeanZ.  With
+// fuseDots (systems without projections) the V-cycle output is left for
+// k_fcg_dots_x, which forms z itself.
+void PoissonDev::minv(const double* r, hipStream_t s, bool fuseDots) {
   SolveCtl* ctl = ctl_.as<SolveCtl>();
   const MgLevel& L = dl_[0];
+  const RedBuf rb = red();
+  double* z = z_.as<double>();
   if (mixed_) {
     launch_to_level0<float>(L, r, b0f_.as<float>(), ctl, s);
     const float* x = nullptr;
     const double* mx = nullptr;
     vcycle<float>(0, b0f_.as<float>(), nullptr, &x, &mx, s);
-    launch_mg_finish<float>(L, x, mx, z_.as<double>(), project_, red(), ctl, s);
+    if (fuseDots) launch_fcg_dots_x<float>(L, x, z, rn_.as<double>(), r_.as<double>(), ctl, rb, s);
+    else launch_mg_finish<float>(L, x, mx, z, project_, rb, ctl, s);
   } else {
     const double* x = nullptr;
     const double* mx = nullptr;
     vcycle<double>(0, r, nullptr, &x, &mx, s);
-    launch_mg_finish<double>(L, x, mx, z_.as<double>(), project_, red(), ctl, s);
+    if (fuseDots) launch_fcg_dots_x<double>(L, x, z, rn_.as<double>(), r_.as<double>(), ctl, rb, s);
+    else launch_mg_finish<double>(L, x, mx, z, project_, rb, ctl, s);
   }
   if (project_) fin(FIN_MEAN, 1, blocks_[0], kSlotMeanZ, L.nf, true, s);
   ++vcount_;
@@ -404,8 +416,12 @@ void PoissonDev::enqueueIteration(hipStream_t s) {
       launch_fcg_norm(L, rn, ctl, rb, s);
     }
     fin(FIN_CONV, 1, nb, 0, 0, true, s);
-    minv(rn, s);
-    launch_fcg_dots(L, z, project_ ? slotPtr(ctl, kSlotMeanZ) : nullptr, rn, r, ctl, rb, s);
+    if (project_) {
+      minv(rn, s, false);
+      launch_fcg_dots(L, z, slotPtr(ctl, kSlotMeanZ), rn, r, ctl, rb, s);
+    } else {
+      minv(rn, s, true);
+    }
     fin(FIN_BETA_FLEX, 2, nb, 0, 0, true, s);
     launch_fcg_pupdate(L, p, z, r, rn, ctl, s);
   } else {
@@ -447,7 +463,7 @@ nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t
   launch_fcg_init2(L, project_, rhs, r, ctl, rb, s);
   fin(FIN_BNORM, 1, nb, 0, 0, false, s, mg ? 1 : 0);
   if (mg) {
-    minv(r, s);
+    minv(r, s, false);
     launch_fcg_start(L, z, project_ ? slotPtr(ctl, kSlotMeanZ) : nullptr, r, p, ctl, rb, s);
   } else {
     launch_cg_z(L, r, z, true, p, ctl, rb, s);
@@ -499,6 +515,28 @@ void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double
   NBL_HIP(hipMemsetAsync(r_.as<void>(), 0, L.n * 8, s));
   NBL_HIP(hipMemsetAsync(p_.as<void>(), 0, L.n * 8, s));
   if (mixed_) NBL_HIP(hipMemsetAsync(b0f_.as<void>(), 0, L.n * 4, s));
+  if (kernel == 3) {  // a non-trivial coarse right-hand side (mean-free for singular systems)
+    const int lc = nlevels() - 1;
+    const size_t nc = size_t(dl_[lc].n);
+    std::vector<double> hb(nc);
+    for (size_t c = 0; c < nc; ++c) hb[c] = std::sin(0.37 * double(c) + 0.11) * std::cos(0.013 * double(c));
+    double mean = 0.0;
+    size_t cnt = 0;
+    for (size_t c = 0; c < nc; ++c)
+      if (hl_[lc].code[c] & kFluidBit) {
+        mean += hb[c];
+        ++cnt;
+      }
+    mean = cnt ? mean / double(cnt) : 0.0;
+    for (size_t c = 0; c < nc; ++c) hb[c] = (hl_[lc].code[c] & kFluidBit) ? hb[c] - (project_ ? mean : 0.0) : 0.0;
+    if (mixed_) {
+      std::vector<float> hf(hb.begin(), hb.end());
+      NBL_HIP(hipMemcpyAsync(lb_[lc].as<void>(), hf.data(), nc * 4, hipMemcpyHostToDevice, s));
+    } else {
+      NBL_HIP(hipMemcpyAsync(lb_[lc].as<void>(), hb.data(), nc * 8, hipMemcpyHostToDevice, s));
+    }
+    NBL_HIP(hipStreamSynchronize(s));
+  }
   const RedBuf rb = red();
   auto one = [&](int q) {
     if (kernel == 0) {

@@ -101,7 +101,7 @@ class PoissonDev {
  private:
   template <class T>
   void vcycle(int l, const T* b, const double* mb, const T** xout, const double** mxout, hipStream_t s);
-  void minv(const double* r, hipStream_t s);
+  void minv(const double* r, hipStream_t s, bool fuseDots);
   bool buildDirectCoarse();
   void fin(int op, int K, int nparts, int slot, long long nf, bool predicated, hipStream_t s, int earlyExit = 0,
            double convTol = 0.0);

@@ -174,6 +174,7 @@ def load(path: os.PathLike | None = None):
     _proto(lib, "nbl_ctx_create", i, i, C.POINTER(V))
     _proto(lib, "nbl_ctx_destroy", i, V)
     _proto(lib, "nbl_ctx_bytes_in_use", i, V, P_U64)
+    _proto(lib, "nbl_ctx_probe_copy", i, V, C.c_uint64, i, C.POINTER(C.c_double))
     _proto(lib, "nbl_run_create", i, V, C.POINTER(GridDesc), C.POINTER(FlowCfg),
            C.POINTER(SolverOpts), C.POINTER(V))
     _proto(lib, "nbl_run_destroy", i, V)
@@ -216,6 +217,8 @@ def load(path: os.PathLike | None = None):
     _proto(lib, "nbl_octree_apply", i, V, P_D, P_D)
     _proto(lib, "nbl_octree_diagonal", i, V, P_D)
     _proto(lib, "nbl_octree_poisson_solve", i, V, P_D, P_D, d, i, C.POINTER(PoissonReport))
+    _proto(lib, "nbl_octree_profile", i, V, i, i, C.POINTER(C.c_double), C.POINTER(C.c_double))
+    _proto(lib, "nbl_octree_last_timing", i, V, C.POINTER(C.c_double), C.POINTER(C.c_double))
     _lib = lib
     return lib
 
@@ -240,6 +243,12 @@ class Ctx:
         v = C.c_uint64()
         check(self.lib.nbl_ctx_bytes_in_use(self.h, C.byref(v)))
         return v.value
+
+    def probe_copy(self, nbytes: int, reps: int = 20) -> float:
+        """Average ms of one device-to-device copy of nbytes."""
+        ms = C.c_double()
+        check(self.lib.nbl_ctx_probe_copy(self.h, nbytes, reps, C.byref(ms)))
+        return ms.value
 
     def close(self):
         if self.h:
@@ -501,6 +510,17 @@ class Octree:
         check(self.lib.nbl_octree_poisson_solve(self.h, dptr(b), dptr(x), tol, max_iters,
                                                 C.byref(rep)))
         return x, rep
+
+    def profile(self, kernel=0, reps=50):
+        ms, by = C.c_double(), C.c_double()
+        check(self.lib.nbl_octree_profile(self.h, kernel, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
+    def last_timing(self):
+        """(device ms of construction, device ms of the last solve)."""
+        a, b = C.c_double(), C.c_double()
+        check(self.lib.nbl_octree_last_timing(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def close(self):
         if self.h:
