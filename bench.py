@@ -153,6 +153,199 @@ def octree_c3(ctx, with_cpu=True):
     return out
 
 
+def batch_octrees(count, seed):
+    """Octrees of 0.5-2M leaves: unit cube, base level 6 (262,144 leaves), then
+    seeded random boxes refined (balanced) to level 8 until the leaf count
+    reaches a seeded target."""
+    u = cases.uniform01(seed, 4096)
+    out, q = [], 0
+    for t in range(count):
+        target = int(500_000 + 1_500_000 * float(u[q]))
+        q += 1
+        tr = nbl.Tree((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 8)
+        tr.refine_uniform(6)
+        while tr.n < target:
+            _k, lv, c = tr.leaves()
+            f = np.zeros(lv.size, bool)
+            for _ in range(3):  # three seeded boxes per balanced pass
+                lo = np.array([float(u[q]), float(u[q + 1]), float(u[q + 2])]) * 0.7
+                hi = lo + 0.08 + 0.22 * float(u[q + 3])
+                q += 4
+                f |= np.all((c >= lo) & (c <= hi), axis=1)
+            f &= lv < 8
+            if f.any():
+                tr.refine_flagged(f)
+        k, lv, _c = tr.leaves()
+        out.append((k, lv))
+    return out
+
+
+def batched(ctx, with_cpu=True, seed=4242, nuni=48, noct=8, concurrency=4):
+    """48 independent uniform Poisson problems (n^3, n drawn from 32/64/96/128,
+    all walls, i.i.d. N(0,1) right-hand sides projected to zero mean, MG-PCG to
+    1e-8) and 8 octree problems (Jacobi-PCG to 1e-8): one solve after another
+    versus nbl_*_solve_batch with `concurrency` solves in flight."""
+    u = cases.uniform01(seed, nuni)
+    sizes = [(32, 64, 96, 128)[int(float(x) * 4) % 4] for x in u]
+    grids = [nbl.Grid(n, n, n, 1.0 / n) for n in sizes]
+    rhs = []
+    for q, g in enumerate(grids):
+        b = cases.normal(seed + 1 + q, g.cells)
+        rhs.append(b - b.mean())
+    t0 = time.perf_counter()
+    solvers = [nbl.Poisson(ctx, g, kind=0, tol=1e-8, max_iters=200) for g in grids]
+    setup_s = time.perf_counter() - t0
+    for p, b in zip(solvers, rhs):  # warm-up (first touches, code objects)
+        p.solve(b)
+    t0 = time.perf_counter()
+    seq = [p.solve(b) for p, b in zip(solvers, rhs)]
+    seq_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    xs, reps = nbl.solve_batch(solvers, rhs, concurrency=concurrency)
+    bat_s = time.perf_counter() - t0
+    same = all(np.array_equal(x, y[0]) for x, y in zip(xs, seq))
+    for p in solvers:
+        p.close()
+    cells = int(sum(g.cells for g in grids))
+    out = {"uniform": {"problems": nuni, "cells": cells, "sizes": {str(n): sizes.count(n) for n in (32, 64, 96, 128)},
+                       "setup_s": round(setup_s, 3), "sequential_s": round(seq_s, 4), "batched_s": round(bat_s, 4),
+                       "speedup": round(seq_s / bat_s, 3), "concurrency": concurrency,
+                       "all_converged": all(r.converged for r in reps),
+                       "max_residual": max(r.residual for r in reps),
+                       "iterations_mean": round(float(np.mean([r.iterations for r in reps])), 2),
+                       "batch_equals_sequential": bool(same),
+                       "mcells_solved_per_s": round(cells / bat_s / 1e6, 2)}}
+    trees = batch_octrees(noct, seed + 99)
+    octs, orhs = [], []
+    for q, (k, lv) in enumerate(trees):
+        octs.append(nbl.Octree(ctx, k, lv, np.zeros(k.size, np.uint8), (0, 0, 0), (1, 1, 1), 8))
+        b = cases.normal(seed + 500 + q, k.size)
+        orhs.append(b - b.mean())
+    t0 = time.perf_counter()
+    oseq = [o.solve(b, 1e-8, 20000) for o, b in zip(octs, orhs)]
+    oseq_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oxs, oreps = nbl.octree_solve_batch(octs, orhs, tol=1e-8, max_iters=20000, concurrency=concurrency)
+    obat_s = time.perf_counter() - t0
+    osame = all(np.array_equal(x, y[0]) for x, y in zip(oxs, oseq))
+    for o in octs:
+        o.close()
+    out["octree"] = {"problems": noct, "leaves": [int(k.size) for k, _ in trees],
+                     "sequential_s": round(oseq_s, 4), "batched_s": round(obat_s, 4),
+                     "speedup": round(oseq_s / obat_s, 3), "all_converged": all(r.converged for r in oreps),
+                     "iterations": [r.iterations for r in oreps], "batch_equals_sequential": bool(osame)}
+    if with_cpu:
+        import oracle_py
+        kind = "reference" if oracle_py.available("ref") else "port"
+        lib = oracle_py.OracleLib("ref" if kind == "reference" else "orc")
+        sample = [q for q, n in enumerate(sizes) if n in (32, 64)][:6]
+        t0 = time.perf_counter()
+        cpu_cells = 0
+        for q in sample:
+            lib.mg(grids[q], rhs[q], 1e-8, 200)
+            cpu_cells += grids[q].cells
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"kind": kind, "cores": 1, "sample_problems": len(sample),
+                               "sample_cells": cpu_cells, "seconds": round(el, 3),
+                               "mcells_solved_per_s": round(cpu_cells / el / 1e6, 4),
+                               "sample": "MgPoisson::solve to 1e-8 on the batch's 32^3 and 64^3 problems (up to 6)"}
+    return out
+
+
+def served_runs(nruns, seed):
+    """32 seeded runs of the two uniform families: Taylor-Green n^3 boxes and
+    cube channels at r cells per cube side (192 r^3 cells), sizes 0.3-4M
+    cells, Re uniform in [500, 3000]."""
+    u = cases.uniform01(seed, 3 * nruns)
+    out = []
+    for q in range(nruns):
+        re = 500.0 + 2500.0 * float(u[3 * q + 1])
+        if u[3 * q] < 0.5:
+            n = 2 * int(round((67 + (158 - 67) * float(u[3 * q + 2])) / 2))
+            g, cfg = cases.taylor_green(n, reynolds=re)
+            st = cases.taylor_green_state(g)
+            name = f"tgv{n}"
+        else:
+            r = int(round(12 + (27 - 12) * float(u[3 * q + 2])))
+            g, cfg = cases.cube_channel(nx=12 * r, ny=4 * r, nz=4 * r, res=r, reynolds=re)
+            st = cases.cube_channel_state(g, seed=seed + q)
+            name = f"cube{12 * r}x{4 * r}x{4 * r}"
+        out.append((name, g, cfg, st))
+    return out
+
+
+def served(ctx, window_s, rate, nruns=32, slots=4, seed=777):
+    """Poisson arrivals at `rate` requests/s over `window_s` seconds, each one
+    RK3 step of a uniformly drawn run; latency = submit to completion."""
+    import threading
+    runs = served_runs(nruns, seed)
+    srv = nbl.Server(ctx, slots=slots, hbm_budget=200 << 30)
+    t0 = time.perf_counter()
+    ids, cells = [], []
+    for name, g, cfg, st in runs:
+        rid = srv.admit(g, cfg)
+        srv.run(rid, g).set_state(*st)
+        ids.append(rid)
+        cells.append(g.cells)
+    admit_s = time.perf_counter() - t0
+    used, _ = srv.footprint()
+    u = cases.uniform01(seed + 1, 2 * int(rate * window_s * 2) + 16)
+    arrivals, pick, t = [], [], 0.0
+    q = 0
+    while True:
+        t += -np.log(max(float(u[2 * q]), 1e-300)) / rate
+        if t > window_s:
+            break
+        arrivals.append(t)
+        pick.append(int(float(u[2 * q + 1]) * nruns) % nruns)
+        q += 1
+    tickets = []
+    lat, done_cells = [], []
+    lock = threading.Lock()
+
+    def collector():
+        k = 0
+        while True:
+            with lock:
+                if k >= len(tickets) and feeding_done[0]:
+                    return
+                have = k < len(tickets)
+                if have:
+                    tk, rid = tickets[k]
+            if not have:
+                time.sleep(0.0005)
+                continue
+            _rep, ms = srv.wait(tk)
+            lat.append(ms)
+            done_cells.append(cells[rid])
+            k += 1
+
+    feeding_done = [False]
+    th = threading.Thread(target=collector)
+    th.start()
+    start = time.perf_counter()
+    for at, rid in zip(arrivals, pick):
+        dly = start + at - time.perf_counter()
+        if dly > 0:
+            time.sleep(dly)
+        tk = srv.submit(ids[rid])
+        with lock:
+            tickets.append((tk, rid))
+    with lock:
+        feeding_done[0] = True
+    th.join()
+    elapsed = time.perf_counter() - start
+    srv.close()
+    lat = np.array(lat)
+    return {"requests": len(lat), "window_s": window_s, "rate_per_s": rate, "runs": nruns, "slots": slots,
+            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "mean_ms": round(float(lat.mean()), 3), "drain_s": round(elapsed - window_s, 3),
+            "mcell_steps_per_s": round(float(np.sum(done_cells)) / elapsed / 1e6, 3),
+            "cells_min": int(min(cells)), "cells_max": int(max(cells)),
+            "admit_s": round(admit_s, 3), "hbm_bytes": int(used),
+            "families": "Taylor-Green n^3 and cube channel 12r x 4r x 4r (uniform MAC); Re U[500,3000]"}
+
+
 def cpu_baseline(g, cfg, st, steps):
     import oracle_py
     kind = "reference" if oracle_py.available("ref") else "port"
@@ -188,6 +381,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-octree", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--served-window", type=float, default=20.0,
+                    help="seconds of Poisson arrivals in the served-stream measurement (0: skip)")
+    ap.add_argument("--served-rate", type=float, default=200.0)
     ap.add_argument("--profile-only", action="store_true",
                     help="only the timed FP64 steps (for profiler runs)")
     ap.add_argument("--roofline-only", action="store_true",
@@ -195,7 +392,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-              "(independent runs shard across GPUs without exchange; see DESIGN.md)", file=sys.stderr)
+              "(independent runs shard across GPUs without exchange: one server per GPU)", file=sys.stderr)
         sys.exit(2)
 
     g, cfg = cases.cube_channel()
@@ -251,6 +448,15 @@ def main():
     out["roofline_256cubed"] = roofline(ctx, big, reps=50)
     if not a.no_octree:
         out["octree_c3"] = octree_c3(ctx, with_cpu=not a.no_cpu)
+    if not a.no_batch:
+        out["batched_solves"] = batched(ctx, with_cpu=not a.no_cpu)
+        log(f"batched: {out['batched_solves']['uniform']['speedup']}x uniform, "
+            f"{out['batched_solves']['octree']['speedup']}x octree")
+    if a.served_window > 0:
+        sv = served(ctx, a.served_window, a.served_rate)
+        out["served"] = sv
+        out["served_p50_ms"], out["served_p99_ms"] = sv["p50_ms"], sv["p99_ms"]
+        log(f"served: {sv['requests']} requests, p50 {sv['p50_ms']} ms, p99 {sv['p99_ms']} ms")
     ctx.close()
     if not a.no_cpu and a.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(g, cfg, st, a.cpu_steps)

@@ -139,6 +139,7 @@ typedef struct nbl_ctx nbl_ctx;
 typedef struct nbl_run nbl_run;
 typedef struct nbl_poisson nbl_poisson;
 typedef struct nbl_octree nbl_octree;
+typedef struct nbl_server nbl_server;
 typedef struct nbl_tree nbl_tree;
 
 /* ---- defaults and errors ------------------------------------------------ */
@@ -211,6 +212,10 @@ int nbl_poisson_solve(nbl_poisson* ps, const double* b, double* x, nbl_poisson_r
 int nbl_poisson_solve_device(nbl_poisson* ps, const double* d_b, double* d_x,
                              nbl_poisson_report* rep);
 /* CgPoisson::applyA (poisson.cpp:102-134), host buffers. */
+/* One application of the MG-PCG preconditioner, z = Minv r: a V(pre,post)
+ * cycle from zero followed by the null-space projection on singular systems
+ * (the Minv of MgPoisson::solve, multigrid.cpp:283-289).  Host vectors. */
+int nbl_poisson_precondition(nbl_poisson* ps, const double* r, double* z);
 int nbl_poisson_apply(nbl_poisson* ps, const double* x, double* out);
 /* CgPoisson::buildDiagonal (poisson.cpp:78-91) and the number of multigrid
  * levels with each level's dims (dims: 3*nlevels ints, may be NULL). */
@@ -283,6 +288,40 @@ int nbl_octree_poisson_solve(nbl_octree* oc, const double* b, double* x, double 
  * validation, face table, operator) and of the last poisson_solve. */
 int nbl_octree_profile(nbl_octree* oc, int kernel, int reps, double* ms, double* bytes);
 int nbl_octree_last_timing(nbl_octree* oc, double* setup_ms, double* solve_ms);
+
+/* ---- serving: device-resident request loop -------------------------------
+ * No counterpart in the reference: its backend (backend/app/jobs.py
+ * LocalJobRunner) runs one nabla_solve subprocess per job.  A server owns many
+ * runs, each resident in HBM with its own stream, and `slots` worker threads
+ * that execute step requests (nbl_run_step with the CFL dt) in arrival order,
+ * letting later requests for idle runs overtake a request whose run is busy,
+ * so different runs' steps overlap on the device.  Results do not depend on
+ * what else is in flight.  Admission builds the run, measures its device
+ * footprint and refuses it (NBL_ERR_CAPACITY) if the total would exceed
+ * `hbm_budget` bytes; admission allocates and should happen outside latency-
+ * critical windows.  Runs returned by nbl_server_run belong to the server
+ * (never nbl_run_destroy them) and may be used with nbl_run_* only while they
+ * have no request in flight.  A ticket is collected exactly once by
+ * nbl_server_wait, which reports the step and its latency (submit to
+ * completion, ms). */
+int nbl_server_create(nbl_ctx* ctx, int slots, uint64_t hbm_budget, nbl_server** out);
+int nbl_server_destroy(nbl_server* srv);   /* finishes queued requests first */
+int nbl_server_admit(nbl_server* srv, const nbl_grid_desc* grid, const nbl_flow_cfg* cfg,
+                     const nbl_solver_opts* opts, int* run_id);
+int nbl_server_run(nbl_server* srv, int run_id, nbl_run** run);
+int nbl_server_submit(nbl_server* srv, int run_id, uint64_t* ticket);
+int nbl_server_wait(nbl_server* srv, uint64_t ticket, nbl_step_report* rep, double* latency_ms);
+int nbl_server_footprint(nbl_server* srv, uint64_t* used, uint64_t* budget);
+
+/* Batched pressure solves: n independent solves (each solver as created by
+ * nbl_poisson_create; host vectors), at most `concurrency` in flight at once
+ * on the solvers' own streams.  Same per-problem results as n calls of
+ * nbl_poisson_solve.  No counterpart in the reference. */
+int nbl_poisson_solve_batch(nbl_poisson* const* ps, const double* const* b, double* const* x, int n,
+                            int concurrency, nbl_poisson_report* reps);
+/* The same for octree Jacobi-PCG solves (as nbl_octree_poisson_solve each). */
+int nbl_octree_solve_batch(nbl_octree* const* os, const double* const* b, double* const* x, int n,
+                           int concurrency, double tol, int max_iters, nbl_poisson_report* reps);
 
 #ifdef __cplusplus
 }

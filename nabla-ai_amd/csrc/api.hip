@@ -8,6 +8,7 @@
 
 #include "nbl_api.h"
 #include "octree.hpp"
+#include "serve.hpp"
 #include "solver.hpp"
 #include "tree.hpp"
 
@@ -28,6 +29,9 @@ int guarded(F&& f) {
     return NBL_ERR_DEVICE;
   } catch (const std::bad_alloc&) {
     g_err = "host allocation failed";
+    return NBL_ERR_CAPACITY;
+  } catch (const nbl::CapacityError& e) {
+    g_err = e.what();
     return NBL_ERR_CAPACITY;
   } catch (const std::exception& e) {
     g_err = e.what();
@@ -294,6 +298,19 @@ int nbl_poisson_solve_device(nbl_poisson* ps, const double* d_b, double* d_x, nb
   });
 }
 
+int nbl_poisson_precondition(nbl_poisson* ps, const double* r, double* z) {
+  return guarded([&] {
+    need(ps && r && z, "NULL argument");
+    useDevice(ps->ctx);
+    const size_t n = ps->grid.cells();
+    Staging st(ps->ctx, n);
+    NBL_HIP(hipMemcpyAsync(st.a.as<void>(), r, n * 8, hipMemcpyHostToDevice, ps->stream));
+    ps->impl->precondition(st.a.as<double>(), st.b.as<double>(), ps->stream);
+    NBL_HIP(hipMemcpyAsync(z, st.b.as<void>(), n * 8, hipMemcpyDeviceToHost, ps->stream));
+    NBL_HIP(hipStreamSynchronize(ps->stream));
+  });
+}
+
 int nbl_poisson_apply(nbl_poisson* ps, const double* x, double* out) {
   return guarded([&] {
     need(ps && x && out, "NULL argument");
@@ -520,6 +537,77 @@ int nbl_octree_last_timing(nbl_octree* oc, double* setup_ms, double* solve_ms) {
     need(oc, "NULL argument");
     if (setup_ms) *setup_ms = oc->impl->setupMs();
     if (solve_ms) *solve_ms = oc->impl->lastSolveMs();
+  });
+}
+
+// ---- serving and batched solves -------------------------------------------
+int nbl_server_create(nbl_ctx* ctx, int slots, uint64_t hbm_budget, nbl_server** out) {
+  return guarded([&] {
+    need(ctx && out, "NULL argument");
+    useDevice(ctx);
+    auto s = std::make_unique<nbl_server>();
+    s->ctx = ctx;
+    s->impl = std::make_unique<nbl::Server>(ctx, slots, hbm_budget);
+    *out = s.release();
+  });
+}
+
+int nbl_server_destroy(nbl_server* srv) {
+  return guarded([&] { delete srv; });
+}
+
+int nbl_server_admit(nbl_server* srv, const nbl_grid_desc* grid, const nbl_flow_cfg* cfg,
+                     const nbl_solver_opts* opts, int* run_id) {
+  return guarded([&] {
+    need(srv && grid && cfg && run_id, "NULL argument");
+    useDevice(srv->ctx);
+    *run_id = srv->impl->admit(nbl::HostGrid::from(grid), *cfg, optsOr(opts));
+  });
+}
+
+int nbl_server_run(nbl_server* srv, int run_id, nbl_run** run) {
+  return guarded([&] {
+    need(srv && run, "NULL argument");
+    *run = srv->impl->run(run_id);
+  });
+}
+
+int nbl_server_submit(nbl_server* srv, int run_id, uint64_t* ticket) {
+  return guarded([&] {
+    need(srv && ticket, "NULL argument");
+    *ticket = srv->impl->submit(run_id);
+  });
+}
+
+int nbl_server_wait(nbl_server* srv, uint64_t ticket, nbl_step_report* rep, double* latency_ms) {
+  return guarded([&] {
+    need(srv, "server is NULL");
+    srv->impl->wait(ticket, rep, latency_ms);
+  });
+}
+
+int nbl_server_footprint(nbl_server* srv, uint64_t* used, uint64_t* budget) {
+  return guarded([&] {
+    need(srv, "server is NULL");
+    srv->impl->footprint(used, budget);
+  });
+}
+
+int nbl_poisson_solve_batch(nbl_poisson* const* ps, const double* const* b, double* const* x, int n,
+                            int concurrency, nbl_poisson_report* reps) {
+  return guarded([&] {
+    need(ps && b && x && reps && n >= 0, "NULL argument");
+    if (n > 0) useDevice(ps[0]->ctx);
+    nbl::solveBatch(ps, b, x, n, concurrency, reps);
+  });
+}
+
+int nbl_octree_solve_batch(nbl_octree* const* os, const double* const* b, double* const* x, int n,
+                           int concurrency, double tol, int max_iters, nbl_poisson_report* reps) {
+  return guarded([&] {
+    need(os && b && x && reps && n >= 0, "NULL argument");
+    if (n > 0) useDevice(os[0]->ctx);
+    nbl::octreeSolveBatch(os, b, x, n, concurrency, tol, max_iters, reps);
   });
 }
 

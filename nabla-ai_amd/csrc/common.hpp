@@ -53,6 +53,18 @@ struct RedBuf {
 // ---------------------------------------------------------------------------
 // Device helpers
 // ---------------------------------------------------------------------------
+// Scalars that one kernel hands to the next (per-block partials, the solve's
+// alpha/beta/means/flags) are written and read with agent-scope (device-wide
+// coherent) accesses: the producer is a single block on one XCD and the
+// consumers run on all eight, each XCD with its own L2.
+template <class T>
+__device__ __forceinline__ T ldA(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void stA(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 // Wavefront reductions with DPP lane moves (no LDS traffic): pairs and quads
 // (quad_perm), rows of 16 (row_ror 4, 8), then row_bcast15 / row_bcast31
 // carry rows 0..2 into row 3, so lane 63 ends with the total, which readlane
@@ -118,9 +130,11 @@ template <int K, unsigned MAXMASK = 0u>
 __device__ __forceinline__ void blockToPartials(double (&v)[K], double* partials, int stride, int blockLinear) {
   __shared__ double sh[K * 16];
   blockFold<K, MAXMASK>(v, sh);
-  if (threadIdx.x + threadIdx.y * blockDim.x == 0) {
+  // the block total sits in linear thread 0 only (every index counts: blocks
+  // may be 3-D)
+  if (threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0) {
 #pragma unroll
-    for (int q = 0; q < K; ++q) partials[q * stride + blockLinear] = v[q];
+    for (int q = 0; q < K; ++q) stA(&partials[q * stride + blockLinear], v[q]);
   }
 }
 
@@ -135,7 +149,7 @@ __device__ __forceinline__ void foldPartials(const double* partials, int stride,
   for (int q = 0; q < K; ++q) {
     a[q] = 0.0;
     for (int b = threadIdx.x; b < P; b += blockDim.x) {
-      const double pv = partials[q * stride + b];
+      const double pv = ldA(&partials[q * stride + b]);
       a[q] = ((MAXMASK >> q) & 1u) ? fmax(a[q], pv) : a[q] + pv;
     }
   }

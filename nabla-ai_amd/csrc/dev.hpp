@@ -154,6 +154,10 @@ void launch_fill(double* a, long long n, double v, hipStream_t s);
 // Multigrid pieces are templated on the storage/arithmetic type T (double for
 // the plain path, float for the mixed V-cycle).
 void launch_fin(RedBuf rb, const FinArgs& a, SolveCtl* ctl, hipStream_t s);
+// All solve scalars to zero, tolerance set (a kernel, ordered on `s`).
+void launch_ctl_init(SolveCtl* ctl, double tol, hipStream_t s);
+// Step scalars (step index, time, last dt) set by a kernel on `s`.
+void launch_set_step(StepCtl* sc, int step, double t, double dt, hipStream_t s);
 template <class T>
 void launch_mg_sweep(const MgLevel& L, const T* b, const double* mb, const T* x, const double* mx,
                      bool zeroGuess, const T* ec, const double* mec, const MgLevel* Lc, T* xo, bool fold,

@@ -168,7 +168,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(MgLevel L, const T* __restrict
                                                   const T* __restrict__ x, const double* mx, const T* ec,
                                                   const double* mec, int ncx, int ncy, int sx, int sy, int sz,
                                                   T* __restrict__ xo, int fold, RedBuf rb, const SolveCtl* ctl) {
-  if (ctl && ctl->done) return;
+  if (ctl && ldA(&ctl->done)) return;
   const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
   if (q.valid) {
@@ -182,18 +182,18 @@ __global__ void __launch_bounds__(kBlock) k_sweep(MgLevel L, const T* __restrict
         Ax = applyRow<T, false>(L, code, q.c, q.i, q.j, q.k, X, dg);
         xc = X.at(q.c, q.i, q.j, q.k);
       } else if constexpr (MODE == SW_PROLONG) {
-        const XView<T, true> X{x, mx ? T(*mx) : T(0), mx != nullptr, ec, ncx, ncy, sx, sy, sz,
-                               mec ? T(*mec) : T(0), mec != nullptr};
+        const XView<T, true> X{x, mx ? T(ldA(mx)) : T(0), mx != nullptr, ec, ncx, ncy, sx, sy, sz,
+                               mec ? T(ldA(mec)) : T(0), mec != nullptr};
         Ax = applyRow<T, false>(L, code, q.c, q.i, q.j, q.k, X, dg);
         xc = X.at(q.c, q.i, q.j, q.k);
       } else {
-        const XView<T> X{x, mx ? T(*mx) : T(0), mx != nullptr, nullptr, 1, 1, 0, 0, 0, T(0), false};
+        const XView<T> X{x, mx ? T(ldA(mx)) : T(0), mx != nullptr, nullptr, 1, 1, 0, 0, 0, T(0), false};
         Ax = applyRow<T, MODE == SW_ZERO>(L, code, q.c, q.i, q.j, q.k, X, dg);
         xc = MODE == SW_ZERO ? T(0) : X.at(q.c, q.i, q.j, q.k);
       }
       const T diag = dg > T(0) ? dg : T(1);
       T bv = b[q.c];
-      if (mb) bv = bv - T(*mb);
+      if (mb) bv = bv - T(ldA(mb));
       const T xn = xc + T(0.8) * (bv - Ax) / diag;
       xo[q.c] = xn;
       acc[0] = double(xn);
@@ -213,10 +213,10 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc
                                                            const double* mb, const T* __restrict__ x,
                                                            const double* mx, T* __restrict__ rc, int fold,
                                                            RedBuf rb, const SolveCtl* ctl) {
-  if (ctl && ctl->done) return;
+  if (ctl && ldA(&ctl->done)) return;
   __shared__ T rs[kBlock];
   __shared__ unsigned char fls[kBlock];
-  const XView<T> X{x, mx ? T(*mx) : T(0), mx != nullptr, nullptr, 1, 1, 0, 0, 0, T(0), false};
+  const XView<T> X{x, mx ? T(ldA(mx)) : T(0), mx != nullptr, nullptr, 1, 1, 0, 0, 0, T(0), false};
   const Blk3 B = blk3();
   const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
   const int bx = blockDim.x, by = blockDim.y;
@@ -232,7 +232,7 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc
       T dg;
       const T Ax = applyRow<T, ZEROX>(L, code, c, i, j, k, X, dg);
       T bv = b[c];
-      if (mb) bv = bv - T(*mb);
+      if (mb) bv = bv - T(ldA(mb));
       r = bv - Ax;
     }
   }
@@ -310,7 +310,7 @@ template <class T>
 __global__ void __launch_bounds__(kCoarseT) k_coarse_lds(MgLevel L, int singular, const T* __restrict__ b,
                                                          const double* mb, T* __restrict__ xout,
                                                          const SolveCtl* ctl) {
-  if (ctl && ctl->done) return;
+  if (ctl && ldA(&ctl->done)) return;
   __shared__ T ul[kCoarseMax + 1];  // [kCoarseMax] stays 0
   __shared__ double red1[1][kCoarseT / kWave];
   __shared__ double red3[3][kCoarseT / kWave];
@@ -320,7 +320,7 @@ __global__ void __launch_bounds__(kCoarseT) k_coarse_lds(MgLevel L, int singular
   T x[kCPT], r[kCPT], p[kCPT], sv[kCPT], inv[kCPT], dsum[kCPT];
   int nb[kCPT][6];
   bool fl[kCPT], live[kCPT];
-  const T bm = mb ? T(*mb) : T(0);
+  const T bm = mb ? T(ldA(mb)) : T(0);
   if (tid == 0) ul[kCoarseMax] = T(0);
   double s0[1] = {0.0};
 #pragma unroll
@@ -510,10 +510,10 @@ template <class T, class MT>
 __global__ void __launch_bounds__(kBlock) k_coarse_direct(MgLevel L, const MT* __restrict__ M,
                                                           const T* __restrict__ b, const double* mb,
                                                           T* __restrict__ x, const SolveCtl* ctl) {
-  if (ctl && ctl->done) return;
+  if (ctl && ldA(&ctl->done)) return;
   const int n = int(L.n);
   const int r0 = blockIdx.x * kDirectRows;
-  const double m = mb ? *mb : 0.0;
+  const double m = mb ? ldA(mb) : 0.0;
   double acc[kDirectRows];
 #pragma unroll
   for (int r = 0; r < kDirectRows; ++r) acc[r] = 0.0;
@@ -549,7 +549,7 @@ template <class T>
 __global__ void __launch_bounds__(1024) k_coarse_global(MgLevel L, int singular, const T* __restrict__ b,
                                                         const double* mb, T* __restrict__ x, T* __restrict__ scr,
                                                         const SolveCtl* ctl) {
-  if (ctl && ctl->done) return;
+  if (ctl && ldA(&ctl->done)) return;
   __shared__ double sh[2 * 16];
   __shared__ double bc[2];
   const int N = int(L.n);
@@ -557,7 +557,7 @@ __global__ void __launch_bounds__(1024) k_coarse_global(MgLevel L, int singular,
   T* z = scr + N;
   T* p = scr + 2 * N;
   T* Ap = scr + 3 * N;
-  const T bm = mb ? T(*mb) : T(0);
+  const T bm = mb ? T(ldA(mb)) : T(0);
   const int tid = threadIdx.x, nt = blockDim.x;
   auto ijk = [&](int c, int& i, int& j, int& k) {
     i = c % L.nx;
@@ -654,13 +654,13 @@ template <class T>
 __global__ void __launch_bounds__(kBlock) k_finish(MgLevel L, const T* __restrict__ x, const double* mx,
                                                    double* __restrict__ z, int fold, RedBuf rb,
                                                    const SolveCtl* ctl) {
-  if (ctl && ctl->done) return;
+  if (ctl && ldA(&ctl->done)) return;
   const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
   if (q.valid) {
     const bool fl = L.code[q.c] & kFluidBit;
     T v = x[q.c];
-    if (mx && fl) v = v - T(*mx);
+    if (mx && fl) v = v - T(ldA(mx));
     const double zv = fl ? double(v) : 0.0;
     z[q.c] = zv;
     acc[0] = fl ? zv : 0.0;
@@ -670,7 +670,7 @@ __global__ void __launch_bounds__(kBlock) k_finish(MgLevel L, const T* __restric
 
 template <class T>
 __global__ void k_to_level0(int n, const double* __restrict__ r, T* __restrict__ b, const SolveCtl* ctl) {
-  if (ctl && ctl->done) return;
+  if (ctl && ldA(&ctl->done)) return;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < n) b[c] = T(r[c]);
 }
@@ -738,7 +738,7 @@ __global__ void __launch_bounds__(kBlock) k_fcg_init2(MgLevel L, int singular, d
   if (q.valid) {
     double v = rhs[q.c];
     if (singular) {
-      v = (L.code[q.c] & kFluidBit) ? v - ctl->slot[S_MEAN_RHS] : 0.0;
+      v = (L.code[q.c] & kFluidBit) ? v - ldA(&ctl->slot[S_MEAN_RHS]) : 0.0;
       rhs[q.c] = v;
     }
     r[q.c] = v;
@@ -751,12 +751,12 @@ __global__ void __launch_bounds__(kBlock) k_fcg_init2(MgLevel L, int singular, d
 __global__ void __launch_bounds__(kBlock) k_fcg_start(MgLevel L, double* __restrict__ z, const double* mz,
                                                       const double* __restrict__ r, double* __restrict__ p,
                                                       const SolveCtl* ctl, RedBuf rb) {
-  if (ctl->done) return;
+  if (ldA(&ctl->done)) return;
   const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
   if (q.valid) {
     double v = z[q.c];
-    if (mz && (L.code[q.c] & kFluidBit)) v = v - *mz;
+    if (mz && (L.code[q.c] & kFluidBit)) v = v - ldA(mz);
     z[q.c] = v;
     p[q.c] = v;
     acc[0] = r[q.c] * v;
@@ -766,7 +766,7 @@ __global__ void __launch_bounds__(kBlock) k_fcg_start(MgLevel L, double* __restr
 
 __global__ void __launch_bounds__(kBlock) k_fcg_apply(MgLevel L, const double* __restrict__ p,
                                                       double* __restrict__ Ap, const SolveCtl* ctl, RedBuf rb) {
-  if (ctl->done) return;
+  if (ldA(&ctl->done)) return;
   const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
   if (q.valid) {
@@ -789,8 +789,8 @@ __global__ void __launch_bounds__(kBlock) k_fcg_update(MgLevel L, int singular, 
                                                        const double* __restrict__ p, const double* __restrict__ r,
                                                        const double* __restrict__ Ap, double* __restrict__ rn,
                                                        const SolveCtl* ctl, RedBuf rb) {
-  if (ctl->done) return;
-  const double a = ctl->alpha;
+  if (ldA(&ctl->done)) return;
+  const double a = ldA(&ctl->alpha);
   const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
   if (q.valid) {
@@ -805,11 +805,11 @@ __global__ void __launch_bounds__(kBlock) k_fcg_update(MgLevel L, int singular, 
 
 __global__ void __launch_bounds__(kBlock) k_fcg_norm(MgLevel L, double* __restrict__ rn, const SolveCtl* ctl,
                                                      RedBuf rb) {
-  if (ctl->done) return;
+  if (ldA(&ctl->done)) return;
   const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
   if (q.valid) {
-    const double v = (L.code[q.c] & kFluidBit) ? rn[q.c] - ctl->slot[S_MEAN_RN] : 0.0;
+    const double v = (L.code[q.c] & kFluidBit) ? rn[q.c] - ldA(&ctl->slot[S_MEAN_RN]) : 0.0;
     rn[q.c] = v;
     acc[0] = v * v;
   }
@@ -821,12 +821,12 @@ __global__ void __launch_bounds__(kBlock) k_fcg_norm(MgLevel L, double* __restri
 __global__ void __launch_bounds__(kBlock) k_fcg_dots(MgLevel L, double* __restrict__ z, const double* mz,
                                                      const double* __restrict__ rn, const double* __restrict__ r,
                                                      const SolveCtl* ctl, RedBuf rb) {
-  if (ctl->done) return;
+  if (ldA(&ctl->done)) return;
   const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[2] = {0.0, 0.0};
   if (q.valid) {
     double v = z[q.c];
-    if (mz && (L.code[q.c] & kFluidBit)) v = v - *mz;
+    if (mz && (L.code[q.c] & kFluidBit)) v = v - ldA(mz);
     z[q.c] = v;
     acc[0] = v * rn[q.c];
     acc[1] = v * r[q.c];
@@ -840,7 +840,7 @@ template <class T>
 __global__ void __launch_bounds__(kBlock) k_fcg_dots_x(MgLevel L, const T* __restrict__ x, double* __restrict__ z,
                                                        const double* __restrict__ rn, const double* __restrict__ r,
                                                        const SolveCtl* ctl, RedBuf rb) {
-  if (ctl->done) return;
+  if (ldA(&ctl->done)) return;
   const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[2] = {0.0, 0.0};
   if (q.valid) {
@@ -855,8 +855,8 @@ __global__ void __launch_bounds__(kBlock) k_fcg_dots_x(MgLevel L, const T* __res
 __global__ void __launch_bounds__(kBlock) k_fcg_pupdate(int n, double* __restrict__ p, const double* __restrict__ z,
                                                         double* __restrict__ r, const double* __restrict__ rn,
                                                         const SolveCtl* ctl) {
-  if (ctl->done) return;
-  const double be = ctl->beta;
+  if (ldA(&ctl->done)) return;
+  const double be = ldA(&ctl->beta);
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   p[c] = z[c] + be * p[c];
@@ -874,7 +874,7 @@ __global__ void __launch_bounds__(kBlock) k_fluid_sum(MgLevel L, const double* _
 __global__ void __launch_bounds__(kBlock) k_fcg_final(MgLevel L, int singular, const double* __restrict__ x,
                                                       const double* __restrict__ rhs, double* __restrict__ xo,
                                                       const SolveCtl* ctl, RedBuf rb) {
-  const double m = singular ? ctl->slot[S_MEAN_X] : 0.0;
+  const double m = singular ? ldA(&ctl->slot[S_MEAN_X]) : 0.0;
   const XView<double> X{x, m, singular != 0, nullptr, 1, 1, 0, 0, 0, 0.0, false};
   const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
@@ -899,7 +899,7 @@ __global__ void __launch_bounds__(kBlock) k_fcg_final(MgLevel L, int singular, c
 // Jacobi-CG: z = r/diag and r.z (first: also p = z).
 __global__ void __launch_bounds__(kBlock) k_cg_z(MgLevel L, const double* __restrict__ r, double* __restrict__ z,
                                                  double* __restrict__ p, int first, const SolveCtl* ctl, RedBuf rb) {
-  if (ctl->done) return;
+  if (ldA(&ctl->done)) return;
   const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
   if (q.valid) {
@@ -921,8 +921,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_z(MgLevel L, const double* __rest
 __global__ void __launch_bounds__(kBlock) k_cg_update(MgLevel L, double* __restrict__ x, const double* __restrict__ p,
                                                       double* __restrict__ r, const double* __restrict__ Ap,
                                                       const SolveCtl* ctl, RedBuf rb) {
-  if (ctl->done) return;
-  const double a = ctl->alpha;
+  if (ldA(&ctl->done)) return;
+  const double a = ldA(&ctl->alpha);
   const Cell q = cellOf(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
   if (q.valid) {
@@ -935,9 +935,10 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(MgLevel L, double* __restr
 }
 
 // One-block finalize: fold the partials of the preceding reducing kernel and
-// apply the scalar update `op`.
+// apply the scalar update `op`.  Thread 0 reads the scalars coherently into a
+// local copy, updates it, and writes it back.
 __global__ void __launch_bounds__(kFinThreads) k_fin(RedBuf rb, FinArgs a, SolveCtl* ctl) {
-  if (a.predicated && ctl->done) return;
+  if (a.predicated && ldA(&ctl->done)) return;
   double t[2];
   if (a.K == 2) {
     foldPartials<2>(rb.partials, rb.stride, a.nparts, t);
@@ -948,79 +949,106 @@ __global__ void __launch_bounds__(kFinThreads) k_fin(RedBuf rb, FinArgs a, Solve
     t[1] = 0.0;
   }
   if (threadIdx.x != 0) return;
+  double tol = ldA(&ctl->tol), bnorm = ldA(&ctl->bnorm), rz = ldA(&ctl->rz), pAp = ldA(&ctl->pAp);
+  double alpha = ldA(&ctl->alpha), beta = ldA(&ctl->beta), rr = ldA(&ctl->rr), resid = ldA(&ctl->resid);
+  int it = ldA(&ctl->it), done = ldA(&ctl->done), conv = ldA(&ctl->converged), broke = ldA(&ctl->broke);
+  int early = ldA(&ctl->early);
+  const double mean0 = a.nf > 0 ? t[0] / double(a.nf) : 0.0;
+  const double mean1 = a.nf > 0 ? t[1] / double(a.nf) : 0.0;
+  auto reset = [&] {
+    it = 0;
+    done = 0;
+    conv = 0;
+    broke = 0;
+    early = 0;
+  };
   switch (a.op) {
     case FIN_MEAN:
-      ctl->slot[a.slot] = a.nf > 0 ? t[0] / double(a.nf) : 0.0;
-      break;
-    case FIN_BNORM: {  // multigrid.cpp:281, 292-298
-      const double bn = sqrt(t[0]) + 1e-300;
-      ctl->bnorm = bn;
-      ctl->it = 0;
-      ctl->done = 0;
-      ctl->converged = 0;
-      ctl->broke = 0;
-      ctl->early = 0;
-      if (a.earlyExit && sqrt(t[0]) <= ctl->tol * bn) {
-        ctl->done = 1;
-        ctl->converged = 1;
-        ctl->early = 1;
-        ctl->resid = sqrt(t[0]) / bn;
+      stA(&ctl->slot[a.slot], mean0);
+      return;
+    case FIN_BNORM:  // multigrid.cpp:281, 292-298
+      bnorm = sqrt(t[0]) + 1e-300;
+      reset();
+      if (a.earlyExit && sqrt(t[0]) <= tol * bnorm) {
+        done = 1;
+        conv = 1;
+        early = 1;
+        resid = sqrt(t[0]) / bnorm;
       }
       break;
-    }
     case FIN_RZ:
-      ctl->rz = t[0];
+      rz = t[0];
       break;
     case FIN_ALPHA:
-      ctl->pAp = t[0];
+      pAp = t[0];
       if (fabs(t[0]) < 1e-300) {
-        ctl->done = 1;
-        ctl->broke = 1;
+        done = 1;
+        broke = 1;
       } else {
-        ctl->alpha = ctl->rz / t[0];
+        alpha = rz / t[0];
       }
       break;
     case FIN_CONV:
-      ctl->rr = t[0];
-      if (sqrt(t[0]) <= ctl->tol * ctl->bnorm) {
-        ctl->it += 1;
-        ctl->converged = 1;
-        ctl->done = 1;
+      rr = t[0];
+      if (sqrt(t[0]) <= tol * bnorm) {
+        it += 1;
+        conv = 1;
+        done = 1;
       }
       break;
     case FIN_BETA_FLEX:
-      ctl->beta = (t[0] - t[1]) / ctl->rz;
-      ctl->rz = t[0];
-      ctl->it += 1;
+      beta = (t[0] - t[1]) / rz;
+      rz = t[0];
+      it += 1;
       break;
     case FIN_BETA:
-      ctl->beta = t[0] / ctl->rz;
-      ctl->rz = t[0];
-      ctl->it += 1;
+      beta = t[0] / rz;
+      rz = t[0];
+      it += 1;
       break;
     case FIN_RESID:
-      if (ctl->early) break;  // returned before the loop: keep that report
-      ctl->resid = sqrt(t[0]) / ctl->bnorm;
-      ctl->converged = (ctl->converged || ctl->resid <= a.convTol) ? 1 : 0;
+      if (early) return;  // returned before the loop: keep that report
+      resid = sqrt(t[0]) / bnorm;
+      conv = (conv || resid <= a.convTol) ? 1 : 0;
       break;
     case FIN_BNORM_MEAN:
-      ctl->bnorm = sqrt(t[0]) + 1e-300;
-      ctl->slot[a.slot] = a.nf > 0 ? t[1] / double(a.nf) : 0.0;
-      ctl->it = 0;
-      ctl->done = 0;
-      ctl->converged = 0;
-      ctl->broke = 0;
-      ctl->early = 0;
+      bnorm = sqrt(t[0]) + 1e-300;
+      stA(&ctl->slot[a.slot], mean1);
+      reset();
       break;
     case FIN_CONV_MEAN:
-      ctl->slot[a.slot] = a.nf > 0 ? t[1] / double(a.nf) : 0.0;
-      ctl->rr = t[0];
-      if (sqrt(t[0]) <= ctl->tol * ctl->bnorm) {
-        ctl->it += 1;
-        ctl->converged = 1;
-        ctl->done = 1;
+      stA(&ctl->slot[a.slot], mean1);
+      rr = t[0];
+      if (sqrt(t[0]) <= tol * bnorm) {
+        it += 1;
+        conv = 1;
+        done = 1;
       }
       break;
+  }
+  stA(&ctl->bnorm, bnorm);
+  stA(&ctl->rz, rz);
+  stA(&ctl->pAp, pAp);
+  stA(&ctl->alpha, alpha);
+  stA(&ctl->beta, beta);
+  stA(&ctl->rr, rr);
+  stA(&ctl->resid, resid);
+  stA(&ctl->it, it);
+  stA(&ctl->converged, conv);
+  stA(&ctl->broke, broke);
+  stA(&ctl->early, early);
+  stA(&ctl->done, done);
+}
+
+// Fresh solve scalars written by a kernel (not a host copy), so they reach
+// memory through the same ordered, cache-coherent path as every later write.
+__global__ void k_ctl_init(SolveCtl* ctl, double tol) {
+  const int t = threadIdx.x;
+  if (t < 64) stA(&ctl->slot[t], 0.0);
+  if (t == 0) {
+    stA(&ctl->tol, tol);
+    for (double* f : {&ctl->bnorm, &ctl->rz, &ctl->pAp, &ctl->alpha, &ctl->beta, &ctl->rr, &ctl->resid}) stA(f, 0.0);
+    for (int* f : {&ctl->it, &ctl->done, &ctl->converged, &ctl->broke, &ctl->early, &ctl->pad_}) stA(f, 0);
   }
 }
 
@@ -1029,6 +1057,11 @@ __global__ void __launch_bounds__(kFinThreads) k_fin(RedBuf rb, FinArgs a, Solve
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
+void launch_ctl_init(SolveCtl* ctl, double tol, hipStream_t s) {
+  k_ctl_init<<<1, 64, 0, s>>>(ctl, tol);
+  NBL_LAUNCH_CHECK();
+}
+
 void launch_fin(RedBuf rb, const FinArgs& a, SolveCtl* ctl, hipStream_t s) {
   k_fin<<<1, kFinThreads, 0, s>>>(rb, a, ctl);
   NBL_LAUNCH_CHECK();

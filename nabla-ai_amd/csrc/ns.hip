@@ -433,7 +433,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs_rk(NsArgs A, PadBufs pb, const d
   }
   const double* a0 = comp == 0 ? u0 : (comp == 1 ? v0 : w0);
   double* ao = comp == 0 ? uo : (comp == 1 ? vo : wo);
-  const double dt = sc->dt;
+  const double dt = ldA(&sc->dt);
   double out;
   if (stage == 0) out = R;
   else if (stage == 1) out = a0[t] + dt * R;
@@ -603,7 +603,7 @@ __global__ void __launch_bounds__(kBlock) k_correct(NsArgs A, double* __restrict
     return;
   }
   t -= nw;
-  if (t < nc) p[t] = phi[t] / sc->dt;
+  if (t < nc) p[t] = phi[t] / ldA(&sc->dt);
 }
 
 // CFL time step (ns_solver.cpp:581-603): per-block max over fluid cells, then
@@ -645,11 +645,19 @@ __global__ void __launch_bounds__(kFinThreads) k_cfl_fin(NsArgs A, RedBuf rb, in
   foldPartials<1, 1u>(rb.partials, rb.stride, nparts, t);
   if (threadIdx.x != 0) return;
   const double mx = fmax(viscTerm(A) + 1e-300, t[0]);
-  sc->cfl_max = mx;
-  sc->dt = A.cfl / mx;
+  stA(&sc->cfl_max, mx);
+  stA(&sc->dt, A.cfl / mx);
 }
 
-__global__ void k_set_dt(StepCtl* sc, double dt) { sc->dt = dt; }
+__global__ void k_set_dt(StepCtl* sc, double dt) { stA(&sc->dt, dt); }
+
+__global__ void k_set_step(StepCtl* sc, int step, double t, double dt) {
+  stA(&sc->step, step);
+  stA(&sc->t, t);
+  stA(&sc->dt, dt);
+  stA(&sc->pad_, 0);
+  stA(&sc->cfl_max, 0.0);
+}
 
 __global__ void k_fill(double* a, long long n, double v) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -742,7 +750,7 @@ __global__ void __launch_bounds__(kFinThreads) k_diag_fin(NsArgs A, RedBuf rb, i
   double tot[7];
   foldPartials<7, 1u << 6>(rb.partials, rb.stride, nparts, tot);
   if (threadIdx.x != 0) return;
-  const double dt = sc->dt;
+  const double dt = ldA(&sc->dt);
   const double n = nu + nv + nw;
   const double mom = sqrt(tot[0] / n) / (dt * A.velocityScale + 1e-300);
   const double divl2 = tot[2] > 0.0 ? sqrt(tot[1] / tot[2]) : 0.0;
@@ -755,10 +763,12 @@ __global__ void __launch_bounds__(kFinThreads) k_diag_fin(NsArgs A, RedBuf rb, i
     cd = tot[4] / qd;
     cl = tot[5] / qd;
   }
-  sc->step += 1;
-  sc->t += dt;
-  out->step = sc->step;
-  out->t = sc->t;
+  const int step = ldA(&sc->step) + 1;
+  const double tnow = ldA(&sc->t) + dt;
+  stA(&sc->step, step);
+  stA(&sc->t, tnow);
+  out->step = step;
+  out->t = tnow;
   out->dt = dt;
   out->cfl = A.cfl;
   out->mom = mom;
@@ -900,6 +910,11 @@ void launch_diagnostics(const NsArgs& a, const double* u, const double* v, const
   const int g = gridFor(n, kBlock, kNsBlocks);
   k_diagnostics<<<g, kBlock, 0, s>>>(a, u, v, w, u0, v0, w0, p, sc, out, rb);
   k_diag_fin<<<1, kFinThreads, 0, s>>>(a, rb, g, sc, out);
+  NBL_LAUNCH_CHECK();
+}
+
+void launch_set_step(StepCtl* sc, int step, double t, double dt, hipStream_t s) {
+  k_set_step<<<1, 1, 0, s>>>(sc, step, t, dt);
   NBL_LAUNCH_CHECK();
 }
 

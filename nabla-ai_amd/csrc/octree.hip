@@ -261,7 +261,7 @@ __global__ void __launch_bounds__(kBlock) k_o_init2(long long n, long long nf, c
                                                     const double* __restrict__ diag, double* __restrict__ rhs,
                                                     double* __restrict__ r, double* __restrict__ z,
                                                     const SolveCtl* ctl, RedBuf rb) {
-  const double m = ctl->slot[O_MEAN_RHS];
+  const double m = ldA(&ctl->slot[O_MEAN_RHS]);
   double acc[2] = {0.0, 0.0};
   OGRID(c, n) {
     const bool fl = mask[c] == 0;
@@ -280,8 +280,8 @@ __global__ void __launch_bounds__(kBlock) k_o_init2(long long n, long long nf, c
 __global__ void __launch_bounds__(kBlock) k_o_zdot(long long n, long long nf, const uint8_t* __restrict__ mask,
                                                    const double* __restrict__ r, double* __restrict__ z,
                                                    double* __restrict__ p, int first, const SolveCtl* ctl, RedBuf rb) {
-  if (ctl->done) return;
-  const double m = ctl->slot[O_MEAN_Z];
+  if (ldA(&ctl->done)) return;
+  const double m = ldA(&ctl->slot[O_MEAN_Z]);
   double acc[1] = {0.0};
   OGRID(c, n) {
     const double zv = mask[c] == 0 ? (nf ? z[c] - m : z[c]) : 0.0;
@@ -296,7 +296,7 @@ __global__ void __launch_bounds__(kBlock) k_o_apply_dot(long long n, const uint8
                                                         const int32_t* __restrict__ rowPtr, const int32_t* __restrict__ col,
                                                         const double* __restrict__ coef, const double* __restrict__ p,
                                                         double* __restrict__ Ap, const SolveCtl* ctl, RedBuf rb) {
-  if (ctl->done) return;
+  if (ldA(&ctl->done)) return;
   double acc[1] = {0.0};
   OGRID(c, n) {
     const double v = octRow(c, mask, rowPtr, col, coef, p, 0.0, false);
@@ -311,8 +311,8 @@ __global__ void __launch_bounds__(kBlock) k_o_update(long long n, const uint8_t*
                                                      double* __restrict__ x, const double* __restrict__ p,
                                                      double* __restrict__ r, const double* __restrict__ Ap,
                                                      const SolveCtl* ctl, RedBuf rb) {
-  if (ctl->done) return;
-  const double a = ctl->alpha;
+  if (ldA(&ctl->done)) return;
+  const double a = ldA(&ctl->alpha);
   double acc[1] = {0.0};
   OGRID(c, n) {
     x[c] = x[c] + a * p[c];
@@ -328,8 +328,8 @@ __global__ void __launch_bounds__(kBlock) k_o_update(long long n, const uint8_t*
 __global__ void __launch_bounds__(kBlock) k_o_rnorm(long long n, long long nf, const uint8_t* __restrict__ mask,
                                                     const double* __restrict__ diag, double* __restrict__ r,
                                                     double* __restrict__ z, const SolveCtl* ctl, RedBuf rb) {
-  if (ctl->done) return;
-  const double m = ctl->slot[O_MEAN_R];
+  if (ldA(&ctl->done)) return;
+  const double m = ldA(&ctl->slot[O_MEAN_R]);
   double acc[2] = {0.0, 0.0};
   OGRID(c, n) {
     const bool fl = mask[c] == 0;
@@ -345,8 +345,8 @@ __global__ void __launch_bounds__(kBlock) k_o_rnorm(long long n, long long nf, c
 
 __global__ void __launch_bounds__(kBlock) k_o_pupdate(long long n, double* __restrict__ p, const double* __restrict__ z,
                                                       const SolveCtl* ctl) {
-  if (ctl->done) return;
-  const double be = ctl->beta;
+  if (ldA(&ctl->done)) return;
+  const double be = ldA(&ctl->beta);
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c < n) p[c] = z[c] + be * p[c];
 }
@@ -363,7 +363,7 @@ __global__ void __launch_bounds__(kBlock) k_o_final(long long n, long long nf, c
                                                     const double* __restrict__ coef, const double* __restrict__ x,
                                                     const double* __restrict__ rhs, double* __restrict__ xo,
                                                     const SolveCtl* ctl, RedBuf rb) {
-  const double m = ctl->slot[O_MEAN_X];
+  const double m = ldA(&ctl->slot[O_MEAN_X]);
   const bool hasM = nf > 0;
   double acc[1] = {0.0};
   OGRID(c, n) {
@@ -548,9 +548,7 @@ nbl_poisson_report OctreeDev::solveSorted(const double* b, double* xo, double to
   const RedBuf rb{red_.as<double>(), kOctBlocks};
   const long long N = (long long)n_, nf = (long long)nf_;
   const int g = ogrid(N);
-  hctl_[1] = SolveCtl{};
-  hctl_[1].tol = tol;
-  NBL_HIP(hipMemcpyAsync(ctl, &hctl_[1], sizeof(SolveCtl), hipMemcpyHostToDevice, s));
+  launch_ctl_init(ctl, tol, s);
   NBL_HIP(hipEventRecord(e0_, s));
   const uint8_t* mk = mask_.as<uint8_t>();
   const int32_t* rp = rowPtr_.as<int32_t>();

@@ -1,0 +1,83 @@
+// serve.hpp — request loop over concurrent device-resident runs, and batched
+// pressure solves (no counterpart in the reference, whose LocalJobRunner runs
+// one subprocess per job).
+#pragma once
+
+#include <chrono>
+#include <condition_variable>
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+#include <deque>
+
+#include "solver.hpp"
+
+namespace nbl {
+
+struct CapacityError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+class Server {
+ public:
+  Server(nbl_ctx* ctx, int slots, uint64_t budget);
+  ~Server();
+  int admit(const HostGrid& g, const nbl_flow_cfg& cfg, const nbl_solver_opts& opts);
+  nbl_run* run(int id);  // only while the run has no request in flight
+  uint64_t submit(int id);
+  void wait(uint64_t ticket, nbl_step_report* rep, double* latencyMs);
+  void footprint(uint64_t* used, uint64_t* budget) const;
+
+ private:
+  struct Req {
+    uint64_t ticket = 0;
+    int run = -1;
+    std::chrono::steady_clock::time_point arrival;
+  };
+  struct Slot {
+    std::unique_ptr<nbl_run> run;
+    uint64_t bytes = 0;
+    bool busy = false;
+    int pending = 0;
+  };
+  struct Done {
+    nbl_step_report rep{};
+    double latencyMs = 0.0;
+    std::string error;
+  };
+  void work();
+
+  nbl_ctx* ctx_;
+  uint64_t budget_, used_ = 0;
+  mutable std::mutex mu_;
+  std::mutex admitMu_;
+  std::condition_variable cv_, doneCv_;
+  std::vector<std::unique_ptr<Slot>> runs_;
+  std::deque<Req> queue_;
+  std::unordered_map<uint64_t, Done> done_;
+  std::unordered_set<uint64_t> live_;  // submitted, not yet waited for
+  std::vector<std::thread> workers_;
+  uint64_t nextTicket_ = 1;
+  bool stop_ = false;
+};
+
+void solveBatch(nbl_poisson* const* ps, const double* const* b, double* const* x, int n, int concurrency,
+                nbl_poisson_report* reps);
+}  // namespace nbl
+struct nbl_octree;
+namespace nbl {
+void octreeSolveBatch(nbl_octree* const* os, const double* const* b, double* const* x, int n, int concurrency,
+                      double tol, int maxIters, nbl_poisson_report* reps);
+
+}  // namespace nbl
+
+struct nbl_server {
+  nbl_ctx* ctx;
+  std::unique_ptr<nbl::Server> impl;
+};

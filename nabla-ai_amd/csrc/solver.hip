@@ -335,7 +335,9 @@ void PoissonDev::vcycle(int l, const T* b, const double* mb, const T** xout, con
                    project_ ? slotPtr(ctl, slotX(l, 1)) : nullptr};
   int cur = -1;
   int q0 = 0;
-  if (!project_ && opts_.pre_smooth >= 2) {  // sweeps 1 and 2 in one pass
+  // sweeps 1 and 2 in one pass where launches, not arithmetic, bound the level
+  // (the pass re-evaluates the pointwise first sweep at every stencil point)
+  if (!project_ && opts_.pre_smooth >= 2 && L.n < (1LL << 18)) {
     launch_mg_sweep_pair<T>(L, b, bufs[0], ctl, s);
     cur = 0;
     q0 = 2;
@@ -433,6 +435,20 @@ void PoissonDev::enqueueIteration(hipStream_t s) {
   }
 }
 
+// z = Minv r alone (multigrid.cpp:283-289): one V-cycle from zero and the
+// level-0 projection.  d_r and d_z are device vectors of the level-0 size.
+void PoissonDev::precondition(const double* d_r, double* d_z, hipStream_t s) {
+  if (kind_ != 0) throw std::invalid_argument("precondition: mg-pcg solver only");
+  SolveCtl* ctl = ctl_.as<SolveCtl>();
+  const MgLevel& L = dl_[0];
+  launch_ctl_init(ctl, tol_, s);
+  minv(d_r, s, false);
+  // z - pending mean (projection) into z, p; the partials it writes are unused
+  launch_fcg_start(L, z_.as<double>(), project_ ? slotPtr(ctl, kSlotMeanZ) : nullptr, d_r, p_.as<double>(), ctl,
+                   red(), s);
+  NBL_HIP(hipMemcpyAsync(d_z, z_.as<void>(), size_t(L.n) * 8, hipMemcpyDeviceToDevice, s));
+}
+
 void PoissonDev::readCtl(hipStream_t s) {
   NBL_HIP(hipMemcpyAsync(hctl_, ctl_.as<void>(), sizeof(SolveCtl), hipMemcpyDeviceToHost, s));
   NBL_HIP(hipStreamSynchronize(s));
@@ -446,9 +462,7 @@ nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t
   const MgLevel& L = dl_[0];
   const RedBuf rb = red();
   const int nb = blocks_[0];
-  hctl_[1] = SolveCtl{};
-  hctl_[1].tol = tol_;
-  NBL_HIP(hipMemcpyAsync(ctl_.as<void>(), &hctl_[1], sizeof(SolveCtl), hipMemcpyHostToDevice, s));
+  launch_ctl_init(ctl, tol_, s);
   NBL_HIP(hipEventRecord(e0_, s));
   vcount_ = 0;
   double* x = x_.as<double>();
@@ -508,8 +522,7 @@ void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double
   double* m1 = project_ ? slotPtr(ctl, slotX(0, 1)) : nullptr;
   // a live solve leaves ctl->done set; clear it (and every pending mean) so the
   // kernels do their work
-  hctl_[1] = SolveCtl{};
-  NBL_HIP(hipMemcpyAsync(ctl, &hctl_[1], sizeof(SolveCtl), hipMemcpyHostToDevice, s));
+  launch_ctl_init(ctl, 0.0, s);
   NBL_HIP(hipMemsetAsync(lx0_[0].as<void>(), 0, L.n * size_t(sz), s));
   NBL_HIP(hipMemsetAsync(lx1_[0].as<void>(), 0, L.n * size_t(sz), s));
   NBL_HIP(hipMemsetAsync(r_.as<void>(), 0, L.n * 8, s));
@@ -711,10 +724,7 @@ void RunDev::initialize() {  // ns_solver.cpp:95-112
       break;
     }
   bc(true);
-  hsc_->step = 0;
-  hsc_->t = 0.0;
-  hsc_->dt = 0.0;
-  NBL_HIP(hipMemcpyAsync(sc_.as<void>(), hsc_, sizeof(StepCtl), hipMemcpyHostToDevice, stream_));
+  launch_set_step(sc_.as<StepCtl>(), 0, 0.0, 0.0, stream_);
   NBL_HIP(hipStreamSynchronize(stream_));
   dtLast_ = 0.0;
 }
@@ -728,10 +738,7 @@ void RunDev::setState(const double* u, size_t nu, const double* v, size_t nv, co
   for (int q = 0; q < 3; ++q)
     NBL_HIP(hipMemcpyAsync(cur_[q], src[q], sz_[q] * 8, hipMemcpyHostToDevice, stream_));
   NBL_HIP(hipMemcpyAsync(p_.as<void>(), p, sz_[3] * 8, hipMemcpyHostToDevice, stream_));
-  hsc_->step = step;
-  hsc_->t = t;
-  hsc_->dt = dtLast;
-  NBL_HIP(hipMemcpyAsync(sc_.as<void>(), hsc_, sizeof(StepCtl), hipMemcpyHostToDevice, stream_));
+  launch_set_step(sc_.as<StepCtl>(), step, t, dtLast, stream_);
   NBL_HIP(hipStreamSynchronize(stream_));
   dtLast_ = dtLast;
 }

@@ -84,6 +84,8 @@ class PoissonDev {
   // Solve with device buffers on `stream` (b and x may be any device arrays of
   // length n).  Returns the reference's Report.
   nbl_poisson_report solve(const double* d_b, double* d_x, hipStream_t stream);
+  // One application of the preconditioner (device vectors).
+  void precondition(const double* d_r, double* d_z, hipStream_t stream);
   const std::vector<HostLevel>& levels() const { return hl_; }
   const MgLevel& level(int l) const { return dl_[l]; }
   int nlevels() const { return int(dl_.size()); }

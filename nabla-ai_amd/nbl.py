@@ -195,6 +195,7 @@ def load(path: os.PathLike | None = None):
     _proto(lib, "nbl_poisson_destroy", i, V)
     _proto(lib, "nbl_poisson_solve", i, V, P_D, P_D, C.POINTER(PoissonReport))
     _proto(lib, "nbl_poisson_solve_device", i, V, C.c_void_p, C.c_void_p, C.POINTER(PoissonReport))
+    _proto(lib, "nbl_poisson_precondition", i, V, P_D, P_D)
     _proto(lib, "nbl_poisson_apply", i, V, P_D, P_D)
     _proto(lib, "nbl_poisson_diagonal", i, V, P_D)
     _proto(lib, "nbl_poisson_levels", i, V, C.POINTER(i), C.POINTER(i))
@@ -219,18 +220,33 @@ def load(path: os.PathLike | None = None):
     _proto(lib, "nbl_octree_poisson_solve", i, V, P_D, P_D, d, i, C.POINTER(PoissonReport))
     _proto(lib, "nbl_octree_profile", i, V, i, i, C.POINTER(C.c_double), C.POINTER(C.c_double))
     _proto(lib, "nbl_octree_last_timing", i, V, C.POINTER(C.c_double), C.POINTER(C.c_double))
+    _proto(lib, "nbl_server_create", i, V, i, C.c_uint64, C.POINTER(V))
+    _proto(lib, "nbl_server_destroy", i, V)
+    _proto(lib, "nbl_server_admit", i, V, V, V, V, C.POINTER(C.c_int))
+    _proto(lib, "nbl_server_run", i, V, i, C.POINTER(V))
+    _proto(lib, "nbl_server_submit", i, V, i, C.POINTER(C.c_uint64))
+    _proto(lib, "nbl_server_wait", i, V, C.c_uint64, C.POINTER(StepReport), C.POINTER(C.c_double))
+    _proto(lib, "nbl_server_footprint", i, V, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64))
+    _proto(lib, "nbl_poisson_solve_batch", i, C.POINTER(V), C.POINTER(P_D), C.POINTER(P_D), i, i,
+           C.POINTER(PoissonReport))
+    _proto(lib, "nbl_octree_solve_batch", i, C.POINTER(V), C.POINTER(P_D), C.POINTER(P_D), i, i, d, i,
+           C.POINTER(PoissonReport))
     _lib = lib
     return lib
 
 
 class NblError(RuntimeError):
-    pass
+    """A non-zero status from the C-ABI; `code` is the NBL_ERR_* value."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"nbl error {code}: {msg}")
+        self.code = code
 
 
 def check(rc: int):
     if rc != 0:
         msg = _lib.nbl_last_error().decode() if _lib else "?"
-        raise NblError(f"nbl error {rc}: {msg}")
+        raise NblError(rc, msg)
 
 
 class Ctx:
@@ -318,9 +334,9 @@ class Run:
         return out
 
     def close(self):
-        if self.h:
+        if self.h and not getattr(self, "_borrowed", False):
             check(self.lib.nbl_run_destroy(self.h))
-            self.h = C.c_void_p()
+        self.h = C.c_void_p()
 
 
 class Poisson:
@@ -349,6 +365,12 @@ class Poisson:
         check(self.lib.nbl_poisson_solve_device(self.h, C.c_void_p(d_b), C.c_void_p(d_x),
                                                 C.byref(rep)))
         return rep
+
+    def precondition(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.empty(self.grid.cells)
+        check(self.lib.nbl_poisson_precondition(self.h, dptr(r), dptr(z)))
+        return z
 
     def apply(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)
@@ -526,3 +548,76 @@ class Octree:
         if self.h:
             check(self.lib.nbl_octree_destroy(self.h))
             self.h = C.c_void_p()
+
+
+class Server:
+    """nbl_server: device-resident request loop over concurrent runs."""
+
+    def __init__(self, ctx: Ctx, slots: int = 4, hbm_budget: int = 64 << 30):
+        self.lib = ctx.lib
+        self.h = C.c_void_p()
+        check(self.lib.nbl_server_create(ctx.h, slots, hbm_budget, C.byref(self.h)))
+        self._keep = []
+
+    def admit(self, grid: Grid, cfg: FlowCfg, opts: SolverOpts | None = None) -> int:
+        d = grid.desc()
+        rid = C.c_int()
+        check(self.lib.nbl_server_admit(self.h, C.byref(d), C.byref(cfg),
+                                        C.byref(opts) if opts is not None else None, C.byref(rid)))
+        self._keep.append(grid)
+        return rid.value
+
+    def run(self, run_id: int, grid: Grid) -> "Run":
+        """A borrowed handle (owned by the server) for set/get state."""
+        h = C.c_void_p()
+        check(self.lib.nbl_server_run(self.h, run_id, C.byref(h)))
+        r = Run.__new__(Run)
+        r.lib, r.grid, r.h, r._borrowed = self.lib, grid, h, True
+        return r
+
+    def submit(self, run_id: int) -> int:
+        t = C.c_uint64()
+        check(self.lib.nbl_server_submit(self.h, run_id, C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int):
+        rep = StepReport()
+        ms = C.c_double()
+        check(self.lib.nbl_server_wait(self.h, ticket, C.byref(rep), C.byref(ms)))
+        return rep, ms.value
+
+    def footprint(self):
+        u, b = C.c_uint64(), C.c_uint64()
+        check(self.lib.nbl_server_footprint(self.h, C.byref(u), C.byref(b)))
+        return u.value, b.value
+
+    def close(self):
+        if self.h:
+            check(self.lib.nbl_server_destroy(self.h))
+            self.h = C.c_void_p()
+
+
+def solve_batch(solvers, rhs, concurrency=4):
+    """nbl_poisson_solve_batch over Poisson objects and host vectors."""
+    n = len(solvers)
+    hs = (C.c_void_p * n)(*[p.h for p in solvers])
+    bs = [np.ascontiguousarray(b, dtype=np.float64) for b in rhs]
+    xs = [np.empty(p.grid.cells) for p in solvers]
+    bp = (P_D * n)(*[dptr(b) for b in bs])
+    xp = (P_D * n)(*[dptr(x) for x in xs])
+    reps = (PoissonReport * n)()
+    check(load().nbl_poisson_solve_batch(hs, bp, xp, n, concurrency, reps))
+    return xs, list(reps)
+
+
+def octree_solve_batch(octs, rhs, tol=1e-9, max_iters=2000, concurrency=4):
+    """nbl_octree_solve_batch over Octree objects and host vectors."""
+    n = len(octs)
+    hs = (C.c_void_p * n)(*[o.h for o in octs])
+    bs = [np.ascontiguousarray(b, dtype=np.float64) for b in rhs]
+    xs = [np.empty(o.n) for o in octs]
+    bp = (P_D * n)(*[dptr(b) for b in bs])
+    xp = (P_D * n)(*[dptr(x) for x in xs])
+    reps = (PoissonReport * n)()
+    check(load().nbl_octree_solve_batch(hs, bp, xp, n, concurrency, tol, max_iters, reps))
+    return xs, list(reps)

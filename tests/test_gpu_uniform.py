@@ -259,3 +259,22 @@ def test_mixed_steps_track_fp64(ctx):
         assert b.cd == pytest.approx(a.cd, rel=1e-6)
     for fa, fb in zip(s64[:3], s32[:3]):
         assert rel(fb, fa) < 1e-7
+
+
+@pytest.mark.parametrize("n", [24, 32])
+def test_singular_mg_is_deterministic(ctx, n):
+    """Repeated solves of a singular system (3-D restriction blocks, lazy mean
+    projections) return the same bits; a partial written by two threads of a
+    3-D block once made these differ in the last place."""
+    g, _ = cases.taylor_green(n)
+    b = cases.poisson_rhs(g, 3)
+    ps = nbl.Poisson(ctx, g, kind=0, tol=1e-8, max_iters=100)
+    try:
+        xs = [ps.solve(b)[0] for _ in range(8)]
+        zs = [ps.precondition(b - b.mean()) for _ in range(4)]
+    finally:
+        ps.close()
+    for x in xs[1:]:
+        assert np.array_equal(x, xs[0])
+    for z in zs[1:]:
+        assert np.array_equal(z, zs[0])
