@@ -42,12 +42,14 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_steps(ctx, g, cfg, st, precision, warmup, steps):
+def run_steps(ctx, g, cfg, st, precision, warmup, steps, timing=None):
     run = nbl.Run(ctx, g, cfg, nbl.solver_opts(precision=precision))
     run.set_state(*st)
     for _ in range(warmup):
         run.step()
     reps, solve_ms, iters = [], [], []
+    if timing is not None:
+        run.sweep_timing(1)
     t0 = time.perf_counter()
     for s in range(steps):
         r = run.step()
@@ -58,6 +60,8 @@ def run_steps(ctx, g, cfg, st, precision, warmup, steps):
         if (s + 1) % max(1, steps // 5) == 0:
             log(f"prec={precision} step {s + 1}/{steps}")
     elapsed = time.perf_counter() - t0
+    if timing is not None:
+        timing.update(zip(("avg_ms", "launches", "bytes"), run.sweep_timing(0)))
     state = run.get_state()
     run.close()
     return reps, solve_ms, iters, elapsed, state
@@ -72,10 +76,36 @@ def roofline(ctx, g, reps=200):
     ms, by = ps.profile(0, reps)
     ps.close()
     ach = by / (ms * 1e-3) / 1e9
+    key = f"{g.nx}x{g.ny}x{g.nz}" if g.nx != 192 else "cube192x64x64"
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(key),
             "kernel": "k_sweep<double> (finest-level damped-Jacobi sweep; fluid-sum fold when singular)",
             "avg_launch_us": round(ms * 1e3, 3), "bytes_per_launch": by, "cells": g.cells}
+
+
+def pmc_traffic(key):
+    """HBM bytes per launch from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE passes, copy-calibrated), or None."""
+    p = ROOT / "profiles" / "k_sweep_traffic.json"
+    if not p.exists():
+        return None
+    try:
+        d = json.loads(p.read_text()).get(key)
+        return None if d is None else d.get("bytes_per_launch")
+    except (ValueError, OSError):
+        return None
+
+
+def roofline_insitu(sweep, g):
+    """Finest-level plain sweep timed inside the timed steps (HIP events on the
+    run's stream around each launch that did work)."""
+    ms, n, by = sweep["avg_ms"], sweep["launches"], sweep["bytes"]
+    ach = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic("cube192x64x64"),
+            "kernel": "k_sweep<double, SW_PLAIN> on the 192x64x64 level (2 of the 4 finest sweeps per V-cycle)",
+            "avg_launch_us": round(ms * 1e3, 3), "launches_timed": n, "bytes_per_launch": by, "cells": g.cells,
+            "measured": "HIP events on the run's stream around each launch inside the timed steps"}
 
 
 def coarse_solve(ctx, g, reps=50):
@@ -93,8 +123,9 @@ def coarse_solve(ctx, g, reps=50):
 def octree_c3(ctx, with_cpu=True):
     """The cube-wake octree (4,294,942 leaves, levels 5..8, the cube's leaves
     InsideSolid): device setup (sort, validation, face table, operator), the
-    SpMV roofline, and a Jacobi-PCG solve to 1e-9 of a seeded mean-free
-    right-hand side (OctreePoisson::solve).  CPU side: the reference's
+    SpMV roofline, a Jacobi-PCG solve to 1e-9 of a seeded mean-free
+    right-hand side (OctreePoisson::solve) and the multigrid-preconditioned CG
+    on the same system (hierarchy built beforehand, reported separately).  CPU side: the reference's
     faceNeighbors sweep over all leaves and one applyA, on one core."""
     t0 = time.perf_counter()
     t = nbl.build_tree(cases.C3_ORIGIN, cases.C3_EXTENT, 8, 5, cases.c3_predicate)
@@ -109,8 +140,11 @@ def octree_c3(ctx, with_cpu=True):
     b = cases.normal(11, k.size)
     b[~fl] = 0.0
     b[fl] -= b[fl].mean()
-    _x, rep = d.solve(b, 1e-9, 20000)
+    xj, rep = d.solve(b, 1e-9, 20000)
     _, solve_ms = d.last_timing()
+    levels, mg_build_ms = d.mg_levels()
+    xm, mrep = d.mg_solve(b, 1e-9, 500)
+    _, mg_ms = d.last_timing()
     d.close()
     ach = by / (ms * 1e-3) / 1e9
     out = {"leaves": n, "couplings": nnz, "host_build_s": round(host_build_s, 3),
@@ -118,10 +152,15 @@ def octree_c3(ctx, with_cpu=True):
            "jacobi_pcg": {"tol": 1e-9, "iterations": rep.iterations, "residual": rep.residual,
                           "converged": bool(rep.converged), "solve_ms": round(solve_ms, 3),
                           "ms_per_iteration": round(solve_ms / max(rep.iterations, 1), 5)},
+           "mg_pcg": {"tol": 1e-9, "levels": levels, "hierarchy_build_ms": round(mg_build_ms, 3),
+                      "iterations": mrep.iterations, "residual": mrep.residual, "converged": bool(mrep.converged),
+                      "solve_ms": round(mg_ms, 3), "speedup_vs_jacobi_pcg": round(solve_ms / max(mg_ms, 1e-9), 2),
+                      "rel_l2_vs_jacobi_pcg": rel(xm, xj)},
            "spmv_roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": round(ach / HBM_PEAK_GBS, 4), "avg_launch_us": round(ms * 1e3, 3),
                              "bytes_per_launch": by}}
-    log(f"octree: {n} leaves, setup {setup_ms:.1f} ms, {rep.iterations} iterations in {solve_ms:.1f} ms")
+    log(f"octree: {n} leaves, setup {setup_ms:.1f} ms, Jacobi-PCG {rep.iterations} iterations in {solve_ms:.1f} ms, "
+        f"MG-PCG {mrep.iterations} iterations in {mg_ms:.1f} ms")
     if with_cpu:
         import oracle_py
         kind = "reference" if oracle_py.available("ref") else "port"
@@ -405,7 +444,8 @@ def main():
         print(json.dumps(out), flush=True)
         return
     log(f"workload 192x64x64 cube channel, {g.cells} cells, warmup {a.warmup}, steps {a.steps}")
-    reps, solve_ms, iters, el, state = run_steps(ctx, g, cfg, st, nbl.PREC_FP64, a.warmup, a.steps)
+    sweep = {}
+    reps, solve_ms, iters, el, state = run_steps(ctx, g, cfg, st, nbl.PREC_FP64, a.warmup, a.steps, timing=sweep)
     ms_step = el / a.steps * 1e3
     value = g.cells * a.steps / el / 1e6
     log(f"fp64: {ms_step:.3f} ms/step, {value:.2f} Mcell-steps/s")
@@ -442,7 +482,8 @@ def main():
             "cl_max_abs_err": float(np.max(np.abs(mcl - cl))),
         }
         log(f"mixed: {mval:.2f} Mcell-steps/s")
-    out["roofline"] = roofline(ctx, g)
+    out["roofline"] = roofline_insitu(sweep, g)
+    out["roofline_isolated"] = roofline(ctx, g)
     out["coarse_solve"] = coarse_solve(ctx, g)
     big = nbl.Grid(256, 256, 256, 1.0 / 256)
     out["roofline_256cubed"] = roofline(ctx, big, reps=50)

@@ -281,6 +281,23 @@ int nbl_octree_diagonal(nbl_octree* oc, double* diag);
 /* OctreePoisson::solve (octree_poisson.cpp:107-197): Jacobi-PCG, singular. */
 int nbl_octree_poisson_solve(nbl_octree* oc, const double* b, double* x, double tol,
                              int max_iters, nbl_poisson_report* rep);
+/* Octree multigrid-preconditioned flexible CG on the OctreePoisson system.
+ * No counterpart in the reference, which solves it by Jacobi-PCG only
+ * (octree_poisson.cpp:107-197) and names octree levels as the intended
+ * multigrid levels (docs/adr/0007-*.md).  The iteration is MgPoisson::solve's
+ * (multigrid.cpp:267-355, singular form) with one V(2,2) cycle over the
+ * tree's coarsenings: level l+1 merges the finest leaves of level l into
+ * their parents, the operator is rediscretised with the same face
+ * coefficients, residuals restrict by summation, corrections prolong by
+ * injection, and the coarsest level (<= 2048 leaves) is solved exactly.
+ * Same arguments and report as nbl_octree_poisson_solve; the hierarchy is
+ * built on the first call. */
+int nbl_octree_mg_solve(nbl_octree* oc, const double* b, double* x, double tol, int max_iters,
+                        nbl_poisson_report* rep);
+/* Hierarchy of nbl_octree_mg_solve (built if needed): level count, leaves per
+ * level (up to cap entries) and the device time of its construction. */
+int nbl_octree_mg_levels(nbl_octree* oc, long long* counts, int cap, int* levels, double* build_ms);
+
 /* No counterpart in the reference: measurement.  Average ms and algorithmic
  * bytes of one launch of an octree kernel (kernel 0: the SpMV, reading row
  * pointers, masks, x and writing y once per leaf plus a column index and a
@@ -288,6 +305,14 @@ int nbl_octree_poisson_solve(nbl_octree* oc, const double* b, double* x, double 
  * validation, face table, operator) and of the last poisson_solve. */
 int nbl_octree_profile(nbl_octree* oc, int kernel, int reps, double* ms, double* bytes);
 int nbl_octree_last_timing(nbl_octree* oc, double* setup_ms, double* solve_ms);
+
+/* No counterpart in the reference: measurement.  In-situ timing of the finest
+ * level's plain smoother sweeps inside this run's pressure solves (an event
+ * pair around each such launch in the first V-cycle of every solve).
+ * enable: 1 on (resets the totals), 0 off (keeps them), -1 leave as is.
+ * Reports the average ms per launch, the number of launches timed and the
+ * sweep's algorithmic bytes (x, b, x' and the 2-byte code per cell). */
+int nbl_run_sweep_timing(nbl_run* run, int enable, double* avg_ms, long long* launches, double* bytes);
 
 /* ---- serving: device-resident request loop -------------------------------
  * No counterpart in the reference: its backend (backend/app/jobs.py

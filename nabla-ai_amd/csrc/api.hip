@@ -524,6 +524,25 @@ int nbl_octree_poisson_solve(nbl_octree* oc, const double* b, double* x, double 
   });
 }
 
+int nbl_octree_mg_solve(nbl_octree* oc, const double* b, double* x, double tol, int max_iters,
+                        nbl_poisson_report* rep) {
+  return guarded([&] {
+    need(oc && b && x, "NULL argument");
+    useDevice(oc->ctx);
+    const nbl_poisson_report r = oc->impl->solveMg(b, x, tol, max_iters);
+    if (rep) *rep = r;
+  });
+}
+
+int nbl_octree_mg_levels(nbl_octree* oc, long long* counts, int cap, int* levels, double* build_ms) {
+  return guarded([&] {
+    need(oc && levels, "NULL argument");
+    need(cap >= 0 && (cap == 0 || counts), "counts must hold cap entries");
+    useDevice(oc->ctx);
+    *levels = oc->impl->mgLevels(counts, cap, build_ms);
+  });
+}
+
 int nbl_octree_profile(nbl_octree* oc, int kernel, int reps, double* ms, double* bytes) {
   return guarded([&] {
     need(oc && ms && bytes, "NULL argument");
@@ -537,6 +556,14 @@ int nbl_octree_last_timing(nbl_octree* oc, double* setup_ms, double* solve_ms) {
     need(oc, "NULL argument");
     if (setup_ms) *setup_ms = oc->impl->setupMs();
     if (solve_ms) *solve_ms = oc->impl->lastSolveMs();
+  });
+}
+
+int nbl_run_sweep_timing(nbl_run* run, int enable, double* avg_ms, long long* launches, double* bytes) {
+  return guarded([&] {
+    need(run, "run is NULL");
+    if (enable >= 0) run->impl->poisson().setSweepTiming(enable != 0);
+    run->impl->poisson().sweepTiming(avg_ms, launches, bytes);
   });
 }
 

@@ -19,6 +19,7 @@
 #include <stdexcept>
 #include <vector>
 
+#include "octmg.hpp"
 #include "octree.hpp"
 
 namespace nbl {
@@ -376,16 +377,182 @@ __global__ void __launch_bounds__(kBlock) k_o_final(long long n, long long nf, c
   blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
 }
 
+__global__ void __launch_bounds__(kBlock) k_count_fluid(long long n, const uint8_t* __restrict__ mask,
+                                                        unsigned long long* __restrict__ cnt) {
+  __shared__ unsigned sh[kBlock / 64];
+  unsigned c = 0;
+  OGRID(i, n) c += mask[i] == 0 ? 1u : 0u;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kBlock / 64; ++w) t += sh[w];
+    atomicAdd(cnt, t);
+  }
+}
+
+// ---- multigrid-preconditioned flexible CG (multigrid.cpp:267-355 on leaves) --
+constexpr double kOmegaFirst = 0.8;  // first V-cycle sweep from zero: omega*r/diag
+
+// rhs <- rhs - mean (fluid), 0 elsewhere; r = rhs; first sweep of the
+// V-cycle; partials of |rhs|^2.
+__global__ void __launch_bounds__(kBlock) k_om_init2(long long n, long long nf, const uint8_t* __restrict__ mask,
+                                                     const double* __restrict__ diag, double* __restrict__ rhs,
+                                                     double* __restrict__ r, double* __restrict__ x0,
+                                                     const SolveCtl* ctl, RedBuf rb) {
+  const double m = ldA(&ctl->slot[O_MEAN_RHS]);
+  double acc[1] = {0.0};
+  OGRID(c, n) {
+    const bool fl = mask[c] == 0;
+    const double v = fl ? (nf ? rhs[c] - m : rhs[c]) : 0.0;
+    rhs[c] = v;
+    r[c] = v;
+    x0[c] = fl ? kOmegaFirst * v / diag[c] : 0.0;
+    acc[0] += v * v;
+  }
+  blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
+}
+
+// z = V-cycle output - mean (fluid), 0 elsewhere; p = z; partials of r.z.
+__global__ void __launch_bounds__(kBlock) k_om_start(long long n, long long nf, const uint8_t* __restrict__ mask,
+                                                     const double* __restrict__ zc, const double* __restrict__ r,
+                                                     double* __restrict__ z, double* __restrict__ p,
+                                                     const SolveCtl* ctl, RedBuf rb) {
+  if (ldA(&ctl->done)) return;
+  const double m = ldA(&ctl->slot[O_MEAN_Z]);
+  double acc[1] = {0.0};
+  OGRID(c, n) {
+    const double zv = mask[c] == 0 ? (nf ? zc[c] - m : zc[c]) : 0.0;
+    z[c] = zv;
+    p[c] = zv;
+    acc[0] += r[c] * zv;
+  }
+  blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
+}
+
+// x += alpha p; rn = r - alpha Ap; partial fluid sums of rn.
+__global__ void __launch_bounds__(kBlock) k_om_update(long long n, const uint8_t* __restrict__ mask,
+                                                      double* __restrict__ x, const double* __restrict__ p,
+                                                      const double* __restrict__ r, const double* __restrict__ Ap,
+                                                      double* __restrict__ rn, const SolveCtl* ctl, RedBuf rb) {
+  if (ldA(&ctl->done)) return;
+  const double a = ldA(&ctl->alpha);
+  double acc[1] = {0.0};
+  OGRID(c, n) {
+    x[c] = x[c] + a * p[c];
+    const double v = r[c] - a * Ap[c];
+    rn[c] = v;
+    if (mask[c] == 0) acc[0] += v;
+  }
+  blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
+}
+
+// rn <- rn - mean (fluid), 0 elsewhere; first sweep of the next V-cycle;
+// partials of |rn|^2.
+__global__ void __launch_bounds__(kBlock) k_om_rfix(long long n, long long nf, const uint8_t* __restrict__ mask,
+                                                    const double* __restrict__ diag, double* __restrict__ rn,
+                                                    double* __restrict__ x0, const SolveCtl* ctl, RedBuf rb) {
+  if (ldA(&ctl->done)) return;
+  const double m = ldA(&ctl->slot[O_MEAN_R]);
+  double acc[1] = {0.0};
+  OGRID(c, n) {
+    const bool fl = mask[c] == 0;
+    const double v = fl ? (nf ? rn[c] - m : rn[c]) : 0.0;
+    rn[c] = v;
+    x0[c] = fl ? kOmegaFirst * v / diag[c] : 0.0;
+    acc[0] += v * v;
+  }
+  blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
+}
+
+// z = V-cycle output - mean (fluid), 0 elsewhere; partials of z.rn and z.r.
+__global__ void __launch_bounds__(kBlock) k_om_dots(long long n, long long nf, const uint8_t* __restrict__ mask,
+                                                    const double* __restrict__ zc, const double* __restrict__ rn,
+                                                    const double* __restrict__ r, double* __restrict__ z,
+                                                    const SolveCtl* ctl, RedBuf rb) {
+  if (ldA(&ctl->done)) return;
+  const double m = ldA(&ctl->slot[O_MEAN_Z]);
+  double acc[2] = {0.0, 0.0};
+  OGRID(c, n) {
+    const double zv = mask[c] == 0 ? (nf ? zc[c] - m : zc[c]) : 0.0;
+    z[c] = zv;
+    acc[0] += zv * rn[c];
+    acc[1] += zv * r[c];
+  }
+  blockToPartials<2>(acc, rb.partials, rb.stride, blockIdx.x);
+}
+
+// p = z + beta p; r = rn.
+__global__ void __launch_bounds__(kBlock) k_om_pupdate(long long n, double* __restrict__ p, const double* __restrict__ z,
+                                                       double* __restrict__ r, const double* __restrict__ rn,
+                                                       const SolveCtl* ctl) {
+  if (ldA(&ctl->done)) return;
+  const double be = ldA(&ctl->beta);
+  OGRID(c, n) {
+    p[c] = z[c] + be * p[c];
+    r[c] = rn[c];
+  }
+}
+
 int ogrid(long long n) { return gridFor(n, kBlock, kOctBlocks); }
 
 }  // namespace
+
+void buildOctOperator(nbl_ctx* ctx, int maxL, const double org[3], const double fine[3], OctLevel& L,
+                      DevBuf* cntOut, DevBuf* nbrOut, hipStream_t s) {
+  const size_t n = L.n;
+  const long long N = (long long)n;
+  DevBuf cnt(ctx, 6 * n), nbr(ctx, 24 * n * sizeof(int32_t));
+  k_faces<<<gridFor(6 * N), kBlock, 0, s>>>(N, maxL, L.keys.as<uint64_t>(), L.lev.as<uint8_t>(), cnt.as<uint8_t>(),
+                                            nbr.as<int32_t>());
+  NBL_LAUNCH_CHECK();
+  // CSR of the fluid-fluid couplings
+  DevBuf rowCount(ctx, (n + 1) * sizeof(int32_t)), nfl(ctx, sizeof(unsigned long long));
+  L.rowPtr = DevBuf(ctx, (n + 1) * sizeof(int32_t));
+  NBL_HIP(hipMemsetAsync(rowCount.as<void>(), 0, (n + 1) * sizeof(int32_t), s));
+  NBL_HIP(hipMemsetAsync(nfl.as<void>(), 0, sizeof(unsigned long long), s));
+  k_row_count<<<gridFor(N), kBlock, 0, s>>>(N, L.mask.as<uint8_t>(), cnt.as<uint8_t>(), nbr.as<int32_t>(),
+                                            rowCount.as<int32_t>());
+  k_count_fluid<<<ogrid(N), kBlock, 0, s>>>(N, L.mask.as<uint8_t>(), nfl.as<unsigned long long>());
+  NBL_LAUNCH_CHECK();
+  size_t scanBytes = 0;
+  NBL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, rowCount.as<int32_t>(), L.rowPtr.as<int32_t>(),
+                                           int(n + 1), s));
+  DevBuf scanTmp(ctx, scanBytes);
+  NBL_HIP(hipcub::DeviceScan::ExclusiveSum(scanTmp.as<void>(), scanBytes, rowCount.as<int32_t>(),
+                                           L.rowPtr.as<int32_t>(), int(n + 1), s));
+  int32_t nnz = 0;
+  unsigned long long nf = 0;
+  NBL_HIP(hipMemcpyAsync(&nnz, L.rowPtr.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NBL_HIP(hipMemcpyAsync(&nf, nfl.as<void>(), sizeof(nf), hipMemcpyDeviceToHost, s));
+  NBL_HIP(hipStreamSynchronize(s));
+  L.nnz = size_t(nnz);
+  L.nf = size_t(nf);
+  L.col = DevBuf(ctx, std::max<size_t>(1, L.nnz) * sizeof(int32_t));
+  L.coef = DevBuf(ctx, std::max<size_t>(1, L.nnz) * sizeof(double));
+  L.diag = DevBuf(ctx, n * sizeof(double));
+  Geo G;
+  for (int a = 0; a < 3; ++a) {
+    G.org[a] = org[a];
+    G.fine[a] = fine[a];
+  }
+  G.maxL = maxL;
+  k_row_fill<<<gridFor(N), kBlock, 0, s>>>(N, G, L.keys.as<uint64_t>(), L.lev.as<uint8_t>(), L.mask.as<uint8_t>(),
+                                           cnt.as<uint8_t>(), nbr.as<int32_t>(), L.rowPtr.as<int32_t>(),
+                                           L.col.as<int32_t>(), L.coef.as<double>(), L.diag.as<double>());
+  NBL_LAUNCH_CHECK();
+  if (cntOut) *cntOut = std::move(cnt);
+  if (nbrOut) *nbrOut = std::move(nbr);
+  // a freed face table synchronises the device; the fill above has then completed
+}
 
 // ---------------------------------------------------------------------------
 // OctreeDev
 // ---------------------------------------------------------------------------
 OctreeDev::OctreeDev(nbl_ctx* ctx, const uint64_t* keys, const uint8_t* levels, const uint8_t* masks, size_t n,
                      const double origin[3], const double extent[3], int maxLevel)
-    : ctx_(ctx), n_(n), maxL_(maxLevel) {
+    : ctx_(ctx), maxL_(maxLevel) {
   if (n == 0) throw std::invalid_argument("octree: no leaves");
   if (maxLevel < 1 || maxLevel > 21) throw std::invalid_argument("Octree: maxLevel must be in [1, 21]");
   if (!(extent[0] > 0 && extent[1] > 0 && extent[2] > 0)) throw std::invalid_argument("Octree: extent must be positive");
@@ -401,10 +568,11 @@ OctreeDev::OctreeDev(nbl_ctx* ctx, const uint64_t* keys, const uint8_t* levels, 
   hipStream_t s = stream_;
   const long long N = (long long)n;
   DevBuf kin(ctx, n * 8), idx(ctx, n * 8), lin(ctx, n), min_(ctx, n);
-  keys_ = DevBuf(ctx, n * 8);
+  L0_.n = n;
+  L0_.keys = DevBuf(ctx, n * 8);
   perm_ = DevBuf(ctx, n * 8);
-  lev_ = DevBuf(ctx, n);
-  mask_ = DevBuf(ctx, n);
+  L0_.lev = DevBuf(ctx, n);
+  L0_.mask = DevBuf(ctx, n);
   NBL_HIP(hipMemcpyAsync(kin.as<void>(), keys, n * 8, hipMemcpyHostToDevice, s));
   NBL_HIP(hipMemcpyAsync(lin.as<void>(), levels, n, hipMemcpyHostToDevice, s));
   NBL_HIP(hipMemcpyAsync(min_.as<void>(), masks, n, hipMemcpyHostToDevice, s));
@@ -412,17 +580,17 @@ OctreeDev::OctreeDev(nbl_ctx* ctx, const uint64_t* keys, const uint8_t* levels, 
   k_iota<<<gridFor(N), kBlock, 0, s>>>(N, idx.as<uint64_t>());
   NBL_LAUNCH_CHECK();
   size_t tmpBytes = 0;
-  NBL_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpBytes, kin.as<uint64_t>(), keys_.as<uint64_t>(),
+  NBL_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpBytes, kin.as<uint64_t>(), L0_.keys.as<uint64_t>(),
                                              idx.as<uint64_t>(), perm_.as<uint64_t>(), int(n), 0, 64, s));
   DevBuf tmp(ctx, tmpBytes);
-  NBL_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.as<void>(), tmpBytes, kin.as<uint64_t>(), keys_.as<uint64_t>(),
+  NBL_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.as<void>(), tmpBytes, kin.as<uint64_t>(), L0_.keys.as<uint64_t>(),
                                              idx.as<uint64_t>(), perm_.as<uint64_t>(), int(n), 0, 64, s));
-  k_permute<uint8_t><<<gridFor(N), kBlock, 0, s>>>(N, perm_.as<uint64_t>(), lin.as<uint8_t>(), lev_.as<uint8_t>());
-  k_permute<uint8_t><<<gridFor(N), kBlock, 0, s>>>(N, perm_.as<uint64_t>(), min_.as<uint8_t>(), mask_.as<uint8_t>());
+  k_permute<uint8_t><<<gridFor(N), kBlock, 0, s>>>(N, perm_.as<uint64_t>(), lin.as<uint8_t>(), L0_.lev.as<uint8_t>());
+  k_permute<uint8_t><<<gridFor(N), kBlock, 0, s>>>(N, perm_.as<uint64_t>(), min_.as<uint8_t>(), L0_.mask.as<uint8_t>());
   NBL_LAUNCH_CHECK();
   DevBuf bad(ctx, 4);
   NBL_HIP(hipMemsetAsync(bad.as<void>(), 0, 4, s));
-  k_validate<<<gridFor(N), kBlock, 0, s>>>(N, maxLevel, keys_.as<uint64_t>(), lev_.as<uint8_t>(), bad.as<unsigned>());
+  k_validate<<<gridFor(N), kBlock, 0, s>>>(N, maxLevel, L0_.keys.as<uint64_t>(), L0_.lev.as<uint8_t>(), bad.as<unsigned>());
   NBL_LAUNCH_CHECK();
   unsigned hbad = 0;
   NBL_HIP(hipMemcpyAsync(&hbad, bad.as<void>(), 4, hipMemcpyDeviceToHost, s));
@@ -430,46 +598,8 @@ OctreeDev::OctreeDev(nbl_ctx* ctx, const uint64_t* keys, const uint8_t* levels, 
   if (hbad & 1u) throw std::invalid_argument("octree: level above maxLevel");
   if (hbad & 2u) throw std::invalid_argument("octree: key is not a valid anchor for its level");
   if (hbad & 4u) throw std::invalid_argument("octree: duplicate leaf key");
-  // face table
-  cnt_ = DevBuf(ctx, 6 * n);
-  nbr_ = DevBuf(ctx, 24 * n * sizeof(int32_t));
-  k_faces<<<gridFor(6 * N), kBlock, 0, s>>>(N, maxLevel, keys_.as<uint64_t>(), lev_.as<uint8_t>(), cnt_.as<uint8_t>(),
-                                            nbr_.as<int32_t>());
-  NBL_LAUNCH_CHECK();
-  // CSR of the fluid-fluid couplings
-  DevBuf rowCount(ctx, (n + 1) * sizeof(int32_t));
-  rowPtr_ = DevBuf(ctx, (n + 1) * sizeof(int32_t));
-  NBL_HIP(hipMemsetAsync(rowCount.as<void>(), 0, (n + 1) * sizeof(int32_t), s));
-  k_row_count<<<gridFor(N), kBlock, 0, s>>>(N, mask_.as<uint8_t>(), cnt_.as<uint8_t>(), nbr_.as<int32_t>(),
-                                            rowCount.as<int32_t>());
-  NBL_LAUNCH_CHECK();
-  size_t scanBytes = 0;
-  NBL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, rowCount.as<int32_t>(), rowPtr_.as<int32_t>(),
-                                           int(n + 1), s));
-  DevBuf scanTmp(ctx, scanBytes);
-  NBL_HIP(hipcub::DeviceScan::ExclusiveSum(scanTmp.as<void>(), scanBytes, rowCount.as<int32_t>(),
-                                           rowPtr_.as<int32_t>(), int(n + 1), s));
-  int32_t nnz = 0;
-  NBL_HIP(hipMemcpyAsync(&nnz, rowPtr_.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NBL_HIP(hipStreamSynchronize(s));
-  nnz_ = size_t(nnz);
-  col_ = DevBuf(ctx, std::max<size_t>(1, nnz_) * sizeof(int32_t));
-  coef_ = DevBuf(ctx, std::max<size_t>(1, nnz_) * sizeof(double));
-  diag_ = DevBuf(ctx, n * sizeof(double));
-  Geo G;
-  for (int a = 0; a < 3; ++a) {
-    G.org[a] = org_[a];
-    G.fine[a] = fine_[a];
-  }
-  G.maxL = maxLevel;
-  k_row_fill<<<gridFor(N), kBlock, 0, s>>>(N, G, keys_.as<uint64_t>(), lev_.as<uint8_t>(), mask_.as<uint8_t>(),
-                                           cnt_.as<uint8_t>(), nbr_.as<int32_t>(), rowPtr_.as<int32_t>(),
-                                           col_.as<int32_t>(), coef_.as<double>(), diag_.as<double>());
-  NBL_LAUNCH_CHECK();
+  buildOctOperator(ctx, maxLevel, org_, fine_, L0_, &cnt_, &nbr_, s);
   NBL_HIP(hipEventRecord(e1_, s));
-  std::vector<uint8_t> hm(masks, masks + n);
-  nf_ = 0;
-  for (uint8_t m : hm) nf_ += m == 0;
   // solver vectors
   for (auto* b : {&x_, &rhs_, &r_, &z_, &p_, &Ap_, &t0_, &t1_}) *b = DevBuf(ctx, n * 8);
   ctl_ = DevBuf(ctx, sizeof(SolveCtl));
@@ -483,6 +613,7 @@ OctreeDev::OctreeDev(nbl_ctx* ctx, const uint64_t* keys, const uint8_t* levels, 
 
 OctreeDev::~OctreeDev() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  mg_.reset();
   if (e0_) (void)hipEventDestroy(e0_);
   if (e1_) (void)hipEventDestroy(e1_);
   if (hctl_) (void)hipHostFree(hctl_);
@@ -490,32 +621,32 @@ OctreeDev::~OctreeDev() {
 }
 
 void OctreeDev::permutation(uint64_t* perm) {
-  NBL_HIP(hipMemcpyAsync(perm, perm_.as<void>(), n_ * 8, hipMemcpyDeviceToHost, stream_));
+  NBL_HIP(hipMemcpyAsync(perm, perm_.as<void>(), L0_.n * 8, hipMemcpyDeviceToHost, stream_));
   NBL_HIP(hipStreamSynchronize(stream_));
 }
 
 void OctreeDev::faceTable(uint8_t* count, int32_t* nbr) {
-  if (count) NBL_HIP(hipMemcpyAsync(count, cnt_.as<void>(), 6 * n_, hipMemcpyDeviceToHost, stream_));
-  if (nbr) NBL_HIP(hipMemcpyAsync(nbr, nbr_.as<void>(), 24 * n_ * sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
+  if (count) NBL_HIP(hipMemcpyAsync(count, cnt_.as<void>(), 6 * L0_.n, hipMemcpyDeviceToHost, stream_));
+  if (nbr) NBL_HIP(hipMemcpyAsync(nbr, nbr_.as<void>(), 24 * L0_.n * sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
   NBL_HIP(hipStreamSynchronize(stream_));
 }
 
 void OctreeDev::toSorted(const double* hostIn, double* dSorted) {
-  NBL_HIP(hipMemcpyAsync(t1_.as<void>(), hostIn, n_ * 8, hipMemcpyHostToDevice, stream_));
-  k_gather<<<gridFor((long long)n_), kBlock, 0, stream_>>>((long long)n_, perm_.as<uint64_t>(), t1_.as<double>(), dSorted);
+  NBL_HIP(hipMemcpyAsync(t1_.as<void>(), hostIn, L0_.n * 8, hipMemcpyHostToDevice, stream_));
+  k_gather<<<gridFor((long long)L0_.n), kBlock, 0, stream_>>>((long long)L0_.n, perm_.as<uint64_t>(), t1_.as<double>(), dSorted);
   NBL_LAUNCH_CHECK();
 }
 
 void OctreeDev::fromSorted(const double* dSorted, double* hostOut) {
-  k_scatter<<<gridFor((long long)n_), kBlock, 0, stream_>>>((long long)n_, perm_.as<uint64_t>(), dSorted, t1_.as<double>());
+  k_scatter<<<gridFor((long long)L0_.n), kBlock, 0, stream_>>>((long long)L0_.n, perm_.as<uint64_t>(), dSorted, t1_.as<double>());
   NBL_LAUNCH_CHECK();
-  NBL_HIP(hipMemcpyAsync(hostOut, t1_.as<void>(), n_ * 8, hipMemcpyDeviceToHost, stream_));
+  NBL_HIP(hipMemcpyAsync(hostOut, t1_.as<void>(), L0_.n * 8, hipMemcpyDeviceToHost, stream_));
   NBL_HIP(hipStreamSynchronize(stream_));
 }
 
 void OctreeDev::applySorted(const double* x, double* out, hipStream_t s) {
-  k_oct_apply<<<gridFor((long long)n_), kBlock, 0, s>>>((long long)n_, mask_.as<uint8_t>(), rowPtr_.as<int32_t>(),
-                                                         col_.as<int32_t>(), coef_.as<double>(), x, out);
+  k_oct_apply<<<gridFor((long long)L0_.n), kBlock, 0, s>>>((long long)L0_.n, L0_.mask.as<uint8_t>(), L0_.rowPtr.as<int32_t>(),
+                                                         L0_.col.as<int32_t>(), L0_.coef.as<double>(), x, out);
   NBL_LAUNCH_CHECK();
 }
 
@@ -525,12 +656,12 @@ void OctreeDev::apply(const double* x, double* out) {
   fromSorted(Ap_.as<double>(), out);
 }
 
-void OctreeDev::diagonal(double* out) { fromSorted(diag_.as<double>(), out); }
+void OctreeDev::diagonal(double* out) { fromSorted(L0_.diag.as<double>(), out); }
 
 void OctreeDev::profile(int kernel, int reps, double* ms, double* bytes) {
   if (kernel != 0) throw std::invalid_argument("octree profile: kernel must be 0 (SpMV)");
   if (reps < 1) throw std::invalid_argument("octree profile: reps must be >= 1");
-  NBL_HIP(hipMemsetAsync(t0_.as<void>(), 0, n_ * 8, stream_));
+  NBL_HIP(hipMemsetAsync(t0_.as<void>(), 0, L0_.n * 8, stream_));
   applySorted(t0_.as<double>(), Ap_.as<double>(), stream_);
   NBL_HIP(hipEventRecord(e0_, stream_));
   for (int q = 0; q < reps; ++q) applySorted(t0_.as<double>(), Ap_.as<double>(), stream_);
@@ -540,21 +671,21 @@ void OctreeDev::profile(int kernel, int reps, double* ms, double* bytes) {
   NBL_HIP(hipEventElapsedTime(&t, e0_, e1_));
   *ms = double(t) / reps;
   // row pointer, mask, x and y once per leaf; column index and coefficient per coupling
-  *bytes = double(n_) * (4.0 + 1.0 + 8.0 + 8.0) + double(nnz_) * (4.0 + 8.0);
+  *bytes = double(L0_.n) * (4.0 + 1.0 + 8.0 + 8.0) + double(L0_.nnz) * (4.0 + 8.0);
 }
 
 nbl_poisson_report OctreeDev::solveSorted(const double* b, double* xo, double tol, int maxIters, hipStream_t s) {
   SolveCtl* ctl = ctl_.as<SolveCtl>();
   const RedBuf rb{red_.as<double>(), kOctBlocks};
-  const long long N = (long long)n_, nf = (long long)nf_;
+  const long long N = (long long)L0_.n, nf = (long long)L0_.nf;
   const int g = ogrid(N);
   launch_ctl_init(ctl, tol, s);
   NBL_HIP(hipEventRecord(e0_, s));
-  const uint8_t* mk = mask_.as<uint8_t>();
-  const int32_t* rp = rowPtr_.as<int32_t>();
-  const int32_t* cl = col_.as<int32_t>();
-  const double* cf = coef_.as<double>();
-  const double* dg = diag_.as<double>();
+  const uint8_t* mk = L0_.mask.as<uint8_t>();
+  const int32_t* rp = L0_.rowPtr.as<int32_t>();
+  const int32_t* cl = L0_.col.as<int32_t>();
+  const double* cf = L0_.coef.as<double>();
+  const double* dg = L0_.diag.as<double>();
   double *x = x_.as<double>(), *rhs = rhs_.as<double>(), *r = r_.as<double>(), *z = z_.as<double>();
   double *p = p_.as<double>(), *Ap = Ap_.as<double>();
   auto fin = [&](int op, int K, int slot, bool pred) {
@@ -616,6 +747,103 @@ nbl_poisson_report OctreeDev::solve(const double* b, double* x, double tol, int 
   // b is consumed by the first kernel of the solve, so the result may reuse its buffer
   toSorted(b, t0_.as<double>());
   const nbl_poisson_report rep = solveSorted(t0_.as<double>(), t0_.as<double>(), tol, maxIters, stream_);
+  fromSorted(t0_.as<double>(), x);
+  return rep;
+}
+
+
+OctMg& OctreeDev::mg() {
+  if (!mg_) mg_ = std::make_unique<OctMg>(ctx_, *this, stream_);
+  return *mg_;
+}
+
+int OctreeDev::mgLevels(long long* counts, int cap, double* buildMs) {
+  OctMg& m = mg();
+  for (int l = 0; l < std::min(cap, m.levels()); ++l) counts[l] = m.count(l);
+  if (buildMs) *buildMs = m.buildMs();
+  return m.levels();
+}
+
+nbl_poisson_report OctreeDev::solveMgSorted(const double* b, double* xo, double tol, int maxIters, hipStream_t s) {
+  OctMg& M = mg();
+  SolveCtl* ctl = ctl_.as<SolveCtl>();
+  const RedBuf rb{red_.as<double>(), kOctBlocks};
+  const long long N = (long long)L0_.n, nf = (long long)L0_.nf;
+  const int g = ogrid(N);
+  if (M.partials() != g) throw std::logic_error("octree multigrid: partial count mismatch");
+  launch_ctl_init(ctl, tol, s);
+  NBL_HIP(hipEventRecord(e0_, s));
+  const uint8_t* mk = L0_.mask.as<uint8_t>();
+  const int32_t* rp = L0_.rowPtr.as<int32_t>();
+  const int32_t* cl = L0_.col.as<int32_t>();
+  const double* cf = L0_.coef.as<double>();
+  const double* dg = L0_.diag.as<double>();
+  double *x = x_.as<double>(), *rhs = rhs_.as<double>(), *r = r_.as<double>(), *z = z_.as<double>();
+  double *p = p_.as<double>(), *Ap = Ap_.as<double>(), *rn = t1_.as<double>();
+  auto fin = [&](int op, int K, int slot, bool pred, int early = 0) {
+    FinArgs a{};
+    a.op = op;
+    a.K = K;
+    a.nparts = g;
+    a.slot = slot;
+    a.nf = nf;
+    a.convTol = tol;
+    a.earlyExit = early;
+    a.predicated = pred ? 1 : 0;
+    launch_fin(rb, a, ctl, s);
+  };
+  k_o_init<<<g, kBlock, 0, s>>>(N, mk, b, rhs, x, rb);
+  fin(FIN_MEAN, 1, O_MEAN_RHS, false);
+  k_om_init2<<<g, kBlock, 0, s>>>(N, nf, mk, dg, rhs, r, M.firstSweep(), ctl, rb);
+  fin(FIN_BNORM, 1, 0, false, 1);
+  M.vcycle(r, rb, ctl, s);
+  fin(FIN_MEAN, 1, O_MEAN_Z, true);
+  k_om_start<<<g, kBlock, 0, s>>>(N, nf, mk, M.result(), r, z, p, ctl, rb);
+  fin(FIN_RZ, 1, 0, true);
+  NBL_LAUNCH_CHECK();
+  int enq = 0;
+  int batch = std::clamp(predictedMg_, 1, std::max(1, maxIters));
+  while (enq < maxIters) {
+    for (int q = 0; q < batch && enq < maxIters; ++q, ++enq) {
+      k_o_apply_dot<<<g, kBlock, 0, s>>>(N, mk, rp, cl, cf, p, Ap, ctl, rb);
+      fin(FIN_ALPHA, 1, 0, true);
+      k_om_update<<<g, kBlock, 0, s>>>(N, mk, x, p, r, Ap, rn, ctl, rb);
+      fin(FIN_MEAN, 1, O_MEAN_R, true);
+      k_om_rfix<<<g, kBlock, 0, s>>>(N, nf, mk, dg, rn, M.firstSweep(), ctl, rb);
+      fin(FIN_CONV, 1, 0, true);
+      M.vcycle(rn, rb, ctl, s);
+      fin(FIN_MEAN, 1, O_MEAN_Z, true);
+      k_om_dots<<<g, kBlock, 0, s>>>(N, nf, mk, M.result(), rn, r, z, ctl, rb);
+      fin(FIN_BETA_FLEX, 2, 0, true);
+      k_om_pupdate<<<g, kBlock, 0, s>>>(N, p, z, r, rn, ctl);
+    }
+    NBL_LAUNCH_CHECK();
+    NBL_HIP(hipMemcpyAsync(hctl_, ctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, s));
+    NBL_HIP(hipStreamSynchronize(s));
+    if (hctl_[0].done) break;
+    batch = std::max(4, batch / 4);
+  }
+  k_o_xsum<<<g, kBlock, 0, s>>>(N, mk, x, rb);
+  fin(FIN_MEAN, 1, O_MEAN_X, false);
+  k_o_final<<<g, kBlock, 0, s>>>(N, nf, mk, rp, cl, cf, x, rhs, xo, ctl, rb);
+  fin(FIN_RESID, 1, 0, false);
+  NBL_LAUNCH_CHECK();
+  NBL_HIP(hipEventRecord(e1_, s));
+  NBL_HIP(hipMemcpyAsync(hctl_, ctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, s));
+  NBL_HIP(hipStreamSynchronize(s));
+  float tms = 0.f;
+  NBL_HIP(hipEventElapsedTime(&tms, e0_, e1_));
+  lastMs_ = tms;
+  const SolveCtl& h = hctl_[0];
+  if (h.converged && h.it > 0) predictedMg_ = h.it;
+  return nbl_poisson_report{h.it, h.resid, h.converged};
+}
+
+nbl_poisson_report OctreeDev::solveMg(const double* b, double* x, double tol, int maxIters) {
+  if (maxIters < 0) throw std::invalid_argument("max iterations must be >= 0");
+  mg();  // hierarchy set-up stays outside the timed solve
+  toSorted(b, t0_.as<double>());
+  const nbl_poisson_report rep = solveMgSorted(t0_.as<double>(), t0_.as<double>(), tol, maxIters, stream_);
   fromSorted(t0_.as<double>(), x);
   return rep;
 }

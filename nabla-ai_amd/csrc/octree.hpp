@@ -1,50 +1,85 @@
-// octree.hpp — device octree: Morton-sorted leaves, face tables, CSR operator
-// and the Jacobi-PCG pressure solve on the leaves.
+// octree.hpp — device octree: Morton-sorted leaves, face tables, CSR operator,
+// the Jacobi-PCG pressure solve on the leaves and the multigrid-preconditioned
+// flexible CG whose levels are the tree's own coarsenings.
 #pragma once
+
+#include <memory>
 
 #include "solver.hpp"
 
 namespace nbl {
+
+// One set of Morton-sorted leaves with its fluid-fluid operator in CSR form:
+// row i holds the fluid neighbours of fluid leaf i in the reference's
+// accumulation order (direction -x,+x,-y,+y,-z,+z, then face order) and the
+// face coefficients area/dist; diag is their sum (1 on non-fluid rows and on
+// fluid rows without fluid neighbours).
+struct OctLevel {
+  size_t n = 0, nnz = 0, nf = 0;
+  DevBuf keys, lev, mask, rowPtr, col, coef, diag;
+};
+
+// Face table (count u8[6n], neighbours i32[24n]) and CSR operator of a level
+// whose keys, levels and masks are already on the device and sorted.  The face
+// table is returned through cnt/nbr when they are non-null, else freed.
+void buildOctOperator(nbl_ctx* ctx, int maxL, const double org[3], const double fine[3], OctLevel& L,
+                      DevBuf* cnt, DevBuf* nbr, hipStream_t s);
+
+class OctMg;
 
 class OctreeDev {
  public:
   OctreeDev(nbl_ctx* ctx, const uint64_t* keys, const uint8_t* levels, const uint8_t* masks, size_t n,
             const double origin[3], const double extent[3], int maxLevel);
   ~OctreeDev();
-  size_t size() const { return n_; }
-  size_t nnz() const { return nnz_; }
+  size_t size() const { return L0_.n; }
+  size_t nnz() const { return L0_.nnz; }
   void permutation(uint64_t* perm);
   void faceTable(uint8_t* count, int32_t* nbr);
   // caller-order host arrays
   void apply(const double* x, double* out);
   void diagonal(double* out);
   nbl_poisson_report solve(const double* b, double* x, double tol, int maxIters);
+  // Multigrid-preconditioned flexible CG on the same system (hierarchy built on
+  // the first call); caller-order host arrays.
+  nbl_poisson_report solveMg(const double* b, double* x, double tol, int maxIters);
   // sorted-order device arrays
   void applySorted(const double* x, double* out, hipStream_t s);
   nbl_poisson_report solveSorted(const double* b, double* x, double tol, int maxIters, hipStream_t s);
+  nbl_poisson_report solveMgSorted(const double* b, double* x, double tol, int maxIters, hipStream_t s);
   hipStream_t stream() const { return stream_; }
   // Device time of construction (sort, validation, face table, CSR) and of the
   // last solve (ms), and a timing loop over one kernel (0: the SpMV).
   double setupMs() const { return setupMs_; }
   double lastSolveMs() const { return lastMs_; }
   void profile(int kernel, int reps, double* ms, double* bytes);
+  // Multigrid hierarchy: level count, leaves per level (up to `cap` entries)
+  // and the device time of its construction (ms).  Builds it if needed.
+  int mgLevels(long long* counts, int cap, double* buildMs);
+
+  const OctLevel& level0() const { return L0_; }
+  int maxLevel() const { return maxL_; }
+  const double* origin() const { return org_; }
+  const double* fine() const { return fine_; }
 
  private:
   void toSorted(const double* hostIn, double* dSorted);
   void fromSorted(const double* dSorted, double* hostOut);
+  OctMg& mg();
 
   nbl_ctx* ctx_;
-  size_t n_ = 0, nnz_ = 0, nf_ = 0;
   int maxL_ = 0;
   double org_[3], fine_[3];
   hipStream_t stream_ = nullptr;
-  DevBuf keys_, perm_, lev_, mask_, cnt_, nbr_, rowPtr_, col_, coef_, diag_;
+  OctLevel L0_;
+  DevBuf perm_, cnt_, nbr_;
   DevBuf x_, rhs_, r_, z_, p_, Ap_, t0_, t1_;
   DevBuf ctl_, red_;
   SolveCtl* hctl_ = nullptr;
-  int predicted_ = 16;
+  int predicted_ = 16, predictedMg_ = 8;
   hipEvent_t e0_ = nullptr, e1_ = nullptr;
   double setupMs_ = 0.0, lastMs_ = 0.0;
+  std::unique_ptr<OctMg> mg_;
 };
 
 }  // namespace nbl

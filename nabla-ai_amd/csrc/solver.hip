@@ -292,6 +292,7 @@ bool PoissonDev::buildDirectCoarse() {
 }
 
 PoissonDev::~PoissonDev() {
+  for (hipEvent_t e : evPool_) (void)hipEventDestroy(e);
   if (hctl_) (void)hipHostFree(hctl_);
   if (e0_) (void)hipEventDestroy(e0_);
   if (e1_) (void)hipEventDestroy(e1_);
@@ -309,6 +310,41 @@ void PoissonDev::fin(int op, int K, int nparts, int slot, long long nf, bool pre
   a.earlyExit = earlyExit;
   a.predicated = predicated ? 1 : 0;
   launch_fin(red(), a, ctl_.as<SolveCtl>(), s);
+}
+
+// In-situ timing of the finest level's plain sweeps (the roofline kernel):
+// an event pair around each such launch in the first V-cycle of a solve (the
+// one that always runs; sampling one cycle per solve keeps the event markers
+// from slowing the solve itself).  After the solve the recorded pairs are
+// summed.
+void PoissonDev::timedSweepBegin(int l, bool plain, hipStream_t s) {
+  curTimed_ = timeSweeps_ && l == 0 && plain && vcount_ == 0;
+  if (!curTimed_) return;
+  if (evUsed_ + 2 > evPool_.size()) {
+    for (int q = 0; q < 64; ++q) {
+      hipEvent_t e;
+      NBL_HIP(hipEventCreate(&e));
+      evPool_.push_back(e);
+    }
+  }
+  NBL_HIP(hipEventRecord(evPool_[evUsed_], s));
+}
+void PoissonDev::timedSweepEnd(hipStream_t s) {
+  if (!curTimed_) return;
+  NBL_HIP(hipEventRecord(evPool_[evUsed_ + 1], s));
+  evCycle_.push_back(vcount_);
+  evUsed_ += 2;
+}
+void PoissonDev::collectSweepTimes(int cyclesRun) {
+  for (size_t q = 0; q + 1 < evUsed_ + 1 && q / 2 < evCycle_.size(); q += 2) {
+    if (evCycle_[q / 2] >= cyclesRun) continue;
+    float ms = 0.f;
+    NBL_HIP(hipEventElapsedTime(&ms, evPool_[q], evPool_[q + 1]));
+    sweepMs_ += ms;
+    ++sweepN_;
+  }
+  evUsed_ = 0;
+  evCycle_.clear();
 }
 
 // One V(pre, post) cycle from level l (multigrid.cpp:185-198).  Each sweep and
@@ -344,8 +380,10 @@ void PoissonDev::vcycle(int l, const T* b, const double* mb, const T** xout, con
   }
   for (int q = q0; q < opts_.pre_smooth; ++q) {
     const int o = cur == 0 ? 1 : 0;
+    timedSweepBegin(l, cur >= 0, s);
     launch_mg_sweep<T>(L, b, mb, cur < 0 ? nullptr : bufs[cur], cur < 0 ? nullptr : ms[cur], cur < 0, nullptr,
                        nullptr, nullptr, bufs[o], project_, rb, ctl, s);
+    timedSweepEnd(s);
     if (project_) fin(FIN_MEAN, 1, blocks_[l], slotX(l, o), L.nf, true, s);
     cur = o;
   }
@@ -359,8 +397,10 @@ void PoissonDev::vcycle(int l, const T* b, const double* mb, const T** xout, con
   vcycle<T>(l + 1, bc, mbc, &ec, &mec, s);
   for (int q = 0; q < opts_.post_smooth; ++q) {
     const int o = 1 - cur;
+    timedSweepBegin(l, q > 0, s);
     launch_mg_sweep<T>(L, b, mb, bufs[cur], ms[cur], false, q == 0 ? ec : nullptr, q == 0 ? mec : nullptr, &Lc,
                        bufs[o], project_, rb, ctl, s);
+    timedSweepEnd(s);
     if (project_) fin(FIN_MEAN, 1, blocks_[l], slotX(l, o), L.nf, true, s);
     cur = o;
   }
@@ -505,6 +545,9 @@ nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t
   lastMs_ = ms;
   lastV_ = vcount_;
   const SolveCtl& h = hctl_[0];
+  // V-cycles that ran: the initial one and one per iteration, except after
+  // the converging update (multigrid.cpp:319-325); with an early exit none
+  if (timeSweeps_) collectSweepTimes(h.early ? 0 : (h.converged ? h.it : h.it + 1));
   if (h.converged && h.it > 0) predicted_ = h.it;
   return nbl_poisson_report{h.it, h.resid, h.converged};
 }

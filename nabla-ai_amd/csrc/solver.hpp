@@ -97,6 +97,19 @@ class PoissonDev {
   RedBuf red() const { return RedBuf{red_.as<double>(), redStride_}; }
   int blocks(int l) const { return blocks_[l]; }
   bool projects() const { return project_; }
+  // In-situ timing of the finest level's plain sweeps inside solves.
+  void setSweepTiming(bool on) {
+    timeSweeps_ = on;
+    if (on) {
+      sweepMs_ = 0.0;
+      sweepN_ = 0;
+    }
+  }
+  void sweepTiming(double* ms, long long* n, double* bytes) const {
+    if (ms) *ms = sweepN_ ? sweepMs_ / double(sweepN_) : 0.0;
+    if (n) *n = sweepN_;
+    if (bytes) *bytes = double(dl_[0].n) * (3.0 * (mixed_ ? 4.0 : 8.0) + 2.0);
+  }
   // Average ms per launch and algorithmic bytes of one solve kernel.
   void profile(int kernel, int reps, hipStream_t s, double* ms, double* bytes);
 
@@ -109,6 +122,9 @@ class PoissonDev {
            double convTol = 0.0);
   void enqueueIteration(hipStream_t s);
   void readCtl(hipStream_t s);
+  void timedSweepBegin(int l, bool plain, hipStream_t s);
+  void timedSweepEnd(hipStream_t s);
+  void collectSweepTimes(int cyclesRun);
 
   nbl_ctx* ctx_;
   int kind_;
@@ -140,6 +156,12 @@ class PoissonDev {
   int lastV_ = 0;
   int predicted_ = 1;
   int vcount_ = 0;
+  bool timeSweeps_ = false, curTimed_ = false;
+  std::vector<hipEvent_t> evPool_;
+  std::vector<int> evCycle_;  // V-cycle index of each recorded pair
+  size_t evUsed_ = 0;
+  double sweepMs_ = 0.0;
+  long long sweepN_ = 0;
 };
 
 // A device-resident MAC Navier-Stokes run (NSSolver).
@@ -168,6 +190,7 @@ class RunDev {
     }
   }
   size_t sizes(int q) const { return sz_[q]; }
+  PoissonDev& poisson() { return *poisson_; }
 
  private:
   void bc(bool outflow);

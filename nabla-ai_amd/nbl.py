@@ -218,8 +218,13 @@ def load(path: os.PathLike | None = None):
     _proto(lib, "nbl_octree_apply", i, V, P_D, P_D)
     _proto(lib, "nbl_octree_diagonal", i, V, P_D)
     _proto(lib, "nbl_octree_poisson_solve", i, V, P_D, P_D, d, i, C.POINTER(PoissonReport))
+    _proto(lib, "nbl_octree_mg_solve", i, V, P_D, P_D, d, i, C.POINTER(PoissonReport))
+    _proto(lib, "nbl_octree_mg_levels", i, V, C.POINTER(C.c_longlong), i, C.POINTER(i),
+           C.POINTER(C.c_double))
     _proto(lib, "nbl_octree_profile", i, V, i, i, C.POINTER(C.c_double), C.POINTER(C.c_double))
     _proto(lib, "nbl_octree_last_timing", i, V, C.POINTER(C.c_double), C.POINTER(C.c_double))
+    _proto(lib, "nbl_run_sweep_timing", i, V, i, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
+           C.POINTER(C.c_double))
     _proto(lib, "nbl_server_create", i, V, i, C.c_uint64, C.POINTER(V))
     _proto(lib, "nbl_server_destroy", i, V)
     _proto(lib, "nbl_server_admit", i, V, V, V, V, C.POINTER(C.c_int))
@@ -327,6 +332,12 @@ class Run:
         vc = (C.c_int * 3)()
         check(self.lib.nbl_run_last_solves(self.h, ms, it, vc))
         return list(ms), list(it), list(vc)
+
+    def sweep_timing(self, enable: int = -1):
+        """(avg ms per finest-level plain sweep, launches timed, bytes per launch)."""
+        ms, n, by = C.c_double(), C.c_longlong(), C.c_double()
+        check(self.lib.nbl_run_sweep_timing(self.h, enable, C.byref(ms), C.byref(n), C.byref(by)))
+        return ms.value, n.value, by.value
 
     def cell_velocity(self, comp: int):
         out = np.empty(self.grid.cells)
@@ -532,6 +543,22 @@ class Octree:
         check(self.lib.nbl_octree_poisson_solve(self.h, dptr(b), dptr(x), tol, max_iters,
                                                 C.byref(rep)))
         return x, rep
+
+    def mg_solve(self, b, tol=1e-9, max_iters=2000):
+        """Multigrid-preconditioned flexible CG on the same system as solve()."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.empty(self.n)
+        rep = PoissonReport()
+        check(self.lib.nbl_octree_mg_solve(self.h, dptr(b), dptr(x), tol, max_iters,
+                                           C.byref(rep)))
+        return x, rep
+
+    def mg_levels(self, cap=32):
+        """(leaves per multigrid level, device ms of the hierarchy build)."""
+        counts = (C.c_longlong * cap)()
+        nl, ms = C.c_int(), C.c_double()
+        check(self.lib.nbl_octree_mg_levels(self.h, counts, cap, C.byref(nl), C.byref(ms)))
+        return [int(counts[q]) for q in range(min(cap, nl.value))], ms.value
 
     def profile(self, kernel=0, reps=50):
         ms, by = C.c_double(), C.c_double()

@@ -1246,6 +1246,271 @@ struct Oct {
   }
 };
 
+
+// ---- octree multigrid (no reference counterpart) -----------------------------
+// Restates the device octree multigrid: level 0 = the leaves in Morton order;
+// level l+1 clips every leaf finer than (finest level in l) - 1 to its ancestor
+// at that depth; coarse cell fluid iff a child is fluid; operator rebuilt with
+// the reference face coefficient (octree_poisson.cpp:24-32, 64-105); residuals
+// restrict by summation in child order, corrections prolong by injection;
+// V(2,2) damped Jacobi (omega 0.8, multigrid.cpp:102-114); coarsest level
+// (<= 2048 leaves) solved exactly through (A + alpha 11^T) with the result
+// projected onto mean-free vectors; outer loop = MgPoisson::solve's flexible
+// CG in its singular form (multigrid.cpp:267-355).
+struct OctMgOrc {
+  struct Lvl {
+    std::unique_ptr<Oct> t;
+    std::vector<std::vector<std::pair<size_t, double>>> rows;  // fluid couplings, reference order
+    Vec diag;
+    std::vector<size_t> up;  // parent in level l+1
+    size_t nf = 0;
+  };
+  std::vector<Lvl> L;
+  std::vector<size_t> order;  // Morton position -> caller index
+  bool direct = false;
+  std::vector<double> lu;     // LU of the regularised coarsest matrix (row-major, no pivoting needed: SPD)
+  static constexpr double kOm = 0.8;
+
+  static void fillOp(Lvl& v) {
+    const Oct& t = *v.t;
+    const size_t n = t.key.size();
+    v.rows.assign(n, {});
+    v.diag.assign(n, 1.0);
+    v.nf = 0;
+    for (size_t i = 0; i < n; ++i) {
+      if (!t.fluid(i)) continue;
+      ++v.nf;
+      double d = 0.0;
+      for (int dd = 0; dd < 6; ++dd)
+        for (size_t j : t.nbrs(i, dd)) {
+          if (!t.fluid(j)) continue;
+          const double c = t.coeff(i, j, dd / 2);
+          v.rows[i].push_back({j, c});
+          d += c;
+        }
+      v.diag[i] = d > 0.0 ? d : 1.0;
+    }
+  }
+  static double rowA(const Lvl& v, size_t i, const Vec& x) {
+    double acc = 0.0;
+    for (const auto& [j, c] : v.rows[i]) acc += c * (x[i] - x[j]);
+    return acc;
+  }
+  static std::unique_ptr<Oct> fromLeaves(const Oct& like, const std::vector<uint64_t>& k,
+                                         const std::vector<uint8_t>& l, const std::vector<uint8_t>& m) {
+    auto t = std::make_unique<Oct>(like.org, like.ext, like.maxL);
+    t->key.clear();
+    t->lev.clear();
+    t->msk.clear();
+    t->at.clear();
+    for (size_t i = 0; i < k.size(); ++i) t->push(k[i], l[i], m[i]);
+    return t;
+  }
+
+  explicit OctMgOrc(const Oct& o) {
+    const size_t n0 = o.key.size();
+    order.resize(n0);
+    for (size_t i = 0; i < n0; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return o.key[a] < o.key[b]; });
+    std::vector<uint64_t> k(n0);
+    std::vector<uint8_t> lv(n0), m(n0);
+    int top = 0;
+    for (size_t q = 0; q < n0; ++q) {
+      k[q] = o.key[order[q]];
+      lv[q] = o.lev[order[q]];
+      m[q] = o.msk[order[q]];
+      top = std::max(top, int(lv[q]));
+    }
+    L.emplace_back();
+    L[0].t = fromLeaves(o, k, lv, m);
+    fillOp(L[0]);
+    while (L.back().t->key.size() > 2048 && L.size() < 16 && top > 0) {
+      const int clip = top - 1;
+      const Oct& f = *L.back().t;
+      const size_t n = f.key.size();
+      std::vector<uint64_t> ck;
+      std::vector<uint8_t> cl, cm;
+      std::vector<size_t> up(n);
+      for (size_t i = 0; i < n; ++i) {
+        uint64_t key = f.key[i];
+        int lev = f.lev[i];
+        if (lev > clip) {
+          key &= ~((uint64_t(1) << (3 * (f.maxL - clip))) - 1);
+          lev = clip;
+        }
+        if (ck.empty() || ck.back() != key) {
+          ck.push_back(key);
+          cl.push_back(uint8_t(lev));
+          cm.push_back(2);
+        }
+        up[i] = ck.size() - 1;
+        if (f.msk[i] == 0) cm.back() = 0;
+      }
+      if (ck.size() >= n) break;
+      L.back().up = up;
+      Lvl c;
+      c.t = fromLeaves(f, ck, cl, cm);
+      fillOp(c);
+      L.push_back(std::move(c));
+      top = clip;
+    }
+    direct = L.back().t->key.size() <= 2048 && factor();
+  }
+
+  // LU (Doolittle, no pivoting) of the coarsest matrix: fluid rows A + alpha
+  // on the fluid block, other rows identity.
+  bool factor() {
+    const Lvl& v = L.back();
+    const size_t n = v.t->key.size();
+    lu.assign(n * n, 0.0);
+    double dfl = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+      lu[i * n + i] = v.diag[i];
+      if (!v.t->fluid(i)) continue;
+      for (const auto& [j, c] : v.rows[i]) lu[i * n + j] -= c;
+      dfl += v.diag[i];
+    }
+    if (v.nf > 0) {
+      const double alpha = dfl / double(v.nf) / double(v.nf);
+      for (size_t i = 0; i < n; ++i)
+        for (size_t j = 0; j < n; ++j)
+          if (v.t->fluid(i) && v.t->fluid(j)) lu[i * n + j] += alpha;
+    }
+    for (size_t kk = 0; kk < n; ++kk) {
+      const double piv = lu[kk * n + kk];
+      if (!(std::abs(piv) > 1e-12)) return false;
+      for (size_t i = kk + 1; i < n; ++i) {
+        const double f = lu[i * n + kk] / piv;
+        if (f == 0.0) continue;
+        lu[i * n + kk] = f;
+        for (size_t j = kk + 1; j < n; ++j) lu[i * n + j] -= f * lu[kk * n + j];
+      }
+    }
+    return true;
+  }
+  void project(const Lvl& v, Vec& a) const {
+    if (v.nf == 0) return;
+    double m = 0.0;
+    for (size_t c = 0; c < a.size(); ++c) {
+      if (v.t->fluid(c)) m += a[c];
+      else a[c] = 0.0;
+    }
+    m /= double(v.nf);
+    for (size_t c = 0; c < a.size(); ++c)
+      if (v.t->fluid(c)) a[c] -= m;
+  }
+  void coarsest(const Vec& b, Vec& x) const {
+    const Lvl& v = L.back();
+    const size_t n = v.t->key.size();
+    if (direct) {
+      Vec y = b;
+      project(v, y);
+      for (size_t i = 0; i < n; ++i)
+        for (size_t j = 0; j < i; ++j) y[i] -= lu[i * n + j] * y[j];
+      for (size_t i = n; i-- > 0;) {
+        for (size_t j = i + 1; j < n; ++j) y[i] -= lu[i * n + j] * y[j];
+        y[i] /= lu[i * n + i];
+      }
+      project(v, y);
+      x = y;
+      return;
+    }
+    x.assign(n, 0.0);
+    for (int q = 0; q < 16; ++q) sweep(v, b, x);
+  }
+  static void sweep(const Lvl& v, const Vec& b, Vec& x) {
+    const size_t n = x.size();
+    Vec xn(n, 0.0);
+    for (size_t i = 0; i < n; ++i)
+      if (v.t->fluid(i)) xn[i] = x[i] + kOm * (b[i] - rowA(v, i, x)) / v.diag[i];
+    x.swap(xn);
+  }
+  void cycle(size_t l, const Vec& b, Vec& x) const {
+    const Lvl& v = L[l];
+    const size_t n = v.t->key.size();
+    if (l + 1 == L.size()) {
+      coarsest(b, x);
+      return;
+    }
+    x.assign(n, 0.0);
+    sweep(v, b, x);
+    sweep(v, b, x);
+    const Lvl& c = L[l + 1];
+    Vec bc(c.t->key.size(), 0.0), ec;
+    for (size_t i = 0; i < n; ++i)
+      if (v.t->fluid(i)) bc[v.up[i]] += b[i] - rowA(v, i, x);
+    cycle(l + 1, bc, ec);
+    for (size_t i = 0; i < n; ++i) x[i] += ec[v.up[i]];
+    sweep(v, b, x);
+    sweep(v, b, x);
+  }
+
+  nbl_poisson_report solve(const Vec& bIn, Vec& xOut, double tol, int maxIt) const {
+    const Lvl& v = L[0];
+    const size_t n = v.t->key.size();
+    Vec rhs(n), x(n, 0.0);
+    for (size_t q = 0; q < n; ++q) rhs[q] = v.t->fluid(q) ? bIn[order[q]] : 0.0;
+    project(v, rhs);
+    const double bn = std::sqrt(dotv(rhs, rhs)) + 1e-300;
+    nbl_poisson_report rep{0, 0.0, 0};
+    Vec r = rhs;
+    xOut.assign(n, 0.0);
+    if (std::sqrt(dotv(r, r)) <= tol * bn) {
+      rep.residual = std::sqrt(dotv(r, r)) / bn;
+      rep.converged = 1;
+      return rep;
+    }
+    auto minv = [&](const Vec& rr, Vec& zz) {
+      cycle(0, rr, zz);
+      project(v, zz);
+    };
+    Vec z, p, Ap(n), rn(n), zn;
+    minv(r, z);
+    p = z;
+    double rz = dotv(r, z);
+    int it = 0;
+    for (; it < maxIt; ++it) {
+      for (size_t i = 0; i < n; ++i) Ap[i] = v.t->fluid(i) ? rowA(v, i, p) : p[i];
+      const double pAp = dotv(p, Ap);
+      if (std::abs(pAp) < 1e-300) break;
+      const double al = rz / pAp;
+      for (size_t i = 0; i < n; ++i) {
+        x[i] += al * p[i];
+        rn[i] = r[i] - al * Ap[i];
+      }
+      project(v, rn);
+      if (std::sqrt(dotv(rn, rn)) <= tol * bn) {
+        ++it;
+        r = rn;
+        rep.converged = 1;
+        break;
+      }
+      minv(rn, zn);
+      double a = 0.0, c = 0.0;
+      for (size_t i = 0; i < n; ++i) {
+        a += zn[i] * rn[i];
+        c += zn[i] * r[i];
+      }
+      const double be = (a - c) / rz;
+      for (size_t i = 0; i < n; ++i) p[i] = zn[i] + be * p[i];
+      rz = a;
+      r = rn;
+    }
+    project(v, x);
+    double rr = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+      const double ax = v.t->fluid(i) ? rowA(v, i, x) : x[i];
+      const double e = rhs[i] - ax;
+      rr += e * e;
+    }
+    rep.iterations = it;
+    rep.residual = std::sqrt(rr) / bn;
+    rep.converged = (rep.converged || rep.residual <= tol) ? 1 : 0;
+    for (size_t q = 0; q < n; ++q) xOut[order[q]] = x[q];
+    return rep;
+  }
+};
+
 }  // namespace orc
 
 using namespace orc;
@@ -1478,6 +1743,21 @@ void orc_oct_solve(void* h, const double* b, double* x, double tol, int max_iter
   const nbl_poisson_report r = o.solve(bb, xx, tol, max_iters);
   std::copy(xx.begin(), xx.end(), x);
   if (rep) *rep = r;
+}
+
+
+void orc_oct_mg_solve(void* h, const double* b, double* x, double tol, int max_iters, nbl_poisson_report* rep) {
+  const Oct& o = *static_cast<Oct*>(h);
+  const OctMgOrc mg(o);
+  Vec bb(b, b + o.key.size()), xx;
+  const nbl_poisson_report r = mg.solve(bb, xx, tol, max_iters);
+  std::copy(xx.begin(), xx.end(), x);
+  if (rep) *rep = r;
+}
+int orc_oct_mg_levels(void* h, long long* counts, int cap) {
+  const OctMgOrc mg(*static_cast<Oct*>(h));
+  for (int l = 0; l < std::min(cap, int(mg.L.size())); ++l) counts[l] = (long long)mg.L[size_t(l)].t->key.size();
+  return int(mg.L.size());
 }
 
 }  // extern "C"

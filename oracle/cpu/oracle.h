@@ -75,6 +75,12 @@ void orc_oct_apply(void* h, const double* x, double* out);
 void orc_oct_diagonal(void* h, double* diag);
 void orc_oct_solve(void* h, const double* b, double* x, double tol, int max_iters,
                    nbl_poisson_report* rep);
+/* Octree multigrid-preconditioned flexible CG (no reference counterpart: the
+ * restatement of the device design, pinned by its reduction to the uniform
+ * multigrid on uniform trees and by the OctreePoisson system it solves). */
+void orc_oct_mg_solve(void* h, const double* b, double* x, double tol, int max_iters,
+                      nbl_poisson_report* rep);
+int orc_oct_mg_levels(void* h, long long* counts, int cap);
 
 #ifdef __cplusplus
 }

@@ -66,6 +66,8 @@ class OracleLib:
                 "ns_rhs": (None, V, P_D, P_D, P_D), "mg_levels": (i, G, C.POINTER(i)),
                 "mg_smooth_once": (None, G, P_D, P_D), "mg_restrict": (None, G, P_D, P_D),
                 "mg_prolong": (None, G, P_D, P_D),
+                "oct_mg_solve": (None, V, P_D, P_D, d, i, PR),
+                "oct_mg_levels": (i, V, C.POINTER(C.c_longlong), i),
             })
         for name, (res, *args) in spec.items():
             f = getattr(self.lib, f"{kind}_{name}")
@@ -268,6 +270,19 @@ class Oct:
         rep = nbl.PoissonReport()
         self.o.oct_solve(self.h, dptr(b), dptr(x), tol, max_iters, C.byref(rep))
         return x, rep
+
+    def mg_solve(self, b, tol=1e-9, max_iters=2000):
+        """Octree multigrid flexible CG (restatement; oracle library only)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.empty(self.n)
+        rep = nbl.PoissonReport()
+        self.o.oct_mg_solve(self.h, dptr(b), dptr(x), tol, max_iters, C.byref(rep))
+        return x, rep
+
+    def mg_levels(self, cap=32):
+        counts = (C.c_longlong * cap)()
+        nl = self.o.oct_mg_levels(self.h, counts, cap)
+        return [int(counts[q]) for q in range(min(cap, nl))]
 
     def close(self):
         if self.h:

@@ -9,6 +9,7 @@ import pytest
 
 import cases
 import nbl
+from octree_trees import adaptive_octree
 
 
 def periodic_grid(nx, ny, nz, h):
@@ -257,3 +258,63 @@ def test_octree_uniform_equals_scaled_uniform_operator(orc):
     xu[lin] = x
     Au = orc.apply(g, xu)[lin]
     assert np.allclose(t.apply(x), h ** 3 * Au, rtol=1e-12, atol=1e-12)
+
+
+# ---- octree multigrid (no reference counterpart) -----------------------------
+def _uniform_tree_case(orc, n, levels):
+    t = orc.octree((0, 0, 0), (1, 1, 1), 6)
+    t.refine_uniform(levels)
+    c = t.centers()
+    idx = np.floor(c * n).astype(int)
+    lin = idx[:, 0] + n * (idx[:, 1] + n * idx[:, 2])
+    return t, lin
+
+
+@pytest.mark.parametrize("n,levels", [(16, 4), (32, 5)])
+def test_octree_mg_on_uniform_tree_is_the_uniform_mg(orc, n, levels):
+    """On a uniform all-fluid tree the octree hierarchy is the uniform one with
+    wall boundaries (32^3 -> 16^3 -> 8^3): summed residuals against the
+    volume-scaled coarse operator give the uniform solver's averaged residuals
+    against its own, so the octree multigrid CG with b scaled by the cell
+    volume takes the uniform MgPoisson's iterations and returns its solution.
+    A restriction by averaging, a wrong damping or sweep count, or a coarse
+    operator without the volume scaling changes the iteration counts here."""
+    t, lin = _uniform_tree_case(orc, n, levels)
+    assert t.mg_levels() == [n ** 3 // 8 ** q for q in range(levels - 2)]
+    g = nbl.Grid(n, n, n, 1.0 / n)
+    b = cases.normal(31, t.n)
+    b -= b.mean()
+    bu = np.empty(t.n)
+    bu[lin] = b
+    for tol in (1e-8, 1e-10):
+        x, rep = t.mg_solve(b / n ** 3, tol, 200)
+        xu, ru = orc.mg(g, bu, tol, 200)
+        assert rep.converged and ru.converged
+        assert rep.iterations == ru.iterations
+        assert rep.residual == pytest.approx(ru.residual, rel=1e-4)
+        assert np.linalg.norm(x - xu[lin]) <= 1e-11 * np.linalg.norm(xu)
+
+
+def test_octree_mg_solves_the_octree_system(orc):
+    """On an adaptive, anisotropic tree with a solid block the multigrid CG
+    solves OctreePoisson's system: A x reproduces the projected right-hand side
+    to the tolerance (A is the pinned OctreePoisson::applyA), x is mean-free on
+    fluid leaves and agrees with the Jacobi-PCG solution, in far fewer
+    iterations."""
+    t = adaptive_octree(orc)
+    lv = t.mg_levels()
+    assert len(lv) >= 3 and lv[-1] <= 2048 and all(a > b for a, b in zip(lv, lv[1:]))
+    _k, _l, m = t.dump()
+    fl = m == 0
+    b = cases.normal(5, t.n)
+    b[~fl] = 0.0
+    b[fl] -= b[fl].mean()
+    x, rep = t.mg_solve(b, 1e-11, 300)
+    xj, rj = t.solve(b, 1e-12, 20000)
+    assert rep.converged and rj.converged
+    assert rep.iterations * 5 < rj.iterations
+    r = t.apply(x) - b
+    assert np.linalg.norm(r[fl]) <= 2e-11 * np.linalg.norm(b)
+    assert abs(x[fl].mean()) < 1e-12 * np.abs(x).max()
+    assert np.all(x[~fl] == 0.0)
+    assert np.linalg.norm(x - xj) <= 1e-8 * np.linalg.norm(xj)
