@@ -97,15 +97,17 @@ def pmc_traffic(key):
 
 
 def roofline_insitu(sweep, g):
-    """Finest-level plain sweep timed inside the timed steps (HIP events on the
-    run's stream around each launch that did work)."""
+    """Finest-level plain sweep timed inside the timed steps: every block of a
+    launch stores the device wall clock at its start and when all its waves
+    are done; a launch lasts from its first block start to its last block end
+    (launches a converged solve skipped are left out)."""
     ms, n, by = sweep["avg_ms"], sweep["launches"], sweep["bytes"]
     ach = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic("cube192x64x64"),
-            "kernel": "k_sweep<double, SW_PLAIN> on the 192x64x64 level (2 of the 4 finest sweeps per V-cycle)",
+            "kernel": "k_sweep<double, SW_PLAIN> on the 192x64x64 level (pre-smoothing sweep 2 of every V-cycle)",
             "avg_launch_us": round(ms * 1e3, 3), "launches_timed": n, "bytes_per_launch": by, "cells": g.cells,
-            "measured": "HIP events on the run's stream around each launch inside the timed steps"}
+            "measured": "device wall-clock stamps per block (first block start to last block end), every launch inside the timed steps"}
 
 
 def coarse_solve(ctx, g, reps=50):

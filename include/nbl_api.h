@@ -208,7 +208,8 @@ int nbl_poisson_create(nbl_ctx* ctx, const nbl_grid_desc* grid, int kind, double
 int nbl_poisson_destroy(nbl_poisson* ps);
 /* Host buffers. */
 int nbl_poisson_solve(nbl_poisson* ps, const double* b, double* x, nbl_poisson_report* rep);
-/* Device buffers (HBM resident, length nx*ny*nz). */
+/* Device buffers (HBM resident, length nx*ny*nz, 16-byte aligned as any
+ * hipMalloc allocation is). */
 int nbl_poisson_solve_device(nbl_poisson* ps, const double* d_b, double* d_x,
                              nbl_poisson_report* rep);
 /* CgPoisson::applyA (poisson.cpp:102-134), host buffers. */
@@ -307,9 +308,14 @@ int nbl_octree_profile(nbl_octree* oc, int kernel, int reps, double* ms, double*
 int nbl_octree_last_timing(nbl_octree* oc, double* setup_ms, double* solve_ms);
 
 /* No counterpart in the reference: measurement.  In-situ timing of the finest
- * level's plain smoother sweeps inside this run's pressure solves (an event
- * pair around each such launch in the first V-cycle of every solve).
- * enable: 1 on (resets the totals), 0 off (keeps them), -1 leave as is.
+ * level's plain smoother sweeps inside this run's pressure solves: every
+ * block of such a launch stores the device wall clock at its start and when
+ * all its waves are done, and a launch lasts from its first start to its last
+ * end (the first 2048 timed launches after enabling; launches a converged
+ * solve skips are not counted).  This is the kernel's execution span; a
+ * profiler's dispatch timestamps add the command processor's dispatch and
+ * completion handling.  enable: 1 on (resets the totals), 0 off (keeps them),
+ * -1 leave as is.  Reading the totals synchronises the device.
  * Reports the average ms per launch, the number of launches timed and the
  * sweep's algorithmic bytes (x, b, x' and the 2-byte code per cell). */
 int nbl_run_sweep_timing(nbl_run* run, int enable, double* avg_ms, long long* launches, double* bytes);

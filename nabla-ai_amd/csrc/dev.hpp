@@ -35,6 +35,7 @@ struct MgLevel {
   const uint16_t* code;
   int fx, fy, fz;         // factor from the next-finer level (coarse levels)
   long long nf;           // fluid cells
+  int vec;                // cells per thread along i in the cell kernels (1, or 2 when nx is even)
 };
 
 constexpr uint16_t kFluidBit = 1u << 12;
@@ -58,6 +59,8 @@ struct Geo3 {
   int blocks;
 };
 Geo3 geo3(int nx, int ny, int nz);
+Geo3 geo3n(int nx, int ny, int nz, int cells);  // `cells` consecutive cells along i per thread
+Geo3 levelGeo(const MgLevel& L);                // the launch of a level's cell kernels (L.vec)
 // Launch shape of the fused residual + restriction from L to Lc: one thread per
 // fine cell, blocks covering whole coarse cells (its block count is the
 // number of partials the fold writes).
@@ -158,10 +161,12 @@ void launch_fin(RedBuf rb, const FinArgs& a, SolveCtl* ctl, hipStream_t s);
 void launch_ctl_init(SolveCtl* ctl, double tol, hipStream_t s);
 // Step scalars (step index, time, last dt) set by a kernel on `s`.
 void launch_set_step(StepCtl* sc, int step, double t, double dt, hipStream_t s);
+// stamps: optional per-block device wall-clock stamps (start, and end once
+// the block's stores are visible), 2 per block.
 template <class T>
 void launch_mg_sweep(const MgLevel& L, const T* b, const double* mb, const T* x, const double* mx,
                      bool zeroGuess, const T* ec, const double* mec, const MgLevel* Lc, T* xo, bool fold,
-                     RedBuf rb, const SolveCtl* ctl, hipStream_t s);
+                     RedBuf rb, const SolveCtl* ctl, hipStream_t s, long long* stamps = nullptr);
 // The first two sweeps from a zero guess in one pass (no projection between
 // them: non-singular systems only).
 template <class T>
@@ -197,7 +202,7 @@ void launch_prolong_plain(const MgLevel& L, const MgLevel& Lc, const double* ec,
                           hipStream_t s);
 
 // Flexible CG (MgPoisson::solve) and Jacobi CG (CgPoisson::solve) pieces.
-// Each reducing launcher writes one partial per block of geo3(level 0).
+// Each reducing launcher writes one partial per block of levelGeo(level 0).
 void launch_fcg_init(const MgLevel& L, bool maskSolid, const double* b, double* rhs, double* x, RedBuf rb,
                      hipStream_t s);
 void launch_fcg_init2(const MgLevel& L, bool singular, double* rhs, double* r, const SolveCtl* ctl, RedBuf rb,
@@ -208,6 +213,18 @@ void launch_fcg_apply(const MgLevel& L, const double* p, double* Ap, const Solve
                       hipStream_t s);
 void launch_fcg_update(const MgLevel& L, bool singular, double* x, const double* p, const double* r,
                        const double* Ap, double* rn, const SolveCtl* ctl, RedBuf rb, hipStream_t s);
+// Non-singular systems: the update plus the next V-cycle's first level-0 sweep
+// from zero (x1) and, for an FP32 cycle, its right-hand side b0 = float(rn).
+template <class T>
+void launch_fcg_update_first(const MgLevel& L, double* x, const double* p, const double* r, const double* Ap,
+                             double* rn, T* x1, T* b0, const SolveCtl* ctl, RedBuf rb, hipStream_t s);
+// p_new = z + beta p_old and A p_new, p_new.Ap in one pass.
+void launch_fcg_apply_pu(const MgLevel& L, const double* z, const double* pOld, double* pNew, double* Ap,
+                         const SolveCtl* ctl, RedBuf rb, hipStream_t s);
+// Last level-0 sweep of a V-cycle writing z and the partials of z.rn, z.r.
+template <class T>
+void launch_mg_sweep_dots(const MgLevel& L, const T* b, const T* x, double* z, const double* rn, const double* r,
+                          RedBuf rb, const SolveCtl* ctl, hipStream_t s);
 void launch_fcg_norm(const MgLevel& L, double* rn, const SolveCtl* ctl, RedBuf rb, hipStream_t s);
 void launch_fcg_dots(const MgLevel& L, double* z, const double* mz, const double* rn, const double* r,
                      const SolveCtl* ctl, RedBuf rb, hipStream_t s);
@@ -215,8 +232,6 @@ void launch_fcg_dots(const MgLevel& L, double* z, const double* mz, const double
 template <class T>
 void launch_fcg_dots_x(const MgLevel& L, const T* x, double* z, const double* rn, const double* r,
                        const SolveCtl* ctl, RedBuf rb, hipStream_t s);
-void launch_fcg_pupdate(const MgLevel& L, double* p, const double* z, double* r, const double* rn,
-                        const SolveCtl* ctl, hipStream_t s);
 void launch_fluid_sum(const MgLevel& L, const double* x, RedBuf rb, hipStream_t s);
 void launch_fcg_final(const MgLevel& L, bool singular, const double* x, const double* rhs, double* xo,
                       const SolveCtl* ctl, RedBuf rb, hipStream_t s);

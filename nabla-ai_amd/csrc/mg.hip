@@ -17,19 +17,25 @@
 //
 // Every kernel of a flexible-CG iteration first checks ctl->done, so the host
 // can enqueue iterations before it knows whether the solve has converged.
+#include <stdexcept>
+#include <type_traits>
+
 #include "dev.hpp"
 
 namespace nbl {
 
-Geo3 geo3(int nx, int ny, int nz) {
-  const int bx = nx >= 64 ? 64 : nx >= 32 ? 32 : nx >= 16 ? 16 : 8;
+Geo3 geo3n(int nx, int ny, int nz, int cells) {
+  const int tx = nx / cells;
+  const int bx = tx >= 64 ? 64 : tx >= 32 ? 32 : tx >= 16 ? 16 : 8;
   const int by = kBlock / bx;
   Geo3 g;
   g.block = dim3(bx, by, 1);
-  g.grid = dim3((nx + bx - 1) / bx, (ny + by - 1) / by, nz);
+  g.grid = dim3((tx + bx - 1) / bx, (ny + by - 1) / by, nz);
   g.blocks = int(g.grid.x * g.grid.y * g.grid.z);
   return g;
 }
+Geo3 geo3(int nx, int ny, int nz) { return geo3n(nx, ny, nz, 1); }
+Geo3 levelGeo(const MgLevel& L) { return geo3n(L.nx, L.ny, L.nz, L.vec); }
 
 namespace {
 
@@ -48,6 +54,79 @@ __device__ __forceinline__ Cell cellOf(int nx, int ny, int nz) {
   q.c = q.i + nx * (q.j + ny * q.k);
   q.blk = B.lin;
   return q;
+}
+
+// NC consecutive cells along i per thread (NC = 1 or 2; nx % NC == 0).  With
+// NC = 2 every vector access of a cell pair is one 16-byte (FP64) or 8-byte
+// (FP32) load or store and the two face codes are one 32-bit word.
+template <int NC>
+struct CellN {
+  int i, j, k, c, blk;  // i: first cell of the group
+  bool valid;
+};
+
+template <int NC>
+__device__ __forceinline__ CellN<NC> cellOfN(int nx, int ny, int nz) {
+  CellN<NC> q;
+  const Blk3 B = blk3();
+  q.i = NC * (B.x * int(blockDim.x) + int(threadIdx.x));
+  q.j = B.y * int(blockDim.y) + int(threadIdx.y);
+  q.k = B.z;
+  q.valid = q.i < nx && q.j < ny && q.k < nz;
+  q.c = q.i + nx * (q.j + ny * q.k);
+  q.blk = B.lin;
+  return q;
+}
+
+// Runs f(std::integral_constant<int, NC>, launch shape) with the level's
+// cells per thread.
+template <class F>
+void withCells(const MgLevel& L, F&& f) {
+  if (L.vec == 2) f(std::integral_constant<int, 2>{}, geo3n(L.nx, L.ny, L.nz, 2));
+  else f(std::integral_constant<int, 1>{}, geo3n(L.nx, L.ny, L.nz, 1));
+}
+
+template <class T>
+struct Vec2Of;
+template <>
+struct Vec2Of<double> {
+  using type = double2;
+};
+template <>
+struct Vec2Of<float> {
+  using type = float2;
+};
+
+template <int NC, class T>
+__device__ __forceinline__ void ldN(const T* __restrict__ p, int c, T (&v)[NC]) {
+  if constexpr (NC == 2) {
+    const typename Vec2Of<T>::type w = *reinterpret_cast<const typename Vec2Of<T>::type*>(p + c);
+    v[0] = w.x;
+    v[1] = w.y;
+  } else {
+    v[0] = p[c];
+  }
+}
+template <int NC, class T>
+__device__ __forceinline__ void stN(T* __restrict__ p, int c, const T (&v)[NC]) {
+  if constexpr (NC == 2) {
+    typename Vec2Of<T>::type w;
+    w.x = v[0];
+    w.y = v[1];
+    *reinterpret_cast<typename Vec2Of<T>::type*>(p + c) = w;
+  } else {
+    p[c] = v[0];
+  }
+}
+template <int NC>
+__device__ __forceinline__ void ldCodes(const uint16_t* __restrict__ code, int c, uint16_t (&v)[NC]) {
+  if constexpr (NC == 2) {
+    const uint32_t w = *reinterpret_cast<const uint32_t*>(code + c);
+    v[0] = uint16_t(w & 0xffffu);
+    v[1] = uint16_t(w >> 16);
+  } else {
+    v[0] = code[c];
+  }
 }
 
 // Neighbour across face `dir` (periodic wrap where the code says kind 1 at a
@@ -79,8 +158,7 @@ struct XView {
   int ncx, ncy, sx, sy, sz;
   T em;
   bool hasEm;
-  __device__ __forceinline__ T at(int c, int i, int j, int k) const {
-    T v = x[c];
+  __device__ __forceinline__ T fix(T v, int i, int j, int k) const {
     if (hasM) v = v - m;
     if constexpr (PROLONG) {
       T e = ec[(i >> sx) + ncx * ((j >> sy) + ncy * (k >> sz))];
@@ -89,6 +167,13 @@ struct XView {
     }
     return v;
   }
+  __device__ __forceinline__ T at(int c, int i, int j, int k) const { return fix(x[c], i, j, k); }
+  template <int NC>
+  __device__ __forceinline__ void pack(int c, int i, int j, int k, T (&v)[NC]) const {
+    ldN<NC>(x, c, v);
+#pragma unroll
+    for (int e = 0; e < NC; ++e) v[e] = fix(v[e], i + e, j, k);
+  }
 };
 
 template <class T>
@@ -96,26 +181,12 @@ __device__ __forceinline__ XView<T> plainView(const T* x) {
   return XView<T>{x, T(0), false, nullptr, 1, 1, 0, 0, 0, T(0), false};
 }
 
-// (A x)_c for a fluid cell and the diagonal sum dg (poisson.cpp:102-134): the
-// couplings are accumulated in the reference's face order.  The six neighbour
-// values are read before the face kinds are decoded (every neighbour index is
-// in range: nbOf wraps), so a wavefront issues all its stencil loads in one
-// round trip instead of waiting for the face code first.
-template <class T, bool ZERO, class XV>
-__device__ __forceinline__ T applyRow(const MgLevel& L, uint16_t code, int c, int i, int j, int k, const XV& X,
-                                      T& dg) {
+// (A x)_c from the centre value and the six neighbour values (order -x, +x,
+// -y, +y, -z, +z) and the diagonal sum dg: the couplings are accumulated in
+// the reference's face order (poisson.cpp:102-134).
+template <class T, bool ZERO>
+__device__ __forceinline__ T applyDecode(const MgLevel& L, uint16_t code, T xc, const T (&xn)[6], T& dg) {
   const T h2[3] = {T(L.h2[0]), T(L.h2[1]), T(L.h2[2])};
-  T xn[6];
-  T xc = T(0);
-  if constexpr (!ZERO) {
-#pragma unroll
-    for (int dir = 0; dir < 6; ++dir) {
-      int ii, jj, kk;
-      const int nb = nbOf(L, c, i, j, k, dir, ii, jj, kk);
-      xn[dir] = X.at(nb, ii, jj, kk);
-    }
-    xc = X.at(c, i, j, k);
-  }
   T d = T(0), off = T(0);
 #pragma unroll
   for (int dir = 0; dir < 6; ++dir) {
@@ -130,6 +201,28 @@ __device__ __forceinline__ T applyRow(const MgLevel& L, uint16_t code, int c, in
   }
   dg = d;
   return d * xc - off;
+}
+
+// (A x)_c for a fluid cell and the diagonal sum dg (poisson.cpp:102-134): the
+// couplings are accumulated in the reference's face order.  The six neighbour
+// values are read before the face kinds are decoded (every neighbour index is
+// in range: nbOf wraps), so a wavefront issues all its stencil loads in one
+// round trip instead of waiting for the face code first.
+template <class T, bool ZERO, class XV>
+__device__ __forceinline__ T applyRow(const MgLevel& L, uint16_t code, int c, int i, int j, int k, const XV& X,
+                                      T& dg) {
+  T xn[6];
+  T xc = T(0);
+  if constexpr (!ZERO) {
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir) {
+      int ii, jj, kk;
+      const int nb = nbOf(L, c, i, j, k, dir, ii, jj, kk);
+      xn[dir] = X.at(nb, ii, jj, kk);
+    }
+    xc = X.at(c, i, j, k);
+  }
+  return applyDecode<T, ZERO>(L, code, xc, xn, dg);
 }
 
 // ---------------------------------------------------------------------------
@@ -161,45 +254,106 @@ struct FirstSweepView {
   __device__ __forceinline__ T at(int c, int i, int j, int k) const {
     return firstSweep<T>(*L, L->code[c], c, i, j, k, b[c]);
   }
+  template <int NC>
+  __device__ __forceinline__ void pack(int c, int i, int j, int k, T (&v)[NC]) const {
+    uint16_t cd[NC];
+    T bv[NC];
+    ldCodes<NC>(L->code, c, cd);
+    ldN<NC>(b, c, bv);
+#pragma unroll
+    for (int e = 0; e < NC; ++e) v[e] = firstSweep<T>(*L, cd[e], c + e, i + e, j, k, bv[e]);
+  }
 };
 
-template <class T, int MODE>
+// Centre values and the six neighbour values (order -x, +x, -y, +y, -z, +z)
+// of the NC cells of a group, read through a view X (plain, pending mean,
+// prolongation, ...).  Neighbour indices wrap periodically like nbOf; the
+// group's inner neighbours along i come from the centre pack.
+template <class T, int NC, class XV>
+__device__ __forceinline__ void gatherN(const MgLevel& L, const XV& X, int c, int i, int j, int k, T (&xc)[NC],
+                                        T (&xn)[NC][6]) {
+  const int sy = L.nx, sz = L.nx * L.ny;
+  T jm[NC], jp[NC], km[NC], kp[NC];
+  X.template pack<NC>(c, i, j, k, xc);
+  const int im = i > 0 ? i - 1 : L.nx - 1;
+  const int cm = i > 0 ? c - 1 : c + L.nx - 1;
+  const int ip = i + NC < L.nx ? i + NC : 0;
+  const int cp = i + NC < L.nx ? c + NC : c + NC - L.nx;
+  const T xm = X.at(cm, im, j, k);
+  const T xp = X.at(cp, ip, j, k);
+  const int jjm = j > 0 ? j - 1 : L.ny - 1, jjp = j < L.ny - 1 ? j + 1 : 0;
+  const int kkm = k > 0 ? k - 1 : L.nz - 1, kkp = k < L.nz - 1 ? k + 1 : 0;
+  X.template pack<NC>(j > 0 ? c - sy : c + (L.ny - 1) * sy, i, jjm, k, jm);
+  X.template pack<NC>(j < L.ny - 1 ? c + sy : c - (L.ny - 1) * sy, i, jjp, k, jp);
+  X.template pack<NC>(k > 0 ? c - sz : c + (L.nz - 1) * sz, i, j, kkm, km);
+  X.template pack<NC>(k < L.nz - 1 ? c + sz : c - (L.nz - 1) * sz, i, j, kkp, kp);
+#pragma unroll
+  for (int e = 0; e < NC; ++e) {
+    xn[e][0] = e == 0 ? xm : xc[e - 1];
+    xn[e][1] = e == NC - 1 ? xp : xc[e + 1];
+    xn[e][2] = jm[e];
+    xn[e][3] = jp[e];
+    xn[e][4] = km[e];
+    xn[e][5] = kp[e];
+  }
+}
+
+template <class T, int MODE, int NC>
 __global__ void __launch_bounds__(kBlock) k_sweep(MgLevel L, const T* __restrict__ b, const double* mb,
                                                   const T* __restrict__ x, const double* mx, const T* ec,
                                                   const double* mec, int ncx, int ncy, int sx, int sy, int sz,
-                                                  T* __restrict__ xo, int fold, RedBuf rb, const SolveCtl* ctl) {
+                                                  T* __restrict__ xo, int fold, RedBuf rb, const SolveCtl* ctl,
+                                                  long long* stamps) {
   if (ctl && ldA(&ctl->done)) return;
-  const Cell q = cellOf(L.nx, L.ny, L.nz);
+  const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz);
+  const bool t0 = threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0;
+  if (stamps && t0) stamps[2 * q.blk] = wall_clock64();
   double acc[1] = {0.0};
   if (q.valid) {
-    const uint16_t code = L.code[q.c];
-    if (!(code & kFluidBit)) {
-      xo[q.c] = T(0);
-    } else {
-      T dg, Ax, xc;
-      if constexpr (MODE == SW_PAIR) {
-        const FirstSweepView<T> X{&L, b};
-        Ax = applyRow<T, false>(L, code, q.c, q.i, q.j, q.k, X, dg);
-        xc = X.at(q.c, q.i, q.j, q.k);
-      } else if constexpr (MODE == SW_PROLONG) {
-        const XView<T, true> X{x, mx ? T(ldA(mx)) : T(0), mx != nullptr, ec, ncx, ncy, sx, sy, sz,
-                               mec ? T(ldA(mec)) : T(0), mec != nullptr};
-        Ax = applyRow<T, false>(L, code, q.c, q.i, q.j, q.k, X, dg);
-        xc = X.at(q.c, q.i, q.j, q.k);
+    uint16_t code[NC];
+    ldCodes<NC>(L.code, q.c, code);
+    T xc[NC], xn[NC][6];
+    if constexpr (MODE == SW_PAIR) {
+      gatherN<T, NC>(L, FirstSweepView<T>{&L, b}, q.c, q.i, q.j, q.k, xc, xn);
+    } else if constexpr (MODE == SW_PROLONG) {
+      const XView<T, true> X{x, mx ? T(ldA(mx)) : T(0), mx != nullptr, ec, ncx, ncy, sx, sy, sz,
+                             mec ? T(ldA(mec)) : T(0), mec != nullptr};
+      gatherN<T, NC>(L, X, q.c, q.i, q.j, q.k, xc, xn);
+    } else if constexpr (MODE == SW_PLAIN) {
+      const XView<T> X{x, mx ? T(ldA(mx)) : T(0), mx != nullptr, nullptr, 1, 1, 0, 0, 0, T(0), false};
+      gatherN<T, NC>(L, X, q.c, q.i, q.j, q.k, xc, xn);
+    }
+    T bv[NC], out[NC];
+    ldN<NC>(b, q.c, bv);
+#pragma unroll
+    for (int e = 0; e < NC; ++e) {
+      if (!(code[e] & kFluidBit)) {
+        out[e] = T(0);
+        continue;
+      }
+      T dg, Ax, x0;
+      if constexpr (MODE == SW_ZERO) {
+        const T zn[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+        Ax = applyDecode<T, true>(L, code[e], T(0), zn, dg);
+        x0 = T(0);
       } else {
-        const XView<T> X{x, mx ? T(ldA(mx)) : T(0), mx != nullptr, nullptr, 1, 1, 0, 0, 0, T(0), false};
-        Ax = applyRow<T, MODE == SW_ZERO>(L, code, q.c, q.i, q.j, q.k, X, dg);
-        xc = MODE == SW_ZERO ? T(0) : X.at(q.c, q.i, q.j, q.k);
+        Ax = applyDecode<T, false>(L, code[e], xc[e], xn[e], dg);
+        x0 = xc[e];
       }
       const T diag = dg > T(0) ? dg : T(1);
-      T bv = b[q.c];
-      if (mb) bv = bv - T(ldA(mb));
-      const T xn = xc + T(0.8) * (bv - Ax) / diag;
-      xo[q.c] = xn;
-      acc[0] = double(xn);
+      T bb = bv[e];
+      if (mb) bb = bb - T(ldA(mb));
+      const T xnew = x0 + T(0.8) * (bb - Ax) / diag;
+      out[e] = xnew;
+      acc[0] += double(xnew);
     }
+    stN<NC>(xo, q.c, out);
   }
   if (fold) blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
+  if (stamps) {  // uniform across the grid: end stamp once every wave of the block is done
+    __syncthreads();
+    if (t0) stamps[2 * q.blk + 1] = wall_clock64();
+  }
 }
 
 // Residual r = b - A x on fine fluid cells restricted to the coarse grid as
@@ -503,33 +657,32 @@ __global__ void k_inv_store(const double* __restrict__ G, int n, const uint16_t*
   Mf[size_t(i) * n + j] = float(v);
 }
 
-// x = Minv (b - mb on fluid cells): kDirectRows rows per block, each thread
+// x = Minv (b - mb on fluid cells): ROWS rows per block, each thread
 // striding the columns; fixed-order block fold.
-constexpr int kDirectRows = 4;
-template <class T, class MT>
+template <class T, class MT, int ROWS>
 __global__ void __launch_bounds__(kBlock) k_coarse_direct(MgLevel L, const MT* __restrict__ M,
                                                           const T* __restrict__ b, const double* mb,
                                                           T* __restrict__ x, const SolveCtl* ctl) {
   if (ctl && ldA(&ctl->done)) return;
   const int n = int(L.n);
-  const int r0 = blockIdx.x * kDirectRows;
+  const int r0 = blockIdx.x * ROWS;
   const double m = mb ? ldA(mb) : 0.0;
-  double acc[kDirectRows];
+  double acc[ROWS];
 #pragma unroll
-  for (int r = 0; r < kDirectRows; ++r) acc[r] = 0.0;
+  for (int r = 0; r < ROWS; ++r) acc[r] = 0.0;
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
     T bj = b[j];
     if (mb && (L.code[j] & kFluidBit)) bj = bj - T(m);
     const double bv = double(bj);
 #pragma unroll
-    for (int r = 0; r < kDirectRows; ++r)
+    for (int r = 0; r < ROWS; ++r)
       if (r0 + r < n) acc[r] = fma(double(M[size_t(r0 + r) * n + j]), bv, acc[r]);
   }
-  __shared__ double sh[kDirectRows * 16];
-  blockFold<kDirectRows, 0u>(acc, sh);
+  __shared__ double sh[ROWS * 16];
+  blockFold<ROWS, 0u>(acc, sh);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int r = 0; r < kDirectRows; ++r)
+    for (int r = 0; r < ROWS; ++r)
       if (r0 + r < n) x[r0 + r] = T(acc[r]);
 }
 
@@ -650,20 +803,28 @@ __global__ void __launch_bounds__(1024) k_coarse_global(MgLevel L, int singular,
 
 // Level-0 preconditioner output: z = x - pending mean (fluid), 0 elsewhere,
 // with fluid partial sums for Minv's own projection (multigrid.cpp:283-289).
-template <class T>
+template <class T, int NC>
 __global__ void __launch_bounds__(kBlock) k_finish(MgLevel L, const T* __restrict__ x, const double* mx,
                                                    double* __restrict__ z, int fold, RedBuf rb,
                                                    const SolveCtl* ctl) {
   if (ctl && ldA(&ctl->done)) return;
-  const Cell q = cellOf(L.nx, L.ny, L.nz);
+  const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
   if (q.valid) {
-    const bool fl = L.code[q.c] & kFluidBit;
-    T v = x[q.c];
-    if (mx && fl) v = v - T(ldA(mx));
-    const double zv = fl ? double(v) : 0.0;
-    z[q.c] = zv;
-    acc[0] = fl ? zv : 0.0;
+    uint16_t code[NC];
+    T xv[NC];
+    double zv[NC];
+    ldCodes<NC>(L.code, q.c, code);
+    ldN<NC>(x, q.c, xv);
+#pragma unroll
+    for (int e = 0; e < NC; ++e) {
+      const bool fl = code[e] & kFluidBit;
+      T v = xv[e];
+      if (mx && fl) v = v - T(ldA(mx));
+      zv[e] = fl ? double(v) : 0.0;
+      acc[0] += zv[e];
+    }
+    stN<NC>(z, q.c, zv);
   }
   if (fold) blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
@@ -712,227 +873,427 @@ __global__ void k_prolong(MgLevel L, MgLevel Lc, const double* __restrict__ ec, 
 // ---------------------------------------------------------------------------
 // Flexible CG (MgPoisson::solve, multigrid.cpp:206-355) and Jacobi CG
 // (CgPoisson::solve, poisson.cpp:136-252).  Scalars live in SolveCtl; each
-// reducing kernel writes per-block partials and is followed by k_fin.
+// reducing kernel writes per-block partials and is followed by k_fin.  All
+// run NC cells per thread along i (the level's vec) on levelGeo's launch.
 // ---------------------------------------------------------------------------
 enum : int { S_MEAN_RHS = 0, S_MEAN_RN = 1, S_MEAN_X = 2 };
 
+#define NBL_CELLS(NC)                                   \
+  const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz);   \
+  uint16_t code[NC];                                    \
+  if (q.valid) ldCodes<NC>(L.code, q.c, code)
+
+template <int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_init(MgLevel L, int maskSolid, const double* __restrict__ b,
                                                      double* __restrict__ rhs, double* __restrict__ x, RedBuf rb) {
-  const Cell q = cellOf(L.nx, L.ny, L.nz);
+  NBL_CELLS(NC);
   double acc[1] = {0.0};
   if (q.valid) {
-    const bool fl = L.code[q.c] & kFluidBit;
-    const double v = (fl || !maskSolid) ? b[q.c] : 0.0;
-    rhs[q.c] = v;
-    x[q.c] = 0.0;
-    acc[0] = fl ? v : 0.0;
+    double bv[NC], v[NC], z[NC];
+    ldN<NC>(b, q.c, bv);
+#pragma unroll
+    for (int e = 0; e < NC; ++e) {
+      const bool fl = code[e] & kFluidBit;
+      v[e] = (fl || !maskSolid) ? bv[e] : 0.0;
+      z[e] = 0.0;
+      acc[0] += fl ? v[e] : 0.0;
+    }
+    stN<NC>(rhs, q.c, v);
+    stN<NC>(x, q.c, z);
   }
   blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
 // rhs <- rhs - mean (fluid) / 0 (solid) on singular systems; r = rhs; |rhs|^2.
+template <int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_init2(MgLevel L, int singular, double* __restrict__ rhs,
                                                       double* __restrict__ r, const SolveCtl* ctl, RedBuf rb) {
-  const Cell q = cellOf(L.nx, L.ny, L.nz);
+  NBL_CELLS(NC);
   double acc[1] = {0.0};
   if (q.valid) {
-    double v = rhs[q.c];
+    double v[NC];
+    ldN<NC>(rhs, q.c, v);
     if (singular) {
-      v = (L.code[q.c] & kFluidBit) ? v - ldA(&ctl->slot[S_MEAN_RHS]) : 0.0;
-      rhs[q.c] = v;
+      const double m = ldA(&ctl->slot[S_MEAN_RHS]);
+#pragma unroll
+      for (int e = 0; e < NC; ++e) v[e] = (code[e] & kFluidBit) ? v[e] - m : 0.0;
+      stN<NC>(rhs, q.c, v);
     }
-    r[q.c] = v;
-    acc[0] = v * v;
+    stN<NC>(r, q.c, v);
+#pragma unroll
+    for (int e = 0; e < NC; ++e) acc[0] += v[e] * v[e];
   }
   blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
 // z <- z - pending mean (fluid); p = z; r.z
+template <int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_start(MgLevel L, double* __restrict__ z, const double* mz,
                                                       const double* __restrict__ r, double* __restrict__ p,
                                                       const SolveCtl* ctl, RedBuf rb) {
   if (ldA(&ctl->done)) return;
-  const Cell q = cellOf(L.nx, L.ny, L.nz);
+  NBL_CELLS(NC);
   double acc[1] = {0.0};
   if (q.valid) {
-    double v = z[q.c];
-    if (mz && (L.code[q.c] & kFluidBit)) v = v - ldA(mz);
-    z[q.c] = v;
-    p[q.c] = v;
-    acc[0] = r[q.c] * v;
+    double v[NC], rv[NC];
+    ldN<NC>(z, q.c, v);
+    ldN<NC>(r, q.c, rv);
+#pragma unroll
+    for (int e = 0; e < NC; ++e) {
+      if (mz && (code[e] & kFluidBit)) v[e] = v[e] - ldA(mz);
+      acc[0] += rv[e] * v[e];
+    }
+    stN<NC>(z, q.c, v);
+    stN<NC>(p, q.c, v);
   }
   blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
+// Ap (identity rows on solid cells) and p.Ap, p read through the view P.
+template <int NC, class PV>
+__device__ __forceinline__ void applyN(const MgLevel& L, const CellN<NC>& q, const uint16_t (&code)[NC],
+                                       const PV& P, double (&pv)[NC], double (&Ap)[NC]) {
+  double xn[NC][6];
+  gatherN<double, NC>(L, P, q.c, q.i, q.j, q.k, pv, xn);
+#pragma unroll
+  for (int e = 0; e < NC; ++e) {
+    if (code[e] & kFluidBit) {
+      double dg;
+      Ap[e] = applyDecode<double, false>(L, code[e], pv[e], xn[e], dg);
+    } else {
+      Ap[e] = pv[e];
+    }
+  }
+}
+
+template <int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_apply(MgLevel L, const double* __restrict__ p,
                                                       double* __restrict__ Ap, const SolveCtl* ctl, RedBuf rb) {
   if (ldA(&ctl->done)) return;
-  const Cell q = cellOf(L.nx, L.ny, L.nz);
+  NBL_CELLS(NC);
   double acc[1] = {0.0};
   if (q.valid) {
-    const uint16_t code = L.code[q.c];
-    double v;
-    if (code & kFluidBit) {
-      double dg;
-      v = applyRow<double, false>(L, code, q.c, q.i, q.j, q.k, plainView<double>(p), dg);
-    } else {
-      v = p[q.c];
-    }
-    Ap[q.c] = v;
-    acc[0] = p[q.c] * v;
+    double pv[NC], av[NC];
+    applyN<NC>(L, q, code, plainView<double>(p), pv, av);
+#pragma unroll
+    for (int e = 0; e < NC; ++e) acc[0] += pv[e] * av[e];
+    stN<NC>(Ap, q.c, av);
+  }
+  blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
+}
+
+// p_new = z + beta p_old (multigrid.cpp:336-338) fused into the next
+// iteration's A p and p.Ap: every stencil point re-forms z + beta p_old with
+// the same expression, so A p_new is bit-identical to applying A to the
+// stored p_new.  p_old and p_new are different buffers.
+struct PNewView {
+  const double* __restrict__ z;
+  const double* __restrict__ p;
+  double be;
+  __device__ __forceinline__ double at(int c, int, int, int) const { return z[c] + be * p[c]; }
+  template <int NC>
+  __device__ __forceinline__ void pack(int c, int, int, int, double (&v)[NC]) const {
+    double zv[NC], pv[NC];
+    ldN<NC>(z, c, zv);
+    ldN<NC>(p, c, pv);
+#pragma unroll
+    for (int e = 0; e < NC; ++e) v[e] = zv[e] + be * pv[e];
+  }
+};
+
+template <int NC>
+__global__ void __launch_bounds__(kBlock) k_fcg_apply_pu(MgLevel L, const double* __restrict__ z,
+                                                         const double* __restrict__ pOld, double* __restrict__ pNew,
+                                                         double* __restrict__ Ap, const SolveCtl* ctl, RedBuf rb) {
+  if (ldA(&ctl->done)) return;
+  NBL_CELLS(NC);
+  double acc[1] = {0.0};
+  if (q.valid) {
+    double pv[NC], av[NC];
+    applyN<NC>(L, q, code, PNewView{z, pOld, ldA(&ctl->beta)}, pv, av);
+#pragma unroll
+    for (int e = 0; e < NC; ++e) acc[0] += pv[e] * av[e];
+    stN<NC>(pNew, q.c, pv);
+    stN<NC>(Ap, q.c, av);
   }
   blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
 // x += alpha p; rn = r - alpha Ap; fluid sum of rn (singular) or |rn|^2.
+template <int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_update(MgLevel L, int singular, double* __restrict__ x,
                                                        const double* __restrict__ p, const double* __restrict__ r,
                                                        const double* __restrict__ Ap, double* __restrict__ rn,
                                                        const SolveCtl* ctl, RedBuf rb) {
   if (ldA(&ctl->done)) return;
   const double a = ldA(&ctl->alpha);
-  const Cell q = cellOf(L.nx, L.ny, L.nz);
+  NBL_CELLS(NC);
   double acc[1] = {0.0};
   if (q.valid) {
-    x[q.c] = x[q.c] + a * p[q.c];
-    const double v = r[q.c] - a * Ap[q.c];
-    rn[q.c] = v;
-    if (singular) acc[0] = (L.code[q.c] & kFluidBit) ? v : 0.0;
-    else acc[0] = v * v;
+    double xv[NC], pv[NC], rv[NC], av[NC];
+    ldN<NC>(x, q.c, xv);
+    ldN<NC>(p, q.c, pv);
+    ldN<NC>(r, q.c, rv);
+    ldN<NC>(Ap, q.c, av);
+#pragma unroll
+    for (int e = 0; e < NC; ++e) {
+      xv[e] = xv[e] + a * pv[e];
+      rv[e] = rv[e] - a * av[e];
+      if (singular) acc[0] += (code[e] & kFluidBit) ? rv[e] : 0.0;
+      else acc[0] += rv[e] * rv[e];
+    }
+    stN<NC>(x, q.c, xv);
+    stN<NC>(rn, q.c, rv);
   }
   blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
+// Non-singular systems: k_fcg_update plus the first smoother sweep of the
+// next V-cycle from a zero guess, which is pointwise (x1 = 0.8 rn / diag on
+// fluid cells, k_sweep<SW_ZERO>'s expression), and for an FP32 V-cycle the
+// level-0 right-hand side float(rn); partials of |rn|^2.
+template <class T, int NC>
+__global__ void __launch_bounds__(kBlock) k_fcg_update_first(MgLevel L, double* __restrict__ x,
+                                                             const double* __restrict__ p,
+                                                             const double* __restrict__ r,
+                                                             const double* __restrict__ Ap, double* __restrict__ rn,
+                                                             T* __restrict__ x1, T* __restrict__ b0,
+                                                             const SolveCtl* ctl, RedBuf rb) {
+  if (ldA(&ctl->done)) return;
+  const double a = ldA(&ctl->alpha);
+  NBL_CELLS(NC);
+  double acc[1] = {0.0};
+  if (q.valid) {
+    double xv[NC], pv[NC], rv[NC], av[NC];
+    T bv[NC], fv[NC];
+    ldN<NC>(x, q.c, xv);
+    ldN<NC>(p, q.c, pv);
+    ldN<NC>(r, q.c, rv);
+    ldN<NC>(Ap, q.c, av);
+#pragma unroll
+    for (int e = 0; e < NC; ++e) {
+      xv[e] = xv[e] + a * pv[e];
+      rv[e] = rv[e] - a * av[e];
+      bv[e] = T(rv[e]);
+      fv[e] = firstSweep<T>(L, code[e], q.c + e, q.i + e, q.j, q.k, bv[e]);
+      acc[0] += rv[e] * rv[e];
+    }
+    stN<NC>(x, q.c, xv);
+    stN<NC>(rn, q.c, rv);
+    if (b0) stN<NC>(b0, q.c, bv);
+    stN<NC>(x1, q.c, fv);
+  }
+  blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
+}
+
+template <int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_norm(MgLevel L, double* __restrict__ rn, const SolveCtl* ctl,
                                                      RedBuf rb) {
   if (ldA(&ctl->done)) return;
-  const Cell q = cellOf(L.nx, L.ny, L.nz);
+  NBL_CELLS(NC);
   double acc[1] = {0.0};
   if (q.valid) {
-    const double v = (L.code[q.c] & kFluidBit) ? rn[q.c] - ldA(&ctl->slot[S_MEAN_RN]) : 0.0;
-    rn[q.c] = v;
-    acc[0] = v * v;
+    const double m = ldA(&ctl->slot[S_MEAN_RN]);
+    double v[NC];
+    ldN<NC>(rn, q.c, v);
+#pragma unroll
+    for (int e = 0; e < NC; ++e) {
+      v[e] = (code[e] & kFluidBit) ? v[e] - m : 0.0;
+      acc[0] += v[e] * v[e];
+    }
+    stN<NC>(rn, q.c, v);
   }
   blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
 // z <- z - pending mean; z.rn and z.r (flexible Polak-Ribiere beta,
 // multigrid.cpp:328-339)
+template <int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_dots(MgLevel L, double* __restrict__ z, const double* mz,
                                                      const double* __restrict__ rn, const double* __restrict__ r,
                                                      const SolveCtl* ctl, RedBuf rb) {
   if (ldA(&ctl->done)) return;
-  const Cell q = cellOf(L.nx, L.ny, L.nz);
+  NBL_CELLS(NC);
   double acc[2] = {0.0, 0.0};
   if (q.valid) {
-    double v = z[q.c];
-    if (mz && (L.code[q.c] & kFluidBit)) v = v - ldA(mz);
-    z[q.c] = v;
-    acc[0] = v * rn[q.c];
-    acc[1] = v * r[q.c];
+    double v[NC], a[NC], c[NC];
+    ldN<NC>(z, q.c, v);
+    ldN<NC>(rn, q.c, a);
+    ldN<NC>(r, q.c, c);
+#pragma unroll
+    for (int e = 0; e < NC; ++e) {
+      if (mz && (code[e] & kFluidBit)) v[e] = v[e] - ldA(mz);
+      acc[0] += v[e] * a[e];
+      acc[1] += v[e] * c[e];
+    }
+    stN<NC>(z, q.c, v);
   }
   blockToPartials<2>(acc, rb.partials, rb.stride, q.blk);
 }
 
 // k_finish + k_fcg_dots in one pass for systems without projections: z is the
 // V-cycle output on fluid cells (converted to FP64), 0 elsewhere.
-template <class T>
+template <class T, int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_dots_x(MgLevel L, const T* __restrict__ x, double* __restrict__ z,
                                                        const double* __restrict__ rn, const double* __restrict__ r,
                                                        const SolveCtl* ctl, RedBuf rb) {
   if (ldA(&ctl->done)) return;
-  const Cell q = cellOf(L.nx, L.ny, L.nz);
+  NBL_CELLS(NC);
   double acc[2] = {0.0, 0.0};
   if (q.valid) {
-    const double v = (L.code[q.c] & kFluidBit) ? double(x[q.c]) : 0.0;
-    z[q.c] = v;
-    acc[0] = v * rn[q.c];
-    acc[1] = v * r[q.c];
+    T xv[NC];
+    double v[NC], a[NC], c[NC];
+    ldN<NC>(x, q.c, xv);
+    ldN<NC>(rn, q.c, a);
+    ldN<NC>(r, q.c, c);
+#pragma unroll
+    for (int e = 0; e < NC; ++e) {
+      v[e] = (code[e] & kFluidBit) ? double(xv[e]) : 0.0;
+      acc[0] += v[e] * a[e];
+      acc[1] += v[e] * c[e];
+    }
+    stN<NC>(z, q.c, v);
   }
   blockToPartials<2>(acc, rb.partials, rb.stride, q.blk);
 }
 
-__global__ void __launch_bounds__(kBlock) k_fcg_pupdate(int n, double* __restrict__ p, const double* __restrict__ z,
-                                                        double* __restrict__ r, const double* __restrict__ rn,
-                                                        const SolveCtl* ctl) {
+// The last level-0 sweep of a V-cycle with the preconditioner output and the
+// flexible-CG dots folded in (non-singular systems): z = the sweep's result
+// (FP64) on fluid cells, 0 elsewhere; partials of z.rn and z.r.  Same values
+// as k_sweep<SW_PLAIN> followed by k_fcg_dots_x.
+template <class T, int NC>
+__global__ void __launch_bounds__(kBlock) k_sweep_dots(MgLevel L, const T* __restrict__ b, const T* __restrict__ x,
+                                                       double* __restrict__ z, const double* __restrict__ rn,
+                                                       const double* __restrict__ r, RedBuf rb, const SolveCtl* ctl) {
   if (ldA(&ctl->done)) return;
-  const double be = ldA(&ctl->beta);
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  p[c] = z[c] + be * p[c];
-  if (r) r[c] = rn[c];
+  NBL_CELLS(NC);
+  double acc[2] = {0.0, 0.0};
+  if (q.valid) {
+    T xc[NC], xn[NC][6], bv[NC];
+    double v[NC], a[NC], c[NC];
+    gatherN<T, NC>(L, plainView<T>(x), q.c, q.i, q.j, q.k, xc, xn);
+    ldN<NC>(b, q.c, bv);
+    ldN<NC>(rn, q.c, a);
+    ldN<NC>(r, q.c, c);
+#pragma unroll
+    for (int e = 0; e < NC; ++e) {
+      v[e] = 0.0;
+      if (code[e] & kFluidBit) {
+        T dg;
+        const T Ax = applyDecode<T, false>(L, code[e], xc[e], xn[e], dg);
+        const T diag = dg > T(0) ? dg : T(1);
+        v[e] = double(xc[e] + T(0.8) * (bv[e] - Ax) / diag);
+      }
+      acc[0] += v[e] * a[e];
+      acc[1] += v[e] * c[e];
+    }
+    stN<NC>(z, q.c, v);
+  }
+  blockToPartials<2>(acc, rb.partials, rb.stride, q.blk);
 }
 
+template <int NC>
 __global__ void __launch_bounds__(kBlock) k_fluid_sum(MgLevel L, const double* __restrict__ x, RedBuf rb) {
-  const Cell q = cellOf(L.nx, L.ny, L.nz);
+  NBL_CELLS(NC);
   double acc[1] = {0.0};
-  if (q.valid && (L.code[q.c] & kFluidBit)) acc[0] = x[q.c];
+  if (q.valid) {
+    double v[NC];
+    ldN<NC>(x, q.c, v);
+#pragma unroll
+    for (int e = 0; e < NC; ++e)
+      if (code[e] & kFluidBit) acc[0] += v[e];
+  }
   blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
 // Final projection of x and the true residual (multigrid.cpp:342-354).
+template <int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_final(MgLevel L, int singular, const double* __restrict__ x,
                                                       const double* __restrict__ rhs, double* __restrict__ xo,
                                                       const SolveCtl* ctl, RedBuf rb) {
   const double m = singular ? ldA(&ctl->slot[S_MEAN_X]) : 0.0;
   const XView<double> X{x, m, singular != 0, nullptr, 1, 1, 0, 0, 0, 0.0, false};
-  const Cell q = cellOf(L.nx, L.ny, L.nz);
+  NBL_CELLS(NC);
   double acc[1] = {0.0};
   if (q.valid) {
-    const uint16_t code = L.code[q.c];
-    double Ax, xv;
-    if (code & kFluidBit) {
-      double dg;
-      Ax = applyRow<double, false>(L, code, q.c, q.i, q.j, q.k, X, dg);
-      xv = singular ? x[q.c] - m : x[q.c];
-    } else {
-      xv = singular ? 0.0 : x[q.c];
-      Ax = xv;
+    double xc[NC], xn[NC][6], raw[NC], hv[NC], out[NC];
+    gatherN<double, NC>(L, X, q.c, q.i, q.j, q.k, xc, xn);
+    ldN<NC>(x, q.c, raw);
+    ldN<NC>(rhs, q.c, hv);
+#pragma unroll
+    for (int e = 0; e < NC; ++e) {
+      double Ax;
+      if (code[e] & kFluidBit) {
+        double dg;
+        Ax = applyDecode<double, false>(L, code[e], xc[e], xn[e], dg);
+        out[e] = singular ? raw[e] - m : raw[e];
+      } else {
+        out[e] = singular ? 0.0 : raw[e];
+        Ax = out[e];
+      }
+      const double er = hv[e] - Ax;
+      acc[0] += er * er;
     }
-    xo[q.c] = xv;
-    const double e = rhs[q.c] - Ax;
-    acc[0] = e * e;
+    stN<NC>(xo, q.c, out);
   }
   blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
 // Jacobi-CG: z = r/diag and r.z (first: also p = z).
+template <int NC>
 __global__ void __launch_bounds__(kBlock) k_cg_z(MgLevel L, const double* __restrict__ r, double* __restrict__ z,
                                                  double* __restrict__ p, int first, const SolveCtl* ctl, RedBuf rb) {
   if (ldA(&ctl->done)) return;
-  const Cell q = cellOf(L.nx, L.ny, L.nz);
+  NBL_CELLS(NC);
   double acc[1] = {0.0};
   if (q.valid) {
-    const uint16_t code = L.code[q.c];
-    double diag = 1.0;
-    if (code & kFluidBit) {
-      double dg;
-      applyRow<double, true>(L, code, q.c, q.i, q.j, q.k, plainView<double>(nullptr), dg);
-      diag = dg > 0.0 ? dg : 1.0;
+    double rv[NC], zv[NC];
+    ldN<NC>(r, q.c, rv);
+#pragma unroll
+    for (int e = 0; e < NC; ++e) {
+      double diag = 1.0;
+      if (code[e] & kFluidBit) {
+        double dg;
+        const double zn[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        applyDecode<double, true>(L, code[e], 0.0, zn, dg);
+        diag = dg > 0.0 ? dg : 1.0;
+      }
+      zv[e] = rv[e] / diag;
+      acc[0] += rv[e] * zv[e];
     }
-    const double zv = r[q.c] / diag;
-    z[q.c] = zv;
-    if (first) p[q.c] = zv;
-    acc[0] = r[q.c] * zv;
+    stN<NC>(z, q.c, zv);
+    if (first) stN<NC>(p, q.c, zv);
   }
   blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
+template <int NC>
 __global__ void __launch_bounds__(kBlock) k_cg_update(MgLevel L, double* __restrict__ x, const double* __restrict__ p,
                                                       double* __restrict__ r, const double* __restrict__ Ap,
                                                       const SolveCtl* ctl, RedBuf rb) {
   if (ldA(&ctl->done)) return;
   const double a = ldA(&ctl->alpha);
-  const Cell q = cellOf(L.nx, L.ny, L.nz);
+  NBL_CELLS(NC);
   double acc[1] = {0.0};
   if (q.valid) {
-    x[q.c] = x[q.c] + a * p[q.c];
-    const double v = r[q.c] - a * Ap[q.c];
-    r[q.c] = v;
-    acc[0] = v * v;
+    double xv[NC], pv[NC], rv[NC], av[NC];
+    ldN<NC>(x, q.c, xv);
+    ldN<NC>(p, q.c, pv);
+    ldN<NC>(r, q.c, rv);
+    ldN<NC>(Ap, q.c, av);
+#pragma unroll
+    for (int e = 0; e < NC; ++e) {
+      xv[e] = xv[e] + a * pv[e];
+      rv[e] = rv[e] - a * av[e];
+      acc[0] += rv[e] * rv[e];
+    }
+    stN<NC>(x, q.c, xv);
+    stN<NC>(r, q.c, rv);
   }
   blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
+
+#undef NBL_CELLS
 
 // One-block finalize: fold the partials of the preceding reducing kernel and
 // apply the scalar update `op`.  Thread 0 reads the scalars coherently into a
@@ -1070,25 +1431,31 @@ void launch_fin(RedBuf rb, const FinArgs& a, SolveCtl* ctl, hipStream_t s) {
 template <class T>
 void launch_mg_sweep(const MgLevel& L, const T* b, const double* mb, const T* x, const double* mx,
                      bool zeroGuess, const T* ec, const double* mec, const MgLevel* Lc, T* xo, bool fold,
-                     RedBuf rb, const SolveCtl* ctl, hipStream_t s) {
-  const Geo3 g = geo3(L.nx, L.ny, L.nz);
+                     RedBuf rb, const SolveCtl* ctl, hipStream_t s, long long* stamps) {
   const int ncx = Lc ? Lc->nx : 1, ncy = Lc ? Lc->ny : 1;
   const int sx = Lc ? Lc->fx - 1 : 0, sy = Lc ? Lc->fy - 1 : 0, sz = Lc ? Lc->fz - 1 : 0;
   const int f = fold ? 1 : 0;
-  if (zeroGuess)
-    k_sweep<T, SW_ZERO><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, nullptr, nullptr, 1, 1, 0, 0, 0, xo, f, rb, ctl);
-  else if (ec)
-    k_sweep<T, SW_PROLONG><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, ec, mec, ncx, ncy, sx, sy, sz, xo, f, rb, ctl);
-  else
-    k_sweep<T, SW_PLAIN><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, nullptr, nullptr, 1, 1, 0, 0, 0, xo, f, rb, ctl);
+  withCells(L, [&](auto nc, const Geo3& g) {
+    constexpr int NC = decltype(nc)::value;
+    if (zeroGuess)
+      k_sweep<T, SW_ZERO, NC><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, nullptr, nullptr, 1, 1, 0, 0, 0, xo, f, rb,
+                                                         ctl, stamps);
+    else if (ec)
+      k_sweep<T, SW_PROLONG, NC><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, ec, mec, ncx, ncy, sx, sy, sz, xo, f,
+                                                            rb, ctl, stamps);
+    else
+      k_sweep<T, SW_PLAIN, NC><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, nullptr, nullptr, 1, 1, 0, 0, 0, xo, f, rb,
+                                                          ctl, stamps);
+  });
   NBL_LAUNCH_CHECK();
 }
 
 template <class T>
 void launch_mg_sweep_pair(const MgLevel& L, const T* b, T* xo, const SolveCtl* ctl, hipStream_t s) {
-  const Geo3 g = geo3(L.nx, L.ny, L.nz);
-  k_sweep<T, SW_PAIR><<<g.grid, g.block, 0, s>>>(L, b, nullptr, nullptr, nullptr, nullptr, nullptr, 1, 1, 0, 0, 0, xo,
-                                                 0, RedBuf{nullptr, 0}, ctl);
+  withCells(L, [&](auto nc, const Geo3& g) {
+    k_sweep<T, SW_PAIR, decltype(nc)::value><<<g.grid, g.block, 0, s>>>(
+        L, b, nullptr, nullptr, nullptr, nullptr, nullptr, 1, 1, 0, 0, 0, xo, 0, RedBuf{nullptr, 0}, ctl, nullptr);
+  });
   NBL_LAUNCH_CHECK();
 }
 
@@ -1124,8 +1491,9 @@ void launch_mg_coarse(const MgLevel& L, bool singular, const T* b, const double*
 template <class T>
 void launch_mg_finish(const MgLevel& L, const T* x, const double* mx, double* z, bool fold, RedBuf rb,
                       const SolveCtl* ctl, hipStream_t s) {
-  const Geo3 g = geo3(L.nx, L.ny, L.nz);
-  k_finish<T><<<g.grid, g.block, 0, s>>>(L, x, mx, z, fold ? 1 : 0, rb, ctl);
+  withCells(L, [&](auto nc, const Geo3& g) {
+    k_finish<T, decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, x, mx, z, fold ? 1 : 0, rb, ctl);
+  });
   NBL_LAUNCH_CHECK();
 }
 
@@ -1156,18 +1524,22 @@ void launch_inv_store(const double* G, int n, const uint16_t* code, long long nf
 template <class T>
 void launch_mg_coarse_direct(const MgLevel& L, const double* M, const float* Mf, const T* b, const double* mb, T* x,
                              const SolveCtl* ctl, hipStream_t s) {
-  const int blocks = int((L.n + kDirectRows - 1) / kDirectRows);
+  // two rows per block: measured fastest of 1, 2, 4, 8 rows per block and one
+  // wavefront per row on the 1536-cell cube-channel level (5.1 us vs 6.7 us
+  // for 4 rows, back-to-back)
+  constexpr int kRows = 2;
+  const int blocks = int((L.n + kRows - 1) / kRows);
   if constexpr (sizeof(T) == sizeof(float))
-    k_coarse_direct<T, float><<<blocks, kBlock, 0, s>>>(L, Mf, b, mb, x, ctl);
+    k_coarse_direct<T, float, kRows><<<blocks, kBlock, 0, s>>>(L, Mf, b, mb, x, ctl);
   else
-    k_coarse_direct<T, double><<<blocks, kBlock, 0, s>>>(L, M, b, mb, x, ctl);
+    k_coarse_direct<T, double, kRows><<<blocks, kBlock, 0, s>>>(L, M, b, mb, x, ctl);
   NBL_LAUNCH_CHECK();
 }
 
 #define NBL_INST(T)                                                                                          \
   template void launch_mg_sweep<T>(const MgLevel&, const T*, const double*, const T*, const double*, bool,    \
                                    const T*, const double*, const MgLevel*, T*, bool, RedBuf, const SolveCtl*, \
-                                   hipStream_t);                                                             \
+                                   hipStream_t, long long*);                                                 \
   template void launch_mg_resid_restrict<T>(const MgLevel&, const MgLevel&, const T*, const double*,         \
                                             const T*, const double*, T*, bool, RedBuf, const SolveCtl*,      \
                                             hipStream_t);                                                    \
@@ -1208,82 +1580,120 @@ void launch_prolong_plain(const MgLevel& L, const MgLevel& Lc, const double* ec,
 
 void launch_fcg_init(const MgLevel& L, bool maskSolid, const double* b, double* rhs, double* x, RedBuf rb,
                      hipStream_t s) {
-  const Geo3 g = geo3(L.nx, L.ny, L.nz);
-  k_fcg_init<<<g.grid, g.block, 0, s>>>(L, maskSolid ? 1 : 0, b, rhs, x, rb);
+  withCells(L, [&](auto nc, const Geo3& g) {
+    k_fcg_init<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, maskSolid ? 1 : 0, b, rhs, x, rb);
+  });
   NBL_LAUNCH_CHECK();
 }
 void launch_fcg_init2(const MgLevel& L, bool singular, double* rhs, double* r, const SolveCtl* ctl, RedBuf rb,
                       hipStream_t s) {
-  const Geo3 g = geo3(L.nx, L.ny, L.nz);
-  k_fcg_init2<<<g.grid, g.block, 0, s>>>(L, singular ? 1 : 0, rhs, r, ctl, rb);
+  withCells(L, [&](auto nc, const Geo3& g) {
+    k_fcg_init2<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, singular ? 1 : 0, rhs, r, ctl, rb);
+  });
   NBL_LAUNCH_CHECK();
 }
 void launch_fcg_start(const MgLevel& L, double* z, const double* mz, const double* r, double* p,
                       const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
-  const Geo3 g = geo3(L.nx, L.ny, L.nz);
-  k_fcg_start<<<g.grid, g.block, 0, s>>>(L, z, mz, r, p, ctl, rb);
+  withCells(L, [&](auto nc, const Geo3& g) {
+    k_fcg_start<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, z, mz, r, p, ctl, rb);
+  });
   NBL_LAUNCH_CHECK();
 }
 void launch_fcg_apply(const MgLevel& L, const double* p, double* Ap, const SolveCtl* ctl, RedBuf rb,
                       hipStream_t s) {
-  const Geo3 g = geo3(L.nx, L.ny, L.nz);
-  k_fcg_apply<<<g.grid, g.block, 0, s>>>(L, p, Ap, ctl, rb);
+  withCells(L, [&](auto nc, const Geo3& g) {
+    k_fcg_apply<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, p, Ap, ctl, rb);
+  });
   NBL_LAUNCH_CHECK();
 }
 void launch_fcg_update(const MgLevel& L, bool singular, double* x, const double* p, const double* r,
                        const double* Ap, double* rn, const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
-  const Geo3 g = geo3(L.nx, L.ny, L.nz);
-  k_fcg_update<<<g.grid, g.block, 0, s>>>(L, singular ? 1 : 0, x, p, r, Ap, rn, ctl, rb);
+  withCells(L, [&](auto nc, const Geo3& g) {
+    k_fcg_update<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, singular ? 1 : 0, x, p, r, Ap, rn, ctl, rb);
+  });
   NBL_LAUNCH_CHECK();
 }
+template <class T>
+void launch_fcg_update_first(const MgLevel& L, double* x, const double* p, const double* r, const double* Ap,
+                             double* rn, T* x1, T* b0, const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
+  withCells(L, [&](auto nc, const Geo3& g) {
+    k_fcg_update_first<T, decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, x, p, r, Ap, rn, x1, b0, ctl, rb);
+  });
+  NBL_LAUNCH_CHECK();
+}
+template void launch_fcg_update_first<double>(const MgLevel&, double*, const double*, const double*, const double*,
+                                              double*, double*, double*, const SolveCtl*, RedBuf, hipStream_t);
+template void launch_fcg_update_first<float>(const MgLevel&, double*, const double*, const double*, const double*,
+                                             double*, float*, float*, const SolveCtl*, RedBuf, hipStream_t);
+void launch_fcg_apply_pu(const MgLevel& L, const double* z, const double* pOld, double* pNew, double* Ap,
+                         const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
+  withCells(L, [&](auto nc, const Geo3& g) {
+    k_fcg_apply_pu<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, z, pOld, pNew, Ap, ctl, rb);
+  });
+  NBL_LAUNCH_CHECK();
+}
+template <class T>
+void launch_mg_sweep_dots(const MgLevel& L, const T* b, const T* x, double* z, const double* rn, const double* r,
+                          RedBuf rb, const SolveCtl* ctl, hipStream_t s) {
+  withCells(L, [&](auto nc, const Geo3& g) {
+    k_sweep_dots<T, decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, b, x, z, rn, r, rb, ctl);
+  });
+  NBL_LAUNCH_CHECK();
+}
+template void launch_mg_sweep_dots<double>(const MgLevel&, const double*, const double*, double*, const double*,
+                                           const double*, RedBuf, const SolveCtl*, hipStream_t);
+template void launch_mg_sweep_dots<float>(const MgLevel&, const float*, const float*, double*, const double*,
+                                          const double*, RedBuf, const SolveCtl*, hipStream_t);
 void launch_fcg_norm(const MgLevel& L, double* rn, const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
-  const Geo3 g = geo3(L.nx, L.ny, L.nz);
-  k_fcg_norm<<<g.grid, g.block, 0, s>>>(L, rn, ctl, rb);
+  withCells(L, [&](auto nc, const Geo3& g) {
+    k_fcg_norm<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, rn, ctl, rb);
+  });
   NBL_LAUNCH_CHECK();
 }
 void launch_fcg_dots(const MgLevel& L, double* z, const double* mz, const double* rn, const double* r,
                      const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
-  const Geo3 g = geo3(L.nx, L.ny, L.nz);
-  k_fcg_dots<<<g.grid, g.block, 0, s>>>(L, z, mz, rn, r, ctl, rb);
+  withCells(L, [&](auto nc, const Geo3& g) {
+    k_fcg_dots<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, z, mz, rn, r, ctl, rb);
+  });
   NBL_LAUNCH_CHECK();
 }
 template <class T>
 void launch_fcg_dots_x(const MgLevel& L, const T* x, double* z, const double* rn, const double* r,
                        const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
-  const Geo3 g = geo3(L.nx, L.ny, L.nz);
-  k_fcg_dots_x<T><<<g.grid, g.block, 0, s>>>(L, x, z, rn, r, ctl, rb);
+  withCells(L, [&](auto nc, const Geo3& g) {
+    k_fcg_dots_x<T, decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, x, z, rn, r, ctl, rb);
+  });
   NBL_LAUNCH_CHECK();
 }
 template void launch_fcg_dots_x<double>(const MgLevel&, const double*, double*, const double*, const double*,
                                         const SolveCtl*, RedBuf, hipStream_t);
 template void launch_fcg_dots_x<float>(const MgLevel&, const float*, double*, const double*, const double*,
                                        const SolveCtl*, RedBuf, hipStream_t);
-void launch_fcg_pupdate(const MgLevel& L, double* p, const double* z, double* r, const double* rn,
-                        const SolveCtl* ctl, hipStream_t s) {
-  k_fcg_pupdate<<<gridFor(L.n), kBlock, 0, s>>>(int(L.n), p, z, r, rn, ctl);
-  NBL_LAUNCH_CHECK();
-}
 void launch_fluid_sum(const MgLevel& L, const double* x, RedBuf rb, hipStream_t s) {
-  const Geo3 g = geo3(L.nx, L.ny, L.nz);
-  k_fluid_sum<<<g.grid, g.block, 0, s>>>(L, x, rb);
+  withCells(L, [&](auto nc, const Geo3& g) {
+    k_fluid_sum<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, x, rb);
+  });
   NBL_LAUNCH_CHECK();
 }
 void launch_fcg_final(const MgLevel& L, bool singular, const double* x, const double* rhs, double* xo,
                       const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
-  const Geo3 g = geo3(L.nx, L.ny, L.nz);
-  k_fcg_final<<<g.grid, g.block, 0, s>>>(L, singular ? 1 : 0, x, rhs, xo, ctl, rb);
+  withCells(L, [&](auto nc, const Geo3& g) {
+    k_fcg_final<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, singular ? 1 : 0, x, rhs, xo, ctl, rb);
+  });
   NBL_LAUNCH_CHECK();
 }
 void launch_cg_z(const MgLevel& L, const double* r, double* z, bool first, double* p, const SolveCtl* ctl,
                  RedBuf rb, hipStream_t s) {
-  const Geo3 g = geo3(L.nx, L.ny, L.nz);
-  k_cg_z<<<g.grid, g.block, 0, s>>>(L, r, z, p, first ? 1 : 0, ctl, rb);
+  withCells(L, [&](auto nc, const Geo3& g) {
+    k_cg_z<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, r, z, p, first ? 1 : 0, ctl, rb);
+  });
   NBL_LAUNCH_CHECK();
 }
 void launch_cg_update(const MgLevel& L, double* x, const double* p, double* r, const double* Ap,
                       const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
-  const Geo3 g = geo3(L.nx, L.ny, L.nz);
-  k_cg_update<<<g.grid, g.block, 0, s>>>(L, x, p, r, Ap, ctl, rb);
+  withCells(L, [&](auto nc, const Geo3& g) {
+    k_cg_update<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, x, p, r, Ap, ctl, rb);
+  });
   NBL_LAUNCH_CHECK();
 }
 

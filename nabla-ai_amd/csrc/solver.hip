@@ -186,8 +186,11 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
     m.fy = H.f[1];
     m.fz = H.f[2];
     m.nf = H.nf;
+    // two cells per thread (vector loads of each pair) where a row splits
+    // into at least 16 pairs
+    m.vec = (m.nx % 2 == 0 && m.nx >= 32) ? 2 : 1;
     dl_.push_back(m);
-    blocks_.push_back(geo3(m.nx, m.ny, m.nz).blocks);
+    blocks_.push_back(levelGeo(m).blocks);
     stride = std::max(stride, blocks_.back());
     rblocks_.push_back(l > 0 ? restrictGeo(dl_[l - 1], m).blocks : 0);
     stride = std::max(stride, rblocks_.back());
@@ -209,6 +212,7 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
   rn_ = DevBuf(ctx, n0 * 8);
   z_ = DevBuf(ctx, n0 * 8);
   p_ = DevBuf(ctx, n0 * 8);
+  p2_ = DevBuf(ctx, n0 * 8);
   Ap_ = DevBuf(ctx, n0 * 8);
   ctl_ = DevBuf(ctx, sizeof(SolveCtl));
   redStride_ = stride;
@@ -292,7 +296,6 @@ bool PoissonDev::buildDirectCoarse() {
 }
 
 PoissonDev::~PoissonDev() {
-  for (hipEvent_t e : evPool_) (void)hipEventDestroy(e);
   if (hctl_) (void)hipHostFree(hctl_);
   if (e0_) (void)hipEventDestroy(e0_);
   if (e1_) (void)hipEventDestroy(e1_);
@@ -313,38 +316,58 @@ void PoissonDev::fin(int op, int K, int nparts, int slot, long long nf, bool pre
 }
 
 // In-situ timing of the finest level's plain sweeps (the roofline kernel):
-// an event pair around each such launch in the first V-cycle of a solve (the
-// one that always runs; sampling one cycle per solve keeps the event markers
-// from slowing the solve itself).  After the solve the recorded pairs are
-// summed.
-void PoissonDev::timedSweepBegin(int l, bool plain, hipStream_t s) {
-  curTimed_ = timeSweeps_ && l == 0 && plain && vcount_ == 0;
-  if (!curTimed_) return;
-  if (evUsed_ + 2 > evPool_.size()) {
-    for (int q = 0; q < 64; ++q) {
-      hipEvent_t e;
-      NBL_HIP(hipEventCreate(&e));
-      evPool_.push_back(e);
+// each timed launch gets a slice of the stamp buffer in which every block
+// records the device wall clock when it starts and when all its waves are
+// done (no events or extra packets in the stream, and no fence: the solve
+// runs as untimed).  A launch lasts from its first block start to its last block end;
+// launches a converged solve skips on the device leave their slice at zero.
+long long* PoissonDev::timedSweep(int l, bool plain) {
+  if (!timeSweeps_ || l != 0 || !plain || stampUsed_ >= stampCap_) return nullptr;
+  return stamps_.as<long long>() + size_t(stampUsed_++) * 2 * size_t(blocks_[0]);
+}
+
+void PoissonDev::setSweepTiming(bool on) {
+  timeSweeps_ = on;
+  if (!on) return;
+  sweepMs_ = 0.0;
+  sweepN_ = 0;
+  stampUsed_ = 0;
+  if (!stamps_.bytes()) {
+    stampCap_ = kStampLaunches;
+    stamps_ = DevBuf(ctx_, size_t(stampCap_) * 2 * size_t(blocks_[0]) * sizeof(long long));
+    int dev = 0, khz = 0;
+    NBL_HIP(hipGetDevice(&dev));
+    NBL_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    wallKHz_ = khz > 0 ? khz : 100000;
+  }
+  NBL_HIP(hipMemset(stamps_.as<void>(), 0, stamps_.bytes()));
+}
+
+void PoissonDev::sweepTiming(double* ms, long long* n, double* bytes) {
+  if (stampUsed_ > 0) {
+    NBL_HIP(hipDeviceSynchronize());
+    const size_t nb = size_t(blocks_[0]);
+    std::vector<long long> h(size_t(stampUsed_) * 2 * nb);
+    NBL_HIP(hipMemcpy(h.data(), stamps_.as<void>(), h.size() * sizeof(long long), hipMemcpyDeviceToHost));
+    for (int q = 0; q < stampUsed_; ++q) {
+      const long long* t = h.data() + size_t(q) * 2 * nb;
+      long long t0 = t[0], t1 = t[1];
+      bool ran = true;
+      for (size_t b = 0; b < nb && ran; ++b) {
+        if (t[2 * b] == 0 || t[2 * b + 1] == 0) ran = false;
+        t0 = std::min(t0, t[2 * b]);
+        t1 = std::max(t1, t[2 * b + 1]);
+      }
+      if (!ran) continue;
+      sweepMs_ += double(t1 - t0) / double(wallKHz_);
+      ++sweepN_;
     }
+    NBL_HIP(hipMemset(stamps_.as<void>(), 0, size_t(stampUsed_) * 2 * nb * sizeof(long long)));
+    stampUsed_ = 0;
   }
-  NBL_HIP(hipEventRecord(evPool_[evUsed_], s));
-}
-void PoissonDev::timedSweepEnd(hipStream_t s) {
-  if (!curTimed_) return;
-  NBL_HIP(hipEventRecord(evPool_[evUsed_ + 1], s));
-  evCycle_.push_back(vcount_);
-  evUsed_ += 2;
-}
-void PoissonDev::collectSweepTimes(int cyclesRun) {
-  for (size_t q = 0; q + 1 < evUsed_ + 1 && q / 2 < evCycle_.size(); q += 2) {
-    if (evCycle_[q / 2] >= cyclesRun) continue;
-    float ms = 0.f;
-    NBL_HIP(hipEventElapsedTime(&ms, evPool_[q], evPool_[q + 1]));
-    sweepMs_ += ms;
-    ++sweepN_;
-  }
-  evUsed_ = 0;
-  evCycle_.clear();
+  if (ms) *ms = sweepN_ ? sweepMs_ / double(sweepN_) : 0.0;
+  if (n) *n = sweepN_;
+  if (bytes) *bytes = double(dl_[0].n) * (3.0 * (mixed_ ? 4.0 : 8.0) + 2.0);
 }
 
 // One V(pre, post) cycle from level l (multigrid.cpp:185-198).  Each sweep and
@@ -371,19 +394,23 @@ void PoissonDev::vcycle(int l, const T* b, const double* mb, const T** xout, con
                    project_ ? slotPtr(ctl, slotX(l, 1)) : nullptr};
   int cur = -1;
   int q0 = 0;
-  // sweeps 1 and 2 in one pass where launches, not arithmetic, bound the level
-  // (the pass re-evaluates the pointwise first sweep at every stencil point)
-  if (!project_ && opts_.pre_smooth >= 2 && L.n < (1LL << 18)) {
+  const bool fusedL0 = l == 0 && l0Fused_;
+  if (fusedL0) {
+    // the first sweep from zero was written into bufs[0] with the residual
+    cur = 0;
+    q0 = 1;
+  } else if (!project_ && opts_.pre_smooth >= 2 && L.n < (1LL << 18)) {
+    // sweeps 1 and 2 in one pass where launches, not arithmetic, bound the
+    // level (the pass re-evaluates the pointwise first sweep at every stencil
+    // point)
     launch_mg_sweep_pair<T>(L, b, bufs[0], ctl, s);
     cur = 0;
     q0 = 2;
   }
   for (int q = q0; q < opts_.pre_smooth; ++q) {
     const int o = cur == 0 ? 1 : 0;
-    timedSweepBegin(l, cur >= 0, s);
     launch_mg_sweep<T>(L, b, mb, cur < 0 ? nullptr : bufs[cur], cur < 0 ? nullptr : ms[cur], cur < 0, nullptr,
-                       nullptr, nullptr, bufs[o], project_, rb, ctl, s);
-    timedSweepEnd(s);
+                       nullptr, nullptr, bufs[o], project_, rb, ctl, s, timedSweep(l, cur >= 0));
     if (project_) fin(FIN_MEAN, 1, blocks_[l], slotX(l, o), L.nf, true, s);
     cur = o;
   }
@@ -397,10 +424,15 @@ void PoissonDev::vcycle(int l, const T* b, const double* mb, const T** xout, con
   vcycle<T>(l + 1, bc, mbc, &ec, &mec, s);
   for (int q = 0; q < opts_.post_smooth; ++q) {
     const int o = 1 - cur;
-    timedSweepBegin(l, q > 0, s);
+    if (fusedL0 && q == opts_.post_smooth - 1 && q > 0) {
+      // last sweep: writes z and the flexible-CG dots instead of x
+      launch_mg_sweep_dots<T>(L, b, bufs[cur], z_.as<double>(), l0Rn_, l0R_, rb, ctl, s);
+      *xout = nullptr;
+      *mxout = nullptr;
+      return;
+    }
     launch_mg_sweep<T>(L, b, mb, bufs[cur], ms[cur], false, q == 0 ? ec : nullptr, q == 0 ? mec : nullptr, &Lc,
-                       bufs[o], project_, rb, ctl, s);
-    timedSweepEnd(s);
+                       bufs[o], project_, rb, ctl, s, timedSweep(l, q > 0));
     if (project_) fin(FIN_MEAN, 1, blocks_[l], slotX(l, o), L.nf, true, s);
     cur = o;
   }
@@ -412,66 +444,84 @@ void PoissonDev::vcycle(int l, const T* b, const double* mb, const T** xout, con
 // (multigrid.cpp:283-289).  z's mean is left pending in kSlotMeanZ.  With
 // fuseDots (systems without projections) the V-cycle output is left for
 // k_fcg_dots_x, which forms z itself.
-void PoissonDev::minv(const double* r, hipStream_t s, bool fuseDots) {
+void PoissonDev::minv(const double* r, hipStream_t s, const double* rOld, bool fused) {
   SolveCtl* ctl = ctl_.as<SolveCtl>();
   const MgLevel& L = dl_[0];
   const RedBuf rb = red();
   double* z = z_.as<double>();
+  const bool dots = rOld != nullptr;
+  l0Fused_ = fused;
+  l0Rn_ = r;
+  l0R_ = rOld;
   if (mixed_) {
-    launch_to_level0<float>(L, r, b0f_.as<float>(), ctl, s);
+    if (!fused) launch_to_level0<float>(L, r, b0f_.as<float>(), ctl, s);
     const float* x = nullptr;
     const double* mx = nullptr;
     vcycle<float>(0, b0f_.as<float>(), nullptr, &x, &mx, s);
-    if (fuseDots) launch_fcg_dots_x<float>(L, x, z, rn_.as<double>(), r_.as<double>(), ctl, rb, s);
-    else launch_mg_finish<float>(L, x, mx, z, project_, rb, ctl, s);
+    if (x && dots) launch_fcg_dots_x<float>(L, x, z, r, rOld, ctl, rb, s);
+    else if (x) launch_mg_finish<float>(L, x, mx, z, project_, rb, ctl, s);
   } else {
     const double* x = nullptr;
     const double* mx = nullptr;
     vcycle<double>(0, r, nullptr, &x, &mx, s);
-    if (fuseDots) launch_fcg_dots_x<double>(L, x, z, rn_.as<double>(), r_.as<double>(), ctl, rb, s);
-    else launch_mg_finish<double>(L, x, mx, z, project_, rb, ctl, s);
+    if (x && dots) launch_fcg_dots_x<double>(L, x, z, r, rOld, ctl, rb, s);
+    else if (x) launch_mg_finish<double>(L, x, mx, z, project_, rb, ctl, s);
   }
+  l0Fused_ = false;
   if (project_) fin(FIN_MEAN, 1, blocks_[0], kSlotMeanZ, L.nf, true, s);
   ++vcount_;
 }
 
 // One iteration of the flexible CG loop (multigrid.cpp:309-340) or of the
 // Jacobi CG loop (poisson.cpp:197-225); every launch is a no-op once the
-// device has set ctl->done.
-void PoissonDev::enqueueIteration(hipStream_t s) {
+// device has set ctl->done.  Iteration k >= 1 forms p = z + beta p of the
+// previous iteration inside its A p, so p alternates between two buffers,
+// and the flexible CG's r / r_new alternate instead of being copied.
+void PoissonDev::enqueueIteration(hipStream_t s, int k) {
   SolveCtl* ctl = ctl_.as<SolveCtl>();
   const MgLevel& L = dl_[0];
   const RedBuf rb = red();
   const int nb = blocks_[0];
   double* x = x_.as<double>();
-  double* r = r_.as<double>();
   double* z = z_.as<double>();
-  double* p = p_.as<double>();
   double* Ap = Ap_.as<double>();
-  launch_fcg_apply(L, p, Ap, ctl, rb, s);
+  double* p = k == 0 ? p_.as<double>() : ((k & 1) ? p2_.as<double>() : p_.as<double>());
+  if (k == 0) {
+    launch_fcg_apply(L, p, Ap, ctl, rb, s);
+  } else {
+    const double* pOld = (k & 1) ? p_.as<double>() : p2_.as<double>();
+    launch_fcg_apply_pu(L, z, pOld, p, Ap, ctl, rb, s);
+  }
   fin(FIN_ALPHA, 1, nb, 0, 0, true, s);
   if (kind_ == 0) {
-    double* rn = rn_.as<double>();
-    launch_fcg_update(L, project_, x, p, r, Ap, rn, ctl, rb, s);
+    double* r = (k & 1) ? rn_.as<double>() : r_.as<double>();
+    double* rn = (k & 1) ? r_.as<double>() : rn_.as<double>();
     if (project_) {
+      launch_fcg_update(L, true, x, p, r, Ap, rn, ctl, rb, s);
       fin(FIN_MEAN, 1, nb, kSlotMeanRn, L.nf, true, s);
       launch_fcg_norm(L, rn, ctl, rb, s);
-    }
-    fin(FIN_CONV, 1, nb, 0, 0, true, s);
-    if (project_) {
-      minv(rn, s, false);
+      fin(FIN_CONV, 1, nb, 0, 0, true, s);
+      minv(rn, s, nullptr, false);
       launch_fcg_dots(L, z, slotPtr(ctl, kSlotMeanZ), rn, r, ctl, rb, s);
     } else {
-      minv(rn, s, true);
+      // fusions need a first pre-sweep and a coarser level to smooth above
+      const bool fused = nlevels() > 1 && opts_.pre_smooth >= 1;
+      if (fused && mixed_)
+        launch_fcg_update_first<float>(L, x, p, r, Ap, rn, lx0_[0].as<float>(), b0f_.as<float>(), ctl, rb, s);
+      else if (fused)
+        launch_fcg_update_first<double>(L, x, p, r, Ap, rn, lx0_[0].as<double>(), nullptr, ctl, rb, s);
+      else
+        launch_fcg_update(L, false, x, p, r, Ap, rn, ctl, rb, s);
+      fin(FIN_CONV, 1, nb, 0, 0, true, s);
+      minv(rn, s, r, fused);
     }
     fin(FIN_BETA_FLEX, 2, nb, 0, 0, true, s);
-    launch_fcg_pupdate(L, p, z, r, rn, ctl, s);
   } else {
+    double* r = r_.as<double>();
     launch_cg_update(L, x, p, r, Ap, ctl, rb, s);
     fin(FIN_CONV, 1, nb, 0, 0, true, s);
-    launch_cg_z(L, r, z, false, p, ctl, rb, s);
+    launch_cg_z(L, r, z, false, nullptr, ctl, rb, s);
     fin(FIN_BETA, 1, nb, 0, 0, true, s);
-    launch_fcg_pupdate(L, p, z, nullptr, nullptr, ctl, s);
   }
 }
 
@@ -482,7 +532,7 @@ void PoissonDev::precondition(const double* d_r, double* d_z, hipStream_t s) {
   SolveCtl* ctl = ctl_.as<SolveCtl>();
   const MgLevel& L = dl_[0];
   launch_ctl_init(ctl, tol_, s);
-  minv(d_r, s, false);
+  minv(d_r, s, nullptr, false);
   // z - pending mean (projection) into z, p; the partials it writes are unused
   launch_fcg_start(L, z_.as<double>(), project_ ? slotPtr(ctl, kSlotMeanZ) : nullptr, d_r, p_.as<double>(), ctl,
                    red(), s);
@@ -498,6 +548,9 @@ void PoissonDev::readCtl(hipStream_t s) {
 // (poisson.cpp:136-252).  The host enqueues the iteration count the previous
 // solve needed, then polls the device flag every check_every iterations.
 nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t s) {
+  // the cell kernels read and write pairs of cells as 16-byte vectors
+  if ((reinterpret_cast<uintptr_t>(d_b) | reinterpret_cast<uintptr_t>(d_x)) % 16)
+    throw std::invalid_argument("poisson solve: device vectors must be 16-byte aligned");
   SolveCtl* ctl = ctl_.as<SolveCtl>();
   const MgLevel& L = dl_[0];
   const RedBuf rb = red();
@@ -517,7 +570,7 @@ nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t
   launch_fcg_init2(L, project_, rhs, r, ctl, rb, s);
   fin(FIN_BNORM, 1, nb, 0, 0, false, s, mg ? 1 : 0);
   if (mg) {
-    minv(r, s, false);
+    minv(r, s, nullptr, false);
     launch_fcg_start(L, z, project_ ? slotPtr(ctl, kSlotMeanZ) : nullptr, r, p, ctl, rb, s);
   } else {
     launch_cg_z(L, r, z, true, p, ctl, rb, s);
@@ -527,7 +580,7 @@ nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t
   int batch = std::clamp(predicted_, 1, std::max(1, maxIt_));
   const int step = std::max(1, opts_.check_every);
   while (enq < maxIt_) {
-    for (int q = 0; q < batch && enq < maxIt_; ++q, ++enq) enqueueIteration(s);
+    for (int q = 0; q < batch && enq < maxIt_; ++q, ++enq) enqueueIteration(s, enq);
     readCtl(s);
     if (hctl_[0].done) break;
     batch = step;
@@ -545,9 +598,6 @@ nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t
   lastMs_ = ms;
   lastV_ = vcount_;
   const SolveCtl& h = hctl_[0];
-  // V-cycles that ran: the initial one and one per iteration, except after
-  // the converging update (multigrid.cpp:319-325); with an early exit none
-  if (timeSweeps_) collectSweepTimes(h.early ? 0 : (h.converged ? h.it : h.it + 1));
   if (h.converged && h.it > 0) predicted_ = h.it;
   return nbl_poisson_report{h.it, h.resid, h.converged};
 }

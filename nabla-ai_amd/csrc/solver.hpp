@@ -97,34 +97,27 @@ class PoissonDev {
   RedBuf red() const { return RedBuf{red_.as<double>(), redStride_}; }
   int blocks(int l) const { return blocks_[l]; }
   bool projects() const { return project_; }
-  // In-situ timing of the finest level's plain sweeps inside solves.
-  void setSweepTiming(bool on) {
-    timeSweeps_ = on;
-    if (on) {
-      sweepMs_ = 0.0;
-      sweepN_ = 0;
-    }
-  }
-  void sweepTiming(double* ms, long long* n, double* bytes) const {
-    if (ms) *ms = sweepN_ ? sweepMs_ / double(sweepN_) : 0.0;
-    if (n) *n = sweepN_;
-    if (bytes) *bytes = double(dl_[0].n) * (3.0 * (mixed_ ? 4.0 : 8.0) + 2.0);
-  }
+  // In-situ timing of the finest level's plain sweeps inside solves (device
+  // wall-clock stamps per block; on: resets the totals).  sweepTiming folds
+  // the stamps recorded so far (it synchronises the device).
+  void setSweepTiming(bool on);
+  void sweepTiming(double* ms, long long* n, double* bytes);
   // Average ms per launch and algorithmic bytes of one solve kernel.
   void profile(int kernel, int reps, hipStream_t s, double* ms, double* bytes);
 
  private:
   template <class T>
   void vcycle(int l, const T* b, const double* mb, const T** xout, const double** mxout, hipStream_t s);
-  void minv(const double* r, hipStream_t s, bool fuseDots);
+  // z = Minv r.  rOld: fold the flexible-CG dots z.r, z.rOld into the output
+  // pass.  fused: the level-0 first sweep (and FP32 right-hand side) were
+  // written by the update, and the last level-0 sweep writes z and the dots.
+  void minv(const double* r, hipStream_t s, const double* rOld, bool fused);
   bool buildDirectCoarse();
   void fin(int op, int K, int nparts, int slot, long long nf, bool predicated, hipStream_t s, int earlyExit = 0,
            double convTol = 0.0);
-  void enqueueIteration(hipStream_t s);
+  void enqueueIteration(hipStream_t s, int k);
   void readCtl(hipStream_t s);
-  void timedSweepBegin(int l, bool plain, hipStream_t s);
-  void timedSweepEnd(hipStream_t s);
-  void collectSweepTimes(int cyclesRun);
+  long long* timedSweep(int l, bool plain);
 
   nbl_ctx* ctx_;
   int kind_;
@@ -147,7 +140,11 @@ class PoissonDev {
   DevBuf coarseM_, coarseMf_;  // coarsest-level inverse (FP64, FP32)
   DevBuf b0f_;  // FP32 level-0 right-hand side (mixed)
   // flexible-CG vectors (FP64)
-  DevBuf x_, rhs_, r_, rn_, z_, p_, Ap_;
+  DevBuf x_, rhs_, r_, rn_, z_, p_, p2_, Ap_;
+  // level-0 fusions of the current V-cycle (set by minv)
+  bool l0Fused_ = false;
+  const double* l0Rn_ = nullptr;
+  const double* l0R_ = nullptr;
   DevBuf ctl_, red_;
   int redStride_ = 1;
   SolveCtl* hctl_ = nullptr;  // pinned mirror
@@ -156,10 +153,10 @@ class PoissonDev {
   int lastV_ = 0;
   int predicted_ = 1;
   int vcount_ = 0;
-  bool timeSweeps_ = false, curTimed_ = false;
-  std::vector<hipEvent_t> evPool_;
-  std::vector<int> evCycle_;  // V-cycle index of each recorded pair
-  size_t evUsed_ = 0;
+  static constexpr int kStampLaunches = 2048;  // timed launches held before a fold
+  bool timeSweeps_ = false;
+  DevBuf stamps_;  // per timed launch and block: start, end (device wall clock)
+  int stampCap_ = 0, stampUsed_ = 0, wallKHz_ = 100000;
   double sweepMs_ = 0.0;
   long long sweepN_ = 0;
 };
