@@ -464,6 +464,7 @@ def main():
         "served_p50_ms": None, "served_p99_ms": None,
     }
     if a.profile_only:
+        out["roofline"] = roofline_insitu(sweep, g)
         print(json.dumps(out), flush=True)
         return
     if not a.no_mixed:

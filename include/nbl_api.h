@@ -108,6 +108,10 @@ typedef struct nbl_solver_opts {
   int check_every;          /* host polls the device convergence flag after
                                this many flexible-CG iterations (>=1)       */
   int coarse;               /* NBL_COARSE_* (default DIRECT)                */
+  int cells_per_thread;     /* consecutive cells along x per thread in the
+                               level kernels: 0 automatic, 1, 2 or 4 (a level
+                               whose nx it does not divide uses 1); results
+                               agree to rounding across choices            */
 } nbl_solver_opts;
 
 /* StepReport (ns_solver.hpp:34-49). */

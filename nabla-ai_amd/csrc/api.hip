@@ -81,6 +81,7 @@ void nbl_solver_opts_default(nbl_solver_opts* o) {
   o->post_smooth = 2;
   o->check_every = 1;
   o->coarse = NBL_COARSE_DIRECT;
+  o->cells_per_thread = 0;
 }
 
 const char* nbl_last_error(void) { return g_err.c_str(); }

@@ -35,7 +35,7 @@ struct MgLevel {
   const uint16_t* code;
   int fx, fy, fz;         // factor from the next-finer level (coarse levels)
   long long nf;           // fluid cells
-  int vec;                // cells per thread along i in the cell kernels (1, or 2 when nx is even)
+  int vec;                // cells per thread along i in the cell kernels (1, 2 or 4; divides nx)
 };
 
 constexpr uint16_t kFluidBit = 1u << 12;

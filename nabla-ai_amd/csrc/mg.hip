@@ -26,7 +26,13 @@ namespace nbl {
 
 Geo3 geo3n(int nx, int ny, int nz, int cells) {
   const int tx = nx / cells;
-  const int bx = tx >= 64 ? 64 : tx >= 32 ? 32 : tx >= 16 ? 16 : 8;
+  // widest row of threads (64..16) that divides the row exactly, else the widest
+  int bx = tx >= 64 ? 64 : tx >= 32 ? 32 : tx >= 16 ? 16 : 8;
+  for (int w : {64, 32, 16})
+    if (w <= tx && tx % w == 0) {
+      bx = w;
+      break;
+    }
   const int by = kBlock / bx;
   Geo3 g;
   g.block = dim3(bx, by, 1);
@@ -56,9 +62,10 @@ __device__ __forceinline__ Cell cellOf(int nx, int ny, int nz) {
   return q;
 }
 
-// NC consecutive cells along i per thread (NC = 1 or 2; nx % NC == 0).  With
-// NC = 2 every vector access of a cell pair is one 16-byte (FP64) or 8-byte
-// (FP32) load or store and the two face codes are one 32-bit word.
+// NC consecutive cells along i per thread (NC = 1, 2 or 4; nx % NC == 0).
+// With NC >= 2 every vector access of a cell pair is one 16-byte (FP64) or
+// 8-byte (FP32) load or store and the face codes of the group are one 32- or
+// 64-bit word.
 template <int NC>
 struct CellN {
   int i, j, k, c, blk;  // i: first cell of the group
@@ -82,7 +89,8 @@ __device__ __forceinline__ CellN<NC> cellOfN(int nx, int ny, int nz) {
 // cells per thread.
 template <class F>
 void withCells(const MgLevel& L, F&& f) {
-  if (L.vec == 2) f(std::integral_constant<int, 2>{}, geo3n(L.nx, L.ny, L.nz, 2));
+  if (L.vec == 4) f(std::integral_constant<int, 4>{}, geo3n(L.nx, L.ny, L.nz, 4));
+  else if (L.vec == 2) f(std::integral_constant<int, 2>{}, geo3n(L.nx, L.ny, L.nz, 2));
   else f(std::integral_constant<int, 1>{}, geo3n(L.nx, L.ny, L.nz, 1));
 }
 
@@ -99,33 +107,47 @@ struct Vec2Of<float> {
 
 template <int NC, class T>
 __device__ __forceinline__ void ldN(const T* __restrict__ p, int c, T (&v)[NC]) {
-  if constexpr (NC == 2) {
-    const typename Vec2Of<T>::type w = *reinterpret_cast<const typename Vec2Of<T>::type*>(p + c);
-    v[0] = w.x;
-    v[1] = w.y;
-  } else {
+  if constexpr (NC == 1) {
     v[0] = p[c];
+  } else {
+    using V2 = typename Vec2Of<T>::type;
+#pragma unroll
+    for (int h = 0; h < NC / 2; ++h) {
+      const V2 w = *reinterpret_cast<const V2*>(p + c + 2 * h);
+      v[2 * h] = w.x;
+      v[2 * h + 1] = w.y;
+    }
   }
 }
 template <int NC, class T>
 __device__ __forceinline__ void stN(T* __restrict__ p, int c, const T (&v)[NC]) {
-  if constexpr (NC == 2) {
-    typename Vec2Of<T>::type w;
-    w.x = v[0];
-    w.y = v[1];
-    *reinterpret_cast<typename Vec2Of<T>::type*>(p + c) = w;
-  } else {
+  if constexpr (NC == 1) {
     p[c] = v[0];
+  } else {
+    using V2 = typename Vec2Of<T>::type;
+#pragma unroll
+    for (int h = 0; h < NC / 2; ++h) {
+      V2 w;
+      w.x = v[2 * h];
+      w.y = v[2 * h + 1];
+      *reinterpret_cast<V2*>(p + c + 2 * h) = w;
+    }
   }
 }
 template <int NC>
 __device__ __forceinline__ void ldCodes(const uint16_t* __restrict__ code, int c, uint16_t (&v)[NC]) {
-  if constexpr (NC == 2) {
+  if constexpr (NC == 1) {
+    v[0] = code[c];
+  } else if constexpr (NC == 2) {
     const uint32_t w = *reinterpret_cast<const uint32_t*>(code + c);
     v[0] = uint16_t(w & 0xffffu);
     v[1] = uint16_t(w >> 16);
   } else {
-    v[0] = code[c];
+    const uint2 w = *reinterpret_cast<const uint2*>(code + c);
+    v[0] = uint16_t(w.x & 0xffffu);
+    v[1] = uint16_t(w.x >> 16);
+    v[2] = uint16_t(w.y & 0xffffu);
+    v[3] = uint16_t(w.y >> 16);
   }
 }
 
@@ -357,44 +379,65 @@ __global__ void __launch_bounds__(kBlock) k_sweep(MgLevel L, const T* __restrict
 }
 
 // Residual r = b - A x on fine fluid cells restricted to the coarse grid as
-// the mean over fluid children (multigrid.cpp:116-154).  One thread per fine
-// cell; a block covers whole coarse cells (blockDim.z = fz, even x/y extents),
-// stages its residuals in LDS, and the thread at each coarse cell's first
-// child sums the children in the reference's k, j, i order.  ZEROX: x == 0
-// (restriction of b itself).
-template <class T, bool ZEROX>
+// the mean over fluid children (multigrid.cpp:116-154).  NC fine cells per
+// thread along i (NC = 2 only with an x coarsening factor of 2: a thread then
+// holds both x-children of one coarse cell); a block covers whole coarse
+// cells (blockDim.z = fz, even x/y extents), stages its residuals in LDS, and
+// one thread per coarse cell sums the children in the reference's k, j, i
+// order.  ZEROX: x == 0 (restriction of b itself).
+template <class T, bool ZEROX, int NC>
 __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc, const T* __restrict__ b,
                                                            const double* mb, const T* __restrict__ x,
                                                            const double* mx, T* __restrict__ rc, int fold,
                                                            RedBuf rb, const SolveCtl* ctl) {
   if (ctl && ldA(&ctl->done)) return;
-  __shared__ T rs[kBlock];
-  __shared__ unsigned char fls[kBlock];
+  __shared__ T rs[kBlock * NC];
+  __shared__ unsigned char fls[kBlock * NC];
   const XView<T> X{x, mx ? T(ldA(mx)) : T(0), mx != nullptr, nullptr, 1, 1, 0, 0, 0, T(0), false};
   const Blk3 B = blk3();
   const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
   const int bx = blockDim.x, by = blockDim.y;
-  const int i = B.x * bx + tx, j = B.y * by + ty, k = B.z * int(blockDim.z) + tz;
+  const int i = NC * (B.x * bx + tx), j = B.y * by + ty, k = B.z * int(blockDim.z) + tz;
   const int tid = tx + bx * (ty + by * tz);
-  T r = T(0);
-  bool fl = false;
+  T r[NC];
+  bool fl[NC];
+#pragma unroll
+  for (int e = 0; e < NC; ++e) {
+    r[e] = T(0);
+    fl[e] = false;
+  }
   if (i < L.nx && j < L.ny && k < L.nz) {
     const int c = i + L.nx * (j + L.ny * k);
-    const uint16_t code = L.code[c];
-    if (code & kFluidBit) {
-      fl = true;
-      T dg;
-      const T Ax = applyRow<T, ZEROX>(L, code, c, i, j, k, X, dg);
-      T bv = b[c];
-      if (mb) bv = bv - T(ldA(mb));
-      r = bv - Ax;
+    uint16_t code[NC];
+    T xc[NC], xn[NC][6], bv[NC];
+    ldCodes<NC>(L.code, c, code);
+    if constexpr (!ZEROX) gatherN<T, NC>(L, X, c, i, j, k, xc, xn);
+    ldN<NC>(b, c, bv);
+#pragma unroll
+    for (int e = 0; e < NC; ++e) {
+      if (!(code[e] & kFluidBit)) continue;
+      fl[e] = true;
+      T dg, Ax;
+      if constexpr (ZEROX) {
+        const T zn[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+        Ax = applyDecode<T, true>(L, code[e], T(0), zn, dg);
+      } else {
+        Ax = applyDecode<T, false>(L, code[e], xc[e], xn[e], dg);
+      }
+      T bb = bv[e];
+      if (mb) bb = bb - T(ldA(mb));
+      r[e] = bb - Ax;
     }
   }
-  rs[tid] = r;
-  fls[tid] = fl ? 1 : 0;
+#pragma unroll
+  for (int e = 0; e < NC; ++e) {
+    rs[NC * tid + e] = r[e];
+    fls[NC * tid + e] = fl[e] ? 1 : 0;
+  }
   __syncthreads();
   double acc[1] = {0.0};
-  if (tz == 0 && tx % Lc.fx == 0 && ty % Lc.fy == 0) {
+  const int fxT = NC == 2 ? 1 : Lc.fx;  // x-children held by one thread: 2 (NC = 2) or 1
+  if (tz == 0 && tx % fxT == 0 && ty % Lc.fy == 0) {
     const int I = i / Lc.fx, J = j / Lc.fy, K = k / Lc.fz;
     if (I < Lc.nx && J < Lc.ny && K < Lc.nz) {
       const int C = I + Lc.nx * (J + Lc.ny * K);
@@ -406,7 +449,8 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc
         for (int dk = 0; dk < Lc.fz; ++dk)
           for (int dj = 0; dj < Lc.fy; ++dj)
             for (int di = 0; di < Lc.fx; ++di) {
-              const int t = (tx + di) + bx * ((ty + dj) + by * dk);
+              const int t = NC == 2 ? 2 * (tx + bx * ((ty + dj) + by * dk)) + di
+                                    : (tx + di) + bx * ((ty + dj) + by * dk);
               if (fls[t]) {
                 s += rs[t];
                 ++cnt;
@@ -1459,13 +1503,24 @@ void launch_mg_sweep_pair(const MgLevel& L, const T* b, T* xo, const SolveCtl* c
   NBL_LAUNCH_CHECK();
 }
 
+// Fine cells per thread of the restriction from L to Lc.
+static int restrictCells(const MgLevel& L, const MgLevel& Lc) { return (L.vec >= 2 && Lc.fx == 2) ? 2 : 1; }
+
 Geo3 restrictGeo(const MgLevel& L, const MgLevel& Lc) {
+  const int nc = restrictCells(L, Lc);
+  const int tx = L.nx / nc;
   const int bz = Lc.fz;
-  const int bx = L.nx >= 64 ? 64 : L.nx >= 32 ? 32 : 16;
+  // widest row of threads that divides the row exactly, else the widest
+  int bx = tx >= 64 ? 64 : tx >= 32 ? 32 : 16;
+  for (int w : {64, 32, 16})
+    if (w <= tx && tx % w == 0) {
+      bx = w;
+      break;
+    }
   const int by = kBlock / (bx * bz);
   Geo3 g;
   g.block = dim3(bx, by, bz);
-  g.grid = dim3((L.nx + bx - 1) / bx, (L.ny + by - 1) / by, (L.nz + bz - 1) / bz);
+  g.grid = dim3((tx + bx - 1) / bx, (L.ny + by - 1) / by, (L.nz + bz - 1) / bz);
   g.blocks = int(g.grid.x * g.grid.y * g.grid.z);
   return g;
 }
@@ -1474,7 +1529,10 @@ template <class T>
 void launch_mg_resid_restrict(const MgLevel& L, const MgLevel& Lc, const T* b, const double* mb, const T* x,
                               const double* mx, T* rc, bool fold, RedBuf rb, const SolveCtl* ctl, hipStream_t s) {
   const Geo3 g = restrictGeo(L, Lc);
-  k_resid_restrict<T, false><<<g.grid, g.block, 0, s>>>(L, Lc, b, mb, x, mx, rc, fold ? 1 : 0, rb, ctl);
+  if (restrictCells(L, Lc) == 2)
+    k_resid_restrict<T, false, 2><<<g.grid, g.block, 0, s>>>(L, Lc, b, mb, x, mx, rc, fold ? 1 : 0, rb, ctl);
+  else
+    k_resid_restrict<T, false, 1><<<g.grid, g.block, 0, s>>>(L, Lc, b, mb, x, mx, rc, fold ? 1 : 0, rb, ctl);
   NBL_LAUNCH_CHECK();
 }
 
@@ -1568,8 +1626,12 @@ void launch_diag(const MgLevel& L, double* out, hipStream_t s) {
 void launch_restrict_plain(const MgLevel& L, const MgLevel& Lc, const double* rf, double* rc, hipStream_t s) {
   // rf - A*0 == rf exactly, so the fused kernel with a zero x restricts rf itself
   const Geo3 g = restrictGeo(L, Lc);
-  k_resid_restrict<double, true><<<g.grid, g.block, 0, s>>>(L, Lc, rf, nullptr, nullptr, nullptr, rc, 0,
-                                                            RedBuf{nullptr, 0}, nullptr);
+  if (restrictCells(L, Lc) == 2)
+    k_resid_restrict<double, true, 2><<<g.grid, g.block, 0, s>>>(L, Lc, rf, nullptr, nullptr, nullptr, rc, 0,
+                                                                 RedBuf{nullptr, 0}, nullptr);
+  else
+    k_resid_restrict<double, true, 1><<<g.grid, g.block, 0, s>>>(L, Lc, rf, nullptr, nullptr, nullptr, rc, 0,
+                                                                 RedBuf{nullptr, 0}, nullptr);
   NBL_LAUNCH_CHECK();
 }
 void launch_prolong_plain(const MgLevel& L, const MgLevel& Lc, const double* ec, double* xf, hipStream_t s) {
