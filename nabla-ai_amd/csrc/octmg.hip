@@ -33,19 +33,6 @@ constexpr int kFallbackSweeps = 16;    // coarsest level when its inverse cannot
 
 #define OGRID(c, n) for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < (n); c += (long long)gridDim.x * blockDim.x)
 
-struct OctView {
-  long long n;
-  const uint8_t* mask;
-  const int32_t* rowPtr;
-  const int32_t* col;
-  const double* coef;
-  const double* diag;
-};
-OctView view(const OctLevel& L) {
-  return OctView{(long long)L.n, L.mask.as<uint8_t>(), L.rowPtr.as<int32_t>(), L.col.as<int32_t>(),
-                 L.coef.as<double>(), L.diag.as<double>()};
-}
-
 __device__ __forceinline__ uint64_t clipKey(uint64_t key, int lev, int maxL, int clip) {
   return lev > clip ? key & ~((uint64_t(1) << (3 * (maxL - clip))) - 1) : key;
 }
@@ -85,16 +72,6 @@ __global__ void k_clip_mask(long long nc, const int32_t* __restrict__ cstart, co
   cmask[I] = m;
 }
 
-// sum_e coef_e (x_i - x_col_e), with x read through X (octree_poisson.cpp:64-85)
-template <class Rd>
-__device__ __forceinline__ double rowA(const OctView& L, long long i, const Rd& X) {
-  const double xi = X(i);
-  double acc = 0.0;
-  const int e1 = L.rowPtr[i + 1];
-  for (int e = L.rowPtr[i]; e < e1; ++e) acc += L.coef[e] * (xi - X(L.col[e]));
-  return acc;
-}
-
 struct PlainRd {
   const double* x;
   __device__ double operator()(long long j) const { return x[j]; }
@@ -124,11 +101,11 @@ __global__ void __launch_bounds__(kBlock) k_om_sweep(OctView L, const double* __
       if constexpr (MODE == OM_PROLONG) {
         const ProlongRd X{x, ec, up};
         xi = X(i);
-        Ax = rowA(L, i, X);
+        Ax = octRowA(L, i, X);
       } else {
         const PlainRd X{x};
         xi = X(i);
-        Ax = rowA(L, i, X);
+        Ax = octRowA(L, i, X);
       }
       v = xi + kOmega * (b[i] - Ax) / L.diag[i];
       if (FOLD) acc[0] += v;
@@ -153,7 +130,7 @@ __global__ void __launch_bounds__(kBlock) k_om_restrict(OctView F, OctView C, co
   const int32_t f0 = cstart[I0], f1 = cstart[I1];
   const long long i = f0 + (long long)threadIdx.x;
   double r = 0.0;
-  if (i < f1 && F.mask[i] == 0) r = b[i] - rowA(F, i, PlainRd{x});
+  if (i < f1 && F.mask[i] == 0) r = b[i] - octRowA(F, i, PlainRd{x});
   rs[threadIdx.x] = r;
   __syncthreads();
   const long long I = I0 + threadIdx.x;
@@ -323,7 +300,7 @@ bool OctMg::buildDirect(hipStream_t s) {
 // zero guess is already in xa.  *out receives the level's result.
 void OctMg::cycle(size_t l, const double* b, RedBuf rb, const SolveCtl* ctl, hipStream_t s, const double** out) {
   Lvl& L = lv_[l];
-  const OctView V = view(*L.op);
+  const OctView V = octView(*L.op);
   const long long n = V.n;
   const bool fold = l == 0;
   const bool last = l + 1 == lv_.size();
@@ -354,7 +331,7 @@ void OctMg::cycle(size_t l, const double* b, RedBuf rb, const SolveCtl* ctl, hip
   // pre-smoothing: first sweep (in xa) then one plain sweep
   k_om_sweep<OM_PLAIN, false><<<omGrid(n), kBlock, 0, s>>>(V, b, xa, nullptr, nullptr, xb, rb, ctl);
   Lvl& C = lv_[l + 1];
-  const OctView VC = view(*C.op);
+  const OctView VC = octView(*C.op);
   const int rgrid = int((VC.n + kRestrictCoarse - 1) / kRestrictCoarse);
   k_om_restrict<<<rgrid, kBlock, 0, s>>>(V, VC, C.cstart.as<int32_t>(), b, xb, C.b.as<double>(),
                                          C.xa.as<double>(), ctl);

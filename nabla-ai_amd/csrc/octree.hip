@@ -169,30 +169,53 @@ __global__ void k_row_fill(long long n, Geo G, const uint64_t* __restrict__ keys
   diag[i] = d > 0.0 ? d : 1.0;
 }
 
+struct XRd {  // x, or x - m with a pending mean
+  const double* __restrict__ x;
+  double m;
+  bool hasM;
+  __device__ __forceinline__ double operator()(long long j) const { return hasM ? x[j] - m : x[j]; }
+};
+
 // (A x)_i = sum_e coef_e (x_i - x_col_e) on fluid rows, x_i elsewhere
-// (octree_poisson.cpp:64-85).
-__device__ __forceinline__ double octRow(long long i, const uint8_t* __restrict__ mask, const int32_t* __restrict__ rowPtr,
-                                         const int32_t* __restrict__ col, const double* __restrict__ coef,
-                                         const double* __restrict__ x, double m, bool hasM) {
-  double xi = x[i];
-  if (mask[i] != 0) return hasM ? 0.0 : xi;
-  if (hasM) xi = xi - m;
-  double acc = 0.0;
-  const int e1 = rowPtr[i + 1];
-  for (int e = rowPtr[i]; e < e1; ++e) {
-    double xj = x[col[e]];
-    if (hasM) xj = xj - m;
-    acc += coef[e] * (xi - xj);
-  }
-  return acc;
+// (octree_poisson.cpp:64-85); with a pending mean non-fluid rows give 0.
+__device__ __forceinline__ double octRow(const OctView& L, long long i, const double* __restrict__ x, double m,
+                                         bool hasM) {
+  if (L.mask[i] != 0) return hasM ? 0.0 : x[i];
+  return octRowA(L, i, XRd{x, m, hasM});
 }
 
-__global__ void k_oct_apply(long long n, const uint8_t* __restrict__ mask, const int32_t* __restrict__ rowPtr,
-                            const int32_t* __restrict__ col, const double* __restrict__ coef,
-                            const double* __restrict__ x, double* __restrict__ out) {
+__global__ void k_oct_apply(OctView L, const double* __restrict__ x, double* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L.n) return;
+  out[i] = octRow(L, i, x, 0.0, false);
+}
+
+// Slot-major copy of the CSR rows (first kOctSlots couplings per row) and the
+// overflow counts.
+__global__ void k_slot_count(long long n, const int32_t* __restrict__ rowPtr, uint8_t* __restrict__ cnt,
+                             int32_t* __restrict__ xcount) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  out[i] = octRow(i, mask, rowPtr, col, coef, x, 0.0, false);
+  const int c = rowPtr[i + 1] - rowPtr[i];
+  cnt[i] = uint8_t(c);
+  xcount[i] = c > kOctSlots ? c - kOctSlots : 0;
+}
+__global__ void k_slot_fill(long long n, const int32_t* __restrict__ rowPtr, const int32_t* __restrict__ col,
+                            const double* __restrict__ coef, const int32_t* __restrict__ xrow,
+                            int32_t* __restrict__ scol, double* __restrict__ scoef, int32_t* __restrict__ xcol,
+                            double* __restrict__ xcoef) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int r0 = rowPtr[i], c = rowPtr[i + 1] - r0;
+  for (int s = 0; s < kOctSlots; ++s) {
+    const long long q = (long long)s * n + i;
+    scol[q] = s < c ? col[r0 + s] : int32_t(i);
+    scoef[q] = s < c ? coef[r0 + s] : 0.0;
+  }
+  for (int e = kOctSlots; e < c; ++e) {
+    xcol[xrow[i] + e - kOctSlots] = col[r0 + e];
+    xcoef[xrow[i] + e - kOctSlots] = coef[r0 + e];
+  }
 }
 
 __global__ void k_gather(long long n, const uint64_t* __restrict__ perm, const double* __restrict__ src,
@@ -293,14 +316,12 @@ __global__ void __launch_bounds__(kBlock) k_o_zdot(long long n, long long nf, co
   blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
 }
 
-__global__ void __launch_bounds__(kBlock) k_o_apply_dot(long long n, const uint8_t* __restrict__ mask,
-                                                        const int32_t* __restrict__ rowPtr, const int32_t* __restrict__ col,
-                                                        const double* __restrict__ coef, const double* __restrict__ p,
+__global__ void __launch_bounds__(kBlock) k_o_apply_dot(OctView L, const double* __restrict__ p,
                                                         double* __restrict__ Ap, const SolveCtl* ctl, RedBuf rb) {
   if (ldA(&ctl->done)) return;
   double acc[1] = {0.0};
-  OGRID(c, n) {
-    const double v = octRow(c, mask, rowPtr, col, coef, p, 0.0, false);
+  OGRID(c, L.n) {
+    const double v = octRow(L, c, p, 0.0, false);
     Ap[c] = v;
     acc[0] += p[c] * v;
   }
@@ -359,17 +380,15 @@ __global__ void __launch_bounds__(kBlock) k_o_xsum(long long n, const uint8_t* _
   blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
 }
 
-__global__ void __launch_bounds__(kBlock) k_o_final(long long n, long long nf, const uint8_t* __restrict__ mask,
-                                                    const int32_t* __restrict__ rowPtr, const int32_t* __restrict__ col,
-                                                    const double* __restrict__ coef, const double* __restrict__ x,
+__global__ void __launch_bounds__(kBlock) k_o_final(OctView L, long long nf, const double* __restrict__ x,
                                                     const double* __restrict__ rhs, double* __restrict__ xo,
                                                     const SolveCtl* ctl, RedBuf rb) {
   const double m = ldA(&ctl->slot[O_MEAN_X]);
   const bool hasM = nf > 0;
   double acc[1] = {0.0};
-  OGRID(c, n) {
-    const double Ax = octRow(c, mask, rowPtr, col, coef, x, m, hasM);
-    const double xv = mask[c] == 0 ? (hasM ? x[c] - m : x[c]) : (hasM ? 0.0 : x[c]);
+  OGRID(c, L.n) {
+    const double Ax = octRow(L, c, x, m, hasM);
+    const double xv = L.mask[c] == 0 ? (hasM ? x[c] - m : x[c]) : (hasM ? 0.0 : x[c]);
     xo[c] = xv;
     const double e = rhs[c] - Ax;
     acc[0] += e * e;
@@ -542,6 +561,26 @@ void buildOctOperator(nbl_ctx* ctx, int maxL, const double org[3], const double 
                                            cnt.as<uint8_t>(), nbr.as<int32_t>(), L.rowPtr.as<int32_t>(),
                                            L.col.as<int32_t>(), L.coef.as<double>(), L.diag.as<double>());
   NBL_LAUNCH_CHECK();
+  // slot-major operator for the solve kernels
+  L.cnt = DevBuf(ctx, n);
+  L.scol = DevBuf(ctx, size_t(kOctSlots) * n * sizeof(int32_t));
+  L.scoef = DevBuf(ctx, size_t(kOctSlots) * n * sizeof(double));
+  L.xrow = DevBuf(ctx, (n + 1) * sizeof(int32_t));
+  k_slot_count<<<gridFor(N), kBlock, 0, s>>>(N, L.rowPtr.as<int32_t>(), L.cnt.as<uint8_t>(), rowCount.as<int32_t>());
+  NBL_HIP(hipMemsetAsync(rowCount.as<int32_t>() + n, 0, sizeof(int32_t), s));
+  NBL_LAUNCH_CHECK();
+  NBL_HIP(hipcub::DeviceScan::ExclusiveSum(scanTmp.as<void>(), scanBytes, rowCount.as<int32_t>(),
+                                           L.xrow.as<int32_t>(), int(n + 1), s));
+  int32_t nxtra = 0;
+  NBL_HIP(hipMemcpyAsync(&nxtra, L.xrow.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NBL_HIP(hipStreamSynchronize(s));
+  L.nx = size_t(nxtra);
+  L.xcol = DevBuf(ctx, std::max<size_t>(1, L.nx) * sizeof(int32_t));
+  L.xcoef = DevBuf(ctx, std::max<size_t>(1, L.nx) * sizeof(double));
+  k_slot_fill<<<gridFor(N), kBlock, 0, s>>>(N, L.rowPtr.as<int32_t>(), L.col.as<int32_t>(), L.coef.as<double>(),
+                                            L.xrow.as<int32_t>(), L.scol.as<int32_t>(), L.scoef.as<double>(),
+                                            L.xcol.as<int32_t>(), L.xcoef.as<double>());
+  NBL_LAUNCH_CHECK();
   if (cntOut) *cntOut = std::move(cnt);
   if (nbrOut) *nbrOut = std::move(nbr);
   // a freed face table synchronises the device; the fill above has then completed
@@ -645,8 +684,7 @@ void OctreeDev::fromSorted(const double* dSorted, double* hostOut) {
 }
 
 void OctreeDev::applySorted(const double* x, double* out, hipStream_t s) {
-  k_oct_apply<<<gridFor((long long)L0_.n), kBlock, 0, s>>>((long long)L0_.n, L0_.mask.as<uint8_t>(), L0_.rowPtr.as<int32_t>(),
-                                                         L0_.col.as<int32_t>(), L0_.coef.as<double>(), x, out);
+  k_oct_apply<<<gridFor((long long)L0_.n), kBlock, 0, s>>>(octView(L0_), x, out);
   NBL_LAUNCH_CHECK();
 }
 
@@ -670,8 +708,8 @@ void OctreeDev::profile(int kernel, int reps, double* ms, double* bytes) {
   float t = 0.f;
   NBL_HIP(hipEventElapsedTime(&t, e0_, e1_));
   *ms = double(t) / reps;
-  // row pointer, mask, x and y once per leaf; column index and coefficient per coupling
-  *bytes = double(L0_.n) * (4.0 + 1.0 + 8.0 + 8.0) + double(L0_.nnz) * (4.0 + 8.0);
+  // coupling count, mask, x and y once per leaf; column index and coefficient per coupling
+  *bytes = double(L0_.n) * (1.0 + 1.0 + 8.0 + 8.0) + double(L0_.nnz) * (4.0 + 8.0);
 }
 
 nbl_poisson_report OctreeDev::solveSorted(const double* b, double* xo, double tol, int maxIters, hipStream_t s) {
@@ -682,10 +720,8 @@ nbl_poisson_report OctreeDev::solveSorted(const double* b, double* xo, double to
   launch_ctl_init(ctl, tol, s);
   NBL_HIP(hipEventRecord(e0_, s));
   const uint8_t* mk = L0_.mask.as<uint8_t>();
-  const int32_t* rp = L0_.rowPtr.as<int32_t>();
-  const int32_t* cl = L0_.col.as<int32_t>();
-  const double* cf = L0_.coef.as<double>();
   const double* dg = L0_.diag.as<double>();
+  const OctView V = octView(L0_);
   double *x = x_.as<double>(), *rhs = rhs_.as<double>(), *r = r_.as<double>(), *z = z_.as<double>();
   double *p = p_.as<double>(), *Ap = Ap_.as<double>();
   auto fin = [&](int op, int K, int slot, bool pred) {
@@ -710,7 +746,7 @@ nbl_poisson_report OctreeDev::solveSorted(const double* b, double* xo, double to
   int batch = std::clamp(predicted_, 1, std::max(1, maxIters));
   while (enq < maxIters) {
     for (int q = 0; q < batch && enq < maxIters; ++q, ++enq) {
-      k_o_apply_dot<<<g, kBlock, 0, s>>>(N, mk, rp, cl, cf, p, Ap, ctl, rb);
+      k_o_apply_dot<<<g, kBlock, 0, s>>>(V, p, Ap, ctl, rb);
       fin(FIN_ALPHA, 1, 0, true);
       k_o_update<<<g, kBlock, 0, s>>>(N, mk, x, p, r, Ap, ctl, rb);
       fin(FIN_MEAN, 1, O_MEAN_R, true);
@@ -728,7 +764,7 @@ nbl_poisson_report OctreeDev::solveSorted(const double* b, double* xo, double to
   }
   k_o_xsum<<<g, kBlock, 0, s>>>(N, mk, x, rb);
   fin(FIN_MEAN, 1, O_MEAN_X, false);
-  k_o_final<<<g, kBlock, 0, s>>>(N, nf, mk, rp, cl, cf, x, rhs, xo, ctl, rb);
+  k_o_final<<<g, kBlock, 0, s>>>(V, nf, x, rhs, xo, ctl, rb);
   fin(FIN_RESID, 1, 0, false);
   NBL_LAUNCH_CHECK();
   NBL_HIP(hipEventRecord(e1_, s));
@@ -774,10 +810,8 @@ nbl_poisson_report OctreeDev::solveMgSorted(const double* b, double* xo, double 
   launch_ctl_init(ctl, tol, s);
   NBL_HIP(hipEventRecord(e0_, s));
   const uint8_t* mk = L0_.mask.as<uint8_t>();
-  const int32_t* rp = L0_.rowPtr.as<int32_t>();
-  const int32_t* cl = L0_.col.as<int32_t>();
-  const double* cf = L0_.coef.as<double>();
   const double* dg = L0_.diag.as<double>();
+  const OctView V = octView(L0_);
   double *x = x_.as<double>(), *rhs = rhs_.as<double>(), *r = r_.as<double>(), *z = z_.as<double>();
   double *p = p_.as<double>(), *Ap = Ap_.as<double>(), *rn = t1_.as<double>();
   auto fin = [&](int op, int K, int slot, bool pred, int early = 0) {
@@ -805,7 +839,7 @@ nbl_poisson_report OctreeDev::solveMgSorted(const double* b, double* xo, double 
   int batch = std::clamp(predictedMg_, 1, std::max(1, maxIters));
   while (enq < maxIters) {
     for (int q = 0; q < batch && enq < maxIters; ++q, ++enq) {
-      k_o_apply_dot<<<g, kBlock, 0, s>>>(N, mk, rp, cl, cf, p, Ap, ctl, rb);
+      k_o_apply_dot<<<g, kBlock, 0, s>>>(V, p, Ap, ctl, rb);
       fin(FIN_ALPHA, 1, 0, true);
       k_om_update<<<g, kBlock, 0, s>>>(N, mk, x, p, r, Ap, rn, ctl, rb);
       fin(FIN_MEAN, 1, O_MEAN_R, true);
@@ -825,7 +859,7 @@ nbl_poisson_report OctreeDev::solveMgSorted(const double* b, double* xo, double 
   }
   k_o_xsum<<<g, kBlock, 0, s>>>(N, mk, x, rb);
   fin(FIN_MEAN, 1, O_MEAN_X, false);
-  k_o_final<<<g, kBlock, 0, s>>>(N, nf, mk, rp, cl, cf, x, rhs, xo, ctl, rb);
+  k_o_final<<<g, kBlock, 0, s>>>(V, nf, x, rhs, xo, ctl, rb);
   fin(FIN_RESID, 1, 0, false);
   NBL_LAUNCH_CHECK();
   NBL_HIP(hipEventRecord(e1_, s));

@@ -15,9 +15,51 @@ namespace nbl {
 // face coefficients area/dist; diag is their sum (1 on non-fluid rows and on
 // fluid rows without fluid neighbours).
 struct OctLevel {
-  size_t n = 0, nnz = 0, nf = 0;
+  size_t n = 0, nnz = 0, nf = 0, nx = 0;
   DevBuf keys, lev, mask, rowPtr, col, coef, diag;
+  // The same couplings for the operator kernels: the first kOctSlots of every
+  // row in slot-major arrays (slot s of row i at s*n + i, so a wavefront's
+  // reads of one slot coalesce), the rest in an overflow CSR.
+  DevBuf cnt, scol, scoef, xrow, xcol, xcoef;
 };
+
+constexpr int kOctSlots = 6;
+
+// Device view of a level's operator.
+struct OctView {
+  long long n;
+  const uint8_t* mask;
+  const uint8_t* cnt;
+  const int32_t* scol;
+  const double* scoef;
+  const int32_t* xrow;
+  const int32_t* xcol;
+  const double* xcoef;
+  const double* diag;
+};
+inline OctView octView(const OctLevel& L) {
+  return OctView{(long long)L.n,       L.mask.as<uint8_t>(),  L.cnt.as<uint8_t>(),  L.scol.as<int32_t>(),
+                 L.scoef.as<double>(), L.xrow.as<int32_t>(),  L.xcol.as<int32_t>(), L.xcoef.as<double>(),
+                 L.diag.as<double>()};
+}
+
+// sum_e coef_e (x_i - x_col_e) over row i's couplings in the reference's order
+// (octree_poisson.cpp:64-85), x read through X(j).
+template <class Rd>
+__device__ __forceinline__ double octRowA(const OctView& L, long long i, const Rd& X) {
+  const double xi = X(i);
+  const int c = L.cnt[i];
+  double acc = 0.0;
+#pragma unroll
+  for (int s = 0; s < kOctSlots; ++s)
+    if (s < c) {
+      const long long q = (long long)s * L.n + i;
+      acc += L.scoef[q] * (xi - X(L.scol[q]));
+    }
+  if (c > kOctSlots)
+    for (int e = L.xrow[i]; e < L.xrow[i + 1]; ++e) acc += L.xcoef[e] * (xi - X(L.xcol[e]));
+  return acc;
+}
 
 // Face table (count u8[6n], neighbours i32[24n]) and CSR operator of a level
 // whose keys, levels and masks are already on the device and sorted.  The face
