@@ -5,13 +5,24 @@
 // levels as the intended multigrid levels (docs/adr/0007, 0008) without an
 // implementation.  This file builds that hierarchy on the device and applies
 // the same V(2,2) cycle as the uniform solver (core/src/flow/multigrid.cpp:
-// 102-194: damped Jacobi, omega 0.8, two pre- and two post-sweeps, exact-enough
+// 102-194: damped Jacobi, omega 0.8, two pre- and two post-sweeps, exact
 // coarsest solve), with the restriction and prolongation of a
 // volume-integrated finite-volume operator.
 //
-// Device layout per level: Morton-sorted leaves (OctLevel), up[i] = parent of
-// leaf i in the next coarser level, cstart[I] = first child of coarse cell I
-// in the next finer level (children are contiguous, at most 8).
+// Cells of every level are boxes in finest-index units (Morton key of the
+// anchor, log2 size per axis).  A coarsening step raises a target log size S
+// per axis: the axes whose physical target size is at most 1.5x the smallest
+// grow by one, so cells of an anisotropic box (the cube-wake domain's 3:1:1
+// leaves) first merge along the strongly coupled axes.  Cells no larger than
+// S on every axis group by their S-aligned box; a group that covers its box
+// exactly (volume sum) becomes one coarse cell, every other cell stays.  The
+// coarse couplings sum the fine face areas between two coarse cells and use
+// area / centre distance, the reference's face coefficient on octree cells.
+//
+// Device layout per level: OctLevel (keys, lsz, mask, CSR with areas and
+// axes, slot-major operator), up[i] = parent of cell i in the next coarser
+// level, and on coarse levels cstart[I] .. cstart[I+1] indexing child[], the
+// cells of the next finer level that form cell I (increasing index).
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -28,49 +39,9 @@ constexpr double kOmega = 0.8;        // multigrid.cpp:38
 constexpr size_t kCoarsestMax = 2048;  // multigrid.cpp:40
 constexpr int kMaxLevels = 16;
 constexpr int kOmBlocks = 4096;        // partial slots of the reducing sweep
-constexpr int kRestrictCoarse = 32;    // coarse cells per restriction block (<= 8 children each)
 constexpr int kFallbackSweeps = 16;    // coarsest level when its inverse cannot be formed (even)
 
 #define OGRID(c, n) for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < (n); c += (long long)gridDim.x * blockDim.x)
-
-__device__ __forceinline__ uint64_t clipKey(uint64_t key, int lev, int maxL, int clip) {
-  return lev > clip ? key & ~((uint64_t(1) << (3 * (maxL - clip))) - 1) : key;
-}
-
-__global__ void k_clip_head(long long n, int maxL, int clip, const uint64_t* __restrict__ keys,
-                            const uint8_t* __restrict__ lev, int32_t* __restrict__ head) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint64_t k = clipKey(keys[i], lev[i], maxL, clip);
-  head[i] = (i == 0 || clipKey(keys[i - 1], lev[i - 1], maxL, clip) != k) ? 1 : 0;
-}
-
-__global__ void k_clip_scatter(long long n, long long nc, int maxL, int clip, const uint64_t* __restrict__ keys,
-                               const uint8_t* __restrict__ lev, const int32_t* __restrict__ head,
-                               const int32_t* __restrict__ pos, uint64_t* __restrict__ ckeys,
-                               uint8_t* __restrict__ clev, int32_t* __restrict__ cstart, int32_t* __restrict__ up) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int32_t I = pos[i] - 1;
-  up[i] = I;
-  if (head[i]) {
-    ckeys[I] = clipKey(keys[i], lev[i], maxL, clip);
-    clev[I] = uint8_t(min(int(lev[i]), clip));
-    cstart[I] = int32_t(i);
-  }
-  if (i == 0) cstart[nc] = int32_t(n);
-}
-
-// A coarse cell is fluid iff any child is fluid (multigrid.cpp:243-244).
-__global__ void k_clip_mask(long long nc, const int32_t* __restrict__ cstart, const uint8_t* __restrict__ fmask,
-                            uint8_t* __restrict__ cmask) {
-  const long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (I >= nc) return;
-  uint8_t m = 2;
-  for (int32_t f = cstart[I]; f < cstart[I + 1]; ++f)
-    if (fmask[f] == 0) m = 0;
-  cmask[I] = m;
-}
 
 struct PlainRd {
   const double* x;
@@ -115,31 +86,172 @@ __global__ void __launch_bounds__(kBlock) k_om_sweep(OctView L, const double* __
   if constexpr (FOLD) blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
 }
 
-// Residual b - A x of the fine leaves, summed per coarse cell in child order
-// (a block owns kRestrictCoarse coarse cells and their <= 8*kRestrictCoarse
-// contiguous children).  With xc: the coarse level's first sweep from a zero
-// guess, omega*bc/diag_c (fluid cells).
+// Residuals b - A x of a coarse cell's children summed in child order
+// (thread per coarse cell); with xc, the coarse level's first sweep from a
+// zero guess, omega*bc/diag_c (fluid cells).
 __global__ void __launch_bounds__(kBlock) k_om_restrict(OctView F, OctView C, const int32_t* __restrict__ cstart,
+                                                        const int32_t* __restrict__ child,
                                                         const double* __restrict__ b, const double* __restrict__ x,
                                                         double* __restrict__ bc, double* __restrict__ xc,
                                                         const SolveCtl* ctl) {
   if (ctl && ldA(&ctl->done)) return;
-  __shared__ double rs[kRestrictCoarse * 8];
-  const long long I0 = (long long)blockIdx.x * kRestrictCoarse;
-  const long long I1 = min(I0 + kRestrictCoarse, C.n);
-  const int32_t f0 = cstart[I0], f1 = cstart[I1];
-  const long long i = f0 + (long long)threadIdx.x;
-  double r = 0.0;
-  if (i < f1 && F.mask[i] == 0) r = b[i] - octRowA(F, i, PlainRd{x});
-  rs[threadIdx.x] = r;
-  __syncthreads();
-  const long long I = I0 + threadIdx.x;
-  if (threadIdx.x < kRestrictCoarse && I < I1) {
-    double s = 0.0;
-    for (int32_t f = cstart[I]; f < cstart[I + 1]; ++f) s += rs[f - f0];
-    bc[I] = s;
-    if (xc) xc[I] = C.mask[I] == 0 ? kOmega * s / C.diag[I] : 0.0;
+  const long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= C.n) return;
+  double s = 0.0;
+  for (int32_t p = cstart[I]; p < cstart[I + 1]; ++p) {
+    const int32_t i = child[p];
+    if (F.mask[i] == 0) s += b[i] - octRowA(F, i, PlainRd{x});
   }
+  bc[I] = s;
+  if (xc) xc[I] = C.mask[I] == 0 ? kOmega * s / C.diag[I] : 0.0;
+}
+
+// ---- coarsening ---------------------------------------------------------------
+__device__ __forceinline__ int lszAxis(uint32_t w, int ax) { return int((w >> (8 * ax)) & 0xffu); }
+
+// Cells no larger than S on every axis: key of their S-aligned box and their
+// volume (2^(lx+ly+lz)); other cells: the sentinel key and volume 0.
+__global__ void k_sc_target(long long n, int S0, int S1, int S2, const uint64_t* __restrict__ keys,
+                            const uint32_t* __restrict__ lsz, uint64_t* __restrict__ tkey,
+                            uint64_t* __restrict__ vol, int32_t* __restrict__ idx) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t w = lsz[i];
+  const int l0 = lszAxis(w, 0), l1 = lszAxis(w, 1), l2 = lszAxis(w, 2);
+  idx[i] = int32_t(i);
+  if (l0 > S0 || l1 > S1 || l2 > S2) {
+    tkey[i] = ~0ull;
+    vol[i] = 0;
+    return;
+  }
+  const uint64_t k = keys[i];
+  tkey[i] = mortonEnc3(mortonAxis(k, 0) & ~((1u << S0) - 1u), mortonAxis(k, 1) & ~((1u << S1) - 1u),
+                       mortonAxis(k, 2) & ~((1u << S2) - 1u));
+  vol[i] = 1ull << (l0 + l1 + l2);
+}
+
+// Per group (run of equal target keys in sorted order): complete iff it is
+// not the sentinel, has more than one member and its volumes fill the box.
+__global__ void k_sc_complete(long long runs, const uint64_t* __restrict__ ukey, const int32_t* __restrict__ len,
+                              const int32_t* __restrict__ off, const int32_t* __restrict__ idxS,
+                              const uint64_t* __restrict__ vol, uint64_t full, int32_t* __restrict__ complete) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= runs) return;
+  uint64_t v = 0;
+  for (int32_t p = off[r]; p < off[r] + len[r]; ++p) v += vol[idxS[p]];
+  complete[r] = (ukey[r] != ~0ull && len[r] > 1 && v == full) ? 1 : 0;
+}
+
+// Coarse key and size of every cell: its group's box when complete, else its own.
+__global__ void k_sc_assign(long long runs, const uint64_t* __restrict__ ukey, const int32_t* __restrict__ len,
+                            const int32_t* __restrict__ off, const int32_t* __restrict__ idxS,
+                            const int32_t* __restrict__ complete, uint32_t Spacked,
+                            const uint64_t* __restrict__ keys, const uint32_t* __restrict__ lsz,
+                            uint64_t* __restrict__ ckey, uint32_t* __restrict__ clsz) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= runs) return;
+  for (int32_t p = off[r]; p < off[r] + len[r]; ++p) {
+    const int32_t i = idxS[p];
+    ckey[i] = complete[r] ? ukey[r] : keys[i];
+    clsz[i] = complete[r] ? Spacked : lsz[i];
+  }
+}
+
+__global__ void k_iota32(long long n, int32_t* __restrict__ a) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = int32_t(i);
+}
+
+__global__ void k_sc_heads(long long n, const uint64_t* __restrict__ k, int32_t* __restrict__ head) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) head[p] = (p == 0 || k[p] != k[p - 1]) ? 1 : 0;
+}
+
+// Parents from the cells sorted by coarse key: up, children ranges, coarse
+// keys and sizes.
+__global__ void k_sc_parents(long long n, long long nc, const uint64_t* __restrict__ ckeyS,
+                             const int32_t* __restrict__ child, const int32_t* __restrict__ pid,
+                             const int32_t* __restrict__ head, const uint32_t* __restrict__ clsz,
+                             int32_t* __restrict__ up, int32_t* __restrict__ cstart, uint64_t* __restrict__ keysC,
+                             uint32_t* __restrict__ lszC) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int32_t I = pid[p] - 1;
+  const int32_t i = child[p];
+  up[i] = I;
+  if (head[p]) {
+    cstart[I] = int32_t(p);
+    keysC[I] = ckeyS[p];
+    lszC[I] = clsz[i];
+  }
+  if (p == 0) cstart[nc] = int32_t(n);
+}
+
+// A coarse cell is fluid iff any child is fluid (multigrid.cpp:243-244).
+__global__ void k_sc_mask(long long nc, const int32_t* __restrict__ cstart, const int32_t* __restrict__ child,
+                          const uint8_t* __restrict__ fmask, uint8_t* __restrict__ cmask) {
+  const long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nc) return;
+  uint8_t m = 2;
+  for (int32_t p = cstart[I]; p < cstart[I + 1]; ++p)
+    if (fmask[child[p]] == 0) m = 0;
+  cmask[I] = m;
+}
+
+// Coarse pair key of every fine coupling (sentinel inside one coarse cell).
+__global__ void k_sc_emit(long long n, const int32_t* __restrict__ rowPtr, const int32_t* __restrict__ col,
+                          const int32_t* __restrict__ up, uint64_t* __restrict__ ekey, int32_t* __restrict__ eid) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t I = uint32_t(up[i]);
+  for (int32_t e = rowPtr[i]; e < rowPtr[i + 1]; ++e) {
+    const uint32_t J = uint32_t(up[col[e]]);
+    ekey[e] = I != J ? (uint64_t(I) << 32) | J : ~0ull;
+    eid[e] = e;
+  }
+}
+
+__global__ void k_sc_rowcount(long long nu, const uint64_t* __restrict__ ukey, int32_t* __restrict__ cnt) {
+  const long long u = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u < nu) atomicAdd(&cnt[ukey[u] >> 32], 1);
+}
+
+struct BoxGeo {
+  double org[3], fine[3];
+};
+__device__ __forceinline__ double boxCentre(const BoxGeo& G, uint64_t key, uint32_t lsz, int ax) {
+  return G.org[ax] + (double(mortonAxis(key, ax)) + 0.5 * double(1ull << lszAxis(lsz, ax))) * G.fine[ax];
+}
+
+// Coarse coupling u: the fine face areas of its run summed in fine coupling
+// order, the run's axis, and coefficient area / centre distance.
+__global__ void k_sc_fill(long long nu, const uint64_t* __restrict__ ukey, const int32_t* __restrict__ len,
+                          const int32_t* __restrict__ off, const int32_t* __restrict__ eidS,
+                          const double* __restrict__ farea, const uint8_t* __restrict__ fax, BoxGeo G,
+                          const uint64_t* __restrict__ keysC, const uint32_t* __restrict__ lszC,
+                          int32_t* __restrict__ col, double* __restrict__ coef, double* __restrict__ area,
+                          uint8_t* __restrict__ cax) {
+  const long long u = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= nu) return;
+  double a = 0.0;
+  for (int32_t p = off[u]; p < off[u] + len[u]; ++p) a += farea[eidS[p]];
+  const int ax = fax[eidS[off[u]]];
+  const uint32_t I = uint32_t(ukey[u] >> 32), J = uint32_t(ukey[u] & 0xffffffffu);
+  const double dist = fabs(boxCentre(G, keysC[I], lszC[I], ax) - boxCentre(G, keysC[J], lszC[J], ax));
+  col[u] = int32_t(J);
+  area[u] = a;
+  cax[u] = uint8_t(ax);
+  coef[u] = dist > 0.0 ? a / dist : 0.0;
+}
+
+__global__ void k_sc_diag(long long nc, const uint8_t* __restrict__ mask, const int32_t* __restrict__ rowPtr,
+                          const double* __restrict__ coef, double* __restrict__ diag) {
+  const long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nc) return;
+  double d = 0.0;
+  if (mask[I] == 0)
+    for (int32_t u = rowPtr[I]; u < rowPtr[I + 1]; ++u) d += coef[u];
+  diag[I] = d > 0.0 ? d : 1.0;
 }
 
 __global__ void __launch_bounds__(kBlock) k_om_fsum(OctView L, const double* __restrict__ x, RedBuf rb,
@@ -185,56 +297,182 @@ OctMg::OctMg(nbl_ctx* ctx, const OctreeDev& fine, hipStream_t s) : ctx_(ctx) {
   parts_ = omGrid((long long)lv_[0].op->n);
 }
 
+namespace {
+// hipcub helpers: query the temporary size, allocate, run.
+template <class F>
+void cub(nbl_ctx* ctx, F&& f) {
+  size_t tb = 0;
+  NBL_HIP(f(nullptr, tb));
+  DevBuf tmp(ctx, std::max<size_t>(tb, 1));
+  NBL_HIP(f(tmp.as<void>(), tb));
+}
+template <class T>
+T readBack(const T* d, hipStream_t s) {
+  T h{};
+  NBL_HIP(hipMemcpyAsync(&h, d, sizeof(T), hipMemcpyDeviceToHost, s));
+  NBL_HIP(hipStreamSynchronize(s));
+  return h;
+}
+}  // namespace
+
 void OctMg::buildCoarse(const OctreeDev& fine, hipStream_t s) {
   const int maxL = fine.maxLevel();
-  // finest level present in level 0
-  const OctLevel& L0 = fine.level0();
-  std::vector<uint8_t> hl(L0.n);
-  NBL_HIP(hipMemcpyAsync(hl.data(), L0.lev.as<void>(), L0.n, hipMemcpyDeviceToHost, s));
-  NBL_HIP(hipStreamSynchronize(s));
-  int top = 0;
-  for (uint8_t v : hl) top = std::max(top, int(v));
-  while (lv_.back().op->n > kCoarsestMax && int(lv_.size()) < kMaxLevels && top > 0) {
-    const int clip = top - 1;
+  BoxGeo G;
+  for (int a = 0; a < 3; ++a) {
+    G.org[a] = fine.origin()[a];
+    G.fine[a] = fine.fine()[a];
+  }
+  // target size of level 0: the finest leaves
+  {
+    const OctLevel& L0 = fine.level0();
+    std::vector<uint32_t> h(L0.n);
+    NBL_HIP(hipMemcpyAsync(h.data(), L0.lsz.as<void>(), L0.n * 4, hipMemcpyDeviceToHost, s));
+    NBL_HIP(hipStreamSynchronize(s));
+    int m = 1 << 30;
+    for (uint32_t w : h) m = std::min(m, int(w & 0xffu));
+    for (int a = 0; a < 3; ++a) lv_[0].S[a] = m;
+  }
+  while (lv_.back().op->n > kCoarsestMax && int(lv_.size()) < kMaxLevels) {
     Lvl& F = lv_.back();
     const OctLevel& fo = *F.op;
     const long long n = (long long)fo.n;
-    DevBuf head(ctx_, fo.n * 4), pos(ctx_, fo.n * 4);
-    k_clip_head<<<gridFor(n), kBlock, 0, s>>>(n, maxL, clip, fo.keys.as<uint64_t>(), fo.lev.as<uint8_t>(),
-                                              head.as<int32_t>());
-    NBL_LAUNCH_CHECK();
-    size_t tb = 0;
-    NBL_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tb, head.as<int32_t>(), pos.as<int32_t>(), int(n), s));
-    DevBuf tmp(ctx_, tb);
-    NBL_HIP(hipcub::DeviceScan::InclusiveSum(tmp.as<void>(), tb, head.as<int32_t>(), pos.as<int32_t>(), int(n), s));
-    int32_t nc = 0;
-    NBL_HIP(hipMemcpyAsync(&nc, pos.as<int32_t>() + (n - 1), 4, hipMemcpyDeviceToHost, s));
-    NBL_HIP(hipStreamSynchronize(s));
-    if (nc <= 0 || nc >= n) break;  // nothing merges
+    const int ni = int(n);
+    DevBuf tkey(ctx_, fo.n * 8), vol(ctx_, fo.n * 8), idx(ctx_, fo.n * 4), tkeyS(ctx_, fo.n * 8), idxS(ctx_, fo.n * 4);
+    DevBuf ukey(ctx_, fo.n * 8), len(ctx_, fo.n * 4), off(ctx_, fo.n * 4), nrun(ctx_, 4), cmpl(ctx_, fo.n * 4);
+    DevBuf ncmpl(ctx_, 4), ckey(ctx_, fo.n * 8), clsz(ctx_, fo.n * 4);
+    int T[3] = {F.S[0], F.S[1], F.S[2]};
+    bool merged = false;
+    while (!(T[0] >= maxL && T[1] >= maxL && T[2] >= maxL)) {
+      // grow the target along the axes whose physical size is within 1.5x of the smallest
+      double h[3], hmin = 1e300;
+      for (int a = 0; a < 3; ++a) {
+        h[a] = G.fine[a] * double(1ull << T[a]);
+        hmin = std::min(hmin, h[a]);
+      }
+      for (int a = 0; a < 3; ++a)
+        if (h[a] <= 1.5 * hmin && T[a] < maxL) ++T[a];
+      k_sc_target<<<gridFor(n), kBlock, 0, s>>>(n, T[0], T[1], T[2], fo.keys.as<uint64_t>(), fo.lsz.as<uint32_t>(),
+                                                tkey.as<uint64_t>(), vol.as<uint64_t>(), idx.as<int32_t>());
+      NBL_LAUNCH_CHECK();
+      cub(ctx_, [&](void* t, size_t& tb) {
+        return hipcub::DeviceRadixSort::SortPairs(t, tb, tkey.as<uint64_t>(), tkeyS.as<uint64_t>(), idx.as<int32_t>(),
+                                                  idxS.as<int32_t>(), ni, 0, 64, s);
+      });
+      cub(ctx_, [&](void* t, size_t& tb) {
+        return hipcub::DeviceRunLengthEncode::Encode(t, tb, tkeyS.as<uint64_t>(), ukey.as<uint64_t>(),
+                                                     len.as<int32_t>(), nrun.as<int32_t>(), ni, s);
+      });
+      const int runs = readBack(nrun.as<int32_t>(), s);
+      cub(ctx_, [&](void* t, size_t& tb) {
+        return hipcub::DeviceScan::ExclusiveSum(t, tb, len.as<int32_t>(), off.as<int32_t>(), runs, s);
+      });
+      k_sc_complete<<<gridFor(runs), kBlock, 0, s>>>(runs, ukey.as<uint64_t>(), len.as<int32_t>(), off.as<int32_t>(),
+                                                     idxS.as<int32_t>(), vol.as<uint64_t>(),
+                                                     1ull << (T[0] + T[1] + T[2]), cmpl.as<int32_t>());
+      NBL_LAUNCH_CHECK();
+      cub(ctx_, [&](void* t, size_t& tb) {
+        return hipcub::DeviceReduce::Sum(t, tb, cmpl.as<int32_t>(), ncmpl.as<int32_t>(), runs, s);
+      });
+      if (readBack(ncmpl.as<int32_t>(), s) == 0) continue;  // nothing merges at this size: grow further
+      const uint32_t Sp = uint32_t(T[0]) | (uint32_t(T[1]) << 8) | (uint32_t(T[2]) << 16);
+      k_sc_assign<<<gridFor(runs), kBlock, 0, s>>>(runs, ukey.as<uint64_t>(), len.as<int32_t>(), off.as<int32_t>(),
+                                                   idxS.as<int32_t>(), cmpl.as<int32_t>(), Sp, fo.keys.as<uint64_t>(),
+                                                   fo.lsz.as<uint32_t>(), ckey.as<uint64_t>(), clsz.as<uint32_t>());
+      NBL_LAUNCH_CHECK();
+      merged = true;
+      break;
+    }
+    if (!merged) break;
+    // cells sorted by coarse key: children grouped by parent, increasing index
     Lvl C;
-    C.own.n = size_t(nc);
-    C.own.keys = DevBuf(ctx_, size_t(nc) * 8);
-    C.own.lev = DevBuf(ctx_, size_t(nc));
-    C.own.mask = DevBuf(ctx_, size_t(nc));
-    C.cstart = DevBuf(ctx_, size_t(nc + 1) * 4);
-    F.up = DevBuf(ctx_, fo.n * 4);
-    k_clip_scatter<<<gridFor(n), kBlock, 0, s>>>(n, nc, maxL, clip, fo.keys.as<uint64_t>(), fo.lev.as<uint8_t>(),
-                                                 head.as<int32_t>(), pos.as<int32_t>(), C.own.keys.as<uint64_t>(),
-                                                 C.own.lev.as<uint8_t>(), C.cstart.as<int32_t>(), F.up.as<int32_t>());
-    k_clip_mask<<<gridFor(nc), kBlock, 0, s>>>(nc, C.cstart.as<int32_t>(), fo.mask.as<uint8_t>(),
-                                               C.own.mask.as<uint8_t>());
+    C.child = DevBuf(ctx_, fo.n * 4);
+    DevBuf ckeyS(ctx_, fo.n * 8), head(ctx_, fo.n * 4), pid(ctx_, fo.n * 4);
+    k_iota32<<<gridFor(n), kBlock, 0, s>>>(n, idx.as<int32_t>());
     NBL_LAUNCH_CHECK();
-    // the restriction kernel stages at most 8 children per coarse cell
-    std::vector<int32_t> hs(size_t(nc) + 1);
-    NBL_HIP(hipMemcpyAsync(hs.data(), C.cstart.as<void>(), hs.size() * 4, hipMemcpyDeviceToHost, s));
-    NBL_HIP(hipStreamSynchronize(s));
-    for (size_t I = 0; I < size_t(nc); ++I)
-      if (hs[I + 1] - hs[I] < 1 || hs[I + 1] - hs[I] > 8)
-        throw std::logic_error("octree multigrid: a coarse cell has more than 8 children");
-    buildOctOperator(ctx_, maxL, fine.origin(), fine.fine(), C.own, nullptr, nullptr, s);
+    cub(ctx_, [&](void* t, size_t& tb) {
+      return hipcub::DeviceRadixSort::SortPairs(t, tb, ckey.as<uint64_t>(), ckeyS.as<uint64_t>(), idx.as<int32_t>(),
+                                                C.child.as<int32_t>(), ni, 0, 64, s);
+    });
+    k_sc_heads<<<gridFor(n), kBlock, 0, s>>>(n, ckeyS.as<uint64_t>(), head.as<int32_t>());
+    NBL_LAUNCH_CHECK();
+    cub(ctx_, [&](void* t, size_t& tb) {
+      return hipcub::DeviceScan::InclusiveSum(t, tb, head.as<int32_t>(), pid.as<int32_t>(), ni, s);
+    });
+    const int nc = readBack(pid.as<int32_t>() + (n - 1), s);
+    if (nc <= 0 || nc >= n) break;
+    OctLevel& co = C.own;
+    co.n = size_t(nc);
+    co.keys = DevBuf(ctx_, co.n * 8);
+    co.lsz = DevBuf(ctx_, co.n * 4);
+    co.mask = DevBuf(ctx_, co.n);
+    C.cstart = DevBuf(ctx_, (co.n + 1) * 4);
+    F.up = DevBuf(ctx_, fo.n * 4);
+    k_sc_parents<<<gridFor(n), kBlock, 0, s>>>(n, nc, ckeyS.as<uint64_t>(), C.child.as<int32_t>(), pid.as<int32_t>(),
+                                               head.as<int32_t>(), clsz.as<uint32_t>(), F.up.as<int32_t>(),
+                                               C.cstart.as<int32_t>(), co.keys.as<uint64_t>(), co.lsz.as<uint32_t>());
+    k_sc_mask<<<gridFor(nc), kBlock, 0, s>>>(nc, C.cstart.as<int32_t>(), C.child.as<int32_t>(), fo.mask.as<uint8_t>(),
+                                             co.mask.as<uint8_t>());
+    NBL_LAUNCH_CHECK();
+    // coarse couplings: fine face areas summed per coarse pair
+    const long long ne = (long long)fo.nnz;
+    const int nei = int(ne);
+    int nu = 0;
+    DevBuf ekey(ctx_, std::max<size_t>(1, fo.nnz) * 8), eid(ctx_, std::max<size_t>(1, fo.nnz) * 4);
+    DevBuf ekeyS(ctx_, std::max<size_t>(1, fo.nnz) * 8), eidS(ctx_, std::max<size_t>(1, fo.nnz) * 4);
+    DevBuf uk(ctx_, std::max<size_t>(1, fo.nnz) * 8), ulen(ctx_, std::max<size_t>(1, fo.nnz) * 4);
+    DevBuf uoff(ctx_, std::max<size_t>(1, fo.nnz) * 4);
+    if (ne > 0) {
+      k_sc_emit<<<gridFor(n), kBlock, 0, s>>>(n, fo.rowPtr.as<int32_t>(), fo.col.as<int32_t>(), F.up.as<int32_t>(),
+                                              ekey.as<uint64_t>(), eid.as<int32_t>());
+      NBL_LAUNCH_CHECK();
+      cub(ctx_, [&](void* t, size_t& tb) {
+        return hipcub::DeviceRadixSort::SortPairs(t, tb, ekey.as<uint64_t>(), ekeyS.as<uint64_t>(), eid.as<int32_t>(),
+                                                  eidS.as<int32_t>(), nei, 0, 64, s);
+      });
+      cub(ctx_, [&](void* t, size_t& tb) {
+        return hipcub::DeviceRunLengthEncode::Encode(t, tb, ekeyS.as<uint64_t>(), uk.as<uint64_t>(), ulen.as<int32_t>(),
+                                                     nrun.as<int32_t>(), nei, s);
+      });
+      nu = readBack(nrun.as<int32_t>(), s);
+      if (nu > 0 && readBack(uk.as<uint64_t>() + (nu - 1), s) == ~0ull) --nu;  // the sentinel run sorts last
+      if (nu > 0)
+        cub(ctx_, [&](void* t, size_t& tb) {
+          return hipcub::DeviceScan::ExclusiveSum(t, tb, ulen.as<int32_t>(), uoff.as<int32_t>(), nu, s);
+        });
+    }
+    co.nnz = size_t(nu);
+    DevBuf rc(ctx_, (co.n + 1) * 4);
+    co.rowPtr = DevBuf(ctx_, (co.n + 1) * 4);
+    NBL_HIP(hipMemsetAsync(rc.as<void>(), 0, (co.n + 1) * 4, s));
+    if (nu > 0) k_sc_rowcount<<<gridFor(nu), kBlock, 0, s>>>(nu, uk.as<uint64_t>(), rc.as<int32_t>());
+    NBL_LAUNCH_CHECK();
+    cub(ctx_, [&](void* t, size_t& tb) {
+      return hipcub::DeviceScan::ExclusiveSum(t, tb, rc.as<int32_t>(), co.rowPtr.as<int32_t>(), nc + 1, s);
+    });
+    co.col = DevBuf(ctx_, std::max<size_t>(1, co.nnz) * 4);
+    co.coef = DevBuf(ctx_, std::max<size_t>(1, co.nnz) * 8);
+    co.area = DevBuf(ctx_, std::max<size_t>(1, co.nnz) * 8);
+    co.cax = DevBuf(ctx_, std::max<size_t>(1, co.nnz));
+    co.diag = DevBuf(ctx_, co.n * 8);
+    if (nu > 0)
+      k_sc_fill<<<gridFor(nu), kBlock, 0, s>>>(nu, uk.as<uint64_t>(), ulen.as<int32_t>(), uoff.as<int32_t>(),
+                                               eidS.as<int32_t>(), fo.area.as<double>(), fo.cax.as<uint8_t>(), G,
+                                               co.keys.as<uint64_t>(), co.lsz.as<uint32_t>(), co.col.as<int32_t>(),
+                                               co.coef.as<double>(), co.area.as<double>(), co.cax.as<uint8_t>());
+    k_sc_diag<<<gridFor(nc), kBlock, 0, s>>>(nc, co.mask.as<uint8_t>(), co.rowPtr.as<int32_t>(), co.coef.as<double>(),
+                                             co.diag.as<double>());
+    NBL_LAUNCH_CHECK();
+    {
+      std::vector<uint8_t> hm(co.n);
+      NBL_HIP(hipMemcpyAsync(hm.data(), co.mask.as<void>(), co.n, hipMemcpyDeviceToHost, s));
+      NBL_HIP(hipStreamSynchronize(s));
+      co.nf = 0;
+      for (uint8_t v : hm) co.nf += v == 0;
+    }
+    buildOctSlots(ctx_, co, s);
+    for (int a = 0; a < 3; ++a) C.S[a] = T[a];
     lv_.push_back(std::move(C));
     lv_.back().op = &lv_.back().own;
-    top = clip;
   }
 }
 
@@ -332,9 +570,8 @@ void OctMg::cycle(size_t l, const double* b, RedBuf rb, const SolveCtl* ctl, hip
   k_om_sweep<OM_PLAIN, false><<<omGrid(n), kBlock, 0, s>>>(V, b, xa, nullptr, nullptr, xb, rb, ctl);
   Lvl& C = lv_[l + 1];
   const OctView VC = octView(*C.op);
-  const int rgrid = int((VC.n + kRestrictCoarse - 1) / kRestrictCoarse);
-  k_om_restrict<<<rgrid, kBlock, 0, s>>>(V, VC, C.cstart.as<int32_t>(), b, xb, C.b.as<double>(),
-                                         C.xa.as<double>(), ctl);
+  k_om_restrict<<<gridFor(VC.n), kBlock, 0, s>>>(V, VC, C.cstart.as<int32_t>(), C.child.as<int32_t>(), b, xb,
+                                                 C.b.as<double>(), C.xa.as<double>(), ctl);
   NBL_LAUNCH_CHECK();
   const double* ec = nullptr;
   cycle(l + 1, C.b.as<double>(), rb, ctl, s, &ec);

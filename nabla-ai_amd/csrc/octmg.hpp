@@ -1,16 +1,14 @@
 // octmg.hpp — geometric multigrid on octree leaves.
 //
-// Level 0 is the tree itself.  Level l+1 clips every leaf of level l that is
-// finer than (finest level present in l) - 1 to its ancestor at that depth,
-// so each coarse cell is either one leaf or the eight children of a split
-// cell, contiguous in Morton order.  A clipped tree is again a 2:1-balanced
-// octree, and its operator is the reference's face-coefficient discretisation
-// (area of the finer cell / centre distance) rebuilt on the coarse leaves; a
-// coarse cell is fluid iff any child is fluid.  Restriction sums the children's
-// residuals (the operator is volume-integrated), prolongation injects.  On a
-// uniform all-fluid tree this is exactly the uniform solver's hierarchy with
-// wall boundaries: cell-volume scaling cancels between the restricted residual
-// and the coarse operator.
+// Level 0 is the tree itself.  Each coarser level merges cells into boxes of a
+// target size that grows first along the axes where the cells are physically
+// smallest (semi-coarsening for anisotropic boxes, plain octree coarsening
+// for cubes); the coarse operator is the two-point finite-volume one (summed
+// face areas / centre distance), a coarse cell is fluid iff any child is
+// fluid, residuals restrict by summation and corrections prolong by
+// injection.  On a uniform all-fluid tree in a cube this is exactly the
+// uniform solver's hierarchy with wall boundaries: cell-volume scaling
+// cancels between the restricted residual and the coarse operator.
 #pragma once
 
 #include <vector>
@@ -41,8 +39,10 @@ class OctMg {
     OctLevel own;               // coarse levels
     const OctLevel* op = nullptr;
     DevBuf up;                  // i32[n]: parent in level l+1
-    DevBuf cstart;              // i32[n+1]: first child in level l-1
+    DevBuf cstart;              // i32[n+1]: children of cell I at child[cstart[I] .. cstart[I+1])
+    DevBuf child;               // i32[n of level l-1]: cells of level l-1 grouped by parent
     DevBuf b, xa, xb;
+    int S[3] = {0, 0, 0};       // target log2 box size per axis of this level
   };
   void buildCoarse(const OctreeDev& fine, hipStream_t s);
   bool buildDirect(hipStream_t s);

@@ -144,10 +144,13 @@ __global__ void k_row_count(long long n, const uint8_t* __restrict__ mask, const
   rowCount[i] = c;
 }
 
+// CSR rows plus, per coupling, the shared face area (the finer cell's
+// transverse area) and the axis, which the multigrid coarsening sums.
 __global__ void k_row_fill(long long n, Geo G, const uint64_t* __restrict__ keys, const uint8_t* __restrict__ lev,
                            const uint8_t* __restrict__ mask, const uint8_t* __restrict__ cnt,
                            const int32_t* __restrict__ nbr, const int32_t* __restrict__ rowPtr,
-                           int32_t* __restrict__ col, double* __restrict__ coef, double* __restrict__ diag) {
+                           int32_t* __restrict__ col, double* __restrict__ coef, double* __restrict__ area,
+                           uint8_t* __restrict__ cax, double* __restrict__ diag) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (mask[i] != 0) {
@@ -160,13 +163,24 @@ __global__ void k_row_fill(long long n, Geo G, const uint64_t* __restrict__ keys
     for (int m = 0; m < cnt[6 * i + dd]; ++m) {
       const int j = nbr[24 * i + 4 * dd + m];
       if (mask[j] != 0) continue;
-      const double c = faceCoeff(G, keys[i], lev[i], keys[j], lev[j], dd >> 1);
+      const int ax = dd >> 1, t1 = (ax + 1) % 3, t2 = (ax + 2) % 3;
+      const int lf = lev[j] > lev[i] ? lev[j] : lev[i];
+      const double c = faceCoeff(G, keys[i], lev[i], keys[j], lev[j], ax);
       col[e] = j;
       coef[e] = c;
+      area[e] = sizeAxis(G, lf, t1) * sizeAxis(G, lf, t2);
+      cax[e] = uint8_t(ax);
       ++e;
       d += c;
     }
   diag[i] = d > 0.0 ? d : 1.0;
+}
+
+__global__ void k_lsz_from_level(long long n, int maxL, const uint8_t* __restrict__ lev, uint32_t* __restrict__ lsz) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t l = uint32_t(maxL - lev[i]);
+  lsz[i] = l | (l << 8) | (l << 16);
 }
 
 struct XRd {  // x, or x - m with a pending mean
@@ -518,6 +532,35 @@ int ogrid(long long n) { return gridFor(n, kBlock, kOctBlocks); }
 
 }  // namespace
 
+void buildOctSlots(nbl_ctx* ctx, OctLevel& L, hipStream_t s) {
+  const size_t n = L.n;
+  const long long N = (long long)n;
+  L.cnt = DevBuf(ctx, n);
+  L.scol = DevBuf(ctx, size_t(kOctSlots) * n * sizeof(int32_t));
+  L.scoef = DevBuf(ctx, size_t(kOctSlots) * n * sizeof(double));
+  L.xrow = DevBuf(ctx, (n + 1) * sizeof(int32_t));
+  DevBuf xcount(ctx, (n + 1) * sizeof(int32_t));
+  NBL_HIP(hipMemsetAsync(xcount.as<int32_t>() + n, 0, sizeof(int32_t), s));
+  k_slot_count<<<gridFor(N), kBlock, 0, s>>>(N, L.rowPtr.as<int32_t>(), L.cnt.as<uint8_t>(), xcount.as<int32_t>());
+  NBL_LAUNCH_CHECK();
+  size_t tb = 0;
+  NBL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, xcount.as<int32_t>(), L.xrow.as<int32_t>(), int(n + 1), s));
+  DevBuf tmp(ctx, tb);
+  NBL_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.as<void>(), tb, xcount.as<int32_t>(), L.xrow.as<int32_t>(),
+                                           int(n + 1), s));
+  int32_t nxtra = 0;
+  NBL_HIP(hipMemcpyAsync(&nxtra, L.xrow.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  NBL_HIP(hipStreamSynchronize(s));
+  L.nx = size_t(nxtra);
+  L.xcol = DevBuf(ctx, std::max<size_t>(1, L.nx) * sizeof(int32_t));
+  L.xcoef = DevBuf(ctx, std::max<size_t>(1, L.nx) * sizeof(double));
+  k_slot_fill<<<gridFor(N), kBlock, 0, s>>>(N, L.rowPtr.as<int32_t>(), L.col.as<int32_t>(), L.coef.as<double>(),
+                                            L.xrow.as<int32_t>(), L.scol.as<int32_t>(), L.scoef.as<double>(),
+                                            L.xcol.as<int32_t>(), L.xcoef.as<double>());
+  NBL_LAUNCH_CHECK();
+  NBL_HIP(hipStreamSynchronize(s));
+}
+
 void buildOctOperator(nbl_ctx* ctx, int maxL, const double org[3], const double fine[3], OctLevel& L,
                       DevBuf* cntOut, DevBuf* nbrOut, hipStream_t s) {
   const size_t n = L.n;
@@ -557,30 +600,16 @@ void buildOctOperator(nbl_ctx* ctx, int maxL, const double org[3], const double 
     G.fine[a] = fine[a];
   }
   G.maxL = maxL;
+  L.area = DevBuf(ctx, std::max<size_t>(1, L.nnz) * sizeof(double));
+  L.cax = DevBuf(ctx, std::max<size_t>(1, L.nnz));
+  L.lsz = DevBuf(ctx, n * sizeof(uint32_t));
   k_row_fill<<<gridFor(N), kBlock, 0, s>>>(N, G, L.keys.as<uint64_t>(), L.lev.as<uint8_t>(), L.mask.as<uint8_t>(),
                                            cnt.as<uint8_t>(), nbr.as<int32_t>(), L.rowPtr.as<int32_t>(),
-                                           L.col.as<int32_t>(), L.coef.as<double>(), L.diag.as<double>());
+                                           L.col.as<int32_t>(), L.coef.as<double>(), L.area.as<double>(),
+                                           L.cax.as<uint8_t>(), L.diag.as<double>());
+  k_lsz_from_level<<<gridFor(N), kBlock, 0, s>>>(N, maxL, L.lev.as<uint8_t>(), L.lsz.as<uint32_t>());
   NBL_LAUNCH_CHECK();
-  // slot-major operator for the solve kernels
-  L.cnt = DevBuf(ctx, n);
-  L.scol = DevBuf(ctx, size_t(kOctSlots) * n * sizeof(int32_t));
-  L.scoef = DevBuf(ctx, size_t(kOctSlots) * n * sizeof(double));
-  L.xrow = DevBuf(ctx, (n + 1) * sizeof(int32_t));
-  k_slot_count<<<gridFor(N), kBlock, 0, s>>>(N, L.rowPtr.as<int32_t>(), L.cnt.as<uint8_t>(), rowCount.as<int32_t>());
-  NBL_HIP(hipMemsetAsync(rowCount.as<int32_t>() + n, 0, sizeof(int32_t), s));
-  NBL_LAUNCH_CHECK();
-  NBL_HIP(hipcub::DeviceScan::ExclusiveSum(scanTmp.as<void>(), scanBytes, rowCount.as<int32_t>(),
-                                           L.xrow.as<int32_t>(), int(n + 1), s));
-  int32_t nxtra = 0;
-  NBL_HIP(hipMemcpyAsync(&nxtra, L.xrow.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  NBL_HIP(hipStreamSynchronize(s));
-  L.nx = size_t(nxtra);
-  L.xcol = DevBuf(ctx, std::max<size_t>(1, L.nx) * sizeof(int32_t));
-  L.xcoef = DevBuf(ctx, std::max<size_t>(1, L.nx) * sizeof(double));
-  k_slot_fill<<<gridFor(N), kBlock, 0, s>>>(N, L.rowPtr.as<int32_t>(), L.col.as<int32_t>(), L.coef.as<double>(),
-                                            L.xrow.as<int32_t>(), L.scol.as<int32_t>(), L.scoef.as<double>(),
-                                            L.xcol.as<int32_t>(), L.xcoef.as<double>());
-  NBL_LAUNCH_CHECK();
+  buildOctSlots(ctx, L, s);
   if (cntOut) *cntOut = std::move(cnt);
   if (nbrOut) *nbrOut = std::move(nbr);
   // a freed face table synchronises the device; the fill above has then completed

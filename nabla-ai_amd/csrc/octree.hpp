@@ -17,6 +17,10 @@ namespace nbl {
 struct OctLevel {
   size_t n = 0, nnz = 0, nf = 0, nx = 0;
   DevBuf keys, lev, mask, rowPtr, col, coef, diag;
+  // Cell boxes in finest-index units: keys = Morton code of the anchor, lsz =
+  // log2 size per axis (x | y << 8 | z << 16; octree leaves are cubes).  Per
+  // coupling: the shared face area and the axis (u8).
+  DevBuf lsz, area, cax;
   // The same couplings for the operator kernels: the first kOctSlots of every
   // row in slot-major arrays (slot s of row i at s*n + i, so a wavefront's
   // reads of one slot coalesce), the rest in an overflow CSR.
@@ -24,6 +28,30 @@ struct OctLevel {
 };
 
 constexpr int kOctSlots = 6;
+
+// Morton code with 21 bits per axis: x at bit 3i, y at 3i+1, z at 3i+2
+// (morton.hpp:15-48).
+__host__ __device__ inline uint64_t mortonSpread(uint32_t a) {
+  uint64_t x = a & 0x1fffffull;
+  x = (x | (x << 32)) & 0x001f00000000ffffull;
+  x = (x | (x << 16)) & 0x001f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+__host__ __device__ inline uint32_t mortonAxis(uint64_t key, int axis) {
+  uint64_t x = (key >> axis) & 0x1249249249249249ull;
+  x = (x ^ (x >> 2)) & 0x10c30c30c30c30c3ull;
+  x = (x ^ (x >> 4)) & 0x100f00f00f00f00full;
+  x = (x ^ (x >> 8)) & 0x001f0000ff0000ffull;
+  x = (x ^ (x >> 16)) & 0x001f00000000ffffull;
+  x = (x ^ (x >> 32)) & 0x1fffffull;
+  return uint32_t(x);
+}
+__host__ __device__ inline uint64_t mortonEnc3(uint32_t x, uint32_t y, uint32_t z) {
+  return mortonSpread(x) | (mortonSpread(y) << 1) | (mortonSpread(z) << 2);
+}
 
 // Device view of a level's operator.
 struct OctView {
@@ -66,6 +94,8 @@ __device__ __forceinline__ double octRowA(const OctView& L, long long i, const R
 // table is returned through cnt/nbr when they are non-null, else freed.
 void buildOctOperator(nbl_ctx* ctx, int maxL, const double org[3], const double fine[3], OctLevel& L,
                       DevBuf* cnt, DevBuf* nbr, hipStream_t s);
+// Slot-major copy (cnt, scol, scoef, xrow, xcol, xcoef) of a level's CSR.
+void buildOctSlots(nbl_ctx* ctx, OctLevel& L, hipStream_t s);
 
 class OctMg;
 

@@ -7,6 +7,8 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <array>
+#include <map>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -1248,133 +1250,233 @@ struct Oct {
 
 
 // ---- octree multigrid (no reference counterpart) -----------------------------
-// Restates the device octree multigrid: level 0 = the leaves in Morton order;
-// level l+1 clips every leaf finer than (finest level in l) - 1 to its ancestor
-// at that depth; coarse cell fluid iff a child is fluid; operator rebuilt with
-// the reference face coefficient (octree_poisson.cpp:24-32, 64-105); residuals
-// restrict by summation in child order, corrections prolong by injection;
-// V(2,2) damped Jacobi (omega 0.8, multigrid.cpp:102-114); coarsest level
-// (<= 2048 leaves) solved exactly through (A + alpha 11^T) with the result
-// projected onto mean-free vectors; outer loop = MgPoisson::solve's flexible
-// CG in its singular form (multigrid.cpp:267-355).
+// Restates the device octree multigrid.  Cells of every level are boxes in
+// finest-index units (anchor, log2 size per axis); level 0 = the leaves in
+// Morton order.  A coarsening step raises the target log size S per axis:
+// from S = the finest leaf size, the axes whose physical target size is at
+// most 1.5x the smallest grow by one (semi-coarsening along the strongly
+// coupled axes of anisotropic cells).  Cells no larger than S on every axis
+// group by their S-aligned box; a group that covers its box exactly becomes
+// one coarse cell, every other cell stays as it is.  Coarse couplings sum the
+// fine face areas between two coarse cells (fluid-fluid faces only) and use
+// the two-point coefficient area / centre distance (on octree cells this is
+// the reference's face coefficient, octree_poisson.cpp:24-32); a coarse cell
+// is fluid iff a child is fluid.  Residuals restrict by summation, corrections
+// prolong by injection; V(2,2) damped Jacobi (omega 0.8, multigrid.cpp:
+// 102-114); coarsest level (<= 2048 cells) solved exactly through
+// (A + alpha 11^T) with the result projected onto mean-free vectors; outer
+// loop = MgPoisson::solve's flexible CG in its singular form
+// (multigrid.cpp:267-355).
 struct OctMgOrc {
+  struct Cpl {
+    size_t j;
+    double coef, area;
+    int axis;
+  };
   struct Lvl {
-    std::unique_ptr<Oct> t;
-    std::vector<std::vector<std::pair<size_t, double>>> rows;  // fluid couplings, reference order
+    std::vector<uint32_t> a[3];
+    std::vector<int> l[3];
+    std::vector<uint8_t> fl;
+    std::vector<std::vector<Cpl>> rows;
     Vec diag;
     std::vector<size_t> up;  // parent in level l+1
-    size_t nf = 0;
+    size_t n = 0, nf = 0;
+    int S[3] = {0, 0, 0};
+    bool fluid(size_t i) const { return fl[i] != 0; }
   };
   std::vector<Lvl> L;
   std::vector<size_t> order;  // Morton position -> caller index
+  double org[3], fine[3];
+  int maxL = 0;
   bool direct = false;
   std::vector<double> lu;     // LU of the regularised coarsest matrix (row-major, no pivoting needed: SPD)
   static constexpr double kOm = 0.8;
 
-  static void fillOp(Lvl& v) {
-    const Oct& t = *v.t;
-    const size_t n = t.key.size();
-    v.rows.assign(n, {});
-    v.diag.assign(n, 1.0);
+  double centre(const Lvl& v, size_t i, int ax) const {
+    return org[ax] + (double(v.a[ax][i]) + 0.5 * double(uint64_t(1) << v.l[ax][i])) * fine[ax];
+  }
+  static void finishRows(Lvl& v) {
+    v.diag.assign(v.n, 1.0);
     v.nf = 0;
-    for (size_t i = 0; i < n; ++i) {
-      if (!t.fluid(i)) continue;
+    for (size_t i = 0; i < v.n; ++i) {
+      if (!v.fluid(i)) continue;
       ++v.nf;
       double d = 0.0;
-      for (int dd = 0; dd < 6; ++dd)
-        for (size_t j : t.nbrs(i, dd)) {
-          if (!t.fluid(j)) continue;
-          const double c = t.coeff(i, j, dd / 2);
-          v.rows[i].push_back({j, c});
-          d += c;
-        }
+      for (const Cpl& c : v.rows[i]) d += c.coef;
       v.diag[i] = d > 0.0 ? d : 1.0;
     }
   }
   static double rowA(const Lvl& v, size_t i, const Vec& x) {
     double acc = 0.0;
-    for (const auto& [j, c] : v.rows[i]) acc += c * (x[i] - x[j]);
+    for (const Cpl& c : v.rows[i]) acc += c.coef * (x[i] - x[c.j]);
     return acc;
   }
-  static std::unique_ptr<Oct> fromLeaves(const Oct& like, const std::vector<uint64_t>& k,
-                                         const std::vector<uint8_t>& l, const std::vector<uint8_t>& m) {
-    auto t = std::make_unique<Oct>(like.org, like.ext, like.maxL);
-    t->key.clear();
-    t->lev.clear();
-    t->msk.clear();
-    t->at.clear();
-    for (size_t i = 0; i < k.size(); ++i) t->push(k[i], l[i], m[i]);
-    return t;
-  }
 
-  explicit OctMgOrc(const Oct& o) {
+  explicit OctMgOrc(const Oct& o) : maxL(o.maxL) {
+    for (int q = 0; q < 3; ++q) {
+      org[q] = o.org[q];
+      fine[q] = o.fine[q];
+    }
     const size_t n0 = o.key.size();
     order.resize(n0);
     for (size_t i = 0; i < n0; ++i) order[i] = i;
     std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return o.key[a] < o.key[b]; });
-    std::vector<uint64_t> k(n0);
-    std::vector<uint8_t> lv(n0), m(n0);
-    int top = 0;
-    for (size_t q = 0; q < n0; ++q) {
-      k[q] = o.key[order[q]];
-      lv[q] = o.lev[order[q]];
-      m[q] = o.msk[order[q]];
-      top = std::max(top, int(lv[q]));
-    }
+    std::vector<size_t> pos(n0);
+    for (size_t q = 0; q < n0; ++q) pos[order[q]] = q;
+    // level 0: the leaves in Morton order, couplings in the reference's order
     L.emplace_back();
-    L[0].t = fromLeaves(o, k, lv, m);
-    fillOp(L[0]);
-    while (L.back().t->key.size() > 2048 && L.size() < 16 && top > 0) {
-      const int clip = top - 1;
-      const Oct& f = *L.back().t;
-      const size_t n = f.key.size();
-      std::vector<uint64_t> ck;
-      std::vector<uint8_t> cl, cm;
-      std::vector<size_t> up(n);
-      for (size_t i = 0; i < n; ++i) {
-        uint64_t key = f.key[i];
-        int lev = f.lev[i];
-        if (lev > clip) {
-          key &= ~((uint64_t(1) << (3 * (f.maxL - clip))) - 1);
-          lev = clip;
-        }
-        if (ck.empty() || ck.back() != key) {
-          ck.push_back(key);
-          cl.push_back(uint8_t(lev));
-          cm.push_back(2);
-        }
-        up[i] = ck.size() - 1;
-        if (f.msk[i] == 0) cm.back() = 0;
-      }
-      if (ck.size() >= n) break;
-      L.back().up = up;
-      Lvl c;
-      c.t = fromLeaves(f, ck, cl, cm);
-      fillOp(c);
-      L.push_back(std::move(c));
-      top = clip;
+    Lvl& z = L[0];
+    z.n = n0;
+    int lmin = 1 << 30;
+    for (int q = 0; q < 3; ++q) {
+      z.a[q].resize(n0);
+      z.l[q].resize(n0);
     }
-    direct = L.back().t->key.size() <= 2048 && factor();
+    z.fl.resize(n0);
+    z.rows.assign(n0, {});
+    for (size_t q = 0; q < n0; ++q) {
+      const size_t i = order[q];
+      const uint64_t k = o.key[i];
+      const uint32_t an[3] = {gather(k), gather(k >> 1), gather(k >> 2)};
+      for (int ax = 0; ax < 3; ++ax) {
+        z.a[ax][q] = an[ax];
+        z.l[ax][q] = o.maxL - o.lev[i];
+      }
+      lmin = std::min(lmin, o.maxL - int(o.lev[i]));
+      z.fl[q] = o.fluid(i) ? 1 : 0;
+      if (!o.fluid(i)) continue;
+      for (int dd = 0; dd < 6; ++dd)
+        for (size_t j : o.nbrs(i, dd)) {
+          if (!o.fluid(j)) continue;
+          const int ax = dd / 2, t1 = (ax + 1) % 3, t2 = (ax + 2) % 3;
+          const size_t f = o.lev[j] > o.lev[i] ? j : i;
+          z.rows[q].push_back({pos[j], o.coeff(i, j, ax), o.size(f, t1) * o.size(f, t2), ax});
+        }
+    }
+    finishRows(z);
+    for (int ax = 0; ax < 3; ++ax) z.S[ax] = lmin;
+    while (L.back().n > 2048 && L.size() < 16) {
+      if (!coarsen()) break;
+    }
+    direct = L.back().n <= 2048 && factor();
+  }
+
+  // One coarsening step from the last level; false when nothing merges up to
+  // the root size.
+  bool coarsen() {
+    const Lvl& f = L.back();
+    int S[3] = {f.S[0], f.S[1], f.S[2]};
+    for (;;) {
+      if (S[0] >= maxL && S[1] >= maxL && S[2] >= maxL) return false;
+      double h[3], hmin = 1e300;
+      for (int ax = 0; ax < 3; ++ax) {
+        h[ax] = fine[ax] * double(uint64_t(1) << S[ax]);
+        hmin = std::min(hmin, h[ax]);
+      }
+      for (int ax = 0; ax < 3; ++ax)
+        if (h[ax] <= 1.5 * hmin && S[ax] < maxL) ++S[ax];
+      // group the cells no larger than S by their S-aligned box
+      std::map<std::array<uint32_t, 3>, std::pair<uint64_t, std::vector<size_t>>> grp;
+      for (size_t i = 0; i < f.n; ++i) {
+        if (f.l[0][i] > S[0] || f.l[1][i] > S[1] || f.l[2][i] > S[2]) continue;
+        std::array<uint32_t, 3> t;
+        for (int ax = 0; ax < 3; ++ax) t[ax] = f.a[ax][i] & ~((uint32_t(1) << S[ax]) - 1u);
+        auto& g = grp[t];
+        g.first += uint64_t(1) << (f.l[0][i] + f.l[1][i] + f.l[2][i]);
+        g.second.push_back(i);
+      }
+      const uint64_t full = uint64_t(1) << (S[0] + S[1] + S[2]);
+      // coarse cells keyed by anchor (Morton order)
+      std::vector<std::array<uint32_t, 3>> ca;
+      std::vector<std::array<int, 3>> cl;
+      std::vector<size_t> tag(f.n, SIZE_MAX);
+      std::map<uint64_t, size_t> byKey;
+      auto add = [&](const std::array<uint32_t, 3>& an, const std::array<int, 3>& lg) {
+        const uint64_t k = menc(an[0], an[1], an[2]);
+        auto it = byKey.find(k);
+        if (it != byKey.end()) return it->second;
+        ca.push_back(an);
+        cl.push_back(lg);
+        byKey[k] = ca.size() - 1;
+        return ca.size() - 1;
+      };
+      for (auto& [t, g] : grp)
+        if (g.first == full && g.second.size() > 1) {
+          const size_t c = add(t, {S[0], S[1], S[2]});
+          for (size_t i : g.second) tag[i] = c;
+        }
+      for (size_t i = 0; i < f.n; ++i)
+        if (tag[i] == SIZE_MAX) tag[i] = add({f.a[0][i], f.a[1][i], f.a[2][i]}, {f.l[0][i], f.l[1][i], f.l[2][i]});
+      if (ca.size() == f.n) {
+        if (S[0] >= maxL && S[1] >= maxL && S[2] >= maxL) return false;
+        continue;  // nothing merged at this size: grow the target further
+      }
+      // renumber coarse cells in Morton order of their anchors
+      std::vector<size_t> rank(ca.size());
+      {
+        size_t r = 0;
+        for (auto& [k, c] : byKey) rank[c] = r++;
+      }
+      Lvl c;
+      c.n = ca.size();
+      for (int ax = 0; ax < 3; ++ax) {
+        c.a[ax].resize(c.n);
+        c.l[ax].resize(c.n);
+        c.S[ax] = S[ax];
+      }
+      c.fl.assign(c.n, 0);
+      for (size_t q = 0; q < ca.size(); ++q)
+        for (int ax = 0; ax < 3; ++ax) {
+          c.a[ax][rank[q]] = ca[q][ax];
+          c.l[ax][rank[q]] = cl[q][ax];
+        }
+      std::vector<size_t> up(f.n);
+      for (size_t i = 0; i < f.n; ++i) {
+        up[i] = rank[tag[i]];
+        if (f.fluid(i)) c.fl[up[i]] = 1;
+      }
+      // couplings: fine face areas summed per coarse pair in fine coupling order
+      std::map<std::pair<size_t, size_t>, std::pair<double, int>> acc;
+      for (size_t i = 0; i < f.n; ++i)
+        for (const Cpl& e : f.rows[i]) {
+          const size_t I = up[i], J = up[e.j];
+          if (I == J) continue;
+          auto it = acc.find({I, J});
+          if (it == acc.end()) acc[{I, J}] = {e.area, e.axis};
+          else it->second.first += e.area;
+        }
+      c.rows.assign(c.n, {});
+      for (auto& [ij, v] : acc) {
+        const auto [I, J] = ij;
+        const int ax = v.second;
+        const double dist = std::abs(centre(c, I, ax) - centre(c, J, ax));
+        c.rows[I].push_back({J, dist > 0.0 ? v.first / dist : 0.0, v.first, ax});
+      }
+      finishRows(c);
+      L.back().up = up;
+      L.push_back(std::move(c));
+      return true;
+    }
   }
 
   // LU (Doolittle, no pivoting) of the coarsest matrix: fluid rows A + alpha
   // on the fluid block, other rows identity.
   bool factor() {
     const Lvl& v = L.back();
-    const size_t n = v.t->key.size();
+    const size_t n = v.n;
     lu.assign(n * n, 0.0);
     double dfl = 0.0;
     for (size_t i = 0; i < n; ++i) {
       lu[i * n + i] = v.diag[i];
-      if (!v.t->fluid(i)) continue;
-      for (const auto& [j, c] : v.rows[i]) lu[i * n + j] -= c;
+      if (!v.fluid(i)) continue;
+      for (const Cpl& c : v.rows[i]) lu[i * n + c.j] -= c.coef;
       dfl += v.diag[i];
     }
     if (v.nf > 0) {
       const double alpha = dfl / double(v.nf) / double(v.nf);
       for (size_t i = 0; i < n; ++i)
         for (size_t j = 0; j < n; ++j)
-          if (v.t->fluid(i) && v.t->fluid(j)) lu[i * n + j] += alpha;
+          if (v.fluid(i) && v.fluid(j)) lu[i * n + j] += alpha;
     }
     for (size_t kk = 0; kk < n; ++kk) {
       const double piv = lu[kk * n + kk];
@@ -1392,16 +1494,16 @@ struct OctMgOrc {
     if (v.nf == 0) return;
     double m = 0.0;
     for (size_t c = 0; c < a.size(); ++c) {
-      if (v.t->fluid(c)) m += a[c];
+      if (v.fluid(c)) m += a[c];
       else a[c] = 0.0;
     }
     m /= double(v.nf);
     for (size_t c = 0; c < a.size(); ++c)
-      if (v.t->fluid(c)) a[c] -= m;
+      if (v.fluid(c)) a[c] -= m;
   }
   void coarsest(const Vec& b, Vec& x) const {
     const Lvl& v = L.back();
-    const size_t n = v.t->key.size();
+    const size_t n = v.n;
     if (direct) {
       Vec y = b;
       project(v, y);
@@ -1422,34 +1524,33 @@ struct OctMgOrc {
     const size_t n = x.size();
     Vec xn(n, 0.0);
     for (size_t i = 0; i < n; ++i)
-      if (v.t->fluid(i)) xn[i] = x[i] + kOm * (b[i] - rowA(v, i, x)) / v.diag[i];
+      if (v.fluid(i)) xn[i] = x[i] + kOm * (b[i] - rowA(v, i, x)) / v.diag[i];
     x.swap(xn);
   }
   void cycle(size_t l, const Vec& b, Vec& x) const {
     const Lvl& v = L[l];
-    const size_t n = v.t->key.size();
     if (l + 1 == L.size()) {
       coarsest(b, x);
       return;
     }
-    x.assign(n, 0.0);
+    x.assign(v.n, 0.0);
     sweep(v, b, x);
     sweep(v, b, x);
     const Lvl& c = L[l + 1];
-    Vec bc(c.t->key.size(), 0.0), ec;
-    for (size_t i = 0; i < n; ++i)
-      if (v.t->fluid(i)) bc[v.up[i]] += b[i] - rowA(v, i, x);
+    Vec bc(c.n, 0.0), ec;
+    for (size_t i = 0; i < v.n; ++i)
+      if (v.fluid(i)) bc[v.up[i]] += b[i] - rowA(v, i, x);
     cycle(l + 1, bc, ec);
-    for (size_t i = 0; i < n; ++i) x[i] += ec[v.up[i]];
+    for (size_t i = 0; i < v.n; ++i) x[i] += ec[v.up[i]];
     sweep(v, b, x);
     sweep(v, b, x);
   }
 
   nbl_poisson_report solve(const Vec& bIn, Vec& xOut, double tol, int maxIt) const {
     const Lvl& v = L[0];
-    const size_t n = v.t->key.size();
+    const size_t n = v.n;
     Vec rhs(n), x(n, 0.0);
-    for (size_t q = 0; q < n; ++q) rhs[q] = v.t->fluid(q) ? bIn[order[q]] : 0.0;
+    for (size_t q = 0; q < n; ++q) rhs[q] = v.fluid(q) ? bIn[order[q]] : 0.0;
     project(v, rhs);
     const double bn = std::sqrt(dotv(rhs, rhs)) + 1e-300;
     nbl_poisson_report rep{0, 0.0, 0};
@@ -1470,7 +1571,7 @@ struct OctMgOrc {
     double rz = dotv(r, z);
     int it = 0;
     for (; it < maxIt; ++it) {
-      for (size_t i = 0; i < n; ++i) Ap[i] = v.t->fluid(i) ? rowA(v, i, p) : p[i];
+      for (size_t i = 0; i < n; ++i) Ap[i] = v.fluid(i) ? rowA(v, i, p) : p[i];
       const double pAp = dotv(p, Ap);
       if (std::abs(pAp) < 1e-300) break;
       const double al = rz / pAp;
@@ -1499,7 +1600,7 @@ struct OctMgOrc {
     project(v, x);
     double rr = 0.0;
     for (size_t i = 0; i < n; ++i) {
-      const double ax = v.t->fluid(i) ? rowA(v, i, x) : x[i];
+      const double ax = v.fluid(i) ? rowA(v, i, x) : x[i];
       const double e = rhs[i] - ax;
       rr += e * e;
     }
@@ -1756,7 +1857,7 @@ void orc_oct_mg_solve(void* h, const double* b, double* x, double tol, int max_i
 }
 int orc_oct_mg_levels(void* h, long long* counts, int cap) {
   const OctMgOrc mg(*static_cast<Oct*>(h));
-  for (int l = 0; l < std::min(cap, int(mg.L.size())); ++l) counts[l] = (long long)mg.L[size_t(l)].t->key.size();
+  for (int l = 0; l < std::min(cap, int(mg.L.size())); ++l) counts[l] = (long long)mg.L[size_t(l)].n;
   return int(mg.L.size());
 }
 

@@ -28,3 +28,51 @@ def uniform_octree(orc, levels, max_level=6):
     t = orc.octree((0, 0, 0), (1, 1, 1), max_level)
     t.refine_uniform(levels)
     return t
+
+
+def _morton_axis(k, axis):
+    x = (k >> np.uint64(axis)) & np.uint64(0x1249249249249249)
+    for sh, m in ((2, 0x10c30c30c30c30c3), (4, 0x100f00f00f00f00f), (8, 0x001f0000ff0000ff),
+                  (16, 0x001f00000000ffff), (32, 0x1fffff)):
+        x = (x ^ (x >> np.uint64(sh))) & np.uint64(m)
+    return x
+
+
+def mg_level_counts(keys, levels, max_level, extent, coarsest=2048, max_levels=16):
+    """Cells per level of the octree multigrid hierarchy, restated with numpy:
+    the target log2 box size S grows along the axes whose physical size is
+    within 1.5x of the smallest; cells no larger than S group by their
+    S-aligned box and a group that fills its box merges."""
+    keys = np.asarray(keys, np.uint64)
+    n = keys.size
+    a = np.stack([_morton_axis(keys, q).astype(np.int64) for q in range(3)], axis=1)
+    lg = np.repeat((max_level - np.asarray(levels, np.int64))[:, None], 3, axis=1)
+    fine = np.asarray(extent, float) / float(1 << max_level)
+    S = np.full(3, lg.min(), np.int64)
+    counts = [n]
+    while counts[-1] > coarsest and len(counts) < max_levels:
+        merged = False
+        while not (S >= max_level).all():
+            h = fine * (2.0 ** S)
+            S = np.where((h <= 1.5 * h.min()) & (S < max_level), S + 1, S)
+            part = (lg <= S).all(axis=1)
+            t = a & ~((np.int64(1) << S) - 1)
+            tk = (t[:, 0] << 42) | (t[:, 1] << 21) | t[:, 2]
+            tk = np.where(part, tk, -1)
+            u, inv, cnt = np.unique(tk, return_inverse=True, return_counts=True)
+            vol = np.bincount(inv, weights=np.where(part, 2.0 ** lg.sum(axis=1), 0.0))
+            comp = (u >= 0) & (cnt > 1) & (vol == 2.0 ** S.sum())
+            if not comp.any():
+                continue
+            cm = comp[inv]
+            a = np.where(cm[:, None], t, a)
+            lg = np.where(cm[:, None], S[None, :], lg)
+            key = (a[:, 0] << 42) | (a[:, 1] << 21) | a[:, 2]
+            uk, first = np.unique(key, return_index=True)
+            a, lg = a[first], lg[first]
+            counts.append(uk.size)
+            merged = True
+            break
+        if not merged:
+            break
+    return counts
