@@ -357,6 +357,10 @@ int nbl_poisson_solve_batch(nbl_poisson* const* ps, const double* const* b, doub
 /* The same for octree Jacobi-PCG solves (as nbl_octree_poisson_solve each). */
 int nbl_octree_solve_batch(nbl_octree* const* os, const double* const* b, double* const* x, int n,
                            int concurrency, double tol, int max_iters, nbl_poisson_report* reps);
+/* The same batch with nbl_octree_mg_solve (the hierarchy of each octree is
+ * built on its first multigrid solve). */
+int nbl_octree_mg_solve_batch(nbl_octree* const* os, const double* const* b, double* const* x, int n,
+                              int concurrency, double tol, int max_iters, nbl_poisson_report* reps);
 
 #ifdef __cplusplus
 }

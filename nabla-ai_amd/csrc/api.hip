@@ -635,7 +635,16 @@ int nbl_octree_solve_batch(nbl_octree* const* os, const double* const* b, double
   return guarded([&] {
     need(os && b && x && reps && n >= 0, "NULL argument");
     if (n > 0) useDevice(os[0]->ctx);
-    nbl::octreeSolveBatch(os, b, x, n, concurrency, tol, max_iters, reps);
+    nbl::octreeSolveBatch(os, b, x, n, concurrency, tol, max_iters, reps, false);
+  });
+}
+
+int nbl_octree_mg_solve_batch(nbl_octree* const* os, const double* const* b, double* const* x, int n,
+                              int concurrency, double tol, int max_iters, nbl_poisson_report* reps) {
+  return guarded([&] {
+    need(os && b && x && reps && n >= 0, "NULL argument");
+    if (n > 0) useDevice(os[0]->ctx);
+    nbl::octreeSolveBatch(os, b, x, n, concurrency, tol, max_iters, reps, true);
   });
 }
 

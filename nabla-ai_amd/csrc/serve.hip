@@ -186,9 +186,9 @@ void solveBatch(nbl_poisson* const* ps, const double* const* b, double* const* x
 }
 
 // Batched octree solves, as solveBatch: each octree's own stream, host vectors
-// in caller order.
+// in caller order; Jacobi-PCG or (mg) the multigrid-preconditioned CG.
 void octreeSolveBatch(nbl_octree* const* os, const double* const* b, double* const* x, int n, int concurrency,
-                      double tol, int maxIters, nbl_poisson_report* reps) {
+                      double tol, int maxIters, nbl_poisson_report* reps, bool mg) {
   if (n <= 0) return;
   if (concurrency < 1) throw std::invalid_argument("batch: concurrency must be >= 1");
   for (int q = 0; q < n; ++q)
@@ -206,7 +206,7 @@ void octreeSolveBatch(nbl_octree* const* os, const double* const* b, double* con
       }
       try {
         NBL_HIP(hipSetDevice(os[q]->ctx->device));
-        reps[q] = os[q]->impl->solve(b[q], x[q], tol, maxIters);
+        reps[q] = mg ? os[q]->impl->solveMg(b[q], x[q], tol, maxIters) : os[q]->impl->solve(b[q], x[q], tol, maxIters);
       } catch (const std::exception& e) {
         std::lock_guard<std::mutex> lk(mu);
         if (err.empty()) err = e.what();

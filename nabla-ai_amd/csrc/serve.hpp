@@ -73,7 +73,7 @@ void solveBatch(nbl_poisson* const* ps, const double* const* b, double* const* x
 struct nbl_octree;
 namespace nbl {
 void octreeSolveBatch(nbl_octree* const* os, const double* const* b, double* const* x, int n, int concurrency,
-                      double tol, int maxIters, nbl_poisson_report* reps);
+                      double tol, int maxIters, nbl_poisson_report* reps, bool mg);
 
 }  // namespace nbl
 

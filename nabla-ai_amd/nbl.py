@@ -237,6 +237,8 @@ def load(path: os.PathLike | None = None):
            C.POINTER(PoissonReport))
     _proto(lib, "nbl_octree_solve_batch", i, C.POINTER(V), C.POINTER(P_D), C.POINTER(P_D), i, i, d, i,
            C.POINTER(PoissonReport))
+    _proto(lib, "nbl_octree_mg_solve_batch", i, C.POINTER(V), C.POINTER(P_D), C.POINTER(P_D), i, i, d, i,
+           C.POINTER(PoissonReport))
     _lib = lib
     return lib
 
@@ -638,8 +640,9 @@ def solve_batch(solvers, rhs, concurrency=4):
     return xs, list(reps)
 
 
-def octree_solve_batch(octs, rhs, tol=1e-9, max_iters=2000, concurrency=4):
-    """nbl_octree_solve_batch over Octree objects and host vectors."""
+def octree_solve_batch(octs, rhs, tol=1e-9, max_iters=2000, concurrency=4, mg=False):
+    """nbl_octree_solve_batch (mg: nbl_octree_mg_solve_batch) over Octree
+    objects and host vectors."""
     n = len(octs)
     hs = (C.c_void_p * n)(*[o.h for o in octs])
     bs = [np.ascontiguousarray(b, dtype=np.float64) for b in rhs]
@@ -647,5 +650,6 @@ def octree_solve_batch(octs, rhs, tol=1e-9, max_iters=2000, concurrency=4):
     bp = (P_D * n)(*[dptr(b) for b in bs])
     xp = (P_D * n)(*[dptr(x) for x in xs])
     reps = (PoissonReport * n)()
-    check(load().nbl_octree_solve_batch(hs, bp, xp, n, concurrency, tol, max_iters, reps))
+    fn = load().nbl_octree_mg_solve_batch if mg else load().nbl_octree_solve_batch
+    check(fn(hs, bp, xp, n, concurrency, tol, max_iters, reps))
     return xs, list(reps)

@@ -129,3 +129,32 @@ def test_batch_equals_sequential(ctx):
     for (x0, r0), x1, r1 in zip(seq, xs, reps):
         assert r1.converged and r1.iterations == r0.iterations
         assert np.array_equal(x0, x1)
+
+
+@pytest.mark.parametrize("mg", [False, True], ids=["jacobi", "mg"])
+def test_octree_batch_equals_sequential(ctx, orc, mg):
+    """Batched octree solves (Jacobi-PCG or the multigrid CG, each octree on
+    its own stream) return the same solutions bit for bit as one solve after
+    another."""
+    from octree_trees import adaptive_octree, uniform_octree
+    trees = [uniform_octree(orc, 4), adaptive_octree(orc), uniform_octree(orc, 5)]
+    octs, rhs = [], []
+    for q, t in enumerate(trees):
+        k, l, m = t.dump()
+        octs.append(nbl.Octree(ctx, k, l, m, t.origin, t.extent, t.max_level))
+        fl = m == 0
+        b = cases.normal(300 + q, t.n)
+        b[~fl] = 0.0
+        b[fl] -= b[fl].mean()
+        rhs.append(b)
+    try:
+        solve = (lambda o, b: o.mg_solve(b, 1e-9, 300)) if mg else (lambda o, b: o.solve(b, 1e-9, 5000))
+        seq = [solve(o, b) for o, b in zip(octs, rhs)]
+        xs, reps = nbl.octree_solve_batch(octs, rhs, tol=1e-9, max_iters=300 if mg else 5000,
+                                          concurrency=2, mg=mg)
+    finally:
+        for o in octs:
+            o.close()
+    for (x0, r0), x1, r1 in zip(seq, xs, reps):
+        assert r1.converged and r1.iterations == r0.iterations
+        assert np.array_equal(x0, x1)
