@@ -138,6 +138,7 @@ def octree_c3(ctx, with_cpu=True):
     setup_ms, _ = d.last_timing()
     n, nnz = d.counts()
     ms, by = d.profile(0, 50)
+    tms, tby = d.profile(1, 50)
     fl = m == 0
     b = cases.normal(11, k.size)
     b[~fl] = 0.0
@@ -160,7 +161,10 @@ def octree_c3(ctx, with_cpu=True):
                       "rel_l2_vs_jacobi_pcg": rel(xm, xj)},
            "spmv_roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": round(ach / HBM_PEAK_GBS, 4), "avg_launch_us": round(ms * 1e3, 3),
-                             "bytes_per_launch": by}}
+                             "bytes_per_launch": by},
+           "transport_step": {"kernel": "k_oct_transport (octreeTransportStep, first-order upwind)",
+                              "avg_launch_us": round(tms * 1e3, 3), "mleaves_per_s": round(n / tms / 1e3, 1),
+                              "achieved_gbs": round(tby / tms / 1e6, 1), "bytes_per_launch": tby}}
     log(f"octree: {n} leaves, setup {setup_ms:.1f} ms, Jacobi-PCG {rep.iterations} iterations in {solve_ms:.1f} ms, "
         f"MG-PCG {mrep.iterations} iterations in {mg_ms:.1f} ms")
     if with_cpu:
@@ -185,12 +189,17 @@ def octree_c3(ctx, with_cpu=True):
         t1 = time.perf_counter()
         rt.apply(xr)
         apply_s = time.perf_counter() - t1
+        z = np.zeros(rt.n)
+        t1 = time.perf_counter()
+        rt.transport(xr, xr, z, xr, 1e-4)
+        transport_s = time.perf_counter() - t1
         rt.close()
         out["cpu_baseline"] = {"kind": kind, "cores": 1, "connectivity_s": round(conn_s, 3),
-                               "applyA_s": round(apply_s, 3),
+                               "applyA_s": round(apply_s, 3), "transport_step_s": round(transport_s, 3),
                                "est_solve_s": round(apply_s * rep.iterations, 1),
-                               "sample": "faceNeighbors over every leaf and one OctreePoisson::applyA on the same "
-                                         "tree; est_solve_s = applyA_s x the device iteration count"}
+                               "sample": "faceNeighbors over every leaf, one OctreePoisson::applyA and one "
+                                         "octreeTransportStep on the same tree; est_solve_s = applyA_s x the "
+                                         "device Jacobi-PCG iteration count"}
     return out
 
 

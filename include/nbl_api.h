@@ -286,6 +286,15 @@ int nbl_octree_diagonal(nbl_octree* oc, double* diag);
 /* OctreePoisson::solve (octree_poisson.cpp:107-197): Jacobi-PCG, singular. */
 int nbl_octree_poisson_solve(nbl_octree* oc, const double* b, double* x, double tol,
                              int max_iters, nbl_poisson_report* rep);
+/* octreeTransportStep (adaptive/transport.cpp:24-60): first-order
+ * conservative upwind update of the leaf scalar phi with cell-centred
+ * velocities u, v, w, closed domain boundaries, applied `steps` times with the
+ * fields resident on the device; phi in/out and the velocities are in the
+ * caller's leaf order.  *integral = sum_i phi_i vol_i after the last step
+ * (octreeFieldIntegral, transport.cpp:9-17). */
+int nbl_octree_transport(nbl_octree* oc, const double* u, const double* v, const double* w, double* phi,
+                         double dt, int steps, double* integral);
+
 /* Octree multigrid-preconditioned flexible CG on the OctreePoisson system.
  * No counterpart in the reference, which solves it by Jacobi-PCG only
  * (octree_poisson.cpp:107-197) and names octree levels as the intended
@@ -304,10 +313,12 @@ int nbl_octree_mg_solve(nbl_octree* oc, const double* b, double* x, double tol, 
 int nbl_octree_mg_levels(nbl_octree* oc, long long* counts, int cap, int* levels, double* build_ms);
 
 /* No counterpart in the reference: measurement.  Average ms and algorithmic
- * bytes of one launch of an octree kernel (kernel 0: the SpMV, reading row
- * pointers, masks, x and writing y once per leaf plus a column index and a
- * coefficient per coupling), and the device time of construction (sort,
- * validation, face table, operator) and of the last poisson_solve. */
+ * bytes of one launch of an octree kernel (kernel 0: the SpMV, reading
+ * coupling counts, masks, x and writing y once per leaf plus a column index
+ * and a coefficient per coupling; kernel 1: the transport step, per leaf its
+ * level, face counts, velocities and phi in and out, per face neighbour its
+ * index, level, velocity and phi), and the device time of construction (sort,
+ * validation, face table, operator) and of the last solve or transport. */
 int nbl_octree_profile(nbl_octree* oc, int kernel, int reps, double* ms, double* bytes);
 int nbl_octree_last_timing(nbl_octree* oc, double* setup_ms, double* solve_ms);
 

@@ -525,6 +525,16 @@ int nbl_octree_poisson_solve(nbl_octree* oc, const double* b, double* x, double 
   });
 }
 
+int nbl_octree_transport(nbl_octree* oc, const double* u, const double* v, const double* w, double* phi,
+                         double dt, int steps, double* integral) {
+  return guarded([&] {
+    need(oc && u && v && w && phi, "NULL argument");
+    useDevice(oc->ctx);
+    const double I = oc->impl->transport(u, v, w, phi, dt, steps);
+    if (integral) *integral = I;
+  });
+}
+
 int nbl_octree_mg_solve(nbl_octree* oc, const double* b, double* x, double tol, int max_iters,
                         nbl_poisson_report* rep) {
   return guarded([&] {

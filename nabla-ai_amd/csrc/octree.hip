@@ -275,7 +275,7 @@ __global__ void k_validate(long long n, int maxL, const uint64_t* __restrict__ k
 // ---- Jacobi-PCG on the leaves (octree_poisson.cpp:107-197) ------------------
 // Every reducing kernel writes per-block partials (grid-stride over at most
 // kOctBlocks blocks) and is followed by the shared finalize kernel (k_fin).
-enum : int { O_MEAN_RHS = 0, O_MEAN_R = 1, O_MEAN_Z = 2, O_MEAN_X = 3 };
+enum : int { O_MEAN_RHS = 0, O_MEAN_R = 1, O_MEAN_Z = 2, O_MEAN_X = 3, O_INTEGRAL = 4 };
 constexpr int kOctBlocks = 4096;
 
 #define OGRID(c, n) for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < (n); c += (long long)gridDim.x * blockDim.x)
@@ -528,6 +528,56 @@ __global__ void __launch_bounds__(kBlock) k_om_pupdate(long long n, double* __re
   }
 }
 
+// ---- first-order conservative transport (adaptive/transport.cpp:24-60) ------
+// One leaf per thread in sorted order, neighbours from the face table in the
+// reference's order; every expression as the reference writes it.
+__global__ void __launch_bounds__(kBlock) k_oct_transport(long long n, Geo G, const uint8_t* __restrict__ lev,
+                                                          const uint8_t* __restrict__ cnt,
+                                                          const int32_t* __restrict__ nbr,
+                                                          const double* __restrict__ u, const double* __restrict__ v,
+                                                          const double* __restrict__ w,
+                                                          const double* __restrict__ phi, double dt,
+                                                          double* __restrict__ phiOut) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int li = lev[i];
+  const double szi[3] = {sizeAxis(G, li, 0), sizeAxis(G, li, 1), sizeAxis(G, li, 2)};
+  const double vol = szi[0] * szi[1] * szi[2];
+  const double pi = phi[i];
+  double netOut = 0.0;
+  for (int dir = 0; dir < 6; ++dir) {
+    const int axis = dir >> 1;
+    const double sign = (dir & 1) ? 1.0 : -1.0;  // +face vs -face normal
+    const double* __restrict__ vel = axis == 0 ? u : axis == 1 ? v : w;
+    const double vi = vel[i];
+    const int c = cnt[6 * i + dir];
+    for (int m = 0; m < c; ++m) {
+      const int nb = nbr[24 * i + 4 * dir + m];
+      const int ln = lev[nb];
+      const double szn[3] = {sizeAxis(G, ln, 0), sizeAxis(G, ln, 1), sizeAxis(G, ln, 2)};
+      const double areaI = axis == 0 ? szi[1] * szi[2] : axis == 1 ? szi[0] * szi[2] : szi[0] * szi[1];
+      const double areaN = axis == 0 ? szn[1] * szn[2] : axis == 1 ? szn[0] * szn[2] : szn[0] * szn[1];
+      const double area = areaN < areaI ? areaN : areaI;  // std::min
+      const double vn = sign * 0.5 * (vi + vel[nb]);    // outward normal velocity
+      const double up = vn >= 0.0 ? pi : phi[nb];
+      netOut += vn * area * up;
+    }
+  }
+  const double dphi = -dt / vol * netOut;
+  phiOut[i] = pi + dphi;
+}
+
+// partials of sum_i phi_i vol_i (octreeFieldIntegral, transport.cpp:9-17)
+__global__ void __launch_bounds__(kBlock) k_oct_integral(long long n, Geo G, const uint8_t* __restrict__ lev,
+                                                         const double* __restrict__ phi, RedBuf rb) {
+  double acc[1] = {0.0};
+  OGRID(i, n) {
+    const int l = lev[i];
+    acc[0] += phi[i] * (sizeAxis(G, l, 0) * sizeAxis(G, l, 1) * sizeAxis(G, l, 2));
+  }
+  blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
+}
+
 int ogrid(long long n) { return gridFor(n, kBlock, kOctBlocks); }
 
 }  // namespace
@@ -725,9 +775,86 @@ void OctreeDev::apply(const double* x, double* out) {
 
 void OctreeDev::diagonal(double* out) { fromSorted(L0_.diag.as<double>(), out); }
 
+double OctreeDev::transport(const double* u, const double* v, const double* w, double* phi, double dt, int steps) {
+  if (steps < 1) throw std::invalid_argument("transport: steps must be >= 1");
+  const long long N = (long long)L0_.n;
+  hipStream_t s = stream_;
+  toSorted(u, r_.as<double>());
+  toSorted(v, z_.as<double>());
+  toSorted(w, p_.as<double>());
+  toSorted(phi, x_.as<double>());
+  Geo G;
+  for (int a = 0; a < 3; ++a) {
+    G.org[a] = org_[a];
+    G.fine[a] = fine_[a];
+  }
+  G.maxL = maxL_;
+  double* a = x_.as<double>();
+  double* b = Ap_.as<double>();
+  NBL_HIP(hipEventRecord(e0_, s));
+  for (int q = 0; q < steps; ++q) {
+    k_oct_transport<<<gridFor(N), kBlock, 0, s>>>(N, G, L0_.lev.as<uint8_t>(), cnt_.as<uint8_t>(), nbr_.as<int32_t>(),
+                                                  r_.as<double>(), z_.as<double>(), p_.as<double>(), a, dt, b);
+    std::swap(a, b);
+  }
+  NBL_HIP(hipEventRecord(e1_, s));
+  NBL_LAUNCH_CHECK();
+  const RedBuf rb{red_.as<double>(), kOctBlocks};
+  SolveCtl* ctl = ctl_.as<SolveCtl>();
+  const int g = ogrid(N);
+  k_oct_integral<<<g, kBlock, 0, s>>>(N, G, L0_.lev.as<uint8_t>(), a, rb);
+  FinArgs fa{};
+  fa.op = FIN_MEAN;
+  fa.K = 1;
+  fa.nparts = g;
+  fa.slot = O_INTEGRAL;
+  fa.nf = 1;  // the "mean" over one is the sum
+  launch_fin(rb, fa, ctl, s);
+  NBL_LAUNCH_CHECK();
+  fromSorted(a, phi);  // synchronises the stream
+  NBL_HIP(hipMemcpyAsync(hctl_, ctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, s));
+  NBL_HIP(hipStreamSynchronize(s));
+  float t = 0.f;
+  NBL_HIP(hipEventElapsedTime(&t, e0_, e1_));
+  lastMs_ = t;
+  return hctl_[0].slot[O_INTEGRAL];
+}
+
 void OctreeDev::profile(int kernel, int reps, double* ms, double* bytes) {
-  if (kernel != 0) throw std::invalid_argument("octree profile: kernel must be 0 (SpMV)");
+  if (kernel != 0 && kernel != 1) throw std::invalid_argument("octree profile: kernel must be 0 (SpMV) or 1 (transport)");
   if (reps < 1) throw std::invalid_argument("octree profile: reps must be >= 1");
+  if (kernel == 1) {
+    const long long N = (long long)L0_.n;
+    for (DevBuf* d : {&r_, &z_, &p_, &x_}) NBL_HIP(hipMemsetAsync(d->as<void>(), 0, L0_.n * 8, stream_));
+    Geo G;
+    for (int a = 0; a < 3; ++a) {
+      G.org[a] = org_[a];
+      G.fine[a] = fine_[a];
+    }
+    G.maxL = maxL_;
+    auto one = [&](int q) {
+      k_oct_transport<<<gridFor(N), kBlock, 0, stream_>>>(
+          N, G, L0_.lev.as<uint8_t>(), cnt_.as<uint8_t>(), nbr_.as<int32_t>(), r_.as<double>(), z_.as<double>(),
+          p_.as<double>(), (q & 1) ? Ap_.as<double>() : x_.as<double>(), 0.01, (q & 1) ? x_.as<double>() : Ap_.as<double>());
+    };
+    one(0);
+    NBL_HIP(hipEventRecord(e0_, stream_));
+    for (int q = 0; q < reps; ++q) one(q + 1);
+    NBL_HIP(hipEventRecord(e1_, stream_));
+    NBL_LAUNCH_CHECK();
+    NBL_HIP(hipEventSynchronize(e1_));
+    float t = 0.f;
+    NBL_HIP(hipEventElapsedTime(&t, e0_, e1_));
+    *ms = double(t) / reps;
+    // per leaf: level, six counts, the three velocities, phi in and out; per
+    // face neighbour: its index, level, normal velocity and (upwind) phi
+    std::vector<uint8_t> hc(6 * L0_.n);
+    NBL_HIP(hipMemcpy(hc.data(), cnt_.as<void>(), hc.size(), hipMemcpyDeviceToHost));
+    double faces = 0.0;
+    for (uint8_t c : hc) faces += c;
+    *bytes = double(L0_.n) * (1.0 + 6.0 + 3.0 * 8.0 + 8.0 + 8.0) + faces * (4.0 + 1.0 + 8.0 + 8.0);
+    return;
+  }
   NBL_HIP(hipMemsetAsync(t0_.as<void>(), 0, L0_.n * 8, stream_));
   applySorted(t0_.as<double>(), Ap_.as<double>(), stream_);
   NBL_HIP(hipEventRecord(e0_, stream_));

@@ -125,6 +125,9 @@ class OctreeDev {
   double setupMs() const { return setupMs_; }
   double lastSolveMs() const { return lastMs_; }
   void profile(int kernel, int reps, double* ms, double* bytes);
+  // octreeTransportStep on phi (caller-order host arrays), `steps` times with
+  // the fields resident; returns the integral sum phi_i vol_i after the last.
+  double transport(const double* u, const double* v, const double* w, double* phi, double dt, int steps);
   // Multigrid hierarchy: level count, leaves per level (up to `cap` entries)
   // and the device time of its construction (ms).  Builds it if needed.
   int mgLevels(long long* counts, int cap, double* buildMs);

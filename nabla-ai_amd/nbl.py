@@ -219,6 +219,7 @@ def load(path: os.PathLike | None = None):
     _proto(lib, "nbl_octree_apply", i, V, P_D, P_D)
     _proto(lib, "nbl_octree_diagonal", i, V, P_D)
     _proto(lib, "nbl_octree_poisson_solve", i, V, P_D, P_D, d, i, C.POINTER(PoissonReport))
+    _proto(lib, "nbl_octree_transport", i, V, P_D, P_D, P_D, P_D, d, i, C.POINTER(C.c_double))
     _proto(lib, "nbl_octree_mg_solve", i, V, P_D, P_D, d, i, C.POINTER(PoissonReport))
     _proto(lib, "nbl_octree_mg_levels", i, V, C.POINTER(C.c_longlong), i, C.POINTER(i),
            C.POINTER(C.c_double))
@@ -546,6 +547,15 @@ class Octree:
         check(self.lib.nbl_octree_poisson_solve(self.h, dptr(b), dptr(x), tol, max_iters,
                                                 C.byref(rep)))
         return x, rep
+
+    def transport(self, u, v, w, phi, dt, steps=1):
+        """octreeTransportStep `steps` times: returns (phi, integral)."""
+        u, v, w = (np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w))
+        p = np.array(phi, dtype=np.float64)
+        integral = C.c_double()
+        check(self.lib.nbl_octree_transport(self.h, dptr(u), dptr(v), dptr(w), dptr(p), dt, steps,
+                                            C.byref(integral)))
+        return p, integral.value
 
     def mg_solve(self, b, tol=1e-9, max_iters=2000):
         """Multigrid-preconditioned flexible CG on the same system as solve()."""

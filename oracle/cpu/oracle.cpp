@@ -1176,6 +1176,38 @@ struct Oct {
       dg[i] = d > 0.0 ? d : 1.0;
     }
   }
+  // First-order conservative upwind step of a cell scalar with cell-centred
+  // velocities (transport.cpp:24-60): closed boundaries, shared face area =
+  // the smaller transverse area, outward velocity = the face average; returns
+  // the integral sum_i phi_i vol_i (transport.cpp:9-17).
+  double transport(const Vec& u, const Vec& v, const Vec& w, Vec& phi, double dt) const {
+    const size_t n = key.size();
+    Vec dphi(n, 0.0);
+    for (size_t i = 0; i < n; ++i) {
+      const double szi[3] = {size(i, 0), size(i, 1), size(i, 2)};
+      const double vol = szi[0] * szi[1] * szi[2];
+      double netOut = 0.0;
+      for (int dir = 0; dir < 6; ++dir) {
+        const int axis = dir / 2;
+        const double sign = (dir & 1) ? 1.0 : -1.0;
+        for (size_t nb : nbrs(i, dir)) {
+          const double szn[3] = {size(nb, 0), size(nb, 1), size(nb, 2)};
+          const double areaI = axis == 0 ? szi[1] * szi[2] : axis == 1 ? szi[0] * szi[2] : szi[0] * szi[1];
+          const double areaN = axis == 0 ? szn[1] * szn[2] : axis == 1 ? szn[0] * szn[2] : szn[0] * szn[1];
+          const double area = std::min(areaI, areaN);
+          const Vec& vel = axis == 0 ? u : axis == 1 ? v : w;
+          const double vn = sign * 0.5 * (vel[i] + vel[nb]);
+          const double up = vn >= 0.0 ? phi[i] : phi[nb];
+          netOut += vn * area * up;
+        }
+      }
+      dphi[i] = -dt / vol * netOut;
+    }
+    for (size_t i = 0; i < n; ++i) phi[i] += dphi[i];
+    double sum = 0.0;
+    for (size_t i = 0; i < n; ++i) sum += phi[i] * (size(i, 0) * size(i, 1) * size(i, 2));
+    return sum;
+  }
   // Jacobi-PCG, always singular (octree_poisson.cpp:107-197).
   nbl_poisson_report solve(const Vec& b, Vec& x, double tol, int maxIt) const {
     const size_t n = key.size();
@@ -1859,6 +1891,16 @@ int orc_oct_mg_levels(void* h, long long* counts, int cap) {
   const OctMgOrc mg(*static_cast<Oct*>(h));
   for (int l = 0; l < std::min(cap, int(mg.L.size())); ++l) counts[l] = (long long)mg.L[size_t(l)].n;
   return int(mg.L.size());
+}
+
+
+double orc_oct_transport(void* h, const double* u, const double* v, const double* w, double* phi, double dt) {
+  const Oct& o = *static_cast<Oct*>(h);
+  const size_t n = o.key.size();
+  Vec uu(u, u + n), vv(v, v + n), ww(w, w + n), pp(phi, phi + n);
+  const double I = o.transport(uu, vv, ww, pp, dt);
+  std::copy(pp.begin(), pp.end(), phi);
+  return I;
 }
 
 }  // extern "C"

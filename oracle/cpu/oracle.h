@@ -81,6 +81,9 @@ void orc_oct_solve(void* h, const double* b, double* x, double tol, int max_iter
 void orc_oct_mg_solve(void* h, const double* b, double* x, double tol, int max_iters,
                       nbl_poisson_report* rep);
 int orc_oct_mg_levels(void* h, long long* counts, int cap);
+/* octreeTransportStep (adaptive/transport.cpp:24-60) on scalar phi (in/out),
+ * cell-centred u, v, w; returns the integral sum phi_i vol_i. */
+double orc_oct_transport(void* h, const double* u, const double* v, const double* w, double* phi, double dt);
 
 #ifdef __cplusplus
 }

@@ -60,6 +60,7 @@ class OracleLib:
             "oct_face_table": (None, V, P_U8, P_I64),
             "oct_centers": (None, V, P_D), "oct_apply": (None, V, P_D, P_D),
             "oct_diagonal": (None, V, P_D), "oct_solve": (None, V, P_D, P_D, d, i, PR),
+            "oct_transport": (d, V, P_D, P_D, P_D, P_D, d),
         }
         if kind == "orc":
             spec.update({
@@ -270,6 +271,13 @@ class Oct:
         rep = nbl.PoissonReport()
         self.o.oct_solve(self.h, dptr(b), dptr(x), tol, max_iters, C.byref(rep))
         return x, rep
+
+    def transport(self, u, v, w, phi, dt):
+        """octreeTransportStep: returns (new phi, integral)."""
+        u, v, w = (np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w))
+        p = np.array(phi, dtype=np.float64)
+        integral = self.o.oct_transport(self.h, dptr(u), dptr(v), dptr(w), dptr(p), dt)
+        return p, integral
 
     def mg_solve(self, b, tol=1e-9, max_iters=2000):
         """Octree multigrid flexible CG (restatement; oracle library only)."""

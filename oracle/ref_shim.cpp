@@ -9,6 +9,7 @@
 #include <memory>
 #include <vector>
 
+#include "nabla/adaptive/adaptive.hpp"
 #include "nabla/flow/ns_solver.hpp"
 #include "nabla/flow/octree_poisson.hpp"
 #include "nabla/flow/poisson.hpp"
@@ -247,4 +248,16 @@ void ref_oct_solve(void* h, const double* b, double* x, double tol, int max_iter
   toPrep(r, rep);
 }
 
+double ref_oct_transport(void* h, const double* u, const double* v, const double* w, double* phi, double dt) {
+  auto& t = *static_cast<mesh::Octree*>(h);
+  const size_t n = t.cellCount();
+  t.u().assign(u, u + n);
+  t.v().assign(v, v + n);
+  t.w().assign(w, w + n);
+  t.registerScalar("phi");
+  t.scalar("phi").assign(phi, phi + n);
+  const double I = adaptive::octreeTransportStep(t, "phi", dt);
+  std::memcpy(phi, t.scalar("phi").data(), n * 8);
+  return I;
+}
 }  // extern "C"

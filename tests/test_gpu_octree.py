@@ -277,3 +277,45 @@ def test_c3_mg_solve(ctx, c3):
         assert rj.converged and rel(x, xj) < 1e-5
     finally:
         d.close()
+
+
+# ---- octreeTransportStep (adaptive/transport.cpp:24-60) ----------------------
+@pytest.mark.parametrize("which", [0, 1, 2], ids=["uniform8", "jump", "storm_masked"])
+def test_transport_bitwise(ctx, tree_list, which):
+    """One step: every leaf's new phi is bit-identical to the oracle's (which
+    is bit-identical to the reference's own octreeTransportStep); the integral
+    agrees to round-off (summation order differs).  Ten resident steps equal
+    ten oracle steps bit for bit."""
+    _name, t = tree_list[which]
+    c = t.centers()
+    phi = 1.0 + np.sin(6.0 * c[:, 0]) * np.cos(5.0 * c[:, 1])
+    u, v, w = cases.normal(41, t.n), cases.normal(42, t.n), cases.normal(43, t.n)
+    d = device_tree(ctx, t)
+    try:
+        p1, i1 = d.transport(u, v, w, phi, 0.02)
+        po, io = t.transport(u, v, w, phi, 0.02)
+        assert np.array_equal(p1, po)
+        assert i1 == pytest.approx(io, rel=1e-13)
+        p10, _ = d.transport(u, v, w, phi, 0.005, steps=10)
+        q = phi
+        for _ in range(10):
+            q, _ = t.transport(u, v, w, q, 0.005)
+        assert np.array_equal(p10, q)
+    finally:
+        d.close()
+
+
+def test_c3_transport_conserves(ctx, c3):
+    """Fifty resident steps on the 4.3M-leaf tree conserve the integral of phi
+    to round-off."""
+    _f, k, l, m, _h = c3
+    d = nbl.Octree(ctx, k, l, m, cases.C3_ORIGIN, cases.C3_EXTENT, 8)
+    try:
+        n = k.size
+        u, v, w = cases.normal(1, n), cases.normal(2, n), cases.normal(3, n)
+        phi = 1.0 + 0.1 * cases.normal(4, n)
+        _, i0 = d.transport(np.zeros(n), np.zeros(n), np.zeros(n), phi, 0.0)
+        _, i1 = d.transport(u, v, w, phi, 1e-4, steps=50)
+        assert abs(i1 - i0) <= 1e-12 * abs(i0)
+    finally:
+        d.close()

@@ -342,3 +342,39 @@ def test_octree_mg_semicoarsens_anisotropic_cells(orc):
     k, l, _m = box.dump()
     assert mg_level_counts(k, l, 6, (12, 4, 4)) == box.mg_levels()
     assert its[1] <= 1.25 * its[0]
+
+
+def test_octree_transport_conserves_and_is_upwind(orc):
+    """octreeTransportStep (adaptive/transport.cpp:24-60) is conservative:
+    every interior face flux is computed identically from both sides, so the
+    integral of phi is constant to round-off over many steps, across 2:1 jumps
+    (core/tests/test_adaptive.cpp:37-60 checks 1e-8 under concurrent AMR); a
+    constant field stays constant under a uniform velocity; and on a uniform
+    tree a uniform x-velocity gives the 1-D first-order upwind update."""
+    t = orc.octree((0, 0, 0), (1, 1, 1), 5)
+    t.refine_uniform(3)
+    t.refine_box((0.2, 0.2, 0.2), (0.6, 0.6, 0.6), 1)
+    c = t.centers()
+    phi = 1.0 + np.sin(6.0 * c[:, 0]) * np.cos(5.0 * c[:, 1])
+    u, v, w = cases.normal(5, t.n), cases.normal(6, t.n), cases.normal(7, t.n)
+    _, i0 = t.transport(np.zeros(t.n), np.zeros(t.n), np.zeros(t.n), phi, 0.0)
+    p = phi
+    for _ in range(20):
+        p, integral = t.transport(u, v, w, p, 0.002)
+    assert abs(integral - i0) <= 1e-12 * abs(i0)
+    const, _ = t.transport(np.full(t.n, 0.3), np.full(t.n, -0.2), np.full(t.n, 0.1), np.full(t.n, 2.5), 0.01)
+    inner = (c > 0.13).all(axis=1) & (c < 0.87).all(axis=1)  # away from the closed boundary
+    assert np.allclose(const[inner], 2.5, rtol=0, atol=1e-13)
+    # uniform 8^3 tree, u = 0.5: phi_i - dt/h * 0.5 * (phi_i - phi_{i-1}) in the interior
+    t8 = orc.octree((0, 0, 0), (1, 1, 1), 3)
+    t8.refine_uniform(3)
+    c8 = t8.centers()
+    idx = np.floor(c8 * 8).astype(int)
+    q = cases.normal(9, t8.n)
+    out, _ = t8.transport(np.full(t8.n, 0.5), np.zeros(t8.n), np.zeros(t8.n), q, 0.01)
+    lin = {tuple(r): n for n, r in enumerate(idx)}
+    for n, (i, j, k) in enumerate(idx):
+        if i == 0 or i == 7:
+            continue
+        want = q[n] - 0.01 / (1.0 / 8) * 0.5 * (q[n] - q[lin[(i - 1, j, k)]])
+        assert out[n] == pytest.approx(want, rel=1e-13, abs=1e-14)

@@ -123,6 +123,15 @@ def _check_oct(a, b):
     x = cases.normal(3, a.n)
     assert same(a.apply(x), b.apply(x))
     assert same(a.diagonal(), b.diagonal())
+    # octreeTransportStep (adaptive/transport.cpp:24-60): the reference test's
+    # fields (test_adaptive.cpp:30-34) and seeded random velocities
+    c = a.centers()
+    phi = 1.0 + np.sin(6.0 * c[:, 0]) * np.cos(5.0 * c[:, 1])
+    for u, v, w in ((np.full(a.n, 0.3), np.full(a.n, 0.1), np.zeros(a.n)),
+                    (cases.normal(41, a.n), cases.normal(42, a.n), cases.normal(43, a.n))):
+        pa, ia = a.transport(u, v, w, phi, 0.02)
+        pb, ib = b.transport(u, v, w, phi, 0.02)
+        assert same(pa, pb) and ia == ib
     bb = a.apply(x)
     xa, ra = a.solve(bb, 1e-10, 3000)
     xb, rb = b.solve(bb, 1e-10, 3000)
