@@ -316,8 +316,8 @@ int nbl_octree_mg_levels(nbl_octree* oc, long long* counts, int cap, int* levels
  * bytes of one launch of an octree kernel (kernel 0: the SpMV, reading
  * coupling counts, masks, x and writing y once per leaf plus a column index
  * and a coefficient per coupling; kernel 1: the transport step, per leaf its
- * level, face counts, velocities and phi in and out, per face neighbour its
- * index, level, velocity and phi), and the device time of construction (sort,
+ * level, face counts, first-neighbour slots, velocities and phi in and out,
+ * plus an index per further neighbour), and the device time of construction (sort,
  * validation, face table, operator) and of the last solve or transport. */
 int nbl_octree_profile(nbl_octree* oc, int kernel, int reps, double* ms, double* bytes);
 int nbl_octree_last_timing(nbl_octree* oc, double* setup_ms, double* solve_ms);

@@ -533,7 +533,9 @@ __global__ void __launch_bounds__(kBlock) k_om_pupdate(long long n, double* __re
 // reference's order; every expression as the reference writes it.
 __global__ void __launch_bounds__(kBlock) k_oct_transport(long long n, Geo G, const uint8_t* __restrict__ lev,
                                                           const uint8_t* __restrict__ cnt,
-                                                          const int32_t* __restrict__ nbr,
+                                                          const int32_t* __restrict__ first,
+                                                          const int32_t* __restrict__ xrow,
+                                                          const int32_t* __restrict__ xnb,
                                                           const double* __restrict__ u, const double* __restrict__ v,
                                                           const double* __restrict__ w,
                                                           const double* __restrict__ phi, double dt,
@@ -544,6 +546,7 @@ __global__ void __launch_bounds__(kBlock) k_oct_transport(long long n, Geo G, co
   const double szi[3] = {sizeAxis(G, li, 0), sizeAxis(G, li, 1), sizeAxis(G, li, 2)};
   const double vol = szi[0] * szi[1] * szi[2];
   const double pi = phi[i];
+  int x = xrow[i];  // this leaf's further neighbours (directions with 2..4), in order
   double netOut = 0.0;
   for (int dir = 0; dir < 6; ++dir) {
     const int axis = dir >> 1;
@@ -552,7 +555,7 @@ __global__ void __launch_bounds__(kBlock) k_oct_transport(long long n, Geo G, co
     const double vi = vel[i];
     const int c = cnt[6 * i + dir];
     for (int m = 0; m < c; ++m) {
-      const int nb = nbr[24 * i + 4 * dir + m];
+      const int nb = m == 0 ? first[(long long)dir * n + i] : xnb[x++];
       const int ln = lev[nb];
       const double szn[3] = {sizeAxis(G, ln, 0), sizeAxis(G, ln, 1), sizeAxis(G, ln, 2)};
       const double areaI = axis == 0 ? szi[1] * szi[2] : axis == 1 ? szi[0] * szi[2] : szi[0] * szi[1];
@@ -565,6 +568,28 @@ __global__ void __launch_bounds__(kBlock) k_oct_transport(long long n, Geo G, co
   }
   const double dphi = -dt / vol * netOut;
   phiOut[i] = pi + dphi;
+}
+
+// Face neighbours for the transport kernel: the first neighbour of every
+// direction slot-major (first[dir*n + i], -1 when none), the rest of each
+// direction (2..4 finer neighbours) per leaf in an overflow list.
+__global__ void k_tab_count(long long n, const uint8_t* __restrict__ cnt, int32_t* __restrict__ extra) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int e = 0;
+  for (int d = 0; d < 6; ++d) e += cnt[6 * i + d] > 1 ? cnt[6 * i + d] - 1 : 0;
+  extra[i] = e;
+}
+__global__ void k_tab_fill(long long n, const uint8_t* __restrict__ cnt, const int32_t* __restrict__ nbr,
+                           const int32_t* __restrict__ xrow, int32_t* __restrict__ first, int32_t* __restrict__ xnb) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int x = xrow[i];
+  for (int d = 0; d < 6; ++d) {
+    const int c = cnt[6 * i + d];
+    first[(long long)d * n + i] = c > 0 ? nbr[24 * i + 4 * d] : -1;
+    for (int m = 1; m < c; ++m) xnb[x++] = nbr[24 * i + 4 * d + m];
+  }
 }
 
 // partials of sum_i phi_i vol_i (octreeFieldIntegral, transport.cpp:9-17)
@@ -717,6 +742,27 @@ OctreeDev::OctreeDev(nbl_ctx* ctx, const uint64_t* keys, const uint8_t* levels, 
   if (hbad & 2u) throw std::invalid_argument("octree: key is not a valid anchor for its level");
   if (hbad & 4u) throw std::invalid_argument("octree: duplicate leaf key");
   buildOctOperator(ctx, maxLevel, org_, fine_, L0_, &cnt_, &nbr_, s);
+  {
+    DevBuf extra(ctx, (n + 1) * sizeof(int32_t));
+    tfirst_ = DevBuf(ctx, 6 * n * sizeof(int32_t));
+    txrow_ = DevBuf(ctx, (n + 1) * sizeof(int32_t));
+    NBL_HIP(hipMemsetAsync(extra.as<int32_t>() + n, 0, sizeof(int32_t), s));
+    k_tab_count<<<gridFor(N), kBlock, 0, s>>>(N, cnt_.as<uint8_t>(), extra.as<int32_t>());
+    NBL_LAUNCH_CHECK();
+    size_t tb = 0;
+    NBL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, extra.as<int32_t>(), txrow_.as<int32_t>(), int(n + 1), s));
+    DevBuf tmp2(ctx, tb);
+    NBL_HIP(hipcub::DeviceScan::ExclusiveSum(tmp2.as<void>(), tb, extra.as<int32_t>(), txrow_.as<int32_t>(),
+                                             int(n + 1), s));
+    int32_t nx = 0;
+    NBL_HIP(hipMemcpyAsync(&nx, txrow_.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    NBL_HIP(hipStreamSynchronize(s));
+    txnb_ = DevBuf(ctx, std::max<size_t>(1, size_t(nx)) * sizeof(int32_t));
+    tnx_ = size_t(nx);
+    k_tab_fill<<<gridFor(N), kBlock, 0, s>>>(N, cnt_.as<uint8_t>(), nbr_.as<int32_t>(), txrow_.as<int32_t>(),
+                                             tfirst_.as<int32_t>(), txnb_.as<int32_t>());
+    NBL_LAUNCH_CHECK();
+  }
   NBL_HIP(hipEventRecord(e1_, s));
   // solver vectors
   for (auto* b : {&x_, &rhs_, &r_, &z_, &p_, &Ap_, &t0_, &t1_}) *b = DevBuf(ctx, n * 8);
@@ -793,7 +839,8 @@ double OctreeDev::transport(const double* u, const double* v, const double* w, d
   double* b = Ap_.as<double>();
   NBL_HIP(hipEventRecord(e0_, s));
   for (int q = 0; q < steps; ++q) {
-    k_oct_transport<<<gridFor(N), kBlock, 0, s>>>(N, G, L0_.lev.as<uint8_t>(), cnt_.as<uint8_t>(), nbr_.as<int32_t>(),
+    k_oct_transport<<<gridFor(N), kBlock, 0, s>>>(N, G, L0_.lev.as<uint8_t>(), cnt_.as<uint8_t>(),
+                                                  tfirst_.as<int32_t>(), txrow_.as<int32_t>(), txnb_.as<int32_t>(),
                                                   r_.as<double>(), z_.as<double>(), p_.as<double>(), a, dt, b);
     std::swap(a, b);
   }
@@ -834,8 +881,9 @@ void OctreeDev::profile(int kernel, int reps, double* ms, double* bytes) {
     G.maxL = maxL_;
     auto one = [&](int q) {
       k_oct_transport<<<gridFor(N), kBlock, 0, stream_>>>(
-          N, G, L0_.lev.as<uint8_t>(), cnt_.as<uint8_t>(), nbr_.as<int32_t>(), r_.as<double>(), z_.as<double>(),
-          p_.as<double>(), (q & 1) ? Ap_.as<double>() : x_.as<double>(), 0.01, (q & 1) ? x_.as<double>() : Ap_.as<double>());
+          N, G, L0_.lev.as<uint8_t>(), cnt_.as<uint8_t>(), tfirst_.as<int32_t>(), txrow_.as<int32_t>(),
+          txnb_.as<int32_t>(), r_.as<double>(), z_.as<double>(), p_.as<double>(),
+          (q & 1) ? Ap_.as<double>() : x_.as<double>(), 0.01, (q & 1) ? x_.as<double>() : Ap_.as<double>());
     };
     one(0);
     NBL_HIP(hipEventRecord(e0_, stream_));
@@ -846,13 +894,10 @@ void OctreeDev::profile(int kernel, int reps, double* ms, double* bytes) {
     float t = 0.f;
     NBL_HIP(hipEventElapsedTime(&t, e0_, e1_));
     *ms = double(t) / reps;
-    // per leaf: level, six counts, the three velocities, phi in and out; per
-    // face neighbour: its index, level, normal velocity and (upwind) phi
-    std::vector<uint8_t> hc(6 * L0_.n);
-    NBL_HIP(hipMemcpy(hc.data(), cnt_.as<void>(), hc.size(), hipMemcpyDeviceToHost));
-    double faces = 0.0;
-    for (uint8_t c : hc) faces += c;
-    *bytes = double(L0_.n) * (1.0 + 6.0 + 3.0 * 8.0 + 8.0 + 8.0) + faces * (4.0 + 1.0 + 8.0 + 8.0);
+    // compulsory traffic (a neighbour's level, velocity and phi are another
+    // leaf's own): per leaf its level, six counts, six first-neighbour slots,
+    // the three velocities, phi in and out; per further neighbour its index
+    *bytes = double(L0_.n) * (1.0 + 6.0 + 24.0 + 3.0 * 8.0 + 8.0 + 8.0) + double(tnx_) * 4.0;
     return;
   }
   NBL_HIP(hipMemsetAsync(t0_.as<void>(), 0, L0_.n * 8, stream_));

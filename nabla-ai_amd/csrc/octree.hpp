@@ -148,6 +148,8 @@ class OctreeDev {
   hipStream_t stream_ = nullptr;
   OctLevel L0_;
   DevBuf perm_, cnt_, nbr_;
+  DevBuf tfirst_, txrow_, txnb_;  // transport's face neighbours (first per direction slot-major, overflow)
+  size_t tnx_ = 0;
   DevBuf x_, rhs_, r_, z_, p_, Ap_, t0_, t1_;
   DevBuf ctl_, red_;
   SolveCtl* hctl_ = nullptr;
