@@ -49,7 +49,21 @@ struct SolveCtl {
   double tol, bnorm, rz, pAp, alpha, beta, rr, resid;
   int it, done, converged, broke;
   int early, pad_;  // early: converged before the first iteration
+  double rzq[2];    // r.z of flexible-CG iteration k in rzq[k & 1] (finalizes folded into kernels)
   double slot[64];  // reduction results / pending means
+};
+
+// A finalize folded into the kernel that consumes it, in place of a one-block
+// k_fin launch (flexible CG on systems without projections): every block of
+// the consumer folds the previous kernel's `n` per-block partials (rows of
+// `stride`) in the same fixed order, so all blocks reach the same bits, and
+// block 0 records the scalars in SolveCtl.  Scalars that a consumer reads
+// while its own block 0 writes their successor alternate by iteration parity
+// (SolveCtl::rzq).  n == 0: off, the scalars come from SolveCtl.
+struct InFold {
+  const double* parts = nullptr;
+  int stride = 0, n = 0;
+  int k = 0;  // flexible-CG iteration
 };
 
 // Launch shape of a cell kernel on an nx x ny x nz level: 256-thread blocks,
@@ -162,19 +176,20 @@ void launch_ctl_init(SolveCtl* ctl, double tol, hipStream_t s);
 // Step scalars (step index, time, last dt) set by a kernel on `s`.
 void launch_set_step(StepCtl* sc, int step, double t, double dt, hipStream_t s);
 // stamps: optional per-block device wall-clock stamps (start, and end once
-// the block's stores are visible), 2 per block.
+// the block's stores are visible), 2 per block.  conv: the flexible CG's
+// convergence test on the partials of |r_new|^2 folded in (the sweep does
+// nothing when it stops the solve).
 template <class T>
 void launch_mg_sweep(const MgLevel& L, const T* b, const double* mb, const T* x, const double* mx,
                      bool zeroGuess, const T* ec, const double* mec, const MgLevel* Lc, T* xo, bool fold,
-                     RedBuf rb, const SolveCtl* ctl, hipStream_t s, long long* stamps = nullptr);
-// The first two sweeps from a zero guess in one pass (no projection between
-// them: non-singular systems only).
-template <class T>
-void launch_mg_sweep_pair(const MgLevel& L, const T* b, T* xo, const SolveCtl* ctl, hipStream_t s);
+                     RedBuf rb, SolveCtl* ctl, hipStream_t s, long long* stamps = nullptr, InFold conv = {});
+// x1c (optional): the coarse level's first sweep from a zero guess, written
+// beside the restricted residual (pointwise: no projection may come between,
+// so non-singular systems only).
 template <class T>
 void launch_mg_resid_restrict(const MgLevel& L, const MgLevel& Lc, const T* b, const double* mb,
                               const T* x, const double* mx, T* rc, bool fold, RedBuf rb,
-                              const SolveCtl* ctl, hipStream_t s);
+                              const SolveCtl* ctl, hipStream_t s, T* x1c = nullptr);
 template <class T>
 void launch_mg_coarse(const MgLevel& L, bool singular, const T* b, const double* mb, T* x,
                       T* scratch, const SolveCtl* ctl, hipStream_t s);
@@ -215,16 +230,19 @@ void launch_fcg_update(const MgLevel& L, bool singular, double* x, const double*
                        const double* Ap, double* rn, const SolveCtl* ctl, RedBuf rb, hipStream_t s);
 // Non-singular systems: the update plus the next V-cycle's first level-0 sweep
 // from zero (x1) and, for an FP32 cycle, its right-hand side b0 = float(rn).
+// alpha: alpha = rz / p.Ap folded in from the partials of p.Ap.
 template <class T>
 void launch_fcg_update_first(const MgLevel& L, double* x, const double* p, const double* r, const double* Ap,
-                             double* rn, T* x1, T* b0, const SolveCtl* ctl, RedBuf rb, hipStream_t s);
-// p_new = z + beta p_old and A p_new, p_new.Ap in one pass.
+                             double* rn, T* x1, T* b0, SolveCtl* ctl, RedBuf rb, hipStream_t s, InFold alpha = {});
+// p_new = z + beta p_old and A p_new, p_new.Ap in one pass.  beta: the
+// flexible beta folded in from the partials of z.r_new and z.r.
 void launch_fcg_apply_pu(const MgLevel& L, const double* z, const double* pOld, double* pNew, double* Ap,
-                         const SolveCtl* ctl, RedBuf rb, hipStream_t s);
-// Last level-0 sweep of a V-cycle writing z and the partials of z.rn, z.r.
+                         SolveCtl* ctl, RedBuf rb, hipStream_t s, InFold beta = {});
+// Last level-0 sweep of a V-cycle writing z and the partials of z.rn, z.r;
+// countIt: the sweep also counts the finished iteration (block 0).
 template <class T>
 void launch_mg_sweep_dots(const MgLevel& L, const T* b, const T* x, double* z, const double* rn, const double* r,
-                          RedBuf rb, const SolveCtl* ctl, hipStream_t s);
+                          RedBuf rb, SolveCtl* ctl, hipStream_t s, bool countIt = false);
 void launch_fcg_norm(const MgLevel& L, double* rn, const SolveCtl* ctl, RedBuf rb, hipStream_t s);
 void launch_fcg_dots(const MgLevel& L, double* z, const double* mz, const double* rn, const double* r,
                      const SolveCtl* ctl, RedBuf rb, hipStream_t s);

@@ -253,11 +253,8 @@ __device__ __forceinline__ T applyRow(const MgLevel& L, uint16_t code, int c, in
 // correction (multigrid.cpp:158-173) in SW_PROLONG, and per-block fluid sums
 // of the result for the projection that follows a sweep on singular systems.
 // Pending means (`mb`, `mx`, `mec`) are device slots, read here.
-// SW_PAIR runs the first two sweeps from a zero guess in one pass (systems
-// without the projection between sweeps): the first sweep is pointwise,
-// x1 = 0 + 0.8 (b - 0)/diag, so each cell re-evaluates it at its stencil points.
 // ---------------------------------------------------------------------------
-enum SweepMode : int { SW_PLAIN = 0, SW_ZERO = 1, SW_PROLONG = 2, SW_PAIR = 3 };
+enum SweepMode : int { SW_PLAIN = 0, SW_ZERO = 1, SW_PROLONG = 2 };
 
 // First sweep from zero at one cell, in the expression order of k_sweep.
 template <class T>
@@ -268,24 +265,6 @@ __device__ __forceinline__ T firstSweep(const MgLevel& L, uint16_t code, int c, 
   const T diag = dg > T(0) ? dg : T(1);
   return T(0) + T(0.8) * (bv - Ax) / diag;
 }
-
-template <class T>
-struct FirstSweepView {
-  const MgLevel* L;
-  const T* __restrict__ b;
-  __device__ __forceinline__ T at(int c, int i, int j, int k) const {
-    return firstSweep<T>(*L, L->code[c], c, i, j, k, b[c]);
-  }
-  template <int NC>
-  __device__ __forceinline__ void pack(int c, int i, int j, int k, T (&v)[NC]) const {
-    uint16_t cd[NC];
-    T bv[NC];
-    ldCodes<NC>(L->code, c, cd);
-    ldN<NC>(b, c, bv);
-#pragma unroll
-    for (int e = 0; e < NC; ++e) v[e] = firstSweep<T>(*L, cd[e], c + e, i + e, j, k, bv[e]);
-  }
-};
 
 // Centre values and the six neighbour values (order -x, +x, -y, +y, -z, +z)
 // of the NC cells of a group, read through a view X (plain, pending mean,
@@ -320,13 +299,107 @@ __device__ __forceinline__ void gatherN(const MgLevel& L, const XV& X, int c, in
   }
 }
 
+// ---------------------------------------------------------------------------
+// Finalizes folded into the consuming kernel (InFold).  Every block reads the
+// previous kernel's partials with plain loads (the kernel boundary orders
+// them) and folds them in one fixed order.  A kernel whose own block 0 may
+// stop the solve reads `done` once per block, so that every thread of a block
+// takes the same branch.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int linearTid() { return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z); }
+__device__ __forceinline__ bool firstBlock() { return blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0; }
+
+template <int K>
+__device__ __forceinline__ void foldInline(const InFold& f, double (&out)[K]) {
+  __shared__ double sh[K * 16];
+  __shared__ double res[K];
+  const int tid = linearTid(), nt = blockDim.x * blockDim.y * blockDim.z;
+  double a[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    a[q] = 0.0;
+    for (int b = tid; b < f.n; b += nt) a[q] += f.parts[q * f.stride + b];
+  }
+  blockFold<K, 0u>(a, sh);
+  if (tid == 0)
+    for (int q = 0; q < K; ++q) res[q] = a[q];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < K; ++q) out[q] = res[q];
+}
+
+__device__ __forceinline__ bool blockStopped(const SolveCtl* ctl) {
+  __shared__ int stop;
+  if (linearTid() == 0) stop = ldA(&ctl->done);
+  __syncthreads();
+  return stop != 0;
+}
+
+// FIN_CONV: |r_new| <= tol |b| stops the solve.  True when this kernel must
+// do nothing (stopped before or now).
+__device__ __forceinline__ bool inlineConv(const InFold& f, SolveCtl* ctl) {
+  if (blockStopped(ctl)) return true;
+  double t[1];
+  foldInline<1>(f, t);
+  const bool conv = sqrt(t[0]) <= ldA(&ctl->tol) * ldA(&ctl->bnorm);
+  if (firstBlock() && linearTid() == 0) {
+    stA(&ctl->rr, t[0]);
+    if (conv) {
+      stA(&ctl->it, ldA(&ctl->it) + 1);
+      stA(&ctl->converged, 1);
+      stA(&ctl->done, 1);
+    }
+  }
+  return conv;
+}
+
+// FIN_ALPHA: alpha = rz / p.Ap; a vanishing p.Ap stops the solve (breakdown).
+// False when this kernel must do nothing.
+__device__ __forceinline__ bool inlineAlpha(const InFold& f, SolveCtl* ctl, double& alpha) {
+  if (blockStopped(ctl)) return false;
+  double t[1];
+  foldInline<1>(f, t);
+  const bool w = firstBlock() && linearTid() == 0;
+  if (fabs(t[0]) < 1e-300) {
+    if (w) {
+      stA(&ctl->pAp, t[0]);
+      stA(&ctl->broke, 1);
+      stA(&ctl->done, 1);
+    }
+    return false;
+  }
+  alpha = ldA(&ctl->rzq[f.k & 1]) / t[0];
+  if (w) {
+    stA(&ctl->pAp, t[0]);
+    stA(&ctl->alpha, alpha);
+  }
+  return true;
+}
+
+// FIN_BETA_FLEX: beta = (z.r_new - z.r) / rz_prev, rz = z.r_new.
+__device__ __forceinline__ double inlineBeta(const InFold& f, SolveCtl* ctl) {
+  double t[2];
+  foldInline<2>(f, t);
+  const double be = (t[0] - t[1]) / ldA(&ctl->rzq[(f.k - 1) & 1]);
+  if (firstBlock() && linearTid() == 0) {
+    stA(&ctl->beta, be);
+    stA(&ctl->rz, t[0]);
+    stA(&ctl->rzq[f.k & 1], t[0]);
+  }
+  return be;
+}
+
 template <class T, int MODE, int NC>
 __global__ void __launch_bounds__(kBlock) k_sweep(MgLevel L, const T* __restrict__ b, const double* mb,
                                                   const T* __restrict__ x, const double* mx, const T* ec,
                                                   const double* mec, int ncx, int ncy, int sx, int sy, int sz,
-                                                  T* __restrict__ xo, int fold, RedBuf rb, const SolveCtl* ctl,
-                                                  long long* stamps) {
-  if (ctl && ldA(&ctl->done)) return;
+                                                  T* __restrict__ xo, int fold, RedBuf rb, SolveCtl* ctl,
+                                                  long long* stamps, InFold conv) {
+  if (conv.n) {
+    if (inlineConv(conv, ctl)) return;
+  } else if (ctl && ldA(&ctl->done)) {
+    return;
+  }
   const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz);
   const bool t0 = threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0;
   if (stamps && t0) stamps[2 * q.blk] = wall_clock64();
@@ -335,9 +408,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(MgLevel L, const T* __restrict
     uint16_t code[NC];
     ldCodes<NC>(L.code, q.c, code);
     T xc[NC], xn[NC][6];
-    if constexpr (MODE == SW_PAIR) {
-      gatherN<T, NC>(L, FirstSweepView<T>{&L, b}, q.c, q.i, q.j, q.k, xc, xn);
-    } else if constexpr (MODE == SW_PROLONG) {
+    if constexpr (MODE == SW_PROLONG) {
       const XView<T, true> X{x, mx ? T(ldA(mx)) : T(0), mx != nullptr, ec, ncx, ncy, sx, sy, sz,
                              mec ? T(ldA(mec)) : T(0), mec != nullptr};
       gatherN<T, NC>(L, X, q.c, q.i, q.j, q.k, xc, xn);
@@ -384,12 +455,15 @@ __global__ void __launch_bounds__(kBlock) k_sweep(MgLevel L, const T* __restrict
 // holds both x-children of one coarse cell); a block covers whole coarse
 // cells (blockDim.z = fz, even x/y extents), stages its residuals in LDS, and
 // one thread per coarse cell sums the children in the reference's k, j, i
-// order.  ZEROX: x == 0 (restriction of b itself).
+// order.  ZEROX: x == 0 (restriction of b itself).  x1c (optional): the
+// coarse level's first sweep from zero, which is pointwise (k_sweep<SW_ZERO>'s
+// expression on the restricted value), so the coarse level starts at its
+// second sweep.
 template <class T, bool ZEROX, int NC>
 __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc, const T* __restrict__ b,
                                                            const double* mb, const T* __restrict__ x,
                                                            const double* mx, T* __restrict__ rc, int fold,
-                                                           RedBuf rb, const SolveCtl* ctl) {
+                                                           RedBuf rb, const SolveCtl* ctl, T* __restrict__ x1c) {
   if (ctl && ldA(&ctl->done)) return;
   __shared__ T rs[kBlock * NC];
   __shared__ unsigned char fls[kBlock * NC];
@@ -441,8 +515,10 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc
     const int I = i / Lc.fx, J = j / Lc.fy, K = k / Lc.fz;
     if (I < Lc.nx && J < Lc.ny && K < Lc.nz) {
       const int C = I + Lc.nx * (J + Lc.ny * K);
-      if (!(Lc.code[C] & kFluidBit)) {
+      const uint16_t cc = Lc.code[C];
+      if (!(cc & kFluidBit)) {
         rc[C] = T(0);
+        if (x1c) x1c[C] = T(0);
       } else {
         T s = T(0);
         int cnt = 0;
@@ -458,6 +534,7 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc
             }
         const T rv = s / T(cnt);
         rc[C] = rv;
+        if (x1c) x1c[C] = firstSweep<T>(Lc, cc, C, I, J, K, rv);
         acc[0] = double(rv);
       }
     }
@@ -1048,13 +1125,16 @@ struct PNewView {
 template <int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_apply_pu(MgLevel L, const double* __restrict__ z,
                                                          const double* __restrict__ pOld, double* __restrict__ pNew,
-                                                         double* __restrict__ Ap, const SolveCtl* ctl, RedBuf rb) {
+                                                         double* __restrict__ Ap, SolveCtl* ctl, RedBuf rb,
+                                                         InFold bf) {
+  // `done` is not written by this kernel: the per-thread read is uniform
   if (ldA(&ctl->done)) return;
+  const double be = bf.n ? inlineBeta(bf, ctl) : ldA(&ctl->beta);
   NBL_CELLS(NC);
   double acc[1] = {0.0};
   if (q.valid) {
     double pv[NC], av[NC];
-    applyN<NC>(L, q, code, PNewView{z, pOld, ldA(&ctl->beta)}, pv, av);
+    applyN<NC>(L, q, code, PNewView{z, pOld, be}, pv, av);
 #pragma unroll
     for (int e = 0; e < NC; ++e) acc[0] += pv[e] * av[e];
     stN<NC>(pNew, q.c, pv);
@@ -1102,9 +1182,14 @@ __global__ void __launch_bounds__(kBlock) k_fcg_update_first(MgLevel L, double* 
                                                              const double* __restrict__ r,
                                                              const double* __restrict__ Ap, double* __restrict__ rn,
                                                              T* __restrict__ x1, T* __restrict__ b0,
-                                                             const SolveCtl* ctl, RedBuf rb) {
-  if (ldA(&ctl->done)) return;
-  const double a = ldA(&ctl->alpha);
+                                                             SolveCtl* ctl, RedBuf rb, InFold af) {
+  double a;
+  if (af.n) {
+    if (!inlineAlpha(af, ctl, a)) return;
+  } else {
+    if (ldA(&ctl->done)) return;
+    a = ldA(&ctl->alpha);
+  }
   NBL_CELLS(NC);
   double acc[1] = {0.0};
   if (q.valid) {
@@ -1208,8 +1293,10 @@ __global__ void __launch_bounds__(kBlock) k_fcg_dots_x(MgLevel L, const T* __res
 template <class T, int NC>
 __global__ void __launch_bounds__(kBlock) k_sweep_dots(MgLevel L, const T* __restrict__ b, const T* __restrict__ x,
                                                        double* __restrict__ z, const double* __restrict__ rn,
-                                                       const double* __restrict__ r, RedBuf rb, const SolveCtl* ctl) {
+                                                       const double* __restrict__ r, RedBuf rb, SolveCtl* ctl,
+                                                       int countIt) {
   if (ldA(&ctl->done)) return;
+  if (countIt && firstBlock() && linearTid() == 0) stA(&ctl->it, ldA(&ctl->it) + 1);
   NBL_CELLS(NC);
   double acc[2] = {0.0, 0.0};
   if (q.valid) {
@@ -1383,6 +1470,7 @@ __global__ void __launch_bounds__(kFinThreads) k_fin(RedBuf rb, FinArgs a, Solve
       break;
     case FIN_RZ:
       rz = t[0];
+      stA(&ctl->rzq[0], rz);
       break;
     case FIN_ALPHA:
       pAp = t[0];
@@ -1452,7 +1540,9 @@ __global__ void k_ctl_init(SolveCtl* ctl, double tol) {
   if (t < 64) stA(&ctl->slot[t], 0.0);
   if (t == 0) {
     stA(&ctl->tol, tol);
-    for (double* f : {&ctl->bnorm, &ctl->rz, &ctl->pAp, &ctl->alpha, &ctl->beta, &ctl->rr, &ctl->resid}) stA(f, 0.0);
+    for (double* f : {&ctl->bnorm, &ctl->rz, &ctl->pAp, &ctl->alpha, &ctl->beta, &ctl->rr, &ctl->resid, &ctl->rzq[0],
+                      &ctl->rzq[1]})
+      stA(f, 0.0);
     for (int* f : {&ctl->it, &ctl->done, &ctl->converged, &ctl->broke, &ctl->early, &ctl->pad_}) stA(f, 0);
   }
 }
@@ -1475,7 +1565,7 @@ void launch_fin(RedBuf rb, const FinArgs& a, SolveCtl* ctl, hipStream_t s) {
 template <class T>
 void launch_mg_sweep(const MgLevel& L, const T* b, const double* mb, const T* x, const double* mx,
                      bool zeroGuess, const T* ec, const double* mec, const MgLevel* Lc, T* xo, bool fold,
-                     RedBuf rb, const SolveCtl* ctl, hipStream_t s, long long* stamps) {
+                     RedBuf rb, SolveCtl* ctl, hipStream_t s, long long* stamps, InFold conv) {
   const int ncx = Lc ? Lc->nx : 1, ncy = Lc ? Lc->ny : 1;
   const int sx = Lc ? Lc->fx - 1 : 0, sy = Lc ? Lc->fy - 1 : 0, sz = Lc ? Lc->fz - 1 : 0;
   const int f = fold ? 1 : 0;
@@ -1483,22 +1573,13 @@ void launch_mg_sweep(const MgLevel& L, const T* b, const double* mb, const T* x,
     constexpr int NC = decltype(nc)::value;
     if (zeroGuess)
       k_sweep<T, SW_ZERO, NC><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, nullptr, nullptr, 1, 1, 0, 0, 0, xo, f, rb,
-                                                         ctl, stamps);
+                                                         ctl, stamps, conv);
     else if (ec)
       k_sweep<T, SW_PROLONG, NC><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, ec, mec, ncx, ncy, sx, sy, sz, xo, f,
-                                                            rb, ctl, stamps);
+                                                            rb, ctl, stamps, conv);
     else
       k_sweep<T, SW_PLAIN, NC><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, nullptr, nullptr, 1, 1, 0, 0, 0, xo, f, rb,
-                                                          ctl, stamps);
-  });
-  NBL_LAUNCH_CHECK();
-}
-
-template <class T>
-void launch_mg_sweep_pair(const MgLevel& L, const T* b, T* xo, const SolveCtl* ctl, hipStream_t s) {
-  withCells(L, [&](auto nc, const Geo3& g) {
-    k_sweep<T, SW_PAIR, decltype(nc)::value><<<g.grid, g.block, 0, s>>>(
-        L, b, nullptr, nullptr, nullptr, nullptr, nullptr, 1, 1, 0, 0, 0, xo, 0, RedBuf{nullptr, 0}, ctl, nullptr);
+                                                          ctl, stamps, conv);
   });
   NBL_LAUNCH_CHECK();
 }
@@ -1527,12 +1608,13 @@ Geo3 restrictGeo(const MgLevel& L, const MgLevel& Lc) {
 
 template <class T>
 void launch_mg_resid_restrict(const MgLevel& L, const MgLevel& Lc, const T* b, const double* mb, const T* x,
-                              const double* mx, T* rc, bool fold, RedBuf rb, const SolveCtl* ctl, hipStream_t s) {
+                              const double* mx, T* rc, bool fold, RedBuf rb, const SolveCtl* ctl, hipStream_t s,
+                              T* x1c) {
   const Geo3 g = restrictGeo(L, Lc);
   if (restrictCells(L, Lc) == 2)
-    k_resid_restrict<T, false, 2><<<g.grid, g.block, 0, s>>>(L, Lc, b, mb, x, mx, rc, fold ? 1 : 0, rb, ctl);
+    k_resid_restrict<T, false, 2><<<g.grid, g.block, 0, s>>>(L, Lc, b, mb, x, mx, rc, fold ? 1 : 0, rb, ctl, x1c);
   else
-    k_resid_restrict<T, false, 1><<<g.grid, g.block, 0, s>>>(L, Lc, b, mb, x, mx, rc, fold ? 1 : 0, rb, ctl);
+    k_resid_restrict<T, false, 1><<<g.grid, g.block, 0, s>>>(L, Lc, b, mb, x, mx, rc, fold ? 1 : 0, rb, ctl, x1c);
   NBL_LAUNCH_CHECK();
 }
 
@@ -1596,17 +1678,16 @@ void launch_mg_coarse_direct(const MgLevel& L, const double* M, const float* Mf,
 
 #define NBL_INST(T)                                                                                          \
   template void launch_mg_sweep<T>(const MgLevel&, const T*, const double*, const T*, const double*, bool,    \
-                                   const T*, const double*, const MgLevel*, T*, bool, RedBuf, const SolveCtl*, \
-                                   hipStream_t, long long*);                                                 \
+                                   const T*, const double*, const MgLevel*, T*, bool, RedBuf, SolveCtl*,     \
+                                   hipStream_t, long long*, InFold);                                         \
   template void launch_mg_resid_restrict<T>(const MgLevel&, const MgLevel&, const T*, const double*,         \
                                             const T*, const double*, T*, bool, RedBuf, const SolveCtl*,      \
-                                            hipStream_t);                                                    \
+                                            hipStream_t, T*);                                                \
   template void launch_mg_coarse<T>(const MgLevel&, bool, const T*, const double*, T*, T*,                   \
                                     const SolveCtl*, hipStream_t);                                           \
   template void launch_mg_finish<T>(const MgLevel&, const T*, const double*, double*, bool, RedBuf,          \
                                     const SolveCtl*, hipStream_t);                                           \
   template void launch_to_level0<T>(const MgLevel&, const double*, T*, const SolveCtl*, hipStream_t);        \
-  template void launch_mg_sweep_pair<T>(const MgLevel&, const T*, T*, const SolveCtl*, hipStream_t);        \
   template void launch_mg_coarse_direct<T>(const MgLevel&, const double*, const float*, const T*,            \
                                            const double*, T*, const SolveCtl*, hipStream_t);
 NBL_INST(double)
@@ -1628,10 +1709,10 @@ void launch_restrict_plain(const MgLevel& L, const MgLevel& Lc, const double* rf
   const Geo3 g = restrictGeo(L, Lc);
   if (restrictCells(L, Lc) == 2)
     k_resid_restrict<double, true, 2><<<g.grid, g.block, 0, s>>>(L, Lc, rf, nullptr, nullptr, nullptr, rc, 0,
-                                                                 RedBuf{nullptr, 0}, nullptr);
+                                                                 RedBuf{nullptr, 0}, nullptr, nullptr);
   else
     k_resid_restrict<double, true, 1><<<g.grid, g.block, 0, s>>>(L, Lc, rf, nullptr, nullptr, nullptr, rc, 0,
-                                                                 RedBuf{nullptr, 0}, nullptr);
+                                                                 RedBuf{nullptr, 0}, nullptr, nullptr);
   NBL_LAUNCH_CHECK();
 }
 void launch_prolong_plain(const MgLevel& L, const MgLevel& Lc, const double* ec, double* xf, hipStream_t s) {
@@ -1677,35 +1758,35 @@ void launch_fcg_update(const MgLevel& L, bool singular, double* x, const double*
 }
 template <class T>
 void launch_fcg_update_first(const MgLevel& L, double* x, const double* p, const double* r, const double* Ap,
-                             double* rn, T* x1, T* b0, const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
+                             double* rn, T* x1, T* b0, SolveCtl* ctl, RedBuf rb, hipStream_t s, InFold alpha) {
   withCells(L, [&](auto nc, const Geo3& g) {
-    k_fcg_update_first<T, decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, x, p, r, Ap, rn, x1, b0, ctl, rb);
+    k_fcg_update_first<T, decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, x, p, r, Ap, rn, x1, b0, ctl, rb, alpha);
   });
   NBL_LAUNCH_CHECK();
 }
 template void launch_fcg_update_first<double>(const MgLevel&, double*, const double*, const double*, const double*,
-                                              double*, double*, double*, const SolveCtl*, RedBuf, hipStream_t);
+                                              double*, double*, double*, SolveCtl*, RedBuf, hipStream_t, InFold);
 template void launch_fcg_update_first<float>(const MgLevel&, double*, const double*, const double*, const double*,
-                                             double*, float*, float*, const SolveCtl*, RedBuf, hipStream_t);
+                                             double*, float*, float*, SolveCtl*, RedBuf, hipStream_t, InFold);
 void launch_fcg_apply_pu(const MgLevel& L, const double* z, const double* pOld, double* pNew, double* Ap,
-                         const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
+                         SolveCtl* ctl, RedBuf rb, hipStream_t s, InFold beta) {
   withCells(L, [&](auto nc, const Geo3& g) {
-    k_fcg_apply_pu<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, z, pOld, pNew, Ap, ctl, rb);
+    k_fcg_apply_pu<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, z, pOld, pNew, Ap, ctl, rb, beta);
   });
   NBL_LAUNCH_CHECK();
 }
 template <class T>
 void launch_mg_sweep_dots(const MgLevel& L, const T* b, const T* x, double* z, const double* rn, const double* r,
-                          RedBuf rb, const SolveCtl* ctl, hipStream_t s) {
+                          RedBuf rb, SolveCtl* ctl, hipStream_t s, bool countIt) {
   withCells(L, [&](auto nc, const Geo3& g) {
-    k_sweep_dots<T, decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, b, x, z, rn, r, rb, ctl);
+    k_sweep_dots<T, decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, b, x, z, rn, r, rb, ctl, countIt ? 1 : 0);
   });
   NBL_LAUNCH_CHECK();
 }
 template void launch_mg_sweep_dots<double>(const MgLevel&, const double*, const double*, double*, const double*,
-                                           const double*, RedBuf, const SolveCtl*, hipStream_t);
+                                           const double*, RedBuf, SolveCtl*, hipStream_t, bool);
 template void launch_mg_sweep_dots<float>(const MgLevel&, const float*, const float*, double*, const double*,
-                                          const double*, RedBuf, const SolveCtl*, hipStream_t);
+                                          const double*, RedBuf, SolveCtl*, hipStream_t, bool);
 void launch_fcg_norm(const MgLevel& L, double* rn, const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
   withCells(L, [&](auto nc, const Geo3& g) {
     k_fcg_norm<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, rn, ctl, rb);

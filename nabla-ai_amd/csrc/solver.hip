@@ -403,29 +403,31 @@ void PoissonDev::vcycle(int l, const T* b, const double* mb, const T** xout, con
   int cur = -1;
   int q0 = 0;
   const bool fusedL0 = l == 0 && l0Fused_;
-  if (fusedL0) {
-    // the first sweep from zero was written into bufs[0] with the residual
+  // Without projections the first sweep from zero is pointwise: the level-0
+  // update and each restriction write it with the right-hand side.
+  const bool firstDone = !project_ && opts_.pre_smooth >= 1 && (fusedL0 || l > 0);
+  if (firstDone) {
     cur = 0;
     q0 = 1;
-  } else if (!project_ && opts_.pre_smooth >= 2 && L.n < (1LL << 18)) {
-    // sweeps 1 and 2 in one pass where launches, not arithmetic, bound the
-    // level (the pass re-evaluates the pointwise first sweep at every stencil
-    // point)
-    launch_mg_sweep_pair<T>(L, b, bufs[0], ctl, s);
-    cur = 0;
-    q0 = 2;
   }
   for (int q = q0; q < opts_.pre_smooth; ++q) {
     const int o = cur == 0 ? 1 : 0;
+    InFold conv{};
+    if (l == 0 && conv_.n) {
+      conv = conv_;  // the flexible CG's convergence test rides on this sweep
+      conv_ = InFold{};
+    }
     launch_mg_sweep<T>(L, b, mb, cur < 0 ? nullptr : bufs[cur], cur < 0 ? nullptr : ms[cur], cur < 0, nullptr,
-                       nullptr, nullptr, bufs[o], project_, rb, ctl, s, timedSweep(l, cur >= 0));
+                       nullptr, nullptr, bufs[o], project_, rb, ctl, s, timedSweep(l, cur >= 0), conv);
     if (project_) fin(FIN_MEAN, 1, blocks_[l], slotX(l, o), L.nf, true, s);
     cur = o;
   }
   const MgLevel& Lc = dl_[l + 1];
   T* bc = lb_[l + 1].as<T>();
   double* mbc = project_ ? slotPtr(ctl, slotB(l + 1)) : nullptr;
-  launch_mg_resid_restrict<T>(L, Lc, b, mb, bufs[cur], ms[cur], bc, project_, rb, ctl, s);
+  // the coarse level's first sweep, unless the coarse level is the coarsest
+  T* x1c = (!project_ && opts_.pre_smooth >= 1 && l + 2 < nlevels()) ? lx0_[l + 1].as<T>() : nullptr;
+  launch_mg_resid_restrict<T>(L, Lc, b, mb, bufs[cur], ms[cur], bc, project_, rb, ctl, s, x1c);
   if (project_) fin(FIN_MEAN, 1, rblocks_[l + 1], slotB(l + 1), Lc.nf, true, s);
   const T* ec = nullptr;
   const double* mec = nullptr;
@@ -434,7 +436,7 @@ void PoissonDev::vcycle(int l, const T* b, const double* mb, const T** xout, con
     const int o = 1 - cur;
     if (fusedL0 && q == opts_.post_smooth - 1 && q > 0) {
       // last sweep: writes z and the flexible-CG dots instead of x
-      launch_mg_sweep_dots<T>(L, b, bufs[cur], z_.as<double>(), l0Rn_, l0R_, rb, ctl, s);
+      launch_mg_sweep_dots<T>(L, b, bufs[cur], z_.as<double>(), l0Rn_, l0R_, rb, ctl, s, inline_);
       *xout = nullptr;
       *mxout = nullptr;
       return;
@@ -485,22 +487,30 @@ void PoissonDev::minv(const double* r, hipStream_t s, const double* rOld, bool f
 // device has set ctl->done.  Iteration k >= 1 forms p = z + beta p of the
 // previous iteration inside its A p, so p alternates between two buffers,
 // and the flexible CG's r / r_new alternate instead of being copied.
+// With inline_ the three finalizes of an iteration run inside their
+// consumers: beta in the next A p, alpha in the update, the convergence test
+// in the first level-0 sweep; the partial rows then alternate so that no
+// kernel overwrites the rows it folds (z.r_new, z.r: rows 0-1; p.Ap: row 2;
+// |r_new|^2: row 3).
 void PoissonDev::enqueueIteration(hipStream_t s, int k) {
   SolveCtl* ctl = ctl_.as<SolveCtl>();
   const MgLevel& L = dl_[0];
   const RedBuf rb = red();
   const int nb = blocks_[0];
+  const RedBuf rbP = inline_ ? RedBuf{rb.partials + 2 * size_t(rb.stride), rb.stride} : rb;
+  const RedBuf rbR = inline_ ? RedBuf{rb.partials + 3 * size_t(rb.stride), rb.stride} : rb;
+  auto fold = [&](const RedBuf& r) { return inline_ ? InFold{r.partials, r.stride, nb, k} : InFold{}; };
   double* x = x_.as<double>();
   double* z = z_.as<double>();
   double* Ap = Ap_.as<double>();
   double* p = k == 0 ? p_.as<double>() : ((k & 1) ? p2_.as<double>() : p_.as<double>());
   if (k == 0) {
-    launch_fcg_apply(L, p, Ap, ctl, rb, s);
+    launch_fcg_apply(L, p, Ap, ctl, rbP, s);
   } else {
     const double* pOld = (k & 1) ? p_.as<double>() : p2_.as<double>();
-    launch_fcg_apply_pu(L, z, pOld, p, Ap, ctl, rb, s);
+    launch_fcg_apply_pu(L, z, pOld, p, Ap, ctl, rbP, s, fold(rb));
   }
-  fin(FIN_ALPHA, 1, nb, 0, 0, true, s);
+  if (!inline_) fin(FIN_ALPHA, 1, nb, 0, 0, true, s);
   if (kind_ == 0) {
     double* r = (k & 1) ? rn_.as<double>() : r_.as<double>();
     double* rn = (k & 1) ? r_.as<double>() : rn_.as<double>();
@@ -515,15 +525,19 @@ void PoissonDev::enqueueIteration(hipStream_t s, int k) {
       // fusions need a first pre-sweep and a coarser level to smooth above
       const bool fused = nlevels() > 1 && opts_.pre_smooth >= 1;
       if (fused && mixed_)
-        launch_fcg_update_first<float>(L, x, p, r, Ap, rn, lx0_[0].as<float>(), b0f_.as<float>(), ctl, rb, s);
+        launch_fcg_update_first<float>(L, x, p, r, Ap, rn, lx0_[0].as<float>(), b0f_.as<float>(), ctl, rbR, s,
+                                       fold(rbP));
       else if (fused)
-        launch_fcg_update_first<double>(L, x, p, r, Ap, rn, lx0_[0].as<double>(), nullptr, ctl, rb, s);
+        launch_fcg_update_first<double>(L, x, p, r, Ap, rn, lx0_[0].as<double>(), nullptr, ctl, rbR, s,
+                                        fold(rbP));
       else
         launch_fcg_update(L, false, x, p, r, Ap, rn, ctl, rb, s);
-      fin(FIN_CONV, 1, nb, 0, 0, true, s);
+      if (inline_) conv_ = fold(rbR);
+      else fin(FIN_CONV, 1, nb, 0, 0, true, s);
       minv(rn, s, r, fused);
+      if (conv_.n) throw std::logic_error("flexible CG: convergence test not placed");
     }
-    fin(FIN_BETA_FLEX, 2, nb, 0, 0, true, s);
+    if (!inline_) fin(FIN_BETA_FLEX, 2, nb, 0, 0, true, s);
   } else {
     double* r = r_.as<double>();
     launch_cg_update(L, x, p, r, Ap, ctl, rb, s);
@@ -566,6 +580,9 @@ nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t
   launch_ctl_init(ctl, tol_, s);
   NBL_HIP(hipEventRecord(e0_, s));
   vcount_ = 0;
+  inline_ = kind_ == 0 && !project_ && nlevels() > 1 && opts_.pre_smooth >= 2 && opts_.post_smooth >= 2 &&
+            nb <= kInlineFoldMax;
+  conv_ = InFold{};
   double* x = x_.as<double>();
   double* rhs = rhs_.as<double>();
   double* r = r_.as<double>();

@@ -145,6 +145,12 @@ class PoissonDev {
   bool l0Fused_ = false;
   const double* l0Rn_ = nullptr;
   const double* l0R_ = nullptr;
+  // Finalizes folded into their consumers for the current solve (flexible CG
+  // without projections, few enough level-0 blocks); conv_: the convergence
+  // test the V-cycle's first level-0 sweep performs.
+  static constexpr int kInlineFoldMax = 4096;
+  bool inline_ = false;
+  InFold conv_{};
   DevBuf ctl_, red_;
   int redStride_ = 1;
   SolveCtl* hctl_ = nullptr;  // pinned mirror
