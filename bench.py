@@ -341,7 +341,7 @@ def served_runs(nruns, seed):
     return out
 
 
-def served(ctx, window_s, rate, nruns=32, slots=4, seed=777, sustain_window_s=10.0):
+def served(ctx, window_s, rate, nruns=32, slots=4, seed=777, sustain_window_s=10.0, precision=None):
     """Poisson arrivals at `rate` requests/s over `window_s` seconds, each one
     RK3 step of a uniformly drawn run; latency = submit to completion.  The
     same server then takes a second stream at 70% of the request rate it
@@ -351,8 +351,9 @@ def served(ctx, window_s, rate, nruns=32, slots=4, seed=777, sustain_window_s=10
     srv = nbl.Server(ctx, slots=slots, hbm_budget=200 << 30)
     t0 = time.perf_counter()
     ids, cells = [], []
+    opts = nbl.solver_opts(precision=precision) if precision is not None else None
     for name, g, cfg, st in runs:
-        rid = srv.admit(g, cfg)
+        rid = srv.admit(g, cfg, opts)
         srv.run(rid, g).set_state(*st)
         ids.append(rid)
         cells.append(g.cells)

@@ -361,9 +361,9 @@ int nbl_poisson_smooth_once(nbl_poisson* ps, const double* b, double* x) {
     nbl::DevBuf xo(ps->ctx, n * 8);
     NBL_HIP(hipMemcpyAsync(st.a.as<void>(), b, n * 8, hipMemcpyHostToDevice, ps->stream));
     NBL_HIP(hipMemcpyAsync(st.b.as<void>(), x, n * 8, hipMemcpyHostToDevice, ps->stream));
-    nbl::launch_mg_sweep<double>(ps->impl->level(0), st.a.as<double>(), nullptr, st.b.as<double>(), nullptr, false,
-                                 nullptr, nullptr, nullptr, xo.as<double>(), false, ps->impl->red(), nullptr,
-                                 ps->stream);
+    nbl::launch_mg_sweep<double>(ps->impl->level(0), st.a.as<double>(), nbl::MeanRef{}, st.b.as<double>(),
+                                 nbl::MeanRef{}, false, nullptr, nbl::MeanRef{}, nullptr, xo.as<double>(), false,
+                                 ps->impl->red(), nullptr, ps->stream);
     NBL_HIP(hipMemcpyAsync(x, xo.as<void>(), n * 8, hipMemcpyDeviceToHost, ps->stream));
     NBL_HIP(hipStreamSynchronize(ps->stream));
   });

@@ -53,6 +53,23 @@ struct SolveCtl {
   double slot[64];  // reduction results / pending means
 };
 
+// A pending null-space mean (fluid sum / nf) that a kernel subtracts on read:
+// either finalized into a SolveCtl slot by k_fin (`slot`), or left as the
+// producer's `n` per-block partial sums, which every block of the consumer
+// folds itself (`parts`, as InFold).  Neither: no mean.
+struct MeanRef {
+  const double* slot = nullptr;
+  const double* parts = nullptr;
+  int n = 0;
+  double nf = 1.0;
+  __host__ __device__ bool on() const { return slot != nullptr || parts != nullptr; }
+};
+inline MeanRef slotMean(const double* slot) {
+  MeanRef m;
+  m.slot = slot;
+  return m;
+}
+
 // A finalize folded into the kernel that consumes it, in place of a one-block
 // k_fin launch (flexible CG on systems without projections): every block of
 // the consumer folds the previous kernel's `n` per-block partials (rows of
@@ -180,15 +197,15 @@ void launch_set_step(StepCtl* sc, int step, double t, double dt, hipStream_t s);
 // convergence test on the partials of |r_new|^2 folded in (the sweep does
 // nothing when it stops the solve).
 template <class T>
-void launch_mg_sweep(const MgLevel& L, const T* b, const double* mb, const T* x, const double* mx,
-                     bool zeroGuess, const T* ec, const double* mec, const MgLevel* Lc, T* xo, bool fold,
+void launch_mg_sweep(const MgLevel& L, const T* b, MeanRef mb, const T* x, MeanRef mx,
+                     bool zeroGuess, const T* ec, MeanRef mec, const MgLevel* Lc, T* xo, bool fold,
                      RedBuf rb, SolveCtl* ctl, hipStream_t s, long long* stamps = nullptr, InFold conv = {});
 // x1c (optional): the coarse level's first sweep from a zero guess, written
 // beside the restricted residual (pointwise: no projection may come between,
 // so non-singular systems only).
 template <class T>
-void launch_mg_resid_restrict(const MgLevel& L, const MgLevel& Lc, const T* b, const double* mb,
-                              const T* x, const double* mx, T* rc, bool fold, RedBuf rb,
+void launch_mg_resid_restrict(const MgLevel& L, const MgLevel& Lc, const T* b, MeanRef mb,
+                              const T* x, MeanRef mx, T* rc, bool fold, RedBuf rb,
                               const SolveCtl* ctl, hipStream_t s, T* x1c = nullptr);
 template <class T>
 void launch_mg_coarse(const MgLevel& L, bool singular, const T* b, const double* mb, T* x,
@@ -201,10 +218,10 @@ void launch_gj_step(double* G, int n, int k, double* colk, double* rowk, int* fa
 void launch_inv_store(const double* G, int n, const uint16_t* code, long long nf, bool singular, double* scratch,
                       double* M, float* Mf, hipStream_t s);
 template <class T>
-void launch_mg_coarse_direct(const MgLevel& L, const double* M, const float* Mf, const T* b, const double* mb, T* x,
+void launch_mg_coarse_direct(const MgLevel& L, const double* M, const float* Mf, const T* b, MeanRef mb, T* x,
                              const SolveCtl* ctl, hipStream_t s);
 template <class T>
-void launch_mg_finish(const MgLevel& L, const T* x, const double* mx, double* z, bool fold,
+void launch_mg_finish(const MgLevel& L, const T* x, MeanRef mx, double* z, bool fold,
                       RedBuf rb, const SolveCtl* ctl, hipStream_t s);
 template <class T>
 void launch_to_level0(const MgLevel& L, const double* r, T* b, const SolveCtl* ctl, hipStream_t s);
@@ -222,12 +239,12 @@ void launch_fcg_init(const MgLevel& L, bool maskSolid, const double* b, double* 
                      hipStream_t s);
 void launch_fcg_init2(const MgLevel& L, bool singular, double* rhs, double* r, const SolveCtl* ctl, RedBuf rb,
                       hipStream_t s);
-void launch_fcg_start(const MgLevel& L, double* z, const double* mz, const double* r, double* p,
+void launch_fcg_start(const MgLevel& L, double* z, MeanRef mz, const double* r, double* p,
                       const SolveCtl* ctl, RedBuf rb, hipStream_t s);
 void launch_fcg_apply(const MgLevel& L, const double* p, double* Ap, const SolveCtl* ctl, RedBuf rb,
                       hipStream_t s);
 void launch_fcg_update(const MgLevel& L, bool singular, double* x, const double* p, const double* r,
-                       const double* Ap, double* rn, const SolveCtl* ctl, RedBuf rb, hipStream_t s);
+                       const double* Ap, double* rn, SolveCtl* ctl, RedBuf rb, hipStream_t s, InFold alpha = {});
 // Non-singular systems: the update plus the next V-cycle's first level-0 sweep
 // from zero (x1) and, for an FP32 cycle, its right-hand side b0 = float(rn).
 // alpha: alpha = rz / p.Ap folded in from the partials of p.Ap.
@@ -243,9 +260,10 @@ void launch_fcg_apply_pu(const MgLevel& L, const double* z, const double* pOld, 
 template <class T>
 void launch_mg_sweep_dots(const MgLevel& L, const T* b, const T* x, double* z, const double* rn, const double* r,
                           RedBuf rb, SolveCtl* ctl, hipStream_t s, bool countIt = false);
-void launch_fcg_norm(const MgLevel& L, double* rn, const SolveCtl* ctl, RedBuf rb, hipStream_t s);
-void launch_fcg_dots(const MgLevel& L, double* z, const double* mz, const double* rn, const double* r,
-                     const SolveCtl* ctl, RedBuf rb, hipStream_t s);
+// rn <- rn - mean (fluid) / 0 (solid), partials of |rn|^2.
+void launch_fcg_norm(const MgLevel& L, double* rn, MeanRef mrn, const SolveCtl* ctl, RedBuf rb, hipStream_t s);
+void launch_fcg_dots(const MgLevel& L, double* z, MeanRef mz, const double* rn, const double* r,
+                     SolveCtl* ctl, RedBuf rb, hipStream_t s, bool countIt = false);
 // k_finish fused into k_fcg_dots (no pending means): z = V-cycle output x.
 template <class T>
 void launch_fcg_dots_x(const MgLevel& L, const T* x, double* z, const double* rn, const double* r,

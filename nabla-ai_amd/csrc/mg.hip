@@ -328,6 +328,39 @@ __device__ __forceinline__ void foldInline(const InFold& f, double (&out)[K]) {
   for (int q = 0; q < K; ++q) out[q] = res[q];
 }
 
+// Values of K pending means, in every thread (block-wide: every thread of the
+// block calls it).  The folded ones share one pass over their partials.
+template <int K>
+__device__ __forceinline__ void resolveMeans(const MeanRef (&m)[K], double (&v)[K]) {
+  bool fold = false;
+#pragma unroll
+  for (int q = 0; q < K; ++q) fold = fold || m[q].parts != nullptr;
+  if (fold) {
+    __shared__ double sh[K * 16];
+    __shared__ double res[K];
+    const int tid = linearTid(), nt = blockDim.x * blockDim.y * blockDim.z;
+    double a[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      a[q] = 0.0;
+      if (m[q].parts)
+        for (int b = tid; b < m[q].n; b += nt) a[q] += m[q].parts[b];
+    }
+    blockFold<K, 0u>(a, sh);
+    if (tid == 0)
+      for (int q = 0; q < K; ++q) res[q] = a[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < K; ++q) a[q] = res[q];
+#pragma unroll
+    for (int q = 0; q < K; ++q)
+      v[q] = m[q].parts ? (m[q].nf > 0.0 ? a[q] / m[q].nf : 0.0) : (m[q].slot ? ldA(m[q].slot) : 0.0);
+  } else {
+#pragma unroll
+    for (int q = 0; q < K; ++q) v[q] = m[q].slot ? ldA(m[q].slot) : 0.0;
+  }
+}
+
 __device__ __forceinline__ bool blockStopped(const SolveCtl* ctl) {
   __shared__ int stop;
   if (linearTid() == 0) stop = ldA(&ctl->done);
@@ -390,30 +423,31 @@ __device__ __forceinline__ double inlineBeta(const InFold& f, SolveCtl* ctl) {
 }
 
 template <class T, int MODE, int NC>
-__global__ void __launch_bounds__(kBlock) k_sweep(MgLevel L, const T* __restrict__ b, const double* mb,
-                                                  const T* __restrict__ x, const double* mx, const T* ec,
-                                                  const double* mec, int ncx, int ncy, int sx, int sy, int sz,
+__global__ void __launch_bounds__(kBlock) k_sweep(MgLevel L, const T* __restrict__ b, MeanRef mb,
+                                                  const T* __restrict__ x, MeanRef mx, const T* ec,
+                                                  MeanRef mec, int ncx, int ncy, int sx, int sy, int sz,
                                                   T* __restrict__ xo, int fold, RedBuf rb, SolveCtl* ctl,
                                                   long long* stamps, InFold conv) {
+  const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz);
+  const bool t0 = threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0;
+  if (stamps && t0) stamps[2 * q.blk] = wall_clock64();
   if (conv.n) {
     if (inlineConv(conv, ctl)) return;
   } else if (ctl && ldA(&ctl->done)) {
     return;
   }
-  const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz);
-  const bool t0 = threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0;
-  if (stamps && t0) stamps[2 * q.blk] = wall_clock64();
+  double mv[3];
+  resolveMeans<3>({mb, mx, mec}, mv);
   double acc[1] = {0.0};
   if (q.valid) {
     uint16_t code[NC];
     ldCodes<NC>(L.code, q.c, code);
     T xc[NC], xn[NC][6];
     if constexpr (MODE == SW_PROLONG) {
-      const XView<T, true> X{x, mx ? T(ldA(mx)) : T(0), mx != nullptr, ec, ncx, ncy, sx, sy, sz,
-                             mec ? T(ldA(mec)) : T(0), mec != nullptr};
+      const XView<T, true> X{x, T(mv[1]), mx.on(), ec, ncx, ncy, sx, sy, sz, T(mv[2]), mec.on()};
       gatherN<T, NC>(L, X, q.c, q.i, q.j, q.k, xc, xn);
     } else if constexpr (MODE == SW_PLAIN) {
-      const XView<T> X{x, mx ? T(ldA(mx)) : T(0), mx != nullptr, nullptr, 1, 1, 0, 0, 0, T(0), false};
+      const XView<T> X{x, T(mv[1]), mx.on(), nullptr, 1, 1, 0, 0, 0, T(0), false};
       gatherN<T, NC>(L, X, q.c, q.i, q.j, q.k, xc, xn);
     }
     T bv[NC], out[NC];
@@ -435,7 +469,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(MgLevel L, const T* __restrict
       }
       const T diag = dg > T(0) ? dg : T(1);
       T bb = bv[e];
-      if (mb) bb = bb - T(ldA(mb));
+      if (mb.on()) bb = bb - T(mv[0]);
       const T xnew = x0 + T(0.8) * (bb - Ax) / diag;
       out[e] = xnew;
       acc[0] += double(xnew);
@@ -461,18 +495,20 @@ __global__ void __launch_bounds__(kBlock) k_sweep(MgLevel L, const T* __restrict
 // second sweep.
 template <class T, bool ZEROX, int NC>
 __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc, const T* __restrict__ b,
-                                                           const double* mb, const T* __restrict__ x,
-                                                           const double* mx, T* __restrict__ rc, int fold,
-                                                           RedBuf rb, const SolveCtl* ctl, T* __restrict__ x1c) {
-  if (ctl && ldA(&ctl->done)) return;
+                                                           MeanRef mb, const T* __restrict__ x, MeanRef mx,
+                                                           T* __restrict__ rc, int fold, RedBuf rb,
+                                                           const SolveCtl* ctl, T* __restrict__ x1c) {
   __shared__ T rs[kBlock * NC];
   __shared__ unsigned char fls[kBlock * NC];
-  const XView<T> X{x, mx ? T(ldA(mx)) : T(0), mx != nullptr, nullptr, 1, 1, 0, 0, 0, T(0), false};
   const Blk3 B = blk3();
   const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
   const int bx = blockDim.x, by = blockDim.y;
   const int i = NC * (B.x * bx + tx), j = B.y * by + ty, k = B.z * int(blockDim.z) + tz;
   const int tid = tx + bx * (ty + by * tz);
+  if (ctl && ldA(&ctl->done)) return;
+  double mv[2];
+  resolveMeans<2>({mb, mx}, mv);
+  const XView<T> X{x, T(mv[1]), mx.on(), nullptr, 1, 1, 0, 0, 0, T(0), false};
   T r[NC];
   bool fl[NC];
 #pragma unroll
@@ -499,7 +535,7 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc
         Ax = applyDecode<T, false>(L, code[e], xc[e], xn[e], dg);
       }
       T bb = bv[e];
-      if (mb) bb = bb - T(ldA(mb));
+      if (mb.on()) bb = bb - T(mv[0]);
       r[e] = bb - Ax;
     }
   }
@@ -782,18 +818,20 @@ __global__ void k_inv_store(const double* __restrict__ G, int n, const uint16_t*
 // striding the columns; fixed-order block fold.
 template <class T, class MT, int ROWS>
 __global__ void __launch_bounds__(kBlock) k_coarse_direct(MgLevel L, const MT* __restrict__ M,
-                                                          const T* __restrict__ b, const double* mb,
+                                                          const T* __restrict__ b, MeanRef mb,
                                                           T* __restrict__ x, const SolveCtl* ctl) {
   if (ctl && ldA(&ctl->done)) return;
   const int n = int(L.n);
   const int r0 = blockIdx.x * ROWS;
-  const double m = mb ? ldA(mb) : 0.0;
+  double mv[1];
+  resolveMeans<1>({mb}, mv);
+  const double m = mv[0];
   double acc[ROWS];
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) acc[r] = 0.0;
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
     T bj = b[j];
-    if (mb && (L.code[j] & kFluidBit)) bj = bj - T(m);
+    if (mb.on() && (L.code[j] & kFluidBit)) bj = bj - T(m);
     const double bv = double(bj);
 #pragma unroll
     for (int r = 0; r < ROWS; ++r)
@@ -925,10 +963,12 @@ __global__ void __launch_bounds__(1024) k_coarse_global(MgLevel L, int singular,
 // Level-0 preconditioner output: z = x - pending mean (fluid), 0 elsewhere,
 // with fluid partial sums for Minv's own projection (multigrid.cpp:283-289).
 template <class T, int NC>
-__global__ void __launch_bounds__(kBlock) k_finish(MgLevel L, const T* __restrict__ x, const double* mx,
+__global__ void __launch_bounds__(kBlock) k_finish(MgLevel L, const T* __restrict__ x, MeanRef mx,
                                                    double* __restrict__ z, int fold, RedBuf rb,
                                                    const SolveCtl* ctl) {
   if (ctl && ldA(&ctl->done)) return;
+  double mv[1];
+  resolveMeans<1>({mx}, mv);
   const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
   if (q.valid) {
@@ -941,7 +981,7 @@ __global__ void __launch_bounds__(kBlock) k_finish(MgLevel L, const T* __restric
     for (int e = 0; e < NC; ++e) {
       const bool fl = code[e] & kFluidBit;
       T v = xv[e];
-      if (mx && fl) v = v - T(ldA(mx));
+      if (mx.on() && fl) v = v - T(mv[0]);
       zv[e] = fl ? double(v) : 0.0;
       acc[0] += zv[e];
     }
@@ -1049,10 +1089,12 @@ __global__ void __launch_bounds__(kBlock) k_fcg_init2(MgLevel L, int singular, d
 
 // z <- z - pending mean (fluid); p = z; r.z
 template <int NC>
-__global__ void __launch_bounds__(kBlock) k_fcg_start(MgLevel L, double* __restrict__ z, const double* mz,
+__global__ void __launch_bounds__(kBlock) k_fcg_start(MgLevel L, double* __restrict__ z, MeanRef mz,
                                                       const double* __restrict__ r, double* __restrict__ p,
                                                       const SolveCtl* ctl, RedBuf rb) {
   if (ldA(&ctl->done)) return;
+  double mv[1];
+  resolveMeans<1>({mz}, mv);
   NBL_CELLS(NC);
   double acc[1] = {0.0};
   if (q.valid) {
@@ -1061,7 +1103,7 @@ __global__ void __launch_bounds__(kBlock) k_fcg_start(MgLevel L, double* __restr
     ldN<NC>(r, q.c, rv);
 #pragma unroll
     for (int e = 0; e < NC; ++e) {
-      if (mz && (code[e] & kFluidBit)) v[e] = v[e] - ldA(mz);
+      if (mz.on() && (code[e] & kFluidBit)) v[e] = v[e] - mv[0];
       acc[0] += rv[e] * v[e];
     }
     stN<NC>(z, q.c, v);
@@ -1148,9 +1190,14 @@ template <int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_update(MgLevel L, int singular, double* __restrict__ x,
                                                        const double* __restrict__ p, const double* __restrict__ r,
                                                        const double* __restrict__ Ap, double* __restrict__ rn,
-                                                       const SolveCtl* ctl, RedBuf rb) {
-  if (ldA(&ctl->done)) return;
-  const double a = ldA(&ctl->alpha);
+                                                       SolveCtl* ctl, RedBuf rb, InFold af) {
+  double a;
+  if (af.n) {
+    if (!inlineAlpha(af, ctl, a)) return;
+  } else {
+    if (ldA(&ctl->done)) return;
+    a = ldA(&ctl->alpha);
+  }
   NBL_CELLS(NC);
   double acc[1] = {0.0};
   if (q.valid) {
@@ -1216,13 +1263,15 @@ __global__ void __launch_bounds__(kBlock) k_fcg_update_first(MgLevel L, double* 
 }
 
 template <int NC>
-__global__ void __launch_bounds__(kBlock) k_fcg_norm(MgLevel L, double* __restrict__ rn, const SolveCtl* ctl,
-                                                     RedBuf rb) {
+__global__ void __launch_bounds__(kBlock) k_fcg_norm(MgLevel L, double* __restrict__ rn, MeanRef mrn,
+                                                     const SolveCtl* ctl, RedBuf rb) {
   if (ldA(&ctl->done)) return;
+  double mv[1];
+  resolveMeans<1>({mrn}, mv);
   NBL_CELLS(NC);
   double acc[1] = {0.0};
   if (q.valid) {
-    const double m = ldA(&ctl->slot[S_MEAN_RN]);
+    const double m = mv[0];
     double v[NC];
     ldN<NC>(rn, q.c, v);
 #pragma unroll
@@ -1238,10 +1287,13 @@ __global__ void __launch_bounds__(kBlock) k_fcg_norm(MgLevel L, double* __restri
 // z <- z - pending mean; z.rn and z.r (flexible Polak-Ribiere beta,
 // multigrid.cpp:328-339)
 template <int NC>
-__global__ void __launch_bounds__(kBlock) k_fcg_dots(MgLevel L, double* __restrict__ z, const double* mz,
+__global__ void __launch_bounds__(kBlock) k_fcg_dots(MgLevel L, double* __restrict__ z, MeanRef mz,
                                                      const double* __restrict__ rn, const double* __restrict__ r,
-                                                     const SolveCtl* ctl, RedBuf rb) {
+                                                     SolveCtl* ctl, RedBuf rb, int countIt) {
   if (ldA(&ctl->done)) return;
+  if (countIt && firstBlock() && linearTid() == 0) stA(&ctl->it, ldA(&ctl->it) + 1);
+  double mv[1];
+  resolveMeans<1>({mz}, mv);
   NBL_CELLS(NC);
   double acc[2] = {0.0, 0.0};
   if (q.valid) {
@@ -1251,7 +1303,7 @@ __global__ void __launch_bounds__(kBlock) k_fcg_dots(MgLevel L, double* __restri
     ldN<NC>(r, q.c, c);
 #pragma unroll
     for (int e = 0; e < NC; ++e) {
-      if (mz && (code[e] & kFluidBit)) v[e] = v[e] - ldA(mz);
+      if (mz.on() && (code[e] & kFluidBit)) v[e] = v[e] - mv[0];
       acc[0] += v[e] * a[e];
       acc[1] += v[e] * c[e];
     }
@@ -1563,8 +1615,8 @@ void launch_fin(RedBuf rb, const FinArgs& a, SolveCtl* ctl, hipStream_t s) {
 }
 
 template <class T>
-void launch_mg_sweep(const MgLevel& L, const T* b, const double* mb, const T* x, const double* mx,
-                     bool zeroGuess, const T* ec, const double* mec, const MgLevel* Lc, T* xo, bool fold,
+void launch_mg_sweep(const MgLevel& L, const T* b, MeanRef mb, const T* x, MeanRef mx,
+                     bool zeroGuess, const T* ec, MeanRef mec, const MgLevel* Lc, T* xo, bool fold,
                      RedBuf rb, SolveCtl* ctl, hipStream_t s, long long* stamps, InFold conv) {
   const int ncx = Lc ? Lc->nx : 1, ncy = Lc ? Lc->ny : 1;
   const int sx = Lc ? Lc->fx - 1 : 0, sy = Lc ? Lc->fy - 1 : 0, sz = Lc ? Lc->fz - 1 : 0;
@@ -1572,14 +1624,14 @@ void launch_mg_sweep(const MgLevel& L, const T* b, const double* mb, const T* x,
   withCells(L, [&](auto nc, const Geo3& g) {
     constexpr int NC = decltype(nc)::value;
     if (zeroGuess)
-      k_sweep<T, SW_ZERO, NC><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, nullptr, nullptr, 1, 1, 0, 0, 0, xo, f, rb,
-                                                         ctl, stamps, conv);
+      k_sweep<T, SW_ZERO, NC><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, nullptr, MeanRef{}, 1, 1, 0, 0, 0, xo, f,
+                                                         rb, ctl, stamps, conv);
     else if (ec)
       k_sweep<T, SW_PROLONG, NC><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, ec, mec, ncx, ncy, sx, sy, sz, xo, f,
                                                             rb, ctl, stamps, conv);
     else
-      k_sweep<T, SW_PLAIN, NC><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, nullptr, nullptr, 1, 1, 0, 0, 0, xo, f, rb,
-                                                          ctl, stamps, conv);
+      k_sweep<T, SW_PLAIN, NC><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, nullptr, MeanRef{}, 1, 1, 0, 0, 0, xo, f,
+                                                          rb, ctl, stamps, conv);
   });
   NBL_LAUNCH_CHECK();
 }
@@ -1607,8 +1659,8 @@ Geo3 restrictGeo(const MgLevel& L, const MgLevel& Lc) {
 }
 
 template <class T>
-void launch_mg_resid_restrict(const MgLevel& L, const MgLevel& Lc, const T* b, const double* mb, const T* x,
-                              const double* mx, T* rc, bool fold, RedBuf rb, const SolveCtl* ctl, hipStream_t s,
+void launch_mg_resid_restrict(const MgLevel& L, const MgLevel& Lc, const T* b, MeanRef mb, const T* x,
+                              MeanRef mx, T* rc, bool fold, RedBuf rb, const SolveCtl* ctl, hipStream_t s,
                               T* x1c) {
   const Geo3 g = restrictGeo(L, Lc);
   if (restrictCells(L, Lc) == 2)
@@ -1629,7 +1681,7 @@ void launch_mg_coarse(const MgLevel& L, bool singular, const T* b, const double*
 }
 
 template <class T>
-void launch_mg_finish(const MgLevel& L, const T* x, const double* mx, double* z, bool fold, RedBuf rb,
+void launch_mg_finish(const MgLevel& L, const T* x, MeanRef mx, double* z, bool fold, RedBuf rb,
                       const SolveCtl* ctl, hipStream_t s) {
   withCells(L, [&](auto nc, const Geo3& g) {
     k_finish<T, decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, x, mx, z, fold ? 1 : 0, rb, ctl);
@@ -1662,7 +1714,7 @@ void launch_inv_store(const double* G, int n, const uint16_t* code, long long nf
 }
 
 template <class T>
-void launch_mg_coarse_direct(const MgLevel& L, const double* M, const float* Mf, const T* b, const double* mb, T* x,
+void launch_mg_coarse_direct(const MgLevel& L, const double* M, const float* Mf, const T* b, MeanRef mb, T* x,
                              const SolveCtl* ctl, hipStream_t s) {
   // two rows per block: measured fastest of 1, 2, 4, 8 rows per block and one
   // wavefront per row on the 1536-cell cube-channel level (5.1 us vs 6.7 us
@@ -1677,19 +1729,18 @@ void launch_mg_coarse_direct(const MgLevel& L, const double* M, const float* Mf,
 }
 
 #define NBL_INST(T)                                                                                          \
-  template void launch_mg_sweep<T>(const MgLevel&, const T*, const double*, const T*, const double*, bool,    \
-                                   const T*, const double*, const MgLevel*, T*, bool, RedBuf, SolveCtl*,     \
-                                   hipStream_t, long long*, InFold);                                         \
-  template void launch_mg_resid_restrict<T>(const MgLevel&, const MgLevel&, const T*, const double*,         \
-                                            const T*, const double*, T*, bool, RedBuf, const SolveCtl*,      \
-                                            hipStream_t, T*);                                                \
+  template void launch_mg_sweep<T>(const MgLevel&, const T*, MeanRef, const T*, MeanRef, bool, const T*,     \
+                                   MeanRef, const MgLevel*, T*, bool, RedBuf, SolveCtl*, hipStream_t,        \
+                                   long long*, InFold);                                                      \
+  template void launch_mg_resid_restrict<T>(const MgLevel&, const MgLevel&, const T*, MeanRef, const T*,     \
+                                            MeanRef, T*, bool, RedBuf, const SolveCtl*, hipStream_t, T*);    \
   template void launch_mg_coarse<T>(const MgLevel&, bool, const T*, const double*, T*, T*,                   \
                                     const SolveCtl*, hipStream_t);                                           \
-  template void launch_mg_finish<T>(const MgLevel&, const T*, const double*, double*, bool, RedBuf,          \
+  template void launch_mg_finish<T>(const MgLevel&, const T*, MeanRef, double*, bool, RedBuf,               \
                                     const SolveCtl*, hipStream_t);                                           \
   template void launch_to_level0<T>(const MgLevel&, const double*, T*, const SolveCtl*, hipStream_t);        \
-  template void launch_mg_coarse_direct<T>(const MgLevel&, const double*, const float*, const T*,            \
-                                           const double*, T*, const SolveCtl*, hipStream_t);
+  template void launch_mg_coarse_direct<T>(const MgLevel&, const double*, const float*, const T*, MeanRef,   \
+                                           T*, const SolveCtl*, hipStream_t);
 NBL_INST(double)
 NBL_INST(float)
 #undef NBL_INST
@@ -1708,10 +1759,10 @@ void launch_restrict_plain(const MgLevel& L, const MgLevel& Lc, const double* rf
   // rf - A*0 == rf exactly, so the fused kernel with a zero x restricts rf itself
   const Geo3 g = restrictGeo(L, Lc);
   if (restrictCells(L, Lc) == 2)
-    k_resid_restrict<double, true, 2><<<g.grid, g.block, 0, s>>>(L, Lc, rf, nullptr, nullptr, nullptr, rc, 0,
+    k_resid_restrict<double, true, 2><<<g.grid, g.block, 0, s>>>(L, Lc, rf, MeanRef{}, nullptr, MeanRef{}, rc, 0,
                                                                  RedBuf{nullptr, 0}, nullptr, nullptr);
   else
-    k_resid_restrict<double, true, 1><<<g.grid, g.block, 0, s>>>(L, Lc, rf, nullptr, nullptr, nullptr, rc, 0,
+    k_resid_restrict<double, true, 1><<<g.grid, g.block, 0, s>>>(L, Lc, rf, MeanRef{}, nullptr, MeanRef{}, rc, 0,
                                                                  RedBuf{nullptr, 0}, nullptr, nullptr);
   NBL_LAUNCH_CHECK();
 }
@@ -1735,7 +1786,7 @@ void launch_fcg_init2(const MgLevel& L, bool singular, double* rhs, double* r, c
   });
   NBL_LAUNCH_CHECK();
 }
-void launch_fcg_start(const MgLevel& L, double* z, const double* mz, const double* r, double* p,
+void launch_fcg_start(const MgLevel& L, double* z, MeanRef mz, const double* r, double* p,
                       const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
   withCells(L, [&](auto nc, const Geo3& g) {
     k_fcg_start<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, z, mz, r, p, ctl, rb);
@@ -1750,9 +1801,10 @@ void launch_fcg_apply(const MgLevel& L, const double* p, double* Ap, const Solve
   NBL_LAUNCH_CHECK();
 }
 void launch_fcg_update(const MgLevel& L, bool singular, double* x, const double* p, const double* r,
-                       const double* Ap, double* rn, const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
+                       const double* Ap, double* rn, SolveCtl* ctl, RedBuf rb, hipStream_t s, InFold alpha) {
   withCells(L, [&](auto nc, const Geo3& g) {
-    k_fcg_update<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, singular ? 1 : 0, x, p, r, Ap, rn, ctl, rb);
+    k_fcg_update<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, singular ? 1 : 0, x, p, r, Ap, rn, ctl, rb,
+                                                                 alpha);
   });
   NBL_LAUNCH_CHECK();
 }
@@ -1787,16 +1839,16 @@ template void launch_mg_sweep_dots<double>(const MgLevel&, const double*, const 
                                            const double*, RedBuf, SolveCtl*, hipStream_t, bool);
 template void launch_mg_sweep_dots<float>(const MgLevel&, const float*, const float*, double*, const double*,
                                           const double*, RedBuf, SolveCtl*, hipStream_t, bool);
-void launch_fcg_norm(const MgLevel& L, double* rn, const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
+void launch_fcg_norm(const MgLevel& L, double* rn, MeanRef mrn, const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
   withCells(L, [&](auto nc, const Geo3& g) {
-    k_fcg_norm<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, rn, ctl, rb);
+    k_fcg_norm<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, rn, mrn, ctl, rb);
   });
   NBL_LAUNCH_CHECK();
 }
-void launch_fcg_dots(const MgLevel& L, double* z, const double* mz, const double* rn, const double* r,
-                     const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
+void launch_fcg_dots(const MgLevel& L, double* z, MeanRef mz, const double* rn, const double* r,
+                     SolveCtl* ctl, RedBuf rb, hipStream_t s, bool countIt) {
   withCells(L, [&](auto nc, const Geo3& g) {
-    k_fcg_dots<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, z, mz, rn, r, ctl, rb);
+    k_fcg_dots<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, z, mz, rn, r, ctl, rb, countIt ? 1 : 0);
   });
   NBL_LAUNCH_CHECK();
 }

@@ -550,7 +550,7 @@ void OctMg::cycle(size_t l, const double* b, RedBuf rb, const SolveCtl* ctl, hip
       ml.n = n;
       ml.code = code_.as<uint16_t>();
       ml.nf = (long long)L.op->nf;
-      launch_mg_coarse_direct<double>(ml, M_.as<double>(), Mf_.as<float>(), b, nullptr, xb, ctl, s);
+      launch_mg_coarse_direct<double>(ml, M_.as<double>(), Mf_.as<float>(), b, MeanRef{}, xb, ctl, s);
     } else {
       // damped Jacobi from the first sweep already in xa; an odd number of
       // further sweeps ends in xb

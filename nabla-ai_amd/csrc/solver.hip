@@ -224,7 +224,7 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
   Ap_ = DevBuf(ctx, n0 * 8);
   ctl_ = DevBuf(ctx, sizeof(SolveCtl));
   redStride_ = stride;
-  red_ = DevBuf(ctx, sizeof(double) * kMaxRedVals * size_t(stride));
+  red_ = DevBuf(ctx, sizeof(double) * (kMaxRedVals + kSlotRows) * size_t(stride));
   NBL_HIP(hipMemset(ctl_.as<void>(), 0, sizeof(SolveCtl)));
   NBL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hctl_), sizeof(SolveCtl) * 2, hipHostMallocDefault));
   std::memset(hctl_, 0, sizeof(SolveCtl) * 2);
@@ -309,6 +309,29 @@ PoissonDev::~PoissonDev() {
   if (e1_) (void)hipEventDestroy(e1_);
 }
 
+MeanRef PoissonDev::meanRef(int slot, bool inl, int blocks, long long nf) const {
+  if (!inl) return slotMean(slotPtr(ctl_.as<SolveCtl>(), slot));
+  MeanRef m;
+  m.parts = red_.as<double>() + size_t(kMaxRedVals + slot) * size_t(redStride_);
+  m.n = blocks;
+  m.nf = double(nf);
+  return m;
+}
+
+MeanRef PoissonDev::zMean() const {
+  if (!project_) return MeanRef{};
+  return meanRef(kSlotMeanZ, inlineMean(blocks_[0]), blocks_[0], dl_[0].nf);
+}
+
+RedBuf PoissonDev::meanRed(int slot, bool inl) const {
+  if (!inl) return red();
+  return RedBuf{red_.as<double>() + size_t(kMaxRedVals + slot) * size_t(redStride_), redStride_};
+}
+
+void PoissonDev::meanFin(int slot, bool inl, int blocks, long long nf, bool predicated, hipStream_t s) {
+  if (!inl) fin(FIN_MEAN, 1, blocks, slot, nf, predicated, s);
+}
+
 void PoissonDev::fin(int op, int K, int nparts, int slot, long long nf, bool predicated, hipStream_t s,
                      int earlyExit, double convTol) {
   FinArgs a{};
@@ -382,8 +405,7 @@ void PoissonDev::sweepTiming(double* ms, long long* n, double* bytes) {
 // each restriction leaves its null-space mean pending in a slot; consumers
 // subtract it on read.
 template <class T>
-void PoissonDev::vcycle(int l, const T* b, const double* mb, const T** xout, const double** mxout,
-                        hipStream_t s) {
+void PoissonDev::vcycle(int l, const T* b, MeanRef mb, const T** xout, MeanRef* mxout, hipStream_t s) {
   SolveCtl* ctl = ctl_.as<SolveCtl>();
   const MgLevel& L = dl_[l];
   const RedBuf rb = red();
@@ -391,15 +413,18 @@ void PoissonDev::vcycle(int l, const T* b, const double* mb, const T** xout, con
     T* x = lx0_[l].as<T>();
     if (direct_)
       launch_mg_coarse_direct<T>(L, coarseM_.as<double>(), coarseMf_.as<float>(), b, mb, x, ctl, s);
-    else
-      launch_mg_coarse<T>(L, project_, b, mb, x, coarseScr_.as<T>(), ctl, s);
+    else  // the one-workgroup CG reads its right-hand side mean from SolveCtl
+      launch_mg_coarse<T>(L, project_, b, mb.slot, x, coarseScr_.as<T>(), ctl, s);
     *xout = x;
-    *mxout = nullptr;
+    *mxout = MeanRef{};
     return;
   }
   T* bufs[2] = {lx0_[l].as<T>(), lx1_[l].as<T>()};
-  double* ms[2] = {project_ ? slotPtr(ctl, slotX(l, 0)) : nullptr,
-                   project_ ? slotPtr(ctl, slotX(l, 1)) : nullptr};
+  const int nbl = blocks_[l];
+  const bool inlX = inlineMean(nbl);
+  const MeanRef ms[2] = {project_ ? meanRef(slotX(l, 0), inlX, nbl, L.nf) : MeanRef{},
+                         project_ ? meanRef(slotX(l, 1), inlX, nbl, L.nf) : MeanRef{}};
+  auto xRed = [&](int o) { return project_ ? meanRed(slotX(l, o), inlX) : rb; };
   int cur = -1;
   int q0 = 0;
   const bool fusedL0 = l == 0 && l0Fused_;
@@ -417,20 +442,23 @@ void PoissonDev::vcycle(int l, const T* b, const double* mb, const T** xout, con
       conv = conv_;  // the flexible CG's convergence test rides on this sweep
       conv_ = InFold{};
     }
-    launch_mg_sweep<T>(L, b, mb, cur < 0 ? nullptr : bufs[cur], cur < 0 ? nullptr : ms[cur], cur < 0, nullptr,
-                       nullptr, nullptr, bufs[o], project_, rb, ctl, s, timedSweep(l, cur >= 0), conv);
-    if (project_) fin(FIN_MEAN, 1, blocks_[l], slotX(l, o), L.nf, true, s);
+    launch_mg_sweep<T>(L, b, mb, cur < 0 ? nullptr : bufs[cur], cur < 0 ? MeanRef{} : ms[cur], cur < 0, nullptr,
+                       MeanRef{}, nullptr, bufs[o], project_, xRed(o), ctl, s, timedSweep(l, cur >= 0), conv);
+    if (project_) meanFin(slotX(l, o), inlX, nbl, L.nf, true, s);
     cur = o;
   }
   const MgLevel& Lc = dl_[l + 1];
   T* bc = lb_[l + 1].as<T>();
-  double* mbc = project_ ? slotPtr(ctl, slotB(l + 1)) : nullptr;
+  const int rbl = rblocks_[l + 1];
+  const bool inlB = inlineMean(rbl) && !(l + 2 == nlevels() && !direct_);
+  const MeanRef mbc = project_ ? meanRef(slotB(l + 1), inlB, rbl, Lc.nf) : MeanRef{};
   // the coarse level's first sweep, unless the coarse level is the coarsest
   T* x1c = (!project_ && opts_.pre_smooth >= 1 && l + 2 < nlevels()) ? lx0_[l + 1].as<T>() : nullptr;
-  launch_mg_resid_restrict<T>(L, Lc, b, mb, bufs[cur], ms[cur], bc, project_, rb, ctl, s, x1c);
-  if (project_) fin(FIN_MEAN, 1, rblocks_[l + 1], slotB(l + 1), Lc.nf, true, s);
+  launch_mg_resid_restrict<T>(L, Lc, b, mb, bufs[cur], ms[cur], bc, project_,
+                              project_ ? meanRed(slotB(l + 1), inlB) : rb, ctl, s, x1c);
+  if (project_) meanFin(slotB(l + 1), inlB, rbl, Lc.nf, true, s);
   const T* ec = nullptr;
-  const double* mec = nullptr;
+  MeanRef mec{};
   vcycle<T>(l + 1, bc, mbc, &ec, &mec, s);
   for (int q = 0; q < opts_.post_smooth; ++q) {
     const int o = 1 - cur;
@@ -438,12 +466,12 @@ void PoissonDev::vcycle(int l, const T* b, const double* mb, const T** xout, con
       // last sweep: writes z and the flexible-CG dots instead of x
       launch_mg_sweep_dots<T>(L, b, bufs[cur], z_.as<double>(), l0Rn_, l0R_, rb, ctl, s, inline_);
       *xout = nullptr;
-      *mxout = nullptr;
+      *mxout = MeanRef{};
       return;
     }
-    launch_mg_sweep<T>(L, b, mb, bufs[cur], ms[cur], false, q == 0 ? ec : nullptr, q == 0 ? mec : nullptr, &Lc,
-                       bufs[o], project_, rb, ctl, s, timedSweep(l, q > 0));
-    if (project_) fin(FIN_MEAN, 1, blocks_[l], slotX(l, o), L.nf, true, s);
+    launch_mg_sweep<T>(L, b, mb, bufs[cur], ms[cur], false, q == 0 ? ec : nullptr, q == 0 ? mec : MeanRef{}, &Lc,
+                       bufs[o], project_, xRed(o), ctl, s, timedSweep(l, q > 0));
+    if (project_) meanFin(slotX(l, o), inlX, nbl, L.nf, true, s);
     cur = o;
   }
   *xout = bufs[cur];
@@ -451,7 +479,7 @@ void PoissonDev::vcycle(int l, const T* b, const double* mb, const T** xout, con
 }
 
 // z = Minv r: one V-cycle from a zero guess, then the level-0 projection
-// (multigrid.cpp:283-289).  z's mean is left pending in kSlotMeanZ.  With
+// (multigrid.cpp:283-289).  z's mean is left pending in kSlotMeanZ (zMean()).  With
 // fuseDots (systems without projections) the V-cycle output is left for
 // k_fcg_dots_x, which forms z itself.
 void PoissonDev::minv(const double* r, hipStream_t s, const double* rOld, bool fused) {
@@ -463,22 +491,24 @@ void PoissonDev::minv(const double* r, hipStream_t s, const double* rOld, bool f
   l0Fused_ = fused;
   l0Rn_ = r;
   l0R_ = rOld;
+  const bool inlZ = inlineMean(blocks_[0]);
+  const RedBuf zRed = project_ ? meanRed(kSlotMeanZ, inlZ) : rb;
   if (mixed_) {
     if (!fused) launch_to_level0<float>(L, r, b0f_.as<float>(), ctl, s);
     const float* x = nullptr;
-    const double* mx = nullptr;
-    vcycle<float>(0, b0f_.as<float>(), nullptr, &x, &mx, s);
+    MeanRef mx{};
+    vcycle<float>(0, b0f_.as<float>(), MeanRef{}, &x, &mx, s);
     if (x && dots) launch_fcg_dots_x<float>(L, x, z, r, rOld, ctl, rb, s);
-    else if (x) launch_mg_finish<float>(L, x, mx, z, project_, rb, ctl, s);
+    else if (x) launch_mg_finish<float>(L, x, mx, z, project_, zRed, ctl, s);
   } else {
     const double* x = nullptr;
-    const double* mx = nullptr;
-    vcycle<double>(0, r, nullptr, &x, &mx, s);
+    MeanRef mx{};
+    vcycle<double>(0, r, MeanRef{}, &x, &mx, s);
     if (x && dots) launch_fcg_dots_x<double>(L, x, z, r, rOld, ctl, rb, s);
-    else if (x) launch_mg_finish<double>(L, x, mx, z, project_, rb, ctl, s);
+    else if (x) launch_mg_finish<double>(L, x, mx, z, project_, zRed, ctl, s);
   }
   l0Fused_ = false;
-  if (project_) fin(FIN_MEAN, 1, blocks_[0], kSlotMeanZ, L.nf, true, s);
+  if (project_) meanFin(kSlotMeanZ, inlZ, blocks_[0], L.nf, true, s);
   ++vcount_;
 }
 
@@ -515,12 +545,15 @@ void PoissonDev::enqueueIteration(hipStream_t s, int k) {
     double* r = (k & 1) ? rn_.as<double>() : r_.as<double>();
     double* rn = (k & 1) ? r_.as<double>() : rn_.as<double>();
     if (project_) {
-      launch_fcg_update(L, true, x, p, r, Ap, rn, ctl, rb, s);
-      fin(FIN_MEAN, 1, nb, kSlotMeanRn, L.nf, true, s);
-      launch_fcg_norm(L, rn, ctl, rb, s);
-      fin(FIN_CONV, 1, nb, 0, 0, true, s);
+      const bool inlM = inlineMean(nb);
+      launch_fcg_update(L, true, x, p, r, Ap, rn, ctl, meanRed(kSlotMeanRn, inlM), s, fold(rbP));
+      meanFin(kSlotMeanRn, inlM, nb, L.nf, true, s);
+      launch_fcg_norm(L, rn, meanRef(kSlotMeanRn, inlM, nb, L.nf), ctl, rbR, s);
+      if (inline_) conv_ = fold(rbR);
+      else fin(FIN_CONV, 1, nb, 0, 0, true, s);
       minv(rn, s, nullptr, false);
-      launch_fcg_dots(L, z, slotPtr(ctl, kSlotMeanZ), rn, r, ctl, rb, s);
+      if (conv_.n) throw std::logic_error("flexible CG: convergence test not placed");
+      launch_fcg_dots(L, z, zMean(), rn, r, ctl, rb, s, inline_);
     } else {
       // fusions need a first pre-sweep and a coarser level to smooth above
       const bool fused = nlevels() > 1 && opts_.pre_smooth >= 1;
@@ -556,7 +589,7 @@ void PoissonDev::precondition(const double* d_r, double* d_z, hipStream_t s) {
   launch_ctl_init(ctl, tol_, s);
   minv(d_r, s, nullptr, false);
   // z - pending mean (projection) into z, p; the partials it writes are unused
-  launch_fcg_start(L, z_.as<double>(), project_ ? slotPtr(ctl, kSlotMeanZ) : nullptr, d_r, p_.as<double>(), ctl,
+  launch_fcg_start(L, z_.as<double>(), zMean(), d_r, p_.as<double>(), ctl,
                    red(), s);
   NBL_HIP(hipMemcpyAsync(d_z, z_.as<void>(), size_t(L.n) * 8, hipMemcpyDeviceToDevice, s));
 }
@@ -580,8 +613,7 @@ nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t
   launch_ctl_init(ctl, tol_, s);
   NBL_HIP(hipEventRecord(e0_, s));
   vcount_ = 0;
-  inline_ = kind_ == 0 && !project_ && nlevels() > 1 && opts_.pre_smooth >= 2 && opts_.post_smooth >= 2 &&
-            nb <= kInlineFoldMax;
+  inline_ = kind_ == 0 && nlevels() > 1 && opts_.pre_smooth >= 2 && opts_.post_smooth >= 2 && nb <= kInlineFoldMax;
   conv_ = InFold{};
   double* x = x_.as<double>();
   double* rhs = rhs_.as<double>();
@@ -596,7 +628,7 @@ nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t
   fin(FIN_BNORM, 1, nb, 0, 0, false, s, mg ? 1 : 0);
   if (mg) {
     minv(r, s, nullptr, false);
-    launch_fcg_start(L, z, project_ ? slotPtr(ctl, kSlotMeanZ) : nullptr, r, p, ctl, rb, s);
+    launch_fcg_start(L, z, zMean(), r, p, ctl, rb, s);
   } else {
     launch_cg_z(L, r, z, true, p, ctl, rb, s);
   }
@@ -636,8 +668,8 @@ void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double
   const double n = double(L.n);
   const double sz = mixed_ ? 4.0 : 8.0;
   SolveCtl* ctl = ctl_.as<SolveCtl>();
-  double* m0 = project_ ? slotPtr(ctl, slotX(0, 0)) : nullptr;
-  double* m1 = project_ ? slotPtr(ctl, slotX(0, 1)) : nullptr;
+  const MeanRef m0 = project_ ? slotMean(slotPtr(ctl, slotX(0, 0))) : MeanRef{};
+  const MeanRef m1 = project_ ? slotMean(slotPtr(ctl, slotX(0, 1))) : MeanRef{};
   // a live solve leaves ctl->done set; clear it (and every pending mean) so the
   // kernels do their work
   launch_ctl_init(ctl, 0.0, s);
@@ -672,12 +704,12 @@ void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double
   auto one = [&](int q) {
     if (kernel == 0) {
       if (mixed_)
-        launch_mg_sweep<float>(L, b0f_.as<float>(), nullptr, (q & 1) ? lx1_[0].as<float>() : lx0_[0].as<float>(),
-                               (q & 1) ? m1 : m0, false, nullptr, nullptr, nullptr,
+        launch_mg_sweep<float>(L, b0f_.as<float>(), MeanRef{}, (q & 1) ? lx1_[0].as<float>() : lx0_[0].as<float>(),
+                               (q & 1) ? m1 : m0, false, nullptr, MeanRef{}, nullptr,
                                (q & 1) ? lx0_[0].as<float>() : lx1_[0].as<float>(), project_, rb, ctl, s);
       else
-        launch_mg_sweep<double>(L, r_.as<double>(), nullptr, (q & 1) ? lx1_[0].as<double>() : lx0_[0].as<double>(),
-                                (q & 1) ? m1 : m0, false, nullptr, nullptr, nullptr,
+        launch_mg_sweep<double>(L, r_.as<double>(), MeanRef{}, (q & 1) ? lx1_[0].as<double>() : lx0_[0].as<double>(),
+                                (q & 1) ? m1 : m0, false, nullptr, MeanRef{}, nullptr,
                                 (q & 1) ? lx0_[0].as<double>() : lx1_[0].as<double>(), project_, rb, ctl, s);
     } else if (kernel == 1) {
       launch_fcg_apply(L, p_.as<double>(), Ap_.as<double>(), ctl, rb, s);
@@ -685,19 +717,19 @@ void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double
       const int lc = nlevels() - 1;
       if (mixed_) {
         const float* xo = nullptr;
-        const double* mo = nullptr;
-        vcycle<float>(lc, lb_[lc].as<float>(), nullptr, &xo, &mo, s);
+        MeanRef mo{};
+        vcycle<float>(lc, lb_[lc].as<float>(), MeanRef{}, &xo, &mo, s);
       } else {
         const double* xo = nullptr;
-        const double* mo = nullptr;
-        vcycle<double>(lc, lb_[lc].as<double>(), nullptr, &xo, &mo, s);
+        MeanRef mo{};
+        vcycle<double>(lc, lb_[lc].as<double>(), MeanRef{}, &xo, &mo, s);
       }
     } else {
       if (mixed_)
-        launch_mg_resid_restrict<float>(L, dl_[1], b0f_.as<float>(), nullptr, lx0_[0].as<float>(), m0,
+        launch_mg_resid_restrict<float>(L, dl_[1], b0f_.as<float>(), MeanRef{}, lx0_[0].as<float>(), m0,
                                         lb_[1].as<float>(), project_, rb, ctl, s);
       else
-        launch_mg_resid_restrict<double>(L, dl_[1], r_.as<double>(), nullptr, lx0_[0].as<double>(), m0,
+        launch_mg_resid_restrict<double>(L, dl_[1], r_.as<double>(), MeanRef{}, lx0_[0].as<double>(), m0,
                                          lb_[1].as<double>(), project_, rb, ctl, s);
     }
   };

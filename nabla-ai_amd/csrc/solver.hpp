@@ -107,7 +107,7 @@ class PoissonDev {
 
  private:
   template <class T>
-  void vcycle(int l, const T* b, const double* mb, const T** xout, const double** mxout, hipStream_t s);
+  void vcycle(int l, const T* b, MeanRef mb, const T** xout, MeanRef* mxout, hipStream_t s);
   // z = Minv r.  rOld: fold the flexible-CG dots z.r, z.rOld into the output
   // pass.  fused: the level-0 first sweep (and FP32 right-hand side) were
   // written by the update, and the last level-0 sweep writes z and the dots.
@@ -115,6 +115,15 @@ class PoissonDev {
   bool buildDirectCoarse();
   void fin(int op, int K, int nparts, int slot, long long nf, bool predicated, hipStream_t s, int earlyExit = 0,
            double convTol = 0.0);
+  // Pending means of the null-space projections.  The mean in slot i of a
+  // producer with `blocks` blocks is folded by its consumers from the slot's
+  // own partial row when `inl` (few enough partials), else finalized into
+  // SolveCtl by k_fin.
+  static bool inlineMean(int blocks) { return blocks <= kInlineFoldMax; }
+  MeanRef meanRef(int slot, bool inl, int blocks, long long nf) const;
+  RedBuf meanRed(int slot, bool inl) const;
+  MeanRef zMean() const;  // pending mean of the preconditioner output z
+  void meanFin(int slot, bool inl, int blocks, long long nf, bool predicated, hipStream_t s);
   void enqueueIteration(hipStream_t s, int k);
   void readCtl(hipStream_t s);
   long long* timedSweep(int l, bool plain);
@@ -149,6 +158,7 @@ class PoissonDev {
   // without projections, few enough level-0 blocks); conv_: the convergence
   // test the V-cycle's first level-0 sweep performs.
   static constexpr int kInlineFoldMax = 4096;
+  static constexpr int kSlotRows = 64;  // partial rows of the pending-mean slots, after the kMaxRedVals rows
   bool inline_ = false;
   InFold conv_{};
   DevBuf ctl_, red_;
