@@ -42,8 +42,8 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_steps(ctx, g, cfg, st, precision, warmup, steps, timing=None):
-    run = nbl.Run(ctx, g, cfg, nbl.solver_opts(precision=precision))
+def run_steps(ctx, g, cfg, st, precision, warmup, steps, timing=None, loops=False):
+    run = nbl.Run(ctx, g, cfg, nbl.solver_opts(precision=precision, loops=loops))
     run.set_state(*st)
     for _ in range(warmup):
         run.step()
@@ -523,6 +523,22 @@ def main():
             "cl_max_abs_err": float(np.max(np.abs(mcl - cl))),
         }
         log(f"mixed: {mval:.2f} Mcell-steps/s")
+    # The reference's loop structure on the GPU, one kernel per loop with its
+    # CgPoisson(1e-6, 500) coarsest solve: the unfused FP64 baseline
+    lreps, lms, lit, lel, lstate = run_steps(ctx, g, cfg, st, nbl.PREC_FP64, a.warmup, a.steps, loops=True)
+    lval = g.cells * a.steps / lel / 1e6
+    out["fp64_reference_loops"] = {
+        "value": round(lval, 3), "ms_per_step": round(lel / a.steps * 1e3, 4),
+        "mg_pcg_solve_ms_median": round(statistics.median(lms), 4),
+        "poisson_iters_per_solve_mean": round(float(np.mean(lit)), 2),
+        "max_poisson_residual": max(r["poisson_residual"] for r in lreps),
+        "rel_l2_vs_fp64": {k: rel(x, y) for k, x, y in zip("uvwp", lstate[:4], state[:4])},
+        "fused_fp64_speedup": round(value / lval, 3),
+        "note": "one kernel per loop of multigrid.cpp / poisson.cpp (smooth, residual, restrict, prolong, "
+                "removeMean, every dot and update), CgPoisson(1e-6, 500) on the coarsest level"}
+    if "mixed_fp32_vcycle" in out:
+        out["mixed_fp32_vcycle"]["speedup_vs_fp64_reference_loops"] = round(out["mixed_fp32_vcycle"]["value"] / lval, 3)
+    log(f"fp64 reference loops: {lval:.2f} Mcell-steps/s")
     out["roofline"] = roofline_insitu(sweep, g)
     out["roofline_isolated"] = roofline(ctx, g)
     out["coarse_solve"] = coarse_solve(ctx, g)
@@ -535,7 +551,11 @@ def main():
         log(f"batched: {out['batched_solves']['uniform']['speedup']}x uniform, "
             f"{out['batched_solves']['octree']['speedup']}x octree")
     if a.served_window > 0:
-        sv = served(ctx, a.served_window, a.served_rate)
+        # the served form: FP32 V-cycle inside the FP64 flexible CG
+        sv = served(ctx, a.served_window, a.served_rate, precision=nbl.PREC_MIXED)
+        sv["precision"] = "mixed (FP32 V-cycle, FP64 outer CG)"
+        sv["speedup_vs_fp64_reference_loops"] = round(sv["mcell_steps_per_s"] / lval, 3)
+        sv["speedup_vs_fused_fp64"] = round(sv["mcell_steps_per_s"] / value, 3)
         out["served"] = sv
         out["served_p50_ms"], out["served_p99_ms"] = sv["p50_ms"], sv["p99_ms"]
         log(f"served: {sv['requests']} requests, p50 {sv['p50_ms']} ms, p99 {sv['p99_ms']} ms")

@@ -112,6 +112,11 @@ typedef struct nbl_solver_opts {
                                level kernels: 0 automatic, 1, 2 or 4 (a level
                                whose nx it does not divide uses 1); results
                                agree to rounding across choices            */
+  int loops;                /* 1: the reference's loop structure, one kernel
+                               per loop of multigrid.cpp / poisson.cpp with
+                               its CgPoisson(1e-6, 500) coarse solve (FP64
+                               only): the unfused baseline the fused path is
+                               measured against; 0 (default): fused         */
 } nbl_solver_opts;
 
 /* StepReport (ns_solver.hpp:34-49). */

@@ -82,6 +82,7 @@ void nbl_solver_opts_default(nbl_solver_opts* o) {
   o->check_every = 1;
   o->coarse = NBL_COARSE_DIRECT;
   o->cells_per_thread = 0;
+  o->loops = 0;
 }
 
 const char* nbl_last_error(void) { return g_err.c_str(); }

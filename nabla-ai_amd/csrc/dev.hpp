@@ -226,6 +226,28 @@ void launch_mg_finish(const MgLevel& L, const T* x, MeanRef mx, double* z, bool 
 template <class T>
 void launch_to_level0(const MgLevel& L, const double* r, T* b, const SolveCtl* ctl, hipStream_t s);
 
+// The reference's multigrid / flexible-CG loops one kernel per loop (the
+// unfused FP64 baseline).  Reductions write one partial per block of
+// loopBlocks(n) blocks.
+constexpr int kLoopBlocks = 2048;
+int loopBlocks(long long n);
+void launch_loop_apply(const MgLevel& L, const double* x, double* out, const SolveCtl* ctl, hipStream_t s);
+void launch_loop_jacobi(const MgLevel& L, const double* b, const double* Ax, const double* diag, double* x,
+                        const SolveCtl* ctl, hipStream_t s);
+void launch_loop_msum(const MgLevel& L, double* x, RedBuf rb, const SolveCtl* ctl, hipStream_t s);
+void launch_loop_msub(const MgLevel& L, double* x, const double* mean, const SolveCtl* ctl, hipStream_t s);
+void launch_loop_resid(const MgLevel& L, const double* b, double* r, const SolveCtl* ctl, hipStream_t s);
+void launch_loop_restrict(const MgLevel& L, const MgLevel& Lc, const double* rf, double* rc, const SolveCtl* ctl,
+                          hipStream_t s);
+void launch_loop_prolong(const MgLevel& L, const MgLevel& Lc, const double* ec, double* xf, const SolveCtl* ctl,
+                         hipStream_t s);
+void launch_loop_dot(long long n, const double* a, const double* b, const double* b2, RedBuf rb,
+                     const SolveCtl* ctl, hipStream_t s);
+void launch_loop_update(long long n, double* x, const double* p, const double* r, const double* Ap, double* rn,
+                        const SolveCtl* ctl, hipStream_t s);
+void launch_loop_pupdate(long long n, double* p, const double* z, const SolveCtl* ctl, hipStream_t s);
+void launch_loop_copy(long long n, double* dst, const double* src, const SolveCtl* ctl, hipStream_t s);
+
 void launch_apply(const MgLevel& L, const double* x, double* out, hipStream_t s);
 void launch_diag(const MgLevel& L, double* out, hipStream_t s);
 void launch_restrict_plain(const MgLevel& L, const MgLevel& Lc, const double* rf, double* rc,

@@ -998,6 +998,128 @@ __global__ void k_to_level0(int n, const double* __restrict__ r, T* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// The reference's multigrid and flexible-CG loops, one kernel per loop
+// (multigrid.cpp:81-194, 268-355; poisson.cpp:102-134): the unfused FP64
+// form, a measurement baseline for what the fused kernels above save.  Every
+// kernel is one loop of the reference over all cells (grid-stride), does
+// nothing once the solve has stopped, and reductions write per-block partials.
+// ---------------------------------------------------------------------------
+#define NBL_LOOP(c, n) for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < (n); \
+                            c += (long long)gridDim.x * blockDim.x)
+
+// CgPoisson::applyA: out = A x, identity rows on solid cells
+__global__ void k_loop_apply(MgLevel L, const double* __restrict__ x, double* __restrict__ out, const SolveCtl* ctl) {
+  if (ldA(&ctl->done)) return;
+  const long long sz = (long long)L.nx * L.ny;
+  NBL_LOOP(c, L.n) {
+    const int k = int(c / sz), j = int((c - k * sz) / L.nx), i = int(c - k * sz - (long long)j * L.nx);
+    const uint16_t code = L.code[c];
+    if (!(code & kFluidBit)) {
+      out[c] = x[c];
+      continue;
+    }
+    double dg;
+    out[c] = applyRow<double, false>(L, code, int(c), i, j, k, plainView<double>(x), dg);
+  }
+}
+// smooth's update loop: x = fluid ? x + 0.8 (b - Ax) / diag : 0
+__global__ void k_loop_jacobi(MgLevel L, const double* __restrict__ b, const double* __restrict__ Ax,
+                              const double* __restrict__ diag, double* __restrict__ x, const SolveCtl* ctl) {
+  if (ldA(&ctl->done)) return;
+  NBL_LOOP(c, L.n) x[c] = (L.code[c] & kFluidBit) ? x[c] + 0.8 * (b[c] - Ax[c]) / diag[c] : 0.0;
+}
+// removeMean, first loop: fluid sum, solid cells set to 0
+__global__ void k_loop_msum(MgLevel L, double* __restrict__ x, RedBuf rb, const SolveCtl* ctl) {
+  if (ldA(&ctl->done)) return;
+  double acc[1] = {0.0};
+  NBL_LOOP(c, L.n) {
+    if (L.code[c] & kFluidBit) acc[0] += x[c];
+    else x[c] = 0.0;
+  }
+  blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
+}
+// removeMean, second loop: fluid cells minus the mean (a SolveCtl slot)
+__global__ void k_loop_msub(MgLevel L, double* __restrict__ x, const double* mean, const SolveCtl* ctl) {
+  if (ldA(&ctl->done)) return;
+  const double m = ldA(mean);
+  NBL_LOOP(c, L.n) if (L.code[c] & kFluidBit) x[c] -= m;
+}
+// residual's loop: r = fluid ? b - r : 0 (r holds A x)
+__global__ void k_loop_resid(MgLevel L, const double* __restrict__ b, double* __restrict__ r, const SolveCtl* ctl) {
+  if (ldA(&ctl->done)) return;
+  NBL_LOOP(c, L.n) r[c] = (L.code[c] & kFluidBit) ? b[c] - r[c] : 0.0;
+}
+// restrictResidual: each coarse cell the mean of its fluid children, summed
+// in the fine loop's k, j, i order
+__global__ void k_loop_restrict(MgLevel L, MgLevel Lc, const double* __restrict__ rf, double* __restrict__ rc,
+                                const SolveCtl* ctl) {
+  if (ldA(&ctl->done)) return;
+  const long long szc = (long long)Lc.nx * Lc.ny;
+  NBL_LOOP(C, Lc.n) {
+    const int K = int(C / szc), J = int((C - K * szc) / Lc.nx), I = int(C - K * szc - (long long)J * Lc.nx);
+    double sum = 0.0;
+    int cnt = 0;
+    for (int k = K * Lc.fz; k < min(K * Lc.fz + Lc.fz, L.nz); ++k)
+      for (int j = J * Lc.fy; j < min(J * Lc.fy + Lc.fy, L.ny); ++j)
+        for (int i = I * Lc.fx; i < min(I * Lc.fx + Lc.fx, L.nx); ++i) {
+          const long long f = i + (long long)L.nx * (j + (long long)L.ny * k);
+          if (L.code[f] & kFluidBit) {
+            sum += rf[f];
+            ++cnt;
+          }
+        }
+    rc[C] = cnt > 0 ? sum / double(cnt) : 0.0;
+  }
+}
+// prolongAdd: fluid fine cells add their parent's correction
+__global__ void k_loop_prolong(MgLevel L, MgLevel Lc, const double* __restrict__ ec, double* __restrict__ xf,
+                               const SolveCtl* ctl) {
+  if (ldA(&ctl->done)) return;
+  const long long sz = (long long)L.nx * L.ny;
+  NBL_LOOP(c, L.n) {
+    if (!(L.code[c] & kFluidBit)) continue;
+    const int k = int(c / sz), j = int((c - k * sz) / L.nx), i = int(c - k * sz - (long long)j * L.nx);
+    xf[c] += ec[(i / Lc.fx) + (long long)Lc.nx * ((j / Lc.fy) + (long long)Lc.ny * (k / Lc.fz))];
+  }
+}
+// dot(a, b); K = 2: z.rn and z.r in one loop (the flexible beta's loop)
+__global__ void k_loop_dot(long long n, const double* __restrict__ a, const double* __restrict__ b,
+                           const double* __restrict__ b2, RedBuf rb, const SolveCtl* ctl) {
+  if (ldA(&ctl->done)) return;
+  double acc[2] = {0.0, 0.0};
+  NBL_LOOP(c, n) {
+    acc[0] += a[c] * b[c];
+    if (b2) acc[1] += a[c] * b2[c];
+  }
+  blockToPartials<2>(acc, rb.partials, rb.stride, blockIdx.x);
+}
+// x += alpha p; rnew = r - alpha Ap
+__global__ void k_loop_update(long long n, double* __restrict__ x, const double* __restrict__ p,
+                              const double* __restrict__ r, const double* __restrict__ Ap,
+                              double* __restrict__ rn, const SolveCtl* ctl) {
+  if (ldA(&ctl->done)) return;
+  const double a = ldA(&ctl->alpha);
+  NBL_LOOP(c, n) {
+    x[c] += a * p[c];
+    rn[c] = r[c] - a * Ap[c];
+  }
+}
+// p = znew + beta p
+__global__ void k_loop_pupdate(long long n, double* __restrict__ p, const double* __restrict__ z,
+                               const SolveCtl* ctl) {
+  if (ldA(&ctl->done)) return;
+  const double be = ldA(&ctl->beta);
+  NBL_LOOP(c, n) p[c] = z[c] + be * p[c];
+}
+// dst = src (r = rnew), dst = 0 (zz.assign)
+__global__ void k_loop_copy(long long n, double* __restrict__ dst, const double* __restrict__ src,
+                            const SolveCtl* ctl) {
+  if (ldA(&ctl->done)) return;
+  NBL_LOOP(c, n) dst[c] = src ? src[c] : 0.0;
+}
+#undef NBL_LOOP
+
+// ---------------------------------------------------------------------------
 // Plain operator pieces (kernel-level parity hooks)
 // ---------------------------------------------------------------------------
 __global__ void k_apply(MgLevel L, const double* __restrict__ x, double* __restrict__ out) {
@@ -1744,6 +1866,57 @@ void launch_mg_coarse_direct(const MgLevel& L, const double* M, const float* Mf,
 NBL_INST(double)
 NBL_INST(float)
 #undef NBL_INST
+
+int loopBlocks(long long n) { return gridFor(n, kBlock, kLoopBlocks); }
+void launch_loop_apply(const MgLevel& L, const double* x, double* out, const SolveCtl* ctl, hipStream_t s) {
+  k_loop_apply<<<loopBlocks(L.n), kBlock, 0, s>>>(L, x, out, ctl);
+  NBL_LAUNCH_CHECK();
+}
+void launch_loop_jacobi(const MgLevel& L, const double* b, const double* Ax, const double* diag, double* x,
+                        const SolveCtl* ctl, hipStream_t s) {
+  k_loop_jacobi<<<loopBlocks(L.n), kBlock, 0, s>>>(L, b, Ax, diag, x, ctl);
+  NBL_LAUNCH_CHECK();
+}
+void launch_loop_msum(const MgLevel& L, double* x, RedBuf rb, const SolveCtl* ctl, hipStream_t s) {
+  k_loop_msum<<<loopBlocks(L.n), kBlock, 0, s>>>(L, x, rb, ctl);
+  NBL_LAUNCH_CHECK();
+}
+void launch_loop_msub(const MgLevel& L, double* x, const double* mean, const SolveCtl* ctl, hipStream_t s) {
+  k_loop_msub<<<loopBlocks(L.n), kBlock, 0, s>>>(L, x, mean, ctl);
+  NBL_LAUNCH_CHECK();
+}
+void launch_loop_resid(const MgLevel& L, const double* b, double* r, const SolveCtl* ctl, hipStream_t s) {
+  k_loop_resid<<<loopBlocks(L.n), kBlock, 0, s>>>(L, b, r, ctl);
+  NBL_LAUNCH_CHECK();
+}
+void launch_loop_restrict(const MgLevel& L, const MgLevel& Lc, const double* rf, double* rc, const SolveCtl* ctl,
+                          hipStream_t s) {
+  k_loop_restrict<<<loopBlocks(Lc.n), kBlock, 0, s>>>(L, Lc, rf, rc, ctl);
+  NBL_LAUNCH_CHECK();
+}
+void launch_loop_prolong(const MgLevel& L, const MgLevel& Lc, const double* ec, double* xf, const SolveCtl* ctl,
+                         hipStream_t s) {
+  k_loop_prolong<<<loopBlocks(L.n), kBlock, 0, s>>>(L, Lc, ec, xf, ctl);
+  NBL_LAUNCH_CHECK();
+}
+void launch_loop_dot(long long n, const double* a, const double* b, const double* b2, RedBuf rb,
+                     const SolveCtl* ctl, hipStream_t s) {
+  k_loop_dot<<<loopBlocks(n), kBlock, 0, s>>>(n, a, b, b2, rb, ctl);
+  NBL_LAUNCH_CHECK();
+}
+void launch_loop_update(long long n, double* x, const double* p, const double* r, const double* Ap, double* rn,
+                        const SolveCtl* ctl, hipStream_t s) {
+  k_loop_update<<<loopBlocks(n), kBlock, 0, s>>>(n, x, p, r, Ap, rn, ctl);
+  NBL_LAUNCH_CHECK();
+}
+void launch_loop_pupdate(long long n, double* p, const double* z, const SolveCtl* ctl, hipStream_t s) {
+  k_loop_pupdate<<<loopBlocks(n), kBlock, 0, s>>>(n, p, z, ctl);
+  NBL_LAUNCH_CHECK();
+}
+void launch_loop_copy(long long n, double* dst, const double* src, const SolveCtl* ctl, hipStream_t s) {
+  k_loop_copy<<<loopBlocks(n), kBlock, 0, s>>>(n, dst, src, ctl);
+  NBL_LAUNCH_CHECK();
+}
 
 void launch_apply(const MgLevel& L, const double* x, double* out, hipStream_t s) {
   const Geo3 g = geo3(L.nx, L.ny, L.nz);

@@ -162,6 +162,9 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
     throw std::invalid_argument("unknown precision");
   singular_ = !hasOutlet(g);
   mixed_ = kind == 0 && opts.precision != NBL_PREC_FP64;
+  if (opts.loops != 0 && opts.loops != 1) throw std::invalid_argument("loops must be 0 or 1");
+  if (opts.loops && (kind != 0 || mixed_)) throw std::invalid_argument("loops: FP64 mg-pcg only");
+  loops_ = opts.loops == 1;
   hl_ = buildLevels(g, kind == 0);
   // the null-space projections are no-ops without fluid cells
   project_ = singular_ && hl_[0].nf > 0;
@@ -213,6 +216,12 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
       opts.cells_per_thread != 4)
     throw std::invalid_argument("cells_per_thread must be 0, 1, 2 or 4");
   const size_t n0 = hl_[0].g.cells();
+  stride = std::max(stride, loopBlocks((long long)n0));
+  if (loops_)
+    for (int l = 0; l < nl; ++l) {
+      ldiag_.emplace_back(ctx, hl_[l].g.cells() * 8);
+      launch_diag(dl_[l], ldiag_.back().as<double>(), nullptr);
+    }
   if (mixed_) b0f_ = DevBuf(ctx, n0 * sizeof(float));
   x_ = DevBuf(ctx, n0 * 8);
   rhs_ = DevBuf(ctx, n0 * 8);
@@ -231,7 +240,8 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
   NBL_HIP(hipEventCreate(&e0_));
   NBL_HIP(hipEventCreate(&e1_));
   predicted_ = std::max(1, opts.check_every);
-  if (kind == 0 && nl > 1 && opts.coarse == NBL_COARSE_DIRECT) direct_ = buildDirectCoarse();
+  if (kind == 0 && nl > 1 && opts.coarse == NBL_COARSE_DIRECT && !loops_) direct_ = buildDirectCoarse();
+  NBL_HIP(hipDeviceSynchronize());
 }
 
 // Dense inverse of the coarsest level's operator (multigrid.cpp:178-183 runs
@@ -523,6 +533,10 @@ void PoissonDev::minv(const double* r, hipStream_t s, const double* rOld, bool f
 // kernel overwrites the rows it folds (z.r_new, z.r: rows 0-1; p.Ap: row 2;
 // |r_new|^2: row 3).
 void PoissonDev::enqueueIteration(hipStream_t s, int k) {
+  if (loops_) {
+    enqueueIterationLoops(s);
+    return;
+  }
   SolveCtl* ctl = ctl_.as<SolveCtl>();
   const MgLevel& L = dl_[0];
   const RedBuf rb = red();
@@ -580,6 +594,90 @@ void PoissonDev::enqueueIteration(hipStream_t s, int k) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// The reference's loop structure, one kernel per loop (opts.loops): smooth,
+// residual, restrictResidual, prolongAdd, vcycle with CgPoisson(1e-6, 500) on
+// the coarsest level, removeMean, and the flexible-CG loop of
+// MgPoisson::solve (multigrid.cpp:81-194, 283-340), each loop a launch and each
+// dot product a reduction with its finalize.  Level buffers: x in lx0_, the
+// right-hand side in lb_ (level 0: the caller's vector), A x / r in lx1_.
+// ---------------------------------------------------------------------------
+void PoissonDev::removeMeanLoops(int l, double* x, hipStream_t s) {
+  if (!project_) return;
+  SolveCtl* ctl = ctl_.as<SolveCtl>();
+  const MgLevel& L = dl_[l];
+  launch_loop_msum(L, x, red(), ctl, s);
+  fin(FIN_MEAN, 1, loopBlocks(L.n), slotX(l, 0), L.nf, true, s);
+  launch_loop_msub(L, x, slotPtr(ctl, slotX(l, 0)), ctl, s);
+}
+
+void PoissonDev::smoothLoops(int l, const double* b, double* x, int sweeps, hipStream_t s) {
+  SolveCtl* ctl = ctl_.as<SolveCtl>();
+  const MgLevel& L = dl_[l];
+  double* Ax = lx1_[l].as<double>();
+  for (int q = 0; q < sweeps; ++q) {
+    launch_loop_apply(L, x, Ax, ctl, s);
+    launch_loop_jacobi(L, b, Ax, ldiag_[l].as<double>(), x, ctl, s);
+    removeMeanLoops(l, x, s);
+  }
+}
+
+void PoissonDev::vcycleLoops(int l, const double* b, double* x, hipStream_t s) {
+  SolveCtl* ctl = ctl_.as<SolveCtl>();
+  const MgLevel& L = dl_[l];
+  if (l + 1 == nlevels()) {
+    launch_mg_coarse<double>(L, project_, b, nullptr, x, coarseScr_.as<double>(), ctl, s);
+    return;
+  }
+  smoothLoops(l, b, x, opts_.pre_smooth, s);
+  double* r = lx1_[l].as<double>();
+  launch_loop_apply(L, x, r, ctl, s);
+  launch_loop_resid(L, b, r, ctl, s);
+  const MgLevel& Lc = dl_[l + 1];
+  double* rc = lb_[l + 1].as<double>();
+  launch_loop_restrict(L, Lc, r, rc, ctl, s);
+  removeMeanLoops(l + 1, rc, s);
+  double* ec = lx0_[l + 1].as<double>();
+  launch_loop_copy(Lc.n, ec, nullptr, ctl, s);
+  vcycleLoops(l + 1, rc, ec, s);
+  launch_loop_prolong(L, Lc, ec, x, ctl, s);
+  smoothLoops(l, b, x, opts_.post_smooth, s);
+}
+
+// Minv (multigrid.cpp:283-289): z = 0, one V-cycle, removeMean(z).
+void PoissonDev::minvLoops(const double* r, hipStream_t s) {
+  SolveCtl* ctl = ctl_.as<SolveCtl>();
+  double* z = z_.as<double>();
+  launch_loop_copy(dl_[0].n, z, nullptr, ctl, s);
+  vcycleLoops(0, r, z, s);
+  removeMeanLoops(0, z, s);
+  ++vcount_;
+}
+
+// One iteration of multigrid.cpp:309-340, one kernel per loop; r and r_new
+// are copied like the reference's `r = rnew`.
+void PoissonDev::enqueueIterationLoops(hipStream_t s) {
+  SolveCtl* ctl = ctl_.as<SolveCtl>();
+  const MgLevel& L = dl_[0];
+  const RedBuf rb = red();
+  const long long n = L.n;
+  const int nb = loopBlocks(n);
+  double *x = x_.as<double>(), *z = z_.as<double>(), *p = p_.as<double>(), *Ap = Ap_.as<double>();
+  double *r = r_.as<double>(), *rn = rn_.as<double>();
+  launch_loop_apply(L, p, Ap, ctl, s);
+  launch_loop_dot(n, p, Ap, nullptr, rb, ctl, s);
+  fin(FIN_ALPHA, 1, nb, 0, 0, true, s);
+  launch_loop_update(n, x, p, r, Ap, rn, ctl, s);
+  removeMeanLoops(0, rn, s);
+  launch_loop_dot(n, rn, rn, nullptr, rb, ctl, s);
+  fin(FIN_CONV, 1, nb, 0, 0, true, s);
+  minvLoops(rn, s);
+  launch_loop_dot(n, z, rn, r, rb, ctl, s);
+  fin(FIN_BETA_FLEX, 2, nb, 0, 0, true, s);
+  launch_loop_pupdate(n, p, z, ctl, s);
+  launch_loop_copy(n, r, rn, ctl, s);
+}
+
 // z = Minv r alone (multigrid.cpp:283-289): one V-cycle from zero and the
 // level-0 projection.  d_r and d_z are device vectors of the level-0 size.
 void PoissonDev::precondition(const double* d_r, double* d_z, hipStream_t s) {
@@ -587,10 +685,10 @@ void PoissonDev::precondition(const double* d_r, double* d_z, hipStream_t s) {
   SolveCtl* ctl = ctl_.as<SolveCtl>();
   const MgLevel& L = dl_[0];
   launch_ctl_init(ctl, tol_, s);
-  minv(d_r, s, nullptr, false);
+  if (loops_) minvLoops(d_r, s);
+  else minv(d_r, s, nullptr, false);
   // z - pending mean (projection) into z, p; the partials it writes are unused
-  launch_fcg_start(L, z_.as<double>(), zMean(), d_r, p_.as<double>(), ctl,
-                   red(), s);
+  launch_fcg_start(L, z_.as<double>(), loops_ ? MeanRef{} : zMean(), d_r, p_.as<double>(), ctl, red(), s);
   NBL_HIP(hipMemcpyAsync(d_z, z_.as<void>(), size_t(L.n) * 8, hipMemcpyDeviceToDevice, s));
 }
 
@@ -613,7 +711,8 @@ nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t
   launch_ctl_init(ctl, tol_, s);
   NBL_HIP(hipEventRecord(e0_, s));
   vcount_ = 0;
-  inline_ = kind_ == 0 && nlevels() > 1 && opts_.pre_smooth >= 2 && opts_.post_smooth >= 2 && nb <= kInlineFoldMax;
+  inline_ = kind_ == 0 && !loops_ && nlevels() > 1 && opts_.pre_smooth >= 2 && opts_.post_smooth >= 2 &&
+            nb <= kInlineFoldMax;
   conv_ = InFold{};
   double* x = x_.as<double>();
   double* rhs = rhs_.as<double>();
@@ -627,8 +726,9 @@ nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t
   launch_fcg_init2(L, project_, rhs, r, ctl, rb, s);
   fin(FIN_BNORM, 1, nb, 0, 0, false, s, mg ? 1 : 0);
   if (mg) {
-    minv(r, s, nullptr, false);
-    launch_fcg_start(L, z, zMean(), r, p, ctl, rb, s);
+    if (loops_) minvLoops(r, s);
+    else minv(r, s, nullptr, false);
+    launch_fcg_start(L, z, loops_ ? MeanRef{} : zMean(), r, p, ctl, rb, s);
   } else {
     launch_cg_z(L, r, z, true, p, ctl, rb, s);
   }

@@ -125,6 +125,12 @@ class PoissonDev {
   MeanRef zMean() const;  // pending mean of the preconditioner output z
   void meanFin(int slot, bool inl, int blocks, long long nf, bool predicated, hipStream_t s);
   void enqueueIteration(hipStream_t s, int k);
+  // The reference's loop structure, one kernel per loop (opts.loops).
+  void removeMeanLoops(int l, double* x, hipStream_t s);
+  void smoothLoops(int l, const double* b, double* x, int sweeps, hipStream_t s);
+  void vcycleLoops(int l, const double* b, double* x, hipStream_t s);
+  void minvLoops(const double* r, hipStream_t s);
+  void enqueueIterationLoops(hipStream_t s);
   void readCtl(hipStream_t s);
   long long* timedSweep(int l, bool plain);
 
@@ -161,6 +167,8 @@ class PoissonDev {
   static constexpr int kSlotRows = 64;  // partial rows of the pending-mean slots, after the kMaxRedVals rows
   bool inline_ = false;
   InFold conv_{};
+  bool loops_ = false;
+  std::vector<DevBuf> ldiag_;  // loops: per-level diagonal (CgPoisson::buildDiagonal)
   DevBuf ctl_, red_;
   int redStride_ = 1;
   SolveCtl* hctl_ = nullptr;  // pinned mirror

@@ -91,6 +91,54 @@ def test_mg_pcg_matches_oracle(ctx, orc, name, g, coarse):
         ps.close()
 
 
+@pytest.mark.parametrize("name,g", GRIDS, ids=IDS)
+def test_reference_loops_match_oracle(ctx, orc, name, g):
+    """The unfused baseline (one kernel per loop of multigrid.cpp with its
+    CgPoisson(1e-6, 500) coarsest solve) converges as the reference does: the
+    iteration count within 1 of the oracle's, solution within rel-L2 1e-9, and
+    it agrees with the fused path to the same bar."""
+    b = cases.poisson_rhs(g, 11)
+    xo, ro = orc.mg(g, b, 1e-10, 100)
+    out = []
+    for loops in (True, False):
+        ps = nbl.Poisson(ctx, g, kind=0, tol=1e-10, max_iters=100, opts=nbl.solver_opts(loops=loops))
+        try:
+            out.append(ps.solve(b))
+        finally:
+            ps.close()
+    (xl, rl), (xf, rf) = out
+    assert rl.converged and ro.converged
+    assert abs(rl.iterations - ro.iterations) <= 1
+    assert rl.residual <= 1e-10 * 1.01
+    assert rel(xl, xo) < 1e-9 and rel(xl, xf) < 1e-9
+
+
+def test_reference_loops_step_tracks_fused(ctx):
+    """Ten cube-channel steps with the unfused baseline solve agree with the
+    fused FP64 path (fields within rel-L2 1e-10)."""
+    g, cfg = cases.cube_channel(nx=48, ny=16, nz=16, res=4)
+    st = cases.cube_channel_state(g)
+    res = []
+    for loops in (True, False):
+        run = nbl.Run(ctx, g, cfg, nbl.solver_opts(loops=loops))
+        try:
+            run.set_state(*st)
+            for _ in range(10):
+                rep = run.step()
+                assert rep.poisson_residual <= cfg.poisson_tol
+            res.append(run.get_state())
+        finally:
+            run.close()
+    for a, b in zip(res[0][:4], res[1][:4]):
+        assert rel(a, b) < 1e-10
+
+
+def test_reference_loops_rejects_mixed(ctx):
+    g, _ = cases.taylor_green(8)
+    with pytest.raises(Exception):
+        nbl.Poisson(ctx, g, kind=0, tol=1e-8, max_iters=10, opts=nbl.solver_opts(precision=nbl.PREC_MIXED, loops=True))
+
+
 @pytest.mark.parametrize("prec", [nbl.PREC_FP64, nbl.PREC_MIXED], ids=["fp64", "mixed"])
 @pytest.mark.parametrize("name", ["cube192", "tgv64", "odd3d"])
 def test_cells_per_thread_do_not_change_results(ctx, name, prec):
