@@ -71,16 +71,23 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def roofline(ctx, g, reps=200):
-    ps = nbl.Poisson(ctx, g, kind=0, tol=1e-8, max_iters=5000)
-    ms, by = ps.profile(0, reps)
+def roofline(ctx, g, reps=200, precision=nbl.PREC_FP64, kernel=0):
+    """One solve kernel in a loop of back-to-back launches: 0 the finest-level
+    sweep (FP64, or FP32 as in the mixed V-cycle), 1 the FP64 operator
+    application with p.Ap (the outer CG's SpMV)."""
+    ps = nbl.Poisson(ctx, g, kind=0, tol=1e-8, max_iters=5000, opts=nbl.solver_opts(precision=precision))
+    ms, by = ps.profile(kernel, reps)
     ps.close()
     ach = by / (ms * 1e-3) / 1e9
     key = f"{g.nx}x{g.ny}x{g.nz}" if g.nx != 192 else "cube192x64x64"
+    fp32 = precision != nbl.PREC_FP64
+    name = ("k_fcg_apply (A p and p.Ap, FP64; 18 B/cell)" if kernel == 1 else
+            f"k_sweep<{'float' if fp32 else 'double'}> (finest-level damped-Jacobi sweep; fluid-sum fold when "
+            f"singular; {14 if fp32 else 26} B/cell)")
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(key),
-            "kernel": "k_sweep<double> (finest-level damped-Jacobi sweep; fluid-sum fold when singular)",
-            "avg_launch_us": round(ms * 1e3, 3), "bytes_per_launch": by, "cells": g.cells}
+            "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": pmc_traffic(key) if (kernel == 0 and not fp32) else None,
+            "kernel": name, "avg_launch_us": round(ms * 1e3, 3), "bytes_per_launch": by, "cells": g.cells}
 
 
 def pmc_traffic(key):
@@ -544,6 +551,10 @@ def main():
     out["coarse_solve"] = coarse_solve(ctx, g)
     big = nbl.Grid(256, 256, 256, 1.0 / 256)
     out["roofline_256cubed"] = roofline(ctx, big, reps=50)
+    out["roofline_256cubed"]["fp32_sweep"] = roofline(ctx, big, reps=50, precision=nbl.PREC_MIXED)
+    out["roofline_256cubed"]["spmv_fp64"] = roofline(ctx, big, reps=50, kernel=1)
+    log(f"256^3: fp64 sweep {out['roofline_256cubed']['frac']}, fp32 sweep "
+        f"{out['roofline_256cubed']['fp32_sweep']['frac']}, spmv {out['roofline_256cubed']['spmv_fp64']['frac']}")
     if not a.no_octree:
         out["octree_c3"] = octree_c3(ctx, with_cpu=not a.no_cpu)
     if not a.no_batch:

@@ -328,53 +328,96 @@ __device__ __forceinline__ void foldInline(const InFold& f, double (&out)[K]) {
   for (int q = 0; q < K; ++q) out[q] = res[q];
 }
 
-// Values of K pending means, in every thread (block-wide: every thread of the
-// block calls it).  The folded ones share one pass over their partials.
+// The solve's stop flag (ctl may be null: never stopped) and the values of K
+// pending means, in every thread (block-wide: every thread of the block calls
+// it).  Folded means share one pass over their partials, in the same memory
+// round trip as the flag (thread 0 reads it).  True: the solve has stopped.
 template <int K>
-__device__ __forceinline__ void resolveMeans(const MeanRef (&m)[K], double (&v)[K]) {
+__device__ __forceinline__ bool stopOrMeans(const SolveCtl* ctl, const MeanRef (&m)[K], double (&v)[K]) {
   bool fold = false;
 #pragma unroll
   for (int q = 0; q < K; ++q) fold = fold || m[q].parts != nullptr;
-  if (fold) {
-    __shared__ double sh[K * 16];
-    __shared__ double res[K];
-    const int tid = linearTid(), nt = blockDim.x * blockDim.y * blockDim.z;
-    double a[K];
-#pragma unroll
-    for (int q = 0; q < K; ++q) {
-      a[q] = 0.0;
-      if (m[q].parts)
-        for (int b = tid; b < m[q].n; b += nt) a[q] += m[q].parts[b];
-    }
-    blockFold<K, 0u>(a, sh);
-    if (tid == 0)
-      for (int q = 0; q < K; ++q) res[q] = a[q];
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < K; ++q) a[q] = res[q];
-#pragma unroll
-    for (int q = 0; q < K; ++q)
-      v[q] = m[q].parts ? (m[q].nf > 0.0 ? a[q] / m[q].nf : 0.0) : (m[q].slot ? ldA(m[q].slot) : 0.0);
-  } else {
+  if (!fold) {
+    if (ctl && ldA(&ctl->done)) return true;
 #pragma unroll
     for (int q = 0; q < K; ++q) v[q] = m[q].slot ? ldA(m[q].slot) : 0.0;
+    return false;
   }
+  __shared__ double sh[K * 16];
+  __shared__ double res[K];
+  __shared__ int st;
+  const int tid = linearTid(), nt = blockDim.x * blockDim.y * blockDim.z;
+  const int d = (tid == 0 && ctl) ? ldA(&ctl->done) : 0;
+  double a[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    a[q] = 0.0;
+    if (m[q].parts)
+      for (int b = tid; b < m[q].n; b += nt) a[q] += m[q].parts[b];
+  }
+  blockFold<K, 0u>(a, sh);
+  if (tid == 0) {
+    for (int q = 0; q < K; ++q) res[q] = a[q];
+    st = d;
+  }
+  __syncthreads();
+  if (st) return true;
+#pragma unroll
+  for (int q = 0; q < K; ++q) a[q] = res[q];
+#pragma unroll
+  for (int q = 0; q < K; ++q)
+    v[q] = m[q].parts ? (m[q].nf > 0.0 ? a[q] / m[q].nf : 0.0) : (m[q].slot ? ldA(m[q].slot) : 0.0);
+  return false;
 }
 
-__device__ __forceinline__ bool blockStopped(const SolveCtl* ctl) {
-  __shared__ int stop;
-  if (linearTid() == 0) stop = ldA(&ctl->done);
+// The partials of an InFold folded together with the solve's stop flag and
+// up to two SolveCtl scalars (e0, e1; may be null) in one memory round trip:
+// thread 0 reads the scalars while every thread reads its share of the
+// partials.  Every thread gets the totals, the scalars and a block-uniform
+// stop flag.
+template <int K>
+__device__ __forceinline__ void foldStop(const InFold& f, const SolveCtl* ctl, const double* e0, const double* e1,
+                                         double (&out)[K], double (&ex)[2], bool& stop) {
+  __shared__ double sh[K * 16];
+  __shared__ double res[K + 2];
+  __shared__ int st;
+  const int tid = linearTid(), nt = blockDim.x * blockDim.y * blockDim.z;
+  int d = 0;
+  double x0 = 0.0, x1 = 0.0;
+  if (tid == 0) {
+    d = ldA(&ctl->done);
+    if (e0) x0 = ldA(e0);
+    if (e1) x1 = ldA(e1);
+  }
+  double a[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    a[q] = 0.0;
+    for (int b = tid; b < f.n; b += nt) a[q] += f.parts[q * f.stride + b];
+  }
+  blockFold<K, 0u>(a, sh);
+  if (tid == 0) {
+    for (int q = 0; q < K; ++q) res[q] = a[q];
+    res[K] = x0;
+    res[K + 1] = x1;
+    st = d;
+  }
   __syncthreads();
-  return stop != 0;
+#pragma unroll
+  for (int q = 0; q < K; ++q) out[q] = res[q];
+  ex[0] = res[K];
+  ex[1] = res[K + 1];
+  stop = st != 0;
 }
 
 // FIN_CONV: |r_new| <= tol |b| stops the solve.  True when this kernel must
 // do nothing (stopped before or now).
 __device__ __forceinline__ bool inlineConv(const InFold& f, SolveCtl* ctl) {
-  if (blockStopped(ctl)) return true;
-  double t[1];
-  foldInline<1>(f, t);
-  const bool conv = sqrt(t[0]) <= ldA(&ctl->tol) * ldA(&ctl->bnorm);
+  double t[1], ex[2];
+  bool stop;
+  foldStop<1>(f, ctl, &ctl->tol, &ctl->bnorm, t, ex, stop);
+  if (stop) return true;
+  const bool conv = sqrt(t[0]) <= ex[0] * ex[1];
   if (firstBlock() && linearTid() == 0) {
     stA(&ctl->rr, t[0]);
     if (conv) {
@@ -389,9 +432,10 @@ __device__ __forceinline__ bool inlineConv(const InFold& f, SolveCtl* ctl) {
 // FIN_ALPHA: alpha = rz / p.Ap; a vanishing p.Ap stops the solve (breakdown).
 // False when this kernel must do nothing.
 __device__ __forceinline__ bool inlineAlpha(const InFold& f, SolveCtl* ctl, double& alpha) {
-  if (blockStopped(ctl)) return false;
-  double t[1];
-  foldInline<1>(f, t);
+  double t[1], ex[2];
+  bool stop;
+  foldStop<1>(f, ctl, &ctl->rzq[f.k & 1], nullptr, t, ex, stop);
+  if (stop) return false;
   const bool w = firstBlock() && linearTid() == 0;
   if (fabs(t[0]) < 1e-300) {
     if (w) {
@@ -401,7 +445,7 @@ __device__ __forceinline__ bool inlineAlpha(const InFold& f, SolveCtl* ctl, doub
     }
     return false;
   }
-  alpha = ldA(&ctl->rzq[f.k & 1]) / t[0];
+  alpha = ex[0] / t[0];
   if (w) {
     stA(&ctl->pAp, t[0]);
     stA(&ctl->alpha, alpha);
@@ -409,17 +453,20 @@ __device__ __forceinline__ bool inlineAlpha(const InFold& f, SolveCtl* ctl, doub
   return true;
 }
 
-// FIN_BETA_FLEX: beta = (z.r_new - z.r) / rz_prev, rz = z.r_new.
-__device__ __forceinline__ double inlineBeta(const InFold& f, SolveCtl* ctl) {
-  double t[2];
-  foldInline<2>(f, t);
-  const double be = (t[0] - t[1]) / ldA(&ctl->rzq[(f.k - 1) & 1]);
+// FIN_BETA_FLEX: beta = (z.r_new - z.r) / rz_prev, rz = z.r_new.  False when
+// the solve has stopped.
+__device__ __forceinline__ bool inlineBeta(const InFold& f, SolveCtl* ctl, double& be) {
+  double t[2], ex[2];
+  bool stop;
+  foldStop<2>(f, ctl, &ctl->rzq[(f.k - 1) & 1], nullptr, t, ex, stop);
+  if (stop) return false;
+  be = (t[0] - t[1]) / ex[0];
   if (firstBlock() && linearTid() == 0) {
     stA(&ctl->beta, be);
     stA(&ctl->rz, t[0]);
     stA(&ctl->rzq[f.k & 1], t[0]);
   }
-  return be;
+  return true;
 }
 
 template <class T, int MODE, int NC>
@@ -431,27 +478,36 @@ __global__ void __launch_bounds__(kBlock) k_sweep(MgLevel L, const T* __restrict
   const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz);
   const bool t0 = threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0;
   if (stamps && t0) stamps[2 * q.blk] = wall_clock64();
+  uint16_t code[NC];
+  T xc[NC], xn[NC][6], bv[NC];
+  double mv[3] = {0.0, 0.0, 0.0};
   if (conv.n) {
+    // The sweep that carries the convergence test has no pending means (the
+    // host passes none): its cell loads go out before the test's fold, so
+    // the two memory round trips overlap; a stopped solve discards them.
+    if (q.valid) {
+      ldCodes<NC>(L.code, q.c, code);
+      ldN<NC>(b, q.c, bv);
+      if constexpr (MODE == SW_PLAIN) gatherN<T, NC>(L, plainView<T>(x), q.c, q.i, q.j, q.k, xc, xn);
+    }
     if (inlineConv(conv, ctl)) return;
-  } else if (ctl && ldA(&ctl->done)) {
-    return;
+  } else {
+    if (stopOrMeans<3>(ctl, {mb, mx, mec}, mv)) return;
+    if (q.valid) {
+      ldCodes<NC>(L.code, q.c, code);
+      if constexpr (MODE == SW_PROLONG) {
+        const XView<T, true> X{x, T(mv[1]), mx.on(), ec, ncx, ncy, sx, sy, sz, T(mv[2]), mec.on()};
+        gatherN<T, NC>(L, X, q.c, q.i, q.j, q.k, xc, xn);
+      } else if constexpr (MODE == SW_PLAIN) {
+        const XView<T> X{x, T(mv[1]), mx.on(), nullptr, 1, 1, 0, 0, 0, T(0), false};
+        gatherN<T, NC>(L, X, q.c, q.i, q.j, q.k, xc, xn);
+      }
+      ldN<NC>(b, q.c, bv);
+    }
   }
-  double mv[3];
-  resolveMeans<3>({mb, mx, mec}, mv);
   double acc[1] = {0.0};
   if (q.valid) {
-    uint16_t code[NC];
-    ldCodes<NC>(L.code, q.c, code);
-    T xc[NC], xn[NC][6];
-    if constexpr (MODE == SW_PROLONG) {
-      const XView<T, true> X{x, T(mv[1]), mx.on(), ec, ncx, ncy, sx, sy, sz, T(mv[2]), mec.on()};
-      gatherN<T, NC>(L, X, q.c, q.i, q.j, q.k, xc, xn);
-    } else if constexpr (MODE == SW_PLAIN) {
-      const XView<T> X{x, T(mv[1]), mx.on(), nullptr, 1, 1, 0, 0, 0, T(0), false};
-      gatherN<T, NC>(L, X, q.c, q.i, q.j, q.k, xc, xn);
-    }
-    T bv[NC], out[NC];
-    ldN<NC>(b, q.c, bv);
+    T out[NC];
 #pragma unroll
     for (int e = 0; e < NC; ++e) {
       if (!(code[e] & kFluidBit)) {
@@ -505,9 +561,8 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc
   const int bx = blockDim.x, by = blockDim.y;
   const int i = NC * (B.x * bx + tx), j = B.y * by + ty, k = B.z * int(blockDim.z) + tz;
   const int tid = tx + bx * (ty + by * tz);
-  if (ctl && ldA(&ctl->done)) return;
   double mv[2];
-  resolveMeans<2>({mb, mx}, mv);
+  if (stopOrMeans<2>(ctl, {mb, mx}, mv)) return;
   const XView<T> X{x, T(mv[1]), mx.on(), nullptr, 1, 1, 0, 0, 0, T(0), false};
   T r[NC];
   bool fl[NC];
@@ -820,11 +875,10 @@ template <class T, class MT, int ROWS>
 __global__ void __launch_bounds__(kBlock) k_coarse_direct(MgLevel L, const MT* __restrict__ M,
                                                           const T* __restrict__ b, MeanRef mb,
                                                           T* __restrict__ x, const SolveCtl* ctl) {
-  if (ctl && ldA(&ctl->done)) return;
+  double mv[1];
+  if (stopOrMeans<1>(ctl, {mb}, mv)) return;
   const int n = int(L.n);
   const int r0 = blockIdx.x * ROWS;
-  double mv[1];
-  resolveMeans<1>({mb}, mv);
   const double m = mv[0];
   double acc[ROWS];
 #pragma unroll
@@ -966,9 +1020,8 @@ template <class T, int NC>
 __global__ void __launch_bounds__(kBlock) k_finish(MgLevel L, const T* __restrict__ x, MeanRef mx,
                                                    double* __restrict__ z, int fold, RedBuf rb,
                                                    const SolveCtl* ctl) {
-  if (ctl && ldA(&ctl->done)) return;
   double mv[1];
-  resolveMeans<1>({mx}, mv);
+  if (stopOrMeans<1>(ctl, {mx}, mv)) return;
   const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz);
   double acc[1] = {0.0};
   if (q.valid) {
@@ -1214,9 +1267,8 @@ template <int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_start(MgLevel L, double* __restrict__ z, MeanRef mz,
                                                       const double* __restrict__ r, double* __restrict__ p,
                                                       const SolveCtl* ctl, RedBuf rb) {
-  if (ldA(&ctl->done)) return;
   double mv[1];
-  resolveMeans<1>({mz}, mv);
+  if (stopOrMeans<1>(ctl, {mz}, mv)) return;
   NBL_CELLS(NC);
   double acc[1] = {0.0};
   if (q.valid) {
@@ -1291,9 +1343,14 @@ __global__ void __launch_bounds__(kBlock) k_fcg_apply_pu(MgLevel L, const double
                                                          const double* __restrict__ pOld, double* __restrict__ pNew,
                                                          double* __restrict__ Ap, SolveCtl* ctl, RedBuf rb,
                                                          InFold bf) {
-  // `done` is not written by this kernel: the per-thread read is uniform
-  if (ldA(&ctl->done)) return;
-  const double be = bf.n ? inlineBeta(bf, ctl) : ldA(&ctl->beta);
+  double be;
+  if (bf.n) {
+    if (!inlineBeta(bf, ctl, be)) return;
+  } else {
+    // `done` is not written by this kernel: the per-thread read is uniform
+    if (ldA(&ctl->done)) return;
+    be = ldA(&ctl->beta);
+  }
   NBL_CELLS(NC);
   double acc[1] = {0.0};
   if (q.valid) {
@@ -1313,21 +1370,27 @@ __global__ void __launch_bounds__(kBlock) k_fcg_update(MgLevel L, int singular, 
                                                        const double* __restrict__ p, const double* __restrict__ r,
                                                        const double* __restrict__ Ap, double* __restrict__ rn,
                                                        SolveCtl* ctl, RedBuf rb, InFold af) {
+  NBL_CELLS(NC);
+  double xv[NC], pv[NC], rv[NC], av[NC];
+  auto load = [&] {
+    if (q.valid) {
+      ldN<NC>(x, q.c, xv);
+      ldN<NC>(p, q.c, pv);
+      ldN<NC>(r, q.c, rv);
+      ldN<NC>(Ap, q.c, av);
+    }
+  };
   double a;
   if (af.n) {
+    load();  // before the fold, so the two memory round trips overlap
     if (!inlineAlpha(af, ctl, a)) return;
   } else {
     if (ldA(&ctl->done)) return;
     a = ldA(&ctl->alpha);
+    load();
   }
-  NBL_CELLS(NC);
   double acc[1] = {0.0};
   if (q.valid) {
-    double xv[NC], pv[NC], rv[NC], av[NC];
-    ldN<NC>(x, q.c, xv);
-    ldN<NC>(p, q.c, pv);
-    ldN<NC>(r, q.c, rv);
-    ldN<NC>(Ap, q.c, av);
 #pragma unroll
     for (int e = 0; e < NC; ++e) {
       xv[e] = xv[e] + a * pv[e];
@@ -1352,22 +1415,28 @@ __global__ void __launch_bounds__(kBlock) k_fcg_update_first(MgLevel L, double* 
                                                              const double* __restrict__ Ap, double* __restrict__ rn,
                                                              T* __restrict__ x1, T* __restrict__ b0,
                                                              SolveCtl* ctl, RedBuf rb, InFold af) {
+  NBL_CELLS(NC);
+  double xv[NC], pv[NC], rv[NC], av[NC];
+  auto load = [&] {
+    if (q.valid) {
+      ldN<NC>(x, q.c, xv);
+      ldN<NC>(p, q.c, pv);
+      ldN<NC>(r, q.c, rv);
+      ldN<NC>(Ap, q.c, av);
+    }
+  };
   double a;
   if (af.n) {
+    load();  // before the fold, so the two memory round trips overlap
     if (!inlineAlpha(af, ctl, a)) return;
   } else {
     if (ldA(&ctl->done)) return;
     a = ldA(&ctl->alpha);
+    load();
   }
-  NBL_CELLS(NC);
   double acc[1] = {0.0};
   if (q.valid) {
-    double xv[NC], pv[NC], rv[NC], av[NC];
     T bv[NC], fv[NC];
-    ldN<NC>(x, q.c, xv);
-    ldN<NC>(p, q.c, pv);
-    ldN<NC>(r, q.c, rv);
-    ldN<NC>(Ap, q.c, av);
 #pragma unroll
     for (int e = 0; e < NC; ++e) {
       xv[e] = xv[e] + a * pv[e];
@@ -1387,9 +1456,8 @@ __global__ void __launch_bounds__(kBlock) k_fcg_update_first(MgLevel L, double* 
 template <int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_norm(MgLevel L, double* __restrict__ rn, MeanRef mrn,
                                                      const SolveCtl* ctl, RedBuf rb) {
-  if (ldA(&ctl->done)) return;
   double mv[1];
-  resolveMeans<1>({mrn}, mv);
+  if (stopOrMeans<1>(ctl, {mrn}, mv)) return;
   NBL_CELLS(NC);
   double acc[1] = {0.0};
   if (q.valid) {
@@ -1412,10 +1480,9 @@ template <int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_dots(MgLevel L, double* __restrict__ z, MeanRef mz,
                                                      const double* __restrict__ rn, const double* __restrict__ r,
                                                      SolveCtl* ctl, RedBuf rb, int countIt) {
-  if (ldA(&ctl->done)) return;
-  if (countIt && firstBlock() && linearTid() == 0) stA(&ctl->it, ldA(&ctl->it) + 1);
   double mv[1];
-  resolveMeans<1>({mz}, mv);
+  if (stopOrMeans<1>(ctl, {mz}, mv)) return;
+  if (countIt && firstBlock() && linearTid() == 0) stA(&ctl->it, ldA(&ctl->it) + 1);
   NBL_CELLS(NC);
   double acc[2] = {0.0, 0.0};
   if (q.valid) {

@@ -449,7 +449,10 @@ void PoissonDev::vcycle(int l, const T* b, MeanRef mb, const T** xout, MeanRef* 
     const int o = cur == 0 ? 1 : 0;
     InFold conv{};
     if (l == 0 && conv_.n) {
-      conv = conv_;  // the flexible CG's convergence test rides on this sweep
+      // the flexible CG's convergence test rides on this sweep, which reads
+      // no pending mean (the kernel issues its loads before the test)
+      if (mb.on() || (cur >= 0 && ms[cur].on())) throw std::logic_error("convergence sweep with a pending mean");
+      conv = conv_;
       conv_ = InFold{};
     }
     launch_mg_sweep<T>(L, b, mb, cur < 0 ? nullptr : bufs[cur], cur < 0 ? MeanRef{} : ms[cur], cur < 0, nullptr,
