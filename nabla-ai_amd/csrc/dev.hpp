@@ -206,7 +206,7 @@ void launch_mg_sweep(const MgLevel& L, const T* b, MeanRef mb, const T* x, MeanR
 template <class T>
 void launch_mg_resid_restrict(const MgLevel& L, const MgLevel& Lc, const T* b, MeanRef mb,
                               const T* x, MeanRef mx, T* rc, bool fold, RedBuf rb,
-                              const SolveCtl* ctl, hipStream_t s, T* x1c = nullptr);
+                              SolveCtl* ctl, hipStream_t s, T* x1c = nullptr, InFold conv = {});
 template <class T>
 void launch_mg_coarse(const MgLevel& L, bool singular, const T* b, const double* mb, T* x,
                       T* scratch, const SolveCtl* ctl, hipStream_t s);

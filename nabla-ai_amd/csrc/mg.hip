@@ -478,32 +478,30 @@ __global__ void __launch_bounds__(kBlock) k_sweep(MgLevel L, const T* __restrict
   const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz);
   const bool t0 = threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0;
   if (stamps && t0) stamps[2 * q.blk] = wall_clock64();
+  // Only a sweep from zero carries the convergence test (the host passes it
+  // to no other mode); it has no pending means, so its cell loads go out
+  // before the test's fold and the two memory round trips overlap.
   uint16_t code[NC];
   T xc[NC], xn[NC][6], bv[NC];
   double mv[3] = {0.0, 0.0, 0.0};
-  if (conv.n) {
-    // The sweep that carries the convergence test has no pending means (the
-    // host passes none): its cell loads go out before the test's fold, so
-    // the two memory round trips overlap; a stopped solve discards them.
-    if (q.valid) {
-      ldCodes<NC>(L.code, q.c, code);
-      ldN<NC>(b, q.c, bv);
-      if constexpr (MODE == SW_PLAIN) gatherN<T, NC>(L, plainView<T>(x), q.c, q.i, q.j, q.k, xc, xn);
-    }
-    if (inlineConv(conv, ctl)) return;
+  if constexpr (MODE == SW_ZERO) {
+    if (!conv.n && stopOrMeans<3>(ctl, {mb, mx, mec}, mv)) return;
   } else {
     if (stopOrMeans<3>(ctl, {mb, mx, mec}, mv)) return;
-    if (q.valid) {
-      ldCodes<NC>(L.code, q.c, code);
-      if constexpr (MODE == SW_PROLONG) {
-        const XView<T, true> X{x, T(mv[1]), mx.on(), ec, ncx, ncy, sx, sy, sz, T(mv[2]), mec.on()};
-        gatherN<T, NC>(L, X, q.c, q.i, q.j, q.k, xc, xn);
-      } else if constexpr (MODE == SW_PLAIN) {
-        const XView<T> X{x, T(mv[1]), mx.on(), nullptr, 1, 1, 0, 0, 0, T(0), false};
-        gatherN<T, NC>(L, X, q.c, q.i, q.j, q.k, xc, xn);
-      }
-      ldN<NC>(b, q.c, bv);
+  }
+  if (q.valid) {
+    ldCodes<NC>(L.code, q.c, code);
+    if constexpr (MODE == SW_PROLONG) {
+      const XView<T, true> X{x, T(mv[1]), mx.on(), ec, ncx, ncy, sx, sy, sz, T(mv[2]), mec.on()};
+      gatherN<T, NC>(L, X, q.c, q.i, q.j, q.k, xc, xn);
+    } else if constexpr (MODE == SW_PLAIN) {
+      const XView<T> X{x, T(mv[1]), mx.on(), nullptr, 1, 1, 0, 0, 0, T(0), false};
+      gatherN<T, NC>(L, X, q.c, q.i, q.j, q.k, xc, xn);
     }
+    ldN<NC>(b, q.c, bv);
+  }
+  if constexpr (MODE == SW_ZERO) {
+    if (conv.n && inlineConv(conv, ctl)) return;
   }
   double acc[1] = {0.0};
   if (q.valid) {
@@ -552,8 +550,8 @@ __global__ void __launch_bounds__(kBlock) k_sweep(MgLevel L, const T* __restrict
 template <class T, bool ZEROX, int NC>
 __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc, const T* __restrict__ b,
                                                            MeanRef mb, const T* __restrict__ x, MeanRef mx,
-                                                           T* __restrict__ rc, int fold, RedBuf rb,
-                                                           const SolveCtl* ctl, T* __restrict__ x1c) {
+                                                           T* __restrict__ rc, int fold, RedBuf rb, SolveCtl* ctl,
+                                                           T* __restrict__ x1c, InFold conv) {
   __shared__ T rs[kBlock * NC];
   __shared__ unsigned char fls[kBlock * NC];
   const Blk3 B = blk3();
@@ -561,9 +559,20 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc
   const int bx = blockDim.x, by = blockDim.y;
   const int i = NC * (B.x * bx + tx), j = B.y * by + ty, k = B.z * int(blockDim.z) + tz;
   const int tid = tx + bx * (ty + by * tz);
-  double mv[2];
-  if (stopOrMeans<2>(ctl, {mb, mx}, mv)) return;
-  const XView<T> X{x, T(mv[1]), mx.on(), nullptr, 1, 1, 0, 0, 0, T(0), false};
+  const bool in = i < L.nx && j < L.ny && k < L.nz;
+  const int c = i + L.nx * (j + L.ny * k);
+  uint16_t code[NC];
+  T xc[NC], xn[NC][6], bv[NC];
+  // a restriction that carries the convergence test: as k_sweep
+  double mv[2] = {0.0, 0.0};
+  if (!conv.n && stopOrMeans<2>(ctl, {mb, mx}, mv)) return;
+  if (in) {
+    const XView<T> X{x, T(mv[1]), mx.on(), nullptr, 1, 1, 0, 0, 0, T(0), false};
+    ldCodes<NC>(L.code, c, code);
+    if constexpr (!ZEROX) gatherN<T, NC>(L, X, c, i, j, k, xc, xn);
+    ldN<NC>(b, c, bv);
+  }
+  if (conv.n && inlineConv(conv, ctl)) return;
   T r[NC];
   bool fl[NC];
 #pragma unroll
@@ -571,13 +580,7 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc
     r[e] = T(0);
     fl[e] = false;
   }
-  if (i < L.nx && j < L.ny && k < L.nz) {
-    const int c = i + L.nx * (j + L.ny * k);
-    uint16_t code[NC];
-    T xc[NC], xn[NC][6], bv[NC];
-    ldCodes<NC>(L.code, c, code);
-    if constexpr (!ZEROX) gatherN<T, NC>(L, X, c, i, j, k, xc, xn);
-    ldN<NC>(b, c, bv);
+  if (in) {
 #pragma unroll
     for (int e = 0; e < NC; ++e) {
       if (!(code[e] & kFluidBit)) continue;
@@ -1810,6 +1813,7 @@ void launch_mg_sweep(const MgLevel& L, const T* b, MeanRef mb, const T* x, MeanR
   const int ncx = Lc ? Lc->nx : 1, ncy = Lc ? Lc->ny : 1;
   const int sx = Lc ? Lc->fx - 1 : 0, sy = Lc ? Lc->fy - 1 : 0, sz = Lc ? Lc->fz - 1 : 0;
   const int f = fold ? 1 : 0;
+  if (conv.n && !zeroGuess) throw std::logic_error("mg sweep: only a sweep from zero carries the convergence test");
   withCells(L, [&](auto nc, const Geo3& g) {
     constexpr int NC = decltype(nc)::value;
     if (zeroGuess)
@@ -1849,13 +1853,15 @@ Geo3 restrictGeo(const MgLevel& L, const MgLevel& Lc) {
 
 template <class T>
 void launch_mg_resid_restrict(const MgLevel& L, const MgLevel& Lc, const T* b, MeanRef mb, const T* x,
-                              MeanRef mx, T* rc, bool fold, RedBuf rb, const SolveCtl* ctl, hipStream_t s,
-                              T* x1c) {
+                              MeanRef mx, T* rc, bool fold, RedBuf rb, SolveCtl* ctl, hipStream_t s, T* x1c,
+                              InFold conv) {
   const Geo3 g = restrictGeo(L, Lc);
   if (restrictCells(L, Lc) == 2)
-    k_resid_restrict<T, false, 2><<<g.grid, g.block, 0, s>>>(L, Lc, b, mb, x, mx, rc, fold ? 1 : 0, rb, ctl, x1c);
+    k_resid_restrict<T, false, 2><<<g.grid, g.block, 0, s>>>(L, Lc, b, mb, x, mx, rc, fold ? 1 : 0, rb, ctl, x1c,
+                                                             conv);
   else
-    k_resid_restrict<T, false, 1><<<g.grid, g.block, 0, s>>>(L, Lc, b, mb, x, mx, rc, fold ? 1 : 0, rb, ctl, x1c);
+    k_resid_restrict<T, false, 1><<<g.grid, g.block, 0, s>>>(L, Lc, b, mb, x, mx, rc, fold ? 1 : 0, rb, ctl, x1c,
+                                                             conv);
   NBL_LAUNCH_CHECK();
 }
 
@@ -1922,7 +1928,7 @@ void launch_mg_coarse_direct(const MgLevel& L, const double* M, const float* Mf,
                                    MeanRef, const MgLevel*, T*, bool, RedBuf, SolveCtl*, hipStream_t,        \
                                    long long*, InFold);                                                      \
   template void launch_mg_resid_restrict<T>(const MgLevel&, const MgLevel&, const T*, MeanRef, const T*,     \
-                                            MeanRef, T*, bool, RedBuf, const SolveCtl*, hipStream_t, T*);    \
+                                            MeanRef, T*, bool, RedBuf, SolveCtl*, hipStream_t, T*, InFold);  \
   template void launch_mg_coarse<T>(const MgLevel&, bool, const T*, const double*, T*, T*,                   \
                                     const SolveCtl*, hipStream_t);                                           \
   template void launch_mg_finish<T>(const MgLevel&, const T*, MeanRef, double*, bool, RedBuf,               \
@@ -2000,10 +2006,10 @@ void launch_restrict_plain(const MgLevel& L, const MgLevel& Lc, const double* rf
   const Geo3 g = restrictGeo(L, Lc);
   if (restrictCells(L, Lc) == 2)
     k_resid_restrict<double, true, 2><<<g.grid, g.block, 0, s>>>(L, Lc, rf, MeanRef{}, nullptr, MeanRef{}, rc, 0,
-                                                                 RedBuf{nullptr, 0}, nullptr, nullptr);
+                                                                 RedBuf{nullptr, 0}, nullptr, nullptr, InFold{});
   else
     k_resid_restrict<double, true, 1><<<g.grid, g.block, 0, s>>>(L, Lc, rf, MeanRef{}, nullptr, MeanRef{}, rc, 0,
-                                                                 RedBuf{nullptr, 0}, nullptr, nullptr);
+                                                                 RedBuf{nullptr, 0}, nullptr, nullptr, InFold{});
   NBL_LAUNCH_CHECK();
 }
 void launch_prolong_plain(const MgLevel& L, const MgLevel& Lc, const double* ec, double* xf, hipStream_t s) {

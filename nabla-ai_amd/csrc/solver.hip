@@ -448,10 +448,10 @@ void PoissonDev::vcycle(int l, const T* b, MeanRef mb, const T** xout, MeanRef* 
   for (int q = q0; q < opts_.pre_smooth; ++q) {
     const int o = cur == 0 ? 1 : 0;
     InFold conv{};
-    if (l == 0 && conv_.n) {
-      // the flexible CG's convergence test rides on this sweep, which reads
-      // no pending mean (the kernel issues its loads before the test)
-      if (mb.on() || (cur >= 0 && ms[cur].on())) throw std::logic_error("convergence sweep with a pending mean");
+    if (l == 0 && conv_.n && cur < 0) {
+      // the flexible CG's convergence test rides on the first sweep from zero
+      // (it reads no pending mean; the kernel issues its loads before the test)
+      if (mb.on()) throw std::logic_error("convergence sweep with a pending mean");
       conv = conv_;
       conv_ = InFold{};
     }
@@ -467,8 +467,16 @@ void PoissonDev::vcycle(int l, const T* b, MeanRef mb, const T** xout, MeanRef* 
   const MeanRef mbc = project_ ? meanRef(slotB(l + 1), inlB, rbl, Lc.nf) : MeanRef{};
   // the coarse level's first sweep, unless the coarse level is the coarsest
   T* x1c = (!project_ && opts_.pre_smooth >= 1 && l + 2 < nlevels()) ? lx0_[l + 1].as<T>() : nullptr;
+  // Otherwise the test rides on the level-0 restriction (the first level-0
+  // sweep of the fused path is then the plain smoother sweep, kept free of it).
+  InFold conv{};
+  if (l == 0 && conv_.n) {
+    if (mb.on() || ms[cur].on()) throw std::logic_error("convergence restriction with a pending mean");
+    conv = conv_;
+    conv_ = InFold{};
+  }
   launch_mg_resid_restrict<T>(L, Lc, b, mb, bufs[cur], ms[cur], bc, project_,
-                              project_ ? meanRed(slotB(l + 1), inlB) : rb, ctl, s, x1c);
+                              project_ ? meanRed(slotB(l + 1), inlB) : rb, ctl, s, x1c, conv);
   if (project_) meanFin(slotB(l + 1), inlB, rbl, Lc.nf, true, s);
   const T* ec = nullptr;
   MeanRef mec{};
