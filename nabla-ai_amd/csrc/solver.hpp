@@ -163,7 +163,7 @@ class PoissonDev {
   // Finalizes folded into their consumers for the current solve (flexible CG
   // without projections, few enough level-0 blocks); conv_: the convergence
   // test the V-cycle's first level-0 sweep performs.
-  static constexpr int kInlineFoldMax = 4096;
+  static constexpr int kInlineFoldMax = 2048;
   static constexpr int kSlotRows = 64;  // partial rows of the pending-mean slots, after the kMaxRedVals rows
   bool inline_ = false;
   InFold conv_{};
