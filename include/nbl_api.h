@@ -300,6 +300,32 @@ int nbl_octree_poisson_solve(nbl_octree* oc, const double* b, double* x, double 
 int nbl_octree_transport(nbl_octree* oc, const double* u, const double* v, const double* w, double* phi,
                          double dt, int steps, double* integral);
 
+/* Results of PhysicalSensorEvaluator::evaluate (adaptive/adaptive_core.cpp:73-176,
+ * CellSensors in adaptive.hpp:51-63), one entry per leaf in the caller's
+ * order; any pointer may be NULL. */
+typedef struct nbl_sensor_fields {
+  double* vorticity_mag;
+  double* strain_mag;
+  double* q_criterion;
+  double* grad_p_mag;
+  double* grad_u_mag;
+  double* y_plus;        /* -1 on cells that are not near a wall */
+  double* tau_wall;
+  double* trunc_err;
+  uint8_t* near_wall;
+  uint8_t* reverse_flow;
+  uint8_t* resid_stagnation;
+} nbl_sensor_fields;
+
+/* PhysicalSensorEvaluator(nu).evaluate on the leaves: gradients of u, v, w, p
+ * across level jumps (Octree::gradient, octree.cpp:491-539) from the device
+ * face table, vorticity, strain, Q criterion, near-wall shear and y+, the
+ * truncation estimate and the residual-stagnation flag.  u, v, w, p are
+ * cell-centred leaf fields; speed_prev / resid_prev are the evaluator's
+ * carried per-leaf state ("speed_prev", "resid_prev"), read and updated. */
+int nbl_octree_sensors(nbl_octree* oc, const double* u, const double* v, const double* w, const double* p,
+                       double nu, double* speed_prev, double* resid_prev, const nbl_sensor_fields* out);
+
 /* Octree multigrid-preconditioned flexible CG on the OctreePoisson system.
  * No counterpart in the reference, which solves it by Jacobi-PCG only
  * (octree_poisson.cpp:107-197) and names octree levels as the intended
@@ -322,8 +348,12 @@ int nbl_octree_mg_levels(nbl_octree* oc, long long* counts, int cap, int* levels
  * coupling counts, masks, x and writing y once per leaf plus a column index
  * and a coefficient per coupling; kernel 1: the transport step, per leaf its
  * level, face counts, first-neighbour slots, velocities and phi in and out,
- * plus an index per further neighbour), and the device time of construction (sort,
- * validation, face table, operator) and of the last solve or transport. */
+ * plus an index per further neighbour; kernel 2: the sensors, per leaf its
+ * key, level, mask, face counts, first-neighbour slots, u, v, w, p, the
+ * carried state, the permutation, ten results and three flags, plus an index
+ * per further neighbour), and the device time of construction (sort,
+ * validation, face table, operator) and of the last solve, transport or
+ * sensor evaluation. */
 int nbl_octree_profile(nbl_octree* oc, int kernel, int reps, double* ms, double* bytes);
 int nbl_octree_last_timing(nbl_octree* oc, double* setup_ms, double* solve_ms);
 

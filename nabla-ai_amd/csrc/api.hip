@@ -5,6 +5,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "nbl_api.h"
 #include "octree.hpp"
@@ -533,6 +534,27 @@ int nbl_octree_transport(nbl_octree* oc, const double* u, const double* v, const
     useDevice(oc->ctx);
     const double I = oc->impl->transport(u, v, w, phi, dt, steps);
     if (integral) *integral = I;
+  });
+}
+
+int nbl_octree_sensors(nbl_octree* oc, const double* u, const double* v, const double* w, const double* p,
+                       double nu, double* speed_prev, double* resid_prev, const nbl_sensor_fields* out) {
+  return guarded([&] {
+    need(oc && u && v && w && p && speed_prev && resid_prev, "NULL argument");
+    useDevice(oc->ctx);
+    const size_t n = oc->impl->size();
+    std::vector<double> vals(out ? 8 * n : 0);
+    std::vector<uint8_t> fl(out ? 3 * n : 0);
+    oc->impl->sensors(u, v, w, p, nu, speed_prev, resid_prev, out ? vals.data() : nullptr,
+                      out ? fl.data() : nullptr);
+    if (!out) return;
+    double* const dst[8] = {out->vorticity_mag, out->strain_mag, out->q_criterion, out->grad_p_mag,
+                            out->grad_u_mag,    out->y_plus,     out->tau_wall,    out->trunc_err};
+    for (int f = 0; f < 8; ++f)
+      if (dst[f]) std::memcpy(dst[f], vals.data() + size_t(f) * n, n * sizeof(double));
+    uint8_t* const fdst[3] = {out->near_wall, out->reverse_flow, out->resid_stagnation};
+    for (int f = 0; f < 3; ++f)
+      if (fdst[f]) std::memcpy(fdst[f], fl.data() + size_t(f) * n, n);
   });
 }
 

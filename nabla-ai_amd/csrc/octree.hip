@@ -570,6 +570,126 @@ __global__ void __launch_bounds__(kBlock) k_oct_transport(long long n, Geo G, co
   phiOut[i] = pi + dphi;
 }
 
+// ---- adaptive sensors (PhysicalSensorEvaluator::evaluate, adaptive_core.cpp:73-176)
+// One leaf per thread in sorted order.  The gradients across level jumps
+// (Octree::gradient, octree.cpp:491-539) of u, v, w and p take one walk over the
+// face table: per axis and side the neighbours' values and centres are summed
+// in the reference's order, each field on its own, then averaged and
+// differenced; the sensor formulas follow as the reference writes them.
+// Results go to the caller's order through the permutation: out[f*n + perm[i]]
+// for the eight real sensors and the carried speed / residual (rows 8, 9),
+// flags[f*n + perm[i]] for near-wall, reverse flow and residual stagnation.
+struct FaceWalk {
+  const uint8_t* __restrict__ cnt;    // u8[6n]
+  const int32_t* __restrict__ first;  // i32[6][n], -1 when none
+  const int32_t* __restrict__ xrow;   // further neighbours of leaf i from xnb[xrow[i]]
+  const int32_t* __restrict__ xnb;
+};
+
+__global__ void __launch_bounds__(kBlock) k_oct_sensors(long long n, Geo G, const uint64_t* __restrict__ keys,
+                                                        const uint8_t* __restrict__ lev,
+                                                        const uint8_t* __restrict__ mask, FaceWalk F,
+                                                        const double* __restrict__ U, const double* __restrict__ V,
+                                                        const double* __restrict__ W, const double* __restrict__ P,
+                                                        const double* __restrict__ sPrev,
+                                                        const double* __restrict__ rPrev, double nu,
+                                                        const uint64_t* __restrict__ perm, double* __restrict__ out,
+                                                        uint8_t* __restrict__ flags) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int cnt[6], off[6];
+  int x = F.xrow[i];
+  for (int d = 0; d < 6; ++d) {
+    cnt[d] = F.cnt[6 * i + d];
+    off[d] = x;
+    x += cnt[d] > 1 ? cnt[d] - 1 : 0;
+  }
+  auto nbAt = [&](int d, int m) { return m == 0 ? F.first[(long long)d * n + i] : F.xnb[off[d] + m - 1]; };
+  const uint64_t ki = keys[i];
+  const int li = lev[i];
+  const double self[4] = {U[i], V[i], W[i], P[i]};
+  double g[4][3];  // d(field)/d(axis) for u, v, w, p
+  for (int a = 0; a < 3; ++a) {
+    const double ci = centerAxis(G, ki, li, a);
+    double val[2][4], pos[2] = {ci, ci};
+    bool has[2] = {false, false};
+    for (int s = 0; s < 2; ++s) {  // s = 0: + face (2a+1), s = 1: - face (2a)
+      for (int f = 0; f < 4; ++f) val[s][f] = self[f];
+      const int d = s == 0 ? 2 * a + 1 : 2 * a;
+      const int c = cnt[d];
+      if (c == 0) continue;
+      double sv[4] = {0.0, 0.0, 0.0, 0.0}, sp = 0.0;
+      for (int m = 0; m < c; ++m) {
+        const int nb = nbAt(d, m);
+        sv[0] += U[nb];
+        sv[1] += V[nb];
+        sv[2] += W[nb];
+        sv[3] += P[nb];
+        sp += centerAxis(G, keys[nb], lev[nb], a);
+      }
+      const double inv = 1.0 / double(c);
+      for (int f = 0; f < 4; ++f) val[s][f] = sv[f] * inv;
+      pos[s] = sp * inv;
+      has[s] = true;
+    }
+    if (!has[0] && !has[1]) {
+      for (int f = 0; f < 4; ++f) g[f][a] = 0.0;
+      continue;
+    }
+    const double dd = pos[0] - pos[1];
+    for (int f = 0; f < 4; ++f) g[f][a] = dd != 0.0 ? (val[0][f] - val[1][f]) / dd : 0.0;
+  }
+  const double* du = g[0];
+  const double* dv = g[1];
+  const double* dw = g[2];
+  const double* dp = g[3];
+  const double ox = dw[1] - dv[2], oy = du[2] - dw[0], oz = dv[0] - du[1];
+  const double vort = sqrt(ox * ox + oy * oy + oz * oz);
+  const double sxx = du[0], syy = dv[1], szz = dw[2];
+  const double sxy = 0.5 * (du[1] + dv[0]), sxz = 0.5 * (du[2] + dw[0]), syz = 0.5 * (dv[2] + dw[1]);
+  const double ss = sxx * sxx + syy * syy + szz * szz + 2.0 * (sxy * sxy + sxz * sxz + syz * syz);
+  const double rxy = 0.5 * (du[1] - dv[0]), rxz = 0.5 * (du[2] - dw[0]), ryz = 0.5 * (dv[2] - dw[1]);
+  const double oo = 2.0 * (rxy * rxy + rxz * rxz + ryz * ryz);
+  const double gu = sqrt(du[0] * du[0] + du[1] * du[1] + du[2] * du[2] + dv[0] * dv[0] + dv[1] * dv[1] +
+                         dv[2] * dv[2] + dw[0] * dw[0] + dw[1] * dw[1] + dw[2] * dw[2]);
+  const double gp = sqrt(dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2]);
+  // near a wall: no leaf across a y face (floor, ceiling) or an InsideSolid neighbour
+  bool wall = cnt[2] == 0 || cnt[3] == 0;
+  double msum = 0.0;  // neighbours' speeds for the truncation estimate, in order
+  int mc = 0;
+  for (int d = 0; d < 6; ++d)
+    for (int m = 0; m < cnt[d]; ++m) {
+      const int nb = nbAt(d, m);
+      if (mask[nb] == 2) wall = true;
+      msum += sqrt(U[nb] * U[nb] + V[nb] * V[nb] + W[nb] * W[nb]);
+      ++mc;
+    }
+  const double speed = sqrt(self[0] * self[0] + self[1] * self[1] + self[2] * self[2]);
+  double yplus = -1.0, tau = 0.0;
+  if (wall) {
+    const double yw = 0.5 * sizeAxis(G, li, 1);
+    const double ut = sqrt(self[0] * self[0] + self[2] * self[2]);
+    tau = nu * ut / (yw + 1e-300);
+    yplus = sqrt(tau) * yw / (nu + 1e-300);
+  }
+  const double res = fabs(speed - sPrev[i]);
+  const double rp = rPrev[i];
+  const long long o = (long long)perm[i];
+  out[o] = vort;
+  out[n + o] = sqrt(2.0 * ss);
+  out[2 * n + o] = 0.5 * (oo - ss);
+  out[3 * n + o] = gp;
+  out[4 * n + o] = gu;
+  out[5 * n + o] = yplus;
+  out[6 * n + o] = tau;
+  out[7 * n + o] = mc ? fabs(speed - msum / mc) : 0.0;
+  out[8 * n + o] = speed;
+  out[9 * n + o] = res;
+  flags[o] = wall ? 1 : 0;
+  flags[n + o] = (wall && self[0] < 0.0) ? 1 : 0;
+  flags[2 * n + o] = (rp > 1e-9 && res >= 0.95 * rp) ? 1 : 0;
+}
+
 // Face neighbours for the transport kernel: the first neighbour of every
 // direction slot-major (first[dir*n + i], -1 when none), the rest of each
 // direction (2..4 finer neighbours) per leaf in an overflow list.
@@ -867,9 +987,89 @@ double OctreeDev::transport(const double* u, const double* v, const double* w, d
   return hctl_[0].slot[O_INTEGRAL];
 }
 
+void OctreeDev::sensors(const double* u, const double* v, const double* w, const double* p, double nu,
+                        double* speedPrev, double* residPrev, double* out, uint8_t* flags) {
+  const size_t n = L0_.n;
+  const long long N = (long long)n;
+  hipStream_t s = stream_;
+  if (!sensOut_.bytes()) {
+    sensOut_ = DevBuf(ctx_, 10 * n * sizeof(double));
+    sensFlags_ = DevBuf(ctx_, 3 * n);
+  }
+  toSorted(u, r_.as<double>());
+  toSorted(v, z_.as<double>());
+  toSorted(w, p_.as<double>());
+  toSorted(p, x_.as<double>());
+  toSorted(speedPrev, rhs_.as<double>());
+  toSorted(residPrev, Ap_.as<double>());
+  Geo G;
+  for (int a = 0; a < 3; ++a) {
+    G.org[a] = org_[a];
+    G.fine[a] = fine_[a];
+  }
+  G.maxL = maxL_;
+  const FaceWalk F{cnt_.as<uint8_t>(), tfirst_.as<int32_t>(), txrow_.as<int32_t>(), txnb_.as<int32_t>()};
+  NBL_HIP(hipEventRecord(e0_, s));
+  k_oct_sensors<<<gridFor(N), kBlock, 0, s>>>(N, G, L0_.keys.as<uint64_t>(), L0_.lev.as<uint8_t>(),
+                                              L0_.mask.as<uint8_t>(), F, r_.as<double>(), z_.as<double>(),
+                                              p_.as<double>(), x_.as<double>(), rhs_.as<double>(), Ap_.as<double>(),
+                                              nu, perm_.as<uint64_t>(), sensOut_.as<double>(),
+                                              sensFlags_.as<uint8_t>());
+  NBL_LAUNCH_CHECK();
+  NBL_HIP(hipEventRecord(e1_, s));
+  const double* d = sensOut_.as<double>();
+  if (out) NBL_HIP(hipMemcpyAsync(out, d, 8 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+  NBL_HIP(hipMemcpyAsync(speedPrev, d + 8 * n, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  NBL_HIP(hipMemcpyAsync(residPrev, d + 9 * n, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (flags) NBL_HIP(hipMemcpyAsync(flags, sensFlags_.as<void>(), 3 * n, hipMemcpyDeviceToHost, s));
+  NBL_HIP(hipStreamSynchronize(s));
+  float t = 0.f;
+  NBL_HIP(hipEventElapsedTime(&t, e0_, e1_));
+  lastMs_ = t;
+}
+
 void OctreeDev::profile(int kernel, int reps, double* ms, double* bytes) {
-  if (kernel != 0 && kernel != 1) throw std::invalid_argument("octree profile: kernel must be 0 (SpMV) or 1 (transport)");
+  if (kernel < 0 || kernel > 2)
+    throw std::invalid_argument("octree profile: kernel must be 0 (SpMV), 1 (transport) or 2 (sensors)");
   if (reps < 1) throw std::invalid_argument("octree profile: reps must be >= 1");
+  if (kernel == 2) {
+    const size_t n = L0_.n;
+    const long long N = (long long)n;
+    if (!sensOut_.bytes()) {
+      sensOut_ = DevBuf(ctx_, 10 * n * sizeof(double));
+      sensFlags_ = DevBuf(ctx_, 3 * n);
+    }
+    for (DevBuf* d : {&r_, &z_, &p_, &x_, &rhs_, &Ap_}) NBL_HIP(hipMemsetAsync(d->as<void>(), 0, n * 8, stream_));
+    Geo G;
+    for (int a = 0; a < 3; ++a) {
+      G.org[a] = org_[a];
+      G.fine[a] = fine_[a];
+    }
+    G.maxL = maxL_;
+    const FaceWalk F{cnt_.as<uint8_t>(), tfirst_.as<int32_t>(), txrow_.as<int32_t>(), txnb_.as<int32_t>()};
+    auto one = [&] {
+      k_oct_sensors<<<gridFor(N), kBlock, 0, stream_>>>(
+          N, G, L0_.keys.as<uint64_t>(), L0_.lev.as<uint8_t>(), L0_.mask.as<uint8_t>(), F, r_.as<double>(),
+          z_.as<double>(), p_.as<double>(), x_.as<double>(), rhs_.as<double>(), Ap_.as<double>(), 1e-3,
+          perm_.as<uint64_t>(), sensOut_.as<double>(), sensFlags_.as<uint8_t>());
+    };
+    one();
+    NBL_HIP(hipEventRecord(e0_, stream_));
+    for (int q = 0; q < reps; ++q) one();
+    NBL_HIP(hipEventRecord(e1_, stream_));
+    NBL_LAUNCH_CHECK();
+    NBL_HIP(hipEventSynchronize(e1_));
+    float t = 0.f;
+    NBL_HIP(hipEventElapsedTime(&t, e0_, e1_));
+    *ms = double(t) / reps;
+    // compulsory traffic (a neighbour's key, level, mask and fields are another
+    // leaf's own): per leaf its key, level, mask, six counts, six first-neighbour
+    // slots, u, v, w, p and the carried speed and residual, the permutation,
+    // ten results and three flags; per further neighbour its index
+    *bytes = double(n) * (8.0 + 1.0 + 1.0 + 6.0 + 24.0 + 4.0 * 8.0 + 2.0 * 8.0 + 8.0 + 10.0 * 8.0 + 3.0) +
+             double(tnx_) * 4.0;
+    return;
+  }
   if (kernel == 1) {
     const long long N = (long long)L0_.n;
     for (DevBuf* d : {&r_, &z_, &p_, &x_}) NBL_HIP(hipMemsetAsync(d->as<void>(), 0, L0_.n * 8, stream_));

@@ -128,6 +128,13 @@ class OctreeDev {
   // octreeTransportStep on phi (caller-order host arrays), `steps` times with
   // the fields resident; returns the integral sum phi_i vol_i after the last.
   double transport(const double* u, const double* v, const double* w, double* phi, double dt, int steps);
+  // PhysicalSensorEvaluator::evaluate on the leaves (caller-order host arrays):
+  // out[8n] field-major (vorticity, strain, Q, |grad p|, |grad u|, y+, wall
+  // shear, truncation estimate), flags[3n] (near wall, reverse flow, residual
+  // stagnation); speedPrev / residPrev are read and updated.  Any output may
+  // be null.
+  void sensors(const double* u, const double* v, const double* w, const double* p, double nu, double* speedPrev,
+               double* residPrev, double* out, uint8_t* flags);
   // Multigrid hierarchy: level count, leaves per level (up to `cap` entries)
   // and the device time of its construction (ms).  Builds it if needed.
   int mgLevels(long long* counts, int cap, double* buildMs);
@@ -149,6 +156,7 @@ class OctreeDev {
   OctLevel L0_;
   DevBuf perm_, cnt_, nbr_;
   DevBuf tfirst_, txrow_, txnb_;  // transport's face neighbours (first per direction slot-major, overflow)
+  DevBuf sensOut_, sensFlags_;    // sensor results in caller order (allocated on first use)
   size_t tnx_ = 0;
   DevBuf x_, rhs_, r_, z_, p_, Ap_, t0_, t1_;
   DevBuf ctl_, red_;

@@ -74,6 +74,12 @@ P_U64 = C.POINTER(C.c_uint64)
 P_I32 = C.POINTER(C.c_int32)
 
 
+class SensorFields(C.Structure):
+    _fields_ = [(f, P_D) for f in ("vorticity_mag", "strain_mag", "q_criterion", "grad_p_mag", "grad_u_mag",
+                                   "y_plus", "tau_wall", "trunc_err")] + \
+               [(f, P_U8) for f in ("near_wall", "reverse_flow", "resid_stagnation")]
+
+
 def dptr(a: np.ndarray):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(P_D)
@@ -225,6 +231,7 @@ def load(path: os.PathLike | None = None):
     _proto(lib, "nbl_octree_diagonal", i, V, P_D)
     _proto(lib, "nbl_octree_poisson_solve", i, V, P_D, P_D, d, i, C.POINTER(PoissonReport))
     _proto(lib, "nbl_octree_transport", i, V, P_D, P_D, P_D, P_D, d, i, C.POINTER(C.c_double))
+    _proto(lib, "nbl_octree_sensors", i, V, P_D, P_D, P_D, P_D, d, P_D, P_D, C.POINTER(SensorFields))
     _proto(lib, "nbl_octree_mg_solve", i, V, P_D, P_D, d, i, C.POINTER(PoissonReport))
     _proto(lib, "nbl_octree_mg_levels", i, V, C.POINTER(C.c_longlong), i, C.POINTER(i),
            C.POINTER(C.c_double))
@@ -561,6 +568,22 @@ class Octree:
         check(self.lib.nbl_octree_transport(self.h, dptr(u), dptr(v), dptr(w), dptr(p), dt, steps,
                                             C.byref(integral)))
         return p, integral.value
+
+    def sensors(self, u, v, w, p, nu, speed_prev, resid_prev):
+        """PhysicalSensorEvaluator(nu).evaluate on the leaves: (dict of the
+        per-leaf sensors and flags, updated speed_prev, updated resid_prev)."""
+        u, v, w, p = (np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w, p))
+        sp = np.array(speed_prev, dtype=np.float64)
+        rp = np.array(resid_prev, dtype=np.float64)
+        res = {}
+        out = SensorFields()
+        for f, t in SensorFields._fields_:
+            a = np.empty(self.n, np.float64 if t is P_D else np.uint8)
+            res[f] = a
+            setattr(out, f, a.ctypes.data_as(t))
+        check(self.lib.nbl_octree_sensors(self.h, dptr(u), dptr(v), dptr(w), dptr(p), nu, dptr(sp), dptr(rp),
+                                          C.byref(out)))
+        return res, sp, rp
 
     def mg_solve(self, b, tol=1e-9, max_iters=2000):
         """Multigrid-preconditioned flexible CG on the same system as solve()."""

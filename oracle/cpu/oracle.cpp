@@ -1208,6 +1208,86 @@ struct Oct {
     for (size_t i = 0; i < n; ++i) sum += phi[i] * (size(i, 0) * size(i, 1) * size(i, 2));
     return sum;
   }
+  // Gradient of f at leaf i across level jumps (octree.cpp:491-539): per axis
+  // the mean value and mean centre of the leaves across the + and - faces (the
+  // leaf itself stands in for a missing side), differenced.
+  std::array<double, 3> gradient(const Vec& f, size_t i) const {
+    std::array<double, 3> g{0.0, 0.0, 0.0};
+    for (int a = 0; a < 3; ++a) {
+      double val[2] = {f[i], f[i]}, pos[2] = {center(i, a), center(i, a)};
+      bool has[2] = {false, false};
+      for (int side = 0; side < 2; ++side) {  // 0: + face (2a+1), 1: - face (2a)
+        const auto nb = nbrs(i, side == 0 ? 2 * a + 1 : 2 * a);
+        if (nb.empty()) continue;
+        double sv = 0.0, sp = 0.0;
+        for (size_t j : nb) {
+          sv += f[j];
+          sp += center(j, a);
+        }
+        const double inv = 1.0 / double(nb.size());
+        val[side] = sv * inv;
+        pos[side] = sp * inv;
+        has[side] = true;
+      }
+      if (!has[0] && !has[1]) continue;
+      const double d = pos[0] - pos[1];
+      g[a] = d != 0.0 ? (val[0] - val[1]) / d : 0.0;
+    }
+    return g;
+  }
+  // PhysicalSensorEvaluator::evaluate (adaptive_core.cpp:73-176): out[f*n+i]
+  // for f = vorticity magnitude, strain magnitude, Q, |grad p|, |grad u|, y+,
+  // wall shear, truncation estimate; flags[f*n+i] for near-wall, reverse flow,
+  // residual stagnation; speedPrev / residPrev are the carried state.
+  void sensors(const Vec& U, const Vec& V, const Vec& W, const Vec& P, double nu, Vec& speedPrev, Vec& residPrev,
+               double* out, uint8_t* flags) const {
+    const size_t n = key.size();
+    for (size_t i = 0; i < n; ++i) {
+      const auto du = gradient(U, i), dv = gradient(V, i), dw = gradient(W, i), dp = gradient(P, i);
+      const double ox = dw[1] - dv[2], oy = du[2] - dw[0], oz = dv[0] - du[1];
+      const double vort = std::sqrt(ox * ox + oy * oy + oz * oz);
+      const double sxx = du[0], syy = dv[1], szz = dw[2];
+      const double sxy = 0.5 * (du[1] + dv[0]), sxz = 0.5 * (du[2] + dw[0]), syz = 0.5 * (dv[2] + dw[1]);
+      const double ss = sxx * sxx + syy * syy + szz * szz + 2.0 * (sxy * sxy + sxz * sxz + syz * syz);
+      const double rxy = 0.5 * (du[1] - dv[0]), rxz = 0.5 * (du[2] - dw[0]), ryz = 0.5 * (dv[2] - dw[1]);
+      const double oo = 2.0 * (rxy * rxy + rxz * rxz + ryz * ryz);
+      const double gu = std::sqrt(du[0] * du[0] + du[1] * du[1] + du[2] * du[2] + dv[0] * dv[0] + dv[1] * dv[1] +
+                                  dv[2] * dv[2] + dw[0] * dw[0] + dw[1] * dw[1] + dw[2] * dw[2]);
+      const double gp = std::sqrt(dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2]);
+      // near a wall: no leaf across a y face (floor, ceiling) or a solid neighbour
+      bool wall = nbrs(i, 2).empty() || nbrs(i, 3).empty();
+      for (int d = 0; d < 6 && !wall; ++d)
+        for (size_t j : nbrs(i, d))
+          if (msk[j] == 2) wall = true;
+      const double speed = std::sqrt(U[i] * U[i] + V[i] * V[i] + W[i] * W[i]);
+      double yplus = -1.0, tau = 0.0;
+      if (wall) {
+        const double yw = 0.5 * size(i, 1);
+        const double ut = std::sqrt(U[i] * U[i] + W[i] * W[i]);
+        tau = nu * ut / (yw + 1e-300);
+        yplus = std::sqrt(tau) * yw / (nu + 1e-300);
+      }
+      double msum = 0.0;
+      int cnt = 0;
+      for (int d = 0; d < 6; ++d)
+        for (size_t j : nbrs(i, d)) {
+          msum += std::sqrt(U[j] * U[j] + V[j] * V[j] + W[j] * W[j]);
+          ++cnt;
+        }
+      const double vals[8] = {vort, std::sqrt(2.0 * ss), 0.5 * (oo - ss), gp, gu, yplus, tau,
+                              cnt ? std::abs(speed - msum / cnt) : 0.0};
+      for (int f = 0; f < 8; ++f) out[size_t(f) * n + i] = vals[f];
+      flags[i] = wall ? 1 : 0;
+      flags[n + i] = (wall && U[i] < 0.0) ? 1 : 0;
+    }
+    for (size_t i = 0; i < n; ++i) {  // residual stagnation: the second pass
+      const double speed = std::sqrt(U[i] * U[i] + V[i] * V[i] + W[i] * W[i]);
+      const double res = std::abs(speed - speedPrev[i]);
+      flags[2 * n + i] = (residPrev[i] > 1e-9 && res >= 0.95 * residPrev[i]) ? 1 : 0;
+      residPrev[i] = res;
+      speedPrev[i] = speed;
+    }
+  }
   // Jacobi-PCG, always singular (octree_poisson.cpp:107-197).
   nbl_poisson_report solve(const Vec& b, Vec& x, double tol, int maxIt) const {
     const size_t n = key.size();
@@ -1893,6 +1973,16 @@ int orc_oct_mg_levels(void* h, long long* counts, int cap) {
   return int(mg.L.size());
 }
 
+
+void orc_oct_sensors(void* h, const double* u, const double* v, const double* w, const double* p, double nu,
+                     double* speedPrev, double* residPrev, double* out, uint8_t* flags) {
+  const Oct& o = *static_cast<Oct*>(h);
+  const size_t n = o.key.size();
+  Vec U(u, u + n), V(v, v + n), W(w, w + n), P(p, p + n), sp(speedPrev, speedPrev + n), rp(residPrev, residPrev + n);
+  o.sensors(U, V, W, P, nu, sp, rp, out, flags);
+  std::copy(sp.begin(), sp.end(), speedPrev);
+  std::copy(rp.begin(), rp.end(), residPrev);
+}
 
 double orc_oct_transport(void* h, const double* u, const double* v, const double* w, double* phi, double dt) {
   const Oct& o = *static_cast<Oct*>(h);

@@ -29,6 +29,17 @@ P_I64 = C.POINTER(C.c_int64)
 dptr = nbl.dptr
 
 
+SENSOR_FIELDS = ("vorticity_mag", "strain_mag", "q_criterion", "grad_p_mag", "grad_u_mag", "y_plus",
+                 "tau_wall", "trunc_err")
+SENSOR_FLAGS = ("near_wall", "reverse_flow", "resid_stagnation")
+
+
+def sensor_dict(out, flags, n):
+    d = {f: out[k * n:(k + 1) * n].copy() for k, f in enumerate(SENSOR_FIELDS)}
+    d.update({f: flags[k * n:(k + 1) * n].copy() for k, f in enumerate(SENSOR_FLAGS)})
+    return d
+
+
 class OracleLib:
     def __init__(self, kind: str = "orc"):
         path = ORACLE_LIB if kind == "orc" else REF_LIB
@@ -61,6 +72,7 @@ class OracleLib:
             "oct_centers": (None, V, P_D), "oct_apply": (None, V, P_D, P_D),
             "oct_diagonal": (None, V, P_D), "oct_solve": (None, V, P_D, P_D, d, i, PR),
             "oct_transport": (d, V, P_D, P_D, P_D, P_D, d),
+            "oct_sensors": (None, V, P_D, P_D, P_D, P_D, d, P_D, P_D, P_D, P_U8),
         }
         if kind == "orc":
             spec.update({
@@ -278,6 +290,18 @@ class Oct:
         p = np.array(phi, dtype=np.float64)
         integral = self.o.oct_transport(self.h, dptr(u), dptr(v), dptr(w), dptr(p), dt)
         return p, integral
+
+    def sensors(self, u, v, w, p, nu, speed_prev, resid_prev):
+        """PhysicalSensorEvaluator::evaluate: (dict of sensor arrays, new
+        speed_prev, new resid_prev)."""
+        u, v, w, p = (np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w, p))
+        sp = np.array(speed_prev, dtype=np.float64)
+        rp = np.array(resid_prev, dtype=np.float64)
+        out = np.empty(8 * self.n)
+        flags = np.empty(3 * self.n, np.uint8)
+        self.o.oct_sensors(self.h, dptr(u), dptr(v), dptr(w), dptr(p), nu, dptr(sp), dptr(rp), dptr(out),
+                           flags.ctypes.data_as(P_U8))
+        return sensor_dict(out, flags, self.n), sp, rp
 
     def mg_solve(self, b, tol=1e-9, max_iters=2000):
         """Octree multigrid flexible CG (restatement; oracle library only)."""

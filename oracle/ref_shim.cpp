@@ -260,4 +260,34 @@ double ref_oct_transport(void* h, const double* u, const double* v, const double
   std::memcpy(phi, t.scalar("phi").data(), n * 8);
   return I;
 }
+
+// PhysicalSensorEvaluator::evaluate on the tree's leaves: out holds the eight
+// real sensors field-major (vorticity, strain, Q, |grad p|, |grad u|, y+,
+// tau_wall, truncation estimate), flags the near-wall, reverse-flow and
+// residual-stagnation flags; speed_prev / resid_prev are read and updated.
+void ref_oct_sensors(void* h, const double* u, const double* v, const double* w, const double* p, double nu,
+                     double* speedPrev, double* residPrev, double* out, uint8_t* flags) {
+  auto& t = *static_cast<mesh::Octree*>(h);
+  const size_t n = t.cellCount();
+  t.u().assign(u, u + n);
+  t.v().assign(v, v + n);
+  t.w().assign(w, w + n);
+  t.p().assign(p, p + n);
+  t.registerScalar("speed_prev");
+  t.registerScalar("resid_prev");
+  t.scalar("speed_prev").assign(speedPrev, speedPrev + n);
+  t.scalar("resid_prev").assign(residPrev, residPrev + n);
+  const auto rows = adaptive::PhysicalSensorEvaluator(nu).evaluate(t);
+  for (size_t i = 0; i < n; ++i) {
+    const auto& r = rows[i];
+    const double vals[8] = {r.vorticityMag, r.strainMag, r.qCriterion, r.gradPMag,
+                            r.gradUMag,     r.yPlus,     r.tauWall,    r.truncErr};
+    for (int f = 0; f < 8; ++f) out[size_t(f) * n + i] = vals[f];
+    flags[i] = r.nearWall ? 1 : 0;
+    flags[n + i] = r.reverseFlow ? 1 : 0;
+    flags[2 * n + i] = r.residStagnation ? 1 : 0;
+  }
+  std::memcpy(speedPrev, t.scalar("speed_prev").data(), n * 8);
+  std::memcpy(residPrev, t.scalar("resid_prev").data(), n * 8);
+}
 }  // extern "C"

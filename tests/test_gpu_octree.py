@@ -319,3 +319,76 @@ def test_c3_transport_conserves(ctx, c3):
         assert abs(i1 - i0) <= 1e-12 * abs(i0)
     finally:
         d.close()
+
+
+# ---- PhysicalSensorEvaluator::evaluate (adaptive/adaptive_core.cpp:73-176) ---
+def _sensor_fields(t, seed):
+    c = t.centers()
+    u = np.sin(2.0 * c[:, 0]) * np.cos(3.0 * c[:, 1]) + 0.3 * cases.normal(seed, t.n)
+    v = np.cos(1.5 * c[:, 2]) - 0.2 * cases.normal(seed + 1, t.n)
+    w = c[:, 0] * c[:, 1] + 0.1 * cases.normal(seed + 2, t.n)
+    p = cases.normal(seed + 3, t.n)
+    return [u, v, w, p]
+
+
+@pytest.mark.parametrize("which", [0, 1, 2], ids=["uniform8", "jump", "storm_masked"])
+def test_sensors_bitwise(ctx, tree_list, which):
+    """Every sensor, flag and the carried speed / residual state of every leaf
+    are bit-identical to the oracle's (itself bit-identical to the reference's
+    evaluator), over three evaluations with the state carried between them."""
+    _name, t = tree_list[which]
+    f = _sensor_fields(t, 61)
+    scale = 1.0 + np.abs(cases.normal(65, t.n))
+    d = device_tree(ctx, t)
+    try:
+        spg = rpg = spo = rpo = np.zeros(t.n)
+        for _ in range(3):
+            sg, spg, rpg = d.sensors(*f, 1.7e-3, spg, rpg)
+            so, spo, rpo = t.sensors(*f, 1.7e-3, spo, rpo)
+            for k in so:
+                assert np.array_equal(sg[k], so[k]), k
+            assert np.array_equal(spg, spo) and np.array_equal(rpg, rpo)
+            f = [f[0] * scale, f[1] * scale, f[2] * scale, f[3]]
+    finally:
+        d.close()
+
+
+def test_sensors_adaptive_tree_bitwise(ctx, orc):
+    """The same on the 31k-leaf anisotropic tree with 2:1 jumps and an
+    InsideSolid block."""
+    from octree_trees import adaptive_octree
+    t = adaptive_octree(orc)
+    f = _sensor_fields(t, 71)
+    d = device_tree(ctx, t)
+    try:
+        sg, spg, rpg = d.sensors(*f, 2e-3, np.zeros(t.n), np.zeros(t.n))
+        so, spo, rpo = t.sensors(*f, 2e-3, np.zeros(t.n), np.zeros(t.n))
+        for k in so:
+            assert np.array_equal(sg[k], so[k]), k
+        assert np.array_equal(spg, spo) and np.array_equal(rpg, rpo)
+        assert so["near_wall"].sum() > 0 and so["reverse_flow"].sum() > 0
+    finally:
+        d.close()
+
+
+def test_c3_sensors_invariants(ctx, c3):
+    """On the 4.3M-leaf tree: a uniform stream has exactly zero gradients,
+    vorticity, strain and Q on every leaf (2:1 jumps included) and a
+    truncation estimate at round-off, a positive wall shear on
+    the near-wall leaves and y+ = -1 elsewhere, and the carried speed and
+    residual equal the stream speed after a first evaluation from zero."""
+    _f, k, l, m, _h = c3
+    d = nbl.Octree(ctx, k, l, m, cases.C3_ORIGIN, cases.C3_EXTENT, 8)
+    try:
+        n = k.size
+        z = np.zeros(n)
+        s, sp, rp = d.sensors(np.full(n, 1.1), z, z, z, 2e-3, z, z)
+        for key in ("vorticity_mag", "strain_mag", "q_criterion", "grad_u_mag", "grad_p_mag"):
+            assert np.all(s[key] == 0.0), key
+        # the neighbours' mean speed is a sum of up to 24 equal terms divided back
+        assert np.all(s["trunc_err"] <= 1e-15)
+        wall = s["near_wall"] == 1
+        assert wall.sum() > 0 and np.all(s["y_plus"][~wall] == -1.0)
+        assert np.all(s["tau_wall"][wall] > 0.0) and np.all(sp == 1.1) and np.all(rp == 1.1)
+    finally:
+        d.close()

@@ -378,3 +378,56 @@ def test_octree_transport_conserves_and_is_upwind(orc):
             continue
         want = q[n] - 0.01 / (1.0 / 8) * 0.5 * (q[n] - q[lin[(i - 1, j, k)]])
         assert out[n] == pytest.approx(want, rel=1e-13, abs=1e-14)
+
+
+def test_octree_sensors_closed_forms(orc):
+    """PhysicalSensorEvaluator::evaluate (adaptive_core.cpp:73-176) against
+    what the mathematics fixes.  On a uniform tree the gradient across faces
+    (octree.cpp:491-539) is exact for linear fields, one-sided at the domain
+    boundary; solid-body rotation u = -w y, v = w x has |vorticity| = 2w,
+    zero strain and Q = w^2; a simple shear u = g y has |vorticity| = strain
+    = g and Q = 0; a uniform stream near the floor has the wall shear
+    nu U / (dy/2) and y+ = sqrt(tau) (dy/2) / nu.  A dropped term, a wrong
+    sign or a transposed gradient component fails one of them."""
+    t = orc.octree((0, 0, 0), (2.0, 1.0, 1.5), 4)
+    t.refine_uniform(4)
+    n = t.n
+    c = t.centers()
+    x, y, z = c[:, 0], c[:, 1], c[:, 2]
+    zero = np.zeros(n)
+    # linear fields: grad p = (0.7, -1.3, 0.4), grad u = (0.2, 0.5, -0.1)
+    p = 0.7 * x - 1.3 * y + 0.4 * z
+    u = 0.2 * x + 0.5 * y - 0.1 * z
+    s, _, _ = t.sensors(u, zero, zero, p, 1e-3, zero, zero)
+    assert np.allclose(s["grad_p_mag"], np.sqrt(0.7**2 + 1.3**2 + 0.4**2), rtol=1e-12)
+    assert np.allclose(s["grad_u_mag"], np.sqrt(0.2**2 + 0.5**2 + 0.1**2), rtol=1e-12)
+    # vorticity of u alone: (0, du/dz, -du/dy) = (0, -0.1, -0.5)
+    assert np.allclose(s["vorticity_mag"], np.sqrt(0.1**2 + 0.5**2), rtol=1e-12)
+    om = 0.8
+    s, _, _ = t.sensors(-om * y, om * x, zero, zero, 1e-3, zero, zero)
+    assert np.allclose(s["vorticity_mag"], 2 * om, rtol=1e-12)
+    assert np.allclose(s["strain_mag"], 0.0, atol=1e-12)
+    assert np.allclose(s["q_criterion"], om * om, rtol=1e-12)
+    g = 1.7
+    s, _, _ = t.sensors(g * y, zero, zero, zero, 1e-3, zero, zero)
+    assert np.allclose(s["vorticity_mag"], g, rtol=1e-12)
+    assert np.allclose(s["strain_mag"], g, rtol=1e-12)
+    assert np.allclose(s["q_criterion"], 0.0, atol=1e-12)
+    # wall sensors: only the floor and ceiling layers (no leaf across a y face)
+    nu, U0 = 2e-3, 1.25
+    s, sp, rp = t.sensors(np.full(n, U0), zero, zero, zero, nu, zero, zero)
+    dy = 1.0 / 16
+    wall = (y < dy) | (y > 1.0 - dy)
+    assert (s["near_wall"] == wall).all() and (s["reverse_flow"] == 0).all()
+    tau = nu * U0 / (0.5 * dy)
+    assert np.allclose(s["tau_wall"][wall], tau, rtol=1e-14)
+    assert np.allclose(s["y_plus"][wall], np.sqrt(tau) * 0.5 * dy / nu, rtol=1e-14)
+    assert (s["y_plus"][~wall] == -1.0).all() and (s["tau_wall"][~wall] == 0.0).all()
+    assert np.allclose(s["trunc_err"], 0.0, atol=1e-15)  # uniform speed
+    # carried state: speed and |speed - previous speed|; stagnation once the
+    # change stops shrinking by 5%
+    assert np.allclose(sp, U0) and np.allclose(rp, U0)
+    s2, sp2, rp2 = t.sensors(np.full(n, 2 * U0), zero, zero, zero, nu, sp, rp)
+    assert (s2["resid_stagnation"] == 1).all() and np.allclose(rp2, U0)
+    s3, _, _ = t.sensors(np.full(n, 2 * U0), zero, zero, zero, nu, sp2, rp2)
+    assert (s3["resid_stagnation"] == 0).all()

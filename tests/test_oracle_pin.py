@@ -132,6 +132,19 @@ def _check_oct(a, b):
         pa, ia = a.transport(u, v, w, phi, 0.02)
         pb, ib = b.transport(u, v, w, phi, 0.02)
         assert same(pa, pb) and ia == ib
+    # PhysicalSensorEvaluator::evaluate (adaptive_core.cpp:73-176): seeded
+    # fields over three evaluations, so the carried speed / residual state and
+    # the stagnation flag take both values
+    f = [cases.normal(51 + k, a.n) for k in range(4)]
+    scale = 1.0 + np.abs(cases.normal(55, a.n))
+    spa = spb = rpa = rpb = np.zeros(a.n)
+    for _ in range(3):
+        sa, spa, rpa = a.sensors(*f, 1.3e-3, spa, rpa)
+        sb, spb, rpb = b.sensors(*f, 1.3e-3, spb, rpb)
+        for k in sa:
+            assert same(sa[k], sb[k]), k
+        assert same(spa, spb) and same(rpa, rpb)
+        f = [f[0] * scale, f[1] * scale, f[2] * scale, f[3]]
     bb = a.apply(x)
     xa, ra = a.solve(bb, 1e-10, 3000)
     xb, rb = b.solve(bb, 1e-10, 3000)
