@@ -146,6 +146,12 @@ def octree_c3(ctx, with_cpu=True):
     n, nnz = d.counts()
     ms, by = d.profile(0, 50)
     tms, tby = d.profile(1, 50)
+    sms, sby = d.profile(2, 20)
+    zs = np.zeros(k.size)
+    ones = np.ones(k.size)
+    t1 = time.perf_counter()
+    d.sensors(ones, zs, zs, zs, 1e-3, zs, zs)
+    sens_call_s = time.perf_counter() - t1
     fl = m == 0
     b = cases.normal(11, k.size)
     b[~fl] = 0.0
@@ -171,7 +177,14 @@ def octree_c3(ctx, with_cpu=True):
                              "bytes_per_launch": by},
            "transport_step": {"kernel": "k_oct_transport (octreeTransportStep, first-order upwind)",
                               "avg_launch_us": round(tms * 1e3, 3), "mleaves_per_s": round(n / tms / 1e3, 1),
-                              "achieved_gbs": round(tby / tms / 1e6, 1), "bytes_per_launch": tby}}
+                              "achieved_gbs": round(tby / tms / 1e6, 1), "bytes_per_launch": tby},
+           "sensors": {"kernel": "k_oct_sensors (PhysicalSensorEvaluator::evaluate: gradients of u, v, w, p across "
+                                 "level jumps, vorticity, strain, Q, wall shear, y+, truncation, stagnation)",
+                       "avg_launch_us": round(sms * 1e3, 3), "mleaves_per_s": round(n / sms / 1e3, 1),
+                       "achieved_gbs": round(sby / sms / 1e6, 1), "bytes_per_launch": sby,
+                       "host_call_s": round(sens_call_s, 4),
+                       "host_call_note": "nbl_octree_sensors with host arrays: PCIe copies of 6 inputs and 12 "
+                                         "outputs included"}}
     log(f"octree: {n} leaves, setup {setup_ms:.1f} ms, Jacobi-PCG {rep.iterations} iterations in {solve_ms:.1f} ms, "
         f"MG-PCG {mrep.iterations} iterations in {mg_ms:.1f} ms")
     if with_cpu:
@@ -200,12 +213,17 @@ def octree_c3(ctx, with_cpu=True):
         t1 = time.perf_counter()
         rt.transport(xr, xr, z, xr, 1e-4)
         transport_s = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        rt.sensors(xr, xr, z, xr, 1e-3, z, z)
+        sensors_s = time.perf_counter() - t1
         rt.close()
         out["cpu_baseline"] = {"kind": kind, "cores": 1, "connectivity_s": round(conn_s, 3),
                                "applyA_s": round(apply_s, 3), "transport_step_s": round(transport_s, 3),
+                               "sensors_s": round(sensors_s, 3),
                                "est_solve_s": round(apply_s * rep.iterations, 1),
-                               "sample": "faceNeighbors over every leaf, one OctreePoisson::applyA and one "
-                                         "octreeTransportStep on the same tree; est_solve_s = applyA_s x the "
+                               "sample": "faceNeighbors over every leaf, one OctreePoisson::applyA, one "
+                                         "octreeTransportStep and one PhysicalSensorEvaluator::evaluate on the "
+                                         "same tree; est_solve_s = applyA_s x the "
                                          "device Jacobi-PCG iteration count"}
     return out
 
