@@ -4,12 +4,18 @@
 #pragma once
 
 #include <cstddef>
+#include <memory>
 #include <string>
+#include <vector>
 
 #include "nabla/flow/grid.hpp"
 #include "nabla/flow/ns_solver.hpp"
 
 namespace nabla::flow {
+
+// VTU encoding of snapshots ("vtu_format": "ascii", the reference's, or
+// "binary": VTK appended raw data, the same arrays).
+enum class VtuFormat { Ascii, Binary };
 
 struct RunOptions {
   std::string runDir = "run";
@@ -20,6 +26,7 @@ struct RunOptions {
   int checkpointEvery = 0;
   std::string restartFrom;
   bool quiet = false;
+  VtuFormat vtuFormat = VtuFormat::Ascii;
 };
 
 struct CaseSpec {
@@ -51,6 +58,30 @@ CaseSpec loadCase(const std::string& jsonPath);
 RunSummary runCase(const CaseSpec& spec);
 
 void writeGridVtu(const NSSolver& solver, const std::string& path);
+
+// The cell fields one VTU snapshot prints, copied to the host.
+struct VtuSnapshot {
+  UniformGrid grid;
+  std::vector<double> u, v, w, p;
+};
+VtuSnapshot captureVtu(const NSSolver& solver);
+void writeVtu(const VtuSnapshot& s, const std::string& path, VtuFormat fmt);
+
+// Writes submitted snapshots in order on a thread of its own; drain() waits
+// for all of them and rethrows the first write error.
+class SnapshotWriter {
+ public:
+  explicit SnapshotWriter(VtuFormat fmt);
+  ~SnapshotWriter();
+  SnapshotWriter(const SnapshotWriter&) = delete;
+  SnapshotWriter& operator=(const SnapshotWriter&) = delete;
+  void submit(VtuSnapshot s, std::string path);
+  void drain();
+
+ private:
+  struct Impl;
+  std::unique_ptr<Impl> impl_;
+};
 void writeCheckpoint(const NSSolver& solver, const std::string& path);
 void readCheckpoint(NSSolver& solver, const std::string& path);
 

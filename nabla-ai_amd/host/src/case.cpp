@@ -230,60 +230,16 @@ CaseSpec loadCase(const std::string& jsonPath) {  // case.cpp:136-170
   s.run.runDir = j.string("run_dir", "run_" + type);
   s.run.maxSteps = j.integer("max_steps", s.run.maxSteps);
   s.run.snapshotEvery = j.integer("snapshot_every", s.run.snapshotEvery);
+  {
+    const std::string vf = j.string("vtu_format", "ascii");
+    if (vf == "binary") s.run.vtuFormat = VtuFormat::Binary;
+    else if (vf != "ascii") throw std::invalid_argument("vtu_format must be \"ascii\" or \"binary\"");
+  }
   s.run.checkpointEvery = j.integer("checkpoint_every", s.run.checkpointEvery);
   s.run.stopTime = j.number("stop_time", s.run.stopTime);
   s.run.steadyTol = j.number("steady_tol", s.run.steadyTol);
   s.run.restartFrom = j.string("restart_from", "");
   return s;
-}
-
-// ---- VTU snapshot (case.cpp:175-241) ---------------------------------------
-void writeGridVtu(const NSSolver& solver, const std::string& path) {
-  const UniformGrid& g = solver.grid();
-  std::ofstream f(path);
-  if (!f) throw std::runtime_error("writeGridVtu: cannot open " + path);
-  f.precision(8);
-  const std::size_t ncell = g.cellCount(), npts = ncell * 8;
-  f << "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"1.0\" "
-       "byte_order=\"LittleEndian\" header_type=\"UInt64\">\n  <UnstructuredGrid>\n";
-  f << "    <Piece NumberOfPoints=\"" << npts << "\" NumberOfCells=\"" << ncell << "\">\n";
-  f << "      <Points>\n        <DataArray type=\"Float64\" NumberOfComponents=\"3\" format=\"ascii\">\n";
-  for (int k = 0; k < g.nz; ++k)
-    for (int j = 0; j < g.ny; ++j)
-      for (int i = 0; i < g.nx; ++i) {
-        const double x0 = g.origin.x + i * g.dx, x1 = x0 + g.dx;
-        const double y0 = g.origin.y + j * g.dy, y1 = y0 + g.dy;
-        const double z0 = g.origin.z + k * g.dz, z1 = z0 + g.dz;
-        f << x0 << ' ' << y0 << ' ' << z0 << ' ' << x1 << ' ' << y0 << ' ' << z0 << ' ' << x1 << ' ' << y1 << ' '
-          << z0 << ' ' << x0 << ' ' << y1 << ' ' << z0 << ' ' << x0 << ' ' << y0 << ' ' << z1 << ' ' << x1 << ' '
-          << y0 << ' ' << z1 << ' ' << x1 << ' ' << y1 << ' ' << z1 << ' ' << x0 << ' ' << y1 << ' ' << z1 << '\n';
-      }
-  f << "        </DataArray>\n      </Points>\n      <Cells>\n";
-  f << "        <DataArray type=\"Int64\" Name=\"connectivity\" format=\"ascii\">\n";
-  for (std::size_t i = 0; i < npts; ++i) f << i << ((i + 1) % 8 == 0 ? '\n' : ' ');
-  f << "        </DataArray>\n        <DataArray type=\"Int64\" Name=\"offsets\" format=\"ascii\">\n";
-  for (std::size_t i = 1; i <= ncell; ++i) f << i * 8 << ' ';
-  f << "\n        </DataArray>\n        <DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n";
-  for (std::size_t i = 0; i < ncell; ++i) f << "12 ";
-  f << "\n        </DataArray>\n      </Cells>\n      <CellData Scalars=\"speed\">\n";
-  auto dump = [&](const char* name, const std::vector<double>& a) {
-    f << "        <DataArray type=\"Float64\" Name=\"" << name << "\" format=\"ascii\">\n";
-    for (double x : a) f << x << ' ';
-    f << "\n        </DataArray>\n";
-  };
-  const std::vector<double> u = solver.cellVelocity(0), v = solver.cellVelocity(1), w = solver.cellVelocity(2);
-  std::vector<double> speed(ncell), solid(ncell);
-  for (std::size_t c = 0; c < ncell; ++c) {
-    speed[c] = std::sqrt(u[c] * u[c] + v[c] * v[c] + w[c] * w[c]);
-    solid[c] = g.solid[c] ? 1.0 : 0.0;
-  }
-  dump("u", u);
-  dump("v", v);
-  dump("w", w);
-  dump("p", solver.cellPressure());
-  dump("speed", speed);
-  dump("solid", solid);
-  f << "      </CellData>\n    </Piece>\n  </UnstructuredGrid>\n</VTKFile>\n";
 }
 
 // ---- NBFCK01 checkpoint (case.cpp:247-323) ---------------------------------
@@ -418,12 +374,15 @@ RunSummary runCase(const CaseSpec& spec) {
   sum.runDir = spec.run.runDir;
   sum.cells = solver.grid().cellCount();
   StepReport last;
+  // snapshots are formatted and written on a thread of their own while the
+  // steps go on; the host copy of the fields is taken at the step itself
+  SnapshotWriter vtu(spec.run.vtuFormat);
   for (int s = 0; s < spec.run.maxSteps; ++s) {
     last = solver.step();
     appendDiagnostics(diag, last, last.accepted);
     diag.flush();
     if (spec.run.snapshotEvery > 0 && last.step % spec.run.snapshotEvery == 0)
-      writeGridVtu(solver, spec.run.runDir + "/snapshot_" + std::to_string(last.step) + ".vtu");
+      vtu.submit(captureVtu(solver), spec.run.runDir + "/snapshot_" + std::to_string(last.step) + ".vtu");
     if (spec.run.checkpointEvery > 0 && last.step % spec.run.checkpointEvery == 0)
       writeCheckpoint(solver, spec.run.runDir + "/checkpoint_" + std::to_string(last.step) + ".ckpt");
     if (spec.run.stopTime > 0.0 && solver.time() >= spec.run.stopTime) break;
@@ -432,7 +391,8 @@ RunSummary runCase(const CaseSpec& spec) {
       break;
     }
   }
-  writeGridVtu(solver, spec.run.runDir + "/final.vtu");
+  vtu.submit(captureVtu(solver), spec.run.runDir + "/final.vtu");
+  vtu.drain();
   writeCheckpoint(solver, spec.run.runDir + "/final.ckpt");
   sum.steps = last.step;
   sum.time = last.t;

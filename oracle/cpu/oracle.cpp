@@ -11,6 +11,7 @@
 #include <map>
 #include <cmath>
 #include <cstring>
+#include <fstream>
 #include <memory>
 #include <stdexcept>
 #include <unordered_map>
@@ -1812,6 +1813,57 @@ void orc_ns_cell_velocity(void* h, int comp, double* out) {
     for (int j = 0; j < s.g.ny; ++j)
       for (int i = 0; i < s.g.nx; ++i)
         out[s.g.c(i, j, k)] = comp == 0 ? s.cU(i, j, k) : (comp == 1 ? s.cV(i, j, k) : s.cW(i, j, k));
+}
+
+// writeGridVtu (case.cpp:175-241): the ASCII snapshot as the reference prints
+// it, through an ostream with precision 8.  Returns 0, or -1 when the file
+// cannot be opened.
+int orc_ns_write_vtu(void* h, const char* path) {
+  NS& s = *static_cast<NS*>(h);
+  const Grid& g = s.g;
+  std::ofstream f(path);
+  if (!f) return -1;
+  f.precision(8);
+  const size_t ncell = g.cells(), npts = ncell * 8;
+  f << "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"1.0\" "
+       "byte_order=\"LittleEndian\" header_type=\"UInt64\">\n  <UnstructuredGrid>\n";
+  f << "    <Piece NumberOfPoints=\"" << npts << "\" NumberOfCells=\"" << ncell << "\">\n";
+  f << "      <Points>\n        <DataArray type=\"Float64\" NumberOfComponents=\"3\" format=\"ascii\">\n";
+  for (int k = 0; k < g.nz; ++k)
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i) {
+        const double x0 = g.org[0] + i * g.dx, x1 = x0 + g.dx;
+        const double y0 = g.org[1] + j * g.dy, y1 = y0 + g.dy;
+        const double z0 = g.org[2] + k * g.dz, z1 = z0 + g.dz;
+        const double q[24] = {x0, y0, z0, x1, y0, z0, x1, y1, z0, x0, y1, z0,
+                              x0, y0, z1, x1, y0, z1, x1, y1, z1, x0, y1, z1};
+        for (int e = 0; e < 24; ++e) f << q[e] << (e == 23 ? '\n' : ' ');
+      }
+  f << "        </DataArray>\n      </Points>\n      <Cells>\n";
+  f << "        <DataArray type=\"Int64\" Name=\"connectivity\" format=\"ascii\">\n";
+  for (size_t i = 0; i < npts; ++i) f << i << ((i + 1) % 8 == 0 ? '\n' : ' ');
+  f << "        </DataArray>\n        <DataArray type=\"Int64\" Name=\"offsets\" format=\"ascii\">\n";
+  for (size_t i = 1; i <= ncell; ++i) f << i * 8 << ' ';
+  f << "\n        </DataArray>\n        <DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n";
+  for (size_t i = 0; i < ncell; ++i) f << "12 ";
+  f << "\n        </DataArray>\n      </Cells>\n      <CellData Scalars=\"speed\">\n";
+  Vec cu(ncell), cv(ncell), cw(ncell), speed(ncell), solid(ncell);
+  orc_ns_cell_velocity(h, 0, cu.data());
+  orc_ns_cell_velocity(h, 1, cv.data());
+  orc_ns_cell_velocity(h, 2, cw.data());
+  for (size_t c = 0; c < ncell; ++c) {
+    speed[c] = std::sqrt(cu[c] * cu[c] + cv[c] * cv[c] + cw[c] * cw[c]);
+    solid[c] = g.solid[c] ? 1.0 : 0.0;
+  }
+  const std::pair<const char*, const Vec*> arrays[6] = {{"u", &cu},        {"v", &cv},         {"w", &cw},
+                                                        {"p", &s.p},       {"speed", &speed}, {"solid", &solid}};
+  for (const auto& [name, a] : arrays) {
+    f << "        <DataArray type=\"Float64\" Name=\"" << name << "\" format=\"ascii\">\n";
+    for (double x : *a) f << x << ' ';
+    f << "\n        </DataArray>\n";
+  }
+  f << "      </CellData>\n    </Piece>\n  </UnstructuredGrid>\n</VTKFile>\n";
+  return f ? 0 : -1;
 }
 
 void orc_apply_a(const nbl_grid_desc* d, const double* x, double* out) {

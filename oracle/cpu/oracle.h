@@ -84,6 +84,8 @@ int orc_oct_mg_levels(void* h, long long* counts, int cap);
 /* octreeTransportStep (adaptive/transport.cpp:24-60) on scalar phi (in/out),
  * cell-centred u, v, w; returns the integral sum phi_i vol_i. */
 double orc_oct_transport(void* h, const double* u, const double* v, const double* w, double* phi, double dt);
+// writeGridVtu (case.cpp:175-241) of the solver's state; 0 or -1 (cannot open).
+int orc_ns_write_vtu(void* h, const char* path);
 // PhysicalSensorEvaluator::evaluate (adaptive_core.cpp:73-176): out[8*n] field-major, flags[3*n].
 void orc_oct_sensors(void* h, const double* u, const double* v, const double* w, const double* p, double nu,
                      double* speedPrev, double* residPrev, double* out, uint8_t* flags);

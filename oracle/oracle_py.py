@@ -81,6 +81,7 @@ class OracleLib:
                 "mg_prolong": (None, G, P_D, P_D),
                 "oct_mg_solve": (None, V, P_D, P_D, d, i, PR),
                 "oct_mg_levels": (i, V, C.POINTER(C.c_longlong), i),
+                "ns_write_vtu": (i, V, C.c_char_p),
             })
         for name, (res, *args) in spec.items():
             f = getattr(self.lib, f"{kind}_{name}")
@@ -195,6 +196,11 @@ class NS:
         out = np.empty(self.grid.cells)
         self.o.ns_cell_velocity(self.h, comp, dptr(out))
         return out
+
+    def write_vtu(self, path):
+        """writeGridVtu of the current state (oracle library only)."""
+        if self.o.ns_write_vtu(self.h, str(path).encode()) != 0:
+            raise OSError(f"cannot write {path}")
 
     def close(self):
         if self.h:

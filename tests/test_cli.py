@@ -107,3 +107,62 @@ def test_bad_checkpoint_magic_exits_1(tmp_path):
     (tmp_path / "c.json").write_text(json.dumps(case))
     r = run("run", str(tmp_path / "c.json"), "--restart", str(bad))
     assert r.returncode == 1 and "bad magic" in r.stderr
+
+
+def read_appended_vtu(path):
+    """Arrays of an appended raw VTU, in file order (UInt64 headers)."""
+    b = Path(path).read_bytes()
+    head, _, rest = b.partition(b'<AppendedData encoding="raw">\n   _')
+    specs = []
+    for part in head.split(b"<DataArray ")[1:]:
+        attrs = dict(kv.split(b"=", 1) for kv in part.split(b"/>")[0].split())
+        specs.append((attrs[b"type"].strip(b'"').decode(), int(attrs[b"offset"].strip(b'"'))))
+    out = []
+    for typ, off in specs:
+        (n,) = struct.unpack_from("<Q", rest, off)
+        dt = {"Float64": np.float64, "Int64": np.int64, "UInt8": np.uint8}[typ]
+        out.append(np.frombuffer(rest, dt, n // np.dtype(dt).itemsize, off + 8).copy())
+    assert rest.endswith(b"\n  </AppendedData>\n</VTKFile>\n")
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fmt", ["ascii", "binary"])
+def test_vtu_snapshots_match_the_reference_writer(tmp_path, orc, fmt):
+    """Snapshots written on the run's writer thread: the ASCII files are byte
+    for byte what the reference's writeGridVtu (case.cpp:175-241, restated in
+    the oracle) prints for the same state (read back from the checkpoint of
+    the same step); the binary files hold the same arrays at full precision."""
+    case = {"type": "cube-channel", "resolution": 4, "reynolds": 500.0, "max_steps": 6, "snapshot_every": 3,
+            "checkpoint_every": 3, "vtu_format": fmt, "run_dir": str(tmp_path / "r")}
+    (tmp_path / "c.json").write_text(json.dumps(case))
+    r = run("run", str(tmp_path / "c.json"))
+    assert r.returncode == 0, r.stderr
+    g, cfg = cases.cube_channel(nx=48, ny=16, nz=16, res=4, reynolds=500.0)
+    for vtu, ckpt in (("snapshot_3.vtu", "checkpoint_3.ckpt"), ("snapshot_6.vtu", "checkpoint_6.ckpt"),
+                      ("final.vtu", "final.ckpt")):
+        hdr, (u, v, w, p) = read_ckpt(tmp_path / "r" / ckpt)
+        o = orc.ns(g, cfg)
+        o.set_state(u, v, w, p, hdr[3], hdr[4], hdr[5])
+        if fmt == "ascii":
+            o.write_vtu(tmp_path / "expected.vtu")
+            assert (tmp_path / "r" / vtu).read_bytes() == (tmp_path / "expected.vtu").read_bytes(), vtu
+        else:
+            arrs = read_appended_vtu(tmp_path / "r" / vtu)
+            n = g.cells
+            cu, cv, cw = (o.cell_velocity(k) for k in range(3))
+            i = np.arange(n) % g.nx
+            j = (np.arange(n) // g.nx) % g.ny
+            k = np.arange(n) // (g.nx * g.ny)
+            x0, y0, z0 = g.origin[0] + i * g.dx, g.origin[1] + j * g.dy, g.origin[2] + k * g.dz
+            x1, y1, z1 = x0 + g.dx, y0 + g.dy, z0 + g.dz
+            pts = np.stack([x0, y0, z0, x1, y0, z0, x1, y1, z0, x0, y1, z0,
+                            x0, y0, z1, x1, y0, z1, x1, y1, z1, x0, y1, z1], axis=1).reshape(-1)
+            assert np.array_equal(arrs[0], pts)
+            assert np.array_equal(arrs[1], np.arange(8 * n))
+            assert np.array_equal(arrs[2], 8 * np.arange(1, n + 1))
+            assert np.array_equal(arrs[3], np.full(n, 12, np.uint8))
+            speed = np.sqrt(cu * cu + cv * cv + cw * cw)
+            for got, want in zip(arrs[4:], (cu, cv, cw, p, speed, g.solid.astype(np.float64))):
+                assert np.array_equal(got, want)
+        o.close()
