@@ -5,9 +5,9 @@
 // which is what std::to_chars' general format with precision 8 produces; the
 // text is formatted in parallel chunks and written in one pass.  Binary: the
 // same arrays in VTK's appended raw encoding (UInt64 byte-count headers, as
-// the header_type attribute already declares), about a quarter of the size
-// and no formatting.  SnapshotWriter takes host copies of a step's cell fields
-// and writes them on a thread of its own while the run continues.
+// the header_type attribute already declares): full precision, no formatting,
+// about the ASCII file's size.  SnapshotWriter takes host copies of a step's
+// cell fields and writes them on a thread of its own while the run continues.
 #include <algorithm>
 #include <atomic>
 #include <charconv>
