@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <memory>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -27,6 +28,13 @@ struct RunOptions {
   std::string restartFrom;
   bool quiet = false;
   VtuFormat vtuFormat = VtuFormat::Ascii;
+  // Set to non-zero by another thread to stop the run between steps (the
+  // in-process job runner's cancel); runCase then throws RunCancelled.
+  const volatile int* cancel = nullptr;
+};
+
+struct RunCancelled : std::runtime_error {
+  explicit RunCancelled(int step) : std::runtime_error("cancelled after step " + std::to_string(step)) {}
 };
 
 struct CaseSpec {

@@ -378,6 +378,10 @@ RunSummary runCase(const CaseSpec& spec) {
   // steps go on; the host copy of the fields is taken at the step itself
   SnapshotWriter vtu(spec.run.vtuFormat);
   for (int s = 0; s < spec.run.maxSteps; ++s) {
+    if (spec.run.cancel && *spec.run.cancel) {
+      vtu.drain();
+      throw RunCancelled(last.step);
+    }
     last = solver.step();
     appendDiagnostics(diag, last, last.accepted);
     diag.flush();
