@@ -25,14 +25,21 @@
 namespace nbl {
 
 Geo3 geo3n(int nx, int ny, int nz, int cells) {
-  const int tx = nx / cells;
-  // widest row of threads (64..16) that divides the row exactly, else the widest
-  int bx = tx >= 64 ? 64 : tx >= 32 ? 32 : tx >= 16 ? 16 : 8;
-  for (int w : {64, 32, 16})
-    if (w <= tx && tx % w == 0) {
+  const int tx = (nx + cells - 1) / cells;
+  // the row width (64, 32, 16 or 8 threads; 256/width rows of j) that leaves
+  // the fewest idle threads in the padded tx x ny rectangle, the widest of
+  // equals (a width that divides the row exactly wastes nothing along i)
+  int bx = 8;
+  double best = -1.0;
+  for (int w : {64, 32, 16, 8}) {
+    if (w > 8 && w > 2 * tx) continue;
+    const int h = kBlock / w;
+    const double eff = double(tx) / double((tx + w - 1) / w * w) * double(ny) / double((ny + h - 1) / h * h);
+    if (eff > best + 1e-12) {
+      best = eff;
       bx = w;
-      break;
     }
+  }
   const int by = kBlock / bx;
   Geo3 g;
   g.block = dim3(bx, by, 1);
