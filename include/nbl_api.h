@@ -349,7 +349,7 @@ int nbl_octree_mg_levels(nbl_octree* oc, long long* counts, int cap, int* levels
  * and a coefficient per coupling; kernel 1: the transport step, per leaf its
  * level, face counts, first-neighbour slots, velocities and phi in and out,
  * plus an index per further neighbour; kernel 2: the sensors, per leaf its
- * key, level, mask, face counts, first-neighbour slots, u, v, w, p, the
+ * centre, level, mask, face counts, first-neighbour slots, u, v, w, p, the
  * carried state, the permutation, ten results and three flags, plus an index
  * per further neighbour), and the device time of construction (sort,
  * validation, face table, operator) and of the last solve, transport or

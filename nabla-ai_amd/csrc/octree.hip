@@ -586,7 +586,16 @@ struct FaceWalk {
   const int32_t* __restrict__ xnb;
 };
 
-__global__ void __launch_bounds__(kBlock) k_oct_sensors(long long n, Geo G, const uint64_t* __restrict__ keys,
+// Leaf centres (octree.cpp:84-91) per axis, cen[a*n + i], for the sensors'
+// neighbour walks.
+__global__ void k_oct_centers(long long n, Geo G, const uint64_t* __restrict__ keys, const uint8_t* __restrict__ lev,
+                              double* __restrict__ cen) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int a = 0; a < 3; ++a) cen[a * n + i] = centerAxis(G, keys[i], lev[i], a);
+}
+
+__global__ void __launch_bounds__(kBlock) k_oct_sensors(long long n, Geo G, const double* __restrict__ cen,
                                                         const uint8_t* __restrict__ lev,
                                                         const uint8_t* __restrict__ mask, FaceWalk F,
                                                         const double* __restrict__ U, const double* __restrict__ V,
@@ -605,27 +614,39 @@ __global__ void __launch_bounds__(kBlock) k_oct_sensors(long long n, Geo G, cons
     x += cnt[d] > 1 ? cnt[d] - 1 : 0;
   }
   auto nbAt = [&](int d, int m) { return m == 0 ? F.first[(long long)d * n + i] : F.xnb[off[d] + m - 1]; };
-  const uint64_t ki = keys[i];
   const int li = lev[i];
   const double self[4] = {U[i], V[i], W[i], P[i]};
   double g[4][3];  // d(field)/d(axis) for u, v, w, p
+  // One walk over the faces in direction order (-x, +x, -y, +y, -z, +z): the
+  // gradient sums per side, and on the way the neighbours' speeds for the
+  // truncation estimate (summed in the reference's direction order) and the
+  // InsideSolid test of the near-wall flag.
+  bool wall = cnt[2] == 0 || cnt[3] == 0;  // no leaf across a y face (floor, ceiling)
+  double msum = 0.0;
+  int mc = 0;
   for (int a = 0; a < 3; ++a) {
-    const double ci = centerAxis(G, ki, li, a);
-    double val[2][4], pos[2] = {ci, ci};
+    const double* __restrict__ ca = cen + a * n;
+    const double ci = ca[i];
+    double val[2][4], pos[2] = {ci, ci};  // [0]: + face (2a+1), [1]: - face (2a)
     bool has[2] = {false, false};
-    for (int s = 0; s < 2; ++s) {  // s = 0: + face (2a+1), s = 1: - face (2a)
+    for (int t = 0; t < 2; ++t) {
+      const int s = 1 - t;  // - face first
+      const int d = 2 * a + t;
       for (int f = 0; f < 4; ++f) val[s][f] = self[f];
-      const int d = s == 0 ? 2 * a + 1 : 2 * a;
       const int c = cnt[d];
       if (c == 0) continue;
       double sv[4] = {0.0, 0.0, 0.0, 0.0}, sp = 0.0;
       for (int m = 0; m < c; ++m) {
         const int nb = nbAt(d, m);
-        sv[0] += U[nb];
-        sv[1] += V[nb];
-        sv[2] += W[nb];
+        const double un = U[nb], vn = V[nb], wn = W[nb];
+        sv[0] += un;
+        sv[1] += vn;
+        sv[2] += wn;
         sv[3] += P[nb];
-        sp += centerAxis(G, keys[nb], lev[nb], a);
+        sp += ca[nb];
+        if (mask[nb] == 2) wall = true;
+        msum += sqrt(un * un + vn * vn + wn * wn);
+        ++mc;
       }
       const double inv = 1.0 / double(c);
       for (int f = 0; f < 4; ++f) val[s][f] = sv[f] * inv;
@@ -653,17 +674,6 @@ __global__ void __launch_bounds__(kBlock) k_oct_sensors(long long n, Geo G, cons
   const double gu = sqrt(du[0] * du[0] + du[1] * du[1] + du[2] * du[2] + dv[0] * dv[0] + dv[1] * dv[1] +
                          dv[2] * dv[2] + dw[0] * dw[0] + dw[1] * dw[1] + dw[2] * dw[2]);
   const double gp = sqrt(dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2]);
-  // near a wall: no leaf across a y face (floor, ceiling) or an InsideSolid neighbour
-  bool wall = cnt[2] == 0 || cnt[3] == 0;
-  double msum = 0.0;  // neighbours' speeds for the truncation estimate, in order
-  int mc = 0;
-  for (int d = 0; d < 6; ++d)
-    for (int m = 0; m < cnt[d]; ++m) {
-      const int nb = nbAt(d, m);
-      if (mask[nb] == 2) wall = true;
-      msum += sqrt(U[nb] * U[nb] + V[nb] * V[nb] + W[nb] * W[nb]);
-      ++mc;
-    }
   const double speed = sqrt(self[0] * self[0] + self[1] * self[1] + self[2] * self[2]);
   double yplus = -1.0, tau = 0.0;
   if (wall) {
@@ -987,6 +997,24 @@ double OctreeDev::transport(const double* u, const double* v, const double* w, d
   return hctl_[0].slot[O_INTEGRAL];
 }
 
+// Leaf centres for the sensors, computed on first use.
+const double* OctreeDev::sensCen(hipStream_t s) {
+  if (!cen_.bytes()) {
+    const long long N = (long long)L0_.n;
+    Geo G;
+    for (int a = 0; a < 3; ++a) {
+      G.org[a] = org_[a];
+      G.fine[a] = fine_[a];
+    }
+    G.maxL = maxL_;
+    cen_ = DevBuf(ctx_, 3 * L0_.n * sizeof(double));
+    k_oct_centers<<<gridFor(N), kBlock, 0, s>>>(N, G, L0_.keys.as<uint64_t>(), L0_.lev.as<uint8_t>(),
+                                                cen_.as<double>());
+    NBL_LAUNCH_CHECK();
+  }
+  return cen_.as<double>();
+}
+
 void OctreeDev::sensors(const double* u, const double* v, const double* w, const double* p, double nu,
                         double* speedPrev, double* residPrev, double* out, uint8_t* flags) {
   const size_t n = L0_.n;
@@ -1010,7 +1038,7 @@ void OctreeDev::sensors(const double* u, const double* v, const double* w, const
   G.maxL = maxL_;
   const FaceWalk F{cnt_.as<uint8_t>(), tfirst_.as<int32_t>(), txrow_.as<int32_t>(), txnb_.as<int32_t>()};
   NBL_HIP(hipEventRecord(e0_, s));
-  k_oct_sensors<<<gridFor(N), kBlock, 0, s>>>(N, G, L0_.keys.as<uint64_t>(), L0_.lev.as<uint8_t>(),
+  k_oct_sensors<<<gridFor(N), kBlock, 0, s>>>(N, G, sensCen(s), L0_.lev.as<uint8_t>(),
                                               L0_.mask.as<uint8_t>(), F, r_.as<double>(), z_.as<double>(),
                                               p_.as<double>(), x_.as<double>(), rhs_.as<double>(), Ap_.as<double>(),
                                               nu, perm_.as<uint64_t>(), sensOut_.as<double>(),
@@ -1047,9 +1075,10 @@ void OctreeDev::profile(int kernel, int reps, double* ms, double* bytes) {
     }
     G.maxL = maxL_;
     const FaceWalk F{cnt_.as<uint8_t>(), tfirst_.as<int32_t>(), txrow_.as<int32_t>(), txnb_.as<int32_t>()};
+    const double* cen = sensCen(stream_);
     auto one = [&] {
       k_oct_sensors<<<gridFor(N), kBlock, 0, stream_>>>(
-          N, G, L0_.keys.as<uint64_t>(), L0_.lev.as<uint8_t>(), L0_.mask.as<uint8_t>(), F, r_.as<double>(),
+          N, G, cen, L0_.lev.as<uint8_t>(), L0_.mask.as<uint8_t>(), F, r_.as<double>(),
           z_.as<double>(), p_.as<double>(), x_.as<double>(), rhs_.as<double>(), Ap_.as<double>(), 1e-3,
           perm_.as<uint64_t>(), sensOut_.as<double>(), sensFlags_.as<uint8_t>());
     };
@@ -1062,11 +1091,12 @@ void OctreeDev::profile(int kernel, int reps, double* ms, double* bytes) {
     float t = 0.f;
     NBL_HIP(hipEventElapsedTime(&t, e0_, e1_));
     *ms = double(t) / reps;
-    // compulsory traffic (a neighbour's key, level, mask and fields are another
-    // leaf's own): per leaf its key, level, mask, six counts, six first-neighbour
-    // slots, u, v, w, p and the carried speed and residual, the permutation,
-    // ten results and three flags; per further neighbour its index
-    *bytes = double(n) * (8.0 + 1.0 + 1.0 + 6.0 + 24.0 + 4.0 * 8.0 + 2.0 * 8.0 + 8.0 + 10.0 * 8.0 + 3.0) +
+    // compulsory traffic (a neighbour's centre, mask and fields are another
+    // leaf's own): per leaf its three centre coordinates, level, mask, six
+    // counts, six first-neighbour slots, u, v, w, p and the carried speed and
+    // residual, the permutation, ten results and three flags; per further
+    // neighbour its index
+    *bytes = double(n) * (24.0 + 1.0 + 1.0 + 6.0 + 24.0 + 4.0 * 8.0 + 2.0 * 8.0 + 8.0 + 10.0 * 8.0 + 3.0) +
              double(tnx_) * 4.0;
     return;
   }

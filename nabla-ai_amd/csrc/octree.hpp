@@ -157,6 +157,8 @@ class OctreeDev {
   DevBuf perm_, cnt_, nbr_;
   DevBuf tfirst_, txrow_, txnb_;  // transport's face neighbours (first per direction slot-major, overflow)
   DevBuf sensOut_, sensFlags_;    // sensor results in caller order (allocated on first use)
+  DevBuf cen_;                     // leaf centres per axis (sensors; computed on first use)
+  const double* sensCen(hipStream_t s);
   size_t tnx_ = 0;
   DevBuf x_, rhs_, r_, z_, p_, Ap_, t0_, t1_;
   DevBuf ctl_, red_;
