@@ -117,6 +117,10 @@ typedef struct nbl_solver_opts {
                                its CgPoisson(1e-6, 500) coarse solve (FP64
                                only): the unfused baseline the fused path is
                                measured against; 0 (default): fused         */
+  int rhs_planes;           /* WENO-5 right-hand side: planes along z each
+                               64 x 4 face brick marches through, 0
+                               automatic or 1..16; results are bit-identical
+                               across choices                              */
 } nbl_solver_opts;
 
 /* StepReport (ns_solver.hpp:34-49). */

@@ -161,6 +161,21 @@ struct PadBufs {
 };
 PadBufs padLayout(const DevGrid& g);  // sizes and strides; the caller sets p[]
 
+// Brick tiling of the WENO-5 right-hand side (ns.hip): per component the box
+// of faces that can be active (origin, extents), its 64 x 4 x planes bricks,
+// and the faces outside the boxes (right-hand side zero) per component.
+constexpr int kBrickX = 64, kBrickY = 4;
+struct BrickComp {
+  int ox, oy, oz, ex, ey, ez;
+  int tx, ty, tz, blocks;
+};
+struct BrickArgs {
+  BrickComp c[3];
+  int planes, tileBlocks;
+  long long edge[3];
+};
+BrickArgs brickArgs(const DevGrid& g, int planes);
+
 void launch_bc_planes(const NsArgs& a, double* u, double* v, double* w, bool outflow, hipStream_t s);
 void launch_bc_solid(const NsArgs& a, double* u, double* v, double* w, const int* list,
                      const int counts[3], hipStream_t s);
@@ -168,7 +183,7 @@ void launch_bc_solid(const NsArgs& a, double* u, double* v, double* w, const int
 // writes R itself, stages 1-3 the SSP-RK3 combinations with u0.
 void launch_rhs_rk(const NsArgs& a, const DevGrid* gd, const PadBufs& pb, const double* u0, const double* v0,
                    const double* w0, const double* u, const double* v, const double* w, double* uo, double* vo,
-                   double* wo, int stage, const StepCtl* sc, hipStream_t s);
+                   double* wo, int stage, const StepCtl* sc, hipStream_t s, int planes = 0);
 void launch_divergence(const NsArgs& a, const double* u, const double* v, const double* w,
                        double* b, hipStream_t s);
 void launch_correct(const NsArgs& a, double* u, double* v, double* w, const double* phi,

@@ -8,6 +8,9 @@
 // order, and the library is built with -ffp-contract=off, so per-element
 // results are bit-identical to the reference; only reductions (CFL max,
 // diagnostics sums) use a different (deterministic, tree) order.
+#include <algorithm>
+#include <stdexcept>
+
 #include "dev.hpp"
 #include "nbl_api.h"
 
@@ -392,6 +395,16 @@ __global__ void __launch_bounds__(kBlock) k_pad_halo(const DevGrid* __restrict__
   }
 }
 
+// SSP-RK3 stage combine of one face (ns_solver.cpp:776-794): stage 0 returns
+// the right-hand side itself.
+__device__ __forceinline__ double rkCombine(int stage, double dt, const double* __restrict__ a0, int t, double xc,
+                                           double R) {
+  if (stage == 0) return R;
+  if (stage == 1) return a0[t] + dt * R;
+  if (stage == 2) return 0.75 * a0[t] + 0.25 * (xc + dt * R);
+  return (1.0 / 3.0) * a0[t] + (2.0 / 3.0) * (xc + dt * R);
+}
+
 // RHS + SSP-RK3 stage combine (ns_solver.cpp:776-794), one thread per face,
 // block z indexing the planes of u, then v, then w.  Stencils read the padded
 // copies; u0 and the output are the compact arrays.
@@ -433,13 +446,131 @@ __global__ void __launch_bounds__(kBlock) k_rhs_rk(NsArgs A, PadBufs pb, const d
   }
   const double* a0 = comp == 0 ? u0 : (comp == 1 ? v0 : w0);
   double* ao = comp == 0 ? uo : (comp == 1 ? vo : wo);
+  ao[t] = rkCombine(stage, ldA(&sc->dt), a0, t, xc, R);
+}
+
+// ---------------------------------------------------------------------------
+// WENO-5 right-hand side in bricks.  The convective term of a face is a
+// difference of two flux products per axis, adv * recon(adv, 6 values), and
+// the upper product of face p along axis d is, operand for operand, the lower
+// product of face p + e_d (ns_solver.cpp:323-472 evaluate both at every face).
+// A brick is 64 faces along i by 4 rows along j of one component, marching
+// through `planes` consecutive k: each face evaluates its lower x and y
+// products into LDS and its upper z product in a register (the next plane's
+// lower one); one wave adds the fifth row's y products and another the 65th
+// column's x products.  That is 3.5 + 1/planes reconstructions per face where
+// the per-face form evaluates 6; the values and the expression that combines
+// them are the per-face form's, so results are bit-identical.
+//
+// The brick covers the faces the reference can make active (u: 1 <= i < nx,
+// v: 1 <= j < ny, w: 1 <= k < nz); the rest, whose right-hand side is zero,
+// are written by trailing blocks.  Every stencil stays inside the kPadHalo
+// ghost layers with at most one index out of range, as the per-face form's.
+// ---------------------------------------------------------------------------
+// Lower flux product along an axis at face element pF of the advected
+// component F: adv = (A[pA - ecA] + A[pA]) / 2 from the advecting component,
+// reconstructed upwind from the six F values around the face.
+__device__ __forceinline__ double weno5Flux(const double* __restrict__ F, int pF, int sF,
+                                            const double* __restrict__ Av, int pA, int ecA) {
+  const double adv = 0.5 * (Av[pA - ecA] + Av[pA]);
+  const double q = adv >= 0.0 ? weno5(F[pF - 3 * sF], F[pF - 2 * sF], F[pF - sF], F[pF], F[pF + sF])
+                              : weno5(F[pF + 2 * sF], F[pF + sF], F[pF], F[pF - sF], F[pF - 2 * sF]);
+  return adv * q;
+}
+
+__global__ void __launch_bounds__(kBrickX * kBrickY)
+    k_rhs_brick(NsArgs A, PadBufs pb, BrickArgs BA, const double* __restrict__ u0, const double* __restrict__ v0,
+                const double* __restrict__ w0, double* __restrict__ uo, double* __restrict__ vo,
+                double* __restrict__ wo, int stage, const StepCtl* __restrict__ sc) {
+  __shared__ double lx[2][kBrickY][kBrickX + 1];
+  __shared__ double ly[2][kBrickY + 1][kBrickX];
+  const DevGrid& g = A.g;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const bool flat = nz == 1;
+  const int b = xcdTile(int(blockIdx.x), int(gridDim.x));
   const double dt = ldA(&sc->dt);
-  double out;
-  if (stage == 0) out = R;
-  else if (stage == 1) out = a0[t] + dt * R;
-  else if (stage == 2) out = 0.75 * a0[t] + 0.25 * (xc + dt * R);
-  else out = (1.0 / 3.0) * a0[t] + (2.0 / 3.0) * (xc + dt * R);
-  ao[t] = out;
+  if (b >= BA.tileBlocks) {
+    // faces with a zero right-hand side: u at i = 0, nx; v at j = 0, ny; w at k = 0, nz
+    long long q = (long long)(b - BA.tileBlocks) * (kBrickX * kBrickY) + threadIdx.x + kBrickX * threadIdx.y;
+    int c = 0;
+    while (c < 3 && q >= BA.edge[c]) q -= BA.edge[c++];
+    if (c == 3) return;
+    const int s = int(q & 1);
+    const long long h = q >> 1;
+    int i, j, k;
+    if (c == 0) { i = s ? nx : 0; j = int(h % ny); k = int(h / ny); }
+    else if (c == 1) { j = s ? ny : 0; i = int(h % nx); k = int(h / nx); }
+    else { k = s ? nz : 0; i = int(h % nx); j = int(h / nx); }
+    const int X = nx + (c == 0), Y = ny + (c == 1);
+    const int t = i + X * (j + Y * k);
+    const double xc = pb.p[c][pb.origin[c] + i + pb.sx[c] * j + pb.sxy[c] * k];
+    const double* a0 = c == 0 ? u0 : (c == 1 ? v0 : w0);
+    double* ao = c == 0 ? uo : (c == 1 ? vo : wo);
+    ao[t] = rkCombine(stage, dt, a0, t, xc, 0.0);
+    return;
+  }
+  int c = 0, tb = b;
+  while (tb >= BA.c[c].blocks) tb -= BA.c[c++].blocks;
+  const BrickComp& C = BA.c[c];
+  const int bi = tb % C.tx, bj = (tb / C.tx) % C.ty, bk = tb / (C.tx * C.ty);
+  const int l = threadIdx.x, r = threadIdx.y;
+  const int i0 = C.ox + bi * kBrickX, j0 = C.oy + bj * kBrickY, k0 = C.oz + bk * BA.planes;
+  const int xe = C.ox + C.ex, ye = C.oy + C.ey, ke = min(k0 + BA.planes, C.oz + C.ez);
+  const int i = i0 + l, j = j0 + r;
+  const bool inX = i < xe, inY = j < ye, in = inX && inY;
+
+  // F: the advected component; Ad: the component advecting along axis d, with
+  // ec[d] its stride along this component's axis (the face-average offset)
+  const double* F = pb.p[c] + pb.origin[c];
+  const int fsx = pb.sx[c], fsxy = pb.sxy[c];
+  const double* Ax = pb.p[0] + pb.origin[0];
+  const double* Ay = pb.p[1] + pb.origin[1];
+  const double* Az = pb.p[2] + pb.origin[2];
+  const int ecx = c == 0 ? 1 : (c == 1 ? pb.sx[0] : pb.sxy[0]);
+  const int ecy = c == 0 ? 1 : (c == 1 ? pb.sx[1] : pb.sxy[1]);
+  const int ecz = c == 0 ? 1 : (c == 1 ? pb.sx[2] : pb.sxy[2]);
+  auto fI = [&](int ii, int jj, int kk) { return ii + fsx * jj + fsxy * kk; };
+  auto xI = [&](int ii, int jj, int kk) { return ii + pb.sx[0] * jj + pb.sxy[0] * kk; };
+  auto yI = [&](int ii, int jj, int kk) { return ii + pb.sx[1] * jj + pb.sxy[1] * kk; };
+  auto zI = [&](int ii, int jj, int kk) { return ii + pb.sx[2] * jj + pb.sxy[2] * kk; };
+  auto fluxX = [&](int ii, int jj, int kk) { return weno5Flux(F, fI(ii, jj, kk), 1, Ax, xI(ii, jj, kk), ecx); };
+  auto fluxY = [&](int ii, int jj, int kk) { return weno5Flux(F, fI(ii, jj, kk), fsx, Ay, yI(ii, jj, kk), ecy); };
+  auto fluxZ = [&](int ii, int jj, int kk) { return weno5Flux(F, fI(ii, jj, kk), fsxy, Az, zI(ii, jj, kk), ecz); };
+
+  const int X = nx + (c == 0), Y = ny + (c == 1);
+  const double* a0 = c == 0 ? u0 : (c == 1 ? v0 : w0);
+  double* ao = c == 0 ? uo : (c == 1 ? vo : wo);
+  const double idx2 = g.dx * g.dx, idy2 = g.dy * g.dy, idz2 = g.dz * g.dz;
+  double pzLo = 0.0;
+  if (!flat && in) pzLo = fluxZ(i, j, k0);
+  for (int k = k0; k < ke; ++k) {
+    const int buf = (k - k0) & 1;
+    if (i <= xe && inY) lx[buf][r][l] = fluxX(i, j, k);
+    if (inX && j <= ye) ly[buf][r][l] = fluxY(i, j, k);
+    if (r == 0 && inX && j0 + kBrickY <= ye) ly[buf][kBrickY][l] = fluxY(i, j0 + kBrickY, k);
+    if (r == 1 && l < kBrickY && i0 + kBrickX <= xe && j0 + l < ye)
+      lx[buf][l][kBrickX] = fluxX(i0 + kBrickX, j0 + l, k);
+    double pzHi = 0.0;
+    if (!flat && in) pzHi = fluxZ(i, j, k + 1);
+    __syncthreads();
+    if (in) {
+      const int p = fI(i, j, k);
+      const double xc = F[p];
+      double R = 0.0;
+      const bool act = !solidAt(g, i - (c == 0), j - (c == 1), k - (c == 2)) && !solidAt(g, i, j, k);
+      if (act) {
+        const double fx = (lx[buf][r][l + 1] - lx[buf][r][l]) / g.dx;
+        const double fy = (ly[buf][r + 1][l] - ly[buf][r][l]) / g.dy;
+        const double fz = flat ? 0.0 : (pzHi - pzLo) / g.dz;
+        double lap = (F[p + 1] - 2 * xc + F[p - 1]) / idx2 + (F[p + fsx] - 2 * xc + F[p - fsx]) / idy2;
+        if (!flat) lap += (F[p + fsxy] - 2 * xc + F[p - fsxy]) / idz2;
+        R = -(fx + fy + fz) + A.nu * lap + A.bodyForce[c];
+      }
+      const int t = i + X * (j + Y * k);
+      ao[t] = rkCombine(stage, dt, a0, t, xc, R);
+    }
+    pzLo = pzHi;
+  }
 }
 
 // Boundary planes of applyVelocityBC (ns_solver.cpp:224-276).
@@ -855,7 +986,7 @@ static long long haloCount(const DevGrid& g) {
 
 void launch_rhs_rk(const NsArgs& a, const DevGrid* gd, const PadBufs& pb, const double* u0, const double* v0,
                    const double* w0, const double* u, const double* v, const double* w, double* uo, double* vo,
-                   double* wo, int stage, const StepCtl* sc, hipStream_t s) {
+                   double* wo, int stage, const StepCtl* sc, hipStream_t s, int planes) {
   const DevGrid& g = a.g;
   for (int c = 0; c < 3; ++c)
     if (pb.n[c] >= (1LL << 31)) throw std::invalid_argument("grid too large for 32-bit face indexing");
@@ -864,11 +995,59 @@ void launch_rhs_rk(const NsArgs& a, const DevGrid* gd, const PadBufs& pb, const 
   const dim3 grid = geo.grid;
   k_pad_copy<<<grid, geo.block, 0, s>>>(a, u, v, w, pb);
   k_pad_halo<<<gridFor(haloCount(g)), kBlock, 0, s>>>(gd, u, v, w, pb);
-  if (a.convection == NBL_CONV_CENTRAL)
+  if (a.convection == NBL_CONV_CENTRAL) {
     k_rhs_rk<NBL_CONV_CENTRAL><<<grid, geo.block, 0, s>>>(a, pb, u0, v0, w0, uo, vo, wo, stage, sc);
-  else
-    k_rhs_rk<NBL_CONV_WENO5><<<grid, geo.block, 0, s>>>(a, pb, u0, v0, w0, uo, vo, wo, stage, sc);
+  } else {
+    const BrickArgs ba = brickArgs(g, planes);
+    const long long edges = ba.edge[0] + ba.edge[1] + ba.edge[2];
+    const long long blocks = ba.tileBlocks + (edges + kBrickX * kBrickY - 1) / (kBrickX * kBrickY);
+    if (blocks >= (1LL << 31)) throw std::invalid_argument("grid too large for the right-hand side bricks");
+    k_rhs_brick<<<dim3(unsigned(blocks)), dim3(kBrickX, kBrickY), 0, s>>>(a, pb, ba, u0, v0, w0, uo, vo, wo,
+                                                                         stage, sc);
+  }
   NBL_LAUNCH_CHECK();
+}
+
+// Brick decomposition of the WENO-5 right-hand side: per component the box of
+// faces that can be active, tiled 64 x 4 x planes.  `planes` <= 0 picks the
+// longest march of 4, 2 or 1 planes that still gives >= 4096 bricks (16 per
+// CU): on the 192 x 64 x 64 channel, bricks of 2 planes (4608) ran the kernel
+// in 90 us, of 4 in 93 us, of 8 in 117 us and of 16 in 150 us, so enough
+// resident waves to hide the stencil loads count for more than the z fluxes
+// a longer march saves.
+BrickArgs brickArgs(const DevGrid& g, int planes) {
+  if (planes > 16) throw std::invalid_argument("rhs_planes must be 0 (automatic) or 1..16");
+  BrickArgs a{};
+  auto tile = [&](int p) {
+    a.planes = p;
+    a.tileBlocks = 0;
+    for (int c = 0; c < 3; ++c) {
+      BrickComp& C = a.c[c];
+      C.ox = c == 0;
+      C.oy = c == 1;
+      C.oz = c == 2;
+      C.ex = g.nx - (c == 0);
+      C.ey = g.ny - (c == 1);
+      C.ez = g.nz - (c == 2);
+      C.tx = std::max(1, (C.ex + kBrickX - 1) / kBrickX);
+      C.ty = std::max(1, (C.ey + kBrickY - 1) / kBrickY);
+      C.tz = std::max(1, (C.ez + p - 1) / p);
+      C.blocks = (C.ex > 0 && C.ey > 0 && C.ez > 0) ? C.tx * C.ty * C.tz : 0;
+      a.tileBlocks += C.blocks;
+    }
+  };
+  if (planes > 0) {
+    tile(planes);
+  } else {
+    for (int p : {4, 2, 1}) {
+      tile(p);
+      if (a.tileBlocks >= 4096) break;
+    }
+  }
+  a.edge[0] = 2LL * g.ny * g.nz;
+  a.edge[1] = 2LL * g.nx * g.nz;
+  a.edge[2] = 2LL * g.nx * g.ny;
+  return a;
 }
 
 void launch_divergence(const NsArgs& a, const double* u, const double* v, const double* w,

@@ -958,6 +958,9 @@ RunDev::RunDev(nbl_ctx* ctx, const HostGrid& g, const nbl_flow_cfg& cfg, const n
   NBL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hsc_), sizeof(StepCtl), hipHostMallocDefault));
   NBL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hrep_), sizeof(DevReport), hipHostMallocDefault));
   std::memset(hsc_, 0, sizeof(StepCtl));
+  if (opts.rhs_planes < 0 || opts.rhs_planes > 16)
+    throw std::invalid_argument("rhs_planes must be 0 (automatic) or 1..16");
+  rhsPlanes_ = opts.rhs_planes;
   nbl_solver_opts o = opts;
   poisson_ = std::make_unique<PoissonDev>(ctx, g, 0, cfg.poisson_tol, cfg.poisson_max_iters, o);
 }
@@ -1046,7 +1049,7 @@ nbl_step_report RunDev::step(double dt) {  // ns_solver.cpp:765-824
   nbl_poisson_report pr[3];
   for (int stage = 1; stage <= 3; ++stage) {
     launch_rhs_rk(args_, gridDev_.as<DevGrid>(), pad_, u0[0], u0[1], u0[2], cur_[0], cur_[1], cur_[2], alt_[0],
-                  alt_[1], alt_[2], stage, sc, stream_);
+                  alt_[1], alt_[2], stage, sc, stream_, rhsPlanes_);
     std::swap(cur_, alt_);
     project(stage, pr[stage - 1]);
   }
@@ -1082,7 +1085,7 @@ void RunDev::rhs(double* ru, double* rv, double* rw) {
     launch_bc_solid(args_, alt_[0], alt_[1], alt_[2], solidList_.as<int>(), solidCounts_, stream_);
   double* out[3] = {buf0_[0].as<double>(), buf0_[1].as<double>(), buf0_[2].as<double>()};
   launch_rhs_rk(args_, gridDev_.as<DevGrid>(), pad_, alt_[0], alt_[1], alt_[2], alt_[0], alt_[1], alt_[2], out[0],
-                out[1], out[2], 0, sc_.as<StepCtl>(), stream_);
+                out[1], out[2], 0, sc_.as<StepCtl>(), stream_, rhsPlanes_);
   double* dst[3] = {ru, rv, rw};
   for (int q = 0; q < 3; ++q) NBL_HIP(hipMemcpyAsync(dst[q], out[q], sz_[q] * 8, hipMemcpyDeviceToHost, stream_));
   NBL_HIP(hipStreamSynchronize(stream_));

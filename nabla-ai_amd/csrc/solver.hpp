@@ -220,6 +220,7 @@ class RunDev {
   nbl_ctx* ctx_;
   HostGrid g_;
   nbl_flow_cfg cfg_;
+  int rhsPlanes_ = 0;  // WENO-5 brick march length (0 automatic)
   NsArgs args_{};
   hipStream_t stream_ = nullptr;
   size_t sz_[4];
