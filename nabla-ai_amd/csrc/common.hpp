@@ -177,7 +177,7 @@ __device__ __forceinline__ int xcdTile(int b, int B) {
 struct Blk3 {
   int x, y, z, lin;
 };
-__device__ __forceinline__ Blk3 blk3() {
+__device__ __forceinline__ Blk3 blk3Tiled() {
   const int B = int(gridDim.x * gridDim.y * gridDim.z);
   const int b = int(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
   const int t = xcdTile(b, B);
@@ -187,6 +187,27 @@ __device__ __forceinline__ Blk3 blk3() {
   const int u = t / int(gridDim.x);
   r.y = u % int(gridDim.y);
   r.z = u / int(gridDim.y);
+  return r;
+}
+
+// Block coordinates of a cell-kernel launch without integer division.  Plain
+// order (slab = 0): the launch coordinates.  XCD slabs (slab = 1): the launch
+// is 8 gx x gy x ceil(gz / 8) blocks and block x' runs tile x' >> 3 of plane
+// (x' & 7) * gridDim.z + z'; workgroups go to the XCDs round-robin by linear
+// id, which is x' mod 8 here, so XCD s receives planes [s * gridDim.z,
+// (s + 1) * gridDim.z) in order and the k +- 1 planes a stencil shares stay in
+// its L2.  Planes past the grid's last fall outside it like any padding.
+__device__ __forceinline__ Blk3 blk3(int slab) {
+  Blk3 r;
+  r.lin = int(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+  r.y = int(blockIdx.y);
+  if (slab) {
+    r.x = int(blockIdx.x >> 3);
+    r.z = int(blockIdx.x & 7u) * int(gridDim.z) + int(blockIdx.z);
+  } else {
+    r.x = int(blockIdx.x);
+    r.z = int(blockIdx.z);
+  }
   return r;
 }
 

@@ -24,7 +24,7 @@
 
 namespace nbl {
 
-Geo3 geo3n(int nx, int ny, int nz, int cells) {
+Geo3 geo3n(int nx, int ny, int nz, int cells, int slab) {
   const int tx = (nx + cells - 1) / cells;
   // the row width (64, 32, 16 or 8 threads; 256/width rows of j) that leaves
   // the fewest idle threads in the padded tx x ny rectangle, the widest of
@@ -43,12 +43,14 @@ Geo3 geo3n(int nx, int ny, int nz, int cells) {
   const int by = kBlock / bx;
   Geo3 g;
   g.block = dim3(bx, by, 1);
-  g.grid = dim3((tx + bx - 1) / bx, (ny + by - 1) / by, nz);
+  const unsigned gx = (tx + bx - 1) / bx, gy = (ny + by - 1) / by;
+  g.grid = slab ? dim3(8 * gx, gy, (nz + 7) / 8) : dim3(gx, gy, nz);
   g.blocks = int(g.grid.x * g.grid.y * g.grid.z);
   return g;
 }
 Geo3 geo3(int nx, int ny, int nz) { return geo3n(nx, ny, nz, 1); }
-Geo3 levelGeo(const MgLevel& L) { return geo3n(L.nx, L.ny, L.nz, L.vec); }
+Geo3 levelGeo(const MgLevel& L) { return geo3n(L.nx, L.ny, L.nz, L.vec, L.slab); }
+int slabFor(int nx, int ny, int nz) { return nz >= 8 && (long long)nx * ny * nz < (8LL << 20) ? 1 : 0; }
 
 namespace {
 
@@ -59,7 +61,7 @@ struct Cell {
 
 __device__ __forceinline__ Cell cellOf(int nx, int ny, int nz) {
   Cell q;
-  const Blk3 B = blk3();
+  const Blk3 B = blk3(0);
   q.i = B.x * blockDim.x + threadIdx.x;
   q.j = B.y * blockDim.y + threadIdx.y;
   q.k = B.z;
@@ -80,9 +82,9 @@ struct CellN {
 };
 
 template <int NC>
-__device__ __forceinline__ CellN<NC> cellOfN(int nx, int ny, int nz) {
+__device__ __forceinline__ CellN<NC> cellOfN(int nx, int ny, int nz, int slab) {
   CellN<NC> q;
-  const Blk3 B = blk3();
+  const Blk3 B = blk3(slab);
   q.i = NC * (B.x * int(blockDim.x) + int(threadIdx.x));
   q.j = B.y * int(blockDim.y) + int(threadIdx.y);
   q.k = B.z;
@@ -96,9 +98,9 @@ __device__ __forceinline__ CellN<NC> cellOfN(int nx, int ny, int nz) {
 // cells per thread.
 template <class F>
 void withCells(const MgLevel& L, F&& f) {
-  if (L.vec == 4) f(std::integral_constant<int, 4>{}, geo3n(L.nx, L.ny, L.nz, 4));
-  else if (L.vec == 2) f(std::integral_constant<int, 2>{}, geo3n(L.nx, L.ny, L.nz, 2));
-  else f(std::integral_constant<int, 1>{}, geo3n(L.nx, L.ny, L.nz, 1));
+  if (L.vec == 4) f(std::integral_constant<int, 4>{}, geo3n(L.nx, L.ny, L.nz, 4, L.slab));
+  else if (L.vec == 2) f(std::integral_constant<int, 2>{}, geo3n(L.nx, L.ny, L.nz, 2, L.slab));
+  else f(std::integral_constant<int, 1>{}, geo3n(L.nx, L.ny, L.nz, 1, L.slab));
 }
 
 template <class T>
@@ -482,7 +484,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(MgLevel L, const T* __restrict
                                                   MeanRef mec, int ncx, int ncy, int sx, int sy, int sz,
                                                   T* __restrict__ xo, int fold, RedBuf rb, SolveCtl* ctl,
                                                   long long* stamps, InFold conv) {
-  const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz);
+  const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz, L.slab);
   const bool t0 = threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0;
   if (stamps && t0) stamps[2 * q.blk] = wall_clock64();
   // Only a sweep from zero carries the convergence test (the host passes it
@@ -561,7 +563,7 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc
                                                            T* __restrict__ x1c, InFold conv) {
   __shared__ T rs[kBlock * NC];
   __shared__ unsigned char fls[kBlock * NC];
-  const Blk3 B = blk3();
+  const Blk3 B = blk3(L.slab);
   const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
   const int bx = blockDim.x, by = blockDim.y;
   const int i = NC * (B.x * bx + tx), j = B.y * by + ty, k = B.z * int(blockDim.z) + tz;
@@ -1032,7 +1034,7 @@ __global__ void __launch_bounds__(kBlock) k_finish(MgLevel L, const T* __restric
                                                    const SolveCtl* ctl) {
   double mv[1];
   if (stopOrMeans<1>(ctl, {mx}, mv)) return;
-  const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz);
+  const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz, L.slab);
   double acc[1] = {0.0};
   if (q.valid) {
     uint16_t code[NC];
@@ -1225,7 +1227,7 @@ __global__ void k_prolong(MgLevel L, MgLevel Lc, const double* __restrict__ ec, 
 enum : int { S_MEAN_RHS = 0, S_MEAN_RN = 1, S_MEAN_X = 2 };
 
 #define NBL_CELLS(NC)                                   \
-  const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz);   \
+  const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz, L.slab);   \
   uint16_t code[NC];                                    \
   if (q.valid) ldCodes<NC>(L.code, q.c, code)
 
@@ -1853,7 +1855,8 @@ Geo3 restrictGeo(const MgLevel& L, const MgLevel& Lc) {
   const int by = kBlock / (bx * bz);
   Geo3 g;
   g.block = dim3(bx, by, bz);
-  g.grid = dim3((tx + bx - 1) / bx, (L.ny + by - 1) / by, (L.nz + bz - 1) / bz);
+  const unsigned gx = (tx + bx - 1) / bx, gy = (L.ny + by - 1) / by, gz = (L.nz + bz - 1) / bz;
+  g.grid = L.slab ? dim3(8 * gx, gy, (gz + 7) / 8) : dim3(gx, gy, gz);
   g.blocks = int(g.grid.x * g.grid.y * g.grid.z);
   return g;
 }

@@ -416,7 +416,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs_rk(NsArgs A, PadBufs pb, const d
                                                    const StepCtl* __restrict__ sc) {
   const DevGrid& g = A.g;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const Blk3 B = blk3();
+  const Blk3 B = blk3Tiled();
   int comp = 0, k = B.z;
   if (k >= nz) {
     k -= nz;

@@ -197,6 +197,7 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
     int autoVec = (m.nx % 2 == 0 && m.nx >= 32) ? 2 : 1;
     if (mixed_ && m.nx % 4 == 0 && m.nx >= 64 && m.n >= (1LL << 22)) autoVec = 4;
     m.vec = want == 0 ? autoVec : (m.nx % want == 0 ? want : 1);
+    m.slab = slabFor(m.nx, m.ny, m.nz);
     dl_.push_back(m);
     blocks_.push_back(levelGeo(m).blocks);
     stride = std::max(stride, blocks_.back());
