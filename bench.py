@@ -450,6 +450,31 @@ def served(ctx, window_s, rate, nruns=32, slots=4, seed=777, sustain_window_s=10
     return out
 
 
+def tgv_c1(ctx, a, steps=100):
+    """The 64^3 periodic Taylor-Green vortex (Re 1600, CFL 0.5, WENO-5, singular
+    MG-PCG to 1e-8), 100 steps from the analytic state in FP64 and in the mixed
+    form, with one step of the same case on one host core."""
+    g, cfg = cases.taylor_green(64)
+    st = cases.taylor_green_state(g)
+    res = {"workload": "Taylor-Green 64^3 periodic, Re 1600, CFL 0.5, WENO-5, MG-PCG tol 1e-8 (singular)",
+           "cells": g.cells, "steps": steps}
+    state = None
+    for name, prec in (("fp64", nbl.PREC_FP64), ("mixed", nbl.PREC_MIXED)):
+        reps, ms, it, el, stt = run_steps(ctx, g, cfg, st, prec, 3, steps)
+        res[name] = {"value": round(g.cells * steps / el / 1e6, 3), "ms_per_step": round(el / steps * 1e3, 4),
+                     "mg_pcg_solve_ms_median": round(statistics.median(ms), 4),
+                     "poisson_iters_per_solve_mean": round(float(np.mean(it)), 2),
+                     "max_poisson_residual": max(r["poisson_residual"] for r in reps)}
+        if state is None:
+            state = stt
+        else:
+            res[name]["rel_l2_vs_fp64"] = {k: rel(x, y) for k, x, y in zip("uvwp", stt[:4], state[:4])}
+        log(f"tgv64 {name}: {res[name]['value']} Mcell-steps/s")
+    if not a.no_cpu and a.cpu_steps > 0:
+        res["cpu_baseline"] = cpu_baseline(g, cfg, st, 1)
+    return res
+
+
 def cpu_baseline(g, cfg, st, steps):
     import oracle_py
     kind = "reference" if oracle_py.available("ref") else "port"
@@ -486,6 +511,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-octree", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-tgv", action="store_true")
     ap.add_argument("--served-window", type=float, default=20.0,
                     help="seconds of Poisson arrivals in the served-stream measurement (0: skip)")
     ap.add_argument("--served-rate", type=float, default=200.0)
@@ -564,6 +590,8 @@ def main():
     if "mixed_fp32_vcycle" in out:
         out["mixed_fp32_vcycle"]["speedup_vs_fp64_reference_loops"] = round(out["mixed_fp32_vcycle"]["value"] / lval, 3)
     log(f"fp64 reference loops: {lval:.2f} Mcell-steps/s")
+    if not a.no_tgv:
+        out["tgv64_c1"] = tgv_c1(ctx, a)
     out["roofline"] = roofline_insitu(sweep, g)
     out["roofline_isolated"] = roofline(ctx, g)
     out["coarse_solve"] = coarse_solve(ctx, g)
