@@ -304,7 +304,9 @@ template <class T>
 void launch_mg_sweep_dots(const MgLevel& L, const T* b, const T* x, double* z, const double* rn, const double* r,
                           RedBuf rb, SolveCtl* ctl, hipStream_t s, bool countIt = false);
 // rn <- rn - mean (fluid) / 0 (solid), partials of |rn|^2.
-void launch_fcg_norm(const MgLevel& L, double* rn, MeanRef mrn, const SolveCtl* ctl, RedBuf rb, hipStream_t s);
+// b0 (optional): also write float(r_new) there, the FP32 V-cycle's right-hand side
+void launch_fcg_norm(const MgLevel& L, double* rn, MeanRef mrn, const SolveCtl* ctl, RedBuf rb, hipStream_t s,
+                     float* b0 = nullptr);
 void launch_fcg_dots(const MgLevel& L, double* z, MeanRef mz, const double* rn, const double* r,
                      SolveCtl* ctl, RedBuf rb, hipStream_t s, bool countIt = false);
 // k_finish fused into k_fcg_dots (no pending means): z = V-cycle output x.

@@ -1467,7 +1467,7 @@ __global__ void __launch_bounds__(kBlock) k_fcg_update_first(MgLevel L, double* 
 
 template <int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_norm(MgLevel L, double* __restrict__ rn, MeanRef mrn,
-                                                     const SolveCtl* ctl, RedBuf rb) {
+                                                     const SolveCtl* ctl, RedBuf rb, float* __restrict__ b0) {
   double mv[1];
   if (stopOrMeans<1>(ctl, {mrn}, mv)) return;
   NBL_CELLS(NC);
@@ -1482,6 +1482,12 @@ __global__ void __launch_bounds__(kBlock) k_fcg_norm(MgLevel L, double* __restri
       acc[0] += v[e] * v[e];
     }
     stN<NC>(rn, q.c, v);
+    if (b0) {  // the FP32 V-cycle's right-hand side, as k_to_level0 writes it
+      float f[NC];
+#pragma unroll
+      for (int e = 0; e < NC; ++e) f[e] = float(v[e]);
+      stN<NC>(b0, q.c, f);
+    }
   }
   blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
@@ -2095,9 +2101,10 @@ template void launch_mg_sweep_dots<double>(const MgLevel&, const double*, const 
                                            const double*, RedBuf, SolveCtl*, hipStream_t, bool);
 template void launch_mg_sweep_dots<float>(const MgLevel&, const float*, const float*, double*, const double*,
                                           const double*, RedBuf, SolveCtl*, hipStream_t, bool);
-void launch_fcg_norm(const MgLevel& L, double* rn, MeanRef mrn, const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
+void launch_fcg_norm(const MgLevel& L, double* rn, MeanRef mrn, const SolveCtl* ctl, RedBuf rb, hipStream_t s,
+                     float* b0) {
   withCells(L, [&](auto nc, const Geo3& g) {
-    k_fcg_norm<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, rn, mrn, ctl, rb);
+    k_fcg_norm<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, rn, mrn, ctl, rb, b0);
   });
   NBL_LAUNCH_CHECK();
 }

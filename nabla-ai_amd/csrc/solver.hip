@@ -504,7 +504,7 @@ void PoissonDev::vcycle(int l, const T* b, MeanRef mb, const T** xout, MeanRef* 
 // (multigrid.cpp:283-289).  z's mean is left pending in kSlotMeanZ (zMean()).  With
 // fuseDots (systems without projections) the V-cycle output is left for
 // k_fcg_dots_x, which forms z itself.
-void PoissonDev::minv(const double* r, hipStream_t s, const double* rOld, bool fused) {
+void PoissonDev::minv(const double* r, hipStream_t s, const double* rOld, bool fused, bool b0Ready) {
   SolveCtl* ctl = ctl_.as<SolveCtl>();
   const MgLevel& L = dl_[0];
   const RedBuf rb = red();
@@ -516,7 +516,7 @@ void PoissonDev::minv(const double* r, hipStream_t s, const double* rOld, bool f
   const bool inlZ = inlineMean(blocks_[0]);
   const RedBuf zRed = project_ ? meanRed(kSlotMeanZ, inlZ) : rb;
   if (mixed_) {
-    if (!fused) launch_to_level0<float>(L, r, b0f_.as<float>(), ctl, s);
+    if (!fused && !b0Ready) launch_to_level0<float>(L, r, b0f_.as<float>(), ctl, s);
     const float* x = nullptr;
     MeanRef mx{};
     vcycle<float>(0, b0f_.as<float>(), MeanRef{}, &x, &mx, s);
@@ -574,10 +574,12 @@ void PoissonDev::enqueueIteration(hipStream_t s, int k) {
       const bool inlM = inlineMean(nb);
       launch_fcg_update(L, true, x, p, r, Ap, rn, ctl, meanRed(kSlotMeanRn, inlM), s, fold(rbP));
       meanFin(kSlotMeanRn, inlM, nb, L.nf, true, s);
-      launch_fcg_norm(L, rn, meanRef(kSlotMeanRn, inlM, nb, L.nf), ctl, rbR, s);
+      // the mixed form's FP32 copy of r_new is written beside it
+      launch_fcg_norm(L, rn, meanRef(kSlotMeanRn, inlM, nb, L.nf), ctl, rbR, s,
+                      mixed_ ? b0f_.as<float>() : nullptr);
       if (inline_) conv_ = fold(rbR);
       else fin(FIN_CONV, 1, nb, 0, 0, true, s);
-      minv(rn, s, nullptr, false);
+      minv(rn, s, nullptr, false, mixed_);
       if (conv_.n) throw std::logic_error("flexible CG: convergence test not placed");
       launch_fcg_dots(L, z, zMean(), rn, r, ctl, rb, s, inline_);
     } else {

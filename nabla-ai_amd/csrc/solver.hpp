@@ -111,7 +111,8 @@ class PoissonDev {
   // z = Minv r.  rOld: fold the flexible-CG dots z.r, z.rOld into the output
   // pass.  fused: the level-0 first sweep (and FP32 right-hand side) were
   // written by the update, and the last level-0 sweep writes z and the dots.
-  void minv(const double* r, hipStream_t s, const double* rOld, bool fused);
+  // b0Ready: the FP32 level-0 right-hand side is already written (mixed)
+  void minv(const double* r, hipStream_t s, const double* rOld, bool fused, bool b0Ready = false);
   bool buildDirectCoarse();
   void fin(int op, int K, int nparts, int slot, long long nf, bool predicated, hipStream_t s, int earlyExit = 0,
            double convTol = 0.0);
