@@ -86,7 +86,7 @@ def roofline(ctx, g, reps=200, precision=nbl.PREC_FP64, kernel=0):
             f"singular; {14 if fp32 else 26} B/cell)")
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4),
-            "traffic": pmc_traffic(key) if (kernel == 0 and not fp32) else None,
+            "traffic": pmc_traffic(key + ("_spmv" if kernel == 1 else "_fp32" if fp32 else "")),
             "kernel": name, "avg_launch_us": round(ms * 1e3, 3), "bytes_per_launch": by, "cells": g.cells}
 
 
@@ -529,7 +529,10 @@ def main():
     st = cases.cube_channel_state(g)
     ctx = nbl.Ctx(0)
     if a.roofline_only:
-        out = {"roofline": roofline(ctx, g), "roofline_256cubed": roofline(ctx, nbl.Grid(256, 256, 256, 1.0 / 256), 50),
+        big = nbl.Grid(256, 256, 256, 1.0 / 256)
+        out = {"roofline": roofline(ctx, g), "roofline_256cubed": roofline(ctx, big, 50),
+               "roofline_256cubed_fp32": roofline(ctx, big, 50, precision=nbl.PREC_MIXED),
+               "roofline_256cubed_spmv": roofline(ctx, big, 50, kernel=1),
                "copy_256MiB_ms": ctx.probe_copy(256 << 20, 20)}
         ctx.close()
         print(json.dumps(out), flush=True)

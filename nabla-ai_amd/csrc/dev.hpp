@@ -94,8 +94,8 @@ Geo3 geo3(int nx, int ny, int nz);
 Geo3 geo3n(int nx, int ny, int nz, int cells, int slab = 0);  // `cells` consecutive cells along i per thread
 // Whether a level's cell kernels run in XCD slabs: levels below 8M cells with
 // at least 8 planes.  On the 192 x 64 x 64 channel slabs make the step 3%
-// faster than plain order; on 256^3 plain order is as fast (A p 83.6 us with
-// no block-coordinate division, 98 us with the division-based tile order).
+// faster than plain order; on 256^3 plain order is faster (FP64 sweep 96 vs
+// 102 us, A p 84 vs 91 us) although it fetches 24-34% more from HBM.
 int slabFor(int nx, int ny, int nz);
 Geo3 levelGeo(const MgLevel& L);                // the launch of a level's cell kernels (L.vec)
 // Launch shape of the fused residual + restriction from L to Lc: one thread per
