@@ -228,46 +228,13 @@ def octree_c3(ctx, with_cpu=True):
     return out
 
 
-def batch_octrees(count, seed):
-    """Octrees of 0.5-2M leaves: unit cube, base level 6 (262,144 leaves), then
-    seeded random boxes refined (balanced) to level 8 until the leaf count
-    reaches a seeded target."""
-    u = cases.uniform01(seed, 4096)
-    out, q = [], 0
-    for t in range(count):
-        target = int(500_000 + 1_500_000 * float(u[q]))
-        q += 1
-        tr = nbl.Tree((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 8)
-        tr.refine_uniform(6)
-        while tr.n < target:
-            _k, lv, c = tr.leaves()
-            f = np.zeros(lv.size, bool)
-            for _ in range(3):  # three seeded boxes per balanced pass
-                lo = np.array([float(u[q]), float(u[q + 1]), float(u[q + 2])]) * 0.7
-                hi = lo + 0.08 + 0.22 * float(u[q + 3])
-                q += 4
-                f |= np.all((c >= lo) & (c <= hi), axis=1)
-            f &= lv < 8
-            if f.any():
-                tr.refine_flagged(f)
-        k, lv, _c = tr.leaves()
-        out.append((k, lv))
-    return out
-
-
 def batched(ctx, with_cpu=True, seed=4242, nuni=48, noct=8, concurrency=4):
     """48 independent uniform Poisson problems (n^3, n drawn from 32/64/96/128,
     all walls, i.i.d. N(0,1) right-hand sides projected to zero mean, MG-PCG to
     1e-8) and 8 octree problems (Jacobi-PCG and the multigrid CG to 1e-8): one
     solve after another versus nbl_*_solve_batch with `concurrency` solves in
     flight."""
-    u = cases.uniform01(seed, nuni)
-    sizes = [(32, 64, 96, 128)[int(float(x) * 4) % 4] for x in u]
-    grids = [nbl.Grid(n, n, n, 1.0 / n) for n in sizes]
-    rhs = []
-    for q, g in enumerate(grids):
-        b = cases.normal(seed + 1 + q, g.cells)
-        rhs.append(b - b.mean())
+    sizes, grids, rhs = cases.batch_uniform(nuni, seed)
     t0 = time.perf_counter()
     solvers = [nbl.Poisson(ctx, g, kind=0, tol=1e-8, max_iters=200) for g in grids]
     setup_s = time.perf_counter() - t0
@@ -291,7 +258,7 @@ def batched(ctx, with_cpu=True, seed=4242, nuni=48, noct=8, concurrency=4):
                        "iterations_mean": round(float(np.mean([r.iterations for r in reps])), 2),
                        "batch_equals_sequential": bool(same),
                        "mcells_solved_per_s": round(cells / bat_s / 1e6, 2)}}
-    trees = batch_octrees(noct, seed + 99)
+    trees = cases.batch_octrees(noct, seed + 99)
     octs, orhs = [], []
     for q, (k, lv) in enumerate(trees):
         octs.append(nbl.Octree(ctx, k, lv, np.zeros(k.size, np.uint8), (0, 0, 0), (1, 1, 1), 8))
@@ -344,35 +311,13 @@ def batched(ctx, with_cpu=True, seed=4242, nuni=48, noct=8, concurrency=4):
     return out
 
 
-def served_runs(nruns, seed):
-    """32 seeded runs of the two uniform families: Taylor-Green n^3 boxes and
-    cube channels at r cells per cube side (192 r^3 cells), sizes 0.3-4M
-    cells, Re uniform in [500, 3000]."""
-    u = cases.uniform01(seed, 3 * nruns)
-    out = []
-    for q in range(nruns):
-        re = 500.0 + 2500.0 * float(u[3 * q + 1])
-        if u[3 * q] < 0.5:
-            n = 2 * int(round((67 + (158 - 67) * float(u[3 * q + 2])) / 2))
-            g, cfg = cases.taylor_green(n, reynolds=re)
-            st = cases.taylor_green_state(g)
-            name = f"tgv{n}"
-        else:
-            r = int(round(12 + (27 - 12) * float(u[3 * q + 2])))
-            g, cfg = cases.cube_channel(nx=12 * r, ny=4 * r, nz=4 * r, res=r, reynolds=re)
-            st = cases.cube_channel_state(g, seed=seed + q)
-            name = f"cube{12 * r}x{4 * r}x{4 * r}"
-        out.append((name, g, cfg, st))
-    return out
-
-
 def served(ctx, window_s, rate, nruns=32, slots=4, seed=777, sustain_window_s=10.0, precision=None):
     """Poisson arrivals at `rate` requests/s over `window_s` seconds, each one
     RK3 step of a uniformly drawn run; latency = submit to completion.  The
     same server then takes a second stream at 70% of the request rate it
     completed in the first (a load it sustains), over `sustain_window_s`."""
     import threading
-    runs = served_runs(nruns, seed)
+    runs = cases.served_runs(nruns, seed)
     srv = nbl.Server(ctx, slots=slots, hbm_budget=200 << 30)
     t0 = time.perf_counter()
     ids, cells = [], []

@@ -197,6 +197,70 @@ def poisson_rhs(g: nbl.Grid, seed: int):
     return b
 
 
+# ---- batched solves and the served stream --------------------------------------
+def batch_uniform(count: int = 48, seed: int = 4242):
+    """The batched-solve problem set: n^3 unit boxes with n drawn by seed from
+    32/64/96/128, all faces walls (pure Neumann, singular), i.i.d. N(0,1)
+    right-hand sides projected to zero mean.  Returns (sizes, grids, rhs)."""
+    u = uniform01(seed, count)
+    sizes = [(32, 64, 96, 128)[int(float(x) * 4) % 4] for x in u]
+    grids = [nbl.Grid(n, n, n, 1.0 / n) for n in sizes]
+    rhs = []
+    for q, g in enumerate(grids):
+        b = normal(seed + 1 + q, g.cells)
+        rhs.append(b - b.mean())
+    return sizes, grids, rhs
+
+
+def batch_octrees(count, seed):
+    """Octrees of 0.5-2M leaves: unit cube, base level 6 (262,144 leaves), then
+    seeded random boxes refined (balanced) to level 8 until the leaf count
+    reaches a seeded target."""
+    u = uniform01(seed, 4096)
+    out, q = [], 0
+    for t in range(count):
+        target = int(500_000 + 1_500_000 * float(u[q]))
+        q += 1
+        tr = nbl.Tree((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 8)
+        tr.refine_uniform(6)
+        while tr.n < target:
+            _k, lv, c = tr.leaves()
+            f = np.zeros(lv.size, bool)
+            for _ in range(3):  # three seeded boxes per balanced pass
+                lo = np.array([float(u[q]), float(u[q + 1]), float(u[q + 2])]) * 0.7
+                hi = lo + 0.08 + 0.22 * float(u[q + 3])
+                q += 4
+                f |= np.all((c >= lo) & (c <= hi), axis=1)
+            f &= lv < 8
+            if f.any():
+                tr.refine_flagged(f)
+        k, lv, _c = tr.leaves()
+        out.append((k, lv))
+    return out
+
+
+def served_runs(nruns, seed):
+    """32 seeded runs of the two uniform families: Taylor-Green n^3 boxes and
+    cube channels at r cells per cube side (192 r^3 cells), sizes 0.3-4M
+    cells, Re uniform in [500, 3000]."""
+    u = uniform01(seed, 3 * nruns)
+    out = []
+    for q in range(nruns):
+        re = 500.0 + 2500.0 * float(u[3 * q + 1])
+        if u[3 * q] < 0.5:
+            n = 2 * int(round((67 + (158 - 67) * float(u[3 * q + 2])) / 2))
+            g, cfg = taylor_green(n, reynolds=re)
+            st = taylor_green_state(g)
+            name = f"tgv{n}"
+        else:
+            r = int(round(12 + (27 - 12) * float(u[3 * q + 2])))
+            g, cfg = cube_channel(nx=12 * r, ny=4 * r, nz=4 * r, res=r, reynolds=re)
+            st = cube_channel_state(g, seed=seed + q)
+            name = f"cube{12 * r}x{4 * r}x{4 * r}"
+        out.append((name, g, cfg, st))
+    return out
+
+
 # ---- octree workload (cube wake) ---------------------------------------------
 CUBE_BOX = (3.0, 4.0, 0.0, 1.0, 1.5, 2.5)
 WAKE_BOX = (4.0, 10.0, 0.0, 2.0, 1.0, 3.0)

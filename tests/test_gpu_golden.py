@@ -135,3 +135,73 @@ def test_cube192_history(ctx):
             assert np.linalg.norm(a) == pytest.approx(float(f[f"{name}_l2"]), rel=1e-9)
             got = a[f[f"{name}_idx"]]
             assert np.abs(got - f[f"{name}_val"]).max() <= 1e-9 * np.abs(f[f"{name}_val"]).max(), name
+
+
+# ---- the mixed form (FP32 V-cycle inside the FP64 flexible CG) ------------------
+# Tolerances of the mixed form against the reference's FP64 fixtures: the FP64
+# outer residual reaches the same 1e-8 at every solve, drag and lift within 1e-6
+# relative (1e-9 absolute where they cross zero), and the integral histories
+# and fields within the rounding the FP32 preconditioner leaves inside that
+# tolerance.
+def test_tgv64_hundred_steps_mixed(ctx):
+    f = load("tgv64_100.npz")
+    g, cfg = cases.taylor_green(64)
+    run = nbl.Run(ctx, g, cfg, nbl.solver_opts(nbl.PREC_MIXED))
+    ke, ens, reps = [], [], []
+    try:
+        run.set_state(*cases.taylor_green_state(g))
+        for _ in range(100):
+            reps.append(run.step())
+            u, v, w, *_ = run.get_state()
+            a, b = cases.kinetic_energy_enstrophy(g, u, v, w)
+            ke.append(a)
+            ens.append(b)
+        u, v, w, p, *_ = run.get_state()
+    finally:
+        run.close()
+    ke_err = np.abs(np.array(ke) / f["ke"] - 1).max()
+    ens_err = np.abs(np.array(ens) / f["enstrophy"] - 1).max()
+    iters = rep_array(reps, "poisson_iters")
+    print(f"TGV64 mixed: KE {ke_err:.2e}, enstrophy {ens_err:.2e}, iterations "
+          f"{iters.sum()} vs {int(f['rep_poisson_iters'].sum())}", flush=True)
+    assert ke_err <= 1e-10 and ens_err <= 1e-9
+    assert np.abs(iters - f["rep_poisson_iters"]).max() <= 3
+    assert (rep_array(reps, "poisson_residual") <= 1e-8).all()
+    for a, name in zip((u, v, w, p), "uvwp"):
+        e = abs(np.linalg.norm(a) / float(f[f"{name}_l2"]) - 1)
+        d = np.abs(a[f[f"{name}_idx"]] - f[f"{name}_val"]).max() / max(float(f[f"{name}_max"]), 1e-30)
+        print(f"  {name}: l2 {e:.2e}, sampled max {d:.2e}", flush=True)
+        assert e <= 1e-8 and d <= 1e-7, name
+
+
+def test_cube192_history_mixed(ctx):
+    """The 192x64x64 cube channel over the fixture's 500 steps with the FP32
+    V-cycle: residual <= 1e-8 at every solve, drag and lift within 1e-6."""
+    f = load("cube192_history.npz")
+    n = len(f["rep_step"])
+    g, cfg = cases.cube_channel()
+    run = nbl.Run(ctx, g, cfg, nbl.solver_opts(nbl.PREC_MIXED))
+    try:
+        run.set_state(*cases.cube_channel_state(g))
+        reps = [run.step() for _ in range(n)]
+        st = run.get_state()
+    finally:
+        run.close()
+    cd, cl = rep_array(reps, "cd"), rep_array(reps, "cl")
+    cd_err = np.abs(cd / f["rep_cd"] - 1).max()
+    cl_err = (np.abs(cl - f["rep_cl"]) / np.maximum(np.abs(f["rep_cl"]), 1e-3)).max()
+    iters = rep_array(reps, "poisson_iters")
+    print(f"cube192 mixed, {n} steps: cd {cd_err:.2e}, cl {cl_err:.2e}, iterations {iters.sum()} vs "
+          f"{int(f['rep_poisson_iters'].sum())}, max residual {rep_array(reps, 'poisson_residual').max():.2e}",
+          flush=True)
+    assert (rep_array(reps, "poisson_residual") <= 1e-8).all()
+    assert np.abs(iters - f["rep_poisson_iters"]).max() <= 3
+    assert np.allclose(rep_array(reps, "dt"), f["rep_dt"], rtol=1e-8, atol=0)
+    assert np.allclose(cd, f["rep_cd"], rtol=1e-6, atol=1e-9)
+    assert np.allclose(cl, f["rep_cl"], rtol=1e-6, atol=1e-9)
+    assert np.array_equal(rep_array(reps, "accepted"), f["rep_accepted"])
+    for a, name in zip(st[:4], "uvwp"):
+        e = abs(np.linalg.norm(a) / float(f[f"{name}_l2"]) - 1)
+        d = np.abs(a[f[f"{name}_idx"]] - f[f"{name}_val"]).max() / np.abs(f[f"{name}_val"]).max()
+        print(f"  {name}: l2 {e:.2e}, sampled max {d:.2e}", flush=True)
+        assert e <= 1e-8 and d <= 1e-6, name

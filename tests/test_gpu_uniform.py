@@ -396,3 +396,22 @@ def test_singular_mg_is_deterministic(ctx, n):
         assert np.array_equal(x, xs[0])
     for z in zs[1:]:
         assert np.array_equal(z, zs[0])
+
+
+@pytest.mark.parametrize("cpt", [2, 4])
+def test_mixed_singular_vector_paths_match_oracle(ctx, orc, cpt):
+    """The mixed form on a singular grid wide enough for 2 and 4 cells per
+    thread: the kernel that removes r_new's mean also writes the FP32 copy the
+    V-cycle starts from with 8- and 16-byte stores.  Against the FP64 oracle."""
+    g, _ = cases.taylor_green(32)
+    b = cases.poisson_rhs(g, 29)
+    ps = nbl.Poisson(ctx, g, kind=0, tol=1e-10, max_iters=200,
+                     opts=nbl.solver_opts(nbl.PREC_MIXED, cells_per_thread=cpt))
+    try:
+        x, rep = ps.solve(b)
+    finally:
+        ps.close()
+    xo, ro = orc.mg(g, b, 1e-10, 200)
+    assert rep.converged and rep.residual <= 1e-10 * 1.01
+    assert abs(rep.iterations - ro.iterations) <= 2
+    assert rel(x, xo) < 1e-8
