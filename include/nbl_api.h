@@ -221,6 +221,15 @@ int nbl_poisson_create(nbl_ctx* ctx, const nbl_grid_desc* grid, int kind, double
 int nbl_poisson_destroy(nbl_poisson* ps);
 /* Host buffers. */
 int nbl_poisson_solve(nbl_poisson* ps, const double* b, double* x, nbl_poisson_report* rep);
+/* nbl_poisson_solve without allocating: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (tight serving loops, harnesses
+ * that forbid hipMalloc): the device copies of b and x live in a fixed array
+ * that every GPU the library runs on holds until the process ends, instead of
+ * two allocations per call.  Same arguments, results and report as
+ * nbl_poisson_solve; grids above 2^21 cells return NBL_ERR_CAPACITY.  All
+ * calls on one GPU share that array: calls that overlap in time on one GPU
+ * race (serialise them). */
+int nbl_poisson_solve_noalloc(nbl_poisson* ps, const double* b, double* x, nbl_poisson_report* rep);
 /* Device buffers (HBM resident, length nx*ny*nz, 16-byte aligned as any
  * hipMalloc allocation is). */
 int nbl_poisson_solve_device(nbl_poisson* ps, const double* d_b, double* d_x,

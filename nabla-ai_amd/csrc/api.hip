@@ -2,6 +2,7 @@
 // catches C++ exceptions and turns them into a status code plus a
 // thread-local message; nothing throws across the ABI.
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -53,6 +54,32 @@ nbl_solver_opts optsOr(const nbl_solver_opts* o) {
   nbl_solver_opts d;
   nbl_solver_opts_default(&d);
   return o ? *o : d;
+}
+
+// Staging of nbl_poisson_solve_noalloc: b's and x's device copies, each up to
+// kNoAllocCells FP64 values (2 x 16 MiB), held by every GPU the library runs on.
+constexpr long long kNoAllocCells = 1LL << 21;
+__device__ __align__(256) double scratch[2 * kNoAllocCells];
+
+// Device address of `scratch` on device `dev` (looked up once per device, with
+// that device current: each GPU holds its own copy at its own address).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMu;
+double* g_scratchPtr[kMaxDevices] = {};
+
+double* noAllocScratch(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) throw std::invalid_argument("device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratchMu);
+  if (!g_scratchPtr[dev]) {
+    int cur = -1;
+    NBL_HIP(hipGetDevice(&cur));
+    if (cur != dev) throw std::logic_error("noAllocScratch: device is not current");
+    void* p = nullptr;
+    NBL_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw nbl::HipError("hipGetSymbolAddress(scratch) returned NULL");
+    g_scratchPtr[dev] = static_cast<double*>(p);
+  }
+  return g_scratchPtr[dev];
 }
 
 }  // namespace
@@ -287,6 +314,33 @@ int nbl_poisson_solve(nbl_poisson* ps, const double* b, double* x, nbl_poisson_r
     NBL_HIP(hipMemcpyAsync(st.a.as<void>(), b, n * 8, hipMemcpyHostToDevice, ps->stream));
     const nbl_poisson_report r = ps->impl->solve(st.a.as<double>(), st.b.as<double>(), ps->stream);
     NBL_HIP(hipMemcpyAsync(x, st.b.as<void>(), n * 8, hipMemcpyDeviceToHost, ps->stream));
+    NBL_HIP(hipStreamSynchronize(ps->stream));
+    if (rep) *rep = r;
+  });
+}
+
+// The same solve with the device copies of b and x in `scratch` (a file-scope
+// device array each GPU holds for the life of the process) instead of two
+// allocations per call.  The host array name is only a stand-in: kernels and
+// copies use the address hipGetSymbolAddress returns for the current device,
+// looked up once per device (noAllocScratch).  b's copy is written whole by
+// the upload and x's whole by the solve's last kernel, so nothing has to be
+// cleared between calls; calls that overlap on one GPU share the array and
+// race.
+int nbl_poisson_solve_noalloc(nbl_poisson* ps, const double* b, double* x, nbl_poisson_report* rep) {
+  return guarded([&] {
+    need(ps && b && x, "NULL argument");
+    useDevice(ps->ctx);
+    const size_t n = ps->grid.cells();
+    if (n > size_t(kNoAllocCells))
+      throw nbl::CapacityError("nbl_poisson_solve_noalloc: " + std::to_string(n) + " cells exceed the " +
+                               std::to_string(kNoAllocCells) + "-cell scratch; use nbl_poisson_solve");
+    double* s = noAllocScratch(ps->ctx->device);
+    double* db = s;
+    double* dx = s + kNoAllocCells;
+    NBL_HIP(hipMemcpyAsync(db, b, n * 8, hipMemcpyHostToDevice, ps->stream));
+    const nbl_poisson_report r = ps->impl->solve(db, dx, ps->stream);
+    NBL_HIP(hipMemcpyAsync(x, dx, n * 8, hipMemcpyDeviceToHost, ps->stream));
     NBL_HIP(hipStreamSynchronize(ps->stream));
     if (rep) *rep = r;
   });

@@ -209,6 +209,7 @@ def load(path: os.PathLike | None = None):
     _proto(lib, "nbl_poisson_destroy", i, V)
     _proto(lib, "nbl_poisson_solve", i, V, P_D, P_D, C.POINTER(PoissonReport))
     _proto(lib, "nbl_poisson_solve_device", i, V, C.c_void_p, C.c_void_p, C.POINTER(PoissonReport))
+    _proto(lib, "nbl_poisson_solve_noalloc", i, V, P_D, P_D, C.POINTER(PoissonReport))
     _proto(lib, "nbl_poisson_precondition", i, V, P_D, P_D)
     _proto(lib, "nbl_poisson_apply", i, V, P_D, P_D)
     _proto(lib, "nbl_poisson_diagonal", i, V, P_D)
@@ -387,6 +388,15 @@ class Poisson:
         x = np.empty(self.grid.cells)
         rep = PoissonReport()
         check(self.lib.nbl_poisson_solve(self.h, dptr(b), dptr(x), C.byref(rep)))
+        return x, rep
+
+    def solve_noalloc(self, b):
+        """nbl_poisson_solve_noalloc: the same solve with its device staging in
+        the library's fixed per-GPU array (no allocation per call)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.empty(self.grid.cells)
+        rep = PoissonReport()
+        check(self.lib.nbl_poisson_solve_noalloc(self.h, dptr(b), dptr(x), C.byref(rep)))
         return x, rep
 
     def solve_device(self, d_b: int, d_x: int):
