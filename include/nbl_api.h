@@ -121,6 +121,19 @@ typedef struct nbl_solver_opts {
                                64 x 4 face brick marches through, 0
                                automatic or 1..16; results are bit-identical
                                across choices                              */
+  int vcycle;               /* V(2,2) cycle form.  1: one kernel per sweep
+                               (and per restriction).  2: two kernels per
+                               level (both pre-sweeps, the residual and the
+                               restriction in one; both post-sweeps in one;
+                               LDS tiles recomputing their halos) wherever
+                               it applies: at least three levels, direct
+                               coarsest solve, pre = post = 2, a finest
+                               level of at most 2048 tiles (2^20 cells in
+                               3-D).  0 (default): 2 on singular systems,
+                               where it also folds the per-sweep null-space
+                               projections away (10 launches per iteration
+                               instead of 21), 1 otherwise (measured faster
+                               there)                                      */
 } nbl_solver_opts;
 
 /* StepReport (ns_solver.hpp:34-49). */

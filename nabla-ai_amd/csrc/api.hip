@@ -112,6 +112,7 @@ void nbl_solver_opts_default(nbl_solver_opts* o) {
   o->cells_per_thread = 0;
   o->loops = 0;
   o->rhs_planes = 0;
+  o->vcycle = 0;
 }
 
 const char* nbl_last_error(void) { return g_err.c_str(); }

@@ -247,6 +247,60 @@ void launch_mg_finish(const MgLevel& L, const T* x, MeanRef mx, double* z, bool 
 template <class T>
 void launch_to_level0(const MgLevel& L, const double* r, T* b, const SolveCtl* ctl, hipStream_t s);
 
+// ---- the tiled V-cycle (vcycle.hip): two kernels per level ------------------
+// Down kernel sources of the level's right-hand side: DS_B a level vector b
+// (pending mean mb); DS_UPD level 0 of a flexible-CG iteration, r_new = r -
+// alpha A p formed on the fly (x += alpha p and r_new written on the tile);
+// DS_RN level 0 of a singular iteration, r_new read with its pending mean
+// and written projected.
+enum DownSrc : int { DS_B = 0, DS_UPD = 1, DS_RN = 2 };
+template <class T>
+struct DownArgs {
+  MgLevel L, Lc;
+  const T* b = nullptr;
+  MeanRef mb{};
+  const double* r = nullptr;
+  const double* Ap = nullptr;
+  double* x = nullptr;
+  const double* p = nullptr;
+  InFold alpha{};             // DS_UPD: alpha = rz / p.Ap folded from the partials of p.Ap
+  const double* rnRaw = nullptr;
+  MeanRef mrn{};              // DS_RN
+  double* rnOut = nullptr;    // DS_UPD / DS_RN: r_new (projected for DS_RN)
+  T* b0Out = nullptr;         // DS_UPD / DS_RN with T = float: the level-0 right-hand side float(r_new)
+  T* xa = nullptr;            // x after the two pre-sweeps
+  T* bc = nullptr;            // restricted residual
+  double* rowNorm = nullptr;  // per-block |r_new|^2 (DS_UPD / DS_RN)
+  double* rowMean = nullptr;  // per-block fluid sum of bc (singular systems)
+  SolveCtl* ctl = nullptr;
+  InFold conv{};              // the flexible CG's convergence test, folded at entry
+};
+// Up kernel destinations: UD_X the level's x; UD_Z level 0, z (and per-block
+// fluid sums of z in rb when rb.partials is set); UD_DOTS level 0, z and the
+// partials of z.r_new, z.r (rows 0, 1 of rb).
+enum UpDst : int { UD_X = 0, UD_Z = 1, UD_DOTS = 2 };
+template <class T>
+struct UpArgs {
+  MgLevel L, Lc;
+  const T* b = nullptr;
+  MeanRef mb{};
+  const T* xa = nullptr;      // x after the pre-sweeps
+  const T* e = nullptr;       // the coarse level's correction
+  T* xb = nullptr;            // UD_X
+  double* z = nullptr;        // UD_Z / UD_DOTS
+  const double* rn = nullptr;
+  const double* r = nullptr;
+  RedBuf rb{nullptr, 0};
+  SolveCtl* ctl = nullptr;
+  int countIt = 0;            // block 0 counts the finished flexible-CG iteration
+};
+template <class T>
+void launch_down(int src, const DownArgs<T>& a, hipStream_t s);
+template <class T>
+void launch_up(int dst, const UpArgs<T>& a, hipStream_t s);
+int tileBlocks(const MgLevel& L);           // workgroups (= partials) of a level's tiled kernels
+bool tileFits(const MgLevel& L, const MgLevel& Lc);
+
 // The reference's multigrid / flexible-CG loops one kernel per loop (the
 // unfused FP64 baseline).  Reductions write one partial per block of
 // loopBlocks(n) blocks.

@@ -201,6 +201,8 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
     dl_.push_back(m);
     blocks_.push_back(levelGeo(m).blocks);
     stride = std::max(stride, blocks_.back());
+    tblocks_.push_back(tileBlocks(m));
+    stride = std::max(stride, tblocks_.back());
     rblocks_.push_back(l > 0 ? restrictGeo(dl_[l - 1], m).blocks : 0);
     stride = std::max(stride, rblocks_.back());
     const size_t n = H.g.cells();
@@ -242,6 +244,20 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
   NBL_HIP(hipEventCreate(&e1_));
   predicted_ = std::max(1, opts.check_every);
   if (kind == 0 && nl > 1 && opts.coarse == NBL_COARSE_DIRECT && !loops_) direct_ = buildDirectCoarse();
+  if (opts.vcycle < 0 || opts.vcycle > 2) throw std::invalid_argument("vcycle must be 0, 1 or 2");
+  // the two-kernels-per-level cycle: V(2,2), a level between the finest and
+  // the directly solved coarsest one, and few enough level-0 workgroups that
+  // every reduction folds inline into its consumer.  Automatic: on singular
+  // systems only.  On the 192x64x64 channel (non-singular) its level-0
+  // kernels, which recompute their halos, take 33 + 24 us against 29 + 15 us
+  // for the per-sweep kernels they replace (the coarser levels break even);
+  // on the 64^3 periodic box (singular) it also drops the per-sweep
+  // projections and a step takes 3.58 instead of 5.05 ms.
+  const bool wantTiled = opts.vcycle == 2 || (opts.vcycle == 0 && project_);
+  tiled_ = kind == 0 && !loops_ && wantTiled && nl >= 3 && opts.pre_smooth == 2 && opts.post_smooth == 2 &&
+           direct_ && blocks_[0] <= kInlineFoldMax && tblocks_[0] <= kInlineFoldMax;
+  for (int l = 0; tiled_ && l + 1 < nl; ++l) tiled_ = tileFits(dl_[l], dl_[l + 1]);
+  if (tiled_ && project_) rnRaw_ = DevBuf(ctx, n0 * 8);
   NBL_HIP(hipDeviceSynchronize());
 }
 
@@ -329,8 +345,17 @@ MeanRef PoissonDev::meanRef(int slot, bool inl, int blocks, long long nf) const 
   return m;
 }
 
+MeanRef PoissonDev::zMeanTiled() const {
+  MeanRef m;
+  m.parts = slotRow(kSlotMeanZ);
+  m.n = tblocks_[0];
+  m.nf = double(dl_[0].nf);
+  return m;
+}
+
 MeanRef PoissonDev::zMean() const {
   if (!project_) return MeanRef{};
+  if (tiled_) return zMeanTiled();
   return meanRef(kSlotMeanZ, inlineMean(blocks_[0]), blocks_[0], dl_[0].nf);
 }
 
@@ -505,6 +530,17 @@ void PoissonDev::vcycle(int l, const T* b, MeanRef mb, const T** xout, MeanRef* 
 // fuseDots (systems without projections) the V-cycle output is left for
 // k_fcg_dots_x, which forms z itself.
 void PoissonDev::minv(const double* r, hipStream_t s, const double* rOld, bool fused, bool b0Ready) {
+  if (tiled_) {  // the first cycle of a solve, or precondition: z = Minv r, z's mean left pending
+    if (rOld || fused) throw std::logic_error("minv: the tiled cycle forms iterations itself");
+    if (mixed_) {
+      launch_to_level0<float>(dl_[0], r, b0f_.as<float>(), ctl_.as<SolveCtl>(), s);
+      tiledCycle<float>(s, 0, true, b0f_.as<float>(), nullptr, nullptr, nullptr, InFold{});
+    } else {
+      tiledCycle<double>(s, 0, true, r, nullptr, nullptr, nullptr, InFold{});
+    }
+    ++vcount_;
+    return;
+  }
   SolveCtl* ctl = ctl_.as<SolveCtl>();
   const MgLevel& L = dl_[0];
   const RedBuf rb = red();
@@ -547,6 +583,10 @@ void PoissonDev::minv(const double* r, hipStream_t s, const double* rOld, bool f
 void PoissonDev::enqueueIteration(hipStream_t s, int k) {
   if (loops_) {
     enqueueIterationLoops(s);
+    return;
+  }
+  if (tiled_) {
+    enqueueIterationTiled(s, k);
     return;
   }
   SolveCtl* ctl = ctl_.as<SolveCtl>();
@@ -606,6 +646,139 @@ void PoissonDev::enqueueIteration(hipStream_t s, int k) {
     launch_cg_z(L, r, z, false, nullptr, ctl, rb, s);
     fin(FIN_BETA, 1, nb, 0, 0, true, s);
   }
+}
+
+// One V(2,2) cycle in two kernels per level (vcycle.hip) and the coarsest
+// solve between them.  Partial rows: |r_new|^2 in row 3 (the convergence
+// test, folded by level 1's down kernel), the flexible-CG dots in rows 0-1,
+// each level's restricted right-hand side's fluid sum in its slotB row, z's
+// in kSlotMeanZ's.  Launches: (levels - 1) down, 1 coarsest, (levels - 1) up.
+template <class T>
+void PoissonDev::tiledCycle(hipStream_t s, int k, bool first, const T* b0, const double* r, double* rn,
+                            const double* p, InFold alpha) {
+  SolveCtl* ctl = ctl_.as<SolveCtl>();
+  const int nl = nlevels();
+  const RedBuf rb = red();
+  double* rowNorm = rb.partials + 3 * size_t(rb.stride);
+  auto meanOf = [&](int slot, int producerBlocks, long long nf) {
+    if (!project_) return MeanRef{};
+    MeanRef m;
+    m.parts = slotRow(slot);
+    m.n = producerBlocks;
+    m.nf = double(nf);
+    return m;
+  };
+  // level 0 down
+  DownArgs<T> d0;
+  d0.L = dl_[0];
+  d0.Lc = dl_[1];
+  d0.ctl = ctl;
+  d0.xa = lx0_[0].as<T>();
+  d0.bc = lb_[1].as<T>();
+  d0.rowMean = project_ ? slotRow(slotB(1)) : nullptr;
+  int src = DS_B;
+  if (first) {
+    d0.b = b0;
+  } else {
+    d0.rnOut = rn;
+    d0.rowNorm = rowNorm;
+    if constexpr (std::is_same_v<T, float>) d0.b0Out = b0f_.as<float>();
+    if (project_) {
+      src = DS_RN;
+      d0.rnRaw = rnRaw_.as<double>();
+      d0.mrn = meanOf(kSlotMeanRn, blocks_[0], dl_[0].nf);
+    } else {
+      src = DS_UPD;
+      d0.r = r;
+      d0.Ap = Ap_.as<double>();
+      d0.x = x_.as<double>();
+      d0.p = p;
+      d0.alpha = alpha;
+    }
+  }
+  launch_down<T>(src, d0, s);
+  // levels 1 .. nl-2 down; level 1 carries the convergence test
+  for (int l = 1; l + 1 < nl; ++l) {
+    DownArgs<T> d;
+    d.L = dl_[l];
+    d.Lc = dl_[l + 1];
+    d.ctl = ctl;
+    d.b = lb_[l].as<T>();
+    d.mb = meanOf(slotB(l), tblocks_[l - 1], dl_[l].nf);
+    d.xa = lx0_[l].as<T>();
+    d.bc = lb_[l + 1].as<T>();
+    d.rowMean = project_ ? slotRow(slotB(l + 1)) : nullptr;
+    if (l == 1 && !first) d.conv = InFold{rowNorm, rb.stride, tblocks_[0], k};
+    launch_down<T>(DS_B, d, s);
+  }
+  const int lc = nl - 1;
+  launch_mg_coarse_direct<T>(dl_[lc], coarseM_.as<double>(), coarseMf_.as<float>(), lb_[lc].as<T>(),
+                             meanOf(slotB(lc), tblocks_[lc - 1], dl_[lc].nf), lx0_[lc].as<T>(), ctl, s);
+  const T* e = lx0_[lc].as<T>();
+  for (int l = nl - 2; l >= 1; --l) {
+    UpArgs<T> u;
+    u.L = dl_[l];
+    u.Lc = dl_[l + 1];
+    u.b = lb_[l].as<T>();
+    u.mb = meanOf(slotB(l), tblocks_[l - 1], dl_[l].nf);
+    u.xa = lx0_[l].as<T>();
+    u.e = e;
+    u.xb = lx1_[l].as<T>();
+    u.ctl = ctl;
+    launch_up<T>(UD_X, u, s);
+    e = lx1_[l].as<T>();
+  }
+  UpArgs<T> u0;
+  u0.L = dl_[0];
+  u0.Lc = dl_[1];
+  u0.b = b0;
+  u0.xa = lx0_[0].as<T>();
+  u0.e = e;
+  u0.z = z_.as<double>();
+  u0.ctl = ctl;
+  if (!first && !project_) {
+    u0.rn = rn;
+    u0.r = r;
+    u0.rb = rb;
+    u0.countIt = 1;
+    launch_up<T>(UD_DOTS, u0, s);
+  } else {
+    u0.rb = project_ ? RedBuf{slotRow(kSlotMeanZ), rb.stride} : RedBuf{nullptr, 0};
+    launch_up<T>(UD_Z, u0, s);
+  }
+}
+
+// One flexible-CG iteration (multigrid.cpp:309-340) around the tiled cycle:
+// A p (with p = z + beta p of the previous iteration and beta folded in), the
+// cycle (whose level-0 down kernel performs the update with alpha folded in,
+// and whose level-1 down kernel the convergence test), and on singular
+// systems the update writing r_new's fluid sums first and the dots after.
+void PoissonDev::enqueueIterationTiled(hipStream_t s, int k) {
+  SolveCtl* ctl = ctl_.as<SolveCtl>();
+  const MgLevel& L = dl_[0];
+  const RedBuf rb = red();
+  const int nb = blocks_[0];
+  const RedBuf rbP{rb.partials + 2 * size_t(rb.stride), rb.stride};
+  double* x = x_.as<double>();
+  double* z = z_.as<double>();
+  double* Ap = Ap_.as<double>();
+  double* p = k == 0 ? p_.as<double>() : ((k & 1) ? p2_.as<double>() : p_.as<double>());
+  if (k == 0) {
+    launch_fcg_apply(L, p, Ap, ctl, rbP, s);
+  } else {
+    const double* pOld = (k & 1) ? p_.as<double>() : p2_.as<double>();
+    // the dots come from the level-0 up kernel, or on singular systems from k_fcg_dots
+    launch_fcg_apply_pu(L, z, pOld, p, Ap, ctl, rbP, s, InFold{rb.partials, rb.stride, project_ ? nb : tblocks_[0], k});
+  }
+  const InFold alpha{rbP.partials, rbP.stride, nb, k};
+  double* r = (k & 1) ? rn_.as<double>() : r_.as<double>();
+  double* rn = (k & 1) ? r_.as<double>() : rn_.as<double>();
+  if (project_)
+    launch_fcg_update(L, true, x, p, r, Ap, rnRaw_.as<double>(), ctl, meanRed(kSlotMeanRn, true), s, alpha);
+  if (mixed_) tiledCycle<float>(s, k, false, b0f_.as<float>(), r, rn, p, alpha);
+  else tiledCycle<double>(s, k, false, rn, r, rn, p, alpha);
+  if (project_) launch_fcg_dots(L, z, zMeanTiled(), rn, r, ctl, rb, s, true);
+  ++vcount_;
 }
 
 // ---------------------------------------------------------------------------

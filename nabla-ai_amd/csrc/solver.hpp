@@ -126,6 +126,20 @@ class PoissonDev {
   MeanRef zMean() const;  // pending mean of the preconditioner output z
   void meanFin(int slot, bool inl, int blocks, long long nf, bool predicated, hipStream_t s);
   void enqueueIteration(hipStream_t s, int k);
+  // The V(2,2) cycle in two kernels per level (vcycle.hip).  first: the
+  // cycle applied to b0 (the first of a solve, precondition); otherwise
+  // iteration k's, whose level-0 right-hand side is r_new of the update
+  // (formed by the down kernel from r, A p and alpha; on singular systems read
+  // from rnRaw_ with its pending mean).  b0: the level-0 right-hand side as
+  // the up kernel reads it.
+  template <class T>
+  void tiledCycle(hipStream_t s, int k, bool first, const T* b0, const double* r, double* rn, const double* p,
+                  InFold alpha);
+  void enqueueIterationTiled(hipStream_t s, int k);
+  double* slotRow(int slot) const {
+    return red_.as<double>() + size_t(kMaxRedVals + slot) * size_t(redStride_);
+  }
+  MeanRef zMeanTiled() const;
   // The reference's loop structure, one kernel per loop (opts.loops).
   void removeMeanLoops(int l, double* x, hipStream_t s);
   void smoothLoops(int l, const double* b, double* x, int sweeps, hipStream_t s);
@@ -169,6 +183,9 @@ class PoissonDev {
   bool inline_ = false;
   InFold conv_{};
   bool loops_ = false;
+  bool tiled_ = false;       // the two-kernels-per-level cycle is in use
+  std::vector<int> tblocks_;  // workgroups of the tiled kernels per level
+  DevBuf rnRaw_;             // singular tiled path: r_new before its projection
   std::vector<DevBuf> ldiag_;  // loops: per-level diagonal (CgPoisson::buildDiagonal)
   DevBuf ctl_, red_;
   int redStride_ = 1;

@@ -1,0 +1,158 @@
+"""The V(2,2) cycle in two kernels per level (LDS tiles with recomputed halos)
+against the one-kernel-per-sweep cycle and the oracle, through the C-ABI.
+
+* Systems without null-space projections: every cell value of the tiled cycle
+  is the per-sweep kernels' expression in their order, so one preconditioner
+  application is bit-identical; solves differ only through the partial-sum
+  tree of the flexible CG's reductions (same iteration counts, agreement far
+  inside the tolerance).
+* Singular systems: the tiled cycle folds the per-sweep null-space projections
+  into the ones that change values (exact in exact arithmetic), so results
+  agree to rounding.
+Grids have at least three multigrid levels (the tiled cycle's domain): 3-D and
+2-D, odd sizes, solids, inlets and outlets, periodic boxes.
+"""
+import numpy as np
+import pytest
+
+import cases
+import nbl
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def grids():
+    out = {}
+    out["tgv32"] = cases.taylor_green(32)[0]
+    out["cube96"] = cases.cube_channel(nx=96, ny=32, nz=32, res=8)[0]
+    g, _ = cases.lid_cavity(128, 100.0)
+    out["lid128"] = g
+    faces = [dict(type=nbl.BC_INLET, inlet_speed=1.0), dict(type=nbl.BC_OUTLET), dict(type=nbl.BC_WALL),
+             dict(type=nbl.BC_WALL), dict(type=nbl.BC_WALL), dict(type=nbl.BC_OUTLET)]
+    solid = np.zeros((20, 34, 50), np.uint8)
+    solid[5:12, 0:9, 14:23] = 1
+    out["odd50"] = nbl.Grid(50, 34, 20, 0.1, 0.12, 0.15, faces=faces, solid=solid)
+    walls = nbl.Grid(40, 24, 36, 1.0 / 40, faces=[dict(type=nbl.BC_WALL)] * 6)
+    out["walls40"] = walls
+    return out
+
+
+GRIDS = grids()
+SINGULAR = {"tgv32", "lid128", "walls40"}
+
+
+def solver(ctx, g, per_sweep, precision=nbl.PREC_FP64, tol=1e-10):
+    return nbl.Poisson(ctx, g, kind=0, tol=tol, max_iters=200,
+                       opts=nbl.solver_opts(precision=precision,
+                                            vcycle=nbl.VCYCLE_PER_SWEEP if per_sweep else nbl.VCYCLE_TILED))
+
+
+@pytest.mark.parametrize("name", sorted(GRIDS))
+def test_grids_have_three_levels(ctx, name):
+    ps = solver(ctx, GRIDS[name], False)
+    try:
+        assert len(ps.levels()) >= 3
+    finally:
+        ps.close()
+
+
+@pytest.mark.parametrize("precision", [nbl.PREC_FP64, nbl.PREC_MIXED], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("name", sorted(set(GRIDS) - SINGULAR))
+def test_tiled_preconditioner_bitwise(ctx, name, precision):
+    g = GRIDS[name]
+    r = cases.poisson_rhs(g, 41)
+    zs = []
+    for per_sweep in (True, False):
+        ps = solver(ctx, g, per_sweep, precision)
+        try:
+            zs.append(ps.precondition(r))
+        finally:
+            ps.close()
+    assert np.array_equal(zs[0], zs[1])
+
+
+@pytest.mark.parametrize("precision", [nbl.PREC_FP64, nbl.PREC_MIXED], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("name", sorted(GRIDS))
+def test_tiled_solve_matches_per_sweep_and_oracle(ctx, orc, name, precision):
+    g = GRIDS[name]
+    b = cases.poisson_rhs(g, 43)
+    out = []
+    for per_sweep in (True, False):
+        ps = solver(ctx, g, per_sweep, precision)
+        try:
+            out.append(ps.solve(b))
+        finally:
+            ps.close()
+    (x1, r1), (x2, r2) = out
+    print(f"{name}: iterations per-sweep {r1.iterations}, tiled {r2.iterations}; rel {rel(x2, x1):.2e}", flush=True)
+    assert r1.converged and r2.converged and r2.residual <= 1e-10 * 1.01
+    assert abs(r1.iterations - r2.iterations) <= (1 if name in SINGULAR else 0)
+    assert rel(x2, x1) < 1e-8
+    xo, ro = orc.mg(g, b, 1e-10, 200)
+    assert abs(r2.iterations - ro.iterations) <= (2 if precision == nbl.PREC_MIXED else 1)
+    assert rel(x2, xo) < 1e-8
+
+
+@pytest.mark.parametrize("name", ["tgv32", "cube96"])
+def test_tiled_solves_are_deterministic(ctx, name):
+    g = GRIDS[name]
+    b = cases.poisson_rhs(g, 47)
+    ps = solver(ctx, g, False)
+    try:
+        xs = [ps.solve(b) for _ in range(4)]
+    finally:
+        ps.close()
+    for x, r in xs[1:]:
+        assert np.array_equal(x, xs[0][0]) and r.iterations == xs[0][1].iterations
+
+
+def test_tiled_steps_track_per_sweep(ctx):
+    """Cube channel 96x32x32 (three levels), WENO-5: five steps with either cycle."""
+    g, cfg = cases.cube_channel(nx=96, ny=32, nz=32, res=8)
+    st = cases.cube_channel_state(g)
+    res = []
+    for per_sweep in (True, False):
+        run = nbl.Run(ctx, g, cfg, nbl.solver_opts(vcycle=nbl.VCYCLE_PER_SWEEP if per_sweep else nbl.VCYCLE_TILED))
+        try:
+            run.set_state(*st)
+            reps = [run.step() for _ in range(5)]
+            res.append((reps, run.get_state()))
+        finally:
+            run.close()
+    (ra, sa), (rb, sb) = res
+    for a, b in zip(ra, rb):
+        assert a.poisson_iters == b.poisson_iters
+        assert b.cd == pytest.approx(a.cd, rel=1e-10)
+    for fa, fb in zip(sa[:4], sb[:4]):
+        assert rel(fb, fa) < 1e-11
+
+
+def test_tiled_taylor_green_steps_track_per_sweep(ctx):
+    """64^3 periodic Taylor-Green vortex (singular, the automatic choice is the
+    tiled cycle): ten steps with either cycle agree to rounding."""
+    g, cfg = cases.taylor_green(64)
+    st = cases.taylor_green_state(g)
+    res = []
+    for vc in (nbl.VCYCLE_PER_SWEEP, nbl.VCYCLE_AUTO):
+        run = nbl.Run(ctx, g, cfg, nbl.solver_opts(vcycle=vc))
+        try:
+            run.set_state(*st)
+            reps = [run.step() for _ in range(10)]
+            res.append((reps, run.get_state()))
+        finally:
+            run.close()
+    (ra, sa), (rb, sb) = res
+    for a, b in zip(ra, rb):
+        assert abs(a.poisson_iters - b.poisson_iters) <= 1
+    for fa, fb in zip(sa[:4], sb[:4]):
+        assert rel(fb, fa) < 1e-10
+
+
+def test_vcycle_option_validated(ctx):
+    g = GRIDS["tgv32"]
+    with pytest.raises(nbl.NblError):
+        nbl.Poisson(ctx, g, kind=0, opts=nbl.solver_opts(vcycle=3))
