@@ -191,6 +191,7 @@ def octree_c3(ctx, with_cpu=True):
         import oracle_py
         kind = "reference" if oracle_py.available("ref") else "port"
         lib = oracle_py.OracleLib("ref" if kind == "reference" else "orc")
+        core = OneCore().__enter__()
         rt = lib.octree(cases.C3_ORIGIN, cases.C3_EXTENT, 8)
         rt.refine_uniform(5)
         while True:
@@ -217,6 +218,7 @@ def octree_c3(ctx, with_cpu=True):
         rt.sensors(xr, xr, z, xr, 1e-3, z, z)
         sensors_s = time.perf_counter() - t1
         rt.close()
+        core.__exit__(None, None, None)
         out["cpu_baseline"] = {"kind": kind, "cores": 1, "connectivity_s": round(conn_s, 3),
                                "applyA_s": round(apply_s, 3), "transport_step_s": round(transport_s, 3),
                                "sensors_s": round(sensors_s, 3),
@@ -225,6 +227,7 @@ def octree_c3(ctx, with_cpu=True):
                                          "octreeTransportStep and one PhysicalSensorEvaluator::evaluate on the "
                                          "same tree; est_solve_s = applyA_s x the "
                                          "device Jacobi-PCG iteration count"}
+        out["cpu_baseline"].update(core.record())
     return out
 
 
@@ -298,16 +301,18 @@ def batched(ctx, with_cpu=True, seed=4242, nuni=48, noct=8, concurrency=4):
         kind = "reference" if oracle_py.available("ref") else "port"
         lib = oracle_py.OracleLib("ref" if kind == "reference" else "orc")
         sample = [q for q, n in enumerate(sizes) if n in (32, 64)][:6]
-        t0 = time.perf_counter()
-        cpu_cells = 0
-        for q in sample:
-            lib.mg(grids[q], rhs[q], 1e-8, 200)
-            cpu_cells += grids[q].cells
-        el = time.perf_counter() - t0
+        with OneCore() as core:
+            t0 = time.perf_counter()
+            cpu_cells = 0
+            for q in sample:
+                lib.mg(grids[q], rhs[q], 1e-8, 200)
+                cpu_cells += grids[q].cells
+            el = time.perf_counter() - t0
         out["cpu_baseline"] = {"kind": kind, "cores": 1, "sample_problems": len(sample),
                                "sample_cells": cpu_cells, "seconds": round(el, 3),
                                "mcells_solved_per_s": round(cpu_cells / el / 1e6, 4),
                                "sample": "MgPoisson::solve to 1e-8 on the batch's 32^3 and 64^3 problems (up to 6)"}
+        out["cpu_baseline"].update(core.record())
     return out
 
 
@@ -416,20 +421,59 @@ def tgv_c1(ctx, a, steps=100):
             res[name]["rel_l2_vs_fp64"] = {k: rel(x, y) for k, x, y in zip("uvwp", stt[:4], state[:4])}
         log(f"tgv64 {name}: {res[name]['value']} Mcell-steps/s")
     if not a.no_cpu and a.cpu_steps > 0:
-        res["cpu_baseline"] = cpu_baseline(g, cfg, st, 1)
+        res["cpu_baseline"] = cpu_baseline(g, cfg, st, 8)
     return res
+
+
+def host_info():
+    """The GPU host's CPU as the CPU baselines record it: model, logical CPUs
+    (nproc) and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = sorted(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = []
+    return {"cpu_model": model, "nproc": os.cpu_count(), "allowed_cpus": len(allowed)}
+
+
+class OneCore:
+    """Pin this process to one CPU (the lowest it may use) for a single-thread
+    timing, as `taskset -c <cpu>` would; the record names the CPU."""
+
+    def __enter__(self):
+        self.cpu = None
+        try:
+            self.old = os.sched_getaffinity(0)
+            self.cpu = min(self.old)
+            os.sched_setaffinity(0, {self.cpu})
+        except (AttributeError, OSError):
+            self.old = None
+        return self
+
+    def __exit__(self, *exc):
+        if self.old is not None:
+            os.sched_setaffinity(0, self.old)
+        return False
+
+    def record(self):
+        out = host_info()
+        out["affinity"] = [self.cpu] if self.cpu is not None else "unpinned"
+        return out
 
 
 def cpu_baseline(g, cfg, st, steps):
     import oracle_py
     kind = "reference" if oracle_py.available("ref") else "port"
     lib = oracle_py.OracleLib("ref" if kind == "reference" else "orc")
-    try:
-        old = os.sched_getaffinity(0)
-        os.sched_setaffinity(0, {min(old)})
-    except (AttributeError, OSError):
-        old = None
-    try:
+    with OneCore() as core:
         s = lib.ns(g, cfg)
         s.set_state(*st)
         t0 = time.perf_counter()
@@ -438,12 +482,11 @@ def cpu_baseline(g, cfg, st, steps):
             log(f"cpu baseline step {q + 1}/{steps}")
         el = time.perf_counter() - t0
         s.close()
-    finally:
-        if old is not None:
-            os.sched_setaffinity(0, old)
-    return {"value": round(g.cells * steps / el / 1e6, 5), "unit": "Mcell-steps/s", "cores": 1,
-            "kind": kind, "seconds_per_step": round(el / steps, 3),
-            "sample": f"{steps} steps of the same workload from the same initial state, 1 thread"}
+    out = {"value": round(g.cells * steps / el / 1e6, 5), "unit": "Mcell-steps/s", "cores": 1,
+           "kind": kind, "seconds_per_step": round(el / steps, 3),
+           "sample": f"{steps} steps of the same workload from the same initial state, 1 thread"}
+    out.update(core.record())
+    return out
 
 
 def main():
@@ -451,7 +494,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-octree", action="store_true")

@@ -27,6 +27,12 @@ struct HipError : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
 
+// A request beyond a fixed capacity (an HBM budget, a 32-bit index range):
+// NBL_ERR_CAPACITY at the C-ABI.
+struct CapacityError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
 #define NBL_HIP(call)                                                                  \
   do {                                                                                 \
     hipError_t e_ = (call);                                                            \

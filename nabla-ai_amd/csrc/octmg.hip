@@ -23,7 +23,11 @@
 // axes, slot-major operator), up[i] = parent of cell i in the next coarser
 // level, and on coarse levels cstart[I] .. cstart[I+1] indexing child[], the
 // cells of the next finer level that form cell I (increasing index).
-#include <hipcub/hipcub.hpp>
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_run_length_encode.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_reduce.hpp>
 
 #include <algorithm>
 #include <stdexcept>
@@ -298,7 +302,7 @@ OctMg::OctMg(nbl_ctx* ctx, const OctreeDev& fine, hipStream_t s) : ctx_(ctx) {
 }
 
 namespace {
-// hipcub helpers: query the temporary size, allocate, run.
+// rocPRIM helpers: query the temporary size, allocate, run.
 template <class F>
 void cub(nbl_ctx* ctx, F&& f) {
   size_t tb = 0;
@@ -355,23 +359,22 @@ void OctMg::buildCoarse(const OctreeDev& fine, hipStream_t s) {
                                                 tkey.as<uint64_t>(), vol.as<uint64_t>(), idx.as<int32_t>());
       NBL_LAUNCH_CHECK();
       cub(ctx_, [&](void* t, size_t& tb) {
-        return hipcub::DeviceRadixSort::SortPairs(t, tb, tkey.as<uint64_t>(), tkeyS.as<uint64_t>(), idx.as<int32_t>(),
+        return rocprim::radix_sort_pairs(t, tb, tkey.as<uint64_t>(), tkeyS.as<uint64_t>(), idx.as<int32_t>(),
                                                   idxS.as<int32_t>(), ni, 0, 64, s);
       });
       cub(ctx_, [&](void* t, size_t& tb) {
-        return hipcub::DeviceRunLengthEncode::Encode(t, tb, tkeyS.as<uint64_t>(), ukey.as<uint64_t>(),
-                                                     len.as<int32_t>(), nrun.as<int32_t>(), ni, s);
+        return rocprim::run_length_encode(t, tb, tkeyS.as<uint64_t>(), (unsigned)(ni), ukey.as<uint64_t>(), len.as<int32_t>(), nrun.as<int32_t>(), s);
       });
       const int runs = readBack(nrun.as<int32_t>(), s);
       cub(ctx_, [&](void* t, size_t& tb) {
-        return hipcub::DeviceScan::ExclusiveSum(t, tb, len.as<int32_t>(), off.as<int32_t>(), runs, s);
+        return rocprim::exclusive_scan(t, tb, len.as<int32_t>(), off.as<int32_t>(), int32_t(0), size_t(runs), rocprim::plus<int32_t>(), s);
       });
       k_sc_complete<<<gridFor(runs), kBlock, 0, s>>>(runs, ukey.as<uint64_t>(), len.as<int32_t>(), off.as<int32_t>(),
                                                      idxS.as<int32_t>(), vol.as<uint64_t>(),
                                                      1ull << (T[0] + T[1] + T[2]), cmpl.as<int32_t>());
       NBL_LAUNCH_CHECK();
       cub(ctx_, [&](void* t, size_t& tb) {
-        return hipcub::DeviceReduce::Sum(t, tb, cmpl.as<int32_t>(), ncmpl.as<int32_t>(), runs, s);
+        return rocprim::reduce(t, tb, cmpl.as<int32_t>(), ncmpl.as<int32_t>(), int32_t(0), size_t(runs), rocprim::plus<int32_t>(), s);
       });
       if (readBack(ncmpl.as<int32_t>(), s) == 0) continue;  // nothing merges at this size: grow further
       const uint32_t Sp = uint32_t(T[0]) | (uint32_t(T[1]) << 8) | (uint32_t(T[2]) << 16);
@@ -390,13 +393,13 @@ void OctMg::buildCoarse(const OctreeDev& fine, hipStream_t s) {
     k_iota32<<<gridFor(n), kBlock, 0, s>>>(n, idx.as<int32_t>());
     NBL_LAUNCH_CHECK();
     cub(ctx_, [&](void* t, size_t& tb) {
-      return hipcub::DeviceRadixSort::SortPairs(t, tb, ckey.as<uint64_t>(), ckeyS.as<uint64_t>(), idx.as<int32_t>(),
+      return rocprim::radix_sort_pairs(t, tb, ckey.as<uint64_t>(), ckeyS.as<uint64_t>(), idx.as<int32_t>(),
                                                 C.child.as<int32_t>(), ni, 0, 64, s);
     });
     k_sc_heads<<<gridFor(n), kBlock, 0, s>>>(n, ckeyS.as<uint64_t>(), head.as<int32_t>());
     NBL_LAUNCH_CHECK();
     cub(ctx_, [&](void* t, size_t& tb) {
-      return hipcub::DeviceScan::InclusiveSum(t, tb, head.as<int32_t>(), pid.as<int32_t>(), ni, s);
+      return rocprim::inclusive_scan(t, tb, head.as<int32_t>(), pid.as<int32_t>(), size_t(ni), rocprim::plus<int32_t>(), s);
     });
     const int nc = readBack(pid.as<int32_t>() + (n - 1), s);
     if (nc <= 0 || nc >= n) break;
@@ -426,18 +429,17 @@ void OctMg::buildCoarse(const OctreeDev& fine, hipStream_t s) {
                                               ekey.as<uint64_t>(), eid.as<int32_t>());
       NBL_LAUNCH_CHECK();
       cub(ctx_, [&](void* t, size_t& tb) {
-        return hipcub::DeviceRadixSort::SortPairs(t, tb, ekey.as<uint64_t>(), ekeyS.as<uint64_t>(), eid.as<int32_t>(),
+        return rocprim::radix_sort_pairs(t, tb, ekey.as<uint64_t>(), ekeyS.as<uint64_t>(), eid.as<int32_t>(),
                                                   eidS.as<int32_t>(), nei, 0, 64, s);
       });
       cub(ctx_, [&](void* t, size_t& tb) {
-        return hipcub::DeviceRunLengthEncode::Encode(t, tb, ekeyS.as<uint64_t>(), uk.as<uint64_t>(), ulen.as<int32_t>(),
-                                                     nrun.as<int32_t>(), nei, s);
+        return rocprim::run_length_encode(t, tb, ekeyS.as<uint64_t>(), (unsigned)(nei), uk.as<uint64_t>(), ulen.as<int32_t>(), nrun.as<int32_t>(), s);
       });
       nu = readBack(nrun.as<int32_t>(), s);
       if (nu > 0 && readBack(uk.as<uint64_t>() + (nu - 1), s) == ~0ull) --nu;  // the sentinel run sorts last
       if (nu > 0)
         cub(ctx_, [&](void* t, size_t& tb) {
-          return hipcub::DeviceScan::ExclusiveSum(t, tb, ulen.as<int32_t>(), uoff.as<int32_t>(), nu, s);
+          return rocprim::exclusive_scan(t, tb, ulen.as<int32_t>(), uoff.as<int32_t>(), int32_t(0), size_t(nu), rocprim::plus<int32_t>(), s);
         });
     }
     co.nnz = size_t(nu);
@@ -447,7 +449,7 @@ void OctMg::buildCoarse(const OctreeDev& fine, hipStream_t s) {
     if (nu > 0) k_sc_rowcount<<<gridFor(nu), kBlock, 0, s>>>(nu, uk.as<uint64_t>(), rc.as<int32_t>());
     NBL_LAUNCH_CHECK();
     cub(ctx_, [&](void* t, size_t& tb) {
-      return hipcub::DeviceScan::ExclusiveSum(t, tb, rc.as<int32_t>(), co.rowPtr.as<int32_t>(), nc + 1, s);
+      return rocprim::exclusive_scan(t, tb, rc.as<int32_t>(), co.rowPtr.as<int32_t>(), int32_t(0), size_t(nc + 1), rocprim::plus<int32_t>(), s);
     });
     co.col = DevBuf(ctx_, std::max<size_t>(1, co.nnz) * 4);
     co.coef = DevBuf(ctx_, std::max<size_t>(1, co.nnz) * 8);

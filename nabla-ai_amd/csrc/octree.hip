@@ -12,7 +12,9 @@
 //     from the reference's centre/size formulas, plus the diagonal.
 // OctreePoisson::applyA is then one gather pass over the CSR, bit-identical to
 // the reference per element.
-#include <hipcub/hipcub.hpp>
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cstring>
@@ -749,10 +751,9 @@ void buildOctSlots(nbl_ctx* ctx, OctLevel& L, hipStream_t s) {
   k_slot_count<<<gridFor(N), kBlock, 0, s>>>(N, L.rowPtr.as<int32_t>(), L.cnt.as<uint8_t>(), xcount.as<int32_t>());
   NBL_LAUNCH_CHECK();
   size_t tb = 0;
-  NBL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, xcount.as<int32_t>(), L.xrow.as<int32_t>(), int(n + 1), s));
+  NBL_HIP(rocprim::exclusive_scan(nullptr, tb, xcount.as<int32_t>(), L.xrow.as<int32_t>(), int32_t(0), size_t(int(n + 1)), rocprim::plus<int32_t>(), s));
   DevBuf tmp(ctx, tb);
-  NBL_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.as<void>(), tb, xcount.as<int32_t>(), L.xrow.as<int32_t>(),
-                                           int(n + 1), s));
+  NBL_HIP(rocprim::exclusive_scan(tmp.as<void>(), tb, xcount.as<int32_t>(), L.xrow.as<int32_t>(), int32_t(0), size_t(int(n + 1)), rocprim::plus<int32_t>(), s));
   int32_t nxtra = 0;
   NBL_HIP(hipMemcpyAsync(&nxtra, L.xrow.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   NBL_HIP(hipStreamSynchronize(s));
@@ -784,11 +785,9 @@ void buildOctOperator(nbl_ctx* ctx, int maxL, const double org[3], const double 
   k_count_fluid<<<ogrid(N), kBlock, 0, s>>>(N, L.mask.as<uint8_t>(), nfl.as<unsigned long long>());
   NBL_LAUNCH_CHECK();
   size_t scanBytes = 0;
-  NBL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, rowCount.as<int32_t>(), L.rowPtr.as<int32_t>(),
-                                           int(n + 1), s));
+  NBL_HIP(rocprim::exclusive_scan(nullptr, scanBytes, rowCount.as<int32_t>(), L.rowPtr.as<int32_t>(), int32_t(0), size_t(int(n + 1)), rocprim::plus<int32_t>(), s));
   DevBuf scanTmp(ctx, scanBytes);
-  NBL_HIP(hipcub::DeviceScan::ExclusiveSum(scanTmp.as<void>(), scanBytes, rowCount.as<int32_t>(),
-                                           L.rowPtr.as<int32_t>(), int(n + 1), s));
+  NBL_HIP(rocprim::exclusive_scan(scanTmp.as<void>(), scanBytes, rowCount.as<int32_t>(), L.rowPtr.as<int32_t>(), int32_t(0), size_t(int(n + 1)), rocprim::plus<int32_t>(), s));
   int32_t nnz = 0;
   unsigned long long nf = 0;
   NBL_HIP(hipMemcpyAsync(&nnz, L.rowPtr.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -829,7 +828,12 @@ OctreeDev::OctreeDev(nbl_ctx* ctx, const uint64_t* keys, const uint8_t* levels, 
   if (n == 0) throw std::invalid_argument("octree: no leaves");
   if (maxLevel < 1 || maxLevel > 21) throw std::invalid_argument("Octree: maxLevel must be in [1, 21]");
   if (!(extent[0] > 0 && extent[1] > 0 && extent[2] > 0)) throw std::invalid_argument("Octree: extent must be positive");
-  if (n >= (size_t(1) << 31)) throw std::invalid_argument("octree: too many leaves");
+  // The face table holds 24 entries per leaf and the coupling CSR up to 24 per
+  // leaf, indexed in 32 bits: refuse trees whose 24 n would overflow them
+  // before anything is allocated or launched.
+  if (n > size_t(kMaxOctreeLeaves))
+    throw CapacityError("octree: " + std::to_string(n) + " leaves exceed the 32-bit face-table limit of " +
+                        std::to_string(kMaxOctreeLeaves));
   const double inv = 1.0 / double(1u << maxLevel);
   for (int a = 0; a < 3; ++a) {
     org_[a] = origin[a];
@@ -853,10 +857,10 @@ OctreeDev::OctreeDev(nbl_ctx* ctx, const uint64_t* keys, const uint8_t* levels, 
   k_iota<<<gridFor(N), kBlock, 0, s>>>(N, idx.as<uint64_t>());
   NBL_LAUNCH_CHECK();
   size_t tmpBytes = 0;
-  NBL_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpBytes, kin.as<uint64_t>(), L0_.keys.as<uint64_t>(),
+  NBL_HIP(rocprim::radix_sort_pairs(nullptr, tmpBytes, kin.as<uint64_t>(), L0_.keys.as<uint64_t>(),
                                              idx.as<uint64_t>(), perm_.as<uint64_t>(), int(n), 0, 64, s));
   DevBuf tmp(ctx, tmpBytes);
-  NBL_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.as<void>(), tmpBytes, kin.as<uint64_t>(), L0_.keys.as<uint64_t>(),
+  NBL_HIP(rocprim::radix_sort_pairs(tmp.as<void>(), tmpBytes, kin.as<uint64_t>(), L0_.keys.as<uint64_t>(),
                                              idx.as<uint64_t>(), perm_.as<uint64_t>(), int(n), 0, 64, s));
   k_permute<uint8_t><<<gridFor(N), kBlock, 0, s>>>(N, perm_.as<uint64_t>(), lin.as<uint8_t>(), L0_.lev.as<uint8_t>());
   k_permute<uint8_t><<<gridFor(N), kBlock, 0, s>>>(N, perm_.as<uint64_t>(), min_.as<uint8_t>(), L0_.mask.as<uint8_t>());
@@ -880,10 +884,9 @@ OctreeDev::OctreeDev(nbl_ctx* ctx, const uint64_t* keys, const uint8_t* levels, 
     k_tab_count<<<gridFor(N), kBlock, 0, s>>>(N, cnt_.as<uint8_t>(), extra.as<int32_t>());
     NBL_LAUNCH_CHECK();
     size_t tb = 0;
-    NBL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, extra.as<int32_t>(), txrow_.as<int32_t>(), int(n + 1), s));
+    NBL_HIP(rocprim::exclusive_scan(nullptr, tb, extra.as<int32_t>(), txrow_.as<int32_t>(), int32_t(0), size_t(int(n + 1)), rocprim::plus<int32_t>(), s));
     DevBuf tmp2(ctx, tb);
-    NBL_HIP(hipcub::DeviceScan::ExclusiveSum(tmp2.as<void>(), tb, extra.as<int32_t>(), txrow_.as<int32_t>(),
-                                             int(n + 1), s));
+    NBL_HIP(rocprim::exclusive_scan(tmp2.as<void>(), tb, extra.as<int32_t>(), txrow_.as<int32_t>(), int32_t(0), size_t(int(n + 1)), rocprim::plus<int32_t>(), s));
     int32_t nx = 0;
     NBL_HIP(hipMemcpyAsync(&nx, txrow_.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     NBL_HIP(hipStreamSynchronize(s));

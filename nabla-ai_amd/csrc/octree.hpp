@@ -28,6 +28,9 @@ struct OctLevel {
 };
 
 constexpr int kOctSlots = 6;
+// Leaves an octree may hold: its face table (24 entries per leaf) and its
+// coupling CSR (at most 24 per leaf) are indexed in 32 bits.
+constexpr long long kMaxOctreeLeaves = ((1LL << 31) - 1) / 24;
 
 // Morton code with 21 bits per axis: x at bit 3i, y at 3i+1, z at 3i+2
 // (morton.hpp:15-48).

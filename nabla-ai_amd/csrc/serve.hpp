@@ -20,10 +20,6 @@
 
 namespace nbl {
 
-struct CapacityError : std::runtime_error {
-  using std::runtime_error::runtime_error;
-};
-
 class Server {
  public:
   Server(nbl_ctx* ctx, int slots, uint64_t budget);

@@ -160,6 +160,8 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
     throw std::invalid_argument("pre/post smoothing sweeps must be >= 1");
   if (opts.precision < NBL_PREC_FP64 || opts.precision > NBL_PREC_MIXED_BF16)
     throw std::invalid_argument("unknown precision");
+  if (opts.precision == NBL_PREC_MIXED_BF16)
+    throw std::invalid_argument("precision NBL_PREC_MIXED_BF16 (BF16 smoother scratch) is not implemented");
   singular_ = !hasOutlet(g);
   mixed_ = kind == 0 && opts.precision != NBL_PREC_FP64;
   if (opts.loops != 0 && opts.loops != 1) throw std::invalid_argument("loops must be 0 or 1");

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <cstddef>
+#include <cstdint>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -37,12 +38,23 @@ struct RunCancelled : std::runtime_error {
   explicit RunCancelled(int step) : std::runtime_error("cancelled after step " + std::to_string(step)) {}
 };
 
+// Initial state of the measured case types, applied through setState after
+// initialize(): the analytic Taylor-Green field, or the channel's uniform
+// stream with a seeded uniform perturbation of the interior u-faces.
+struct CaseInit {
+  enum class Kind { Default, TaylorGreen, Perturbed };
+  Kind kind = Kind::Default;
+  double amp = 0.0;
+  std::uint64_t seed = 0;
+};
+
 struct CaseSpec {
   std::string name = "case";
   UniformGrid grid;
   FlowConfig flow;
   RunOptions run;
   Precision precision = Precision::FP64;  // optional "precision" key
+  CaseInit init;                          // "taylor-green" / "cube-channel" initial states
 };
 
 struct RunSummary {

@@ -30,13 +30,8 @@ void allFaces(UniformGrid& g, BoundaryType t) {
   for (auto& f : g.face) f.type = t;
 }
 
-// Initial states of the measured case types (set through setState).
-enum class Init { Default, TaylorGreen, Perturbed };
-struct InitSpec {
-  Init kind = Init::Default;
-  double amp = 0.0;
-  std::uint64_t seed = 0;
-};
+using Init = CaseInit::Kind;
+using InitSpec = CaseInit;
 
 std::uint64_t splitmix(std::uint64_t& s) {
   std::uint64_t z = (s += 0x9E3779B97F4A7C15ull);
@@ -183,17 +178,12 @@ static CaseSpec makeCubeChannel(int nx, int ny, int nz, int res, double reynolds
   return s;
 }
 
-namespace {
-thread_local InitSpec g_init;  // init of the last loaded case (consumed by runCase)
-}
-
 CaseSpec loadCase(const std::string& jsonPath) {  // case.cpp:136-170
   const json::Value j = json::parseFile(jsonPath);
   const std::string type = j.string("type", "lid-cavity");
   const double Re = j.number("reynolds", 100.0);
   const int res = j.integer("resolution", 64);
   CaseSpec s;
-  g_init = InitSpec{};
   if (type == "lid-cavity") {
     s = makeLidCavity(res, Re, j.number("lid_velocity", 1.0));
   } else if (type == "channel" || type == "poiseuille") {
@@ -202,15 +192,15 @@ CaseSpec loadCase(const std::string& jsonPath) {  // case.cpp:136-170
     s = makeWallMountedCube(res, Re);
   } else if (type == "taylor-green") {
     s = makeTaylorGreen(res, Re);
-    g_init.kind = Init::TaylorGreen;
+    s.init.kind = Init::TaylorGreen;
   } else if (type == "cube-channel") {
     s = makeCubeChannel(j.integer("nx", 12 * res), j.integer("ny", 4 * res), j.integer("nz", 4 * res), res, Re);
-    g_init.kind = Init::Perturbed;
-    g_init.amp = 0.01;
-    g_init.seed = 20240601;
+    s.init.kind = Init::Perturbed;
+    s.init.amp = 0.01;
+    s.init.seed = 20240601;
     if (const json::Value* pz = j.find("perturbation")) {
-      g_init.amp = pz->number("amp", g_init.amp);
-      g_init.seed = std::uint64_t(pz->number("seed", double(g_init.seed)));
+      s.init.amp = pz->number("amp", s.init.amp);
+      s.init.seed = std::uint64_t(pz->number("seed", double(s.init.seed)));
     }
   } else {
     throw std::runtime_error("loadCase: unknown case type '" + type + "'");
@@ -225,7 +215,8 @@ CaseSpec loadCase(const std::string& jsonPath) {  // case.cpp:136-170
   s.flow.poissonMaxIters = j.integer("poisson_max_iters", s.flow.poissonMaxIters);
   const std::string prec = j.string("precision", "fp64");
   if (prec == "mixed") s.precision = Precision::Mixed;
-  else if (prec == "mixed-bf16") s.precision = Precision::MixedBF16;
+  else if (prec == "mixed-bf16")
+    throw std::runtime_error("loadCase: precision 'mixed-bf16' (BF16 smoother scratch) is not implemented; use 'mixed'");
   else if (prec != "fp64") throw std::runtime_error("loadCase: unknown precision '" + prec + "'");
   s.run.runDir = j.string("run_dir", "run_" + type);
   s.run.maxSteps = j.integer("max_steps", s.run.maxSteps);
@@ -359,7 +350,7 @@ void applyInit(NSSolver& s, const InitSpec& in) {
 }  // namespace
 
 RunSummary runCase(const CaseSpec& spec) {
-  const InitSpec init = g_init;
+  const InitSpec& init = spec.init;
   auto poisson = std::make_shared<HipMgPoisson>(spec.flow.poissonTol, spec.flow.poissonMaxIters, 2, 2, spec.precision);
   NSSolver solver(spec.grid, spec.flow, poisson);
   solver.initialize();
