@@ -88,8 +88,8 @@ __global__ void __launch_bounds__(kBlock) k_sweep(MgLevel L, const T* __restrict
     if (conv.n && inlineConv(conv, ctl)) return;
   }
   double acc[1] = {0.0};
+  T out[NC];
   if (q.valid) {
-    T out[NC];
 #pragma unroll
     for (int e = 0; e < NC; ++e) {
       if (!(code[e] & kFluidBit)) {
@@ -112,9 +112,10 @@ __global__ void __launch_bounds__(kBlock) k_sweep(MgLevel L, const T* __restrict
       out[e] = xnew;
       acc[0] += double(xnew);
     }
-    stN<NC>(xo, q.c, out);
   }
+  // stores after the fold's barriers, which would otherwise wait for them
   if (fold) blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
+  if (q.valid) stN<NC>(xo, q.c, out);
   if (stamps) {  // uniform across the grid: end stamp once every wave of the block is done
     __syncthreads();
     if (t0) stamps[2 * q.blk + 1] = wall_clock64();
@@ -189,15 +190,15 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc
   __syncthreads();
   double acc[1] = {0.0};
   const int fxT = NC == 2 ? 1 : Lc.fx;  // x-children held by one thread: 2 (NC = 2) or 1
+  int Cst = -1;  // the coarse cell this thread writes, stored after the fold
+  T rcv = T(0), x1v = T(0);
   if (tz == 0 && tx % fxT == 0 && ty % Lc.fy == 0) {
     const int I = i / Lc.fx, J = j / Lc.fy, K = k / Lc.fz;
     if (I < Lc.nx && J < Lc.ny && K < Lc.nz) {
       const int C = I + Lc.nx * (J + Lc.ny * K);
+      Cst = C;
       const uint16_t cc = Lc.code[C];
-      if (!(cc & kFluidBit)) {
-        rc[C] = T(0);
-        if (x1c) x1c[C] = T(0);
-      } else {
+      if (cc & kFluidBit) {
         T s = T(0);
         int cnt = 0;
         for (int dk = 0; dk < Lc.fz; ++dk)
@@ -211,13 +212,17 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc
               }
             }
         const T rv = s / T(cnt);
-        rc[C] = rv;
-        if (x1c) x1c[C] = firstSweep<T>(Lc, cc, C, I, J, K, rv);
+        rcv = rv;
+        if (x1c) x1v = firstSweep<T>(Lc, cc, C, I, J, K, rv);
         acc[0] = double(rv);
       }
     }
   }
   if (fold) blockToPartials<1>(acc, rb.partials, rb.stride, B.lin);
+  if (Cst >= 0) {
+    rc[Cst] = rcv;
+    if (x1c) x1c[Cst] = x1v;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -611,10 +616,10 @@ __global__ void __launch_bounds__(kBlock) k_finish(MgLevel L, const T* __restric
   if (stopOrMeans<1>(ctl, {mx}, mv)) return;
   const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz, L.slab);
   double acc[1] = {0.0};
+  double zv[NC];
   if (q.valid) {
     uint16_t code[NC];
     T xv[NC];
-    double zv[NC];
     ldCodes<NC>(L.code, q.c, code);
     ldN<NC>(x, q.c, xv);
 #pragma unroll
@@ -625,9 +630,9 @@ __global__ void __launch_bounds__(kBlock) k_finish(MgLevel L, const T* __restric
       zv[e] = fl ? double(v) : 0.0;
       acc[0] += zv[e];
     }
-    stN<NC>(z, q.c, zv);
   }
   if (fold) blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
+  if (q.valid) stN<NC>(z, q.c, zv);  // after the fold's barriers
 }
 
 template <class T>
@@ -896,14 +901,14 @@ __global__ void __launch_bounds__(kBlock) k_fcg_apply(MgLevel L, const double* _
   if (ldA(&ctl->done)) return;
   NBL_CELLS(NC);
   double acc[1] = {0.0};
+  double pv[NC], av[NC];
   if (q.valid) {
-    double pv[NC], av[NC];
     applyN<NC>(L, q, code, plainView<double>(p), pv, av);
 #pragma unroll
     for (int e = 0; e < NC; ++e) acc[0] += pv[e] * av[e];
-    stN<NC>(Ap, q.c, av);
   }
   blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
+  if (q.valid) stN<NC>(Ap, q.c, av);  // after the fold's barriers, which would wait for it
 }
 
 // p_new = z + beta p_old (multigrid.cpp:336-338) fused into the next
@@ -940,15 +945,17 @@ __global__ void __launch_bounds__(kBlock) k_fcg_apply_pu(MgLevel L, const double
   }
   NBL_CELLS(NC);
   double acc[1] = {0.0};
+  double pv[NC], av[NC];
   if (q.valid) {
-    double pv[NC], av[NC];
     applyN<NC>(L, q, code, PNewView{z, pOld, be}, pv, av);
 #pragma unroll
     for (int e = 0; e < NC; ++e) acc[0] += pv[e] * av[e];
+  }
+  blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
+  if (q.valid) {
     stN<NC>(pNew, q.c, pv);
     stN<NC>(Ap, q.c, av);
   }
-  blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
 // x += alpha p; rn = r - alpha Ap; fluid sum of rn (singular) or |rn|^2.
@@ -985,10 +992,12 @@ __global__ void __launch_bounds__(kBlock) k_fcg_update(MgLevel L, int singular, 
       if (singular) acc[0] += (code[e] & kFluidBit) ? rv[e] : 0.0;
       else acc[0] += rv[e] * rv[e];
     }
+  }
+  blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
+  if (q.valid) {  // after the fold's barriers, which would wait for the stores
     stN<NC>(x, q.c, xv);
     stN<NC>(rn, q.c, rv);
   }
-  blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
 // Non-singular systems: k_fcg_update plus the first smoother sweep of the
@@ -1022,8 +1031,8 @@ __global__ void __launch_bounds__(kBlock) k_fcg_update_first(MgLevel L, double* 
     load();
   }
   double acc[1] = {0.0};
+  T bv[NC], fv[NC];
   if (q.valid) {
-    T bv[NC], fv[NC];
 #pragma unroll
     for (int e = 0; e < NC; ++e) {
       xv[e] = xv[e] + a * pv[e];
@@ -1032,12 +1041,14 @@ __global__ void __launch_bounds__(kBlock) k_fcg_update_first(MgLevel L, double* 
       fv[e] = firstSweep<T>(L, code[e], q.c + e, q.i + e, q.j, q.k, bv[e]);
       acc[0] += rv[e] * rv[e];
     }
+  }
+  blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
+  if (q.valid) {  // after the fold's barriers, which would wait for the stores
     stN<NC>(x, q.c, xv);
     stN<NC>(rn, q.c, rv);
     if (b0) stN<NC>(b0, q.c, bv);
     stN<NC>(x1, q.c, fv);
   }
-  blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
 template <int NC>
@@ -1047,15 +1058,18 @@ __global__ void __launch_bounds__(kBlock) k_fcg_norm(MgLevel L, double* __restri
   if (stopOrMeans<1>(ctl, {mrn}, mv)) return;
   NBL_CELLS(NC);
   double acc[1] = {0.0};
+  double v[NC];
   if (q.valid) {
     const double m = mv[0];
-    double v[NC];
     ldN<NC>(rn, q.c, v);
 #pragma unroll
     for (int e = 0; e < NC; ++e) {
       v[e] = (code[e] & kFluidBit) ? v[e] - m : 0.0;
       acc[0] += v[e] * v[e];
     }
+  }
+  blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
+  if (q.valid) {  // after the fold's barriers, which would wait for the stores
     stN<NC>(rn, q.c, v);
     if (b0) {  // the FP32 V-cycle's right-hand side, as k_to_level0 writes it
       float f[NC];
@@ -1064,7 +1078,6 @@ __global__ void __launch_bounds__(kBlock) k_fcg_norm(MgLevel L, double* __restri
       stN<NC>(b0, q.c, f);
     }
   }
-  blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
 // z <- z - pending mean; z.rn and z.r (flexible Polak-Ribiere beta,
@@ -1078,8 +1091,9 @@ __global__ void __launch_bounds__(kBlock) k_fcg_dots(MgLevel L, double* __restri
   if (countIt && firstBlock() && linearTid() == 0) stA(&ctl->it, ldA(&ctl->it) + 1);
   NBL_CELLS(NC);
   double acc[2] = {0.0, 0.0};
+  double v[NC];
   if (q.valid) {
-    double v[NC], a[NC], c[NC];
+    double a[NC], c[NC];
     ldN<NC>(z, q.c, v);
     ldN<NC>(rn, q.c, a);
     ldN<NC>(r, q.c, c);
@@ -1089,9 +1103,9 @@ __global__ void __launch_bounds__(kBlock) k_fcg_dots(MgLevel L, double* __restri
       acc[0] += v[e] * a[e];
       acc[1] += v[e] * c[e];
     }
-    stN<NC>(z, q.c, v);
   }
   blockToPartials<2>(acc, rb.partials, rb.stride, q.blk);
+  if (q.valid) stN<NC>(z, q.c, v);  // after the fold's barriers
 }
 
 // k_finish + k_fcg_dots in one pass for systems without projections: z is the
@@ -1103,9 +1117,10 @@ __global__ void __launch_bounds__(kBlock) k_fcg_dots_x(MgLevel L, const T* __res
   if (ldA(&ctl->done)) return;
   NBL_CELLS(NC);
   double acc[2] = {0.0, 0.0};
+  double v[NC];
   if (q.valid) {
     T xv[NC];
-    double v[NC], a[NC], c[NC];
+    double a[NC], c[NC];
     ldN<NC>(x, q.c, xv);
     ldN<NC>(rn, q.c, a);
     ldN<NC>(r, q.c, c);
@@ -1115,9 +1130,9 @@ __global__ void __launch_bounds__(kBlock) k_fcg_dots_x(MgLevel L, const T* __res
       acc[0] += v[e] * a[e];
       acc[1] += v[e] * c[e];
     }
-    stN<NC>(z, q.c, v);
   }
   blockToPartials<2>(acc, rb.partials, rb.stride, q.blk);
+  if (q.valid) stN<NC>(z, q.c, v);  // after the fold's barriers
 }
 
 // The last level-0 sweep of a V-cycle with the preconditioner output and the
@@ -1133,9 +1148,10 @@ __global__ void __launch_bounds__(kBlock) k_sweep_dots(MgLevel L, const T* __res
   if (countIt && firstBlock() && linearTid() == 0) stA(&ctl->it, ldA(&ctl->it) + 1);
   NBL_CELLS(NC);
   double acc[2] = {0.0, 0.0};
+  double v[NC];
   if (q.valid) {
     T xc[NC], xn[NC][6], bv[NC];
-    double v[NC], a[NC], c[NC];
+    double a[NC], c[NC];
     gatherN<T, NC>(L, plainView<T>(x), q.c, q.i, q.j, q.k, xc, xn);
     ldN<NC>(b, q.c, bv);
     ldN<NC>(rn, q.c, a);
@@ -1152,9 +1168,9 @@ __global__ void __launch_bounds__(kBlock) k_sweep_dots(MgLevel L, const T* __res
       acc[0] += v[e] * a[e];
       acc[1] += v[e] * c[e];
     }
-    stN<NC>(z, q.c, v);
   }
   blockToPartials<2>(acc, rb.partials, rb.stride, q.blk);
+  if (q.valid) stN<NC>(z, q.c, v);  // after the fold's barriers
 }
 
 template <int NC>

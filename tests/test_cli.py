@@ -175,3 +175,114 @@ def test_vtu_snapshots_match_the_reference_writer(tmp_path, orc, fmt):
             for got, want in zip(arrs[4:], (cu, cv, cw, p, speed, g.solid.astype(np.float64))):
                 assert np.array_equal(got, want)
         o.close()
+
+
+# ---- the output formats pinned by their specification (case.cpp:175-241,
+# 332-345): an ostream's default float formatting at precision p is C's "%.pg"
+# (6 digits for diagnostics.jsonl, 8 for the VTU), so every number the run
+# writes is checked against Python's "%g" of the value it stands for, computed
+# here from the run's own state, independently of any writer restatement.
+def g6(x):
+    return "%g" % x
+
+
+def g8(x):
+    return "%.8g" % x
+
+
+def cell_fields(g, u, v, w):
+    """cellVelocity (ns_solver.cpp:676-701): face pairs averaged, w = 0 in 2-D."""
+    nx, ny, nz = g.nx, g.ny, g.nz
+    U = u.reshape(nz, ny, nx + 1)
+    V = v.reshape(nz, ny + 1, nx)
+    cu = (0.5 * (U[:, :, :-1] + U[:, :, 1:])).reshape(-1)
+    cv = (0.5 * (V[:, :-1, :] + V[:, 1:, :])).reshape(-1)
+    if nz == 1:
+        cw = np.zeros(g.cells)
+    else:
+        W = w.reshape(nz + 1, ny, nx)
+        cw = (0.5 * (W[:-1] + W[1:])).reshape(-1)
+    return cu, cv, cw
+
+
+def expected_vtu_lines(g, u, v, w, p):
+    n = g.cells
+    lines = ['<?xml version="1.0"?>',
+             '<VTKFile type="UnstructuredGrid" version="1.0" byte_order="LittleEndian" header_type="UInt64">',
+             "  <UnstructuredGrid>", f'    <Piece NumberOfPoints="{8 * n}" NumberOfCells="{n}">',
+             "      <Points>", '        <DataArray type="Float64" NumberOfComponents="3" format="ascii">']
+    for k in range(g.nz):
+        for j in range(g.ny):
+            for i in range(g.nx):
+                x0 = g.origin[0] + i * g.dx
+                x1 = x0 + g.dx
+                y0 = g.origin[1] + j * g.dy
+                y1 = y0 + g.dy
+                z0 = g.origin[2] + k * g.dz
+                z1 = z0 + g.dz
+                pts = (x0, y0, z0, x1, y0, z0, x1, y1, z0, x0, y1, z0, x0, y0, z1, x1, y0, z1, x1, y1, z1, x0, y1, z1)
+                lines.append(" ".join(g8(c) for c in pts))
+    lines += ["        </DataArray>", "      </Points>", "      <Cells>",
+              '        <DataArray type="Int64" Name="connectivity" format="ascii">']
+    lines += [" ".join(str(8 * c + q) for q in range(8)) for c in range(n)]
+    lines += ["        </DataArray>", '        <DataArray type="Int64" Name="offsets" format="ascii">',
+              "".join(f"{8 * (c + 1)} " for c in range(n)), "        </DataArray>",
+              '        <DataArray type="UInt8" Name="types" format="ascii">', "12 " * n,
+              "        </DataArray>", "      </Cells>", '      <CellData Scalars="speed">']
+    cu, cv, cw = cell_fields(g, u, v, w)
+    speed = np.sqrt(cu * cu + cv * cv + cw * cw)
+    for name, a in (("u", cu), ("v", cv), ("w", cw), ("p", p), ("speed", speed),
+                    ("solid", g.solid.astype(np.float64))):
+        lines += [f'        <DataArray type="Float64" Name="{name}" format="ascii">',
+                  "".join(g8(x) + " " for x in a), "        </DataArray>"]
+    lines += ["      </CellData>", "    </Piece>", "  </UnstructuredGrid>", "</VTKFile>"]
+    return lines
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["cube-channel", "lid-cavity"])
+def test_output_formats_match_their_specification(tmp_path, ctx, name):
+    if name == "cube-channel":
+        case = {"type": "cube-channel", "resolution": 4, "reynolds": 500.0}
+        g, cfg = cases.cube_channel(nx=48, ny=16, nz=16, res=4, reynolds=500.0)
+        st = cases.cube_channel_state(g)
+    else:
+        case = {"type": "lid-cavity", "resolution": 16, "reynolds": 100.0}
+        g, cfg = cases.lid_cavity(16, 100.0)
+        st = None
+    case.update(max_steps=6, snapshot_every=3, checkpoint_every=3, vtu_format="ascii", run_dir=str(tmp_path / "r"))
+    (tmp_path / "c.json").write_text(json.dumps(case))
+    r = run("run", str(tmp_path / "c.json"))
+    assert r.returncode == 0, r.stderr
+    # diagnostics.jsonl: every step line is the reference's key sequence with
+    # each double printed as "%g" of the step report of the same run
+    run_ = nbl.Run(ctx, g, cfg)
+    try:
+        if st is None:
+            run_.initialize()
+        else:
+            run_.set_state(*st)
+        reps = [run_.step() for _ in range(6)]
+    finally:
+        run_.close()
+    lines = (tmp_path / "r" / "diagnostics.jsonl").read_text().split("\n")
+    assert lines[-1] == "" and len(lines) == 8
+    assert lines[0].startswith('{"event":"meta","solver":"nabla_solve","solver_version":"')
+    for rep, ln in zip(reps, lines[1:7]):
+        acc = json.loads(ln)["accepted"]
+        want = ('{"step":%d,"t":%s,"dt":%s,"cfl":%s,"momentum_residual":%s,"continuity_residual":%s,'
+                '"divergence_max":%s,"mass_error":%s,"cd":%s,"cl":%s,"poisson_iters":%d,"poisson_residual":%s,'
+                '"cells":%d,"accepted":%s}') % (
+            rep.step, g6(rep.t), g6(rep.dt), g6(rep.cfl), g6(rep.momentum_residual), g6(rep.continuity_residual),
+            g6(rep.divergence_max), g6(rep.mass_error), g6(rep.cd), g6(rep.cl), rep.poisson_iters,
+            g6(rep.poisson_residual), rep.cell_count, "true" if acc else "false")
+        assert ln == want
+    # VTU: skeleton, element counts and every number ("%.8g") from the state
+    # the same step's checkpoint holds
+    for vtu, ckpt in (("snapshot_3.vtu", "checkpoint_3.ckpt"), ("final.vtu", "final.ckpt")):
+        _hdr, (u, v, w, p) = read_ckpt(tmp_path / "r" / ckpt)
+        got = (tmp_path / "r" / vtu).read_text().split("\n")
+        want = expected_vtu_lines(g, u, v, w, p)
+        assert got[-1] == "" and len(got) == len(want) + 1, vtu
+        for q, (a, b) in enumerate(zip(got, want)):
+            assert a == b, (vtu, q)
