@@ -220,6 +220,16 @@ int nbl_run_rhs(nbl_run* run, double* ru, double* rv, double* rw);
 /* NSSolver::cellVelocity(component) (ns_solver.cpp:690-701). */
 int nbl_run_cell_velocity(nbl_run* run, int component, double* out);
 
+/* A pressure solver that runs outside the library (the reference's
+ * PoissonSolver seam, ns_solver.hpp:60-61: NSSolver takes any implementation).
+ * After it is set, every projection of this run (ns_solver.cpp:501) copies its
+ * right-hand side b (nx*ny*nz, compact order) to the host, calls
+ * solve(user, b, x, n, rep), which must assign x (length n) and may fill rep
+ * (iterations, residual, converged), and copies x back; a non-zero return
+ * fails the step with NBL_ERR_STATE.  NULL restores the device MG-PCG solve. */
+typedef int (*nbl_host_poisson_fn)(void* user, const double* b, double* x, size_t n, nbl_poisson_report* rep);
+int nbl_run_set_pressure_solver(nbl_run* run, nbl_host_poisson_fn solve, void* user);
+
 /* Per-stage pressure solves of the last step: device milliseconds, flexible-CG
  * iterations and V-cycles (any pointer may be NULL; arrays of 3).  No
  * reference counterpart (the reference keeps no timings). */

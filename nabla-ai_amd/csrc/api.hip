@@ -266,6 +266,13 @@ int nbl_run_cell_velocity(nbl_run* run, int component, double* out) {
   });
 }
 
+int nbl_run_set_pressure_solver(nbl_run* run, nbl_host_poisson_fn solve, void* user) {
+  return guarded([&] {
+    need(run, "run is NULL");
+    run->impl->setHostSolver(solve, user);
+  });
+}
+
 int nbl_run_last_solves(nbl_run* run, double* ms, int* iters, int* vcycles) {
   return guarded([&] {
     need(run, "run is NULL");

@@ -1208,10 +1208,26 @@ double RunDev::computeDt() {
 void RunDev::project(int stageIdx, nbl_poisson_report& rep) {  // ns_solver.cpp:475-579
   bc(true);
   launch_divergence(args_, cur_[0], cur_[1], cur_[2], b_.as<double>(), stream_);
-  rep = poisson_->solve(b_.as<double>(), phi_.as<double>(), stream_);
-  solveMs_[stageIdx - 1] = poisson_->lastMs();
-  solveIt_[stageIdx - 1] = rep.iterations;
-  solveV_[stageIdx - 1] = poisson_->lastVcycles();
+  if (hostFn_) {  // a solver outside the library: host round trip
+    const size_t n = sz_[3];
+    hostB_.resize(n);
+    hostX_.assign(n, 0.0);
+    NBL_HIP(hipMemcpyAsync(hostB_.data(), b_.as<void>(), n * 8, hipMemcpyDeviceToHost, stream_));
+    NBL_HIP(hipStreamSynchronize(stream_));
+    rep = nbl_poisson_report{0, 0.0, 0};
+    if (hostFn_(hostUser_, hostB_.data(), hostX_.data(), n, &rep) != 0)
+      throw std::runtime_error("host pressure solver failed");
+    NBL_HIP(hipMemcpyAsync(phi_.as<void>(), hostX_.data(), n * 8, hipMemcpyHostToDevice, stream_));
+    NBL_HIP(hipStreamSynchronize(stream_));
+    solveMs_[stageIdx - 1] = 0.0;
+    solveIt_[stageIdx - 1] = rep.iterations;
+    solveV_[stageIdx - 1] = 0;
+  } else {
+    rep = poisson_->solve(b_.as<double>(), phi_.as<double>(), stream_);
+    solveMs_[stageIdx - 1] = poisson_->lastMs();
+    solveIt_[stageIdx - 1] = rep.iterations;
+    solveV_[stageIdx - 1] = poisson_->lastVcycles();
+  }
   launch_correct(args_, cur_[0], cur_[1], cur_[2], phi_.as<double>(), p_.as<double>(), sc_.as<StepCtl>(),
                  stream_);
 }

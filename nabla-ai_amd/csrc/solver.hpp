@@ -230,6 +230,11 @@ class RunDev {
   }
   size_t sizes(int q) const { return sz_[q]; }
   PoissonDev& poisson() { return *poisson_; }
+  // A host pressure solver in place of the device MG-PCG (NULL: device).
+  void setHostSolver(nbl_host_poisson_fn fn, void* user) {
+    hostFn_ = fn;
+    hostUser_ = user;
+  }
 
  private:
   void bc(bool outflow);
@@ -254,6 +259,9 @@ class RunDev {
   StepCtl* hsc_ = nullptr;
   DevReport* hrep_ = nullptr;
   std::unique_ptr<PoissonDev> poisson_;
+  nbl_host_poisson_fn hostFn_ = nullptr;
+  void* hostUser_ = nullptr;
+  std::vector<double> hostB_, hostX_;
   double dtLast_ = 0.0;
   double solveMs_[3] = {0, 0, 0};
   int solveIt_[3] = {0, 0, 0}, solveV_[3] = {0, 0, 0};

@@ -48,10 +48,12 @@ struct StepReport {
 
 class NSSolver {
  public:
-  // `poisson` may be null (MG-PCG with the config's tolerance) or a
-  // DevicePoisson, whose tolerance, cycle limit and precision are used by the
-  // device pressure solve.  Other PoissonSolver implementations run on the
-  // host and are rejected (std::invalid_argument).
+  // `poisson` may be null (MG-PCG with the config's tolerance), a device
+  // MG-PCG (HipMgPoisson), whose tolerance, cycle limit and precision the
+  // device pressure solve uses with the state kept in HBM, or any other
+  // PoissonSolver (ns_solver.hpp:60-61 of the reference takes any): each
+  // projection then copies its right-hand side to the host, calls
+  // poisson->solve(grid, b, x) and copies x back.
   NSSolver(UniformGrid grid, FlowConfig config, std::shared_ptr<PoissonSolver> poisson = nullptr);
   ~NSSolver();
   NSSolver(const NSSolver&) = delete;
@@ -86,9 +88,12 @@ class NSSolver {
  private:
   void fetch() const;
 
+  static int hostSolve(void* self, const double* b, double* x, std::size_t n, nbl_poisson_report* rep);
+
   UniformGrid grid_;
   FlowConfig cfg_;
   double nu_;
+  std::shared_ptr<PoissonSolver> hostPoisson_;  // a solver outside the device path
   nbl_run* run_ = nullptr;
   int step_ = 0;
   double t_ = 0.0;

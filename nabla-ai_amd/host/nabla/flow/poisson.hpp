@@ -62,6 +62,15 @@ class HipCgPoisson : public DevicePoisson {
   explicit HipCgPoisson(double tol = 1e-10, int maxIters = 5000)
       : DevicePoisson(1, tol, maxIters, 2, 2, Precision::FP64) {}
   [[nodiscard]] std::string name() const override { return "cg-jacobi"; }
+
+  // CgPoisson's statics (poisson.hpp:47-52): A x for the discrete -Laplacian
+  // with the grid's closure (identity rows on solid cells), whether an outlet
+  // makes the system non-singular, and A's main diagonal (solid cells 1).
+  // The operator and diagonal are evaluated on the device, bit-identical to
+  // the reference's (poisson.cpp:78-134); out and diag are assigned.
+  static void applyA(const UniformGrid& grid, const std::vector<double>& x, std::vector<double>& out);
+  static bool hasDirichlet(const UniformGrid& grid);
+  static void buildDiagonal(const UniformGrid& grid, std::vector<double>& diag);
 };
 
 // The reference's class names for the same solvers.

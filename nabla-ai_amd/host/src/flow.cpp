@@ -1,4 +1,5 @@
 // Host layer over the C-ABI: device context, the Poisson plugin and NSSolver.
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -103,14 +104,71 @@ NSSolver::NSSolver(UniformGrid grid, FlowConfig config, std::shared_ptr<PoissonS
   nbl_solver_opts_default(&o);
   if (poisson) {
     auto* dp = dynamic_cast<DevicePoisson*>(poisson.get());
-    if (!dp || dp->name() != "mg-pcg")
-      throw std::invalid_argument("NSSolver: the device time step needs a device MG-PCG pressure solver");
-    c.poisson_tol = dp->tol();
-    c.poisson_max_iters = dp->maxIters();
-    o = dp->opts();
+    if (dp && dp->name() == "mg-pcg") {
+      c.poisson_tol = dp->tol();
+      c.poisson_max_iters = dp->maxIters();
+      o = dp->opts();
+    } else {
+      hostPoisson_ = std::move(poisson);
+    }
   }
   const nbl_grid_desc d = grid_.desc();
   nblCheck(nbl_run_create(deviceContext(), &d, &c, &o, &run_), "NSSolver");
+  if (hostPoisson_) nblCheck(nbl_run_set_pressure_solver(run_, &NSSolver::hostSolve, this), "NSSolver");
+}
+
+// The projection's pressure solve through a PoissonSolver outside the device
+// path (called back by the run with the right-hand side on the host).
+int NSSolver::hostSolve(void* self, const double* b, double* x, std::size_t n, nbl_poisson_report* rep) {
+  auto* s = static_cast<NSSolver*>(self);
+  try {
+    const std::vector<double> bv(b, b + n);
+    std::vector<double> xv;
+    const PoissonSolver::Report r = s->hostPoisson_->solve(s->grid_, bv, xv);
+    if (xv.size() != n) return 1;
+    std::copy(xv.begin(), xv.end(), x);
+    rep->iterations = r.iterations;
+    rep->residual = r.residual;
+    rep->converged = r.converged ? 1 : 0;
+    return 0;
+  } catch (...) {
+    return 1;  // the run reports the failure; nothing is thrown across the C-ABI
+  }
+}
+
+// CgPoisson's statics on a temporary device solver for the grid.
+namespace {
+struct TmpCg {
+  nbl_poisson* ps = nullptr;
+  explicit TmpCg(const UniformGrid& grid) {
+    UniformGrid g = grid;
+    if (g.solid.empty()) g.allocate();
+    const nbl_grid_desc d = g.desc();
+    nblCheck(nbl_poisson_create(deviceContext(), &d, 1, 1e-10, 1, nullptr, &ps), "CgPoisson");
+  }
+  ~TmpCg() {
+    if (ps) nbl_poisson_destroy(ps);
+  }
+};
+}  // namespace
+
+void HipCgPoisson::applyA(const UniformGrid& grid, const std::vector<double>& x, std::vector<double>& out) {
+  if (x.size() != grid.cellCount()) throw std::invalid_argument("CgPoisson::applyA: x size mismatch");
+  TmpCg t(grid);
+  out.assign(grid.cellCount(), 0.0);
+  nblCheck(nbl_poisson_apply(t.ps, x.data(), out.data()), "CgPoisson::applyA");
+}
+
+bool HipCgPoisson::hasDirichlet(const UniformGrid& grid) {  // poisson.cpp:93-100
+  for (const FaceBC& f : grid.face)
+    if (f.type == BoundaryType::Outlet) return true;
+  return false;
+}
+
+void HipCgPoisson::buildDiagonal(const UniformGrid& grid, std::vector<double>& diag) {
+  TmpCg t(grid);
+  diag.assign(grid.cellCount(), 1.0);
+  nblCheck(nbl_poisson_diagonal(t.ps, diag.data()), "CgPoisson::buildDiagonal");
 }
 
 NSSolver::~NSSolver() {
