@@ -52,8 +52,11 @@ enum { NBL_CONV_CENTRAL = 0, NBL_CONV_WENO5 = 1 };
 enum {
   NBL_PREC_FP64 = 0,  /* plain: every level of the V-cycle in FP64                  */
   NBL_PREC_MIXED = 1, /* FP32 V-cycle inside FP64 flexible CG (FP64 residual/dots)   */
-  NBL_PREC_MIXED_BF16 = 2 /* reserved: BF16 smoother scratch; not implemented, rejected
-                             with NBL_ERR_INVALID                                 */
+  NBL_PREC_MIXED_BF16 = 2 /* as MIXED, the finest level's smoother scratch (the iterate
+                             after the pre-sweeps, passed from the cycle's down to its
+                             up kernel) stored as BF16; arithmetic stays FP32.  Needs
+                             the two-kernels-per-level cycle (see vcycle), else
+                             NBL_ERR_INVALID                                        */
 };
 /* Coarsest-level solve of the V-cycle.  The reference runs CgPoisson(1e-6, 500)
    there ("near-exact", multigrid.cpp:178-183). */

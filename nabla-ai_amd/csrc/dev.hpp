@@ -2,6 +2,7 @@
 // host-side launchers each kernel file exports.
 #pragma once
 
+#include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -269,6 +270,7 @@ struct DownArgs {
   double* rnOut = nullptr;    // DS_UPD / DS_RN: r_new (projected for DS_RN)
   T* b0Out = nullptr;         // DS_UPD / DS_RN with T = float: the level-0 right-hand side float(r_new)
   T* xa = nullptr;            // x after the two pre-sweeps
+  __hip_bfloat16* xaB = nullptr;  // the same stored as BF16 instead (finest level of the BF16 form)
   T* bc = nullptr;            // restricted residual
   double* rowNorm = nullptr;  // per-block |r_new|^2 (DS_UPD / DS_RN)
   double* rowMean = nullptr;  // per-block fluid sum of bc (singular systems)
@@ -285,6 +287,7 @@ struct UpArgs {
   const T* b = nullptr;
   MeanRef mb{};
   const T* xa = nullptr;      // x after the pre-sweeps
+  const __hip_bfloat16* xaB = nullptr;  // or the same stored as BF16
   const T* e = nullptr;       // the coarse level's correction
   T* xb = nullptr;            // UD_X
   double* z = nullptr;        // UD_Z / UD_DOTS

@@ -160,10 +160,9 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
     throw std::invalid_argument("pre/post smoothing sweeps must be >= 1");
   if (opts.precision < NBL_PREC_FP64 || opts.precision > NBL_PREC_MIXED_BF16)
     throw std::invalid_argument("unknown precision");
-  if (opts.precision == NBL_PREC_MIXED_BF16)
-    throw std::invalid_argument("precision NBL_PREC_MIXED_BF16 (BF16 smoother scratch) is not implemented");
   singular_ = !hasOutlet(g);
   mixed_ = kind == 0 && opts.precision != NBL_PREC_FP64;
+  bf16_ = kind == 0 && opts.precision == NBL_PREC_MIXED_BF16;
   if (opts.loops != 0 && opts.loops != 1) throw std::invalid_argument("loops must be 0 or 1");
   if (opts.loops && (kind != 0 || mixed_)) throw std::invalid_argument("loops: FP64 mg-pcg only");
   loops_ = opts.loops == 1;
@@ -255,11 +254,18 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
   // for the per-sweep kernels they replace (the coarser levels break even);
   // on the 64^3 periodic box (singular) it also drops the per-sweep
   // projections and a step takes 3.58 instead of 5.05 ms.
-  const bool wantTiled = opts.vcycle == 2 || (opts.vcycle == 0 && project_);
+  const bool wantTiled = opts.vcycle == 2 || (opts.vcycle == 0 && (project_ || bf16_));
   tiled_ = kind == 0 && !loops_ && wantTiled && nl >= 3 && opts.pre_smooth == 2 && opts.post_smooth == 2 &&
            direct_ && blocks_[0] <= kInlineFoldMax && tblocks_[0] <= kInlineFoldMax;
   for (int l = 0; tiled_ && l + 1 < nl; ++l) tiled_ = tileFits(dl_[l], dl_[l + 1]);
   if (tiled_ && project_) rnRaw_ = DevBuf(ctx, n0 * 8);
+  // BF16 smoother scratch lives in the two-kernels-per-level cycle: the finest
+  // level's pre-smoothed iterate passes from its down to its up kernel as BF16
+  if (bf16_ && !tiled_)
+    throw std::invalid_argument(
+        "precision NBL_PREC_MIXED_BF16 needs the two-kernels-per-level V-cycle (vcycle 0 or 2, pre = post = 2, "
+        "at least three levels, a direct coarsest solve, a finest level of at most 2048 tiles)");
+  if (bf16_) xaB_ = DevBuf(ctx, n0 * sizeof(__hip_bfloat16));
   NBL_HIP(hipDeviceSynchronize());
 }
 
@@ -676,6 +682,7 @@ void PoissonDev::tiledCycle(hipStream_t s, int k, bool first, const T* b0, const
   d0.Lc = dl_[1];
   d0.ctl = ctl;
   d0.xa = lx0_[0].as<T>();
+  if (bf16_) d0.xaB = xaB_.as<__hip_bfloat16>();
   d0.bc = lb_[1].as<T>();
   d0.rowMean = project_ ? slotRow(slotB(1)) : nullptr;
   int src = DS_B;
@@ -735,6 +742,7 @@ void PoissonDev::tiledCycle(hipStream_t s, int k, bool first, const T* b0, const
   u0.Lc = dl_[1];
   u0.b = b0;
   u0.xa = lx0_[0].as<T>();
+  if (bf16_) u0.xaB = xaB_.as<__hip_bfloat16>();
   u0.e = e;
   u0.z = z_.as<double>();
   u0.ctl = ctl;

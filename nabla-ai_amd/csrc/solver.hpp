@@ -157,6 +157,8 @@ class PoissonDev {
   bool singular_ = false;
   bool project_ = false;  // singular with fluid cells: null-space projections apply
   bool mixed_ = false;
+  bool bf16_ = false;   // mixed with the finest level's pre-smoothed iterate stored as BF16
+  DevBuf xaB_;
   std::vector<HostLevel> hl_;
   std::vector<MgLevel> dl_;
   std::vector<int> blocks_;   // geo3 block count per level (= partials per reduction)

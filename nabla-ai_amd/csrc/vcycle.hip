@@ -32,6 +32,9 @@
 // in exact arithmetic.  Here they are folded into the projections that do
 // change values: the restricted right-hand side of every level and r_new
 // (multigrid.cpp:150-152, 318), and z (multigrid.cpp:288).
+#include <stdexcept>
+#include <type_traits>
+
 #include "mgkern.hpp"
 
 namespace nbl {
@@ -121,7 +124,7 @@ __device__ __forceinline__ void nbrs(const T* __restrict__ s, int i, T (&xn)[6])
 // cell's code is decoded into its diagonal sum once per kernel; the sweeps
 // then accumulate only the couplings.
 // ---------------------------------------------------------------------------
-template <class T, int SRC, int TX, int TY, int TZ, int HZ>
+template <class T, int SRC, int TX, int TY, int TZ, int HZ, bool BF>
 __global__ void __launch_bounds__(KT) k_down(DownArgs<T> a) {
   using G = Tile<TX, TY, TZ, HZ>;
   constexpr bool ZD = HZ > 0;
@@ -322,8 +325,11 @@ __global__ void __launch_bounds__(KT) k_down(DownArgs<T> a) {
     }
   }
 #pragma unroll
-  for (int q = 0; q < G::S1; ++q)
-    if (xac[q] >= 0) a.xa[xac[q]] = xav[q];
+  for (int q = 0; q < G::S1; ++q) {
+    if (xac[q] < 0) continue;
+    if constexpr (BF) a.xaB[xac[q]] = __float2bfloat16(float(xav[q]));
+    else a.xa[xac[q]] = xav[q];
+  }
 #pragma unroll
   for (int q = 0; q < G::ST; ++q)
     if (cC[q] >= 0) a.bc[cC[q]] = rcv[q];
@@ -333,7 +339,7 @@ __global__ void __launch_bounds__(KT) k_down(DownArgs<T> a) {
 // Up kernel (stores after the last barrier, codes decoded once, as the down
 // kernel)
 // ---------------------------------------------------------------------------
-template <class T, int DST, int TX, int TY, int TZ, int HZ>
+template <class T, int DST, int TX, int TY, int TZ, int HZ, bool BF>
 __global__ void __launch_bounds__(KT) k_up(UpArgs<T> a) {
   using G = Tile<TX, TY, TZ, HZ>;
   constexpr bool ZD = HZ > 0;
@@ -365,7 +371,8 @@ __global__ void __launch_bounds__(KT) k_up(UpArgs<T> a) {
       const int gk = ZD ? wrapc(k0 - HZ + ck, L.nz) : 0;
       const int c = gi + L.nx * (gj + L.ny * gk);
       r1[q] = G::r1of(ai, bj, ck);
-      xv[q] = a.xa[c] + a.e[(gi >> sx) + Lc.nx * ((gj >> sy) + Lc.ny * (gk >> sz))];
+      const T x2 = BF ? T(__bfloat162float(a.xaB[c])) : a.xa[c];
+      xv[q] = x2 + a.e[(gi >> sx) + Lc.nx * ((gj >> sy) + Lc.ny * (gk >> sz))];
       if (r1[q] >= 0) {
         cv[q] = L.code[c];
         bw[q] = a.b[c];
@@ -504,9 +511,21 @@ void launch_down(int src, const DownArgs<T>& a, hipStream_t s) {
   withTile(a.L, [&](auto tx, auto ty, auto tz, auto hz) {
     constexpr int TX = decltype(tx)::value, TY = decltype(ty)::value, TZ = decltype(tz)::value,
                   HZ = decltype(hz)::value;
-    if (src == DS_B) k_down<T, DS_B, TX, TY, TZ, HZ><<<grid, KT, 0, s>>>(a);
-    else if (src == DS_UPD) k_down<T, DS_UPD, TX, TY, TZ, HZ><<<grid, KT, 0, s>>>(a);
-    else k_down<T, DS_RN, TX, TY, TZ, HZ><<<grid, KT, 0, s>>>(a);
+    if (a.xaB) {
+      if constexpr (std::is_same_v<T, float>) {
+        if (src == DS_B) k_down<T, DS_B, TX, TY, TZ, HZ, true><<<grid, KT, 0, s>>>(a);
+        else if (src == DS_UPD) k_down<T, DS_UPD, TX, TY, TZ, HZ, true><<<grid, KT, 0, s>>>(a);
+        else k_down<T, DS_RN, TX, TY, TZ, HZ, true><<<grid, KT, 0, s>>>(a);
+      } else {
+        throw std::logic_error("down kernel: BF16 scratch needs the FP32 cycle");
+      }
+    } else if (src == DS_B) {
+      k_down<T, DS_B, TX, TY, TZ, HZ, false><<<grid, KT, 0, s>>>(a);
+    } else if (src == DS_UPD) {
+      k_down<T, DS_UPD, TX, TY, TZ, HZ, false><<<grid, KT, 0, s>>>(a);
+    } else {
+      k_down<T, DS_RN, TX, TY, TZ, HZ, false><<<grid, KT, 0, s>>>(a);
+    }
   });
   NBL_LAUNCH_CHECK();
 }
@@ -517,9 +536,21 @@ void launch_up(int dst, const UpArgs<T>& a, hipStream_t s) {
   withTile(a.L, [&](auto tx, auto ty, auto tz, auto hz) {
     constexpr int TX = decltype(tx)::value, TY = decltype(ty)::value, TZ = decltype(tz)::value,
                   HZ = decltype(hz)::value;
-    if (dst == UD_X) k_up<T, UD_X, TX, TY, TZ, HZ><<<grid, KT, 0, s>>>(a);
-    else if (dst == UD_Z) k_up<T, UD_Z, TX, TY, TZ, HZ><<<grid, KT, 0, s>>>(a);
-    else k_up<T, UD_DOTS, TX, TY, TZ, HZ><<<grid, KT, 0, s>>>(a);
+    if (a.xaB) {
+      if constexpr (std::is_same_v<T, float>) {
+        if (dst == UD_X) k_up<T, UD_X, TX, TY, TZ, HZ, true><<<grid, KT, 0, s>>>(a);
+        else if (dst == UD_Z) k_up<T, UD_Z, TX, TY, TZ, HZ, true><<<grid, KT, 0, s>>>(a);
+        else k_up<T, UD_DOTS, TX, TY, TZ, HZ, true><<<grid, KT, 0, s>>>(a);
+      } else {
+        throw std::logic_error("up kernel: BF16 scratch needs the FP32 cycle");
+      }
+    } else if (dst == UD_X) {
+      k_up<T, UD_X, TX, TY, TZ, HZ, false><<<grid, KT, 0, s>>>(a);
+    } else if (dst == UD_Z) {
+      k_up<T, UD_Z, TX, TY, TZ, HZ, false><<<grid, KT, 0, s>>>(a);
+    } else {
+      k_up<T, UD_DOTS, TX, TY, TZ, HZ, false><<<grid, KT, 0, s>>>(a);
+    }
   });
   NBL_LAUNCH_CHECK();
 }

@@ -215,8 +215,7 @@ CaseSpec loadCase(const std::string& jsonPath) {  // case.cpp:136-170
   s.flow.poissonMaxIters = j.integer("poisson_max_iters", s.flow.poissonMaxIters);
   const std::string prec = j.string("precision", "fp64");
   if (prec == "mixed") s.precision = Precision::Mixed;
-  else if (prec == "mixed-bf16")
-    throw std::runtime_error("loadCase: precision 'mixed-bf16' (BF16 smoother scratch) is not implemented; use 'mixed'");
+  else if (prec == "mixed-bf16") s.precision = Precision::MixedBF16;
   else if (prec != "fp64") throw std::runtime_error("loadCase: unknown precision '" + prec + "'");
   s.run.runDir = j.string("run_dir", "run_" + type);
   s.run.maxSteps = j.integer("max_steps", s.run.maxSteps);

@@ -45,15 +45,6 @@ def test_unknown_case_type_exits_1(tmp_path):
     assert r.returncode == 1 and "unknown case type" in r.stderr
 
 
-def test_bf16_precision_is_refused_exits_1(tmp_path):
-    """"mixed-bf16" names a BF16 smoother-scratch form that is not built: the
-    case is refused with an error instead of silently running another form."""
-    p = tmp_path / "c.json"
-    p.write_text(json.dumps({"type": "lid-cavity", "resolution": 8, "precision": "mixed-bf16"}))
-    r = run("run", str(p))
-    assert r.returncode == 1 and "mixed-bf16" in r.stderr and "not implemented" in r.stderr
-
-
 def read_ckpt(path):
     b = Path(path).read_bytes()
     assert b[:8] == b"NBFCK01\0"

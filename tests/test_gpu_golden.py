@@ -143,10 +143,11 @@ def test_cube192_history(ctx):
 # relative (1e-9 absolute where they cross zero), and the integral histories
 # and fields within the rounding the FP32 preconditioner leaves inside that
 # tolerance.
-def test_tgv64_hundred_steps_mixed(ctx):
+@pytest.mark.parametrize("prec", [nbl.PREC_MIXED, nbl.PREC_MIXED_BF16], ids=["fp32", "bf16"])
+def test_tgv64_hundred_steps_mixed(ctx, prec):
     f = load("tgv64_100.npz")
     g, cfg = cases.taylor_green(64)
-    run = nbl.Run(ctx, g, cfg, nbl.solver_opts(nbl.PREC_MIXED))
+    run = nbl.Run(ctx, g, cfg, nbl.solver_opts(prec))
     ke, ens, reps = [], [], []
     try:
         run.set_state(*cases.taylor_green_state(g))
@@ -162,7 +163,7 @@ def test_tgv64_hundred_steps_mixed(ctx):
     ke_err = np.abs(np.array(ke) / f["ke"] - 1).max()
     ens_err = np.abs(np.array(ens) / f["enstrophy"] - 1).max()
     iters = rep_array(reps, "poisson_iters")
-    print(f"TGV64 mixed: KE {ke_err:.2e}, enstrophy {ens_err:.2e}, iterations "
+    print(f"TGV64 mixed ({prec}): KE {ke_err:.2e}, enstrophy {ens_err:.2e}, iterations "
           f"{iters.sum()} vs {int(f['rep_poisson_iters'].sum())}", flush=True)
     assert ke_err <= 1e-10 and ens_err <= 1e-9
     assert np.abs(iters - f["rep_poisson_iters"]).max() <= 3
@@ -174,13 +175,15 @@ def test_tgv64_hundred_steps_mixed(ctx):
         assert e <= 1e-8 and d <= 1e-7, name
 
 
-def test_cube192_history_mixed(ctx):
+@pytest.mark.parametrize("prec", [nbl.PREC_MIXED, nbl.PREC_MIXED_BF16], ids=["fp32", "bf16"])
+def test_cube192_history_mixed(ctx, prec):
     """The 192x64x64 cube channel over the fixture's 500 steps with the FP32
-    V-cycle: residual <= 1e-8 at every solve, drag and lift within 1e-6."""
+    V-cycle (and its BF16 smoother-scratch form): residual <= 1e-8 at every
+    solve, drag and lift within 1e-6."""
     f = load("cube192_history.npz")
     n = len(f["rep_step"])
     g, cfg = cases.cube_channel()
-    run = nbl.Run(ctx, g, cfg, nbl.solver_opts(nbl.PREC_MIXED))
+    run = nbl.Run(ctx, g, cfg, nbl.solver_opts(prec))
     try:
         run.set_state(*cases.cube_channel_state(g))
         reps = [run.step() for _ in range(n)]
@@ -191,7 +194,7 @@ def test_cube192_history_mixed(ctx):
     cd_err = np.abs(cd / f["rep_cd"] - 1).max()
     cl_err = (np.abs(cl - f["rep_cl"]) / np.maximum(np.abs(f["rep_cl"]), 1e-3)).max()
     iters = rep_array(reps, "poisson_iters")
-    print(f"cube192 mixed, {n} steps: cd {cd_err:.2e}, cl {cl_err:.2e}, iterations {iters.sum()} vs "
+    print(f"cube192 mixed ({prec}), {n} steps: cd {cd_err:.2e}, cl {cl_err:.2e}, iterations {iters.sum()} vs "
           f"{int(f['rep_poisson_iters'].sum())}, max residual {rep_array(reps, 'poisson_residual').max():.2e}",
           flush=True)
     assert (rep_array(reps, "poisson_residual") <= 1e-8).all()

@@ -416,10 +416,3 @@ def test_mixed_singular_vector_paths_match_oracle(ctx, orc, cpt):
     assert abs(rep.iterations - ro.iterations) <= 2
     assert rel(x, xo) < 1e-8
 
-
-def test_bf16_precision_rejected(ctx):
-    """NBL_PREC_MIXED_BF16 is reserved and refused, not run as another form."""
-    g, _ = cases.taylor_green(16)
-    with pytest.raises(nbl.NblError) as e:
-        nbl.Poisson(ctx, g, kind=0, opts=nbl.solver_opts(nbl.PREC_MIXED_BF16))
-    assert e.value.code == -1 and "not implemented" in str(e.value)

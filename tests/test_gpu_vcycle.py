@@ -156,3 +156,36 @@ def test_vcycle_option_validated(ctx):
     g = GRIDS["tgv32"]
     with pytest.raises(nbl.NblError):
         nbl.Poisson(ctx, g, kind=0, opts=nbl.solver_opts(vcycle=3))
+
+
+@pytest.mark.parametrize("name", ["tgv32", "cube96", "odd50"])
+def test_bf16_scratch_reaches_fp64_tolerance(ctx, orc, name):
+    """The BF16 form (the finest level's pre-smoothed iterate stored as BF16
+    between the cycle's down and up kernels): the FP64 outer residual still
+    reaches the tolerance, the solution agrees with the FP64 oracle to it, and
+    the rounder preconditioner costs at most a few iterations."""
+    g = GRIDS[name]
+    b = cases.poisson_rhs(g, 53)
+    out = {}
+    for prec in (nbl.PREC_MIXED, nbl.PREC_MIXED_BF16):
+        ps = nbl.Poisson(ctx, g, kind=0, tol=1e-10, max_iters=200, opts=nbl.solver_opts(prec))
+        try:
+            out[prec] = ps.solve(b)
+        finally:
+            ps.close()
+    (x32, r32), (x16, r16) = out[nbl.PREC_MIXED], out[nbl.PREC_MIXED_BF16]
+    xo, ro = orc.mg(g, b, 1e-10, 200)
+    print(f"{name}: iterations fp64 {ro.iterations}, fp32 {r32.iterations}, bf16 {r16.iterations}", flush=True)
+    assert r16.converged and r16.residual <= 1e-10 * 1.01
+    assert r16.iterations <= r32.iterations + 4
+    assert rel(x16, xo) < 1e-8
+
+
+def test_bf16_needs_the_tiled_cycle(ctx):
+    g, _ = cases.taylor_green(16)  # two levels: the two-kernels-per-level cycle does not apply
+    with pytest.raises(nbl.NblError) as e:
+        nbl.Poisson(ctx, g, kind=0, opts=nbl.solver_opts(nbl.PREC_MIXED_BF16))
+    assert e.value.code == -1 and "two-kernels-per-level" in str(e.value)
+    with pytest.raises(nbl.NblError):
+        nbl.Poisson(ctx, GRIDS["cube96"], kind=0, opts=nbl.solver_opts(nbl.PREC_MIXED_BF16,
+                                                                         vcycle=nbl.VCYCLE_PER_SWEEP))
