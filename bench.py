@@ -161,6 +161,17 @@ def octree_c3(ctx, with_cpu=True):
     levels, mg_build_ms = d.mg_levels()
     xm, mrep = d.mg_solve(b, 1e-9, 500)
     _, mg_ms = d.last_timing()
+    variants = {}
+    for name, prec in (("fp32_vcycle", nbl.PREC_MIXED), ("bf16_scratch", nbl.PREC_MIXED_BF16)):
+        d.set_precision(prec)
+        d.mg_solve(b, 1e-9, 500)  # first call of a form makes its FP32 operator copies
+        xq, q = d.mg_solve(b, 1e-9, 500)
+        _, qms = d.last_timing()
+        res = float(np.linalg.norm((d.apply(xq) - b)[fl]) / np.linalg.norm(b))
+        variants[name] = {"iterations": q.iterations, "residual": q.residual, "true_residual": res,
+                          "converged": bool(q.converged), "solve_ms": round(qms, 3),
+                          "speedup_vs_fp64_vcycle": round(mg_ms / max(qms, 1e-9), 3),
+                          "rel_l2_vs_fp64_vcycle": rel(xq, xm)}
     d.close()
     ach = by / (ms * 1e-3) / 1e9
     out = {"leaves": n, "couplings": nnz, "host_build_s": round(host_build_s, 3),
@@ -171,7 +182,11 @@ def octree_c3(ctx, with_cpu=True):
            "mg_pcg": {"tol": 1e-9, "levels": levels, "hierarchy_build_ms": round(mg_build_ms, 3),
                       "iterations": mrep.iterations, "residual": mrep.residual, "converged": bool(mrep.converged),
                       "solve_ms": round(mg_ms, 3), "speedup_vs_jacobi_pcg": round(solve_ms / max(mg_ms, 1e-9), 2),
-                      "rel_l2_vs_jacobi_pcg": rel(xm, xj)},
+                      "rel_l2_vs_jacobi_pcg": rel(xm, xj),
+                      "mixed": variants,
+                      "mixed_note": "the same flexible CG (FP64 residuals, dots and stopping test) with the "
+                                    "V-cycle's levels in FP32, and with level 0's pre-smoothed iterate stored "
+                                    "as BF16"},
            "spmv_roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": round(ach / HBM_PEAK_GBS, 4), "avg_launch_us": round(ms * 1e3, 3),
                              "bytes_per_launch": by},
@@ -186,7 +201,9 @@ def octree_c3(ctx, with_cpu=True):
                        "host_call_note": "nbl_octree_sensors with host arrays: PCIe copies of 6 inputs and 12 "
                                          "outputs included"}}
     log(f"octree: {n} leaves, setup {setup_ms:.1f} ms, Jacobi-PCG {rep.iterations} iterations in {solve_ms:.1f} ms, "
-        f"MG-PCG {mrep.iterations} iterations in {mg_ms:.1f} ms")
+        f"MG-PCG {mrep.iterations} iterations in {mg_ms:.1f} ms; FP32 cycle "
+        f"{variants['fp32_vcycle']['iterations']} in {variants['fp32_vcycle']['solve_ms']:.1f} ms, BF16 "
+        f"{variants['bf16_scratch']['iterations']} in {variants['bf16_scratch']['solve_ms']:.1f} ms")
     if with_cpu:
         import oracle_py
         kind = "reference" if oracle_py.available("ref") else "port"

@@ -382,6 +382,17 @@ int nbl_octree_mg_solve(nbl_octree* oc, const double* b, double* x, double tol, 
 /* Hierarchy of nbl_octree_mg_solve (built if needed): level count, leaves per
  * level (up to cap entries) and the device time of its construction. */
 int nbl_octree_mg_levels(nbl_octree* oc, long long* counts, int cap, int* levels, double* build_ms);
+/* Arithmetic of nbl_octree_mg_solve's V-cycle (NBL_PREC_*; FP64 by default):
+ * NBL_PREC_MIXED runs every level's sweeps, restriction, prolongation and
+ * coarsest solve in FP32 (FP32 copies of the level operators) inside the FP64
+ * flexible CG, whose residuals, dot products, means and stopping test stay
+ * FP64, so the FP64 tolerance is still reached; NBL_PREC_MIXED_BF16 does the
+ * same with level 0's pre-smoothed iterate stored as BF16 between the sweep
+ * that writes it and the restriction and post-sweep that read it.  No
+ * counterpart in the reference (its octree system is solved by FP64
+ * Jacobi-PCG only, octree_poisson.cpp:107-197).  Applies from the next solve;
+ * NBL_ERR_INVALID for other values. */
+int nbl_octree_set_precision(nbl_octree* oc, int precision);
 
 /* No counterpart in the reference: measurement.  Average ms and algorithmic
  * bytes of one launch of an octree kernel (kernel 0: the SpMV, reading

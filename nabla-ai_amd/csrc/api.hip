@@ -640,6 +640,16 @@ int nbl_octree_mg_levels(nbl_octree* oc, long long* counts, int cap, int* levels
   });
 }
 
+int nbl_octree_set_precision(nbl_octree* oc, int precision) {
+  return guarded([&] {
+    need(oc, "NULL argument");
+    need(precision >= NBL_PREC_FP64 && precision <= NBL_PREC_MIXED_BF16,
+         "precision must be NBL_PREC_FP64, NBL_PREC_MIXED or NBL_PREC_MIXED_BF16");
+    useDevice(oc->ctx);
+    oc->impl->setMgPrecision(precision);
+  });
+}
+
 int nbl_octree_profile(nbl_octree* oc, int kernel, int reps, double* ms, double* bytes) {
   return guarded([&] {
     need(oc && ms && bytes, "NULL argument");

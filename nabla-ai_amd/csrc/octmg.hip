@@ -47,45 +47,44 @@ constexpr int kFallbackSweeps = 16;    // coarsest level when its inverse cannot
 
 #define OGRID(c, n) for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < (n); c += (long long)gridDim.x * blockDim.x)
 
+template <class T>
 struct PlainRd {
-  const double* x;
-  __device__ double operator()(long long j) const { return x[j]; }
+  const T* x;
+  __device__ T operator()(long long j) const { return x[j]; }
 };
+struct Bf16Rd {  // FP32 value of an iterate stored as BF16
+  const __hip_bfloat16* x;
+  __device__ float operator()(long long j) const { return __bfloat162float(x[j]); }
+};
+template <class T, class XR>
 struct ProlongRd {  // x + P e: each leaf adds its parent's correction
-  const double* x;
-  const double* e;
+  XR x;
+  const T* e;
   const int32_t* up;
-  __device__ double operator()(long long j) const { return x[j] + e[up[j]]; }
+  __device__ T operator()(long long j) const { return x(j) + e[up[j]]; }
 };
 
-enum : int { OM_PLAIN = 0, OM_PROLONG = 1 };
+__device__ __forceinline__ void storeAs(double* p, double v) { *p = v; }
+__device__ __forceinline__ void storeAs(float* p, float v) { *p = v; }
+__device__ __forceinline__ void storeAs(__hip_bfloat16* p, float v) { *p = __float2bfloat16(v); }
 
 // x <- x + omega (b - A x) / diag on fluid leaves, 0 elsewhere
-// (multigrid.cpp:102-114); FOLD: per-block partial fluid sums of the result.
-template <int MODE, bool FOLD>
-__global__ void __launch_bounds__(kBlock) k_om_sweep(OctView L, const double* __restrict__ b,
-                                                     const double* __restrict__ x, const double* __restrict__ ec,
-                                                     const int32_t* __restrict__ up, double* __restrict__ xo,
-                                                     RedBuf rb, const SolveCtl* ctl) {
+// (multigrid.cpp:102-114), in T arithmetic with x read through X and the
+// result stored as O; FOLD: per-block partial fluid sums of the result.
+template <class T, class XR, class O, bool FOLD>
+__global__ void __launch_bounds__(kBlock) k_om_sweep(OctViewT<T> L, const T* __restrict__ b, XR X,
+                                                     O* __restrict__ xo, RedBuf rb, const SolveCtl* ctl) {
   if (ctl && ldA(&ctl->done)) return;
   double acc[1] = {0.0};
   OGRID(i, L.n) {
-    double v = 0.0;
+    T v = T(0);
     if (L.mask[i] == 0) {
-      double xi, Ax;
-      if constexpr (MODE == OM_PROLONG) {
-        const ProlongRd X{x, ec, up};
-        xi = X(i);
-        Ax = octRowA(L, i, X);
-      } else {
-        const PlainRd X{x};
-        xi = X(i);
-        Ax = octRowA(L, i, X);
-      }
-      v = xi + kOmega * (b[i] - Ax) / L.diag[i];
-      if (FOLD) acc[0] += v;
+      const T xi = X(i);
+      const T Ax = octRowA(L, i, X);
+      v = xi + T(kOmega) * (b[i] - Ax) / L.diag[i];
+      if (FOLD) acc[0] += double(v);
     }
-    xo[i] = v;
+    storeAs(xo + i, v);
   }
   if constexpr (FOLD) blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
 }
@@ -93,21 +92,27 @@ __global__ void __launch_bounds__(kBlock) k_om_sweep(OctView L, const double* __
 // Residuals b - A x of a coarse cell's children summed in child order
 // (thread per coarse cell); with xc, the coarse level's first sweep from a
 // zero guess, omega*bc/diag_c (fluid cells).
-__global__ void __launch_bounds__(kBlock) k_om_restrict(OctView F, OctView C, const int32_t* __restrict__ cstart,
-                                                        const int32_t* __restrict__ child,
-                                                        const double* __restrict__ b, const double* __restrict__ x,
-                                                        double* __restrict__ bc, double* __restrict__ xc,
+template <class T, class XR>
+__global__ void __launch_bounds__(kBlock) k_om_restrict(OctViewT<T> F, OctViewT<T> C,
+                                                        const int32_t* __restrict__ cstart,
+                                                        const int32_t* __restrict__ child, const T* __restrict__ b,
+                                                        XR X, T* __restrict__ bc, T* __restrict__ xc,
                                                         const SolveCtl* ctl) {
   if (ctl && ldA(&ctl->done)) return;
   const long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= C.n) return;
-  double s = 0.0;
+  T s = T(0);
   for (int32_t p = cstart[I]; p < cstart[I + 1]; ++p) {
     const int32_t i = child[p];
-    if (F.mask[i] == 0) s += b[i] - octRowA(F, i, PlainRd{x});
+    if (F.mask[i] == 0) s += b[i] - octRowA(F, i, X);
   }
   bc[I] = s;
-  if (xc) xc[I] = C.mask[I] == 0 ? kOmega * s / C.diag[I] : 0.0;
+  if (xc) xc[I] = C.mask[I] == 0 ? T(kOmega) * s / C.diag[I] : T(0);
+}
+
+__global__ void k_to_f32(long long n, const double* __restrict__ a, float* __restrict__ o) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o[i] = float(a[i]);
 }
 
 // ---- coarsening ---------------------------------------------------------------
@@ -258,11 +263,12 @@ __global__ void k_sc_diag(long long nc, const uint8_t* __restrict__ mask, const 
   diag[I] = d > 0.0 ? d : 1.0;
 }
 
-__global__ void __launch_bounds__(kBlock) k_om_fsum(OctView L, const double* __restrict__ x, RedBuf rb,
-                                                    const SolveCtl* ctl) {
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_om_fsum(long long n, const uint8_t* __restrict__ mask,
+                                                    const T* __restrict__ x, RedBuf rb, const SolveCtl* ctl) {
   if (ctl && ldA(&ctl->done)) return;
   double acc[1] = {0.0};
-  OGRID(i, L.n) if (L.mask[i] == 0) acc[0] += x[i];
+  OGRID(i, n) if (mask[i] == 0) acc[0] += double(x[i]);
   blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
 }
 
@@ -536,59 +542,138 @@ bool OctMg::buildDirect(hipStream_t s) {
   return true;
 }
 
+void OctMg::setPrecision(int prec, hipStream_t s) {
+  if (prec < NBL_PREC_FP64 || prec > NBL_PREC_MIXED_BF16) throw std::invalid_argument("unknown precision");
+  prec_ = prec;
+  if (prec == NBL_PREC_FP64 || lv_[0].xaF.bytes() > 0) {
+    if (prec == NBL_PREC_MIXED_BF16 && lv_.size() > 1 && lv_[0].xbB.bytes() == 0)
+      lv_[0].xbB = DevBuf(ctx_, lv_[0].op->n * sizeof(__hip_bfloat16));
+    return;
+  }
+  auto toF = [&](const DevBuf& d, size_t n) {
+    DevBuf f(ctx_, std::max<size_t>(1, n) * sizeof(float));
+    if (n > 0) k_to_f32<<<gridFor((long long)n), kBlock, 0, s>>>((long long)n, d.as<double>(), f.as<float>());
+    NBL_LAUNCH_CHECK();
+    return f;
+  };
+  for (Lvl& L : lv_) {
+    const OctLevel& o = *L.op;
+    L.scoefF = toF(o.scoef, size_t(kOctSlots) * o.n);
+    L.xcoefF = toF(o.xcoef, o.nx);
+    L.diagF = toF(o.diag, o.n);
+    L.bF = DevBuf(ctx_, o.n * sizeof(float));
+    L.xaF = DevBuf(ctx_, o.n * sizeof(float));
+    L.xbF = DevBuf(ctx_, o.n * sizeof(float));
+  }
+  if (prec == NBL_PREC_MIXED_BF16 && lv_.size() > 1)
+    lv_[0].xbB = DevBuf(ctx_, lv_[0].op->n * sizeof(__hip_bfloat16));
+  NBL_HIP(hipStreamSynchronize(s));
+}
+
+const void* OctMg::diag0() const {
+  return fp32() ? lv_[0].diagF.as<void>() : lv_[0].op->diag.as<void>();
+}
+
+template <>
+OctViewT<double> OctMg::view<double>(size_t l) const {
+  return octView(*lv_[l].op);
+}
+template <>
+OctViewT<float> OctMg::view<float>(size_t l) const {
+  const Lvl& L = lv_[l];
+  const OctLevel& o = *L.op;
+  return OctViewT<float>{(long long)o.n,          o.mask.as<uint8_t>(),  o.cnt.as<uint8_t>(),
+                         o.scol.as<int32_t>(),    L.scoefF.as<float>(),  o.xrow.as<int32_t>(),
+                         o.xcol.as<int32_t>(),    L.xcoefF.as<float>(),  L.diagF.as<float>()};
+}
+
 // V-cycle on level l with right-hand side b; the level's first sweep from a
-// zero guess is already in xa.  *out receives the level's result.
-void OctMg::cycle(size_t l, const double* b, RedBuf rb, const SolveCtl* ctl, hipStream_t s, const double** out) {
+// zero guess is already in xa.  *out receives the level's result.  On level
+// 0 of the BF16 form the pre-smoothed iterate (written by the second
+// pre-sweep, read by the restriction and the prolongating post-sweep) is
+// stored as BF16.
+template <class T>
+void OctMg::cycle(size_t l, const T* b, RedBuf rb, const SolveCtl* ctl, hipStream_t s, const T** out) {
+  constexpr bool kF = sizeof(T) == sizeof(float);
   Lvl& L = lv_[l];
-  const OctView V = octView(*L.op);
+  const OctViewT<T> V = view<T>(l);
   const long long n = V.n;
   const bool fold = l == 0;
   const bool last = l + 1 == lv_.size();
-  double* xa = L.xa.as<double>();
-  double* xb = L.xb.as<double>();
+  T* xa = kF ? L.xaF.as<T>() : L.xa.as<T>();
+  T* xb = kF ? L.xbF.as<T>() : L.xb.as<T>();
+  const int g = omGrid(n);
   if (last) {
     if (direct_) {
       MgLevel ml{};
       ml.n = n;
       ml.code = code_.as<uint16_t>();
       ml.nf = (long long)L.op->nf;
-      launch_mg_coarse_direct<double>(ml, M_.as<double>(), Mf_.as<float>(), b, MeanRef{}, xb, ctl, s);
+      launch_mg_coarse_direct<T>(ml, M_.as<double>(), Mf_.as<float>(), b, MeanRef{}, xb, ctl, s);
     } else {
       // damped Jacobi from the first sweep already in xa; an odd number of
       // further sweeps ends in xb
       for (int q = 0; q < kFallbackSweeps - 1; ++q) {
         const bool even = (q & 1) == 0;
-        k_om_sweep<OM_PLAIN, false><<<omGrid(n), kBlock, 0, s>>>(V, b, even ? xa : xb, nullptr, nullptr,
-                                                                 even ? xb : xa, rb, ctl);
+        k_om_sweep<T, PlainRd<T>, T, false><<<g, kBlock, 0, s>>>(V, b, PlainRd<T>{even ? xa : xb}, even ? xb : xa,
+                                                                 rb, ctl);
       }
       NBL_LAUNCH_CHECK();
     }
-    if (fold) k_om_fsum<<<omGrid(n), kBlock, 0, s>>>(V, xb, rb, ctl);
+    if (fold) k_om_fsum<T><<<g, kBlock, 0, s>>>(n, V.mask, xb, rb, ctl);
     NBL_LAUNCH_CHECK();
     *out = xb;
     return;
   }
-  // pre-smoothing: first sweep (in xa) then one plain sweep
-  k_om_sweep<OM_PLAIN, false><<<omGrid(n), kBlock, 0, s>>>(V, b, xa, nullptr, nullptr, xb, rb, ctl);
   Lvl& C = lv_[l + 1];
-  const OctView VC = octView(*C.op);
-  k_om_restrict<<<gridFor(VC.n), kBlock, 0, s>>>(V, VC, C.cstart.as<int32_t>(), C.child.as<int32_t>(), b, xb,
-                                                 C.b.as<double>(), C.xa.as<double>(), ctl);
+  const OctViewT<T> VC = view<T>(l + 1);
+  T* bc = kF ? C.bF.as<T>() : C.b.as<T>();
+  T* xc = kF ? C.xaF.as<T>() : C.xa.as<T>();
+  const T* ec = nullptr;
+  auto post = [&](auto xr) {
+    using R = ProlongRd<T, decltype(xr)>;
+    k_om_sweep<T, R, T, false><<<g, kBlock, 0, s>>>(V, b, R{xr, ec, L.up.as<int32_t>()}, xa, rb, ctl);
+    if (fold)
+      k_om_sweep<T, PlainRd<T>, T, true><<<g, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xb, rb, ctl);
+    else
+      k_om_sweep<T, PlainRd<T>, T, false><<<g, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xb, rb, ctl);
+    NBL_LAUNCH_CHECK();
+  };
+  const int gc = gridFor(VC.n);
+  if constexpr (kF) {
+    if (l == 0 && prec_ == NBL_PREC_MIXED_BF16) {
+      __hip_bfloat16* xbB = L.xbB.as<__hip_bfloat16>();
+      k_om_sweep<T, PlainRd<T>, __hip_bfloat16, false><<<g, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xbB, rb, ctl);
+      k_om_restrict<T, Bf16Rd><<<gc, kBlock, 0, s>>>(V, VC, C.cstart.as<int32_t>(), C.child.as<int32_t>(), b,
+                                                    Bf16Rd{xbB}, bc, xc, ctl);
+      NBL_LAUNCH_CHECK();
+      cycle<T>(l + 1, bc, rb, ctl, s, &ec);
+      post(Bf16Rd{xbB});
+      *out = xb;
+      return;
+    }
+  }
+  // pre-smoothing: first sweep (in xa) then one plain sweep
+  k_om_sweep<T, PlainRd<T>, T, false><<<g, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xb, rb, ctl);
+  k_om_restrict<T, PlainRd<T>><<<gc, kBlock, 0, s>>>(V, VC, C.cstart.as<int32_t>(), C.child.as<int32_t>(), b,
+                                                     PlainRd<T>{xb}, bc, xc, ctl);
   NBL_LAUNCH_CHECK();
-  const double* ec = nullptr;
-  cycle(l + 1, C.b.as<double>(), rb, ctl, s, &ec);
+  cycle<T>(l + 1, bc, rb, ctl, s, &ec);
   // post-smoothing: prolongation fused into the first sweep
-  k_om_sweep<OM_PROLONG, false><<<omGrid(n), kBlock, 0, s>>>(V, b, xb, ec, L.up.as<int32_t>(), xa, rb, ctl);
-  if (fold)
-    k_om_sweep<OM_PLAIN, true><<<omGrid(n), kBlock, 0, s>>>(V, b, xa, nullptr, nullptr, xb, rb, ctl);
-  else
-    k_om_sweep<OM_PLAIN, false><<<omGrid(n), kBlock, 0, s>>>(V, b, xa, nullptr, nullptr, xb, rb, ctl);
-  NBL_LAUNCH_CHECK();
+  post(PlainRd<T>{xb});
   *out = xb;
 }
 
 void OctMg::vcycle(const double* r, RedBuf rb, const SolveCtl* ctl, hipStream_t s) {
-  cycle(0, r, rb, ctl, s, &res0_);
+  if (fp32()) {
+    const float* out = nullptr;
+    cycle<float>(0, lv_[0].bF.as<float>(), rb, ctl, s, &out);
+    res0_ = out;
+  } else {
+    const double* out = nullptr;
+    cycle<double>(0, r, rb, ctl, s, &out);
+    res0_ = out;
+  }
 }
 
 }  // namespace nbl

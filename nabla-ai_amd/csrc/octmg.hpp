@@ -24,14 +24,26 @@ class OctMg {
   long long count(int l) const { return (long long)lv_[size_t(l)].op->n; }
   double buildMs() const { return buildMs_; }
   bool directCoarse() const { return direct_; }
-  // First sweep of level 0 from a zero guess: the caller writes omega*r/diag
-  // (fluid leaves, 0 elsewhere) here before vcycle().
-  double* firstSweep() { return lv_[0].xa.as<double>(); }
-  // One V(2,2) cycle for right-hand side r on level 0.  Its result is
-  // result(); the last kernel writes `partials()` per-block partial fluid sums
-  // of it into rb.
+  // Arithmetic of the cycle: NBL_PREC_FP64 (level vectors and operators in
+  // FP64), NBL_PREC_MIXED (FP32 copies of both) or NBL_PREC_MIXED_BF16 (FP32,
+  // with level 0's pre-smoothed iterate stored as BF16 between the sweeps
+  // that write and read it).  FP32 copies are made on first use.
+  void setPrecision(int prec, hipStream_t s);
+  int precision() const { return prec_; }
+  bool fp32() const { return prec_ != 0; }
+  // Level 0's right-hand side (FP32 forms: the caller writes float(r) here;
+  // the FP64 form reads the caller's r) and its first sweep from a zero
+  // guess: the caller writes omega*r/diag (fluid leaves, 0 elsewhere) in the
+  // cycle's type before vcycle().  diag0() is level 0's diagonal in that type.
+  float* rhs0F() { return lv_[0].bF.as<float>(); }
+  void* firstSweep() { return fp32() ? lv_[0].xaF.as<void>() : lv_[0].xa.as<void>(); }
+  const void* diag0() const;
+  // One V(2,2) cycle for right-hand side r on level 0 (FP64 form; the FP32
+  // forms read rhs0F()).  Its result (double or float as the form) is
+  // result(); the last kernel writes `partials()` per-block partial fluid
+  // sums of it into rb.
   void vcycle(const double* r, RedBuf rb, const SolveCtl* ctl, hipStream_t s);
-  const double* result() const { return res0_; }
+  const void* result() const { return res0_; }
   int partials() const { return parts_; }
 
  private:
@@ -41,18 +53,24 @@ class OctMg {
     DevBuf up;                  // i32[n]: parent in level l+1
     DevBuf cstart;              // i32[n+1]: children of cell I at child[cstart[I] .. cstart[I+1])
     DevBuf child;               // i32[n of level l-1]: cells of level l-1 grouped by parent
-    DevBuf b, xa, xb;
+    DevBuf b, xa, xb;           // FP64 level vectors
+    DevBuf scoefF, xcoefF, diagF, bF, xaF, xbF;  // FP32 operator copy and level vectors
+    DevBuf xbB;                 // level 0, BF16 form: the pre-smoothed iterate
     int S[3] = {0, 0, 0};       // target log2 box size per axis of this level
   };
   void buildCoarse(const OctreeDev& fine, hipStream_t s);
   bool buildDirect(hipStream_t s);
-  void cycle(size_t l, const double* b, RedBuf rb, const SolveCtl* ctl, hipStream_t s, const double** out);
+  template <class T>
+  OctViewT<T> view(size_t l) const;
+  template <class T>
+  void cycle(size_t l, const T* b, RedBuf rb, const SolveCtl* ctl, hipStream_t s, const T** out);
 
   nbl_ctx* ctx_;
   std::vector<Lvl> lv_;
   bool direct_ = false;
+  int prec_ = 0;
   DevBuf code_, M_, Mf_;
-  const double* res0_ = nullptr;
+  const void* res0_ = nullptr;
   int parts_ = 0;
   double buildMs_ = 0.0;
 };

@@ -430,12 +430,29 @@ __global__ void __launch_bounds__(kBlock) k_count_fluid(long long n, const uint8
 // ---- multigrid-preconditioned flexible CG (multigrid.cpp:267-355 on leaves) --
 constexpr double kOmegaFirst = 0.8;  // first V-cycle sweep from zero: omega*r/diag
 
-// rhs <- rhs - mean (fluid), 0 elsewhere; r = rhs; first sweep of the
-// V-cycle; partials of |rhs|^2.
+// Level 0's right-hand side and first sweep of the V-cycle from residual
+// value v of leaf c: FP64 cycle, omega*v/diag; FP32 cycle, its FP32 copy of v
+// and omega*float(v)/diag in FP32 (the cycle's own arithmetic).
+template <class T>
+__device__ __forceinline__ void cycleInput(long long c, bool fl, double v, const double* diag, const T* diagT,
+                                           T* rT, T* x0) {
+  if constexpr (sizeof(T) == sizeof(float)) {
+    const float vf = float(v);
+    rT[c] = vf;
+    x0[c] = fl ? float(kOmegaFirst) * vf / diagT[c] : 0.f;
+  } else {
+    x0[c] = fl ? kOmegaFirst * v / diag[c] : 0.0;
+  }
+}
+
+// rhs <- rhs - mean (fluid), 0 elsewhere; r = rhs; the V-cycle's input;
+// partials of |rhs|^2.
+template <class T>
 __global__ void __launch_bounds__(kBlock) k_om_init2(long long n, long long nf, const uint8_t* __restrict__ mask,
-                                                     const double* __restrict__ diag, double* __restrict__ rhs,
-                                                     double* __restrict__ r, double* __restrict__ x0,
-                                                     const SolveCtl* ctl, RedBuf rb) {
+                                                     const double* __restrict__ diag, const T* __restrict__ diagT,
+                                                     double* __restrict__ rhs, double* __restrict__ r,
+                                                     T* __restrict__ rT, T* __restrict__ x0, const SolveCtl* ctl,
+                                                     RedBuf rb) {
   const double m = ldA(&ctl->slot[O_MEAN_RHS]);
   double acc[1] = {0.0};
   OGRID(c, n) {
@@ -443,22 +460,24 @@ __global__ void __launch_bounds__(kBlock) k_om_init2(long long n, long long nf, 
     const double v = fl ? (nf ? rhs[c] - m : rhs[c]) : 0.0;
     rhs[c] = v;
     r[c] = v;
-    x0[c] = fl ? kOmegaFirst * v / diag[c] : 0.0;
+    cycleInput(c, fl, v, diag, diagT, rT, x0);
     acc[0] += v * v;
   }
   blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
 }
 
 // z = V-cycle output - mean (fluid), 0 elsewhere; p = z; partials of r.z.
+template <class T>
 __global__ void __launch_bounds__(kBlock) k_om_start(long long n, long long nf, const uint8_t* __restrict__ mask,
-                                                     const double* __restrict__ zc, const double* __restrict__ r,
+                                                     const T* __restrict__ zc, const double* __restrict__ r,
                                                      double* __restrict__ z, double* __restrict__ p,
                                                      const SolveCtl* ctl, RedBuf rb) {
   if (ldA(&ctl->done)) return;
   const double m = ldA(&ctl->slot[O_MEAN_Z]);
   double acc[1] = {0.0};
   OGRID(c, n) {
-    const double zv = mask[c] == 0 ? (nf ? zc[c] - m : zc[c]) : 0.0;
+    const double zo = double(zc[c]);
+    const double zv = mask[c] == 0 ? (nf ? zo - m : zo) : 0.0;
     z[c] = zv;
     p[c] = zv;
     acc[0] += r[c] * zv;
@@ -483,11 +502,13 @@ __global__ void __launch_bounds__(kBlock) k_om_update(long long n, const uint8_t
   blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
 }
 
-// rn <- rn - mean (fluid), 0 elsewhere; first sweep of the next V-cycle;
-// partials of |rn|^2.
+// rn <- rn - mean (fluid), 0 elsewhere; the next V-cycle's input; partials
+// of |rn|^2.
+template <class T>
 __global__ void __launch_bounds__(kBlock) k_om_rfix(long long n, long long nf, const uint8_t* __restrict__ mask,
-                                                    const double* __restrict__ diag, double* __restrict__ rn,
-                                                    double* __restrict__ x0, const SolveCtl* ctl, RedBuf rb) {
+                                                    const double* __restrict__ diag, const T* __restrict__ diagT,
+                                                    double* __restrict__ rn, T* __restrict__ rT, T* __restrict__ x0,
+                                                    const SolveCtl* ctl, RedBuf rb) {
   if (ldA(&ctl->done)) return;
   const double m = ldA(&ctl->slot[O_MEAN_R]);
   double acc[1] = {0.0};
@@ -495,22 +516,24 @@ __global__ void __launch_bounds__(kBlock) k_om_rfix(long long n, long long nf, c
     const bool fl = mask[c] == 0;
     const double v = fl ? (nf ? rn[c] - m : rn[c]) : 0.0;
     rn[c] = v;
-    x0[c] = fl ? kOmegaFirst * v / diag[c] : 0.0;
+    cycleInput(c, fl, v, diag, diagT, rT, x0);
     acc[0] += v * v;
   }
   blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
 }
 
 // z = V-cycle output - mean (fluid), 0 elsewhere; partials of z.rn and z.r.
+template <class T>
 __global__ void __launch_bounds__(kBlock) k_om_dots(long long n, long long nf, const uint8_t* __restrict__ mask,
-                                                    const double* __restrict__ zc, const double* __restrict__ rn,
+                                                    const T* __restrict__ zc, const double* __restrict__ rn,
                                                     const double* __restrict__ r, double* __restrict__ z,
                                                     const SolveCtl* ctl, RedBuf rb) {
   if (ldA(&ctl->done)) return;
   const double m = ldA(&ctl->slot[O_MEAN_Z]);
   double acc[2] = {0.0, 0.0};
   OGRID(c, n) {
-    const double zv = mask[c] == 0 ? (nf ? zc[c] - m : zc[c]) : 0.0;
+    const double zo = double(zc[c]);
+    const double zv = mask[c] == 0 ? (nf ? zo - m : zo) : 0.0;
     z[c] = zv;
     acc[0] += zv * rn[c];
     acc[1] += zv * r[c];
@@ -1224,7 +1247,14 @@ nbl_poisson_report OctreeDev::solve(const double* b, double* x, double tol, int 
 
 OctMg& OctreeDev::mg() {
   if (!mg_) mg_ = std::make_unique<OctMg>(ctx_, *this, stream_);
+  if (mg_->precision() != mgPrec_) mg_->setPrecision(mgPrec_, stream_);
   return *mg_;
+}
+
+void OctreeDev::setMgPrecision(int prec) {
+  if (prec < NBL_PREC_FP64 || prec > NBL_PREC_MIXED_BF16)
+    throw std::invalid_argument("precision must be NBL_PREC_FP64, NBL_PREC_MIXED or NBL_PREC_MIXED_BF16");
+  mgPrec_ = prec;
 }
 
 int OctreeDev::mgLevels(long long* counts, int cap, double* buildMs) {
@@ -1262,11 +1292,23 @@ nbl_poisson_report OctreeDev::solveMgSorted(const double* b, double* xo, double 
   };
   k_o_init<<<g, kBlock, 0, s>>>(N, mk, b, rhs, x, rb);
   fin(FIN_MEAN, 1, O_MEAN_RHS, false);
-  k_om_init2<<<g, kBlock, 0, s>>>(N, nf, mk, dg, rhs, r, M.firstSweep(), ctl, rb);
+  // the V-cycle's level type: its input is written and its result read as double or float
+  const bool f32 = M.fp32();
+  float* rF = M.rhs0F();
+  const float* dgF = static_cast<const float*>(M.diag0());
+  if (f32)
+    k_om_init2<float><<<g, kBlock, 0, s>>>(N, nf, mk, dg, dgF, rhs, r, rF, static_cast<float*>(M.firstSweep()), ctl,
+                                           rb);
+  else
+    k_om_init2<double><<<g, kBlock, 0, s>>>(N, nf, mk, dg, dg, rhs, r, nullptr,
+                                            static_cast<double*>(M.firstSweep()), ctl, rb);
   fin(FIN_BNORM, 1, 0, false, 1);
   M.vcycle(r, rb, ctl, s);
   fin(FIN_MEAN, 1, O_MEAN_Z, true);
-  k_om_start<<<g, kBlock, 0, s>>>(N, nf, mk, M.result(), r, z, p, ctl, rb);
+  if (f32)
+    k_om_start<float><<<g, kBlock, 0, s>>>(N, nf, mk, static_cast<const float*>(M.result()), r, z, p, ctl, rb);
+  else
+    k_om_start<double><<<g, kBlock, 0, s>>>(N, nf, mk, static_cast<const double*>(M.result()), r, z, p, ctl, rb);
   fin(FIN_RZ, 1, 0, true);
   NBL_LAUNCH_CHECK();
   int enq = 0;
@@ -1277,11 +1319,19 @@ nbl_poisson_report OctreeDev::solveMgSorted(const double* b, double* xo, double 
       fin(FIN_ALPHA, 1, 0, true);
       k_om_update<<<g, kBlock, 0, s>>>(N, mk, x, p, r, Ap, rn, ctl, rb);
       fin(FIN_MEAN, 1, O_MEAN_R, true);
-      k_om_rfix<<<g, kBlock, 0, s>>>(N, nf, mk, dg, rn, M.firstSweep(), ctl, rb);
+      if (f32)
+        k_om_rfix<float><<<g, kBlock, 0, s>>>(N, nf, mk, dg, dgF, rn, rF, static_cast<float*>(M.firstSweep()), ctl,
+                                              rb);
+      else
+        k_om_rfix<double><<<g, kBlock, 0, s>>>(N, nf, mk, dg, dg, rn, nullptr, static_cast<double*>(M.firstSweep()),
+                                               ctl, rb);
       fin(FIN_CONV, 1, 0, true);
       M.vcycle(rn, rb, ctl, s);
       fin(FIN_MEAN, 1, O_MEAN_Z, true);
-      k_om_dots<<<g, kBlock, 0, s>>>(N, nf, mk, M.result(), rn, r, z, ctl, rb);
+      if (f32)
+        k_om_dots<float><<<g, kBlock, 0, s>>>(N, nf, mk, static_cast<const float*>(M.result()), rn, r, z, ctl, rb);
+      else
+        k_om_dots<double><<<g, kBlock, 0, s>>>(N, nf, mk, static_cast<const double*>(M.result()), rn, r, z, ctl, rb);
       fin(FIN_BETA_FLEX, 2, 0, true);
       k_om_pupdate<<<g, kBlock, 0, s>>>(N, p, z, r, rn, ctl);
     }

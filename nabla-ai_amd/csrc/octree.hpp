@@ -56,18 +56,22 @@ __host__ __device__ inline uint64_t mortonEnc3(uint32_t x, uint32_t y, uint32_t 
   return mortonSpread(x) | (mortonSpread(y) << 1) | (mortonSpread(z) << 2);
 }
 
-// Device view of a level's operator.
-struct OctView {
+// Device view of a level's operator with coefficients and diagonal in T
+// (double: the operator itself; float: its FP32 copy for the mixed-precision
+// V-cycle).
+template <class T>
+struct OctViewT {
   long long n;
   const uint8_t* mask;
   const uint8_t* cnt;
   const int32_t* scol;
-  const double* scoef;
+  const T* scoef;
   const int32_t* xrow;
   const int32_t* xcol;
-  const double* xcoef;
-  const double* diag;
+  const T* xcoef;
+  const T* diag;
 };
+using OctView = OctViewT<double>;
 inline OctView octView(const OctLevel& L) {
   return OctView{(long long)L.n,       L.mask.as<uint8_t>(),  L.cnt.as<uint8_t>(),  L.scol.as<int32_t>(),
                  L.scoef.as<double>(), L.xrow.as<int32_t>(),  L.xcol.as<int32_t>(), L.xcoef.as<double>(),
@@ -75,12 +79,12 @@ inline OctView octView(const OctLevel& L) {
 }
 
 // sum_e coef_e (x_i - x_col_e) over row i's couplings in the reference's order
-// (octree_poisson.cpp:64-85), x read through X(j).
-template <class Rd>
-__device__ __forceinline__ double octRowA(const OctView& L, long long i, const Rd& X) {
-  const double xi = X(i);
+// (octree_poisson.cpp:64-85), x read through X(j), in T arithmetic.
+template <class T, class Rd>
+__device__ __forceinline__ T octRowA(const OctViewT<T>& L, long long i, const Rd& X) {
+  const T xi = X(i);
   const int c = L.cnt[i];
-  double acc = 0.0;
+  T acc = T(0);
 #pragma unroll
   for (int s = 0; s < kOctSlots; ++s)
     if (s < c) {
@@ -141,6 +145,11 @@ class OctreeDev {
   // Multigrid hierarchy: level count, leaves per level (up to `cap` entries)
   // and the device time of its construction (ms).  Builds it if needed.
   int mgLevels(long long* counts, int cap, double* buildMs);
+  // Arithmetic of the multigrid V-cycle (NBL_PREC_*): FP64 levels, FP32
+  // levels, or FP32 levels with BF16 storage of level 0's pre-smoothed
+  // iterate.  The flexible CG around it stays FP64.
+  void setMgPrecision(int prec);
+  int mgPrecision() const { return mgPrec_; }
 
   const OctLevel& level0() const { return L0_; }
   int maxLevel() const { return maxL_; }
@@ -167,6 +176,7 @@ class OctreeDev {
   DevBuf ctl_, red_;
   SolveCtl* hctl_ = nullptr;
   int predicted_ = 16, predictedMg_ = 8;
+  int mgPrec_ = 0;
   hipEvent_t e0_ = nullptr, e1_ = nullptr;
   double setupMs_ = 0.0, lastMs_ = 0.0;
   std::unique_ptr<OctMg> mg_;

@@ -241,6 +241,7 @@ def load(path: os.PathLike | None = None):
     _proto(lib, "nbl_octree_mg_solve", i, V, P_D, P_D, d, i, C.POINTER(PoissonReport))
     _proto(lib, "nbl_octree_mg_levels", i, V, C.POINTER(C.c_longlong), i, C.POINTER(i),
            C.POINTER(C.c_double))
+    _proto(lib, "nbl_octree_set_precision", i, V, i)
     _proto(lib, "nbl_octree_profile", i, V, i, i, C.POINTER(C.c_double), C.POINTER(C.c_double))
     _proto(lib, "nbl_octree_last_timing", i, V, C.POINTER(C.c_double), C.POINTER(C.c_double))
     _proto(lib, "nbl_run_sweep_timing", i, V, i, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
@@ -608,6 +609,12 @@ class Octree:
         check(self.lib.nbl_octree_mg_solve(self.h, dptr(b), dptr(x), tol, max_iters,
                                            C.byref(rep)))
         return x, rep
+
+    def set_precision(self, precision):
+        """Arithmetic of mg_solve's V-cycle: PREC_FP64, PREC_MIXED (FP32 levels
+        inside the FP64 flexible CG) or PREC_MIXED_BF16 (FP32 levels, level 0's
+        pre-smoothed iterate stored as BF16)."""
+        check(self.lib.nbl_octree_set_precision(self.h, int(precision)))
 
     def mg_levels(self, cap=32):
         """(leaves per multigrid level, device ms of the hierarchy build)."""

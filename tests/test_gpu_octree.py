@@ -279,6 +279,107 @@ def test_c3_mg_solve(ctx, c3):
         d.close()
 
 
+PRECS = [(nbl.PREC_MIXED, "fp32"), (nbl.PREC_MIXED_BF16, "bf16")]
+
+
+@pytest.mark.parametrize("prec,pname", PRECS, ids=[p for _, p in PRECS])
+@pytest.mark.parametrize("which", [0, 1], ids=["uniform32", "adaptive"])
+def test_octree_mg_mixed_matches_oracle(ctx, mg_tree_list, which, prec, pname):
+    """FP32 V-cycle (and its BF16 form) inside the FP64 flexible CG: the FP64
+    residual still reaches the tolerance, the solution agrees with the
+    oracle's FP64 multigrid CG to 1e-8 (rel-L2), and the rounder
+    preconditioner costs at most a few iterations."""
+    name, t = mg_tree_list[which]
+    d = device_tree(ctx, t)
+    try:
+        _k, _l, m = t.dump()
+        fl = m == 0
+        d.set_precision(prec)
+        for seed in (6, 7):
+            b = _projected_rhs(seed, m)
+            x, rep = d.mg_solve(b, 1e-10, 300)
+            xo, ro = t.mg_solve(b, 1e-10, 300)
+            print(f"{name} {pname} seed {seed}: iterations {rep.iterations} (oracle FP64 {ro.iterations}), "
+                  f"rel {rel(x, xo):.2e}", flush=True)
+            assert rep.converged and rep.residual <= 1e-10 * 1.01
+            assert ro.iterations - 1 <= rep.iterations <= ro.iterations + 4
+            assert rel(x, xo) < 1e-8
+            assert abs(x[fl].mean()) < 1e-12 * max(np.abs(x).max(), 1.0)
+            r = d.apply(x) - b
+            assert np.linalg.norm(r[fl]) <= 1.01e-10 * np.linalg.norm(b)
+    finally:
+        d.close()
+
+
+def test_octree_mg_precision_switch(ctx, mg_tree_list):
+    """Switching the cycle's precision on one octree: an FP64 solve after an
+    FP32 and a BF16 one returns bit for bit what the first FP64 solve
+    returned; invalid values are rejected."""
+    _name, t = mg_tree_list[1]
+    d = device_tree(ctx, t)
+    try:
+        _k, _l, m = t.dump()
+        b = _projected_rhs(9, m)
+        x0, r0 = d.mg_solve(b, 1e-10, 300)
+        for p in (nbl.PREC_MIXED, nbl.PREC_MIXED_BF16, nbl.PREC_MIXED):
+            d.set_precision(p)
+            xp, rp = d.mg_solve(b, 1e-10, 300)
+            assert rp.converged
+        d.set_precision(nbl.PREC_FP64)
+        x1, r1 = d.mg_solve(b, 1e-10, 300)
+        assert np.array_equal(x0, x1) and (r0.iterations, r0.residual) == (r1.iterations, r1.residual)
+        for bad in (-1, 3):
+            with pytest.raises(nbl.NblError):
+                d.set_precision(bad)
+    finally:
+        d.close()
+
+
+@pytest.mark.parametrize("which", [0, 1, 2], ids=["uniform8", "jump", "storm_masked"])
+def test_octree_mg_mixed_single_level(ctx, tree_list, which):
+    """Trees of at most 2048 leaves: the FP32 form applies the FP32 copy of the
+    projected inverse, so the FP64 flexible CG needs a few iterations instead
+    of one and still reaches the oracle's solution."""
+    _name, t = tree_list[which]
+    d = device_tree(ctx, t)
+    try:
+        d.set_precision(nbl.PREC_MIXED_BF16)  # single level: no BF16 scratch, the FP32 form
+        _k, _l, m = t.dump()
+        b = _projected_rhs(8, m)
+        x, rep = d.mg_solve(b, 1e-10, 50)
+        xo, ro = t.mg_solve(b, 1e-10, 50)
+        assert rep.converged and 1 <= rep.iterations <= 6
+        assert rel(x, xo) < 1e-8
+    finally:
+        d.close()
+
+
+@pytest.mark.parametrize("prec,pname", PRECS, ids=[p for _, p in PRECS])
+def test_c3_mg_solve_mixed(ctx, c3, prec, pname):
+    """The 4.3M-leaf cube-wake tree with the FP32 (BF16) V-cycle: converges to
+    the FP64 tolerance 1e-9 within a few iterations of the FP64 cycle, A x
+    reproduces the projected right-hand side, and the solution agrees with
+    the FP64 multigrid CG's."""
+    _f, k, l, m, _h = c3
+    d = nbl.Octree(ctx, k, l, m, cases.C3_ORIGIN, cases.C3_EXTENT, 8)
+    try:
+        fl = m == 0
+        b = _projected_rhs(11, m)
+        x64, r64 = d.mg_solve(b, 1e-9, 500)
+        d.set_precision(prec)
+        x, rep = d.mg_solve(b, 1e-9, 500)
+        print(f"C3 {pname}: iterations {rep.iterations} (FP64 cycle {r64.iterations}), rel {rel(x, x64):.2e}",
+              flush=True)
+        assert rep.converged and rep.residual < 1e-9 * 1.01
+        assert rep.iterations <= r64.iterations + 4
+        r = d.apply(x) - b
+        assert np.linalg.norm(r[fl]) <= 2e-9 * np.linalg.norm(b)
+        assert abs(x[fl].mean()) < 1e-12 * max(np.abs(x).max(), 1.0)
+        assert rel(x, x64) < 1e-6
+    finally:
+        d.close()
+
+
 # ---- octreeTransportStep (adaptive/transport.cpp:24-60) ----------------------
 @pytest.mark.parametrize("which", [0, 1, 2], ids=["uniform8", "jump", "storm_masked"])
 def test_transport_bitwise(ctx, tree_list, which):
