@@ -138,6 +138,15 @@ typedef struct nbl_solver_opts {
                                projections away (10 launches per iteration
                                instead of 21), 1 otherwise (measured faster
                                there)                                      */
+  int launch_order;         /* order in which the level kernels' workgroups
+                               walk a level (workgroups go to the 8 XCDs
+                               round-robin, each XCD has its own L2): 1 plain
+                               launch order, 2 slabs of planes per XCD, 3
+                               bands of rows per XCD, 0 (default) 2 on levels
+                               below 8M cells with at least 8 planes, 3 from
+                               8M cells, 1 otherwise.  Per-cell results are
+                               bit-identical across orders; sums differ only
+                               through their partial-sum tree              */
 } nbl_solver_opts;
 
 /* StepReport (ns_solver.hpp:34-49). */

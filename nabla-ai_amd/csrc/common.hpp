@@ -196,22 +196,34 @@ __device__ __forceinline__ Blk3 blk3Tiled() {
   return r;
 }
 
-// Block coordinates of a cell-kernel launch without integer division.  Plain
-// order (slab = 0): the launch coordinates.  XCD slabs (slab = 1): the launch
-// is 8 gx x gy x ceil(gz / 8) blocks and block x' runs tile x' >> 3 of plane
-// (x' & 7) * gridDim.z + z'; workgroups go to the XCDs round-robin by linear
-// id, which is x' mod 8 here, so XCD s receives planes [s * gridDim.z,
-// (s + 1) * gridDim.z) in order and the k +- 1 planes a stencil shares stay in
-// its L2.  Planes past the grid's last fall outside it like any padding.
+// Block coordinates of a cell-kernel launch without integer division.
+// Workgroups go to the 8 XCDs round-robin by linear id, and each XCD has its
+// own L2.  Plain order (slab = 0): the launch coordinates.  XCD slabs
+// (slab = 1): the launch is 8 gx x gy x ceil(gz / 8) blocks and block x' runs
+// tile x' >> 3 of plane (x' & 7) * gridDim.z + z'; its linear id is x' mod 8,
+// so XCD s receives planes [s * gridDim.z, (s + 1) * gridDim.z) in order and
+// the k +- 1 planes a stencil shares stay in its L2.  Planes past the grid's
+// last fall outside it like any padding.  Row bands (slab = 2): the launch is
+// 8 gx x ceil(gy / 8) x gz blocks and block x' runs tile x' >> 3 of block row
+// (x' & 7) * gridDim.y + y' in plane z', so XCD s receives one band of block
+// rows of every plane, plane by plane: the i and j halos a tile shares with
+// its neighbours and the k +- 1 planes stay in one L2 (except on the band's
+// two edge rows), while all XCDs stream the same plane at a time.  Rows past
+// the grid's last fall outside it.
 __device__ __forceinline__ Blk3 blk3(int slab) {
   Blk3 r;
   r.lin = int(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
-  r.y = int(blockIdx.y);
-  if (slab) {
+  if (slab == 1) {
     r.x = int(blockIdx.x >> 3);
+    r.y = int(blockIdx.y);
     r.z = int(blockIdx.x & 7u) * int(gridDim.z) + int(blockIdx.z);
+  } else if (slab == 2) {
+    r.x = int(blockIdx.x >> 3);
+    r.y = int(blockIdx.x & 7u) * int(gridDim.y) + int(blockIdx.y);
+    r.z = int(blockIdx.z);
   } else {
     r.x = int(blockIdx.x);
+    r.y = int(blockIdx.y);
     r.z = int(blockIdx.z);
   }
   return r;

@@ -37,7 +37,7 @@ struct MgLevel {
   int fx, fy, fz;         // factor from the next-finer level (coarse levels)
   long long nf;           // fluid cells
   int vec;                // cells per thread along i in the cell kernels (1, 2 or 4; divides nx)
-  int slab;               // cell-kernel launches in XCD slabs along k (blk3)
+  int slab;               // launch order of the cell kernels (blk3): 0 plain, 1 XCD slabs of planes, 2 XCD bands of rows
 };
 
 constexpr uint16_t kFluidBit = 1u << 12;
@@ -93,10 +93,13 @@ struct Geo3 {
 };
 Geo3 geo3(int nx, int ny, int nz);
 Geo3 geo3n(int nx, int ny, int nz, int cells, int slab = 0);  // `cells` consecutive cells along i per thread
-// Whether a level's cell kernels run in XCD slabs: levels below 8M cells with
-// at least 8 planes.  On the 192 x 64 x 64 channel slabs make the step 3%
-// faster than plain order; on 256^3 plain order is faster (FP64 sweep 96 vs
-// 102 us, A p 84 vs 91 us) although it fetches 24-34% more from HBM.
+// Automatic launch order of a level's cell kernels (blk3's slab argument): XCD
+// slabs of planes (1) on levels below 8M cells with at least 8 planes, bands
+// of rows (2) from 8M cells, plain order (0) otherwise.  On the 192 x 64 x 64
+// channel slabs make the step 3% faster than plain order; on 256^3, 512 x 256
+// x 128 and 320 x 320 x 160 bands run A p 3-4% faster than plain order (78-79
+// vs 81 us) and the FP64 sweep equally fast, while slabs of planes are slower
+// than both (sweep 99-102 vs 93-95 us).
 int slabFor(int nx, int ny, int nz);
 Geo3 levelGeo(const MgLevel& L);                // the launch of a level's cell kernels (L.vec)
 // Launch shape of the fused residual + restriction from L to Lc: one thread per

@@ -44,13 +44,16 @@ Geo3 geo3n(int nx, int ny, int nz, int cells, int slab) {
   Geo3 g;
   g.block = dim3(bx, by, 1);
   const unsigned gx = (tx + bx - 1) / bx, gy = (ny + by - 1) / by;
-  g.grid = slab ? dim3(8 * gx, gy, (nz + 7) / 8) : dim3(gx, gy, nz);
+  g.grid = slab == 1 ? dim3(8 * gx, gy, (nz + 7) / 8) : slab == 2 ? dim3(8 * gx, (gy + 7) / 8, nz) : dim3(gx, gy, nz);
   g.blocks = int(g.grid.x * g.grid.y * g.grid.z);
   return g;
 }
 Geo3 geo3(int nx, int ny, int nz) { return geo3n(nx, ny, nz, 1); }
 Geo3 levelGeo(const MgLevel& L) { return geo3n(L.nx, L.ny, L.nz, L.vec, L.slab); }
-int slabFor(int nx, int ny, int nz) { return nz >= 8 && (long long)nx * ny * nz < (8LL << 20) ? 1 : 0; }
+int slabFor(int nx, int ny, int nz) {
+  if ((long long)nx * ny * nz < (8LL << 20)) return nz >= 8 ? 1 : 0;
+  return 2;
+}
 
 namespace {
 template <class T, int MODE, int NC>
@@ -1453,7 +1456,7 @@ Geo3 restrictGeo(const MgLevel& L, const MgLevel& Lc) {
   Geo3 g;
   g.block = dim3(bx, by, bz);
   const unsigned gx = (tx + bx - 1) / bx, gy = (L.ny + by - 1) / by, gz = (L.nz + bz - 1) / bz;
-  g.grid = L.slab ? dim3(8 * gx, gy, (gz + 7) / 8) : dim3(gx, gy, gz);
+  g.grid = L.slab == 1 ? dim3(8 * gx, gy, (gz + 7) / 8) : L.slab == 2 ? dim3(8 * gx, (gy + 7) / 8, gz) : dim3(gx, gy, gz);
   g.blocks = int(g.grid.x * g.grid.y * g.grid.z);
   return g;
 }

@@ -198,7 +198,7 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
     int autoVec = (m.nx % 2 == 0 && m.nx >= 32) ? 2 : 1;
     if (mixed_ && m.nx % 4 == 0 && m.nx >= 64 && m.n >= (1LL << 22)) autoVec = 4;
     m.vec = want == 0 ? autoVec : (m.nx % want == 0 ? want : 1);
-    m.slab = slabFor(m.nx, m.ny, m.nz);
+    m.slab = opts.launch_order == 0 ? slabFor(m.nx, m.ny, m.nz) : opts.launch_order - 1;
     dl_.push_back(m);
     blocks_.push_back(levelGeo(m).blocks);
     stride = std::max(stride, blocks_.back());
@@ -219,6 +219,7 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
   if (opts.cells_per_thread != 0 && opts.cells_per_thread != 1 && opts.cells_per_thread != 2 &&
       opts.cells_per_thread != 4)
     throw std::invalid_argument("cells_per_thread must be 0, 1, 2 or 4");
+  if (opts.launch_order < 0 || opts.launch_order > 3) throw std::invalid_argument("launch_order must be 0, 1, 2 or 3");
   const size_t n0 = hl_[0].g.cells();
   stride = std::max(stride, loopBlocks((long long)n0));
   if (loops_)

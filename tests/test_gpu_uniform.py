@@ -166,6 +166,53 @@ def test_cells_per_thread_do_not_change_results(ctx, name, prec):
     assert rel(x2, x1) < tol and rel(x4, x1) < tol
 
 
+def _order_grid(name):
+    if name == "odd3d":
+        return dict(GRIDS)["odd3d"]
+    if name == "cube192":
+        return cases.cube_channel()[0]
+    if name == "tgv64":
+        return cases.taylor_green(64)[0]
+    # 256 x 256 x 128 (8.4M cells, bands of rows by default) with an outlet:
+    # non-singular, so the preconditioner has no reductions
+    faces = [dict(type=nbl.BC_WALL)] * 5 + [dict(type=nbl.BC_OUTLET)]
+    return nbl.Grid(256, 256, 128, 1.0 / 256, faces=faces)
+
+
+@pytest.mark.parametrize("name", ["cube192", "odd3d", "tgv64", "big256x256x128"])
+def test_launch_orders_do_not_change_results(ctx, name):
+    """Plain order, slabs of planes and bands of rows per XCD run the same
+    per-cell arithmetic: the preconditioner of a non-singular system (no
+    reductions) is bit-identical across orders; solves have identical
+    iteration counts and agree to rounding (only the partial-sum tree of the
+    reductions changes)."""
+    g = _order_grid(name)
+    b = cases.poisson_rhs(g, 23)
+    zs, out = [], []
+    for order in (1, 2, 3):
+        ps = nbl.Poisson(ctx, g, kind=0, tol=1e-10, max_iters=100, opts=nbl.solver_opts(launch_order=order))
+        try:
+            if name in ("cube192", "big256x256x128"):
+                zs.append(ps.precondition(b))
+            out.append(ps.solve(b))
+        finally:
+            ps.close()
+    for z in zs[1:]:
+        assert np.array_equal(z, zs[0])
+    (x1, r1), (x2, r2), (x3, r3) = out
+    print(f"{name}: iterations {r1.iterations} {r2.iterations} {r3.iterations}", flush=True)
+    assert r1.converged and r2.converged and r3.converged
+    assert r1.iterations == r2.iterations == r3.iterations
+    assert rel(x2, x1) < 1e-12 and rel(x3, x1) < 1e-12
+
+
+def test_launch_order_validated(ctx):
+    g = cases.taylor_green(16)[0]
+    for bad in (-1, 4):
+        with pytest.raises(nbl.NblError):
+            nbl.Poisson(ctx, g, kind=0, opts=nbl.solver_opts(launch_order=bad))
+
+
 @pytest.mark.parametrize("name", ["cube192", "tgv64"])
 def test_direct_coarse_matches_cg_coarse(ctx, name):
     """The exact coarsest-level solve and the reference's CG(1e-6, 500) there
