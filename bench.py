@@ -10,10 +10,15 @@ path).  The same steps are also run with the FP32 V-cycle inside the FP64
 flexible CG ("mixed") and reported beside it.
 
 Prints ONE JSON line (rank 0).  `value` is whole-job Mcell-steps/s of the FP64
-path with the state resident in HBM.  `roofline` is the finest-level smoother
-sweep, timed live with HIP events on its own stream; `cpu_baseline` is the
-reference solver (oracle/_ref, else the oracle restatement) on one host core
-over a bounded sample of the same workload.
+path with the state resident in HBM.  `roofline` is the FP64 damped-Jacobi
+sweep (the workload's dominant kernel) on a 256^3 level of the same system
+class (walls and one outlet face: non-singular, as the channel's), whose
+436 MB per launch stream from HBM, timed with HIP events on the solver's
+stream over 100 back-to-back launches; the channel's own 192x64x64 level
+(20 MB per launch, resident in the 256 MB Infinity Cache) is reported beside
+it as `roofline_cube192_infinity_cache`.  `cpu_baseline` is the reference
+solver (oracle/_ref, else the oracle restatement) on one host core over a
+bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -71,23 +76,37 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def roofline(ctx, g, reps=200, precision=nbl.PREC_FP64, kernel=0):
-    """One solve kernel in a loop of back-to-back launches: 0 the finest-level
-    sweep (FP64, or FP32 as in the mixed V-cycle), 1 the FP64 operator
-    application with p.Ap (the outer CG's SpMV)."""
+def grid256(singular=False):
+    """256^3 unit box: all walls (singular) or walls with an outlet on the +z
+    face (non-singular, the channel's system class)."""
+    faces = [dict(type=nbl.BC_WALL)] * 6 if singular else \
+        [dict(type=nbl.BC_WALL)] * 5 + [dict(type=nbl.BC_OUTLET)]
+    return nbl.Grid(256, 256, 256, 1.0 / 256, faces=faces)
+
+
+def roofline(ctx, g, reps=100, precision=nbl.PREC_FP64, kernel=0, key=None):
+    """One solve kernel in a loop of back-to-back launches on the solver's
+    stream between two HIP events: 0 the finest-level sweep (FP64, or FP32 as
+    in the mixed V-cycle), 1 the FP64 operator application with p.Ap (the
+    outer CG's SpMV), 4 the finest-level down kernel of the
+    two-kernels-per-level cycle."""
     ps = nbl.Poisson(ctx, g, kind=0, tol=1e-8, max_iters=5000, opts=nbl.solver_opts(precision=precision))
+    ps.profile(kernel, 10)
     ms, by = ps.profile(kernel, reps)
     ps.close()
     ach = by / (ms * 1e-3) / 1e9
-    key = f"{g.nx}x{g.ny}x{g.nz}" if g.nx != 192 else "cube192x64x64"
     fp32 = precision != nbl.PREC_FP64
-    name = ("k_fcg_apply (A p and p.Ap, FP64; 18 B/cell)" if kernel == 1 else
-            f"k_sweep<{'float' if fp32 else 'double'}> (finest-level damped-Jacobi sweep; fluid-sum fold when "
-            f"singular; {14 if fp32 else 26} B/cell)")
+    t = "float" if fp32 else "double"
+    name = {1: "k_fcg_apply<2> (A p and p.Ap, FP64; 18 B/cell)",
+            4: f"k_down<{t}, DS_B> (both pre-sweeps, residual and restriction in LDS tiles; "
+               f"{2 * (4 if fp32 else 8) + 2} B/cell + the coarse residual)"}.get(
+        kernel, f"k_sweep<{t}, SW_PLAIN> (finest-level damped-Jacobi sweep; fluid-sum fold when singular; "
+                f"{14 if fp32 else 26} B/cell)")
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4),
-            "traffic": pmc_traffic(key + ("_spmv" if kernel == 1 else "_fp32" if fp32 else "")),
-            "kernel": name, "avg_launch_us": round(ms * 1e3, 3), "bytes_per_launch": by, "cells": g.cells}
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(key) if key else None,
+            "kernel": name, "avg_launch_us": round(ms * 1e3, 3), "bytes_per_launch": by, "cells": g.cells,
+            "grid": f"{g.nx}x{g.ny}x{g.nz}", "launches_timed": reps,
+            "measured": "HIP events on the solver's stream around back-to-back launches"}
 
 
 def pmc_traffic(key):
@@ -107,14 +126,17 @@ def roofline_insitu(sweep, g):
     """Finest-level plain sweep timed inside the timed steps: every block of a
     launch stores the device wall clock at its start and when all its waves
     are done; a launch lasts from its first block start to its last block end
-    (launches a converged solve skipped are left out)."""
+    (launches a converged solve skipped are left out).  This leaves out the
+    dispatch cost every launch pays; the 20 MB the sweep touches on this level
+    sit in the 256 MB Infinity Cache, so the figure is not an HBM one."""
     ms, n, by = sweep["avg_ms"], sweep["launches"], sweep["bytes"]
     ach = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic("cube192x64x64"),
+    return {"achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": pmc_traffic("cube192x64x64"),
             "kernel": "k_sweep<double, SW_PLAIN> on the 192x64x64 level (pre-smoothing sweep 2 of every V-cycle)",
             "avg_launch_us": round(ms * 1e3, 3), "launches_timed": n, "bytes_per_launch": by, "cells": g.cells,
-            "measured": "device wall-clock stamps per block (first block start to last block end), every launch inside the timed steps"}
+            "measured": "device wall-clock stamps per block (first block start to last block end), every launch "
+                        "inside the timed steps"}
 
 
 def coarse_solve(ctx, g, reps=50):
@@ -437,6 +459,10 @@ def tgv_c1(ctx, a, steps=100):
         else:
             res[name]["rel_l2_vs_fp64"] = {k: rel(x, y) for k, x, y in zip("uvwp", stt[:4], state[:4])}
         log(f"tgv64 {name}: {res[name]['value']} Mcell-steps/s")
+    res["roofline"] = roofline(ctx, g, reps=200, kernel=4, key="tgv64_down_fp64")
+    res["roofline"]["note"] = ("64^3 singular: the finest level runs the two-kernels-per-level cycle; its down "
+                               "kernel (6.3 MB per launch at FP64) is latency-bound and Infinity-Cache resident")
+    res["roofline_mixed"] = roofline(ctx, g, reps=200, kernel=4, precision=nbl.PREC_MIXED)
     if not a.no_cpu and a.cpu_steps > 0:
         res["cpu_baseline"] = cpu_baseline(g, cfg, st, 8)
     return res
@@ -517,13 +543,17 @@ def main():
     ap.add_argument("--no-octree", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-tgv", action="store_true")
-    ap.add_argument("--served-window", type=float, default=20.0,
+    ap.add_argument("--served-window", type=float, default=60.0,
                     help="seconds of Poisson arrivals in the served-stream measurement (0: skip)")
     ap.add_argument("--served-rate", type=float, default=200.0)
     ap.add_argument("--profile-only", action="store_true",
                     help="only the timed FP64 steps (for profiler runs)")
+    ap.add_argument("--workload", choices=("cube192", "tgv64"), default="cube192",
+                    help="with --profile-only: the cube channel or the 64^3 Taylor-Green vortex")
     ap.add_argument("--roofline-only", action="store_true",
-                    help="only the roofline kernel loops and a 256 MiB copy (for counter runs)")
+                    help="only the roofline kernel loops and a 256 MiB copy (for profiler and counter runs)")
+    ap.add_argument("--roofline-singular", action="store_true",
+                    help="only the FP64 sweep on the singular (all-wall) 256^3 level and a 256 MiB copy")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -533,11 +563,28 @@ def main():
     g, cfg = cases.cube_channel()
     st = cases.cube_channel_state(g)
     ctx = nbl.Ctx(0)
+    if a.profile_only and a.workload == "tgv64":
+        tg, tcfg = cases.taylor_green(64)
+        _r, _ms, _it, el, _s = run_steps(ctx, tg, tcfg, cases.taylor_green_state(tg), nbl.PREC_FP64, a.warmup, a.steps)
+        ctx.close()
+        print(json.dumps({"workload": "tgv64", "steps": a.steps, "ms_per_step": round(el / a.steps * 1e3, 4),
+                          "value": round(tg.cells * a.steps / el / 1e6, 3)}), flush=True)
+        return
+    if a.roofline_singular:
+        out = {"singular_fp64_sweep": roofline(ctx, grid256(singular=True), key="256x256x256"),
+               "copy_256MiB_ms": ctx.probe_copy(256 << 20, 20)}
+        ctx.close()
+        print(json.dumps(out), flush=True)
+        return
     if a.roofline_only:
-        big = nbl.Grid(256, 256, 256, 1.0 / 256)
-        out = {"roofline": roofline(ctx, g), "roofline_256cubed": roofline(ctx, big, 50),
-               "roofline_256cubed_fp32": roofline(ctx, big, 50, precision=nbl.PREC_MIXED),
-               "roofline_256cubed_spmv": roofline(ctx, big, 50, kernel=1),
+        # the kernels of the roofline figures, each its own template instance
+        # (so a profiler's per-kernel summary of this run is theirs alone)
+        big = grid256()
+        tg = cases.taylor_green(64)[0]
+        out = {"roofline": roofline(ctx, big, key="256x256x256_outlet"),
+               "fp32_sweep": roofline(ctx, big, precision=nbl.PREC_MIXED, key="256x256x256_outlet_fp32"),
+               "spmv_fp64": roofline(ctx, big, kernel=1, key="256x256x256_outlet_spmv"),
+               "tgv64_down_fp64": roofline(ctx, tg, kernel=4, key="tgv64_down_fp64"),
                "copy_256MiB_ms": ctx.probe_copy(256 << 20, 20)}
         ctx.close()
         print(json.dumps(out), flush=True)
@@ -561,7 +608,7 @@ def main():
         "served_p50_ms": None, "served_p99_ms": None,
     }
     if a.profile_only:
-        out["roofline"] = roofline_insitu(sweep, g)
+        out["roofline_cube192_infinity_cache"] = roofline_insitu(sweep, g)
         print(json.dumps(out), flush=True)
         return
     if not a.no_mixed:
@@ -600,15 +647,24 @@ def main():
     log(f"fp64 reference loops: {lval:.2f} Mcell-steps/s")
     if not a.no_tgv:
         out["tgv64_c1"] = tgv_c1(ctx, a)
-    out["roofline"] = roofline_insitu(sweep, g)
-    out["roofline_isolated"] = roofline(ctx, g)
+    big = grid256()
+    out["roofline"] = roofline(ctx, big, key="256x256x256_outlet")
+    out["roofline"]["workload_note"] = ("the channel's dominant kernel at HBM scale: 256^3 walls + outlet "
+                                        "(non-singular, as the channel's system), 436 MB per launch")
+    out["roofline_256cubed"] = {
+        "fp32_sweep": roofline(ctx, big, precision=nbl.PREC_MIXED, key="256x256x256_outlet_fp32"),
+        "spmv_fp64": roofline(ctx, big, kernel=1, key="256x256x256_outlet_spmv"),
+        "singular_fp64_sweep": roofline(ctx, grid256(singular=True), key="256x256x256"),
+    }
+    out["roofline_cube192_infinity_cache"] = {
+        "in_step_stamps": roofline_insitu(sweep, g),
+        "isolated_events": roofline(ctx, g, reps=200, key="cube192x64x64"),
+        "note": "192x64x64: the sweep's 20 MB and the solve's ~170 MB working set are resident in the 256 MB "
+                "Infinity Cache, so these are not HBM figures; the stamps leave out every launch's dispatch cost"}
     out["coarse_solve"] = coarse_solve(ctx, g)
-    big = nbl.Grid(256, 256, 256, 1.0 / 256)
-    out["roofline_256cubed"] = roofline(ctx, big, reps=50)
-    out["roofline_256cubed"]["fp32_sweep"] = roofline(ctx, big, reps=50, precision=nbl.PREC_MIXED)
-    out["roofline_256cubed"]["spmv_fp64"] = roofline(ctx, big, reps=50, kernel=1)
-    log(f"256^3: fp64 sweep {out['roofline_256cubed']['frac']}, fp32 sweep "
-        f"{out['roofline_256cubed']['fp32_sweep']['frac']}, spmv {out['roofline_256cubed']['spmv_fp64']['frac']}")
+    rb = out["roofline_256cubed"]
+    log(f"256^3: fp64 sweep {out['roofline']['frac']}, fp32 sweep {rb['fp32_sweep']['frac']}, "
+        f"spmv {rb['spmv_fp64']['frac']}, singular fp64 sweep {rb['singular_fp64_sweep']['frac']}")
     if not a.no_octree:
         out["octree_c3"] = octree_c3(ctx, with_cpu=not a.no_cpu)
     if not a.no_batch:

@@ -301,7 +301,12 @@ int nbl_poisson_last_timing(nbl_poisson* ps, double* ms, int* vcycles);
  *   kernel 2: finest-level residual + restriction (reads b, x, code; writes
  *             the coarse residual),
  *   kernel 3: one coarsest-level solve (direct: reads the n^2 inverse, b and
- *             code, writes x; CG: counted as b, x and code only). */
+ *             code, writes x; CG: counted as b, x and code only),
+ *   kernel 4: finest-level down kernel of the two-kernels-per-level cycle
+ *             from its right-hand side (both pre-sweeps, the residual and
+ *             the restriction: reads b and code; writes the pre-smoothed
+ *             iterate, BF16 in that form, and the coarse residual, reading
+ *             the coarse code). */
 int nbl_poisson_profile(nbl_poisson* ps, int kernel, int reps, double* ms, double* bytes);
 
 /* ---- octree construction (host; Octree::refineUniform / refineByMorton /

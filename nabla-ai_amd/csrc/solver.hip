@@ -958,10 +958,11 @@ nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t
 }
 
 void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double* bytes) {
-  if (kernel < 0 || kernel > 3) throw std::invalid_argument("profile: kernel must be 0, 1, 2 or 3");
+  if (kernel < 0 || kernel > 4) throw std::invalid_argument("profile: kernel must be 0, 1, 2, 3 or 4");
   if (reps < 1) throw std::invalid_argument("profile: reps must be >= 1");
   if (kind_ != 0) throw std::invalid_argument("profile: mg-pcg solver only");
   if ((kernel == 2 || kernel == 3) && nlevels() < 2) throw std::invalid_argument("profile: single-level grid");
+  if (kernel == 4 && !tiled_) throw std::invalid_argument("profile: kernel 4 needs the two-kernels-per-level cycle");
   const MgLevel& L = dl_[0];
   const double n = double(L.n);
   const double sz = mixed_ ? 4.0 : 8.0;
@@ -1011,6 +1012,29 @@ void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double
                                 (q & 1) ? lx0_[0].as<double>() : lx1_[0].as<double>(), project_, rb, ctl, s);
     } else if (kernel == 1) {
       launch_fcg_apply(L, p_.as<double>(), Ap_.as<double>(), ctl, rb, s);
+    } else if (kernel == 4) {
+      if (mixed_) {
+        DownArgs<float> d0;
+        d0.L = dl_[0];
+        d0.Lc = dl_[1];
+        d0.ctl = ctl;
+        d0.b = b0f_.as<float>();
+        d0.xa = lx0_[0].as<float>();
+        if (bf16_) d0.xaB = xaB_.as<__hip_bfloat16>();
+        d0.bc = lb_[1].as<float>();
+        d0.rowMean = project_ ? slotRow(slotB(1)) : nullptr;
+        launch_down<float>(DS_B, d0, s);
+      } else {
+        DownArgs<double> d0;
+        d0.L = dl_[0];
+        d0.Lc = dl_[1];
+        d0.ctl = ctl;
+        d0.b = r_.as<double>();
+        d0.xa = lx0_[0].as<double>();
+        d0.bc = lb_[1].as<double>();
+        d0.rowMean = project_ ? slotRow(slotB(1)) : nullptr;
+        launch_down<double>(DS_B, d0, s);
+      }
     } else if (kernel == 3) {
       const int lc = nlevels() - 1;
       if (mixed_) {
@@ -1044,6 +1068,7 @@ void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double
   if (kernel == 0) *bytes = n * (3.0 * sz + 2.0);
   else if (kernel == 1) *bytes = n * (8.0 + 8.0 + 2.0);
   else if (kernel == 3) *bytes = direct_ ? nc * nc * sz + nc * (2.0 * sz + 2.0) : nc * (2.0 * sz + 2.0);
+  else if (kernel == 4) *bytes = n * (sz + 2.0 + (bf16_ ? 2.0 : sz)) + double(dl_[1].n) * (sz + 2.0);
   else *bytes = n * (2.0 * sz + 2.0) + double(dl_[1].n) * (sz + 2.0);
 }
 
