@@ -147,6 +147,13 @@ typedef struct nbl_solver_opts {
                                8M cells, 1 otherwise.  Per-cell results are
                                bit-identical across orders; sums differ only
                                through their partial-sum tree              */
+  int iteration_graph;      /* flexible-CG iterations after the first: 2
+                               replayed in pairs from a HIP graph captured on
+                               the first solve (one launch per pair instead
+                               of one per kernel); 0 (default) or 1 launched
+                               kernel by kernel (measured 1-2% faster on the
+                               channel and Taylor-Green steps).  Bit-identical
+                               results either way                          */
 } nbl_solver_opts;
 
 /* StepReport (ns_solver.hpp:34-49). */
@@ -306,7 +313,10 @@ int nbl_poisson_last_timing(nbl_poisson* ps, double* ms, int* vcycles);
  *             from its right-hand side (both pre-sweeps, the residual and
  *             the restriction: reads b and code; writes the pre-smoothed
  *             iterate, BF16 in that form, and the coarse residual, reading
- *             the coarse code). */
+ *             the coarse code),
+ *   kernel 5: finest-level A p of flexible-CG iterations after the first,
+ *             with p = z + beta p_old formed at every stencil point and the
+ *             p.Ap fold (reads z, p_old, code; writes p, Ap). */
 int nbl_poisson_profile(nbl_poisson* ps, int kernel, int reps, double* ms, double* bytes);
 
 /* ---- octree construction (host; Octree::refineUniform / refineByMorton /

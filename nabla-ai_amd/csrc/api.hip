@@ -114,6 +114,7 @@ void nbl_solver_opts_default(nbl_solver_opts* o) {
   o->rhs_planes = 0;
   o->vcycle = 0;
   o->launch_order = 0;
+  o->iteration_graph = 0;
 }
 
 const char* nbl_last_error(void) { return g_err.c_str(); }

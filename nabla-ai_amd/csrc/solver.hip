@@ -220,6 +220,8 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
       opts.cells_per_thread != 4)
     throw std::invalid_argument("cells_per_thread must be 0, 1, 2 or 4");
   if (opts.launch_order < 0 || opts.launch_order > 3) throw std::invalid_argument("launch_order must be 0, 1, 2 or 3");
+  if (opts.iteration_graph < 0 || opts.iteration_graph > 2)
+    throw std::invalid_argument("iteration_graph must be 0, 1 or 2");
   const size_t n0 = hl_[0].g.cells();
   stride = std::max(stride, loopBlocks((long long)n0));
   if (loops_)
@@ -340,6 +342,7 @@ bool PoissonDev::buildDirectCoarse() {
 }
 
 PoissonDev::~PoissonDev() {
+  if (iterGraph_) (void)hipGraphExecDestroy(iterGraph_);
   if (hctl_) (void)hipHostFree(hctl_);
   if (e0_) (void)hipEventDestroy(e0_);
   if (e1_) (void)hipEventDestroy(e1_);
@@ -890,6 +893,39 @@ void PoissonDev::precondition(const double* d_r, double* d_z, hipStream_t s) {
   NBL_HIP(hipMemcpyAsync(d_z, z_.as<void>(), size_t(L.n) * 8, hipMemcpyDeviceToDevice, s));
 }
 
+bool PoissonDev::iterationPair(hipStream_t s) {
+  if (opts_.iteration_graph != 2 || loops_ || timeSweeps_ || graphFailed_) return false;
+  if (!iterGraph_) {
+    const int v0 = vcount_;
+    hipGraph_t g = nullptr;
+    NBL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueueIteration(s, 1);
+      enqueueIteration(s, 2);
+    } catch (...) {
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      vcount_ = v0;
+      graphFailed_ = true;
+      throw;
+    }
+    NBL_HIP(hipStreamEndCapture(s, &g));
+    graphV_ = vcount_ - v0;
+    vcount_ = v0;
+    const hipError_t e = hipGraphInstantiate(&iterGraph_, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      iterGraph_ = nullptr;
+      graphFailed_ = true;
+      return false;
+    }
+  }
+  NBL_HIP(hipGraphLaunch(iterGraph_, s));
+  vcount_ += graphV_;
+  return true;
+}
+
 void PoissonDev::readCtl(hipStream_t s) {
   NBL_HIP(hipMemcpyAsync(hctl_, ctl_.as<void>(), sizeof(SolveCtl), hipMemcpyDeviceToHost, s));
   NBL_HIP(hipStreamSynchronize(s));
@@ -935,7 +971,16 @@ nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t
   int batch = std::clamp(predicted_, 1, std::max(1, maxIt_));
   const int step = std::max(1, opts_.check_every);
   while (enq < maxIt_) {
-    for (int q = 0; q < batch && enq < maxIt_; ++q, ++enq) enqueueIteration(s, enq);
+    for (int q = 0; q < batch && enq < maxIt_;) {
+      if (enq % 2 == 1 && q + 2 <= batch && enq + 2 <= maxIt_ && iterationPair(s)) {
+        q += 2;
+        enq += 2;
+      } else {
+        enqueueIteration(s, enq);
+        ++q;
+        ++enq;
+      }
+    }
     readCtl(s);
     if (hctl_[0].done) break;
     batch = step;
@@ -958,7 +1003,7 @@ nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t
 }
 
 void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double* bytes) {
-  if (kernel < 0 || kernel > 4) throw std::invalid_argument("profile: kernel must be 0, 1, 2, 3 or 4");
+  if (kernel < 0 || kernel > 5) throw std::invalid_argument("profile: kernel must be 0 .. 5");
   if (reps < 1) throw std::invalid_argument("profile: reps must be >= 1");
   if (kind_ != 0) throw std::invalid_argument("profile: mg-pcg solver only");
   if ((kernel == 2 || kernel == 3) && nlevels() < 2) throw std::invalid_argument("profile: single-level grid");
@@ -976,6 +1021,10 @@ void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double
   NBL_HIP(hipMemsetAsync(lx1_[0].as<void>(), 0, L.n * size_t(sz), s));
   NBL_HIP(hipMemsetAsync(r_.as<void>(), 0, L.n * 8, s));
   NBL_HIP(hipMemsetAsync(p_.as<void>(), 0, L.n * 8, s));
+  if (kernel == 5) {
+    NBL_HIP(hipMemsetAsync(p2_.as<void>(), 0, L.n * 8, s));
+    NBL_HIP(hipMemsetAsync(z_.as<void>(), 0, L.n * 8, s));
+  }
   if (mixed_) NBL_HIP(hipMemsetAsync(b0f_.as<void>(), 0, L.n * 4, s));
   if (kernel == 3) {  // a non-trivial coarse right-hand side (mean-free for singular systems)
     const int lc = nlevels() - 1;
@@ -1012,6 +1061,9 @@ void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double
                                 (q & 1) ? lx0_[0].as<double>() : lx1_[0].as<double>(), project_, rb, ctl, s);
     } else if (kernel == 1) {
       launch_fcg_apply(L, p_.as<double>(), Ap_.as<double>(), ctl, rb, s);
+    } else if (kernel == 5) {  // beta from SolveCtl (0 after the reset), p_old and p_new alternate
+      launch_fcg_apply_pu(L, z_.as<double>(), (q & 1) ? p2_.as<double>() : p_.as<double>(),
+                          (q & 1) ? p_.as<double>() : p2_.as<double>(), Ap_.as<double>(), ctl, rb, s, InFold{});
     } else if (kernel == 4) {
       if (mixed_) {
         DownArgs<float> d0;
@@ -1067,6 +1119,7 @@ void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double
   const double nc = double(dl_.back().n);
   if (kernel == 0) *bytes = n * (3.0 * sz + 2.0);
   else if (kernel == 1) *bytes = n * (8.0 + 8.0 + 2.0);
+  else if (kernel == 5) *bytes = n * (4.0 * 8.0 + 2.0);
   else if (kernel == 3) *bytes = direct_ ? nc * nc * sz + nc * (2.0 * sz + 2.0) : nc * (2.0 * sz + 2.0);
   else if (kernel == 4) *bytes = n * (sz + 2.0 + (bf16_ ? 2.0 : sz)) + double(dl_[1].n) * (sz + 2.0);
   else *bytes = n * (2.0 * sz + 2.0) + double(dl_[1].n) * (sz + 2.0);

@@ -147,6 +147,15 @@ class PoissonDev {
   void minvLoops(const double* r, hipStream_t s);
   void enqueueIterationLoops(hipStream_t s);
   void readCtl(hipStream_t s);
+  // Flexible-CG iterations k, k + 1 (k odd) from the captured graph: their
+  // launches depend on k only through k & 1, so one capture of iterations 1
+  // and 2 replays every later pair.  Captured on first use; false: not in
+  // use (iteration_graph other than 2, the reference loop structure, in-situ
+  // sweep timing, or the capture failed).
+  bool iterationPair(hipStream_t s);
+  hipGraphExec_t iterGraph_ = nullptr;
+  bool graphFailed_ = false;
+  int graphV_ = 0;  // V-cycles one replay applies
   long long* timedSweep(int l, bool plain);
 
   nbl_ctx* ctx_;

@@ -256,8 +256,9 @@ double NSSolver::cellV(int i, int j, int k) const {
 double NSSolver::cellW(int i, int j, int k) const {
   if (grid_.nz == 1) return 0.0;
   fetch();
+  // w faces: nx * ny per plane, plane k + 1 is the face above cell (i, j, k)
   const std::size_t a = grid_.cidx(i, j, k);
-  return 0.5 * (w_[a] + w_[a + grid_.cellCount()]);
+  return 0.5 * (w_[a] + w_[a + std::size_t(grid_.nx) * grid_.ny]);
 }
 
 std::vector<double> NSSolver::cellVelocity(int component) const {

@@ -297,6 +297,20 @@ void gpuBackend() {
   for (int s = 0; s < 3; ++s) sb.step();
   check(sb.poissonIters() > 0 && std::isfinite(sb.cd()) && sb.t() > 0.0 && sb.uniformCells() == g.cellCount(),
         "GpuBackend: steps and reports through SolverBackend");
+  {
+    // the host accessors against the device's cell-centred velocity, every cell
+    const std::vector<double> cu = be.solver().cellVelocity(0), cv = be.solver().cellVelocity(1),
+                              cw = be.solver().cellVelocity(2);
+    bool same = true;
+    for (int k = 0; k < g.nz; ++k)
+      for (int j = 0; j < g.ny; ++j)
+        for (int i = 0; i < g.nx; ++i) {
+          const std::size_t c = g.cidx(i, j, k);
+          same = same && cu[c] == be.solver().cellU(i, j, k) && cv[c] == be.solver().cellV(i, j, k) &&
+                 cw[c] == be.solver().cellW(i, j, k);
+        }
+    check(same, "NSSolver::cellU/V/W: the device's cell-centred velocity at every cell");
+  }
   check(sb.sample(0, Vec3{0.6, 2.1, 3.3}) == be.solver().cellU(2, 8, 13) &&
             sb.sample(3, Vec3{100.0, -1.0, 0.1}) == be.solver().pressure()[g.cidx(47, 0, 0)],
         "GpuBackend::sample: the containing cell, indices clamped");

@@ -50,7 +50,8 @@ class FlowCfg(C.Structure):
 class SolverOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("pre_smooth", C.c_int), ("post_smooth", C.c_int),
                 ("check_every", C.c_int), ("coarse", C.c_int), ("cells_per_thread", C.c_int),
-                ("loops", C.c_int), ("rhs_planes", C.c_int), ("vcycle", C.c_int), ("launch_order", C.c_int)]
+                ("loops", C.c_int), ("rhs_planes", C.c_int), ("vcycle", C.c_int), ("launch_order", C.c_int),
+                ("iteration_graph", C.c_int)]
 
 
 class StepReport(C.Structure):
@@ -146,12 +147,14 @@ def flow_cfg(**kw) -> FlowCfg:
 
 
 def solver_opts(precision=PREC_FP64, pre=2, post=2, check_every=1, coarse=None, cells_per_thread=0,
-                loops=False, rhs_planes=0, vcycle=0, launch_order=0) -> SolverOpts:
+                loops=False, rhs_planes=0, vcycle=0, launch_order=0, iteration_graph=0) -> SolverOpts:
     """loops: the reference's loop structure, one kernel per loop (the unfused
     FP64 baseline).  rhs_planes: planes each WENO-5 right-hand-side brick
     marches through (0 automatic).  vcycle: 0 automatic, 1 one kernel per
     sweep, 2 two kernels per level (VCYCLE_*).  launch_order: 0 automatic, 1
-    plain, 2 slabs of planes per XCD, 3 bands of rows per XCD."""
+    plain, 2 slabs of planes per XCD, 3 bands of rows per XCD.
+    iteration_graph: 0 / 1 kernel by kernel (default), 2 pairs of iterations
+    replayed from a captured HIP graph."""
     o = SolverOpts()
     o.precision, o.pre_smooth, o.post_smooth, o.check_every = precision, pre, post, check_every
     o.coarse = COARSE_DIRECT if coarse is None else coarse
@@ -160,6 +163,7 @@ def solver_opts(precision=PREC_FP64, pre=2, post=2, check_every=1, coarse=None, 
     o.rhs_planes = rhs_planes
     o.vcycle = vcycle
     o.launch_order = launch_order
+    o.iteration_graph = iteration_graph
     return o
 
 
