@@ -283,7 +283,10 @@ def batched(ctx, with_cpu=True, seed=4242, nuni=48, noct=8, concurrency=4):
     for p, b in zip(solvers, rhs):  # warm-up (first touches, code objects)
         p.solve(b)
     t0 = time.perf_counter()
-    seq = [p.solve(b) for p, b in zip(solvers, rhs)]
+    seq, seq_dev_ms = [], 0.0
+    for p, b in zip(solvers, rhs):
+        seq.append(p.solve(b))
+        seq_dev_ms += p.last_timing()[0]
     seq_s = time.perf_counter() - t0
     t0 = time.perf_counter()
     xs, reps = nbl.solve_batch(solvers, rhs, concurrency=concurrency)
@@ -291,6 +294,44 @@ def batched(ctx, with_cpu=True, seed=4242, nuni=48, noct=8, concurrency=4):
     same = all(np.array_equal(x, y[0]) for x, y in zip(xs, seq))
     for p in solvers:
         p.close()
+    # one batch per size class: every level kernel of a class is one launch
+    # for all its problems (nbl_poisson_batch_*)
+    classes = {n: [q for q, s in enumerate(sizes) if s == n] for n in (32, 64, 96, 128)}
+    batches = {n: nbl.PoissonBatch(ctx, grids[idx[0]], len(idx), tol=1e-8, max_iters=200)
+               for n, idx in classes.items() if idx}
+    for n, pb in batches.items():  # warm-up
+        pb.solve([rhs[q] for q in classes[n]])
+    t0 = time.perf_counter()
+    cls_dev_ms, cls_dev, cls_iters, cls_x = 0.0, {}, {}, {}
+    for n, pb in batches.items():
+        xs_c, reps_c = pb.solve([rhs[q] for q in classes[n]])
+        ms, _v = pb.last_timing()
+        cls_dev_ms += ms
+        cls_dev[str(n)] = round(ms, 3)
+        cls_iters[str(n)] = sorted({r.iterations for r in reps_c})
+        for q, x in zip(classes[n], xs_c):
+            cls_x[q] = (x, reps_c[classes[n].index(q)])
+    cls_s = time.perf_counter() - t0
+    # the class batches side by side, one host thread (and stream) each
+    import threading
+    cc_out = {}
+
+    def run_class(n):
+        cc_out[n] = batches[n].solve([rhs[q] for q in classes[n]])
+
+    t0 = time.perf_counter()
+    ths = [threading.Thread(target=run_class, args=(n,)) for n in batches]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    cc_s = time.perf_counter() - t0
+    cc_same = all(np.array_equal(x, cls_x[q][0]) for n, (xs_c, _r) in cc_out.items()
+                  for q, x in zip(classes[n], xs_c))
+    for pb in batches.values():
+        pb.close()
+    cls_conv = all(r.converged for _, r in cls_x.values())
+    cls_rel = max(rel(cls_x[q][0], seq[q][0]) for q in cls_x)
     cells = int(sum(g.cells for g in grids))
     out = {"uniform": {"problems": nuni, "cells": cells, "sizes": {str(n): sizes.count(n) for n in (32, 64, 96, 128)},
                        "setup_s": round(setup_s, 3), "sequential_s": round(seq_s, 4), "batched_s": round(bat_s, 4),
@@ -299,7 +340,20 @@ def batched(ctx, with_cpu=True, seed=4242, nuni=48, noct=8, concurrency=4):
                        "max_residual": max(r.residual for r in reps),
                        "iterations_mean": round(float(np.mean([r.iterations for r in reps])), 2),
                        "batch_equals_sequential": bool(same),
-                       "mcells_solved_per_s": round(cells / bat_s / 1e6, 2)}}
+                       "mcells_solved_per_s": round(cells / bat_s / 1e6, 2),
+                       "sequential_device_ms": round(seq_dev_ms, 3),
+                       "size_class_batches": {
+                           "note": "one nbl_poisson_batch per size class (14/12/12/10 problems): one launch per "
+                                   "level kernel for all problems of a class, per-problem scalars and stopping",
+                           "host_arrays_s": round(cls_s, 4), "speedup_vs_sequential": round(seq_s / cls_s, 3),
+                           "speedup_vs_concurrent": round(bat_s / cls_s, 3),
+                           "device_ms": round(cls_dev_ms, 3), "device_ms_per_class": cls_dev,
+                           "device_speedup_vs_sequential": round(seq_dev_ms / max(cls_dev_ms, 1e-9), 3),
+                           "classes_concurrent_s": round(cc_s, 4),
+                           "classes_concurrent_speedup_vs_sequential": round(seq_s / cc_s, 3),
+                           "classes_concurrent_equal": bool(cc_same),
+                           "iterations_per_class": cls_iters, "all_converged": bool(cls_conv),
+                           "max_rel_l2_vs_sequential": cls_rel}}}
     trees = cases.batch_octrees(noct, seed + 99)
     octs, orhs = [], []
     for q, (k, lv) in enumerate(trees):

@@ -475,6 +475,33 @@ int nbl_server_footprint(nbl_server* srv, uint64_t* used, uint64_t* budget);
  * nbl_poisson_solve.  No counterpart in the reference. */
 int nbl_poisson_solve_batch(nbl_poisson* const* ps, const double* const* b, double* const* x, int n,
                             int concurrency, nbl_poisson_report* reps);
+/* Batched pressure solves (no counterpart in the reference, whose jobs each
+ * run MgPoisson::solve on their own, backend/app/jobs.py:50-140): `problems`
+ * independent systems on one grid (the same operator, different right-hand
+ * sides), solved together by MgPoisson::solve's flexible CG (kind 0,
+ * multigrid.cpp:267-355): every level kernel is one launch for all problems
+ * (their vectors one after another in HBM, one copy of the operator codes
+ * per problem, one coarsest-level inverse), while each problem keeps its own
+ * scalars, stopping test and report, so a problem's arithmetic is that of a
+ * batch of one.  The batch runs the one-kernel-per-sweep V-cycle with a
+ * direct coarsest solve, FP64 or FP32 levels (opts precision NBL_PREC_FP64
+ * or NBL_PREC_MIXED), levels in plain launch order or bands of rows; its
+ * reductions are finalized per problem (never folded inline).  Other
+ * options are refused with NBL_ERR_INVALID, as is a batch handle passed to
+ * a single-problem nbl_poisson_* entry point.  Destroy with
+ * nbl_poisson_destroy. */
+int nbl_poisson_batch_create(nbl_ctx* ctx, const nbl_grid_desc* grid, int problems, double tol,
+                             int max_iters, const nbl_solver_opts* opts, nbl_poisson** out);
+/* Problems of a batch (0: a single-problem solver). */
+int nbl_poisson_batch_size(nbl_poisson* ps, int* problems);
+/* Host arrays: b[q], x[q] of the grid's cell count for each problem q;
+ * reps[q] (may be NULL) its report. */
+int nbl_poisson_batch_solve(nbl_poisson* ps, const double* const* b, double* const* x,
+                            nbl_poisson_report* reps);
+/* Device arrays holding the problems one after another (problems x cells,
+ * 16-byte aligned). */
+int nbl_poisson_batch_solve_device(nbl_poisson* ps, const double* d_b, double* d_x,
+                                   nbl_poisson_report* reps);
 /* The same for octree Jacobi-PCG solves (as nbl_octree_poisson_solve each). */
 int nbl_octree_solve_batch(nbl_octree* const* os, const double* const* b, double* const* x, int n,
                            int concurrency, double tol, int max_iters, nbl_poisson_report* reps);

@@ -1,6 +1,7 @@
 // api.hip — the extern "C" boundary (include/nbl_api.h).  Every entry point
 // catches C++ exceptions and turns them into a status code plus a
 // thread-local message; nothing throws across the ABI.
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -309,15 +310,75 @@ int nbl_poisson_destroy(nbl_poisson* ps) {
 }
 
 namespace {
+// The single-problem entry points refuse a batch (its vectors hold every problem).
+void needSingle(const nbl_poisson* ps) {
+  need(!ps->impl->batched(), "this entry point takes a single-problem solver; use nbl_poisson_batch_solve");
+}
+}  // namespace
+
+int nbl_poisson_batch_create(nbl_ctx* ctx, const nbl_grid_desc* grid, int problems, double tol, int max_iters,
+                             const nbl_solver_opts* opts, nbl_poisson** out) {
+  return guarded([&] {
+    need(ctx && grid && out, "NULL argument");
+    need(problems >= 1, "a batch holds at least one problem");
+    useDevice(ctx);
+    auto ps = std::make_unique<nbl_poisson>();
+    ps->ctx = ctx;
+    ps->grid = nbl::HostGrid::from(grid);
+    ps->impl = std::make_unique<nbl::PoissonDev>(ctx, ps->grid, 0, tol, max_iters, optsOr(opts), problems);
+    NBL_HIP(hipStreamCreate(&ps->stream));
+    *out = ps.release();
+  });
+}
+
+int nbl_poisson_batch_size(nbl_poisson* ps, int* problems) {
+  return guarded([&] {
+    need(ps && problems, "NULL argument");
+    *problems = ps->impl->batched() ? ps->impl->problems() : 0;
+  });
+}
+
+int nbl_poisson_batch_solve_device(nbl_poisson* ps, const double* d_b, double* d_x, nbl_poisson_report* reps) {
+  return guarded([&] {
+    need(ps && d_b && d_x, "NULL argument");
+    need(ps->impl->batched(), "not a batch: use nbl_poisson_solve_device");
+    useDevice(ps->ctx);
+    ps->impl->solve(d_b, d_x, ps->stream);
+    NBL_HIP(hipStreamSynchronize(ps->stream));
+    if (reps) std::copy(ps->impl->reports().begin(), ps->impl->reports().end(), reps);
+  });
+}
+
+namespace {
 struct Staging {  // device copies of host vectors for the host-buffer entry points
   nbl::DevBuf a, b;
   Staging(nbl_ctx* ctx, size_t n) : a(ctx, n * 8), b(ctx, n * 8) {}
 };
 }  // namespace
 
+int nbl_poisson_batch_solve(nbl_poisson* ps, const double* const* b, double* const* x, nbl_poisson_report* reps) {
+  return guarded([&] {
+    need(ps && b && x, "NULL argument");
+    need(ps->impl->batched(), "not a batch: use nbl_poisson_solve");
+    const int nb = ps->impl->problems();
+    for (int q = 0; q < nb; ++q) need(b[q] && x[q], "NULL problem vector");
+    useDevice(ps->ctx);
+    const size_t n = ps->grid.cells();
+    Staging st(ps->ctx, n * size_t(nb));
+    for (int q = 0; q < nb; ++q)
+      NBL_HIP(hipMemcpyAsync(st.a.as<double>() + size_t(q) * n, b[q], n * 8, hipMemcpyHostToDevice, ps->stream));
+    ps->impl->solve(st.a.as<double>(), st.b.as<double>(), ps->stream);
+    for (int q = 0; q < nb; ++q)
+      NBL_HIP(hipMemcpyAsync(x[q], st.b.as<double>() + size_t(q) * n, n * 8, hipMemcpyDeviceToHost, ps->stream));
+    NBL_HIP(hipStreamSynchronize(ps->stream));
+    if (reps) std::copy(ps->impl->reports().begin(), ps->impl->reports().end(), reps);
+  });
+}
+
 int nbl_poisson_solve(nbl_poisson* ps, const double* b, double* x, nbl_poisson_report* rep) {
   return guarded([&] {
     need(ps && b && x, "NULL argument");
+    needSingle(ps);
     useDevice(ps->ctx);
     const size_t n = ps->grid.cells();
     Staging st(ps->ctx, n);
@@ -340,6 +401,7 @@ int nbl_poisson_solve(nbl_poisson* ps, const double* b, double* x, nbl_poisson_r
 int nbl_poisson_solve_noalloc(nbl_poisson* ps, const double* b, double* x, nbl_poisson_report* rep) {
   return guarded([&] {
     need(ps && b && x, "NULL argument");
+    needSingle(ps);
     useDevice(ps->ctx);
     const size_t n = ps->grid.cells();
     if (n > size_t(kNoAllocCells))
@@ -359,6 +421,7 @@ int nbl_poisson_solve_noalloc(nbl_poisson* ps, const double* b, double* x, nbl_p
 int nbl_poisson_solve_device(nbl_poisson* ps, const double* d_b, double* d_x, nbl_poisson_report* rep) {
   return guarded([&] {
     need(ps && d_b && d_x, "NULL argument");
+    needSingle(ps);
     useDevice(ps->ctx);
     const nbl_poisson_report r = ps->impl->solve(d_b, d_x, ps->stream);
     NBL_HIP(hipStreamSynchronize(ps->stream));
@@ -369,6 +432,7 @@ int nbl_poisson_solve_device(nbl_poisson* ps, const double* d_b, double* d_x, nb
 int nbl_poisson_precondition(nbl_poisson* ps, const double* r, double* z) {
   return guarded([&] {
     need(ps && r && z, "NULL argument");
+    needSingle(ps);
     useDevice(ps->ctx);
     const size_t n = ps->grid.cells();
     Staging st(ps->ctx, n);
@@ -382,6 +446,7 @@ int nbl_poisson_precondition(nbl_poisson* ps, const double* r, double* z) {
 int nbl_poisson_apply(nbl_poisson* ps, const double* x, double* out) {
   return guarded([&] {
     need(ps && x && out, "NULL argument");
+    needSingle(ps);
     useDevice(ps->ctx);
     const size_t n = ps->grid.cells();
     Staging st(ps->ctx, n);
@@ -395,6 +460,7 @@ int nbl_poisson_apply(nbl_poisson* ps, const double* x, double* out) {
 int nbl_poisson_diagonal(nbl_poisson* ps, double* diag) {
   return guarded([&] {
     need(ps && diag, "NULL argument");
+    needSingle(ps);
     useDevice(ps->ctx);
     const size_t n = ps->grid.cells();
     nbl::DevBuf d(ps->ctx, n * 8);
@@ -421,6 +487,7 @@ int nbl_poisson_levels(nbl_poisson* ps, int* nlevels, int* dims) {
 int nbl_poisson_smooth_once(nbl_poisson* ps, const double* b, double* x) {
   return guarded([&] {
     need(ps && b && x, "NULL argument");
+    needSingle(ps);
     need(ps->impl->kind() == 0, "smoother exists for the mg-pcg solver only");
     useDevice(ps->ctx);
     const size_t n = ps->grid.cells();
@@ -439,6 +506,7 @@ int nbl_poisson_smooth_once(nbl_poisson* ps, const double* b, double* x) {
 int nbl_poisson_restrict(nbl_poisson* ps, const double* rf, double* rc) {
   return guarded([&] {
     need(ps && rf && rc, "NULL argument");
+    needSingle(ps);
     need(ps->impl->nlevels() >= 2, "grid has a single multigrid level");
     useDevice(ps->ctx);
     const nbl::MgLevel& L = ps->impl->level(0);
@@ -454,6 +522,7 @@ int nbl_poisson_restrict(nbl_poisson* ps, const double* rf, double* rc) {
 int nbl_poisson_prolong(nbl_poisson* ps, const double* ec, double* xf) {
   return guarded([&] {
     need(ps && ec && xf, "NULL argument");
+    needSingle(ps);
     need(ps->impl->nlevels() >= 2, "grid has a single multigrid level");
     useDevice(ps->ctx);
     const nbl::MgLevel& L = ps->impl->level(0);
@@ -478,6 +547,7 @@ int nbl_poisson_last_timing(nbl_poisson* ps, double* ms, int* vcycles) {
 int nbl_poisson_profile(nbl_poisson* ps, int kernel, int reps, double* ms, double* bytes) {
   return guarded([&] {
     need(ps && ms && bytes, "NULL argument");
+    needSingle(ps);
     useDevice(ps->ctx);
     ps->impl->profile(kernel, reps, ps->stream, ms, bytes);
   });
@@ -733,6 +803,10 @@ int nbl_poisson_solve_batch(nbl_poisson* const* ps, const double* const* b, doub
                             int concurrency, nbl_poisson_report* reps) {
   return guarded([&] {
     need(ps && b && x && reps && n >= 0, "NULL argument");
+    for (int q = 0; q < n; ++q) {
+      need(ps[q] != nullptr, "NULL solver");
+      needSingle(ps[q]);
+    }
     if (n > 0) useDevice(ps[0]->ctx);
     nbl::solveBatch(ps, b, x, n, concurrency, reps);
   });

@@ -38,6 +38,8 @@ struct MgLevel {
   long long nf;           // fluid cells
   int vec;                // cells per thread along i in the cell kernels (1, 2 or 4; divides nx)
   int slab;               // launch order of the cell kernels (blk3): 0 plain, 1 XCD slabs of planes, 2 XCD bands of rows
+  int nb;                 // problems of a batch: level vectors and codes hold nb blocks of n cells, a
+                          // launch's grid z runs over nb * nz planes (0 or 1: one problem)
 };
 
 constexpr uint16_t kFluidBit = 1u << 12;
@@ -65,7 +67,19 @@ struct MeanRef {
   int n = 0;
   double nf = 1.0;
   __host__ __device__ bool on() const { return slot != nullptr || parts != nullptr; }
+  // The same slot in problem p's SolveCtl (batches: one SolveCtl per problem,
+  // contiguous; batches use slots only).
+  __host__ __device__ MeanRef at(int p) const;
 };
+constexpr int kCtlDoubles = int(sizeof(SolveCtl) / sizeof(double));
+static_assert(sizeof(SolveCtl) % sizeof(double) == 0, "SolveCtl must be a whole number of doubles");
+__host__ __device__ inline MeanRef MeanRef::at(int p) const {
+  MeanRef r = *this;
+  if (r.slot) r.slot += size_t(p) * size_t(kCtlDoubles);
+  return r;
+}
+__host__ __device__ inline SolveCtl* ctlAt(SolveCtl* c, int p) { return c ? c + p : c; }
+__host__ __device__ inline const SolveCtl* ctlAt(const SolveCtl* c, int p) { return c ? c + p : c; }
 inline MeanRef slotMean(const double* slot) {
   MeanRef m;
   m.slot = slot;
@@ -92,7 +106,9 @@ struct Geo3 {
   int blocks;
 };
 Geo3 geo3(int nx, int ny, int nz);
-Geo3 geo3n(int nx, int ny, int nz, int cells, int slab = 0);  // `cells` consecutive cells along i per thread
+// `cells` consecutive cells along i per thread; nb problems of a batch stacked
+// along the grid's z (plain order or bands of rows; blocks counts them all)
+Geo3 geo3n(int nx, int ny, int nz, int cells, int slab = 0, int nb = 1);
 // Automatic launch order of a level's cell kernels (blk3's slab argument): XCD
 // slabs of planes (1) on levels below 8M cells with at least 8 planes, bands
 // of rows (2) from 8M cells, plain order (0) otherwise.  On the 192 x 64 x 64
@@ -212,9 +228,11 @@ void launch_fill(double* a, long long n, double v, hipStream_t s);
 // ---- launchers (mg.hip) ----------------------------------------------------
 // Multigrid pieces are templated on the storage/arithmetic type T (double for
 // the plain path, float for the mixed V-cycle).
-void launch_fin(RedBuf rb, const FinArgs& a, SolveCtl* ctl, hipStream_t s);
+// nb > 1: one block per problem p of a batch, folding the producer's
+// partials p * a.nparts .. (p + 1) * a.nparts - 1 into problem p's SolveCtl
+void launch_fin(RedBuf rb, const FinArgs& a, SolveCtl* ctl, hipStream_t s, int nb = 1);
 // All solve scalars to zero, tolerance set (a kernel, ordered on `s`).
-void launch_ctl_init(SolveCtl* ctl, double tol, hipStream_t s);
+void launch_ctl_init(SolveCtl* ctl, double tol, hipStream_t s, int nb = 1);
 // Step scalars (step index, time, last dt) set by a kernel on `s`.
 void launch_set_step(StepCtl* sc, int step, double t, double dt, hipStream_t s);
 // stamps: optional per-block device wall-clock stamps (start, and end once

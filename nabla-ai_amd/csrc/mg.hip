@@ -17,6 +17,7 @@
 //
 // Every kernel of a flexible-CG iteration first checks ctl->done, so the host
 // can enqueue iterations before it knows whether the solve has converged.
+#include <algorithm>
 #include <stdexcept>
 #include <type_traits>
 
@@ -24,7 +25,7 @@
 
 namespace nbl {
 
-Geo3 geo3n(int nx, int ny, int nz, int cells, int slab) {
+Geo3 geo3n(int nx, int ny, int nz, int cells, int slab, int nb) {
   const int tx = (nx + cells - 1) / cells;
   // the row width (64, 32, 16 or 8 threads; 256/width rows of j) that leaves
   // the fewest idle threads in the padded tx x ny rectangle, the widest of
@@ -44,12 +45,14 @@ Geo3 geo3n(int nx, int ny, int nz, int cells, int slab) {
   Geo3 g;
   g.block = dim3(bx, by, 1);
   const unsigned gx = (tx + bx - 1) / bx, gy = (ny + by - 1) / by;
-  g.grid = slab == 1 ? dim3(8 * gx, gy, (nz + 7) / 8) : slab == 2 ? dim3(8 * gx, (gy + 7) / 8, nz) : dim3(gx, gy, nz);
+  if (nb > 1 && slab == 1) throw std::logic_error("a batch runs its levels in plain order or bands of rows");
+  const unsigned gz = unsigned(nz) * unsigned(std::max(nb, 1));
+  g.grid = slab == 1 ? dim3(8 * gx, gy, (nz + 7) / 8) : slab == 2 ? dim3(8 * gx, (gy + 7) / 8, gz) : dim3(gx, gy, gz);
   g.blocks = int(g.grid.x * g.grid.y * g.grid.z);
   return g;
 }
 Geo3 geo3(int nx, int ny, int nz) { return geo3n(nx, ny, nz, 1); }
-Geo3 levelGeo(const MgLevel& L) { return geo3n(L.nx, L.ny, L.nz, L.vec, L.slab); }
+Geo3 levelGeo(const MgLevel& L) { return geo3n(L.nx, L.ny, L.nz, L.vec, L.slab, L.nb); }
 int slabFor(int nx, int ny, int nz) {
   if ((long long)nx * ny * nz < (8LL << 20)) return nz >= 8 ? 1 : 0;
   return 2;
@@ -59,10 +62,17 @@ namespace {
 template <class T, int MODE, int NC>
 __global__ void __launch_bounds__(kBlock) k_sweep(MgLevel L, const T* __restrict__ b, MeanRef mb,
                                                   const T* __restrict__ x, MeanRef mx, const T* ec,
-                                                  MeanRef mec, int ncx, int ncy, int sx, int sy, int sz,
+                                                  MeanRef mec, int ncx, int ncy, int sx, int sy, int sz, int ecN,
                                                   T* __restrict__ xo, int fold, RedBuf rb, SolveCtl* ctl,
                                                   long long* stamps, InFold conv) {
-  const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz, L.slab);
+  const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz, L.slab, L.nb);
+  if (L.nb > 1) {  // this block's problem: its SolveCtl, mean slots and coarse correction
+    ctl = ctlAt(ctl, q.p);
+    mb = mb.at(q.p);
+    mx = mx.at(q.p);
+    mec = mec.at(q.p);
+    if (ec) ec += size_t(q.p) * size_t(ecN);
+  }
   const bool t0 = threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0;
   if (stamps && t0) stamps[2 * q.blk] = wall_clock64();
   // Only a sweep from zero carries the convergence test (the host passes it
@@ -145,10 +155,20 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc
   const Blk3 B = blk3(L.slab);
   const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
   const int bx = blockDim.x, by = blockDim.y;
-  const int i = NC * (B.x * bx + tx), j = B.y * by + ty, k = B.z * int(blockDim.z) + tz;
+  // a batch's problems follow each other along z, each in its own blocks
+  int bzl = B.z, p = 0;
+  if (L.nb > 1) {
+    const int gzp = (L.nz + int(blockDim.z) - 1) / int(blockDim.z);
+    p = bzl / gzp;
+    bzl -= p * gzp;
+    ctl = ctlAt(ctl, p);
+    mb = mb.at(p);
+    mx = mx.at(p);
+  }
+  const int i = NC * (B.x * bx + tx), j = B.y * by + ty, k = bzl * int(blockDim.z) + tz;
   const int tid = tx + bx * (ty + by * tz);
   const bool in = i < L.nx && j < L.ny && k < L.nz;
-  const int c = i + L.nx * (j + L.ny * k);
+  const int c = i + L.nx * (j + L.ny * (k + L.nz * p));
   uint16_t code[NC];
   T xc[NC], xn[NC][6], bv[NC];
   // a restriction that carries the convergence test: as k_sweep
@@ -198,7 +218,7 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc
   if (tz == 0 && tx % fxT == 0 && ty % Lc.fy == 0) {
     const int I = i / Lc.fx, J = j / Lc.fy, K = k / Lc.fz;
     if (I < Lc.nx && J < Lc.ny && K < Lc.nz) {
-      const int C = I + Lc.nx * (J + Lc.ny * K);
+      const int C = I + Lc.nx * (J + Lc.ny * (K + Lc.nz * p));
       Cst = C;
       const uint16_t cc = Lc.code[C];
       if (cc & kFluidBit) {
@@ -470,10 +490,20 @@ template <class T, class MT, int ROWS>
 __global__ void __launch_bounds__(kBlock) k_coarse_direct(MgLevel L, const MT* __restrict__ M,
                                                           const T* __restrict__ b, MeanRef mb,
                                                           T* __restrict__ x, const SolveCtl* ctl) {
+  const int n = int(L.n);
+  int blk = blockIdx.x;
+  if (L.nb > 1) {  // ceil(n / ROWS) blocks per problem of a batch; one inverse for all
+    const int bpp = (n + ROWS - 1) / ROWS;
+    const int p = blk / bpp;
+    blk -= p * bpp;
+    ctl = ctlAt(ctl, p);
+    mb = mb.at(p);
+    b += size_t(p) * size_t(n);
+    x += size_t(p) * size_t(n);
+  }
   double mv[1];
   if (stopOrMeans<1>(ctl, {mb}, mv)) return;
-  const int n = int(L.n);
-  const int r0 = blockIdx.x * ROWS;
+  const int r0 = blk * ROWS;
   const double m = mv[0];
   double acc[ROWS];
 #pragma unroll
@@ -615,9 +645,13 @@ template <class T, int NC>
 __global__ void __launch_bounds__(kBlock) k_finish(MgLevel L, const T* __restrict__ x, MeanRef mx,
                                                    double* __restrict__ z, int fold, RedBuf rb,
                                                    const SolveCtl* ctl) {
+  const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz, L.slab, L.nb);
+  if (L.nb > 1) {
+    ctl = ctlAt(ctl, q.p);
+    mx = mx.at(q.p);
+  }
   double mv[1];
   if (stopOrMeans<1>(ctl, {mx}, mv)) return;
-  const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz, L.slab);
   double acc[1] = {0.0};
   double zv[NC];
   if (q.valid) {
@@ -638,11 +672,13 @@ __global__ void __launch_bounds__(kBlock) k_finish(MgLevel L, const T* __restric
   if (q.valid) stN<NC>(z, q.c, zv);  // after the fold's barriers
 }
 
+// n cells per problem; a batch's nb problems one after another (problem c / n).
 template <class T>
-__global__ void k_to_level0(int n, const double* __restrict__ r, T* __restrict__ b, const SolveCtl* ctl) {
-  if (ctl && ldA(&ctl->done)) return;
+__global__ void k_to_level0(int n, int nb, const double* __restrict__ r, T* __restrict__ b, const SolveCtl* ctl) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < n) b[c] = T(r[c]);
+  if (c >= n * nb) return;
+  if (ctl && ldA(&ctlAt(ctl, nb > 1 ? c / n : 0)->done)) return;
+  b[c] = T(r[c]);
 }
 
 // ---------------------------------------------------------------------------
@@ -809,9 +845,10 @@ __global__ void k_prolong(MgLevel L, MgLevel Lc, const double* __restrict__ ec, 
 // ---------------------------------------------------------------------------
 enum : int { S_MEAN_RHS = 0, S_MEAN_RN = 1, S_MEAN_X = 2 };
 
-#define NBL_CELLS(NC)                                   \
-  const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz, L.slab);   \
-  uint16_t code[NC];                                    \
+// The thread's cells and their codes; in a batch the block's problem (q.p).
+#define NBL_CELLS(NC)                                               \
+  const CellN<NC> q = cellOfN<NC>(L.nx, L.ny, L.nz, L.slab, L.nb);  \
+  uint16_t code[NC];                                                \
   if (q.valid) ldCodes<NC>(L.code, q.c, code)
 
 template <int NC>
@@ -840,6 +877,7 @@ template <int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_init2(MgLevel L, int singular, double* __restrict__ rhs,
                                                       double* __restrict__ r, const SolveCtl* ctl, RedBuf rb) {
   NBL_CELLS(NC);
+  ctl = ctlAt(ctl, q.p);
   double acc[1] = {0.0};
   if (q.valid) {
     double v[NC];
@@ -862,9 +900,11 @@ template <int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_start(MgLevel L, double* __restrict__ z, MeanRef mz,
                                                       const double* __restrict__ r, double* __restrict__ p,
                                                       const SolveCtl* ctl, RedBuf rb) {
+  NBL_CELLS(NC);
+  ctl = ctlAt(ctl, q.p);
+  mz = mz.at(q.p);
   double mv[1];
   if (stopOrMeans<1>(ctl, {mz}, mv)) return;
-  NBL_CELLS(NC);
   double acc[1] = {0.0};
   if (q.valid) {
     double v[NC], rv[NC];
@@ -901,8 +941,9 @@ __device__ __forceinline__ void applyN(const MgLevel& L, const CellN<NC>& q, con
 template <int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_apply(MgLevel L, const double* __restrict__ p,
                                                       double* __restrict__ Ap, const SolveCtl* ctl, RedBuf rb) {
-  if (ldA(&ctl->done)) return;
   NBL_CELLS(NC);
+  ctl = ctlAt(ctl, q.p);
+  if (ldA(&ctl->done)) return;
   double acc[1] = {0.0};
   double pv[NC], av[NC];
   if (q.valid) {
@@ -938,6 +979,8 @@ __global__ void __launch_bounds__(kBlock) k_fcg_apply_pu(MgLevel L, const double
                                                          const double* __restrict__ pOld, double* __restrict__ pNew,
                                                          double* __restrict__ Ap, SolveCtl* ctl, RedBuf rb,
                                                          InFold bf) {
+  NBL_CELLS(NC);
+  ctl = ctlAt(ctl, q.p);  // (batches fold no partials inline: bf.n == 0)
   double be;
   if (bf.n) {
     if (!inlineBeta(bf, ctl, be)) return;
@@ -946,7 +989,6 @@ __global__ void __launch_bounds__(kBlock) k_fcg_apply_pu(MgLevel L, const double
     if (ldA(&ctl->done)) return;
     be = ldA(&ctl->beta);
   }
-  NBL_CELLS(NC);
   double acc[1] = {0.0};
   double pv[NC], av[NC];
   if (q.valid) {
@@ -968,6 +1010,7 @@ __global__ void __launch_bounds__(kBlock) k_fcg_update(MgLevel L, int singular, 
                                                        const double* __restrict__ Ap, double* __restrict__ rn,
                                                        SolveCtl* ctl, RedBuf rb, InFold af) {
   NBL_CELLS(NC);
+  ctl = ctlAt(ctl, q.p);
   double xv[NC], pv[NC], rv[NC], av[NC];
   auto load = [&] {
     if (q.valid) {
@@ -1015,6 +1058,7 @@ __global__ void __launch_bounds__(kBlock) k_fcg_update_first(MgLevel L, double* 
                                                              T* __restrict__ x1, T* __restrict__ b0,
                                                              SolveCtl* ctl, RedBuf rb, InFold af) {
   NBL_CELLS(NC);
+  ctl = ctlAt(ctl, q.p);
   double xv[NC], pv[NC], rv[NC], av[NC];
   auto load = [&] {
     if (q.valid) {
@@ -1057,9 +1101,11 @@ __global__ void __launch_bounds__(kBlock) k_fcg_update_first(MgLevel L, double* 
 template <int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_norm(MgLevel L, double* __restrict__ rn, MeanRef mrn,
                                                      const SolveCtl* ctl, RedBuf rb, float* __restrict__ b0) {
+  NBL_CELLS(NC);
+  ctl = ctlAt(ctl, q.p);
+  mrn = mrn.at(q.p);
   double mv[1];
   if (stopOrMeans<1>(ctl, {mrn}, mv)) return;
-  NBL_CELLS(NC);
   double acc[1] = {0.0};
   double v[NC];
   if (q.valid) {
@@ -1089,10 +1135,12 @@ template <int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_dots(MgLevel L, double* __restrict__ z, MeanRef mz,
                                                      const double* __restrict__ rn, const double* __restrict__ r,
                                                      SolveCtl* ctl, RedBuf rb, int countIt) {
+  NBL_CELLS(NC);
+  ctl = ctlAt(ctl, q.p);
+  mz = mz.at(q.p);
   double mv[1];
   if (stopOrMeans<1>(ctl, {mz}, mv)) return;
-  if (countIt && firstBlock() && linearTid() == 0) stA(&ctl->it, ldA(&ctl->it) + 1);
-  NBL_CELLS(NC);
+  if (countIt && problemLead(q, L.nz, L.nb) && linearTid() == 0) stA(&ctl->it, ldA(&ctl->it) + 1);
   double acc[2] = {0.0, 0.0};
   double v[NC];
   if (q.valid) {
@@ -1117,8 +1165,9 @@ template <class T, int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_dots_x(MgLevel L, const T* __restrict__ x, double* __restrict__ z,
                                                        const double* __restrict__ rn, const double* __restrict__ r,
                                                        const SolveCtl* ctl, RedBuf rb) {
-  if (ldA(&ctl->done)) return;
   NBL_CELLS(NC);
+  ctl = ctlAt(ctl, q.p);
+  if (ldA(&ctl->done)) return;
   double acc[2] = {0.0, 0.0};
   double v[NC];
   if (q.valid) {
@@ -1147,9 +1196,10 @@ __global__ void __launch_bounds__(kBlock) k_sweep_dots(MgLevel L, const T* __res
                                                        double* __restrict__ z, const double* __restrict__ rn,
                                                        const double* __restrict__ r, RedBuf rb, SolveCtl* ctl,
                                                        int countIt) {
-  if (ldA(&ctl->done)) return;
-  if (countIt && firstBlock() && linearTid() == 0) stA(&ctl->it, ldA(&ctl->it) + 1);
   NBL_CELLS(NC);
+  ctl = ctlAt(ctl, q.p);
+  if (ldA(&ctl->done)) return;
+  if (countIt && problemLead(q, L.nz, L.nb) && linearTid() == 0) stA(&ctl->it, ldA(&ctl->it) + 1);
   double acc[2] = {0.0, 0.0};
   double v[NC];
   if (q.valid) {
@@ -1195,9 +1245,10 @@ template <int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_final(MgLevel L, int singular, const double* __restrict__ x,
                                                       const double* __restrict__ rhs, double* __restrict__ xo,
                                                       const SolveCtl* ctl, RedBuf rb) {
+  NBL_CELLS(NC);
+  ctl = ctlAt(ctl, q.p);
   const double m = singular ? ldA(&ctl->slot[S_MEAN_X]) : 0.0;
   const XView<double> X{x, m, singular != 0, nullptr, 1, 1, 0, 0, 0, 0.0, false};
-  NBL_CELLS(NC);
   double acc[1] = {0.0};
   if (q.valid) {
     double xc[NC], xn[NC][6], raw[NC], hv[NC], out[NC];
@@ -1283,13 +1334,16 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(MgLevel L, double* __restr
 // apply the scalar update `op`.  Thread 0 reads the scalars coherently into a
 // local copy, updates it, and writes it back.
 __global__ void __launch_bounds__(kFinThreads) k_fin(RedBuf rb, FinArgs a, SolveCtl* ctl) {
+  // block p finalizes problem p of a batch from its producer blocks' partials
+  ctl += blockIdx.x;
+  const double* parts = rb.partials + size_t(blockIdx.x) * size_t(a.nparts);
   if (a.predicated && ldA(&ctl->done)) return;
   double t[2];
   if (a.K == 2) {
-    foldPartials<2>(rb.partials, rb.stride, a.nparts, t);
+    foldPartials<2>(parts, rb.stride, a.nparts, t);
   } else {
     double t1[1];
-    foldPartials<1>(rb.partials, rb.stride, a.nparts, t1);
+    foldPartials<1>(parts, rb.stride, a.nparts, t1);
     t[0] = t1[0];
     t[1] = 0.0;
   }
@@ -1389,6 +1443,7 @@ __global__ void __launch_bounds__(kFinThreads) k_fin(RedBuf rb, FinArgs a, Solve
 // Fresh solve scalars written by a kernel (not a host copy), so they reach
 // memory through the same ordered, cache-coherent path as every later write.
 __global__ void k_ctl_init(SolveCtl* ctl, double tol) {
+  ctl += blockIdx.x;  // one block per problem of a batch
   const int t = threadIdx.x;
   if (t < 64) stA(&ctl->slot[t], 0.0);
   if (t == 0) {
@@ -1405,13 +1460,13 @@ __global__ void k_ctl_init(SolveCtl* ctl, double tol) {
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-void launch_ctl_init(SolveCtl* ctl, double tol, hipStream_t s) {
-  k_ctl_init<<<1, 64, 0, s>>>(ctl, tol);
+void launch_ctl_init(SolveCtl* ctl, double tol, hipStream_t s, int nb) {
+  k_ctl_init<<<std::max(nb, 1), 64, 0, s>>>(ctl, tol);
   NBL_LAUNCH_CHECK();
 }
 
-void launch_fin(RedBuf rb, const FinArgs& a, SolveCtl* ctl, hipStream_t s) {
-  k_fin<<<1, kFinThreads, 0, s>>>(rb, a, ctl);
+void launch_fin(RedBuf rb, const FinArgs& a, SolveCtl* ctl, hipStream_t s, int nb) {
+  k_fin<<<std::max(nb, 1), kFinThreads, 0, s>>>(rb, a, ctl);
   NBL_LAUNCH_CHECK();
 }
 
@@ -1421,19 +1476,20 @@ void launch_mg_sweep(const MgLevel& L, const T* b, MeanRef mb, const T* x, MeanR
                      RedBuf rb, SolveCtl* ctl, hipStream_t s, long long* stamps, InFold conv) {
   const int ncx = Lc ? Lc->nx : 1, ncy = Lc ? Lc->ny : 1;
   const int sx = Lc ? Lc->fx - 1 : 0, sy = Lc ? Lc->fy - 1 : 0, sz = Lc ? Lc->fz - 1 : 0;
+  const int ecN = Lc ? int(Lc->n) : 0;
   const int f = fold ? 1 : 0;
   if (conv.n && !zeroGuess) throw std::logic_error("mg sweep: only a sweep from zero carries the convergence test");
   withCells(L, [&](auto nc, const Geo3& g) {
     constexpr int NC = decltype(nc)::value;
     if (zeroGuess)
-      k_sweep<T, SW_ZERO, NC><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, nullptr, MeanRef{}, 1, 1, 0, 0, 0, xo, f,
-                                                         rb, ctl, stamps, conv);
+      k_sweep<T, SW_ZERO, NC><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, nullptr, MeanRef{}, 1, 1, 0, 0, 0, 0, xo,
+                                                         f, rb, ctl, stamps, conv);
     else if (ec)
-      k_sweep<T, SW_PROLONG, NC><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, ec, mec, ncx, ncy, sx, sy, sz, xo, f,
-                                                            rb, ctl, stamps, conv);
+      k_sweep<T, SW_PROLONG, NC><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, ec, mec, ncx, ncy, sx, sy, sz, ecN, xo,
+                                                            f, rb, ctl, stamps, conv);
     else
-      k_sweep<T, SW_PLAIN, NC><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, nullptr, MeanRef{}, 1, 1, 0, 0, 0, xo, f,
-                                                          rb, ctl, stamps, conv);
+      k_sweep<T, SW_PLAIN, NC><<<g.grid, g.block, 0, s>>>(L, b, mb, x, mx, nullptr, MeanRef{}, 1, 1, 0, 0, 0, 0, xo,
+                                                          f, rb, ctl, stamps, conv);
   });
   NBL_LAUNCH_CHECK();
 }
@@ -1456,7 +1512,9 @@ Geo3 restrictGeo(const MgLevel& L, const MgLevel& Lc) {
   Geo3 g;
   g.block = dim3(bx, by, bz);
   const unsigned gx = (tx + bx - 1) / bx, gy = (L.ny + by - 1) / by, gz = (L.nz + bz - 1) / bz;
-  g.grid = L.slab == 1 ? dim3(8 * gx, gy, (gz + 7) / 8) : L.slab == 2 ? dim3(8 * gx, (gy + 7) / 8, gz) : dim3(gx, gy, gz);
+  if (L.nb > 1 && L.slab == 1) throw std::logic_error("a batch runs its levels in plain order or bands of rows");
+  const unsigned gzb = gz * unsigned(std::max(L.nb, 1));  // a batch's problems follow each other along z
+  g.grid = L.slab == 1 ? dim3(8 * gx, gy, (gz + 7) / 8) : L.slab == 2 ? dim3(8 * gx, (gy + 7) / 8, gzb) : dim3(gx, gy, gzb);
   g.blocks = int(g.grid.x * g.grid.y * g.grid.z);
   return g;
 }
@@ -1496,7 +1554,8 @@ void launch_mg_finish(const MgLevel& L, const T* x, MeanRef mx, double* z, bool 
 
 template <class T>
 void launch_to_level0(const MgLevel& L, const double* r, T* b, const SolveCtl* ctl, hipStream_t s) {
-  k_to_level0<T><<<gridFor(L.n), kBlock, 0, s>>>(int(L.n), r, b, ctl);
+  const int nb = std::max(L.nb, 1);
+  k_to_level0<T><<<gridFor(L.n * nb), kBlock, 0, s>>>(int(L.n), nb, r, b, ctl);
   NBL_LAUNCH_CHECK();
 }
 
@@ -1525,7 +1584,7 @@ void launch_mg_coarse_direct(const MgLevel& L, const double* M, const float* Mf,
   // wavefront per row on the 1536-cell cube-channel level (5.1 us vs 6.7 us
   // for 4 rows, back-to-back)
   constexpr int kRows = 2;
-  const int blocks = int((L.n + kRows - 1) / kRows);
+  const int blocks = int((L.n + kRows - 1) / kRows) * std::max(L.nb, 1);
   if constexpr (sizeof(T) == sizeof(float))
     k_coarse_direct<T, float, kRows><<<blocks, kBlock, 0, s>>>(L, Mf, b, mb, x, ctl);
   else

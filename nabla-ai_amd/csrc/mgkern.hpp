@@ -37,30 +37,46 @@ __device__ __forceinline__ Cell cellOf(int nx, int ny, int nz) {
 // 64-bit word.
 template <int NC>
 struct CellN {
-  int i, j, k, c, blk;  // i: first cell of the group
+  int i, j, k, c, blk;  // i: first cell of the group; c counts the batch's earlier problems
+  int p;                // problem of a batch
   bool valid;
 };
 
+// nb > 1: a batch of nb problems whose planes follow each other along the
+// launch's z (plane z of the launch is plane z mod nz of problem z / nz).
 template <int NC>
-__device__ __forceinline__ CellN<NC> cellOfN(int nx, int ny, int nz, int slab) {
+__device__ __forceinline__ CellN<NC> cellOfN(int nx, int ny, int nz, int slab, int nb = 1) {
   CellN<NC> q;
   const Blk3 B = blk3(slab);
   q.i = NC * (B.x * int(blockDim.x) + int(threadIdx.x));
   q.j = B.y * int(blockDim.y) + int(threadIdx.y);
   q.k = B.z;
+  q.p = 0;
+  if (nb > 1) {
+    q.p = q.k / nz;
+    q.k -= q.p * nz;
+  }
   q.valid = q.i < nx && q.j < ny && q.k < nz;
-  q.c = q.i + nx * (q.j + ny * q.k);
+  q.c = q.i + nx * (q.j + ny * (q.k + nz * q.p));
   q.blk = B.lin;
   return q;
+}
+
+// The first block of this cell kernel's problem (block 0 of a single
+// problem): the one that counts the problem's iterations.
+template <int NC>
+__device__ __forceinline__ bool problemLead(const CellN<NC>& q, int nz, int nb) {
+  if (nb <= 1) return blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0;
+  return q.blk == q.p * int(gridDim.x * gridDim.y) * nz;
 }
 
 // Runs f(std::integral_constant<int, NC>, launch shape) with the level's
 // cells per thread.
 template <class F>
 void withCells(const MgLevel& L, F&& f) {
-  if (L.vec == 4) f(std::integral_constant<int, 4>{}, geo3n(L.nx, L.ny, L.nz, 4, L.slab));
-  else if (L.vec == 2) f(std::integral_constant<int, 2>{}, geo3n(L.nx, L.ny, L.nz, 2, L.slab));
-  else f(std::integral_constant<int, 1>{}, geo3n(L.nx, L.ny, L.nz, 1, L.slab));
+  if (L.vec == 4) f(std::integral_constant<int, 4>{}, geo3n(L.nx, L.ny, L.nz, 4, L.slab, L.nb));
+  else if (L.vec == 2) f(std::integral_constant<int, 2>{}, geo3n(L.nx, L.ny, L.nz, 2, L.slab, L.nb));
+  else f(std::integral_constant<int, 1>{}, geo3n(L.nx, L.ny, L.nz, 1, L.slab, L.nb));
 }
 
 template <class T>

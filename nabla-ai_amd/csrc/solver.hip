@@ -152,8 +152,18 @@ double* slotPtr(SolveCtl* ctl, int i) {
 }  // namespace
 
 PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, int maxIters,
-                       const nbl_solver_opts& opts)
+                       const nbl_solver_opts& opts, int problems)
     : ctx_(ctx), kind_(kind), tol_(tol), maxIt_(maxIters), opts_(opts) {
+  if (problems < 0) throw std::invalid_argument("a batch holds at least one problem");
+  batched_ = problems > 0;
+  nb_ = std::max(problems, 1);
+  if (batched_) {
+    if (kind != 0) throw std::invalid_argument("batched solves: mg-pcg only");
+    if (opts.loops) throw std::invalid_argument("batched solves: the fused kernels only");
+    if (opts.vcycle == 2) throw std::invalid_argument("batched solves: the one-kernel-per-sweep V-cycle only");
+    if (opts.precision == NBL_PREC_MIXED_BF16) throw std::invalid_argument("batched solves: FP64 or FP32 levels");
+    if (opts.launch_order == 2) throw std::invalid_argument("batched solves: plain order or bands of rows");
+  }
   if (kind != 0 && kind != 1) throw std::invalid_argument("poisson kind must be 0 (mg-pcg) or 1 (cg-jacobi)");
   if (maxIters < 0) throw std::invalid_argument("max iterations must be >= 0");
   if (opts.pre_smooth < 1 || opts.post_smooth < 1)
@@ -174,9 +184,12 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
   int stride = 1;
   for (int l = 0; l < nl; ++l) {
     const HostLevel& H = hl_[l];
-    codes_.emplace_back(ctx, H.code.size() * sizeof(uint16_t));
-    NBL_HIP(hipMemcpy(codes_.back().as<void>(), H.code.data(), H.code.size() * sizeof(uint16_t),
-                      hipMemcpyHostToDevice));
+    // a batch's problems share the grid: one copy of the codes per problem, so
+    // that every cell kernel indexes codes and vectors alike
+    const size_t cb = H.code.size() * sizeof(uint16_t);
+    codes_.emplace_back(ctx, cb * size_t(nb_));
+    for (int q = 0; q < nb_; ++q)
+      NBL_HIP(hipMemcpy(codes_.back().as<char>() + size_t(q) * cb, H.code.data(), cb, hipMemcpyHostToDevice));
     MgLevel m{};
     m.nx = H.g.nx;
     m.ny = H.g.ny;
@@ -199,20 +212,25 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
     if (mixed_ && m.nx % 4 == 0 && m.nx >= 64 && m.n >= (1LL << 22)) autoVec = 4;
     m.vec = want == 0 ? autoVec : (m.nx % want == 0 ? want : 1);
     m.slab = opts.launch_order == 0 ? slabFor(m.nx, m.ny, m.nz) : opts.launch_order - 1;
+    if (batched_ && m.slab == 1) m.slab = 0;  // a batch's problems follow each other along the launch's z
+    m.nb = nb_;
     dl_.push_back(m);
-    blocks_.push_back(levelGeo(m).blocks);
-    stride = std::max(stride, blocks_.back());
-    tblocks_.push_back(tileBlocks(m));
+    // partials: per problem (the finalizes' counts); the buffer holds the batch's
+    const int lb = levelGeo(m).blocks;
+    blocks_.push_back(lb / nb_);
+    stride = std::max(stride, lb);
+    tblocks_.push_back(batched_ ? 0 : tileBlocks(m));
     stride = std::max(stride, tblocks_.back());
-    rblocks_.push_back(l > 0 ? restrictGeo(dl_[l - 1], m).blocks : 0);
-    stride = std::max(stride, rblocks_.back());
-    const size_t n = H.g.cells();
+    const int rb = l > 0 ? restrictGeo(dl_[l - 1], m).blocks : 0;
+    rblocks_.push_back(rb / nb_);
+    stride = std::max(stride, rb);
+    const size_t n = H.g.cells() * size_t(nb_);
     lb_.emplace_back(ctx, l > 0 ? n * ts : 0);
     lx0_.emplace_back(ctx, kind == 0 ? n * ts : 0);
     lx1_.emplace_back(ctx, kind == 0 ? n * ts : 0);
   }
-  if (size_t(hl_[0].g.cells()) >= (size_t(1) << 31))
-    throw std::invalid_argument("grid too large: cell indices must fit in 32 bits");
+  if (size_t(hl_[0].g.cells()) * size_t(nb_) >= (size_t(1) << 31))
+    throw std::invalid_argument("grid too large: cell indices of the solve (all problems) must fit in 32 bits");
   if (kind == 0) coarseScr_ = DevBuf(ctx, 4 * hl_.back().g.cells() * ts);
   if (opts.coarse != NBL_COARSE_DIRECT && opts.coarse != NBL_COARSE_CG)
     throw std::invalid_argument("unknown coarse solve kind");
@@ -229,25 +247,31 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
       ldiag_.emplace_back(ctx, hl_[l].g.cells() * 8);
       launch_diag(dl_[l], ldiag_.back().as<double>(), nullptr);
     }
-  if (mixed_) b0f_ = DevBuf(ctx, n0 * sizeof(float));
-  x_ = DevBuf(ctx, n0 * 8);
-  rhs_ = DevBuf(ctx, n0 * 8);
-  r_ = DevBuf(ctx, n0 * 8);
-  rn_ = DevBuf(ctx, n0 * 8);
-  z_ = DevBuf(ctx, n0 * 8);
-  p_ = DevBuf(ctx, n0 * 8);
-  p2_ = DevBuf(ctx, n0 * 8);
-  Ap_ = DevBuf(ctx, n0 * 8);
-  ctl_ = DevBuf(ctx, sizeof(SolveCtl));
+  const size_t nv = n0 * size_t(nb_);  // the FP64 flexible-CG vectors of every problem
+  if (mixed_) b0f_ = DevBuf(ctx, nv * sizeof(float));
+  x_ = DevBuf(ctx, nv * 8);
+  rhs_ = DevBuf(ctx, nv * 8);
+  r_ = DevBuf(ctx, nv * 8);
+  rn_ = DevBuf(ctx, nv * 8);
+  z_ = DevBuf(ctx, nv * 8);
+  p_ = DevBuf(ctx, nv * 8);
+  p2_ = DevBuf(ctx, nv * 8);
+  Ap_ = DevBuf(ctx, nv * 8);
+  ctl_ = DevBuf(ctx, sizeof(SolveCtl) * size_t(nb_));
   redStride_ = stride;
   red_ = DevBuf(ctx, sizeof(double) * (kMaxRedVals + kSlotRows) * size_t(stride));
-  NBL_HIP(hipMemset(ctl_.as<void>(), 0, sizeof(SolveCtl)));
-  NBL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hctl_), sizeof(SolveCtl) * 2, hipHostMallocDefault));
-  std::memset(hctl_, 0, sizeof(SolveCtl) * 2);
+  NBL_HIP(hipMemset(ctl_.as<void>(), 0, sizeof(SolveCtl) * size_t(nb_)));
+  const size_t nh = std::max<size_t>(2, size_t(nb_));
+  NBL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hctl_), sizeof(SolveCtl) * nh, hipHostMallocDefault));
+  std::memset(hctl_, 0, sizeof(SolveCtl) * nh);
   NBL_HIP(hipEventCreate(&e0_));
   NBL_HIP(hipEventCreate(&e1_));
   predicted_ = std::max(1, opts.check_every);
   if (kind == 0 && nl > 1 && opts.coarse == NBL_COARSE_DIRECT && !loops_) direct_ = buildDirectCoarse();
+  if (batched_ && nl < 2) throw std::invalid_argument("batched solves: grids of at least two multigrid levels");
+  if (batched_ && !direct_)
+    throw std::invalid_argument("batched solves: the coarsest level must be solved directly (coarse DIRECT, "
+                                "at most 4096 cells, an invertible operator)");
   if (opts.vcycle < 0 || opts.vcycle > 2) throw std::invalid_argument("vcycle must be 0, 1 or 2");
   // the two-kernels-per-level cycle: V(2,2), a level between the finest and
   // the directly solved coarsest one, and few enough level-0 workgroups that
@@ -258,7 +282,7 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
   // on the 64^3 periodic box (singular) it also drops the per-sweep
   // projections and a step takes 3.58 instead of 5.05 ms.
   const bool wantTiled = opts.vcycle == 2 || (opts.vcycle == 0 && (project_ || bf16_));
-  tiled_ = kind == 0 && !loops_ && wantTiled && nl >= 3 && opts.pre_smooth == 2 && opts.post_smooth == 2 &&
+  tiled_ = kind == 0 && !loops_ && !batched_ && wantTiled && nl >= 3 && opts.pre_smooth == 2 && opts.post_smooth == 2 &&
            direct_ && blocks_[0] <= kInlineFoldMax && tblocks_[0] <= kInlineFoldMax;
   for (int l = 0; tiled_ && l + 1 < nl; ++l) tiled_ = tileFits(dl_[l], dl_[l + 1]);
   if (tiled_ && project_) rnRaw_ = DevBuf(ctx, n0 * 8);
@@ -391,7 +415,7 @@ void PoissonDev::fin(int op, int K, int nparts, int slot, long long nf, bool pre
   a.convTol = convTol;
   a.earlyExit = earlyExit;
   a.predicated = predicated ? 1 : 0;
-  launch_fin(red(), a, ctl_.as<SolveCtl>(), s);
+  launch_fin(red(), a, ctl_.as<SolveCtl>(), s, nb_);
 }
 
 // In-situ timing of the finest level's plain sweeps (the roofline kernel):
@@ -401,7 +425,7 @@ void PoissonDev::fin(int op, int K, int nparts, int slot, long long nf, bool pre
 // runs as untimed).  A launch lasts from its first block start to its last block end;
 // launches a converged solve skips on the device leave their slice at zero.
 long long* PoissonDev::timedSweep(int l, bool plain) {
-  if (!timeSweeps_ || l != 0 || !plain || stampUsed_ >= stampCap_) return nullptr;
+  if (batched_ || !timeSweeps_ || l != 0 || !plain || stampUsed_ >= stampCap_) return nullptr;
   return stamps_.as<long long>() + size_t(stampUsed_++) * 2 * size_t(blocks_[0]);
 }
 
@@ -883,6 +907,7 @@ void PoissonDev::enqueueIterationLoops(hipStream_t s) {
 // level-0 projection.  d_r and d_z are device vectors of the level-0 size.
 void PoissonDev::precondition(const double* d_r, double* d_z, hipStream_t s) {
   if (kind_ != 0) throw std::invalid_argument("precondition: mg-pcg solver only");
+  if (batched_) throw std::invalid_argument("precondition: not on a batch");
   SolveCtl* ctl = ctl_.as<SolveCtl>();
   const MgLevel& L = dl_[0];
   launch_ctl_init(ctl, tol_, s);
@@ -927,7 +952,7 @@ bool PoissonDev::iterationPair(hipStream_t s) {
 }
 
 void PoissonDev::readCtl(hipStream_t s) {
-  NBL_HIP(hipMemcpyAsync(hctl_, ctl_.as<void>(), sizeof(SolveCtl), hipMemcpyDeviceToHost, s));
+  NBL_HIP(hipMemcpyAsync(hctl_, ctl_.as<void>(), sizeof(SolveCtl) * size_t(nb_), hipMemcpyDeviceToHost, s));
   NBL_HIP(hipStreamSynchronize(s));
 }
 
@@ -942,11 +967,11 @@ nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t
   const MgLevel& L = dl_[0];
   const RedBuf rb = red();
   const int nb = blocks_[0];
-  launch_ctl_init(ctl, tol_, s);
+  launch_ctl_init(ctl, tol_, s, nb_);
   NBL_HIP(hipEventRecord(e0_, s));
   vcount_ = 0;
-  inline_ = kind_ == 0 && !loops_ && nlevels() > 1 && opts_.pre_smooth >= 2 && opts_.post_smooth >= 2 &&
-            nb <= kInlineFoldMax;
+  inline_ = kind_ == 0 && !loops_ && !batched_ && nlevels() > 1 && opts_.pre_smooth >= 2 &&
+            opts_.post_smooth >= 2 && nb <= kInlineFoldMax;
   conv_ = InFold{};
   double* x = x_.as<double>();
   double* rhs = rhs_.as<double>();
@@ -982,7 +1007,9 @@ nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t
       }
     }
     readCtl(s);
-    if (hctl_[0].done) break;
+    bool done = true;
+    for (int q = 0; q < nb_; ++q) done = done && hctl_[q].done;
+    if (done) break;
     batch = step;
   }
   if (project_) {
@@ -997,12 +1024,21 @@ nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t
   NBL_HIP(hipEventElapsedTime(&ms, e0_, e1_));
   lastMs_ = ms;
   lastV_ = vcount_;
-  const SolveCtl& h = hctl_[0];
-  if (h.converged && h.it > 0) predicted_ = h.it;
-  return nbl_poisson_report{h.it, h.resid, h.converged};
+  reps_.resize(size_t(nb_));
+  int most = 0;
+  bool allConv = true;
+  for (int q = 0; q < nb_; ++q) {
+    const SolveCtl& h = hctl_[q];
+    reps_[size_t(q)] = nbl_poisson_report{h.it, h.resid, h.converged};
+    most = std::max(most, h.it);
+    allConv = allConv && h.converged;
+  }
+  if (allConv && most > 0) predicted_ = most;
+  return reps_[0];
 }
 
 void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double* bytes) {
+  if (batched_) throw std::invalid_argument("profile: not on a batch");
   if (kernel < 0 || kernel > 5) throw std::invalid_argument("profile: kernel must be 0 .. 5");
   if (reps < 1) throw std::invalid_argument("profile: reps must be >= 1");
   if (kind_ != 0) throw std::invalid_argument("profile: mg-pcg solver only");

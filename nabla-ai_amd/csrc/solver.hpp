@@ -78,12 +78,20 @@ bool hasOutlet(const HostGrid& g);
 // V(pre,post) cycle per iteration (MgPoisson), kind 1 = Jacobi CG (CgPoisson).
 class PoissonDev {
  public:
+  // problems > 0: a batch of that many independent problems on the same grid,
+  // solved together (one launch per level kernel for all of them; each
+  // problem has its own scalars, convergence test and report).  Vectors are
+  // then the problems' vectors one after another.
   PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, int maxIters,
-             const nbl_solver_opts& opts);
+             const nbl_solver_opts& opts, int problems = 0);
   ~PoissonDev();
   // Solve with device buffers on `stream` (b and x may be any device arrays of
   // length n).  Returns the reference's Report.
   nbl_poisson_report solve(const double* d_b, double* d_x, hipStream_t stream);
+  int problems() const { return nb_; }
+  bool batched() const { return batched_; }
+  // Reports of the last solve, one per problem.
+  const std::vector<nbl_poisson_report>& reports() const { return reps_; }
   // One application of the preconditioner (device vectors).
   void precondition(const double* d_r, double* d_z, hipStream_t stream);
   const std::vector<HostLevel>& levels() const { return hl_; }
@@ -120,7 +128,7 @@ class PoissonDev {
   // producer with `blocks` blocks is folded by its consumers from the slot's
   // own partial row when `inl` (few enough partials), else finalized into
   // SolveCtl by k_fin.
-  static bool inlineMean(int blocks) { return blocks <= kInlineFoldMax; }
+  bool inlineMean(int blocks) const { return !batched_ && blocks <= kInlineFoldMax; }
   MeanRef meanRef(int slot, bool inl, int blocks, long long nf) const;
   RedBuf meanRed(int slot, bool inl) const;
   MeanRef zMean() const;  // pending mean of the preconditioner output z
@@ -162,6 +170,9 @@ class PoissonDev {
   int kind_;
   double tol_;
   int maxIt_;
+  int nb_ = 1;            // problems (1 unless batched)
+  bool batched_ = false;  // created as a batch: every reduction through k_fin, per problem
+  std::vector<nbl_poisson_report> reps_;
   nbl_solver_opts opts_;
   bool singular_ = false;
   bool project_ = false;  // singular with fluid cells: null-space projections apply

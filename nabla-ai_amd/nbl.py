@@ -261,6 +261,11 @@ def load(path: os.PathLike | None = None):
     _proto(lib, "nbl_server_footprint", i, V, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64))
     _proto(lib, "nbl_poisson_solve_batch", i, C.POINTER(V), C.POINTER(P_D), C.POINTER(P_D), i, i,
            C.POINTER(PoissonReport))
+    _proto(lib, "nbl_poisson_batch_create", i, V, C.POINTER(GridDesc), i, d, i, C.POINTER(SolverOpts),
+           C.POINTER(V))
+    _proto(lib, "nbl_poisson_batch_size", i, V, C.POINTER(i))
+    _proto(lib, "nbl_poisson_batch_solve", i, V, C.POINTER(P_D), C.POINTER(P_D), C.POINTER(PoissonReport))
+    _proto(lib, "nbl_poisson_batch_solve_device", i, V, V, V, C.POINTER(PoissonReport))
     _proto(lib, "nbl_octree_solve_batch", i, C.POINTER(V), C.POINTER(P_D), C.POINTER(P_D), i, i, d, i,
            C.POINTER(PoissonReport))
     _proto(lib, "nbl_octree_mg_solve_batch", i, C.POINTER(V), C.POINTER(P_D), C.POINTER(P_D), i, i, d, i,
@@ -527,6 +532,53 @@ def build_tree(origin, extent, max_level, base, predicate):
         if not f.any():
             return t
         t.refine_flagged(f)
+
+
+class PoissonBatch:
+    """nbl_poisson_batch_*: `problems` independent pressure systems on one
+    grid solved together (one launch per level kernel for all of them), each
+    with its own flexible-CG scalars, stopping test and report."""
+
+    def __init__(self, ctx: Ctx, grid: Grid, problems: int, tol=1e-9, max_iters=100,
+                 opts: SolverOpts | None = None):
+        self.lib, self.grid, self.problems = ctx.lib, grid, int(problems)
+        self._desc = grid.desc()
+        self.h = C.c_void_p()
+        check(self.lib.nbl_poisson_batch_create(ctx.h, C.byref(self._desc), self.problems, tol, max_iters,
+                                                C.byref(opts) if opts is not None else None, C.byref(self.h)))
+
+    def size(self):
+        n = C.c_int()
+        check(self.lib.nbl_poisson_batch_size(self.h, C.byref(n)))
+        return n.value
+
+    def last_timing(self):
+        """(device ms of the last batch solve, V-cycles it applied)."""
+        ms, vc = C.c_double(), C.c_int()
+        check(self.lib.nbl_poisson_last_timing(self.h, C.byref(ms), C.byref(vc)))
+        return ms.value, vc.value
+
+    def solve(self, bs):
+        """bs: one right-hand side per problem (host arrays); returns (xs, reports)."""
+        assert len(bs) == self.problems
+        bs = [np.ascontiguousarray(b, dtype=np.float64) for b in bs]
+        xs = [np.empty(self.grid.cells) for _ in bs]
+        bp = (P_D * self.problems)(*[dptr(b) for b in bs])
+        xp = (P_D * self.problems)(*[dptr(x) for x in xs])
+        reps = (PoissonReport * self.problems)()
+        check(self.lib.nbl_poisson_batch_solve(self.h, bp, xp, reps))
+        return xs, list(reps)
+
+    def close(self):
+        if self.h:
+            check(self.lib.nbl_poisson_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class Octree:

@@ -53,6 +53,34 @@ def test_batched_uniform_solves_at_size(ctx, orc):
     assert checked >= 10
 
 
+def test_batched_uniform_solves_in_size_classes(ctx, orc):
+    """The same 48 problems as batches, one per size class (one launch per
+    level kernel for all problems of a class): every problem converges to
+    1e-8 (true residual recomputed with the oracle's operator), its fluid
+    mean is zero, and it agrees with the same problem solved alone (iterations
+    within one, solution within rounding of the tolerance)."""
+    sizes, grids, rhs = cases.batch_uniform(48, 4242)
+    for n in (32, 64, 96, 128):
+        idx = [q for q, s in enumerate(sizes) if s == n]
+        g = grids[idx[0]]
+        with nbl.PoissonBatch(ctx, g, len(idx), tol=1e-8, max_iters=200) as pb:
+            xs, reps = pb.solve([rhs[q] for q in idx])
+        ps = nbl.Poisson(ctx, g, kind=0, tol=1e-8, max_iters=200)
+        try:
+            for x, r, q in zip(xs, reps, idx):
+                b = rhs[q]
+                assert r.converged, q
+                res = np.linalg.norm(b - orc.apply(g, x)) / np.linalg.norm(b)
+                assert res <= 1e-8 * (1 + 1e-6), (q, res)
+                assert abs(x.mean()) <= 1e-12 * np.abs(x).max(), q
+                x1, r1 = ps.solve(b)
+                assert abs(r.iterations - r1.iterations) <= 1, q
+                assert rel(x, x1) < 1e-7, q
+        finally:
+            ps.close()
+        print(f"{n}^3: {len(idx)} problems in one batch, iterations {[r.iterations for r in reps]}", flush=True)
+
+
 @pytest.mark.parametrize("mg", [False, True], ids=["jacobi_pcg", "mg_pcg"])
 def test_batched_octree_solves_at_size(ctx, mg):
     trees = cases.batch_octrees(8, 4242 + 99)
