@@ -185,6 +185,7 @@ typedef struct nbl_ctx nbl_ctx;
 typedef struct nbl_run nbl_run;
 typedef struct nbl_poisson nbl_poisson;
 typedef struct nbl_octree nbl_octree;
+typedef struct nbl_octree_flow nbl_octree_flow;
 typedef struct nbl_server nbl_server;
 typedef struct nbl_tree nbl_tree;
 
@@ -417,6 +418,68 @@ int nbl_octree_mg_levels(nbl_octree* oc, long long* counts, int cap, int* levels
  * Jacobi-PCG only, octree_poisson.cpp:107-197).  Applies from the next solve;
  * NBL_ERR_INVALID for other values. */
 int nbl_octree_set_precision(nbl_octree* oc, int precision);
+
+/* ---- octree flow step (no reference implementation: the collocated
+ * finite-volume solve the reference's design record plans on the octree
+ * leaves, docs/adr/0008-octree-native-flow-solve.md:35-86, over its
+ * OctreePoisson operator, octree_poisson.cpp:64-105) -------------------------
+ * Cell-centred u, v, w, p on the leaves; face-normal velocities on the face
+ * table's faces.  A step is SSP-RK3 (the MAC solver's stages, ns_solver.cpp:
+ * 773-795) with a projection after each stage: provisional face velocities
+ * 0.5 (u_P + u_N).n, A p = -(sum_f U_f A_f)/dt with the OctreePoisson
+ * operator (area_f / dist_f couplings, Neumann at the domain boundary and at
+ * non-fluid leaves; the octree multigrid CG, nbl_octree_mg_solve), then
+ * U_f -= dt (p_N - p_P)/dist_f and u_P -= dt (Gauss gradient of p)_P.  The
+ * momentum right-hand side is -(1/V) sum_f U_f A_f phi_f + (nu/V) sum_f A_f
+ * (phi_N - phi_P)/dist_f, phi_f WENO-5 where the face has a five-leaf line
+ * of one level through it and first-order upwind otherwise (at 2:1 jumps),
+ * every face term formed identically from both sides (conservative across
+ * level jumps).  Domain faces: NBL_OCT_WALL no-slip, NBL_OCT_SLIP free-slip,
+ * NBL_OCT_INLET uniform inflow of inlet_speed along the inward normal,
+ * NBL_OCT_OUTLET zero-gradient outflow scaled to the inflow's volume flux
+ * (so the pressure system stays compatible).  Non-fluid leaves are no-slip
+ * walls with zero velocity. */
+enum { NBL_OCT_WALL = 0, NBL_OCT_SLIP = 1, NBL_OCT_INLET = 2, NBL_OCT_OUTLET = 3 };
+typedef struct nbl_octree_flow_cfg {
+  double nu;               /* kinematic viscosity                                  */
+  double cfl;              /* dt = cfl / max over fluid leaves of |u|/hx + |v|/hy +
+                              |w|/hz + 2 nu (1/hx^2 + 1/hy^2 + 1/hz^2) (the MAC
+                              solver's rule, ns_solver.cpp:581-603, per leaf)      */
+  double inlet_speed;      /* mean inflow speed                                    */
+  int inlet_profile;       /* 0 uniform; 1 plane Poiseuille across the next axis
+                              (x inlet: y): 6 s (1 - s), s the leaf centre's
+                              fraction of the domain along it, times inlet_speed */
+  int face[6];             /* -x, +x, -y, +y, -z, +z: NBL_OCT_*                    */
+  int convection;          /* 0 first-order upwind, 1 WENO-5 away from level jumps */
+  double poisson_tol;      /* relative residual of each projection's solve         */
+  int poisson_max_iters;
+  int precision;           /* the multigrid cycle's arithmetic, NBL_PREC_*         */
+} nbl_octree_flow_cfg;
+typedef struct nbl_octree_step_report {
+  int step;
+  double t, dt;
+  double divergence_max;   /* max over fluid leaves of |sum_f U_f A_f| / V after
+                              the step's last projection                          */
+  double kinetic_energy;   /* sum over fluid leaves of 0.5 |u|^2 V                 */
+  double inflow, outflow;  /* boundary volume fluxes of the last projection        */
+  int poisson_iters;       /* the three projections' iterations                    */
+  double poisson_residual; /* largest of their final relative residuals            */
+  double solve_ms;         /* device ms of the three pressure solves               */
+} nbl_octree_step_report;
+void nbl_octree_flow_cfg_default(nbl_octree_flow_cfg* cfg);
+/* The flow keeps using the octree: destroy the flow first. */
+int nbl_octree_flow_create(nbl_octree* oc, const nbl_octree_flow_cfg* cfg, nbl_octree_flow** out);
+int nbl_octree_flow_destroy(nbl_octree_flow* f);
+/* Caller-order leaf arrays (n each); p may be NULL (zero).  Face velocities
+ * start as 0.5 (u_P + u_N).n of the given field. */
+int nbl_octree_flow_set_state(nbl_octree_flow* f, const double* u, const double* v, const double* w,
+                              const double* p);
+int nbl_octree_flow_get_state(nbl_octree_flow* f, double* u, double* v, double* w, double* p);
+/* One step; dt <= 0 takes the CFL step. */
+int nbl_octree_flow_step(nbl_octree_flow* f, double dt, nbl_octree_step_report* rep);
+/* sum_f U_f A_f / V per leaf (caller order; 0 on non-fluid leaves) of the
+ * current face velocities. */
+int nbl_octree_flow_divergence(nbl_octree_flow* f, double* div);
 
 /* No counterpart in the reference: measurement.  Average ms and algorithmic
  * bytes of one launch of an octree kernel (kernel 0: the SpMV, reading

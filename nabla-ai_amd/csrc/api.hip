@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "nbl_api.h"
+#include "octflow.hpp"
 #include "octree.hpp"
 #include "serve.hpp"
 #include "solver.hpp"
@@ -719,6 +720,75 @@ int nbl_octree_set_precision(nbl_octree* oc, int precision) {
          "precision must be NBL_PREC_FP64, NBL_PREC_MIXED or NBL_PREC_MIXED_BF16");
     useDevice(oc->ctx);
     oc->impl->setMgPrecision(precision);
+  });
+}
+
+void nbl_octree_flow_cfg_default(nbl_octree_flow_cfg* c) {
+  if (!c) return;
+  c->nu = 1.0 / 500.0;
+  c->cfl = 0.5;
+  c->inlet_speed = 1.0;
+  c->inlet_profile = 0;
+  const int faces[6] = {NBL_OCT_INLET, NBL_OCT_OUTLET, NBL_OCT_WALL, NBL_OCT_WALL, NBL_OCT_SLIP, NBL_OCT_SLIP};
+  for (int d = 0; d < 6; ++d) c->face[d] = faces[d];
+  c->convection = 1;
+  c->poisson_tol = 1e-8;
+  c->poisson_max_iters = 500;
+  c->precision = NBL_PREC_FP64;
+}
+
+int nbl_octree_flow_create(nbl_octree* oc, const nbl_octree_flow_cfg* cfg, nbl_octree_flow** out) {
+  return guarded([&] {
+    need(oc && cfg && out, "NULL argument");
+    need(cfg->precision >= NBL_PREC_FP64 && cfg->precision <= NBL_PREC_MIXED_BF16, "unknown precision");
+    useDevice(oc->ctx);
+    auto f = std::make_unique<nbl_octree_flow>();
+    f->ctx = oc->ctx;
+    f->tree = oc;
+    f->impl = std::make_unique<nbl::OctFlowDev>(oc->ctx, *oc->impl, *cfg);
+    *out = f.release();
+  });
+}
+
+int nbl_octree_flow_destroy(nbl_octree_flow* f) {
+  return guarded([&] {
+    if (!f) return;
+    useDevice(f->ctx);
+    delete f;
+  });
+}
+
+int nbl_octree_flow_set_state(nbl_octree_flow* f, const double* u, const double* v, const double* w,
+                              const double* p) {
+  return guarded([&] {
+    need(f && u && v && w, "NULL argument");
+    useDevice(f->ctx);
+    f->impl->setState(u, v, w, p);
+  });
+}
+
+int nbl_octree_flow_get_state(nbl_octree_flow* f, double* u, double* v, double* w, double* p) {
+  return guarded([&] {
+    need(f, "NULL argument");
+    useDevice(f->ctx);
+    f->impl->getState(u, v, w, p);
+  });
+}
+
+int nbl_octree_flow_step(nbl_octree_flow* f, double dt, nbl_octree_step_report* rep) {
+  return guarded([&] {
+    need(f, "NULL argument");
+    useDevice(f->ctx);
+    const nbl_octree_step_report r = f->impl->step(dt);
+    if (rep) *rep = r;
+  });
+}
+
+int nbl_octree_flow_divergence(nbl_octree_flow* f, double* div) {
+  return guarded([&] {
+    need(f && div, "NULL argument");
+    useDevice(f->ctx);
+    f->impl->divergence(div);
   });
 }
 

@@ -109,21 +109,6 @@ __global__ void k_faces(long long n, int maxL, const uint64_t* __restrict__ keys
   cnt[t] = uint8_t(c);
 }
 
-struct Geo {
-  double org[3], fine[3];
-  int maxL;
-};
-
-// Reference centre (octree.cpp:84-91) and size (octree.cpp:97-100).
-__device__ __forceinline__ double centerAxis(const Geo& G, uint64_t key, int lvl, int axis) {
-  const uint32_t a = gather3(key >> axis);
-  const uint32_t sz = 1u << (G.maxL - lvl);
-  const double units = double(a) + 0.5 * double(sz);
-  return G.org[axis] + units * G.fine[axis];
-}
-__device__ __forceinline__ double sizeAxis(const Geo& G, int lvl, int axis) {
-  return double(1u << (G.maxL - lvl)) * G.fine[axis];
-}
 // area of the finer cell / centre distance (octree_poisson.cpp:24-32)
 __device__ __forceinline__ double faceCoeff(const Geo& G, uint64_t ki, int li, uint64_t kj, int lj, int axis) {
   const int t1 = (axis + 1) % 3, t2 = (axis + 2) % 3;
@@ -604,12 +589,6 @@ __global__ void __launch_bounds__(kBlock) k_oct_transport(long long n, Geo G, co
 // Results go to the caller's order through the permutation: out[f*n + perm[i]]
 // for the eight real sensors and the carried speed / residual (rows 8, 9),
 // flags[f*n + perm[i]] for near-wall, reverse flow and residual stagnation.
-struct FaceWalk {
-  const uint8_t* __restrict__ cnt;    // u8[6n]
-  const int32_t* __restrict__ first;  // i32[6][n], -1 when none
-  const int32_t* __restrict__ xrow;   // further neighbours of leaf i from xnb[xrow[i]]
-  const int32_t* __restrict__ xnb;
-};
 
 // Leaf centres (octree.cpp:84-91) per axis, cen[a*n + i], for the sensors'
 // neighbour walks.

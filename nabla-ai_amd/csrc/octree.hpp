@@ -56,6 +56,34 @@ __host__ __device__ inline uint64_t mortonEnc3(uint32_t x, uint32_t y, uint32_t 
   return mortonSpread(x) | (mortonSpread(y) << 1) | (mortonSpread(z) << 2);
 }
 
+// Geometry of a tree's index space: origin, finest cell size per axis and the
+// finest level.
+struct Geo {
+  double org[3], fine[3];
+  int maxL;
+};
+// Reference centre (octree.cpp:84-91) and size (octree.cpp:97-100) of a leaf.
+__device__ __forceinline__ double centerAxis(const Geo& G, uint64_t key, int lvl, int axis) {
+  const uint32_t a = mortonAxis(key, axis);
+  const uint32_t sz = 1u << (G.maxL - lvl);
+  const double units = double(a) + 0.5 * double(sz);
+  return G.org[axis] + units * G.fine[axis];
+}
+__device__ __forceinline__ double sizeAxis(const Geo& G, int lvl, int axis) {
+  return double(1u << (G.maxL - lvl)) * G.fine[axis];
+}
+
+// The face table as the per-leaf kernels walk it: face counts per direction
+// (u8[6n], directions -x,+x,-y,+y,-z,+z), the first neighbour of each
+// direction slot-major (first[d*n + i], -1 when none), and the further
+// neighbours of leaf i in direction order from xnb[xrow[i]].
+struct FaceWalk {
+  const uint8_t* __restrict__ cnt;
+  const int32_t* __restrict__ first;
+  const int32_t* __restrict__ xrow;
+  const int32_t* __restrict__ xnb;
+};
+
 // Device view of a level's operator with coefficients and diagonal in T
 // (double: the operator itself; float: its FP32 copy for the mixed-precision
 // V-cycle).
@@ -155,10 +183,28 @@ class OctreeDev {
   int maxLevel() const { return maxL_; }
   const double* origin() const { return org_; }
   const double* fine() const { return fine_; }
-
- private:
+  Geo geo() const {
+    Geo G;
+    for (int a = 0; a < 3; ++a) {
+      G.org[a] = org_[a];
+      G.fine[a] = fine_[a];
+    }
+    G.maxL = maxL_;
+    return G;
+  }
+  FaceWalk faceWalk() const {
+    return FaceWalk{cnt_.as<uint8_t>(), tfirst_.as<int32_t>(), txrow_.as<int32_t>(), txnb_.as<int32_t>()};
+  }
+  // Face entries of the walk: the first neighbour of each direction at d*n + i
+  // (also the entry of a domain-boundary face, which has none), the further
+  // ones at 6n + their position in xnb.
+  size_t faceEntries() const { return 6 * L0_.n + tnx_; }
+  // Caller-order host array to sorted-order device array and back (on the
+  // octree's stream; fromSorted synchronises it).
   void toSorted(const double* hostIn, double* dSorted);
   void fromSorted(const double* dSorted, double* hostOut);
+
+ private:
   OctMg& mg();
 
   nbl_ctx* ctx_;

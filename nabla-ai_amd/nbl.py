@@ -66,6 +66,25 @@ class StepReport(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+OCT_WALL, OCT_SLIP, OCT_INLET, OCT_OUTLET = 0, 1, 2, 3
+
+
+class OctFlowCfg(C.Structure):
+    _fields_ = [("nu", C.c_double), ("cfl", C.c_double), ("inlet_speed", C.c_double), ("inlet_profile", C.c_int),
+                ("face", C.c_int * 6),
+                ("convection", C.c_int), ("poisson_tol", C.c_double), ("poisson_max_iters", C.c_int),
+                ("precision", C.c_int)]
+
+
+class OctStepReport(C.Structure):
+    _fields_ = [("step", C.c_int), ("t", C.c_double), ("dt", C.c_double), ("divergence_max", C.c_double),
+                ("kinetic_energy", C.c_double), ("inflow", C.c_double), ("outflow", C.c_double),
+                ("poisson_iters", C.c_int), ("poisson_residual", C.c_double), ("solve_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class PoissonReport(C.Structure):
     _fields_ = [("iterations", C.c_int), ("residual", C.c_double), ("converged", C.c_int)]
 
@@ -248,6 +267,13 @@ def load(path: os.PathLike | None = None):
     _proto(lib, "nbl_octree_mg_levels", i, V, C.POINTER(C.c_longlong), i, C.POINTER(i),
            C.POINTER(C.c_double))
     _proto(lib, "nbl_octree_set_precision", i, V, i)
+    _proto(lib, "nbl_octree_flow_cfg_default", None, C.POINTER(OctFlowCfg))
+    _proto(lib, "nbl_octree_flow_create", i, V, C.POINTER(OctFlowCfg), C.POINTER(V))
+    _proto(lib, "nbl_octree_flow_destroy", i, V)
+    _proto(lib, "nbl_octree_flow_set_state", i, V, P_D, P_D, P_D, P_D)
+    _proto(lib, "nbl_octree_flow_get_state", i, V, P_D, P_D, P_D, P_D)
+    _proto(lib, "nbl_octree_flow_step", i, V, d, C.POINTER(OctStepReport))
+    _proto(lib, "nbl_octree_flow_divergence", i, V, P_D)
     _proto(lib, "nbl_octree_profile", i, V, i, i, C.POINTER(C.c_double), C.POINTER(C.c_double))
     _proto(lib, "nbl_octree_last_timing", i, V, C.POINTER(C.c_double), C.POINTER(C.c_double))
     _proto(lib, "nbl_run_sweep_timing", i, V, i, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
@@ -532,6 +558,55 @@ def build_tree(origin, extent, max_level, base, predicate):
         if not f.any():
             return t
         t.refine_flagged(f)
+
+
+def oct_flow_cfg(**kw) -> OctFlowCfg:
+    """nbl_octree_flow_cfg_default with keyword overrides (face: 6 NBL_OCT_* values)."""
+    c = OctFlowCfg()
+    load().nbl_octree_flow_cfg_default(C.byref(c))
+    for k, v in kw.items():
+        if k == "face":
+            for d in range(6):
+                c.face[d] = int(v[d])
+        else:
+            setattr(c, k, v)
+    return c
+
+
+class OctreeFlow:
+    """nbl_octree_flow_*: the collocated finite-volume flow step on an
+    Octree's leaves (arrays in the octree's caller order)."""
+
+    def __init__(self, octree, cfg: OctFlowCfg):
+        self.lib, self.n, self.tree = octree.lib, octree.n, octree
+        self.h = C.c_void_p()
+        check(self.lib.nbl_octree_flow_create(octree.h, C.byref(cfg), C.byref(self.h)))
+
+    def set_state(self, u, v, w, p=None):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w)]
+        pp = None if p is None else np.ascontiguousarray(p, dtype=np.float64)
+        check(self.lib.nbl_octree_flow_set_state(self.h, *[dptr(a) for a in arrs],
+                                                 None if pp is None else dptr(pp)))
+
+    def get_state(self):
+        out = [np.empty(self.n) for _ in range(4)]
+        check(self.lib.nbl_octree_flow_get_state(self.h, *[dptr(a) for a in out]))
+        return out
+
+    def step(self, dt=0.0):
+        r = OctStepReport()
+        check(self.lib.nbl_octree_flow_step(self.h, dt, C.byref(r)))
+        return r
+
+    def divergence(self):
+        out = np.empty(self.n)
+        check(self.lib.nbl_octree_flow_divergence(self.h, dptr(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            check(self.lib.nbl_octree_flow_destroy(self.h))
+            self.h = C.c_void_p()
 
 
 class PoissonBatch:
