@@ -151,7 +151,55 @@ def coarse_solve(ctx, g, reps=50):
     return out
 
 
-def octree_c3(ctx, with_cpu=True):
+def octree_flow(d, masks, steps, warmup=3, precision=nbl.PREC_MIXED, seed=20240601):
+    """Flow steps on the cube-wake octree's leaves (nbl_octree_flow_step):
+    Re_h 500 (nu = 1/500, h = U = 1), CFL 0.7, WENO-5; uniform inflow at x = 0,
+    outlet at x = 12, no-slip floor and ceiling, free-slip side planes (the
+    uniform-grid cube channel is periodic in the span; the octree has no
+    periodic faces).  Initial u = 1 + 0.01 xi on fluid leaves (xi ~ U(-1,1),
+    splitmix64 seed), v = w = p = 0.  Each step: three RK stages, each followed
+    by a projection through the octree multigrid-preconditioned CG (FP32
+    V-cycle by default) to 1e-8, warm-started from the last pressure."""
+    n = masks.size
+    fl = masks == 0
+    u = np.where(fl, 1.0 + 0.01 * (2.0 * cases.uniform01(seed, n) - 1.0), 0.0)
+    z = np.zeros(n)
+    cfg = nbl.oct_flow_cfg(nu=1.0 / 500.0, cfl=0.7, inlet_speed=1.0, inlet_profile=0,
+                           face=(nbl.OCT_INLET, nbl.OCT_OUTLET, nbl.OCT_WALL, nbl.OCT_WALL,
+                                 nbl.OCT_SLIP, nbl.OCT_SLIP),
+                           convection=1, poisson_tol=1e-8, poisson_max_iters=500, precision=precision)
+    f = nbl.OctreeFlow(d, cfg)
+    try:
+        f.set_state(u, z, z)
+        for _ in range(warmup):
+            f.step()
+        reps = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            reps.append(f.step())
+        el = time.perf_counter() - t0
+        _u, _v, _w, p = f.get_state()
+    finally:
+        f.close()
+    it = np.array([r.poisson_iters for r in reps])
+    solve = float(sum(r.solve_ms for r in reps))
+    last = reps[-1]
+    return {"leaves": int(n), "fluid_leaves": int(fl.sum()), "steps": steps, "warmup": warmup,
+            "precision": {nbl.PREC_FP64: "fp64", nbl.PREC_MIXED: "fp32_vcycle",
+                          nbl.PREC_MIXED_BF16: "bf16_scratch"}[precision],
+            "ms_per_step": round(el / steps * 1e3, 3),
+            "mleaf_steps_per_s": round(n * steps / el / 1e6, 2),
+            "projection_ms_per_step": round(solve / steps, 3),
+            "projection_share": round(solve * 1e-3 / el, 3),
+            "poisson_iters_per_step_mean": round(float(it.mean()), 2), "poisson_iters_per_step_max": int(it.max()),
+            "max_poisson_residual": max(r.poisson_residual for r in reps),
+            "max_divergence": max(r.divergence_max for r in reps),
+            "dt_last": last.dt, "t_end": last.t, "kinetic_energy_end": last.kinetic_energy,
+            "inflow": last.inflow, "outflow": last.outflow,
+            "p_finite": bool(np.isfinite(p[fl]).all())}
+
+
+def octree_c3(ctx, with_cpu=True, flow_steps=200):
     """The cube-wake octree (4,294,942 leaves, levels 5..8, the cube's leaves
     InsideSolid): device setup (sort, validation, face table, operator), the
     SpMV roofline, a Jacobi-PCG solve to 1e-9 of a seeded mean-free
@@ -194,6 +242,7 @@ def octree_c3(ctx, with_cpu=True):
                           "converged": bool(q.converged), "solve_ms": round(qms, 3),
                           "speedup_vs_fp64_vcycle": round(mg_ms / max(qms, 1e-9), 3),
                           "rel_l2_vs_fp64_vcycle": rel(xq, xm)}
+    flow = octree_flow(d, m, flow_steps) if flow_steps > 0 else None
     d.close()
     ach = by / (ms * 1e-3) / 1e9
     out = {"leaves": n, "couplings": nnz, "host_build_s": round(host_build_s, 3),
@@ -222,6 +271,11 @@ def octree_c3(ctx, with_cpu=True):
                        "host_call_s": round(sens_call_s, 4),
                        "host_call_note": "nbl_octree_sensors with host arrays: PCIe copies of 6 inputs and 12 "
                                          "outputs included"}}
+    if flow is not None:
+        out["flow_steps"] = flow
+        log(f"octree flow: {flow['steps']} steps, {flow['ms_per_step']} ms/step "
+            f"({flow['mleaf_steps_per_s']} Mleaf-steps/s), projection {flow['projection_share']:.0%}, "
+            f"{flow['poisson_iters_per_step_mean']} CG iterations/step, max div {flow['max_divergence']:.2e}")
     log(f"octree: {n} leaves, setup {setup_ms:.1f} ms, Jacobi-PCG {rep.iterations} iterations in {solve_ms:.1f} ms, "
         f"MG-PCG {mrep.iterations} iterations in {mg_ms:.1f} ms; FP32 cycle "
         f"{variants['fp32_vcycle']['iterations']} in {variants['fp32_vcycle']['solve_ms']:.1f} ms, BF16 "
@@ -596,6 +650,10 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-octree", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--octree-flow-steps", type=int, default=200,
+                    help="flow steps on the cube-wake octree (0: skip)")
+    ap.add_argument("--octree-flow-only", action="store_true",
+                    help="only the cube-wake octree's flow steps (for profiler runs)")
     ap.add_argument("--no-tgv", action="store_true")
     ap.add_argument("--served-window", type=float, default=60.0,
                     help="seconds of Poisson arrivals in the served-stream measurement (0: skip)")
@@ -623,6 +681,16 @@ def main():
         ctx.close()
         print(json.dumps({"workload": "tgv64", "steps": a.steps, "ms_per_step": round(el / a.steps * 1e3, 4),
                           "value": round(tg.cells * a.steps / el / 1e6, 3)}), flush=True)
+        return
+    if a.octree_flow_only:
+        t = nbl.build_tree(cases.C3_ORIGIN, cases.C3_EXTENT, 8, 5, cases.c3_predicate)
+        k, l, c = t.leaves()
+        m = cases.c3_solid(c)
+        d = nbl.Octree(ctx, k, l, m, cases.C3_ORIGIN, cases.C3_EXTENT, 8)
+        fo = octree_flow(d, m, a.octree_flow_steps, warmup=a.warmup)
+        d.close()
+        ctx.close()
+        print(json.dumps({"octree_flow": fo}), flush=True)
         return
     if a.roofline_singular:
         out = {"singular_fp64_sweep": roofline(ctx, grid256(singular=True), key="256x256x256"),
@@ -720,7 +788,7 @@ def main():
     log(f"256^3: fp64 sweep {out['roofline']['frac']}, fp32 sweep {rb['fp32_sweep']['frac']}, "
         f"spmv {rb['spmv_fp64']['frac']}, singular fp64 sweep {rb['singular_fp64_sweep']['frac']}")
     if not a.no_octree:
-        out["octree_c3"] = octree_c3(ctx, with_cpu=not a.no_cpu)
+        out["octree_c3"] = octree_c3(ctx, with_cpu=not a.no_cpu, flow_steps=a.octree_flow_steps)
     if not a.no_batch:
         out["batched_solves"] = batched(ctx, with_cpu=not a.no_cpu)
         log(f"batched: {out['batched_solves']['uniform']['speedup']}x uniform, "

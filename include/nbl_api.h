@@ -404,6 +404,13 @@ int nbl_octree_sensors(nbl_octree* oc, const double* u, const double* v, const d
  * built on the first call. */
 int nbl_octree_mg_solve(nbl_octree* oc, const double* b, double* x, double tol, int max_iters,
                         nbl_poisson_report* rep);
+/* nbl_octree_mg_solve started from the guess x holds on entry instead of zero
+ * (a previous step's pressure): r0 = b - A x0, the same stopping test
+ * |r| <= tol |b|, no iteration when x0 already meets it; x receives the
+ * solution, mean-free on the fluid leaves.  No reference counterpart: the
+ * reference's solves start from zero (octree_poisson.cpp:110). */
+int nbl_octree_mg_solve_from(nbl_octree* oc, const double* b, double* x, double tol, int max_iters,
+                             nbl_poisson_report* rep);
 /* Hierarchy of nbl_octree_mg_solve (built if needed): level count, leaves per
  * level (up to cap entries) and the device time of its construction. */
 int nbl_octree_mg_levels(nbl_octree* oc, long long* counts, int cap, int* levels, double* build_ms);

@@ -704,6 +704,16 @@ int nbl_octree_mg_solve(nbl_octree* oc, const double* b, double* x, double tol, 
   });
 }
 
+int nbl_octree_mg_solve_from(nbl_octree* oc, const double* b, double* x, double tol, int max_iters,
+                             nbl_poisson_report* rep) {
+  return guarded([&] {
+    need(oc && b && x, "NULL argument");
+    useDevice(oc->ctx);
+    const nbl_poisson_report r = oc->impl->solveMg(b, x, tol, max_iters, true);
+    if (rep) *rep = r;
+  });
+}
+
 int nbl_octree_mg_levels(nbl_octree* oc, long long* counts, int cap, int* levels, double* build_ms) {
   return guarded([&] {
     need(oc && levels, "NULL argument");

@@ -135,6 +135,7 @@ enum FinOp : int {
   FIN_RESID,      // |rhs - A x|^2 -> true relative residual
   FIN_BNORM_MEAN, // (|b|^2, sum) -> bnorm + counters reset, slot[slot] = sum / nf
   FIN_CONV_MEAN,  // (|r|^2, sum) -> convergence test, slot[slot] = sum / nf
+  FIN_BNORM_R0,   // (|b|^2, |r0|^2) -> bnorm, reset counters, optional early exit on |r0|
 };
 struct FinArgs {
   int op;

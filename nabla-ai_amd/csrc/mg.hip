@@ -1416,6 +1416,16 @@ __global__ void __launch_bounds__(kFinThreads) k_fin(RedBuf rb, FinArgs a, Solve
       stA(&ctl->slot[a.slot], mean1);
       reset();
       break;
+    case FIN_BNORM_R0:  // warm start: the stopping test's |b| and the first residual
+      bnorm = sqrt(t[0]) + 1e-300;
+      reset();
+      if (a.earlyExit && sqrt(t[1]) <= tol * bnorm) {
+        done = 1;
+        conv = 1;
+        early = 1;
+        resid = sqrt(t[1]) / bnorm;
+      }
+      break;
     case FIN_CONV_MEAN:
       stA(&ctl->slot[a.slot], mean1);
       rr = t[0];

@@ -271,17 +271,21 @@ __global__ void __launch_bounds__(kBlock) k_of_divergence(TreeV T, const double*
 // of one level (two leaves upstream, two downstream, all fluid), else the
 // upwind leaf's value.  Non-fluid neighbours and walls: no-slip (phi = 0 at
 // the face, half a leaf away); free-slip walls: the normal component only;
-// inlet: the inflow vector; outlet: zero gradient.
+// inlet: the inflow vector; outlet: zero gradient.  The RK stage is applied
+// in the same pass: out = a u0 + b (u + dt R) on fluid leaves, 0 elsewhere
+// (u itself is read at neighbours, so the result goes to a second buffer).
 __global__ void __launch_bounds__(kBlock) k_of_rhs(TreeV T, Bc B, const double* __restrict__ u,
                                                    const double* __restrict__ v, const double* __restrict__ w,
-                                                   const double* __restrict__ U, double* __restrict__ ru,
-                                                   double* __restrict__ rv, double* __restrict__ rw) {
+                                                   const double* __restrict__ U, const double* __restrict__ u0,
+                                                   const double* __restrict__ v0, const double* __restrict__ w0,
+                                                   double a, double b, double dt, double* __restrict__ ou,
+                                                   double* __restrict__ ov, double* __restrict__ ow) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= T.n) return;
   if (T.mask[i] != 0) {
-    ru[i] = 0.0;
-    rv[i] = 0.0;
-    rw[i] = 0.0;
+    ou[i] = 0.0;
+    ov[i] = 0.0;
+    ow[i] = 0.0;
     return;
   }
   const double* __restrict__ ph[3] = {u, v, w};
@@ -350,17 +354,19 @@ __global__ void __launch_bounds__(kBlock) k_of_rhs(TreeV T, Bc B, const double* 
     }
   });
   const double V = hi[0] * hi[1] * hi[2];
-  ru[i] = (conv[0] + diff[0]) / V;
-  rv[i] = (conv[1] + diff[1]) / V;
-  rw[i] = (conv[2] + diff[2]) / V;
+  ou[i] = a * u0[i] + b * (pP[0] + dt * ((conv[0] + diff[0]) / V));
+  ov[i] = a * v0[i] + b * (pP[1] + dt * ((conv[1] + diff[1]) / V));
+  ow[i] = a * w0[i] + b * (pP[2] + dt * ((conv[2] + diff[2]) / V));
 }
 
-// RK stage: u = a u0 + b (u + dt R) on fluid leaves, 0 elsewhere.
-__global__ void k_of_stage(long long n, const uint8_t* __restrict__ mask, const double* __restrict__ u0,
-                           double* __restrict__ u, const double* __restrict__ r, double a, double b, double dt) {
+// u = v = w = 0 on non-fluid leaves.
+__global__ void k_of_solid_zero(long long n, const uint8_t* __restrict__ mask, double* __restrict__ u,
+                                double* __restrict__ v, double* __restrict__ w) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  u[i] = mask[i] == 0 ? a * u0[i] + b * (u[i] + dt * r[i]) : 0.0;
+  if (i >= n || mask[i] == 0) return;
+  u[i] = 0.0;
+  v[i] = 0.0;
+  w[i] = 0.0;
 }
 
 // partials of max over fluid leaves of |u|/hx + |v|/hy + |w|/hz + 2 nu sum 1/h^2
@@ -455,9 +461,8 @@ void OctFlowDev::setState(const double* u, const double* v, const double* w, con
   else NBL_HIP(hipMemsetAsync(p_.as<void>(), 0, n_ * 8, s_));
   // non-fluid leaves carry no velocity
   const long long N = (long long)n_;
-  for (int c = 0; c < 3; ++c)
-    k_of_stage<<<gridFor(N), kBlock, 0, s_>>>(N, tree_.level0().mask.as<uint8_t>(), u_[c].as<double>(),
-                                              u_[c].as<double>(), u_[c].as<double>(), 1.0, 0.0, 0.0);
+  k_of_solid_zero<<<gridFor(N), kBlock, 0, s_>>>(N, tree_.level0().mask.as<uint8_t>(), u_[0].as<double>(),
+                                                 u_[1].as<double>(), u_[2].as<double>());
   NBL_LAUNCH_CHECK();
   faceInit();
   NBL_HIP(hipStreamSynchronize(s_));
@@ -503,7 +508,7 @@ void OctFlowDev::project(double dt, int* it, double* resid, double* ms) {
                                             dt, b_.as<double>());
   NBL_LAUNCH_CHECK();
   const nbl_poisson_report r = tree_.solveMgSorted(b_.as<double>(), p_.as<double>(), cfg_.poisson_tol,
-                                                   cfg_.poisson_max_iters, s_);
+                                                   cfg_.poisson_max_iters, s_, p_.as<double>());
   if (!r.converged) throw std::runtime_error("octree flow: pressure solve did not converge");
   *it += r.iterations;
   *resid = std::max(*resid, r.residual);
@@ -528,13 +533,16 @@ nbl_octree_step_report OctFlowDev::step(double dtIn) {
   double resid = 0.0, ms = 0.0;
   for (int st = 0; st < 3; ++st) {
     k_of_rhs<<<gridFor(N), kBlock, 0, s_>>>(T, B, u_[0].as<double>(), u_[1].as<double>(), u_[2].as<double>(),
-                                            U_.as<double>(), rhs_[0].as<double>(), rhs_[1].as<double>(),
-                                            rhs_[2].as<double>());
-    for (int c = 0; c < 3; ++c)
-      k_of_stage<<<gridFor(N), kBlock, 0, s_>>>(N, T.mask, u0_[c].as<double>(), u_[c].as<double>(),
-                                                rhs_[c].as<double>(), ca[st], cb[st], dt);
+                                            U_.as<double>(), u0_[0].as<double>(), u0_[1].as<double>(),
+                                            u0_[2].as<double>(), ca[st], cb[st], dt, rhs_[0].as<double>(),
+                                            rhs_[1].as<double>(), rhs_[2].as<double>());
     NBL_LAUNCH_CHECK();
-    project(dt, &it, &resid, &ms);
+    for (int c = 0; c < 3; ++c) std::swap(u_[c], rhs_[c]);
+    // The stage's velocity carries its Euler step with weight cb, so it is
+    // projected with the step cb dt: p is then the pressure at every stage
+    // (the same field up to the time the stage lands on), and each solve
+    // starts from the last stage's p.
+    project(cb[st] * dt, &it, &resid, &ms);
   }
   NBL_HIP(hipMemcpyAsync(hscal_, scal_.as<void>(), 16 * 8, hipMemcpyDeviceToHost, s_));
   NBL_HIP(hipStreamSynchronize(s_));

@@ -89,25 +89,56 @@ __global__ void __launch_bounds__(kBlock) k_om_sweep(OctViewT<T> L, const T* __r
   if constexpr (FOLD) blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
 }
 
-// Residuals b - A x of a coarse cell's children summed in child order
-// (thread per coarse cell); with xc, the coarse level's first sweep from a
-// zero guess, omega*bc/diag_c (fluid cells).
-template <class T, class XR>
+// Residuals b - A x of a coarse cell's children, summed; with xc, the coarse
+// level's first sweep from a zero guess, omega*bc/diag_c (fluid cells).
+// G lanes of one wave per coarse cell: lane k takes children k, k+G, ... (one
+// residual row each, in parallel instead of one after another), then the G
+// lane sums are added by a butterfly in a fixed order, so the result does not
+// depend on timing.  G is a power of two dividing the wave; every lane of a
+// group shares I, so a group returns or shuffles as a whole.
+template <class T, class XR, int G>
 __global__ void __launch_bounds__(kBlock) k_om_restrict(OctViewT<T> F, OctViewT<T> C,
                                                         const int32_t* __restrict__ cstart,
                                                         const int32_t* __restrict__ child, const T* __restrict__ b,
                                                         XR X, T* __restrict__ bc, T* __restrict__ xc,
                                                         const SolveCtl* ctl) {
+  static_assert(G >= 1 && G <= 64 && (G & (G - 1)) == 0, "lane group must be a power of two within a wave");
   if (ctl && ldA(&ctl->done)) return;
-  const long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long I = t / G;
+  const int k = int(t % G);
   if (I >= C.n) return;
   T s = T(0);
-  for (int32_t p = cstart[I]; p < cstart[I + 1]; ++p) {
+  const int32_t e = cstart[I + 1];
+  for (int32_t p = cstart[I] + k; p < e; p += G) {
     const int32_t i = child[p];
     if (F.mask[i] == 0) s += b[i] - octRowA(F, i, X);
   }
+#pragma unroll
+  for (int o = 1; o < G; o <<= 1) s += __shfl_xor(s, o, G);
+  if (k != 0) return;
   bc[I] = s;
   if (xc) xc[I] = C.mask[I] == 0 ? T(kOmega) * s / C.diag[I] : T(0);
+}
+
+// Lanes per coarse cell of a restriction: the power of two nearest the mean
+// child count (1 .. 8).
+int restrictLanes(long long nFine, long long nCoarse) {
+  const double m = nCoarse > 0 ? double(nFine) / double(nCoarse) : 1.0;
+  return m < 1.5 ? 1 : m < 3.0 ? 2 : m < 6.0 ? 4 : 8;
+}
+
+template <class T, class XR>
+void launchRestrict(const OctViewT<T>& F, const OctViewT<T>& C, const int32_t* cstart, const int32_t* child,
+                    const T* b, XR X, T* bc, T* xc, const SolveCtl* ctl, hipStream_t s) {
+  const int G = restrictLanes(F.n, C.n);
+  const int g = gridFor(C.n * G);
+  switch (G) {
+    case 1: k_om_restrict<T, XR, 1><<<g, kBlock, 0, s>>>(F, C, cstart, child, b, X, bc, xc, ctl); break;
+    case 2: k_om_restrict<T, XR, 2><<<g, kBlock, 0, s>>>(F, C, cstart, child, b, X, bc, xc, ctl); break;
+    case 4: k_om_restrict<T, XR, 4><<<g, kBlock, 0, s>>>(F, C, cstart, child, b, X, bc, xc, ctl); break;
+    default: k_om_restrict<T, XR, 8><<<g, kBlock, 0, s>>>(F, C, cstart, child, b, X, bc, xc, ctl); break;
+  }
 }
 
 __global__ void k_to_f32(long long n, const double* __restrict__ a, float* __restrict__ o) {
@@ -639,13 +670,11 @@ void OctMg::cycle(size_t l, const T* b, RedBuf rb, const SolveCtl* ctl, hipStrea
       k_om_sweep<T, PlainRd<T>, T, false><<<g, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xb, rb, ctl);
     NBL_LAUNCH_CHECK();
   };
-  const int gc = gridFor(VC.n);
   if constexpr (kF) {
     if (l == 0 && prec_ == NBL_PREC_MIXED_BF16) {
       __hip_bfloat16* xbB = L.xbB.as<__hip_bfloat16>();
       k_om_sweep<T, PlainRd<T>, __hip_bfloat16, false><<<g, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xbB, rb, ctl);
-      k_om_restrict<T, Bf16Rd><<<gc, kBlock, 0, s>>>(V, VC, C.cstart.as<int32_t>(), C.child.as<int32_t>(), b,
-                                                    Bf16Rd{xbB}, bc, xc, ctl);
+      launchRestrict<T>(V, VC, C.cstart.as<int32_t>(), C.child.as<int32_t>(), b, Bf16Rd{xbB}, bc, xc, ctl, s);
       NBL_LAUNCH_CHECK();
       cycle<T>(l + 1, bc, rb, ctl, s, &ec);
       post(Bf16Rd{xbB});
@@ -655,8 +684,7 @@ void OctMg::cycle(size_t l, const T* b, RedBuf rb, const SolveCtl* ctl, hipStrea
   }
   // pre-smoothing: first sweep (in xa) then one plain sweep
   k_om_sweep<T, PlainRd<T>, T, false><<<g, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xb, rb, ctl);
-  k_om_restrict<T, PlainRd<T>><<<gc, kBlock, 0, s>>>(V, VC, C.cstart.as<int32_t>(), C.child.as<int32_t>(), b,
-                                                     PlainRd<T>{xb}, bc, xc, ctl);
+  launchRestrict<T>(V, VC, C.cstart.as<int32_t>(), C.child.as<int32_t>(), b, PlainRd<T>{xb}, bc, xc, ctl, s);
   NBL_LAUNCH_CHECK();
   cycle<T>(l + 1, bc, rb, ctl, s, &ec);
   // post-smoothing: prolongation fused into the first sweep

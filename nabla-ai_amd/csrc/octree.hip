@@ -451,6 +451,44 @@ __global__ void __launch_bounds__(kBlock) k_om_init2(long long n, long long nf, 
   blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
 }
 
+// Warm start, first pass: rhs = b and x = x0 on fluid leaves (0 elsewhere);
+// partials of sum(rhs).
+__global__ void __launch_bounds__(kBlock) k_o_init_warm(long long n, const uint8_t* __restrict__ mask,
+                                                        const double* __restrict__ b, const double* __restrict__ x0,
+                                                        double* __restrict__ rhs, double* __restrict__ x, RedBuf rb) {
+  double acc[1] = {0.0};
+  OGRID(c, n) {
+    const bool fl = mask[c] == 0;
+    const double v = fl ? b[c] : 0.0;
+    rhs[c] = v;
+    x[c] = fl ? x0[c] : 0.0;
+    acc[0] += v;
+  }
+  blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
+}
+
+// Warm start, second pass: rhs <- rhs - mean (fluid), 0 elsewhere;
+// r = rhs - A x; the V-cycle's input from r; partials of |rhs|^2 and |r|^2.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_om_init2_warm(OctView L, long long nf, const T* __restrict__ diagT,
+                                                          const double* __restrict__ x, double* __restrict__ rhs,
+                                                          double* __restrict__ r, T* __restrict__ rT,
+                                                          T* __restrict__ x0, const SolveCtl* ctl, RedBuf rb) {
+  const double m = ldA(&ctl->slot[O_MEAN_RHS]);
+  double acc[2] = {0.0, 0.0};
+  OGRID(c, L.n) {
+    const bool fl = L.mask[c] == 0;
+    const double v = fl ? (nf ? rhs[c] - m : rhs[c]) : 0.0;
+    rhs[c] = v;
+    const double rv = fl ? v - octRow(L, c, x, 0.0, false) : 0.0;
+    r[c] = rv;
+    cycleInput(c, fl, rv, L.diag, diagT, rT, x0);
+    acc[0] += v * v;
+    acc[1] += rv * rv;
+  }
+  blockToPartials<2>(acc, rb.partials, rb.stride, blockIdx.x);
+}
+
 // z = V-cycle output - mean (fluid), 0 elsewhere; p = z; partials of r.z.
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_om_start(long long n, long long nf, const uint8_t* __restrict__ mask,
@@ -1243,7 +1281,8 @@ int OctreeDev::mgLevels(long long* counts, int cap, double* buildMs) {
   return m.levels();
 }
 
-nbl_poisson_report OctreeDev::solveMgSorted(const double* b, double* xo, double tol, int maxIters, hipStream_t s) {
+nbl_poisson_report OctreeDev::solveMgSorted(const double* b, double* xo, double tol, int maxIters, hipStream_t s,
+                                            const double* xs) {
   OctMg& M = mg();
   SolveCtl* ctl = ctl_.as<SolveCtl>();
   const RedBuf rb{red_.as<double>(), kOctBlocks};
@@ -1269,19 +1308,30 @@ nbl_poisson_report OctreeDev::solveMgSorted(const double* b, double* xo, double 
     a.predicated = pred ? 1 : 0;
     launch_fin(rb, a, ctl, s);
   };
-  k_o_init<<<g, kBlock, 0, s>>>(N, mk, b, rhs, x, rb);
+  if (xs) k_o_init_warm<<<g, kBlock, 0, s>>>(N, mk, b, xs, rhs, x, rb);
+  else k_o_init<<<g, kBlock, 0, s>>>(N, mk, b, rhs, x, rb);
   fin(FIN_MEAN, 1, O_MEAN_RHS, false);
   // the V-cycle's level type: its input is written and its result read as double or float
   const bool f32 = M.fp32();
   float* rF = M.rhs0F();
   const float* dgF = static_cast<const float*>(M.diag0());
-  if (f32)
-    k_om_init2<float><<<g, kBlock, 0, s>>>(N, nf, mk, dg, dgF, rhs, r, rF, static_cast<float*>(M.firstSweep()), ctl,
-                                           rb);
-  else
-    k_om_init2<double><<<g, kBlock, 0, s>>>(N, nf, mk, dg, dg, rhs, r, nullptr,
-                                            static_cast<double*>(M.firstSweep()), ctl, rb);
-  fin(FIN_BNORM, 1, 0, false, 1);
+  if (xs) {
+    if (f32)
+      k_om_init2_warm<float><<<g, kBlock, 0, s>>>(V, nf, dgF, x, rhs, r, rF, static_cast<float*>(M.firstSweep()),
+                                                  ctl, rb);
+    else
+      k_om_init2_warm<double><<<g, kBlock, 0, s>>>(V, nf, dg, x, rhs, r, nullptr,
+                                                   static_cast<double*>(M.firstSweep()), ctl, rb);
+    fin(FIN_BNORM_R0, 2, 0, false, 1);
+  } else {
+    if (f32)
+      k_om_init2<float><<<g, kBlock, 0, s>>>(N, nf, mk, dg, dgF, rhs, r, rF, static_cast<float*>(M.firstSweep()),
+                                             ctl, rb);
+    else
+      k_om_init2<double><<<g, kBlock, 0, s>>>(N, nf, mk, dg, dg, rhs, r, nullptr,
+                                              static_cast<double*>(M.firstSweep()), ctl, rb);
+    fin(FIN_BNORM, 1, 0, false, 1);
+  }
   M.vcycle(r, rb, ctl, s);
   fin(FIN_MEAN, 1, O_MEAN_Z, true);
   if (f32)
@@ -1336,11 +1386,15 @@ nbl_poisson_report OctreeDev::solveMgSorted(const double* b, double* xo, double 
   return nbl_poisson_report{h.it, h.resid, h.converged};
 }
 
-nbl_poisson_report OctreeDev::solveMg(const double* b, double* x, double tol, int maxIters) {
+nbl_poisson_report OctreeDev::solveMg(const double* b, double* x, double tol, int maxIters, bool warm) {
   if (maxIters < 0) throw std::invalid_argument("max iterations must be >= 0");
   mg();  // hierarchy set-up stays outside the timed solve
   toSorted(b, t0_.as<double>());
-  const nbl_poisson_report rep = solveMgSorted(t0_.as<double>(), t0_.as<double>(), tol, maxIters, stream_);
+  // the initial guess waits in Ap_, which the iteration first writes after
+  // the initial residual is formed
+  if (warm) toSorted(x, Ap_.as<double>());
+  const nbl_poisson_report rep = solveMgSorted(t0_.as<double>(), t0_.as<double>(), tol, maxIters, stream_,
+                                               warm ? Ap_.as<double>() : nullptr);
   fromSorted(t0_.as<double>(), x);
   return rep;
 }

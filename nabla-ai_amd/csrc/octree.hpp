@@ -148,12 +148,14 @@ class OctreeDev {
   void diagonal(double* out);
   nbl_poisson_report solve(const double* b, double* x, double tol, int maxIters);
   // Multigrid-preconditioned flexible CG on the same system (hierarchy built on
-  // the first call); caller-order host arrays.
-  nbl_poisson_report solveMg(const double* b, double* x, double tol, int maxIters);
+  // the first call); caller-order host arrays.  warm: x holds the initial guess.
+  nbl_poisson_report solveMg(const double* b, double* x, double tol, int maxIters, bool warm = false);
   // sorted-order device arrays
   void applySorted(const double* x, double* out, hipStream_t s);
   nbl_poisson_report solveSorted(const double* b, double* x, double tol, int maxIters, hipStream_t s);
-  nbl_poisson_report solveMgSorted(const double* b, double* x, double tol, int maxIters, hipStream_t s);
+  // x0: optional initial guess (sorted; may be x itself); null starts from zero.
+  nbl_poisson_report solveMgSorted(const double* b, double* x, double tol, int maxIters, hipStream_t s,
+                                   const double* x0 = nullptr);
   hipStream_t stream() const { return stream_; }
   // Device time of construction (sort, validation, face table, CSR) and of the
   // last solve (ms), and a timing loop over one kernel (0: the SpMV).

@@ -264,6 +264,7 @@ def load(path: os.PathLike | None = None):
     _proto(lib, "nbl_octree_transport", i, V, P_D, P_D, P_D, P_D, d, i, C.POINTER(C.c_double))
     _proto(lib, "nbl_octree_sensors", i, V, P_D, P_D, P_D, P_D, d, P_D, P_D, C.POINTER(SensorFields))
     _proto(lib, "nbl_octree_mg_solve", i, V, P_D, P_D, d, i, C.POINTER(PoissonReport))
+    _proto(lib, "nbl_octree_mg_solve_from", i, V, P_D, P_D, d, i, C.POINTER(PoissonReport))
     _proto(lib, "nbl_octree_mg_levels", i, V, C.POINTER(C.c_longlong), i, C.POINTER(i),
            C.POINTER(C.c_double))
     _proto(lib, "nbl_octree_set_precision", i, V, i)
@@ -734,13 +735,15 @@ class Octree:
                                           C.byref(out)))
         return res, sp, rp
 
-    def mg_solve(self, b, tol=1e-9, max_iters=2000):
-        """Multigrid-preconditioned flexible CG on the same system as solve()."""
+    def mg_solve(self, b, tol=1e-9, max_iters=2000, x0=None):
+        """Multigrid-preconditioned flexible CG on the same system as solve()
+        (from zero, or from the initial guess x0: nbl_octree_mg_solve_from)."""
         b = np.ascontiguousarray(b, dtype=np.float64)
-        x = np.empty(self.n)
+        x = np.empty(self.n) if x0 is None else np.array(x0, dtype=np.float64)
+        assert x.size == self.n
         rep = PoissonReport()
-        check(self.lib.nbl_octree_mg_solve(self.h, dptr(b), dptr(x), tol, max_iters,
-                                           C.byref(rep)))
+        fn = self.lib.nbl_octree_mg_solve if x0 is None else self.lib.nbl_octree_mg_solve_from
+        check(fn(self.h, dptr(b), dptr(x), tol, max_iters, C.byref(rep)))
         return x, rep
 
     def set_precision(self, precision):

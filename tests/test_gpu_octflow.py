@@ -10,8 +10,7 @@ the discretization must do, on trees with 2:1 level jumps:
   kinetic energy decays (upwind convection, viscosity, no forcing);
 * plane Poiseuille flow: with the parabolic inflow the analytic profile is a
   steady state; after 100 steps the profile stays within 2% (rel. L2) of it
-  and the core pressure gradient within 2% of the analytic -12 nu U / H^2
-  (scaled by the last RK stage's 2/3);
+  and the core pressure gradient within 2% of the analytic -12 nu U / H^2;
 * solid leaves carry no velocity; repeated runs are bit-identical.
 """
 import numpy as np
@@ -118,12 +117,11 @@ def test_poiseuille_profile(ctx, orc):
         assert err < 0.02
         assert np.abs(v).max() < 0.02 * U and np.abs(w).max() < 1e-12
         # pressure gradient of the core, by least squares in x: every stage is
-        # projected with the full dt (u -= dt grad p) and the last stage
-        # advances 2/3 of the step, so the step's p is 2/3 of the pressure
-        # gradient -12 nu U / H^2 that balances the parabola's viscous force
+        # projected with its own step, so p is the pressure whose gradient
+        # -12 nu U / H^2 balances the parabola's viscous force
         mid = (c[:, 0] > 0.25) & (c[:, 0] < 0.75) & (c[:, 1] > 0.2) & (c[:, 1] < 0.8)
         slope = np.polyfit(c[mid, 0], p[mid], 1)[0]
-        expect = 2.0 / 3.0 * (-12.0 * nu * U / H ** 2)
+        expect = -12.0 * nu * U / H ** 2
         print(f"dp/dx {slope:.5f} against {expect:.5f}", flush=True)
         assert abs(slope - expect) <= 0.02 * abs(expect)
     finally:
@@ -171,3 +169,65 @@ def test_flow_options_validated(ctx, orc):
                 nbl.OctreeFlow(d, nbl.oct_flow_cfg(**kw))
     finally:
         d.close()
+
+
+def duct_runs(ctx, orc, n, steps, dt, nu=0.05):
+    """Developing flow in a 2 x 1 x 1 duct (uniform inflow, outlet, no-slip
+    side walls) on n^3 cells of (2/n, 1/n, 1/n): the octree step on a
+    uniformly refined tree and the MAC step on the same cells, both from
+    u = 1, WENO-5, the same fixed dt.  Returns both cell-centre velocities
+    in the grid's cell order and the cells' x centres."""
+    L = int(round(np.log2(n)))
+    t = orc.octree((0.0, 0.0, 0.0), (2.0, 1.0, 1.0), L)
+    t.refine_uniform(L)
+    walls = (nbl.OCT_WALL,) * 4
+    d, f = make_flow(ctx, t, nu=nu, inlet_speed=1.0, face=(nbl.OCT_INLET, nbl.OCT_OUTLET) + walls, convection=1,
+                     poisson_tol=1e-10)
+    try:
+        f.set_state(np.ones(t.n), np.zeros(t.n), np.zeros(t.n))
+        for _ in range(steps):
+            f.step(dt)
+        uo = f.get_state()[:3]
+    finally:
+        f.close()
+        d.close()
+    c = t.centers()
+    ix = np.floor(c[:, 0] * n / 2.0).astype(np.int64)
+    iy = np.floor(c[:, 1] * n).astype(np.int64)
+    iz = np.floor(c[:, 2] * n).astype(np.int64)
+    cell = ix + n * (iy + n * iz)
+    assert np.array_equal(np.sort(cell), np.arange(n ** 3))
+    faces = [dict(type=nbl.BC_INLET, profile=nbl.INLET_UNIFORM, inlet_speed=1.0, inlet_axis=0),
+             dict(type=nbl.BC_OUTLET)] + [dict(type=nbl.BC_WALL) for _ in range(4)]
+    g = nbl.Grid(n, n, n, 2.0 / n, 1.0 / n, 1.0 / n, faces=faces)
+    cfg = nbl.flow_cfg(reynolds=1.0 / nu, ref_length=1.0, velocity_scale=1.0, convection=nbl.CONV_WENO5,
+                       poisson_tol=1e-10)
+    run = nbl.Run(ctx, g, cfg)
+    try:
+        run.initialize()
+        for _ in range(steps):
+            run.step(dt)
+        um = [run.cell_velocity(q) for q in range(3)]
+    finally:
+        run.close()
+    o = [np.empty(n ** 3) for _ in range(3)]
+    for q in range(3):
+        o[q][cell] = uo[q]
+    xc = (np.arange(n ** 3) % n + 0.5) * 2.0 / n
+    return o, um, xc
+
+
+def test_agrees_with_mac_step_on_a_uniform_tree(ctx, orc):
+    """On a uniformly refined tree the collocated octree step and the MAC step
+    discretize the same developing duct flow: their cell velocities differ by
+    the discretizations' truncation error, which shrinks under refinement."""
+    errs = []
+    for n in (16, 32):
+        o, m, xc = duct_runs(ctx, orc, n, steps=100, dt=0.0025)
+        core = (xc > 0.25) & (xc < 1.25)  # away from the outlet's different closure
+        num = np.sqrt(sum(np.sum((a[core] - b[core]) ** 2) for a, b in zip(o, m)))
+        den = np.sqrt(sum(np.sum(b[core] ** 2) for b in m))
+        errs.append(num / den)
+        print(f"duct {n}^3: octree vs MAC rel L2 {errs[-1]:.3e}", flush=True)
+    assert errs[0] < 0.1
+    assert errs[1] < 0.7 * errs[0]

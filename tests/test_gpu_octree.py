@@ -253,6 +253,32 @@ def test_octree_mg_zero_rhs_returns_at_once(ctx, mg_tree_list):
         d.close()
 
 
+def test_octree_mg_warm_start(ctx, mg_tree_list):
+    """nbl_octree_mg_solve_from: a zero guess reproduces the cold solve bit
+    for bit; the converged solution (shifted by a constant, the null space)
+    stops before the first iteration; a perturbed solution converges to the
+    same answer in fewer iterations than from zero."""
+    _name, t = mg_tree_list[1]
+    d = device_tree(ctx, t)
+    try:
+        _k, _l, m = t.dump()
+        fl = m == 0
+        b = _projected_rhs(9, m)
+        xc, rc = d.mg_solve(b, 1e-10, 300)
+        xz, rz = d.mg_solve(b, 1e-10, 300, x0=np.zeros(t.n))
+        assert rz.iterations == rc.iterations and np.array_equal(xz, xc)
+        xs, rs = d.mg_solve(b, 1e-12, 300)
+        xe, re_ = d.mg_solve(b, 1e-10, 300, x0=xs + 5.0)
+        assert re_.converged and re_.iterations == 0 and re_.residual <= 1e-10
+        assert rel(xe, xs) < 1e-13 and abs(xe[fl].mean()) < 1e-12
+        noisy = xs + 1e-4 * cases.normal(10, t.n) * np.abs(xs).max()
+        xw, rw = d.mg_solve(b, 1e-10, 300, x0=noisy)
+        assert rw.converged and 0 < rw.iterations < rc.iterations
+        assert rel(xw, xs) < 1e-8
+    finally:
+        d.close()
+
+
 def test_c3_mg_solve(ctx, c3):
     """The 4.3M-leaf cube-wake tree (3:1:1 leaves): the hierarchy merges along
     y and z first (4294942 -> 1139794 -> 179260 -> 43346 -> 8192 -> 1024
