@@ -472,13 +472,9 @@ def served(ctx, window_s, rate, nruns=32, slots=4, seed=777, sustain_window_s=10
     runs = cases.served_runs(nruns, seed)
     srv = nbl.Server(ctx, slots=slots, hbm_budget=200 << 30)
     t0 = time.perf_counter()
-    ids, cells = [], []
     opts = nbl.solver_opts(precision=precision) if precision is not None else None
-    for name, g, cfg, st in runs:
-        rid = srv.admit(g, cfg, opts)
-        srv.run(rid, g).set_state(*st)
-        ids.append(rid)
-        cells.append(g.cells)
+    ids = [cases.admit_served(srv, r, opts) for r in runs]
+    cells = [r["cells"] for r in runs]
     admit_s = time.perf_counter() - t0
     used, _ = srv.footprint()
 
@@ -539,7 +535,9 @@ def served(ctx, window_s, rate, nruns=32, slots=4, seed=777, sustain_window_s=10
     out = stream(rate, window_s, seed + 1)
     out.update({"runs": nruns, "slots": slots, "cells_min": int(min(cells)), "cells_max": int(max(cells)),
                 "cells_mean": int(np.mean(cells)), "admit_s": round(admit_s, 3), "hbm_bytes": int(used),
-                "families": "Taylor-Green n^3 and cube channel 12r x 4r x 4r (uniform MAC); Re U[500,3000]"})
+                "octree_runs": sum(r["kind"] == "octree" for r in runs),
+                "families": "Taylor-Green n^3 and cube channel 12r x 4r x 4r (uniform MAC), cube-wake octrees "
+                            "(finest level 7 or 8, collocated octree step); Re U[500,3000]"})
     if sustain_window_s > 0:
         out["sustained"] = stream(0.7 * out["completed_per_s"], sustain_window_s, seed + 2)
         out["sustained"]["note"] = "70% of the request rate the server completed under the overload above"

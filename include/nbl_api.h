@@ -535,6 +535,21 @@ int nbl_server_destroy(nbl_server* srv);   /* finishes queued requests first */
 int nbl_server_admit(nbl_server* srv, const nbl_grid_desc* grid, const nbl_flow_cfg* cfg,
                      const nbl_solver_opts* opts, int* run_id);
 int nbl_server_run(nbl_server* srv, int run_id, nbl_run** run);
+/* An octree flow run (nbl_octree_flow_*): the leaves as nbl_octree_create
+ * takes them and the flow configuration; admission also builds the octree's
+ * multigrid hierarchy, so the measured footprint is the run's whole device
+ * memory.  Its steps are served like the uniform runs' (dt from the CFL
+ * rule); nbl_server_wait reports them in nbl_step_report's fields (cd, cl and
+ * the momentum/continuity residuals 0, mass_error = inflow - outflow,
+ * cell_count = leaves).  nbl_server_octree_flow returns the run's flow state
+ * for set_state / get_state while it has no request in flight; it belongs to
+ * the server (never nbl_octree_flow_destroy it).  nbl_server_run refuses
+ * octree runs and nbl_server_octree_flow uniform ones. */
+int nbl_server_admit_octree(nbl_server* srv, const uint64_t* keys, const uint8_t* levels,
+                            const uint8_t* masks, size_t n, const double origin[3],
+                            const double extent[3], int max_level, const nbl_octree_flow_cfg* cfg,
+                            int* run_id);
+int nbl_server_octree_flow(nbl_server* srv, int run_id, nbl_octree_flow** flow);
 int nbl_server_submit(nbl_server* srv, int run_id, uint64_t* ticket);
 int nbl_server_wait(nbl_server* srv, uint64_t ticket, nbl_step_report* rep, double* latency_ms);
 int nbl_server_footprint(nbl_server* srv, uint64_t* used, uint64_t* budget);

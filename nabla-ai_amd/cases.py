@@ -240,25 +240,93 @@ def batch_octrees(count, seed):
 
 
 def served_runs(nruns, seed):
-    """32 seeded runs of the two uniform families: Taylor-Green n^3 boxes and
-    cube channels at r cells per cube side (192 r^3 cells), sizes 0.3-4M
-    cells, Re uniform in [500, 3000]."""
-    u = uniform01(seed, 3 * nruns)
+    """Seeded runs of the three families, a third each: Taylor-Green n^3
+    boxes, cube channels at r cells per cube side (192 r^3 cells) and
+    cube-wake octrees (finest level 7 or 8, wake box 1.5-5 h long), sizes
+    0.3-4M cells or leaves, Re uniform in [500, 3000].  Each run is a dict:
+    name, kind ("uniform" or "octree"), cells, and either grid, cfg, state
+    (u, v, w, p) or leaves (keys, levels, masks), origin, extent, max_level,
+    cfg (an OctFlowCfg) and state (u, v, w)."""
+    u = uniform01(seed, 4 * nruns)
     out = []
     for q in range(nruns):
-        re = 500.0 + 2500.0 * float(u[3 * q + 1])
-        if u[3 * q] < 0.5:
-            n = 2 * int(round((67 + (158 - 67) * float(u[3 * q + 2])) / 2))
+        f, re, s1, s2 = (float(x) for x in u[4 * q:4 * q + 4])
+        re = 500.0 + 2500.0 * re
+        if f < 1.0 / 3.0:
+            n = 2 * int(round((67 + (158 - 67) * s1) / 2))
             g, cfg = taylor_green(n, reynolds=re)
-            st = taylor_green_state(g)
-            name = f"tgv{n}"
-        else:
-            r = int(round(12 + (27 - 12) * float(u[3 * q + 2])))
+            out.append(dict(name=f"tgv{n}", kind="uniform", cells=g.cells, grid=g, cfg=cfg,
+                            state=taylor_green_state(g)))
+        elif f < 2.0 / 3.0:
+            r = int(round(12 + (27 - 12) * s1))
             g, cfg = cube_channel(nx=12 * r, ny=4 * r, nz=4 * r, res=r, reynolds=re)
-            st = cube_channel_state(g, seed=seed + q)
-            name = f"cube{12 * r}x{4 * r}x{4 * r}"
-        out.append((name, g, cfg, st))
+            out.append(dict(name=f"cube{12 * r}x{4 * r}x{4 * r}", kind="uniform", cells=g.cells, grid=g, cfg=cfg,
+                            state=cube_channel_state(g, seed=seed + q)))
+        else:
+            level = 7 if s1 < 0.5 else 8
+            wake = 1.5 + 3.5 * s2
+            out.append(cube_wake_run(level, wake, re, seed + q))
     return out
+
+
+def cube_wake_run(max_level, wake, reynolds, seed, precision=nbl.PREC_MIXED):
+    """A cube-wake octree flow run: the cube-wake refinement rule with the wake
+    box `wake` h long, finest level max_level, the cube's leaves InsideSolid;
+    uniform inflow, outlet, no-slip floor and ceiling, free-slip sides, CFL
+    0.7, WENO-5, FP32 V-cycle; u = 1 + 0.01 xi on fluid leaves."""
+    t = nbl.build_tree(C3_ORIGIN, C3_EXTENT, max_level, 5, lambda c: c3_predicate(c, wake))
+    k, l, c = t.leaves()
+    t.close()
+    m = c3_solid(c)
+    fl = m == 0
+    n = k.size
+    uu = np.where(fl, 1.0 + 0.01 * (2.0 * uniform01(seed, n) - 1.0), 0.0)
+    cfg = nbl.oct_flow_cfg(nu=1.0 / reynolds, cfl=0.7, inlet_speed=1.0, inlet_profile=0,
+                           face=(nbl.OCT_INLET, nbl.OCT_OUTLET, nbl.OCT_WALL, nbl.OCT_WALL,
+                                 nbl.OCT_SLIP, nbl.OCT_SLIP),
+                           convection=1, poisson_tol=1e-8, poisson_max_iters=500, precision=precision)
+    return dict(name=f"wake{max_level}_{wake:.1f}", kind="octree", cells=int(n), leaves=(k, l, m),
+                origin=C3_ORIGIN, extent=C3_EXTENT, max_level=max_level, cfg=cfg,
+                state=(uu, np.zeros(n), np.zeros(n)))
+
+
+def admit_served(srv, r, opts=None):
+    """Admit a served_runs() entry to an nbl.Server and set its state."""
+    if r["kind"] == "uniform":
+        rid = srv.admit(r["grid"], r["cfg"], opts)
+        srv.run(rid, r["grid"]).set_state(*r["state"])
+    else:
+        rid = srv.admit_octree(*r["leaves"], r["origin"], r["extent"], r["max_level"], r["cfg"])
+        srv.octree_flow(rid).set_state(*r["state"])
+    return rid
+
+
+def served_state(srv, rid, r):
+    """The run's state arrays as the server holds them."""
+    if r["kind"] == "uniform":
+        return srv.run(rid, r["grid"]).get_state()[:4]
+    return srv.octree_flow(rid).get_state()
+
+
+def step_alone(ctx, r, steps, opts=None):
+    """The same run stepped by itself: (reports, state arrays)."""
+    if r["kind"] == "uniform":
+        run = nbl.Run(ctx, r["grid"], r["cfg"], opts)
+        try:
+            run.set_state(*r["state"])
+            reps = [run.step() for _ in range(steps)]
+            return reps, run.get_state()[:4]
+        finally:
+            run.close()
+    d = nbl.Octree(ctx, *r["leaves"], r["origin"], r["extent"], r["max_level"])
+    f = nbl.OctreeFlow(d, r["cfg"])
+    try:
+        f.set_state(*r["state"])
+        reps = [f.step() for _ in range(steps)]
+        return reps, f.get_state()
+    finally:
+        f.close()
+        d.close()
 
 
 # ---- octree workload (cube wake) ---------------------------------------------
@@ -268,19 +336,20 @@ C3_ORIGIN = (0.0, 0.0, 0.0)
 C3_EXTENT = (12.0, 4.0, 4.0)
 
 
-def c3_predicate(centers: np.ndarray) -> np.ndarray:
+def c3_predicate(centers: np.ndarray, wake: float = 6.0) -> np.ndarray:
     """Refinement rule of the cube-wake octree: centre within 1.5 h of the cube's
     bounding box, or inside the wake box [0,6h] behind the rear face, +-1 h about
-    the span centre, [0,2h] above the floor (h = 1)."""
+    the span centre, [0,2h] above the floor (h = 1; `wake` sets the box's
+    length behind the cube)."""
     x, y, z = centers[:, 0], centers[:, 1], centers[:, 2]
     x0, x1, y0, y1, z0, z1 = CUBE_BOX
     dx = np.maximum(np.maximum(x0 - x, 0.0), x - x1)
     dy = np.maximum(np.maximum(y0 - y, 0.0), y - y1)
     dz = np.maximum(np.maximum(z0 - z, 0.0), z - z1)
     near = dx * dx + dy * dy + dz * dz <= 1.5 * 1.5
-    a0, a1, b0, b1, c0, c1 = WAKE_BOX
-    wake = (x >= a0) & (x <= a1) & (y >= b0) & (y <= b1) & (z >= c0) & (z <= c1)
-    return near | wake
+    a0, _a1, b0, b1, c0, c1 = WAKE_BOX
+    in_wake = (x >= a0) & (x <= a0 + wake) & (y >= b0) & (y <= b1) & (z >= c0) & (z <= c1)
+    return near | in_wake
 
 
 def c3_solid(centers: np.ndarray) -> np.ndarray:

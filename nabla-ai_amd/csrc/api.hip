@@ -858,6 +858,24 @@ int nbl_server_run(nbl_server* srv, int run_id, nbl_run** run) {
   });
 }
 
+int nbl_server_admit_octree(nbl_server* srv, const uint64_t* keys, const uint8_t* levels, const uint8_t* masks,
+                            size_t n, const double origin[3], const double extent[3], int max_level,
+                            const nbl_octree_flow_cfg* cfg, int* run_id) {
+  return guarded([&] {
+    need(srv && keys && levels && masks && origin && extent && cfg && run_id, "NULL argument");
+    need(cfg->precision >= NBL_PREC_FP64 && cfg->precision <= NBL_PREC_MIXED_BF16, "unknown precision");
+    useDevice(srv->ctx);
+    *run_id = srv->impl->admitOctree(keys, levels, masks, n, origin, extent, max_level, *cfg);
+  });
+}
+
+int nbl_server_octree_flow(nbl_server* srv, int run_id, nbl_octree_flow** flow) {
+  return guarded([&] {
+    need(srv && flow, "NULL argument");
+    *flow = srv->impl->octreeFlow(run_id);
+  });
+}
+
 int nbl_server_submit(nbl_server* srv, int run_id, uint64_t* ticket) {
   return guarded([&] {
     need(srv && ticket, "NULL argument");

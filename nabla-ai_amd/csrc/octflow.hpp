@@ -19,6 +19,7 @@ class OctFlowDev {
   nbl_octree_step_report step(double dt);
   void divergence(double* out);
   double cflDt();
+  double cfl() const { return cfg_.cfl; }
 
  private:
   // Projection of the cell field in u_ with time step dt: face velocities,

@@ -41,30 +41,83 @@ Server::~Server() {
   for (auto& t : workers_) t.join();
 }
 
-int Server::admit(const HostGrid& g, const nbl_flow_cfg& cfg, const nbl_solver_opts& opts) {
-  std::lock_guard<std::mutex> lk(admitMu_);
-  const uint64_t before = ctx_->bytes.load();
-  auto run = std::make_unique<nbl_run>();
-  run->ctx = ctx_;
-  run->impl = std::make_unique<RunDev>(ctx_, g, cfg, opts);
-  const uint64_t foot = ctx_->bytes.load() - before;
-  std::lock_guard<std::mutex> lk2(mu_);
+int Server::insert(std::unique_ptr<Slot> s, uint64_t foot) {
+  std::lock_guard<std::mutex> lk(mu_);
   if (used_ + foot > budget_) throw CapacityError("server: the run's device footprint exceeds the HBM budget");
   used_ += foot;
-  auto s = std::make_unique<Slot>();
-  s->run = std::move(run);
   s->bytes = foot;
   runs_.push_back(std::move(s));
   return int(runs_.size()) - 1;
 }
 
+int Server::admit(const HostGrid& g, const nbl_flow_cfg& cfg, const nbl_solver_opts& opts) {
+  std::lock_guard<std::mutex> lk(admitMu_);
+  const uint64_t before = ctx_->bytes.load();
+  auto s = std::make_unique<Slot>();
+  s->run = std::make_unique<nbl_run>();
+  s->run->ctx = ctx_;
+  s->run->impl = std::make_unique<RunDev>(ctx_, g, cfg, opts);
+  return insert(std::move(s), ctx_->bytes.load() - before);
+}
+
+int Server::admitOctree(const uint64_t* keys, const uint8_t* levels, const uint8_t* masks, size_t n,
+                        const double origin[3], const double extent[3], int maxLevel,
+                        const nbl_octree_flow_cfg& cfg) {
+  std::lock_guard<std::mutex> lk(admitMu_);
+  const uint64_t before = ctx_->bytes.load();
+  auto s = std::make_unique<Slot>();
+  s->tree = std::make_unique<nbl_octree>();
+  s->tree->ctx = ctx_;
+  s->tree->impl = std::make_unique<OctreeDev>(ctx_, keys, levels, masks, n, origin, extent, maxLevel);
+  s->flow = std::make_unique<nbl_octree_flow>();
+  s->flow->ctx = ctx_;
+  s->flow->tree = s->tree.get();
+  s->flow->impl = std::make_unique<OctFlowDev>(ctx_, *s->tree->impl, cfg);
+  // the multigrid hierarchy (and its FP32 copies) would otherwise be built by
+  // the first served step, outside the measured footprint
+  s->tree->impl->mgLevels(nullptr, 0, nullptr);
+  return insert(std::move(s), ctx_->bytes.load() - before);
+}
+
+Server::Slot& Server::idle(int id) {
+  if (id < 0 || id >= int(runs_.size())) throw std::invalid_argument("server: unknown run id");
+  Slot& s = *runs_[size_t(id)];
+  if (s.busy || s.pending) throw std::runtime_error("server: run has requests in flight");
+  return s;
+}
+
 nbl_run* Server::run(int id) {
   std::lock_guard<std::mutex> lk(mu_);
-  if (id < 0 || id >= int(runs_.size())) throw std::invalid_argument("server: unknown run id");
-  if (runs_[size_t(id)]->busy || runs_[size_t(id)]->pending)
-    throw std::runtime_error("server: run has requests in flight");
-  return runs_[size_t(id)]->run.get();
+  Slot& s = idle(id);
+  if (!s.run) throw std::invalid_argument("server: run is an octree run");
+  return s.run.get();
 }
+
+nbl_octree_flow* Server::octreeFlow(int id) {
+  std::lock_guard<std::mutex> lk(mu_);
+  Slot& s = idle(id);
+  if (!s.flow) throw std::invalid_argument("server: run is a uniform-grid run");
+  return s.flow.get();
+}
+
+namespace {
+// An octree step in the uniform report's fields: no drag or momentum residual;
+// mass_error = inflow - outflow; cfl the configured number the step size obeys.
+nbl_step_report octreeReport(const nbl_octree_step_report& r, double cfl, uint64_t leaves) {
+  nbl_step_report o{};
+  o.step = r.step;
+  o.t = r.t;
+  o.dt = r.dt;
+  o.cfl = cfl;
+  o.divergence_max = r.divergence_max;
+  o.mass_error = r.inflow - r.outflow;
+  o.poisson_iters = r.poisson_iters;
+  o.poisson_residual = r.poisson_residual;
+  o.cell_count = leaves;
+  o.accepted = 1;
+  return o;
+}
+}  // namespace
 
 uint64_t Server::submit(int id) {
   std::lock_guard<std::mutex> lk(mu_);
@@ -120,7 +173,10 @@ void Server::work() {
     }
     Done d;
     try {
-      d.rep = slot->run->impl->step(0.0);
+      if (slot->flow)
+        d.rep = octreeReport(slot->flow->impl->step(0.0), slot->flow->impl->cfl(), slot->tree->impl->size());
+      else
+        d.rep = slot->run->impl->step(0.0);
     } catch (const std::exception& e) {
       d.error = e.what();
     }

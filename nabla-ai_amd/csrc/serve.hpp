@@ -16,6 +16,7 @@
 #include <vector>
 #include <deque>
 
+#include "octflow.hpp"
 #include "solver.hpp"
 
 namespace nbl {
@@ -25,7 +26,12 @@ class Server {
   Server(nbl_ctx* ctx, int slots, uint64_t budget);
   ~Server();
   int admit(const HostGrid& g, const nbl_flow_cfg& cfg, const nbl_solver_opts& opts);
+  // A flow run on octree leaves (caller order, as nbl_octree_create takes
+  // them); the server owns the octree, its multigrid hierarchy and the flow.
+  int admitOctree(const uint64_t* keys, const uint8_t* levels, const uint8_t* masks, size_t n,
+                  const double origin[3], const double extent[3], int maxLevel, const nbl_octree_flow_cfg& cfg);
   nbl_run* run(int id);  // only while the run has no request in flight
+  nbl_octree_flow* octreeFlow(int id);  // the same, for octree runs
   uint64_t submit(int id);
   void wait(uint64_t ticket, nbl_step_report* rep, double* latencyMs);
   void footprint(uint64_t* used, uint64_t* budget) const;
@@ -37,7 +43,9 @@ class Server {
     std::chrono::steady_clock::time_point arrival;
   };
   struct Slot {
-    std::unique_ptr<nbl_run> run;
+    std::unique_ptr<nbl_run> run;           // uniform runs
+    std::unique_ptr<nbl_octree> tree;       // octree runs: the leaves and
+    std::unique_ptr<nbl_octree_flow> flow;  // their flow state
     uint64_t bytes = 0;
     bool busy = false;
     int pending = 0;
@@ -48,6 +56,8 @@ class Server {
     std::string error;
   };
   void work();
+  int insert(std::unique_ptr<Slot> s, uint64_t foot);
+  Slot& idle(int id);
 
   nbl_ctx* ctx_;
   uint64_t budget_, used_ = 0;

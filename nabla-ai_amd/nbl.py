@@ -283,6 +283,9 @@ def load(path: os.PathLike | None = None):
     _proto(lib, "nbl_server_destroy", i, V)
     _proto(lib, "nbl_server_admit", i, V, V, V, V, C.POINTER(C.c_int))
     _proto(lib, "nbl_server_run", i, V, i, C.POINTER(V))
+    _proto(lib, "nbl_server_admit_octree", i, V, P_U64, P_U8, P_U8, sz, P_D, P_D, i, C.POINTER(OctFlowCfg),
+           C.POINTER(C.c_int))
+    _proto(lib, "nbl_server_octree_flow", i, V, i, C.POINTER(V))
     _proto(lib, "nbl_server_submit", i, V, i, C.POINTER(C.c_uint64))
     _proto(lib, "nbl_server_wait", i, V, C.c_uint64, C.POINTER(StepReport), C.POINTER(C.c_double))
     _proto(lib, "nbl_server_footprint", i, V, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64))
@@ -605,9 +608,9 @@ class OctreeFlow:
         return out
 
     def close(self):
-        if self.h:
+        if self.h and not getattr(self, "_borrowed", False):
             check(self.lib.nbl_octree_flow_destroy(self.h))
-            self.h = C.c_void_p()
+        self.h = C.c_void_p()
 
 
 class PoissonBatch:
@@ -784,6 +787,7 @@ class Server:
         self.h = C.c_void_p()
         check(self.lib.nbl_server_create(ctx.h, slots, hbm_budget, C.byref(self.h)))
         self._keep = []
+        self._octree_n = {}
 
     def admit(self, grid: Grid, cfg: FlowCfg, opts: SolverOpts | None = None) -> int:
         d = grid.desc()
@@ -800,6 +804,28 @@ class Server:
         r = Run.__new__(Run)
         r.lib, r.grid, r.h, r._borrowed = self.lib, grid, h, True
         return r
+
+    def admit_octree(self, keys, levels, masks, origin, extent, max_level, cfg: OctFlowCfg) -> int:
+        """An octree flow run (nbl_server_admit_octree); leaves in caller order."""
+        k = np.ascontiguousarray(keys, dtype=np.uint64)
+        lv = np.ascontiguousarray(levels, dtype=np.uint8)
+        m = np.ascontiguousarray(masks, dtype=np.uint8)
+        o = (C.c_double * 3)(*origin)
+        e = (C.c_double * 3)(*extent)
+        rid = C.c_int()
+        check(self.lib.nbl_server_admit_octree(self.h, k.ctypes.data_as(P_U64), lv.ctypes.data_as(P_U8),
+                                               m.ctypes.data_as(P_U8), k.size, o, e, max_level, C.byref(cfg),
+                                               C.byref(rid)))
+        self._octree_n[rid.value] = k.size
+        return rid.value
+
+    def octree_flow(self, run_id: int) -> "OctreeFlow":
+        """A borrowed flow handle (owned by the server) for set/get state."""
+        h = C.c_void_p()
+        check(self.lib.nbl_server_octree_flow(self.h, run_id, C.byref(h)))
+        f = OctreeFlow.__new__(OctreeFlow)
+        f.lib, f.n, f.tree, f.h, f._borrowed = self.lib, self._octree_n[run_id], None, h, True
+        return f
 
     def submit(self, run_id: int) -> int:
         t = C.c_uint64()

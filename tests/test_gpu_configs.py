@@ -107,36 +107,35 @@ def test_batched_octree_solves_at_size(ctx, mg):
 
 
 def test_served_stream_at_size_equals_alone(ctx):
+    """C5's 32 seeded runs (Taylor-Green boxes, cube channels and cube-wake
+    octrees of 0.3-4M cells or leaves) admitted to one server, two requests
+    each submitted together so that different runs' steps overlap on the
+    device: every served step and final state equals the run stepped alone."""
     runs = cases.served_runs(32, 777)
-    cells = [g.cells for _, g, _, _ in runs]
+    cells = [r["cells"] for r in runs]
     assert min(cells) >= 300_000 and max(cells) <= 4_000_000
+    assert {r["kind"] for r in runs} == {"uniform", "octree"}
     opts = nbl.solver_opts(nbl.PREC_MIXED)
     srv = nbl.Server(ctx, slots=4, hbm_budget=200 << 30)
     try:
-        ids = []
-        for _name, g, cfg, st in runs:
-            rid = srv.admit(g, cfg, opts)
-            srv.run(rid, g).set_state(*st)
-            ids.append(rid)
-        # two rounds of requests for every run, submitted together so that
-        # different runs' steps overlap on the device
+        ids = [cases.admit_served(srv, r, opts) for r in runs]
         tickets = [srv.submit(rid) for _ in range(2) for rid in ids]
         served = [srv.wait(t)[0] for t in tickets]
-        states = [srv.run(rid, g).get_state() for rid, (_n, g, _c, _s) in zip(ids, runs)]
+        states = [cases.served_state(srv, rid, r) for rid, r in zip(ids, runs)]
     finally:
         srv.close()
-    for q, (name, g, cfg, st) in enumerate(runs):
-        run = nbl.Run(ctx, g, cfg, opts)
-        try:
-            run.set_state(*st)
-            alone = [run.step(), run.step()]
-            ref = run.get_state()
-        finally:
-            run.close()
+    for q, r in enumerate(runs):
+        alone, ref = cases.step_alone(ctx, r, 2, opts)
         mine = [served[q], served[len(runs) + q]]
         for a, b in zip(mine, alone):
-            assert (a.step, a.poisson_iters) == (b.step, b.poisson_iters), name
-            assert (a.dt, a.cd, a.cl, a.poisson_residual) == (b.dt, b.cd, b.cl, b.poisson_residual), name
-        for fa, fb in zip(states[q][:4], ref[:4]):
-            assert np.array_equal(fa, fb), name
-    print(f"32 runs, {sum(cells)} cells, 64 served steps equal to the runs stepped alone", flush=True)
+            assert (a.step, a.poisson_iters, a.dt, a.poisson_residual) == \
+                   (b.step, b.poisson_iters, b.dt, b.poisson_residual), r["name"]
+            if r["kind"] == "uniform":
+                assert (a.cd, a.cl) == (b.cd, b.cl), r["name"]
+            else:
+                assert a.cell_count == r["cells"] and a.divergence_max == b.divergence_max, r["name"]
+                assert a.mass_error == b.inflow - b.outflow, r["name"]
+        for fa, fb in zip(states[q], ref):
+            assert np.array_equal(fa, fb), r["name"]
+    print(f"32 runs ({sum(r['kind'] == 'octree' for r in runs)} octrees), {sum(cells)} cells, "
+          "64 served steps equal to the runs stepped alone", flush=True)

@@ -5,6 +5,7 @@
   deterministic and a run's arithmetic never depends on other streams);
 * requests for a busy run are served in order and later requests for idle
   runs overtake them;
+* octree flow runs are served beside uniform ones, with the same guarantee;
 * admission refuses a run whose device footprint exceeds the budget;
 * a batch of solves equals the same solves one by one, bit for bit.
 """
@@ -63,6 +64,43 @@ def test_served_steps_equal_alone(ctx):
                 assert np.array_equal(a, b)
     finally:
         srv.close()
+
+
+def test_served_octree_runs_equal_alone(ctx):
+    """Octree flow runs (cube-wake trees of finest level 6 and 7) served beside
+    a uniform run: every served step report and the final state equal the
+    run stepped alone; handles of the wrong kind are refused."""
+    runs = [cases.cube_wake_run(6, 3.0, 800.0, 11), cases.cube_wake_run(7, 1.5, 1500.0, 12)]
+    g, cfg = cases.taylor_green(16)
+    runs.append(dict(name="tgv16", kind="uniform", cells=g.cells, grid=g, cfg=cfg,
+                     state=cases.taylor_green_state(g)))
+    steps = 3
+    srv = nbl.Server(ctx, slots=3)
+    try:
+        ids = [cases.admit_served(srv, r) for r in runs]
+        with pytest.raises(nbl.NblError):
+            srv.run(ids[0], g)
+        with pytest.raises(nbl.NblError):
+            srv.octree_flow(ids[2])
+        tickets = [(q, srv.submit(rid)) for _ in range(steps) for q, rid in enumerate(ids)]
+        got = [[] for _ in runs]
+        for q, t in tickets:
+            got[q].append(srv.wait(t)[0])
+        states = [cases.served_state(srv, rid, r) for rid, r in zip(ids, runs)]
+    finally:
+        srv.close()
+    for q, r in enumerate(runs):
+        reps, ref = cases.step_alone(ctx, r, steps)
+        for a, b in zip(got[q], reps):
+            if r["kind"] == "octree":
+                assert (a.step, a.t, a.dt, a.divergence_max, a.poisson_iters, a.poisson_residual) == \
+                       (b.step, b.t, b.dt, b.divergence_max, b.poisson_iters, b.poisson_residual)
+                assert a.mass_error == b.inflow - b.outflow and a.cell_count == r["cells"]
+                assert a.cfl == r["cfg"].cfl and a.accepted == 1
+            else:
+                assert a.as_dict() == b.as_dict()
+        for fa, fb in zip(states[q], ref):
+            assert np.array_equal(fa, fb), r["name"]
 
 
 def test_server_admission_budget(ctx):
