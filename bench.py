@@ -243,6 +243,7 @@ def octree_c3(ctx, with_cpu=True, flow_steps=200):
                           "speedup_vs_fp64_vcycle": round(mg_ms / max(qms, 1e-9), 3),
                           "rel_l2_vs_fp64_vcycle": rel(xq, xm)}
     flow = octree_flow(d, m, flow_steps) if flow_steps > 0 else None
+    ncls = d.coef_classes()
     d.close()
     ach = by / (ms * 1e-3) / 1e9
     out = {"leaves": n, "couplings": nnz, "host_build_s": round(host_build_s, 3),
@@ -260,7 +261,12 @@ def octree_c3(ctx, with_cpu=True, flow_steps=200):
                                     "as BF16"},
            "spmv_roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": round(ach / HBM_PEAK_GBS, 4), "avg_launch_us": round(ms * 1e3, 3),
-                             "bytes_per_launch": by},
+                             "bytes_per_launch": by, "coef_classes": ncls,
+                             "note": "k_o_apply_dot; bytes: per leaf 8 B of coefficient classes and count, mask, "
+                                     "x and y, per coupling a column index (and the FP64 coefficient of "
+                                     "couplings beyond a row's sixth)" if ncls else
+                                     "k_o_apply_dot; bytes: per leaf count, mask, x and y, per coupling a column "
+                                     "index and an FP64 coefficient"},
            "transport_step": {"kernel": "k_oct_transport (octreeTransportStep, first-order upwind)",
                               "avg_launch_us": round(tms * 1e3, 3), "mleaves_per_s": round(n / tms / 1e3, 1),
                               "achieved_gbs": round(tby / tms / 1e6, 1), "bytes_per_launch": tby},

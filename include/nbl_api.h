@@ -344,6 +344,11 @@ int nbl_octree_create(nbl_ctx* ctx, const uint64_t* keys, const uint8_t* levels,
                       const double extent[3], int max_level, nbl_octree** out);
 int nbl_octree_destroy(nbl_octree* oc);
 int nbl_octree_count(nbl_octree* oc, size_t* n, size_t* nnz);
+/* Storage of the leaf operator's coefficients (no reference counterpart): the
+ * number of distinct coefficient values when they are at most 256 and the
+ * operator kernels read a one-byte class per coupling and a table of the
+ * values (the same bits), or 0 when they read one coefficient per coupling. */
+int nbl_octree_coef_classes(nbl_octree* oc, int* ncls);
 int nbl_octree_permutation(nbl_octree* oc, uint64_t* perm);
 /* Face table in sorted order, as Octree::faceNeighbors (octree.cpp:229-283)
  * returns it: for leaf i and direction d (0..5), count[6*i+d] in {0,1,..,4}
@@ -491,7 +496,10 @@ int nbl_octree_flow_divergence(nbl_octree_flow* f, double* div);
 /* No counterpart in the reference: measurement.  Average ms and algorithmic
  * bytes of one launch of an octree kernel (kernel 0: the SpMV, reading
  * coupling counts, masks, x and writing y once per leaf plus a column index
- * and a coefficient per coupling; kernel 1: the transport step, per leaf its
+ * and a coefficient per coupling, or with coefficient classes
+ * (nbl_octree_coef_classes) 8 bytes of classes and count per leaf, a column
+ * index per coupling and the coefficients of couplings beyond a row's
+ * sixth; kernel 1: the transport step, per leaf its
  * level, face counts, first-neighbour slots, velocities and phi in and out,
  * plus an index per further neighbour; kernel 2: the sensors, per leaf its
  * centre, level, mask, face counts, first-neighbour slots, u, v, w, p, the

@@ -621,6 +621,13 @@ int nbl_octree_count(nbl_octree* oc, size_t* n, size_t* nnz) {
   });
 }
 
+int nbl_octree_coef_classes(nbl_octree* oc, int* ncls) {
+  return guarded([&] {
+    need(oc && ncls, "NULL argument");
+    *ncls = oc->impl->coefClasses();
+  });
+}
+
 int nbl_octree_permutation(nbl_octree* oc, uint64_t* perm) {
   return guarded([&] {
     need(oc && perm, "NULL argument");

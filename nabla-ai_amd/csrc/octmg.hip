@@ -589,7 +589,8 @@ void OctMg::setPrecision(int prec, hipStream_t s) {
   };
   for (Lvl& L : lv_) {
     const OctLevel& o = *L.op;
-    L.scoefF = toF(o.scoef, size_t(kOctSlots) * o.n);
+    if (o.ncls) L.ctabF = toF(o.ctab, size_t(o.ncls));
+    else L.scoefF = toF(o.scoef, size_t(kOctSlots) * o.n);
     L.xcoefF = toF(o.xcoef, o.nx);
     L.diagF = toF(o.diag, o.n);
     L.bF = DevBuf(ctx_, o.n * sizeof(float));
@@ -615,7 +616,8 @@ OctViewT<float> OctMg::view<float>(size_t l) const {
   const OctLevel& o = *L.op;
   return OctViewT<float>{(long long)o.n,          o.mask.as<uint8_t>(),  o.cnt.as<uint8_t>(),
                          o.scol.as<int32_t>(),    L.scoefF.as<float>(),  o.xrow.as<int32_t>(),
-                         o.xcol.as<int32_t>(),    L.xcoefF.as<float>(),  L.diagF.as<float>()};
+                         o.xcol.as<int32_t>(),    L.xcoefF.as<float>(),  L.diagF.as<float>(),
+                         o.ncls ? o.meta.as<uint2>() : nullptr,          L.ctabF.as<float>()};
 }
 
 // V-cycle on level l with right-hand side b; the level's first sweep from a

@@ -55,6 +55,7 @@ class OctMg {
     DevBuf child;               // i32[n of level l-1]: cells of level l-1 grouped by parent
     DevBuf b, xa, xb;           // FP64 level vectors
     DevBuf scoefF, xcoefF, diagF, bF, xaF, xbF;  // FP32 operator copy and level vectors
+    DevBuf ctabF;               // FP32 class values (levels with coefficient classes)
     DevBuf xbB;                 // level 0, BF16 form: the pre-smoothed iterate
     int S[3] = {0, 0, 0};       // target log2 box size per axis of this level
   };

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
 
 #include <algorithm>
 #include <cstring>
@@ -217,6 +218,32 @@ __global__ void k_slot_fill(long long n, const int32_t* __restrict__ rowPtr, con
     xcol[xrow[i] + e - kOctSlots] = col[r0 + e];
     xcoef[xrow[i] + e - kOctSlots] = coef[r0 + e];
   }
+}
+
+__global__ void k_coef_bits(long long n, const double* __restrict__ coef, uint64_t* __restrict__ bits) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) bits[e] = uint64_t(__double_as_longlong(coef[e]));
+}
+// Per leaf: the class of every slot's coefficient (binary search of its bit
+// pattern in the ascending class table; unused slots hold 0.0, which the
+// table contains) and the row's coupling count, packed into 8 bytes.
+__global__ void k_slot_meta(long long n, int ncls, const uint64_t* __restrict__ tab, const uint8_t* __restrict__ cnt,
+                            const double* __restrict__ scoef, uint2* __restrict__ meta) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t w[2] = {0u, 0u};
+  for (int s = 0; s < kOctSlots; ++s) {
+    const uint64_t key = uint64_t(__double_as_longlong(scoef[(long long)s * n + i]));
+    int lo = 0, hi = ncls - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (tab[mid] < key) lo = mid + 1;
+      else hi = mid;
+    }
+    w[s >> 2] |= uint32_t(lo) << (8 * (s & 3));
+  }
+  w[1] |= uint32_t(cnt[i]) << 16;
+  meta[i] = make_uint2(w[0], w[1]);
 }
 
 __global__ void k_gather(long long n, const uint64_t* __restrict__ perm, const double* __restrict__ src,
@@ -805,6 +832,50 @@ void buildOctSlots(nbl_ctx* ctx, OctLevel& L, hipStream_t s) {
                                             L.xcol.as<int32_t>(), L.xcoef.as<double>());
   NBL_LAUNCH_CHECK();
   NBL_HIP(hipStreamSynchronize(s));
+  buildOctClasses(ctx, L, s);
+}
+
+void buildOctClasses(nbl_ctx* ctx, OctLevel& L, hipStream_t s) {
+  L.ncls = 0;
+  L.meta = DevBuf();
+  L.ctab = DevBuf();
+  const size_t n = L.n, nnz = L.nnz;
+  std::vector<uint64_t> tab{0ull};  // unused slots hold 0.0
+  if (nnz > 0) {
+    const long long E = (long long)nnz;
+    DevBuf bits(ctx, nnz * sizeof(uint64_t)), sorted(ctx, nnz * sizeof(uint64_t)), uniq(ctx, nnz * sizeof(uint64_t));
+    DevBuf nu(ctx, sizeof(unsigned));
+    k_coef_bits<<<gridFor(E), kBlock, 0, s>>>(E, L.coef.as<double>(), bits.as<uint64_t>());
+    NBL_LAUNCH_CHECK();
+    size_t tb = 0, tb2 = 0;
+    NBL_HIP(rocprim::radix_sort_keys(nullptr, tb, bits.as<uint64_t>(), sorted.as<uint64_t>(), nnz, 0, 64, s));
+    NBL_HIP(rocprim::unique(nullptr, tb2, sorted.as<uint64_t>(), uniq.as<uint64_t>(), nu.as<unsigned>(), nnz,
+                            rocprim::equal_to<uint64_t>(), s));
+    DevBuf tmp(ctx, std::max(tb, tb2));
+    NBL_HIP(rocprim::radix_sort_keys(tmp.as<void>(), tb, bits.as<uint64_t>(), sorted.as<uint64_t>(), nnz, 0, 64, s));
+    NBL_HIP(rocprim::unique(tmp.as<void>(), tb2, sorted.as<uint64_t>(), uniq.as<uint64_t>(), nu.as<unsigned>(), nnz,
+                            rocprim::equal_to<uint64_t>(), s));
+    unsigned cnt = 0;
+    NBL_HIP(hipMemcpyAsync(&cnt, nu.as<unsigned>(), sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    NBL_HIP(hipStreamSynchronize(s));
+    if (cnt > unsigned(kOctClasses)) return;
+    std::vector<uint64_t> u(cnt);
+    NBL_HIP(hipMemcpyAsync(u.data(), uniq.as<uint64_t>(), cnt * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    NBL_HIP(hipStreamSynchronize(s));
+    tab.insert(tab.end(), u.begin(), u.end());
+    std::sort(tab.begin(), tab.end());
+    tab.erase(std::unique(tab.begin(), tab.end()), tab.end());
+    if (tab.size() > size_t(kOctClasses)) return;
+  }
+  L.ctab = DevBuf(ctx, tab.size() * sizeof(uint64_t));
+  NBL_HIP(hipMemcpyAsync(L.ctab.as<void>(), tab.data(), tab.size() * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+  L.meta = DevBuf(ctx, std::max<size_t>(1, n) * sizeof(uint2));
+  const long long N = (long long)n;
+  k_slot_meta<<<gridFor(N), kBlock, 0, s>>>(N, int(tab.size()), L.ctab.as<uint64_t>(), L.cnt.as<uint8_t>(),
+                                            L.scoef.as<double>(), L.meta.as<uint2>());
+  NBL_LAUNCH_CHECK();
+  NBL_HIP(hipStreamSynchronize(s));
+  L.ncls = int(tab.size());
 }
 
 void buildOctOperator(nbl_ctx* ctx, int maxL, const double org[3], const double fine[3], OctLevel& L,
@@ -1182,8 +1253,15 @@ void OctreeDev::profile(int kernel, int reps, double* ms, double* bytes) {
   float t = 0.f;
   NBL_HIP(hipEventElapsedTime(&t, e0_, e1_));
   *ms = double(t) / reps;
-  // coupling count, mask, x and y once per leaf; column index and coefficient per coupling
-  *bytes = double(L0_.n) * (1.0 + 1.0 + 8.0 + 8.0) + double(L0_.nnz) * (4.0 + 8.0);
+  // coupling count, mask, x and y once per leaf; column index and coefficient
+  // per coupling.  With coefficient classes: 8 bytes of classes and count,
+  // mask, x and y per leaf, a column index per coupling, the FP64 coefficient
+  // of each overflow coupling (beyond the six slots) and the class table.
+  if (L0_.ncls)
+    *bytes = double(L0_.n) * (8.0 + 1.0 + 8.0 + 8.0) + double(L0_.nnz) * 4.0 + double(L0_.nx) * 8.0 +
+             double(L0_.ncls) * 8.0;
+  else
+    *bytes = double(L0_.n) * (1.0 + 1.0 + 8.0 + 8.0) + double(L0_.nnz) * (4.0 + 8.0);
 }
 
 nbl_poisson_report OctreeDev::solveSorted(const double* b, double* xo, double tol, int maxIters, hipStream_t s) {

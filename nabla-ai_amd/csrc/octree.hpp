@@ -25,7 +25,17 @@ struct OctLevel {
   // row in slot-major arrays (slot s of row i at s*n + i, so a wavefront's
   // reads of one slot coalesce), the rest in an overflow CSR.
   DevBuf cnt, scol, scoef, xrow, xcol, xcoef;
+  // Coefficient classes: where a level's coefficients take at most
+  // kOctClasses distinct values (the usual case: face areas and centre
+  // distances of dyadic boxes), ctab holds them (ascending by bit pattern)
+  // and meta holds 8 bytes per leaf: the class of slot s in byte s (s < 6)
+  // and the row's coupling count in byte 6.  ncls == 0: no classes, the
+  // kernels read scoef.
+  DevBuf meta, ctab;
+  int ncls = 0;
 };
+
+constexpr int kOctClasses = 256;
 
 constexpr int kOctSlots = 6;
 // Leaves an octree may hold: its face table (24 entries per leaf) and its
@@ -98,27 +108,44 @@ struct OctViewT {
   const int32_t* xcol;
   const T* xcoef;
   const T* diag;
+  const uint2* meta;  // coefficient classes and counts (null: read cnt and scoef)
+  const T* ctab;      // class values
 };
 using OctView = OctViewT<double>;
 inline OctView octView(const OctLevel& L) {
   return OctView{(long long)L.n,       L.mask.as<uint8_t>(),  L.cnt.as<uint8_t>(),  L.scol.as<int32_t>(),
                  L.scoef.as<double>(), L.xrow.as<int32_t>(),  L.xcol.as<int32_t>(), L.xcoef.as<double>(),
-                 L.diag.as<double>()};
+                 L.diag.as<double>(),  L.ncls ? L.meta.as<uint2>() : nullptr,  L.ctab.as<double>()};
 }
 
 // sum_e coef_e (x_i - x_col_e) over row i's couplings in the reference's order
 // (octree_poisson.cpp:64-85), x read through X(j), in T arithmetic.
+// With coefficient classes one 8-byte load gives the count and every slot's
+// class, and the coefficient is the class value (the same bits as scoef).
 template <class T, class Rd>
 __device__ __forceinline__ T octRowA(const OctViewT<T>& L, long long i, const Rd& X) {
   const T xi = X(i);
-  const int c = L.cnt[i];
   T acc = T(0);
+  int c;
+  if (L.meta) {
+    const uint2 m = L.meta[i];
+    c = int((m.y >> 16) & 0xffu);
 #pragma unroll
-  for (int s = 0; s < kOctSlots; ++s)
-    if (s < c) {
-      const long long q = (long long)s * L.n + i;
-      acc += L.scoef[q] * (xi - X(L.scol[q]));
-    }
+    for (int s = 0; s < kOctSlots; ++s)
+      if (s < c) {
+        const long long q = (long long)s * L.n + i;
+        const unsigned k = ((s < 4 ? m.x : m.y) >> (8 * (s & 3))) & 0xffu;
+        acc += L.ctab[k] * (xi - X(L.scol[q]));
+      }
+  } else {
+    c = L.cnt[i];
+#pragma unroll
+    for (int s = 0; s < kOctSlots; ++s)
+      if (s < c) {
+        const long long q = (long long)s * L.n + i;
+        acc += L.scoef[q] * (xi - X(L.scol[q]));
+      }
+  }
   if (c > kOctSlots)
     for (int e = L.xrow[i]; e < L.xrow[i + 1]; ++e) acc += L.xcoef[e] * (xi - X(L.xcol[e]));
   return acc;
@@ -131,6 +158,9 @@ void buildOctOperator(nbl_ctx* ctx, int maxL, const double org[3], const double 
                       DevBuf* cnt, DevBuf* nbr, hipStream_t s);
 // Slot-major copy (cnt, scol, scoef, xrow, xcol, xcoef) of a level's CSR.
 void buildOctSlots(nbl_ctx* ctx, OctLevel& L, hipStream_t s);
+// Coefficient classes of a level with slots built (ncls = 0 when its
+// coefficients take more than kOctClasses distinct values).
+void buildOctClasses(nbl_ctx* ctx, OctLevel& L, hipStream_t s);
 
 class OctMg;
 
@@ -141,6 +171,7 @@ class OctreeDev {
   ~OctreeDev();
   size_t size() const { return L0_.n; }
   size_t nnz() const { return L0_.nnz; }
+  int coefClasses() const { return L0_.ncls; }
   void permutation(uint64_t* perm);
   void faceTable(uint8_t* count, int32_t* nbr);
   // caller-order host arrays

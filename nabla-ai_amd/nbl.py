@@ -256,6 +256,7 @@ def load(path: os.PathLike | None = None):
     _proto(lib, "nbl_octree_create", i, V, P_U64, P_U8, P_U8, sz, P_D, P_D, i, C.POINTER(V))
     _proto(lib, "nbl_octree_destroy", i, V)
     _proto(lib, "nbl_octree_count", i, V, C.POINTER(sz), C.POINTER(sz))
+    _proto(lib, "nbl_octree_coef_classes", i, V, C.POINTER(i))
     _proto(lib, "nbl_octree_permutation", i, V, P_U64)
     _proto(lib, "nbl_octree_face_table", i, V, P_U8, P_I32)
     _proto(lib, "nbl_octree_apply", i, V, P_D, P_D)
@@ -681,6 +682,13 @@ class Octree:
         n, nnz = C.c_size_t(), C.c_size_t()
         check(self.lib.nbl_octree_count(self.h, C.byref(n), C.byref(nnz)))
         return n.value, nnz.value
+
+    def coef_classes(self):
+        """Distinct coefficient values of the leaf operator when its kernels
+        read them as one-byte classes (at most 256), else 0."""
+        k = C.c_int()
+        check(self.lib.nbl_octree_coef_classes(self.h, C.byref(k)))
+        return k.value
 
     def permutation(self):
         perm = np.empty(self.n, np.uint64)

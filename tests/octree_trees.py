@@ -76,3 +76,51 @@ def mg_level_counts(keys, levels, max_level, extent, coarsest=2048, max_levels=1
         if not merged:
             break
     return counts
+
+
+def coefficient_values(t):
+    """Distinct face coefficients area / dist of the reference's octree
+    operator (octree_poisson.cpp:24-32: the finer leaf's transverse face
+    area, Octree::cellSize, over the distance of the Octree::cellCenter
+    coordinates along the axis) over every fluid-fluid face of the tree."""
+    _k, l, m = t.dump()
+    _cnt, nbr = t.face_table()
+    cen = t.centers()
+    dxf = np.array(t.extent, float) / float(1 << t.max_level)
+    sz = (1 << (t.max_level - l.astype(np.int64))).astype(float)
+    vals = set()
+    for d in range(6):
+        a = d >> 1
+        t1, t2 = (a + 1) % 3, (a + 2) % 3
+        for q in range(4):
+            j = nbr[:, d, q]
+            i = np.flatnonzero((j >= 0) & (m == 0))
+            j = j[i]
+            keep = m[j] == 0
+            i, j = i[keep], j[keep]
+            fin = np.where(l[j] > l[i], j, i)
+            area = (sz[fin] * dxf[t1]) * (sz[fin] * dxf[t2])
+            vals.update((area / np.abs(cen[i, a] - cen[j, a])).tolist())
+    return vals
+
+
+def ladder_octree(orc, points=16, seed=3):
+    """Non-dyadic box, base level 2, refined to level 14 around seeded points
+    (each step refines the leaf holding the point): face coefficients between
+    many level pairs at inexact centres, more than 256 distinct values
+    (about 10k leaves)."""
+    origin, extent, ml = (0.1, -0.37, 0.61), (0.93, 1.37, 0.71), 14
+    t = orc.octree(origin, extent, ml)
+    t.refine_uniform(2)
+    rng = np.random.default_rng(seed)
+    dxf = np.array(extent) / float(1 << ml)
+    for _ in range(points):
+        p = np.array([origin[a] + extent[a] * rng.uniform(0.05, 0.95) for a in range(3)])
+        while True:
+            k, l, _m = t.dump()
+            half = (1 << (ml - l.astype(np.int64)))[:, None] * dxf[None, :] / 2
+            i = int(np.flatnonzero(np.all(np.abs(t.centers() - p[None, :]) <= half, axis=1))[0])
+            if l[i] >= ml:
+                break
+            t.refine_key(k[i])
+    return t

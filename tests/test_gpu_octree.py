@@ -519,3 +519,38 @@ def test_c3_sensors_invariants(ctx, c3):
         assert np.all(s["tau_wall"][wall] > 0.0) and np.all(sp == 1.1) and np.all(rp == 1.1)
     finally:
         d.close()
+
+
+def test_coefficient_classes_and_fallback(ctx, tree_list, orc):
+    """Leaf operators whose coefficients take at most 256 distinct values are
+    read as one-byte classes into a table of those values, others one
+    coefficient per coupling: the class count is the distinct count (0
+    beyond 256), and both forms apply the operator bit for bit as the
+    oracle and solve as it does (Jacobi-PCG, multigrid CG in FP64 and with
+    the FP32 cycle)."""
+    from octree_trees import coefficient_values, ladder_octree
+    cases_ = [tree_list[1], tree_list[2], ("ladder", ladder_octree(orc))]
+    for name, t in cases_:
+        nd = len(coefficient_values(t) | {0.0})
+        d = device_tree(ctx, t)
+        try:
+            assert d.coef_classes() == (nd if nd <= 256 else 0), (name, nd)
+            x = cases.normal(11, t.n)
+            assert np.array_equal(d.apply(x), t.apply(x)), name
+            assert np.array_equal(d.diagonal(), t.diagonal()), name
+            _k, _l, m = t.dump()
+            b = _projected_rhs(12, m)
+            xj, rj = d.solve(b, 1e-10, 20000)
+            xo, ro = t.solve(b, 1e-10, 20000)
+            assert rj.converged and abs(rj.iterations - ro.iterations) <= 1, name
+            assert rel(xj, xo) < 1e-9, name
+            xm, rm = d.mg_solve(b, 1e-10, 300)
+            xmo, rmo = t.mg_solve(b, 1e-10, 300)
+            assert rm.converged and abs(rm.iterations - rmo.iterations) <= 1, (name, rm.iterations, rmo.iterations)
+            assert rel(xm, xmo) < 1e-9, name
+            d.set_precision(nbl.PREC_MIXED)
+            xf, rf = d.mg_solve(b, 1e-10, 300)
+            assert rf.converged and rel(xf, xmo) < 1e-8, name
+        finally:
+            d.close()
+    assert any(name == "ladder" for name, _ in cases_)

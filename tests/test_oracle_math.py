@@ -9,7 +9,7 @@ import pytest
 
 import cases
 import nbl
-from octree_trees import adaptive_octree, mg_level_counts
+from octree_trees import adaptive_octree, coefficient_values, ladder_octree, mg_level_counts
 
 
 def periodic_grid(nx, ny, nz, h):
@@ -431,3 +431,32 @@ def test_octree_sensors_closed_forms(orc):
     assert (s2["resid_stagnation"] == 1).all() and np.allclose(rp2, U0)
     s3, _, _ = t.sensors(np.full(n, 2 * U0), zero, zero, zero, nu, sp2, rp2)
     assert (s3["resid_stagnation"] == 0).all()
+
+
+def test_octree_coefficient_set_is_the_operators(orc):
+    """coefficient_values (the face coefficients restated from the
+    reference's formula) is exactly the set of off-diagonal entries of the
+    oracle's applyA, read off its columns A e_j, on a 2:1 tree and a
+    non-dyadic box; the ladder tree has more than 256 distinct values."""
+    t1 = orc.octree((0, 0, 0), (1, 1, 1), 5)
+    t1.refine_uniform(2)
+    t1.refine_box((0, 0, 0), (0.5, 0.5, 0.5), 1)
+    t2 = orc.octree((0.1, -0.37, 0.61), (0.93, 1.37, 0.71), 6)
+    t2.refine_uniform(2)
+    rng = np.random.default_rng(5)
+    for _ in range(30):
+        k, lv, _m = t2.dump()
+        i = int(rng.integers(0, t2.n))
+        if lv[i] < 6:
+            t2.refine_key(k[i])
+    for i in range(0, t2.n, 7):
+        t2.set_mask(i, 2)
+    for t in (t1, t2):
+        cols = set()
+        for j in range(t.n):
+            e = np.zeros(t.n)
+            e[j] = 1.0
+            col = t.apply(e)
+            cols.update(float(-col[i]) for i in np.flatnonzero(col) if i != j)
+        assert cols == coefficient_values(t)
+    assert len(coefficient_values(ladder_octree(orc)) | {0.0}) > 256
