@@ -77,11 +77,15 @@ __global__ void __launch_bounds__(kBlock) k_om_sweep(OctViewT<T> L, const T* __r
   if (ctl && ldA(&ctl->done)) return;
   double acc[1] = {0.0};
   OGRID(i, L.n) {
+    // evaluated on every leaf (non-fluid rows have no couplings and a unit
+    // diagonal), so no load waits for the mask
+    const bool fl = L.mask[i] == 0;
+    const T xi = X(i);
+    const T Ax = octRowA(L, i, X);
+    const T bi = b[i], di = L.diag[i];
     T v = T(0);
-    if (L.mask[i] == 0) {
-      const T xi = X(i);
-      const T Ax = octRowA(L, i, X);
-      v = xi + T(kOmega) * (b[i] - Ax) / L.diag[i];
+    if (fl) {
+      v = xi + T(kOmega) * (bi - Ax) / di;
       if (FOLD) acc[0] += double(v);
     }
     storeAs(xo + i, v);
@@ -112,7 +116,8 @@ __global__ void __launch_bounds__(kBlock) k_om_restrict(OctViewT<T> F, OctViewT<
   const int32_t e = cstart[I + 1];
   for (int32_t p = cstart[I] + k; p < e; p += G) {
     const int32_t i = child[p];
-    if (F.mask[i] == 0) s += b[i] - octRowA(F, i, X);
+    const T r = b[i] - octRowA(F, i, X);  // valid on every row; loads do not wait for the mask
+    if (F.mask[i] == 0) s += r;
   }
 #pragma unroll
   for (int o = 1; o < G; o <<= 1) s += __shfl_xor(s, o, G);

@@ -182,8 +182,11 @@ struct XRd {  // x, or x - m with a pending mean
 // (octree_poisson.cpp:64-85); with a pending mean non-fluid rows give 0.
 __device__ __forceinline__ double octRow(const OctView& L, long long i, const double* __restrict__ x, double m,
                                          bool hasM) {
-  if (L.mask[i] != 0) return hasM ? 0.0 : x[i];
-  return octRowA(L, i, XRd{x, m, hasM});
+  // the row is evaluated on every leaf (a non-fluid row has no couplings),
+  // so its loads do not wait for the mask
+  const bool fl = L.mask[i] == 0;
+  const double a = octRowA(L, i, XRd{x, m, hasM});
+  return fl ? a : (hasM ? 0.0 : x[i]);
 }
 
 __global__ void k_oct_apply(OctView L, const double* __restrict__ x, double* __restrict__ out) {

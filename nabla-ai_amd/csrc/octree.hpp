@@ -122,30 +122,37 @@ inline OctView octView(const OctLevel& L) {
 // (octree_poisson.cpp:64-85), x read through X(j), in T arithmetic.
 // With coefficient classes one 8-byte load gives the count and every slot's
 // class, and the coefficient is the class value (the same bits as scoef).
+// Every slot holds a valid column (an unused slot names row i itself, with a
+// zero coefficient), so all six columns, coefficients and neighbour values are
+// loaded before the count is known: two memory round trips per row (columns,
+// then neighbour values) instead of one per dependent step.  Only the first
+// count slots are accumulated, in slot order.
 template <class T, class Rd>
 __device__ __forceinline__ T octRowA(const OctViewT<T>& L, long long i, const Rd& X) {
   const T xi = X(i);
   T acc = T(0);
   int c;
+  int32_t col[kOctSlots];
+  T xv[kOctSlots], cf[kOctSlots];
+#pragma unroll
+  for (int s = 0; s < kOctSlots; ++s) col[s] = L.scol[(long long)s * L.n + i];
   if (L.meta) {
     const uint2 m = L.meta[i];
+#pragma unroll
+    for (int s = 0; s < kOctSlots; ++s) xv[s] = X(col[s]);
     c = int((m.y >> 16) & 0xffu);
 #pragma unroll
-    for (int s = 0; s < kOctSlots; ++s)
-      if (s < c) {
-        const long long q = (long long)s * L.n + i;
-        const unsigned k = ((s < 4 ? m.x : m.y) >> (8 * (s & 3))) & 0xffu;
-        acc += L.ctab[k] * (xi - X(L.scol[q]));
-      }
+    for (int s = 0; s < kOctSlots; ++s) cf[s] = L.ctab[((s < 4 ? m.x : m.y) >> (8 * (s & 3))) & 0xffu];
   } else {
-    c = L.cnt[i];
 #pragma unroll
-    for (int s = 0; s < kOctSlots; ++s)
-      if (s < c) {
-        const long long q = (long long)s * L.n + i;
-        acc += L.scoef[q] * (xi - X(L.scol[q]));
-      }
+    for (int s = 0; s < kOctSlots; ++s) cf[s] = L.scoef[(long long)s * L.n + i];
+#pragma unroll
+    for (int s = 0; s < kOctSlots; ++s) xv[s] = X(col[s]);
+    c = L.cnt[i];
   }
+#pragma unroll
+  for (int s = 0; s < kOctSlots; ++s)
+    if (s < c) acc += cf[s] * (xi - xv[s]);
   if (c > kOctSlots)
     for (int e = L.xrow[i]; e < L.xrow[i + 1]; ++e) acc += L.xcoef[e] * (xi - X(L.xcol[e]));
   return acc;
