@@ -640,7 +640,11 @@ void OctMg::cycle(size_t l, const T* b, RedBuf rb, const SolveCtl* ctl, hipStrea
   const bool last = l + 1 == lv_.size();
   T* xa = kF ? L.xaF.as<T>() : L.xa.as<T>();
   T* xb = kF ? L.xbF.as<T>() : L.xb.as<T>();
+  // g: the reducing sweep's launch (at most kOmBlocks partials); gs: sweeps
+  // without partials, one leaf per thread, so no thread walks several leaves
+  // one dependent chain of loads after another
   const int g = omGrid(n);
+  const int gs = gridFor(n);
   if (last) {
     if (direct_) {
       MgLevel ml{};
@@ -653,8 +657,8 @@ void OctMg::cycle(size_t l, const T* b, RedBuf rb, const SolveCtl* ctl, hipStrea
       // further sweeps ends in xb
       for (int q = 0; q < kFallbackSweeps - 1; ++q) {
         const bool even = (q & 1) == 0;
-        k_om_sweep<T, PlainRd<T>, T, false><<<g, kBlock, 0, s>>>(V, b, PlainRd<T>{even ? xa : xb}, even ? xb : xa,
-                                                                 rb, ctl);
+        k_om_sweep<T, PlainRd<T>, T, false><<<gs, kBlock, 0, s>>>(V, b, PlainRd<T>{even ? xa : xb}, even ? xb : xa,
+                                                                  rb, ctl);
       }
       NBL_LAUNCH_CHECK();
     }
@@ -670,17 +674,17 @@ void OctMg::cycle(size_t l, const T* b, RedBuf rb, const SolveCtl* ctl, hipStrea
   const T* ec = nullptr;
   auto post = [&](auto xr) {
     using R = ProlongRd<T, decltype(xr)>;
-    k_om_sweep<T, R, T, false><<<g, kBlock, 0, s>>>(V, b, R{xr, ec, L.up.as<int32_t>()}, xa, rb, ctl);
+    k_om_sweep<T, R, T, false><<<gs, kBlock, 0, s>>>(V, b, R{xr, ec, L.up.as<int32_t>()}, xa, rb, ctl);
     if (fold)
       k_om_sweep<T, PlainRd<T>, T, true><<<g, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xb, rb, ctl);
     else
-      k_om_sweep<T, PlainRd<T>, T, false><<<g, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xb, rb, ctl);
+      k_om_sweep<T, PlainRd<T>, T, false><<<gs, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xb, rb, ctl);
     NBL_LAUNCH_CHECK();
   };
   if constexpr (kF) {
     if (l == 0 && prec_ == NBL_PREC_MIXED_BF16) {
       __hip_bfloat16* xbB = L.xbB.as<__hip_bfloat16>();
-      k_om_sweep<T, PlainRd<T>, __hip_bfloat16, false><<<g, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xbB, rb, ctl);
+      k_om_sweep<T, PlainRd<T>, __hip_bfloat16, false><<<gs, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xbB, rb, ctl);
       launchRestrict<T>(V, VC, C.cstart.as<int32_t>(), C.child.as<int32_t>(), b, Bf16Rd{xbB}, bc, xc, ctl, s);
       NBL_LAUNCH_CHECK();
       cycle<T>(l + 1, bc, rb, ctl, s, &ec);
@@ -690,7 +694,7 @@ void OctMg::cycle(size_t l, const T* b, RedBuf rb, const SolveCtl* ctl, hipStrea
     }
   }
   // pre-smoothing: first sweep (in xa) then one plain sweep
-  k_om_sweep<T, PlainRd<T>, T, false><<<g, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xb, rb, ctl);
+  k_om_sweep<T, PlainRd<T>, T, false><<<gs, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xb, rb, ctl);
   launchRestrict<T>(V, VC, C.cstart.as<int32_t>(), C.child.as<int32_t>(), b, PlainRd<T>{xb}, bc, xc, ctl, s);
   NBL_LAUNCH_CHECK();
   cycle<T>(l + 1, bc, rb, ctl, s, &ec);
