@@ -23,6 +23,26 @@ constexpr int kBlock = 256;          // threads per block for streaming kernels
 constexpr int kWave = 64;            // CDNA wavefront
 constexpr int kMaxRedVals = 8;       // values reduced together by one launch
 
+// Work-group sizes read from the implicit kernel arguments (code object v5:
+// hidden_group_size_x, _y, _z are 16-bit fields at byte offsets 12, 14 and 16
+// of the implicit argument block, which sits in device memory beside the
+// kernel arguments): one scalar load.  In kernels that remap their block
+// coordinates, blockDim.x can compile to a vector load at an offset selected
+// by the block index (the remainder of a non-uniform grid), which every
+// wavefront then waits for before its first address computation.  HIP launches
+// uniform work-groups, so these are blockDim's values.  (The dispatch packet
+// holds the same sizes, but it sits in the queue in host memory: reading it
+// from every wavefront made the pressure solve 2.4x slower.)
+__device__ __forceinline__ int wgSizeX() {
+  return int(((const uint16_t*)__builtin_amdgcn_implicitarg_ptr())[6]);
+}
+__device__ __forceinline__ int wgSizeY() {
+  return int(((const uint16_t*)__builtin_amdgcn_implicitarg_ptr())[7]);
+}
+__device__ __forceinline__ int wgSizeZ() {
+  return int(((const uint16_t*)__builtin_amdgcn_implicitarg_ptr())[8]);
+}
+
 struct HipError : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
@@ -108,9 +128,9 @@ __device__ __forceinline__ double waveMax(double v) { return waveReduce<true>(v)
 // thread 0.  Bit q of MAXMASK selects max instead of sum for value q.
 template <int K, unsigned MAXMASK>
 __device__ __forceinline__ void blockFold(double (&v)[K], double* sh) {
-  const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  const int tid = int(threadIdx.x) + wgSizeX() * (int(threadIdx.y) + wgSizeY() * int(threadIdx.z));
   const int lane = tid & (kWave - 1), wid = tid / kWave;
-  const int nw = (blockDim.x * blockDim.y * blockDim.z) / kWave;
+  const int nw = (wgSizeX() * wgSizeY() * wgSizeZ()) / kWave;
 #pragma unroll
   for (int q = 0; q < K; ++q) {
     v[q] = ((MAXMASK >> q) & 1u) ? waveMax(v[q]) : waveSum(v[q]);
@@ -154,7 +174,7 @@ __device__ __forceinline__ void foldPartials(const double* partials, int stride,
 #pragma unroll
   for (int q = 0; q < K; ++q) {
     a[q] = 0.0;
-    for (int b = threadIdx.x; b < P; b += blockDim.x) {
+    for (int b = threadIdx.x; b < P; b += wgSizeX()) {
       const double pv = ldA(&partials[q * stride + b]);
       a[q] = ((MAXMASK >> q) & 1u) ? fmax(a[q], pv) : a[q] + pv;
     }

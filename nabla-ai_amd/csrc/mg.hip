@@ -154,18 +154,18 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc
   __shared__ unsigned char fls[kBlock * NC];
   const Blk3 B = blk3(L.slab);
   const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
-  const int bx = blockDim.x, by = blockDim.y;
+  const int bx = wgSizeX(), by = wgSizeY(), bz = wgSizeZ();
   // a batch's problems follow each other along z, each in its own blocks
   int bzl = B.z, p = 0;
   if (L.nb > 1) {
-    const int gzp = (L.nz + int(blockDim.z) - 1) / int(blockDim.z);
+    const int gzp = (L.nz + bz - 1) / bz;
     p = bzl / gzp;
     bzl -= p * gzp;
     ctl = ctlAt(ctl, p);
     mb = mb.at(p);
     mx = mx.at(p);
   }
-  const int i = NC * (B.x * bx + tx), j = B.y * by + ty, k = bzl * int(blockDim.z) + tz;
+  const int i = NC * (B.x * bx + tx), j = B.y * by + ty, k = bzl * bz + tz;
   const int tid = tx + bx * (ty + by * tz);
   const bool in = i < L.nx && j < L.ny && k < L.nz;
   const int c = i + L.nx * (j + L.ny * (k + L.nz * p));

@@ -22,8 +22,8 @@ struct Cell {
 __device__ __forceinline__ Cell cellOf(int nx, int ny, int nz) {
   Cell q;
   const Blk3 B = blk3(0);
-  q.i = B.x * blockDim.x + threadIdx.x;
-  q.j = B.y * blockDim.y + threadIdx.y;
+  q.i = B.x * wgSizeX() + int(threadIdx.x);
+  q.j = B.y * wgSizeY() + int(threadIdx.y);
   q.k = B.z;
   q.valid = q.i < nx && q.j < ny && q.k < nz;
   q.c = q.i + nx * (q.j + ny * q.k);
@@ -48,8 +48,8 @@ template <int NC>
 __device__ __forceinline__ CellN<NC> cellOfN(int nx, int ny, int nz, int slab, int nb = 1) {
   CellN<NC> q;
   const Blk3 B = blk3(slab);
-  q.i = NC * (B.x * int(blockDim.x) + int(threadIdx.x));
-  q.j = B.y * int(blockDim.y) + int(threadIdx.y);
+  q.i = NC * (B.x * wgSizeX() + int(threadIdx.x));
+  q.j = B.y * wgSizeY() + int(threadIdx.y);
   q.k = B.z;
   q.p = 0;
   if (nb > 1) {
@@ -291,14 +291,16 @@ __device__ __forceinline__ void gatherN(const MgLevel& L, const XV& X, int c, in
 // stop the solve reads `done` once per block, so that every thread of a block
 // takes the same branch.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int linearTid() { return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z); }
+__device__ __forceinline__ int linearTid() {
+  return int(threadIdx.x) + wgSizeX() * (int(threadIdx.y) + wgSizeY() * int(threadIdx.z));
+}
 __device__ __forceinline__ bool firstBlock() { return blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0; }
 
 template <int K>
 __device__ __forceinline__ void foldInline(const InFold& f, double (&out)[K]) {
   __shared__ double sh[K * 16];
   __shared__ double res[K];
-  const int tid = linearTid(), nt = blockDim.x * blockDim.y * blockDim.z;
+  const int tid = linearTid(), nt = wgSizeX() * wgSizeY() * wgSizeZ();
   double a[K];
 #pragma unroll
   for (int q = 0; q < K; ++q) {
@@ -331,7 +333,7 @@ __device__ __forceinline__ bool stopOrMeans(const SolveCtl* ctl, const MeanRef (
   __shared__ double sh[K * 16];
   __shared__ double res[K];
   __shared__ int st;
-  const int tid = linearTid(), nt = blockDim.x * blockDim.y * blockDim.z;
+  const int tid = linearTid(), nt = wgSizeX() * wgSizeY() * wgSizeZ();
   const int d = (tid == 0 && ctl) ? ldA(&ctl->done) : 0;
   double a[K];
 #pragma unroll
@@ -366,7 +368,7 @@ __device__ __forceinline__ void foldStop(const InFold& f, const SolveCtl* ctl, c
   __shared__ double sh[K * 16];
   __shared__ double res[K + 2];
   __shared__ int st;
-  const int tid = linearTid(), nt = blockDim.x * blockDim.y * blockDim.z;
+  const int tid = linearTid(), nt = wgSizeX() * wgSizeY() * wgSizeZ();
   int d = 0;
   double x0 = 0.0, x1 = 0.0;
   if (tid == 0) {

@@ -411,7 +411,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs_rk(NsArgs A, PadBufs pb, const d
   }
   int X, Y, Z;
   compDims(g, comp, X, Y, Z);
-  const int i = B.x * blockDim.x + threadIdx.x, j = B.y * blockDim.y + threadIdx.y;
+  const int i = B.x * wgSizeX() + int(threadIdx.x), j = B.y * wgSizeY() + int(threadIdx.y);
   if (i >= X || j >= Y) return;
   const PadRd r{pb.p[0] + pb.origin[0], pb.p[1] + pb.origin[1], pb.p[2] + pb.origin[2],
                 pb.sx[0], pb.sxy[0], pb.sx[1], pb.sxy[1], pb.sx[2], pb.sxy[2]};
