@@ -132,7 +132,7 @@ typedef struct nbl_solver_opts {
                                LDS tiles recomputing their halos) wherever
                                it applies: at least three levels, direct
                                coarsest solve, pre = post = 2, a finest
-                               level of at most 2048 tiles (2^20 cells in
+                               level of at most 4096 tiles (2^21 cells in
                                3-D).  0 (default): 2 on singular systems,
                                where it also folds the per-sweep null-space
                                projections away (10 launches per iteration

@@ -274,8 +274,9 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
                                 "at most 4096 cells, an invertible operator)");
   if (opts.vcycle < 0 || opts.vcycle > 2) throw std::invalid_argument("vcycle must be 0, 1 or 2");
   // the two-kernels-per-level cycle: V(2,2), a level between the finest and
-  // the directly solved coarsest one, and few enough level-0 workgroups that
-  // every reduction folds inline into its consumer.  Automatic: on singular
+  // the directly solved coarsest one, and few enough level-0 workgroups
+  // (kTiledFoldMax) that every reduction folds inline into its consumer.
+  // Automatic: on singular
   // systems only.  On the 192x64x64 channel (non-singular) its level-0
   // kernels, which recompute their halos, take 33 + 24 us against 29 + 15 us
   // for the per-sweep kernels they replace (the coarser levels break even);
@@ -283,7 +284,7 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
   // projections and a step takes 3.58 instead of 5.05 ms.
   const bool wantTiled = opts.vcycle == 2 || (opts.vcycle == 0 && (project_ || bf16_));
   tiled_ = kind == 0 && !loops_ && !batched_ && wantTiled && nl >= 3 && opts.pre_smooth == 2 && opts.post_smooth == 2 &&
-           direct_ && blocks_[0] <= kInlineFoldMax && tblocks_[0] <= kInlineFoldMax;
+           direct_ && blocks_[0] <= kTiledFoldMax && tblocks_[0] <= kTiledFoldMax;
   for (int l = 0; tiled_ && l + 1 < nl; ++l) tiled_ = tileFits(dl_[l], dl_[l + 1]);
   if (tiled_ && project_) rnRaw_ = DevBuf(ctx, n0 * 8);
   // BF16 smoother scratch lives in the two-kernels-per-level cycle: the finest
@@ -291,7 +292,7 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
   if (bf16_ && !tiled_)
     throw std::invalid_argument(
         "precision NBL_PREC_MIXED_BF16 needs the two-kernels-per-level V-cycle (vcycle 0 or 2, pre = post = 2, "
-        "at least three levels, a direct coarsest solve, a finest level of at most 2048 tiles)");
+        "at least three levels, a direct coarsest solve, a finest level of at most 4096 tiles)");
   if (bf16_) xaB_ = DevBuf(ctx, n0 * sizeof(__hip_bfloat16));
   NBL_HIP(hipDeviceSynchronize());
 }

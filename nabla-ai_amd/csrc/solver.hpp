@@ -201,6 +201,12 @@ class PoissonDev {
   // without projections, few enough level-0 blocks); conv_: the convergence
   // test the V-cycle's first level-0 sweep performs.
   static constexpr int kInlineFoldMax = 2048;
+  // the two-kernels-per-level cycle folds every reduction inline (every block
+  // reads all partials); it is used up to this many level-0 workgroups.
+  // Measured on periodic boxes (mixed form, ms per step): 100^3 (2275 tiles)
+  // 12.2 -> 9.9 and 116^3 (3480) 13.6 -> 12.8 against one kernel per sweep,
+  // while 146^3 (7030 tiles) went from 24.4 to 29.2
+  static constexpr int kTiledFoldMax = 4096;
   static constexpr int kSlotRows = 64;  // partial rows of the pending-mean slots, after the kMaxRedVals rows
   bool inline_ = false;
   InFold conv_{};

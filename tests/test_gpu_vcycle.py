@@ -189,3 +189,27 @@ def test_bf16_needs_the_tiled_cycle(ctx):
     with pytest.raises(nbl.NblError):
         nbl.Poisson(ctx, GRIDS["cube96"], kind=0, opts=nbl.solver_opts(nbl.PREC_MIXED_BF16,
                                                                          vcycle=nbl.VCYCLE_PER_SWEEP))
+
+
+@pytest.mark.parametrize("precision", [nbl.PREC_FP64, nbl.PREC_MIXED], ids=["fp64", "mixed"])
+def test_tiled_cycle_beyond_2048_tiles(ctx, orc, precision):
+    """A 100^3 periodic box has 2275 level-0 tiles (odd coarse levels 25, 13,
+    7): the automatic choice runs the tiled cycle there, every reduction still
+    folded inline; it solves like the per-sweep cycle and the oracle."""
+    g, _ = cases.taylor_green(100)
+    b = cases.poisson_rhs(g, 53)
+    out = []
+    for vc in (nbl.VCYCLE_PER_SWEEP, 0):
+        ps = nbl.Poisson(ctx, g, kind=0, tol=1e-10, max_iters=200, opts=nbl.solver_opts(precision=precision, vcycle=vc))
+        try:
+            if vc == 0:
+                ps.profile(4, 1)  # the tiled cycle's down kernel: raises unless that cycle is in use
+            out.append(ps.solve(b))
+        finally:
+            ps.close()
+    (x1, r1), (x2, r2) = out
+    assert r1.converged and r2.converged and abs(r1.iterations - r2.iterations) <= 1
+    assert rel(x2, x1) < 1e-8
+    xo, ro = orc.mg(g, b, 1e-10, 200)
+    assert abs(r2.iterations - ro.iterations) <= (2 if precision == nbl.PREC_MIXED else 1)
+    assert rel(x2, xo) < 1e-8
