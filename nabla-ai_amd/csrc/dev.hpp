@@ -246,7 +246,7 @@ void launch_mg_sweep(const MgLevel& L, const T* b, MeanRef mb, const T* x, MeanR
                      RedBuf rb, SolveCtl* ctl, hipStream_t s, long long* stamps = nullptr, InFold conv = {});
 // x1c (optional): the coarse level's first sweep from a zero guess, written
 // beside the restricted residual (pointwise: no projection may come between,
-// so non-singular systems only).
+// so only where the restricted residual is not projected).
 template <class T>
 void launch_mg_resid_restrict(const MgLevel& L, const MgLevel& Lc, const T* b, MeanRef mb,
                               const T* x, MeanRef mx, T* rc, bool fold, RedBuf rb,

@@ -495,15 +495,27 @@ void PoissonDev::vcycle(int l, const T* b, MeanRef mb, const T** xout, MeanRef* 
   T* bufs[2] = {lx0_[l].as<T>(), lx1_[l].as<T>()};
   const int nbl = blocks_[l];
   const bool inlX = inlineMean(nbl);
-  const MeanRef ms[2] = {project_ ? meanRef(slotX(l, 0), inlX, nbl, L.nf) : MeanRef{},
-                         project_ ? meanRef(slotX(l, 1), inlX, nbl, L.nf) : MeanRef{}};
-  auto xRed = [&](int o) { return project_ ? meanRed(slotX(l, o), inlX) : rb; };
+  // Singular systems: the reference removes the fluid mean after every sweep
+  // and of every restricted residual (multigrid.cpp:110-112, 150-152).  The
+  // operator annihilates constants on fluid cells and is symmetric, so a
+  // residual's fluid sum vanishes and those projections change neither a
+  // residual nor, after the final projection of z (launch_mg_finish), the
+  // cycle's output in exact arithmetic.  The cycle keeps only the projection
+  // of z and, where the coarsest level is solved by CG, of its right-hand side
+  // (the projected inverse of the direct solve ignores a mean); results agree
+  // with the reference's to rounding, and the fusions of the non-singular
+  // cycle (first sweeps from zero written by the restrictions) apply.
+  const bool sp = false;
+  const bool rp = project_ && l + 2 == nlevels() && !direct_;
+  const MeanRef ms[2] = {sp ? meanRef(slotX(l, 0), inlX, nbl, L.nf) : MeanRef{},
+                         sp ? meanRef(slotX(l, 1), inlX, nbl, L.nf) : MeanRef{}};
+  auto xRed = [&](int o) { return sp ? meanRed(slotX(l, o), inlX) : rb; };
   int cur = -1;
   int q0 = 0;
   const bool fusedL0 = l == 0 && l0Fused_;
-  // Without projections the first sweep from zero is pointwise: the level-0
-  // update and each restriction write it with the right-hand side.
-  const bool firstDone = !project_ && opts_.pre_smooth >= 1 && (fusedL0 || l > 0);
+  // Without a pending mean on b the first sweep from zero is pointwise: the
+  // level-0 update and each restriction write it with the right-hand side.
+  const bool firstDone = opts_.pre_smooth >= 1 && (fusedL0 || (l > 0 && !mb.on()));
   if (firstDone) {
     cur = 0;
     q0 = 1;
@@ -519,17 +531,17 @@ void PoissonDev::vcycle(int l, const T* b, MeanRef mb, const T** xout, MeanRef* 
       conv_ = InFold{};
     }
     launch_mg_sweep<T>(L, b, mb, cur < 0 ? nullptr : bufs[cur], cur < 0 ? MeanRef{} : ms[cur], cur < 0, nullptr,
-                       MeanRef{}, nullptr, bufs[o], project_, xRed(o), ctl, s, timedSweep(l, cur >= 0), conv);
-    if (project_) meanFin(slotX(l, o), inlX, nbl, L.nf, true, s);
+                       MeanRef{}, nullptr, bufs[o], sp, xRed(o), ctl, s, timedSweep(l, cur >= 0), conv);
+    if (sp) meanFin(slotX(l, o), inlX, nbl, L.nf, true, s);
     cur = o;
   }
   const MgLevel& Lc = dl_[l + 1];
   T* bc = lb_[l + 1].as<T>();
   const int rbl = rblocks_[l + 1];
   const bool inlB = inlineMean(rbl) && !(l + 2 == nlevels() && !direct_);
-  const MeanRef mbc = project_ ? meanRef(slotB(l + 1), inlB, rbl, Lc.nf) : MeanRef{};
+  const MeanRef mbc = rp ? meanRef(slotB(l + 1), inlB, rbl, Lc.nf) : MeanRef{};
   // the coarse level's first sweep, unless the coarse level is the coarsest
-  T* x1c = (!project_ && opts_.pre_smooth >= 1 && l + 2 < nlevels()) ? lx0_[l + 1].as<T>() : nullptr;
+  T* x1c = (!rp && opts_.pre_smooth >= 1 && l + 2 < nlevels()) ? lx0_[l + 1].as<T>() : nullptr;
   // Otherwise the test rides on the level-0 restriction (the first level-0
   // sweep of the fused path is then the plain smoother sweep, kept free of it).
   InFold conv{};
@@ -538,9 +550,9 @@ void PoissonDev::vcycle(int l, const T* b, MeanRef mb, const T** xout, MeanRef* 
     conv = conv_;
     conv_ = InFold{};
   }
-  launch_mg_resid_restrict<T>(L, Lc, b, mb, bufs[cur], ms[cur], bc, project_,
-                              project_ ? meanRed(slotB(l + 1), inlB) : rb, ctl, s, x1c, conv);
-  if (project_) meanFin(slotB(l + 1), inlB, rbl, Lc.nf, true, s);
+  launch_mg_resid_restrict<T>(L, Lc, b, mb, bufs[cur], ms[cur], bc, rp, rp ? meanRed(slotB(l + 1), inlB) : rb,
+                              ctl, s, x1c, conv);
+  if (rp) meanFin(slotB(l + 1), inlB, rbl, Lc.nf, true, s);
   const T* ec = nullptr;
   MeanRef mec{};
   vcycle<T>(l + 1, bc, mbc, &ec, &mec, s);
@@ -554,8 +566,8 @@ void PoissonDev::vcycle(int l, const T* b, MeanRef mb, const T** xout, MeanRef* 
       return;
     }
     launch_mg_sweep<T>(L, b, mb, bufs[cur], ms[cur], false, q == 0 ? ec : nullptr, q == 0 ? mec : MeanRef{}, &Lc,
-                       bufs[o], project_, xRed(o), ctl, s, timedSweep(l, q > 0));
-    if (project_) meanFin(slotX(l, o), inlX, nbl, L.nf, true, s);
+                       bufs[o], sp, xRed(o), ctl, s, timedSweep(l, q > 0));
+    if (sp) meanFin(slotX(l, o), inlX, nbl, L.nf, true, s);
     cur = o;
   }
   *xout = bufs[cur];
