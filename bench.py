@@ -90,7 +90,8 @@ def roofline(ctx, g, reps=100, precision=nbl.PREC_FP64, kernel=0, key=None):
     in the mixed V-cycle), 1 the FP64 operator application with p.Ap (the
     outer CG's SpMV), 4 the finest-level down kernel of the
     two-kernels-per-level cycle."""
-    ps = nbl.Poisson(ctx, g, kind=0, tol=1e-8, max_iters=5000, opts=nbl.solver_opts(precision=precision))
+    vc = nbl.VCYCLE_TILED if kernel == 4 else nbl.VCYCLE_AUTO
+    ps = nbl.Poisson(ctx, g, kind=0, tol=1e-8, max_iters=5000, opts=nbl.solver_opts(precision=precision, vcycle=vc))
     ps.profile(kernel, 10)
     ms, by = ps.profile(kernel, reps)
     ps.close()
@@ -100,8 +101,7 @@ def roofline(ctx, g, reps=100, precision=nbl.PREC_FP64, kernel=0, key=None):
     name = {1: "k_fcg_apply<2> (A p and p.Ap, FP64; 18 B/cell)",
             4: f"k_down<{t}, DS_B> (both pre-sweeps, residual and restriction in LDS tiles; "
                f"{2 * (4 if fp32 else 8) + 2} B/cell + the coarse residual)"}.get(
-        kernel, f"k_sweep<{t}, SW_PLAIN> (finest-level damped-Jacobi sweep; fluid-sum fold when singular; "
-                f"{14 if fp32 else 26} B/cell)")
+        kernel, f"k_sweep<{t}, SW_PLAIN> (finest-level damped-Jacobi sweep; {14 if fp32 else 26} B/cell)")
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(key) if key else None,
             "kernel": name, "avg_launch_us": round(ms * 1e3, 3), "bytes_per_launch": by, "cells": g.cells,
@@ -571,10 +571,10 @@ def tgv_c1(ctx, a, steps=100):
         else:
             res[name]["rel_l2_vs_fp64"] = {k: rel(x, y) for k, x, y in zip("uvwp", stt[:4], state[:4])}
         log(f"tgv64 {name}: {res[name]['value']} Mcell-steps/s")
-    res["roofline"] = roofline(ctx, g, reps=200, kernel=4, key="tgv64_down_fp64")
-    res["roofline"]["note"] = ("64^3 singular: the finest level runs the two-kernels-per-level cycle; its down "
-                               "kernel (6.3 MB per launch at FP64) is latency-bound and Infinity-Cache resident")
-    res["roofline_mixed"] = roofline(ctx, g, reps=200, kernel=4, precision=nbl.PREC_MIXED)
+    res["roofline"] = roofline(ctx, g, reps=200, kernel=0)
+    res["roofline"]["note"] = ("64^3 singular: the finest-level sweep of the one-kernel-per-sweep V-cycle (6.8 MB "
+                               "per launch at FP64), latency-bound and Infinity-Cache resident")
+    res["roofline_mixed"] = roofline(ctx, g, reps=200, kernel=0, precision=nbl.PREC_MIXED)
     if not a.no_cpu and a.cpu_steps > 0:
         res["cpu_baseline"] = cpu_baseline(g, cfg, st, 8)
     return res
@@ -711,6 +711,7 @@ def main():
                "fp32_sweep": roofline(ctx, big, precision=nbl.PREC_MIXED, key="256x256x256_outlet_fp32"),
                "spmv_fp64": roofline(ctx, big, kernel=1, key="256x256x256_outlet_spmv"),
                "tgv64_down_fp64": roofline(ctx, tg, kernel=4, key="tgv64_down_fp64"),
+               "tgv64_sweep_fp64": roofline(ctx, tg, kernel=0),
                "copy_256MiB_ms": ctx.probe_copy(256 << 20, 20)}
         ctx.close()
         print(json.dumps(out), flush=True)

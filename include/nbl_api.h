@@ -133,11 +133,9 @@ typedef struct nbl_solver_opts {
                                it applies: at least three levels, direct
                                coarsest solve, pre = post = 2, a finest
                                level of at most 4096 tiles (2^21 cells in
-                               3-D).  0 (default): 2 on singular systems,
-                               where it also folds the per-sweep null-space
-                               projections away (10 launches per iteration
-                               instead of 21), 1 otherwise (measured faster
-                               there)                                      */
+                               3-D).  0 (default): 2 where the BF16
+                               smoother scratch needs it, 1 otherwise
+                               (measured faster)                           */
   int launch_order;         /* order in which the level kernels' workgroups
                                walk a level (workgroups go to the 8 XCDs
                                round-robin, each XCD has its own L2): 1 plain
@@ -303,7 +301,7 @@ int nbl_poisson_last_timing(nbl_poisson* ps, double* ms, int* vcycles);
  * `reps` times back to back on the solver's stream between two HIP events and
  * return the average duration of one launch (ms) and its algorithmic HBM
  * bytes (every array the kernel must touch counted once):
- *   kernel 0: finest-level damped-Jacobi sweep with its fused fluid sum
+ *   kernel 0: finest-level damped-Jacobi sweep as the V-cycle runs it
  *             (reads x, b, 2-byte code; writes x),
  *   kernel 1: finest-level A p with the p.Ap fold (reads p, code; writes Ap),
  *   kernel 2: finest-level residual + restriction (reads b, x, code; writes

@@ -384,8 +384,10 @@ void launch_mg_sweep_dots(const MgLevel& L, const T* b, const T* x, double* z, c
                           RedBuf rb, SolveCtl* ctl, hipStream_t s, bool countIt = false);
 // rn <- rn - mean (fluid) / 0 (solid), partials of |rn|^2.
 // b0 (optional): also write float(r_new) there, the FP32 V-cycle's right-hand side
+// x1f / x1d (optional, at most one): the level-0 first sweep from a zero guess on
+// that right-hand side, in the cycle's type
 void launch_fcg_norm(const MgLevel& L, double* rn, MeanRef mrn, const SolveCtl* ctl, RedBuf rb, hipStream_t s,
-                     float* b0 = nullptr);
+                     float* b0 = nullptr, float* x1f = nullptr, double* x1d = nullptr);
 void launch_fcg_dots(const MgLevel& L, double* z, MeanRef mz, const double* rn, const double* r,
                      SolveCtl* ctl, RedBuf rb, hipStream_t s, bool countIt = false);
 // k_finish fused into k_fcg_dots (no pending means): z = V-cycle output x.

@@ -1098,9 +1098,13 @@ __global__ void __launch_bounds__(kBlock) k_fcg_update_first(MgLevel L, double* 
   }
 }
 
-template <int NC>
+// r_new <- r_new - mean (fluid) / 0; |r_new|^2; the FP32 V-cycle's right-hand
+// side (b0) and the level-0 first sweep from a zero guess in the cycle's type
+// (x1) beside it, each optional.
+template <class T, int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_norm(MgLevel L, double* __restrict__ rn, MeanRef mrn,
-                                                     const SolveCtl* ctl, RedBuf rb, float* __restrict__ b0) {
+                                                     const SolveCtl* ctl, RedBuf rb, float* __restrict__ b0,
+                                                     T* __restrict__ x1) {
   NBL_CELLS(NC);
   ctl = ctlAt(ctl, q.p);
   mrn = mrn.at(q.p);
@@ -1125,6 +1129,12 @@ __global__ void __launch_bounds__(kBlock) k_fcg_norm(MgLevel L, double* __restri
 #pragma unroll
       for (int e = 0; e < NC; ++e) f[e] = float(v[e]);
       stN<NC>(b0, q.c, f);
+    }
+    if (x1) {  // the first sweep from zero on that right-hand side, as k_sweep<T, SW_ZERO> writes it
+      T fv[NC];
+#pragma unroll
+      for (int e = 0; e < NC; ++e) fv[e] = firstSweep<T>(L, code[e], q.c + e, q.i + e, q.j, q.k, T(v[e]));
+      stN<NC>(x1, q.c, fv);
     }
   }
 }
@@ -1765,9 +1775,13 @@ template void launch_mg_sweep_dots<double>(const MgLevel&, const double*, const 
 template void launch_mg_sweep_dots<float>(const MgLevel&, const float*, const float*, double*, const double*,
                                           const double*, RedBuf, SolveCtl*, hipStream_t, bool);
 void launch_fcg_norm(const MgLevel& L, double* rn, MeanRef mrn, const SolveCtl* ctl, RedBuf rb, hipStream_t s,
-                     float* b0) {
+                     float* b0, float* x1f, double* x1d) {
+  if (x1f && x1d) throw std::logic_error("fcg_norm: one first-sweep output");
   withCells(L, [&](auto nc, const Geo3& g) {
-    k_fcg_norm<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, rn, mrn, ctl, rb, b0);
+    if (x1f)
+      k_fcg_norm<float, decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, rn, mrn, ctl, rb, b0, x1f);
+    else
+      k_fcg_norm<double, decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, rn, mrn, ctl, rb, b0, x1d);
   });
   NBL_LAUNCH_CHECK();
 }

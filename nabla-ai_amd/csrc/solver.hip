@@ -276,13 +276,13 @@ PoissonDev::PoissonDev(nbl_ctx* ctx, const HostGrid& g, int kind, double tol, in
   // the two-kernels-per-level cycle: V(2,2), a level between the finest and
   // the directly solved coarsest one, and few enough level-0 workgroups
   // (kTiledFoldMax) that every reduction folds inline into its consumer.
-  // Automatic: on singular
-  // systems only.  On the 192x64x64 channel (non-singular) its level-0
-  // kernels, which recompute their halos, take 33 + 24 us against 29 + 15 us
-  // for the per-sweep kernels they replace (the coarser levels break even);
-  // on the 64^3 periodic box (singular) it also drops the per-sweep
-  // projections and a step takes 3.58 instead of 5.05 ms.
-  const bool wantTiled = opts.vcycle == 2 || (opts.vcycle == 0 && (project_ || bf16_));
+  // Automatic: only where the BF16 smoother scratch asks for it.  Since the
+  // per-sweep cycle dropped the singular systems' per-sweep projections it is
+  // the faster one everywhere measured (periodic boxes, FP64 ms per step:
+  // 64^3 3.42 vs 3.53, 100^3 9.12 vs 10.62, 128^3 9.86 vs 13.03); on the
+  // 192x64x64 channel the tiled level-0 kernels, which recompute their halos,
+  // take 33 + 24 us against 29 + 15 us for the per-sweep kernels they replace.
+  const bool wantTiled = opts.vcycle == 2 || (opts.vcycle == 0 && bf16_);
   tiled_ = kind == 0 && !loops_ && !batched_ && wantTiled && nl >= 3 && opts.pre_smooth == 2 && opts.post_smooth == 2 &&
            direct_ && blocks_[0] <= kTiledFoldMax && tblocks_[0] <= kTiledFoldMax;
   for (int l = 0; tiled_ && l + 1 < nl; ++l) tiled_ = tileFits(dl_[l], dl_[l + 1]);
@@ -514,8 +514,9 @@ void PoissonDev::vcycle(int l, const T* b, MeanRef mb, const T** xout, MeanRef* 
   int q0 = 0;
   const bool fusedL0 = l == 0 && l0Fused_;
   // Without a pending mean on b the first sweep from zero is pointwise: the
-  // level-0 update and each restriction write it with the right-hand side.
-  const bool firstDone = opts_.pre_smooth >= 1 && (fusedL0 || (l > 0 && !mb.on()));
+  // level-0 update (or r_new's projection) and each restriction write it with
+  // the right-hand side.
+  const bool firstDone = opts_.pre_smooth >= 1 && ((l == 0 && l0First_) || (l > 0 && !mb.on()));
   if (firstDone) {
     cur = 0;
     q0 = 1;
@@ -578,7 +579,7 @@ void PoissonDev::vcycle(int l, const T* b, MeanRef mb, const T** xout, MeanRef* 
 // (multigrid.cpp:283-289).  z's mean is left pending in kSlotMeanZ (zMean()).  With
 // fuseDots (systems without projections) the V-cycle output is left for
 // k_fcg_dots_x, which forms z itself.
-void PoissonDev::minv(const double* r, hipStream_t s, const double* rOld, bool fused, bool b0Ready) {
+void PoissonDev::minv(const double* r, hipStream_t s, const double* rOld, bool fused, bool b0Ready, bool x1Ready) {
   if (tiled_) {  // the first cycle of a solve, or precondition: z = Minv r, z's mean left pending
     if (rOld || fused) throw std::logic_error("minv: the tiled cycle forms iterations itself");
     if (mixed_) {
@@ -596,6 +597,7 @@ void PoissonDev::minv(const double* r, hipStream_t s, const double* rOld, bool f
   double* z = z_.as<double>();
   const bool dots = rOld != nullptr;
   l0Fused_ = fused;
+  l0First_ = fused || x1Ready;
   l0Rn_ = r;
   l0R_ = rOld;
   const bool inlZ = inlineMean(blocks_[0]);
@@ -615,6 +617,7 @@ void PoissonDev::minv(const double* r, hipStream_t s, const double* rOld, bool f
     else if (x) launch_mg_finish<double>(L, x, mx, z, project_, zRed, ctl, s);
   }
   l0Fused_ = false;
+  l0First_ = false;
   if (project_) meanFin(kSlotMeanZ, inlZ, blocks_[0], L.nf, true, s);
   ++vcount_;
 }
@@ -663,12 +666,15 @@ void PoissonDev::enqueueIteration(hipStream_t s, int k) {
       const bool inlM = inlineMean(nb);
       launch_fcg_update(L, true, x, p, r, Ap, rn, ctl, meanRed(kSlotMeanRn, inlM), s, fold(rbP));
       meanFin(kSlotMeanRn, inlM, nb, L.nf, true, s);
-      // the mixed form's FP32 copy of r_new is written beside it
-      launch_fcg_norm(L, rn, meanRef(kSlotMeanRn, inlM, nb, L.nf), ctl, rbR, s,
-                      mixed_ ? b0f_.as<float>() : nullptr);
+      // the mixed form's FP32 copy of r_new is written beside it, and the
+      // level-0 first sweep from zero where a coarser level follows
+      const bool first = nlevels() > 1 && opts_.pre_smooth >= 1;
+      launch_fcg_norm(L, rn, meanRef(kSlotMeanRn, inlM, nb, L.nf), ctl, rbR, s, mixed_ ? b0f_.as<float>() : nullptr,
+                      first && mixed_ ? lx0_[0].as<float>() : nullptr,
+                      first && !mixed_ ? lx0_[0].as<double>() : nullptr);
       if (inline_) conv_ = fold(rbR);
       else fin(FIN_CONV, 1, nb, 0, 0, true, s);
-      minv(rn, s, nullptr, false, mixed_);
+      minv(rn, s, nullptr, false, mixed_, first);
       if (conv_.n) throw std::logic_error("flexible CG: convergence test not placed");
       launch_fcg_dots(L, z, zMean(), rn, r, ctl, rb, s, inline_);
     } else {
@@ -1061,8 +1067,6 @@ void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double
   const double n = double(L.n);
   const double sz = mixed_ ? 4.0 : 8.0;
   SolveCtl* ctl = ctl_.as<SolveCtl>();
-  const MeanRef m0 = project_ ? slotMean(slotPtr(ctl, slotX(0, 0))) : MeanRef{};
-  const MeanRef m1 = project_ ? slotMean(slotPtr(ctl, slotX(0, 1))) : MeanRef{};
   // a live solve leaves ctl->done set; clear it (and every pending mean) so the
   // kernels do their work
   launch_ctl_init(ctl, 0.0, s);
@@ -1102,12 +1106,12 @@ void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double
     if (kernel == 0) {
       if (mixed_)
         launch_mg_sweep<float>(L, b0f_.as<float>(), MeanRef{}, (q & 1) ? lx1_[0].as<float>() : lx0_[0].as<float>(),
-                               (q & 1) ? m1 : m0, false, nullptr, MeanRef{}, nullptr,
-                               (q & 1) ? lx0_[0].as<float>() : lx1_[0].as<float>(), project_, rb, ctl, s);
+                               MeanRef{}, false, nullptr, MeanRef{}, nullptr,
+                               (q & 1) ? lx0_[0].as<float>() : lx1_[0].as<float>(), false, rb, ctl, s);
       else
         launch_mg_sweep<double>(L, r_.as<double>(), MeanRef{}, (q & 1) ? lx1_[0].as<double>() : lx0_[0].as<double>(),
-                                (q & 1) ? m1 : m0, false, nullptr, MeanRef{}, nullptr,
-                                (q & 1) ? lx0_[0].as<double>() : lx1_[0].as<double>(), project_, rb, ctl, s);
+                                MeanRef{}, false, nullptr, MeanRef{}, nullptr,
+                                (q & 1) ? lx0_[0].as<double>() : lx1_[0].as<double>(), false, rb, ctl, s);
     } else if (kernel == 1) {
       launch_fcg_apply(L, p_.as<double>(), Ap_.as<double>(), ctl, rb, s);
     } else if (kernel == 5) {  // beta from SolveCtl (0 after the reset), p_old and p_new alternate
@@ -1148,12 +1152,15 @@ void PoissonDev::profile(int kernel, int reps, hipStream_t s, double* ms, double
         vcycle<double>(lc, lb_[lc].as<double>(), MeanRef{}, &xo, &mo, s);
       }
     } else {
+      // as the per-sweep cycle runs it: the residual is folded only for a
+      // coarsest level solved by CG on a singular system
+      const bool rp = project_ && nlevels() == 2 && !direct_;
       if (mixed_)
-        launch_mg_resid_restrict<float>(L, dl_[1], b0f_.as<float>(), MeanRef{}, lx0_[0].as<float>(), m0,
-                                        lb_[1].as<float>(), project_, rb, ctl, s);
+        launch_mg_resid_restrict<float>(L, dl_[1], b0f_.as<float>(), MeanRef{}, lx0_[0].as<float>(), MeanRef{},
+                                        lb_[1].as<float>(), rp, rb, ctl, s);
       else
-        launch_mg_resid_restrict<double>(L, dl_[1], r_.as<double>(), MeanRef{}, lx0_[0].as<double>(), m0,
-                                         lb_[1].as<double>(), project_, rb, ctl, s);
+        launch_mg_resid_restrict<double>(L, dl_[1], r_.as<double>(), MeanRef{}, lx0_[0].as<double>(), MeanRef{},
+                                         lb_[1].as<double>(), rp, rb, ctl, s);
     }
   };
   one(0);  // warm-up

@@ -120,7 +120,10 @@ class PoissonDev {
   // pass.  fused: the level-0 first sweep (and FP32 right-hand side) were
   // written by the update, and the last level-0 sweep writes z and the dots.
   // b0Ready: the FP32 level-0 right-hand side is already written (mixed)
-  void minv(const double* r, hipStream_t s, const double* rOld, bool fused, bool b0Ready = false);
+  // fused: the level-0 update wrote the first sweep and the last sweep writes
+  // z and the flexible-CG dots; x1Ready: the first sweep alone was written
+  void minv(const double* r, hipStream_t s, const double* rOld, bool fused, bool b0Ready = false,
+            bool x1Ready = false);
   bool buildDirectCoarse();
   void fin(int op, int K, int nparts, int slot, long long nf, bool predicated, hipStream_t s, int earlyExit = 0,
            double convTol = 0.0);
@@ -195,6 +198,7 @@ class PoissonDev {
   DevBuf x_, rhs_, r_, rn_, z_, p_, p2_, Ap_;
   // level-0 fusions of the current V-cycle (set by minv)
   bool l0Fused_ = false;
+  bool l0First_ = false;  // the level-0 first sweep from zero is already in lx0_[0]
   const double* l0Rn_ = nullptr;
   const double* l0R_ = nullptr;
   // Finalizes folded into their consumers for the current solve (flexible CG

@@ -132,12 +132,12 @@ def test_tiled_steps_track_per_sweep(ctx):
 
 
 def test_tiled_taylor_green_steps_track_per_sweep(ctx):
-    """64^3 periodic Taylor-Green vortex (singular, the automatic choice is the
-    tiled cycle): ten steps with either cycle agree to rounding."""
+    """64^3 periodic Taylor-Green vortex (singular): ten steps with either
+    cycle agree to rounding."""
     g, cfg = cases.taylor_green(64)
     st = cases.taylor_green_state(g)
     res = []
-    for vc in (nbl.VCYCLE_PER_SWEEP, nbl.VCYCLE_AUTO):
+    for vc in (nbl.VCYCLE_PER_SWEEP, nbl.VCYCLE_TILED):
         run = nbl.Run(ctx, g, cfg, nbl.solver_opts(vcycle=vc))
         try:
             run.set_state(*st)
@@ -194,15 +194,15 @@ def test_bf16_needs_the_tiled_cycle(ctx):
 @pytest.mark.parametrize("precision", [nbl.PREC_FP64, nbl.PREC_MIXED], ids=["fp64", "mixed"])
 def test_tiled_cycle_beyond_2048_tiles(ctx, orc, precision):
     """A 100^3 periodic box has 2275 level-0 tiles (odd coarse levels 25, 13,
-    7): the automatic choice runs the tiled cycle there, every reduction still
-    folded inline; it solves like the per-sweep cycle and the oracle."""
+    7): the tiled cycle runs there, every reduction still folded inline, and
+    solves like the per-sweep cycle and the oracle."""
     g, _ = cases.taylor_green(100)
     b = cases.poisson_rhs(g, 53)
     out = []
-    for vc in (nbl.VCYCLE_PER_SWEEP, 0):
+    for vc in (nbl.VCYCLE_PER_SWEEP, nbl.VCYCLE_TILED):
         ps = nbl.Poisson(ctx, g, kind=0, tol=1e-10, max_iters=200, opts=nbl.solver_opts(precision=precision, vcycle=vc))
         try:
-            if vc == 0:
+            if vc == nbl.VCYCLE_TILED:
                 ps.profile(4, 1)  # the tiled cycle's down kernel: raises unless that cycle is in use
             out.append(ps.solve(b))
         finally:
