@@ -390,10 +390,11 @@ void launch_fcg_norm(const MgLevel& L, double* rn, MeanRef mrn, const SolveCtl* 
                      float* b0 = nullptr, float* x1f = nullptr, double* x1d = nullptr);
 void launch_fcg_dots(const MgLevel& L, double* z, MeanRef mz, const double* rn, const double* r,
                      SolveCtl* ctl, RedBuf rb, hipStream_t s, bool countIt = false);
-// k_finish fused into k_fcg_dots (no pending means): z = V-cycle output x.
+// k_finish fused into k_fcg_dots: z = V-cycle output x (minus its pending
+// fluid mean mx); countIt: count the iteration (no finalize follows).
 template <class T>
-void launch_fcg_dots_x(const MgLevel& L, const T* x, double* z, const double* rn, const double* r,
-                       const SolveCtl* ctl, RedBuf rb, hipStream_t s);
+void launch_fcg_dots_x(const MgLevel& L, const T* x, MeanRef mx, double* z, const double* rn, const double* r,
+                       SolveCtl* ctl, RedBuf rb, hipStream_t s, bool countIt = false);
 void launch_fluid_sum(const MgLevel& L, const double* x, RedBuf rb, hipStream_t s);
 void launch_fcg_final(const MgLevel& L, bool singular, const double* x, const double* rhs, double* xo,
                       const SolveCtl* ctl, RedBuf rb, hipStream_t s);

@@ -1169,15 +1169,20 @@ __global__ void __launch_bounds__(kBlock) k_fcg_dots(MgLevel L, double* __restri
   if (q.valid) stN<NC>(z, q.c, v);  // after the fold's barriers
 }
 
-// k_finish + k_fcg_dots in one pass for systems without projections: z is the
-// V-cycle output on fluid cells (converted to FP64), 0 elsewhere.
+// k_finish + k_fcg_dots in one pass: z is the V-cycle output on fluid cells
+// (converted to FP64, minus its pending fluid mean on singular systems, as
+// k_finish and k_fcg_dots form it), 0 elsewhere.
 template <class T, int NC>
-__global__ void __launch_bounds__(kBlock) k_fcg_dots_x(MgLevel L, const T* __restrict__ x, double* __restrict__ z,
-                                                       const double* __restrict__ rn, const double* __restrict__ r,
-                                                       const SolveCtl* ctl, RedBuf rb) {
+__global__ void __launch_bounds__(kBlock) k_fcg_dots_x(MgLevel L, const T* __restrict__ x, MeanRef mx,
+                                                       double* __restrict__ z, const double* __restrict__ rn,
+                                                       const double* __restrict__ r, SolveCtl* ctl, RedBuf rb,
+                                                       int countIt) {
   NBL_CELLS(NC);
   ctl = ctlAt(ctl, q.p);
-  if (ldA(&ctl->done)) return;
+  mx = mx.at(q.p);
+  double mv[1];
+  if (stopOrMeans<1>(ctl, {mx}, mv)) return;
+  if (countIt && problemLead(q, L.nz, L.nb) && linearTid() == 0) stA(&ctl->it, ldA(&ctl->it) + 1);
   double acc[2] = {0.0, 0.0};
   double v[NC];
   if (q.valid) {
@@ -1189,6 +1194,7 @@ __global__ void __launch_bounds__(kBlock) k_fcg_dots_x(MgLevel L, const T* __res
 #pragma unroll
     for (int e = 0; e < NC; ++e) {
       v[e] = (code[e] & kFluidBit) ? double(xv[e]) : 0.0;
+      if (mx.on() && (code[e] & kFluidBit)) v[e] = v[e] - mv[0];
       acc[0] += v[e] * a[e];
       acc[1] += v[e] * c[e];
     }
@@ -1793,17 +1799,17 @@ void launch_fcg_dots(const MgLevel& L, double* z, MeanRef mz, const double* rn, 
   NBL_LAUNCH_CHECK();
 }
 template <class T>
-void launch_fcg_dots_x(const MgLevel& L, const T* x, double* z, const double* rn, const double* r,
-                       const SolveCtl* ctl, RedBuf rb, hipStream_t s) {
+void launch_fcg_dots_x(const MgLevel& L, const T* x, MeanRef mx, double* z, const double* rn, const double* r,
+                       SolveCtl* ctl, RedBuf rb, hipStream_t s, bool countIt) {
   withCells(L, [&](auto nc, const Geo3& g) {
-    k_fcg_dots_x<T, decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, x, z, rn, r, ctl, rb);
+    k_fcg_dots_x<T, decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, x, mx, z, rn, r, ctl, rb, countIt ? 1 : 0);
   });
   NBL_LAUNCH_CHECK();
 }
-template void launch_fcg_dots_x<double>(const MgLevel&, const double*, double*, const double*, const double*,
-                                        const SolveCtl*, RedBuf, hipStream_t);
-template void launch_fcg_dots_x<float>(const MgLevel&, const float*, double*, const double*, const double*,
-                                       const SolveCtl*, RedBuf, hipStream_t);
+template void launch_fcg_dots_x<double>(const MgLevel&, const double*, MeanRef, double*, const double*,
+                                        const double*, SolveCtl*, RedBuf, hipStream_t, bool);
+template void launch_fcg_dots_x<float>(const MgLevel&, const float*, MeanRef, double*, const double*,
+                                       const double*, SolveCtl*, RedBuf, hipStream_t, bool);
 void launch_fluid_sum(const MgLevel& L, const double* x, RedBuf rb, hipStream_t s) {
   withCells(L, [&](auto nc, const Geo3& g) {
     k_fluid_sum<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, x, rb);

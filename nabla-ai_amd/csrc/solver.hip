@@ -566,10 +566,18 @@ void PoissonDev::vcycle(int l, const T* b, MeanRef mb, const T** xout, MeanRef* 
       *mxout = MeanRef{};
       return;
     }
+    // the last level-0 sweep folds its fluid sum where the dots kernel removes
+    // z's mean (l0FoldLast_)
+    const bool fl = sp || (l == 0 && l0FoldLast_ && q == opts_.post_smooth - 1);
     launch_mg_sweep<T>(L, b, mb, bufs[cur], ms[cur], false, q == 0 ? ec : nullptr, q == 0 ? mec : MeanRef{}, &Lc,
-                       bufs[o], sp, xRed(o), ctl, s, timedSweep(l, q > 0));
-    if (sp) meanFin(slotX(l, o), inlX, nbl, L.nf, true, s);
+                       bufs[o], fl, fl ? meanRed(slotX(l, o), inlX) : rb, ctl, s, timedSweep(l, q > 0));
+    if (fl) meanFin(slotX(l, o), inlX, nbl, L.nf, true, s);
     cur = o;
+    if (fl && !sp) {
+      *xout = bufs[cur];
+      *mxout = meanRef(slotX(l, cur), inlX, nbl, L.nf);
+      return;
+    }
   }
   *xout = bufs[cur];
   *mxout = ms[cur];
@@ -602,23 +610,27 @@ void PoissonDev::minv(const double* r, hipStream_t s, const double* rOld, bool f
   l0R_ = rOld;
   const bool inlZ = inlineMean(blocks_[0]);
   const RedBuf zRed = project_ ? meanRed(kSlotMeanZ, inlZ) : rb;
+  // singular systems with the flexible-CG dots: the last level-0 sweep folds
+  // its fluid sum and the dots kernel forms z = x - mean (no k_finish)
+  l0FoldLast_ = project_ && dots && !fused;
   if (mixed_) {
     if (!fused && !b0Ready) launch_to_level0<float>(L, r, b0f_.as<float>(), ctl, s);
     const float* x = nullptr;
     MeanRef mx{};
     vcycle<float>(0, b0f_.as<float>(), MeanRef{}, &x, &mx, s);
-    if (x && dots) launch_fcg_dots_x<float>(L, x, z, r, rOld, ctl, rb, s);
+    if (x && dots) launch_fcg_dots_x<float>(L, x, mx, z, r, rOld, ctl, rb, s, project_ && inline_);
     else if (x) launch_mg_finish<float>(L, x, mx, z, project_, zRed, ctl, s);
   } else {
     const double* x = nullptr;
     MeanRef mx{};
     vcycle<double>(0, r, MeanRef{}, &x, &mx, s);
-    if (x && dots) launch_fcg_dots_x<double>(L, x, z, r, rOld, ctl, rb, s);
+    if (x && dots) launch_fcg_dots_x<double>(L, x, mx, z, r, rOld, ctl, rb, s, project_ && inline_);
     else if (x) launch_mg_finish<double>(L, x, mx, z, project_, zRed, ctl, s);
   }
   l0Fused_ = false;
   l0First_ = false;
-  if (project_) meanFin(kSlotMeanZ, inlZ, blocks_[0], L.nf, true, s);
+  l0FoldLast_ = false;
+  if (project_ && !dots) meanFin(kSlotMeanZ, inlZ, blocks_[0], L.nf, true, s);
   ++vcount_;
 }
 
@@ -674,9 +686,8 @@ void PoissonDev::enqueueIteration(hipStream_t s, int k) {
                       first && !mixed_ ? lx0_[0].as<double>() : nullptr);
       if (inline_) conv_ = fold(rbR);
       else fin(FIN_CONV, 1, nb, 0, 0, true, s);
-      minv(rn, s, nullptr, false, mixed_, first);
+      minv(rn, s, r, false, mixed_, first);
       if (conv_.n) throw std::logic_error("flexible CG: convergence test not placed");
-      launch_fcg_dots(L, z, zMean(), rn, r, ctl, rb, s, inline_);
     } else {
       // fusions need a first pre-sweep and a coarser level to smooth above
       const bool fused = nlevels() > 1 && opts_.pre_smooth >= 1;

@@ -199,6 +199,7 @@ class PoissonDev {
   // level-0 fusions of the current V-cycle (set by minv)
   bool l0Fused_ = false;
   bool l0First_ = false;  // the level-0 first sweep from zero is already in lx0_[0]
+  bool l0FoldLast_ = false;  // the last level-0 sweep folds its fluid sum (z's pending mean)
   const double* l0Rn_ = nullptr;
   const double* l0R_ = nullptr;
   // Finalizes folded into their consumers for the current solve (flexible CG
