@@ -4,6 +4,7 @@
 #include "solver.hpp"
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstddef>
 #include <cstring>
@@ -613,24 +614,36 @@ void PoissonDev::minv(const double* r, hipStream_t s, const double* rOld, bool f
   // singular systems with the flexible-CG dots: the last level-0 sweep folds
   // its fluid sum and the dots kernel forms z = x - mean (no k_finish)
   l0FoldLast_ = project_ && dots && !fused;
+  // z from the cycle's output x: with the dots in one pass where x carries
+  // its pending mean (or needs none); otherwise a pass writing z (and, on
+  // singular systems, its fluid sum, a single-level cycle's output having
+  // none folded) before the dots
+  auto finish = [&](const auto* x, MeanRef mx) {
+    using T = std::remove_cv_t<std::remove_pointer_t<decltype(x)>>;
+    if (!x) return;
+    if (dots && (!project_ || mx.on())) {
+      launch_fcg_dots_x<T>(L, x, mx, z, r, rOld, ctl, rb, s, project_ && inline_);
+      return;
+    }
+    launch_mg_finish<T>(L, x, mx, z, project_, zRed, ctl, s);
+    if (project_) meanFin(kSlotMeanZ, inlZ, blocks_[0], L.nf, true, s);
+    if (dots) launch_fcg_dots(L, z, zMean(), r, rOld, ctl, rb, s, inline_);
+  };
   if (mixed_) {
     if (!fused && !b0Ready) launch_to_level0<float>(L, r, b0f_.as<float>(), ctl, s);
     const float* x = nullptr;
     MeanRef mx{};
     vcycle<float>(0, b0f_.as<float>(), MeanRef{}, &x, &mx, s);
-    if (x && dots) launch_fcg_dots_x<float>(L, x, mx, z, r, rOld, ctl, rb, s, project_ && inline_);
-    else if (x) launch_mg_finish<float>(L, x, mx, z, project_, zRed, ctl, s);
+    finish(x, mx);
   } else {
     const double* x = nullptr;
     MeanRef mx{};
     vcycle<double>(0, r, MeanRef{}, &x, &mx, s);
-    if (x && dots) launch_fcg_dots_x<double>(L, x, mx, z, r, rOld, ctl, rb, s, project_ && inline_);
-    else if (x) launch_mg_finish<double>(L, x, mx, z, project_, zRed, ctl, s);
+    finish(x, mx);
   }
   l0Fused_ = false;
   l0First_ = false;
   l0FoldLast_ = false;
-  if (project_ && !dots) meanFin(kSlotMeanZ, inlZ, blocks_[0], L.nf, true, s);
   ++vcount_;
 }
 
