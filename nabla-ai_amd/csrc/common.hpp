@@ -23,7 +23,8 @@ constexpr int kBlock = 256;          // threads per block for streaming kernels
 constexpr int kWave = 64;            // CDNA wavefront
 constexpr int kMaxRedVals = 8;       // values reduced together by one launch
 
-// Work-group sizes read from the implicit kernel arguments (code object v5:
+// Work-group sizes read from the implicit kernel arguments (code object v5
+// and v6, which the build pins:
 // hidden_group_size_x, _y, _z are 16-bit fields at byte offsets 12, 14 and 16
 // of the implicit argument block, which sits in device memory beside the
 // kernel arguments): one scalar load.  In kernels that remap their block
