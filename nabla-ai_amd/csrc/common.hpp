@@ -165,6 +165,32 @@ __device__ __forceinline__ void blockToPartials(double (&v)[K], double* partials
   }
 }
 
+// A thread's strided share of K reductions: a[q] combines ld(q, b) for b = t,
+// t + nt, ... < n in that order (bit q of MAXMASK: max instead of sum).  The
+// loads of kFoldBatch strides of every row are issued before they are
+// combined, so a thread makes one memory round trip per batch instead of one
+// per element (a loop that adds each load as it arrives waits for every one).
+constexpr int kFoldBatch = 8;
+template <int K, unsigned MAXMASK, class LD>
+__device__ __forceinline__ void stridedFold(int t, int nt, int n, const LD& ld, double (&a)[K]) {
+#pragma unroll
+  for (int q = 0; q < K; ++q) a[q] = 0.0;
+  for (int b0 = t; b0 < n; b0 += kFoldBatch * nt) {
+    double v[K][kFoldBatch];
+#pragma unroll
+    for (int u = 0; u < kFoldBatch; ++u) {
+      const int b = b0 + u * nt;
+#pragma unroll
+      for (int q = 0; q < K; ++q) v[q][u] = b < n ? ld(q, b) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kFoldBatch; ++u)
+      if (b0 + u * nt < n)
+#pragma unroll
+        for (int q = 0; q < K; ++q) a[q] = ((MAXMASK >> q) & 1u) ? fmax(a[q], v[q][u]) : a[q] + v[q][u];
+  }
+}
+
 // Fold P partials per value in a fixed order inside one block; the totals are
 // returned in every thread.
 template <int K, unsigned MAXMASK = 0u>
@@ -172,14 +198,8 @@ __device__ __forceinline__ void foldPartials(const double* partials, int stride,
   __shared__ double sh[K * 16];
   __shared__ double res[K];
   double a[K];
-#pragma unroll
-  for (int q = 0; q < K; ++q) {
-    a[q] = 0.0;
-    for (int b = threadIdx.x; b < P; b += wgSizeX()) {
-      const double pv = ldA(&partials[q * stride + b]);
-      a[q] = ((MAXMASK >> q) & 1u) ? fmax(a[q], pv) : a[q] + pv;
-    }
-  }
+  stridedFold<K, MAXMASK>(int(threadIdx.x), wgSizeX(), P,
+                          [&](int q, int b) { return ldA(&partials[q * stride + b]); }, a);
   blockFold<K, MAXMASK>(a, sh);
   if (threadIdx.x == 0)
     for (int q = 0; q < K; ++q) res[q] = a[q];

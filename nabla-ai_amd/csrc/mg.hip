@@ -508,13 +508,33 @@ __global__ void __launch_bounds__(kBlock) k_coarse_direct(MgLevel L, const MT* _
   double acc[ROWS];
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) acc[r] = 0.0;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    T bj = b[j];
-    if (mb.on() && (L.code[j] & kFluidBit)) bj = bj - T(m);
-    const double bv = double(bj);
+  // the loads of four column strides are issued before their products are
+  // accumulated (in column order), one memory round trip per four strides
+  constexpr int CB = 4;
+  const int nt = wgSizeX();
+  for (int j0 = int(threadIdx.x); j0 < n; j0 += CB * nt) {
+    double bv[CB], mv2[CB][ROWS];
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r)
-      if (r0 + r < n) acc[r] = fma(double(M[size_t(r0 + r) * n + j]), bv, acc[r]);
+    for (int u = 0; u < CB; ++u) {
+      const int j = j0 + u * nt;
+      bv[u] = 0.0;
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) mv2[u][r] = 0.0;
+      if (j < n) {
+        T bj = b[j];
+        if (mb.on() && (L.code[j] & kFluidBit)) bj = bj - T(m);
+        bv[u] = double(bj);
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r)
+          if (r0 + r < n) mv2[u][r] = double(M[size_t(r0 + r) * n + j]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < CB; ++u)
+      if (j0 + u * nt < n)
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r)
+          if (r0 + r < n) acc[r] = fma(mv2[u][r], bv[u], acc[r]);
   }
   __shared__ double sh[ROWS * 16];
   blockFold<ROWS, 0u>(acc, sh);

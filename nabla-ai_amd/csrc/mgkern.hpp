@@ -302,11 +302,7 @@ __device__ __forceinline__ void foldInline(const InFold& f, double (&out)[K]) {
   __shared__ double res[K];
   const int tid = linearTid(), nt = wgSizeX() * wgSizeY() * wgSizeZ();
   double a[K];
-#pragma unroll
-  for (int q = 0; q < K; ++q) {
-    a[q] = 0.0;
-    for (int b = tid; b < f.n; b += nt) a[q] += f.parts[q * f.stride + b];
-  }
+  stridedFold<K, 0u>(tid, nt, f.n, [&](int q, int b) { return f.parts[q * f.stride + b]; }, a);
   blockFold<K, 0u>(a, sh);
   if (tid == 0)
     for (int q = 0; q < K; ++q) res[q] = a[q];
@@ -338,9 +334,12 @@ __device__ __forceinline__ bool stopOrMeans(const SolveCtl* ctl, const MeanRef (
   double a[K];
 #pragma unroll
   for (int q = 0; q < K; ++q) {
-    a[q] = 0.0;
-    if (m[q].parts)
-      for (int b = tid; b < m[q].n; b += nt) a[q] += m[q].parts[b];
+    double s[1] = {0.0};
+    if (m[q].parts) {
+      const double* pp = m[q].parts;
+      stridedFold<1, 0u>(tid, nt, m[q].n, [&](int, int b) { return pp[b]; }, s);
+    }
+    a[q] = s[0];
   }
   blockFold<K, 0u>(a, sh);
   if (tid == 0) {
@@ -377,11 +376,7 @@ __device__ __forceinline__ void foldStop(const InFold& f, const SolveCtl* ctl, c
     if (e1) x1 = ldA(e1);
   }
   double a[K];
-#pragma unroll
-  for (int q = 0; q < K; ++q) {
-    a[q] = 0.0;
-    for (int b = tid; b < f.n; b += nt) a[q] += f.parts[q * f.stride + b];
-  }
+  stridedFold<K, 0u>(tid, nt, f.n, [&](int q, int b) { return f.parts[q * f.stride + b]; }, a);
   blockFold<K, 0u>(a, sh);
   if (tid == 0) {
     for (int q = 0; q < K; ++q) res[q] = a[q];
