@@ -171,6 +171,19 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc
   const int c = i + L.nx * (j + L.ny * (k + L.nz * p));
   uint16_t code[NC];
   T xc[NC], xn[NC][6], bv[NC];
+  // the coarse cell this thread sums (one thread per coarse cell, the first
+  // of its children), and its code, loaded with the fine stencil so the sum
+  // after the barrier waits for no further memory round trip
+  const int fxT = NC == 2 ? 1 : Lc.fx;  // x-children held by one thread: 2 (NC = 2) or 1
+  int Cst = -1;
+  uint16_t cc = 0;
+  if (tz == 0 && tx % fxT == 0 && ty % Lc.fy == 0) {
+    const int I = i / Lc.fx, J = j / Lc.fy, K = k / Lc.fz;
+    if (I < Lc.nx && J < Lc.ny && K < Lc.nz) {
+      Cst = I + Lc.nx * (J + Lc.ny * (K + Lc.nz * p));
+      cc = Lc.code[Cst];
+    }
+  }
   // a restriction that carries the convergence test: as k_sweep
   double mv[2] = {0.0, 0.0};
   if (!conv.n && stopOrMeans<2>(ctl, {mb, mx}, mv)) return;
@@ -212,15 +225,11 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(MgLevel L, MgLevel Lc
   }
   __syncthreads();
   double acc[1] = {0.0};
-  const int fxT = NC == 2 ? 1 : Lc.fx;  // x-children held by one thread: 2 (NC = 2) or 1
-  int Cst = -1;  // the coarse cell this thread writes, stored after the fold
-  T rcv = T(0), x1v = T(0);
-  if (tz == 0 && tx % fxT == 0 && ty % Lc.fy == 0) {
+  T rcv = T(0), x1v = T(0);  // the coarse cell's values, stored after the fold
+  if (Cst >= 0) {
     const int I = i / Lc.fx, J = j / Lc.fy, K = k / Lc.fz;
-    if (I < Lc.nx && J < Lc.ny && K < Lc.nz) {
-      const int C = I + Lc.nx * (J + Lc.ny * (K + Lc.nz * p));
-      Cst = C;
-      const uint16_t cc = Lc.code[C];
+    const int C = Cst;
+    {
       if (cc & kFluidBit) {
         T s = T(0);
         int cnt = 0;
