@@ -69,6 +69,32 @@ __device__ __forceinline__ int sameLevel(const TreeV& T, int c, int d) {
   return (T.lev[nb] == T.lev[c] && T.mask[nb] == 0) ? nb : -1;
 }
 
+// WENO-5 line of every first-neighbour face entry (d, i) whose neighbour is one
+// fluid leaf of i's level: (m2, m1, p1, p2), the two same-level fluid leaves
+// behind i and beyond the neighbour along the face normal, or -1s where the
+// line is incomplete (k_of_rhs then uses the upwind value).  The tree and its
+// masks are fixed for a flow's life, so the table is built once; it replaces
+// four dependent neighbour-of-neighbour lookups per face in every stage.
+__global__ void k_of_wenoline(TreeV T, int4* __restrict__ wl) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 6 * T.n) return;
+  const int d = int(t / T.n);
+  const long long i = t - (long long)d * T.n;
+  int4 w = make_int4(-1, -1, -1, -1);
+  if (T.mask[i] == 0 && T.F.cnt[6 * i + d] == 1) {
+    const int nb = T.F.first[(long long)d * T.n + i];
+    if (T.mask[nb] == 0 && T.lev[nb] == T.lev[i]) {
+      const int od = d ^ 1;
+      const int m1 = sameLevel(T, int(i), od);
+      const int m2 = sameLevel(T, m1, od);
+      const int p1 = sameLevel(T, nb, d);
+      const int p2 = sameLevel(T, p1, d);
+      if (m2 >= 0 && p2 >= 0 && m1 >= 0 && p1 >= 0) w = make_int4(m2, m1, p1, p2);
+    }
+  }
+  wl[t] = w;
+}
+
 // Walks leaf i's faces: f(d, nb, entry) with nb < 0 for a domain-boundary face.
 template <class Fn>
 __device__ __forceinline__ void eachFace(const TreeV& T, long long i, Fn&& f) {
@@ -279,7 +305,8 @@ __global__ void __launch_bounds__(kBlock) k_of_rhs(TreeV T, Bc B, const double* 
                                                    const double* __restrict__ U, const double* __restrict__ u0,
                                                    const double* __restrict__ v0, const double* __restrict__ w0,
                                                    double a, double b, double dt, double* __restrict__ ou,
-                                                   double* __restrict__ ov, double* __restrict__ ow) {
+                                                   double* __restrict__ ov, double* __restrict__ ow,
+                                                   const int4* __restrict__ wl) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= T.n) return;
   if (T.mask[i] != 0) {
@@ -336,11 +363,11 @@ __global__ void __launch_bounds__(kBlock) k_of_rhs(TreeV T, Bc B, const double* 
     const double Uf = U[e];
     int m1 = -1, m2 = -1, p1 = -1, p2 = -1;
     if (B.weno && ln == li && T.F.cnt[6 * i + d] == 1) {
-      const int od = d ^ 1;
-      m1 = sameLevel(T, int(i), od);
-      m2 = sameLevel(T, m1, od);
-      p1 = sameLevel(T, nb, d);
-      p2 = sameLevel(T, p1, d);
+      const int4 w = wl[(long long)d * T.n + i];  // (k_of_wenoline)
+      m2 = w.x;
+      m1 = w.y;
+      p1 = w.z;
+      p2 = w.w;
     }
     const bool line = m2 >= 0 && p2 >= 0 && m1 >= 0 && p1 >= 0;
     for (int c = 0; c < 3; ++c) {
@@ -439,6 +466,12 @@ OctFlowDev::OctFlowDev(nbl_ctx* ctx, OctreeDev& tree, const nbl_octree_flow_cfg&
   NBL_HIP(hipMemsetAsync(scal_.as<void>(), 0, 16 * 8, s_));
   NBL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hscal_), 16 * 8, hipHostMallocDefault));
   std::memset(hscal_, 0, 16 * 8);
+  if (bcOf(cfg_).weno) {  // WENO-5 lines of the fixed tree
+    wl_ = DevBuf(ctx, 6 * n_ * sizeof(int4));
+    const TreeV T = treeView(tree_);
+    k_of_wenoline<<<gridFor(6 * T.n), kBlock, 0, s_>>>(T, wl_.as<int4>());
+    NBL_LAUNCH_CHECK();
+  }
   faceInit();
   NBL_HIP(hipStreamSynchronize(s_));
 }
@@ -535,7 +568,7 @@ nbl_octree_step_report OctFlowDev::step(double dtIn) {
     k_of_rhs<<<gridFor(N), kBlock, 0, s_>>>(T, B, u_[0].as<double>(), u_[1].as<double>(), u_[2].as<double>(),
                                             U_.as<double>(), u0_[0].as<double>(), u0_[1].as<double>(),
                                             u0_[2].as<double>(), ca[st], cb[st], dt, rhs_[0].as<double>(),
-                                            rhs_[1].as<double>(), rhs_[2].as<double>());
+                                            rhs_[1].as<double>(), rhs_[2].as<double>(), wl_.as<int4>());
     NBL_LAUNCH_CHECK();
     for (int c = 0; c < 3; ++c) std::swap(u_[c], rhs_[c]);
     // The stage's velocity carries its Euler step with weight cb, so it is

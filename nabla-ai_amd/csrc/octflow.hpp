@@ -34,6 +34,7 @@ class OctFlowDev {
   hipStream_t s_;
   size_t n_;
   DevBuf u_[3], u0_[3], rhs_[3], p_, b_, U_, red_, scal_;
+  DevBuf wl_;  // WENO-5 line (m2, m1, p1, p2) per first-neighbour face entry (WENO-5 runs)
   double* hscal_ = nullptr;  // pinned
   int step_ = 0;
   double t_ = 0.0;
