@@ -1075,8 +1075,9 @@ __global__ void __launch_bounds__(kBlock) k_fcg_update(MgLevel L, int singular, 
   }
 }
 
-// Non-singular systems: k_fcg_update plus the first smoother sweep of the
-// next V-cycle from a zero guess, which is pointwise (x1 = 0.8 rn / diag on
+// k_fcg_update without r_new's projection (a non-singular system's r_new
+// has none; a singular one's stays mean-free to rounding) plus the first
+// smoother sweep of the next V-cycle from a zero guess, which is pointwise (x1 = 0.8 rn / diag on
 // fluid cells, k_sweep<SW_ZERO>'s expression), and for an FP32 V-cycle the
 // level-0 right-hand side float(rn); partials of |rn|^2.
 template <class T, int NC>

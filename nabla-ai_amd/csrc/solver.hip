@@ -688,15 +688,27 @@ void PoissonDev::enqueueIteration(hipStream_t s, int k) {
     double* r = (k & 1) ? rn_.as<double>() : r_.as<double>();
     double* rn = (k & 1) ? r_.as<double>() : rn_.as<double>();
     if (project_) {
-      const bool inlM = inlineMean(nb);
-      launch_fcg_update(L, true, x, p, r, Ap, rn, ctl, meanRed(kSlotMeanRn, inlM), s, fold(rbP));
-      meanFin(kSlotMeanRn, inlM, nb, L.nf, true, s);
-      // the mixed form's FP32 copy of r_new is written beside it, and the
-      // level-0 first sweep from zero where a coarser level follows
       const bool first = nlevels() > 1 && opts_.pre_smooth >= 1;
-      launch_fcg_norm(L, rn, meanRef(kSlotMeanRn, inlM, nb, L.nf), ctl, rbR, s, mixed_ ? b0f_.as<float>() : nullptr,
-                      first && mixed_ ? lx0_[0].as<float>() : nullptr,
-                      first && !mixed_ ? lx0_[0].as<double>() : nullptr);
+      if (first) {
+        // The right-hand side is projected once at the start and A annihilates
+        // constants on fluid cells and is symmetric, so r - alpha Ap stays
+        // mean-free up to rounding: r_new needs no projection of its own, and
+        // the update writes |r_new|^2, the mixed form's FP32 copy and the
+        // level-0 first sweep from zero as on non-singular systems.
+        if (mixed_)
+          launch_fcg_update_first<float>(L, x, p, r, Ap, rn, lx0_[0].as<float>(), b0f_.as<float>(), ctl, rbR, s,
+                                         fold(rbP));
+        else
+          launch_fcg_update_first<double>(L, x, p, r, Ap, rn, lx0_[0].as<double>(), nullptr, ctl, rbR, s,
+                                          fold(rbP));
+      } else {
+        // a single-level cycle: r_new's projection rides on the norm kernel
+        const bool inlM = inlineMean(nb);
+        launch_fcg_update(L, true, x, p, r, Ap, rn, ctl, meanRed(kSlotMeanRn, inlM), s, fold(rbP));
+        meanFin(kSlotMeanRn, inlM, nb, L.nf, true, s);
+        launch_fcg_norm(L, rn, meanRef(kSlotMeanRn, inlM, nb, L.nf), ctl, rbR, s,
+                        mixed_ ? b0f_.as<float>() : nullptr, nullptr, nullptr);
+      }
       if (inline_) conv_ = fold(rbR);
       else fin(FIN_CONV, 1, nb, 0, 0, true, s);
       minv(rn, s, r, false, mixed_, first);
