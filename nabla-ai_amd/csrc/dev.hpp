@@ -375,13 +375,15 @@ void launch_fcg_update_first(const MgLevel& L, double* x, const double* p, const
                              double* rn, T* x1, T* b0, SolveCtl* ctl, RedBuf rb, hipStream_t s, InFold alpha = {});
 // p_new = z + beta p_old and A p_new, p_new.Ap in one pass.  beta: the
 // flexible beta folded in from the partials of z.r_new and z.r.
+// zMeanNf > 0: beta's inline fold carries a third row, z's fluid sum, and
+// p_new = (z - sum / zMeanNf) + beta p_old on fluid cells.
 void launch_fcg_apply_pu(const MgLevel& L, const double* z, const double* pOld, double* pNew, double* Ap,
-                         SolveCtl* ctl, RedBuf rb, hipStream_t s, InFold beta = {});
+                         SolveCtl* ctl, RedBuf rb, hipStream_t s, InFold beta = {}, double zMeanNf = 0.0);
 // Last level-0 sweep of a V-cycle writing z and the partials of z.rn, z.r;
 // countIt: the sweep also counts the finished iteration (block 0).
 template <class T>
 void launch_mg_sweep_dots(const MgLevel& L, const T* b, const T* x, double* z, const double* rn, const double* r,
-                          RedBuf rb, SolveCtl* ctl, hipStream_t s, bool countIt = false);
+                          RedBuf rb, SolveCtl* ctl, hipStream_t s, bool countIt = false, bool zSum = false);
 // rn <- rn - mean (fluid) / 0 (solid), partials of |rn|^2.
 // b0 (optional): also write float(r_new) there, the FP32 V-cycle's right-hand side
 // x1f / x1d (optional, at most one): the level-0 first sweep from a zero guess on

@@ -1003,15 +1003,39 @@ struct PNewView {
   }
 };
 
-template <int NC>
+// The same with z's fluid mean m still pending (singular systems whose last
+// level-0 sweep folded z's fluid sum beside the dots): p_new = (z - m) + beta
+// p_old, the projection's and the update's expressions in their order.  A
+// non-fluid neighbour's value is never read by the operator row; a non-fluid
+// centre's p_new and A p are zero, as projected z would give.
+struct PNewMeanView {
+  const double* __restrict__ z;
+  const double* __restrict__ p;
+  double be, m;
+  __device__ __forceinline__ double at(int c, int, int, int) const { return (z[c] - m) + be * p[c]; }
+  template <int NC>
+  __device__ __forceinline__ void pack(int c, int, int, int, double (&v)[NC]) const {
+    double zv[NC], pv[NC];
+    ldN<NC>(z, c, zv);
+    ldN<NC>(p, c, pv);
+#pragma unroll
+    for (int e = 0; e < NC; ++e) v[e] = (zv[e] - m) + be * pv[e];
+  }
+};
+
+// MZ: beta's partials carry a third row, z's fluid sum (inline folds only;
+// nf = the fluid cell count).
+template <int NC, bool MZ>
 __global__ void __launch_bounds__(kBlock) k_fcg_apply_pu(MgLevel L, const double* __restrict__ z,
                                                          const double* __restrict__ pOld, double* __restrict__ pNew,
                                                          double* __restrict__ Ap, SolveCtl* ctl, RedBuf rb,
-                                                         InFold bf) {
+                                                         InFold bf, double nf) {
   NBL_CELLS(NC);
   ctl = ctlAt(ctl, q.p);  // (batches fold no partials inline: bf.n == 0)
-  double be;
-  if (bf.n) {
+  double be, m = 0.0;
+  if (MZ) {
+    if (!inlineBetaMean(bf, ctl, nf, be, m)) return;
+  } else if (bf.n) {
     if (!inlineBeta(bf, ctl, be)) return;
   } else {
     // `done` is not written by this kernel: the per-thread read is uniform
@@ -1021,7 +1045,14 @@ __global__ void __launch_bounds__(kBlock) k_fcg_apply_pu(MgLevel L, const double
   double acc[1] = {0.0};
   double pv[NC], av[NC];
   if (q.valid) {
-    applyN<NC>(L, q, code, PNewView{z, pOld, be}, pv, av);
+    if constexpr (MZ) {
+      applyN<NC>(L, q, code, PNewMeanView{z, pOld, be, m}, pv, av);
+#pragma unroll
+      for (int e = 0; e < NC; ++e)
+        if (!(code[e] & kFluidBit)) pv[e] = av[e] = 0.0;
+    } else {
+      applyN<NC>(L, q, code, PNewView{z, pOld, be}, pv, av);
+    }
 #pragma unroll
     for (int e = 0; e < NC; ++e) acc[0] += pv[e] * av[e];
   }
@@ -1234,10 +1265,12 @@ __global__ void __launch_bounds__(kBlock) k_fcg_dots_x(MgLevel L, const T* __res
 }
 
 // The last level-0 sweep of a V-cycle with the preconditioner output and the
-// flexible-CG dots folded in (non-singular systems): z = the sweep's result
-// (FP64) on fluid cells, 0 elsewhere; partials of z.rn and z.r.  Same values
-// as k_sweep<SW_PLAIN> followed by k_fcg_dots_x.
-template <class T, int NC>
+// flexible-CG dots folded in: z = the sweep's result (FP64) on fluid cells, 0
+// elsewhere; partials of z.rn and z.r.  Same values as k_sweep<SW_PLAIN>
+// followed by k_fcg_dots_x.  ZS (singular systems): a third row of partials
+// holds z's fluid sum, and z's mean is left pending for the next A p; r_new
+// and r are mean-free to rounding, so z.rn and z.r equal the projected z's.
+template <class T, int NC, bool ZS>
 __global__ void __launch_bounds__(kBlock) k_sweep_dots(MgLevel L, const T* __restrict__ b, const T* __restrict__ x,
                                                        double* __restrict__ z, const double* __restrict__ rn,
                                                        const double* __restrict__ r, RedBuf rb, SolveCtl* ctl,
@@ -1246,7 +1279,8 @@ __global__ void __launch_bounds__(kBlock) k_sweep_dots(MgLevel L, const T* __res
   ctl = ctlAt(ctl, q.p);
   if (ldA(&ctl->done)) return;
   if (countIt && problemLead(q, L.nz, L.nb) && linearTid() == 0) stA(&ctl->it, ldA(&ctl->it) + 1);
-  double acc[2] = {0.0, 0.0};
+  constexpr int K = ZS ? 3 : 2;
+  double acc[K] = {};
   double v[NC];
   if (q.valid) {
     T xc[NC], xn[NC][6], bv[NC];
@@ -1266,9 +1300,10 @@ __global__ void __launch_bounds__(kBlock) k_sweep_dots(MgLevel L, const T* __res
       }
       acc[0] += v[e] * a[e];
       acc[1] += v[e] * c[e];
+      if constexpr (ZS) acc[K - 1] += v[e];
     }
   }
-  blockToPartials<2>(acc, rb.partials, rb.stride, q.blk);
+  blockToPartials<K>(acc, rb.partials, rb.stride, q.blk);
   if (q.valid) stN<NC>(z, q.c, v);  // after the fold's barriers
 }
 
@@ -1792,24 +1827,35 @@ template void launch_fcg_update_first<double>(const MgLevel&, double*, const dou
 template void launch_fcg_update_first<float>(const MgLevel&, double*, const double*, const double*, const double*,
                                              double*, float*, float*, SolveCtl*, RedBuf, hipStream_t, InFold);
 void launch_fcg_apply_pu(const MgLevel& L, const double* z, const double* pOld, double* pNew, double* Ap,
-                         SolveCtl* ctl, RedBuf rb, hipStream_t s, InFold beta) {
+                         SolveCtl* ctl, RedBuf rb, hipStream_t s, InFold beta, double zMeanNf) {
+  if (zMeanNf > 0.0 && !beta.n) throw std::logic_error("apply_pu: a pending z mean needs inline folds");
   withCells(L, [&](auto nc, const Geo3& g) {
-    k_fcg_apply_pu<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, z, pOld, pNew, Ap, ctl, rb, beta);
+    if (zMeanNf > 0.0)
+      k_fcg_apply_pu<decltype(nc)::value, true><<<g.grid, g.block, 0, s>>>(L, z, pOld, pNew, Ap, ctl, rb, beta,
+                                                                          zMeanNf);
+    else
+      k_fcg_apply_pu<decltype(nc)::value, false><<<g.grid, g.block, 0, s>>>(L, z, pOld, pNew, Ap, ctl, rb, beta,
+                                                                           0.0);
   });
   NBL_LAUNCH_CHECK();
 }
 template <class T>
 void launch_mg_sweep_dots(const MgLevel& L, const T* b, const T* x, double* z, const double* rn, const double* r,
-                          RedBuf rb, SolveCtl* ctl, hipStream_t s, bool countIt) {
+                          RedBuf rb, SolveCtl* ctl, hipStream_t s, bool countIt, bool zSum) {
   withCells(L, [&](auto nc, const Geo3& g) {
-    k_sweep_dots<T, decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, b, x, z, rn, r, rb, ctl, countIt ? 1 : 0);
+    if (zSum)
+      k_sweep_dots<T, decltype(nc)::value, true><<<g.grid, g.block, 0, s>>>(L, b, x, z, rn, r, rb, ctl,
+                                                                           countIt ? 1 : 0);
+    else
+      k_sweep_dots<T, decltype(nc)::value, false><<<g.grid, g.block, 0, s>>>(L, b, x, z, rn, r, rb, ctl,
+                                                                            countIt ? 1 : 0);
   });
   NBL_LAUNCH_CHECK();
 }
 template void launch_mg_sweep_dots<double>(const MgLevel&, const double*, const double*, double*, const double*,
-                                           const double*, RedBuf, SolveCtl*, hipStream_t, bool);
+                                           const double*, RedBuf, SolveCtl*, hipStream_t, bool, bool);
 template void launch_mg_sweep_dots<float>(const MgLevel&, const float*, const float*, double*, const double*,
-                                          const double*, RedBuf, SolveCtl*, hipStream_t, bool);
+                                          const double*, RedBuf, SolveCtl*, hipStream_t, bool, bool);
 void launch_fcg_norm(const MgLevel& L, double* rn, MeanRef mrn, const SolveCtl* ctl, RedBuf rb, hipStream_t s,
                      float* b0, float* x1f, double* x1d) {
   if (x1f && x1d) throw std::logic_error("fcg_norm: one first-sweep output");

@@ -451,5 +451,22 @@ __device__ __forceinline__ bool inlineBeta(const InFold& f, SolveCtl* ctl, doubl
   return true;
 }
 
+// FIN_BETA_FLEX with z's fluid sum folded beside the dots in a third row (z's
+// mean pending): m = sum / nf, the mean the projection would remove.
+__device__ __forceinline__ bool inlineBetaMean(const InFold& f, SolveCtl* ctl, double nf, double& be, double& m) {
+  double t[3], ex[2];
+  bool stop;
+  foldStop<3>(f, ctl, &ctl->rzq[(f.k - 1) & 1], nullptr, t, ex, stop);
+  if (stop) return false;
+  be = (t[0] - t[1]) / ex[0];
+  m = t[2] / nf;
+  if (firstBlock() && linearTid() == 0) {
+    stA(&ctl->beta, be);
+    stA(&ctl->rz, t[0]);
+    stA(&ctl->rzq[f.k & 1], t[0]);
+  }
+  return true;
+}
+
 }  // namespace
 }  // namespace nbl

@@ -562,7 +562,10 @@ void PoissonDev::vcycle(int l, const T* b, MeanRef mb, const T** xout, MeanRef* 
     const int o = 1 - cur;
     if (fusedL0 && q == opts_.post_smooth - 1 && q > 0) {
       // last sweep: writes z and the flexible-CG dots instead of x
-      launch_mg_sweep_dots<T>(L, b, bufs[cur], z_.as<double>(), l0Rn_, l0R_, rb, ctl, s, inline_);
+      const bool zs = project_;  // (only with zDotsMean(): minv fuses singular dots then only)
+      launch_mg_sweep_dots<T>(L, b, bufs[cur], z_.as<double>(), l0Rn_, l0R_,
+                              zs ? RedBuf{rb.partials + kZDotsRow * size_t(rb.stride), rb.stride} : rb, ctl, s,
+                              inline_, zs);
       *xout = nullptr;
       *mxout = MeanRef{};
       return;
@@ -605,6 +608,7 @@ void PoissonDev::minv(const double* r, hipStream_t s, const double* rOld, bool f
   const RedBuf rb = red();
   double* z = z_.as<double>();
   const bool dots = rOld != nullptr;
+  if (fused && project_ && !zDotsMean()) throw std::logic_error("minv: singular fused dots need inline folds");
   l0Fused_ = fused;
   l0First_ = fused || x1Ready;
   l0Rn_ = r;
@@ -681,7 +685,11 @@ void PoissonDev::enqueueIteration(hipStream_t s, int k) {
     launch_fcg_apply(L, p, Ap, ctl, rbP, s);
   } else {
     const double* pOld = (k & 1) ? p_.as<double>() : p2_.as<double>();
-    launch_fcg_apply_pu(L, z, pOld, p, Ap, ctl, rbP, s, fold(rb));
+    if (zDotsMean())
+      launch_fcg_apply_pu(L, z, pOld, p, Ap, ctl, rbP, s,
+                          InFold{rb.partials + kZDotsRow * size_t(rb.stride), rb.stride, nb, k}, double(L.nf));
+    else
+      launch_fcg_apply_pu(L, z, pOld, p, Ap, ctl, rbP, s, fold(rb));
   }
   if (!inline_) fin(FIN_ALPHA, 1, nb, 0, 0, true, s);
   if (kind_ == 0) {
@@ -711,7 +719,9 @@ void PoissonDev::enqueueIteration(hipStream_t s, int k) {
       }
       if (inline_) conv_ = fold(rbR);
       else fin(FIN_CONV, 1, nb, 0, 0, true, s);
-      minv(rn, s, r, false, mixed_, first);
+      // with inline folds the last level-0 sweep writes z and the dots, z's
+      // mean left for the next A p (zDotsMean); otherwise k_fcg_dots_x
+      minv(rn, s, r, zDotsMean() && first, mixed_, first);
       if (conv_.n) throw std::logic_error("flexible CG: convergence test not placed");
     } else {
       // fusions need a first pre-sweep and a coarser level to smooth above

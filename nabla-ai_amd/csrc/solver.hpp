@@ -200,6 +200,11 @@ class PoissonDev {
   bool l0Fused_ = false;
   bool l0First_ = false;  // the level-0 first sweep from zero is already in lx0_[0]
   bool l0FoldLast_ = false;  // the last level-0 sweep folds its fluid sum (z's pending mean)
+  // Singular systems with inline folds: the fused last level-0 sweep writes z
+  // with its mean pending (partial rows kZDotsRow.., z.rn, z.r, fluid sum) and
+  // the next A p removes it.
+  bool zDotsMean() const { return project_ && inline_ && dl_[0].nf > 0; }
+  static constexpr int kZDotsRow = 4;
   const double* l0Rn_ = nullptr;
   const double* l0R_ = nullptr;
   // Finalizes folded into their consumers for the current solve (flexible CG
