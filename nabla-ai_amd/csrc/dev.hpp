@@ -357,10 +357,15 @@ void launch_prolong_plain(const MgLevel& L, const MgLevel& Lc, const double* ec,
 
 // Flexible CG (MgPoisson::solve) and Jacobi CG (CgPoisson::solve) pieces.
 // Each reducing launcher writes one partial per block of levelGeo(level 0).
+// rhs = masked b, x = 0, partials of rhs's fluid sum; with r (no projection):
+// r = rhs and partials of |rhs|^2 instead.  b0 / x1f / x1d (optional, need r
+// or init2): the first V-cycle's FP32 right-hand side and its level-0 first
+// sweep from zero.
 void launch_fcg_init(const MgLevel& L, bool maskSolid, const double* b, double* rhs, double* x, RedBuf rb,
-                     hipStream_t s);
+                     hipStream_t s, double* r = nullptr, float* b0 = nullptr, float* x1f = nullptr,
+                     double* x1d = nullptr);
 void launch_fcg_init2(const MgLevel& L, bool singular, double* rhs, double* r, const SolveCtl* ctl, RedBuf rb,
-                      hipStream_t s);
+                      hipStream_t s, float* b0 = nullptr, float* x1f = nullptr, double* x1d = nullptr);
 void launch_fcg_start(const MgLevel& L, double* z, MeanRef mz, const double* r, double* p,
                       const SolveCtl* ctl, RedBuf rb, hipStream_t s);
 void launch_fcg_apply(const MgLevel& L, const double* p, double* Ap, const SolveCtl* ctl, RedBuf rb,

@@ -880,9 +880,34 @@ enum : int { S_MEAN_RHS = 0, S_MEAN_RN = 1, S_MEAN_X = 2 };
   uint16_t code[NC];                                                \
   if (q.valid) ldCodes<NC>(L.code, q.c, code)
 
-template <int NC>
+// The first V-cycle's level-0 inputs beside r: its FP32 right-hand side
+// (b0, the mixed form) and its first sweep from a zero guess (x1), each
+// optional, as k_to_level0 and k_sweep<SW_ZERO> write them.
+template <class T, int NC>
+__device__ __forceinline__ void firstCycleInputs(const MgLevel& L, const CellN<NC>& q, const uint16_t (&code)[NC],
+                                                 const double (&v)[NC], float* b0, T* x1) {
+  if (b0) {
+    float f[NC];
+#pragma unroll
+    for (int e = 0; e < NC; ++e) f[e] = float(v[e]);
+    stN<NC>(b0, q.c, f);
+  }
+  if (x1) {
+    T fv[NC];
+#pragma unroll
+    for (int e = 0; e < NC; ++e) fv[e] = firstSweep<T>(L, code[e], q.c + e, q.i + e, q.j, q.k, T(v[e]));
+    stN<NC>(x1, q.c, fv);
+  }
+}
+
+// rhs = b (0 on solid cells where maskSolid), x = 0, partials of rhs's fluid
+// sum.  With r (systems without projection): also r = rhs, the partials then
+// |rhs|^2, and the first cycle's inputs (b0, x1) beside them.
+template <class T, int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_init(MgLevel L, int maskSolid, const double* __restrict__ b,
-                                                     double* __restrict__ rhs, double* __restrict__ x, RedBuf rb) {
+                                                     double* __restrict__ rhs, double* __restrict__ x,
+                                                     double* __restrict__ r, float* __restrict__ b0,
+                                                     T* __restrict__ x1, RedBuf rb) {
   NBL_CELLS(NC);
   double acc[1] = {0.0};
   if (q.valid) {
@@ -893,18 +918,24 @@ __global__ void __launch_bounds__(kBlock) k_fcg_init(MgLevel L, int maskSolid, c
       const bool fl = code[e] & kFluidBit;
       v[e] = (fl || !maskSolid) ? bv[e] : 0.0;
       z[e] = 0.0;
-      acc[0] += fl ? v[e] : 0.0;
+      acc[0] += r ? v[e] * v[e] : (fl ? v[e] : 0.0);
     }
     stN<NC>(rhs, q.c, v);
     stN<NC>(x, q.c, z);
+    if (r) {
+      stN<NC>(r, q.c, v);
+      firstCycleInputs<T, NC>(L, q, code, v, b0, x1);
+    }
   }
   blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
 
-// rhs <- rhs - mean (fluid) / 0 (solid) on singular systems; r = rhs; |rhs|^2.
-template <int NC>
+// rhs <- rhs - mean (fluid) / 0 (solid) on singular systems; r = rhs; |rhs|^2;
+// the first cycle's inputs (b0, x1) beside them.
+template <class T, int NC>
 __global__ void __launch_bounds__(kBlock) k_fcg_init2(MgLevel L, int singular, double* __restrict__ rhs,
-                                                      double* __restrict__ r, const SolveCtl* ctl, RedBuf rb) {
+                                                      double* __restrict__ r, const SolveCtl* ctl, RedBuf rb,
+                                                      float* __restrict__ b0, T* __restrict__ x1) {
   NBL_CELLS(NC);
   ctl = ctlAt(ctl, q.p);
   double acc[1] = {0.0};
@@ -920,6 +951,7 @@ __global__ void __launch_bounds__(kBlock) k_fcg_init2(MgLevel L, int singular, d
     stN<NC>(r, q.c, v);
 #pragma unroll
     for (int e = 0; e < NC; ++e) acc[0] += v[e] * v[e];
+    firstCycleInputs<T, NC>(L, q, code, v, b0, x1);
   }
   blockToPartials<1>(acc, rb.partials, rb.stride, q.blk);
 }
@@ -1779,16 +1811,29 @@ void launch_prolong_plain(const MgLevel& L, const MgLevel& Lc, const double* ec,
 }
 
 void launch_fcg_init(const MgLevel& L, bool maskSolid, const double* b, double* rhs, double* x, RedBuf rb,
-                     hipStream_t s) {
+                     hipStream_t s, double* r, float* b0, float* x1f, double* x1d) {
+  if (x1f && x1d) throw std::logic_error("fcg_init: one first-sweep output");
+  if (!r && (b0 || x1f || x1d)) throw std::logic_error("fcg_init: first-cycle inputs need r");
   withCells(L, [&](auto nc, const Geo3& g) {
-    k_fcg_init<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, maskSolid ? 1 : 0, b, rhs, x, rb);
+    if (x1f)
+      k_fcg_init<float, decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, maskSolid ? 1 : 0, b, rhs, x, r, b0, x1f,
+                                                                       rb);
+    else
+      k_fcg_init<double, decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, maskSolid ? 1 : 0, b, rhs, x, r, b0,
+                                                                        x1d, rb);
   });
   NBL_LAUNCH_CHECK();
 }
 void launch_fcg_init2(const MgLevel& L, bool singular, double* rhs, double* r, const SolveCtl* ctl, RedBuf rb,
-                      hipStream_t s) {
+                      hipStream_t s, float* b0, float* x1f, double* x1d) {
+  if (x1f && x1d) throw std::logic_error("fcg_init2: one first-sweep output");
   withCells(L, [&](auto nc, const Geo3& g) {
-    k_fcg_init2<decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, singular ? 1 : 0, rhs, r, ctl, rb);
+    if (x1f)
+      k_fcg_init2<float, decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, singular ? 1 : 0, rhs, r, ctl, rb, b0,
+                                                                        x1f);
+    else
+      k_fcg_init2<double, decltype(nc)::value><<<g.grid, g.block, 0, s>>>(L, singular ? 1 : 0, rhs, r, ctl, rb, b0,
+                                                                         x1d);
   });
   NBL_LAUNCH_CHECK();
 }

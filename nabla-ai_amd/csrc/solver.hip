@@ -1044,14 +1044,24 @@ nbl_poisson_report PoissonDev::solve(const double* d_b, double* d_x, hipStream_t
   double* z = z_.as<double>();
   double* p = p_.as<double>();
   const bool mg = kind_ == 0;
-  // MgPoisson zeroes the right-hand side on solid cells; CgPoisson does not
-  launch_fcg_init(L, mg, d_b, rhs, x, rb, s);
-  if (project_) fin(FIN_MEAN, 1, nb, kSlotMeanRhs, L.nf, false, s);
-  launch_fcg_init2(L, project_, rhs, r, ctl, rb, s);
+  // MgPoisson zeroes the right-hand side on solid cells; CgPoisson does not.
+  // The kernel that writes r also writes the first V-cycle's FP32 right-hand
+  // side and level-0 first sweep from zero.
+  const bool first = mg && !loops_ && !tiled_ && nlevels() > 1 && opts_.pre_smooth >= 1;
+  float* b0 = first && mixed_ ? b0f_.as<float>() : nullptr;
+  float* x1f = first && mixed_ ? lx0_[0].as<float>() : nullptr;
+  double* x1d = first && !mixed_ ? lx0_[0].as<double>() : nullptr;
+  if (project_) {
+    launch_fcg_init(L, mg, d_b, rhs, x, rb, s);
+    fin(FIN_MEAN, 1, nb, kSlotMeanRhs, L.nf, false, s);
+    launch_fcg_init2(L, true, rhs, r, ctl, rb, s, b0, x1f, x1d);
+  } else {
+    launch_fcg_init(L, mg, d_b, rhs, x, rb, s, r, b0, x1f, x1d);
+  }
   fin(FIN_BNORM, 1, nb, 0, 0, false, s, mg ? 1 : 0);
   if (mg) {
     if (loops_) minvLoops(r, s);
-    else minv(r, s, nullptr, false);
+    else minv(r, s, nullptr, false, b0 != nullptr, first);
     launch_fcg_start(L, z, loops_ ? MeanRef{} : zMean(), r, p, ctl, rb, s);
   } else {
     launch_cg_z(L, r, z, true, p, ctl, rb, s);
