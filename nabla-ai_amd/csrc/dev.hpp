@@ -136,10 +136,11 @@ enum FinOp : int {
   FIN_BNORM_MEAN, // (|b|^2, sum) -> bnorm + counters reset, slot[slot] = sum / nf
   FIN_CONV_MEAN,  // (|r|^2, sum) -> convergence test, slot[slot] = sum / nf
   FIN_BNORM_R0,   // (|b|^2, |r0|^2) -> bnorm, reset counters, optional early exit on |r0|
+  FIN_BETA_FLEX_MEAN,  // (z.rn, z.r, z's fluid sum) -> FIN_BETA_FLEX, slot[slot] = sum / nf
 };
 struct FinArgs {
   int op;
-  int K;          // values folded (1 or 2)
+  int K;          // values folded (1 to 3)
   int nparts;     // partials written by the reducing kernel (its block count)
   int slot;
   long long nf;

@@ -1451,9 +1451,14 @@ __global__ void __launch_bounds__(kFinThreads) k_fin(RedBuf rb, FinArgs a, Solve
   ctl += blockIdx.x;
   const double* parts = rb.partials + size_t(blockIdx.x) * size_t(a.nparts);
   if (a.predicated && ldA(&ctl->done)) return;
-  double t[2];
-  if (a.K == 2) {
-    foldPartials<2>(parts, rb.stride, a.nparts, t);
+  double t[3] = {0.0, 0.0, 0.0};
+  if (a.K == 3) {
+    foldPartials<3>(parts, rb.stride, a.nparts, t);
+  } else if (a.K == 2) {
+    double t2[2];
+    foldPartials<2>(parts, rb.stride, a.nparts, t2);
+    t[0] = t2[0];
+    t[1] = t2[1];
   } else {
     double t1[1];
     foldPartials<1>(parts, rb.stride, a.nparts, t1);
@@ -1510,6 +1515,12 @@ __global__ void __launch_bounds__(kFinThreads) k_fin(RedBuf rb, FinArgs a, Solve
       }
       break;
     case FIN_BETA_FLEX:
+      beta = (t[0] - t[1]) / rz;
+      rz = t[0];
+      it += 1;
+      break;
+    case FIN_BETA_FLEX_MEAN:
+      stA(&ctl->slot[a.slot], a.nf > 0 ? t[2] / double(a.nf) : 0.0);
       beta = (t[0] - t[1]) / rz;
       rz = t[0];
       it += 1;

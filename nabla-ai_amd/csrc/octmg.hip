@@ -70,12 +70,17 @@ __device__ __forceinline__ void storeAs(__hip_bfloat16* p, float v) { *p = __flo
 
 // x <- x + omega (b - A x) / diag on fluid leaves, 0 elsewhere
 // (multigrid.cpp:102-114), in T arithmetic with x read through X and the
-// result stored as O; FOLD: per-block partial fluid sums of the result.
-template <class T, class XR, class O, bool FOLD>
+// result stored as O.  FOLD 1: per-block partial fluid sums of the result;
+// FOLD 2 (the cycle's last sweep inside the flexible CG): partials of
+// result.rn, result.r and the fluid sum, in that row order.
+template <class T, class XR, class O, int FOLD>
 __global__ void __launch_bounds__(kBlock) k_om_sweep(OctViewT<T> L, const T* __restrict__ b, XR X,
-                                                     O* __restrict__ xo, RedBuf rb, const SolveCtl* ctl) {
+                                                     O* __restrict__ xo, RedBuf rb, const SolveCtl* ctl,
+                                                     const double* __restrict__ rn, const double* __restrict__ ro) {
+  static_assert(FOLD >= 0 && FOLD <= 2, "fold mode");
   if (ctl && ldA(&ctl->done)) return;
-  double acc[1] = {0.0};
+  constexpr int K = FOLD == 2 ? 3 : 1;
+  double acc[K] = {};
   OGRID(i, L.n) {
     // evaluated on every leaf (non-fluid rows have no couplings and a unit
     // diagonal), so no load waits for the mask
@@ -83,14 +88,23 @@ __global__ void __launch_bounds__(kBlock) k_om_sweep(OctViewT<T> L, const T* __r
     const T xi = X(i);
     const T Ax = octRowA(L, i, X);
     const T bi = b[i], di = L.diag[i];
+    double rv = 0.0, r0 = 0.0;
+    if constexpr (FOLD == 2) {
+      rv = rn[i];
+      r0 = ro[i];
+    }
     T v = T(0);
     if (fl) {
       v = xi + T(kOmega) * (bi - Ax) / di;
-      if (FOLD) acc[0] += double(v);
+      if constexpr (FOLD == 2) {
+        acc[0] += double(v) * rv;
+        acc[1] += double(v) * r0;
+      }
+      if constexpr (FOLD >= 1) acc[K - 1] += double(v);
     }
     storeAs(xo + i, v);
   }
-  if constexpr (FOLD) blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
+  if constexpr (FOLD >= 1) blockToPartials<K>(acc, rb.partials, rb.stride, blockIdx.x);
 }
 
 // Residuals b - A x of a coarse cell's children, summed; with xc, the coarse
@@ -299,10 +313,24 @@ __global__ void k_sc_diag(long long nc, const uint8_t* __restrict__ mask, const 
   diag[I] = d > 0.0 ? d : 1.0;
 }
 
+// Partial fluid sums of x; with rn (the flexible CG's dots) the rows of
+// k_om_sweep<FOLD 2>: x.rn, x.r, the fluid sum.
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_om_fsum(long long n, const uint8_t* __restrict__ mask,
-                                                    const T* __restrict__ x, RedBuf rb, const SolveCtl* ctl) {
+                                                    const T* __restrict__ x, RedBuf rb, const SolveCtl* ctl,
+                                                    const double* __restrict__ rn, const double* __restrict__ ro) {
   if (ctl && ldA(&ctl->done)) return;
+  if (rn) {
+    double acc[3] = {0.0, 0.0, 0.0};
+    OGRID(i, n) if (mask[i] == 0) {
+      const double v = double(x[i]);
+      acc[0] += v * rn[i];
+      acc[1] += v * ro[i];
+      acc[2] += v;
+    }
+    blockToPartials<3>(acc, rb.partials, rb.stride, blockIdx.x);
+    return;
+  }
   double acc[1] = {0.0};
   OGRID(i, n) if (mask[i] == 0) acc[0] += double(x[i]);
   blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
@@ -657,12 +685,12 @@ void OctMg::cycle(size_t l, const T* b, RedBuf rb, const SolveCtl* ctl, hipStrea
       // further sweeps ends in xb
       for (int q = 0; q < kFallbackSweeps - 1; ++q) {
         const bool even = (q & 1) == 0;
-        k_om_sweep<T, PlainRd<T>, T, false><<<gs, kBlock, 0, s>>>(V, b, PlainRd<T>{even ? xa : xb}, even ? xb : xa,
-                                                                  rb, ctl);
+        k_om_sweep<T, PlainRd<T>, T, 0><<<gs, kBlock, 0, s>>>(V, b, PlainRd<T>{even ? xa : xb}, even ? xb : xa, rb,
+                                                              ctl, nullptr, nullptr);
       }
       NBL_LAUNCH_CHECK();
     }
-    if (fold) k_om_fsum<T><<<g, kBlock, 0, s>>>(n, V.mask, xb, rb, ctl);
+    if (fold) k_om_fsum<T><<<g, kBlock, 0, s>>>(n, V.mask, xb, rb, ctl, dRn_, dR_);
     NBL_LAUNCH_CHECK();
     *out = xb;
     return;
@@ -674,17 +702,20 @@ void OctMg::cycle(size_t l, const T* b, RedBuf rb, const SolveCtl* ctl, hipStrea
   const T* ec = nullptr;
   auto post = [&](auto xr) {
     using R = ProlongRd<T, decltype(xr)>;
-    k_om_sweep<T, R, T, false><<<gs, kBlock, 0, s>>>(V, b, R{xr, ec, L.up.as<int32_t>()}, xa, rb, ctl);
-    if (fold)
-      k_om_sweep<T, PlainRd<T>, T, true><<<g, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xb, rb, ctl);
+    k_om_sweep<T, R, T, 0><<<gs, kBlock, 0, s>>>(V, b, R{xr, ec, L.up.as<int32_t>()}, xa, rb, ctl, nullptr, nullptr);
+    if (fold && dRn_)
+      k_om_sweep<T, PlainRd<T>, T, 2><<<g, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xb, rb, ctl, dRn_, dR_);
+    else if (fold)
+      k_om_sweep<T, PlainRd<T>, T, 1><<<g, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xb, rb, ctl, nullptr, nullptr);
     else
-      k_om_sweep<T, PlainRd<T>, T, false><<<gs, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xb, rb, ctl);
+      k_om_sweep<T, PlainRd<T>, T, 0><<<gs, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xb, rb, ctl, nullptr, nullptr);
     NBL_LAUNCH_CHECK();
   };
   if constexpr (kF) {
     if (l == 0 && prec_ == NBL_PREC_MIXED_BF16) {
       __hip_bfloat16* xbB = L.xbB.as<__hip_bfloat16>();
-      k_om_sweep<T, PlainRd<T>, __hip_bfloat16, false><<<gs, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xbB, rb, ctl);
+      k_om_sweep<T, PlainRd<T>, __hip_bfloat16, 0><<<gs, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xbB, rb, ctl, nullptr,
+                                                                         nullptr);
       launchRestrict<T>(V, VC, C.cstart.as<int32_t>(), C.child.as<int32_t>(), b, Bf16Rd{xbB}, bc, xc, ctl, s);
       NBL_LAUNCH_CHECK();
       cycle<T>(l + 1, bc, rb, ctl, s, &ec);
@@ -694,13 +725,20 @@ void OctMg::cycle(size_t l, const T* b, RedBuf rb, const SolveCtl* ctl, hipStrea
     }
   }
   // pre-smoothing: first sweep (in xa) then one plain sweep
-  k_om_sweep<T, PlainRd<T>, T, false><<<gs, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xb, rb, ctl);
+  k_om_sweep<T, PlainRd<T>, T, 0><<<gs, kBlock, 0, s>>>(V, b, PlainRd<T>{xa}, xb, rb, ctl, nullptr, nullptr);
   launchRestrict<T>(V, VC, C.cstart.as<int32_t>(), C.child.as<int32_t>(), b, PlainRd<T>{xb}, bc, xc, ctl, s);
   NBL_LAUNCH_CHECK();
   cycle<T>(l + 1, bc, rb, ctl, s, &ec);
   // post-smoothing: prolongation fused into the first sweep
   post(PlainRd<T>{xb});
   *out = xb;
+}
+
+void OctMg::vcycleDots(const double* r, const double* rOld, RedBuf rb, const SolveCtl* ctl, hipStream_t s) {
+  dRn_ = r;
+  dR_ = rOld;
+  vcycle(r, rb, ctl, s);
+  dRn_ = dR_ = nullptr;
 }
 
 void OctMg::vcycle(const double* r, RedBuf rb, const SolveCtl* ctl, hipStream_t s) {

@@ -43,6 +43,10 @@ class OctMg {
   // result(); the last kernel writes `partials()` per-block partial fluid
   // sums of it into rb.
   void vcycle(const double* r, RedBuf rb, const SolveCtl* ctl, hipStream_t s);
+  // The same cycle whose last level-0 sweep writes partials of result.r,
+  // result.rOld and the result's fluid sum (rows 0, 1, 2) instead: the
+  // flexible CG's dots with the result's mean left to the caller.
+  void vcycleDots(const double* r, const double* rOld, RedBuf rb, const SolveCtl* ctl, hipStream_t s);
   const void* result() const { return res0_; }
   int partials() const { return parts_; }
 
@@ -72,6 +76,8 @@ class OctMg {
   int prec_ = 0;
   DevBuf code_, M_, Mf_;
   const void* res0_ = nullptr;
+  const double* dRn_ = nullptr;  // vcycleDots' r and rOld while its cycle is enqueued
+  const double* dR_ = nullptr;
   int parts_ = 0;
   double buildMs_ = 0.0;
 };

@@ -538,36 +538,24 @@ __global__ void __launch_bounds__(kBlock) k_om_start(long long n, long long nf, 
   blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
 }
 
-// x += alpha p; rn = r - alpha Ap; partial fluid sums of rn.
+// x += alpha p; rn = r - alpha Ap on fluid leaves, 0 elsewhere; the next
+// V-cycle's input; partials of |rn|^2.  The right-hand side is projected once
+// at the start and A annihilates constants on fluid leaves and is symmetric,
+// so r - alpha Ap stays mean-free to rounding and rn needs no projection.
+template <class T>
 __global__ void __launch_bounds__(kBlock) k_om_update(long long n, const uint8_t* __restrict__ mask,
+                                                      const double* __restrict__ diag, const T* __restrict__ diagT,
                                                       double* __restrict__ x, const double* __restrict__ p,
                                                       const double* __restrict__ r, const double* __restrict__ Ap,
-                                                      double* __restrict__ rn, const SolveCtl* ctl, RedBuf rb) {
+                                                      double* __restrict__ rn, T* __restrict__ rT,
+                                                      T* __restrict__ x0, const SolveCtl* ctl, RedBuf rb) {
   if (ldA(&ctl->done)) return;
   const double a = ldA(&ctl->alpha);
   double acc[1] = {0.0};
   OGRID(c, n) {
-    x[c] = x[c] + a * p[c];
-    const double v = r[c] - a * Ap[c];
-    rn[c] = v;
-    if (mask[c] == 0) acc[0] += v;
-  }
-  blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
-}
-
-// rn <- rn - mean (fluid), 0 elsewhere; the next V-cycle's input; partials
-// of |rn|^2.
-template <class T>
-__global__ void __launch_bounds__(kBlock) k_om_rfix(long long n, long long nf, const uint8_t* __restrict__ mask,
-                                                    const double* __restrict__ diag, const T* __restrict__ diagT,
-                                                    double* __restrict__ rn, T* __restrict__ rT, T* __restrict__ x0,
-                                                    const SolveCtl* ctl, RedBuf rb) {
-  if (ldA(&ctl->done)) return;
-  const double m = ldA(&ctl->slot[O_MEAN_R]);
-  double acc[1] = {0.0};
-  OGRID(c, n) {
     const bool fl = mask[c] == 0;
-    const double v = fl ? (nf ? rn[c] - m : rn[c]) : 0.0;
+    x[c] = x[c] + a * p[c];
+    const double v = fl ? r[c] - a * Ap[c] : 0.0;
     rn[c] = v;
     cycleInput(c, fl, v, diag, diagT, rT, x0);
     acc[0] += v * v;
@@ -575,34 +563,19 @@ __global__ void __launch_bounds__(kBlock) k_om_rfix(long long n, long long nf, c
   blockToPartials<1>(acc, rb.partials, rb.stride, blockIdx.x);
 }
 
-// z = V-cycle output - mean (fluid), 0 elsewhere; partials of z.rn and z.r.
+// p = z + beta p with z = the V-cycle's output - its fluid mean (fluid
+// leaves), 0 elsewhere: the projection of z formed where p reads it.
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_om_dots(long long n, long long nf, const uint8_t* __restrict__ mask,
-                                                    const T* __restrict__ zc, const double* __restrict__ rn,
-                                                    const double* __restrict__ r, double* __restrict__ z,
-                                                    const SolveCtl* ctl, RedBuf rb) {
-  if (ldA(&ctl->done)) return;
-  const double m = ldA(&ctl->slot[O_MEAN_Z]);
-  double acc[2] = {0.0, 0.0};
-  OGRID(c, n) {
-    const double zo = double(zc[c]);
-    const double zv = mask[c] == 0 ? (nf ? zo - m : zo) : 0.0;
-    z[c] = zv;
-    acc[0] += zv * rn[c];
-    acc[1] += zv * r[c];
-  }
-  blockToPartials<2>(acc, rb.partials, rb.stride, blockIdx.x);
-}
-
-// p = z + beta p; r = rn.
-__global__ void __launch_bounds__(kBlock) k_om_pupdate(long long n, double* __restrict__ p, const double* __restrict__ z,
-                                                       double* __restrict__ r, const double* __restrict__ rn,
+__global__ void __launch_bounds__(kBlock) k_om_pupdate(long long n, long long nf, const uint8_t* __restrict__ mask,
+                                                       const T* __restrict__ zc, double* __restrict__ p,
                                                        const SolveCtl* ctl) {
   if (ldA(&ctl->done)) return;
   const double be = ldA(&ctl->beta);
+  const double m = ldA(&ctl->slot[O_MEAN_Z]);
   OGRID(c, n) {
-    p[c] = z[c] + be * p[c];
-    r[c] = rn[c];
+    const double zo = double(zc[c]);
+    const double zv = mask[c] == 0 ? (nf ? zo - m : zo) : 0.0;
+    p[c] = zv + be * p[c];
   }
 }
 
@@ -1424,26 +1397,28 @@ nbl_poisson_report OctreeDev::solveMgSorted(const double* b, double* xo, double 
   int enq = 0;
   int batch = std::clamp(predictedMg_, 1, std::max(1, maxIters));
   while (enq < maxIters) {
+    // One flexible-CG iteration in seven launches: A p with p.Ap; alpha; x,
+    // r_new and the cycle's input with |r_new|^2; the convergence test; the
+    // V-cycle, whose last sweep folds z.r_new, z.r and z's fluid sum (r_new
+    // and r are mean-free, so these are the projected z's dots); beta and z's
+    // mean; p = (z - mean) + beta p.  r and r_new swap buffers.
     for (int q = 0; q < batch && enq < maxIters; ++q, ++enq) {
       k_o_apply_dot<<<g, kBlock, 0, s>>>(V, p, Ap, ctl, rb);
       fin(FIN_ALPHA, 1, 0, true);
-      k_om_update<<<g, kBlock, 0, s>>>(N, mk, x, p, r, Ap, rn, ctl, rb);
-      fin(FIN_MEAN, 1, O_MEAN_R, true);
       if (f32)
-        k_om_rfix<float><<<g, kBlock, 0, s>>>(N, nf, mk, dg, dgF, rn, rF, static_cast<float*>(M.firstSweep()), ctl,
-                                              rb);
+        k_om_update<float><<<g, kBlock, 0, s>>>(N, mk, dg, dgF, x, p, r, Ap, rn, rF,
+                                                static_cast<float*>(M.firstSweep()), ctl, rb);
       else
-        k_om_rfix<double><<<g, kBlock, 0, s>>>(N, nf, mk, dg, dg, rn, nullptr, static_cast<double*>(M.firstSweep()),
-                                               ctl, rb);
+        k_om_update<double><<<g, kBlock, 0, s>>>(N, mk, dg, dg, x, p, r, Ap, rn, nullptr,
+                                                 static_cast<double*>(M.firstSweep()), ctl, rb);
       fin(FIN_CONV, 1, 0, true);
-      M.vcycle(rn, rb, ctl, s);
-      fin(FIN_MEAN, 1, O_MEAN_Z, true);
+      M.vcycleDots(rn, r, rb, ctl, s);
+      fin(FIN_BETA_FLEX_MEAN, 3, O_MEAN_Z, true);
       if (f32)
-        k_om_dots<float><<<g, kBlock, 0, s>>>(N, nf, mk, static_cast<const float*>(M.result()), rn, r, z, ctl, rb);
+        k_om_pupdate<float><<<g, kBlock, 0, s>>>(N, nf, mk, static_cast<const float*>(M.result()), p, ctl);
       else
-        k_om_dots<double><<<g, kBlock, 0, s>>>(N, nf, mk, static_cast<const double*>(M.result()), rn, r, z, ctl, rb);
-      fin(FIN_BETA_FLEX, 2, 0, true);
-      k_om_pupdate<<<g, kBlock, 0, s>>>(N, p, z, r, rn, ctl);
+        k_om_pupdate<double><<<g, kBlock, 0, s>>>(N, nf, mk, static_cast<const double*>(M.result()), p, ctl);
+      std::swap(r, rn);
     }
     NBL_LAUNCH_CHECK();
     NBL_HIP(hipMemcpyAsync(hctl_, ctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, s));
