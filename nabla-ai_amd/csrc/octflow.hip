@@ -99,6 +99,9 @@ __global__ void k_of_wenoline(TreeV T, int4* __restrict__ wl) {
 template <class Fn>
 __device__ __forceinline__ void eachFace(const TreeV& T, long long i, Fn&& f) {
   int x = T.F.xrow[i];
+  // unrolled: d indexes the kernel arguments' per-face and per-axis arrays,
+  // which a runtime index would copy into scratch memory
+#pragma unroll
   for (int d = 0; d < 6; ++d) {
     const int c = T.F.cnt[6 * i + d];
     if (c == 0) {
@@ -218,7 +221,9 @@ __global__ void __launch_bounds__(kBlock) k_of_correct(TreeV T, Bc B, const doub
     const int li = T.lev[i];
     const uint64_t ki = T.keys[i];
     const double pi = p[i];
-    double div = 0.0, g[3] = {0.0, 0.0, 0.0};
+    // the gradient's components as scalars: an array indexed by the face's
+    // axis would live in scratch memory
+    double div = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
     eachFace(T, i, [&](int d, int nb, long long e) {
       const int axis = d >> 1;
       const double sign = (d & 1) ? 1.0 : -1.0;
@@ -237,12 +242,15 @@ __global__ void __launch_bounds__(kBlock) k_of_correct(TreeV T, Bc B, const doub
         // N lies along the outward normal: dp / dist is the normal derivative
         Uf = sign * 0.5 * (vel[i] + vel[nb]) - dt * dp / dist;
         div += Uf * A;
-        g[axis] += sign * (0.5 * dp) * A;
+        const double ga = sign * (0.5 * dp) * A;
+        if (axis == 0) gx += ga;
+        else if (axis == 1) gy += ga;
+        else gz += ga;
       }
       U[e] = Uf;
     });
     const double V = sizeAxis(T.G, li, 0) * sizeAxis(T.G, li, 1) * sizeAxis(T.G, li, 2);
-    const double un = u[i] - dt * g[0] / V, vn = v[i] - dt * g[1] / V, wn = w[i] - dt * g[2] / V;
+    const double un = u[i] - dt * gx / V, vn = v[i] - dt * gy / V, wn = w[i] - dt * gz / V;
     uo[i] = un;
     vo[i] = vn;
     wo[i] = wn;
