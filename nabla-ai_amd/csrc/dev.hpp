@@ -381,10 +381,12 @@ void launch_fcg_update_first(const MgLevel& L, double* x, const double* p, const
                              double* rn, T* x1, T* b0, SolveCtl* ctl, RedBuf rb, hipStream_t s, InFold alpha = {});
 // p_new = z + beta p_old and A p_new, p_new.Ap in one pass.  beta: the
 // flexible beta folded in from the partials of z.r_new and z.r.
-// zMeanNf > 0: beta's inline fold carries a third row, z's fluid sum, and
-// p_new = (z - sum / zMeanNf) + beta p_old on fluid cells.
+// zMeanNf > 0: z's mean is pending and p_new = (z - mean) + beta p_old on
+// fluid cells; beta's inline fold carries a third row, z's fluid sum, or
+// (no inline fold) beta and the mean were finalized into ctl and zMeanSlot.
 void launch_fcg_apply_pu(const MgLevel& L, const double* z, const double* pOld, double* pNew, double* Ap,
-                         SolveCtl* ctl, RedBuf rb, hipStream_t s, InFold beta = {}, double zMeanNf = 0.0);
+                         SolveCtl* ctl, RedBuf rb, hipStream_t s, InFold beta = {}, double zMeanNf = 0.0,
+                         const double* zMeanSlot = nullptr);
 // Last level-0 sweep of a V-cycle writing z and the partials of z.rn, z.r;
 // countIt: the sweep also counts the finished iteration (block 0).
 template <class T>

@@ -1055,18 +1055,24 @@ struct PNewMeanView {
   }
 };
 
-// MZ: beta's partials carry a third row, z's fluid sum (inline folds only;
-// nf = the fluid cell count).
+// MZ: z's mean is pending; with inline folds beta's partials carry a third
+// row, z's fluid sum (nf = the fluid cell count), otherwise beta and the mean
+// are read from ctl and the slot mz.
 template <int NC, bool MZ>
 __global__ void __launch_bounds__(kBlock) k_fcg_apply_pu(MgLevel L, const double* __restrict__ z,
                                                          const double* __restrict__ pOld, double* __restrict__ pNew,
                                                          double* __restrict__ Ap, SolveCtl* ctl, RedBuf rb,
-                                                         InFold bf, double nf) {
+                                                         InFold bf, double nf, MeanRef mz) {
   NBL_CELLS(NC);
   ctl = ctlAt(ctl, q.p);  // (batches fold no partials inline: bf.n == 0)
+  mz = mz.at(q.p);
   double be, m = 0.0;
-  if (MZ) {
+  if (MZ && bf.n) {
     if (!inlineBetaMean(bf, ctl, nf, be, m)) return;
+  } else if (MZ) {
+    if (ldA(&ctl->done)) return;
+    be = ldA(&ctl->beta);
+    m = ldA(mz.slot);
   } else if (bf.n) {
     if (!inlineBeta(bf, ctl, be)) return;
   } else {
@@ -1883,15 +1889,16 @@ template void launch_fcg_update_first<double>(const MgLevel&, double*, const dou
 template void launch_fcg_update_first<float>(const MgLevel&, double*, const double*, const double*, const double*,
                                              double*, float*, float*, SolveCtl*, RedBuf, hipStream_t, InFold);
 void launch_fcg_apply_pu(const MgLevel& L, const double* z, const double* pOld, double* pNew, double* Ap,
-                         SolveCtl* ctl, RedBuf rb, hipStream_t s, InFold beta, double zMeanNf) {
-  if (zMeanNf > 0.0 && !beta.n) throw std::logic_error("apply_pu: a pending z mean needs inline folds");
+                         SolveCtl* ctl, RedBuf rb, hipStream_t s, InFold beta, double zMeanNf, const double* zMeanSlot) {
+  if (zMeanNf > 0.0 && !beta.n && !zMeanSlot)
+    throw std::logic_error("apply_pu: a pending z mean needs inline folds or its slot");
   withCells(L, [&](auto nc, const Geo3& g) {
     if (zMeanNf > 0.0)
       k_fcg_apply_pu<decltype(nc)::value, true><<<g.grid, g.block, 0, s>>>(L, z, pOld, pNew, Ap, ctl, rb, beta,
-                                                                          zMeanNf);
+                                                                          zMeanNf, slotMean(zMeanSlot));
     else
       k_fcg_apply_pu<decltype(nc)::value, false><<<g.grid, g.block, 0, s>>>(L, z, pOld, pNew, Ap, ctl, rb, beta,
-                                                                           0.0);
+                                                                           0.0, MeanRef{});
   });
   NBL_LAUNCH_CHECK();
 }

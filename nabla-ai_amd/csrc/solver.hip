@@ -407,7 +407,7 @@ void PoissonDev::meanFin(int slot, bool inl, int blocks, long long nf, bool pred
 }
 
 void PoissonDev::fin(int op, int K, int nparts, int slot, long long nf, bool predicated, hipStream_t s,
-                     int earlyExit, double convTol) {
+                     int earlyExit, double convTol, int row0) {
   FinArgs a{};
   a.op = op;
   a.K = K;
@@ -417,7 +417,8 @@ void PoissonDev::fin(int op, int K, int nparts, int slot, long long nf, bool pre
   a.convTol = convTol;
   a.earlyExit = earlyExit;
   a.predicated = predicated ? 1 : 0;
-  launch_fin(red(), a, ctl_.as<SolveCtl>(), s, nb_);
+  const RedBuf rb = red();
+  launch_fin(RedBuf{rb.partials + size_t(row0) * size_t(rb.stride), rb.stride}, a, ctl_.as<SolveCtl>(), s, nb_);
 }
 
 // In-situ timing of the finest level's plain sweeps (the roofline kernel):
@@ -685,9 +686,11 @@ void PoissonDev::enqueueIteration(hipStream_t s, int k) {
     launch_fcg_apply(L, p, Ap, ctl, rbP, s);
   } else {
     const double* pOld = (k & 1) ? p_.as<double>() : p2_.as<double>();
-    if (zDotsMean())
+    if (zDotsMean() && inline_)
       launch_fcg_apply_pu(L, z, pOld, p, Ap, ctl, rbP, s,
                           InFold{rb.partials + kZDotsRow * size_t(rb.stride), rb.stride, nb, k}, double(L.nf));
+    else if (zDotsMean())  // beta and z's mean from FIN_BETA_FLEX_MEAN
+      launch_fcg_apply_pu(L, z, pOld, p, Ap, ctl, rbP, s, InFold{}, double(L.nf), slotPtr(ctl, kSlotMeanZ));
     else
       launch_fcg_apply_pu(L, z, pOld, p, Ap, ctl, rbP, s, fold(rb));
   }
@@ -719,9 +722,9 @@ void PoissonDev::enqueueIteration(hipStream_t s, int k) {
       }
       if (inline_) conv_ = fold(rbR);
       else fin(FIN_CONV, 1, nb, 0, 0, true, s);
-      // with inline folds the last level-0 sweep writes z and the dots, z's
-      // mean left for the next A p (zDotsMean); otherwise k_fcg_dots_x
-      minv(rn, s, r, zDotsMean() && first, mixed_, first);
+      // the last level-0 sweep writes z and the dots, z's mean left for the
+      // next A p (zDotsMean); otherwise k_fcg_dots_x
+      minv(rn, s, r, zDotsMean(), mixed_, first);
       if (conv_.n) throw std::logic_error("flexible CG: convergence test not placed");
     } else {
       // fusions need a first pre-sweep and a coarser level to smooth above
@@ -739,7 +742,10 @@ void PoissonDev::enqueueIteration(hipStream_t s, int k) {
       minv(rn, s, r, fused);
       if (conv_.n) throw std::logic_error("flexible CG: convergence test not placed");
     }
-    if (!inline_) fin(FIN_BETA_FLEX, 2, nb, 0, 0, true, s);
+    if (!inline_ && zDotsMean())
+      fin(FIN_BETA_FLEX_MEAN, 3, nb, kSlotMeanZ, L.nf, true, s, 0, 0.0, kZDotsRow);
+    else if (!inline_)
+      fin(FIN_BETA_FLEX, 2, nb, 0, 0, true, s);
   } else {
     double* r = r_.as<double>();
     launch_cg_update(L, x, p, r, Ap, ctl, rb, s);

@@ -125,8 +125,9 @@ class PoissonDev {
   void minv(const double* r, hipStream_t s, const double* rOld, bool fused, bool b0Ready = false,
             bool x1Ready = false);
   bool buildDirectCoarse();
+  // row0: the first partial row the finalize folds
   void fin(int op, int K, int nparts, int slot, long long nf, bool predicated, hipStream_t s, int earlyExit = 0,
-           double convTol = 0.0);
+           double convTol = 0.0, int row0 = 0);
   // Pending means of the null-space projections.  The mean in slot i of a
   // producer with `blocks` blocks is folded by its consumers from the slot's
   // own partial row when `inl` (few enough partials), else finalized into
@@ -200,10 +201,14 @@ class PoissonDev {
   bool l0Fused_ = false;
   bool l0First_ = false;  // the level-0 first sweep from zero is already in lx0_[0]
   bool l0FoldLast_ = false;  // the last level-0 sweep folds its fluid sum (z's pending mean)
-  // Singular systems with inline folds: the fused last level-0 sweep writes z
-  // with its mean pending (partial rows kZDotsRow.., z.rn, z.r, fluid sum) and
-  // the next A p removes it.
-  bool zDotsMean() const { return project_ && inline_ && dl_[0].nf > 0; }
+  // Singular systems (one kernel per sweep, a coarser level, two or more
+  // post-sweeps): the fused last level-0 sweep writes z with its mean pending
+  // (partial rows kZDotsRow.., z.rn, z.r, fluid sum; folded inline or by
+  // FIN_BETA_FLEX_MEAN) and the next A p removes it.
+  bool zDotsMean() const {
+    return kind_ == 0 && project_ && !tiled_ && !loops_ && nlevels() > 1 && opts_.pre_smooth >= 1 &&
+           opts_.post_smooth >= 2 && dl_[0].nf > 0;
+  }
   static constexpr int kZDotsRow = 4;
   const double* l0Rn_ = nullptr;
   const double* l0R_ = nullptr;
