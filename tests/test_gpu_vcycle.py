@@ -213,3 +213,21 @@ def test_tiled_cycle_beyond_2048_tiles(ctx, orc, precision):
     xo, ro = orc.mg(g, b, 1e-10, 200)
     assert abs(r2.iterations - ro.iterations) <= (2 if precision == nbl.PREC_MIXED else 1)
     assert rel(x2, xo) < 1e-8
+
+
+def test_singular_per_sweep_beyond_inline_folds(ctx, orc):
+    """A 128^3 periodic box has 4096 level-0 workgroups, beyond the inline
+    folds: the flexible CG's reductions run as finalize kernels, z's fluid
+    mean is finalized with beta and removed by the next A p where it forms p.
+    FP64 and FP32 cycles solve like the oracle (iterations, solution)."""
+    g, _ = cases.taylor_green(128)
+    b = cases.poisson_rhs(g, 61)
+    xo, ro = orc.mg(g, b, 1e-10, 200)
+    for precision in (nbl.PREC_FP64, nbl.PREC_MIXED):
+        ps = solver(ctx, g, True, precision)
+        try:
+            x, r = ps.solve(b)
+        finally:
+            ps.close()
+        assert r.converged and abs(r.iterations - ro.iterations) <= (2 if precision == nbl.PREC_MIXED else 1)
+        assert rel(x, xo) < 1e-8, precision
